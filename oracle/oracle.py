@@ -1,0 +1,194 @@
+"""ctypes front-end of the CPU oracle (liboracle.so, built from sgm_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg as the checker.  The product package never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+INVALID = 0xFFFFFFFF
+MAXQ = 32
+
+
+def build(force: bool = False):
+    """Compile liboracle.so / libsortprobe.so in place with the committed Makefile."""
+    args = ["make", "-C", HERE]
+    if force:
+        args += ["-B"]
+    subprocess.run(args, check=True, stdout=subprocess.DEVNULL)
+
+
+_lib = None
+
+
+class Stats(C.Structure):
+    _fields_ = [("count", C.c_uint64), ("intersections", C.c_uint64),
+                ("nodes", C.c_uint64 * MAXQ), ("fs_prunes", C.c_uint64),
+                ("enum_ms", C.c_double), ("timed_out", C.c_int32), ("capped", C.c_int32)]
+
+    def as_dict(self, n):
+        return {"count": int(self.count), "intersections": int(self.intersections),
+                "nodes": [int(x) for x in self.nodes[:n]], "fs_prunes": int(self.fs_prunes),
+                "enum_ms": float(self.enum_ms), "timed_out": bool(self.timed_out),
+                "capped": bool(self.capped)}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        _lib = C.CDLL(path)
+        assert _lib.orc_stats_size() == C.sizeof(Stats)
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def nlf(g, q):
+    V, n = g.getVertexCnt(), q.getVertexCnt()
+    cand = np.zeros((n, V), dtype=np.uint32)
+    cnt = np.zeros(n, dtype=np.uint32)
+    ok = lib().orc_nlf(C.c_uint32(V), _p(g.off), C.c_uint32(n), _p(q.off), _p(cand), _p(cnt))
+    return bool(ok), [cand[u, :cnt[u]].copy() for u in range(n)]
+
+
+def gql(g, q, return_stats=False):
+    V, n = g.getVertexCnt(), q.getVertexCnt()
+    cand = np.zeros((n, V), dtype=np.uint32)
+    cnt = np.zeros(n, dtype=np.uint32)
+    stats = np.zeros(4, dtype=np.uint64)
+    ok = lib().orc_gql(C.c_uint32(V), _p(g.off), _p(g.adj), C.c_uint32(n), _p(q.off), _p(q.adj),
+                       _p(cand), _p(cnt), _p(stats))
+    res = (bool(ok), [cand[u, :cnt[u]].copy() for u in range(n)])
+    if return_stats:
+        return res + ({"checks": int(stats[0]), "removed_r0": int(stats[1]),
+                       "removed_r1": int(stats[2])},)
+    return res
+
+
+def table(g, cand_a, cand_b):
+    """Aux table for one directed query edge: (offsets, idx, vertex ids, max_degree)."""
+    V = g.getVertexCnt()
+    ca, cb = _u32(cand_a), _u32(cand_b)
+    scratch = np.zeros(max(V, 1), dtype=np.int32)
+    L = lib()
+    L.orc_table.restype = C.c_uint64
+    e = L.orc_table(C.c_uint32(V), _p(g.off), _p(g.adj), _p(ca), C.c_uint32(ca.size), _p(cb),
+                    C.c_uint32(cb.size), None, None, None, None, _p(scratch))
+    off = np.zeros(ca.size + 1, dtype=np.uint32)
+    idx = np.zeros(max(e, 1), dtype=np.uint32)
+    vv = np.zeros(max(e, 1), dtype=np.uint32)
+    md = C.c_uint32(0)
+    L.orc_table(C.c_uint32(V), _p(g.off), _p(g.adj), _p(ca), C.c_uint32(ca.size), _p(cb),
+                C.c_uint32(cb.size), _p(off), _p(idx), _p(vv), C.byref(md), _p(scratch))
+    return off, idx[:e], vv[:e], int(md.value)
+
+
+def order_ri(q):
+    n = q.getVertexCnt()
+    o = np.zeros(n, dtype=np.uint32)
+    lib().orc_order_ri(C.c_uint32(n), _p(q.off), _p(q.adj), _p(o))
+    return [int(x) for x in o]
+
+
+def order_gql(g, q, counts):
+    n = q.getVertexCnt()
+    o = np.zeros(n, dtype=np.uint32)
+    cnt = _u32(counts)
+    lib().orc_order_gql(C.c_uint32(g.getVertexCnt()), C.c_uint32(n), _p(q.off), _p(q.adj),
+                        _p(cnt), _p(o))
+    return [int(x) for x in o]
+
+
+def plan(q, order):
+    """PlanIR as a dict: ops (depth, parent, vdepth, edge depth list) and iter_set."""
+    n = q.getVertexCnt()
+    o = _u32(order)
+    nops = C.c_uint32(0)
+    arrs = [np.zeros(512, dtype=np.uint32) for _ in range(4)]
+    it = np.zeros(n, dtype=np.uint32)
+    rc = lib().orc_plan_export(C.c_uint32(n), _p(q.off), _p(q.adj), _p(o), C.byref(nops),
+                               *[_p(a) for a in arrs], _p(it))
+    if rc:
+        raise ValueError("order is not connected (empty iteration set)")
+    k = nops.value
+    depth, parent, vdepth, emask = (a[:k] for a in arrs)
+    ops = []
+    for i in range(k):
+        m = int(emask[i])
+        ops.append({"depth": int(depth[i]),
+                    "parent": None if parent[i] == INVALID else int(parent[i]),
+                    "vdepth": int(vdepth[i]),
+                    "edges": [b for b in range(32) if (m >> b) & 1]})
+    return {"ops": ops, "iter_set": [None if x == INVALID else int(x) for x in it]}
+
+
+def sort_ops_perm(depth, size, elems, use_std=False):
+    """Permutation produced by the oracle's restated sort (or std::sort itself)."""
+    n = len(depth)
+    d, s = _u32(depth), _u32(size)
+    e = _u32(elems).reshape(n, 32)
+    perm = np.zeros(max(n, 1), dtype=np.uint32)
+    if use_std:
+        path = os.path.join(HERE, "libsortprobe.so")
+        if not os.path.exists(path):
+            build()
+        C.CDLL(path).probe_std_sort(C.c_long(n), _p(d), _p(s), _p(e), _p(perm))
+    else:
+        lib().orc_sort_ops_test(C.c_long(n), _p(d), _p(s), _p(e), _p(perm))
+    return [int(x) for x in perm[:n]]
+
+
+def _cand_matrix(cands, V):
+    n = len(cands)
+    stride = max([len(c) for c in cands] + [1])
+    m = np.zeros((n, stride), dtype=np.uint32)
+    cnt = np.zeros(n, dtype=np.uint32)
+    for u, c in enumerate(cands):
+        m[u, :len(c)] = c
+        cnt[u] = len(c)
+    return m, cnt, stride
+
+
+def enum_cache(g, q, cands, order, cap=None, fs=True, time_limit_ms=0.0):
+    """explore_Intersection_Cache_DAF_Filter restated; returns the stats dict."""
+    V, n = g.getVertexCnt(), q.getVertexCnt()
+    m, cnt, stride = _cand_matrix(cands, V)
+    o = _u32(order)
+    st = Stats()
+    rc = lib().orc_enum_cache(C.c_uint32(V), _p(g.off), _p(g.adj), C.c_uint32(n), _p(q.off),
+                              _p(q.adj), _p(m), _p(cnt), C.c_uint32(stride), _p(o),
+                              C.c_uint64(cap if cap is not None else 2**64 - 1),
+                              C.c_int(1 if fs else 0), C.c_double(time_limit_ms), C.byref(st))
+    if rc:
+        raise ValueError(f"orc_enum_cache failed ({rc})")
+    return st.as_dict(n)
+
+
+def enum_all(g, q, cands, order, cap=None, time_limit_ms=0.0):
+    """explore_Intersection_All<false x5> restated; returns the stats dict."""
+    V, n = g.getVertexCnt(), q.getVertexCnt()
+    m, cnt, stride = _cand_matrix(cands, V)
+    o = _u32(order)
+    st = Stats()
+    rc = lib().orc_enum_all(C.c_uint32(V), _p(g.off), _p(g.adj), C.c_uint32(n), _p(q.off),
+                            _p(q.adj), _p(m), _p(cnt), C.c_uint32(stride), _p(o),
+                            C.c_uint64(cap if cap is not None else 2**64 - 1),
+                            C.c_double(time_limit_ms), C.byref(st))
+    if rc:
+        raise ValueError(f"orc_enum_all failed ({rc})")
+    return st.as_dict(n)

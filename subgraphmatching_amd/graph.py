@@ -1,0 +1,187 @@
+"""Host-side graph containers and the reference text format.
+
+``Graph`` / ``Query`` mirror the reference's ``Graph`` / ``Query`` classes
+(inc/graph.h:51-378, :380-495): an undirected simple graph held as a CSR with each
+adjacency list sorted ascending (inc/graph.h:259-260).  Only the unlabeled case is on the
+accelerated path, so every vertex label is 0.
+
+Text format (README.md:64-84, parse rules inc/graph.h:190-266)::
+
+    t <V> <E>
+    v <id> <label> <degree>      one line per vertex, ids 0..V-1 in order
+    e <u> <v>                    one line per undirected edge
+
+The reference fills adjacency slots from the ``v`` lines' degree fields, so those must be
+exact; ``load_text`` checks that instead of trusting it.
+"""
+from __future__ import annotations
+
+import hashlib
+import os
+
+import numpy as np
+
+__all__ = ["Graph", "Query", "csr_from_edges", "load_text", "save_text"]
+
+
+def csr_from_edges(num_vertices: int, edges: np.ndarray):
+    """Build a sorted, deduplicated, self-loop-free CSR from an (m, 2) edge array.
+
+    Returns ``(off, adj)`` as uint32 arrays (``off`` has V+1 entries, ``adj`` 2|E|).
+    """
+    V = int(num_vertices)
+    e = np.asarray(edges, dtype=np.int64).reshape(-1, 2)
+    if e.size:
+        if e.min() < 0 or e.max() >= V:
+            raise ValueError("edge endpoint out of range")
+    a = np.minimum(e[:, 0], e[:, 1])
+    b = np.maximum(e[:, 0], e[:, 1])
+    keep = a != b
+    key = np.unique(a[keep] * V + b[keep])
+    a = key // V
+    b = key % V
+    src = np.concatenate([a, b])
+    dst = np.concatenate([b, a])
+    order = np.lexsort((dst, src))
+    src = src[order]
+    dst = dst[order]
+    deg = np.bincount(src, minlength=V)
+    off = np.zeros(V + 1, dtype=np.int64)
+    np.cumsum(deg, out=off[1:])
+    if off[-1] >= 2**32:
+        raise ValueError("graph too large for 32-bit offsets")
+    return off.astype(np.uint32), dst.astype(np.uint32)
+
+
+class Graph:
+    """Undirected simple graph in sorted CSR form (inc/graph.h:51-378)."""
+
+    def __init__(self, off: np.ndarray, adj: np.ndarray, name: str = ""):
+        self.off = np.ascontiguousarray(off, dtype=np.uint32)
+        self.adj = np.ascontiguousarray(adj, dtype=np.uint32)
+        self.name = name
+        if self.off.ndim != 1 or self.off.size < 1:
+            raise ValueError("offsets must be a 1-D array of V+1 entries")
+        if int(self.off[-1]) != self.adj.size:
+            raise ValueError("offsets do not cover the adjacency array")
+
+    @classmethod
+    def from_edges(cls, num_vertices: int, edges, name: str = ""):
+        off, adj = csr_from_edges(num_vertices, np.asarray(edges))
+        return cls(off, adj, name)
+
+    # reference accessors -------------------------------------------------------------
+    def getVertexCnt(self) -> int:  # inc/graph.h:291
+        return int(self.off.size - 1)
+
+    def getEdgeCnt(self) -> int:  # inc/graph.h:299
+        return int(self.adj.size // 2)
+
+    def getVertexDegree(self, v: int) -> int:  # inc/graph.h:295
+        return int(self.off[v + 1] - self.off[v])
+
+    def getMaxDgree(self) -> int:  # inc/graph.h:307
+        d = self.degrees()
+        return int(d.max()) if d.size else 0
+
+    def getNeb(self, v: int) -> np.ndarray:  # inc/graph.h:347-350
+        return self.adj[self.off[v]:self.off[v + 1]]
+
+    def hasEdge(self, u: int, v: int) -> bool:  # inc/graph.h:376-378
+        nb = self.getNeb(u)
+        i = int(np.searchsorted(nb, v))
+        return i < nb.size and int(nb[i]) == v
+
+    def getVertexLabel(self, v: int) -> int:  # inc/graph.h:359
+        return 0
+
+    # helpers ---------------------------------------------------------------------------
+    def degrees(self) -> np.ndarray:
+        return np.diff(self.off.astype(np.int64))
+
+    def edges(self) -> np.ndarray:
+        """Undirected edge list (u < v), sorted."""
+        src = np.repeat(np.arange(self.getVertexCnt(), dtype=np.int64), self.degrees())
+        dst = self.adj.astype(np.int64)
+        m = src < dst
+        return np.stack([src[m], dst[m]], axis=1)
+
+    def digest(self) -> str:
+        h = hashlib.sha256()
+        h.update(np.uint64(self.getVertexCnt()).tobytes())
+        h.update(self.off.tobytes())
+        h.update(self.adj.tobytes())
+        return h.hexdigest()
+
+    def Load(self, path: str):  # inc/graph.h:190-266
+        g = load_text(path)
+        self.off, self.adj, self.name = g.off, g.adj, g.name
+        return self
+
+
+class Query(Graph):
+    """Query graph (inc/graph.h:380-495).  At most 32 vertices on the accelerated path."""
+
+    MAX_VERTICES = 32
+
+    def __init__(self, off, adj, name: str = ""):
+        super().__init__(off, adj, name)
+        if self.getVertexCnt() > self.MAX_VERTICES:
+            raise ValueError(f"query has {self.getVertexCnt()} vertices; at most 32 are supported")
+
+    def is_connected(self) -> bool:
+        n = self.getVertexCnt()
+        if n == 0:
+            return False
+        seen = {0}
+        stack = [0]
+        while stack:
+            x = stack.pop()
+            for y in self.getNeb(x).tolist():
+                if y not in seen:
+                    seen.add(y)
+                    stack.append(y)
+        return len(seen) == n
+
+
+def load_text(path: str, cls=Graph):
+    """Read the reference text format (inc/graph.h:190-266)."""
+    labels_seen = set()
+    V = E = None
+    decl_deg = None
+    edges = []
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if not parts:
+                continue
+            if parts[0] == "t":
+                V, E = int(parts[1]), int(parts[2])
+                decl_deg = np.zeros(V, dtype=np.int64)
+            elif parts[0] == "v":
+                vid, lab, d = int(parts[1]), int(parts[2]), int(parts[3])
+                labels_seen.add(lab)
+                decl_deg[vid] = d
+            elif parts[0] == "e":
+                edges.append((int(parts[1]), int(parts[2])))
+    if V is None:
+        raise ValueError(f"{path}: missing 't' header")
+    if labels_seen - {0}:
+        raise ValueError(f"{path}: labelled graphs are not on the accelerated path")
+    g = cls.from_edges(V, np.array(edges, dtype=np.int64).reshape(-1, 2),
+                       name=os.path.splitext(os.path.basename(path))[0])
+    if g.getEdgeCnt() != E or not np.array_equal(g.degrees(), decl_deg):
+        raise ValueError(f"{path}: header/degree fields disagree with the edge list "
+                         "(the reference requires exact degrees and a simple graph)")
+    return g
+
+
+def save_text(g: Graph, path: str):
+    """Write the reference text format (all labels 0)."""
+    V = g.getVertexCnt()
+    deg = g.degrees()
+    e = g.edges()
+    with open(path, "w") as f:
+        f.write(f"t {V} {e.shape[0]}\n")
+        f.write("".join(f"v {i} 0 {int(deg[i])}\n" for i in range(V)))
+        f.write("".join(f"e {int(a)} {int(b)}\n" for a, b in e))
