@@ -1,0 +1,136 @@
+/*
+ * sgm_c.h -- C ABI of the MI355X (gfx950) filter -> order -> enumerate engine.
+ *
+ * Plain C types only: host buffers are caller-owned with explicit sizes; device objects
+ * (context, resident data graph, query with its candidate space) are opaque,
+ * library-owned handles.  Every entry point returns 0 (SGM_OK) or a negative SGM_E*
+ * code and never exits the process; sgm_ctx_last_error() holds a message.  One context
+ * per host thread; calls on one context are not thread-safe except sgm_stop().
+ *
+ * Each entry point names the reference interface it replaces (paths relative to the
+ * Orrinn/SubgraphMatching source tree).  Only unlabeled graphs (all labels 0) are on this
+ * path, and queries have 2..32 vertices.
+ */
+#ifndef SGM_C_H
+#define SGM_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SGM_ABI_VERSION 1
+
+enum {
+    SGM_OK = 0,
+    SGM_EINVAL = -1,    /* bad argument or malformed graph */
+    SGM_EHIP = -2,      /* HIP runtime error */
+    SGM_ENOMEM = -3,    /* device allocation failed */
+    SGM_EEMPTY = -4,    /* some candidate set is empty (reference filters return false) */
+    SGM_EORDER = -5,    /* order is not a connected permutation (reference PlanIR exits) */
+    SGM_ESTATE = -6,    /* call out of sequence (e.g. enumerate before filter) */
+    SGM_ESTOPPED = -7,  /* sgm_stop() interrupted the search: counts are partial */
+    SGM_EDEVICE = -8,   /* a device-side capacity check tripped */
+    SGM_ENODEV = -9     /* no usable GPU */
+};
+
+enum { SGM_FILTER_NLF = 1, SGM_FILTER_GQL = 2 };
+
+enum {
+    SGM_ENUM_CACHE_FILTER_FS = 0, /* explore_Intersection_Cache_DAF_Filter (inc/explore.h:1073) */
+    SGM_ENUM_CACHE_FILTER = 1,    /* same search without failing-set pruning */
+    SGM_ENUM_AUX_TABLES = 2       /* explore_Intersection_All<false,false,false,false,false> (inc/explore.h:2757) */
+};
+
+typedef struct sgm_ctx sgm_ctx;
+typedef struct sgm_graph sgm_graph;
+typedef struct sgm_query sgm_query;
+
+typedef struct {
+    uint64_t cap;          /* output limit: count = min(#embeddings, cap); UINT64_MAX = none */
+    int32_t method;        /* SGM_ENUM_* */
+    uint32_t blocks;       /* persistent blocks of 4 waves; 0 = automatic */
+    uint32_t stack_words;  /* per-wave set-op stack (u32 words); 0 = automatic */
+    uint32_t reserved[5];
+} sgm_enum_opts;
+
+typedef struct {
+    uint64_t count;          /* min(#embeddings, cap) */
+    uint64_t intersections;  /* set intersections (Profiler::total_intersection semantics) */
+    uint64_t nodes[32];      /* extensions per depth */
+    uint64_t fs_prunes;      /* failing-set sibling cut-offs */
+    uint64_t elems_read;     /* list elements fed to intersections, sum of |A|+|B| */
+    uint64_t elems_out;      /* intersection results, sum of |A n B| */
+    uint64_t elems_scanned;  /* iteration-set elements tested against filter and injectivity */
+    double kernel_ms;        /* enumeration kernel time (HIP events) */
+    int32_t capped;          /* the cap was reached */
+    int32_t stopped;         /* sgm_stop() interrupted: count is partial */
+    uint32_t retries;        /* re-runs with a larger set-op stack */
+    uint32_t reserved;
+} sgm_stats;
+
+/* ---- context ------------------------------------------------------------------------ */
+const char* sgm_version(void);
+int sgm_device_count(int* count);
+int sgm_ctx_create(int device, sgm_ctx** out);
+void sgm_ctx_destroy(sgm_ctx* ctx);
+const char* sgm_ctx_last_error(const sgm_ctx* ctx);
+/* Device time (HIP events) of the last filter call on this context: the NLF kernel, the
+ * GQL refinement phases and the candidate bitmap/count kernels (no reference
+ * counterpart: the reference times filters with a host clock,
+ * src/optimizations_ablation.cpp:64-82). */
+int sgm_ctx_filter_ms(const sgm_ctx* ctx, double* nlf_ms, double* gql_ms, double* cand_ms);
+/* Cooperative stop of a running enumeration (replaces the std::atomic<bool>& stop_flag
+ * polled by the reference, inc/explore.h:1057, src/overall_performance.cpp:302-305). */
+int sgm_stop(sgm_ctx* ctx);
+
+/* ---- data graph (replaces Graph::Load's CSR, inc/graph.h:190-266) ----------------- */
+/* offsets[V+1], adj[adj_len]: undirected simple graph, every list strictly ascending.  The
+ * graph stays resident in HBM until sgm_graph_destroy. */
+int sgm_graph_upload(sgm_ctx* ctx, const uint32_t* offsets, const uint32_t* adj, uint32_t num_vertices,
+                     uint64_t adj_len, sgm_graph** out);
+void sgm_graph_destroy(sgm_graph* g);
+uint32_t sgm_graph_max_degree(const sgm_graph* g);
+
+/* ---- query and candidate space (replaces Query::Load + CandidateParam, inc/api.h:9-46) */
+int sgm_query_create(sgm_graph* g, const uint32_t* qoff, const uint32_t* qadj, uint32_t n, sgm_query** out);
+void sgm_query_destroy(sgm_query* q);
+
+/* Filter_NLF / Filter_GQL (inc/filter.h:1049-1063, _NLF_Filter :80-93, _GQL_Filter :233-313).
+ * Returns SGM_OK, or SGM_EEMPTY when some C(u) is empty (the reference returns false). */
+int sgm_filter(sgm_query* q, int kind);
+/* Same for many queries of one graph in one set of launches; status[i] gets each result. */
+int sgm_filter_batch(sgm_query* const* qs, uint32_t nq, int kind, int32_t* status);
+/* candidates_count[] (inc/api.h:12) */
+int sgm_cand_counts(const sgm_query* q, uint32_t* counts);
+/* candidates[u] in ascending vertex id (inc/api.h:11); out_cap >= counts[u] */
+int sgm_cand_download(sgm_query* q, uint32_t u, uint32_t* out, uint32_t out_cap);
+/* candidates_exsist (inc/api.h:17) as one word per data vertex: bit u set iff v in C(u) */
+int sgm_cand_exist_download(sgm_query* q, uint32_t* words, uint32_t num_words);
+
+/* buildTables (inc/table.h:7-137) for every directed query edge, on the device */
+int sgm_build_tables(sgm_query* q);
+/* edge_matrix[a][b] (inc/graph.h:13-49): e_cnt and _max_degree */
+int sgm_table_info(sgm_query* q, uint32_t a, uint32_t b, uint64_t* e_cnt, uint32_t* max_degree);
+/* edge_matrix[a][b]: _offset[|C(a)|+1], _edge[e_cnt], _edge_v[e_cnt] */
+int sgm_table_download(sgm_query* q, uint32_t a, uint32_t b, uint32_t* offsets, uint32_t* idx, uint32_t* vtx);
+
+/* ---- orders and plan (host; replaces Order_RI inc/order.h:1138, Order_GQL :563, PlanIR
+ * inc/code_gen.h:699) ------------------------------------------------------------------ */
+int sgm_order_ri(const uint32_t* qoff, const uint32_t* qadj, uint32_t n, uint32_t* order);
+int sgm_order_gql(const sgm_query* q, uint32_t* order);
+/* Plan export for inspection: per op depth, parent (UINT32_MAX = none), v_depth, bitmask
+ * of backward-neighbour depths; iter_set[n].  Arrays hold up to 512 ops. */
+int sgm_plan_export(const uint32_t* qoff, const uint32_t* qadj, uint32_t n, const uint32_t* order, uint32_t* n_ops,
+                    uint32_t* depth, uint32_t* parent, uint32_t* vdepth, uint32_t* emask, uint32_t* iter_set);
+
+/* ---- enumeration (replaces explore_Intersection_Cache_DAF_Filter inc/explore.h:1073 and
+ * explore_Intersection_All<false x5> :2757) -------------------------------------------- */
+int sgm_enumerate(sgm_query* q, const uint32_t* order, const sgm_enum_opts* opts, sgm_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SGM_C_H */

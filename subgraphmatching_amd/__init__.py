@@ -1,0 +1,11 @@
+"""MI355X (gfx950) engine for the filter -> order -> enumerate path of unlabeled subgraph
+matching, with the call surface of Orrinn/SubgraphMatching (inc/api.h, filter.h, order.h,
+explore.h).  See ``subgraphmatching_amd.api`` for the mirrored entry points."""
+from .graph import Graph, Query, load_text, save_text  # noqa: F401
+from . import generators  # noqa: F401
+from .api import (CandidateParam, Config, Context, ExploreResult, Filter_GQL, Filter_NLF,  # noqa: F401
+                  Order_GQL, Order_RI, Profiler, ReuseParam, default_context, enumerate_query,
+                  explore_Intersection_All, explore_Intersection_Cache_DAF_Filter, filter_batch,
+                  plan_export)
+
+__version__ = "0.1.0"

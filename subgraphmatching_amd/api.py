@@ -1,0 +1,406 @@
+"""Python mirror of the reference's filter / order / enumerate call surface, backed by the
+gfx950 library through its C ABI.
+
+Reference surface (inc/api.h, inc/filter.h, inc/order.h, inc/explore.h):
+
+    CandidateParam can(data, query);            -> can = CandidateParam(data, query)
+    Filter_NLF(data, query, can) / Filter_GQL     -> same names, return bool
+    Order_RI(query, order) / Order_GQL(...)      -> return the order list
+    ReuseParam reuse(query, order)               -> ReuseParam(query, order)
+    Config cfg; cfg.useFilter = ...              -> Config(...)
+    explore_Intersection_Cache_DAF_Filter(data, query, order, cfg, time, ans, stop, limit)
+                                                 -> returns ExploreResult(time, ans, ...)
+    explore_Intersection_All<false x5>(...)      -> explore_Intersection_All(...)
+
+Profiler counters (inc/common_type.h:150-226) are mirrored by ``Profiler.getInst()``.
+Everything runs on the GPU; a missing library or device raises instead of falling back.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import threading
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+from .graph import Graph, Query
+
+__all__ = ["Context", "default_context", "DeviceGraph", "CandidateParam", "Edges", "Filter_NLF",
+           "Filter_GQL", "Order_RI", "Order_GQL", "ReuseParam", "Config", "ExploreResult",
+           "explore_Intersection_Cache_DAF_Filter", "explore_Intersection_All", "Profiler",
+           "filter_batch", "enumerate_query", "plan_export"]
+
+
+def _check(code, ctx=None):
+    if code != L.SGM_OK:
+        msg = L.lib().sgm_ctx_last_error(ctx.handle if ctx else None)
+        raise L.SgmError(code, msg.decode() if msg else "")
+
+
+class Context:
+    """One device context (stream, scratch buffers) on one GPU."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        code = L.lib().sgm_ctx_create(device, C.byref(h))
+        if code != L.SGM_OK:
+            raise L.SgmError(code, f"cannot create a context on device {device}")
+        self.handle = h
+        self.device = device
+        self._graphs = {}
+
+    def close(self):
+        if self.handle:
+            for dg in list(self._graphs.values()):
+                dg.close()
+            self._graphs.clear()
+            L.lib().sgm_ctx_destroy(self.handle)
+            self.handle = None
+
+    def filter_ms(self):
+        """(nlf, gql, candidate-count) device milliseconds of the last filter call."""
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        _check(L.lib().sgm_ctx_filter_ms(self.handle, C.byref(a), C.byref(b), C.byref(c)), self)
+        return a.value, b.value, c.value
+
+    def stop(self):
+        """Cooperative stop of the running enumeration (the reference's stop_flag)."""
+        _check(L.lib().sgm_stop(self.handle), self)
+
+    def upload(self, g: Graph) -> "DeviceGraph":
+        key = id(g)
+        dg = self._graphs.get(key)
+        if dg is None or dg.graph is not g:
+            dg = DeviceGraph(self, g)
+            self._graphs[key] = dg
+        return dg
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default = None
+_default_lock = threading.Lock()
+
+
+def default_context() -> Context:
+    global _default
+    with _default_lock:
+        if _default is None:
+            _default = Context(0)
+        return _default
+
+
+class DeviceGraph:
+    """The data graph resident in HBM."""
+
+    def __init__(self, ctx: Context, g: Graph):
+        self.ctx = ctx
+        self.graph = g
+        h = C.c_void_p()
+        off, adj = L.u32(g.off), L.u32(g.adj)
+        _check(L.lib().sgm_graph_upload(ctx.handle, L.ptr(off), L.ptr(adj), g.getVertexCnt(), adj.size,
+                                        C.byref(h)), ctx)
+        self.handle = h
+        self.max_degree = int(L.lib().sgm_graph_max_degree(h))
+
+    def close(self):
+        if self.handle:
+            L.lib().sgm_graph_destroy(self.handle)
+            self.handle = None
+
+
+class Edges:
+    """One auxiliary table edge_matrix[a][b] (inc/graph.h:13-49), downloaded lazily."""
+
+    def __init__(self, can: "CandidateParam", a: int, b: int):
+        e = C.c_uint64()
+        md = C.c_uint32()
+        _check(L.lib().sgm_table_info(can._q, a, b, C.byref(e), C.byref(md)), can.ctx)
+        self._e_cnt = int(e.value)
+        self._max_degree = int(md.value)
+        self._v_cnt = int(can.candidates_count[a])
+        self._offset = np.zeros(self._v_cnt + 1, dtype=np.uint32)
+        self._edge = np.zeros(max(self._e_cnt, 1), dtype=np.uint32)
+        self._edge_v = np.zeros(max(self._e_cnt, 1), dtype=np.uint32)
+        _check(L.lib().sgm_table_download(can._q, a, b, L.ptr(self._offset), L.ptr(self._edge),
+                                          L.ptr(self._edge_v)), can.ctx)
+        self._edge = self._edge[:self._e_cnt]
+        self._edge_v = self._edge_v[:self._e_cnt]
+
+    def getNeb(self, i: int) -> np.ndarray:
+        return self._edge[self._offset[i]:self._offset[i + 1]]
+
+    def getNeb_V(self, i: int) -> np.ndarray:
+        return self._edge_v[self._offset[i]:self._offset[i + 1]]
+
+
+class _EdgeMatrix:
+    def __init__(self, can):
+        self._can = can
+        self._cache = {}
+
+    def __getitem__(self, a):
+        can = self._can
+
+        class _Row:
+            def __getitem__(_, b):
+                if not can.query.hasEdge(a, b):
+                    return None
+                key = (a, b)
+                if key not in self._cache:
+                    self._cache[key] = Edges(can, a, b)
+                return self._cache[key]
+        return _Row()
+
+
+class CandidateParam:
+    """Candidate space of one query on the device (inc/api.h:9-164).
+
+    ``candidates[u]`` / ``candidates_count[u]`` / ``candidates_exsist[u][v]`` /
+    ``edge_matrix[a][b]`` are materialised on the host on first access.
+    """
+
+    def __init__(self, data: Graph, query: Query, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self.data = data
+        self.query = query
+        self.dg = self.ctx.upload(data)
+        h = C.c_void_p()
+        qoff, qadj = L.u32(query.off), L.u32(query.adj)
+        _check(L.lib().sgm_query_create(self.dg.handle, L.ptr(qoff), L.ptr(qadj), query.getVertexCnt(),
+                                        C.byref(h)), self.ctx)
+        self._q = h
+        self._filtered = False
+        self._counts = None
+        self._cands = None
+        self._exist = None
+        self.edge_matrix = _EdgeMatrix(self)
+
+    def close(self):
+        if self._q:
+            L.lib().sgm_query_destroy(self._q)
+            self._q = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _after_filter(self):
+        self._filtered = True
+        n = self.query.getVertexCnt()
+        cnt = np.zeros(n, dtype=np.uint32)
+        _check(L.lib().sgm_cand_counts(self._q, L.ptr(cnt)), self.ctx)
+        self._counts = cnt
+        self._cands = None
+        self._exist = None
+        self.edge_matrix = _EdgeMatrix(self)
+
+    @property
+    def candidates_count(self) -> np.ndarray:
+        if self._counts is None:
+            raise RuntimeError("filter the candidates first")
+        return self._counts
+
+    @property
+    def candidates(self):
+        if self._cands is None:
+            out = []
+            for u in range(self.query.getVertexCnt()):
+                c = int(self.candidates_count[u])
+                buf = np.zeros(max(c, 1), dtype=np.uint32)
+                _check(L.lib().sgm_cand_download(self._q, u, L.ptr(buf), max(c, 1)), self.ctx)
+                out.append(buf[:c])
+            self._cands = out
+        return self._cands
+
+    @property
+    def candidates_exsist(self) -> np.ndarray:
+        """n x V boolean matrix, as the reference's candidates_exsist."""
+        if self._exist is None:
+            V = self.data.getVertexCnt()
+            words = np.zeros(max(V, 1), dtype=np.uint32)
+            _check(L.lib().sgm_cand_exist_download(self._q, L.ptr(words), V), self.ctx)
+            n = self.query.getVertexCnt()
+            self._exist = ((words[None, :V] >> np.arange(n, dtype=np.uint32)[:, None]) & 1).astype(bool)
+        return self._exist
+
+
+def _filter(data, query, can, kind, build_tables):
+    code = L.lib().sgm_filter(can._q, kind)
+    if code == L.SGM_EEMPTY:
+        can._after_filter()
+        return False
+    _check(code, can.ctx)
+    can._after_filter()
+    if build_tables:
+        _check(L.lib().sgm_build_tables(can._q), can.ctx)
+    return True
+
+
+def Filter_NLF(data: Graph, query: Query, can: CandidateParam, build_tables: bool = True) -> bool:
+    """inc/filter.h:1049-1055: NLF candidates, then FilterParamConstruct (tables)."""
+    return _filter(data, query, can, L.SGM_FILTER_NLF, build_tables)
+
+
+def Filter_GQL(data: Graph, query: Query, can: CandidateParam, build_tables: bool = True) -> bool:
+    """inc/filter.h:1057-1063: GQL candidates, then FilterParamConstruct (tables)."""
+    return _filter(data, query, can, L.SGM_FILTER_GQL, build_tables)
+
+
+def filter_batch(cans, kind=L.SGM_FILTER_GQL):
+    """Filter many queries of one data graph in one set of launches; returns [bool]."""
+    if not cans:
+        return []
+    ctx = cans[0].ctx
+    arr = (C.c_void_p * len(cans))(*[c._q.value for c in cans])
+    status = np.zeros(len(cans), dtype=np.int32)
+    _check(L.lib().sgm_filter_batch(arr, len(cans), kind, L.ptr(status)), ctx)
+    for c in cans:
+        c._after_filter()
+    return [int(s) == L.SGM_OK for s in status]
+
+
+def Order_RI(query: Query) -> list:
+    """inc/order.h:1138-1281."""
+    n = query.getVertexCnt()
+    o = np.zeros(n, dtype=np.uint32)
+    qoff, qadj = L.u32(query.off), L.u32(query.adj)
+    code = L.lib().sgm_order_ri(L.ptr(qoff), L.ptr(qadj), n, L.ptr(o))
+    if code != L.SGM_OK:
+        raise L.SgmError(code, "Order_RI")
+    return [int(x) for x in o]
+
+
+def Order_GQL(data: Graph, query: Query, can: CandidateParam) -> list:
+    """inc/order.h:563-597 (needs filtered candidate counts)."""
+    o = np.zeros(query.getVertexCnt(), dtype=np.uint32)
+    _check(L.lib().sgm_order_gql(can._q, L.ptr(o)), can.ctx)
+    return [int(x) for x in o]
+
+
+def plan_export(query: Query, order):
+    """The intersection-cache plan (inc/code_gen.h:682-847) as built by the library."""
+    n = query.getVertexCnt()
+    qoff, qadj, o = L.u32(query.off), L.u32(query.adj), L.u32(order)
+    nops = C.c_uint32()
+    arrs = [np.zeros(512, dtype=np.uint32) for _ in range(4)]
+    it = np.zeros(n, dtype=np.uint32)
+    code = L.lib().sgm_plan_export(L.ptr(qoff), L.ptr(qadj), n, L.ptr(o), C.byref(nops), *[L.ptr(a) for a in arrs],
+                                   L.ptr(it))
+    if code != L.SGM_OK:
+        raise L.SgmError(code, "plan")
+    k = nops.value
+    ops = []
+    for i in range(k):
+        m = int(arrs[3][i])
+        ops.append({"depth": int(arrs[0][i]), "parent": None if arrs[1][i] == 0xFFFFFFFF else int(arrs[1][i]),
+                    "vdepth": int(arrs[2][i]), "edges": [b for b in range(32) if (m >> b) & 1]})
+    return {"ops": ops, "iter_set": [None if x == 0xFFFFFFFF else int(x) for x in it]}
+
+
+class ReuseParam:
+    """inc/api.h:167-182: owns the intersection-cache plan for (query, order)."""
+
+    def __init__(self, query: Query, order):
+        self.order = list(order)
+        self.plan = plan_export(query, order)
+
+
+@dataclass
+class Config:
+    """inc/api.h:184-227 (non-owning)."""
+    useFilter: bool = False
+    filterType: str = "gql"
+    orderType: str = "gql"
+    useCache: bool = False
+    usePrune: bool = False
+    can: CandidateParam | None = None
+    reuse: ReuseParam | None = None
+    useDAFFailingPrune: bool = False
+
+    def Check(self):
+        if self.useFilter and self.can is None:
+            raise ValueError("config error, use filter but canParam is nullptr")
+        if self.useCache and self.reuse is None:
+            raise ValueError("config error, use cache but reuseParam is nullptr")
+
+
+class Profiler:
+    """Mirror of the reference's Profiler counters (inc/common_type.h:150-226)."""
+    _inst = None
+
+    def __init__(self):
+        self.reset()
+
+    @classmethod
+    def getInst(cls):
+        if cls._inst is None:
+            cls._inst = Profiler()
+        return cls._inst
+
+    def reset(self):
+        self.total_intersection = 0
+        self.total_iter_count = [0] * 300
+        self.daf_prune_cnt = 0
+
+
+@dataclass
+class ExploreResult:
+    time: float            # ms, enumeration kernel time
+    ans: int               # min(#embeddings, output_limit)
+    intersections: int = 0
+    nodes: list = field(default_factory=list)
+    fs_prunes: int = 0
+    elems_read: int = 0
+    elems_out: int = 0
+    elems_scanned: int = 0
+    capped: bool = False
+    retries: int = 0
+
+
+def enumerate_query(can: CandidateParam, order, method=L.SGM_ENUM_CACHE_FILTER_FS, cap=None, blocks=0,
+                    stack_words=0) -> ExploreResult:
+    opts = L.EnumOpts()
+    opts.cap = (1 << 64) - 1 if cap is None else int(cap)
+    opts.method = method
+    opts.blocks = blocks
+    opts.stack_words = stack_words
+    st = L.Stats()
+    o = L.u32(order)
+    _check(L.lib().sgm_enumerate(can._q, L.ptr(o), C.byref(opts), C.byref(st)), can.ctx)
+    n = can.query.getVertexCnt()
+    res = ExploreResult(time=float(st.kernel_ms), ans=int(st.count), intersections=int(st.intersections),
+                        nodes=[int(x) for x in st.nodes[:n]], fs_prunes=int(st.fs_prunes),
+                        elems_read=int(st.elems_read), elems_out=int(st.elems_out),
+                        elems_scanned=int(st.elems_scanned), capped=bool(st.capped), retries=int(st.retries))
+    prof = Profiler.getInst()
+    prof.total_intersection += res.intersections
+    for d, x in enumerate(res.nodes):
+        prof.total_iter_count[d] += x
+    prof.daf_prune_cnt += res.fs_prunes
+    return res
+
+
+def explore_Intersection_Cache_DAF_Filter(data: Graph, query: Query, order, cfg: Config,
+                                          output_limit: int | None = None) -> ExploreResult:
+    """inc/explore.h:1073-1129 (RI+cache+filter+fs row)."""
+    cfg.Check()
+    if cfg.can is None:
+        raise ValueError("the cache+filter+DAF search needs filtered candidates (cfg.can)")
+    return enumerate_query(cfg.can, order, L.SGM_ENUM_CACHE_FILTER_FS, output_limit)
+
+
+def explore_Intersection_All(data: Graph, query: Query, order, cfg: Config,
+                             output_limit: int | None = None) -> ExploreResult:
+    """inc/explore.h:2757-2848 with all pruning disabled (the "RI" / "GQL" rows)."""
+    cfg.Check()
+    if cfg.can is None:
+        raise ValueError("the candidate-space search needs filtered candidates (cfg.can)")
+    return enumerate_query(cfg.can, order, L.SGM_ENUM_AUX_TABLES, output_limit)

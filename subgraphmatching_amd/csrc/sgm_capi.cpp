@@ -1,0 +1,696 @@
+// C ABI implementation: device residency, filter / table / enumeration drivers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/sgm_c.h"
+#include "sgm_host.h"
+#include "sgm_internal.h"
+#include "sgm_kernels.h"
+
+struct sgm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipStream_t ctl = nullptr;       // non-blocking stream for sgm_stop
+    std::string err;
+    uint32_t* d_stop = nullptr;      // stop word polled by the enumeration kernel
+    uint32_t* d_work = nullptr;      // work counter
+    sgm_stats_dev* d_stats = nullptr;
+    sgm_plan_dev* d_plan = nullptr;
+    uint32_t* d_stack = nullptr;
+    size_t stack_bytes = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t fev[4] = {nullptr, nullptr, nullptr, nullptr};
+    double filter_ms[3] = {0, 0, 0};
+    int num_cus = 256;
+    volatile int stop_requested = 0;
+};
+
+struct sgm_graph {
+    sgm_ctx* ctx = nullptr;
+    uint32_t V = 0;
+    uint64_t adj_len = 0;
+    uint32_t max_deg = 0;
+    uint32_t* d_off = nullptr;
+    uint32_t* d_adj = nullptr;
+    uint32_t* d_bydeg = nullptr;
+    uint32_t* d_gecnt = nullptr;
+    uint32_t ge_cnt[33] = {0};
+};
+
+struct sgm_table {
+    bool built = false;
+    uint64_t e_cnt = 0;
+    uint32_t max_deg = 0;
+    uint32_t* d_off = nullptr;  // |C(a)|+1
+    uint32_t* d_idx = nullptr;
+    uint32_t* d_v = nullptr;
+};
+
+struct sgm_query {
+    sgm_graph* g = nullptr;
+    sgm::QueryCSR q;
+    uint32_t nbr[SGM_MAXQ] = {0};
+    int filtered = 0;                 // SGM_FILTER_* after a successful or empty filter
+    bool empty = false;
+    bool compacted = false;
+    uint32_t counts[SGM_MAXQ] = {0};
+    uint32_t cand_off[SGM_MAXQ + 1] = {0};
+    uint32_t* d_vmask = nullptr;
+    uint32_t* d_removed = nullptr;
+    uint64_t* d_bm = nullptr;
+    uint32_t* d_rank = nullptr;
+    uint32_t* d_cnt = nullptr;
+    uint32_t* d_candoff = nullptr;
+    uint32_t* d_list = nullptr;
+    sgm_qdesc* d_qdesc = nullptr;
+    sgm_cdesc* d_cdesc = nullptr;
+    sgm_qdesc hq;                     // host copies of the descriptors
+    sgm_cdesc hc;
+    std::vector<sgm_table> tables;    // n*n, directed (a,b)
+};
+
+static int set_err(sgm_ctx* c, int code, const char* fmt, ...)
+{
+    if (c) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        c->err = buf;
+    }
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                          \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return set_err((ctx), e_ == hipErrorOutOfMemory ? SGM_ENOMEM : SGM_EHIP, "%s: %s (%s:%d)", \
+                           #call, hipGetErrorString(e_), __FILE__, __LINE__);                      \
+    } while (0)
+
+template <class T>
+static hipError_t dalloc(T** p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    return hipMalloc((void**)p, count * sizeof(T));
+}
+
+static void dfree(void* p)
+{
+    if (p) (void)hipFree(p);
+}
+
+extern "C" {
+
+const char* sgm_version(void) { return "sgm-mi355x 0.1 (gfx950)"; }
+
+int sgm_device_count(int* count)
+{
+    if (!count) return SGM_EINVAL;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    return SGM_OK;
+}
+
+int sgm_ctx_create(int device, sgm_ctx** out)
+{
+    if (!out) return SGM_EINVAL;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return SGM_ENODEV;
+    if (device < 0 || device >= n) return SGM_EINVAL;
+    sgm_ctx* c = new sgm_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) { delete c; return SGM_EHIP; }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->ctl, hipStreamNonBlocking) != hipSuccess ||
+        dalloc(&c->d_stop, 1) != hipSuccess || dalloc(&c->d_work, 1) != hipSuccess ||
+        dalloc(&c->d_stats, 1) != hipSuccess || dalloc(&c->d_plan, 1) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreate(&c->fev[0]) != hipSuccess || hipEventCreate(&c->fev[1]) != hipSuccess ||
+        hipEventCreate(&c->fev[2]) != hipSuccess || hipEventCreate(&c->fev[3]) != hipSuccess) {
+        sgm_ctx_destroy(c);
+        return SGM_EHIP;
+    }
+    *out = c;
+    return SGM_OK;
+}
+
+void sgm_ctx_destroy(sgm_ctx* c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    dfree(c->d_stop);
+    dfree(c->d_work);
+    dfree(c->d_stats);
+    dfree(c->d_plan);
+    dfree(c->d_stack);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    for (auto& e : c->fev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->ctl) (void)hipStreamDestroy(c->ctl);
+    delete c;
+}
+
+const char* sgm_ctx_last_error(const sgm_ctx* c) { return c ? c->err.c_str() : "no context"; }
+
+int sgm_ctx_filter_ms(const sgm_ctx* c, double* nlf_ms, double* gql_ms, double* cand_ms)
+{
+    if (!c) return SGM_EINVAL;
+    if (nlf_ms) *nlf_ms = c->filter_ms[0];
+    if (gql_ms) *gql_ms = c->filter_ms[1];
+    if (cand_ms) *cand_ms = c->filter_ms[2];
+    return SGM_OK;
+}
+
+int sgm_stop(sgm_ctx* c)
+{
+    if (!c) return SGM_EINVAL;
+    static const uint32_t one = 1;
+    c->stop_requested = 1;
+    HIPCHK(c, hipMemcpyAsync(c->d_stop, &one, sizeof one, hipMemcpyHostToDevice, c->ctl));
+    HIPCHK(c, hipStreamSynchronize(c->ctl));
+    return SGM_OK;
+}
+
+// ------------------------------------------------------------------------------ graph
+int sgm_graph_upload(sgm_ctx* c, const uint32_t* off, const uint32_t* adj, uint32_t V, uint64_t adj_len,
+                     sgm_graph** out)
+{
+    if (!c || !off || (!adj && adj_len) || !out) return set_err(c, SGM_EINVAL, "null argument");
+    *out = nullptr;
+    if (off[0] != 0 || off[V] != adj_len) return set_err(c, SGM_EINVAL, "offsets do not span adj");
+    if (adj_len >= 0x80000000ull) return set_err(c, SGM_EINVAL, "adjacency longer than 2^31 entries");
+    uint32_t maxdeg = 0;
+    std::vector<uint32_t> hist;
+    for (uint32_t v = 0; v < V; v++) {
+        if (off[v + 1] < off[v]) return set_err(c, SGM_EINVAL, "offsets not monotone at %u", v);
+        const uint32_t d = off[v + 1] - off[v];
+        maxdeg = std::max(maxdeg, d);
+        for (uint32_t p = off[v]; p < off[v + 1]; p++) {
+            if (adj[p] >= V) return set_err(c, SGM_EINVAL, "neighbour out of range at vertex %u", v);
+            if (adj[p] == v) return set_err(c, SGM_EINVAL, "self loop at vertex %u", v);
+            if (p > off[v] && adj[p] <= adj[p - 1])
+                return set_err(c, SGM_EINVAL, "adjacency of %u not strictly ascending", v);
+        }
+    }
+    // vertices by degree, descending, ties by id (counting sort)
+    hist.assign((size_t)maxdeg + 2, 0);
+    for (uint32_t v = 0; v < V; v++) hist[off[v + 1] - off[v]]++;
+    std::vector<uint32_t> start((size_t)maxdeg + 2, 0);
+    uint32_t acc = 0;
+    for (int64_t d = maxdeg; d >= 0; d--) { start[d] = acc; acc += hist[d]; }
+    std::vector<uint32_t> bydeg(V ? V : 1);
+    for (uint32_t v = 0; v < V; v++) bydeg[start[off[v + 1] - off[v]]++] = v;
+    sgm_graph* g = new sgm_graph();
+    g->ctx = c;
+    g->V = V;
+    g->adj_len = adj_len;
+    g->max_deg = maxdeg;
+    for (uint32_t d = 0; d <= 32; d++) {
+        uint64_t s = 0;
+        for (uint64_t x = d; x <= maxdeg; x++) s += hist[x];
+        g->ge_cnt[d] = (uint32_t)s;
+    }
+    hipError_t e = hipSetDevice(c->device);
+    if (e == hipSuccess) e = dalloc(&g->d_off, (size_t)V + 1);
+    if (e == hipSuccess) e = dalloc(&g->d_adj, adj_len);
+    if (e == hipSuccess) e = dalloc(&g->d_bydeg, V);
+    if (e == hipSuccess) e = dalloc(&g->d_gecnt, 33);
+    if (e == hipSuccess) e = hipMemcpy(g->d_off, off, sizeof(uint32_t) * ((size_t)V + 1), hipMemcpyHostToDevice);
+    if (e == hipSuccess && adj_len) e = hipMemcpy(g->d_adj, adj, sizeof(uint32_t) * adj_len, hipMemcpyHostToDevice);
+    if (e == hipSuccess && V) e = hipMemcpy(g->d_bydeg, bydeg.data(), sizeof(uint32_t) * V, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(g->d_gecnt, g->ge_cnt, sizeof(g->ge_cnt), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        sgm_graph_destroy(g);
+        return set_err(c, e == hipErrorOutOfMemory ? SGM_ENOMEM : SGM_EHIP, "graph upload: %s", hipGetErrorString(e));
+    }
+    *out = g;
+    return SGM_OK;
+}
+
+void sgm_graph_destroy(sgm_graph* g)
+{
+    if (!g) return;
+    dfree(g->d_off);
+    dfree(g->d_adj);
+    dfree(g->d_bydeg);
+    dfree(g->d_gecnt);
+    delete g;
+}
+
+uint32_t sgm_graph_max_degree(const sgm_graph* g) { return g ? g->max_deg : 0; }
+
+// ------------------------------------------------------------------------------ query
+static void free_tables(sgm_query* q)
+{
+    for (auto& t : q->tables) {
+        dfree(t.d_off);
+        dfree(t.d_idx);
+        dfree(t.d_v);
+        t = sgm_table();
+    }
+}
+
+void sgm_query_destroy(sgm_query* q)
+{
+    if (!q) return;
+    free_tables(q);
+    dfree(q->d_vmask);
+    dfree(q->d_removed);
+    dfree(q->d_bm);
+    dfree(q->d_rank);
+    dfree(q->d_cnt);
+    dfree(q->d_candoff);
+    dfree(q->d_list);
+    dfree(q->d_qdesc);
+    dfree(q->d_cdesc);
+    delete q;
+}
+
+int sgm_query_create(sgm_graph* g, const uint32_t* qoff, const uint32_t* qadj, uint32_t n, sgm_query** out)
+{
+    if (!g || !qoff || !out) return SGM_EINVAL;
+    sgm_ctx* c = g->ctx;
+    *out = nullptr;
+    if (n < 1 || n > SGM_MAXQ) return set_err(c, SGM_EINVAL, "query must have 1..32 vertices (got %u)", n);
+    if (qoff[0] != 0) return set_err(c, SGM_EINVAL, "query offsets must start at 0");
+    sgm_query* q = new sgm_query();
+    q->g = g;
+    q->q.n = n;
+    q->q.off.assign(qoff, qoff + n + 1);
+    q->q.adj.assign(qadj, qadj + qoff[n]);
+    for (uint32_t u = 0; u < n; u++) {
+        if (qoff[u + 1] < qoff[u]) { sgm_query_destroy(q); return set_err(c, SGM_EINVAL, "query offsets not monotone"); }
+        for (uint32_t p = qoff[u]; p < qoff[u + 1]; p++) {
+            const uint32_t w = qadj[p];
+            if (w >= n || w == u || (p > qoff[u] && w <= qadj[p - 1])) {
+                sgm_query_destroy(q);
+                return set_err(c, SGM_EINVAL, "query adjacency of %u malformed", u);
+            }
+            bool back = false;
+            for (uint32_t r = qoff[w]; r < qoff[w + 1]; r++) back |= qadj[r] == u;
+            if (!back) { sgm_query_destroy(q); return set_err(c, SGM_EINVAL, "query graph not symmetric"); }
+            q->nbr[u] |= 1u << w;
+        }
+    }
+    const uint32_t V = g->V, W = (V + 63) / 64;
+    hipError_t e = hipSetDevice(c->device);
+    if (e == hipSuccess) e = dalloc(&q->d_vmask, V);
+    if (e == hipSuccess) e = dalloc(&q->d_removed, 2 * SGM_MAXQ);
+    if (e == hipSuccess) e = dalloc(&q->d_bm, (size_t)n * W);
+    if (e == hipSuccess) e = dalloc(&q->d_rank, (size_t)n * W);
+    if (e == hipSuccess) e = dalloc(&q->d_cnt, SGM_MAXQ);
+    if (e == hipSuccess) e = dalloc(&q->d_candoff, SGM_MAXQ + 1);
+    if (e == hipSuccess) e = dalloc(&q->d_qdesc, 1);
+    if (e == hipSuccess) e = dalloc(&q->d_cdesc, 1);
+    if (e == hipSuccess) {
+        sgm_qdesc qd;
+        std::memset(&qd, 0, sizeof qd);
+        qd.n = n;
+        for (uint32_t u = 0; u < n; u++) { qd.qdeg[u] = qoff[u + 1] - qoff[u]; qd.nbr[u] = q->nbr[u]; }
+        qd.vmask = q->d_vmask;
+        qd.removed = q->d_removed;
+        sgm_cdesc cd;
+        std::memset(&cd, 0, sizeof cd);
+        cd.n = n;
+        cd.vmask = q->d_vmask;
+        cd.bm = q->d_bm;
+        cd.rank = q->d_rank;
+        cd.cnt = q->d_cnt;
+        cd.cand_off = q->d_candoff;
+        cd.list = nullptr;
+        q->hq = qd;
+        q->hc = cd;
+        e = hipMemcpy(q->d_qdesc, &qd, sizeof qd, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(q->d_cdesc, &cd, sizeof cd, hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+        sgm_query_destroy(q);
+        return set_err(c, e == hipErrorOutOfMemory ? SGM_ENOMEM : SGM_EHIP, "query alloc: %s", hipGetErrorString(e));
+    }
+    q->tables.resize((size_t)n * n);
+    *out = q;
+    return SGM_OK;
+}
+
+// ------------------------------------------------------------------------------ filter
+int sgm_filter_batch(sgm_query* const* qs, uint32_t nq, int kind, int32_t* status)
+{
+    if (!qs || nq == 0) return SGM_EINVAL;
+    sgm_graph* g = qs[0]->g;
+    sgm_ctx* c = g->ctx;
+    if (kind != SGM_FILTER_NLF && kind != SGM_FILTER_GQL) return set_err(c, SGM_EINVAL, "unknown filter kind %d", kind);
+    for (uint32_t i = 0; i < nq; i++)
+        if (!qs[i] || qs[i]->g != g) return set_err(c, SGM_EINVAL, "queries of a batch must share one graph");
+    HIPCHK(c, hipSetDevice(c->device));
+    const uint32_t V = g->V;
+    // gather the per-query descriptors into one device array
+    std::vector<sgm_qdesc> qd(nq);
+    std::vector<sgm_cdesc> cd(nq);
+    uint32_t max_n = 0;
+    for (uint32_t i = 0; i < nq; i++) {
+        qd[i] = qs[i]->hq;
+        cd[i] = qs[i]->hc;
+        max_n = std::max(max_n, qs[i]->q.n);
+        free_tables(qs[i]);
+        dfree(qs[i]->d_list);
+        qs[i]->d_list = nullptr;
+        qs[i]->compacted = false;
+        HIPCHK(c, hipMemsetAsync(qs[i]->d_removed, 0, sizeof(uint32_t) * 2 * SGM_MAXQ, c->stream));
+    }
+    sgm_qdesc* d_qd = nullptr;
+    sgm_cdesc* d_cd = nullptr;
+    HIPCHK(c, dalloc(&d_qd, nq));
+    HIPCHK(c, dalloc(&d_cd, nq));
+    HIPCHK(c, hipMemcpyAsync(d_qd, qd.data(), sizeof(sgm_qdesc) * nq, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(d_cd, cd.data(), sizeof(sgm_cdesc) * nq, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipEventRecord(c->fev[0], c->stream));
+    sgm::launch_nlf(c->stream, g->d_off, V, d_qd, nq);
+    HIPCHK(c, hipEventRecord(c->fev[1], c->stream));
+    if (kind == SGM_FILTER_GQL) {
+        for (uint32_t ph = 0; ph < 2 * max_n; ph++) {
+            uint32_t mc = 0;
+            for (uint32_t i = 0; i < nq; i++) {
+                const uint32_t n = qs[i]->q.n;
+                if (ph >= 2 * n) continue;
+                const uint32_t u = ph % n;
+                mc = std::max(mc, g->ge_cnt[std::min<uint32_t>(qd[i].qdeg[u], 32)]);
+            }
+            if (mc) sgm::launch_gql_phase(c->stream, g->d_off, g->d_adj, g->d_bydeg, g->d_gecnt, d_qd, nq, ph, mc);
+        }
+    }
+    HIPCHK(c, hipEventRecord(c->fev[2], c->stream));
+    sgm::launch_bitmaps(c->stream, d_cd, nq, V);
+    sgm::launch_rank_scan(c->stream, d_cd, nq, V, max_n);
+    HIPCHK(c, hipEventRecord(c->fev[3], c->stream));
+    HIPCHK(c, hipGetLastError());
+    for (uint32_t i = 0; i < nq; i++)
+        HIPCHK(c, hipMemcpyAsync(qs[i]->counts, qs[i]->d_cnt, sizeof(uint32_t) * qs[i]->q.n, hipMemcpyDeviceToHost,
+                                 c->stream));
+    hipError_t se = hipStreamSynchronize(c->stream);
+    dfree(d_qd);
+    dfree(d_cd);
+    if (se != hipSuccess) return set_err(c, SGM_EHIP, "filter: %s", hipGetErrorString(se));
+    for (int k = 0; k < 3; k++) {
+        float ms = 0;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->fev[k], c->fev[k + 1]));
+        c->filter_ms[k] = ms;
+    }
+    int rc = SGM_OK;
+    for (uint32_t i = 0; i < nq; i++) {
+        sgm_query* q = qs[i];
+        q->filtered = kind;
+        q->empty = false;
+        for (uint32_t u = 0; u < q->q.n; u++) q->empty |= q->counts[u] == 0;
+        if (status) status[i] = q->empty ? SGM_EEMPTY : SGM_OK;
+        if (q->empty) rc = SGM_EEMPTY;
+    }
+    return nq == 1 ? rc : SGM_OK;
+}
+
+int sgm_filter(sgm_query* q, int kind)
+{
+    if (!q) return SGM_EINVAL;
+    int32_t st = 0;
+    int rc = sgm_filter_batch(&q, 1, kind, &st);
+    return rc;
+}
+
+int sgm_cand_counts(const sgm_query* q, uint32_t* counts)
+{
+    if (!q || !counts) return SGM_EINVAL;
+    if (!q->filtered) return set_err(q->g->ctx, SGM_ESTATE, "filter the query first");
+    std::memcpy(counts, q->counts, sizeof(uint32_t) * q->q.n);
+    return SGM_OK;
+}
+
+static int ensure_compacted(sgm_query* q)
+{
+    sgm_ctx* c = q->g->ctx;
+    if (!q->filtered) return set_err(c, SGM_ESTATE, "filter the query first");
+    if (q->compacted) return SGM_OK;
+    const uint32_t n = q->q.n;
+    q->cand_off[0] = 0;
+    for (uint32_t u = 0; u < n; u++) q->cand_off[u + 1] = q->cand_off[u] + q->counts[u];
+    HIPCHK(c, hipSetDevice(c->device));
+    dfree(q->d_list);
+    HIPCHK(c, dalloc(&q->d_list, q->cand_off[n]));
+    q->hc.list = q->d_list;
+    HIPCHK(c, hipMemcpy(q->d_cdesc, &q->hc, sizeof q->hc, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(q->d_candoff, q->cand_off, sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice));
+    sgm::launch_compact(c->stream, q->d_cdesc, 1, q->g->V);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    q->compacted = true;
+    return SGM_OK;
+}
+
+int sgm_cand_download(sgm_query* q, uint32_t u, uint32_t* out, uint32_t out_cap)
+{
+    if (!q || !out) return SGM_EINVAL;
+    sgm_ctx* c = q->g->ctx;
+    if (u >= q->q.n) return set_err(c, SGM_EINVAL, "query vertex out of range");
+    int rc = ensure_compacted(q);
+    if (rc) return rc;
+    if (out_cap < q->counts[u]) return set_err(c, SGM_EINVAL, "output buffer too small (%u < %u)", out_cap, q->counts[u]);
+    if (q->counts[u])
+        HIPCHK(c, hipMemcpy(out, q->d_list + q->cand_off[u], sizeof(uint32_t) * q->counts[u], hipMemcpyDeviceToHost));
+    return SGM_OK;
+}
+
+int sgm_cand_exist_download(sgm_query* q, uint32_t* words, uint32_t num_words)
+{
+    if (!q || !words) return SGM_EINVAL;
+    sgm_ctx* c = q->g->ctx;
+    if (!q->filtered) return set_err(c, SGM_ESTATE, "filter the query first");
+    if (num_words < q->g->V) return set_err(c, SGM_EINVAL, "need %u words", q->g->V);
+    HIPCHK(c, hipMemcpy(words, q->d_vmask, sizeof(uint32_t) * q->g->V, hipMemcpyDeviceToHost));
+    return SGM_OK;
+}
+
+// ------------------------------------------------------------------------------ tables
+int sgm_build_tables(sgm_query* q)
+{
+    if (!q) return SGM_EINVAL;
+    sgm_ctx* c = q->g->ctx;
+    int rc = ensure_compacted(q);
+    if (rc) return rc;
+    const uint32_t n = q->q.n, W = (q->g->V + 63) / 64;
+    free_tables(q);
+    uint32_t* d_rowcnt = nullptr;
+    uint32_t* d_max = nullptr;
+    uint32_t maxc = 1;
+    for (uint32_t u = 0; u < n; u++) maxc = std::max(maxc, q->counts[u]);
+    HIPCHK(c, dalloc(&d_rowcnt, maxc));
+    HIPCHK(c, dalloc(&d_max, 1));
+    int out_rc = SGM_OK;
+    for (uint32_t a = 0; a < n && out_rc == SGM_OK; a++) {
+        for (uint32_t p = q->q.off[a]; p < q->q.off[a + 1]; p++) {
+            const uint32_t b = q->q.adj[p];
+            sgm_table& t = q->tables[(size_t)a * n + b];
+            const uint32_t rows = q->counts[a];
+            hipError_t e = dalloc(&t.d_off, (size_t)rows + 1);
+            if (e != hipSuccess) { out_rc = set_err(c, SGM_ENOMEM, "table offsets"); break; }
+            sgm::launch_table(c->stream, q->g->d_off, q->g->d_adj, q->d_vmask, q->d_list + q->cand_off[a], rows, b,
+                              nullptr, nullptr, d_rowcnt, t.d_off, d_max, nullptr, nullptr, 0);
+            uint32_t tot = 0, mx = 0;
+            e = hipMemcpyAsync(&tot, t.d_off + rows, sizeof tot, hipMemcpyDeviceToHost, c->stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(&mx, d_max, sizeof mx, hipMemcpyDeviceToHost, c->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+            if (e != hipSuccess) { out_rc = set_err(c, SGM_EHIP, "table count: %s", hipGetErrorString(e)); break; }
+            t.e_cnt = tot;
+            t.max_deg = mx;
+            e = dalloc(&t.d_idx, tot);
+            if (e == hipSuccess) e = dalloc(&t.d_v, tot);
+            if (e != hipSuccess) { out_rc = set_err(c, SGM_ENOMEM, "table of %llu entries", (unsigned long long)tot); break; }
+            sgm::launch_table(c->stream, q->g->d_off, q->g->d_adj, q->d_vmask, q->d_list + q->cand_off[a], rows, b,
+                              q->d_bm + (size_t)b * W, q->d_rank + (size_t)b * W, d_rowcnt, t.d_off, d_max, t.d_idx,
+                              t.d_v, 1);
+            t.built = true;
+        }
+    }
+    hipError_t se = hipStreamSynchronize(c->stream);
+    dfree(d_rowcnt);
+    dfree(d_max);
+    if (out_rc) { free_tables(q); return out_rc; }
+    if (se != hipSuccess) { free_tables(q); return set_err(c, SGM_EHIP, "tables: %s", hipGetErrorString(se)); }
+    return SGM_OK;
+}
+
+int sgm_table_info(sgm_query* q, uint32_t a, uint32_t b, uint64_t* e_cnt, uint32_t* max_degree)
+{
+    if (!q || a >= q->q.n || b >= q->q.n) return SGM_EINVAL;
+    const sgm_table& t = q->tables[(size_t)a * q->q.n + b];
+    if (!t.built) return set_err(q->g->ctx, SGM_ESTATE, "no table for (%u,%u)", a, b);
+    if (e_cnt) *e_cnt = t.e_cnt;
+    if (max_degree) *max_degree = t.max_deg;
+    return SGM_OK;
+}
+
+int sgm_table_download(sgm_query* q, uint32_t a, uint32_t b, uint32_t* offsets, uint32_t* idx, uint32_t* vtx)
+{
+    if (!q || a >= q->q.n || b >= q->q.n) return SGM_EINVAL;
+    sgm_ctx* c = q->g->ctx;
+    const sgm_table& t = q->tables[(size_t)a * q->q.n + b];
+    if (!t.built) return set_err(c, SGM_ESTATE, "no table for (%u,%u)", a, b);
+    if (offsets) HIPCHK(c, hipMemcpy(offsets, t.d_off, sizeof(uint32_t) * (q->counts[a] + 1), hipMemcpyDeviceToHost));
+    if (idx && t.e_cnt) HIPCHK(c, hipMemcpy(idx, t.d_idx, sizeof(uint32_t) * t.e_cnt, hipMemcpyDeviceToHost));
+    if (vtx && t.e_cnt) HIPCHK(c, hipMemcpy(vtx, t.d_v, sizeof(uint32_t) * t.e_cnt, hipMemcpyDeviceToHost));
+    return SGM_OK;
+}
+
+// ------------------------------------------------------------------------------ orders
+static int csr_from(const uint32_t* qoff, const uint32_t* qadj, uint32_t n, sgm::QueryCSR& q)
+{
+    if (!qoff || n < 1 || n > SGM_MAXQ) return SGM_EINVAL;
+    q.n = n;
+    q.off.assign(qoff, qoff + n + 1);
+    q.adj.assign(qadj, qadj + qoff[n]);
+    return SGM_OK;
+}
+
+int sgm_order_ri(const uint32_t* qoff, const uint32_t* qadj, uint32_t n, uint32_t* order)
+{
+    sgm::QueryCSR q;
+    if (!order || csr_from(qoff, qadj, n, q)) return SGM_EINVAL;
+    sgm::order_ri(q, order);
+    return SGM_OK;
+}
+
+int sgm_order_gql(const sgm_query* q, uint32_t* order)
+{
+    if (!q || !order) return SGM_EINVAL;
+    if (!q->filtered) return set_err(q->g->ctx, SGM_ESTATE, "filter the query first");
+    sgm::order_gql(q->q, q->counts, q->g->V, order);
+    return SGM_OK;
+}
+
+int sgm_plan_export(const uint32_t* qoff, const uint32_t* qadj, uint32_t n, const uint32_t* order, uint32_t* n_ops,
+                    uint32_t* depth, uint32_t* parent, uint32_t* vdepth, uint32_t* emask, uint32_t* iter_set)
+{
+    sgm::QueryCSR q;
+    if (!order || !n_ops || csr_from(qoff, qadj, n, q)) return SGM_EINVAL;
+    sgm::Plan pl;
+    const int rc = sgm::build_plan(q, order, pl);
+    if (rc == -2) return SGM_EORDER;
+    if (rc) return SGM_EINVAL;
+    *n_ops = (uint32_t)pl.ops.size();
+    for (size_t i = 0; i < pl.ops.size(); i++) {
+        if (depth) depth[i] = pl.ops[i].depth;
+        if (parent) parent[i] = pl.ops[i].parent;
+        if (vdepth) vdepth[i] = pl.ops[i].vdepth;
+        if (emask) {
+            uint32_t m = 0;
+            for (uint32_t x : pl.ops[i].edges) m |= 1u << x;
+            emask[i] = m;
+        }
+    }
+    if (iter_set)
+        for (uint32_t d = 0; d < n; d++) iter_set[d] = pl.iter_set[d];
+    return SGM_OK;
+}
+
+// ------------------------------------------------------------------------------ enumerate
+static int ensure_stack(sgm_ctx* c, size_t bytes)
+{
+    if (c->stack_bytes >= bytes) return SGM_OK;
+    dfree(c->d_stack);
+    c->d_stack = nullptr;
+    c->stack_bytes = 0;
+    HIPCHK(c, hipMalloc((void**)&c->d_stack, bytes));
+    c->stack_bytes = bytes;
+    return SGM_OK;
+}
+
+int sgm_enumerate(sgm_query* q, const uint32_t* order, const sgm_enum_opts* opts, sgm_stats* out)
+{
+    if (!q || !order || !out) return SGM_EINVAL;
+    sgm_graph* g = q->g;
+    sgm_ctx* c = g->ctx;
+    std::memset(out, 0, sizeof *out);
+    sgm_enum_opts o;
+    std::memset(&o, 0, sizeof o);
+    o.cap = UINT64_MAX;
+    if (opts) o = *opts;
+    if (o.cap == 0) o.cap = 1;  // the reference stops at the first embedding when ans >= 0
+    if (!q->filtered) return set_err(c, SGM_ESTATE, "filter the query first");
+    const uint32_t n = q->q.n;
+    if (n < 2) return set_err(c, SGM_EINVAL, "enumeration needs at least 2 query vertices");
+    if (o.method != SGM_ENUM_CACHE_FILTER_FS && o.method != SGM_ENUM_CACHE_FILTER)
+        return set_err(c, SGM_EINVAL, "enumeration method %d is not available", o.method);
+    sgm::Plan pl;
+    const int prc = sgm::build_plan(q->q, order, pl);
+    if (prc == -2) return set_err(c, SGM_EORDER, "order is not a connected permutation");
+    if (prc) return set_err(c, SGM_EINVAL, "plan construction failed (%d)", prc);
+    if (q->empty) return SGM_OK;  // the reference's driver reports 0 without enumerating
+    sgm_plan_dev pd;
+    sgm::plan_to_device(pl, pd);
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(c->d_plan, &pd, sizeof pd, hipMemcpyHostToDevice, c->stream));
+    const uint32_t u0 = order[0];
+    const uint32_t n_items = g->ge_cnt[std::min<uint32_t>(q->q.off[u0 + 1] - q->q.off[u0], 32)];
+    uint32_t blocks = o.blocks ? o.blocks : (uint32_t)c->num_cus * 8;
+    blocks = std::max<uint32_t>(1, std::min<uint32_t>(blocks, 1u << 16));
+    const uint32_t waves = blocks * 4;
+    uint64_t words = o.stack_words ? o.stack_words : (uint64_t)8 * g->max_deg + 1024;
+    const uint32_t method_fs = o.method == SGM_ENUM_CACHE_FILTER_FS ? 1 : 0;
+    for (uint32_t attempt = 0;; attempt++) {
+        if (words > 0x7FFFFFFFull) return set_err(c, SGM_EDEVICE, "set-op stack would exceed 2^31 words");
+        int rc = ensure_stack(c, (size_t)waves * words * sizeof(uint32_t));
+        if (rc) return rc;
+        HIPCHK(c, hipMemsetAsync(c->d_stats, 0, sizeof(sgm_stats_dev), c->stream));
+        HIPCHK(c, hipMemsetAsync(c->d_work, 0, sizeof(uint32_t), c->stream));
+        HIPCHK(c, hipMemsetAsync(c->d_stop, 0, sizeof(uint32_t), c->stream));
+        c->stop_requested = 0;
+        HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+        sgm::launch_enum_cache(c->stream, blocks, g->d_off, g->d_adj, q->d_vmask, g->d_bydeg, n_items, u0, c->d_plan,
+                               pd.n_ops, c->d_stack, (uint32_t)words, c->d_work, c->d_stats, c->d_stop,
+                               (unsigned long long)o.cap, (int)method_fs);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+        sgm_stats_dev st;
+        HIPCHK(c, hipMemcpyAsync(&st, c->d_stats, sizeof st, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        float ms = 0;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        out->kernel_ms += ms;
+        if (st.overflow) {
+            words *= 4;
+            out->retries++;
+            if (attempt >= 3) return set_err(c, SGM_EDEVICE, "set-op stack overflow after %u retries", attempt);
+            continue;
+        }
+        out->count = st.count < o.cap ? st.count : o.cap;
+        out->capped = st.count >= o.cap;
+        out->intersections = st.intersections;
+        for (uint32_t d = 0; d < SGM_MAXQ; d++) out->nodes[d] = st.nodes[d];
+        out->fs_prunes = st.fs_prunes;
+        out->elems_read = st.elems_read;
+        out->elems_out = st.elems_out;
+        out->elems_scanned = st.elems_scanned;
+        if (c->stop_requested && !out->capped) {
+            out->stopped = 1;
+            return set_err(c, SGM_ESTOPPED, "stopped by sgm_stop");
+        }
+        return SGM_OK;
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,64 @@
+// Internal layouts shared by the host driver and the gfx950 kernels.
+//
+// Device data layout (HBM):
+//   data graph    off[V+1] u32, adj[2E] u32 (each list sorted ascending), by_deg[V] u32
+//                 (vertex ids by degree, descending, ties by id), ge_cnt[33] u32 (number of
+//                 vertices with degree >= d, d = 0..32)
+//   per query     vmask[V] u32: bit u set iff data vertex v is in C(u) -- the reference's
+//                 candidates_exsist[u][v] (inc/api.h:15) transposed so that one load answers
+//                 "which query vertices may take v"; bm[n][W] u64 per-vertex bitmaps with
+//                 rank[n][W] u32 exclusive popcount prefixes (W = ceil(V/64)); cand lists
+//                 (ascending) with cand_off[n+1]; aux tables (CSR per directed query edge).
+#pragma once
+#include <stdint.h>
+
+#define SGM_MAXQ 32
+#define SGM_MAXOPS 512
+#define SGM_INVALID 0xFFFFFFFFu
+#define SGM_NOPARENT 0xFFFFu
+
+// Intersection-cache plan on the device (the reference's PlanIR, inc/code_gen.h:682-847).
+// Ops are sorted by depth, so the ops computed at depth d are the contiguous id range
+// [op_first[d], op_first[d+1]) -- the reference's setOps[d].
+struct sgm_plan_dev {
+    uint32_t n;
+    uint32_t n_ops;
+    uint32_t order[SGM_MAXQ];      // query vertex at each depth
+    uint32_t iter_set[SGM_MAXQ];   // op id iterated at depth d (SGM_INVALID at depth 0)
+    uint32_t op_first[SGM_MAXQ + 1];
+    uint64_t anc[SGM_MAXQ];        // DAF ancestor masks over depths (inc/prune.h:100-104)
+    uint16_t op_parent[SGM_MAXOPS];
+    uint8_t op_vdepth[SGM_MAXOPS];
+    uint8_t op_depth[SGM_MAXOPS];
+};
+
+// Aux-table plan (explore_Intersection_All): backward neighbours by order position and the
+// table id of each (p -> u) directed edge.
+struct sgm_auxplan_dev {
+    uint32_t n;
+    uint32_t order[SGM_MAXQ];
+    uint32_t nprev[SGM_MAXQ];                   // by depth
+    uint32_t prev_depth[SGM_MAXQ][SGM_MAXQ];    // depth of each backward neighbour
+    uint32_t prev_tab[SGM_MAXQ][SGM_MAXQ];      // table index of edge (prev -> order[d])
+};
+
+struct sgm_stats_dev {
+    unsigned long long count;
+    unsigned long long intersections;
+    unsigned long long nodes[SGM_MAXQ];
+    unsigned long long fs_prunes;
+    unsigned long long elems_read;      // list elements fed to intersections (|A|+|B|)
+    unsigned long long elems_out;       // intersection results (|A n B|), incl. the fused last level
+    unsigned long long elems_scanned;   // iteration-set elements classified (filter + injectivity)
+    unsigned long long overflow;        // per-wave stack overflows (result invalid if > 0)
+    unsigned long long items_done;
+};
+
+// Per-query description used by the batched filter kernels.
+struct sgm_qdesc {
+    uint32_t n;
+    uint32_t qdeg[SGM_MAXQ];
+    uint32_t nbr[SGM_MAXQ];          // neighbour bitmask of each query vertex
+    uint32_t* vmask;                 // V words
+    uint32_t* removed;               // [2*n] removals per phase (round*n + u)
+};

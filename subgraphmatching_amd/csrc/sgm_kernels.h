@@ -1,0 +1,35 @@
+// Launch wrappers of the gfx950 kernels (implemented in sgm_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "sgm_internal.h"
+
+// Per-query candidate-space descriptor for the bitmap / rank / compaction kernels.
+struct sgm_cdesc {
+    uint32_t n;
+    uint32_t* vmask;     // V
+    uint64_t* bm;        // n * W
+    uint32_t* rank;      // n * W
+    uint32_t* cnt;       // n (written by k_rank_scan)
+    uint32_t* cand_off;  // n+1 (host-provided prefix of cnt)
+    uint32_t* list;      // sum of cnt
+};
+
+namespace sgm {
+void launch_nlf(hipStream_t s, const uint32_t* off, uint32_t V, const sgm_qdesc* qs, uint32_t nq);
+void launch_gql_phase(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* by_deg,
+                      const uint32_t* ge_cnt, const sgm_qdesc* qs, uint32_t nq, uint32_t phase, uint32_t max_cand);
+void launch_bitmaps(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V);
+void launch_rank_scan(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V, uint32_t max_n);
+void launch_compact(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V);
+void launch_table(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* vmask,
+                  const uint32_t* rows, uint32_t nrows, uint32_t b, const uint64_t* bm_b, const uint32_t* rank_b,
+                  uint32_t* rowcnt, uint32_t* rowoff, uint32_t* maxv, uint32_t* eidx, uint32_t* ev, int pass);
+size_t enum_smem_bytes(uint32_t n_ops, uint32_t waves);
+void launch_enum_cache(hipStream_t s, uint32_t blocks, const uint32_t* off, const uint32_t* adj,
+                       const uint32_t* vmask, const uint32_t* items, uint32_t n_items, uint32_t root_u,
+                       const sgm_plan_dev* plan, uint32_t n_ops, uint32_t* stack, uint32_t stack_words,
+                       uint32_t* work, sgm_stats_dev* st, uint32_t* stop, unsigned long long cap, int use_fs);
+}  // namespace sgm
