@@ -23,6 +23,7 @@ struct sgm_ctx {
     uint32_t* d_work = nullptr;      // work counter
     sgm_stats_dev* d_stats = nullptr;
     sgm_plan_dev* d_plan = nullptr;
+    sgm_auxplan_dev* d_auxplan = nullptr;
     uint32_t* d_stack = nullptr;
     size_t stack_bytes = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -139,6 +140,7 @@ int sgm_ctx_create(int device, sgm_ctx** out)
         hipStreamCreateWithFlags(&c->ctl, hipStreamNonBlocking) != hipSuccess ||
         dalloc(&c->d_stop, 1) != hipSuccess || dalloc(&c->d_work, 1) != hipSuccess ||
         dalloc(&c->d_stats, 1) != hipSuccess || dalloc(&c->d_plan, 1) != hipSuccess ||
+        dalloc(&c->d_auxplan, 1) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipEventCreate(&c->fev[0]) != hipSuccess || hipEventCreate(&c->fev[1]) != hipSuccess ||
         hipEventCreate(&c->fev[2]) != hipSuccess || hipEventCreate(&c->fev[3]) != hipSuccess) {
@@ -158,6 +160,7 @@ void sgm_ctx_destroy(sgm_ctx* c)
     dfree(c->d_work);
     dfree(c->d_stats);
     dfree(c->d_plan);
+    dfree(c->d_auxplan);
     dfree(c->d_stack);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -633,19 +636,54 @@ int sgm_enumerate(sgm_query* q, const uint32_t* order, const sgm_enum_opts* opts
     if (!q->filtered) return set_err(c, SGM_ESTATE, "filter the query first");
     const uint32_t n = q->q.n;
     if (n < 2) return set_err(c, SGM_EINVAL, "enumeration needs at least 2 query vertices");
-    if (o.method != SGM_ENUM_CACHE_FILTER_FS && o.method != SGM_ENUM_CACHE_FILTER)
-        return set_err(c, SGM_EINVAL, "enumeration method %d is not available", o.method);
+    if (o.method != SGM_ENUM_CACHE_FILTER_FS && o.method != SGM_ENUM_CACHE_FILTER && o.method != SGM_ENUM_AUX_TABLES)
+        return set_err(c, SGM_EINVAL, "unknown enumeration method %d", o.method);
     sgm::Plan pl;
     const int prc = sgm::build_plan(q->q, order, pl);
     if (prc == -2) return set_err(c, SGM_EORDER, "order is not a connected permutation");
     if (prc) return set_err(c, SGM_EINVAL, "plan construction failed (%d)", prc);
     if (q->empty) return SGM_OK;  // the reference's driver reports 0 without enumerating
+    const bool aux = o.method == SGM_ENUM_AUX_TABLES;
     sgm_plan_dev pd;
     sgm::plan_to_device(pl, pd);
     HIPCHK(c, hipSetDevice(c->device));
-    HIPCHK(c, hipMemcpyAsync(c->d_plan, &pd, sizeof pd, hipMemcpyHostToDevice, c->stream));
     const uint32_t u0 = order[0];
-    const uint32_t n_items = g->ge_cnt[std::min<uint32_t>(q->q.off[u0 + 1] - q->q.off[u0], 32)];
+    uint32_t n_items = g->ge_cnt[std::min<uint32_t>(q->q.off[u0 + 1] - q->q.off[u0], 32)];
+    if (aux) {
+        bool have = true;
+        for (uint32_t a = 0; a < n && have; a++)
+            for (uint32_t p = q->q.off[a]; p < q->q.off[a + 1]; p++) have &= q->tables[(size_t)a * n + q->q.adj[p]].built;
+        if (!have) {
+            int trc = sgm_build_tables(q);
+            if (trc) return trc;
+        }
+        std::vector<std::vector<uint32_t>> prev;
+        sgm::previous_neighbours(q->q, order, prev);
+        std::vector<uint32_t> pos(n);
+        for (uint32_t i = 0; i < n; i++) pos[order[i]] = i;
+        sgm_auxplan_dev ap;
+        std::memset(&ap, 0, sizeof ap);
+        ap.n = n;
+        uint32_t e = 0;
+        for (uint32_t d = 0; d < n; d++) {
+            const uint32_t u = order[d];
+            ap.order[d] = u;
+            ap.prev_first[d] = e;
+            ap.cand[d] = q->d_list + q->cand_off[u];
+            for (uint32_t p : prev[u]) {
+                const sgm_table& t = q->tables[(size_t)p * n + u];
+                ap.prev_depth[e] = pos[p];
+                ap.tab_off[e] = t.d_off;
+                ap.tab_idx[e] = t.d_idx;
+                e++;
+            }
+        }
+        ap.prev_first[n] = e;
+        HIPCHK(c, hipMemcpyAsync(c->d_auxplan, &ap, sizeof ap, hipMemcpyHostToDevice, c->stream));
+        n_items = q->counts[u0];
+    } else {
+        HIPCHK(c, hipMemcpyAsync(c->d_plan, &pd, sizeof pd, hipMemcpyHostToDevice, c->stream));
+    }
     uint32_t blocks = o.blocks ? o.blocks : (uint32_t)c->num_cus * 8;
     blocks = std::max<uint32_t>(1, std::min<uint32_t>(blocks, 1u << 16));
     const uint32_t waves = blocks * 4;
@@ -660,9 +698,13 @@ int sgm_enumerate(sgm_query* q, const uint32_t* order, const sgm_enum_opts* opts
         HIPCHK(c, hipMemsetAsync(c->d_stop, 0, sizeof(uint32_t), c->stream));
         c->stop_requested = 0;
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-        sgm::launch_enum_cache(c->stream, blocks, g->d_off, g->d_adj, q->d_vmask, g->d_bydeg, n_items, u0, c->d_plan,
-                               pd.n_ops, c->d_stack, (uint32_t)words, c->d_work, c->d_stats, c->d_stop,
-                               (unsigned long long)o.cap, (int)method_fs);
+        if (aux)
+            sgm::launch_enum_aux(c->stream, blocks, c->d_auxplan, n_items, c->d_stack, (uint32_t)words, c->d_work,
+                                 c->d_stats, c->d_stop, (unsigned long long)o.cap);
+        else
+            sgm::launch_enum_cache(c->stream, blocks, g->d_off, g->d_adj, q->d_vmask, g->d_bydeg, n_items, u0,
+                                   c->d_plan, pd.n_ops, c->d_stack, (uint32_t)words, c->d_work, c->d_stats, c->d_stop,
+                                   (unsigned long long)o.cap, (int)method_fs);
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
         sgm_stats_dev st;

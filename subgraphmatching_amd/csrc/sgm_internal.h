@@ -32,14 +32,19 @@ struct sgm_plan_dev {
     uint8_t op_depth[SGM_MAXOPS];
 };
 
-// Aux-table plan (explore_Intersection_All): backward neighbours by order position and the
-// table id of each (p -> u) directed edge.
+// Aux-table plan (explore_Intersection_All, inc/explore.h:2610-2848).  For depth d the
+// backward neighbours (in order position, GetPreviousNeb) are entries
+// [prev_first[d], prev_first[d+1]); entry e names the neighbour's depth and the device
+// table (offsets over C(prev), candidate positions in C(order[d])) of that directed edge.
+#define SGM_MAXPREV 512
 struct sgm_auxplan_dev {
     uint32_t n;
     uint32_t order[SGM_MAXQ];
-    uint32_t nprev[SGM_MAXQ];                   // by depth
-    uint32_t prev_depth[SGM_MAXQ][SGM_MAXQ];    // depth of each backward neighbour
-    uint32_t prev_tab[SGM_MAXQ][SGM_MAXQ];      // table index of edge (prev -> order[d])
+    uint32_t prev_first[SGM_MAXQ + 1];
+    uint32_t prev_depth[SGM_MAXPREV];
+    const uint32_t* tab_off[SGM_MAXPREV];
+    const uint32_t* tab_idx[SGM_MAXPREV];
+    const uint32_t* cand[SGM_MAXQ];   // candidate list of the query vertex at each depth
 };
 
 struct sgm_stats_dev {

@@ -28,6 +28,8 @@ void launch_table(hipStream_t s, const uint32_t* off, const uint32_t* adj, const
                   const uint32_t* rows, uint32_t nrows, uint32_t b, const uint64_t* bm_b, const uint32_t* rank_b,
                   uint32_t* rowcnt, uint32_t* rowoff, uint32_t* maxv, uint32_t* eidx, uint32_t* ev, int pass);
 size_t enum_smem_bytes(uint32_t n_ops, uint32_t waves);
+void launch_enum_aux(hipStream_t s, uint32_t blocks, const sgm_auxplan_dev* plan, uint32_t n_items, uint32_t* stack,
+                     uint32_t stack_words, uint32_t* work, sgm_stats_dev* st, uint32_t* stop, unsigned long long cap);
 void launch_enum_cache(hipStream_t s, uint32_t blocks, const uint32_t* off, const uint32_t* adj,
                        const uint32_t* vmask, const uint32_t* items, uint32_t n_items, uint32_t root_u,
                        const sgm_plan_dev* plan, uint32_t n_ops, uint32_t* stack, uint32_t stack_words,

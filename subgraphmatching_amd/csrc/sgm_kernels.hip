@@ -416,6 +416,23 @@ __device__ uint32_t isect_write(const uint32_t* __restrict__ a, uint32_t na, con
     return cnt;
 }
 
+// A wave publishes its embedding count at least every FLUSH_AT embeddings (and at the end
+// of each depth-0 item), so the cap can stop the whole grid while waves are still deep
+// inside huge subtrees; the stop word is raised once the global count reaches the cap,
+// so the final total is >= cap whenever the cap stopped the search.
+#define FLUSH_AT 4096ull
+
+__device__ __forceinline__ void flush_count(unsigned long long& pending, unsigned long long* gcount, uint32_t* stop,
+                                            unsigned long long cap)
+{
+    if (!pending) return;
+    unsigned long long prev = 0;
+    if (lane_id() == 0) prev = atomicAdd(gcount, pending);
+    prev = __shfl(prev, 0);
+    if (prev + pending >= cap && lane_id() == 0) atomicExch(stop, 1u);
+    pending = 0;
+}
+
 struct LastRes {
     uint32_t size;      // iteration-set size
     uint32_t ok;        // extensions (embeddings)
@@ -508,7 +525,7 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
     const uint32_t uL = P.order[L];
 
     for (uint32_t d = 0; d < SGM_MAXQ; d++) S.nodes[d] = 0;
-    unsigned long long cnt_total = 0, inter = 0, fsp = 0, elems = 0, items_done = 0, outs = 0, scanned = 0;
+    unsigned long long inter = 0, fsp = 0, elems = 0, items_done = 0, outs = 0, scanned = 0;
     unsigned long long cnt_pending = 0;
     bool overflow = false;
     uint32_t poll = 0;
@@ -522,7 +539,6 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
         const uint32_t v0 = items[it];
         if (!((vmask[v0] >> root_u) & 1u)) continue;  // not (or no longer) a candidate of u0
         items_done++;
-        unsigned long long cnt_item = 0;
         S.emb[0] = v0;
         S.nodes[0] += 1;
         S.stack_top[0] = 0;
@@ -536,7 +552,7 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
             op_size[o] = off[v0 + 1] - off[v0];
             last_scan(adj + off[v0], op_size[o], vmask, uL, S, P.anc, L, r);
             scanned += r.size;
-            cnt_item += r.ok;
+            cnt_pending += r.ok;
             S.nodes[L] += r.ok;
         } else {
             bool ok = true;
@@ -566,7 +582,7 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
                 S.fail[0] = 0;
                 uint32_t d = 1;
                 while (true) {
-                    if ((++poll & 1023u) == 0 && __atomic_load_n(stop, __ATOMIC_RELAXED)) { aborted = true; break; }
+                    if ((++poll & 63u) == 0 && __atomic_load_n(stop, __ATOMIC_RELAXED)) { aborted = true; break; }
                     if (S.rem[d] == 0 && ((S.flags[d] & F_PRUNED) || S.pos[d] >= S.tot[d])) {
                         // end of the loop at depth d (inc/explore.h:1068-1070)
                         if (use_fs && !(S.flags[d] & (F_PRUNED | F_ANY))) S.fail[d - 1] = P.anc[d];
@@ -637,7 +653,8 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
                             const uint32_t o = P.iter_set[L];
                             last_scan(op_list(op_off[o], adj, ws), op_size[o], vmask, uL, S, P.anc, L, r);
                         }
-                        cnt_item += r.ok;
+                        cnt_pending += r.ok;
+                        if (cnt_pending >= FLUSH_AT) flush_count(cnt_pending, &gst->count, stop, cap);
                         S.nodes[L] += r.ok;
                         scanned += r.size;
                         if (P.op_first[d] < P.op_first[d + 1] && P.op_parent[P.op_first[d]] != SGM_NOPARENT)
@@ -703,20 +720,11 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
                 }
             }
         }
-        cnt_total += cnt_item;
-        cnt_pending += cnt_item;
-        if (aborted && !overflow) break;
-        if (cnt_pending) {
-            unsigned long long prev = 0;
-            if (lane == 0) prev = atomicAdd(&gst->count, cnt_pending);
-            prev = __shfl(prev, 0);
-            if (prev + cnt_pending >= cap && lane == 0) atomicExch(stop, 1u);
-            cnt_pending = 0;
-        }
-        if (overflow) break;
+        flush_count(cnt_pending, &gst->count, stop, cap);
+        if (aborted || overflow) break;
     }
+    flush_count(cnt_pending, &gst->count, stop, cap);
     if (lane == 0) {
-        if (cnt_pending) atomicAdd(&gst->count, cnt_pending);
         atomicAdd(&gst->intersections, inter);
         atomicAdd(&gst->fs_prunes, fsp);
         atomicAdd(&gst->elems_read, elems);
@@ -727,7 +735,219 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
         for (uint32_t d = 0; d < n; d++)
             if (S.nodes[d]) atomicAdd(&gst->nodes[d], S.nodes[d]);
     }
-    (void)cnt_total;
+}
+
+
+// ================================================================== aux-table enumeration
+// Plain candidate-space search over the auxiliary tables (inc/explore.h:2610-2848 with every
+// pruning option off; CandidatesBuffer::getNextCandidatesImpl :284-354): the candidates of
+// depth d are the intersection, over the backward neighbours p of order[d] in order
+// position, of the table rows edge_matrix[p][order[d]].getNeb(idx(p)) -- positions in
+// C(order[d]), ascending; each extra backward neighbour is one intersection (counted even
+// when an earlier one came out empty, as the reference does).  Injectivity is the only
+// check.  One wave per depth-0 candidate; the last level is intersected and counted in one
+// pass.  Per-wave intermediate lists live on the same HBM stack as the cache search.
+struct WaveAux {
+    uint64_t rem[SGM_MAXQ];
+    unsigned long long nodes[SGM_MAXQ];
+    const uint32_t* lst[SGM_MAXQ];
+    uint32_t emb[SGM_MAXQ];
+    uint32_t iemb[SGM_MAXQ];
+    uint32_t pos[SGM_MAXQ];
+    uint32_t tot[SGM_MAXQ];
+    uint32_t chunk_base[SGM_MAXQ];
+    uint32_t stack_top[SGM_MAXQ + 1];
+};
+
+// Build the candidate list of depth d into (*list, *size); all but the last intersection
+// are materialised on the stack from *top; when `leave_last` is set the last backward
+// neighbour's row is returned separately (for a fused count) in (*last, *nlast).
+__device__ bool aux_list(const sgm_auxplan_dev& A, const WaveAux& S, uint32_t d, uint32_t* ws, uint32_t stack_words,
+                         uint32_t& top, const uint32_t*& list, uint32_t& size, bool leave_last,
+                         const uint32_t*& last, uint32_t& nlast, unsigned long long& elems, unsigned long long& outs)
+{
+    const uint32_t f = A.prev_first[d], e_end = A.prev_first[d + 1];
+    const uint32_t np = e_end - f;
+    {
+        const uint32_t* o = A.tab_off[f];
+        const uint32_t r = S.iemb[A.prev_depth[f]];
+        const uint32_t b = o[r];
+        list = A.tab_idx[f] + b;
+        size = o[r + 1] - b;
+    }
+    const uint32_t stop_at = leave_last ? e_end - 1 : e_end;
+    for (uint32_t e = f + 1; e < stop_at; e++) {
+        const uint32_t* o = A.tab_off[e];
+        const uint32_t r = S.iemb[A.prev_depth[e]];
+        const uint32_t b = o[r], nb = o[r + 1] - b;
+        const uint32_t need = size < nb ? size : nb;
+        if (top + need > stack_words) return false;
+        const uint32_t sz = isect_write(list, size, A.tab_idx[e] + b, nb, ws + top, elems, outs);
+        list = ws + top;
+        size = sz;
+        top += sz;
+    }
+    if (leave_last && np > 1) {
+        const uint32_t e = e_end - 1;
+        const uint32_t* o = A.tab_off[e];
+        const uint32_t r = S.iemb[A.prev_depth[e]];
+        const uint32_t b = o[r];
+        last = A.tab_idx[e] + b;
+        nlast = o[r + 1] - b;
+    } else {
+        last = nullptr;
+        nlast = 0;
+    }
+    return true;
+}
+
+// count positions whose vertex is not used by depths < L
+__device__ __forceinline__ uint32_t aux_count_chunk(bool valid, uint32_t vidx, const uint32_t* __restrict__ cand,
+                                                    const WaveAux& S, uint32_t L)
+{
+    bool ok = false;
+    if (valid) {
+        const uint32_t v = cand[vidx];
+        ok = true;
+        for (uint32_t j = 0; j < L; j++)
+            if (S.emb[j] == v) ok = false;
+    }
+    return (uint32_t)__popcll(__ballot(ok));
+}
+
+__global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restrict__ gA, uint32_t n_items,
+                                                  uint32_t* __restrict__ stack, uint32_t stack_words,
+                                                  uint32_t* __restrict__ work, sgm_stats_dev* __restrict__ gst,
+                                                  uint32_t* __restrict__ stop, unsigned long long cap)
+{
+    extern __shared__ uint64_t smem_u64[];
+    uint8_t* smem = reinterpret_cast<uint8_t*>(smem_u64);
+    const sgm_auxplan_dev& A = *gA;
+    const uint32_t n = A.n;
+    const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
+    WaveAux& S = *reinterpret_cast<WaveAux*>(smem + wave * ((sizeof(WaveAux) + 15) & ~(size_t)15));
+    uint32_t* ws = stack + (size_t)(blockIdx.x * (blockDim.x >> 6) + wave) * stack_words;
+    const uint32_t L = n - 1;
+    for (uint32_t d = 0; d < SGM_MAXQ; d++) S.nodes[d] = 0;
+    unsigned long long inter = 0, elems = 0, outs = 0, scanned = 0, cnt_pending = 0, items_done = 0;
+    bool overflow = false;
+    uint32_t poll = 0;
+    while (true) {
+        uint32_t it = 0;
+        if (lane == 0) it = atomicAdd(work, 1u);
+        it = uni(__shfl((int)it, 0));
+        if (it >= n_items) break;
+        if (__atomic_load_n(stop, __ATOMIC_RELAXED)) break;
+        items_done++;
+        bool aborted = false;
+        S.emb[0] = A.cand[0][it];
+        S.iemb[0] = it;
+        S.nodes[0] += 1;
+        S.stack_top[1] = 0;
+        uint32_t d = 1;
+        // enter depth d (d <= L): build the list; at d == L count directly
+        bool entering = true;
+        while (true) {
+            if ((++poll & 63u) == 0 && __atomic_load_n(stop, __ATOMIC_RELAXED)) { aborted = true; break; }
+            if (entering) {
+                entering = false;
+                const uint32_t np = A.prev_first[d + 1] - A.prev_first[d];
+                inter += np - 1;
+                uint32_t top = S.stack_top[d];
+                const uint32_t* list;
+                uint32_t size;
+                const uint32_t* last;
+                uint32_t nlast;
+                const bool is_last = d == L;
+                if (!aux_list(A, S, d, ws, stack_words, top, list, size, is_last, last, nlast, elems, outs)) {
+                    overflow = true; aborted = true; break;
+                }
+                if (is_last) {
+                    const uint32_t* cand = A.cand[d];
+                    uint32_t ok = 0;
+                    if (!last) {
+                        scanned += size;
+                        for (uint32_t base = 0; base < size; base += 64) {
+                            const uint32_t i = base + lane;
+                            const bool valid = i < size;
+                            ok += aux_count_chunk(valid, valid ? list[i] : 0u, cand, S, L);
+                        }
+                    } else {
+                        const uint32_t* a = list; uint32_t na = size;
+                        const uint32_t* b = last; uint32_t nb = nlast;
+                        if (na > nb) { const uint32_t* t = a; a = b; b = t; uint32_t tn = na; na = nb; nb = tn; }
+                        elems += na + nb;
+                        for (uint32_t base = 0; base < na; base += 64) {
+                            const uint32_t i = base + lane;
+                            const uint32_t key = i < na ? a[i] : 0u;
+                            const bool hit = i < na && lower_found(b, nb, key);
+                            const uint32_t h = (uint32_t)__popcll(__ballot(hit));
+                            outs += h;
+                            scanned += h;
+                            ok += aux_count_chunk(hit, key, cand, S, L);
+                        }
+                    }
+                    cnt_pending += ok;
+                    if (cnt_pending >= FLUSH_AT) flush_count(cnt_pending, &gst->count, stop, cap);
+                    S.nodes[L] += ok;
+                    d--;
+                    if (d == 0) break;
+                    continue;
+                }
+                S.stack_top[d + 1] = top;
+                S.lst[d] = list;
+                S.tot[d] = size;
+                S.pos[d] = 0;
+                S.rem[d] = 0;
+                continue;
+            }
+            if (S.rem[d] == 0 && S.pos[d] >= S.tot[d]) {
+                d--;
+                if (d == 0) break;
+                continue;
+            }
+            const uint32_t* list = S.lst[d];
+            if (S.rem[d] == 0) {
+                const uint32_t base = S.pos[d];
+                S.pos[d] = base + 64;
+                S.chunk_base[d] = base;
+                const uint32_t i = base + lane;
+                const bool valid = i < S.tot[d];
+                scanned += (S.tot[d] - base < 64u) ? (S.tot[d] - base) : 64u;
+                bool ok = false;
+                if (valid) {
+                    const uint32_t v = A.cand[d][list[i]];
+                    ok = true;
+                    for (uint32_t j = 0; j < d; j++)
+                        if (S.emb[j] == v) ok = false;
+                }
+                S.rem[d] = __ballot(ok);
+                continue;
+            }
+            const uint64_t rm = S.rem[d];
+            const uint32_t j = (uint32_t)__ffsll((long long)rm) - 1u;
+            S.rem[d] = rm & (rm - 1);
+            const uint32_t vidx = list[S.chunk_base[d] + j];
+            S.emb[d] = A.cand[d][vidx];
+            S.iemb[d] = vidx;
+            S.nodes[d] += 1;
+            d++;
+            entering = true;
+        }
+        flush_count(cnt_pending, &gst->count, stop, cap);
+        if (aborted || overflow) break;
+    }
+    flush_count(cnt_pending, &gst->count, stop, cap);
+    if (lane == 0) {
+        atomicAdd(&gst->intersections, inter);
+        atomicAdd(&gst->elems_read, elems);
+        atomicAdd(&gst->elems_out, outs);
+        atomicAdd(&gst->elems_scanned, scanned);
+        atomicAdd(&gst->items_done, items_done);
+        if (overflow) { atomicAdd(&gst->overflow, 1ull); atomicExch(stop, 1u); }
+        for (uint32_t dd = 0; dd < n; dd++)
+            if (S.nodes[dd]) atomicAdd(&gst->nodes[dd], S.nodes[dd]);
+    }
 }
 
 }  // namespace sgm
@@ -792,6 +1012,14 @@ size_t enum_smem_bytes(uint32_t n_ops, uint32_t waves)
     const size_t plan_bytes = (sizeof(sgm_plan_dev) + 15) & ~(size_t)15;
     const size_t wave_bytes = ((sizeof(WaveDFS) + 8u * n_ops) + 15) & ~(size_t)15;
     return plan_bytes + waves * wave_bytes;
+}
+
+void launch_enum_aux(hipStream_t s, uint32_t blocks, const sgm_auxplan_dev* plan, uint32_t n_items, uint32_t* stack,
+                     uint32_t stack_words, uint32_t* work, sgm_stats_dev* st, uint32_t* stop, unsigned long long cap)
+{
+    const size_t smem = 4 * ((sizeof(WaveAux) + 15) & ~(size_t)15);
+    hipLaunchKernelGGL(k_enum_aux, dim3(blocks), dim3(256), smem, s, plan, n_items, stack, stack_words, work, st, stop,
+                       cap);
 }
 
 void launch_enum_cache(hipStream_t s, uint32_t blocks, const uint32_t* off, const uint32_t* adj,

@@ -120,6 +120,12 @@ def _compare(g, q, order, cap=None, check_work=True):
             assert res.intersections == ref["intersections"], (q.name, fs)
             assert res.nodes == ref["nodes"], (q.name, fs, res.nodes, ref["nodes"])
             assert res.fs_prunes == ref["fs_prunes"], (q.name, fs)
+    res_a = sgm.enumerate_query(can, order, method=L.SGM_ENUM_AUX_TABLES, cap=cap)
+    ref_a = O.enum_all(g, q, c_r, order, cap=cap)
+    assert res_a.ans == ref_a["count"], (q.name, "aux", res_a.ans, ref_a["count"])
+    if check_work and not ref_a["capped"]:
+        assert res_a.intersections == ref_a["intersections"], (q.name, "aux")
+        assert res_a.nodes == ref_a["nodes"], (q.name, "aux", res_a.nodes, ref_a["nodes"])
     return res
 
 
