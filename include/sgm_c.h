@@ -53,8 +53,13 @@ typedef struct {
     int32_t method;        /* SGM_ENUM_* */
     uint32_t blocks;       /* persistent blocks of 4 waves; 0 = automatic */
     uint32_t stack_words;  /* per-wave set-op stack (u32 words); 0 = automatic */
-    uint32_t reserved[5];
+    uint32_t flags;        /* SGM_ENUM_F_* */
+    uint32_t reserved[4];
 } sgm_enum_opts;
+
+/* sgm_enum_opts.flags: intersect against hub adjacency lists by binary search instead of
+ * their bitmaps (same results; for measurement) */
+#define SGM_ENUM_F_NO_BITMAPS 1u
 
 typedef struct {
     uint64_t count;          /* min(#embeddings, cap) */

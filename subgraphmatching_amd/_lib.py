@@ -47,9 +47,12 @@ EXPORTS = [
 ]
 
 
+SGM_ENUM_F_NO_BITMAPS = 1
+
+
 class EnumOpts(C.Structure):
     _fields_ = [("cap", C.c_uint64), ("method", C.c_int32), ("blocks", C.c_uint32),
-                ("stack_words", C.c_uint32), ("reserved", C.c_uint32 * 5)]
+                ("stack_words", C.c_uint32), ("flags", C.c_uint32), ("reserved", C.c_uint32 * 4)]
 
 
 class Stats(C.Structure):

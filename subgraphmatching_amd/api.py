@@ -366,12 +366,13 @@ class ExploreResult:
 
 
 def enumerate_query(can: CandidateParam, order, method=L.SGM_ENUM_CACHE_FILTER_FS, cap=None, blocks=0,
-                    stack_words=0) -> ExploreResult:
+                    stack_words=0, flags=0) -> ExploreResult:
     opts = L.EnumOpts()
     opts.cap = (1 << 64) - 1 if cap is None else int(cap)
     opts.method = method
     opts.blocks = blocks
     opts.stack_words = stack_words
+    opts.flags = flags
     st = L.Stats()
     o = L.u32(order)
     _check(L.lib().sgm_enumerate(can._q, L.ptr(o), C.byref(opts), C.byref(st)), can.ctx)

@@ -43,7 +43,16 @@ struct sgm_graph {
     uint32_t* d_bydeg = nullptr;
     uint32_t* d_gecnt = nullptr;
     uint32_t ge_cnt[33] = {0};
+    // adjacency bitmaps of the highest-degree vertices (degree >= kHubMinDegree, within
+    // kHubBudgetBytes); hub_slot[v] = bitmap index or SGM_INVALID
+    uint32_t n_hubs = 0;
+    uint32_t hub_words = 0;
+    uint32_t* d_hub_slot = nullptr;
+    uint32_t* d_hub_bits = nullptr;
 };
+
+static const uint32_t kHubMinDegree = 64;
+static const uint64_t kHubBudgetBytes = 1ull << 30;
 
 struct sgm_table {
     bool built = false;
@@ -240,6 +249,27 @@ int sgm_graph_upload(sgm_ctx* c, const uint32_t* off, const uint32_t* adj, uint3
     if (e == hipSuccess && adj_len) e = hipMemcpy(g->d_adj, adj, sizeof(uint32_t) * adj_len, hipMemcpyHostToDevice);
     if (e == hipSuccess && V) e = hipMemcpy(g->d_bydeg, bydeg.data(), sizeof(uint32_t) * V, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(g->d_gecnt, g->ge_cnt, sizeof(g->ge_cnt), hipMemcpyHostToDevice);
+    // hub bitmaps: the highest-degree vertices first (by_deg order), as many as fit
+    if (e == hipSuccess && V) {
+        g->hub_words = (V + 31) / 32;
+        const uint64_t per = (uint64_t)g->hub_words * sizeof(uint32_t);
+        uint64_t h = 0;
+        while (h < V && off[bydeg[h] + 1] - off[bydeg[h]] >= kHubMinDegree && (h + 1) * per <= kHubBudgetBytes) h++;
+        g->n_hubs = (uint32_t)h;
+        std::vector<uint32_t> slot(V, SGM_INVALID);
+        for (uint32_t i = 0; i < g->n_hubs; i++) slot[bydeg[i]] = i;
+        e = dalloc(&g->d_hub_slot, V);
+        if (e == hipSuccess) e = hipMemcpy(g->d_hub_slot, slot.data(), sizeof(uint32_t) * V, hipMemcpyHostToDevice);
+        if (e == hipSuccess && g->n_hubs) {
+            e = dalloc(&g->d_hub_bits, (size_t)g->n_hubs * g->hub_words);
+            if (e == hipSuccess) e = hipMemset(g->d_hub_bits, 0, (size_t)g->n_hubs * per);
+            if (e == hipSuccess) {
+                sgm::launch_hub_bits(nullptr, g->d_off, g->d_adj, g->d_bydeg, g->n_hubs, g->d_hub_bits, g->hub_words);
+                e = hipGetLastError();
+            }
+            if (e == hipSuccess) e = hipDeviceSynchronize();
+        }
+    }
     if (e != hipSuccess) {
         sgm_graph_destroy(g);
         return set_err(c, e == hipErrorOutOfMemory ? SGM_ENOMEM : SGM_EHIP, "graph upload: %s", hipGetErrorString(e));
@@ -255,6 +285,8 @@ void sgm_graph_destroy(sgm_graph* g)
     dfree(g->d_adj);
     dfree(g->d_bydeg);
     dfree(g->d_gecnt);
+    dfree(g->d_hub_slot);
+    dfree(g->d_hub_bits);
     delete g;
 }
 
@@ -704,7 +736,9 @@ int sgm_enumerate(sgm_query* q, const uint32_t* order, const sgm_enum_opts* opts
         else
             sgm::launch_enum_cache(c->stream, blocks, g->d_off, g->d_adj, q->d_vmask, g->d_bydeg, n_items, u0,
                                    c->d_plan, pd.n_ops, c->d_stack, (uint32_t)words, c->d_work, c->d_stats, c->d_stop,
-                                   (unsigned long long)o.cap, (int)method_fs);
+                                   (unsigned long long)o.cap, (int)method_fs,
+                                   (o.flags & SGM_ENUM_F_NO_BITMAPS) || !g->n_hubs ? nullptr : g->d_hub_slot,
+                                   g->d_hub_bits, g->hub_words);
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
         sgm_stats_dev st;

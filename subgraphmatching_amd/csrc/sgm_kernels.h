@@ -33,5 +33,8 @@ void launch_enum_aux(hipStream_t s, uint32_t blocks, const sgm_auxplan_dev* plan
 void launch_enum_cache(hipStream_t s, uint32_t blocks, const uint32_t* off, const uint32_t* adj,
                        const uint32_t* vmask, const uint32_t* items, uint32_t n_items, uint32_t root_u,
                        const sgm_plan_dev* plan, uint32_t n_ops, uint32_t* stack, uint32_t stack_words,
-                       uint32_t* work, sgm_stats_dev* st, uint32_t* stop, unsigned long long cap, int use_fs);
+                       uint32_t* work, sgm_stats_dev* st, uint32_t* stop, unsigned long long cap, int use_fs,
+                       const uint32_t* hub_slot, const uint32_t* hub_bits, uint32_t hub_words);
+void launch_hub_bits(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* hubs, uint32_t n_hubs,
+                     uint32_t* bits, uint32_t words);
 }  // namespace sgm

@@ -193,6 +193,24 @@ __global__ void __launch_bounds__(256) k_gql_phase(const uint32_t* __restrict__ 
     if (lane_id() == 0 && local_removed) atomicAdd(&removed[phase], local_removed);
 }
 
+// ================================================================== hub adjacency bitmaps
+// One V-bit bitmap per high-degree vertex (slot order = hubs[]), so intersections against
+// a hub's adjacency list test membership with one load instead of a binary search.
+// The bitmaps start zeroed; lanes of a block set the bits of one hub's neighbours.
+__global__ void __launch_bounds__(256) k_hub_bits(const uint32_t* __restrict__ off, const uint32_t* __restrict__ adj,
+                                                  const uint32_t* __restrict__ hubs, uint32_t n_hubs,
+                                                  uint32_t* __restrict__ bits, uint32_t words)
+{
+    for (uint32_t h = blockIdx.x; h < n_hubs; h += gridDim.x) {
+        const uint32_t v = hubs[h];
+        uint32_t* __restrict__ b = bits + (size_t)h * words;
+        for (uint32_t p = off[v] + threadIdx.x; p < off[v + 1]; p += blockDim.x) {
+            const uint32_t w = adj[p];
+            atomicOr(&b[w >> 5], 1u << (w & 31u));
+        }
+    }
+}
+
 // ================================================================== candidate bitmaps
 // Per query vertex u: bm[u][w] = 64-vertex bitmap of C(u) (from vmask bit u), then
 // rank[u][w] = exclusive popcount prefix and cnt[u] = |C(u)|.
@@ -394,20 +412,33 @@ __device__ __forceinline__ const uint32_t* op_list(uint32_t o_off, const uint32_
     return (o_off & STACK_FLAG) ? ws + (o_off & ~STACK_FLAG) : adj + o_off;
 }
 
-// Intersect sorted lists a and b into out (ascending); returns the size.  Lanes take 64
-// elements of the shorter list and binary-search the longer one.
-__device__ uint32_t isect_write(const uint32_t* __restrict__ a, uint32_t na, const uint32_t* __restrict__ b,
-                                uint32_t nb, uint32_t* __restrict__ out, unsigned long long& elems,
-                                unsigned long long& outs)
+// Membership of key in the sorted list b: one bit test when b is the adjacency list of a
+// vertex that has a bitmap (bits != nullptr), else a binary search.
+__device__ __forceinline__ bool member(const uint32_t* __restrict__ b, uint32_t nb, const uint32_t* __restrict__ bits,
+                                       uint32_t key)
 {
-    if (na > nb) { const uint32_t* t = a; a = b; b = t; uint32_t tn = na; na = nb; nb = tn; }
+    return bits ? ((bits[key >> 5] >> (key & 31u)) & 1u) != 0u : lower_found(b, nb, key);
+}
+
+// Intersect sorted lists a and b into out (ascending); returns the size.  Lanes take 64
+// elements of the shorter list and test them against the longer one (bitmap or binary
+// search).  abits/bbits: bitmaps of a / b when they are hub adjacency lists, else null.
+__device__ uint32_t isect_write(const uint32_t* __restrict__ a, uint32_t na, const uint32_t* __restrict__ abits,
+                                const uint32_t* __restrict__ b, uint32_t nb, const uint32_t* __restrict__ bbits,
+                                uint32_t* __restrict__ out, unsigned long long& elems, unsigned long long& outs)
+{
+    if (na > nb) {
+        const uint32_t* t = a; a = b; b = t;
+        uint32_t tn = na; na = nb; nb = tn;
+        bbits = abits;
+    }
     const uint32_t lane = lane_id();
     uint32_t cnt = 0;
     elems += na + nb;
     for (uint32_t base = 0; base < na; base += 64) {
         const uint32_t i = base + lane;
         const uint32_t key = i < na ? a[i] : 0u;
-        const bool hit = i < na && lower_found(b, nb, key);
+        const bool hit = i < na && member(b, nb, bbits, key);
         const uint64_t bal = __ballot(hit);
         if (hit) out[cnt + mbcnt64(bal)] = key;
         cnt += (uint32_t)__popcll(bal);
@@ -475,18 +506,23 @@ __device__ void last_scan(const uint32_t* __restrict__ list, uint32_t size, cons
 }
 
 // Last level, intersecting (parent op) with N(v) and classifying in the same pass.
-__device__ void last_isect(const uint32_t* __restrict__ a, uint32_t na, const uint32_t* __restrict__ b,
-                           uint32_t nb, const uint32_t* __restrict__ vmask, uint32_t uL, const WaveDFS& S,
+__device__ void last_isect(const uint32_t* __restrict__ a, uint32_t na, const uint32_t* __restrict__ abits,
+                           const uint32_t* __restrict__ b, uint32_t nb, const uint32_t* __restrict__ bbits,
+                           const uint32_t* __restrict__ vmask, uint32_t uL, const WaveDFS& S,
                            const uint64_t* anc, uint32_t L, LastRes& r, unsigned long long& elems)
 {
-    if (na > nb) { const uint32_t* t = a; a = b; b = t; uint32_t tn = na; na = nb; nb = tn; }
+    if (na > nb) {
+        const uint32_t* t = a; a = b; b = t;
+        uint32_t tn = na; na = nb; nb = tn;
+        bbits = abits;
+    }
     const uint32_t lane = lane_id();
     elems += na + nb;
     uint32_t size = 0;
     for (uint32_t base = 0; base < na; base += 64) {
         const uint32_t i = base + lane;
         const uint32_t key = i < na ? a[i] : 0u;
-        const bool hit = i < na && lower_found(b, nb, key);
+        const bool hit = i < na && member(b, nb, bbits, key);
         size += (uint32_t)__popcll(__ballot(hit));
         classify_last(hit, key, vmask, uL, S, anc, L, r);
     }
@@ -502,11 +538,18 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
                                                     uint32_t* __restrict__ stack, uint32_t stack_words,
                                                     uint32_t* __restrict__ work, sgm_stats_dev* __restrict__ gst,
                                                     uint32_t* __restrict__ stop, unsigned long long cap,
-                                                    int use_fs)
+                                                    int use_fs, const uint32_t* __restrict__ hub_slot,
+                                                    const uint32_t* __restrict__ hub_bits, uint32_t hub_words)
 {
     extern __shared__ uint64_t smem_u64[];
     uint8_t* smem = reinterpret_cast<uint8_t*>(smem_u64);
     sgm_plan_dev& P = *reinterpret_cast<sgm_plan_dev*>(smem);
+    // bitmap of a vertex's adjacency list, or null when it has none
+    auto vbits = [&](uint32_t v) -> const uint32_t* {
+        if (!hub_slot) return nullptr;
+        const uint32_t s = hub_slot[v];
+        return s == SGM_INVALID ? nullptr : hub_bits + (size_t)s * hub_words;
+    };
     {
         const uint32_t* src = reinterpret_cast<const uint32_t*>(gplan);
         uint32_t* dst = reinterpret_cast<uint32_t*>(smem);
@@ -516,10 +559,11 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
     const uint32_t n = P.n, n_ops = P.n_ops;
     const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
     const size_t plan_bytes = (sizeof(sgm_plan_dev) + 15) & ~(size_t)15;
-    const size_t wave_bytes = ((sizeof(WaveDFS) + 8u * n_ops) + 15) & ~(size_t)15;
+    const size_t wave_bytes = ((sizeof(WaveDFS) + 12u * n_ops) + 15) & ~(size_t)15;
     WaveDFS& S = *reinterpret_cast<WaveDFS*>(smem + plan_bytes + wave * wave_bytes);
     uint32_t* op_off = reinterpret_cast<uint32_t*>(smem + plan_bytes + wave * wave_bytes + sizeof(WaveDFS));
     uint32_t* op_size = op_off + n_ops;
+    uint32_t* op_vtx = op_size + n_ops;  // vertex of an adjacency (alias) op, else SGM_INVALID
     uint32_t* ws = stack + (size_t)(blockIdx.x * (blockDim.x >> 6) + wave) * stack_words;
     const uint32_t L = n - 1;
     const uint32_t uL = P.order[L];
@@ -560,15 +604,17 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
             const uint32_t nbo = off[v0], nbs = off[v0 + 1] - nbo;
             for (uint32_t o = P.op_first[0]; o < P.op_first[1]; o++) {
                 const uint32_t p = P.op_parent[o];
-                if (p == SGM_NOPARENT) { op_off[o] = nbo; op_size[o] = nbs; }
+                if (p == SGM_NOPARENT) { op_off[o] = nbo; op_size[o] = nbs; op_vtx[o] = v0; }
                 else {
                     const uint32_t need = op_size[p] < nbs ? op_size[p] : nbs;
                     if (top + need > stack_words) { overflow = true; aborted = true; ok = false; break; }
                     inter++;
-                    const uint32_t sz = isect_write(op_list(op_off[p], adj, ws), op_size[p], adj + nbo, nbs, ws + top,
-                                                    elems, outs);
+                    const uint32_t sz = isect_write(op_list(op_off[p], adj, ws), op_size[p],
+                                                    op_vtx[p] != SGM_INVALID ? vbits(op_vtx[p]) : nullptr,
+                                                    adj + nbo, nbs, vbits(v0), ws + top, elems, outs);
                     op_off[o] = top | STACK_FLAG;
                     op_size[o] = sz;
+                    op_vtx[o] = SGM_INVALID;
                     top += sz;
                 }
                 if (op_size[o] == 0) { ok = false; break; }
@@ -645,8 +691,9 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
                                 last_scan(adj + nbo, nbs, vmask, uL, S, P.anc, L, r);
                             } else {
                                 inter++;
-                                last_isect(op_list(op_off[p], adj, ws), op_size[p], adj + nbo, nbs, vmask, uL, S,
-                                           P.anc, L, r, elems);
+                                last_isect(op_list(op_off[p], adj, ws), op_size[p],
+                                           op_vtx[p] != SGM_INVALID ? vbits(op_vtx[p]) : nullptr, adj + nbo, nbs,
+                                           vbits(v), vmask, uL, S, P.anc, L, r, elems);
                             }
                             if (r.size == 0) { empty = true; conf_v = P.op_vdepth[o]; }
                         } else {
@@ -683,15 +730,17 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
                     uint32_t top = S.stack_top[d];
                     for (uint32_t o = P.op_first[d]; o < P.op_first[d + 1]; o++) {
                         const uint32_t p = P.op_parent[o];
-                        if (p == SGM_NOPARENT) { op_off[o] = nbo; op_size[o] = nbs; }
+                        if (p == SGM_NOPARENT) { op_off[o] = nbo; op_size[o] = nbs; op_vtx[o] = v; }
                         else {
                             const uint32_t need = op_size[p] < nbs ? op_size[p] : nbs;
                             if (top + need > stack_words) { overflow = true; aborted = true; ok2 = false; break; }
                             inter++;
-                            const uint32_t sz = isect_write(op_list(op_off[p], adj, ws), op_size[p], adj + nbo, nbs,
-                                                            ws + top, elems, outs);
+                            const uint32_t sz = isect_write(op_list(op_off[p], adj, ws), op_size[p],
+                                                            op_vtx[p] != SGM_INVALID ? vbits(op_vtx[p]) : nullptr,
+                                                            adj + nbo, nbs, vbits(v), ws + top, elems, outs);
                             op_off[o] = top | STACK_FLAG;
                             op_size[o] = sz;
+                            op_vtx[o] = SGM_INVALID;
                             top += sz;
                         }
                         if (op_size[o] == 0) { ok2 = false; conf = P.op_vdepth[o]; break; }
@@ -782,7 +831,7 @@ __device__ bool aux_list(const sgm_auxplan_dev& A, const WaveAux& S, uint32_t d,
         const uint32_t b = o[r], nb = o[r + 1] - b;
         const uint32_t need = size < nb ? size : nb;
         if (top + need > stack_words) return false;
-        const uint32_t sz = isect_write(list, size, A.tab_idx[e] + b, nb, ws + top, elems, outs);
+        const uint32_t sz = isect_write(list, size, nullptr, A.tab_idx[e] + b, nb, nullptr, ws + top, elems, outs);
         list = ws + top;
         size = sz;
         top += sz;
@@ -1010,8 +1059,14 @@ void launch_table(hipStream_t s, const uint32_t* off, const uint32_t* adj, const
 size_t enum_smem_bytes(uint32_t n_ops, uint32_t waves)
 {
     const size_t plan_bytes = (sizeof(sgm_plan_dev) + 15) & ~(size_t)15;
-    const size_t wave_bytes = ((sizeof(WaveDFS) + 8u * n_ops) + 15) & ~(size_t)15;
+    const size_t wave_bytes = ((sizeof(WaveDFS) + 12u * n_ops) + 15) & ~(size_t)15;
     return plan_bytes + waves * wave_bytes;
+}
+
+void launch_hub_bits(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* hubs, uint32_t n_hubs,
+                     uint32_t* bits, uint32_t words)
+{
+    hipLaunchKernelGGL(k_hub_bits, dim3(clampg(n_hubs, 1, 65535)), dim3(256), 0, s, off, adj, hubs, n_hubs, bits, words);
 }
 
 void launch_enum_aux(hipStream_t s, uint32_t blocks, const sgm_auxplan_dev* plan, uint32_t n_items, uint32_t* stack,
@@ -1025,10 +1080,12 @@ void launch_enum_aux(hipStream_t s, uint32_t blocks, const sgm_auxplan_dev* plan
 void launch_enum_cache(hipStream_t s, uint32_t blocks, const uint32_t* off, const uint32_t* adj,
                        const uint32_t* vmask, const uint32_t* items, uint32_t n_items, uint32_t root_u,
                        const sgm_plan_dev* plan, uint32_t n_ops, uint32_t* stack, uint32_t stack_words,
-                       uint32_t* work, sgm_stats_dev* st, uint32_t* stop, unsigned long long cap, int use_fs)
+                       uint32_t* work, sgm_stats_dev* st, uint32_t* stop, unsigned long long cap, int use_fs,
+                       const uint32_t* hub_slot, const uint32_t* hub_bits, uint32_t hub_words)
 {
     hipLaunchKernelGGL(k_enum_cache, dim3(blocks), dim3(256), enum_smem_bytes(n_ops, 4), s, off, adj, vmask, items,
-                       n_items, root_u, plan, stack, stack_words, work, st, stop, cap, use_fs);
+                       n_items, root_u, plan, stack, stack_words, work, st, stop, cap, use_fs, hub_slot, hub_bits,
+                       hub_words);
 }
 
 }  // namespace sgm

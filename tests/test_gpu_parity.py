@@ -112,8 +112,9 @@ def _compare(g, q, order, cap=None, check_work=True):
     ok = sgm.Filter_GQL(g, q, can, build_tables=False)
     ok_r, c_r = O.gql(g, q)
     assert ok == ok_r
-    for fs, method in ((True, L.SGM_ENUM_CACHE_FILTER_FS), (False, L.SGM_ENUM_CACHE_FILTER)):
-        res = sgm.enumerate_query(can, order, method=method, cap=cap)
+    for fs, method, flags in ((True, L.SGM_ENUM_CACHE_FILTER_FS, 0), (False, L.SGM_ENUM_CACHE_FILTER, 0),
+                              (True, L.SGM_ENUM_CACHE_FILTER_FS, L.SGM_ENUM_F_NO_BITMAPS)):
+        res = sgm.enumerate_query(can, order, method=method, cap=cap, flags=flags)
         ref = O.enum_cache(g, q, c_r, order, cap=cap, fs=fs)
         assert res.ans == ref["count"], (q.name, fs, res.ans, ref["count"])
         if check_work and not ref["capped"]:
