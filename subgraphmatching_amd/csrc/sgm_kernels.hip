@@ -420,12 +420,47 @@ __device__ __forceinline__ bool member(const uint32_t* __restrict__ b, uint32_t 
     return bits ? ((bits[key >> 5] >> (key & 31u)) & 1u) != 0u : lower_found(b, nb, key);
 }
 
-// Intersect sorted lists a and b into out (ascending); returns the size.  Lanes take 64
+// Each lane handles UNR elements per step (256 per wave), so a wave keeps UNR independent
+// memory chains in flight instead of one: intersections and scans are latency-bound, and
+// the time to reach the first embeddings is a chain of them.
+#define UNR 4
+
+// Membership of UNR keys in the sorted list b (nb > 0) at once: a bit test each when b has
+// a bitmap, else UNR branch-free binary searches in lockstep (the step count depends only
+// on nb, so the loads of the UNR searches overlap).
+__device__ __forceinline__ void member4(const uint32_t* __restrict__ b, uint32_t nb, const uint32_t* __restrict__ bits,
+                                        const uint32_t (&key)[UNR], const bool (&valid)[UNR], bool (&hit)[UNR])
+{
+    if (bits) {
+#pragma unroll
+        for (int u = 0; u < UNR; u++) hit[u] = valid[u] && ((bits[key[u] >> 5] >> (key[u] & 31u)) & 1u);
+        return;
+    }
+    uint32_t p[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; u++) p[u] = 0;
+    uint32_t n = nb;
+    while (n > 1) {
+        const uint32_t half = n >> 1;
+#pragma unroll
+        for (int u = 0; u < UNR; u++) p[u] = (b[p[u] + half] < key[u]) ? p[u] + half : p[u];
+        n -= half;
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+        const uint32_t lb = p[u] + (b[p[u]] < key[u] ? 1u : 0u);
+        hit[u] = valid[u] && lb < nb && b[lb] == key[u];
+    }
+}
+
+// Intersect sorted lists a and b into out (ascending); returns the size.  Lanes take
 // elements of the shorter list and test them against the longer one (bitmap or binary
 // search).  abits/bbits: bitmaps of a / b when they are hub adjacency lists, else null.
+// Stops early (result unused) once the stop word is raised.
 __device__ uint32_t isect_write(const uint32_t* __restrict__ a, uint32_t na, const uint32_t* __restrict__ abits,
                                 const uint32_t* __restrict__ b, uint32_t nb, const uint32_t* __restrict__ bbits,
-                                uint32_t* __restrict__ out, unsigned long long& elems, unsigned long long& outs)
+                                uint32_t* __restrict__ out, unsigned long long& elems, unsigned long long& outs,
+                                const uint32_t* stop)
 {
     if (na > nb) {
         const uint32_t* t = a; a = b; b = t;
@@ -435,13 +470,24 @@ __device__ uint32_t isect_write(const uint32_t* __restrict__ a, uint32_t na, con
     const uint32_t lane = lane_id();
     uint32_t cnt = 0;
     elems += na + nb;
-    for (uint32_t base = 0; base < na; base += 64) {
-        const uint32_t i = base + lane;
-        const uint32_t key = i < na ? a[i] : 0u;
-        const bool hit = i < na && member(b, nb, bbits, key);
-        const uint64_t bal = __ballot(hit);
-        if (hit) out[cnt + mbcnt64(bal)] = key;
-        cnt += (uint32_t)__popcll(bal);
+    if (nb == 0) return 0;
+    for (uint32_t base = 0; base < na; base += 64 * UNR) {
+        uint32_t key[UNR];
+        bool valid[UNR], hit[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            const uint32_t i = base + 64 * u + lane;
+            valid[u] = i < na;
+            key[u] = valid[u] ? a[i] : 0u;
+        }
+        member4(b, nb, bbits, key, valid, hit);
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            const uint64_t bal = __ballot(hit[u]);
+            if (hit[u]) out[cnt + mbcnt64(bal)] = key[u];
+            cnt += (uint32_t)__popcll(bal);
+        }
+        if (((base >> 8) & 7u) == 7u && __atomic_load_n(stop, __ATOMIC_RELAXED)) break;
     }
     outs += cnt;
     return cnt;
@@ -451,7 +497,7 @@ __device__ uint32_t isect_write(const uint32_t* __restrict__ a, uint32_t na, con
 // of each depth-0 item), so the cap can stop the whole grid while waves are still deep
 // inside huge subtrees; the stop word is raised once the global count reaches the cap,
 // so the final total is >= cap whenever the cap stopped the search.
-#define FLUSH_AT 4096ull
+#define FLUSH_AT 512ull
 
 __device__ __forceinline__ void flush_count(unsigned long long& pending, unsigned long long* gcount, uint32_t* stop,
                                             unsigned long long cap)
@@ -471,37 +517,54 @@ struct LastRes {
     uint32_t any;       // some element passed the filter
 };
 
-// Classify one element at the last depth L: filter bit of u_L, then injectivity.
-__device__ __forceinline__ void classify_last(bool valid, uint32_t w, const uint32_t* __restrict__ vmask,
-                                              uint32_t uL, const WaveDFS& S, const uint64_t* anc, uint32_t L,
-                                              LastRes& r)
+// Classify UNR elements per lane at the last depth L: filter bit of u_L, then injectivity
+// against the L mapped vertices.
+__device__ __forceinline__ void classify_last4(const bool (&valid)[UNR], const uint32_t (&w)[UNR],
+                                               const uint32_t* __restrict__ vmask, uint32_t uL, const WaveDFS& S,
+                                               const uint64_t* anc, uint32_t L, LastRes& r)
 {
-    const bool f = valid && ((vmask[w] >> uL) & 1u);
-    uint32_t c = SGM_INVALID;
-    if (f)
-        for (uint32_t j = 0; j < L; j++)
-            if (S.emb[j] == w) c = j;
-    const uint64_t okb = __ballot(f && c == SGM_INVALID);
-    const uint64_t cfb = __ballot(f && c != SGM_INVALID);
-    r.ok += (uint32_t)__popcll(okb);
-    if (okb | cfb) r.any = 1;
-    if (cfb) {
-        const uint64_t m = (f && c != SGM_INVALID) ? (anc[L] | anc[c]) : 0ull;
-        r.conf |= wave_or64(m);
+    bool f[UNR];
+    uint32_t c[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+        f[u] = valid[u] && ((vmask[w[u]] >> uL) & 1u);
+        c[u] = SGM_INVALID;
+    }
+    for (uint32_t j = 0; j < L; j++) {
+        const uint32_t e = S.emb[j];
+#pragma unroll
+        for (int u = 0; u < UNR; u++)
+            if (e == w[u]) c[u] = j;
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+        const bool ok = f[u] && c[u] == SGM_INVALID, cf = f[u] && c[u] != SGM_INVALID;
+        const uint64_t okb = __ballot(ok);
+        const uint64_t cfb = __ballot(cf);
+        r.ok += (uint32_t)__popcll(okb);
+        if (okb | cfb) r.any = 1;
+        if (cfb) r.conf |= wave_or64(cf ? (anc[L] | anc[c[u]]) : 0ull);
     }
 }
 
 // Last level, scanning an existing list.
 __device__ void last_scan(const uint32_t* __restrict__ list, uint32_t size, const uint32_t* __restrict__ vmask,
-                          uint32_t uL, const WaveDFS& S, const uint64_t* anc, uint32_t L, LastRes& r)
+                          uint32_t uL, const WaveDFS& S, const uint64_t* anc, uint32_t L, LastRes& r,
+                          const uint32_t* stop)
 {
     const uint32_t lane = lane_id();
     r.size = size;
-    for (uint32_t base = 0; base < size; base += 64) {
-        const uint32_t i = base + lane;
-        const bool valid = i < size;
-        const uint32_t w = valid ? list[i] : 0u;
-        classify_last(valid, w, vmask, uL, S, anc, L, r);
+    for (uint32_t base = 0; base < size; base += 64 * UNR) {
+        uint32_t w[UNR];
+        bool valid[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            const uint32_t i = base + 64 * u + lane;
+            valid[u] = i < size;
+            w[u] = valid[u] ? list[i] : 0u;
+        }
+        classify_last4(valid, w, vmask, uL, S, anc, L, r);
+        if (((base >> 8) & 7u) == 7u && __atomic_load_n(stop, __ATOMIC_RELAXED)) break;
     }
 }
 
@@ -509,7 +572,8 @@ __device__ void last_scan(const uint32_t* __restrict__ list, uint32_t size, cons
 __device__ void last_isect(const uint32_t* __restrict__ a, uint32_t na, const uint32_t* __restrict__ abits,
                            const uint32_t* __restrict__ b, uint32_t nb, const uint32_t* __restrict__ bbits,
                            const uint32_t* __restrict__ vmask, uint32_t uL, const WaveDFS& S,
-                           const uint64_t* anc, uint32_t L, LastRes& r, unsigned long long& elems)
+                           const uint64_t* anc, uint32_t L, LastRes& r, unsigned long long& elems,
+                           const uint32_t* stop)
 {
     if (na > nb) {
         const uint32_t* t = a; a = b; b = t;
@@ -519,12 +583,21 @@ __device__ void last_isect(const uint32_t* __restrict__ a, uint32_t na, const ui
     const uint32_t lane = lane_id();
     elems += na + nb;
     uint32_t size = 0;
-    for (uint32_t base = 0; base < na; base += 64) {
-        const uint32_t i = base + lane;
-        const uint32_t key = i < na ? a[i] : 0u;
-        const bool hit = i < na && member(b, nb, bbits, key);
-        size += (uint32_t)__popcll(__ballot(hit));
-        classify_last(hit, key, vmask, uL, S, anc, L, r);
+    if (nb == 0) { r.size = 0; return; }
+    for (uint32_t base = 0; base < na; base += 64 * UNR) {
+        uint32_t key[UNR];
+        bool valid[UNR], hit[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            const uint32_t i = base + 64 * u + lane;
+            valid[u] = i < na;
+            key[u] = valid[u] ? a[i] : 0u;
+        }
+        member4(b, nb, bbits, key, valid, hit);
+#pragma unroll
+        for (int u = 0; u < UNR; u++) size += (uint32_t)__popcll(__ballot(hit[u]));
+        classify_last4(hit, key, vmask, uL, S, anc, L, r);
+        if (((base >> 8) & 7u) == 7u && __atomic_load_n(stop, __ATOMIC_RELAXED)) break;
     }
     r.size = size;
 }
@@ -594,7 +667,7 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
             const uint32_t o = P.iter_set[1];
             op_off[o] = off[v0];
             op_size[o] = off[v0 + 1] - off[v0];
-            last_scan(adj + off[v0], op_size[o], vmask, uL, S, P.anc, L, r);
+            last_scan(adj + off[v0], op_size[o], vmask, uL, S, P.anc, L, r, stop);
             scanned += r.size;
             cnt_pending += r.ok;
             S.nodes[L] += r.ok;
@@ -611,7 +684,7 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
                     inter++;
                     const uint32_t sz = isect_write(op_list(op_off[p], adj, ws), op_size[p],
                                                     op_vtx[p] != SGM_INVALID ? vbits(op_vtx[p]) : nullptr,
-                                                    adj + nbo, nbs, vbits(v0), ws + top, elems, outs);
+                                                    adj + nbo, nbs, vbits(v0), ws + top, elems, outs, stop);
                     op_off[o] = top | STACK_FLAG;
                     op_size[o] = sz;
                     op_vtx[o] = SGM_INVALID;
@@ -628,7 +701,7 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
                 S.fail[0] = 0;
                 uint32_t d = 1;
                 while (true) {
-                    if ((++poll & 63u) == 0 && __atomic_load_n(stop, __ATOMIC_RELAXED)) { aborted = true; break; }
+                    if ((++poll & 3u) == 0 && __atomic_load_n(stop, __ATOMIC_RELAXED)) { aborted = true; break; }
                     if (S.rem[d] == 0 && ((S.flags[d] & F_PRUNED) || S.pos[d] >= S.tot[d])) {
                         // end of the loop at depth d (inc/explore.h:1068-1070)
                         if (use_fs && !(S.flags[d] & (F_PRUNED | F_ANY))) S.fail[d - 1] = P.anc[d];
@@ -688,17 +761,17 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
                             const uint32_t o = P.op_first[d];
                             const uint32_t p = P.op_parent[o];
                             if (p == SGM_NOPARENT) {
-                                last_scan(adj + nbo, nbs, vmask, uL, S, P.anc, L, r);
+                                last_scan(adj + nbo, nbs, vmask, uL, S, P.anc, L, r, stop);
                             } else {
                                 inter++;
                                 last_isect(op_list(op_off[p], adj, ws), op_size[p],
                                            op_vtx[p] != SGM_INVALID ? vbits(op_vtx[p]) : nullptr, adj + nbo, nbs,
-                                           vbits(v), vmask, uL, S, P.anc, L, r, elems);
+                                           vbits(v), vmask, uL, S, P.anc, L, r, elems, stop);
                             }
                             if (r.size == 0) { empty = true; conf_v = P.op_vdepth[o]; }
                         } else {
                             const uint32_t o = P.iter_set[L];
-                            last_scan(op_list(op_off[o], adj, ws), op_size[o], vmask, uL, S, P.anc, L, r);
+                            last_scan(op_list(op_off[o], adj, ws), op_size[o], vmask, uL, S, P.anc, L, r, stop);
                         }
                         cnt_pending += r.ok;
                         if (cnt_pending >= FLUSH_AT) flush_count(cnt_pending, &gst->count, stop, cap);
@@ -737,7 +810,7 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
                             inter++;
                             const uint32_t sz = isect_write(op_list(op_off[p], adj, ws), op_size[p],
                                                             op_vtx[p] != SGM_INVALID ? vbits(op_vtx[p]) : nullptr,
-                                                            adj + nbo, nbs, vbits(v), ws + top, elems, outs);
+                                                            adj + nbo, nbs, vbits(v), ws + top, elems, outs, stop);
                             op_off[o] = top | STACK_FLAG;
                             op_size[o] = sz;
                             op_vtx[o] = SGM_INVALID;
@@ -813,7 +886,8 @@ struct WaveAux {
 // neighbour's row is returned separately (for a fused count) in (*last, *nlast).
 __device__ bool aux_list(const sgm_auxplan_dev& A, const WaveAux& S, uint32_t d, uint32_t* ws, uint32_t stack_words,
                          uint32_t& top, const uint32_t*& list, uint32_t& size, bool leave_last,
-                         const uint32_t*& last, uint32_t& nlast, unsigned long long& elems, unsigned long long& outs)
+                         const uint32_t*& last, uint32_t& nlast, unsigned long long& elems, unsigned long long& outs,
+                         const uint32_t* stop)
 {
     const uint32_t f = A.prev_first[d], e_end = A.prev_first[d + 1];
     const uint32_t np = e_end - f;
@@ -831,7 +905,7 @@ __device__ bool aux_list(const sgm_auxplan_dev& A, const WaveAux& S, uint32_t d,
         const uint32_t b = o[r], nb = o[r + 1] - b;
         const uint32_t need = size < nb ? size : nb;
         if (top + need > stack_words) return false;
-        const uint32_t sz = isect_write(list, size, nullptr, A.tab_idx[e] + b, nb, nullptr, ws + top, elems, outs);
+        const uint32_t sz = isect_write(list, size, nullptr, A.tab_idx[e] + b, nb, nullptr, ws + top, elems, outs, stop);
         list = ws + top;
         size = sz;
         top += sz;
@@ -908,7 +982,7 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
                 const uint32_t* last;
                 uint32_t nlast;
                 const bool is_last = d == L;
-                if (!aux_list(A, S, d, ws, stack_words, top, list, size, is_last, last, nlast, elems, outs)) {
+                if (!aux_list(A, S, d, ws, stack_words, top, list, size, is_last, last, nlast, elems, outs, stop)) {
                     overflow = true; aborted = true; break;
                 }
                 if (is_last) {
