@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--queries", type=int, default=200)
     ap.add_argument("--cap", type=int, default=10_000_000)
+    ap.add_argument("--blocks", type=int, default=0,
+                    help="persistent enumeration blocks of 4 waves (0 = library default)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="wall-clock budget of the CPU baseline sample (0 disables it)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
@@ -93,7 +95,7 @@ def main():
             if not good:
                 rec["counts"].append(0)
                 continue
-            r = sgm.enumerate_query(can, order, L.SGM_ENUM_CACHE_FILTER_FS, cap=args.cap)
+            r = sgm.enumerate_query(can, order, L.SGM_ENUM_CACHE_FILTER_FS, cap=args.cap, blocks=args.blocks)
             rec["emb"] += r.ans
             rec["inter"] += r.intersections
             rec["elems_read"] += r.elems_read

@@ -155,6 +155,39 @@ __device__ bool lsm_wave(const uint32_t* __restrict__ nb, uint32_t deg, const ui
 // one phase are independent (the reference's sequential order inside a phase does not
 // matter).  A round-2 phase whose left-side sets did not change since round 1 would
 // repeat round 1's answers exactly and is skipped.
+//
+// A wave takes 64 candidates at a time: lanes fetch their ids, validity bits and
+// offsets together.  A candidate of degree <= GQL_LANE_DEG is first decided by its own lane
+// from its right-side masks: a greedy matching that saturates the left side proves the
+// answer true; a left vertex no right vertex covers, or fewer useful right vertices than
+// left vertices, proves it false.  Only the remaining candidates (and all higher-degree
+// ones) go through the exact wave matcher, one after another.
+#define GQL_LANE_DEG 32u
+
+__device__ __forceinline__ int lsm_lane(const uint32_t* __restrict__ adj, uint32_t o, uint32_t deg,
+                                        const uint32_t* __restrict__ vm, uint32_t L, uint32_t k)
+{
+    // returns 1 = passes, 0 = fails, -1 = undecided
+    uint32_t orm = 0, useful = 0, freeL = L, matched = 0;
+    for (uint32_t j0 = 0; j0 < deg; j0 += 8) {
+        uint32_t w[8], m[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) w[t] = (j0 + t < deg) ? adj[o + j0 + t] : 0u;
+#pragma unroll
+        for (int t = 0; t < 8; t++) m[t] = (j0 + t < deg) ? (vm[w[t]] & L) : 0u;
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            orm |= m[t];
+            useful += m[t] != 0u;
+            const uint32_t f = m[t] & freeL;
+            if (f) { freeL &= ~(f & (0u - f)); matched++; }
+        }
+    }
+    if (matched == k) return 1;
+    if (orm != L || useful < k) return 0;
+    return -1;
+}
+
 __global__ void __launch_bounds__(256) k_gql_phase(const uint32_t* __restrict__ off,
                                                    const uint32_t* __restrict__ adj,
                                                    const uint32_t* __restrict__ by_deg,
@@ -177,20 +210,33 @@ __global__ void __launch_bounds__(256) k_gql_phase(const uint32_t* __restrict__ 
     const uint32_t k = __popc(L);
     const uint32_t cnt = ge_cnt[d->qdeg[u]];
     uint32_t* __restrict__ vm = d->vmask;
+    const uint32_t lane = lane_id();
     const uint32_t waves = blockDim.x >> 6;
     const uint32_t w0 = blockIdx.x * waves + (threadIdx.x >> 6);
     const uint32_t stride = gridDim.x * waves;
     uint32_t local_removed = 0;
-    for (uint32_t i = w0; i < cnt; i += stride) {
-        const uint32_t v = by_deg[i];
-        if (!((vm[v] >> u) & 1u)) continue;
-        const uint32_t o = off[v];
-        if (!lsm_wave(adj + o, off[v + 1] - o, vm, L, k)) {
-            if (lane_id() == 0) atomicAnd(&vm[v], ~(1u << u));
-            local_removed++;
+    for (uint32_t base = w0 * 64; base < cnt; base += stride * 64) {
+        const uint32_t i = base + lane;
+        const uint32_t v = i < cnt ? by_deg[i] : 0u;
+        const bool valid = i < cnt && ((vm[v] >> u) & 1u);
+        uint32_t o = 0, dv = 0;
+        if (valid) { o = off[v]; dv = off[v + 1] - o; }
+        int verdict = -1;
+        if (valid && dv <= GQL_LANE_DEG) verdict = lsm_lane(adj, o, dv, vm, L, k);
+        if (valid && verdict == 0) atomicAnd(&vm[v], ~(1u << u));
+        local_removed += (uint32_t)__popcll(__ballot(valid && verdict == 0));
+        uint64_t pend = __ballot(valid && verdict < 0);
+        while (pend) {
+            const uint32_t t = (uint32_t)__ffsll((long long)pend) - 1u;
+            pend &= pend - 1;
+            const uint32_t vt = rdlane(v, t), ot = rdlane(o, t), dt = rdlane(dv, t);
+            if (!lsm_wave(adj + ot, dt, vm, L, k)) {
+                if (lane == 0) atomicAnd(&vm[vt], ~(1u << u));
+                local_removed++;
+            }
         }
     }
-    if (lane_id() == 0 && local_removed) atomicAdd(&removed[phase], local_removed);
+    if (lane == 0 && local_removed) atomicAdd(&removed[phase], local_removed);
 }
 
 // ================================================================== hub adjacency bitmaps
@@ -1092,7 +1138,7 @@ void launch_nlf(hipStream_t s, const uint32_t* off, uint32_t V, const sgm_qdesc*
 void launch_gql_phase(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* by_deg,
                       const uint32_t* ge_cnt, const sgm_qdesc* qs, uint32_t nq, uint32_t phase, uint32_t max_cand)
 {
-    dim3 grid(clampg(((uint64_t)max_cand + 31) / 32, 1, 8192), nq);
+    dim3 grid(clampg(((uint64_t)max_cand + 255) / 256, 1, 4096), nq);
     hipLaunchKernelGGL(k_gql_phase, grid, dim3(256), 0, s, off, adj, by_deg, ge_cnt, qs, phase);
 }
 
