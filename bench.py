@@ -7,7 +7,8 @@ Workload (one step = one pass of the path over one batch):
               (unlabeled, simple, sorted u32 CSR; generated once, resident in HBM)
   queries     200 dense 16-vertex random-walk queries (average degree > 3), seed 22
   path        NLF + GQL candidate filtering for all 200 queries in one batch of launches,
-              RI order (host), cache + filter + failing-set enumeration per query
+              RI order (host), cache + filter + failing-set enumeration of all 200 queries
+              in one persistent launch (per-query caps and statistics)
   output      exact counts, capped at --cap embeddings per query (default 10^7, the same
               cap for the GPU run and the CPU baseline; uncapped counts of these queries
               run past 10^9 and cannot be enumerated in bounded time)
@@ -37,6 +38,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--queries", type=int, default=200)
     ap.add_argument("--cap", type=int, default=10_000_000)
+    ap.add_argument("--per-query", action="store_true",
+                    help="one enumeration launch per query instead of one batched persistent launch")
     ap.add_argument("--blocks", type=int, default=0,
                     help="persistent enumeration blocks of 4 waves (0 = library default)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
@@ -90,19 +93,32 @@ def main():
         ok = sgm.filter_batch(cans, L.SGM_FILTER_GQL)
         nlf_ms, gql_ms, cand_ms = ctx.filter_ms()
         t1 = time.perf_counter()
-        for q, can, good in zip(queries, cans, ok):
-            order = sgm.Order_RI(q)
-            if not good:
+        orders = [sgm.Order_RI(q) for q in queries]
+        if args.per_query:
+            results = [sgm.enumerate_query(can, order, L.SGM_ENUM_CACHE_FILTER_FS, cap=args.cap, blocks=args.blocks)
+                       if good else None for can, order, good in zip(cans, orders, ok)]
+        else:
+            live = [i for i, good in enumerate(ok) if good]
+            batch = sgm.enumerate_batch([cans[i] for i in live], [orders[i] for i in live],
+                                        L.SGM_ENUM_CACHE_FILTER_FS, cap=args.cap, blocks=args.blocks)
+            results = [None] * len(queries)
+            for i, r in zip(live, batch):
+                results[i] = r
+            if batch:
+                rec["enum_ms"] += batch[0].time
+                rec["enum_launches"] += 1
+        for r in results:
+            if r is None:
                 rec["counts"].append(0)
                 continue
-            r = sgm.enumerate_query(can, order, L.SGM_ENUM_CACHE_FILTER_FS, cap=args.cap, blocks=args.blocks)
             rec["emb"] += r.ans
             rec["inter"] += r.intersections
             rec["elems_read"] += r.elems_read
             rec["elems_out"] += r.elems_out
             rec["elems_scanned"] += r.elems_scanned
-            rec["enum_ms"] += r.time
-            rec["enum_launches"] += 1
+            if args.per_query:
+                rec["enum_ms"] += r.time
+                rec["enum_launches"] += 1
             rec["capped"] += int(r.capped)
             rec["counts"].append(r.ans)
         t2 = time.perf_counter()

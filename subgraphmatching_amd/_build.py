@@ -7,13 +7,13 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsgm_hip.so")
-SOURCES = ["sgm_kernels.hip", "sgm_host.cpp", "sgm_capi.cpp"]
+SOURCES = ["sgm_kernels.hip", "sgm_enum.hip", "sgm_host.cpp", "sgm_capi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SGM_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
          # no packed-fp32 instructions anywhere (none are needed: all arithmetic is integer)
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-         "-Wall", "-Wno-unused-function"]
+         "-fno-strict-aliasing", "-Wall", "-Wno-unused-function"]
 
 
 def build(verbose: bool = False) -> str:

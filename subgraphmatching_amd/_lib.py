@@ -43,7 +43,7 @@ EXPORTS = [
     "sgm_stop", "sgm_graph_upload", "sgm_graph_destroy", "sgm_graph_max_degree", "sgm_query_create",
     "sgm_query_destroy", "sgm_filter", "sgm_filter_batch", "sgm_cand_counts", "sgm_cand_download",
     "sgm_cand_exist_download", "sgm_build_tables", "sgm_table_info", "sgm_table_download",
-    "sgm_order_ri", "sgm_order_gql", "sgm_plan_export", "sgm_enumerate",
+    "sgm_order_ri", "sgm_order_gql", "sgm_plan_export", "sgm_enumerate", "sgm_enumerate_batch",
 ]
 
 
@@ -106,6 +106,7 @@ def lib() -> C.CDLL:
         "sgm_order_gql": (C.c_int, [vp, vp]),
         "sgm_plan_export": (C.c_int, [vp, vp, C.c_uint32, vp, u32p, vp, vp, vp, vp, vp]),
         "sgm_enumerate": (C.c_int, [vp, vp, C.POINTER(EnumOpts), C.POINTER(Stats)]),
+        "sgm_enumerate_batch": (C.c_int, [vp, vp, C.c_uint32, C.POINTER(EnumOpts), vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -29,7 +29,7 @@ from .graph import Graph, Query
 __all__ = ["Context", "default_context", "DeviceGraph", "CandidateParam", "Edges", "Filter_NLF",
            "Filter_GQL", "Order_RI", "Order_GQL", "ReuseParam", "Config", "ExploreResult",
            "explore_Intersection_Cache_DAF_Filter", "explore_Intersection_All", "Profiler",
-           "filter_batch", "enumerate_query", "plan_export"]
+           "filter_batch", "enumerate_query", "enumerate_batch", "plan_export"]
 
 
 def _check(code, ctx=None):
@@ -387,6 +387,38 @@ def enumerate_query(can: CandidateParam, order, method=L.SGM_ENUM_CACHE_FILTER_F
         prof.total_iter_count[d] += x
     prof.daf_prune_cnt += res.fs_prunes
     return res
+
+
+def enumerate_batch(cans, orders, method=L.SGM_ENUM_CACHE_FILTER_FS, cap=None, blocks=0, stack_words=0,
+                    flags=0):
+    """Enumerate many filtered queries of one data graph in one persistent launch.
+    Returns one ExploreResult per query (``time`` is the whole batch's kernel time)."""
+    if not cans:
+        return []
+    ctx = cans[0].ctx
+    opts = L.EnumOpts()
+    opts.cap = (1 << 64) - 1 if cap is None else int(cap)
+    opts.method = method
+    opts.blocks = blocks
+    opts.stack_words = stack_words
+    opts.flags = flags
+    nq = len(cans)
+    ords = [L.u32(o) for o in orders]
+    qarr = (C.c_void_p * nq)(*[c._q.value for c in cans])
+    oarr = (C.c_void_p * nq)(*[o.ctypes.data for o in ords])
+    stats = (L.Stats * nq)()
+    _check(L.lib().sgm_enumerate_batch(qarr, oarr, nq, C.byref(opts), stats), ctx)
+    out = []
+    prof = Profiler.getInst()
+    for can, st in zip(cans, stats):
+        n = can.query.getVertexCnt()
+        r = ExploreResult(time=float(st.kernel_ms), ans=int(st.count), intersections=int(st.intersections),
+                          nodes=[int(x) for x in st.nodes[:n]], fs_prunes=int(st.fs_prunes),
+                          elems_read=int(st.elems_read), elems_out=int(st.elems_out),
+                          elems_scanned=int(st.elems_scanned), capped=bool(st.capped), retries=int(st.retries))
+        prof.total_intersection += r.intersections
+        out.append(r)
+    return out
 
 
 def explore_Intersection_Cache_DAF_Filter(data: Graph, query: Query, order, cfg: Config,

@@ -22,8 +22,13 @@ struct sgm_ctx {
     uint32_t* d_stop = nullptr;      // stop word polled by the enumeration kernel
     uint32_t* d_work = nullptr;      // work counter
     sgm_stats_dev* d_stats = nullptr;
-    sgm_plan_dev* d_plan = nullptr;
     sgm_auxplan_dev* d_auxplan = nullptr;
+    unsigned long long* d_work64 = nullptr;  // item counter of the batched search
+    sgm_plan_dev* d_plans = nullptr;          // per-query plans of a batch
+    sgm_equery* d_eq = nullptr;
+    sgm_stats_dev* d_qstats = nullptr;
+    uint32_t* d_qstop = nullptr;
+    uint32_t batch_cap = 0, batch_cap2 = 0;
     uint32_t* d_stack = nullptr;
     size_t stack_bytes = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -120,6 +125,19 @@ static void dfree(void* p)
     if (p) (void)hipFree(p);
 }
 
+// (Re)allocate a context buffer so that it holds at least `need` elements.
+template <class T>
+static int grow(sgm_ctx* c, T** p, uint32_t& cap_now, uint32_t need)
+{
+    if (cap_now >= need) return SGM_OK;
+    dfree(*p);
+    *p = nullptr;
+    cap_now = 0;
+    HIPCHK(c, dalloc(p, need));
+    cap_now = need;
+    return SGM_OK;
+}
+
 extern "C" {
 
 const char* sgm_version(void) { return "sgm-mi355x 0.1 (gfx950)"; }
@@ -148,8 +166,8 @@ int sgm_ctx_create(int device, sgm_ctx** out)
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->ctl, hipStreamNonBlocking) != hipSuccess ||
         dalloc(&c->d_stop, 1) != hipSuccess || dalloc(&c->d_work, 1) != hipSuccess ||
-        dalloc(&c->d_stats, 1) != hipSuccess || dalloc(&c->d_plan, 1) != hipSuccess ||
-        dalloc(&c->d_auxplan, 1) != hipSuccess ||
+        dalloc(&c->d_stats, 1) != hipSuccess || dalloc(&c->d_auxplan, 1) != hipSuccess ||
+        dalloc(&c->d_work64, 1) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipEventCreate(&c->fev[0]) != hipSuccess || hipEventCreate(&c->fev[1]) != hipSuccess ||
         hipEventCreate(&c->fev[2]) != hipSuccess || hipEventCreate(&c->fev[3]) != hipSuccess) {
@@ -168,8 +186,12 @@ void sgm_ctx_destroy(sgm_ctx* c)
     dfree(c->d_stop);
     dfree(c->d_work);
     dfree(c->d_stats);
-    dfree(c->d_plan);
     dfree(c->d_auxplan);
+    dfree(c->d_work64);
+    dfree(c->d_plans);
+    dfree(c->d_eq);
+    dfree(c->d_qstats);
+    dfree(c->d_qstop);
     dfree(c->d_stack);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -654,73 +676,77 @@ static int ensure_stack(sgm_ctx* c, size_t bytes)
     return SGM_OK;
 }
 
-int sgm_enumerate(sgm_query* q, const uint32_t* order, const sgm_enum_opts* opts, sgm_stats* out)
+static sgm_enum_opts normalize_opts(const sgm_enum_opts* opts)
 {
-    if (!q || !order || !out) return SGM_EINVAL;
-    sgm_graph* g = q->g;
-    sgm_ctx* c = g->ctx;
-    std::memset(out, 0, sizeof *out);
     sgm_enum_opts o;
     std::memset(&o, 0, sizeof o);
     o.cap = UINT64_MAX;
     if (opts) o = *opts;
     if (o.cap == 0) o.cap = 1;  // the reference stops at the first embedding when ans >= 0
-    if (!q->filtered) return set_err(c, SGM_ESTATE, "filter the query first");
+    return o;
+}
+
+static uint32_t default_blocks(const sgm_ctx* c, const sgm_enum_opts& o, uint32_t nq)
+{
+    // A single capped query: only its first embeddings matter and extra waves add overshoot.
+    // A batch (or an uncapped search) wants the wave slots busy.
+    const uint32_t per_cu = o.cap == UINT64_MAX ? 8u : (nq > 1 ? 4u : 2u);
+    uint32_t b = o.blocks ? o.blocks : (uint32_t)c->num_cus * per_cu;
+    return std::max<uint32_t>(1, std::min<uint32_t>(b, 1u << 16));
+}
+
+static void stats_out(const sgm_stats_dev& st, uint64_t cap, sgm_stats* out)
+{
+    out->count = st.count < cap ? st.count : cap;
+    out->capped = st.count >= cap;
+    out->intersections = st.intersections;
+    for (uint32_t d = 0; d < SGM_MAXQ; d++) out->nodes[d] = st.nodes[d];
+    out->fs_prunes = st.fs_prunes;
+    out->elems_read = st.elems_read;
+    out->elems_out = st.elems_out;
+    out->elems_scanned = st.elems_scanned;
+}
+
+// explore_Intersection_All<false x5> for one query over its device tables.
+static int enumerate_aux(sgm_query* q, const uint32_t* order, const sgm_enum_opts& o, sgm_stats* out)
+{
+    sgm_graph* g = q->g;
+    sgm_ctx* c = g->ctx;
     const uint32_t n = q->q.n;
-    if (n < 2) return set_err(c, SGM_EINVAL, "enumeration needs at least 2 query vertices");
-    if (o.method != SGM_ENUM_CACHE_FILTER_FS && o.method != SGM_ENUM_CACHE_FILTER && o.method != SGM_ENUM_AUX_TABLES)
-        return set_err(c, SGM_EINVAL, "unknown enumeration method %d", o.method);
-    sgm::Plan pl;
-    const int prc = sgm::build_plan(q->q, order, pl);
-    if (prc == -2) return set_err(c, SGM_EORDER, "order is not a connected permutation");
-    if (prc) return set_err(c, SGM_EINVAL, "plan construction failed (%d)", prc);
-    if (q->empty) return SGM_OK;  // the reference's driver reports 0 without enumerating
-    const bool aux = o.method == SGM_ENUM_AUX_TABLES;
-    sgm_plan_dev pd;
-    sgm::plan_to_device(pl, pd);
-    HIPCHK(c, hipSetDevice(c->device));
-    const uint32_t u0 = order[0];
-    uint32_t n_items = g->ge_cnt[std::min<uint32_t>(q->q.off[u0 + 1] - q->q.off[u0], 32)];
-    if (aux) {
-        bool have = true;
-        for (uint32_t a = 0; a < n && have; a++)
-            for (uint32_t p = q->q.off[a]; p < q->q.off[a + 1]; p++) have &= q->tables[(size_t)a * n + q->q.adj[p]].built;
-        if (!have) {
-            int trc = sgm_build_tables(q);
-            if (trc) return trc;
-        }
-        std::vector<std::vector<uint32_t>> prev;
-        sgm::previous_neighbours(q->q, order, prev);
-        std::vector<uint32_t> pos(n);
-        for (uint32_t i = 0; i < n; i++) pos[order[i]] = i;
-        sgm_auxplan_dev ap;
-        std::memset(&ap, 0, sizeof ap);
-        ap.n = n;
-        uint32_t e = 0;
-        for (uint32_t d = 0; d < n; d++) {
-            const uint32_t u = order[d];
-            ap.order[d] = u;
-            ap.prev_first[d] = e;
-            ap.cand[d] = q->d_list + q->cand_off[u];
-            for (uint32_t p : prev[u]) {
-                const sgm_table& t = q->tables[(size_t)p * n + u];
-                ap.prev_depth[e] = pos[p];
-                ap.tab_off[e] = t.d_off;
-                ap.tab_idx[e] = t.d_idx;
-                e++;
-            }
-        }
-        ap.prev_first[n] = e;
-        HIPCHK(c, hipMemcpyAsync(c->d_auxplan, &ap, sizeof ap, hipMemcpyHostToDevice, c->stream));
-        n_items = q->counts[u0];
-    } else {
-        HIPCHK(c, hipMemcpyAsync(c->d_plan, &pd, sizeof pd, hipMemcpyHostToDevice, c->stream));
+    bool have = true;
+    for (uint32_t a = 0; a < n && have; a++)
+        for (uint32_t p = q->q.off[a]; p < q->q.off[a + 1]; p++) have &= q->tables[(size_t)a * n + q->q.adj[p]].built;
+    if (!have) {
+        int trc = sgm_build_tables(q);
+        if (trc) return trc;
     }
-    uint32_t blocks = o.blocks ? o.blocks : (uint32_t)c->num_cus * 8;
-    blocks = std::max<uint32_t>(1, std::min<uint32_t>(blocks, 1u << 16));
+    std::vector<std::vector<uint32_t>> prev;
+    sgm::previous_neighbours(q->q, order, prev);
+    std::vector<uint32_t> pos(n);
+    for (uint32_t i = 0; i < n; i++) pos[order[i]] = i;
+    sgm_auxplan_dev ap;
+    std::memset(&ap, 0, sizeof ap);
+    ap.n = n;
+    uint32_t e = 0;
+    for (uint32_t d = 0; d < n; d++) {
+        const uint32_t u = order[d];
+        ap.order[d] = u;
+        ap.prev_first[d] = e;
+        ap.cand[d] = q->d_list + q->cand_off[u];
+        for (uint32_t p : prev[u]) {
+            const sgm_table& t = q->tables[(size_t)p * n + u];
+            ap.prev_depth[e] = pos[p];
+            ap.tab_off[e] = t.d_off;
+            ap.tab_idx[e] = t.d_idx;
+            e++;
+        }
+    }
+    ap.prev_first[n] = e;
+    HIPCHK(c, hipMemcpyAsync(c->d_auxplan, &ap, sizeof ap, hipMemcpyHostToDevice, c->stream));
+    const uint32_t n_items = q->counts[order[0]];
+    const uint32_t blocks = default_blocks(c, o, 1);
     const uint32_t waves = blocks * 4;
     uint64_t words = o.stack_words ? o.stack_words : (uint64_t)8 * g->max_deg + 1024;
-    const uint32_t method_fs = o.method == SGM_ENUM_CACHE_FILTER_FS ? 1 : 0;
     for (uint32_t attempt = 0;; attempt++) {
         if (words > 0x7FFFFFFFull) return set_err(c, SGM_EDEVICE, "set-op stack would exceed 2^31 words");
         int rc = ensure_stack(c, (size_t)waves * words * sizeof(uint32_t));
@@ -730,15 +756,8 @@ int sgm_enumerate(sgm_query* q, const uint32_t* order, const sgm_enum_opts* opts
         HIPCHK(c, hipMemsetAsync(c->d_stop, 0, sizeof(uint32_t), c->stream));
         c->stop_requested = 0;
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-        if (aux)
-            sgm::launch_enum_aux(c->stream, blocks, c->d_auxplan, n_items, c->d_stack, (uint32_t)words, c->d_work,
-                                 c->d_stats, c->d_stop, (unsigned long long)o.cap);
-        else
-            sgm::launch_enum_cache(c->stream, blocks, g->d_off, g->d_adj, q->d_vmask, g->d_bydeg, n_items, u0,
-                                   c->d_plan, pd.n_ops, c->d_stack, (uint32_t)words, c->d_work, c->d_stats, c->d_stop,
-                                   (unsigned long long)o.cap, (int)method_fs,
-                                   (o.flags & SGM_ENUM_F_NO_BITMAPS) || !g->n_hubs ? nullptr : g->d_hub_slot,
-                                   g->d_hub_bits, g->hub_words);
+        sgm::launch_enum_aux(c->stream, blocks, c->d_auxplan, n_items, c->d_stack, (uint32_t)words, c->d_work,
+                             c->d_stats, c->d_stop, (unsigned long long)o.cap);
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
         sgm_stats_dev st;
@@ -753,20 +772,141 @@ int sgm_enumerate(sgm_query* q, const uint32_t* order, const sgm_enum_opts* opts
             if (attempt >= 3) return set_err(c, SGM_EDEVICE, "set-op stack overflow after %u retries", attempt);
             continue;
         }
-        out->count = st.count < o.cap ? st.count : o.cap;
-        out->capped = st.count >= o.cap;
-        out->intersections = st.intersections;
-        for (uint32_t d = 0; d < SGM_MAXQ; d++) out->nodes[d] = st.nodes[d];
-        out->fs_prunes = st.fs_prunes;
-        out->elems_read = st.elems_read;
-        out->elems_out = st.elems_out;
-        out->elems_scanned = st.elems_scanned;
+        stats_out(st, o.cap, out);
         if (c->stop_requested && !out->capped) {
             out->stopped = 1;
             return set_err(c, SGM_ESTOPPED, "stopped by sgm_stop");
         }
         return SGM_OK;
     }
+}
+
+int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uint32_t nq, const sgm_enum_opts* opts,
+                        sgm_stats* out)
+{
+    if (!qs || !orders || !out || nq == 0) return SGM_EINVAL;
+    for (uint32_t i = 0; i < nq; i++)
+        if (!qs[i] || !orders[i]) return SGM_EINVAL;
+    sgm_graph* g = qs[0]->g;
+    sgm_ctx* c = g->ctx;
+    const sgm_enum_opts o = normalize_opts(opts);
+    if (o.method != SGM_ENUM_CACHE_FILTER_FS && o.method != SGM_ENUM_CACHE_FILTER)
+        return set_err(c, SGM_EINVAL, "batched enumeration supports the cache + filter methods (got %d)", o.method);
+    std::memset(out, 0, sizeof(sgm_stats) * nq);
+    // plans (host), validated before anything runs
+    std::vector<sgm_plan_dev> plans(nq);
+    std::vector<uint32_t> live;
+    uint32_t max_ops = 1;
+    for (uint32_t i = 0; i < nq; i++) {
+        sgm_query* q = qs[i];
+        if (q->g != g) return set_err(c, SGM_EINVAL, "queries of a batch must share one graph");
+        if (!q->filtered) return set_err(c, SGM_ESTATE, "filter query %u first", i);
+        if (q->q.n < 2) return set_err(c, SGM_EINVAL, "enumeration needs at least 2 query vertices (query %u)", i);
+        sgm::Plan pl;
+        const int prc = sgm::build_plan(q->q, orders[i], pl);
+        if (prc == -2) return set_err(c, SGM_EORDER, "order of query %u is not a connected permutation", i);
+        if (prc) return set_err(c, SGM_EINVAL, "plan construction failed for query %u (%d)", i, prc);
+        sgm::plan_to_device(pl, plans[i]);
+        max_ops = std::max<uint32_t>(max_ops, plans[i].n_ops);
+        if (!q->empty) live.push_back(i);  // the reference's driver reports 0 without enumerating
+    }
+    if (live.empty()) return SGM_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = grow(c, &c->d_plans, c->batch_cap, nq);
+    if (rc) return rc;
+    uint32_t cap2 = c->batch_cap2;
+    if ((rc = grow(c, &c->d_eq, cap2, nq))) return rc;
+    uint32_t cap3 = c->batch_cap2;
+    if ((rc = grow(c, &c->d_qstats, cap3, nq))) return rc;
+    uint32_t cap4 = 2 * c->batch_cap2;
+    if ((rc = grow(c, &c->d_qstop, cap4, 2 * nq))) return rc;  // stop words + item counters
+    c->batch_cap2 = std::max(c->batch_cap2, nq);
+    HIPCHK(c, hipMemcpyAsync(c->d_plans, plans.data(), sizeof(sgm_plan_dev) * nq, hipMemcpyHostToDevice, c->stream));
+    const uint32_t blocks = default_blocks(c, o, (uint32_t)live.size());
+    const uint32_t waves = blocks * 4;
+    uint64_t words = o.stack_words ? o.stack_words : (uint64_t)8 * g->max_deg + 1024;
+    const int use_fs = o.method == SGM_ENUM_CACHE_FILTER_FS ? 1 : 0;
+    const uint32_t* hub_slot = (o.flags & SGM_ENUM_F_NO_BITMAPS) || !g->n_hubs ? nullptr : g->d_hub_slot;
+    std::vector<sgm_stats_dev> st(nq);
+    std::vector<sgm_equery> eq;
+    for (uint32_t attempt = 0; !live.empty(); attempt++) {
+        if (attempt > 3) return set_err(c, SGM_EDEVICE, "set-op stack overflow after %u retries", attempt - 1);
+        if (words > 0x7FFFFFFFull) return set_err(c, SGM_EDEVICE, "set-op stack would exceed 2^31 words");
+        rc = ensure_stack(c, (size_t)waves * words * sizeof(uint32_t));
+        if (rc) return rc;
+        eq.assign(live.size(), sgm_equery());
+        const uint32_t m = (uint32_t)live.size();
+        for (size_t k = 0; k < live.size(); k++) {
+            const uint32_t i = live[k];
+            sgm_query* q = qs[i];
+            const uint32_t u0 = orders[i][0];
+            eq[k].plan = c->d_plans + i;
+            eq[k].vmask = q->d_vmask;
+            eq[k].stop = c->d_qstop + k;       // stop words: [0, m)
+            eq[k].next = c->d_qstop + m + k;   // item counters: [m, 2m)
+            eq[k].st = c->d_qstats + k;
+            eq[k].cap = (unsigned long long)o.cap;
+            eq[k].n_items = g->ge_cnt[std::min<uint32_t>(q->q.off[u0 + 1] - q->q.off[u0], 32)];
+            eq[k].root_u = u0;
+        }
+        HIPCHK(c, hipMemcpyAsync(c->d_eq, eq.data(), sizeof(sgm_equery) * m, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->d_qstats, 0, sizeof(sgm_stats_dev) * m, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->d_qstop, 0, sizeof(uint32_t) * 2 * m, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->d_stop, 0, sizeof(uint32_t), c->stream));
+        c->stop_requested = 0;
+        HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+        sgm::launch_enum_cache(c->stream, blocks, g->d_off, g->d_adj, g->d_bydeg, c->d_eq, m, max_ops, c->d_stack,
+                               (uint32_t)words, c->d_stop, use_fs, hub_slot, g->d_hub_bits, g->hub_words);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+        std::vector<sgm_stats_dev> got(m);
+        HIPCHK(c, hipMemcpyAsync(got.data(), c->d_qstats, sizeof(sgm_stats_dev) * m, hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        float ms = 0;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        std::vector<uint32_t> again;
+        for (size_t k = 0; k < live.size(); k++) {
+            const uint32_t i = live[k];
+            out[i].kernel_ms += ms;
+            if (got[k].overflow) {
+                out[i].retries++;
+                again.push_back(i);
+                continue;
+            }
+            stats_out(got[k], o.cap, &out[i]);
+            if (c->stop_requested && !out[i].capped) out[i].stopped = 1;
+        }
+        live.swap(again);
+        words *= 4;
+    }
+    if (c->stop_requested) {
+        for (uint32_t i = 0; i < nq; i++)
+            if (out[i].stopped) return set_err(c, SGM_ESTOPPED, "stopped by sgm_stop");
+    }
+    return SGM_OK;
+}
+
+int sgm_enumerate(sgm_query* q, const uint32_t* order, const sgm_enum_opts* opts, sgm_stats* out)
+{
+    if (!q || !order || !out) return SGM_EINVAL;
+    sgm_ctx* c = q->g->ctx;
+    const sgm_enum_opts o = normalize_opts(opts);
+    if (o.method == SGM_ENUM_AUX_TABLES) {
+        std::memset(out, 0, sizeof *out);
+        if (!q->filtered) return set_err(c, SGM_ESTATE, "filter the query first");
+        if (q->q.n < 2) return set_err(c, SGM_EINVAL, "enumeration needs at least 2 query vertices");
+        sgm::Plan pl;
+        const int prc = sgm::build_plan(q->q, order, pl);
+        if (prc == -2) return set_err(c, SGM_EORDER, "order is not a connected permutation");
+        if (prc) return set_err(c, SGM_EINVAL, "plan construction failed (%d)", prc);
+        if (q->empty) return SGM_OK;
+        HIPCHK(c, hipSetDevice(c->device));
+        return enumerate_aux(q, order, o, out);
+    }
+    if (o.method != SGM_ENUM_CACHE_FILTER_FS && o.method != SGM_ENUM_CACHE_FILTER)
+        return set_err(c, SGM_EINVAL, "unknown enumeration method %d", o.method);
+    return sgm_enumerate_batch(&q, &order, 1, &o, out);
 }
 
 }  // extern "C"

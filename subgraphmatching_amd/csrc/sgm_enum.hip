@@ -1,0 +1,793 @@
+// gfx950 enumeration kernels: cache + filter + failing-set search (batched persistent
+// launch) and the plain aux-table search.  All matching arithmetic is integer.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sgm_internal.h"
+#include "sgm_kernels.h"
+#include "sgm_wave.h"
+
+namespace sgm {
+
+// ================================================================== enumeration
+// Cache + filter + DAF failing-set search (inc/explore.h:968-1129, inc/prune.h:82-171),
+// one wave per depth-0 candidate.  The reference never cuts at depth 0 and resets every
+// failing-set slot before reading it, so the subtrees of different depth-0 candidates are
+// independent: each wave reproduces the reference's search of its subtree exactly
+// (same set-op evaluations, same failing-set cut-offs), and per-depth node and
+// intersection counts add up to the reference's.
+//
+// Per wave: the partial embedding, per-depth iteration cursors and failing masks, and the
+// location/size of every cached set op live in LDS; materialised ops are kept on a per-wave
+// stack in HBM (ops computed at depth d start at stack_top[d] and are overwritten when the
+// vertex at depth d changes).  Each level is scanned 64 candidates at a time: lanes test
+// the candidate filter (vmask bit) and injectivity, ballots give the ordered list of
+// extensions, and the conflicts of a chunk are OR-ed into the parent's failing mask at once
+// (a later cut-off overwrites that mask anyway, so this equals the one-by-one order).
+// The last level is never materialised: its set op is intersected and counted in one pass.
+struct WaveDFS {
+    uint64_t rem[SGM_MAXQ];
+    uint64_t fail[SGM_MAXQ];
+    unsigned long long nodes[SGM_MAXQ];
+    uint32_t emb[SGM_MAXQ];
+    uint32_t pos[SGM_MAXQ];
+    uint32_t tot[SGM_MAXQ];
+    uint32_t lst[SGM_MAXQ];
+    uint32_t chunk_base[SGM_MAXQ];
+    uint32_t flags[SGM_MAXQ];
+    uint32_t stack_top[SGM_MAXQ + 1];
+};
+
+#define F_ANY 1u
+#define F_PRUNED 2u
+#define STACK_FLAG 0x80000000u
+
+// LDS slot of one wave in the batched search: its copy of the current query's plan, its
+// search state, and per-op list location / size / alias vertex (max_ops entries each).
+__host__ __device__ inline size_t enum_plan_bytes() { return (sizeof(sgm_plan_dev) + 15) & ~(size_t)15; }
+
+__host__ __device__ inline size_t enum_wave_bytes(uint32_t max_ops)
+{
+    return enum_plan_bytes() + (((sizeof(WaveDFS) + 12u * max_ops) + 15) & ~(size_t)15);
+}
+
+__device__ __forceinline__ const uint32_t* op_list(uint32_t o_off, const uint32_t* adj, const uint32_t* ws)
+{
+    return (o_off & STACK_FLAG) ? ws + (o_off & ~STACK_FLAG) : adj + o_off;
+}
+
+// Membership of key in the sorted list b: one bit test when b is the adjacency list of a
+// vertex that has a bitmap (bits != nullptr), else a binary search.
+__device__ __forceinline__ bool member(const uint32_t* __restrict__ b, uint32_t nb, const uint32_t* __restrict__ bits,
+                                       uint32_t key)
+{
+    return bits ? ((bits[key >> 5] >> (key & 31u)) & 1u) != 0u : lower_found(b, nb, key);
+}
+
+// Each lane handles UNR elements per step (256 per wave), so a wave keeps UNR independent
+// memory chains in flight instead of one: intersections and scans are latency-bound, and
+// the time to reach the first embeddings is a chain of them.
+#define UNR 4
+
+// Membership of UNR keys in the sorted list b (nb > 0) at once: a bit test each when b has
+// a bitmap, else UNR branch-free binary searches in lockstep (the step count depends only
+// on nb, so the loads of the UNR searches overlap).
+__device__ __forceinline__ void member4(const uint32_t* __restrict__ b, uint32_t nb, const uint32_t* __restrict__ bits,
+                                        const uint32_t (&key)[UNR], const bool (&valid)[UNR], bool (&hit)[UNR])
+{
+    if (bits) {
+#pragma unroll
+        for (int u = 0; u < UNR; u++) hit[u] = valid[u] && ((bits[key[u] >> 5] >> (key[u] & 31u)) & 1u);
+        return;
+    }
+    uint32_t p[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; u++) p[u] = 0;
+    uint32_t n = nb;
+    while (n > 1) {
+        const uint32_t half = n >> 1;
+#pragma unroll
+        for (int u = 0; u < UNR; u++) p[u] = (b[p[u] + half] < key[u]) ? p[u] + half : p[u];
+        n -= half;
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+        const uint32_t lb = p[u] + (b[p[u]] < key[u] ? 1u : 0u);
+        hit[u] = valid[u] && lb < nb && b[lb] == key[u];
+    }
+}
+
+// Intersect sorted lists a and b into out (ascending); returns the size.  Lanes take
+// elements of the shorter list and test them against the longer one (bitmap or binary
+// search).  abits/bbits: bitmaps of a / b when they are hub adjacency lists, else null.
+// Stops early (result unused) once the stop word is raised.
+__device__ uint32_t isect_write(const uint32_t* __restrict__ a, uint32_t na, const uint32_t* __restrict__ abits,
+                                const uint32_t* __restrict__ b, uint32_t nb, const uint32_t* __restrict__ bbits,
+                                uint32_t* __restrict__ out, unsigned long long& elems, unsigned long long& outs,
+                                const uint32_t* stop)
+{
+    if (na > nb) {
+        const uint32_t* t = a; a = b; b = t;
+        uint32_t tn = na; na = nb; nb = tn;
+        bbits = abits;
+    }
+    const uint32_t lane = lane_id();
+    uint32_t cnt = 0;
+    elems += na + nb;
+    if (nb == 0) return 0;
+    for (uint32_t base = 0; base < na; base += 64 * UNR) {
+        uint32_t key[UNR];
+        bool valid[UNR], hit[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            const uint32_t i = base + 64 * u + lane;
+            valid[u] = i < na;
+            key[u] = valid[u] ? a[i] : 0u;
+        }
+        member4(b, nb, bbits, key, valid, hit);
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            const uint64_t bal = __ballot(hit[u]);
+            if (hit[u]) out[cnt + mbcnt64(bal)] = key[u];
+            cnt += (uint32_t)__popcll(bal);
+        }
+        if (((base >> 8) & 7u) == 7u && __atomic_load_n(stop, __ATOMIC_RELAXED)) break;
+    }
+    outs += cnt;
+    return cnt;
+}
+
+// A wave publishes its embedding count at least every FLUSH_AT embeddings (and at the end
+// of each depth-0 item), so the cap can stop the whole grid while waves are still deep
+// inside huge subtrees; the stop word is raised once the global count reaches the cap,
+// so the final total is >= cap whenever the cap stopped the search.
+#define FLUSH_AT 512ull
+
+__device__ __forceinline__ void flush_count(unsigned long long& pending, unsigned long long* gcount, uint32_t* stop,
+                                            unsigned long long cap)
+{
+    if (!pending) return;
+    unsigned long long prev = 0;
+    if (lane_id() == 0) prev = atomicAdd(gcount, pending);
+    prev = __shfl(prev, 0);
+    if (prev + pending >= cap && lane_id() == 0) atomicExch(stop, 1u);
+    pending = 0;
+}
+
+struct LastRes {
+    uint32_t size;      // iteration-set size
+    uint32_t ok;        // extensions (embeddings)
+    uint64_t conf;      // OR of conflict failing sets
+    uint32_t any;       // some element passed the filter
+};
+
+// Classify UNR elements per lane at the last depth L: filter bit of u_L, then injectivity
+// against the L mapped vertices.
+__device__ __forceinline__ void classify_last4(const bool (&valid)[UNR], const uint32_t (&w)[UNR],
+                                               const uint32_t* __restrict__ vmask, uint32_t uL, const WaveDFS& S,
+                                               const uint64_t* anc, uint32_t L, LastRes& r)
+{
+    bool f[UNR];
+    uint32_t c[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+        f[u] = valid[u] && ((vmask[w[u]] >> uL) & 1u);
+        c[u] = SGM_INVALID;
+    }
+    for (uint32_t j = 0; j < L; j++) {
+        const uint32_t e = S.emb[j];
+#pragma unroll
+        for (int u = 0; u < UNR; u++)
+            if (e == w[u]) c[u] = j;
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+        const bool ok = f[u] && c[u] == SGM_INVALID, cf = f[u] && c[u] != SGM_INVALID;
+        const uint64_t okb = __ballot(ok);
+        const uint64_t cfb = __ballot(cf);
+        r.ok += (uint32_t)__popcll(okb);
+        if (okb | cfb) r.any = 1;
+        if (cfb) r.conf |= wave_or64(cf ? (anc[L] | anc[c[u]]) : 0ull);
+    }
+}
+
+// Last level, scanning an existing list.
+__device__ void last_scan(const uint32_t* __restrict__ list, uint32_t size, const uint32_t* __restrict__ vmask,
+                          uint32_t uL, const WaveDFS& S, const uint64_t* anc, uint32_t L, LastRes& r,
+                          const uint32_t* stop)
+{
+    const uint32_t lane = lane_id();
+    r.size = size;
+    for (uint32_t base = 0; base < size; base += 64 * UNR) {
+        uint32_t w[UNR];
+        bool valid[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            const uint32_t i = base + 64 * u + lane;
+            valid[u] = i < size;
+            w[u] = valid[u] ? list[i] : 0u;
+        }
+        classify_last4(valid, w, vmask, uL, S, anc, L, r);
+        if (((base >> 8) & 7u) == 7u && __atomic_load_n(stop, __ATOMIC_RELAXED)) break;
+    }
+}
+
+// Last level, intersecting (parent op) with N(v) and classifying in the same pass.
+__device__ void last_isect(const uint32_t* __restrict__ a, uint32_t na, const uint32_t* __restrict__ abits,
+                           const uint32_t* __restrict__ b, uint32_t nb, const uint32_t* __restrict__ bbits,
+                           const uint32_t* __restrict__ vmask, uint32_t uL, const WaveDFS& S,
+                           const uint64_t* anc, uint32_t L, LastRes& r, unsigned long long& elems,
+                           const uint32_t* stop)
+{
+    if (na > nb) {
+        const uint32_t* t = a; a = b; b = t;
+        uint32_t tn = na; na = nb; nb = tn;
+        bbits = abits;
+    }
+    const uint32_t lane = lane_id();
+    elems += na + nb;
+    uint32_t size = 0;
+    if (nb == 0) { r.size = 0; return; }
+    for (uint32_t base = 0; base < na; base += 64 * UNR) {
+        uint32_t key[UNR];
+        bool valid[UNR], hit[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            const uint32_t i = base + 64 * u + lane;
+            valid[u] = i < na;
+            key[u] = valid[u] ? a[i] : 0u;
+        }
+        member4(b, nb, bbits, key, valid, hit);
+#pragma unroll
+        for (int u = 0; u < UNR; u++) size += (uint32_t)__popcll(__ballot(hit[u]));
+        classify_last4(hit, key, vmask, uL, S, anc, L, r);
+        if (((base >> 8) & 7u) == 7u && __atomic_load_n(stop, __ATOMIC_RELAXED)) break;
+    }
+    r.size = size;
+}
+
+// Batched persistent launch: every query keeps its own counter over its depth-0 candidates
+// (its roots in degree order, hubs first).  Wave w starts on query w mod nq and takes roots
+// from that query's counter; when the query is exhausted or stopped (cap reached), the wave
+// moves on to the next query, and it exits after a full lap over the batch finds no work.
+// So all queries progress at once -- one query's slow start or drain overlaps the others'
+// work -- and waves flow to the queries that still have roots.  A wave keeps a copy of its
+// current query's plan in its own LDS slot and publishes counts and statistics to that
+// query's slot when it moves on.  A batch of one query is the single-query search.
+struct WaveAcc {
+    unsigned long long inter, fsp, elems, outs, scanned, items_done, cnt_pending;
+};
+
+__device__ void flush_stats(const sgm_equery& Q, WaveDFS& S, WaveAcc& a, uint32_t n, bool overflow)
+{
+    flush_count(a.cnt_pending, &Q.st->count, Q.stop, Q.cap);
+    if (lane_id() == 0) {
+        if (a.inter) atomicAdd(&Q.st->intersections, a.inter);
+        if (a.fsp) atomicAdd(&Q.st->fs_prunes, a.fsp);
+        if (a.elems) atomicAdd(&Q.st->elems_read, a.elems);
+        if (a.outs) atomicAdd(&Q.st->elems_out, a.outs);
+        if (a.scanned) atomicAdd(&Q.st->elems_scanned, a.scanned);
+        if (a.items_done) atomicAdd(&Q.st->items_done, a.items_done);
+        if (overflow) { atomicAdd(&Q.st->overflow, 1ull); atomicExch(Q.stop, 1u); }
+        for (uint32_t d = 0; d < n; d++)
+            if (S.nodes[d]) atomicAdd(&Q.st->nodes[d], S.nodes[d]);
+    }
+    a = WaveAcc{0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t d = 0; d < SGM_MAXQ; d++) S.nodes[d] = 0;
+}
+
+__global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__ off,
+                                                    const uint32_t* __restrict__ adj,
+                                                    const uint32_t* __restrict__ items,
+                                                    const sgm_equery* __restrict__ qs, uint32_t nq,
+                                                    uint32_t max_ops,
+                                                    uint32_t* __restrict__ stack, uint32_t stack_words,
+                                                    const uint32_t* __restrict__ host_stop, int use_fs,
+                                                    const uint32_t* __restrict__ hub_slot,
+                                                    const uint32_t* __restrict__ hub_bits, uint32_t hub_words)
+{
+    extern __shared__ uint64_t smem_u64[];
+    uint8_t* smem = reinterpret_cast<uint8_t*>(smem_u64);
+    const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
+    const size_t wave_bytes = enum_wave_bytes(max_ops);
+    uint8_t* wbase = smem + wave * wave_bytes;
+    sgm_plan_dev& P = *reinterpret_cast<sgm_plan_dev*>(wbase);
+    WaveDFS& S = *reinterpret_cast<WaveDFS*>(wbase + enum_plan_bytes());
+    uint32_t* op_off = reinterpret_cast<uint32_t*>(wbase + enum_plan_bytes() + sizeof(WaveDFS));
+    uint32_t* op_size = op_off + max_ops;
+    uint32_t* op_vtx = op_size + max_ops;  // vertex of an adjacency (alias) op, else SGM_INVALID
+    uint32_t* ws = stack + (size_t)(blockIdx.x * (blockDim.x >> 6) + wave) * stack_words;
+    // bitmap of a vertex's adjacency list, or null when it has none
+    auto vbits = [&](uint32_t v) -> const uint32_t* {
+        if (!hub_slot) return nullptr;
+        const uint32_t s = hub_slot[v];
+        return s == SGM_INVALID ? nullptr : hub_bits + (size_t)s * hub_words;
+    };
+
+    for (uint32_t d = 0; d < SGM_MAXQ; d++) S.nodes[d] = 0;
+    WaveAcc acc = {0, 0, 0, 0, 0, 0, 0};
+    uint32_t cur = SGM_INVALID;  // query whose plan sits in this wave's LDS
+    uint32_t n = 0, L = 0, uL = 0, root_u = 0;
+    const uint32_t* vmask = nullptr;
+    uint32_t* stop = nullptr;
+    bool overflow = false;
+    uint32_t poll = 0;
+
+    const uint32_t gwave = blockIdx.x * (blockDim.x >> 6) + wave;
+    uint32_t q = gwave % nq, misses = 0;
+    while (misses < nq) {
+        if (__atomic_load_n(host_stop, __ATOMIC_RELAXED)) break;
+        const sgm_equery& Q = qs[q];
+        uint32_t it = SGM_INVALID;
+        if (!__atomic_load_n(Q.stop, __ATOMIC_RELAXED) && __atomic_load_n(Q.next, __ATOMIC_RELAXED) < Q.n_items) {
+            if (lane == 0) it = atomicAdd(Q.next, 1u);
+            it = uni((uint32_t)__shfl((int)it, 0));
+        }
+        if (it >= Q.n_items) {
+            // exhausted or stopped (cap reached / overflow): try the next query of the batch
+            q = q + 1 == nq ? 0 : q + 1;
+            misses++;
+            continue;
+        }
+        misses = 0;
+        if (q != cur) {
+            if (cur != SGM_INVALID) flush_stats(qs[cur], S, acc, n, false);
+            const uint32_t* src = reinterpret_cast<const uint32_t*>(Q.plan);
+            uint32_t* dst = reinterpret_cast<uint32_t*>(wbase);
+            for (uint32_t i = lane; i < sizeof(sgm_plan_dev) / 4; i += 64) dst[i] = src[i];
+            // every lane reads the whole copy: order the LDS stores before those loads
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            cur = q;
+            n = P.n;
+            L = n - 1;
+            uL = P.order[L];
+            root_u = Q.root_u;
+            vmask = Q.vmask;
+            stop = Q.stop;
+        }
+        const unsigned long long cap = Q.cap;
+        const uint32_t v0 = items[it];
+        if (!((vmask[v0] >> root_u) & 1u)) continue;  // not (or no longer) a candidate of u0
+        acc.items_done++;
+        S.emb[0] = v0;
+        S.nodes[0] += 1;
+        S.stack_top[0] = 0;
+        bool aborted = false;
+
+        // ---- ops of depth 0 (or the whole last level when n == 2)
+        if (n == 2) {
+            LastRes r = {0, 0, 0, 0};
+            const uint32_t o = P.iter_set[1];
+            op_off[o] = off[v0];
+            op_size[o] = off[v0 + 1] - off[v0];
+            last_scan(adj + off[v0], op_size[o], vmask, uL, S, P.anc, L, r, stop);
+            acc.scanned += r.size;
+            acc.cnt_pending += r.ok;
+            S.nodes[L] += r.ok;
+        } else {
+            bool ok = true;
+            uint32_t top = 0;
+            const uint32_t nbo = off[v0], nbs = off[v0 + 1] - nbo;
+            for (uint32_t o = P.op_first[0]; o < P.op_first[1]; o++) {
+                const uint32_t p = P.op_parent[o];
+                if (p == SGM_NOPARENT) { op_off[o] = nbo; op_size[o] = nbs; op_vtx[o] = v0; }
+                else {
+                    const uint32_t need = op_size[p] < nbs ? op_size[p] : nbs;
+                    if (top + need > stack_words) { overflow = true; aborted = true; ok = false; break; }
+                    acc.inter++;
+                    const uint32_t sz = isect_write(op_list(op_off[p], adj, ws), op_size[p],
+                                                    op_vtx[p] != SGM_INVALID ? vbits(op_vtx[p]) : nullptr,
+                                                    adj + nbo, nbs, vbits(v0), ws + top, acc.elems, acc.outs, stop);
+                    op_off[o] = top | STACK_FLAG;
+                    op_size[o] = sz;
+                    op_vtx[o] = SGM_INVALID;
+                    top += sz;
+                }
+                if (op_size[o] == 0) { ok = false; break; }
+            }
+            if (ok) {
+                S.stack_top[1] = top;
+                // enter depth 1
+                S.pos[1] = 0; S.rem[1] = 0; S.flags[1] = 0;
+                S.tot[1] = op_size[P.iter_set[1]];
+                S.lst[1] = op_off[P.iter_set[1]];
+                S.fail[0] = 0;
+                uint32_t d = 1;
+                while (true) {
+                    if ((++poll & 3u) == 0 &&
+                        (__atomic_load_n(stop, __ATOMIC_RELAXED) | __atomic_load_n(host_stop, __ATOMIC_RELAXED))) {
+                        aborted = true;
+                        break;
+                    }
+                    if (S.rem[d] == 0 && ((S.flags[d] & F_PRUNED) || S.pos[d] >= S.tot[d])) {
+                        // end of the loop at depth d (inc/explore.h:1068-1070)
+                        if (use_fs && !(S.flags[d] & (F_PRUNED | F_ANY))) S.fail[d - 1] = P.anc[d];
+                        d--;
+                        if (d == 0) break;
+                        // PruneCheck(d) after the child returned (inc/prune.h:119-137)
+                        if (use_fs) {
+                            if (!((S.fail[d] >> d) & 1ull)) {
+                                S.fail[d - 1] = S.fail[d];
+                                S.flags[d] |= F_PRUNED;
+                                S.rem[d] = 0;
+                                acc.fsp++;
+                            } else S.fail[d - 1] |= S.fail[d];
+                        }
+                        continue;
+                    }
+                    const uint32_t* list = op_list(S.lst[d], adj, ws);
+                    if (S.rem[d] == 0) {
+                        // classify the next 64 elements of depth d's iteration set
+                        const uint32_t u = P.order[d];
+                        const uint32_t base = S.pos[d];
+                        S.pos[d] = base + 64;
+                        acc.scanned += (S.tot[d] - base < 64u) ? (S.tot[d] - base) : 64u;
+                        S.chunk_base[d] = base;
+                        const uint32_t i = base + lane;
+                        const bool valid = i < S.tot[d];
+                        const uint32_t w = valid ? list[i] : 0u;
+                        const bool f = valid && ((vmask[w] >> u) & 1u);
+                        uint32_t c = SGM_INVALID;
+                        if (f)
+                            for (uint32_t j = 0; j < d; j++)
+                                if (S.emb[j] == w) c = j;
+                        const uint64_t okb = __ballot(f && c == SGM_INVALID);
+                        const uint64_t cfb = __ballot(f && c != SGM_INVALID);
+                        if (okb | cfb) S.flags[d] |= F_ANY;
+                        if (use_fs && cfb) {
+                            const uint64_t m = (f && c != SGM_INVALID) ? (P.anc[d] | P.anc[c]) : 0ull;
+                            S.fail[d - 1] |= wave_or64(m);
+                        }
+                        S.rem[d] = okb;
+                        continue;
+                    }
+                    // extend with the next candidate of this chunk
+                    const uint64_t rm = S.rem[d];
+                    const uint32_t j = (uint32_t)__ffsll((long long)rm) - 1u;
+                    S.rem[d] = rm & (rm - 1);
+                    const uint32_t v = list[S.chunk_base[d] + j];
+                    S.emb[d] = v;
+                    S.nodes[d] += 1;
+                    const uint32_t nbo = off[v], nbs = off[v + 1] - nbo;
+                    if (d == n - 2) {
+                        // last level: child at depth L evaluated in one pass
+                        LastRes r = {0, 0, 0, 0};
+                        bool empty = false;
+                        uint32_t conf_v = 0;
+                        if (P.op_first[d] < P.op_first[d + 1]) {
+                            const uint32_t o = P.op_first[d];
+                            const uint32_t p = P.op_parent[o];
+                            if (p == SGM_NOPARENT) {
+                                last_scan(adj + nbo, nbs, vmask, uL, S, P.anc, L, r, stop);
+                            } else {
+                                acc.inter++;
+                                last_isect(op_list(op_off[p], adj, ws), op_size[p],
+                                           op_vtx[p] != SGM_INVALID ? vbits(op_vtx[p]) : nullptr, adj + nbo, nbs,
+                                           vbits(v), vmask, uL, S, P.anc, L, r, acc.elems, stop);
+                                acc.outs += r.size;
+                            }
+                            if (r.size == 0) { empty = true; conf_v = P.op_vdepth[o]; }
+                        } else {
+                            const uint32_t o = P.iter_set[L];
+                            last_scan(op_list(op_off[o], adj, ws), op_size[o], vmask, uL, S, P.anc, L, r, stop);
+                        }
+                        acc.cnt_pending += r.ok;
+                        if (acc.cnt_pending >= FLUSH_AT) flush_count(acc.cnt_pending, &Q.st->count, stop, cap);
+                        S.nodes[L] += r.ok;
+                        acc.scanned += r.size;
+                        if (use_fs) {
+                            uint64_t f;
+                            if (empty) f = P.anc[conf_v];
+                            else {
+                                f = r.conf;
+                                if (r.ok) f = ~0ull;
+                                if (!r.any) f = P.anc[L];
+                            }
+                            S.fail[d] = f;
+                            if (!((f >> d) & 1ull)) {
+                                S.fail[d - 1] = f;
+                                S.flags[d] |= F_PRUNED;
+                                S.rem[d] = 0;
+                                acc.fsp++;
+                            } else S.fail[d - 1] |= f;
+                        }
+                        continue;
+                    }
+                    // compute the set ops of depth d (CacheBuffer::getNextCandidatesImpl)
+                    bool ok2 = true;
+                    uint32_t conf = 0;
+                    uint32_t top = S.stack_top[d];
+                    for (uint32_t o = P.op_first[d]; o < P.op_first[d + 1]; o++) {
+                        const uint32_t p = P.op_parent[o];
+                        if (p == SGM_NOPARENT) { op_off[o] = nbo; op_size[o] = nbs; op_vtx[o] = v; }
+                        else {
+                            const uint32_t need = op_size[p] < nbs ? op_size[p] : nbs;
+                            if (top + need > stack_words) { overflow = true; aborted = true; ok2 = false; break; }
+                            acc.inter++;
+                            const uint32_t sz = isect_write(op_list(op_off[p], adj, ws), op_size[p],
+                                                            op_vtx[p] != SGM_INVALID ? vbits(op_vtx[p]) : nullptr,
+                                                            adj + nbo, nbs, vbits(v), ws + top, acc.elems, acc.outs,
+                                                            stop);
+                            op_off[o] = top | STACK_FLAG;
+                            op_size[o] = sz;
+                            op_vtx[o] = SGM_INVALID;
+                            top += sz;
+                        }
+                        if (op_size[o] == 0) { ok2 = false; conf = P.op_vdepth[o]; break; }
+                    }
+                    if (aborted) break;
+                    if (!ok2) {
+                        if (use_fs) {
+                            const uint64_t f = P.anc[conf];
+                            S.fail[d] = f;
+                            if (!((f >> d) & 1ull)) {
+                                S.fail[d - 1] = f;
+                                S.flags[d] |= F_PRUNED;
+                                S.rem[d] = 0;
+                                acc.fsp++;
+                            } else S.fail[d - 1] |= f;
+                        }
+                        continue;
+                    }
+                    S.stack_top[d + 1] = top;
+                    const uint32_t nd = d + 1;
+                    S.pos[nd] = 0; S.rem[nd] = 0; S.flags[nd] = 0;
+                    S.tot[nd] = op_size[P.iter_set[nd]];
+                    S.lst[nd] = op_off[P.iter_set[nd]];
+                    S.fail[d] = 0;
+                    d = nd;
+                }
+            }
+        }
+        flush_count(acc.cnt_pending, &Q.st->count, stop, cap);
+        if (overflow) {
+            // this query's result is void (its stop word is raised); the host re-runs it
+            flush_stats(Q, S, acc, n, true);
+            overflow = false;
+            cur = SGM_INVALID;
+            continue;
+        }
+        (void)aborted;  // an aborted item means its query (or the whole batch) was stopped
+    }
+    if (cur != SGM_INVALID) flush_stats(qs[cur], S, acc, n, false);
+}
+
+// ================================================================== aux-table enumeration
+// Plain candidate-space search over the auxiliary tables (inc/explore.h:2610-2848 with every
+// pruning option off; CandidatesBuffer::getNextCandidatesImpl :284-354): the candidates of
+// depth d are the intersection, over the backward neighbours p of order[d] in order
+// position, of the table rows edge_matrix[p][order[d]].getNeb(idx(p)) -- positions in
+// C(order[d]), ascending; each extra backward neighbour is one intersection (counted even
+// when an earlier one came out empty, as the reference does).  Injectivity is the only
+// check.  One wave per depth-0 candidate; the last level is intersected and counted in one
+// pass.  Per-wave intermediate lists live on the same HBM stack as the cache search.
+struct WaveAux {
+    uint64_t rem[SGM_MAXQ];
+    unsigned long long nodes[SGM_MAXQ];
+    const uint32_t* lst[SGM_MAXQ];
+    uint32_t emb[SGM_MAXQ];
+    uint32_t iemb[SGM_MAXQ];
+    uint32_t pos[SGM_MAXQ];
+    uint32_t tot[SGM_MAXQ];
+    uint32_t chunk_base[SGM_MAXQ];
+    uint32_t stack_top[SGM_MAXQ + 1];
+};
+
+// Build the candidate list of depth d into (*list, *size); all but the last intersection
+// are materialised on the stack from *top; when `leave_last` is set the last backward
+// neighbour's row is returned separately (for a fused count) in (*last, *nlast).
+__device__ bool aux_list(const sgm_auxplan_dev& A, const WaveAux& S, uint32_t d, uint32_t* ws, uint32_t stack_words,
+                         uint32_t& top, const uint32_t*& list, uint32_t& size, bool leave_last,
+                         const uint32_t*& last, uint32_t& nlast, unsigned long long& elems, unsigned long long& outs,
+                         const uint32_t* stop)
+{
+    const uint32_t f = A.prev_first[d], e_end = A.prev_first[d + 1];
+    const uint32_t np = e_end - f;
+    {
+        const uint32_t* o = A.tab_off[f];
+        const uint32_t r = S.iemb[A.prev_depth[f]];
+        const uint32_t b = o[r];
+        list = A.tab_idx[f] + b;
+        size = o[r + 1] - b;
+    }
+    const uint32_t stop_at = leave_last ? e_end - 1 : e_end;
+    for (uint32_t e = f + 1; e < stop_at; e++) {
+        const uint32_t* o = A.tab_off[e];
+        const uint32_t r = S.iemb[A.prev_depth[e]];
+        const uint32_t b = o[r], nb = o[r + 1] - b;
+        const uint32_t need = size < nb ? size : nb;
+        if (top + need > stack_words) return false;
+        const uint32_t sz = isect_write(list, size, nullptr, A.tab_idx[e] + b, nb, nullptr, ws + top, elems, outs, stop);
+        list = ws + top;
+        size = sz;
+        top += sz;
+    }
+    if (leave_last && np > 1) {
+        const uint32_t e = e_end - 1;
+        const uint32_t* o = A.tab_off[e];
+        const uint32_t r = S.iemb[A.prev_depth[e]];
+        const uint32_t b = o[r];
+        last = A.tab_idx[e] + b;
+        nlast = o[r + 1] - b;
+    } else {
+        last = nullptr;
+        nlast = 0;
+    }
+    return true;
+}
+
+// count positions whose vertex is not used by depths < L
+__device__ __forceinline__ uint32_t aux_count_chunk(bool valid, uint32_t vidx, const uint32_t* __restrict__ cand,
+                                                    const WaveAux& S, uint32_t L)
+{
+    bool ok = false;
+    if (valid) {
+        const uint32_t v = cand[vidx];
+        ok = true;
+        for (uint32_t j = 0; j < L; j++)
+            if (S.emb[j] == v) ok = false;
+    }
+    return (uint32_t)__popcll(__ballot(ok));
+}
+
+__global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restrict__ gA, uint32_t n_items,
+                                                  uint32_t* __restrict__ stack, uint32_t stack_words,
+                                                  uint32_t* __restrict__ work, sgm_stats_dev* __restrict__ gst,
+                                                  uint32_t* __restrict__ stop, unsigned long long cap)
+{
+    extern __shared__ uint64_t smem_u64[];
+    uint8_t* smem = reinterpret_cast<uint8_t*>(smem_u64);
+    const sgm_auxplan_dev& A = *gA;
+    const uint32_t n = A.n;
+    const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
+    WaveAux& S = *reinterpret_cast<WaveAux*>(smem + wave * ((sizeof(WaveAux) + 15) & ~(size_t)15));
+    uint32_t* ws = stack + (size_t)(blockIdx.x * (blockDim.x >> 6) + wave) * stack_words;
+    const uint32_t L = n - 1;
+    for (uint32_t d = 0; d < SGM_MAXQ; d++) S.nodes[d] = 0;
+    unsigned long long inter = 0, elems = 0, outs = 0, scanned = 0, cnt_pending = 0, items_done = 0;
+    bool overflow = false;
+    uint32_t poll = 0;
+    while (true) {
+        uint32_t it = 0;
+        if (lane == 0) it = atomicAdd(work, 1u);
+        it = uni(__shfl((int)it, 0));
+        if (it >= n_items) break;
+        if (__atomic_load_n(stop, __ATOMIC_RELAXED)) break;
+        items_done++;
+        bool aborted = false;
+        S.emb[0] = A.cand[0][it];
+        S.iemb[0] = it;
+        S.nodes[0] += 1;
+        S.stack_top[1] = 0;
+        uint32_t d = 1;
+        // enter depth d (d <= L): build the list; at d == L count directly
+        bool entering = true;
+        while (true) {
+            if ((++poll & 63u) == 0 && __atomic_load_n(stop, __ATOMIC_RELAXED)) { aborted = true; break; }
+            if (entering) {
+                entering = false;
+                const uint32_t np = A.prev_first[d + 1] - A.prev_first[d];
+                inter += np - 1;
+                uint32_t top = S.stack_top[d];
+                const uint32_t* list;
+                uint32_t size;
+                const uint32_t* last;
+                uint32_t nlast;
+                const bool is_last = d == L;
+                if (!aux_list(A, S, d, ws, stack_words, top, list, size, is_last, last, nlast, elems, outs, stop)) {
+                    overflow = true; aborted = true; break;
+                }
+                if (is_last) {
+                    const uint32_t* cand = A.cand[d];
+                    uint32_t ok = 0;
+                    if (!last) {
+                        scanned += size;
+                        for (uint32_t base = 0; base < size; base += 64) {
+                            const uint32_t i = base + lane;
+                            const bool valid = i < size;
+                            ok += aux_count_chunk(valid, valid ? list[i] : 0u, cand, S, L);
+                        }
+                    } else {
+                        const uint32_t* a = list; uint32_t na = size;
+                        const uint32_t* b = last; uint32_t nb = nlast;
+                        if (na > nb) { const uint32_t* t = a; a = b; b = t; uint32_t tn = na; na = nb; nb = tn; }
+                        elems += na + nb;
+                        for (uint32_t base = 0; base < na; base += 64) {
+                            const uint32_t i = base + lane;
+                            const uint32_t key = i < na ? a[i] : 0u;
+                            const bool hit = i < na && lower_found(b, nb, key);
+                            const uint32_t h = (uint32_t)__popcll(__ballot(hit));
+                            outs += h;
+                            scanned += h;
+                            ok += aux_count_chunk(hit, key, cand, S, L);
+                        }
+                    }
+                    cnt_pending += ok;
+                    if (cnt_pending >= FLUSH_AT) flush_count(cnt_pending, &gst->count, stop, cap);
+                    S.nodes[L] += ok;
+                    d--;
+                    if (d == 0) break;
+                    continue;
+                }
+                S.stack_top[d + 1] = top;
+                S.lst[d] = list;
+                S.tot[d] = size;
+                S.pos[d] = 0;
+                S.rem[d] = 0;
+                continue;
+            }
+            if (S.rem[d] == 0 && S.pos[d] >= S.tot[d]) {
+                d--;
+                if (d == 0) break;
+                continue;
+            }
+            const uint32_t* list = S.lst[d];
+            if (S.rem[d] == 0) {
+                const uint32_t base = S.pos[d];
+                S.pos[d] = base + 64;
+                S.chunk_base[d] = base;
+                const uint32_t i = base + lane;
+                const bool valid = i < S.tot[d];
+                scanned += (S.tot[d] - base < 64u) ? (S.tot[d] - base) : 64u;
+                bool ok = false;
+                if (valid) {
+                    const uint32_t v = A.cand[d][list[i]];
+                    ok = true;
+                    for (uint32_t j = 0; j < d; j++)
+                        if (S.emb[j] == v) ok = false;
+                }
+                S.rem[d] = __ballot(ok);
+                continue;
+            }
+            const uint64_t rm = S.rem[d];
+            const uint32_t j = (uint32_t)__ffsll((long long)rm) - 1u;
+            S.rem[d] = rm & (rm - 1);
+            const uint32_t vidx = list[S.chunk_base[d] + j];
+            S.emb[d] = A.cand[d][vidx];
+            S.iemb[d] = vidx;
+            S.nodes[d] += 1;
+            d++;
+            entering = true;
+        }
+        flush_count(cnt_pending, &gst->count, stop, cap);
+        if (aborted || overflow) break;
+    }
+    flush_count(cnt_pending, &gst->count, stop, cap);
+    if (lane == 0) {
+        atomicAdd(&gst->intersections, inter);
+        atomicAdd(&gst->elems_read, elems);
+        atomicAdd(&gst->elems_out, outs);
+        atomicAdd(&gst->elems_scanned, scanned);
+        atomicAdd(&gst->items_done, items_done);
+        if (overflow) { atomicAdd(&gst->overflow, 1ull); atomicExch(stop, 1u); }
+        for (uint32_t dd = 0; dd < n; dd++)
+            if (S.nodes[dd]) atomicAdd(&gst->nodes[dd], S.nodes[dd]);
+    }
+}
+
+}  // namespace sgm
+
+// ================================================================== launch wrappers
+namespace sgm {
+
+size_t enum_smem_bytes(uint32_t max_ops) { return 4 * enum_wave_bytes(max_ops); }
+
+void launch_enum_cache(hipStream_t s, uint32_t blocks, const uint32_t* off, const uint32_t* adj, const uint32_t* items,
+                       const sgm_equery* qs, uint32_t nq, uint32_t max_ops, uint32_t* stack, uint32_t stack_words,
+                       const uint32_t* host_stop, int use_fs, const uint32_t* hub_slot, const uint32_t* hub_bits,
+                       uint32_t hub_words)
+{
+    hipLaunchKernelGGL(k_enum_cache, dim3(blocks), dim3(256), 4 * enum_wave_bytes(max_ops), s, off, adj, items, qs, nq,
+                       max_ops, stack, stack_words, host_stop, use_fs, hub_slot, hub_bits, hub_words);
+}
+
+void launch_enum_aux(hipStream_t s, uint32_t blocks, const sgm_auxplan_dev* plan, uint32_t n_items, uint32_t* stack,
+                     uint32_t stack_words, uint32_t* work, sgm_stats_dev* st, uint32_t* stop, unsigned long long cap)
+{
+    const size_t smem = 4 * ((sizeof(WaveAux) + 15) & ~(size_t)15);
+    hipLaunchKernelGGL(k_enum_aux, dim3(blocks), dim3(256), smem, s, plan, n_items, stack, stack_words, work, st, stop,
+                       cap);
+}
+
+}  // namespace sgm

@@ -59,6 +59,19 @@ struct sgm_stats_dev {
     unsigned long long items_done;
 };
 
+// One query of a batched cache + filter + failing-set enumeration.  Its depth-0 items are
+// the prefix of length n_items of the degree-sorted vertex list, handed out by `next`.
+struct sgm_equery {
+    const sgm_plan_dev* plan;
+    const uint32_t* vmask;
+    uint32_t* stop;                 // raised when the cap is reached (or the run overflowed)
+    sgm_stats_dev* st;
+    uint32_t* next;                 // next depth-0 item of this query
+    unsigned long long cap;
+    uint32_t n_items;
+    uint32_t root_u;
+};
+
 // Per-query description used by the batched filter kernels.
 struct sgm_qdesc {
     uint32_t n;

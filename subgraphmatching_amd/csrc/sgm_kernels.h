@@ -1,4 +1,5 @@
-// Launch wrappers of the gfx950 kernels (implemented in sgm_kernels.hip).
+// Launch wrappers of the gfx950 kernels (filters and tables in sgm_kernels.hip,
+// enumeration in sgm_enum.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stddef.h>
@@ -27,14 +28,14 @@ void launch_compact(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V)
 void launch_table(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* vmask,
                   const uint32_t* rows, uint32_t nrows, uint32_t b, const uint64_t* bm_b, const uint32_t* rank_b,
                   uint32_t* rowcnt, uint32_t* rowoff, uint32_t* maxv, uint32_t* eidx, uint32_t* ev, int pass);
-size_t enum_smem_bytes(uint32_t n_ops, uint32_t waves);
+// sgm_enum.hip
+size_t enum_smem_bytes(uint32_t max_ops);
 void launch_enum_aux(hipStream_t s, uint32_t blocks, const sgm_auxplan_dev* plan, uint32_t n_items, uint32_t* stack,
                      uint32_t stack_words, uint32_t* work, sgm_stats_dev* st, uint32_t* stop, unsigned long long cap);
-void launch_enum_cache(hipStream_t s, uint32_t blocks, const uint32_t* off, const uint32_t* adj,
-                       const uint32_t* vmask, const uint32_t* items, uint32_t n_items, uint32_t root_u,
-                       const sgm_plan_dev* plan, uint32_t n_ops, uint32_t* stack, uint32_t stack_words,
-                       uint32_t* work, sgm_stats_dev* st, uint32_t* stop, unsigned long long cap, int use_fs,
-                       const uint32_t* hub_slot, const uint32_t* hub_bits, uint32_t hub_words);
+void launch_enum_cache(hipStream_t s, uint32_t blocks, const uint32_t* off, const uint32_t* adj, const uint32_t* items,
+                       const sgm_equery* qs, uint32_t nq, uint32_t max_ops, uint32_t* stack, uint32_t stack_words,
+                       const uint32_t* host_stop, int use_fs, const uint32_t* hub_slot, const uint32_t* hub_bits,
+                       uint32_t hub_words);
 void launch_hub_bits(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* hubs, uint32_t n_hubs,
                      uint32_t* bits, uint32_t words);
 }  // namespace sgm

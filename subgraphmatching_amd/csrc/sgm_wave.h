@@ -1,0 +1,49 @@
+// Wave64 primitives shared by the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sgm {
+
+// ------------------------------------------------------------------ wave primitives
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint32_t mbcnt64(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ uint32_t rdlane(uint32_t x, uint32_t l)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)x, (int)l);
+}
+
+__device__ __forceinline__ uint32_t uni(uint32_t x)
+{
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)x);
+}
+
+__device__ __forceinline__ uint32_t wave_or32(uint32_t x)
+{
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) x |= (uint32_t)__shfl_xor((int)x, s);
+    return x;
+}
+
+__device__ __forceinline__ uint64_t wave_or64(uint64_t x)
+{
+    uint32_t lo = wave_or32((uint32_t)x), hi = wave_or32((uint32_t)(x >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ bool lower_found(const uint32_t* __restrict__ a, uint32_t n, uint32_t key)
+{
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    return lo < n && a[lo] == key;
+}
+
+}  // namespace sgm

@@ -169,6 +169,30 @@ def test_enum_capped_config1(g1):
         assert res.ans <= 1_000_000
 
 
+def test_enum_batch_equals_oracle(g_small):
+    """One persistent launch over many queries: per-query counts (capped and uncapped) and,
+    for uncapped queries, intersections / nodes / prunes equal the oracle's."""
+    qs = _mixed(g_small, 24, 108, 3, 12)
+    cans = [sgm.CandidateParam(g_small, q) for q in qs]
+    oks = sgm.filter_batch(cans)
+    orders = [sgm.Order_RI(q) for q in qs]
+    for cap in (None, 50_000):
+        say("batch cap", cap)
+        refs = []
+        for q, can, order in zip(qs, cans, orders):
+            ok_r, c_r = O.gql(g_small, q)
+            ref = O.enum_cache(g_small, q, c_r, order, cap=cap if cap else 3_000_000)
+            refs.append(ref)
+        keep = [i for i in range(len(qs)) if cap or not refs[i]["capped"]]
+        res = sgm.enumerate_batch([cans[i] for i in keep], [orders[i] for i in keep], cap=cap)
+        for r, i in zip(res, keep):
+            assert r.ans == refs[i]["count"], (qs[i].name, cap, r.ans, refs[i]["count"])
+            if not cap:
+                assert r.intersections == refs[i]["intersections"], qs[i].name
+                assert r.nodes == refs[i]["nodes"], qs[i].name
+                assert r.fs_prunes == refs[i]["fs_prunes"], qs[i].name
+
+
 def test_single_edge_and_star_queries(g_small):
     deg = g_small.degrees().astype(np.int64)
     cases = [(query_from_edges(2, [(0, 1)]), int(deg.sum())),
