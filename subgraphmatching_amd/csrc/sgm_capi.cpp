@@ -52,6 +52,7 @@ struct sgm_graph {
     // kHubBudgetBytes); hub_slot[v] = bitmap index or SGM_INVALID
     uint32_t n_hubs = 0;
     uint32_t hub_words = 0;
+    uint32_t n_above_lane = 0;  // vertices with degree > sgm::gql_lane_degree() (a by_deg prefix)
     uint32_t* d_hub_slot = nullptr;
     uint32_t* d_hub_bits = nullptr;
 };
@@ -262,6 +263,11 @@ int sgm_graph_upload(sgm_ctx* c, const uint32_t* off, const uint32_t* adj, uint3
         for (uint64_t x = d; x <= maxdeg; x++) s += hist[x];
         g->ge_cnt[d] = (uint32_t)s;
     }
+    {
+        uint64_t s = 0;
+        for (uint64_t x = (uint64_t)sgm::gql_lane_degree() + 1; x <= maxdeg; x++) s += hist[x];
+        g->n_above_lane = (uint32_t)s;
+    }
     hipError_t e = hipSetDevice(c->device);
     if (e == hipSuccess) e = dalloc(&g->d_off, (size_t)V + 1);
     if (e == hipSuccess) e = dalloc(&g->d_adj, adj_len);
@@ -450,7 +456,9 @@ int sgm_filter_batch(sgm_query* const* qs, uint32_t nq, int kind, int32_t* statu
                 const uint32_t u = ph % n;
                 mc = std::max(mc, g->ge_cnt[std::min<uint32_t>(qd[i].qdeg[u], 32)]);
             }
-            if (mc) sgm::launch_gql_phase(c->stream, g->d_off, g->d_adj, g->d_bydeg, g->d_gecnt, d_qd, nq, ph, mc);
+            if (mc)
+                sgm::launch_gql_phase(c->stream, g->d_off, g->d_adj, g->d_bydeg, g->d_gecnt, d_qd, nq, ph, mc,
+                                      g->n_above_lane);
         }
     }
     HIPCHK(c, hipEventRecord(c->fev[2], c->stream));

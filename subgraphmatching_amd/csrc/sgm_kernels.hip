@@ -116,13 +116,14 @@ __device__ bool lsm_wave(const uint32_t* __restrict__ nb, uint32_t deg, const ui
 // matter).  A round-2 phase whose left-side sets did not change since round 1 would
 // repeat round 1's answers exactly and is skipped.
 //
-// A wave takes 64 candidates at a time: lanes fetch their ids, validity bits and
-// offsets together.  A candidate of degree <= GQL_LANE_DEG is first decided by its own lane
-// from its right-side masks: a greedy matching that saturates the left side proves the
-// answer true; a left vertex no right vertex covers, or fewer useful right vertices than
-// left vertices, proves it false.  Only the remaining candidates (and all higher-degree
-// ones) go through the exact wave matcher, one after another.
-#define GQL_LANE_DEG 32u
+// Work units: each candidate of degree > GQL_LANE_DEG (the first n_big entries of the
+// degree order) is one unit for a whole wave (the exact wave matcher); the remaining
+// candidates come in units of 64, one per lane: lanes fetch their ids, validity bits and
+// offsets together and decide their own candidate from its right-side masks where they
+// can -- a greedy matching that saturates the left side proves the answer true; a left
+// vertex no right vertex covers, or fewer useful right vertices than left vertices, proves
+// it false -- and the wave runs the exact matcher on the few undecided ones.
+#define GQL_LANE_DEG 128u
 
 __device__ __forceinline__ int lsm_lane(const uint32_t* __restrict__ adj, uint32_t o, uint32_t deg,
                                         const uint32_t* __restrict__ vm, uint32_t L, uint32_t k)
@@ -153,7 +154,7 @@ __global__ void __launch_bounds__(256) k_gql_phase(const uint32_t* __restrict__ 
                                                    const uint32_t* __restrict__ adj,
                                                    const uint32_t* __restrict__ by_deg,
                                                    const uint32_t* __restrict__ ge_cnt,
-                                                   const sgm_qdesc* __restrict__ qs, uint32_t phase)
+                                                   const sgm_qdesc* __restrict__ qs, uint32_t phase, uint32_t n_big)
 {
     const sgm_qdesc* d = qs + blockIdx.y;
     const uint32_t n = d->n;
@@ -176,7 +177,20 @@ __global__ void __launch_bounds__(256) k_gql_phase(const uint32_t* __restrict__ 
     const uint32_t w0 = blockIdx.x * waves + (threadIdx.x >> 6);
     const uint32_t stride = gridDim.x * waves;
     uint32_t local_removed = 0;
-    for (uint32_t base = w0 * 64; base < cnt; base += stride * 64) {
+    const uint32_t nh = cnt < n_big ? cnt : n_big;
+    const uint32_t nunits = nh + (cnt - nh + 63) / 64;
+    for (uint32_t unit = w0; unit < nunits; unit += stride) {
+        if (unit < nh) {
+            const uint32_t v = by_deg[unit];
+            if (!((vm[v] >> u) & 1u)) continue;
+            const uint32_t o = off[v];
+            if (!lsm_wave(adj + o, off[v + 1] - o, vm, L, k)) {
+                if (lane == 0) atomicAnd(&vm[v], ~(1u << u));
+                local_removed++;
+            }
+            continue;
+        }
+        const uint32_t base = nh + (unit - nh) * 64;
         const uint32_t i = base + lane;
         const uint32_t v = i < cnt ? by_deg[i] : 0u;
         const bool valid = i < cnt && ((vm[v] >> u) & 1u);
@@ -398,11 +412,16 @@ void launch_nlf(hipStream_t s, const uint32_t* off, uint32_t V, const sgm_qdesc*
 }
 
 void launch_gql_phase(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* by_deg,
-                      const uint32_t* ge_cnt, const sgm_qdesc* qs, uint32_t nq, uint32_t phase, uint32_t max_cand)
+                      const uint32_t* ge_cnt, const sgm_qdesc* qs, uint32_t nq, uint32_t phase, uint32_t max_cand,
+                      uint32_t n_big)
 {
-    dim3 grid(clampg(((uint64_t)max_cand + 255) / 256, 1, 4096), nq);
-    hipLaunchKernelGGL(k_gql_phase, grid, dim3(256), 0, s, off, adj, by_deg, ge_cnt, qs, phase);
+    const uint64_t nh = max_cand < n_big ? max_cand : n_big;
+    const uint64_t units = nh + (max_cand - nh + 63) / 64;
+    dim3 grid(clampg((units + 3) / 4, 1, 4096), nq);
+    hipLaunchKernelGGL(k_gql_phase, grid, dim3(256), 0, s, off, adj, by_deg, ge_cnt, qs, phase, n_big);
 }
+
+uint32_t gql_lane_degree() { return GQL_LANE_DEG; }
 
 void launch_bitmaps(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V)
 {
