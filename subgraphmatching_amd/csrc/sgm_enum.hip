@@ -313,10 +313,19 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
     bool overflow = false;
     uint32_t poll = 0;
 
+    // Visiting order (L2 locality): workgroups are dispatched round-robin over the 8 XCDs, so
+    // blocks with the same blockIdx.x mod 8 share an L2.  Queries are split into 8 classes
+    // (q mod 8); a wave serves its own class first (starting at a wave-dependent query of
+    // the class) and only then helps the other classes, so each L2 mostly holds the
+    // candidate words of one eighth of the batch.
     const uint32_t gwave = blockIdx.x * (blockDim.x >> 6) + wave;
-    uint32_t q = gwave % nq, misses = 0;
-    while (misses < nq) {
+    const uint32_t ncls = nq < 8u ? nq : 8u;
+    auto csize = [&](uint32_t c) { return (nq - c + ncls - 1) / ncls; };
+    uint32_t cls = blockIdx.x % ncls;
+    uint32_t k = (gwave / ncls) % csize(cls), seen = 0, classes = 0;
+    while (classes < ncls) {
         if (__atomic_load_n(host_stop, __ATOMIC_RELAXED)) break;
+        const uint32_t q = cls + ncls * k;
         const sgm_equery& Q = qs[q];
         uint32_t it = SGM_INVALID;
         if (!__atomic_load_n(Q.stop, __ATOMIC_RELAXED) && __atomic_load_n(Q.next, __ATOMIC_RELAXED) < Q.n_items) {
@@ -324,12 +333,20 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
             it = uni((uint32_t)__shfl((int)it, 0));
         }
         if (it >= Q.n_items) {
-            // exhausted or stopped (cap reached / overflow): try the next query of the batch
-            q = q + 1 == nq ? 0 : q + 1;
-            misses++;
+            // exhausted or stopped (cap reached / overflow): try the next query of the class,
+            // then the next class; a full pass over every class without work ends the wave
+            const uint32_t cs = csize(cls);
+            k = k + 1 == cs ? 0 : k + 1;
+            if (++seen == cs) {
+                cls = cls + 1 == ncls ? 0 : cls + 1;
+                k = (gwave / ncls) % csize(cls);
+                seen = 0;
+                classes++;
+            }
             continue;
         }
-        misses = 0;
+        seen = 0;
+        classes = 0;
         if (q != cur) {
             if (cur != SGM_INVALID) flush_stats(qs[cur], S, acc, n, false);
             const uint32_t* src = reinterpret_cast<const uint32_t*>(Q.plan);
