@@ -31,6 +31,8 @@ struct sgm_ctx {
     uint32_t batch_cap = 0, batch_cap2 = 0;
     uint32_t* d_stack = nullptr;
     size_t stack_bytes = 0;
+    uint32_t* d_gqls = nullptr;      // GQL filter scratch: touch + dirty marks, 2*V words per query
+    size_t gqls_words = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t fev[4] = {nullptr, nullptr, nullptr, nullptr};
     double filter_ms[3] = {0, 0, 0};
@@ -52,10 +54,14 @@ struct sgm_graph {
     // kHubBudgetBytes); hub_slot[v] = bitmap index or SGM_INVALID
     uint32_t n_hubs = 0;
     uint32_t hub_words = 0;
-    uint32_t n_above_lane = 0;  // vertices with degree > sgm::gql_lane_degree() (a by_deg prefix)
     uint32_t* d_hub_slot = nullptr;
     uint32_t* d_hub_bits = nullptr;
+    uint8_t* d_nbrdeg = nullptr;  // 32*V bytes: order statistics of neighbour degrees (GQL shortcut)
+    uint8_t* d_degc = nullptr;    // V bytes: min(degree, 255)
 };
+
+// GQL scratch is allocated per filter batch; larger batches run in chunks of this size.
+static const uint64_t kGqlScratchBytes = 8ull << 30;
 
 static const uint32_t kHubMinDegree = 64;
 static const uint64_t kHubBudgetBytes = 1ull << 30;
@@ -194,6 +200,7 @@ void sgm_ctx_destroy(sgm_ctx* c)
     dfree(c->d_qstats);
     dfree(c->d_qstop);
     dfree(c->d_stack);
+    dfree(c->d_gqls);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     for (auto& e : c->fev)
@@ -263,11 +270,6 @@ int sgm_graph_upload(sgm_ctx* c, const uint32_t* off, const uint32_t* adj, uint3
         for (uint64_t x = d; x <= maxdeg; x++) s += hist[x];
         g->ge_cnt[d] = (uint32_t)s;
     }
-    {
-        uint64_t s = 0;
-        for (uint64_t x = (uint64_t)sgm::gql_lane_degree() + 1; x <= maxdeg; x++) s += hist[x];
-        g->n_above_lane = (uint32_t)s;
-    }
     hipError_t e = hipSetDevice(c->device);
     if (e == hipSuccess) e = dalloc(&g->d_off, (size_t)V + 1);
     if (e == hipSuccess) e = dalloc(&g->d_adj, adj_len);
@@ -277,6 +279,18 @@ int sgm_graph_upload(sgm_ctx* c, const uint32_t* off, const uint32_t* adj, uint3
     if (e == hipSuccess && adj_len) e = hipMemcpy(g->d_adj, adj, sizeof(uint32_t) * adj_len, hipMemcpyHostToDevice);
     if (e == hipSuccess && V) e = hipMemcpy(g->d_bydeg, bydeg.data(), sizeof(uint32_t) * V, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(g->d_gecnt, g->ge_cnt, sizeof(g->ge_cnt), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = dalloc(&g->d_nbrdeg, (size_t)32 * V);
+    if (e == hipSuccess) e = dalloc(&g->d_degc, V);
+    if (e == hipSuccess && V) {
+        std::vector<uint8_t> degc(V);
+        for (uint32_t v = 0; v < V; v++) degc[v] = (uint8_t)std::min<uint32_t>(off[v + 1] - off[v], 255);
+        e = hipMemcpy(g->d_degc, degc.data(), V, hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess && V) {
+        sgm::launch_nbr_deg(nullptr, g->d_off, g->d_adj, V, g->d_nbrdeg);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+    }
     // hub bitmaps: the highest-degree vertices first (by_deg order), as many as fit
     if (e == hipSuccess && V) {
         g->hub_words = (V + 31) / 32;
@@ -315,6 +329,8 @@ void sgm_graph_destroy(sgm_graph* g)
     dfree(g->d_gecnt);
     dfree(g->d_hub_slot);
     dfree(g->d_hub_bits);
+    dfree(g->d_nbrdeg);
+    dfree(g->d_degc);
     delete g;
 }
 
@@ -376,7 +392,7 @@ int sgm_query_create(sgm_graph* g, const uint32_t* qoff, const uint32_t* qadj, u
     const uint32_t V = g->V, W = (V + 63) / 64;
     hipError_t e = hipSetDevice(c->device);
     if (e == hipSuccess) e = dalloc(&q->d_vmask, V);
-    if (e == hipSuccess) e = dalloc(&q->d_removed, 2 * SGM_MAXQ);
+    if (e == hipSuccess) e = dalloc(&q->d_removed, SGM_GQL_FLAG_WORDS);
     if (e == hipSuccess) e = dalloc(&q->d_bm, (size_t)n * W);
     if (e == hipSuccess) e = dalloc(&q->d_rank, (size_t)n * W);
     if (e == hipSuccess) e = dalloc(&q->d_cnt, SGM_MAXQ);
@@ -388,6 +404,12 @@ int sgm_query_create(sgm_graph* g, const uint32_t* qoff, const uint32_t* qadj, u
         std::memset(&qd, 0, sizeof qd);
         qd.n = n;
         for (uint32_t u = 0; u < n; u++) { qd.qdeg[u] = qoff[u + 1] - qoff[u]; qd.nbr[u] = q->nbr[u]; }
+        for (uint32_t u = 0; u < n; u++) {  // neighbour degrees, descending (GQL degree-table test)
+            std::vector<uint32_t> r;
+            for (uint32_t p = qoff[u]; p < qoff[u + 1]; p++) r.push_back(qd.qdeg[qadj[p]]);
+            std::sort(r.begin(), r.end(), [](uint32_t a, uint32_t b) { return a > b; });
+            for (size_t j = 0; j < r.size(); j++) qd.req[u][j] = (uint8_t)r[j];
+        }
         qd.vmask = q->d_vmask;
         qd.removed = q->d_removed;
         sgm_cdesc cd;
@@ -414,30 +436,30 @@ int sgm_query_create(sgm_graph* g, const uint32_t* qoff, const uint32_t* qadj, u
 }
 
 // ------------------------------------------------------------------------------ filter
-int sgm_filter_batch(sgm_query* const* qs, uint32_t nq, int kind, int32_t* status)
+// One chunk of a filter batch: NLF (+ GQL) for nq queries in one set of launches, then the
+// candidate bitmaps and counts.  Adds the device time of the three stages to ms[].
+static int filter_chunk(sgm_ctx* c, sgm_graph* g, sgm_query* const* qs, uint32_t nq, int kind, double ms[3])
 {
-    if (!qs || nq == 0) return SGM_EINVAL;
-    sgm_graph* g = qs[0]->g;
-    sgm_ctx* c = g->ctx;
-    if (kind != SGM_FILTER_NLF && kind != SGM_FILTER_GQL) return set_err(c, SGM_EINVAL, "unknown filter kind %d", kind);
-    for (uint32_t i = 0; i < nq; i++)
-        if (!qs[i] || qs[i]->g != g) return set_err(c, SGM_EINVAL, "queries of a batch must share one graph");
-    HIPCHK(c, hipSetDevice(c->device));
     const uint32_t V = g->V;
-    // gather the per-query descriptors into one device array
     std::vector<sgm_qdesc> qd(nq);
     std::vector<sgm_cdesc> cd(nq);
     uint32_t max_n = 0;
     for (uint32_t i = 0; i < nq; i++) {
         qd[i] = qs[i]->hq;
         cd[i] = qs[i]->hc;
+        if (kind == SGM_FILTER_GQL) {
+            qd[i].touch = c->d_gqls + (size_t)2 * V * i;
+            qd[i].dirty = qd[i].touch + V;
+        }
         max_n = std::max(max_n, qs[i]->q.n);
         free_tables(qs[i]);
         dfree(qs[i]->d_list);
         qs[i]->d_list = nullptr;
         qs[i]->compacted = false;
-        HIPCHK(c, hipMemsetAsync(qs[i]->d_removed, 0, sizeof(uint32_t) * 2 * SGM_MAXQ, c->stream));
+        HIPCHK(c, hipMemsetAsync(qs[i]->d_removed, 0, sizeof(uint32_t) * SGM_GQL_FLAG_WORDS, c->stream));
     }
+    if (kind == SGM_FILTER_GQL && V)
+        HIPCHK(c, hipMemsetAsync(c->d_gqls, 0, sizeof(uint32_t) * 2 * (size_t)V * nq, c->stream));
     sgm_qdesc* d_qd = nullptr;
     sgm_cdesc* d_cd = nullptr;
     HIPCHK(c, dalloc(&d_qd, nq));
@@ -447,20 +469,9 @@ int sgm_filter_batch(sgm_query* const* qs, uint32_t nq, int kind, int32_t* statu
     HIPCHK(c, hipEventRecord(c->fev[0], c->stream));
     sgm::launch_nlf(c->stream, g->d_off, V, d_qd, nq);
     HIPCHK(c, hipEventRecord(c->fev[1], c->stream));
-    if (kind == SGM_FILTER_GQL) {
-        for (uint32_t ph = 0; ph < 2 * max_n; ph++) {
-            uint32_t mc = 0;
-            for (uint32_t i = 0; i < nq; i++) {
-                const uint32_t n = qs[i]->q.n;
-                if (ph >= 2 * n) continue;
-                const uint32_t u = ph % n;
-                mc = std::max(mc, g->ge_cnt[std::min<uint32_t>(qd[i].qdeg[u], 32)]);
-            }
-            if (mc)
-                sgm::launch_gql_phase(c->stream, g->d_off, g->d_adj, g->d_bydeg, g->d_gecnt, d_qd, nq, ph, mc,
-                                      g->n_above_lane);
-        }
-    }
+    if (kind == SGM_FILTER_GQL && V)
+        for (uint32_t ph = 0; ph < 2 * max_n; ph++)
+            sgm::launch_gql_phase(c->stream, g->d_off, g->d_adj, g->d_degc, g->d_nbrdeg, V, d_qd, nq, ph);
     HIPCHK(c, hipEventRecord(c->fev[2], c->stream));
     sgm::launch_bitmaps(c->stream, d_cd, nq, V);
     sgm::launch_rank_scan(c->stream, d_cd, nq, V, max_n);
@@ -474,10 +485,41 @@ int sgm_filter_batch(sgm_query* const* qs, uint32_t nq, int kind, int32_t* statu
     dfree(d_cd);
     if (se != hipSuccess) return set_err(c, SGM_EHIP, "filter: %s", hipGetErrorString(se));
     for (int k = 0; k < 3; k++) {
-        float ms = 0;
-        HIPCHK(c, hipEventElapsedTime(&ms, c->fev[k], c->fev[k + 1]));
-        c->filter_ms[k] = ms;
+        float t = 0;
+        HIPCHK(c, hipEventElapsedTime(&t, c->fev[k], c->fev[k + 1]));
+        ms[k] += t;
     }
+    return SGM_OK;
+}
+
+int sgm_filter_batch(sgm_query* const* qs, uint32_t nq, int kind, int32_t* status)
+{
+    if (!qs || nq == 0) return SGM_EINVAL;
+    sgm_graph* g = qs[0]->g;
+    sgm_ctx* c = g->ctx;
+    if (kind != SGM_FILTER_NLF && kind != SGM_FILTER_GQL) return set_err(c, SGM_EINVAL, "unknown filter kind %d", kind);
+    for (uint32_t i = 0; i < nq; i++)
+        if (!qs[i] || qs[i]->g != g) return set_err(c, SGM_EINVAL, "queries of a batch must share one graph");
+    HIPCHK(c, hipSetDevice(c->device));
+    const uint64_t per_q = (uint64_t)2 * g->V * sizeof(uint32_t);
+    uint32_t chunk = nq;
+    if (kind == SGM_FILTER_GQL && per_q) {
+        chunk = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(nq, kGqlScratchBytes / per_q));
+        const size_t words = (size_t)2 * g->V * chunk;
+        if (c->gqls_words < words) {
+            dfree(c->d_gqls);
+            c->d_gqls = nullptr;
+            c->gqls_words = 0;
+            HIPCHK(c, dalloc(&c->d_gqls, words));
+            c->gqls_words = words;
+        }
+    }
+    double ms[3] = {0, 0, 0};
+    for (uint32_t s0 = 0; s0 < nq; s0 += chunk) {
+        const int rc = filter_chunk(c, g, qs + s0, std::min(chunk, nq - s0), kind, ms);
+        if (rc) return rc;
+    }
+    for (int k = 0; k < 3; k++) c->filter_ms[k] = ms[k];
     int rc = SGM_OK;
     for (uint32_t i = 0; i < nq; i++) {
         sgm_query* q = qs[i];

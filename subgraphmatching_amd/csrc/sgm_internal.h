@@ -73,10 +73,19 @@ struct sgm_equery {
 };
 
 // Per-query description used by the batched filter kernels.
+//   flags[0 .. 2*SGM_MAXQ)   removals per GQL phase (round*n + u)
+//   flags[SGM_GQL_TOUCH_ANY] bit u: some touch[] word has bit u (round-1 checks of u need it)
+//   flags[SGM_GQL_DIRTY_ANY] bit u: some dirty[] word has bit u (round 2 must revisit u)
+#define SGM_GQL_TOUCH_ANY (2 * SGM_MAXQ)
+#define SGM_GQL_DIRTY_ANY (2 * SGM_MAXQ + 1)
+#define SGM_GQL_FLAG_WORDS (2 * SGM_MAXQ + 2)
 struct sgm_qdesc {
     uint32_t n;
     uint32_t qdeg[SGM_MAXQ];
     uint32_t nbr[SGM_MAXQ];          // neighbour bitmask of each query vertex
+    uint8_t req[SGM_MAXQ][SGM_MAXQ]; // req[u][j]: (j+1)-th largest degree among N_q(u)
     uint32_t* vmask;                 // V words
-    uint32_t* removed;               // [2*n] removals per phase (round*n + u)
+    uint32_t* removed;               // SGM_GQL_FLAG_WORDS words (see above)
+    uint32_t* touch;                 // V words of GQL scratch: bit u = a neighbour lost a bit of N_q(u)
+    uint32_t* dirty;                 // V words of GQL scratch: bit u = recheck (round 2, u)
 };

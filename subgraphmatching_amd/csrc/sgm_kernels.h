@@ -20,10 +20,10 @@ struct sgm_cdesc {
 
 namespace sgm {
 void launch_nlf(hipStream_t s, const uint32_t* off, uint32_t V, const sgm_qdesc* qs, uint32_t nq);
-void launch_gql_phase(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* by_deg,
-                      const uint32_t* ge_cnt, const sgm_qdesc* qs, uint32_t nq, uint32_t phase, uint32_t max_cand,
-                      uint32_t n_big);
-uint32_t gql_lane_degree();  // candidates of higher degree get a whole wave in the GQL phase
+void launch_gql_phase(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint8_t* degc,
+                      const uint8_t* nbr_deg, uint32_t V, const sgm_qdesc* qs, uint32_t nq, uint32_t phase);
+// degree table for the GQL shortcut: nbr_deg[j*V + v], 32 bytes per vertex
+void launch_nbr_deg(hipStream_t s, const uint32_t* off, const uint32_t* adj, uint32_t V, uint8_t* tab);
 void launch_bitmaps(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V);
 void launch_rank_scan(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V, uint32_t max_n);
 void launch_compact(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V);

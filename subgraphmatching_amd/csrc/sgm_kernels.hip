@@ -108,34 +108,44 @@ __device__ bool lsm_wave(const uint32_t* __restrict__ nb, uint32_t deg, const ui
 }
 
 // One refinement phase (round r, query vertex u) of the reference's GQL schedule
-// (inc/filter.h:273-290) for every query of a batch (blockIdx.y = query).  Candidates of u
-// are walked hub-first through the degree-sorted vertex list: its prefix of length
-// ge_cnt[deg_q(u)] is exactly the NLF set, and vmask bit u says whether v is still valid.
-// Only bit u changes during the phase and u is not its own neighbour, so the checks of
-// one phase are independent (the reference's sequential order inside a phase does not
-// matter).  A round-2 phase whose left-side sets did not change since round 1 would
-// repeat round 1's answers exactly and is skipped.
+// (inc/filter.h:273-290) for every query of a batch (blockIdx.y = query).  Only bit u
+// changes during the phase and u is not its own neighbour, so the checks of one phase are
+// independent (the reference's sequential order inside a phase does not matter).
 //
-// Work units: each candidate of degree > GQL_LANE_DEG (the first n_big entries of the
-// degree order) is one unit for a whole wave (the exact wave matcher); the remaining
-// candidates come in units of 64, one per lane: lanes fetch their ids, validity bits and
-// offsets together and decide their own candidate from its right-side masks where they
-// can -- a greedy matching that saturates the left side proves the answer true; a left
-// vertex no right vertex covers, or fewer useful right vertices than left vertices, proves
-// it false -- and the wave runs the exact matcher on the few undecided ones.
+// Exact shortcuts (the answers are the reference's, only fewer checks run):
+//   * Before phase (round 1, u) bit u of every vertex still holds its NLF value, so the
+//     candidates of u are the vertices of degree >= deg_q(u) and need no vmask read.
+//   * When no neighbour w of v has lost a bit u' of N_q(u) (touch[v] bit u clear), every
+//     bit the check reads is an NLF bit: left vertex u' is adjacent to w iff
+//     deg(w) >= deg_q(u').  These neighbourhoods are nested, so Hall's condition reduces to
+//     "the j-th largest neighbour degree of v is at least the j-th largest requirement"
+//     for every j -- one byte of the degree table per left vertex.
+//   * Round 2 rechecks v only if a neighbour lost a bit of N_q(u) after v's round-1 check
+//     of u (dirty[v] bit u): otherwise every input of the check is unchanged and v passes
+//     again.  A removal of v from C(u') in phase (1, u') therefore marks its neighbours
+//     dirty for u in N_q(u') with u < u', and in phase (2, u') for u > u'.
+//
+// Work units: vertices are streamed in id order, 64 per unit, one per lane, with coalesced
+// loads of the capped degree, the marks and the table bytes.  Lanes decide their vertex
+// themselves where they can -- the degree table, or, over at most GQL_LANE_DEG neighbours,
+// a greedy matching that saturates the left side (true), a left vertex no right vertex
+// covers, or fewer useful right vertices than left vertices (false, only when the whole
+// list was seen) -- and the wave runs the exact matcher on the few undecided ones and
+// marks the neighbours of removed high-degree vertices together.
 #define GQL_LANE_DEG 128u
 
 __device__ __forceinline__ int lsm_lane(const uint32_t* __restrict__ adj, uint32_t o, uint32_t deg,
                                         const uint32_t* __restrict__ vm, uint32_t L, uint32_t k)
 {
     // returns 1 = passes, 0 = fails, -1 = undecided
+    const uint32_t scan = deg < GQL_LANE_DEG ? deg : GQL_LANE_DEG;
     uint32_t orm = 0, useful = 0, freeL = L, matched = 0;
-    for (uint32_t j0 = 0; j0 < deg; j0 += 8) {
+    for (uint32_t j0 = 0; j0 < scan; j0 += 8) {
         uint32_t w[8], m[8];
 #pragma unroll
-        for (int t = 0; t < 8; t++) w[t] = (j0 + t < deg) ? adj[o + j0 + t] : 0u;
+        for (int t = 0; t < 8; t++) w[t] = (j0 + t < scan) ? adj[o + j0 + t] : 0u;
 #pragma unroll
-        for (int t = 0; t < 8; t++) m[t] = (j0 + t < deg) ? (vm[w[t]] & L) : 0u;
+        for (int t = 0; t < 8; t++) m[t] = (j0 + t < scan) ? (vm[w[t]] & L) : 0u;
 #pragma unroll
         for (int t = 0; t < 8; t++) {
             orm |= m[t];
@@ -146,15 +156,25 @@ __device__ __forceinline__ int lsm_lane(const uint32_t* __restrict__ adj, uint32
         if (matched == k) return 1;
     }
     if (matched == k) return 1;
+    if (scan < deg) return -1;  // a partial view proves only success
     if (orm != L || useful < k) return 0;
     return -1;
 }
 
+// Record that w's neighbour lost a bit: touch bits tm, dirty bits dm (read first, so words
+// that already hold the bits cost no atomic; bits are only ever added during a filter).
+__device__ __forceinline__ void gql_mark(uint32_t* __restrict__ touch, uint32_t* __restrict__ dirty, uint32_t w,
+                                         uint32_t tm, uint32_t dm)
+{
+    if (tm && (touch[w] & tm) != tm) atomicOr(&touch[w], tm);
+    if (dm && (dirty[w] & dm) != dm) atomicOr(&dirty[w], dm);
+}
+
 __global__ void __launch_bounds__(256) k_gql_phase(const uint32_t* __restrict__ off,
                                                    const uint32_t* __restrict__ adj,
-                                                   const uint32_t* __restrict__ by_deg,
-                                                   const uint32_t* __restrict__ ge_cnt,
-                                                   const sgm_qdesc* __restrict__ qs, uint32_t phase, uint32_t n_big)
+                                                   const uint8_t* __restrict__ degc,
+                                                   const uint8_t* __restrict__ nbr_deg, uint32_t V,
+                                                   const sgm_qdesc* __restrict__ qs, uint32_t phase)
 {
     const sgm_qdesc* d = qs + blockIdx.y;
     const uint32_t n = d->n;
@@ -162,56 +182,113 @@ __global__ void __launch_bounds__(256) k_gql_phase(const uint32_t* __restrict__ 
     const uint32_t r = phase / n, u = phase % n;
     const uint32_t L = d->nbr[u];
     if (L == 0) return;  // empty left side: the matching trivially saturates it
-    uint32_t* __restrict__ removed = d->removed;
+    uint32_t* __restrict__ flags = d->removed;
+    bool use_touch = false;
     if (r == 1) {
-        uint32_t touched = 0;
-        for (uint32_t p = u + 1; p < n; p++) if (removed[p]) touched |= 1u << p;
-        for (uint32_t p = 0; p < u; p++) if (removed[n + p]) touched |= 1u << p;
-        if (!(touched & L)) return;
+        if (!((flags[SGM_GQL_DIRTY_ANY] >> u) & 1u)) return;  // no input of any check changed
+    } else {
+        use_touch = (flags[SGM_GQL_TOUCH_ANY] >> u) & 1u;
     }
-    const uint32_t k = __popc(L);
-    const uint32_t cnt = ge_cnt[d->qdeg[u]];
+    const uint32_t low = L & ((1u << u) - 1u);
+    const uint32_t high = u == 31u ? 0u : (L & (~0u << (u + 1u)));
+    const uint32_t tm = r == 0 ? high : 0u;  // touch bits only matter to later round-1 phases
+    const uint32_t dm = r == 0 ? low : high;
+    const uint32_t k = __popc(L), qd = d->qdeg[u];
+    const uint8_t* __restrict__ req = d->req[u];
     uint32_t* __restrict__ vm = d->vmask;
+    uint32_t* __restrict__ touch = d->touch;
+    uint32_t* __restrict__ dirty = d->dirty;
     const uint32_t lane = lane_id();
     const uint32_t waves = blockDim.x >> 6;
     const uint32_t w0 = blockIdx.x * waves + (threadIdx.x >> 6);
     const uint32_t stride = gridDim.x * waves;
     uint32_t local_removed = 0;
-    const uint32_t nh = cnt < n_big ? cnt : n_big;
-    const uint32_t nunits = nh + (cnt - nh + 63) / 64;
+    const uint32_t nunits = (V + 63) / 64;
     for (uint32_t unit = w0; unit < nunits; unit += stride) {
-        if (unit < nh) {
-            const uint32_t v = by_deg[unit];
-            if (!((vm[v] >> u) & 1u)) continue;
-            const uint32_t o = off[v];
-            if (!lsm_wave(adj + o, off[v + 1] - o, vm, L, k)) {
-                if (lane == 0) atomicAnd(&vm[v], ~(1u << u));
-                local_removed++;
-            }
-            continue;
-        }
-        const uint32_t base = nh + (unit - nh) * 64;
-        const uint32_t i = base + lane;
-        const uint32_t v = i < cnt ? by_deg[i] : 0u;
-        const bool valid = i < cnt && ((vm[v] >> u) & 1u);
+        const uint32_t v = unit * 64 + lane;
+        bool cand = v < V && degc[v] >= qd;  // NLF (degrees are capped at 255 >= 31 >= qd)
         uint32_t o = 0, dv = 0;
-        if (valid) { o = off[v]; dv = off[v + 1] - o; }
         int verdict = -1;
-        if (valid && dv <= GQL_LANE_DEG) verdict = lsm_lane(adj, o, dv, vm, L, k);
-        if (valid && verdict == 0) atomicAnd(&vm[v], ~(1u << u));
-        local_removed += (uint32_t)__popcll(__ballot(valid && verdict == 0));
-        uint64_t pend = __ballot(valid && verdict < 0);
+        if (r == 0) {
+            if (cand && (!use_touch || !((touch[v] >> u) & 1u))) {
+                bool ok = true;
+                for (uint32_t j = 0; j < k && ok; j++) ok = nbr_deg[(size_t)j * V + v] >= req[j];
+                verdict = ok ? 1 : 0;
+            } else if (cand) {
+                o = off[v];
+                dv = off[v + 1] - o;
+                verdict = lsm_lane(adj, o, dv, vm, L, k);
+            }
+        } else {
+            cand = cand && ((dirty[v] >> u) & 1u) && ((vm[v] >> u) & 1u);
+            if (cand) {
+                o = off[v];
+                dv = off[v + 1] - o;
+                verdict = lsm_lane(adj, o, dv, vm, L, k);
+            }
+        }
+        if (cand && verdict == 0) {
+            if (dv == 0) { o = off[v]; dv = off[v + 1] - o; }  // decided by the table
+            atomicAnd(&vm[v], ~(1u << u));
+            if (dv <= GQL_LANE_DEG)
+                for (uint32_t p = o; p < o + dv; p++) gql_mark(touch, dirty, adj[p], tm, dm);
+        }
+        local_removed += (uint32_t)__popcll(__ballot(cand && verdict == 0));
+        // the wave: exact matcher for undecided lanes, shared marking for long lists
+        uint64_t pend = __ballot(cand && (verdict < 0 || (verdict == 0 && dv > GQL_LANE_DEG)));
         while (pend) {
             const uint32_t t = (uint32_t)__ffsll((long long)pend) - 1u;
             pend &= pend - 1;
-            const uint32_t vt = rdlane(v, t), ot = rdlane(o, t), dt = rdlane(dv, t);
-            if (!lsm_wave(adj + ot, dt, vm, L, k)) {
+            const uint32_t vt = rdlane(v, t);
+            const uint32_t ot = off[vt], dt = off[vt + 1] - ot;
+            bool fail = rdlane((uint32_t)verdict, t) == 0u;
+            if (!fail && !lsm_wave(adj + ot, dt, vm, L, k)) {
                 if (lane == 0) atomicAnd(&vm[vt], ~(1u << u));
                 local_removed++;
+                fail = true;
             }
+            if (fail)
+                for (uint32_t p = ot + lane; p < ot + dt; p += 64) gql_mark(touch, dirty, adj[p], tm, dm);
         }
     }
-    if (lane == 0 && local_removed) atomicAdd(&removed[phase], local_removed);
+    if (lane == 0 && local_removed) {
+        atomicAdd(&flags[phase], local_removed);
+        if (tm) atomicOr(&flags[SGM_GQL_TOUCH_ANY], tm);
+        if (dm) atomicOr(&flags[SGM_GQL_DIRTY_ANY], dm);
+    }
+}
+
+// Degree table of the data graph for the GQL shortcut: nbr_deg[j*V + v] = the (j+1)-th
+// largest min(deg(w), 31) over the neighbours w of v, 0 if v has at most j neighbours
+// (j < 32).  Query degrees are at most 31, so the cap loses nothing.  One wave per vertex:
+// a 31-bin histogram by ballots, then lane j reads off its order statistic.
+__global__ void __launch_bounds__(256) k_nbr_deg(const uint32_t* __restrict__ off, const uint32_t* __restrict__ adj,
+                                                 uint32_t V, uint8_t* __restrict__ tab)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t waves = blockDim.x >> 6;
+    for (uint32_t v = blockIdx.x * waves + (threadIdx.x >> 6); v < V; v += gridDim.x * waves) {
+        const uint32_t o = off[v], dv = off[v + 1] - o;
+        uint32_t hist = 0;  // lane b: neighbours with min(deg, 31) == b
+        for (uint32_t base = 0; base < dv; base += 64) {
+            uint32_t c = 0;  // neighbours have degree >= 1, so bin 0 stays empty
+            if (base + lane < dv) {
+                const uint32_t w = adj[o + base + lane];
+                const uint32_t dw = off[w + 1] - off[w];
+                c = dw < 31u ? dw : 31u;
+            }
+            for (uint32_t b = 1; b < 32; b++) {
+                const uint32_t cnt = (uint32_t)__popcll(__ballot(c == b));
+                if (lane == b) hist += cnt;
+            }
+        }
+        uint32_t ge = 0, D = 0;
+        for (uint32_t b = 31; b >= 1; b--) {
+            ge += rdlane(hist, b);
+            if (D == 0 && ge >= lane + 1) D = b;
+        }
+        if (lane < 32) tab[(size_t)lane * V + v] = (uint8_t)D;
+    }
 }
 
 // ================================================================== hub adjacency bitmaps
@@ -411,17 +488,22 @@ void launch_nlf(hipStream_t s, const uint32_t* off, uint32_t V, const sgm_qdesc*
     hipLaunchKernelGGL(k_nlf, grid, dim3(256), 0, s, off, V, qs);
 }
 
-void launch_gql_phase(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* by_deg,
-                      const uint32_t* ge_cnt, const sgm_qdesc* qs, uint32_t nq, uint32_t phase, uint32_t max_cand,
-                      uint32_t n_big)
+void launch_gql_phase(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint8_t* degc,
+                      const uint8_t* nbr_deg, uint32_t V, const sgm_qdesc* qs, uint32_t nq, uint32_t phase)
 {
-    const uint64_t nh = max_cand < n_big ? max_cand : n_big;
-    const uint64_t units = nh + (max_cand - nh + 63) / 64;
-    dim3 grid(clampg((units + 3) / 4, 1, 4096), nq);
-    hipLaunchKernelGGL(k_gql_phase, grid, dim3(256), 0, s, off, adj, by_deg, ge_cnt, qs, phase, n_big);
+    // about 16k blocks of 4 waves in all (at least 8 per query, at most one per 4 units)
+    const uint64_t units = ((uint64_t)V + 63) / 64;
+    const uint64_t max_blocks = (units + 3) / 4 ? (units + 3) / 4 : 1;
+    uint32_t per_q = clampg(16384 / (nq ? nq : 1), 8, 4096);
+    if (per_q > max_blocks) per_q = (uint32_t)max_blocks;
+    dim3 grid(per_q, nq);
+    hipLaunchKernelGGL(k_gql_phase, grid, dim3(256), 0, s, off, adj, degc, nbr_deg, V, qs, phase);
 }
 
-uint32_t gql_lane_degree() { return GQL_LANE_DEG; }
+void launch_nbr_deg(hipStream_t s, const uint32_t* off, const uint32_t* adj, uint32_t V, uint8_t* tab)
+{
+    hipLaunchKernelGGL(k_nbr_deg, dim3(clampg(((uint64_t)V + 3) / 4, 1, 16384)), dim3(256), 0, s, off, adj, V, tab);
+}
 
 void launch_bitmaps(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V)
 {
