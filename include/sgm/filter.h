@@ -1,0 +1,34 @@
+// Candidate filters (inc/filter.h:1049-1063), drop-in over the C ABI: the NLF and GQL
+// kernels run on the device, then FilterParamConstruct's host arrays and aux-edge tables
+// are filled from the device candidate space.  Both return false when some C(u) is empty,
+// as the reference does; candidates / candidates_count then hold the filtered sets and the
+// membership arrays and tables stay unbuilt.
+#pragma once
+
+#include "api.h"
+
+namespace sgm {
+inline bool run_filter(const Graph& data, const Query& query, CandidateParam& can, int kind)
+{
+    if (!data.unlabeled() || !query.unlabeled())
+        throw Error(SGM_EINVAL, "labelled graphs are not on the accelerated path (all labels must be 0)");
+    if (&can.data() != &data || &can.query() != &query)
+        throw Error(SGM_EINVAL, "CandidateParam was built for another data graph or query");
+    std::lock_guard<std::mutex> lk(device_mutex());
+    const int rc = sgm_filter(can.device(), kind);
+    if (rc != SGM_OK && rc != SGM_EEMPTY) check(rc, "filter");
+    if (rc == SGM_OK) check(sgm_build_tables(can.device()), "tables");
+    can.fetch_host_arrays(rc == SGM_OK);
+    return rc == SGM_OK;
+}
+}  // namespace sgm
+
+inline bool Filter_NLF(const Graph& data, const Query& query, CandidateParam& can)
+{
+    return sgm::run_filter(data, query, can, SGM_FILTER_NLF);
+}
+
+inline bool Filter_GQL(const Graph& data, const Query& query, CandidateParam& can)
+{
+    return sgm::run_filter(data, query, can, SGM_FILTER_GQL);
+}

@@ -1,0 +1,194 @@
+// Data and query graphs (inc/graph.h:150-495) and auxiliary edge tables (Edges,
+// inc/graph.h:13-49) for the drop-in C++ surface.
+//
+// A Graph keeps the reference's CSR on the host (sorted neighbour lists) and uploads itself
+// to HBM on first use; it stays resident until the Graph is destroyed.  Only unlabeled
+// graphs (every label 0) are on the accelerated path: Load accepts labels, and the filter
+// calls reject graphs that carry any other label.
+#pragma once
+
+#include <algorithm>
+#include <fstream>
+#include <map>
+#include <sstream>
+
+#include "common.h"
+
+class Edges {
+public:
+    uint32_t* _offset = nullptr;
+    uint32_t* _edge = nullptr;    // candidate index k into C(b)
+    uint32_t* _edge_v = nullptr;  // the same entries as vertex ids
+    uint32_t _v_cnt = 0;
+    uint32_t _e_cnt = 0;
+    uint32_t _max_degree = 0;
+
+    Edges() = default;
+    Edges(const Edges&) = delete;
+    Edges& operator=(const Edges&) = delete;
+    ~Edges()
+    {
+        delete[] _offset;
+        delete[] _edge;
+        delete[] _edge_v;
+    }
+    const uint32_t* getNeb(const uint32_t vertex_id, uint32_t& cnt)
+    {
+        cnt = _offset[vertex_id + 1] - _offset[vertex_id];
+        return _edge + _offset[vertex_id];
+    }
+    const uint32_t* getNeb_V(const uint32_t vertex_id, uint32_t& cnt)
+    {
+        cnt = _offset[vertex_id + 1] - _offset[vertex_id];
+        return _edge_v + _offset[vertex_id];
+    }
+};
+
+class Graph {
+public:
+    std::string path;
+
+    Graph() = default;
+    Graph(const Graph&) = delete;
+    Graph& operator=(const Graph&) = delete;
+    virtual ~Graph() { release_device(); }
+
+    // Reference text format: "t V E", then "v id label degree" lines, then "e u v" lines
+    // (inc/graph.h:190-266).  Degrees must match the edge list (the reference sizes its
+    // CSR from them); a mismatch, a self loop or a duplicate edge throws.
+    virtual void Load(const std::string& file_path)
+    {
+        std::ifstream in(file_path);
+        if (!in.is_open()) throw sgm::Error(SGM_EINVAL, "can't open " + file_path);
+        path = file_path;
+        char type = 0;
+        uint64_t V = 0, E = 0;
+        if (!(in >> type >> V >> E) || type != 't') throw sgm::Error(SGM_EINVAL, file_path + ": missing 't V E' header");
+        std::vector<uint32_t> decl(V, 0);
+        std::vector<LabelID> labels(V, 0);
+        std::vector<std::pair<uint32_t, uint32_t>> edges;
+        edges.reserve(E);
+        while (in >> type) {
+            if (type == 'v') {
+                uint64_t id, lab, d;
+                in >> id >> lab >> d;
+                if (id >= V) throw sgm::Error(SGM_EINVAL, file_path + ": vertex id out of range");
+                labels[id] = (LabelID)lab;
+                decl[id] = (uint32_t)d;
+            } else if (type == 'e') {
+                uint64_t a, b;
+                in >> a >> b;
+                if (a >= V || b >= V || a == b) throw sgm::Error(SGM_EINVAL, file_path + ": bad edge");
+                edges.emplace_back((uint32_t)a, (uint32_t)b);
+            } else {
+                std::string rest;
+                std::getline(in, rest);
+            }
+        }
+        if (edges.size() != E) throw sgm::Error(SGM_EINVAL, file_path + ": edge count differs from the header");
+        AssignEdges((uint32_t)V, edges);
+        for (uint32_t v = 0; v < _v_cnt; v++)
+            if (getVertexDegree(v) != decl[v])
+                throw sgm::Error(SGM_EINVAL, file_path + ": declared degree differs from the edge list");
+        _labels = std::move(labels);
+        count_labels();
+    }
+
+    // Programmatic construction from an undirected edge list (each edge once).
+    void AssignEdges(uint32_t V, const std::vector<std::pair<uint32_t, uint32_t>>& edges)
+    {
+        release_device();
+        _v_cnt = V;
+        _offset.assign((size_t)V + 1, 0);
+        for (auto& e : edges) { _offset[e.first + 1]++; _offset[e.second + 1]++; }
+        for (uint32_t v = 0; v < V; v++) _offset[v + 1] += _offset[v];
+        _neb.assign(_offset[V], 0);
+        std::vector<uint32_t> fill(_offset.begin(), _offset.end() - 1);
+        for (auto& e : edges) { _neb[fill[e.first]++] = e.second; _neb[fill[e.second]++] = e.first; }
+        _max_degree = 0;
+        for (uint32_t v = 0; v < V; v++) {
+            std::sort(_neb.begin() + _offset[v], _neb.begin() + _offset[v + 1]);
+            for (uint32_t p = _offset[v] + 1; p < _offset[v + 1]; p++)
+                if (_neb[p] == _neb[p - 1]) throw sgm::Error(SGM_EINVAL, "duplicate edge");
+            _max_degree = std::max(_max_degree, _offset[v + 1] - _offset[v]);
+        }
+        _e_cnt = (uint32_t)edges.size();
+        _labels.assign(V, 0);
+        count_labels();
+    }
+
+    uint32_t getVertexCnt() const { return _v_cnt; }
+    uint32_t getVertexDegree(const VertexID id) const { return _offset[id + 1] - _offset[id]; }
+    uint32_t getEdgeCnt() const { return _e_cnt; }
+    uint32_t getLabelCnt() const { return _label_cnt; }
+    uint32_t getMaxDgree() const { return _max_degree; }
+    uint32_t getMaxLabelFreq() const { return _max_label_freq; }
+    uint32_t getLabelFreq(LabelID label) const
+    {
+        auto it = _labels_freq.find(label);
+        return it == _labels_freq.end() ? 0 : it->second;
+    }
+    const VertexID* getNeb(VertexID id, uint32_t& neb_count) const
+    {
+        neb_count = _offset[id + 1] - _offset[id];
+        return _neb.data() + _offset[id];
+    }
+    LabelID getVertexLabel(VertexID id) const { return _labels[id]; }
+    const LabelID* getVertexLabels() const { return _labels.data(); }
+    bool hasEdge(VertexID u, VertexID v) const
+    {
+        return std::binary_search(_neb.begin() + _offset[u], _neb.begin() + _offset[u + 1], v);
+    }
+    const uint32_t* offsets() const { return _offset.data(); }
+    const uint32_t* neighbours() const { return _neb.data(); }
+    bool unlabeled() const { return _label_cnt <= 1 && getLabelFreq(0) == _v_cnt; }
+
+    // The resident device copy (uploaded on first call).
+    sgm_graph* device() const
+    {
+        std::lock_guard<std::mutex> lk(sgm::device_mutex());
+        if (!_dev) {
+            sgm_graph* g = nullptr;
+            sgm::check(sgm_graph_upload(sgm::context(), _offset.data(), _neb.data(), _v_cnt, _neb.size(), &g),
+                       "graph upload");
+            _dev = g;
+        }
+        return _dev;
+    }
+
+protected:
+    uint32_t _v_cnt = 0, _e_cnt = 0, _label_cnt = 0, _max_degree = 0, _max_label_freq = 0;
+    std::vector<uint32_t> _offset{0};
+    std::vector<VertexID> _neb;
+    std::vector<LabelID> _labels;
+    std::map<LabelID, uint32_t> _labels_freq;
+    mutable sgm_graph* _dev = nullptr;
+
+    void count_labels()
+    {
+        _labels_freq.clear();
+        LabelID max_label = 0;
+        for (LabelID l : _labels) { _labels_freq[l]++; max_label = std::max(max_label, l); }
+        _label_cnt = _v_cnt ? std::max<uint32_t>((uint32_t)_labels_freq.size(), max_label + 1) : 0;
+        _max_label_freq = 0;
+        for (auto& e : _labels_freq) _max_label_freq = std::max(_max_label_freq, e.second);
+    }
+    void release_device()
+    {
+        if (_dev) {
+            std::lock_guard<std::mutex> lk(sgm::device_mutex());
+            sgm_graph_destroy(_dev);
+            _dev = nullptr;
+        }
+    }
+};
+
+// Query graph (inc/graph.h:382-495): same format, at most 32 vertices on this path.
+class Query : public Graph {
+public:
+    void Load(const std::string& file_path) override
+    {
+        Graph::Load(file_path);
+        if (getVertexCnt() > 32) throw sgm::Error(SGM_EINVAL, file_path + ": queries have at most 32 vertices");
+    }
+};

@@ -1,4 +1,5 @@
-"""Builds the HIP library libsgm_hip.so in place (gfx950)."""
+"""Builds the HIP library libsgm_hip.so in place (gfx950), then the C++ row driver
+sgm_overall, which uses only the drop-in headers include/sgm/*.h and the library."""
 from __future__ import annotations
 
 import os
@@ -7,6 +8,8 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsgm_hip.so")
+DRIVER = os.path.join(HERE, "sgm_overall")
+ROOT = os.path.dirname(HERE)
 SOURCES = ["sgm_kernels.hip", "sgm_enum.hip", "sgm_host.cpp", "sgm_capi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SGM_OFFLOAD_ARCH", "gfx950")
@@ -38,7 +41,21 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)
+    build_driver()
     return LIB
+
+
+def build_driver() -> str:
+    """g++ build of the C++ row driver against the drop-in headers (no HIP headers needed)."""
+    tmp = DRIVER + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+           os.path.join(HERE, "cpp", "overall_performance.cpp"), "-o", tmp,
+           "-L", HERE, "-lsgm_hip", "-Wl,-rpath,$ORIGIN", "-pthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed on the row driver:\n{r.stderr}")
+    os.replace(tmp, DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

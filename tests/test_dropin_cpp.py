@@ -1,0 +1,130 @@
+"""The drop-in C++ surface (include/sgm/{api,filter,order,explore}.h): the headers compile
+against the reference's call pattern, and the row driver built on them
+(subgraphmatching_amd/cpp/overall_performance.cpp, the reference harness's rows and CSV
+columns) gives the oracle's counts and intersection counts on the GPU."""
+import os
+import subprocess
+
+import pytest
+
+import oracle as O
+from subgraphmatching_amd.generators import chung_lu, query_set
+from subgraphmatching_amd.graph import save_text
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "subgraphmatching_amd")
+DRIVER = os.path.join(PKG, "sgm_overall")
+
+# the reference harness's calling sequence (src/overall_performance.cpp:243-262)
+USAGE = r'''
+#include "sgm/explore.h"
+#include "sgm/filter.h"
+#include "sgm/order.h"
+size_t row(const Graph& data, const Query& query, std::atomic<bool>& stop_flag) {
+    double time = 0; size_t ans = 0;
+    Order order;
+    Order_RI(query, order);
+    CandidateParam can(data, query);
+    if (Filter_GQL(data, query, can) == false) return 0;
+    Config cfg;
+    cfg.useFilter = true;
+    cfg.can = &can;
+    ReuseParam reuse(query, order);
+    cfg.useCache = true;
+    cfg.reuse = &reuse;
+    explore_Intersection_Cache_DAF_Filter(data, query, order, cfg, time, ans, stop_flag, 100000);
+    Order o2;
+    Order_GQL(data, query, can, o2);
+    explore_Intersection_All<false, false, false, false, false>(data, query, o2, cfg, time, ans, stop_flag);
+    uint32_t cnt;
+    const uint32_t* nb = can.edge_matrix[0][1]->getNeb(0, cnt);
+    (void)nb;
+    return ans + Profiler::getInst().total_intersection + can.candidates_count[0] + can.candidates_exsist[0][0];
+}
+'''
+
+
+def test_headers_compile_with_reference_call_pattern(tmp_path):
+    src = tmp_path / "usage.cpp"
+    src.write_text(USAGE)
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
+                        str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_pruning_variants_are_rejected_at_compile_time(tmp_path):
+    src = tmp_path / "bad.cpp"
+    src.write_text('#include "sgm/explore.h"\n'
+                   "void f(const Graph& d, const Query& q, const Order& o, const Config& c, std::atomic<bool>& s) {\n"
+                   "  double t; size_t a; explore_Intersection_All<true,false,false,false,false>(d, q, o, c, t, a, s); }\n")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), str(src)],
+                       capture_output=True, text=True)
+    assert r.returncode != 0 and "accelerated path" in r.stderr
+
+
+def _write_inputs(tmp_path, g, qs):
+    dp = tmp_path / "data.txt"
+    save_text(g, str(dp))
+    paths = []
+    for i, q in enumerate(qs):
+        qp = tmp_path / f"q{i}.txt"
+        save_text(q, str(qp))
+        paths.append(str(qp))
+    return str(dp), paths
+
+
+def _run(data, query, limit, rows, timeout=120):
+    r = subprocess.run([DRIVER, data, query, str(limit), "300", *rows], capture_output=True, text=True,
+                       timeout=timeout)
+    return r
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="row driver not built (run __graft_entry__.build())")
+def test_driver_reports_missing_gpu(tmp_path):
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    g = chung_lu(200, 2.5, 4.0, seed=1)
+    qs = query_set(g, 1, 4, False, seed=2)
+    data, qp = _write_inputs(tmp_path, g, qs)
+    r = _run(data, qp[0], 1000, ["RI"])
+    assert r.returncode == 1 and "no usable GPU" in r.stderr, (r.returncode, r.stderr)
+
+
+@pytest.mark.gpu
+def test_driver_rows_match_oracle(tmp_path):
+    assert os.path.exists(DRIVER), "row driver not built"
+    g = chung_lu(2000, 2.3, 8.0, seed=3)
+    qs = query_set(g, 6, lambda i, r: int(r.integers(4, 10)), lambda i, r: bool(i % 2), seed=71)
+    data, paths = _write_inputs(tmp_path, g, qs)
+    limit = 100000
+    for q, qp in zip(qs, paths):
+        print("query", qp, q.getVertexCnt(), flush=True)
+        r = _run(data, qp, limit, ["RI", "GQL", "RI+cache+filter+fs"])
+        assert r.returncode == 0, r.stderr
+        lines = r.stdout.strip().splitlines()
+        assert lines[0] == "data,query_name,algorithm,time_ms,ans,output_limit,intersection_cnt"
+        got = {}
+        for ln in lines[1:]:
+            f = ln.split(",")
+            assert f[0] == "data" and f[5] == str(limit)
+            got[f[2]] = (int(f[4]), int(f[6]))
+        # RI row: NLF + RI order + plain candidate-space search
+        ok, c = O.nlf(g, q)
+        ref = O.enum_all(g, q, c, O.order_ri(q), cap=limit) if ok else {"count": 0, "intersections": 0}
+        _check(got["RI"], ref, limit)
+        # GQL row: GQL filter + GQL order + plain search
+        ok, c = O.gql(g, q)
+        if ok:
+            order = O.order_gql(g, q, [len(x) for x in c])
+            _check(got["GQL"], O.enum_all(g, q, c, order, cap=limit), limit)
+            _check(got["RI+cache+filter+fs"], O.enum_cache(g, q, c, O.order_ri(q), cap=limit), limit)
+        else:
+            assert got["GQL"][0] == 0 and got["RI+cache+filter+fs"][0] == 0
+
+
+def _check(got, ref, limit):
+    ans, inter = got
+    assert ans == ref["count"]
+    if ref["count"] < limit:  # uncapped: the whole search ran, so the work is exact too
+        assert inter == ref["intersections"]
