@@ -54,7 +54,9 @@ typedef struct {
     uint32_t blocks;       /* persistent blocks of 4 waves; 0 = automatic */
     uint32_t stack_words;  /* per-wave set-op stack (u32 words); 0 = automatic */
     uint32_t flags;        /* SGM_ENUM_F_* */
-    uint32_t reserved[4];
+    uint32_t hub_min_degree; /* intersect against adjacency bitmaps only for lists of at least
+                                this many vertices (0 = 64, the smallest with a bitmap) */
+    uint32_t reserved[3];
 } sgm_enum_opts;
 
 /* sgm_enum_opts.flags: intersect against hub adjacency lists by binary search instead of
@@ -139,6 +141,36 @@ int sgm_enumerate(sgm_query* q, const uint32_t* order, const sgm_enum_opts* opts
  * cap and its own sgm_stats; out[i].kernel_ms is the time of the whole batch's launch. */
 int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uint32_t nq, const sgm_enum_opts* opts,
                         sgm_stats* out);
+
+/* ---- batched runner (no reference counterpart: the reference runs one query per call) --
+ * Up to `slots` queries of one data graph per batch.  Per-slot device buffers are
+ * allocated once, so a batch allocates nothing (except a rare stack-overflow rerun); the
+ * filter and the enumeration of a batch each replay a hipGraph captured on first use.
+ * Orders are chosen on the host between the two calls: RI (orders == NULL or
+ * orders[i] == NULL), the GQL order from the batch's candidate counts, or any connected
+ * order of the caller's (a learned one).  Calls on one batch are not thread-safe. */
+typedef struct sgm_batch sgm_batch;
+int sgm_batch_create(sgm_graph* g, uint32_t slots, sgm_batch** out);
+void sgm_batch_destroy(sgm_batch* b);
+/* Load nq <= slots queries (CSR each, 2..32 vertices) and run NLF / NLF + GQL on them
+ * (Filter_NLF / Filter_GQL, inc/filter.h:1049-1063).  status[i] = SGM_OK or SGM_EEMPTY;
+ * optional cand_counts[i*32 + u] = |C(u)| and cand_degsum[i*32 + u] = sum of the data
+ * degrees of C(u).  Candidate lists and tables are not built on this path. */
+int sgm_batch_filter(sgm_batch* b, uint32_t nq, const uint32_t* const* qoffs, const uint32_t* const* qadjs,
+                     const uint32_t* ns, int kind, int32_t* status, uint32_t* cand_counts, uint64_t* cand_degsum);
+/* Order_GQL (inc/order.h:563-597) of loaded query i from the batch's candidate counts. */
+int sgm_batch_order_gql(const sgm_batch* b, uint32_t i, uint32_t* order);
+/* Enumerate the loaded batch (cache + filter methods) in one persistent launch; stats[i]
+ * per query, stats[i].kernel_ms = the batch's enumeration time. */
+int sgm_batch_enumerate(sgm_batch* b, const uint32_t* const* orders, const sgm_enum_opts* opts, sgm_stats* stats);
+/* Device time of the last sgm_batch_filter / sgm_batch_enumerate (HIP events). */
+int sgm_batch_last_ms(const sgm_batch* b, double* filter_ms, double* enum_ms);
+/* Autotuner: enumerates the loaded batch with each persistent grid size (2, 4, 8 blocks per
+ * CU) and intersection strategy (adjacency bitmaps for lists of >= 64, 256, 1024 vertices,
+ * or binary search only), checks that every setting gives the same counts, and returns the
+ * fastest setting in *best (cap and method from *base). */
+int sgm_batch_autotune(sgm_batch* b, const uint32_t* const* orders, const sgm_enum_opts* base, sgm_enum_opts* best,
+                       double* best_ms);
 
 #ifdef __cplusplus
 }

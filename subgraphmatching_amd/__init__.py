@@ -3,7 +3,7 @@ matching, with the call surface of Orrinn/SubgraphMatching (inc/api.h, filter.h,
 explore.h).  See ``subgraphmatching_amd.api`` for the mirrored entry points."""
 from .graph import Graph, Query, load_text, save_text  # noqa: F401
 from . import generators  # noqa: F401
-from .api import (CandidateParam, Config, Context, ExploreResult, Filter_GQL, Filter_NLF,  # noqa: F401
+from .api import (Batch, CandidateParam, Config, Context, ExploreResult, Filter_GQL, Filter_NLF,  # noqa: F401
                   Order_GQL, Order_RI, Profiler, ReuseParam, default_context, enumerate_batch, enumerate_query,
                   explore_Intersection_All, explore_Intersection_Cache_DAF_Filter, filter_batch,
                   plan_export)

@@ -44,6 +44,8 @@ EXPORTS = [
     "sgm_query_destroy", "sgm_filter", "sgm_filter_batch", "sgm_cand_counts", "sgm_cand_download",
     "sgm_cand_exist_download", "sgm_build_tables", "sgm_table_info", "sgm_table_download",
     "sgm_order_ri", "sgm_order_gql", "sgm_plan_export", "sgm_enumerate", "sgm_enumerate_batch",
+    "sgm_batch_create", "sgm_batch_destroy", "sgm_batch_filter", "sgm_batch_order_gql", "sgm_batch_enumerate",
+    "sgm_batch_last_ms", "sgm_batch_autotune",
 ]
 
 
@@ -52,7 +54,8 @@ SGM_ENUM_F_NO_BITMAPS = 1
 
 class EnumOpts(C.Structure):
     _fields_ = [("cap", C.c_uint64), ("method", C.c_int32), ("blocks", C.c_uint32),
-                ("stack_words", C.c_uint32), ("flags", C.c_uint32), ("reserved", C.c_uint32 * 4)]
+                ("stack_words", C.c_uint32), ("flags", C.c_uint32), ("hub_min_degree", C.c_uint32),
+                ("reserved", C.c_uint32 * 3)]
 
 
 class Stats(C.Structure):
@@ -107,6 +110,14 @@ def lib() -> C.CDLL:
         "sgm_plan_export": (C.c_int, [vp, vp, C.c_uint32, vp, u32p, vp, vp, vp, vp, vp]),
         "sgm_enumerate": (C.c_int, [vp, vp, C.POINTER(EnumOpts), C.POINTER(Stats)]),
         "sgm_enumerate_batch": (C.c_int, [vp, vp, C.c_uint32, C.POINTER(EnumOpts), vp]),
+        "sgm_batch_create": (C.c_int, [vp, C.c_uint32, C.POINTER(vp)]),
+        "sgm_batch_destroy": (None, [vp]),
+        "sgm_batch_filter": (C.c_int, [vp, C.c_uint32, vp, vp, vp, C.c_int, vp, vp, vp]),
+        "sgm_batch_order_gql": (C.c_int, [vp, C.c_uint32, vp]),
+        "sgm_batch_enumerate": (C.c_int, [vp, vp, C.POINTER(EnumOpts), vp]),
+        "sgm_batch_last_ms": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "sgm_batch_autotune": (C.c_int, [vp, vp, C.POINTER(EnumOpts), C.POINTER(EnumOpts),
+                                         C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -29,7 +29,7 @@ from .graph import Graph, Query
 __all__ = ["Context", "default_context", "DeviceGraph", "CandidateParam", "Edges", "Filter_NLF",
            "Filter_GQL", "Order_RI", "Order_GQL", "ReuseParam", "Config", "ExploreResult",
            "explore_Intersection_Cache_DAF_Filter", "explore_Intersection_All", "Profiler",
-           "filter_batch", "enumerate_query", "enumerate_batch", "plan_export"]
+           "filter_batch", "enumerate_query", "enumerate_batch", "plan_export", "Batch"]
 
 
 def _check(code, ctx=None):
@@ -437,3 +437,99 @@ def explore_Intersection_All(data: Graph, query: Query, order, cfg: Config,
     if cfg.can is None:
         raise ValueError("the candidate-space search needs filtered candidates (cfg.can)")
     return enumerate_query(cfg.can, order, L.SGM_ENUM_AUX_TABLES, output_limit)
+
+
+def _opts(method, cap, blocks=0, stack_words=0, flags=0, hub_min_degree=0):
+    o = L.EnumOpts()
+    o.cap = (1 << 64) - 1 if cap is None else int(cap)
+    o.method = method
+    o.blocks = blocks
+    o.stack_words = stack_words
+    o.flags = flags
+    o.hub_min_degree = hub_min_degree
+    return o
+
+
+def _result(st, n):
+    return ExploreResult(time=float(st.kernel_ms), ans=int(st.count), intersections=int(st.intersections),
+                         nodes=[int(x) for x in st.nodes[:n]], fs_prunes=int(st.fs_prunes),
+                         elems_read=int(st.elems_read), elems_out=int(st.elems_out),
+                         elems_scanned=int(st.elems_scanned), capped=bool(st.capped), retries=int(st.retries))
+
+
+class Batch:
+    """Batched runner over one resident data graph (sgm_batch_*): up to ``slots`` queries
+    per batch, filtered and enumerated by replayed hipGraphs, with per-slot device state
+    allocated once.  ``filter`` loads a batch; ``enumerate`` runs it with RI orders (None),
+    or per-query orders chosen in between (``order_gql`` or any connected order)."""
+
+    def __init__(self, data: Graph, slots: int = 64, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self.data = data
+        self.dg = self.ctx.upload(data)
+        self.slots = int(slots)
+        h = C.c_void_p()
+        _check(L.lib().sgm_batch_create(self.dg.handle, self.slots, C.byref(h)), self.ctx)
+        self.handle = h
+        self.queries = []
+
+    def close(self):
+        if self.handle:
+            L.lib().sgm_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def filter(self, queries, kind=L.SGM_FILTER_GQL):
+        """Returns (ok[nq] bools, cand_counts[nq, 32], cand_degsum[nq, 32])."""
+        nq = len(queries)
+        self._keep = [(L.u32(q.off), L.u32(q.adj)) for q in queries]
+        offs = (C.c_void_p * nq)(*[a.ctypes.data for a, _ in self._keep])
+        adjs = (C.c_void_p * nq)(*[b.ctypes.data for _, b in self._keep])
+        ns = L.u32([q.getVertexCnt() for q in queries])
+        status = np.zeros(nq, dtype=np.int32)
+        cnt = np.zeros((nq, 32), dtype=np.uint32)
+        dsum = np.zeros((nq, 32), dtype=np.uint64)
+        _check(L.lib().sgm_batch_filter(self.handle, nq, offs, adjs, L.ptr(ns), kind, L.ptr(status), L.ptr(cnt),
+                                        L.ptr(dsum)), self.ctx)
+        self.queries = list(queries)
+        return [int(x) == L.SGM_OK for x in status], cnt, dsum
+
+    def order_gql(self, i: int) -> list:
+        o = np.zeros(self.queries[i].getVertexCnt(), dtype=np.uint32)
+        _check(L.lib().sgm_batch_order_gql(self.handle, i, L.ptr(o)), self.ctx)
+        return [int(x) for x in o]
+
+    def _orders(self, orders):
+        if orders is None:
+            return None, None
+        keep = [None if o is None else L.u32(o) for o in orders]
+        arr = (C.c_void_p * len(keep))(*[None if o is None else o.ctypes.data for o in keep])
+        return arr, keep
+
+    def enumerate(self, orders=None, method=L.SGM_ENUM_CACHE_FILTER_FS, cap=None, blocks=0, stack_words=0,
+                  flags=0, hub_min_degree=0):
+        arr, _keep = self._orders(orders)
+        opts = _opts(method, cap, blocks, stack_words, flags, hub_min_degree)
+        stats = (L.Stats * len(self.queries))()
+        _check(L.lib().sgm_batch_enumerate(self.handle, arr, C.byref(opts), stats), self.ctx)
+        return [_result(st, q.getVertexCnt()) for st, q in zip(stats, self.queries)]
+
+    def autotune(self, orders=None, method=L.SGM_ENUM_CACHE_FILTER_FS, cap=None):
+        """Returns ({blocks, flags, hub_min_degree}, best_ms) of the fastest setting."""
+        arr, _keep = self._orders(orders)
+        base = _opts(method, cap)
+        best = L.EnumOpts()
+        ms = C.c_double()
+        _check(L.lib().sgm_batch_autotune(self.handle, arr, C.byref(base), C.byref(best), C.byref(ms)), self.ctx)
+        return {"blocks": int(best.blocks), "flags": int(best.flags),
+                "hub_min_degree": int(best.hub_min_degree)}, float(ms.value)
+
+    def last_ms(self):
+        f, e = C.c_double(), C.c_double()
+        _check(L.lib().sgm_batch_last_ms(self.handle, C.byref(f), C.byref(e)), self.ctx)
+        return float(f.value), float(e.value)

@@ -1,0 +1,445 @@
+// Batched runner: up to `slots` queries of one resident data graph per batch, with
+// per-slot device state reused across batches, and the filter and enumeration launches of
+// a batch captured once as hipGraphs and replayed for every later batch.
+//
+// Filter graph (per filter kind): clear flags / GQL marks / statistics with fill kernels,
+// NLF, the 64 GQL refinement phases (a phase with nothing to do for a slot returns at
+// once), candidate counts and degree sums.  Enumeration graph (per launch shape): clear the
+// per-query statistics, stop words and item counters, then one persistent cache + filter
+// (+ failing-set) launch over all slots.  Unused slots have n = 0 (filter) and no items
+// (enumeration).  Orders are chosen on the host between the two graphs, from the query
+// alone (RI) or from the candidate statistics (GQL order, a learned cost model).
+#include "sgm_objects.h"
+
+namespace {
+
+struct EnumGraph {
+    uint32_t max_ops, use_fs, blocks, words;
+    const uint32_t* hub;
+    hipGraphExec_t exec;
+};
+
+}  // namespace
+
+struct sgm_batch {
+    sgm_graph* g = nullptr;
+    sgm_ctx* c = nullptr;
+    uint32_t slots = 0;
+    uint32_t* d_vmask = nullptr;             // slots * V
+    uint32_t* d_flags = nullptr;             // slots * SGM_GQL_FLAG_WORDS
+    uint32_t* d_scratch = nullptr;           // slots * 2V: GQL touch / dirty marks
+    sgm_qdesc* d_qd = nullptr;               // slots
+    uint32_t* d_cnt = nullptr;               // slots * 32
+    unsigned long long* d_degsum = nullptr;  // slots * 32
+    sgm_plan_dev* d_plans = nullptr;         // slots
+    sgm_equery* d_eq = nullptr;              // slots
+    sgm_stats_dev* d_st = nullptr;           // slots
+    uint32_t* d_qstop = nullptr;             // 2 * slots: stop words, then item counters
+    uint32_t* d_stack = nullptr;
+    size_t stack_words_total = 0;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // the loaded batch
+    uint32_t nq = 0;
+    int kind = 0;
+    std::vector<sgm::QueryCSR> q;
+    std::vector<sgm_qdesc> hqd;
+    std::vector<uint32_t> counts;
+    std::vector<unsigned long long> degsum;
+    std::vector<int32_t> status;
+    double filter_ms = 0, enum_ms = 0;
+    hipGraphExec_t gf[3] = {nullptr, nullptr, nullptr};
+    std::vector<EnumGraph> ge;
+};
+
+namespace {
+
+int load_query(sgm_ctx* c, const uint32_t* qoff, const uint32_t* qadj, uint32_t n, sgm::QueryCSR& q, sgm_qdesc& qd)
+{
+    if (!qoff || (!qadj && qoff[n]) || n < 2 || n > SGM_MAXQ)
+        return set_err(c, SGM_EINVAL, "batch queries need 2..32 vertices");
+    if (qoff[0] != 0) return set_err(c, SGM_EINVAL, "query offsets must start at 0");
+    q.n = n;
+    q.off.assign(qoff, qoff + n + 1);
+    q.adj.assign(qadj, qadj + qoff[n]);
+    std::memset(&qd.qdeg, 0, sizeof qd.qdeg);
+    std::memset(&qd.nbr, 0, sizeof qd.nbr);
+    std::memset(&qd.req, 0, sizeof qd.req);
+    qd.n = n;
+    for (uint32_t u = 0; u < n; u++) {
+        if (qoff[u + 1] < qoff[u]) return set_err(c, SGM_EINVAL, "query offsets not monotone");
+        for (uint32_t p = qoff[u]; p < qoff[u + 1]; p++) {
+            const uint32_t w = qadj[p];
+            if (w >= n || w == u || (p > qoff[u] && w <= qadj[p - 1]))
+                return set_err(c, SGM_EINVAL, "query adjacency of %u malformed", u);
+            bool back = false;
+            for (uint32_t r = qoff[w]; r < qoff[w + 1]; r++) back |= qadj[r] == u;
+            if (!back) return set_err(c, SGM_EINVAL, "query graph not symmetric");
+            qd.nbr[u] |= 1u << w;
+        }
+        qd.qdeg[u] = qoff[u + 1] - qoff[u];
+    }
+    for (uint32_t u = 0; u < n; u++) {
+        std::vector<uint32_t> r;
+        for (uint32_t p = qoff[u]; p < qoff[u + 1]; p++) r.push_back(qd.qdeg[qadj[p]]);
+        std::sort(r.begin(), r.end(), [](uint32_t a, uint32_t b) { return a > b; });
+        for (size_t j = 0; j < r.size(); j++) qd.req[u][j] = (uint8_t)r[j];
+    }
+    return SGM_OK;
+}
+
+// Records `body` on the context stream into an executable graph.
+template <class F>
+int capture(sgm_ctx* c, F&& body, hipGraphExec_t* out)
+{
+    hipGraph_t graph = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    body();
+    const hipError_t le = hipGetLastError();
+    const hipError_t ee = hipStreamEndCapture(c->stream, &graph);
+    if (le != hipSuccess || ee != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return set_err(c, SGM_EHIP, "graph capture: %s", hipGetErrorString(le != hipSuccess ? le : ee));
+    }
+    const hipError_t ie = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) return set_err(c, SGM_EHIP, "graph instantiate: %s", hipGetErrorString(ie));
+    return SGM_OK;
+}
+
+uint32_t ops_bucket(uint32_t max_ops)
+{
+    uint32_t b = 64;
+    while (b < max_ops) b <<= 1;
+    return std::min<uint32_t>(b, SGM_MAXOPS);
+}
+
+}  // namespace
+
+extern "C" {
+
+void sgm_batch_destroy(sgm_batch* b)
+{
+    if (!b) return;
+    if (b->c) {
+        (void)hipSetDevice(b->c->device);
+        (void)hipStreamSynchronize(b->c->stream);
+    }
+    for (auto& x : b->gf)
+        if (x) (void)hipGraphExecDestroy(x);
+    for (auto& x : b->ge) (void)hipGraphExecDestroy(x.exec);
+    for (auto& e : b->ev)
+        if (e) (void)hipEventDestroy(e);
+    dfree(b->d_vmask);
+    dfree(b->d_flags);
+    dfree(b->d_scratch);
+    dfree(b->d_qd);
+    dfree(b->d_cnt);
+    dfree(b->d_degsum);
+    dfree(b->d_plans);
+    dfree(b->d_eq);
+    dfree(b->d_st);
+    dfree(b->d_qstop);
+    dfree(b->d_stack);
+    delete b;
+}
+
+int sgm_batch_create(sgm_graph* g, uint32_t slots, sgm_batch** out)
+{
+    if (!g || !out || slots == 0 || slots > 4096) return SGM_EINVAL;
+    *out = nullptr;
+    sgm_ctx* c = g->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    sgm_batch* b = new sgm_batch();
+    b->g = g;
+    b->c = c;
+    b->slots = slots;
+    const size_t V = g->V;
+    hipError_t e = dalloc(&b->d_vmask, slots * V);
+    if (e == hipSuccess) e = dalloc(&b->d_flags, (size_t)slots * SGM_GQL_FLAG_WORDS);
+    if (e == hipSuccess) e = dalloc(&b->d_scratch, 2 * slots * V);
+    if (e == hipSuccess) e = dalloc(&b->d_qd, slots);
+    if (e == hipSuccess) e = dalloc(&b->d_cnt, (size_t)slots * SGM_MAXQ);
+    if (e == hipSuccess) e = dalloc(&b->d_degsum, (size_t)slots * SGM_MAXQ);
+    if (e == hipSuccess) e = dalloc(&b->d_plans, slots);
+    if (e == hipSuccess) e = dalloc(&b->d_eq, slots);
+    if (e == hipSuccess) e = dalloc(&b->d_st, slots);
+    if (e == hipSuccess) e = dalloc(&b->d_qstop, 2 * (size_t)slots);
+    for (auto& ev : b->ev)
+        if (e == hipSuccess) e = hipEventCreate(&ev);
+    if (e != hipSuccess) {
+        sgm_batch_destroy(b);
+        return set_err(c, e == hipErrorOutOfMemory ? SGM_ENOMEM : SGM_EHIP, "batch alloc: %s", hipGetErrorString(e));
+    }
+    b->hqd.assign(slots, sgm_qdesc());
+    for (uint32_t i = 0; i < slots; i++) {
+        sgm_qdesc& d = b->hqd[i];
+        std::memset(&d, 0, sizeof d);
+        d.vmask = b->d_vmask + i * V;
+        d.removed = b->d_flags + (size_t)i * SGM_GQL_FLAG_WORDS;
+        d.touch = b->d_scratch + 2 * i * V;
+        d.dirty = d.touch + V;
+    }
+    b->counts.assign((size_t)slots * SGM_MAXQ, 0);
+    b->degsum.assign((size_t)slots * SGM_MAXQ, 0);
+    *out = b;
+    return SGM_OK;
+}
+
+int sgm_batch_filter(sgm_batch* b, uint32_t nq, const uint32_t* const* qoffs, const uint32_t* const* qadjs,
+                     const uint32_t* ns, int kind, int32_t* status, uint32_t* cand_counts, uint64_t* cand_degsum)
+{
+    if (!b || !qoffs || !qadjs || !ns) return SGM_EINVAL;
+    sgm_ctx* c = b->c;
+    sgm_graph* g = b->g;
+    if (nq == 0 || nq > b->slots) return set_err(c, SGM_EINVAL, "batch of %u queries for %u slots", nq, b->slots);
+    if (kind != SGM_FILTER_NLF && kind != SGM_FILTER_GQL) return set_err(c, SGM_EINVAL, "unknown filter kind %d", kind);
+    HIPCHK(c, hipSetDevice(c->device));
+    b->nq = 0;
+    b->q.assign(nq, sgm::QueryCSR());
+    for (uint32_t i = 0; i < b->slots; i++) {
+        sgm_qdesc& d = b->hqd[i];
+        if (i < nq) {
+            const int rc = load_query(c, qoffs[i], qadjs[i], ns[i], b->q[i], d);
+            if (rc) return rc;
+        } else {
+            d.n = 0;
+        }
+    }
+    const uint32_t slots = b->slots, V = g->V;
+    if (!b->gf[kind]) {
+        const int rc = capture(c, [&] {
+            sgm::launch_fill_u32(c->stream, b->d_flags, (size_t)slots * SGM_GQL_FLAG_WORDS, 0);
+            if (kind == SGM_FILTER_GQL) sgm::launch_fill_u32(c->stream, b->d_scratch, 2 * (size_t)slots * V, 0);
+            sgm::launch_fill_u32(c->stream, b->d_cnt, (size_t)slots * SGM_MAXQ, 0);
+            sgm::launch_fill_u32(c->stream, reinterpret_cast<uint32_t*>(b->d_degsum), 2 * (size_t)slots * SGM_MAXQ, 0);
+            sgm::launch_nlf(c->stream, g->d_off, V, b->d_qd, slots);
+            if (kind == SGM_FILTER_GQL && V)
+                for (uint32_t ph = 0; ph < 2 * SGM_MAXQ; ph++)
+                    sgm::launch_gql_phase(c->stream, g->d_off, g->d_adj, g->d_degc, g->d_nbrdeg, V, b->d_qd, slots, ph);
+            sgm::launch_cand_stats(c->stream, g->d_off, V, b->d_qd, slots, b->d_cnt, b->d_degsum);
+        }, &b->gf[kind]);
+        if (rc) return rc;
+    }
+    HIPCHK(c, hipMemcpyAsync(b->d_qd, b->hqd.data(), sizeof(sgm_qdesc) * slots, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipEventRecord(b->ev[0], c->stream));
+    HIPCHK(c, hipGraphLaunch(b->gf[kind], c->stream));
+    HIPCHK(c, hipEventRecord(b->ev[1], c->stream));
+    HIPCHK(c, hipMemcpyAsync(b->counts.data(), b->d_cnt, sizeof(uint32_t) * SGM_MAXQ * nq, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipMemcpyAsync(b->degsum.data(), b->d_degsum, sizeof(unsigned long long) * SGM_MAXQ * nq,
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+    b->filter_ms = ms;
+    b->status.assign(nq, SGM_OK);
+    for (uint32_t i = 0; i < nq; i++) {
+        for (uint32_t u = 0; u < b->q[i].n; u++)
+            if (b->counts[(size_t)i * SGM_MAXQ + u] == 0) b->status[i] = SGM_EEMPTY;
+        if (status) status[i] = b->status[i];
+    }
+    if (cand_counts) std::memcpy(cand_counts, b->counts.data(), sizeof(uint32_t) * SGM_MAXQ * nq);
+    if (cand_degsum) std::memcpy(cand_degsum, b->degsum.data(), sizeof(uint64_t) * SGM_MAXQ * nq);
+    b->nq = nq;
+    b->kind = kind;
+    return SGM_OK;
+}
+
+int sgm_batch_order_gql(const sgm_batch* b, uint32_t i, uint32_t* order)
+{
+    if (!b || !order || i >= b->nq) return SGM_EINVAL;
+    sgm::order_gql(b->q[i], b->counts.data() + (size_t)i * SGM_MAXQ, b->g->V, order);
+    return SGM_OK;
+}
+
+int sgm_batch_enumerate(sgm_batch* b, const uint32_t* const* orders, const sgm_enum_opts* opts, sgm_stats* stats)
+{
+    if (!b || !stats) return SGM_EINVAL;
+    sgm_ctx* c = b->c;
+    sgm_graph* g = b->g;
+    if (b->nq == 0) return set_err(c, SGM_ESTATE, "filter a batch first");
+    const sgm_enum_opts o = normalize_opts(opts);
+    if (o.method != SGM_ENUM_CACHE_FILTER_FS && o.method != SGM_ENUM_CACHE_FILTER)
+        return set_err(c, SGM_EINVAL, "the batch runner supports the cache + filter methods (got %d)", o.method);
+    HIPCHK(c, hipSetDevice(c->device));
+    const uint32_t nq = b->nq, slots = b->slots, V = g->V;
+    std::memset(stats, 0, sizeof(sgm_stats) * nq);
+    std::vector<sgm_plan_dev> plans(nq);
+    std::vector<sgm_equery> eq(slots);
+    uint32_t max_ops = 1, live = 0;
+    std::vector<uint32_t> ord(SGM_MAXQ);
+    for (uint32_t i = 0; i < slots; i++) {
+        sgm_equery& e = eq[i];
+        std::memset(&e, 0, sizeof e);
+        e.plan = b->d_plans + std::min(i, nq - 1);
+        e.vmask = b->d_vmask + (size_t)i * V;
+        e.stop = b->d_qstop + i;
+        e.next = b->d_qstop + slots + i;
+        e.st = b->d_st + i;
+        e.cap = (unsigned long long)o.cap;
+        if (i >= nq) continue;
+        const sgm::QueryCSR& q = b->q[i];
+        if (orders && orders[i]) std::copy(orders[i], orders[i] + q.n, ord.begin());
+        else sgm::order_ri(q, ord.data());
+        sgm::Plan pl;
+        const int prc = sgm::build_plan(q, ord.data(), pl);
+        if (prc == -2) return set_err(c, SGM_EORDER, "order of query %u is not a connected permutation", i);
+        if (prc) return set_err(c, SGM_EINVAL, "plan construction failed for query %u (%d)", i, prc);
+        sgm::plan_to_device(pl, plans[i]);
+        max_ops = std::max<uint32_t>(max_ops, plans[i].n_ops);
+        e.root_u = ord[0];
+        if (b->status[i] == SGM_OK) {  // the reference's driver reports 0 without enumerating
+            e.n_items = g->ge_cnt[std::min<uint32_t>(q.off[ord[0] + 1] - q.off[ord[0]], 32)];
+            live++;
+        }
+    }
+    if (!live) return SGM_OK;
+    HIPCHK(c, hipMemcpyAsync(b->d_plans, plans.data(), sizeof(sgm_plan_dev) * nq, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(b->d_eq, eq.data(), sizeof(sgm_equery) * slots, hipMemcpyHostToDevice, c->stream));
+    const uint32_t bucket = ops_bucket(max_ops);
+    const uint32_t blocks = default_blocks(c, o, live);
+    uint64_t words = o.stack_words ? o.stack_words : std::min<uint64_t>((uint64_t)8 * g->max_deg + 1024, 1u << 20);
+    const int use_fs = o.method == SGM_ENUM_CACHE_FILTER_FS ? 1 : 0;
+    const uint32_t* hub = nullptr;
+    int rc = hub_slot_map(g, o, &hub);
+    if (rc) return rc;
+    const size_t need = (size_t)blocks * 4 * words;
+    if (b->stack_words_total < need) {
+        dfree(b->d_stack);
+        b->d_stack = nullptr;
+        b->stack_words_total = 0;
+        HIPCHK(c, dalloc(&b->d_stack, need));
+        b->stack_words_total = need;
+        for (auto& x : b->ge) (void)hipGraphExecDestroy(x.exec);  // captured with the old stack
+        b->ge.clear();
+    }
+    hipGraphExec_t exec = nullptr;
+    for (auto& x : b->ge)
+        if (x.max_ops == bucket && x.use_fs == (uint32_t)use_fs && x.blocks == blocks && x.words == words && x.hub == hub)
+            exec = x.exec;
+    const size_t st_words = sizeof(sgm_stats_dev) / sizeof(uint32_t) * slots;
+    if (!exec) {
+        rc = capture(c, [&] {
+            sgm::launch_fill_u32(c->stream, reinterpret_cast<uint32_t*>(b->d_st), st_words, 0);
+            sgm::launch_fill_u32(c->stream, b->d_qstop, 2 * (size_t)slots, 0);
+            sgm::launch_fill_u32(c->stream, c->d_stop, 1, 0);
+            sgm::launch_enum_cache(c->stream, blocks, g->d_off, g->d_adj, g->d_bydeg, b->d_eq, slots, bucket, b->d_stack,
+                                   (uint32_t)words, c->d_stop, use_fs, hub, g->d_hub_bits, g->hub_words);
+        }, &exec);
+        if (rc) return rc;
+        b->ge.push_back(EnumGraph{bucket, (uint32_t)use_fs, blocks, (uint32_t)words, hub, exec});
+    }
+    c->stop_requested = 0;
+    HIPCHK(c, hipEventRecord(b->ev[2], c->stream));
+    HIPCHK(c, hipGraphLaunch(exec, c->stream));
+    HIPCHK(c, hipEventRecord(b->ev[3], c->stream));
+    std::vector<sgm_stats_dev> got(nq);
+    HIPCHK(c, hipMemcpyAsync(got.data(), b->d_st, sizeof(sgm_stats_dev) * nq, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, b->ev[2], b->ev[3]));
+    b->enum_ms = ms;
+    std::vector<uint32_t> again;
+    for (uint32_t i = 0; i < nq; i++) {
+        stats[i].kernel_ms = ms;
+        if (got[i].overflow) { again.push_back(i); continue; }
+        stats_out(got[i], o.cap, &stats[i]);
+        if (c->stop_requested && !stats[i].capped && b->status[i] == SGM_OK) stats[i].stopped = 1;
+    }
+    // set-op stack overflow (rare): rerun those queries alone with a larger stack, outside
+    // the graph (this allocates)
+    for (uint32_t attempt = 0; !again.empty(); attempt++) {
+        if (attempt >= 3) return set_err(c, SGM_EDEVICE, "set-op stack overflow after 3 retries");
+        words *= 4;
+        if (words > 0x7FFFFFFFull) return set_err(c, SGM_EDEVICE, "set-op stack would exceed 2^31 words");
+        rc = ensure_stack(c, (size_t)blocks * 4 * words * sizeof(uint32_t));
+        if (rc) return rc;
+        std::vector<sgm_equery> sub;
+        for (uint32_t i : again) {
+            sgm_equery e = eq[i];
+            const uint32_t k = (uint32_t)sub.size();
+            e.stop = b->d_qstop + k;
+            e.next = b->d_qstop + slots + k;
+            e.st = b->d_st + k;
+            sub.push_back(e);
+        }
+        HIPCHK(c, hipMemcpyAsync(b->d_eq, sub.data(), sizeof(sgm_equery) * sub.size(), hipMemcpyHostToDevice,
+                                 c->stream));
+        sgm::launch_fill_u32(c->stream, reinterpret_cast<uint32_t*>(b->d_st), st_words, 0);
+        sgm::launch_fill_u32(c->stream, b->d_qstop, 2 * (size_t)slots, 0);
+        sgm::launch_fill_u32(c->stream, c->d_stop, 1, 0);
+        HIPCHK(c, hipEventRecord(b->ev[2], c->stream));
+        sgm::launch_enum_cache(c->stream, blocks, g->d_off, g->d_adj, g->d_bydeg, b->d_eq, (uint32_t)sub.size(),
+                               bucket, c->d_stack, (uint32_t)words, c->d_stop, use_fs, hub, g->d_hub_bits,
+                               g->hub_words);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipEventRecord(b->ev[3], c->stream));
+        std::vector<sgm_stats_dev> got2(nq);
+        HIPCHK(c, hipMemcpyAsync(got2.data(), b->d_st, sizeof(sgm_stats_dev) * sub.size(), hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipEventElapsedTime(&ms, b->ev[2], b->ev[3]));
+        b->enum_ms += ms;
+        std::vector<uint32_t> next;
+        for (size_t k = 0; k < again.size(); k++) {
+            const uint32_t i = again[k];
+            stats[i].kernel_ms += ms;
+            stats[i].retries = attempt + 1;
+            if (got2[k].overflow) { next.push_back(i); continue; }
+            stats_out(got2[k], o.cap, &stats[i]);
+        }
+        again.swap(next);
+        // the slot layout of d_eq was replaced: restore it for the captured graph
+        HIPCHK(c, hipMemcpyAsync(b->d_eq, eq.data(), sizeof(sgm_equery) * slots, hipMemcpyHostToDevice, c->stream));
+    }
+    if (c->stop_requested)
+        for (uint32_t i = 0; i < nq; i++)
+            if (stats[i].stopped) return set_err(c, SGM_ESTOPPED, "stopped by sgm_stop");
+    return SGM_OK;
+}
+
+int sgm_batch_last_ms(const sgm_batch* b, double* filter_ms, double* enum_ms)
+{
+    if (!b) return SGM_EINVAL;
+    if (filter_ms) *filter_ms = b->filter_ms;
+    if (enum_ms) *enum_ms = b->enum_ms;
+    return SGM_OK;
+}
+
+int sgm_batch_autotune(sgm_batch* b, const uint32_t* const* orders, const sgm_enum_opts* base, sgm_enum_opts* best,
+                       double* best_ms)
+{
+    if (!b || !best) return SGM_EINVAL;
+    sgm_ctx* c = b->c;
+    if (b->nq == 0) return set_err(c, SGM_ESTATE, "filter a batch first");
+    const sgm_enum_opts o0 = normalize_opts(base);
+    std::vector<sgm_stats> st(b->nq), ref;
+    double best_t = 1e300;
+    sgm_enum_opts winner = o0;
+    // persistent grid (blocks of 4 waves per CU) x intersection strategy (adjacency bitmaps
+    // for lists of at least T vertices; the rest by binary search)
+    static const uint32_t per_cu[] = {2, 4, 8};
+    static const uint32_t hub_min[] = {64, 256, 1024, 0};  // 0: no bitmaps
+    for (uint32_t pc : per_cu)
+        for (uint32_t hm : hub_min) {
+            sgm_enum_opts o = o0;
+            o.blocks = (uint32_t)c->num_cus * pc;
+            o.hub_min_degree = hm;
+            o.flags = hm ? (o0.flags & ~SGM_ENUM_F_NO_BITMAPS) : (o0.flags | SGM_ENUM_F_NO_BITMAPS);
+            const int rc = sgm_batch_enumerate(b, orders, &o, st.data());
+            if (rc) return rc;
+            if (ref.empty()) {
+                ref = st;
+            } else {
+                for (uint32_t i = 0; i < b->nq; i++)
+                    if (st[i].count != ref[i].count)
+                        return set_err(c, SGM_EDEVICE, "autotune: counts differ between settings (query %u)", i);
+            }
+            if (b->enum_ms < best_t) { best_t = b->enum_ms; winner = o; }
+        }
+    *best = winner;
+    if (best_ms) *best_ms = best_t;
+    return SGM_OK;
+}
+
+}  // extern "C"

@@ -9,141 +9,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/sgm_c.h"
-#include "sgm_host.h"
-#include "sgm_internal.h"
-#include "sgm_kernels.h"
-
-struct sgm_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipStream_t ctl = nullptr;       // non-blocking stream for sgm_stop
-    std::string err;
-    uint32_t* d_stop = nullptr;      // stop word polled by the enumeration kernel
-    uint32_t* d_work = nullptr;      // work counter
-    sgm_stats_dev* d_stats = nullptr;
-    sgm_auxplan_dev* d_auxplan = nullptr;
-    unsigned long long* d_work64 = nullptr;  // item counter of the batched search
-    sgm_plan_dev* d_plans = nullptr;          // per-query plans of a batch
-    sgm_equery* d_eq = nullptr;
-    sgm_stats_dev* d_qstats = nullptr;
-    uint32_t* d_qstop = nullptr;
-    uint32_t batch_cap = 0, batch_cap2 = 0;
-    uint32_t* d_stack = nullptr;
-    size_t stack_bytes = 0;
-    uint32_t* d_gqls = nullptr;      // GQL filter scratch: touch + dirty marks, 2*V words per query
-    size_t gqls_words = 0;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    hipEvent_t fev[4] = {nullptr, nullptr, nullptr, nullptr};
-    double filter_ms[3] = {0, 0, 0};
-    int num_cus = 256;
-    volatile int stop_requested = 0;
-};
-
-struct sgm_graph {
-    sgm_ctx* ctx = nullptr;
-    uint32_t V = 0;
-    uint64_t adj_len = 0;
-    uint32_t max_deg = 0;
-    uint32_t* d_off = nullptr;
-    uint32_t* d_adj = nullptr;
-    uint32_t* d_bydeg = nullptr;
-    uint32_t* d_gecnt = nullptr;
-    uint32_t ge_cnt[33] = {0};
-    // adjacency bitmaps of the highest-degree vertices (degree >= kHubMinDegree, within
-    // kHubBudgetBytes); hub_slot[v] = bitmap index or SGM_INVALID
-    uint32_t n_hubs = 0;
-    uint32_t hub_words = 0;
-    uint32_t* d_hub_slot = nullptr;
-    uint32_t* d_hub_bits = nullptr;
-    uint8_t* d_nbrdeg = nullptr;  // 32*V bytes: order statistics of neighbour degrees (GQL shortcut)
-    uint8_t* d_degc = nullptr;    // V bytes: min(degree, 255)
-};
-
-// GQL scratch is allocated per filter batch; larger batches run in chunks of this size.
-static const uint64_t kGqlScratchBytes = 8ull << 30;
-
-static const uint32_t kHubMinDegree = 64;
-static const uint64_t kHubBudgetBytes = 1ull << 30;
-
-struct sgm_table {
-    bool built = false;
-    uint64_t e_cnt = 0;
-    uint32_t max_deg = 0;
-    uint32_t* d_off = nullptr;  // |C(a)|+1
-    uint32_t* d_idx = nullptr;
-    uint32_t* d_v = nullptr;
-};
-
-struct sgm_query {
-    sgm_graph* g = nullptr;
-    sgm::QueryCSR q;
-    uint32_t nbr[SGM_MAXQ] = {0};
-    int filtered = 0;                 // SGM_FILTER_* after a successful or empty filter
-    bool empty = false;
-    bool compacted = false;
-    uint32_t counts[SGM_MAXQ] = {0};
-    uint32_t cand_off[SGM_MAXQ + 1] = {0};
-    uint32_t* d_vmask = nullptr;
-    uint32_t* d_removed = nullptr;
-    uint64_t* d_bm = nullptr;
-    uint32_t* d_rank = nullptr;
-    uint32_t* d_cnt = nullptr;
-    uint32_t* d_candoff = nullptr;
-    uint32_t* d_list = nullptr;
-    sgm_qdesc* d_qdesc = nullptr;
-    sgm_cdesc* d_cdesc = nullptr;
-    sgm_qdesc hq;                     // host copies of the descriptors
-    sgm_cdesc hc;
-    std::vector<sgm_table> tables;    // n*n, directed (a,b)
-};
-
-static int set_err(sgm_ctx* c, int code, const char* fmt, ...)
-{
-    if (c) {
-        char buf[512];
-        va_list ap;
-        va_start(ap, fmt);
-        vsnprintf(buf, sizeof buf, fmt, ap);
-        va_end(ap);
-        c->err = buf;
-    }
-    return code;
-}
-
-#define HIPCHK(ctx, call)                                                                          \
-    do {                                                                                           \
-        hipError_t e_ = (call);                                                                    \
-        if (e_ != hipSuccess)                                                                      \
-            return set_err((ctx), e_ == hipErrorOutOfMemory ? SGM_ENOMEM : SGM_EHIP, "%s: %s (%s:%d)", \
-                           #call, hipGetErrorString(e_), __FILE__, __LINE__);                      \
-    } while (0)
-
-template <class T>
-static hipError_t dalloc(T** p, size_t count)
-{
-    *p = nullptr;
-    if (count == 0) count = 1;
-    return hipMalloc((void**)p, count * sizeof(T));
-}
-
-static void dfree(void* p)
-{
-    if (p) (void)hipFree(p);
-}
-
-// (Re)allocate a context buffer so that it holds at least `need` elements.
-template <class T>
-static int grow(sgm_ctx* c, T** p, uint32_t& cap_now, uint32_t need)
-{
-    if (cap_now >= need) return SGM_OK;
-    dfree(*p);
-    *p = nullptr;
-    cap_now = 0;
-    HIPCHK(c, dalloc(p, need));
-    cap_now = need;
-    return SGM_OK;
-}
+#include "sgm_objects.h"
 
 extern "C" {
 
@@ -299,7 +165,11 @@ int sgm_graph_upload(sgm_ctx* c, const uint32_t* off, const uint32_t* adj, uint3
         while (h < V && off[bydeg[h] + 1] - off[bydeg[h]] >= kHubMinDegree && (h + 1) * per <= kHubBudgetBytes) h++;
         g->n_hubs = (uint32_t)h;
         std::vector<uint32_t> slot(V, SGM_INVALID);
-        for (uint32_t i = 0; i < g->n_hubs; i++) slot[bydeg[i]] = i;
+        for (uint32_t i = 0; i < g->n_hubs; i++) {
+            slot[bydeg[i]] = i;
+            g->hub_vertex.push_back(bydeg[i]);
+            g->hub_degree.push_back(off[bydeg[i] + 1] - off[bydeg[i]]);
+        }
         e = dalloc(&g->d_hub_slot, V);
         if (e == hipSuccess) e = hipMemcpy(g->d_hub_slot, slot.data(), sizeof(uint32_t) * V, hipMemcpyHostToDevice);
         if (e == hipSuccess && g->n_hubs) {
@@ -331,6 +201,7 @@ void sgm_graph_destroy(sgm_graph* g)
     dfree(g->d_hub_bits);
     dfree(g->d_nbrdeg);
     dfree(g->d_degc);
+    for (auto& m : g->hub_maps) dfree(m.second);
     delete g;
 }
 
@@ -715,7 +586,9 @@ int sgm_plan_export(const uint32_t* qoff, const uint32_t* qadj, uint32_t n, cons
 }
 
 // ------------------------------------------------------------------------------ enumerate
-static int ensure_stack(sgm_ctx* c, size_t bytes)
+}  // extern "C"
+
+int ensure_stack(sgm_ctx* c, size_t bytes)
 {
     if (c->stack_bytes >= bytes) return SGM_OK;
     dfree(c->d_stack);
@@ -726,7 +599,25 @@ static int ensure_stack(sgm_ctx* c, size_t bytes)
     return SGM_OK;
 }
 
-static sgm_enum_opts normalize_opts(const sgm_enum_opts* opts)
+int hub_slot_map(sgm_graph* g, const sgm_enum_opts& o, const uint32_t** out)
+{
+    *out = nullptr;
+    if ((o.flags & SGM_ENUM_F_NO_BITMAPS) || !g->n_hubs) return SGM_OK;
+    if (o.hub_min_degree <= kHubMinDegree) { *out = g->d_hub_slot; return SGM_OK; }
+    for (auto& m : g->hub_maps)
+        if (m.first == o.hub_min_degree) { *out = m.second; return SGM_OK; }
+    // the hubs are in degree order, so the threshold keeps a prefix of the slots
+    std::vector<uint32_t> slot(g->V, SGM_INVALID);
+    for (uint32_t i = 0; i < g->n_hubs && g->hub_degree[i] >= o.hub_min_degree; i++) slot[g->hub_vertex[i]] = i;
+    uint32_t* d = nullptr;
+    HIPCHK(g->ctx, dalloc(&d, g->V));
+    HIPCHK(g->ctx, hipMemcpy(d, slot.data(), sizeof(uint32_t) * g->V, hipMemcpyHostToDevice));
+    g->hub_maps.emplace_back(o.hub_min_degree, d);
+    *out = d;
+    return SGM_OK;
+}
+
+sgm_enum_opts normalize_opts(const sgm_enum_opts* opts)
 {
     sgm_enum_opts o;
     std::memset(&o, 0, sizeof o);
@@ -736,7 +627,7 @@ static sgm_enum_opts normalize_opts(const sgm_enum_opts* opts)
     return o;
 }
 
-static uint32_t default_blocks(const sgm_ctx* c, const sgm_enum_opts& o, uint32_t nq)
+uint32_t default_blocks(const sgm_ctx* c, const sgm_enum_opts& o, uint32_t nq)
 {
     // A single capped query: only its first embeddings matter and extra waves add overshoot.
     // A batch (or an uncapped search) wants the wave slots busy.
@@ -745,7 +636,7 @@ static uint32_t default_blocks(const sgm_ctx* c, const sgm_enum_opts& o, uint32_
     return std::max<uint32_t>(1, std::min<uint32_t>(b, 1u << 16));
 }
 
-static void stats_out(const sgm_stats_dev& st, uint64_t cap, sgm_stats* out)
+void stats_out(const sgm_stats_dev& st, uint64_t cap, sgm_stats* out)
 {
     out->count = st.count < cap ? st.count : cap;
     out->capped = st.count >= cap;
@@ -756,6 +647,8 @@ static void stats_out(const sgm_stats_dev& st, uint64_t cap, sgm_stats* out)
     out->elems_out = st.elems_out;
     out->elems_scanned = st.elems_scanned;
 }
+
+extern "C" {
 
 // explore_Intersection_All<false x5> for one query over its device tables.
 static int enumerate_aux(sgm_query* q, const uint32_t* order, const sgm_enum_opts& o, sgm_stats* out)
@@ -876,7 +769,8 @@ int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uin
     const uint32_t waves = blocks * 4;
     uint64_t words = o.stack_words ? o.stack_words : (uint64_t)8 * g->max_deg + 1024;
     const int use_fs = o.method == SGM_ENUM_CACHE_FILTER_FS ? 1 : 0;
-    const uint32_t* hub_slot = (o.flags & SGM_ENUM_F_NO_BITMAPS) || !g->n_hubs ? nullptr : g->d_hub_slot;
+    const uint32_t* hub_slot = nullptr;
+    if ((rc = hub_slot_map(g, o, &hub_slot))) return rc;
     std::vector<sgm_stats_dev> st(nq);
     std::vector<sgm_equery> eq;
     for (uint32_t attempt = 0; !live.empty(); attempt++) {

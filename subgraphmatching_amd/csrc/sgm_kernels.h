@@ -22,6 +22,10 @@ namespace sgm {
 void launch_nlf(hipStream_t s, const uint32_t* off, uint32_t V, const sgm_qdesc* qs, uint32_t nq);
 void launch_gql_phase(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint8_t* degc,
                       const uint8_t* nbr_deg, uint32_t V, const sgm_qdesc* qs, uint32_t nq, uint32_t phase);
+// batch candidate statistics: cnt[q*32+u] = |C(u)|, degsum[q*32+u] = sum of degrees (zeroed first)
+void launch_cand_stats(hipStream_t s, const uint32_t* off, uint32_t V, const sgm_qdesc* qs, uint32_t nq,
+                       uint32_t* cnt, unsigned long long* degsum);
+void launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t val);
 // degree table for the GQL shortcut: nbr_deg[j*V + v], 32 bytes per vertex
 void launch_nbr_deg(hipStream_t s, const uint32_t* off, const uint32_t* adj, uint32_t V, uint8_t* tab);
 void launch_bitmaps(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V);

@@ -20,6 +20,7 @@ __global__ void __launch_bounds__(256) k_nlf(const uint32_t* __restrict__ off, u
 {
     const sgm_qdesc* d = qs + blockIdx.y;
     const uint32_t n = d->n;
+    if (n == 0) return;  // unused batch slot
     uint32_t* __restrict__ vm = d->vmask;
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
         uint32_t dv = off[v + 1] - off[v], m = 0;
@@ -316,6 +317,7 @@ __global__ void __launch_bounds__(256) k_bitmaps(const sgm_cdesc* __restrict__ c
 {
     const sgm_cdesc* c = cs + blockIdx.y;
     const uint32_t n = c->n, W = (V + 63) / 64;
+    if (n == 0) return;
     const uint32_t waves = blockDim.x >> 6;
     const uint32_t lane = lane_id();
     for (uint32_t w = blockIdx.x * waves + (threadIdx.x >> 6); w < W; w += gridDim.x * waves) {
@@ -362,6 +364,53 @@ __global__ void __launch_bounds__(256) k_rank_scan(const sgm_cdesc* __restrict__
         __syncthreads();
     }
     if (threadIdx.x == 0) c->cnt[u] = carry;
+}
+
+// Candidate statistics of a batch without lists: cnt[q*32 + u] = |C(u)| and
+// degsum[q*32 + u] = sum of deg(v) over C(u) (the cost model's mean candidate degree).
+// Both start zeroed.  Waves stream 64 vertices; block totals go out with one atomic each.
+__global__ void __launch_bounds__(256) k_cand_stats(const uint32_t* __restrict__ off, uint32_t V,
+                                                    const sgm_qdesc* __restrict__ qs, uint32_t* __restrict__ cnt,
+                                                    unsigned long long* __restrict__ degsum)
+{
+    const sgm_qdesc* d = qs + blockIdx.y;
+    const uint32_t n = d->n;
+    if (n == 0) return;
+    __shared__ uint32_t s_cnt[SGM_MAXQ];
+    __shared__ unsigned long long s_sum[SGM_MAXQ];
+    if (threadIdx.x < SGM_MAXQ) { s_cnt[threadIdx.x] = 0; s_sum[threadIdx.x] = 0; }
+    __syncthreads();
+    const uint32_t* __restrict__ vm = d->vmask;
+    const uint32_t waves = blockDim.x >> 6, lane = lane_id();
+    for (uint32_t w = blockIdx.x * waves + (threadIdx.x >> 6); w < (V + 63) / 64; w += gridDim.x * waves) {
+        const uint32_t v = w * 64 + lane;
+        const uint32_t m = v < V ? vm[v] : 0u;
+        const uint32_t dv = m ? off[v + 1] - off[v] : 0u;
+        if (__ballot(m != 0u) == 0) continue;
+        for (uint32_t u = 0; u < n; u++) {
+            const uint64_t b = __ballot((m >> u) & 1u);
+            if (!b) continue;
+            long long x = ((m >> u) & 1u) ? (long long)dv : 0ll;  // 64 degrees may exceed 2^32
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o);
+            if (lane == 0) {
+                atomicAdd(&s_cnt[u], (uint32_t)__popcll(b));
+                atomicAdd(&s_sum[u], (unsigned long long)x);
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < n) {
+        const size_t i = (size_t)blockIdx.y * SGM_MAXQ + threadIdx.x;
+        if (s_cnt[threadIdx.x]) atomicAdd(&cnt[i], s_cnt[threadIdx.x]);
+        if (s_sum[threadIdx.x]) atomicAdd(&degsum[i], s_sum[threadIdx.x]);
+    }
+}
+
+// Word fill with vector stores (graph-capturable, unlike a memset node).
+__global__ void __launch_bounds__(256) k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t val)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = val;
 }
 
 // Compact C(u) in ascending vertex order into list[cand_off[u] ...].
@@ -498,6 +547,20 @@ void launch_gql_phase(hipStream_t s, const uint32_t* off, const uint32_t* adj, c
     if (per_q > max_blocks) per_q = (uint32_t)max_blocks;
     dim3 grid(per_q, nq);
     hipLaunchKernelGGL(k_gql_phase, grid, dim3(256), 0, s, off, adj, degc, nbr_deg, V, qs, phase);
+}
+
+void launch_cand_stats(hipStream_t s, const uint32_t* off, uint32_t V, const sgm_qdesc* qs, uint32_t nq,
+                       uint32_t* cnt, unsigned long long* degsum)
+{
+    const uint64_t W = ((uint64_t)V + 63) / 64;
+    dim3 grid(clampg((W + 3) / 4, 1, 256), nq);
+    hipLaunchKernelGGL(k_cand_stats, grid, dim3(256), 0, s, off, V, qs, cnt, degsum);
+}
+
+void launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t val)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_fill_u32, dim3(clampg((n + 255) / 256, 1, 8192)), dim3(256), 0, s, p, n, val);
 }
 
 void launch_nbr_deg(hipStream_t s, const uint32_t* off, const uint32_t* adj, uint32_t V, uint8_t* tab)

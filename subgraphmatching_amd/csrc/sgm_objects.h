@@ -1,0 +1,158 @@
+// Library objects behind the C ABI's opaque handles (context, resident graph, query with
+// its candidate space) and the helpers the driver sources share.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sgm_c.h"
+#include "sgm_host.h"
+#include "sgm_internal.h"
+#include "sgm_kernels.h"
+
+struct sgm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipStream_t ctl = nullptr;       // non-blocking stream for sgm_stop
+    std::string err;
+    uint32_t* d_stop = nullptr;      // stop word polled by the enumeration kernel
+    uint32_t* d_work = nullptr;      // work counter
+    sgm_stats_dev* d_stats = nullptr;
+    sgm_auxplan_dev* d_auxplan = nullptr;
+    unsigned long long* d_work64 = nullptr;  // item counter of the batched search
+    sgm_plan_dev* d_plans = nullptr;          // per-query plans of a batch
+    sgm_equery* d_eq = nullptr;
+    sgm_stats_dev* d_qstats = nullptr;
+    uint32_t* d_qstop = nullptr;
+    uint32_t batch_cap = 0, batch_cap2 = 0;
+    uint32_t* d_stack = nullptr;
+    size_t stack_bytes = 0;
+    uint32_t* d_gqls = nullptr;      // GQL filter scratch: touch + dirty marks, 2*V words per query
+    size_t gqls_words = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t fev[4] = {nullptr, nullptr, nullptr, nullptr};
+    double filter_ms[3] = {0, 0, 0};
+    int num_cus = 256;
+    volatile int stop_requested = 0;
+};
+
+struct sgm_graph {
+    sgm_ctx* ctx = nullptr;
+    uint32_t V = 0;
+    uint64_t adj_len = 0;
+    uint32_t max_deg = 0;
+    uint32_t* d_off = nullptr;
+    uint32_t* d_adj = nullptr;
+    uint32_t* d_bydeg = nullptr;
+    uint32_t* d_gecnt = nullptr;
+    uint32_t ge_cnt[33] = {0};
+    // adjacency bitmaps of the highest-degree vertices (degree >= kHubMinDegree, within
+    // kHubBudgetBytes); hub_slot[v] = bitmap index or SGM_INVALID
+    uint32_t n_hubs = 0;
+    uint32_t hub_words = 0;
+    uint32_t* d_hub_slot = nullptr;
+    uint32_t* d_hub_bits = nullptr;
+    uint8_t* d_nbrdeg = nullptr;  // 32*V bytes: order statistics of neighbour degrees (GQL shortcut)
+    uint8_t* d_degc = nullptr;    // V bytes: min(degree, 255)
+    std::vector<uint32_t> hub_vertex;                   // bitmap slot -> vertex (degree descending)
+    std::vector<uint32_t> hub_degree;
+    std::vector<std::pair<uint32_t, uint32_t*>> hub_maps;  // (min degree, slot map) for higher thresholds
+};
+
+// GQL scratch is allocated per filter batch; larger batches run in chunks of this size.
+static const uint64_t kGqlScratchBytes = 8ull << 30;
+
+static const uint32_t kHubMinDegree = 64;
+static const uint64_t kHubBudgetBytes = 1ull << 30;
+
+struct sgm_table {
+    bool built = false;
+    uint64_t e_cnt = 0;
+    uint32_t max_deg = 0;
+    uint32_t* d_off = nullptr;  // |C(a)|+1
+    uint32_t* d_idx = nullptr;
+    uint32_t* d_v = nullptr;
+};
+
+struct sgm_query {
+    sgm_graph* g = nullptr;
+    sgm::QueryCSR q;
+    uint32_t nbr[SGM_MAXQ] = {0};
+    int filtered = 0;                 // SGM_FILTER_* after a successful or empty filter
+    bool empty = false;
+    bool compacted = false;
+    uint32_t counts[SGM_MAXQ] = {0};
+    uint32_t cand_off[SGM_MAXQ + 1] = {0};
+    uint32_t* d_vmask = nullptr;
+    uint32_t* d_removed = nullptr;
+    uint64_t* d_bm = nullptr;
+    uint32_t* d_rank = nullptr;
+    uint32_t* d_cnt = nullptr;
+    uint32_t* d_candoff = nullptr;
+    uint32_t* d_list = nullptr;
+    sgm_qdesc* d_qdesc = nullptr;
+    sgm_cdesc* d_cdesc = nullptr;
+    sgm_qdesc hq;                     // host copies of the descriptors
+    sgm_cdesc hc;
+    std::vector<sgm_table> tables;    // n*n, directed (a,b)
+};
+
+inline int set_err(sgm_ctx* c, int code, const char* fmt, ...)
+{
+    if (c) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        c->err = buf;
+    }
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                          \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return set_err((ctx), e_ == hipErrorOutOfMemory ? SGM_ENOMEM : SGM_EHIP, "%s: %s (%s:%d)", \
+                           #call, hipGetErrorString(e_), __FILE__, __LINE__);                      \
+    } while (0)
+
+template <class T>
+inline hipError_t dalloc(T** p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    return hipMalloc((void**)p, count * sizeof(T));
+}
+
+inline void dfree(void* p)
+{
+    if (p) (void)hipFree(p);
+}
+
+// (Re)allocate a context buffer so that it holds at least `need` elements.
+template <class T>
+inline int grow(sgm_ctx* c, T** p, uint32_t& cap_now, uint32_t need)
+{
+    if (cap_now >= need) return SGM_OK;
+    dfree(*p);
+    *p = nullptr;
+    cap_now = 0;
+    HIPCHK(c, dalloc(p, need));
+    cap_now = need;
+    return SGM_OK;
+}
+
+// Shared by the enumeration drivers (sgm_capi.cpp, sgm_batch.cpp).
+// Slot map of the hub bitmaps to use for opts (nullptr = binary search everywhere).
+int hub_slot_map(sgm_graph* g, const sgm_enum_opts& o, const uint32_t** out);
+int ensure_stack(sgm_ctx* c, size_t bytes);
+sgm_enum_opts normalize_opts(const sgm_enum_opts* opts);
+uint32_t default_blocks(const sgm_ctx* c, const sgm_enum_opts& o, uint32_t nq);
+void stats_out(const sgm_stats_dev& st, uint64_t cap, sgm_stats* out);
