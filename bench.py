@@ -2,20 +2,28 @@
 """Headline benchmark: exact embeddings/s and queries/s on one MI355X for the
 filter -> order -> enumerate path of unlabeled subgraph matching.
 
-Workload (one step = one pass of the path over one batch):
-  data graph  Chung-Lu, 100,000 vertices, degree exponent 2.3, average degree 16, seed 2
-              (unlabeled, simple, sorted u32 CSR; generated once, resident in HBM)
-  queries     200 dense 16-vertex random-walk queries (average degree > 3), seed 22
-  path        NLF + GQL candidate filtering for all 200 queries in one batch of launches,
-              RI order (host), cache + filter + failing-set enumeration of all 200 queries
-              in one persistent launch (per-query caps and statistics)
-  output      exact counts, capped at --cap embeddings per query (default 10^7, the same
-              cap for the GPU run and the CPU baseline; uncapped counts of these queries
-              run past 10^9 and cannot be enumerated in bounded time)
+Workloads (one step = one pass of the path over the whole query set):
 
-The CPU baseline is the project's C restatement of the reference path (oracle/, the
-checker), run single-threaded on this host over a bounded sample of the same queries; the
-GPU counts of the sampled queries are compared with it.
+  chunglu100k (default)
+    data graph  Chung-Lu, 100,000 vertices, degree exponent 2.3, average degree 16, seed 2
+                (unlabeled, simple, sorted u32 CSR; generated once, resident in HBM)
+    queries     200 dense 16-vertex random-walk queries (average degree > 3), seed 22
+    path        NLF + GQL filtering of all 200 queries in one set of launches, RI order
+                (host), cache + filter + failing-set enumeration of all 200 queries in one
+                persistent launch (per-query caps and statistics)
+  rmat20
+    data graph  R-MAT scale 20 (a,b,c,d = .57,.19,.19,.05), 2*10^7 undirected edges, seed 3
+    queries     1,000 random-walk queries, sizes uniform in 4..32, even ids dense / odd
+                sparse, seed 33
+    path        batches of 64 through the batched runner: hipGraph-replayed NLF + GQL
+                filter, RI orders, hipGraph-replayed persistent enumeration with the
+                autotuned grid size and intersection strategy (tuned once on the first
+                batch, untimed); a single-launch pass over all 1,000 queries is also timed
+
+Counts are exact, capped at --cap embeddings per query (default 10^7, the same cap for the
+GPU run and the CPU baseline).  The CPU baseline is the project's C restatement of the
+reference path (oracle/, the checker), run single-threaded on this host over a bounded
+sample of the same queries; the GPU counts of the sampled queries are compared with it.
 """
 from __future__ import annotations
 
@@ -29,6 +37,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "exact embeddings/sec & queries/sec on 1 MI355X; learned vs RI order speedup"
 
 
 def parse():
@@ -36,12 +45,14 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--queries", type=int, default=200)
+    ap.add_argument("--workload", choices=["chunglu100k", "rmat20"], default="chunglu100k")
+    ap.add_argument("--queries", type=int, default=0, help="number of queries (0 = the workload's own)")
     ap.add_argument("--cap", type=int, default=10_000_000)
     ap.add_argument("--per-query", action="store_true",
-                    help="one enumeration launch per query instead of one batched persistent launch")
+                    help="chunglu100k: one enumeration launch per query instead of one batched launch")
     ap.add_argument("--blocks", type=int, default=0,
-                    help="persistent enumeration blocks of 4 waves (0 = library default)")
+                    help="persistent enumeration blocks of 4 waves (0 = library default / autotuned)")
+    ap.add_argument("--batch", type=int, default=64, help="rmat20: queries per graph-replayed batch")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="wall-clock budget of the CPU baseline sample (0 disables it)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
@@ -62,41 +73,53 @@ def copy_calibration(nbytes):
     print(json.dumps({"copy_calibration_bytes": nbytes, "ok": bool(y[-1].item() == 1)}), flush=True)
 
 
-def main():
-    args = parse()
-    if args.copy_calibration:
-        copy_calibration(args.copy_calibration)
-        return
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}): the path shards by query "
-              "and nothing in this build uses a second GPU", file=sys.stderr)
-        sys.exit(2)
+def new_rec():
+    return {"emb": 0, "inter": 0, "elems_read": 0, "elems_out": 0, "elems_scanned": 0, "enum_ms": 0.0,
+            "enum_launches": 0, "capped": 0, "counts": [], "gql_ms": 0.0, "nlf_ms": 0.0, "cand_ms": 0.0,
+            "filter_ms": 0.0}
 
-    import numpy as np
+
+def add_result(rec, r):
+    if r is None:
+        rec["counts"].append(0)
+        return
+    rec["emb"] += r.ans
+    rec["inter"] += r.intersections
+    rec["elems_read"] += r.elems_read
+    rec["elems_out"] += r.elems_out
+    rec["elems_scanned"] += r.elems_scanned
+    rec["capped"] += int(r.capped)
+    rec["counts"].append(r.ans)
+
+
+# ------------------------------------------------------------------------- chunglu100k
+def setup_chunglu(args):
     import subgraphmatching_amd as sgm
     from subgraphmatching_amd import _lib as L
     from subgraphmatching_amd.generators import chung_lu, query_set
 
-    t_setup = time.perf_counter()
     g = chung_lu(100_000, 2.3, 16.0, seed=2)
-    queries = query_set(g, args.queries, 16, True, seed=22)
+    queries = query_set(g, args.queries or 200, 16, True, seed=22)
     ctx = sgm.default_context()
     cans = [sgm.CandidateParam(g, q, ctx) for q in queries]
-    setup_s = time.perf_counter() - t_setup
-    print(f"[bench] graph V={g.getVertexCnt()} E={g.getEdgeCnt()} maxdeg={g.getMaxDgree()}; "
-          f"{len(queries)} queries; setup {setup_s:.1f}s", flush=True)
 
     def step():
-        rec = {"emb": 0, "inter": 0, "elems_read": 0, "elems_out": 0, "elems_scanned": 0,
-               "enum_ms": 0.0, "enum_launches": 0, "capped": 0, "counts": []}
+        rec = new_rec()
         t0 = time.perf_counter()
         ok = sgm.filter_batch(cans, L.SGM_FILTER_GQL)
-        nlf_ms, gql_ms, cand_ms = ctx.filter_ms()
+        rec["nlf_ms"], rec["gql_ms"], rec["cand_ms"] = ctx.filter_ms()
+        rec["filter_ms"] = rec["nlf_ms"] + rec["gql_ms"] + rec["cand_ms"]
         t1 = time.perf_counter()
         orders = [sgm.Order_RI(q) for q in queries]
         if args.per_query:
-            results = [sgm.enumerate_query(can, order, L.SGM_ENUM_CACHE_FILTER_FS, cap=args.cap, blocks=args.blocks)
-                       if good else None for can, order, good in zip(cans, orders, ok)]
+            results = []
+            for can, order, good in zip(cans, orders, ok):
+                r = sgm.enumerate_query(can, order, L.SGM_ENUM_CACHE_FILTER_FS, cap=args.cap,
+                                        blocks=args.blocks) if good else None
+                if r is not None:
+                    rec["enum_ms"] += r.time
+                    rec["enum_launches"] += 1
+                results.append(r)
         else:
             live = [i for i, good in enumerate(ok) if good]
             batch = sgm.enumerate_batch([cans[i] for i in live], [orders[i] for i in live],
@@ -108,22 +131,97 @@ def main():
                 rec["enum_ms"] += batch[0].time
                 rec["enum_launches"] += 1
         for r in results:
-            if r is None:
-                rec["counts"].append(0)
-                continue
-            rec["emb"] += r.ans
-            rec["inter"] += r.intersections
-            rec["elems_read"] += r.elems_read
-            rec["elems_out"] += r.elems_out
-            rec["elems_scanned"] += r.elems_scanned
-            if args.per_query:
-                rec["enum_ms"] += r.time
-                rec["enum_launches"] += 1
-            rec["capped"] += int(r.capped)
-            rec["counts"].append(r.ans)
-        t2 = time.perf_counter()
-        rec.update(wall_s=t2 - t0, filter_wall_s=t1 - t0, nlf_ms=nlf_ms, gql_ms=gql_ms, cand_ms=cand_ms)
+            add_result(rec, r)
+        rec["wall_s"] = time.perf_counter() - t0
+        rec["filter_wall_s"] = t1 - t0
         return rec
+
+    config = {"workload": "chunglu_V100000_g2.3_d16_seed2__200x16v_dense_rw_seed22__gql_ri_cache_filter_fs",
+              "queries_per_step": len(queries), "cap_per_query": args.cap, "order": "RI",
+              "filter": "NLF+GQL", "enumeration": "cache+filter+failing-sets",
+              "launch": "per-query" if args.per_query else "one filter batch + one persistent enumeration launch"}
+    return g, queries, step, config, {}
+
+
+# ------------------------------------------------------------------------------ rmat20
+def setup_rmat(args):
+    import subgraphmatching_amd as sgm
+    from subgraphmatching_amd import _lib as L
+    from subgraphmatching_amd.generators import query_set, rmat
+
+    g = rmat(20, 20_000_000, seed=3)
+    nq = args.queries or 1000
+    queries = query_set(g, nq, lambda i, r: int(r.integers(4, 33)), lambda i, r: i % 2 == 0, seed=33)
+    B = args.batch
+    runner = sgm.Batch(g, slots=B)
+    chunks = [queries[i:i + B] for i in range(0, nq, B)]
+    # autotune once on the first batch (untimed)
+    runner.filter(chunks[0])
+    tuned, tuned_ms = runner.autotune(cap=args.cap)
+    if args.blocks:
+        tuned["blocks"] = args.blocks
+    print(f"[bench] autotuned on the first batch: {tuned} ({tuned_ms:.2f} ms)", flush=True)
+
+    def step():
+        rec = new_rec()
+        t0 = time.perf_counter()
+        for part in chunks:
+            ok, _, _ = runner.filter(part)
+            f_ms, _ = runner.last_ms()
+            rec["filter_ms"] += f_ms
+            rec["gql_ms"] += f_ms
+            res = runner.enumerate(cap=args.cap, **tuned)
+            _, e_ms = runner.last_ms()
+            rec["enum_ms"] += e_ms
+            rec["enum_launches"] += 1
+            for good, r in zip(ok, res):
+                add_result(rec, r if good else None)
+        rec["wall_s"] = time.perf_counter() - t0
+        rec["filter_wall_s"] = rec["filter_ms"] / 1e3
+        return rec
+
+    def single_launch():
+        """The same work as one filter batch and one persistent launch over all queries."""
+        ctx = sgm.default_context()
+        cans = [sgm.CandidateParam(g, q, ctx) for q in queries]
+        t0 = time.perf_counter()
+        ok = sgm.filter_batch(cans, L.SGM_FILTER_GQL)
+        nlf_ms, gql_ms, cand_ms = ctx.filter_ms()
+        live = [i for i, good in enumerate(ok) if good]
+        res = sgm.enumerate_batch([cans[i] for i in live], [sgm.Order_RI(queries[i]) for i in live],
+                                  L.SGM_ENUM_CACHE_FILTER_FS, cap=args.cap, blocks=tuned["blocks"])
+        wall = time.perf_counter() - t0
+        counts = [0] * len(queries)
+        for i, r in zip(live, res):
+            counts[i] = r.ans
+        for c in cans:
+            c.close()
+        return {"wall_ms": 1e3 * wall, "filter_device_ms": nlf_ms + gql_ms + cand_ms,
+                "enum_device_ms": res[0].time if res else 0.0}, counts
+
+    config = {"workload": "rmat_s20_e2e7_abcd.57.19.19.05_seed3__1000x4-32v_mixed_rw_seed33__gql_ri_cache_filter_fs",
+              "queries_per_step": nq, "cap_per_query": args.cap, "order": "RI", "filter": "NLF+GQL",
+              "enumeration": "cache+filter+failing-sets",
+              "launch": f"hipGraph-replayed batches of {B} (filter graph + persistent enumeration graph)",
+              "autotuned": tuned}
+    return g, queries, step, config, {"single_launch": single_launch}
+
+
+def main():
+    args = parse()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): the path shards by query "
+              "and nothing in this build uses a second GPU", file=sys.stderr)
+        sys.exit(2)
+
+    t_setup = time.perf_counter()
+    setup = setup_rmat if args.workload == "rmat20" else setup_chunglu
+    g, queries, step, config, extra = setup(args)
+    print(f"[bench] {args.workload}: graph V={g.getVertexCnt()} E={g.getEdgeCnt()} maxdeg={g.getMaxDgree()}; "
+          f"{len(queries)} queries; setup {time.perf_counter() - t_setup:.1f}s", flush=True)
 
     for i in range(args.warmup):
         r = step()
@@ -133,8 +231,8 @@ def main():
     for i in range(args.steps):
         r = step()
         recs.append(r)
-        print(f"[bench] step {i}: {r['wall_s']:.3f}s  filter {r['filter_wall_s']:.3f}s "
-              f"(gql {r['gql_ms']:.1f} ms)  enum {r['enum_ms']:.1f} ms  emb {r['emb']}", flush=True)
+        print(f"[bench] step {i}: {r['wall_s']:.3f}s  filter {r['filter_ms']:.1f} ms (gql {r['gql_ms']:.1f} ms)  "
+              f"enum {r['enum_ms']:.1f} ms  emb {r['emb']}", flush=True)
     total_s = time.perf_counter() - t_all
     counts0 = recs[0]["counts"]
     assert all(r["counts"] == counts0 for r in recs), "counts differ between steps"
@@ -144,28 +242,35 @@ def main():
     nq = len(queries) * K
     inter = sum(r["inter"] for r in recs)
     enum_ms = sum(r["enum_ms"] for r in recs)
-    gql_ms = sum(r["gql_ms"] for r in recs)
+    filter_ms = sum(r["filter_ms"] for r in recs)
     launches = sum(r["enum_launches"] for r in recs)
     # algorithmic bytes of the enumeration kernel (merge-intersection model): every
     # intersection reads |A|+|B| and writes |A n B| u32; every iterated candidate costs a
     # 4-byte id plus one filter bit
     algo_bytes = sum(4 * (r["elems_read"] + r["elems_out"]) + 4.125 * r["elems_scanned"] for r in recs)
     achieved = algo_bytes / (enum_ms / 1e3) / 1e9 if enum_ms > 0 else 0.0
-    traffic = None
-    if os.path.exists(args.traffic):
+    traffic = measured = None
+    if args.workload == "chunglu100k" and os.path.exists(args.traffic):
         try:
             with open(args.traffic) as f:
                 traffic = json.load(f).get("k_enum_cache_bytes_per_launch")
         except Exception:
             traffic = None
-    dominant = "k_enum_cache" if enum_ms >= gql_ms else "k_gql_phase"
+    if traffic and launches and not args.per_query:
+        measured = traffic / (enum_ms / launches / 1e3) / 1e9
+
+    extras = {}
+    if "single_launch" in extra:
+        sl, sl_counts = extra["single_launch"]()
+        extras["single_launch"] = dict(sl, counts_match=sl_counts == counts0)
+        print(f"[bench] single launch over all queries: {sl}", flush=True)
 
     cpu = None
     if args.cpu_seconds > 0:
         cpu = cpu_baseline(g, queries, counts0, args.cap, args.cpu_seconds)
 
     out = {
-        "metric": "exact embeddings/sec & queries/sec on 1 MI355X; learned vs RI order speedup",
+        "metric": METRIC,
         "value": emb / total_s,
         "unit": "embeddings/s",
         "n_gpus": 1,
@@ -177,29 +282,32 @@ def main():
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic",
-        "config": {"workload": "chunglu_V100000_g2.3_d16_seed2__200x16v_dense_rw_seed22__gql_ri_cache_filter_fs",
-                   "queries_per_step": len(queries), "cap_per_query": args.cap, "order": "RI",
-                   "filter": "NLF+GQL", "enumeration": "cache+filter+failing-sets"},
+        "config": config,
         "queries_per_s": nq / total_s,
         "intersections_per_s": inter / total_s,
         "embeddings_per_step": emb // K,
         "capped_queries_per_step": recs[0]["capped"],
-        "device_ms_per_step": {"gql_filter": gql_ms / K, "enumerate": enum_ms / K,
-                               "nlf": sum(r["nlf_ms"] for r in recs) / K,
-                               "candidate_counts": sum(r["cand_ms"] for r in recs) / K},
+        "device_ms_per_step": {"filter": filter_ms / K, "gql_filter": sum(r["gql_ms"] for r in recs) / K,
+                               "enumerate": enum_ms / K},
         "roofline": {"kernel": "k_enum_cache", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "dominant_kernel": dominant,
+                     "achieved_basis": "merge-model algorithmic bytes per launch / average launch time",
+                     "measured_gbs": measured,
+                     "measured_frac": measured / HBM_PEAK_GBS if measured is not None else None,
                      "algorithmic_bytes_per_launch": algo_bytes / max(launches, 1),
-                     "avg_launch_ms": enum_ms / max(launches, 1)},
+                     "avg_launch_ms": enum_ms / max(launches, 1),
+                     "dominant_kernel": "k_enum_cache" if enum_ms >= filter_ms else "k_gql_phase"},
         "cpu_baseline": cpu,
     }
+    out.update(extras)
     print(json.dumps(out), flush=True)
 
 
 def cpu_baseline(g, queries, gpu_counts, cap, budget_s):
     """Single-threaded CPU restatement of the same path on a bounded sample of the same
-    queries (GQL filter + RI order + cache/filter/failing-set enumeration, same cap)."""
+    queries (GQL filter + RI order + cache/filter/failing-set enumeration, same cap).  A
+    query still running when the budget ends is cut off by the oracle's own time limit and
+    left out of the sample."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     O.build()
@@ -208,13 +316,16 @@ def cpu_baseline(g, queries, gpu_counts, cap, budget_s):
     enum_ms = 0.0
     parity = True
     for i, q in enumerate(queries):
-        if time.perf_counter() - t0 > budget_s:
+        left = budget_s - (time.perf_counter() - t0)
+        if left <= 0:
             break
         ok, c = O.gql(g, q)
         order = O.order_ri(q)
         cnt = 0
         if ok:
-            st = O.enum_cache(g, q, c, order, cap=cap)
+            st = O.enum_cache(g, q, c, order, cap=cap, time_limit_ms=max(1.0, 1e3 * left))
+            if st["timed_out"]:
+                break
             cnt = st["count"]
             enum_ms += st["enum_ms"]
         parity &= cnt == gpu_counts[i]
@@ -222,10 +333,11 @@ def cpu_baseline(g, queries, gpu_counts, cap, budget_s):
         done += 1
     wall = time.perf_counter() - t0
     print(f"[bench] cpu baseline: {done} queries in {wall:.1f}s, parity={parity}", flush=True)
-    return {"value": emb / wall, "unit": "embeddings/s", "cores": 1, "kind": "port",
+    return {"value": emb / wall if wall > 0 else 0.0, "unit": "embeddings/s", "cores": 1, "kind": "port",
             "sample": f"first {done} of the {len(queries)} queries (GQL + RI + cache/filter/fs enumeration, "
                       f"cap {cap}), single thread, {wall:.1f} s wall",
-            "queries_per_s": done / wall, "enum_thread_ms": enum_ms, "counts_match_gpu": bool(parity)}
+            "queries_per_s": done / wall if wall > 0 else 0.0, "enum_thread_ms": enum_ms,
+            "counts_match_gpu": bool(parity)}
 
 
 if __name__ == "__main__":
