@@ -172,6 +172,46 @@ int sgm_batch_last_ms(const sgm_batch* b, double* filter_ms, double* enum_ms);
 int sgm_batch_autotune(sgm_batch* b, const uint32_t* const* orders, const sgm_enum_opts* base, sgm_enum_opts* best,
                        double* best_ms);
 
+/* ---- learned-order cost model (no reference counterpart; the reference samples orders,
+ * inc/order.h:1883-1959, but has no learned ranking) -----------------------------------
+ * A 3-layer message-passing network over the query graph (hidden 64, BF16 operands on
+ * MFMA with FP32 accumulation) that predicts log2(1 + search-tree nodes) of a (query,
+ * order) pair from per-vertex features: query degree, |C(u)|, mean candidate degree and the
+ * order position with its backward neighbours.  It only ranks orders; counts never depend
+ * on it. */
+typedef struct sgm_costmodel sgm_costmodel;
+typedef struct {
+    uint32_t n;                   /* query vertices (1..32) */
+    const uint32_t* qoff;         /* query CSR */
+    const uint32_t* qadj;
+    const uint32_t* order;        /* a permutation of 0..n-1 */
+    const uint32_t* cand_count;   /* n: |C(u)| */
+    const uint64_t* cand_degsum;  /* n: sum of the data degrees over C(u) */
+} sgm_cm_sample;
+int sgm_costmodel_create(sgm_ctx* ctx, uint64_t seed, sgm_costmodel** out);
+void sgm_costmodel_destroy(sgm_costmodel* m);
+uint32_t sgm_costmodel_param_count(void);
+/* Copy the fp32 parameters out (get) and/or in (set); each sgm_costmodel_param_count() floats. */
+int sgm_costmodel_params(sgm_costmodel* m, float* get, const float* set);
+/* The model input of one sample: feats[32*32] (row = order position), backward / forward
+ * neighbour masks per row. */
+int sgm_costmodel_features(const sgm_cm_sample* s, float* feats, uint32_t* back, uint32_t* fwd);
+/* Predicted log2(1 + nodes) of each sample. */
+int sgm_costmodel_predict(sgm_costmodel* m, const sgm_cm_sample* s, uint32_t count, float* out);
+/* Mean squared error and its gradient over the samples (no update). */
+int sgm_costmodel_grad(sgm_costmodel* m, const sgm_cm_sample* s, const float* target, uint32_t count, float* grad,
+                       float* loss);
+/* Adam training (targets: log2(1 + nodes)); epoch_loss[epochs] gets each epoch's mean loss. */
+int sgm_costmodel_train(sgm_costmodel* m, const sgm_cm_sample* s, const float* target, uint32_t count,
+                        uint32_t epochs, uint32_t minibatch, float lr, uint64_t seed, float* epoch_loss);
+/* Accumulated device time (ms) and MFMA flops of training and prediction. */
+int sgm_costmodel_timing(const sgm_costmodel* m, double* train_ms, double* train_flops, double* infer_ms,
+                         double* infer_flops);
+/* A seeded random connected order of a query (the sampled orders the model ranks): start
+ * at a uniformly chosen vertex, then repeatedly a uniformly chosen unvisited vertex adjacent
+ * to the visited ones. */
+int sgm_order_sample(const uint32_t* qoff, const uint32_t* qadj, uint32_t n, uint64_t seed, uint32_t* order);
+
 #ifdef __cplusplus
 }
 #endif

@@ -46,6 +46,9 @@ EXPORTS = [
     "sgm_order_ri", "sgm_order_gql", "sgm_plan_export", "sgm_enumerate", "sgm_enumerate_batch",
     "sgm_batch_create", "sgm_batch_destroy", "sgm_batch_filter", "sgm_batch_order_gql", "sgm_batch_enumerate",
     "sgm_batch_last_ms", "sgm_batch_autotune",
+    "sgm_costmodel_create", "sgm_costmodel_destroy", "sgm_costmodel_param_count", "sgm_costmodel_params",
+    "sgm_costmodel_features", "sgm_costmodel_predict", "sgm_costmodel_grad", "sgm_costmodel_train",
+    "sgm_costmodel_timing", "sgm_order_sample",
 ]
 
 
@@ -64,6 +67,11 @@ class Stats(C.Structure):
                 ("elems_scanned", C.c_uint64), ("kernel_ms", C.c_double),
                 ("capped", C.c_int32), ("stopped", C.c_int32), ("retries", C.c_uint32),
                 ("reserved", C.c_uint32)]
+
+
+class CmSample(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("qoff", C.c_void_p), ("qadj", C.c_void_p), ("order", C.c_void_p),
+                ("cand_count", C.c_void_p), ("cand_degsum", C.c_void_p)]
 
 
 class SgmError(RuntimeError):
@@ -118,6 +126,18 @@ def lib() -> C.CDLL:
         "sgm_batch_last_ms": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "sgm_batch_autotune": (C.c_int, [vp, vp, C.POINTER(EnumOpts), C.POINTER(EnumOpts),
                                          C.POINTER(C.c_double)]),
+        "sgm_costmodel_create": (C.c_int, [vp, C.c_uint64, C.POINTER(vp)]),
+        "sgm_costmodel_destroy": (None, [vp]),
+        "sgm_costmodel_param_count": (C.c_uint32, []),
+        "sgm_costmodel_params": (C.c_int, [vp, vp, vp]),
+        "sgm_costmodel_features": (C.c_int, [C.POINTER(CmSample), vp, vp, vp]),
+        "sgm_costmodel_predict": (C.c_int, [vp, vp, C.c_uint32, vp]),
+        "sgm_costmodel_grad": (C.c_int, [vp, vp, vp, C.c_uint32, vp, vp]),
+        "sgm_costmodel_train": (C.c_int, [vp, vp, vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64,
+                                          vp]),
+        "sgm_costmodel_timing": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                           C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "sgm_order_sample": (C.c_int, [vp, vp, C.c_uint32, C.c_uint64, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

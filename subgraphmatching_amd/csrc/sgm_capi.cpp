@@ -552,6 +552,41 @@ int sgm_order_ri(const uint32_t* qoff, const uint32_t* qadj, uint32_t n, uint32_
     return SGM_OK;
 }
 
+int sgm_order_sample(const uint32_t* qoff, const uint32_t* qadj, uint32_t n, uint64_t seed, uint32_t* order)
+{
+    sgm::QueryCSR q;
+    if (!order || csr_from(qoff, qadj, n, q)) return SGM_EINVAL;
+    // splitmix64 stream; uniform choice among k by the high half of a 64x32 product
+    uint64_t st = seed;
+    auto next = [&]() {
+        uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    };
+    auto pick = [&](uint32_t k) { return (uint32_t)(((next() >> 32) * (uint64_t)k) >> 32); };
+    uint32_t visited = 0, frontier = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        uint32_t u;
+        if (i == 0 || !frontier) {  // start (or a new component of a disconnected query)
+            uint32_t cand[SGM_MAXQ], k = 0;
+            for (uint32_t x = 0; x < n; x++)
+                if (!((visited >> x) & 1u)) cand[k++] = x;
+            u = cand[pick(k)];
+        } else {
+            uint32_t cand[SGM_MAXQ], k = 0;
+            for (uint32_t x = 0; x < n; x++)
+                if ((frontier >> x) & 1u) cand[k++] = x;
+            u = cand[pick(k)];
+        }
+        order[i] = u;
+        visited |= 1u << u;
+        for (uint32_t p = q.off[u]; p < q.off[u + 1]; p++) frontier |= 1u << q.adj[p];
+        frontier &= ~visited;
+    }
+    return SGM_OK;
+}
+
 int sgm_order_gql(const sgm_query* q, uint32_t* order)
 {
     if (!q || !order) return SGM_EINVAL;

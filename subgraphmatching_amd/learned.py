@@ -1,0 +1,159 @@
+"""Learned matching orders: the cost model (sgm_costmodel_*) and the order ranking built on it.
+
+The model predicts log2(1 + search-tree nodes) of a (query, order) pair from the query graph,
+the order and the candidate statistics of the filtered query; its kernels run on the GPU
+(BF16 MFMA, FP32 accumulation) for training and inference.  Ranking changes only the order
+of a query, never its count: every order of a connected query enumerates the same
+embeddings.  ``choose_orders`` falls back to the heuristic order unless the model predicts a
+clear gain.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib as L
+from .api import Context, _check, default_context
+from .graph import Query
+
+__all__ = ["CostModel", "sample_order", "candidate_orders", "choose_orders", "node_label"]
+
+
+def sample_order(q: Query, seed: int) -> list:
+    """A seeded random connected order (sgm_order_sample)."""
+    o = np.zeros(q.getVertexCnt(), dtype=np.uint32)
+    qo, qa = L.u32(q.off), L.u32(q.adj)
+    _check(L.lib().sgm_order_sample(L.ptr(qo), L.ptr(qa), q.getVertexCnt(), int(seed) & (2**64 - 1), L.ptr(o)))
+    return [int(x) for x in o]
+
+
+def node_label(nodes) -> float:
+    """Training target of an enumeration: log2(1 + total search-tree nodes)."""
+    return math.log2(1.0 + float(sum(nodes)))
+
+
+class _Samples:
+    """ctypes array of sgm_cm_sample with the numpy buffers it points into."""
+
+    def __init__(self, items):
+        self.keep = []
+        self.arr = (L.CmSample * len(items))()
+        for k, (q, order, counts, degsum) in enumerate(items):
+            n = q.getVertexCnt()
+            bufs = (L.u32(q.off), L.u32(q.adj), L.u32(order), L.u32(np.asarray(counts)[:n]),
+                    np.ascontiguousarray(np.asarray(degsum)[:n], dtype=np.uint64))
+            self.keep.append(bufs)
+            s = self.arr[k]
+            s.n = n
+            s.qoff, s.qadj, s.order, s.cand_count, s.cand_degsum = [b.ctypes.data for b in bufs]
+
+
+class CostModel:
+    """3-layer message-passing cost model, hidden 64 (see include/sgm_c.h)."""
+
+    def __init__(self, seed: int = 0, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        h = C.c_void_p()
+        _check(L.lib().sgm_costmodel_create(self.ctx.handle, int(seed), C.byref(h)), self.ctx)
+        self.handle = h
+        self.n_params = int(L.lib().sgm_costmodel_param_count())
+
+    def close(self):
+        if self.handle:
+            L.lib().sgm_costmodel_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.n_params, dtype=np.float32)
+        _check(L.lib().sgm_costmodel_params(self.handle, L.ptr(p), None), self.ctx)
+        return p
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.n_params
+        _check(L.lib().sgm_costmodel_params(self.handle, None, L.ptr(p)), self.ctx)
+
+    @staticmethod
+    def features(item):
+        """(feats[32, 32], back[32], fwd[32]) of one (query, order, counts, degsum) item."""
+        s = _Samples([item])
+        f = np.zeros((32, 32), dtype=np.float32)
+        b = np.zeros(32, dtype=np.uint32)
+        w = np.zeros(32, dtype=np.uint32)
+        _check(L.lib().sgm_costmodel_features(C.byref(s.arr[0]), L.ptr(f), L.ptr(b), L.ptr(w)))
+        return f, b, w
+
+    def predict(self, items) -> np.ndarray:
+        out = np.zeros(len(items), dtype=np.float32)
+        if items:
+            s = _Samples(items)
+            _check(L.lib().sgm_costmodel_predict(self.handle, s.arr, len(items), L.ptr(out)), self.ctx)
+        return out
+
+    def grad(self, items, targets):
+        s = _Samples(items)
+        t = np.ascontiguousarray(targets, dtype=np.float32)
+        g = np.zeros(self.n_params, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        _check(L.lib().sgm_costmodel_grad(self.handle, s.arr, L.ptr(t), len(items), L.ptr(g), L.ptr(loss)), self.ctx)
+        return g, float(loss[0])
+
+    def train(self, items, targets, epochs=30, minibatch=256, lr=2e-3, seed=0) -> np.ndarray:
+        s = _Samples(items)
+        t = np.ascontiguousarray(targets, dtype=np.float32)
+        losses = np.zeros(epochs, dtype=np.float32)
+        _check(L.lib().sgm_costmodel_train(self.handle, s.arr, L.ptr(t), len(items), epochs, minibatch,
+                                           C.c_float(lr), int(seed), L.ptr(losses)), self.ctx)
+        return losses
+
+    def timing(self) -> dict:
+        v = [C.c_double() for _ in range(4)]
+        _check(L.lib().sgm_costmodel_timing(self.handle, *[C.byref(x) for x in v]), self.ctx)
+        return {"train_ms": v[0].value, "train_flops": v[1].value, "infer_ms": v[2].value,
+                "infer_flops": v[3].value}
+
+
+def candidate_orders(q: Query, ri, gql, n_samples: int, seed: int) -> list:
+    """RI, GQL and n_samples seeded sampled orders (duplicates removed, first kept)."""
+    out, seen = [], set()
+    for o in [ri, gql] + [sample_order(q, (seed << 20) + k) for k in range(n_samples)]:
+        if o is None:
+            continue
+        t = tuple(o)
+        if t not in seen:
+            seen.add(t)
+            out.append(list(o))
+    return out
+
+
+def choose_orders(model: CostModel, queries, counts, degsums, cands, fallback, margin: float = 0.5):
+    """Per query, the candidate order with the lowest predicted log2 node count, unless it
+    beats the fallback order's prediction by less than ``margin`` (then the fallback).
+    Returns (orders, predicted, predicted_fallback)."""
+    items, owner = [], []
+    for i, q in enumerate(queries):
+        for o in cands[i]:
+            items.append((q, o, counts[i], degsums[i]))
+            owner.append(i)
+    pred = model.predict(items)
+    best = [None] * len(queries)
+    best_p = [math.inf] * len(queries)
+    fb_p = [math.inf] * len(queries)
+    for (q, o, _, _), i, p in zip(items, owner, pred):
+        if list(o) == list(fallback[i]):
+            fb_p[i] = float(p)
+        if p < best_p[i]:
+            best_p[i], best[i] = float(p), list(o)
+    orders = []
+    for i in range(len(queries)):
+        use = best[i] if best[i] is not None and best_p[i] < fb_p[i] - margin else list(fallback[i])
+        orders.append(use)
+    return orders, best_p, fb_p
