@@ -82,6 +82,8 @@ typedef struct {
 const char* sgm_version(void);
 int sgm_device_count(int* count);
 int sgm_ctx_create(int device, sgm_ctx** out);
+/* Also destroys the context's batches and cost models that are still alive (their handles
+ * become invalid).  Graphs and queries are destroyed by their own calls. */
 void sgm_ctx_destroy(sgm_ctx* ctx);
 const char* sgm_ctx_last_error(const sgm_ctx* ctx);
 /* Device time (HIP events) of the last filter call on this context: the NLF kernel, the
@@ -207,6 +209,13 @@ int sgm_costmodel_train(sgm_costmodel* m, const sgm_cm_sample* s, const float* t
 /* Accumulated device time (ms) and MFMA flops of training and prediction. */
 int sgm_costmodel_timing(const sgm_costmodel* m, double* train_ms, double* train_flops, double* infer_ms,
                          double* infer_flops);
+/* Learned orders for the loaded batch (RI fallback): per query, RI, the GQL order and
+ * n_samples seeded sampled orders are scored by the model on the GPU; the best-scored order
+ * is taken when its predicted log2 node count is below RI's by more than `margin`, else RI.
+ * orders[i*32 ..] receives query i's order; optional pred_best / pred_ri (predictions of
+ * the chosen and the RI order) and learned[i] (1 when the learned order was taken). */
+int sgm_batch_rank_orders(sgm_batch* b, sgm_costmodel* m, uint32_t n_samples, uint64_t seed, float margin,
+                          uint32_t* orders, float* pred_best, float* pred_ri, int32_t* learned);
 /* A seeded random connected order of a query (the sampled orders the model ranks): start
  * at a uniformly chosen vertex, then repeatedly a uniformly chosen unvisited vertex adjacent
  * to the visited ones. */

@@ -17,8 +17,10 @@ Everything runs on the GPU; a missing library or device raises instead of fallin
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import threading
+import weakref
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -49,9 +51,17 @@ class Context:
         self.handle = h
         self.device = device
         self._graphs = {}
+        self._children = weakref.WeakSet()  # batches and cost models on this context
+        _live_contexts.add(self)
+
+    def adopt(self, obj):
+        """Register an object whose library handle needs this context (closed first)."""
+        self._children.add(obj)
 
     def close(self):
         if self.handle:
+            for ch in list(self._children):
+                ch.close()
             for dg in list(self._graphs.values()):
                 dg.close()
             self._graphs.clear()
@@ -85,6 +95,18 @@ class Context:
 
 _default = None
 _default_lock = threading.Lock()
+_live_contexts = weakref.WeakSet()
+
+
+@atexit.register
+def _close_all_contexts():
+    """Release every device object in dependency order while the interpreter is intact
+    (finalizers at exit run in no fixed order)."""
+    for ctx in list(_live_contexts):
+        try:
+            ctx.close()
+        except Exception:
+            pass
 
 
 def default_context() -> Context:
@@ -363,6 +385,7 @@ class ExploreResult:
     elems_scanned: int = 0
     capped: bool = False
     retries: int = 0
+    stopped: bool = False  # ended early by a stop request: the count is partial
 
 
 def enumerate_query(can: CandidateParam, order, method=L.SGM_ENUM_CACHE_FILTER_FS, cap=None, blocks=0,
@@ -454,7 +477,8 @@ def _result(st, n):
     return ExploreResult(time=float(st.kernel_ms), ans=int(st.count), intersections=int(st.intersections),
                          nodes=[int(x) for x in st.nodes[:n]], fs_prunes=int(st.fs_prunes),
                          elems_read=int(st.elems_read), elems_out=int(st.elems_out),
-                         elems_scanned=int(st.elems_scanned), capped=bool(st.capped), retries=int(st.retries))
+                         elems_scanned=int(st.elems_scanned), capped=bool(st.capped), retries=int(st.retries),
+                         stopped=bool(st.stopped))
 
 
 class Batch:
@@ -472,6 +496,7 @@ class Batch:
         _check(L.lib().sgm_batch_create(self.dg.handle, self.slots, C.byref(h)), self.ctx)
         self.handle = h
         self.queries = []
+        self.ctx.adopt(self)
 
     def close(self):
         if self.handle:
@@ -512,12 +537,39 @@ class Batch:
         return arr, keep
 
     def enumerate(self, orders=None, method=L.SGM_ENUM_CACHE_FILTER_FS, cap=None, blocks=0, stack_words=0,
-                  flags=0, hub_min_degree=0):
+                  flags=0, hub_min_degree=0, time_limit_s=None):
+        """With ``time_limit_s``, a stop request ends the launch after that many seconds;
+        queries still running then come back with ``stopped`` set and partial counts."""
         arr, _keep = self._orders(orders)
         opts = _opts(method, cap, blocks, stack_words, flags, hub_min_degree)
         stats = (L.Stats * len(self.queries))()
-        _check(L.lib().sgm_batch_enumerate(self.handle, arr, C.byref(opts), stats), self.ctx)
+        timer = None
+        if time_limit_s:
+            import threading
+            timer = threading.Timer(time_limit_s, self.ctx.stop)
+            timer.start()
+        try:
+            code = L.lib().sgm_batch_enumerate(self.handle, arr, C.byref(opts), stats)
+        finally:
+            if timer:
+                timer.cancel()
+        if code != L.SGM_ESTOPPED or not time_limit_s:
+            _check(code, self.ctx)
         return [_result(st, q.getVertexCnt()) for st, q in zip(stats, self.queries)]
+
+    def rank_orders(self, model, n_samples=32, seed=0, margin=0.5):
+        """Learned orders of the loaded batch (sgm_batch_rank_orders): returns (orders,
+        predicted log2 nodes of the chosen order, of the RI order, learned-order flags)."""
+        nq = len(self.queries)
+        o = np.zeros((nq, 32), dtype=np.uint32)
+        pb = np.zeros(nq, dtype=np.float32)
+        pr = np.zeros(nq, dtype=np.float32)
+        fl = np.zeros(nq, dtype=np.int32)
+        _check(L.lib().sgm_batch_rank_orders(self.handle, model.handle, n_samples, int(seed) & (2**64 - 1),
+                                             C.c_float(margin), L.ptr(o), L.ptr(pb), L.ptr(pr), L.ptr(fl)),
+               self.ctx)
+        orders = [[int(x) for x in o[i, :q.getVertexCnt()]] for i, q in enumerate(self.queries)]
+        return orders, pb, pr, fl.astype(bool)
 
     def autotune(self, orders=None, method=L.SGM_ENUM_CACHE_FILTER_FS, cap=None):
         """Returns ({blocks, flags, hub_min_degree}, best_ms) of the fastest setting."""

@@ -121,6 +121,7 @@ void sgm_batch_destroy(sgm_batch* b)
 {
     if (!b) return;
     if (b->c) {
+        forget(b->c->batches, b);
         (void)hipSetDevice(b->c->device);
         (void)hipStreamSynchronize(b->c->stream);
     }
@@ -181,6 +182,7 @@ int sgm_batch_create(sgm_graph* g, uint32_t slots, sgm_batch** out)
     }
     b->counts.assign((size_t)slots * SGM_MAXQ, 0);
     b->degsum.assign((size_t)slots * SGM_MAXQ, 0);
+    c->batches.push_back(b);
     *out = b;
     return SGM_OK;
 }
@@ -395,6 +397,67 @@ int sgm_batch_enumerate(sgm_batch* b, const uint32_t* const* orders, const sgm_e
     if (c->stop_requested)
         for (uint32_t i = 0; i < nq; i++)
             if (stats[i].stopped) return set_err(c, SGM_ESTOPPED, "stopped by sgm_stop");
+    return SGM_OK;
+}
+
+int sgm_batch_rank_orders(sgm_batch* b, sgm_costmodel* m, uint32_t n_samples, uint64_t seed, float margin,
+                          uint32_t* orders, float* pred_best, float* pred_ri, int32_t* learned)
+{
+    if (!b || !m || !orders) return SGM_EINVAL;
+    sgm_ctx* c = b->c;
+    if (b->nq == 0) return set_err(c, SGM_ESTATE, "filter a batch first");
+    const uint32_t nq = b->nq;
+    // candidate orders per query: RI, GQL, then seeded samples; duplicates dropped
+    std::vector<std::vector<uint32_t>> cand;
+    std::vector<uint32_t> first(nq + 1, 0);
+    for (uint32_t i = 0; i < nq; i++) {
+        const sgm::QueryCSR& q = b->q[i];
+        std::vector<std::vector<uint32_t>> mine;
+        std::vector<uint32_t> o(q.n);
+        auto add = [&]() {
+            for (auto& x : mine)
+                if (x == o) return;
+            mine.push_back(o);
+        };
+        sgm::order_ri(q, o.data());
+        add();
+        sgm::order_gql(q, b->counts.data() + (size_t)i * SGM_MAXQ, b->g->V, o.data());
+        add();
+        for (uint32_t k = 0; k < n_samples; k++) {
+            const int rc = sgm_order_sample(q.off.data(), q.adj.data(), q.n, seed ^ ((uint64_t)i << 32) ^ (k * 0x9E3779B97F4A7C15ull),
+                                            o.data());
+            if (rc) return set_err(c, rc, "order sampling failed for query %u", i);
+            add();
+        }
+        first[i] = (uint32_t)cand.size();
+        for (auto& x : mine) cand.push_back(std::move(x));
+    }
+    first[nq] = (uint32_t)cand.size();
+    std::vector<sgm_cm_sample> smp(cand.size());
+    for (uint32_t i = 0; i < nq; i++)
+        for (uint32_t k = first[i]; k < first[i + 1]; k++) {
+            sgm_cm_sample& x = smp[k];
+            x.n = b->q[i].n;
+            x.qoff = b->q[i].off.data();
+            x.qadj = b->q[i].adj.data();
+            x.order = cand[k].data();
+            x.cand_count = b->counts.data() + (size_t)i * SGM_MAXQ;
+            x.cand_degsum = reinterpret_cast<const uint64_t*>(b->degsum.data()) + (size_t)i * SGM_MAXQ;
+        }
+    std::vector<float> pred(cand.size());
+    const int rc = sgm_costmodel_predict(m, smp.data(), (uint32_t)smp.size(), pred.data());
+    if (rc) return rc;
+    for (uint32_t i = 0; i < nq; i++) {
+        uint32_t best = first[i];  // RI is the first candidate of every query
+        for (uint32_t k = first[i]; k < first[i + 1]; k++)
+            if (pred[k] < pred[best]) best = k;
+        const bool use = pred[best] < pred[first[i]] - margin;
+        const uint32_t pick = use ? best : first[i];
+        std::copy(cand[pick].begin(), cand[pick].end(), orders + (size_t)i * SGM_MAXQ);
+        if (pred_best) pred_best[i] = pred[pick];
+        if (pred_ri) pred_ri[i] = pred[first[i]];
+        if (learned) learned[i] = use ? 1 : 0;
+    }
     return SGM_OK;
 }
 

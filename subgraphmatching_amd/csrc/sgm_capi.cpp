@@ -54,6 +54,8 @@ int sgm_ctx_create(int device, sgm_ctx** out)
 void sgm_ctx_destroy(sgm_ctx* c)
 {
     if (!c) return;
+    while (!c->batches.empty()) sgm_batch_destroy(c->batches.back());
+    while (!c->models.empty()) sgm_costmodel_destroy(c->models.back());
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     dfree(c->d_stop);

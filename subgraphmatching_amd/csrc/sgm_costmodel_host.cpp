@@ -220,6 +220,7 @@ uint32_t sgm_costmodel_param_count(void) { return NPARAM; }
 void sgm_costmodel_destroy(sgm_costmodel* m)
 {
     if (!m) return;
+    forget(m->c->models, m);
     (void)hipSetDevice(m->c->device);
     (void)hipStreamSynchronize(m->c->stream);
     dfree(m->p); dfree(m->g); dfree(m->m); dfree(m->v); dfree(m->loss);
@@ -273,6 +274,7 @@ int sgm_costmodel_create(sgm_ctx* c, uint64_t seed, sgm_costmodel** out)
     HIPCHK(c, hipMemset(m->v, 0, sizeof(float) * NPARAM));
     cast_weights(m);
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->models.push_back(m);
     *out = m;
     return SGM_OK;
 }

@@ -39,7 +39,17 @@ struct sgm_ctx {
     double filter_ms[3] = {0, 0, 0};
     int num_cus = 256;
     volatile int stop_requested = 0;
+    // objects that hold this context (destroyed with it if the caller has not done so)
+    std::vector<sgm_batch*> batches;
+    std::vector<sgm_costmodel*> models;
 };
+
+template <class T>
+inline void forget(std::vector<T*>& v, T* x)
+{
+    for (size_t i = 0; i < v.size(); i++)
+        if (v[i] == x) { v[i] = v.back(); v.pop_back(); return; }
+}
 
 struct sgm_graph {
     sgm_ctx* ctx = nullptr;

@@ -18,7 +18,8 @@ from . import _lib as L
 from .api import Context, _check, default_context
 from .graph import Query
 
-__all__ = ["CostModel", "sample_order", "candidate_orders", "choose_orders", "node_label"]
+__all__ = ["CostModel", "sample_order", "candidate_orders", "choose_orders", "node_label", "collect_training",
+           "train_model"]
 
 
 def sample_order(q: Query, seed: int) -> list:
@@ -30,8 +31,10 @@ def sample_order(q: Query, seed: int) -> list:
 
 
 def node_label(nodes) -> float:
-    """Training target of an enumeration: log2(1 + total search-tree nodes)."""
-    return math.log2(1.0 + float(sum(nodes)))
+    """Training target of an enumeration: log2(1 + internal search-tree nodes), i.e. the
+    nodes of every depth but the last.  Last-depth nodes are the embeddings themselves (at
+    most the cap, and the same for every order), so they carry no ranking signal."""
+    return math.log2(1.0 + float(sum(nodes[:-1])))
 
 
 class _Samples:
@@ -58,6 +61,7 @@ class CostModel:
         h = C.c_void_p()
         _check(L.lib().sgm_costmodel_create(self.ctx.handle, int(seed), C.byref(h)), self.ctx)
         self.handle = h
+        self.ctx.adopt(self)
         self.n_params = int(L.lib().sgm_costmodel_param_count())
 
     def close(self):
@@ -157,3 +161,47 @@ def choose_orders(model: CostModel, queries, counts, degsums, cands, fallback, m
         use = best[i] if best[i] is not None and best_p[i] < fb_p[i] - margin else list(fallback[i])
         orders.append(use)
     return orders, best_p, fb_p
+
+
+def collect_training(runner, queries, n_samples=8, cap=10_000_000, seed=0, time_limit_s=5.0, log=None):
+    """Training triples logged by the enumerator: for every query, its RI order, GQL order
+    and ``n_samples`` sampled orders are enumerated through the batched runner (cache +
+    filter + failing sets, the same search the orders are chosen for) and labelled with
+    log2(1 + search-tree nodes).  Launches cut off by the time limit leave their unfinished
+    queries out.  Returns (items, labels, info)."""
+    from .api import Order_RI
+    items, labels = [], []
+    dropped = enumerated = 0
+    B = runner.slots
+    for lo in range(0, len(queries), B):
+        part = queries[lo:lo + B]
+        ok, cnt, dsum = runner.filter(part)
+        sets = []
+        for i, q in enumerate(part):
+            cands = candidate_orders(q, Order_RI(q), runner.order_gql(i), n_samples, seed * 1000003 + lo + i)
+            while len(cands) < 2 + n_samples:  # tiny queries have few distinct orders
+                cands.append(cands[len(cands) % max(1, len(cands))])
+            sets.append(cands)
+        for k in range(2 + n_samples):
+            orders = [sets[i][k] for i in range(len(part))]
+            res = runner.enumerate(orders=orders, cap=cap, time_limit_s=time_limit_s)
+            for i, r in enumerate(res):
+                if not ok[i]:
+                    continue
+                enumerated += 1
+                if r.stopped:
+                    dropped += 1
+                    continue
+                n = part[i].getVertexCnt()
+                items.append((part[i], orders[i], cnt[i, :n].copy(), dsum[i, :n].copy()))
+                labels.append(node_label(r.nodes))
+        if log:
+            log(f"collected {len(items)} samples from {lo + len(part)} queries ({dropped} cut off)")
+    return items, np.asarray(labels, dtype=np.float32), {"enumerated": enumerated, "cut_off": dropped}
+
+
+def train_model(items, labels, seed=0, epochs=30, minibatch=256, lr=2e-3, ctx=None):
+    """A cost model trained on the triples; returns (model, per-epoch losses)."""
+    m = CostModel(seed=seed, ctx=ctx)
+    losses = m.train(items, labels, epochs=epochs, minibatch=minibatch, lr=lr, seed=seed + 1)
+    return m, losses
