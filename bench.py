@@ -11,6 +11,12 @@ Workloads (one step = one pass of the path over the whole query set):
     path        NLF + GQL filtering of all 200 queries in one set of launches, RI order
                 (host), cache + filter + failing-set enumeration of all 200 queries in one
                 persistent launch (per-query caps and statistics)
+  rmat20-learned
+    the rmat20 graph and query mix with learned orders: a cost model is trained (untimed) on
+    node counts the enumerator logs for 5,000 training queries (seed 44) x 10 orders, then
+    each step filters, ranks RI + GQL + 32 sampled orders per query on the GPU, keeps RI
+    unless the model predicts a clear gain, and enumerates; the same steps with RI orders
+    are timed after it for the learned-vs-RI speedup
   rmat20
     data graph  R-MAT scale 20 (a,b,c,d = .57,.19,.19,.05), 2*10^7 undirected edges, seed 3
     queries     1,000 random-walk queries, sizes uniform in 4..32, even ids dense / odd
@@ -45,7 +51,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["chunglu100k", "rmat20"], default="chunglu100k")
+    ap.add_argument("--workload", choices=["chunglu100k", "rmat20", "rmat20-learned"], default="chunglu100k")
+    ap.add_argument("--train-queries", type=int, default=5000,
+                    help="rmat20-learned: training queries (10 logged orders each: RI, GQL, 8 sampled)")
+    ap.add_argument("--rank-samples", type=int, default=32,
+                    help="rmat20-learned: sampled orders ranked per query besides RI and GQL")
     ap.add_argument("--queries", type=int, default=0, help="number of queries (0 = the workload's own)")
     ap.add_argument("--cap", type=int, default=10_000_000)
     ap.add_argument("--per-query", action="store_true",
@@ -207,6 +217,96 @@ def setup_rmat(args):
     return g, queries, step, config, {"single_launch": single_launch}
 
 
+# ----------------------------------------------------------------------- rmat20-learned
+def setup_rmat_learned(args):
+    import numpy as np
+    import subgraphmatching_amd as sgm
+    from subgraphmatching_amd.generators import query_set, rmat
+    from subgraphmatching_amd.learned import collect_training, train_model
+
+    g = rmat(20, 20_000_000, seed=3)
+    nq = args.queries or 1000
+    mix = lambda count, seed: query_set(g, count, lambda i, r: int(r.integers(4, 33)),  # noqa: E731
+                                        lambda i, r: i % 2 == 0, seed=seed)
+    queries = mix(nq, 33)
+    train_q = mix(args.train_queries, 44)
+    B = args.batch
+    runner = sgm.Batch(g, slots=B)
+    chunks = [queries[i:i + B] for i in range(0, nq, B)]
+    runner.filter(chunks[0])
+    tuned, _ = runner.autotune(cap=args.cap)
+    t0 = time.perf_counter()
+    items, labels, info = collect_training(runner, train_q, n_samples=8, cap=args.cap, seed=1, time_limit_s=5.0)
+    collect_s = time.perf_counter() - t0
+    model, losses = train_model(items, labels, seed=7, epochs=30, minibatch=256, lr=2e-3)
+    tm = model.timing()
+    print(f"[bench] cost model: {len(items)} triples ({info['cut_off']} cut off) logged in {collect_s:.1f}s, "
+          f"loss {losses[0]:.3f} -> {losses[-1]:.3f}, training {tm['train_ms']:.0f} ms", flush=True)
+    stats = {"learned_used": 0, "pred": [], "meas": [], "nodes": 0, "rank_s": 0.0}
+
+    def run(learned):
+        rec = new_rec()
+        t_start = time.perf_counter()
+        st = {"learned_used": 0, "pred": [], "meas": [], "nodes": 0, "rank_s": 0.0}
+        for part in chunks:
+            ok, _, _ = runner.filter(part)
+            f_ms, _ = runner.last_ms()
+            rec["filter_ms"] += f_ms
+            rec["gql_ms"] += f_ms
+            orders = None
+            if learned:
+                t1 = time.perf_counter()
+                orders, pb, _, fl = runner.rank_orders(model, n_samples=args.rank_samples, seed=5, margin=0.5)
+                st["rank_s"] += time.perf_counter() - t1
+                st["learned_used"] += int(fl.sum())
+            res = runner.enumerate(orders=orders, cap=args.cap, **tuned)
+            rec["enum_ms"] += runner.last_ms()[1]
+            rec["enum_launches"] += 1
+            for i, (good, r) in enumerate(zip(ok, res)):
+                add_result(rec, r if good else None)
+                if good:
+                    st["nodes"] += sum(r.nodes[:-1])
+                    if learned:
+                        st["pred"].append(float(pb[i]))
+                        st["meas"].append(float(np.log2(1.0 + sum(r.nodes[:-1]))))
+        rec["wall_s"] = time.perf_counter() - t_start
+        rec["filter_wall_s"] = rec["filter_ms"] / 1e3
+        return rec, st
+
+    def step():
+        rec, st = run(True)
+        stats.update(st)
+        return rec
+
+    def compare(K, learned_counts, learned_s):
+        ri = [run(False) for _ in range(K)]
+        ri_s = sum(r["wall_s"] for r, _ in ri)
+        err = np.abs(np.array(stats["pred"]) - np.array(stats["meas"]))
+        flops, ms = tm["train_flops"], tm["train_ms"]
+        t2 = model.timing()
+        return {"ri_queries_per_s": nq * K / ri_s, "learned_queries_per_s": nq * K / learned_s,
+                "learned_vs_ri_speedup": ri_s / learned_s,
+                "counts_identical": all(r["counts"] == learned_counts for r, _ in ri),
+                "internal_nodes_per_step": {"ri": ri[0][1]["nodes"], "learned": stats["nodes"]},
+                "learned_orders_used_per_step": stats["learned_used"],
+                "ranking_host_s_per_step": stats["rank_s"],
+                "prediction_abs_error_log2": {"mean": float(err.mean()), "median": float(np.median(err))},
+                "costmodel": {"training_triples": len(items), "cut_off": info["cut_off"], "epochs": 30,
+                              "loss_first_epoch": float(losses[0]), "loss_last_epoch": float(losses[-1]),
+                              "train_ms": ms, "train_tflops": flops / ms / 1e9 if ms else 0.0,
+                              "train_mfma_util_vs_bf16_dense_2.5pf": flops / ms / 1e9 / 2500.0 if ms else 0.0,
+                              "infer_ms": t2["infer_ms"], "infer_flops": t2["infer_flops"],
+                              "layers": 3, "hidden": 64, "dtype": "bf16 operands, fp32 accumulation"}}
+
+    config = {"workload": "rmat_s20_e2e7_abcd.57.19.19.05_seed3__1000x4-32v_mixed_rw_seed33__gql_learned_order_cache_filter_fs",
+              "queries_per_step": nq, "cap_per_query": args.cap, "order": "learned (RI fallback, margin 0.5 log2)",
+              "filter": "NLF+GQL", "enumeration": "cache+filter+failing-sets",
+              "launch": f"hipGraph-replayed batches of {B}", "autotuned": tuned,
+              "training": f"{args.train_queries} queries (seed 44) x 10 orders, labels log2(1+internal nodes)",
+              "ranked_orders_per_query": f"RI + GQL + {args.rank_samples} sampled"}
+    return g, queries, step, config, {"compare": compare}
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -218,7 +318,7 @@ def main():
         sys.exit(2)
 
     t_setup = time.perf_counter()
-    setup = setup_rmat if args.workload == "rmat20" else setup_chunglu
+    setup = {"rmat20": setup_rmat, "rmat20-learned": setup_rmat_learned}.get(args.workload, setup_chunglu)
     g, queries, step, config, extra = setup(args)
     print(f"[bench] {args.workload}: graph V={g.getVertexCnt()} E={g.getEdgeCnt()} maxdeg={g.getMaxDgree()}; "
           f"{len(queries)} queries; setup {time.perf_counter() - t_setup:.1f}s", flush=True)
@@ -260,6 +360,9 @@ def main():
         measured = traffic / (enum_ms / launches / 1e3) / 1e9
 
     extras = {}
+    if "compare" in extra:
+        extras["learned_vs_ri"] = extra["compare"](K, counts0, total_s)
+        print(f"[bench] learned vs RI: {extras['learned_vs_ri']}", flush=True)
     if "single_launch" in extra:
         sl, sl_counts = extra["single_launch"]()
         extras["single_launch"] = dict(sl, counts_match=sl_counts == counts0)

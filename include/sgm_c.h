@@ -112,6 +112,10 @@ void sgm_query_destroy(sgm_query* q);
 int sgm_filter(sgm_query* q, int kind);
 /* Same for many queries of one graph in one set of launches; status[i] gets each result. */
 int sgm_filter_batch(sgm_query* const* qs, uint32_t nq, int kind, int32_t* status);
+/* Work of the last GQL filter of q (no reference counterpart): work[0] checks decided by
+ * the degree table, work[1] by a lane-level matcher, work[2] by the wave matcher, work[3]
+ * removals. */
+int sgm_gql_work(const sgm_query* q, uint64_t* work);
 /* candidates_count[] (inc/api.h:12) */
 int sgm_cand_counts(const sgm_query* q, uint32_t* counts);
 /* candidates[u] in ascending vertex id (inc/api.h:11); out_cap >= counts[u] */

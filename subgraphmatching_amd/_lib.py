@@ -48,7 +48,7 @@ EXPORTS = [
     "sgm_batch_last_ms", "sgm_batch_autotune",
     "sgm_costmodel_create", "sgm_costmodel_destroy", "sgm_costmodel_param_count", "sgm_costmodel_params",
     "sgm_costmodel_features", "sgm_costmodel_predict", "sgm_costmodel_grad", "sgm_costmodel_train",
-    "sgm_costmodel_timing", "sgm_order_sample", "sgm_batch_rank_orders",
+    "sgm_costmodel_timing", "sgm_order_sample", "sgm_batch_rank_orders", "sgm_gql_work",
 ]
 
 
@@ -139,6 +139,7 @@ def lib() -> C.CDLL:
                                            C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "sgm_order_sample": (C.c_int, [vp, vp, C.c_uint32, C.c_uint64, vp]),
         "sgm_batch_rank_orders": (C.c_int, [vp, vp, C.c_uint32, C.c_uint64, C.c_float, vp, vp, vp, vp]),
+        "sgm_gql_work": (C.c_int, [vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -405,6 +405,15 @@ int sgm_filter_batch(sgm_query* const* qs, uint32_t nq, int kind, int32_t* statu
     return nq == 1 ? rc : SGM_OK;
 }
 
+int sgm_gql_work(const sgm_query* q, uint64_t* work)
+{
+    if (!q || !work) return SGM_EINVAL;
+    uint32_t w[4];
+    HIPCHK(q->g->ctx, hipMemcpy(w, q->d_removed + SGM_GQL_WORK, sizeof w, hipMemcpyDeviceToHost));
+    for (int i = 0; i < 4; i++) work[i] = w[i];
+    return SGM_OK;
+}
+
 int sgm_filter(sgm_query* q, int kind)
 {
     if (!q) return SGM_EINVAL;

@@ -33,6 +33,7 @@ uint16_t host_f2bf(float f)
 //   0 deg_q(u) / 8            1 log2(1 + |C(u)|) / 16     2 log2(1 + mean candidate degree) / 16
 //   3 i / 32                  4 backward neighbours / deg_q(u)
 //   5 backward neighbours / 8 6 [i == 0]                  7 n / 32
+// f: n rows of NFEAT values (row-major, stride NFEAT).
 int features(const sgm_cm_sample& s, float* f, uint8_t* valid, uint32_t* back, uint32_t* fwd)
 {
     const uint32_t n = s.n;
@@ -44,7 +45,6 @@ int features(const sgm_cm_sample& s, float* f, uint8_t* valid, uint32_t* back, u
         seen |= 1u << u;
         pos[u] = i;
     }
-    std::memset(f, 0, sizeof(float) * CM_ROWS * CM_F);
     for (uint32_t i = 0; i < CM_ROWS; i++) { valid[i] = i < n; back[i] = 0; fwd[i] = 0; }
     for (uint32_t i = 0; i < n; i++) {
         const uint32_t u = s.order[i];
@@ -57,7 +57,7 @@ int features(const sgm_cm_sample& s, float* f, uint8_t* valid, uint32_t* back, u
         const double cnt = (double)s.cand_count[u];
         const double mean = cnt > 0 ? (double)s.cand_degsum[u] / cnt : 0.0;
         const float nb = (float)__builtin_popcount(back[i]);
-        float* r = f + (size_t)i * CM_F;
+        float* r = f + (size_t)i * NFEAT;
         r[0] = deg / 8.f;
         r[1] = (float)(std::log2(1.0 + cnt) / 16.0);
         r[2] = (float)(std::log2(1.0 + mean) / 16.0);
@@ -146,15 +146,17 @@ int upload(sgm_costmodel* m, Bufs& b, const sgm_cm_sample* s, uint32_t count, co
 {
     sgm_ctx* c = m->c;
     const size_t R = (size_t)count * CM_ROWS;
-    std::vector<uint16_t> F(R * CM_F);
+    std::vector<uint16_t> F(R * CM_F, 0);  // columns NFEAT.. and padding rows stay zero
     std::vector<uint8_t> valid(R);
     std::vector<uint32_t> back(R), fwd(R);
-    std::vector<float> f(CM_ROWS * CM_F);
+    float f[CM_ROWS * NFEAT];
     for (uint32_t k = 0; k < count; k++) {
-        const int rc = features(s[k], f.data(), valid.data() + (size_t)k * CM_ROWS, back.data() + (size_t)k * CM_ROWS,
+        const int rc = features(s[k], f, valid.data() + (size_t)k * CM_ROWS, back.data() + (size_t)k * CM_ROWS,
                                 fwd.data() + (size_t)k * CM_ROWS);
         if (rc) return set_err(c, rc, "cost model: bad sample %u", k);
-        for (size_t e = 0; e < f.size(); e++) F[(size_t)k * CM_ROWS * CM_F + e] = host_f2bf(f[e]);
+        uint16_t* dst = F.data() + (size_t)k * CM_ROWS * CM_F;
+        for (uint32_t i = 0; i < s[k].n; i++)
+            for (uint32_t j = 0; j < NFEAT; j++) dst[(size_t)i * CM_F + j] = host_f2bf(f[i * NFEAT + j]);
     }
     HIPCHK(c, hipMemcpyAsync(b.F, F.data(), F.size() * 2, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(b.valid, valid.data(), R, hipMemcpyHostToDevice, c->stream));
@@ -297,7 +299,13 @@ int sgm_costmodel_features(const sgm_cm_sample* s, float* feats, uint32_t* back,
 {
     if (!s || !feats || !back || !fwd) return SGM_EINVAL;
     uint8_t valid[CM_ROWS];
-    return features(*s, feats, valid, back, fwd);
+    float f[CM_ROWS * NFEAT];
+    const int rc = features(*s, f, valid, back, fwd);
+    if (rc) return rc;
+    std::memset(feats, 0, sizeof(float) * CM_ROWS * CM_F);
+    for (uint32_t i = 0; i < s->n; i++)
+        for (uint32_t j = 0; j < NFEAT; j++) feats[i * CM_F + j] = f[i * NFEAT + j];
+    return SGM_OK;
 }
 
 int sgm_costmodel_predict(sgm_costmodel* m, const sgm_cm_sample* s, uint32_t count, float* out)

@@ -76,9 +76,12 @@ struct sgm_equery {
 //   flags[0 .. 2*SGM_MAXQ)   removals per GQL phase (round*n + u)
 //   flags[SGM_GQL_TOUCH_ANY] bit u: some touch[] word has bit u (round-1 checks of u need it)
 //   flags[SGM_GQL_DIRTY_ANY] bit u: some dirty[] word has bit u (round 2 must revisit u)
+//   flags[SGM_GQL_WORK ..+3]  checks decided by the degree table, by a lane matcher, by the
+//                             wave matcher, and vertices whose removal marked neighbours
 #define SGM_GQL_TOUCH_ANY (2 * SGM_MAXQ)
 #define SGM_GQL_DIRTY_ANY (2 * SGM_MAXQ + 1)
-#define SGM_GQL_FLAG_WORDS (2 * SGM_MAXQ + 2)
+#define SGM_GQL_WORK (2 * SGM_MAXQ + 2)
+#define SGM_GQL_FLAG_WORDS (2 * SGM_MAXQ + 6)
 struct sgm_qdesc {
     uint32_t n;
     uint32_t qdeg[SGM_MAXQ];

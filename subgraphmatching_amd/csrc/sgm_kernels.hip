@@ -203,18 +203,20 @@ __global__ void __launch_bounds__(256) k_gql_phase(const uint32_t* __restrict__ 
     const uint32_t waves = blockDim.x >> 6;
     const uint32_t w0 = blockIdx.x * waves + (threadIdx.x >> 6);
     const uint32_t stride = gridDim.x * waves;
-    uint32_t local_removed = 0;
+    uint32_t local_removed = 0, n_table = 0, n_lane = 0, n_wave = 0;
     const uint32_t nunits = (V + 63) / 64;
     for (uint32_t unit = w0; unit < nunits; unit += stride) {
         const uint32_t v = unit * 64 + lane;
         bool cand = v < V && degc[v] >= qd;  // NLF (degrees are capped at 255 >= 31 >= qd)
         uint32_t o = 0, dv = 0;
         int verdict = -1;
+        bool by_table = false;
         if (r == 0) {
             if (cand && (!use_touch || !((touch[v] >> u) & 1u))) {
                 bool ok = true;
                 for (uint32_t j = 0; j < k && ok; j++) ok = nbr_deg[(size_t)j * V + v] >= req[j];
                 verdict = ok ? 1 : 0;
+                by_table = true;
             } else if (cand) {
                 o = off[v];
                 dv = off[v + 1] - o;
@@ -235,6 +237,8 @@ __global__ void __launch_bounds__(256) k_gql_phase(const uint32_t* __restrict__ 
                 for (uint32_t p = o; p < o + dv; p++) gql_mark(touch, dirty, adj[p], tm, dm);
         }
         local_removed += (uint32_t)__popcll(__ballot(cand && verdict == 0));
+        n_table += (uint32_t)__popcll(__ballot(by_table));
+        n_lane += (uint32_t)__popcll(__ballot(cand && !by_table));
         // the wave: exact matcher for undecided lanes, shared marking for long lists
         uint64_t pend = __ballot(cand && (verdict < 0 || (verdict == 0 && dv > GQL_LANE_DEG)));
         while (pend) {
@@ -243,6 +247,7 @@ __global__ void __launch_bounds__(256) k_gql_phase(const uint32_t* __restrict__ 
             const uint32_t vt = rdlane(v, t);
             const uint32_t ot = off[vt], dt = off[vt + 1] - ot;
             bool fail = rdlane((uint32_t)verdict, t) == 0u;
+            n_wave += fail ? 0u : 1u;
             if (!fail && !lsm_wave(adj + ot, dt, vm, L, k)) {
                 if (lane == 0) atomicAnd(&vm[vt], ~(1u << u));
                 local_removed++;
@@ -252,10 +257,16 @@ __global__ void __launch_bounds__(256) k_gql_phase(const uint32_t* __restrict__ 
                 for (uint32_t p = ot + lane; p < ot + dt; p += 64) gql_mark(touch, dirty, adj[p], tm, dm);
         }
     }
-    if (lane == 0 && local_removed) {
-        atomicAdd(&flags[phase], local_removed);
-        if (tm) atomicOr(&flags[SGM_GQL_TOUCH_ANY], tm);
-        if (dm) atomicOr(&flags[SGM_GQL_DIRTY_ANY], dm);
+    if (lane == 0) {
+        if (local_removed) {
+            atomicAdd(&flags[phase], local_removed);
+            atomicAdd(&flags[SGM_GQL_WORK + 3], local_removed);
+            if (tm) atomicOr(&flags[SGM_GQL_TOUCH_ANY], tm);
+            if (dm) atomicOr(&flags[SGM_GQL_DIRTY_ANY], dm);
+        }
+        if (n_table) atomicAdd(&flags[SGM_GQL_WORK], n_table);
+        if (n_lane) atomicAdd(&flags[SGM_GQL_WORK + 1], n_lane);
+        if (n_wave) atomicAdd(&flags[SGM_GQL_WORK + 2], n_wave);
     }
 }
 
