@@ -21,7 +21,8 @@ Workloads (one step = one pass of the path over the whole query set):
     data graph  R-MAT scale 20 (a,b,c,d = .57,.19,.19,.05), 2*10^7 undirected edges, seed 3
     queries     1,000 random-walk queries, sizes uniform in 4..32, even ids dense / odd
                 sparse, seed 33
-    path        batches of 64 through the batched runner: hipGraph-replayed NLF + GQL
+    path        batches of 64 queries of similar size (grouped by vertex count) through the
+                batched runner: hipGraph-replayed NLF + GQL
                 filter, RI orders, hipGraph-replayed persistent enumeration with the
                 autotuned grid size and intersection strategy (tuned once on the first
                 batch, untimed); a single-launch pass over all 1,000 queries is also timed
@@ -164,7 +165,9 @@ def setup_rmat(args):
     queries = query_set(g, nq, lambda i, r: int(r.integers(4, 33)), lambda i, r: i % 2 == 0, seed=33)
     B = args.batch
     runner = sgm.Batch(g, slots=B)
-    chunks = [queries[i:i + B] for i in range(0, nq, B)]
+    # batches group queries of similar size (a batch runs 2 x its largest query's GQL phases)
+    by_size = sorted(range(nq), key=lambda i: queries[i].getVertexCnt())
+    chunks = [[queries[i] for i in by_size[lo:lo + B]] for lo in range(0, nq, B)]
     # autotune once on the first batch (untimed)
     runner.filter(chunks[0])
     tuned, tuned_ms = runner.autotune(cap=args.cap)
@@ -175,6 +178,8 @@ def setup_rmat(args):
     def step():
         rec = new_rec()
         t0 = time.perf_counter()
+        counts = [0] * nq
+        pos = 0
         for part in chunks:
             ok, _, _ = runner.filter(part)
             f_ms, _ = runner.last_ms()
@@ -186,6 +191,9 @@ def setup_rmat(args):
             rec["enum_launches"] += 1
             for good, r in zip(ok, res):
                 add_result(rec, r if good else None)
+                counts[by_size[pos]] = rec["counts"][-1]
+                pos += 1
+        rec["counts"] = counts  # in query order
         rec["wall_s"] = time.perf_counter() - t0
         rec["filter_wall_s"] = rec["filter_ms"] / 1e3
         return rec
@@ -232,7 +240,8 @@ def setup_rmat_learned(args):
     train_q = mix(args.train_queries, 44)
     B = args.batch
     runner = sgm.Batch(g, slots=B)
-    chunks = [queries[i:i + B] for i in range(0, nq, B)]
+    by_size = sorted(range(nq), key=lambda i: queries[i].getVertexCnt())  # batches of similar size
+    chunks = [[queries[i] for i in by_size[lo:lo + B]] for lo in range(0, nq, B)]
     runner.filter(chunks[0])
     tuned, _ = runner.autotune(cap=args.cap)
     t0 = time.perf_counter()

@@ -27,7 +27,7 @@ struct sgm_batch {
     uint32_t slots = 0;
     uint32_t* d_vmask = nullptr;             // slots * V
     uint32_t* d_flags = nullptr;             // slots * SGM_GQL_FLAG_WORDS
-    uint32_t* d_scratch = nullptr;           // slots * 2V: GQL touch / dirty marks
+    uint32_t* d_scratch = nullptr;           // slots * 3V: GQL rank-space membership, touch / dirty marks
     sgm_qdesc* d_qd = nullptr;               // slots
     uint32_t* d_cnt = nullptr;               // slots * 32
     unsigned long long* d_degsum = nullptr;  // slots * 32
@@ -47,7 +47,8 @@ struct sgm_batch {
     std::vector<unsigned long long> degsum;
     std::vector<int32_t> status;
     double filter_ms = 0, enum_ms = 0;
-    hipGraphExec_t gf[3] = {nullptr, nullptr, nullptr};
+    hipGraphExec_t gf_nlf = nullptr;
+    std::vector<std::pair<uint32_t, hipGraphExec_t>> gf_gql;  // by number of GQL phases
     std::vector<EnumGraph> ge;
 };
 
@@ -82,7 +83,7 @@ int load_query(sgm_ctx* c, const uint32_t* qoff, const uint32_t* qadj, uint32_t 
         std::vector<uint32_t> r;
         for (uint32_t p = qoff[u]; p < qoff[u + 1]; p++) r.push_back(qd.qdeg[qadj[p]]);
         std::sort(r.begin(), r.end(), [](uint32_t a, uint32_t b) { return a > b; });
-        for (size_t j = 0; j < r.size(); j++) qd.req[u][j] = (uint8_t)r[j];
+        for (size_t j = 0; j < r.size(); j++) reinterpret_cast<uint8_t*>(qd.req[u])[j] = (uint8_t)r[j];
     }
     return SGM_OK;
 }
@@ -125,8 +126,8 @@ void sgm_batch_destroy(sgm_batch* b)
         (void)hipSetDevice(b->c->device);
         (void)hipStreamSynchronize(b->c->stream);
     }
-    for (auto& x : b->gf)
-        if (x) (void)hipGraphExecDestroy(x);
+    if (b->gf_nlf) (void)hipGraphExecDestroy(b->gf_nlf);
+    for (auto& x : b->gf_gql) (void)hipGraphExecDestroy(x.second);
     for (auto& x : b->ge) (void)hipGraphExecDestroy(x.exec);
     for (auto& e : b->ev)
         if (e) (void)hipEventDestroy(e);
@@ -157,7 +158,7 @@ int sgm_batch_create(sgm_graph* g, uint32_t slots, sgm_batch** out)
     const size_t V = g->V;
     hipError_t e = dalloc(&b->d_vmask, slots * V);
     if (e == hipSuccess) e = dalloc(&b->d_flags, (size_t)slots * SGM_GQL_FLAG_WORDS);
-    if (e == hipSuccess) e = dalloc(&b->d_scratch, 2 * slots * V);
+    if (e == hipSuccess) e = dalloc(&b->d_scratch, 3 * slots * V);
     if (e == hipSuccess) e = dalloc(&b->d_qd, slots);
     if (e == hipSuccess) e = dalloc(&b->d_cnt, (size_t)slots * SGM_MAXQ);
     if (e == hipSuccess) e = dalloc(&b->d_degsum, (size_t)slots * SGM_MAXQ);
@@ -177,7 +178,8 @@ int sgm_batch_create(sgm_graph* g, uint32_t slots, sgm_batch** out)
         std::memset(&d, 0, sizeof d);
         d.vmask = b->d_vmask + i * V;
         d.removed = b->d_flags + (size_t)i * SGM_GQL_FLAG_WORDS;
-        d.touch = b->d_scratch + 2 * i * V;
+        d.vmr = b->d_scratch + 3 * i * V;
+        d.touch = d.vmr + V;
         d.dirty = d.touch + V;
     }
     b->counts.assign((size_t)slots * SGM_MAXQ, 0);
@@ -208,23 +210,43 @@ int sgm_batch_filter(sgm_batch* b, uint32_t nq, const uint32_t* const* qoffs, co
         }
     }
     const uint32_t slots = b->slots, V = g->V;
-    if (!b->gf[kind]) {
+    // GQL runs 2n phases for a query of n vertices: one graph per phase count, sized for the
+    // batch's largest query (phases past a smaller query's own return at once)
+    uint32_t max_n = 0;
+    for (uint32_t i = 0; i < nq; i++) max_n = std::max(max_n, b->q[i].n);
+    const uint32_t n_phases = kind == SGM_FILTER_GQL ? 2 * max_n : 0;
+    hipGraphExec_t* gexec = &b->gf_nlf;
+    if (kind == SGM_FILTER_GQL) {
+        gexec = nullptr;
+        for (auto& x : b->gf_gql)
+            if (x.first == n_phases) gexec = &x.second;
+        if (!gexec) {
+            b->gf_gql.emplace_back(n_phases, nullptr);
+            gexec = &b->gf_gql.back().second;
+        }
+    }
+    if (!*gexec) {
         const int rc = capture(c, [&] {
             sgm::launch_fill_u32(c->stream, b->d_flags, (size_t)slots * SGM_GQL_FLAG_WORDS, 0);
-            if (kind == SGM_FILTER_GQL) sgm::launch_fill_u32(c->stream, b->d_scratch, 2 * (size_t)slots * V, 0);
+            if (kind == SGM_FILTER_GQL) sgm::launch_fill_u32(c->stream, b->d_scratch, 3 * (size_t)slots * V, 0);
             sgm::launch_fill_u32(c->stream, b->d_cnt, (size_t)slots * SGM_MAXQ, 0);
             sgm::launch_fill_u32(c->stream, reinterpret_cast<uint32_t*>(b->d_degsum), 2 * (size_t)slots * SGM_MAXQ, 0);
-            sgm::launch_nlf(c->stream, g->d_off, V, b->d_qd, slots);
-            if (kind == SGM_FILTER_GQL && V)
-                for (uint32_t ph = 0; ph < 2 * SGM_MAXQ; ph++)
-                    sgm::launch_gql_phase(c->stream, g->d_off, g->d_adj, g->d_degc, g->d_nbrdeg, V, b->d_qd, slots, ph);
+            if (kind == SGM_FILTER_GQL && V) {
+                sgm::launch_nlf_rank(c->stream, g->d_gecnt, V, b->d_qd, slots);
+                for (uint32_t ph = 0; ph < n_phases; ph++)  // grid sized for the largest prefix
+                    sgm::launch_gql_phase(c->stream, g->d_roff, g->d_radj, g->d_dtab, V, g->d_gecnt, b->d_qd, slots,
+                                          ph, g->ge_cnt[1]);
+                sgm::launch_unrank(c->stream, g->d_rank, V, b->d_qd, slots);
+            } else {
+                sgm::launch_nlf(c->stream, g->d_off, V, b->d_qd, slots);
+            }
             sgm::launch_cand_stats(c->stream, g->d_off, V, b->d_qd, slots, b->d_cnt, b->d_degsum);
-        }, &b->gf[kind]);
+        }, gexec);
         if (rc) return rc;
     }
     HIPCHK(c, hipMemcpyAsync(b->d_qd, b->hqd.data(), sizeof(sgm_qdesc) * slots, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipEventRecord(b->ev[0], c->stream));
-    HIPCHK(c, hipGraphLaunch(b->gf[kind], c->stream));
+    HIPCHK(c, hipGraphLaunch(*gexec, c->stream));
     HIPCHK(c, hipEventRecord(b->ev[1], c->stream));
     HIPCHK(c, hipMemcpyAsync(b->counts.data(), b->d_cnt, sizeof(uint32_t) * SGM_MAXQ * nq, hipMemcpyDeviceToHost,
                              c->stream));

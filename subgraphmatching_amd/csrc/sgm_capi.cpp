@@ -147,16 +147,24 @@ int sgm_graph_upload(sgm_ctx* c, const uint32_t* off, const uint32_t* adj, uint3
     if (e == hipSuccess && adj_len) e = hipMemcpy(g->d_adj, adj, sizeof(uint32_t) * adj_len, hipMemcpyHostToDevice);
     if (e == hipSuccess && V) e = hipMemcpy(g->d_bydeg, bydeg.data(), sizeof(uint32_t) * V, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(g->d_gecnt, g->ge_cnt, sizeof(g->ge_cnt), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = dalloc(&g->d_nbrdeg, (size_t)32 * V);
-    if (e == hipSuccess) e = dalloc(&g->d_degc, V);
+    // degree-rank space for GQL: rank of each vertex, CSR of ranks, neighbour-degree table
+    if (e == hipSuccess) e = dalloc(&g->d_rank, V);
+    if (e == hipSuccess) e = dalloc(&g->d_roff, (size_t)V + 1);
+    if (e == hipSuccess) e = dalloc(&g->d_radj, adj_len);
+    if (e == hipSuccess) e = dalloc(&g->d_dtab, (size_t)4 * V);
     if (e == hipSuccess && V) {
-        std::vector<uint8_t> degc(V);
-        for (uint32_t v = 0; v < V; v++) degc[v] = (uint8_t)std::min<uint32_t>(off[v + 1] - off[v], 255);
-        e = hipMemcpy(g->d_degc, degc.data(), V, hipMemcpyHostToDevice);
-    }
-    if (e == hipSuccess && V) {
-        sgm::launch_nbr_deg(nullptr, g->d_off, g->d_adj, V, g->d_nbrdeg);
-        e = hipGetLastError();
+        std::vector<uint32_t> rank(V), roff((size_t)V + 1, 0);
+        for (uint32_t x = 0; x < V; x++) {
+            rank[bydeg[x]] = x;
+            roff[x + 1] = roff[x] + (off[bydeg[x] + 1] - off[bydeg[x]]);
+        }
+        e = hipMemcpy(g->d_rank, rank.data(), sizeof(uint32_t) * V, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(g->d_roff, roff.data(), sizeof(uint32_t) * ((size_t)V + 1), hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+            sgm::launch_rank_adj(nullptr, g->d_off, g->d_adj, g->d_bydeg, g->d_rank, g->d_roff, V, g->d_radj);
+            sgm::launch_nbr_deg(nullptr, g->d_roff, g->d_radj, V, g->d_dtab);
+            e = hipGetLastError();
+        }
         if (e == hipSuccess) e = hipDeviceSynchronize();
     }
     // hub bitmaps: the highest-degree vertices first (by_deg order), as many as fit
@@ -201,8 +209,10 @@ void sgm_graph_destroy(sgm_graph* g)
     dfree(g->d_gecnt);
     dfree(g->d_hub_slot);
     dfree(g->d_hub_bits);
-    dfree(g->d_nbrdeg);
-    dfree(g->d_degc);
+    dfree(g->d_rank);
+    dfree(g->d_roff);
+    dfree(g->d_radj);
+    dfree(g->d_dtab);
     for (auto& m : g->hub_maps) dfree(m.second);
     delete g;
 }
@@ -281,7 +291,7 @@ int sgm_query_create(sgm_graph* g, const uint32_t* qoff, const uint32_t* qadj, u
             std::vector<uint32_t> r;
             for (uint32_t p = qoff[u]; p < qoff[u + 1]; p++) r.push_back(qd.qdeg[qadj[p]]);
             std::sort(r.begin(), r.end(), [](uint32_t a, uint32_t b) { return a > b; });
-            for (size_t j = 0; j < r.size(); j++) qd.req[u][j] = (uint8_t)r[j];
+            for (size_t j = 0; j < r.size(); j++) reinterpret_cast<uint8_t*>(qd.req[u])[j] = (uint8_t)r[j];
         }
         qd.vmask = q->d_vmask;
         qd.removed = q->d_removed;
@@ -321,7 +331,8 @@ static int filter_chunk(sgm_ctx* c, sgm_graph* g, sgm_query* const* qs, uint32_t
         qd[i] = qs[i]->hq;
         cd[i] = qs[i]->hc;
         if (kind == SGM_FILTER_GQL) {
-            qd[i].touch = c->d_gqls + (size_t)2 * V * i;
+            qd[i].vmr = c->d_gqls + (size_t)3 * V * i;
+            qd[i].touch = qd[i].vmr + V;
             qd[i].dirty = qd[i].touch + V;
         }
         max_n = std::max(max_n, qs[i]->q.n);
@@ -332,7 +343,7 @@ static int filter_chunk(sgm_ctx* c, sgm_graph* g, sgm_query* const* qs, uint32_t
         HIPCHK(c, hipMemsetAsync(qs[i]->d_removed, 0, sizeof(uint32_t) * SGM_GQL_FLAG_WORDS, c->stream));
     }
     if (kind == SGM_FILTER_GQL && V)
-        HIPCHK(c, hipMemsetAsync(c->d_gqls, 0, sizeof(uint32_t) * 2 * (size_t)V * nq, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->d_gqls, 0, sizeof(uint32_t) * 3 * (size_t)V * nq, c->stream));
     sgm_qdesc* d_qd = nullptr;
     sgm_cdesc* d_cd = nullptr;
     HIPCHK(c, dalloc(&d_qd, nq));
@@ -340,11 +351,20 @@ static int filter_chunk(sgm_ctx* c, sgm_graph* g, sgm_query* const* qs, uint32_t
     HIPCHK(c, hipMemcpyAsync(d_qd, qd.data(), sizeof(sgm_qdesc) * nq, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(d_cd, cd.data(), sizeof(sgm_cdesc) * nq, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipEventRecord(c->fev[0], c->stream));
-    sgm::launch_nlf(c->stream, g->d_off, V, d_qd, nq);
+    if (kind == SGM_FILTER_GQL && V) sgm::launch_nlf_rank(c->stream, g->d_gecnt, V, d_qd, nq);
+    else sgm::launch_nlf(c->stream, g->d_off, V, d_qd, nq);
     HIPCHK(c, hipEventRecord(c->fev[1], c->stream));
-    if (kind == SGM_FILTER_GQL && V)
-        for (uint32_t ph = 0; ph < 2 * max_n; ph++)
-            sgm::launch_gql_phase(c->stream, g->d_off, g->d_adj, g->d_degc, g->d_nbrdeg, V, d_qd, nq, ph);
+    if (kind == SGM_FILTER_GQL && V) {
+        for (uint32_t ph = 0; ph < 2 * max_n; ph++) {
+            uint32_t mp = 0;  // the largest candidate prefix of this phase
+            for (uint32_t i = 0; i < nq; i++) {
+                const uint32_t n = qs[i]->q.n;
+                if (ph < 2 * n) mp = std::max(mp, g->ge_cnt[std::min<uint32_t>(qd[i].qdeg[ph % n], 32)]);
+            }
+            if (mp) sgm::launch_gql_phase(c->stream, g->d_roff, g->d_radj, g->d_dtab, V, g->d_gecnt, d_qd, nq, ph, mp);
+        }
+        sgm::launch_unrank(c->stream, g->d_rank, V, d_qd, nq);
+    }
     HIPCHK(c, hipEventRecord(c->fev[2], c->stream));
     sgm::launch_bitmaps(c->stream, d_cd, nq, V);
     sgm::launch_rank_scan(c->stream, d_cd, nq, V, max_n);
@@ -374,11 +394,11 @@ int sgm_filter_batch(sgm_query* const* qs, uint32_t nq, int kind, int32_t* statu
     for (uint32_t i = 0; i < nq; i++)
         if (!qs[i] || qs[i]->g != g) return set_err(c, SGM_EINVAL, "queries of a batch must share one graph");
     HIPCHK(c, hipSetDevice(c->device));
-    const uint64_t per_q = (uint64_t)2 * g->V * sizeof(uint32_t);
+    const uint64_t per_q = (uint64_t)3 * g->V * sizeof(uint32_t);
     uint32_t chunk = nq;
     if (kind == SGM_FILTER_GQL && per_q) {
         chunk = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(nq, kGqlScratchBytes / per_q));
-        const size_t words = (size_t)2 * g->V * chunk;
+        const size_t words = (size_t)3 * g->V * chunk;
         if (c->gqls_words < words) {
             dfree(c->d_gqls);
             c->d_gqls = nullptr;

@@ -82,13 +82,17 @@ struct sgm_equery {
 #define SGM_GQL_DIRTY_ANY (2 * SGM_MAXQ + 1)
 #define SGM_GQL_WORK (2 * SGM_MAXQ + 2)
 #define SGM_GQL_FLAG_WORDS (2 * SGM_MAXQ + 6)
+// GQL runs in degree-rank space (rank r = position in the degree-sorted vertex list), where
+// the candidates of every NLF threshold are a prefix; `vmr` holds the rank-space membership
+// words and is copied back to vmask (vertex ids) after the last phase.
 struct sgm_qdesc {
+    uint64_t req[SGM_MAXQ][4];       // byte j of req[u]: (j+1)-th largest degree in N_q(u), 0 past deg_q(u)
     uint32_t n;
     uint32_t qdeg[SGM_MAXQ];
     uint32_t nbr[SGM_MAXQ];          // neighbour bitmask of each query vertex
-    uint8_t req[SGM_MAXQ][SGM_MAXQ]; // req[u][j]: (j+1)-th largest degree among N_q(u)
-    uint32_t* vmask;                 // V words
+    uint32_t* vmask;                 // V words, by vertex id
     uint32_t* removed;               // SGM_GQL_FLAG_WORDS words (see above)
-    uint32_t* touch;                 // V words of GQL scratch: bit u = a neighbour lost a bit of N_q(u)
-    uint32_t* dirty;                 // V words of GQL scratch: bit u = recheck (round 2, u)
+    uint32_t* vmr;                   // V words of GQL scratch: membership by degree rank
+    uint32_t* touch;                 // V words of GQL scratch (by rank): a neighbour lost a bit of N_q(u)
+    uint32_t* dirty;                 // V words of GQL scratch (by rank): recheck (round 2, u)
 };

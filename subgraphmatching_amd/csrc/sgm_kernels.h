@@ -20,14 +20,20 @@ struct sgm_cdesc {
 
 namespace sgm {
 void launch_nlf(hipStream_t s, const uint32_t* off, uint32_t V, const sgm_qdesc* qs, uint32_t nq);
-void launch_gql_phase(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint8_t* degc,
-                      const uint8_t* nbr_deg, uint32_t V, const sgm_qdesc* qs, uint32_t nq, uint32_t phase);
+// GQL in degree-rank space (see k_gql_phase); max_prefix = the largest candidate prefix of
+// the batch in this phase (sizes the grid)
+void launch_gql_phase(hipStream_t s, const uint32_t* roff, const uint32_t* radj, const uint64_t* dtab, uint32_t V,
+                      const uint32_t* ge_cnt, const sgm_qdesc* qs, uint32_t nq, uint32_t phase, uint32_t max_prefix);
+void launch_nlf_rank(hipStream_t s, const uint32_t* ge_cnt, uint32_t V, const sgm_qdesc* qs, uint32_t nq);
+void launch_unrank(hipStream_t s, const uint32_t* rank, uint32_t V, const sgm_qdesc* qs, uint32_t nq);
+void launch_rank_adj(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* by_deg,
+                     const uint32_t* rank, const uint32_t* roff, uint32_t V, uint32_t* radj);
+// degree table for the GQL shortcut, by rank: 4 planes of V words, 32 bytes per vertex
+void launch_nbr_deg(hipStream_t s, const uint32_t* roff, const uint32_t* radj, uint32_t V, uint64_t* dtab);
 // batch candidate statistics: cnt[q*32+u] = |C(u)|, degsum[q*32+u] = sum of degrees (zeroed first)
 void launch_cand_stats(hipStream_t s, const uint32_t* off, uint32_t V, const sgm_qdesc* qs, uint32_t nq,
                        uint32_t* cnt, unsigned long long* degsum);
 void launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t val);
-// degree table for the GQL shortcut: nbr_deg[j*V + v], 32 bytes per vertex
-void launch_nbr_deg(hipStream_t s, const uint32_t* off, const uint32_t* adj, uint32_t V, uint8_t* tab);
 void launch_bitmaps(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V);
 void launch_rank_scan(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V, uint32_t max_n);
 void launch_compact(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V);

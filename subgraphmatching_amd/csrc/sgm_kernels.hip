@@ -113,27 +113,32 @@ __device__ bool lsm_wave(const uint32_t* __restrict__ nb, uint32_t deg, const ui
 // changes during the phase and u is not its own neighbour, so the checks of one phase are
 // independent (the reference's sequential order inside a phase does not matter).
 //
-// Exact shortcuts (the answers are the reference's, only fewer checks run):
-//   * Before phase (round 1, u) bit u of every vertex still holds its NLF value, so the
-//     candidates of u are the vertices of degree >= deg_q(u) and need no vmask read.
-//   * When no neighbour w of v has lost a bit u' of N_q(u) (touch[v] bit u clear), every
-//     bit the check reads is an NLF bit: left vertex u' is adjacent to w iff
-//     deg(w) >= deg_q(u').  These neighbourhoods are nested, so Hall's condition reduces to
-//     "the j-th largest neighbour degree of v is at least the j-th largest requirement"
-//     for every j -- one byte of the degree table per left vertex.
-//   * Round 2 rechecks v only if a neighbour lost a bit of N_q(u) after v's round-1 check
-//     of u (dirty[v] bit u): otherwise every input of the check is unchanged and v passes
-//     again.  A removal of v from C(u') in phase (1, u') therefore marks its neighbours
-//     dirty for u in N_q(u') with u < u', and in phase (2, u') for u > u'.
+// The phase works in degree-rank space: rank r is the r-th vertex of the degree-sorted
+// list, the rank-space adjacency lists hold ranks, and the NLF candidates of u are exactly
+// the ranks below ge_cnt[deg_q(u)] -- a contiguous prefix, streamed with coalesced loads.
 //
-// Work units: vertices are streamed in id order, 64 per unit, one per lane, with coalesced
-// loads of the capped degree, the marks and the table bytes.  Lanes decide their vertex
-// themselves where they can -- the degree table, or, over at most GQL_LANE_DEG neighbours,
-// a greedy matching that saturates the left side (true), a left vertex no right vertex
-// covers, or fewer useful right vertices than left vertices (false, only when the whole
-// list was seen) -- and the wave runs the exact matcher on the few undecided ones and
-// marks the neighbours of removed high-degree vertices together.
+// Exact shortcuts (the answers are the reference's, only fewer checks run):
+//   * Before phase (round 1, u) bit u of every vertex still holds its NLF value, so every
+//     rank of the prefix is a candidate and needs no membership read.
+//   * When no neighbour w of v has lost a bit u' of N_q(u) (touch bit u clear), every bit
+//     the check reads is an NLF bit: left vertex u' is adjacent to w iff deg(w) >=
+//     deg_q(u').  These neighbourhoods are nested, so Hall's condition reduces to "the j-th
+//     largest neighbour degree of v is at least the j-th largest requirement" for every j:
+//     eight byte comparisons per 64-bit word of the degree table (SWAR, values <= 31).
+//   * Round 2 rechecks v only if a neighbour lost a bit of N_q(u) after v's round-1 check
+//     of u (dirty bit u): otherwise every input of the check is unchanged and v passes
+//     again.  A removal from C(u') in phase (1, u') therefore marks the neighbours dirty for
+//     u in N_q(u') with u < u', and in phase (2, u') for u > u'.
+//
+// Work units: GQL_VPL x 64 ranks per wave step, GQL_VPL per lane with all loads issued
+// before any decision.  Lanes decide their vertices themselves where they can -- the
+// degree table, or, over at most GQL_LANE_DEG neighbours, a greedy matching that saturates
+// the left side (true), a left vertex no right vertex covers, or fewer useful right
+// vertices than left vertices (false, only when the whole list was seen) -- and the wave
+// runs the exact matcher on the few undecided ones and marks the neighbours of removed
+// high-degree vertices together.
 #define GQL_LANE_DEG 128u
+#define GQL_VPL 4u
 
 __device__ __forceinline__ int lsm_lane(const uint32_t* __restrict__ adj, uint32_t o, uint32_t deg,
                                         const uint32_t* __restrict__ vm, uint32_t L, uint32_t k)
@@ -171,10 +176,19 @@ __device__ __forceinline__ void gql_mark(uint32_t* __restrict__ touch, uint32_t*
     if (dm && (dirty[w] & dm) != dm) atomicOr(&dirty[w], dm);
 }
 
-__global__ void __launch_bounds__(256) k_gql_phase(const uint32_t* __restrict__ off,
-                                                   const uint32_t* __restrict__ adj,
-                                                   const uint8_t* __restrict__ degc,
-                                                   const uint8_t* __restrict__ nbr_deg, uint32_t V,
+// Hall's test on the degree table: byte j of D (the (j+1)-th largest capped neighbour
+// degree) >= byte j of R (the (j+1)-th largest requirement, 0 past the left side) for all
+// eight bytes.  Bytes are <= 31, so (D | 0x80..) - R never borrows across bytes.
+__device__ __forceinline__ bool hall_word(uint64_t D, uint64_t R)
+{
+    const uint64_t H = 0x8080808080808080ull;
+    return (((D | H) - R) & H) == H;
+}
+
+__global__ void __launch_bounds__(256) k_gql_phase(const uint32_t* __restrict__ roff,
+                                                   const uint32_t* __restrict__ radj,
+                                                   const uint64_t* __restrict__ dtab, uint32_t V,
+                                                   const uint32_t* __restrict__ ge_cnt,
                                                    const sgm_qdesc* __restrict__ qs, uint32_t phase)
 {
     const sgm_qdesc* d = qs + blockIdx.y;
@@ -194,9 +208,13 @@ __global__ void __launch_bounds__(256) k_gql_phase(const uint32_t* __restrict__ 
     const uint32_t high = u == 31u ? 0u : (L & (~0u << (u + 1u)));
     const uint32_t tm = r == 0 ? high : 0u;  // touch bits only matter to later round-1 phases
     const uint32_t dm = r == 0 ? low : high;
-    const uint32_t k = __popc(L), qd = d->qdeg[u];
-    const uint8_t* __restrict__ req = d->req[u];
-    uint32_t* __restrict__ vm = d->vmask;
+    const uint32_t k = __popc(L);
+    const uint32_t P = ge_cnt[d->qdeg[u]];  // candidates of u: ranks [0, P)
+    const uint32_t np = (k + 7) / 8;        // table words the test needs
+    uint64_t R[4];
+#pragma unroll
+    for (int p = 0; p < 4; p++) R[p] = d->req[u][p];
+    uint32_t* __restrict__ vm = d->vmr;
     uint32_t* __restrict__ touch = d->touch;
     uint32_t* __restrict__ dirty = d->dirty;
     const uint32_t lane = lane_id();
@@ -204,57 +222,78 @@ __global__ void __launch_bounds__(256) k_gql_phase(const uint32_t* __restrict__ 
     const uint32_t w0 = blockIdx.x * waves + (threadIdx.x >> 6);
     const uint32_t stride = gridDim.x * waves;
     uint32_t local_removed = 0, n_table = 0, n_lane = 0, n_wave = 0;
-    const uint32_t nunits = (V + 63) / 64;
+    const uint32_t nunits = (P + 64 * GQL_VPL - 1) / (64 * GQL_VPL);
     for (uint32_t unit = w0; unit < nunits; unit += stride) {
-        const uint32_t v = unit * 64 + lane;
-        bool cand = v < V && degc[v] >= qd;  // NLF (degrees are capped at 255 >= 31 >= qd)
-        uint32_t o = 0, dv = 0;
-        int verdict = -1;
-        bool by_table = false;
+        uint32_t x[GQL_VPL], w1[GQL_VPL], w2[GQL_VPL];
+        bool cand[GQL_VPL];
+        uint64_t D[GQL_VPL][4];
+#pragma unroll
+        for (uint32_t t = 0; t < GQL_VPL; t++) {
+            x[t] = unit * 64 * GQL_VPL + t * 64 + lane;
+            cand[t] = x[t] < P;
+        }
         if (r == 0) {
-            if (cand && (!use_touch || !((touch[v] >> u) & 1u))) {
-                bool ok = true;
-                for (uint32_t j = 0; j < k && ok; j++) ok = nbr_deg[(size_t)j * V + v] >= req[j];
-                verdict = ok ? 1 : 0;
-                by_table = true;
-            } else if (cand) {
-                o = off[v];
-                dv = off[v + 1] - o;
-                verdict = lsm_lane(adj, o, dv, vm, L, k);
+#pragma unroll
+            for (uint32_t t = 0; t < GQL_VPL; t++) {
+                w1[t] = (use_touch && cand[t]) ? touch[x[t]] : 0u;
+#pragma unroll
+                for (uint32_t p = 0; p < 4; p++) D[t][p] = (p < np && cand[t]) ? dtab[(size_t)p * V + x[t]] : 0ull;
             }
         } else {
-            cand = cand && ((dirty[v] >> u) & 1u) && ((vm[v] >> u) & 1u);
-            if (cand) {
-                o = off[v];
-                dv = off[v + 1] - o;
-                verdict = lsm_lane(adj, o, dv, vm, L, k);
+#pragma unroll
+            for (uint32_t t = 0; t < GQL_VPL; t++) {
+                w1[t] = cand[t] ? dirty[x[t]] : 0u;
+                w2[t] = cand[t] ? vm[x[t]] : 0u;
             }
         }
-        if (cand && verdict == 0) {
-            if (dv == 0) { o = off[v]; dv = off[v + 1] - o; }  // decided by the table
-            atomicAnd(&vm[v], ~(1u << u));
-            if (dv <= GQL_LANE_DEG)
-                for (uint32_t p = o; p < o + dv; p++) gql_mark(touch, dirty, adj[p], tm, dm);
-        }
-        local_removed += (uint32_t)__popcll(__ballot(cand && verdict == 0));
-        n_table += (uint32_t)__popcll(__ballot(by_table));
-        n_lane += (uint32_t)__popcll(__ballot(cand && !by_table));
-        // the wave: exact matcher for undecided lanes, shared marking for long lists
-        uint64_t pend = __ballot(cand && (verdict < 0 || (verdict == 0 && dv > GQL_LANE_DEG)));
-        while (pend) {
-            const uint32_t t = (uint32_t)__ffsll((long long)pend) - 1u;
-            pend &= pend - 1;
-            const uint32_t vt = rdlane(v, t);
-            const uint32_t ot = off[vt], dt = off[vt + 1] - ot;
-            bool fail = rdlane((uint32_t)verdict, t) == 0u;
-            n_wave += fail ? 0u : 1u;
-            if (!fail && !lsm_wave(adj + ot, dt, vm, L, k)) {
-                if (lane == 0) atomicAnd(&vm[vt], ~(1u << u));
-                local_removed++;
-                fail = true;
+#pragma unroll
+        for (uint32_t t = 0; t < GQL_VPL; t++) {
+            int verdict = -1;
+            bool by_table = false;
+            uint32_t o = 0, dv = 0;
+            bool c = cand[t];
+            if (r == 0) {
+                if (c && !((w1[t] >> u) & 1u)) {
+                    bool ok = true;
+#pragma unroll
+                    for (uint32_t p = 0; p < 4; p++) ok = ok && (p >= np || hall_word(D[t][p], R[p]));
+                    verdict = ok ? 1 : 0;
+                    by_table = true;
+                }
+            } else {
+                c = c && ((w1[t] >> u) & 1u) && ((w2[t] >> u) & 1u);
             }
-            if (fail)
-                for (uint32_t p = ot + lane; p < ot + dt; p += 64) gql_mark(touch, dirty, adj[p], tm, dm);
+            if (c && !by_table) {
+                o = roff[x[t]];
+                dv = roff[x[t] + 1] - o;
+                verdict = lsm_lane(radj, o, dv, vm, L, k);
+            }
+            if (c && verdict == 0) {
+                if (by_table) { o = roff[x[t]]; dv = roff[x[t] + 1] - o; }
+                atomicAnd(&vm[x[t]], ~(1u << u));
+                if (dv <= GQL_LANE_DEG)
+                    for (uint32_t q = o; q < o + dv; q++) gql_mark(touch, dirty, radj[q], tm, dm);
+            }
+            local_removed += (uint32_t)__popcll(__ballot(c && verdict == 0));
+            n_table += (uint32_t)__popcll(__ballot(by_table));
+            n_lane += (uint32_t)__popcll(__ballot(c && !by_table));
+            // the wave: exact matcher for undecided lanes, shared marking for long lists
+            uint64_t pend = __ballot(c && (verdict < 0 || (verdict == 0 && dv > GQL_LANE_DEG)));
+            while (pend) {
+                const uint32_t s = (uint32_t)__ffsll((long long)pend) - 1u;
+                pend &= pend - 1;
+                const uint32_t xs = rdlane(x[t], s);
+                const uint32_t os = roff[xs], ds = roff[xs + 1] - os;
+                bool fail = rdlane((uint32_t)verdict, s) == 0u;
+                n_wave += fail ? 0u : 1u;
+                if (!fail && !lsm_wave(radj + os, ds, vm, L, k)) {
+                    if (lane == 0) atomicAnd(&vm[xs], ~(1u << u));
+                    local_removed++;
+                    fail = true;
+                }
+                if (fail)
+                    for (uint32_t q = os + lane; q < os + ds; q += 64) gql_mark(touch, dirty, radj[q], tm, dm);
+            }
         }
     }
     if (lane == 0) {
@@ -270,23 +309,66 @@ __global__ void __launch_bounds__(256) k_gql_phase(const uint32_t* __restrict__ 
     }
 }
 
-// Degree table of the data graph for the GQL shortcut: nbr_deg[j*V + v] = the (j+1)-th
-// largest min(deg(w), 31) over the neighbours w of v, 0 if v has at most j neighbours
-// (j < 32).  Query degrees are at most 31, so the cap loses nothing.  One wave per vertex:
-// a 31-bin histogram by ballots, then lane j reads off its order statistic.
-__global__ void __launch_bounds__(256) k_nbr_deg(const uint32_t* __restrict__ off, const uint32_t* __restrict__ adj,
-                                                 uint32_t V, uint8_t* __restrict__ tab)
+// NLF in rank space: degrees are non-increasing in rank, so rank x takes u iff
+// x < ge_cnt[deg_q(u)].
+__global__ void __launch_bounds__(256) k_nlf_rank(const uint32_t* __restrict__ ge_cnt, uint32_t V,
+                                                  const sgm_qdesc* __restrict__ qs)
+{
+    const sgm_qdesc* d = qs + blockIdx.y;
+    const uint32_t n = d->n;
+    if (n == 0) return;
+    uint32_t* __restrict__ vmr = d->vmr;
+    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < V; x += gridDim.x * blockDim.x) {
+        uint32_t m = 0;
+        for (uint32_t u = 0; u < n; u++) m |= (x < ge_cnt[d->qdeg[u]] ? 1u : 0u) << u;  // uniform loads
+        vmr[x] = m;
+    }
+}
+
+// Back to vertex ids after the last phase: vmask[v] = vmr[rank[v]].
+__global__ void __launch_bounds__(256) k_unrank(const uint32_t* __restrict__ rank, uint32_t V,
+                                                const sgm_qdesc* __restrict__ qs)
+{
+    const sgm_qdesc* d = qs + blockIdx.y;
+    if (d->n == 0) return;
+    const uint32_t* __restrict__ vmr = d->vmr;
+    uint32_t* __restrict__ vm = d->vmask;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) vm[v] = vmr[rank[v]];
+}
+
+// Rank-space adjacency: list of rank x = the ranks of by_deg[x]'s neighbours (in the
+// vertex-id order of the original list).  One wave per rank.
+__global__ void __launch_bounds__(256) k_rank_adj(const uint32_t* __restrict__ off, const uint32_t* __restrict__ adj,
+                                                  const uint32_t* __restrict__ by_deg, const uint32_t* __restrict__ rank,
+                                                  const uint32_t* __restrict__ roff, uint32_t V,
+                                                  uint32_t* __restrict__ radj)
 {
     const uint32_t lane = lane_id();
     const uint32_t waves = blockDim.x >> 6;
-    for (uint32_t v = blockIdx.x * waves + (threadIdx.x >> 6); v < V; v += gridDim.x * waves) {
-        const uint32_t o = off[v], dv = off[v + 1] - o;
+    for (uint32_t x = blockIdx.x * waves + (threadIdx.x >> 6); x < V; x += gridDim.x * waves) {
+        const uint32_t v = by_deg[x];
+        const uint32_t o = off[v], dv = off[v + 1] - o, ro = roff[x];
+        for (uint32_t j = lane; j < dv; j += 64) radj[ro + j] = rank[adj[o + j]];
+    }
+}
+
+// Degree table for the GQL shortcut, by rank: byte j of dtab[(j/8)*V + x] is the (j+1)-th
+// largest min(deg(w), 31) over the neighbours w of rank x, 0 if x has at most j neighbours
+// (j < 32).  Query degrees are at most 31, so the cap loses nothing.  One wave per rank: a
+// 31-bin histogram by ballots, then lane j reads off its order statistic.
+__global__ void __launch_bounds__(256) k_nbr_deg(const uint32_t* __restrict__ roff, const uint32_t* __restrict__ radj,
+                                                 uint32_t V, uint8_t* __restrict__ dtab)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t waves = blockDim.x >> 6;
+    for (uint32_t x = blockIdx.x * waves + (threadIdx.x >> 6); x < V; x += gridDim.x * waves) {
+        const uint32_t o = roff[x], dv = roff[x + 1] - o;
         uint32_t hist = 0;  // lane b: neighbours with min(deg, 31) == b
         for (uint32_t base = 0; base < dv; base += 64) {
             uint32_t c = 0;  // neighbours have degree >= 1, so bin 0 stays empty
             if (base + lane < dv) {
-                const uint32_t w = adj[o + base + lane];
-                const uint32_t dw = off[w + 1] - off[w];
+                const uint32_t w = radj[o + base + lane];
+                const uint32_t dw = roff[w + 1] - roff[w];
                 c = dw < 31u ? dw : 31u;
             }
             for (uint32_t b = 1; b < 32; b++) {
@@ -294,12 +376,12 @@ __global__ void __launch_bounds__(256) k_nbr_deg(const uint32_t* __restrict__ of
                 if (lane == b) hist += cnt;
             }
         }
-        uint32_t ge = 0, D = 0;
+        uint32_t ge = 0, Dj = 0;
         for (uint32_t b = 31; b >= 1; b--) {
             ge += rdlane(hist, b);
-            if (D == 0 && ge >= lane + 1) D = b;
+            if (Dj == 0 && ge >= lane + 1) Dj = b;
         }
-        if (lane < 32) tab[(size_t)lane * V + v] = (uint8_t)D;
+        if (lane < 32) dtab[((size_t)(lane >> 3) * V + x) * 8 + (lane & 7u)] = (uint8_t)Dj;
     }
 }
 
@@ -379,7 +461,9 @@ __global__ void __launch_bounds__(256) k_rank_scan(const sgm_cdesc* __restrict__
 
 // Candidate statistics of a batch without lists: cnt[q*32 + u] = |C(u)| and
 // degsum[q*32 + u] = sum of deg(v) over C(u) (the cost model's mean candidate degree).
-// Both start zeroed.  Waves stream 64 vertices; block totals go out with one atomic each.
+// Both start zeroed.  Each lane accumulates all 32 query vertices in registers over its
+// strided vertices (compile-time indices, no cross-lane work in the loop); the wave then
+// reduces once and lane 0 adds its totals with one atomic per query vertex.
 __global__ void __launch_bounds__(256) k_cand_stats(const uint32_t* __restrict__ off, uint32_t V,
                                                     const sgm_qdesc* __restrict__ qs, uint32_t* __restrict__ cnt,
                                                     unsigned long long* __restrict__ degsum)
@@ -387,34 +471,37 @@ __global__ void __launch_bounds__(256) k_cand_stats(const uint32_t* __restrict__
     const sgm_qdesc* d = qs + blockIdx.y;
     const uint32_t n = d->n;
     if (n == 0) return;
-    __shared__ uint32_t s_cnt[SGM_MAXQ];
-    __shared__ unsigned long long s_sum[SGM_MAXQ];
-    if (threadIdx.x < SGM_MAXQ) { s_cnt[threadIdx.x] = 0; s_sum[threadIdx.x] = 0; }
-    __syncthreads();
     const uint32_t* __restrict__ vm = d->vmask;
-    const uint32_t waves = blockDim.x >> 6, lane = lane_id();
-    for (uint32_t w = blockIdx.x * waves + (threadIdx.x >> 6); w < (V + 63) / 64; w += gridDim.x * waves) {
-        const uint32_t v = w * 64 + lane;
-        const uint32_t m = v < V ? vm[v] : 0u;
-        const uint32_t dv = m ? off[v + 1] - off[v] : 0u;
-        if (__ballot(m != 0u) == 0) continue;
-        for (uint32_t u = 0; u < n; u++) {
-            const uint64_t b = __ballot((m >> u) & 1u);
-            if (!b) continue;
-            long long x = ((m >> u) & 1u) ? (long long)dv : 0ll;  // 64 degrees may exceed 2^32
+    const uint32_t lane = lane_id();
+    uint32_t c[SGM_MAXQ];
+    unsigned long long sm[SGM_MAXQ];
 #pragma unroll
-            for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o);
-            if (lane == 0) {
-                atomicAdd(&s_cnt[u], (uint32_t)__popcll(b));
-                atomicAdd(&s_sum[u], (unsigned long long)x);
-            }
+    for (int u = 0; u < SGM_MAXQ; u++) { c[u] = 0; sm[u] = 0; }
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
+        const uint32_t m = vm[v];
+        if (!m) continue;
+        const unsigned long long dv = off[v + 1] - off[v];
+#pragma unroll
+        for (int u = 0; u < SGM_MAXQ; u++) {
+            const uint32_t bit = (m >> u) & 1u;
+            c[u] += bit;
+            sm[u] += bit ? dv : 0ull;
         }
     }
-    __syncthreads();
-    if (threadIdx.x < n) {
-        const size_t i = (size_t)blockIdx.y * SGM_MAXQ + threadIdx.x;
-        if (s_cnt[threadIdx.x]) atomicAdd(&cnt[i], s_cnt[threadIdx.x]);
-        if (s_sum[threadIdx.x]) atomicAdd(&degsum[i], s_sum[threadIdx.x]);
+    const size_t base = (size_t)blockIdx.y * SGM_MAXQ;
+#pragma unroll
+    for (int u = 0; u < SGM_MAXQ; u++) {
+        uint32_t x = c[u];
+        long long y = (long long)sm[u];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            x += (uint32_t)__shfl_xor((int)x, o);
+            y += __shfl_xor(y, o);
+        }
+        if (lane == 0 && x && (uint32_t)u < n) {
+            atomicAdd(&cnt[base + u], x);
+            atomicAdd(&degsum[base + u], (unsigned long long)y);
+        }
     }
 }
 
@@ -548,16 +635,35 @@ void launch_nlf(hipStream_t s, const uint32_t* off, uint32_t V, const sgm_qdesc*
     hipLaunchKernelGGL(k_nlf, grid, dim3(256), 0, s, off, V, qs);
 }
 
-void launch_gql_phase(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint8_t* degc,
-                      const uint8_t* nbr_deg, uint32_t V, const sgm_qdesc* qs, uint32_t nq, uint32_t phase)
+void launch_gql_phase(hipStream_t s, const uint32_t* roff, const uint32_t* radj, const uint64_t* dtab, uint32_t V,
+                      const uint32_t* ge_cnt, const sgm_qdesc* qs, uint32_t nq, uint32_t phase, uint32_t max_prefix)
 {
     // about 16k blocks of 4 waves in all (at least 8 per query, at most one per 4 units)
-    const uint64_t units = ((uint64_t)V + 63) / 64;
+    const uint64_t units = ((uint64_t)max_prefix + 64 * GQL_VPL - 1) / (64 * GQL_VPL);
     const uint64_t max_blocks = (units + 3) / 4 ? (units + 3) / 4 : 1;
     uint32_t per_q = clampg(16384 / (nq ? nq : 1), 8, 4096);
     if (per_q > max_blocks) per_q = (uint32_t)max_blocks;
     dim3 grid(per_q, nq);
-    hipLaunchKernelGGL(k_gql_phase, grid, dim3(256), 0, s, off, adj, degc, nbr_deg, V, qs, phase);
+    hipLaunchKernelGGL(k_gql_phase, grid, dim3(256), 0, s, roff, radj, dtab, V, ge_cnt, qs, phase);
+}
+
+void launch_nlf_rank(hipStream_t s, const uint32_t* ge_cnt, uint32_t V, const sgm_qdesc* qs, uint32_t nq)
+{
+    dim3 grid(clampg((V + 255) / 256, 1, 4096), nq);
+    hipLaunchKernelGGL(k_nlf_rank, grid, dim3(256), 0, s, ge_cnt, V, qs);
+}
+
+void launch_unrank(hipStream_t s, const uint32_t* rank, uint32_t V, const sgm_qdesc* qs, uint32_t nq)
+{
+    dim3 grid(clampg((V + 255) / 256, 1, 4096), nq);
+    hipLaunchKernelGGL(k_unrank, grid, dim3(256), 0, s, rank, V, qs);
+}
+
+void launch_rank_adj(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* by_deg,
+                     const uint32_t* rank, const uint32_t* roff, uint32_t V, uint32_t* radj)
+{
+    hipLaunchKernelGGL(k_rank_adj, dim3(clampg(((uint64_t)V + 3) / 4, 1, 16384)), dim3(256), 0, s, off, adj, by_deg,
+                       rank, roff, V, radj);
 }
 
 void launch_cand_stats(hipStream_t s, const uint32_t* off, uint32_t V, const sgm_qdesc* qs, uint32_t nq,
@@ -574,9 +680,10 @@ void launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t val)
     hipLaunchKernelGGL(k_fill_u32, dim3(clampg((n + 255) / 256, 1, 8192)), dim3(256), 0, s, p, n, val);
 }
 
-void launch_nbr_deg(hipStream_t s, const uint32_t* off, const uint32_t* adj, uint32_t V, uint8_t* tab)
+void launch_nbr_deg(hipStream_t s, const uint32_t* roff, const uint32_t* radj, uint32_t V, uint64_t* dtab)
 {
-    hipLaunchKernelGGL(k_nbr_deg, dim3(clampg(((uint64_t)V + 3) / 4, 1, 16384)), dim3(256), 0, s, off, adj, V, tab);
+    hipLaunchKernelGGL(k_nbr_deg, dim3(clampg(((uint64_t)V + 3) / 4, 1, 16384)), dim3(256), 0, s, roff, radj, V,
+                       reinterpret_cast<uint8_t*>(dtab));
 }
 
 void launch_bitmaps(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V)

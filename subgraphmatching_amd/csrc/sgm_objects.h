@@ -67,14 +67,19 @@ struct sgm_graph {
     uint32_t hub_words = 0;
     uint32_t* d_hub_slot = nullptr;
     uint32_t* d_hub_bits = nullptr;
-    uint8_t* d_nbrdeg = nullptr;  // 32*V bytes: order statistics of neighbour degrees (GQL shortcut)
-    uint8_t* d_degc = nullptr;    // V bytes: min(degree, 255)
+    // degree-rank space of the GQL filter: rank[v] (position in by_deg), the CSR of ranks
+    // (roff[V+1], radj[2E]) and the neighbour-degree table (4 planes of V words)
+    uint32_t* d_rank = nullptr;
+    uint32_t* d_roff = nullptr;
+    uint32_t* d_radj = nullptr;
+    uint64_t* d_dtab = nullptr;
     std::vector<uint32_t> hub_vertex;                   // bitmap slot -> vertex (degree descending)
     std::vector<uint32_t> hub_degree;
     std::vector<std::pair<uint32_t, uint32_t*>> hub_maps;  // (min degree, slot map) for higher thresholds
 };
 
-// GQL scratch is allocated per filter batch; larger batches run in chunks of this size.
+// GQL scratch (3 V words per query: rank-space membership, touch and dirty marks) is
+// allocated per filter batch; larger batches run in chunks of this size.
 static const uint64_t kGqlScratchBytes = 8ull << 30;
 
 static const uint32_t kHubMinDegree = 64;
