@@ -17,6 +17,16 @@ Workloads (one step = one pass of the path over the whole query set):
     each step filters, ranks RI + GQL + 32 sampled orders per query on the GPU, keeps RI
     unless the model predicts a clear gain, and enumerates; the same steps with RI orders
     are timed after it for the learned-vs-RI speedup
+  rmat22-serve
+    data graph  R-MAT scale 22, average degree 32 (2^26 undirected edges), seed 5
+    queries     1,200 random-walk queries, sizes uniform in 8..32, mixed sparse / dense,
+                seed 55, cap 10^6, arriving as a Poisson stream at --serve-load (0.7) of the
+                capacity measured by a back-to-back pass (seed 77)
+    path        serving loop: dynamic batches of every arrived query, up to the slot count
+                admitted by a device-memory budget (--serve-mem-gb), learned orders with RI
+                fallback (a cost model trained untimed on 600 queries, seed 66); one step = one
+                serving run over the stream; the same stream served with RI orders is
+                reported beside it (queries/s, p50 / p99 latency)
   rmat20
     data graph  R-MAT scale 20 (a,b,c,d = .57,.19,.19,.05), 2*10^7 undirected edges, seed 3
     queries     1,000 random-walk queries, sizes uniform in 4..32, even ids dense / odd
@@ -52,7 +62,12 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["chunglu100k", "rmat20", "rmat20-learned"], default="chunglu100k")
+    ap.add_argument("--workload", choices=["chunglu100k", "rmat20", "rmat20-learned", "rmat22-serve"],
+                    default="chunglu100k")
+    ap.add_argument("--serve-mem-gb", type=float, default=32.0,
+                    help="rmat22-serve: device memory budget for query slots and DFS stacks")
+    ap.add_argument("--serve-load", type=float, default=0.7,
+                    help="rmat22-serve: arrival rate as a fraction of the measured capacity")
     ap.add_argument("--train-queries", type=int, default=5000,
                     help="rmat20-learned: training queries (10 logged orders each: RI, GQL, 8 sampled)")
     ap.add_argument("--rank-samples", type=int, default=32,
@@ -316,6 +331,71 @@ def setup_rmat_learned(args):
     return g, queries, step, config, {"compare": compare}
 
 
+# ------------------------------------------------------------------------- rmat22-serve
+def setup_rmat_serve(args):
+    import subgraphmatching_amd as sgm
+    from subgraphmatching_amd.generators import query_set, rmat
+    from subgraphmatching_amd.learned import collect_training, train_model
+    from subgraphmatching_amd.serving import admit_slots, poisson_arrivals, serve
+
+    cap = args.cap if args.cap != 10_000_000 else 1_000_000
+    g = rmat(22, 1 << 26, seed=5)
+    nq = args.queries or 1200
+    mix = lambda count, seed: query_set(g, count, lambda i, r: int(r.integers(8, 33)),  # noqa: E731
+                                        lambda i, r: i % 2 == 0, seed=seed)
+    queries = mix(nq, 55)
+    train_q = mix(600, 66)
+    ctx = sgm.default_context()
+    num_cus = 256
+    slots, stack, per_slot = admit_slots(g.getVertexCnt(), g.getMaxDgree(), int(args.serve_mem_gb * (1 << 30)),
+                                         waves=num_cus * 4 * 4)
+    runner = sgm.Batch(g, slots=slots, ctx=ctx)
+    opts = {"blocks": num_cus * 4}
+    items, labels, info = collect_training(runner, train_q, n_samples=8, cap=cap, seed=2, time_limit_s=5.0)
+    model, losses = train_model(items, labels, seed=8, epochs=30, minibatch=256, lr=2e-3)
+    rank = lambda r: (lambda o: (o[0], o[3]))(r.rank_orders(model, n_samples=32, seed=5, margin=0.5))  # noqa: E731
+    # capacity: the whole stream at once, RI orders
+    zero = [0.0] * nq
+    serve(runner, queries[:2 * slots], zero[:2 * slots], cap, enum_opts=opts)  # warm the captured graphs
+    capacity = serve(runner, queries, zero, cap, enum_opts=opts).queries_per_s
+    rate = args.serve_load * capacity
+    arrivals = poisson_arrivals(nq, rate, seed=77)
+    print(f"[bench] serving: {slots} slots admitted ({per_slot / 2**20:.0f} MiB each, stacks {stack / 2**30:.1f} GiB), "
+          f"capacity {capacity:.0f} q/s, Poisson rate {rate:.0f} q/s; cost model on {len(items)} triples, "
+          f"loss {losses[0]:.2f} -> {losses[-1]:.2f}", flush=True)
+    last = {}
+
+    def step():
+        res = serve(runner, queries, arrivals, cap, order_fn=rank, enum_opts=opts)
+        last["learned"] = res
+        rec = new_rec()
+        rec["emb"] = sum(res.counts)
+        rec["counts"] = res.counts
+        rec["filter_ms"] = rec["gql_ms"] = res.device_ms["filter"]
+        rec["enum_ms"] = res.device_ms["enumerate"]
+        rec["enum_launches"] = res.batches
+        rec["capped"] = sum(int(c >= cap) for c in res.counts)
+        rec["wall_s"] = res.elapsed_s
+        return rec
+
+    def compare(K, learned_counts, learned_s):
+        ri = serve(runner, queries, arrivals, cap, enum_opts=opts)
+        le = last["learned"]
+        summary = lambda r: {"queries_per_s": r.queries_per_s, "latency_ms": r.latency_ms,  # noqa: E731
+                             "batches": r.batches, "mean_batch": r.mean_batch, "device_ms": r.device_ms}
+        return {"arrival_rate_q_per_s": rate, "capacity_q_per_s_ri_back_to_back": capacity, "slots": slots,
+                "learned": dict(summary(le), learned_orders_used=le.learned_used),
+                "ri": summary(ri), "counts_identical": ri.counts == learned_counts,
+                "p99_ri_over_learned": ri.latency_ms["p99"] / le.latency_ms["p99"]}
+
+    config = {"workload": "rmat_s22_e2^26_seed5__1200x8-32v_mixed_rw_seed55__poisson_serving_learned_order",
+              "queries_per_step": nq, "cap_per_query": cap, "order": "learned (RI fallback)",
+              "filter": "NLF+GQL", "enumeration": "cache+filter+failing-sets",
+              "launch": f"dynamic batches of up to {slots} (hipGraph-replayed)",
+              "memory_budget_gb": args.serve_mem_gb, "arrival_load": args.serve_load}
+    return g, queries, step, config, {"compare": compare, "serving": True}
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -327,7 +407,8 @@ def main():
         sys.exit(2)
 
     t_setup = time.perf_counter()
-    setup = {"rmat20": setup_rmat, "rmat20-learned": setup_rmat_learned}.get(args.workload, setup_chunglu)
+    setup = {"rmat20": setup_rmat, "rmat20-learned": setup_rmat_learned,
+             "rmat22-serve": setup_rmat_serve}.get(args.workload, setup_chunglu)
     g, queries, step, config, extra = setup(args)
     print(f"[bench] {args.workload}: graph V={g.getVertexCnt()} E={g.getEdgeCnt()} maxdeg={g.getMaxDgree()}; "
           f"{len(queries)} queries; setup {time.perf_counter() - t_setup:.1f}s", flush=True)
@@ -371,7 +452,9 @@ def main():
     extras = {}
     if "compare" in extra:
         extras["learned_vs_ri"] = extra["compare"](K, counts0, total_s)
-        print(f"[bench] learned vs RI: {extras['learned_vs_ri']}", flush=True)
+        key = "serving" if extra.get("serving") else "learned_vs_ri"
+        extras[key] = extras.pop("learned_vs_ri")
+        print(f"[bench] {key}: {extras[key]}", flush=True)
     if "single_launch" in extra:
         sl, sl_counts = extra["single_launch"]()
         extras["single_launch"] = dict(sl, counts_match=sl_counts == counts0)
@@ -379,7 +462,7 @@ def main():
 
     cpu = None
     if args.cpu_seconds > 0:
-        cpu = cpu_baseline(g, queries, counts0, args.cap, args.cpu_seconds)
+        cpu = cpu_baseline(g, queries, counts0, config["cap_per_query"], args.cpu_seconds)
 
     out = {
         "metric": METRIC,
