@@ -139,6 +139,7 @@ __device__ bool lsm_wave(const uint32_t* __restrict__ nb, uint32_t deg, const ui
 // high-degree vertices together.
 #define GQL_LANE_DEG 128u
 #define GQL_VPL 4u
+#define GQL_QCAP 512u  // per-wave matcher queue: < 64 left over + 64 * GQL_VPL new per unit
 
 __device__ __forceinline__ int lsm_lane(const uint32_t* __restrict__ adj, uint32_t o, uint32_t deg,
                                         const uint32_t* __restrict__ vm, uint32_t L, uint32_t k)
@@ -222,6 +223,55 @@ __global__ void __launch_bounds__(256) k_gql_phase(const uint32_t* __restrict__ 
     const uint32_t w0 = blockIdx.x * waves + (threadIdx.x >> 6);
     const uint32_t stride = gridDim.x * waves;
     uint32_t local_removed = 0, n_table = 0, n_lane = 0, n_wave = 0;
+    // Candidates that need a matcher (touched in round 1, dirty in round 2) are queued in a
+    // per-wave LDS ring and matched 64 at a time, one per lane, so the few such candidates
+    // do not each stall a whole wave behind one divergent lane.
+    __shared__ uint32_t s_queue[4][GQL_QCAP];
+    uint32_t* __restrict__ wq = s_queue[(threadIdx.x >> 6) & 3u];
+    uint32_t qh = 0, qt = 0;  // wave-uniform head / tail (ring index = value mod GQL_QCAP)
+    // wave-cooperative removal marking of a vertex with a long list (lane 0 already cleared its bit)
+    auto mark_all = [&](uint32_t xs) {
+        const uint32_t os = roff[xs], ds = roff[xs + 1] - os;
+        for (uint32_t q = os + lane; q < os + ds; q += 64) gql_mark(touch, dirty, radj[q], tm, dm);
+    };
+    auto drain = [&](uint32_t cnt) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const bool has = lane < cnt;
+        const uint32_t xv = has ? wq[(qh + lane) & (GQL_QCAP - 1u)] : 0u;
+        qh += cnt;
+        uint32_t o = 0, dv = 0;
+        int verdict = -1;
+        if (has) {
+            o = roff[xv];
+            dv = roff[xv + 1] - o;
+            verdict = lsm_lane(radj, o, dv, vm, L, k);
+            n_lane++;
+        }
+        if (has && verdict == 0) {
+            atomicAnd(&vm[xv], ~(1u << u));
+            local_removed++;
+            if (dv <= GQL_LANE_DEG)
+                for (uint32_t q = o; q < o + dv; q++) gql_mark(touch, dirty, radj[q], tm, dm);
+        }
+        // the wave: exact matcher for undecided lanes, shared marking for long lists
+        uint64_t pend = __ballot(has && (verdict < 0 || (verdict == 0 && dv > GQL_LANE_DEG)));
+        while (pend) {
+            const uint32_t s2 = (uint32_t)__ffsll((long long)pend) - 1u;
+            pend &= pend - 1;
+            const uint32_t xs = rdlane(xv, s2);
+            bool fail = rdlane((uint32_t)verdict, s2) == 0u;
+            if (!fail) {
+                const uint32_t os = roff[xs], ds = roff[xs + 1] - os;
+                if (lane == 0) n_wave++;
+                if (!lsm_wave(radj + os, ds, vm, L, k)) {
+                    if (lane == 0) { atomicAnd(&vm[xs], ~(1u << u)); local_removed++; }
+                    fail = true;
+                }
+            }
+            if (fail) mark_all(xs);
+        }
+    };
     const uint32_t nunits = (P + 64 * GQL_VPL - 1) / (64 * GQL_VPL);
     for (uint32_t unit = w0; unit < nunits; unit += stride) {
         uint32_t x[GQL_VPL], w1[GQL_VPL], w2[GQL_VPL];
@@ -248,53 +298,47 @@ __global__ void __launch_bounds__(256) k_gql_phase(const uint32_t* __restrict__ 
         }
 #pragma unroll
         for (uint32_t t = 0; t < GQL_VPL; t++) {
-            int verdict = -1;
-            bool by_table = false;
-            uint32_t o = 0, dv = 0;
-            bool c = cand[t];
+            bool need = false, table_fail = false;
             if (r == 0) {
-                if (c && !((w1[t] >> u) & 1u)) {
+                if (cand[t] && !((w1[t] >> u) & 1u)) {
                     bool ok = true;
 #pragma unroll
                     for (uint32_t p = 0; p < 4; p++) ok = ok && (p >= np || hall_word(D[t][p], R[p]));
-                    verdict = ok ? 1 : 0;
-                    by_table = true;
+                    n_table++;
+                    table_fail = !ok;
+                } else {
+                    need = cand[t];
                 }
             } else {
-                c = c && ((w1[t] >> u) & 1u) && ((w2[t] >> u) & 1u);
+                need = cand[t] && ((w1[t] >> u) & 1u) && ((w2[t] >> u) & 1u);
             }
-            if (c && !by_table) {
-                o = roff[x[t]];
-                dv = roff[x[t] + 1] - o;
-                verdict = lsm_lane(radj, o, dv, vm, L, k);
-            }
-            if (c && verdict == 0) {
-                if (by_table) { o = roff[x[t]]; dv = roff[x[t] + 1] - o; }
+            if (table_fail) {
                 atomicAnd(&vm[x[t]], ~(1u << u));
+                local_removed++;
+                const uint32_t o = roff[x[t]], dv = roff[x[t] + 1] - o;
                 if (dv <= GQL_LANE_DEG)
                     for (uint32_t q = o; q < o + dv; q++) gql_mark(touch, dirty, radj[q], tm, dm);
+                table_fail = dv > GQL_LANE_DEG;  // a long list is marked by the whole wave below
             }
-            local_removed += (uint32_t)__popcll(__ballot(c && verdict == 0));
-            n_table += (uint32_t)__popcll(__ballot(by_table));
-            n_lane += (uint32_t)__popcll(__ballot(c && !by_table));
-            // the wave: exact matcher for undecided lanes, shared marking for long lists
-            uint64_t pend = __ballot(c && (verdict < 0 || (verdict == 0 && dv > GQL_LANE_DEG)));
-            while (pend) {
-                const uint32_t s = (uint32_t)__ffsll((long long)pend) - 1u;
-                pend &= pend - 1;
-                const uint32_t xs = rdlane(x[t], s);
-                const uint32_t os = roff[xs], ds = roff[xs + 1] - os;
-                bool fail = rdlane((uint32_t)verdict, s) == 0u;
-                n_wave += fail ? 0u : 1u;
-                if (!fail && !lsm_wave(radj + os, ds, vm, L, k)) {
-                    if (lane == 0) atomicAnd(&vm[xs], ~(1u << u));
-                    local_removed++;
-                    fail = true;
-                }
-                if (fail)
-                    for (uint32_t q = os + lane; q < os + ds; q += 64) gql_mark(touch, dirty, radj[q], tm, dm);
+            uint64_t longf = __ballot(table_fail);
+            while (longf) {
+                const uint32_t s2 = (uint32_t)__ffsll((long long)longf) - 1u;
+                longf &= longf - 1;
+                mark_all(rdlane(x[t], s2));
             }
+            const uint64_t bal = __ballot(need);
+            if (need) wq[(qt + mbcnt64(bal)) & (GQL_QCAP - 1u)] = x[t];
+            qt += (uint32_t)__popcll(bal);
         }
+        while (qt - qh >= 64) drain(64);
+    }
+    while (qt != qh) drain(qt - qh < 64 ? qt - qh : 64u);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        local_removed += (uint32_t)__shfl_xor((int)local_removed, o);
+        n_table += (uint32_t)__shfl_xor((int)n_table, o);
+        n_lane += (uint32_t)__shfl_xor((int)n_lane, o);
+        n_wave += (uint32_t)__shfl_xor((int)n_wave, o);
     }
     if (lane == 0) {
         if (local_removed) {
@@ -638,9 +682,9 @@ void launch_nlf(hipStream_t s, const uint32_t* off, uint32_t V, const sgm_qdesc*
 void launch_gql_phase(hipStream_t s, const uint32_t* roff, const uint32_t* radj, const uint64_t* dtab, uint32_t V,
                       const uint32_t* ge_cnt, const sgm_qdesc* qs, uint32_t nq, uint32_t phase, uint32_t max_prefix)
 {
-    // about 16k blocks of 4 waves in all (at least 8 per query, at most one per 4 units)
+    // about 16k blocks of 4 waves in all (at least 8 per query, at most one per 16 units)
     const uint64_t units = ((uint64_t)max_prefix + 64 * GQL_VPL - 1) / (64 * GQL_VPL);
-    const uint64_t max_blocks = (units + 3) / 4 ? (units + 3) / 4 : 1;
+    const uint64_t max_blocks = (units + 15) / 16 ? (units + 15) / 16 : 1;  // >= 4 units per wave
     uint32_t per_q = clampg(16384 / (nq ? nq : 1), 8, 4096);
     if (per_q > max_blocks) per_q = (uint32_t)max_blocks;
     dim3 grid(per_q, nq);
