@@ -9,6 +9,7 @@
 // (+ failing-set) launch over all slots.  Unused slots have n = 0 (filter) and no items
 // (enumeration).  Orders are chosen on the host between the two graphs, from the query
 // alone (RI) or from the candidate statistics (GQL order, a learned cost model).
+#include "sgm_costmodel.h"
 #include "sgm_objects.h"
 
 namespace {
@@ -37,6 +38,13 @@ struct sgm_batch {
     uint32_t* d_qstop = nullptr;             // 2 * slots: stop words, then item counters
     uint32_t* d_stack = nullptr;
     size_t stack_words_total = 0;
+    // learned-order ranking: given RI / GQL orders, candidate orders, the choice per query
+    uint8_t* d_rank_given = nullptr;         // slots * 2 * 32
+    uint8_t* d_rank_orders = nullptr;        // rank_cap * 32
+    uint32_t rank_cap = 0;
+    uint32_t* d_rank_out = nullptr;          // slots * 32
+    float* d_rank_pred = nullptr;            // 2 * slots: chosen, RI
+    int32_t* d_rank_flag = nullptr;          // slots
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     // the loaded batch
     uint32_t nq = 0;
@@ -142,6 +150,11 @@ void sgm_batch_destroy(sgm_batch* b)
     dfree(b->d_st);
     dfree(b->d_qstop);
     dfree(b->d_stack);
+    dfree(b->d_rank_given);
+    dfree(b->d_rank_orders);
+    dfree(b->d_rank_out);
+    dfree(b->d_rank_pred);
+    dfree(b->d_rank_flag);
     delete b;
 }
 
@@ -166,6 +179,10 @@ int sgm_batch_create(sgm_graph* g, uint32_t slots, sgm_batch** out)
     if (e == hipSuccess) e = dalloc(&b->d_eq, slots);
     if (e == hipSuccess) e = dalloc(&b->d_st, slots);
     if (e == hipSuccess) e = dalloc(&b->d_qstop, 2 * (size_t)slots);
+    if (e == hipSuccess) e = dalloc(&b->d_rank_given, (size_t)slots * 2 * CM_ROWS);
+    if (e == hipSuccess) e = dalloc(&b->d_rank_out, (size_t)slots * CM_ROWS);
+    if (e == hipSuccess) e = dalloc(&b->d_rank_pred, 2 * (size_t)slots);
+    if (e == hipSuccess) e = dalloc(&b->d_rank_flag, slots);
     for (auto& ev : b->ev)
         if (e == hipSuccess) e = hipEventCreate(&ev);
     if (e != hipSuccess) {
@@ -428,57 +445,54 @@ int sgm_batch_rank_orders(sgm_batch* b, sgm_costmodel* m, uint32_t n_samples, ui
     if (!b || !m || !orders) return SGM_EINVAL;
     sgm_ctx* c = b->c;
     if (b->nq == 0) return set_err(c, SGM_ESTATE, "filter a batch first");
-    const uint32_t nq = b->nq;
-    // candidate orders per query: RI, GQL, then seeded samples; duplicates dropped
-    std::vector<std::vector<uint32_t>> cand;
-    std::vector<uint32_t> first(nq + 1, 0);
+    HIPCHK(c, hipSetDevice(c->device));
+    const uint32_t nq = b->nq, K = 2 + n_samples, S = nq * K;
+    // RI and GQL orders on the host (cheap, and the GQL order needs the host's counts);
+    // the sampled candidates, features, scores and the choice are built on the device
+    std::vector<uint8_t> given((size_t)nq * 2 * CM_ROWS, 0);
+    std::vector<uint32_t> o(SGM_MAXQ);
     for (uint32_t i = 0; i < nq; i++) {
         const sgm::QueryCSR& q = b->q[i];
-        std::vector<std::vector<uint32_t>> mine;
-        std::vector<uint32_t> o(q.n);
-        auto add = [&]() {
-            for (auto& x : mine)
-                if (x == o) return;
-            mine.push_back(o);
-        };
         sgm::order_ri(q, o.data());
-        add();
+        for (uint32_t j = 0; j < q.n; j++) given[((size_t)i * 2) * CM_ROWS + j] = (uint8_t)o[j];
         sgm::order_gql(q, b->counts.data() + (size_t)i * SGM_MAXQ, b->g->V, o.data());
-        add();
-        for (uint32_t k = 0; k < n_samples; k++) {
-            const int rc = sgm_order_sample(q.off.data(), q.adj.data(), q.n, seed ^ ((uint64_t)i << 32) ^ (k * 0x9E3779B97F4A7C15ull),
-                                            o.data());
-            if (rc) return set_err(c, rc, "order sampling failed for query %u", i);
-            add();
-        }
-        first[i] = (uint32_t)cand.size();
-        for (auto& x : mine) cand.push_back(std::move(x));
+        for (uint32_t j = 0; j < q.n; j++) given[((size_t)i * 2 + 1) * CM_ROWS + j] = (uint8_t)o[j];
     }
-    first[nq] = (uint32_t)cand.size();
-    std::vector<sgm_cm_sample> smp(cand.size());
-    for (uint32_t i = 0; i < nq; i++)
-        for (uint32_t k = first[i]; k < first[i + 1]; k++) {
-            sgm_cm_sample& x = smp[k];
-            x.n = b->q[i].n;
-            x.qoff = b->q[i].off.data();
-            x.qadj = b->q[i].adj.data();
-            x.order = cand[k].data();
-            x.cand_count = b->counts.data() + (size_t)i * SGM_MAXQ;
-            x.cand_degsum = reinterpret_cast<const uint64_t*>(b->degsum.data()) + (size_t)i * SGM_MAXQ;
-        }
-    std::vector<float> pred(cand.size());
-    const int rc = sgm_costmodel_predict(m, smp.data(), (uint32_t)smp.size(), pred.data());
+    if (b->rank_cap < S) {
+        dfree(b->d_rank_orders);
+        b->d_rank_orders = nullptr;
+        b->rank_cap = 0;
+        HIPCHK(c, dalloc(&b->d_rank_orders, (size_t)S * CM_ROWS));
+        b->rank_cap = S;
+    }
+    uint16_t* F;
+    uint8_t* valid;
+    uint32_t *back, *fwd;
+    int rc = costmodel_work_inputs(m, S, &F, &valid, &back, &fwd);
     if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(b->d_rank_given, given.data(), given.size(), hipMemcpyHostToDevice, c->stream));
+    sgm::cm_rank_inputs(c->stream, b->d_qd, b->d_cnt, b->d_degsum, b->d_rank_given, nq, K, seed, b->d_rank_orders, F,
+                        valid, back, fwd);
+    const float* y = nullptr;
+    rc = costmodel_forward_work(m, S, &y);
+    if (rc) return rc;
+    sgm::cm_choose(c->stream, y, b->d_rank_orders, nq, K, margin, b->d_rank_out, b->d_rank_pred,
+                   b->d_rank_pred + b->slots, b->d_rank_flag);
+    HIPCHK(c, hipGetLastError());
+    std::vector<uint32_t> out((size_t)nq * CM_ROWS);
+    std::vector<float> pb(nq), pr(nq);
+    std::vector<int32_t> fl(nq);
+    HIPCHK(c, hipMemcpyAsync(out.data(), b->d_rank_out, sizeof(uint32_t) * out.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(pb.data(), b->d_rank_pred, sizeof(float) * nq, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(pr.data(), b->d_rank_pred + b->slots, sizeof(float) * nq, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(fl.data(), b->d_rank_flag, sizeof(int32_t) * nq, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     for (uint32_t i = 0; i < nq; i++) {
-        uint32_t best = first[i];  // RI is the first candidate of every query
-        for (uint32_t k = first[i]; k < first[i + 1]; k++)
-            if (pred[k] < pred[best]) best = k;
-        const bool use = pred[best] < pred[first[i]] - margin;
-        const uint32_t pick = use ? best : first[i];
-        std::copy(cand[pick].begin(), cand[pick].end(), orders + (size_t)i * SGM_MAXQ);
-        if (pred_best) pred_best[i] = pred[pick];
-        if (pred_ri) pred_ri[i] = pred[first[i]];
-        if (learned) learned[i] = use ? 1 : 0;
+        std::copy(out.begin() + (size_t)i * CM_ROWS, out.begin() + (size_t)i * CM_ROWS + SGM_MAXQ,
+                  orders + (size_t)i * SGM_MAXQ);
+        if (pred_best) pred_best[i] = pb[i];
+        if (pred_ri) pred_ri[i] = pr[i];
+        if (learned) learned[i] = fl[i];
     }
     return SGM_OK;
 }

@@ -18,6 +18,7 @@
 #include <stdint.h>
 
 #include "sgm_costmodel.h"
+#include "sgm_internal.h"
 #include "sgm_wave.h"
 
 namespace sgm {
@@ -274,6 +275,120 @@ __global__ void __launch_bounds__(256) k_cm_zero(float* __restrict__ p, size_t n
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.f;
 }
 
+// ------------------------------------------------------------------ order ranking
+// Model inputs of the ranking pass, built on the device.  Sample s is candidate k = s % K
+// of batch query i = s / K: k = 0 the RI order and k = 1 the GQL order (given), k >= 2 a
+// seeded random connected order (start at a uniform vertex, then a uniform unvisited
+// neighbour of the visited set).  One wave per sample, lane = order position; the features
+// are the host's (sgm_costmodel_host.cpp): deg/8, log2(1+|C|)/16, log2(1+mean degree)/16,
+// i/32, backward/deg, backward/8, [i == 0], n/32.
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ uint32_t nth_bit(uint32_t x, uint32_t r)
+{
+    for (uint32_t t = 0; t < r; t++) x &= x - 1;
+    return (uint32_t)__ffs(x) - 1u;
+}
+
+__global__ void __launch_bounds__(256) k_cm_rank_inputs(const sgm_qdesc* __restrict__ qs, const uint32_t* __restrict__ cnt,
+                                                        const unsigned long long* __restrict__ degsum,
+                                                        const uint8_t* __restrict__ given, uint32_t nq, uint32_t K,
+                                                        uint64_t seed, uint8_t* __restrict__ orders,
+                                                        uint16_t* __restrict__ F, uint8_t* __restrict__ valid,
+                                                        uint32_t* __restrict__ back, uint32_t* __restrict__ fwd)
+{
+    __shared__ uint8_t s_ord[4][CM_ROWS];
+    __shared__ uint8_t s_pos[4][CM_ROWS];
+    const uint32_t lane = lane_id(), wv = (threadIdx.x >> 6) & 3u;
+    const uint32_t smp = blockIdx.x * 4 + wv;
+    if (smp >= nq * K) return;  // whole wave
+    const uint32_t i = smp / K, k = smp % K;
+    const sgm_qdesc& d = qs[i];
+    const uint32_t n = d.n;
+    if (lane == 0) {
+        if (k < 2) {
+            for (uint32_t j = 0; j < n; j++) s_ord[wv][j] = given[((size_t)i * 2 + k) * CM_ROWS + j];
+        } else {
+            uint64_t st = seed ^ ((uint64_t)i << 32) ^ (k * 0x9E3779B97F4A7C15ull);
+            uint32_t visited = 0, frontier = 0;
+            const uint32_t all = n == 32 ? 0xFFFFFFFFu : ((1u << n) - 1u);
+            for (uint32_t j = 0; j < n; j++) {
+                const uint32_t pool = frontier ? frontier : (all & ~visited);
+                st += 0x9E3779B97F4A7C15ull;
+                const uint32_t r = (uint32_t)(((mix64(st) >> 32) * (uint64_t)__popc(pool)) >> 32);
+                const uint32_t u = nth_bit(pool, r);
+                s_ord[wv][j] = (uint8_t)u;
+                visited |= 1u << u;
+                frontier = (frontier | d.nbr[u]) & ~visited;
+            }
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t u = lane < n ? s_ord[wv][lane] : 0u;
+    if (lane < n) s_pos[wv][u] = (uint8_t)lane;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (lane >= CM_ROWS) return;
+    const size_t row = (size_t)smp * CM_ROWS + lane;
+    uint32_t bm = 0, fm = 0;
+    float f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (lane < n) {
+        uint32_t nb = d.nbr[u];
+        while (nb) {
+            const uint32_t w = (uint32_t)__ffs(nb) - 1u;
+            nb &= nb - 1u;
+            const uint32_t p = s_pos[wv][w];
+            if (p < lane) bm |= 1u << p;
+            else fm |= 1u << p;
+        }
+        const float deg = (float)d.qdeg[u];
+        const float c = (float)cnt[(size_t)i * SGM_MAXQ + u];
+        const float mean = c > 0.f ? (float)degsum[(size_t)i * SGM_MAXQ + u] / c : 0.f;
+        const float nbk = (float)__popc(bm);
+        f[0] = deg / 8.f;
+        f[1] = log2f(1.f + c) / 16.f;
+        f[2] = log2f(1.f + mean) / 16.f;
+        f[3] = (float)lane / 32.f;
+        f[4] = deg > 0.f ? nbk / deg : 0.f;
+        f[5] = nbk / 8.f;
+        f[6] = lane == 0 ? 1.f : 0.f;
+        f[7] = (float)n / 32.f;
+    }
+#pragma unroll
+    for (int j = 0; j < CM_F; j++) F[row * CM_F + j] = j < 8 ? f2bf(f[j]) : (uint16_t)0;
+    valid[row] = lane < n;
+    back[row] = bm;
+    fwd[row] = fm;
+    orders[row] = (uint8_t)u;
+}
+
+// Per query: the best-scored candidate, kept only if it beats the RI prediction (candidate
+// 0) by more than `margin`; its order goes to out[i*32 ..].
+__global__ void __launch_bounds__(256) k_cm_choose(const float* __restrict__ y, const uint8_t* __restrict__ orders,
+                                                   uint32_t nq, uint32_t K, float margin, uint32_t* __restrict__ out,
+                                                   float* __restrict__ pbest, float* __restrict__ pri,
+                                                   int32_t* __restrict__ learned)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    const float* yi = y + (size_t)i * K;
+    uint32_t best = 0;
+    for (uint32_t k = 1; k < K; k++)
+        if (yi[k] < yi[best]) best = k;
+    const bool use = yi[best] < yi[0] - margin;
+    const uint32_t pick = use ? best : 0u;
+    for (uint32_t j = 0; j < CM_ROWS; j++) out[(size_t)i * CM_ROWS + j] = orders[((size_t)i * K + pick) * CM_ROWS + j];
+    pbest[i] = yi[pick];
+    pri[i] = yi[0];
+    learned[i] = use ? 1 : 0;
+}
+
 // ------------------------------------------------------------------ launch wrappers
 static inline uint32_t grid_for(size_t work, uint32_t per_block, uint32_t cap)
 {
@@ -346,6 +461,22 @@ void cm_gather(hipStream_t s, const uint32_t* idx, uint32_t S, const uint16_t* F
 {
     hipLaunchKernelGGL(k_cm_gather, dim3(grid_for((size_t)S * CM_ROWS * CM_F, 256, 8192)), dim3(256), 0, s, idx, S, F,
                        valid, back, fwd, target, oF, ovalid, oback, ofwd, otarget);
+}
+
+void cm_rank_inputs(hipStream_t s, const sgm_qdesc* qs, const uint32_t* cnt, const unsigned long long* degsum,
+                    const uint8_t* given, uint32_t nq, uint32_t K, uint64_t seed, uint8_t* orders, uint16_t* F,
+                    uint8_t* valid, uint32_t* back, uint32_t* fwd)
+{
+    const uint32_t S = nq * K;
+    hipLaunchKernelGGL(k_cm_rank_inputs, dim3((S + 3) / 4), dim3(256), 0, s, qs, cnt, degsum, given, nq, K, seed,
+                       orders, F, valid, back, fwd);
+}
+
+void cm_choose(hipStream_t s, const float* y, const uint8_t* orders, uint32_t nq, uint32_t K, float margin,
+               uint32_t* out, float* pbest, float* pri, int32_t* learned)
+{
+    hipLaunchKernelGGL(k_cm_choose, dim3((nq + 255) / 256), dim3(256), 0, s, y, orders, nq, K, margin, out, pbest,
+                       pri, learned);
 }
 
 void cm_zero(hipStream_t s, float* p, size_t n)

@@ -215,6 +215,35 @@ double bwd_flops(uint32_t S) { return 2.0 * S * CM_ROWS * ((double)CM_F * CM_H +
 
 }  // namespace
 
+// Device-built inputs (the batch runner's ranking pass): the model's work buffers for S
+// samples, then a forward pass over them in place.
+int costmodel_work_inputs(sgm_costmodel* m, uint32_t S, uint16_t** F, uint8_t** valid, uint32_t** back, uint32_t** fwd)
+{
+    const int rc = alloc_bufs(m->c, m->work, std::max(S, 256u), true);
+    if (rc) return rc;
+    *F = m->work.F;
+    *valid = m->work.valid;
+    *back = m->work.back;
+    *fwd = m->work.fwd;
+    return SGM_OK;
+}
+
+int costmodel_forward_work(sgm_costmodel* m, uint32_t S, const float** y)
+{
+    sgm_ctx* c = m->c;
+    HIPCHK(c, hipEventRecord(m->ev[0], c->stream));
+    forward(m, m->work, S, false);
+    HIPCHK(c, hipEventRecord(m->ev[1], c->stream));
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventSynchronize(m->ev[1]));
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, m->ev[0], m->ev[1]));
+    m->infer_ms += ms;
+    m->infer_flops += fwd_flops(S);
+    *y = m->work.y;
+    return SGM_OK;
+}
+
 extern "C" {
 
 uint32_t sgm_costmodel_param_count(void) { return NPARAM; }

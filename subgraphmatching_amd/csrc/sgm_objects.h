@@ -165,6 +165,9 @@ inline int grow(sgm_ctx* c, T** p, uint32_t& cap_now, uint32_t need)
 }
 
 // Shared by the enumeration drivers (sgm_capi.cpp, sgm_batch.cpp).
+// Cost-model work buffers filled on the device, and a forward pass over them (ranking).
+int costmodel_work_inputs(sgm_costmodel* m, uint32_t S, uint16_t** F, uint8_t** valid, uint32_t** back, uint32_t** fwd);
+int costmodel_forward_work(sgm_costmodel* m, uint32_t S, const float** y);
 // Slot map of the hub bitmaps to use for opts (nullptr = binary search everywhere).
 int hub_slot_map(sgm_graph* g, const sgm_enum_opts& o, const uint32_t** out);
 int ensure_stack(sgm_ctx* c, size_t bytes);

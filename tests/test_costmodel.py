@@ -169,3 +169,28 @@ def test_training_lowers_loss():
     assert losses[-1] < 0.25 * losses[0]
     t = m.timing()
     assert t["train_ms"] > 0 and t["train_flops"] > 0
+
+
+@pytest.mark.gpu
+def test_device_ranking_matches_host_features_and_keeps_counts():
+    import subgraphmatching_amd as sgm
+    from subgraphmatching_amd.generators import chung_lu, query_set
+    g = chung_lu(3000, 2.3, 8.0, seed=13)
+    qs = query_set(g, 24, lambda i, r: int(r.integers(4, 20)), lambda i, r: bool(i % 2), seed=131)
+    runner = sgm.Batch(g, slots=24)
+    ok, cnt, dsum = runner.filter(qs)
+    m = CostModel(seed=31)
+    orders, pbest, pri, used = runner.rank_orders(m, n_samples=16, seed=7, margin=-100.0)  # always take the best
+    ri_items = [(q, sgm.Order_RI(q), cnt[i], dsum[i]) for i, q in enumerate(qs)]
+    host = m.predict(ri_items)
+    print("max |device - host| RI score", float(np.abs(host - pri).max()), flush=True)
+    assert np.allclose(pri, host, atol=2e-2, rtol=1e-2)  # device features = host features
+    assert np.all(pbest <= pri + 1e-6)
+    for q, o in zip(qs, orders):
+        assert sorted(o) == list(range(q.getVertexCnt())) and _connected(q, o)
+    ri_res = runner.enumerate(cap=50000)
+    learned_res = runner.enumerate(orders=orders, cap=50000)
+    assert [r.ans for r in learned_res] == [r.ans for r in ri_res]
+    # with a prohibitive margin every query keeps RI
+    orders2, _, _, used2 = runner.rank_orders(m, n_samples=16, seed=7, margin=1e9)
+    assert not used2.any() and orders2 == [sgm.Order_RI(q) for q in qs]
