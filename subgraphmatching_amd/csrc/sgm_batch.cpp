@@ -28,7 +28,7 @@ struct sgm_batch {
     uint32_t slots = 0;
     uint32_t* d_vmask = nullptr;             // slots * V
     uint32_t* d_flags = nullptr;             // slots * SGM_GQL_FLAG_WORDS
-    uint32_t* d_scratch = nullptr;           // slots * 3V: GQL rank-space membership, touch / dirty marks
+    uint32_t* d_scratch = nullptr;           // slots * sgm_gql_scratch_words(V): GQL scratch
     sgm_qdesc* d_qd = nullptr;               // slots
     uint32_t* d_cnt = nullptr;               // slots * 32
     unsigned long long* d_degsum = nullptr;  // slots * 32
@@ -171,7 +171,7 @@ int sgm_batch_create(sgm_graph* g, uint32_t slots, sgm_batch** out)
     const size_t V = g->V;
     hipError_t e = dalloc(&b->d_vmask, slots * V);
     if (e == hipSuccess) e = dalloc(&b->d_flags, (size_t)slots * SGM_GQL_FLAG_WORDS);
-    if (e == hipSuccess) e = dalloc(&b->d_scratch, 3 * slots * V);
+    if (e == hipSuccess) e = dalloc(&b->d_scratch, sgm_gql_scratch_words(g->V) * slots);
     if (e == hipSuccess) e = dalloc(&b->d_qd, slots);
     if (e == hipSuccess) e = dalloc(&b->d_cnt, (size_t)slots * SGM_MAXQ);
     if (e == hipSuccess) e = dalloc(&b->d_degsum, (size_t)slots * SGM_MAXQ);
@@ -195,9 +195,14 @@ int sgm_batch_create(sgm_graph* g, uint32_t slots, sgm_batch** out)
         std::memset(&d, 0, sizeof d);
         d.vmask = b->d_vmask + i * V;
         d.removed = b->d_flags + (size_t)i * SGM_GQL_FLAG_WORDS;
-        d.vmr = b->d_scratch + 3 * i * V;
+        const size_t G = sgm_gql_groups(g->V);
+        d.vmr = b->d_scratch + sgm_gql_scratch_words(g->V) * i;
         d.touch = d.vmr + V;
         d.dirty = d.touch + V;
+        d.tsum = d.dirty + V;
+        d.dsum = d.tsum + G;
+        d.f0 = d.dsum + G;
+        d.fsum = d.f0 + V;
     }
     b->counts.assign((size_t)slots * SGM_MAXQ, 0);
     b->degsum.assign((size_t)slots * SGM_MAXQ, 0);
@@ -245,14 +250,16 @@ int sgm_batch_filter(sgm_batch* b, uint32_t nq, const uint32_t* const* qoffs, co
     if (!*gexec) {
         const int rc = capture(c, [&] {
             sgm::launch_fill_u32(c->stream, b->d_flags, (size_t)slots * SGM_GQL_FLAG_WORDS, 0);
-            if (kind == SGM_FILTER_GQL) sgm::launch_fill_u32(c->stream, b->d_scratch, 3 * (size_t)slots * V, 0);
+            if (kind == SGM_FILTER_GQL)  // touch and dirty words of every slot
+                sgm::launch_fill_strided(c->stream, b->d_scratch + V, sgm_gql_clear_words(V),
+                                         sgm_gql_scratch_words(V), slots, 0);
             sgm::launch_fill_u32(c->stream, b->d_cnt, (size_t)slots * SGM_MAXQ, 0);
             sgm::launch_fill_u32(c->stream, reinterpret_cast<uint32_t*>(b->d_degsum), 2 * (size_t)slots * SGM_MAXQ, 0);
             if (kind == SGM_FILTER_GQL && V) {
                 sgm::launch_nlf_rank(c->stream, g->d_gecnt, V, b->d_qd, slots);
-                for (uint32_t ph = 0; ph < n_phases; ph++)  // grid sized for the largest prefix
-                    sgm::launch_gql_phase(c->stream, g->d_roff, g->d_radj, g->d_dtab, V, g->d_gecnt, b->d_qd, slots,
-                                          ph, g->ge_cnt[1]);
+                sgm::launch_gql_prepass(c->stream, g->d_dtab, V, g->d_gecnt, b->d_qd, slots);
+                for (uint32_t ph = 0; ph < n_phases; ph++)
+                    sgm::launch_gql_phase(c->stream, g->d_roff, g->d_radj, V, g->d_gecnt, b->d_qd, slots, ph);
                 sgm::launch_unrank(c->stream, g->d_rank, V, b->d_qd, slots);
             } else {
                 sgm::launch_nlf(c->stream, g->d_off, V, b->d_qd, slots);

@@ -331,9 +331,14 @@ static int filter_chunk(sgm_ctx* c, sgm_graph* g, sgm_query* const* qs, uint32_t
         qd[i] = qs[i]->hq;
         cd[i] = qs[i]->hc;
         if (kind == SGM_FILTER_GQL) {
-            qd[i].vmr = c->d_gqls + (size_t)3 * V * i;
+            const size_t G = sgm_gql_groups(V);
+            qd[i].vmr = c->d_gqls + sgm_gql_scratch_words(V) * i;
             qd[i].touch = qd[i].vmr + V;
             qd[i].dirty = qd[i].touch + V;
+            qd[i].tsum = qd[i].dirty + V;
+            qd[i].dsum = qd[i].tsum + G;
+            qd[i].f0 = qd[i].dsum + G;
+            qd[i].fsum = qd[i].f0 + V;
         }
         max_n = std::max(max_n, qs[i]->q.n);
         free_tables(qs[i]);
@@ -342,8 +347,8 @@ static int filter_chunk(sgm_ctx* c, sgm_graph* g, sgm_query* const* qs, uint32_t
         qs[i]->compacted = false;
         HIPCHK(c, hipMemsetAsync(qs[i]->d_removed, 0, sizeof(uint32_t) * SGM_GQL_FLAG_WORDS, c->stream));
     }
-    if (kind == SGM_FILTER_GQL && V)
-        HIPCHK(c, hipMemsetAsync(c->d_gqls, 0, sizeof(uint32_t) * 3 * (size_t)V * nq, c->stream));
+    if (kind == SGM_FILTER_GQL && V)  // touch and dirty words start clear; the rest is written before it is read
+        sgm::launch_fill_strided(c->stream, c->d_gqls + V, sgm_gql_clear_words(V), sgm_gql_scratch_words(V), nq, 0);
     sgm_qdesc* d_qd = nullptr;
     sgm_cdesc* d_cd = nullptr;
     HIPCHK(c, dalloc(&d_qd, nq));
@@ -355,14 +360,9 @@ static int filter_chunk(sgm_ctx* c, sgm_graph* g, sgm_query* const* qs, uint32_t
     else sgm::launch_nlf(c->stream, g->d_off, V, d_qd, nq);
     HIPCHK(c, hipEventRecord(c->fev[1], c->stream));
     if (kind == SGM_FILTER_GQL && V) {
-        for (uint32_t ph = 0; ph < 2 * max_n; ph++) {
-            uint32_t mp = 0;  // the largest candidate prefix of this phase
-            for (uint32_t i = 0; i < nq; i++) {
-                const uint32_t n = qs[i]->q.n;
-                if (ph < 2 * n) mp = std::max(mp, g->ge_cnt[std::min<uint32_t>(qd[i].qdeg[ph % n], 32)]);
-            }
-            if (mp) sgm::launch_gql_phase(c->stream, g->d_roff, g->d_radj, g->d_dtab, V, g->d_gecnt, d_qd, nq, ph, mp);
-        }
+        sgm::launch_gql_prepass(c->stream, g->d_dtab, V, g->d_gecnt, d_qd, nq);
+        for (uint32_t ph = 0; ph < 2 * max_n; ph++)
+            sgm::launch_gql_phase(c->stream, g->d_roff, g->d_radj, V, g->d_gecnt, d_qd, nq, ph);
         sgm::launch_unrank(c->stream, g->d_rank, V, d_qd, nq);
     }
     HIPCHK(c, hipEventRecord(c->fev[2], c->stream));
@@ -394,11 +394,11 @@ int sgm_filter_batch(sgm_query* const* qs, uint32_t nq, int kind, int32_t* statu
     for (uint32_t i = 0; i < nq; i++)
         if (!qs[i] || qs[i]->g != g) return set_err(c, SGM_EINVAL, "queries of a batch must share one graph");
     HIPCHK(c, hipSetDevice(c->device));
-    const uint64_t per_q = (uint64_t)3 * g->V * sizeof(uint32_t);
+    const uint64_t per_q = (uint64_t)sgm_gql_scratch_words(g->V) * sizeof(uint32_t);
     uint32_t chunk = nq;
     if (kind == SGM_FILTER_GQL && per_q) {
         chunk = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(nq, kGqlScratchBytes / per_q));
-        const size_t words = (size_t)3 * g->V * chunk;
+        const size_t words = sgm_gql_scratch_words(g->V) * chunk;
         if (c->gqls_words < words) {
             dfree(c->d_gqls);
             c->d_gqls = nullptr;

@@ -82,6 +82,13 @@ struct sgm_equery {
 #define SGM_GQL_DIRTY_ANY (2 * SGM_MAXQ + 1)
 #define SGM_GQL_WORK (2 * SGM_MAXQ + 2)
 #define SGM_GQL_FLAG_WORDS (2 * SGM_MAXQ + 6)
+// GQL scratch per query, in words, by rank (G = SGM_GQL_GROUP ranks per summary word):
+//   vmr[V] | touch[V] | dirty[V] | tsum[V/G] | dsum[V/G] | f0[V] | fsum[V/G]
+// (touch .. dsum start clear; the rest is written before it is read)
+#define SGM_GQL_GROUP 8u
+inline size_t sgm_gql_groups(uint32_t V) { return ((size_t)V + SGM_GQL_GROUP - 1) / SGM_GQL_GROUP; }
+inline size_t sgm_gql_scratch_words(uint32_t V) { return 4 * (size_t)V + 3 * sgm_gql_groups(V); }
+inline size_t sgm_gql_clear_words(uint32_t V) { return 2 * (size_t)V + 2 * sgm_gql_groups(V); }
 // GQL runs in degree-rank space (rank r = position in the degree-sorted vertex list), where
 // the candidates of every NLF threshold are a prefix; `vmr` holds the rank-space membership
 // words and is copied back to vmask (vertex ids) after the last phase.
@@ -95,4 +102,8 @@ struct sgm_qdesc {
     uint32_t* vmr;                   // V words of GQL scratch: membership by degree rank
     uint32_t* touch;                 // V words of GQL scratch (by rank): a neighbour lost a bit of N_q(u)
     uint32_t* dirty;                 // V words of GQL scratch (by rank): recheck (round 2, u)
+    uint32_t* f0;                    // V words (by rank): bit u = fails the NLF table test of u
+    uint32_t* fsum;                  // per group of SGM_GQL_GROUP ranks: OR of f0
+    uint32_t* tsum;                  // per group: OR of touch
+    uint32_t* dsum;                  // per group: OR of dirty
 };

@@ -20,10 +20,14 @@ struct sgm_cdesc {
 
 namespace sgm {
 void launch_nlf(hipStream_t s, const uint32_t* off, uint32_t V, const sgm_qdesc* qs, uint32_t nq);
-// GQL in degree-rank space (see k_gql_phase); max_prefix = the largest candidate prefix of
-// the batch in this phase (sizes the grid)
-void launch_gql_phase(hipStream_t s, const uint32_t* roff, const uint32_t* radj, const uint64_t* dtab, uint32_t V,
-                      const uint32_t* ge_cnt, const sgm_qdesc* qs, uint32_t nq, uint32_t phase, uint32_t max_prefix);
+// GQL in degree-rank space (see k_gql_phase): the pre-pass once per filter, then one
+// launch per phase (round * n + u) for the whole batch
+void launch_gql_prepass(hipStream_t s, const uint64_t* dtab, uint32_t V, const uint32_t* ge_cnt, const sgm_qdesc* qs,
+                        uint32_t nq);
+void launch_gql_phase(hipStream_t s, const uint32_t* roff, const uint32_t* radj, uint32_t V, const uint32_t* ge_cnt,
+                      const sgm_qdesc* qs, uint32_t nq, uint32_t phase);
+// p[rep * stride + i] = val for rep < reps, i < len
+void launch_fill_strided(hipStream_t s, uint32_t* p, size_t len, size_t stride, uint32_t reps, uint32_t val);
 void launch_nlf_rank(hipStream_t s, const uint32_t* ge_cnt, uint32_t V, const sgm_qdesc* qs, uint32_t nq);
 void launch_unrank(hipStream_t s, const uint32_t* rank, uint32_t V, const sgm_qdesc* qs, uint32_t nq);
 void launch_rank_adj(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* by_deg,

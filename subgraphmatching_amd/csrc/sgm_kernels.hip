@@ -113,30 +113,30 @@ __device__ bool lsm_wave(const uint32_t* __restrict__ nb, uint32_t deg, const ui
 // changes during the phase and u is not its own neighbour, so the checks of one phase are
 // independent (the reference's sequential order inside a phase does not matter).
 //
-// The phase works in degree-rank space: rank r is the r-th vertex of the degree-sorted
+// The filter works in degree-rank space: rank r is the r-th vertex of the degree-sorted
 // list, the rank-space adjacency lists hold ranks, and the NLF candidates of u are exactly
-// the ranks below ge_cnt[deg_q(u)] -- a contiguous prefix, streamed with coalesced loads.
+// the ranks below P_u = ge_cnt[deg_q(u)].
 //
 // Exact shortcuts (the answers are the reference's, only fewer checks run):
-//   * Before phase (round 1, u) bit u of every vertex still holds its NLF value, so every
-//     rank of the prefix is a candidate and needs no membership read.
-//   * When no neighbour w of v has lost a bit u' of N_q(u) (touch bit u clear), every bit
-//     the check reads is an NLF bit: left vertex u' is adjacent to w iff deg(w) >=
-//     deg_q(u').  These neighbourhoods are nested, so Hall's condition reduces to "the j-th
-//     largest neighbour degree of v is at least the j-th largest requirement" for every j:
-//     eight byte comparisons per 64-bit word of the degree table (SWAR, values <= 31).
+//   * With only NLF bits, left vertex u' is adjacent to w iff deg(w) >= deg_q(u'): nested
+//     neighbourhoods, so Hall's condition is "the j-th largest neighbour degree of v is at
+//     least the j-th largest requirement" for every j -- eight byte comparisons per 64-bit
+//     word of the degree table (SWAR, values <= 31).  The pre-pass evaluates this test once
+//     per rank for every u (f0 bit u = fails).
+//   * Removals only take edges away, so a rank failing the NLF test fails every later check
+//     too: phase (1, u) removes exactly the ranks with f0 bit u, plus those a
+//     matcher rejects among the ranks whose neighbours lost a bit of N_q(u) (touch bit u).
+//     Every other candidate passes without any work.
 //   * Round 2 rechecks v only if a neighbour lost a bit of N_q(u) after v's round-1 check
-//     of u (dirty bit u): otherwise every input of the check is unchanged and v passes
-//     again.  A removal from C(u') in phase (1, u') therefore marks the neighbours dirty for
-//     u in N_q(u') with u < u', and in phase (2, u') for u > u'.
-//
-// Work units: GQL_VPL x 64 ranks per wave step, GQL_VPL per lane with all loads issued
-// before any decision.  Lanes decide their vertices themselves where they can -- the
-// degree table, or, over at most GQL_LANE_DEG neighbours, a greedy matching that saturates
-// the left side (true), a left vertex no right vertex covers, or fewer useful right
-// vertices than left vertices (false, only when the whole list was seen) -- and the wave
-// runs the exact matcher on the few undecided ones and marks the neighbours of removed
-// high-degree vertices together.
+//     of u (dirty bit u, set by removals in (1, u') for u < u' and in (2, u') for u > u');
+//     otherwise every input is unchanged and v passes again.
+// Summary words (OR over groups of SGM_GQL_GROUP ranks) of the failures, touch and dirty
+// bits let a phase stream half a byte per rank and open only the groups that hold work.
+// Matchers: lsm_lane (one lane, <= GQL_LANE_DEG neighbours: a greedy matching that
+// saturates the left side proves true; a left vertex no right vertex covers, or fewer
+// useful right vertices than left vertices, proves false when the whole list was seen),
+// then lsm_wave for the undecided; matcher candidates are queued per wave and taken 64 at
+// a time, one per lane.
 #define GQL_LANE_DEG 128u
 #define GQL_VPL 4u
 #define GQL_QCAP 512u  // per-wave matcher queue: < 64 left over + 64 * GQL_VPL new per unit
@@ -168,15 +168,6 @@ __device__ __forceinline__ int lsm_lane(const uint32_t* __restrict__ adj, uint32
     return -1;
 }
 
-// Record that w's neighbour lost a bit: touch bits tm, dirty bits dm (read first, so words
-// that already hold the bits cost no atomic; bits are only ever added during a filter).
-__device__ __forceinline__ void gql_mark(uint32_t* __restrict__ touch, uint32_t* __restrict__ dirty, uint32_t w,
-                                         uint32_t tm, uint32_t dm)
-{
-    if (tm && (touch[w] & tm) != tm) atomicOr(&touch[w], tm);
-    if (dm && (dirty[w] & dm) != dm) atomicOr(&dirty[w], dm);
-}
-
 // Hall's test on the degree table: byte j of D (the (j+1)-th largest capped neighbour
 // degree) >= byte j of R (the (j+1)-th largest requirement, 0 past the left side) for all
 // eight bytes.  Bytes are <= 31, so (D | 0x80..) - R never borrows across bytes.
@@ -186,9 +177,102 @@ __device__ __forceinline__ bool hall_word(uint64_t D, uint64_t R)
     return (((D | H) - R) & H) == H;
 }
 
+// Removals of the lanes' ranks x with rm set: their neighbours get touch bits tm and dirty
+// bits dm, and the summary words of their groups the same bits (each word read first: bits
+// are only ever added during a filter, so a word that already holds them needs no atomic).
+// The removed lanes' lists are concatenated (wave prefix sum of their lengths) and the wave
+// walks the concatenation 64 neighbours at a time, so many short lists share a step and a
+// long one is split over all lanes.  Called by the whole wave.
+__device__ void gql_mark_nbrs(const sgm_qdesc* __restrict__ d, const uint32_t* __restrict__ roff,
+                              const uint32_t* __restrict__ radj, bool rm, uint32_t x, uint32_t tm, uint32_t dm)
+{
+    const uint32_t o = rm ? roff[x] : 0u;
+    const uint32_t dv = rm ? roff[x + 1] - o : 0u;
+    const uint32_t incl = wave_incl_scan(dv);
+    const uint32_t total = rdlane(incl, 63);
+    const uint32_t lane = lane_id();
+    for (uint32_t b = 0; b < total; b += 64) {
+        const uint32_t t = b + lane;
+        // owner lane j: the number of lanes whose inclusive end is <= t (ends are sorted)
+        uint32_t j = 0;
+#pragma unroll
+        for (uint32_t s = 32; s > 0; s >>= 1) {
+            const uint32_t e = (uint32_t)__shfl((int)incl, (int)(j + s - 1));
+            if (e <= t) j += s;
+        }
+        j = j > 63u ? 63u : j;
+        const uint32_t oj = (uint32_t)__shfl((int)o, (int)j);
+        const uint32_t sj = (uint32_t)__shfl((int)(incl - dv), (int)j);
+        if (t >= total) continue;
+        const uint32_t w = radj[oj + (t - sj)], gw = w / SGM_GQL_GROUP;
+        if (tm && (d->touch[w] & tm) != tm) {
+            atomicOr(&d->touch[w], tm);
+            if ((d->tsum[gw] & tm) != tm) atomicOr(&d->tsum[gw], tm);
+        }
+        if (dm && (d->dirty[w] & dm) != dm) {
+            atomicOr(&d->dirty[w], dm);
+            if ((d->dsum[gw] & dm) != dm) atomicOr(&d->dsum[gw], dm);
+        }
+    }
+}
+
+// Pre-pass: f0[x] = bits u (x < P_u, N_q(u) non-empty) whose NLF table test rank x fails, and
+// fsum[g] = OR of f0 over group g.
+__global__ void __launch_bounds__(256) k_gql_prepass(const uint64_t* __restrict__ dtab, uint32_t V,
+                                                     const uint32_t* __restrict__ ge_cnt,
+                                                     const sgm_qdesc* __restrict__ qs)
+{
+    const sgm_qdesc* d = qs + blockIdx.y;
+    const uint32_t n = d->n;
+    if (n == 0) return;
+    uint32_t* __restrict__ flags = d->removed;
+    // the query's thresholds and requirement words, staged once per block
+    __shared__ uint32_t s_P[SGM_MAXQ];
+    __shared__ uint64_t s_req[SGM_MAXQ][4];
+    if (threadIdx.x < n) {
+        const uint32_t u = threadIdx.x;
+        s_P[u] = d->nbr[u] ? ge_cnt[d->qdeg[u]] : 0u;  // 0: no test for an empty left side
+#pragma unroll
+        for (uint32_t p = 0; p < 4; p++) s_req[u][p] = d->req[u][p];
+    }
+    __syncthreads();
+    uint32_t Pmax = 0, kmax = 0;
+    for (uint32_t u = 0; u < n; u++) {
+        Pmax = s_P[u] > Pmax ? s_P[u] : Pmax;
+        const uint32_t k = __popc(d->nbr[u]);
+        kmax = k > kmax ? k : kmax;
+    }
+    const uint32_t np = (kmax + 7) / 8;
+    const uint32_t lane = lane_id();
+    uint32_t tests = 0;
+    const uint32_t span = (Pmax + 63) & ~63u;  // whole waves (and whole groups) iterate together
+    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < span; x += gridDim.x * blockDim.x) {
+        const bool in = x < Pmax;
+        uint64_t D[4];
+#pragma unroll
+        for (uint32_t p = 0; p < 4; p++) D[p] = (in && p < np) ? dtab[(size_t)p * V + x] : 0ull;
+        uint32_t f = 0;
+        for (uint32_t u = 0; u < n; u++) {
+            if (x >= s_P[u]) continue;
+            bool ok = true;
+#pragma unroll
+            for (uint32_t p = 0; p < 4; p++) ok = ok && (p >= np || hall_word(D[p], s_req[u][p]));
+            f |= ok ? 0u : (1u << u);
+            tests++;
+        }
+        if (in) d->f0[x] = f;
+        uint32_t g = f;
+#pragma unroll
+        for (int o = 1; o < (int)SGM_GQL_GROUP; o <<= 1) g |= (uint32_t)__shfl_xor((int)g, o);
+        if ((lane % SGM_GQL_GROUP) == 0 && in) d->fsum[x / SGM_GQL_GROUP] = g;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) tests += (uint32_t)__shfl_xor((int)tests, o);
+    if (lane == 0 && tests) atomicAdd(&flags[SGM_GQL_WORK], tests);
+}
+
 __global__ void __launch_bounds__(256) k_gql_phase(const uint32_t* __restrict__ roff,
-                                                   const uint32_t* __restrict__ radj,
-                                                   const uint64_t* __restrict__ dtab, uint32_t V,
+                                                   const uint32_t* __restrict__ radj, uint32_t V,
                                                    const uint32_t* __restrict__ ge_cnt,
                                                    const sgm_qdesc* __restrict__ qs, uint32_t phase)
 {
@@ -199,40 +283,35 @@ __global__ void __launch_bounds__(256) k_gql_phase(const uint32_t* __restrict__ 
     const uint32_t L = d->nbr[u];
     if (L == 0) return;  // empty left side: the matching trivially saturates it
     uint32_t* __restrict__ flags = d->removed;
-    bool use_touch = false;
-    if (r == 1) {
-        if (!((flags[SGM_GQL_DIRTY_ANY] >> u) & 1u)) return;  // no input of any check changed
-    } else {
-        use_touch = (flags[SGM_GQL_TOUCH_ANY] >> u) & 1u;
-    }
+    if (r == 1 && !((flags[SGM_GQL_DIRTY_ANY] >> u) & 1u)) return;  // no input of any check changed
+    const bool use_touch = r == 0 && ((flags[SGM_GQL_TOUCH_ANY] >> u) & 1u);
     const uint32_t low = L & ((1u << u) - 1u);
     const uint32_t high = u == 31u ? 0u : (L & (~0u << (u + 1u)));
     const uint32_t tm = r == 0 ? high : 0u;  // touch bits only matter to later round-1 phases
     const uint32_t dm = r == 0 ? low : high;
     const uint32_t k = __popc(L);
-    const uint32_t P = ge_cnt[d->qdeg[u]];  // candidates of u: ranks [0, P)
-    const uint32_t np = (k + 7) / 8;        // table words the test needs
-    uint64_t R[4];
-#pragma unroll
-    for (int p = 0; p < 4; p++) R[p] = d->req[u][p];
+    const uint32_t P = ge_cnt[d->qdeg[u]];  // NLF candidates of u: ranks [0, P)
+    const uint32_t ng = (P + SGM_GQL_GROUP - 1) / SGM_GQL_GROUP;
+    const uint32_t* __restrict__ sumA = r == 0 ? d->fsum : d->dsum;
+    const uint32_t* __restrict__ sumB = use_touch ? d->tsum : nullptr;
+    const uint32_t* __restrict__ word = r == 0 ? d->touch : d->dirty;
     uint32_t* __restrict__ vm = d->vmr;
-    uint32_t* __restrict__ touch = d->touch;
-    uint32_t* __restrict__ dirty = d->dirty;
+    const uint32_t* __restrict__ f0 = d->f0;
     const uint32_t lane = lane_id();
     const uint32_t waves = blockDim.x >> 6;
     const uint32_t w0 = blockIdx.x * waves + (threadIdx.x >> 6);
     const uint32_t stride = gridDim.x * waves;
-    uint32_t local_removed = 0, n_table = 0, n_lane = 0, n_wave = 0;
-    // Candidates that need a matcher (touched in round 1, dirty in round 2) are queued in a
-    // per-wave LDS ring and matched 64 at a time, one per lane, so the few such candidates
-    // do not each stall a whole wave behind one divergent lane.
+    uint32_t local_removed = 0, n_lane = 0, n_wave = 0;
     __shared__ uint32_t s_queue[4][GQL_QCAP];
     uint32_t* __restrict__ wq = s_queue[(threadIdx.x >> 6) & 3u];
     uint32_t qh = 0, qt = 0;  // wave-uniform head / tail (ring index = value mod GQL_QCAP)
-    // wave-cooperative removal marking of a vertex with a long list (lane 0 already cleared its bit)
-    auto mark_all = [&](uint32_t xs) {
-        const uint32_t os = roff[xs], ds = roff[xs + 1] - os;
-        for (uint32_t q = os + lane; q < os + ds; q += 64) gql_mark(touch, dirty, radj[q], tm, dm);
+    // clear bit u of the lanes' ranks with rm set, then mark their neighbours (whole wave)
+    auto remove_now = [&](bool rm, uint32_t x) {
+        if (rm) {
+            atomicAnd(&vm[x], ~(1u << u));
+            local_removed++;
+        }
+        if (__ballot(rm)) gql_mark_nbrs(d, roff, radj, rm, x, tm, dm);
     };
     auto drain = [&](uint32_t cnt) {
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -240,103 +319,58 @@ __global__ void __launch_bounds__(256) k_gql_phase(const uint32_t* __restrict__ 
         const bool has = lane < cnt;
         const uint32_t xv = has ? wq[(qh + lane) & (GQL_QCAP - 1u)] : 0u;
         qh += cnt;
-        uint32_t o = 0, dv = 0;
         int verdict = -1;
         if (has) {
-            o = roff[xv];
-            dv = roff[xv + 1] - o;
+            const uint32_t o = roff[xv], dv = roff[xv + 1] - o;
             verdict = lsm_lane(radj, o, dv, vm, L, k);
             n_lane++;
         }
-        if (has && verdict == 0) {
-            atomicAnd(&vm[xv], ~(1u << u));
-            local_removed++;
-            if (dv <= GQL_LANE_DEG)
-                for (uint32_t q = o; q < o + dv; q++) gql_mark(touch, dirty, radj[q], tm, dm);
-        }
-        // the wave: exact matcher for undecided lanes, shared marking for long lists
-        uint64_t pend = __ballot(has && (verdict < 0 || (verdict == 0 && dv > GQL_LANE_DEG)));
+        bool rm = has && verdict == 0;
+        // the wave: exact matcher for the undecided lanes (only bit u changes in a phase and
+        // no check reads it, so the removals can wait until all checks are done)
+        uint64_t pend = __ballot(has && verdict < 0);
         while (pend) {
             const uint32_t s2 = (uint32_t)__ffsll((long long)pend) - 1u;
             pend &= pend - 1;
             const uint32_t xs = rdlane(xv, s2);
-            bool fail = rdlane((uint32_t)verdict, s2) == 0u;
-            if (!fail) {
-                const uint32_t os = roff[xs], ds = roff[xs + 1] - os;
-                if (lane == 0) n_wave++;
-                if (!lsm_wave(radj + os, ds, vm, L, k)) {
-                    if (lane == 0) { atomicAnd(&vm[xs], ~(1u << u)); local_removed++; }
-                    fail = true;
-                }
-            }
-            if (fail) mark_all(xs);
+            const uint32_t os = roff[xs], ds = roff[xs + 1] - os;
+            if (lane == 0) n_wave++;
+            if (!lsm_wave(radj + os, ds, vm, L, k) && lane == s2) rm = true;
         }
+        remove_now(rm, xv);
     };
-    const uint32_t nunits = (P + 64 * GQL_VPL - 1) / (64 * GQL_VPL);
-    for (uint32_t unit = w0; unit < nunits; unit += stride) {
-        uint32_t x[GQL_VPL], w1[GQL_VPL], w2[GQL_VPL];
-        bool cand[GQL_VPL];
-        uint64_t D[GQL_VPL][4];
-#pragma unroll
-        for (uint32_t t = 0; t < GQL_VPL; t++) {
-            x[t] = unit * 64 * GQL_VPL + t * 64 + lane;
-            cand[t] = x[t] < P;
-        }
-        if (r == 0) {
-#pragma unroll
-            for (uint32_t t = 0; t < GQL_VPL; t++) {
-                w1[t] = (use_touch && cand[t]) ? touch[x[t]] : 0u;
-#pragma unroll
-                for (uint32_t p = 0; p < 4; p++) D[t][p] = (p < np && cand[t]) ? dtab[(size_t)p * V + x[t]] : 0ull;
+    // 64 summary words per wave step; the groups whose word holds bit u are then opened 8 at
+    // a time, 8 lanes per group (coalesced rank segments)
+    const uint32_t sub = lane / SGM_GQL_GROUP;  // which of the 8 selected groups
+    for (uint32_t base = w0 * 64; base < ng; base += stride * 64) {
+        const uint32_t gi = base + lane;
+        const uint32_t sw = gi < ng ? (sumA[gi] | (sumB ? sumB[gi] : 0u)) : 0u;
+        uint64_t hm = __ballot((sw >> u) & 1u);
+        while (hm) {
+            uint32_t mine = SGM_INVALID;
+            for (uint32_t j = 0; j < 64 / SGM_GQL_GROUP && hm; j++) {
+                const uint32_t bit = (uint32_t)__ffsll((long long)hm) - 1u;
+                hm &= hm - 1;
+                if (sub == j) mine = base + bit;
             }
-        } else {
-#pragma unroll
-            for (uint32_t t = 0; t < GQL_VPL; t++) {
-                w1[t] = cand[t] ? dirty[x[t]] : 0u;
-                w2[t] = cand[t] ? vm[x[t]] : 0u;
-            }
-        }
-#pragma unroll
-        for (uint32_t t = 0; t < GQL_VPL; t++) {
-            bool need = false, table_fail = false;
-            if (r == 0) {
-                if (cand[t] && !((w1[t] >> u) & 1u)) {
-                    bool ok = true;
-#pragma unroll
-                    for (uint32_t p = 0; p < 4; p++) ok = ok && (p >= np || hall_word(D[t][p], R[p]));
-                    n_table++;
-                    table_fail = !ok;
-                } else {
-                    need = cand[t];
-                }
-            } else {
-                need = cand[t] && ((w1[t] >> u) & 1u) && ((w2[t] >> u) & 1u);
-            }
-            if (table_fail) {
-                atomicAnd(&vm[x[t]], ~(1u << u));
-                local_removed++;
-                const uint32_t o = roff[x[t]], dv = roff[x[t] + 1] - o;
-                if (dv <= GQL_LANE_DEG)
-                    for (uint32_t q = o; q < o + dv; q++) gql_mark(touch, dirty, radj[q], tm, dm);
-                table_fail = dv > GQL_LANE_DEG;  // a long list is marked by the whole wave below
-            }
-            uint64_t longf = __ballot(table_fail);
-            while (longf) {
-                const uint32_t s2 = (uint32_t)__ffsll((long long)longf) - 1u;
-                longf &= longf - 1;
-                mark_all(rdlane(x[t], s2));
-            }
+            const uint32_t x = mine != SGM_INVALID ? mine * SGM_GQL_GROUP + lane % SGM_GQL_GROUP : SGM_INVALID;
+            const bool c = x < P;
+            const uint32_t fw = (c && r == 0) ? f0[x] : 0u;
+            const uint32_t tw = (c && (r == 1 || use_touch)) ? word[x] : 0u;
+            const uint32_t vw = (c && r == 1) ? vm[x] : 0u;
+            const bool rm = (fw >> u) & 1u;  // fails the NLF table test: fails now too
+            const bool need = !rm && ((tw >> u) & 1u) && (r == 0 || ((vw >> u) & 1u));
+            remove_now(rm, x);
             const uint64_t bal = __ballot(need);
-            if (need) wq[(qt + mbcnt64(bal)) & (GQL_QCAP - 1u)] = x[t];
+            if (need) wq[(qt + mbcnt64(bal)) & (GQL_QCAP - 1u)] = x;
             qt += (uint32_t)__popcll(bal);
+            while (qt - qh >= 64) drain(64);
         }
-        while (qt - qh >= 64) drain(64);
     }
     while (qt != qh) drain(qt - qh < 64 ? qt - qh : 64u);
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) {
         local_removed += (uint32_t)__shfl_xor((int)local_removed, o);
-        n_table += (uint32_t)__shfl_xor((int)n_table, o);
         n_lane += (uint32_t)__shfl_xor((int)n_lane, o);
         n_wave += (uint32_t)__shfl_xor((int)n_wave, o);
     }
@@ -347,7 +381,6 @@ __global__ void __launch_bounds__(256) k_gql_phase(const uint32_t* __restrict__ 
             if (tm) atomicOr(&flags[SGM_GQL_TOUCH_ANY], tm);
             if (dm) atomicOr(&flags[SGM_GQL_DIRTY_ANY], dm);
         }
-        if (n_table) atomicAdd(&flags[SGM_GQL_WORK], n_table);
         if (n_lane) atomicAdd(&flags[SGM_GQL_WORK + 1], n_lane);
         if (n_wave) atomicAdd(&flags[SGM_GQL_WORK + 2], n_wave);
     }
@@ -549,6 +582,14 @@ __global__ void __launch_bounds__(256) k_cand_stats(const uint32_t* __restrict__
     }
 }
 
+// Strided word fill: rep blockIdx.y clears p[rep * stride ... + len) (per-query scratch).
+__global__ void __launch_bounds__(256) k_fill_strided(uint32_t* __restrict__ p, size_t len, size_t stride,
+                                                      uint32_t val)
+{
+    uint32_t* q = p + (size_t)blockIdx.y * stride;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (size_t)gridDim.x * blockDim.x) q[i] = val;
+}
+
 // Word fill with vector stores (graph-capturable, unlike a memset node).
 __global__ void __launch_bounds__(256) k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t val)
 {
@@ -679,16 +720,25 @@ void launch_nlf(hipStream_t s, const uint32_t* off, uint32_t V, const sgm_qdesc*
     hipLaunchKernelGGL(k_nlf, grid, dim3(256), 0, s, off, V, qs);
 }
 
-void launch_gql_phase(hipStream_t s, const uint32_t* roff, const uint32_t* radj, const uint64_t* dtab, uint32_t V,
-                      const uint32_t* ge_cnt, const sgm_qdesc* qs, uint32_t nq, uint32_t phase, uint32_t max_prefix)
+void launch_gql_prepass(hipStream_t s, const uint64_t* dtab, uint32_t V, const uint32_t* ge_cnt, const sgm_qdesc* qs,
+                        uint32_t nq)
 {
-    // about 16k blocks of 4 waves in all (at least 8 per query, at most one per 16 units)
-    const uint64_t units = ((uint64_t)max_prefix + 64 * GQL_VPL - 1) / (64 * GQL_VPL);
-    const uint64_t max_blocks = (units + 15) / 16 ? (units + 15) / 16 : 1;  // >= 4 units per wave
-    uint32_t per_q = clampg(16384 / (nq ? nq : 1), 8, 4096);
-    if (per_q > max_blocks) per_q = (uint32_t)max_blocks;
-    dim3 grid(per_q, nq);
-    hipLaunchKernelGGL(k_gql_phase, grid, dim3(256), 0, s, roff, radj, dtab, V, ge_cnt, qs, phase);
+    dim3 grid(clampg(((uint64_t)V + 1023) / 1024, 1, 16384 / (nq ? nq : 1) + 1), nq);
+    hipLaunchKernelGGL(k_gql_prepass, grid, dim3(256), 0, s, dtab, V, ge_cnt, qs);
+}
+
+void launch_gql_phase(hipStream_t s, const uint32_t* roff, const uint32_t* radj, uint32_t V, const uint32_t* ge_cnt,
+                      const sgm_qdesc* qs, uint32_t nq, uint32_t phase)
+{
+    // the phase visits short lists: a modest number of blocks per query
+    dim3 grid(clampg(8192 / (nq ? nq : 1), 8, 256), nq);
+    hipLaunchKernelGGL(k_gql_phase, grid, dim3(256), 0, s, roff, radj, V, ge_cnt, qs, phase);
+}
+
+void launch_fill_strided(hipStream_t s, uint32_t* p, size_t len, size_t stride, uint32_t reps, uint32_t val)
+{
+    dim3 grid(clampg((len + 255) / 256, 1, 4096), reps);
+    hipLaunchKernelGGL(k_fill_strided, grid, dim3(256), 0, s, p, len, stride, val);
 }
 
 void launch_nlf_rank(hipStream_t s, const uint32_t* ge_cnt, uint32_t V, const sgm_qdesc* qs, uint32_t nq)

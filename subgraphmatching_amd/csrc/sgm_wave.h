@@ -30,6 +30,18 @@ __device__ __forceinline__ uint32_t wave_or32(uint32_t x)
     return x;
 }
 
+// inclusive prefix sum over the 64 lanes (every lane must be active)
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
+{
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = (uint32_t)__shfl_up((int)v, o);
+        if (lane >= (uint32_t)o) v += t;
+    }
+    return v;
+}
+
 __device__ __forceinline__ uint64_t wave_or64(uint64_t x)
 {
     uint32_t lo = wave_or32((uint32_t)x), hi = wave_or32((uint32_t)(x >> 32));

@@ -28,10 +28,12 @@ def poisson_arrivals(count: int, rate_per_s: float, seed: int) -> np.ndarray:
 def admit_slots(num_vertices: int, max_degree: int, mem_budget_bytes: int, waves: int, max_slots: int = 64):
     """Largest slot count whose device state fits the budget together with the DFS stacks.
 
-    Per slot: membership words by id and by rank, GQL touch / dirty marks (4 words per data
-    vertex) plus descriptors; per wave: the set-op stack the enumeration reserves
-    (min(8 * max_degree + 1024, 2^20) words).  Returns (slots, stack_bytes, per_slot_bytes)."""
-    per_slot = 16 * num_vertices + (1 << 14)
+    Per slot: membership words by id plus the GQL scratch (rank-space membership, touch /
+    dirty marks and NLF-test failures, 4 words per data vertex, and three summary words per 8
+    vertices), i.e. 22 bytes per data vertex, plus descriptors; per wave: the set-op stack
+    the enumeration reserves (min(8 * max_degree + 1024, 2^20) words).  Returns (slots,
+    stack_bytes, per_slot_bytes)."""
+    per_slot = 22 * num_vertices + (1 << 14)
     stack = waves * 4 * min(8 * max_degree + 1024, 1 << 20)
     free = mem_budget_bytes - stack
     if free < per_slot:
