@@ -62,6 +62,9 @@ typedef struct {
 /* sgm_enum_opts.flags: intersect against hub adjacency lists by binary search instead of
  * their bitmaps (same results; for measurement) */
 #define SGM_ENUM_F_NO_BITMAPS 1u
+/* sgm_enum_opts.flags: evaluate the last level one extension at a time instead of a chunk of
+ * extensions per pass (same results and statistics; for measurement) */
+#define SGM_ENUM_F_SERIAL_LAST 2u
 
 typedef struct {
     uint64_t count;          /* min(#embeddings, cap) */

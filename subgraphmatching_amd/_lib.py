@@ -53,6 +53,7 @@ EXPORTS = [
 
 
 SGM_ENUM_F_NO_BITMAPS = 1
+SGM_ENUM_F_SERIAL_LAST = 2
 
 
 class EnumOpts(C.Structure):

@@ -15,7 +15,7 @@
 namespace {
 
 struct EnumGraph {
-    uint32_t max_ops, use_fs, blocks, words;
+    uint32_t max_ops, use_fs, blocks, words, serial;
     const uint32_t* hub;
     hipGraphExec_t exec;
 };
@@ -348,6 +348,7 @@ int sgm_batch_enumerate(sgm_batch* b, const uint32_t* const* orders, const sgm_e
     const uint32_t blocks = default_blocks(c, o, live);
     uint64_t words = o.stack_words ? o.stack_words : std::min<uint64_t>((uint64_t)8 * g->max_deg + 1024, 1u << 20);
     const int use_fs = o.method == SGM_ENUM_CACHE_FILTER_FS ? 1 : 0;
+    const int serial = (o.flags & SGM_ENUM_F_SERIAL_LAST) ? 1 : 0;
     const uint32_t* hub = nullptr;
     int rc = hub_slot_map(g, o, &hub);
     if (rc) return rc;
@@ -363,7 +364,8 @@ int sgm_batch_enumerate(sgm_batch* b, const uint32_t* const* orders, const sgm_e
     }
     hipGraphExec_t exec = nullptr;
     for (auto& x : b->ge)
-        if (x.max_ops == bucket && x.use_fs == (uint32_t)use_fs && x.blocks == blocks && x.words == words && x.hub == hub)
+        if (x.max_ops == bucket && x.use_fs == (uint32_t)use_fs && x.blocks == blocks && x.words == words && x.hub == hub &&
+            x.serial == (uint32_t)serial)
             exec = x.exec;
     const size_t st_words = sizeof(sgm_stats_dev) / sizeof(uint32_t) * slots;
     if (!exec) {
@@ -372,10 +374,10 @@ int sgm_batch_enumerate(sgm_batch* b, const uint32_t* const* orders, const sgm_e
             sgm::launch_fill_u32(c->stream, b->d_qstop, 2 * (size_t)slots, 0);
             sgm::launch_fill_u32(c->stream, c->d_stop, 1, 0);
             sgm::launch_enum_cache(c->stream, blocks, g->d_off, g->d_adj, g->d_bydeg, b->d_eq, slots, bucket, b->d_stack,
-                                   (uint32_t)words, c->d_stop, use_fs, hub, g->d_hub_bits, g->hub_words);
+                                   (uint32_t)words, c->d_stop, use_fs, hub, g->d_hub_bits, g->hub_words, serial);
         }, &exec);
         if (rc) return rc;
-        b->ge.push_back(EnumGraph{bucket, (uint32_t)use_fs, blocks, (uint32_t)words, hub, exec});
+        b->ge.push_back(EnumGraph{bucket, (uint32_t)use_fs, blocks, (uint32_t)words, (uint32_t)serial, hub, exec});
     }
     c->stop_requested = 0;
     HIPCHK(c, hipEventRecord(b->ev[2], c->stream));
@@ -419,7 +421,7 @@ int sgm_batch_enumerate(sgm_batch* b, const uint32_t* const* orders, const sgm_e
         HIPCHK(c, hipEventRecord(b->ev[2], c->stream));
         sgm::launch_enum_cache(c->stream, blocks, g->d_off, g->d_adj, g->d_bydeg, b->d_eq, (uint32_t)sub.size(),
                                bucket, c->d_stack, (uint32_t)words, c->d_stop, use_fs, hub, g->d_hub_bits,
-                               g->hub_words);
+                               g->hub_words, serial);
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipEventRecord(b->ev[3], c->stream));
         std::vector<sgm_stats_dev> got2(nq);

@@ -835,6 +835,7 @@ int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uin
     const uint32_t waves = blocks * 4;
     uint64_t words = o.stack_words ? o.stack_words : (uint64_t)8 * g->max_deg + 1024;
     const int use_fs = o.method == SGM_ENUM_CACHE_FILTER_FS ? 1 : 0;
+    const int serial = (o.flags & SGM_ENUM_F_SERIAL_LAST) ? 1 : 0;
     const uint32_t* hub_slot = nullptr;
     if ((rc = hub_slot_map(g, o, &hub_slot))) return rc;
     std::vector<sgm_stats_dev> st(nq);
@@ -866,7 +867,7 @@ int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uin
         c->stop_requested = 0;
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
         sgm::launch_enum_cache(c->stream, blocks, g->d_off, g->d_adj, g->d_bydeg, c->d_eq, m, max_ops, c->d_stack,
-                               (uint32_t)words, c->d_stop, use_fs, hub_slot, g->d_hub_bits, g->hub_words);
+                               (uint32_t)words, c->d_stop, use_fs, hub_slot, g->d_hub_bits, g->hub_words, serial);
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
         std::vector<sgm_stats_dev> got(m);

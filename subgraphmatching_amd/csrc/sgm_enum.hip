@@ -187,7 +187,7 @@ __device__ __forceinline__ void classify_last4(const bool (&valid)[UNR], const u
         const uint64_t cfb = __ballot(cf);
         r.ok += (uint32_t)__popcll(okb);
         if (okb | cfb) r.any = 1;
-        if (cfb) r.conf |= wave_or64(cf ? (anc[L] | anc[c[u]]) : 0ull);
+        if (cfb) r.conf |= lanes_or64(cfb, cf ? (anc[L] | anc[c[u]]) : 0ull);
     }
 }
 
@@ -258,6 +258,20 @@ struct WaveAcc {
     unsigned long long inter, fsp, elems, outs, scanned, items_done, cnt_pending;
 };
 
+// Per-depth node counters: lane d publishes (when out != null) and clears depth d, one LDS
+// address per lane (a loop over the depths would keep one address register per depth).
+__device__ __forceinline__ void clear_nodes(WaveDFS& S, unsigned long long* out, uint32_t n)
+{
+    const uint32_t lane = lane_id();
+    if (lane < SGM_MAXQ) {
+        if (out && lane < n && S.nodes[lane]) atomicAdd(out + lane, S.nodes[lane]);
+        S.nodes[lane] = 0;
+    }
+    // later uniform reads of any depth's counter see the clears
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 __device__ void flush_stats(const sgm_equery& Q, WaveDFS& S, WaveAcc& a, uint32_t n, bool overflow)
 {
     flush_count(a.cnt_pending, &Q.st->count, Q.stop, Q.cap);
@@ -269,11 +283,275 @@ __device__ void flush_stats(const sgm_equery& Q, WaveDFS& S, WaveAcc& a, uint32_
         if (a.scanned) atomicAdd(&Q.st->elems_scanned, a.scanned);
         if (a.items_done) atomicAdd(&Q.st->items_done, a.items_done);
         if (overflow) { atomicAdd(&Q.st->overflow, 1ull); atomicExch(Q.stop, 1u); }
-        for (uint32_t d = 0; d < n; d++)
-            if (S.nodes[d]) atomicAdd(&Q.st->nodes[d], S.nodes[d]);
     }
     a = WaveAcc{0, 0, 0, 0, 0, 0, 0};
-    for (uint32_t d = 0; d < SGM_MAXQ; d++) S.nodes[d] = 0;
+    clear_nodes(S, &Q.st->nodes[0], n);
+}
+
+// ---- last level for all remaining siblings of a chunk at once
+//
+// At depth d = n-2 every extension v of the current chunk (the set bits of S.rem[d]) ends
+// in one last-level evaluation over the candidates w of u_L: (parent op) ∩ N(v) when the
+// last-level op has a parent, N(v) when it has none, or a list cached at an earlier depth
+// when depth d computes no op.  One sibling at a time, each evaluation is a chain of
+// dependent loads (keys, membership, filter word) over a few dozen elements.  Here the
+// siblings' (v, element) pairs are concatenated -- wave prefix sum of the per-sibling
+// lengths, each the shorter list of its intersection -- and 64 * UNR pairs are evaluated
+// per step; per-sibling sums accumulate in the sibling's own lane (lane i = i-th sibling).
+// A cached list is evaluated once for the chunk: sibling v then differs only at w == v
+// (an extension becomes a conflict at depth d).  Results are applied in sibling order with
+// the same failing-set cut-off as one at a time, so the search, its counts and its
+// statistics are unchanged; siblings after a cut-off were evaluated for nothing.
+
+// Membership of K keys, each in its own sorted list b[u] (length nb[u] > 0) or bitmap.
+// Branch-free search for the last element <= key: every step issues the K loads together
+// (lanes with nothing to search read safe[0]); the wave steps until its longest list is done.
+template <int K>
+__device__ __forceinline__ void member4v(const uint32_t* const (&b)[K], const uint32_t (&nb)[K],
+                                         const uint32_t* const (&bits)[K], const uint32_t (&key)[K],
+                                         const bool (&valid)[K], bool (&hit)[K], const uint32_t* safe)
+{
+    uint32_t p[K], n[K];
+    const uint32_t* bb[K];
+    bool more = false;
+#pragma unroll
+    for (int u = 0; u < K; u++) {
+        const bool srch = valid[u] && !bits[u];
+        p[u] = 0;
+        n[u] = srch ? nb[u] : 1u;
+        bb[u] = srch ? b[u] : safe;
+        more |= n[u] > 1;
+    }
+    while (__ballot(more)) {
+        more = false;
+        uint32_t x[K], half[K];
+#pragma unroll
+        for (int u = 0; u < K; u++) {
+            half[u] = n[u] >> 1;
+            x[u] = bb[u][p[u] + half[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < K; u++) {
+            p[u] = x[u] <= key[u] ? p[u] + half[u] : p[u];
+            n[u] -= half[u];
+            more |= n[u] > 1;
+        }
+    }
+    // one load per key, bitmap word or list element, all issued before the first use
+    uint32_t wd[K];
+#pragma unroll
+    for (int u = 0; u < K; u++) wd[u] = bits[u] ? bits[u][key[u] >> 5] : bb[u][p[u]];
+#pragma unroll
+    for (int u = 0; u < K; u++) {
+        const bool e = bits[u] ? ((wd[u] >> (key[u] & 31u)) & 1u) != 0u : wd[u] == key[u];
+        hit[u] = valid[u] && e;
+    }
+}
+
+#define LAST_PU 2  // 4 gives more loads in flight per wave but costs a wave per SIMD
+
+struct SibAcc {
+    uint32_t ok, size, any;
+    uint64_t conf;
+};
+
+// Adds one step's pair results to the owner lanes: the pairs of one owner occupy
+// consecutive lanes (owners ascend with the pair index); `head` marks the first valid lane
+// of each owner's run (lane 0, or the owner's first pair).
+__device__ __forceinline__ void sib_accumulate(SibAcc& r, bool head, uint32_t own, bool hit, bool filt,
+                                               uint32_t c, const uint64_t* anc, uint32_t L)
+{
+    const uint32_t lane = lane_id();
+    const bool ok = filt && c == SGM_INVALID, cf = filt && c != SGM_INVALID;
+    const uint64_t H = __ballot(hit), O = __ballot(ok), F = __ballot(filt);
+    uint64_t CF = __ballot(cf);
+    uint64_t hd = __ballot(head);
+    if (H) {
+        while (hd) {
+            const uint32_t h = (uint32_t)__ffsll((long long)hd) - 1u;
+            hd &= hd - 1;
+            const uint32_t e = hd ? (uint32_t)__ffsll((long long)hd) - 1u : 64u;
+            const uint64_t seg = (e == 64u ? ~0ull : ((1ull << e) - 1ull)) & ~((1ull << h) - 1ull);
+            if (!(H & seg)) continue;
+            const uint32_t j = rdlane(own, h);
+            const uint32_t cs = (uint32_t)__popcll(H & seg), co = (uint32_t)__popcll(O & seg);
+            const uint32_t ca = (F & seg) ? 1u : 0u;
+            if (lane == j) { r.size += cs; r.ok += co; r.any |= ca; }
+        }
+    }
+    while (CF) {
+        const uint32_t l = (uint32_t)__ffsll((long long)CF) - 1u;
+        CF &= CF - 1;
+        const uint32_t j = rdlane(own, l), cc = rdlane(c, l);
+        if (lane == j) r.conf |= anc[L] | anc[cc];
+    }
+}
+
+__device__ void last_chunk(const sgm_plan_dev& P, WaveDFS& S, uint32_t d, const uint32_t* __restrict__ list,
+                           const uint32_t* __restrict__ off, const uint32_t* __restrict__ adj,
+                           const uint32_t* __restrict__ ws, const uint32_t* op_off, const uint32_t* op_size,
+                           const uint32_t* op_vtx, const uint32_t* __restrict__ hub_slot,
+                           const uint32_t* __restrict__ hub_bits, uint32_t hub_words,
+                           const uint32_t* __restrict__ vmask, uint32_t uL, int use_fs, WaveAcc& acc,
+                           const uint32_t* stop)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t L = d + 1;
+    const uint64_t rm = S.rem[d];
+    const uint32_t m = (uint32_t)__popcll(rm);
+    const bool mine = lane < m;
+    // chunk position of the lane-th set bit of rm: the largest p with at most `lane` set
+    // bits below it
+    uint32_t pos = 0;
+#pragma unroll
+    for (uint32_t s = 32; s > 0; s >>= 1) {
+        const uint32_t q = pos + s;
+        const uint64_t below = q >= 64u ? ~0ull : ((1ull << q) - 1ull);
+        if ((uint32_t)__popcll(rm & below) <= lane) pos = q;
+    }
+    pos = pos > 63u ? 63u : pos;
+    const uint32_t v = mine ? list[S.chunk_base[d] + pos] : 0u;
+    auto bits_of = [&](uint32_t x) -> const uint32_t* {
+        if (!hub_slot) return nullptr;
+        const uint32_t s = hub_slot[x];
+        return s == SGM_INVALID ? nullptr : hub_bits + (size_t)s * hub_words;
+    };
+    const bool has_op = P.op_first[d] < P.op_first[d + 1];
+    const uint32_t o = P.op_first[d];
+    const uint32_t par = has_op ? P.op_parent[o] : SGM_NOPARENT;
+    const int mode = !has_op ? 2 : (par == SGM_NOPARENT ? 1 : 0);  // 0 isect, 1 N(v), 2 cached list
+    SibAcc r = {0, 0, 0, 0};
+    uint32_t nbs = 0, pn = 0;
+    constexpr int PU = LAST_PU;  // pairs per lane per step
+    if (mode == 2) {
+        const uint32_t ci = P.iter_set[L];
+        const uint32_t* __restrict__ cl = op_list(op_off[ci], adj, ws);
+        const uint32_t cn = op_size[ci];
+        uint32_t b_ok = 0, b_any = 0;
+        uint64_t b_conf = 0;
+        for (uint32_t base = 0; base < cn; base += 64 * UNR) {
+            uint32_t w[UNR];
+            bool f[UNR];
+#pragma unroll
+            for (int u = 0; u < UNR; u++) {
+                const uint32_t i = base + 64 * u + lane;
+                f[u] = i < cn;
+                w[u] = f[u] ? cl[i] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < UNR; u++) f[u] = f[u] && ((vmask[w[u]] >> uL) & 1u);
+#pragma unroll
+            for (int u = 0; u < UNR; u++) {
+                uint32_t c = SGM_INVALID;
+                for (uint32_t j = 0; j < d; j++)
+                    if (S.emb[j] == w[u]) c = j;
+                const bool ok = f[u] && c == SGM_INVALID, cf = f[u] && c != SGM_INVALID;
+                const uint64_t okb = __ballot(ok), cfb = __ballot(cf);
+                b_ok += (uint32_t)__popcll(okb);
+                if (okb | cfb) b_any = 1;
+                if (cfb) b_conf |= lanes_or64(cfb, cf ? (P.anc[L] | P.anc[c]) : 0ull);
+            }
+            if (((base >> 8) & 7u) == 7u && __atomic_load_n(stop, __ATOMIC_RELAXED)) break;
+        }
+        // sibling v: an extension w == v of the cached list becomes a conflict at depth d
+        const bool self = mine && lower_found(cl, cn, v) && ((vmask[v] >> uL) & 1u);
+        r.ok = b_ok - (self ? 1u : 0u);
+        r.size = cn;
+        r.any = b_any;
+        r.conf = b_conf | (self ? (P.anc[L] | P.anc[d]) : 0ull);
+    } else {
+        uint32_t nbo = 0;
+        if (mine) { nbo = off[v]; nbs = off[v + 1] - nbo; }
+        const uint32_t* __restrict__ pl = nullptr;
+        const uint32_t* pbits = nullptr;
+        if (mode == 0) {
+            pl = op_list(op_off[par], adj, ws);
+            pn = op_size[par];
+            pbits = op_vtx[par] != SGM_INVALID ? bits_of(op_vtx[par]) : nullptr;
+        }
+        // keys of sibling i: N(v) when it is the shorter list (always without a parent op),
+        // else the parent op, tested against N(v) (its bitmap when v is a hub)
+        const uint32_t kv = (mode == 1 || pn > nbs) ? 1u : 0u;
+        const uint32_t len = mine ? (mode == 1 ? nbs : (pn < nbs ? pn : nbs)) : 0u;
+        const uint32_t slot = (mine && !kv && hub_slot) ? hub_slot[v] : SGM_INVALID;
+        const uint32_t incl = wave_incl_scan(len);
+        const uint32_t total = rdlane(incl, 63);
+        const uint32_t excl = incl - len;
+        for (uint32_t base = 0; base < total; base += 64 * PU) {
+            uint32_t key[PU], own[PU], ov[PU], tn[PU];
+            const uint32_t* tb[PU];
+            const uint32_t* tbits[PU];
+            bool valid[PU], hit[PU], head[PU];
+#pragma unroll
+            for (int u = 0; u < PU; u++) {
+                const uint32_t t = base + 64 * u + lane;
+                own[u] = wave_seg_owner(incl, t);
+                valid[u] = t < total;
+                const uint32_t q = t - (uint32_t)__shfl((int)excl, (int)own[u]);
+                head[u] = valid[u] && (lane == 0 || q == 0);
+                const uint32_t onbo = (uint32_t)__shfl((int)nbo, (int)own[u]);
+                const uint32_t onbs = (uint32_t)__shfl((int)nbs, (int)own[u]);
+                const uint32_t okv = (uint32_t)__shfl((int)kv, (int)own[u]);
+                const uint32_t oslot = (uint32_t)__shfl((int)slot, (int)own[u]);
+                ov[u] = (uint32_t)__shfl((int)v, (int)own[u]);
+                key[u] = valid[u] ? (okv ? adj[onbo + q] : pl[q]) : 0u;
+                tb[u] = okv ? pl : adj + onbo;
+                tn[u] = okv ? pn : onbs;
+                tbits[u] = okv ? pbits
+                               : (oslot == SGM_INVALID ? nullptr : hub_bits + (size_t)oslot * hub_words);
+            }
+            if (mode == 1) {
+#pragma unroll
+                for (int u = 0; u < PU; u++) hit[u] = valid[u];
+            } else {
+                member4v(tb, tn, tbits, key, valid, hit, adj);
+            }
+            bool filt[PU];
+#pragma unroll
+            for (int u = 0; u < PU; u++) filt[u] = hit[u] && ((vmask[key[u]] >> uL) & 1u);
+#pragma unroll
+            for (int u = 0; u < PU; u++) {
+                uint32_t c = SGM_INVALID;
+                if (filt[u]) {
+                    for (uint32_t j = 0; j < d; j++)
+                        if (S.emb[j] == key[u]) c = j;
+                    if (key[u] == ov[u]) c = d;
+                }
+                sib_accumulate(r, head[u], own[u], hit[u], filt[u], c, P.anc, L);
+            }
+            if (((base >> 8) & 7u) == 7u && __atomic_load_n(stop, __ATOMIC_RELAXED)) break;
+        }
+    }
+    // apply the siblings' results in order (inc/explore.h:1036-1066 per extension)
+    for (uint32_t i = 0; i < m; i++) {
+        S.emb[d] = rdlane(v, i);
+        S.nodes[d] += 1;
+        const uint32_t ok_i = rdlane(r.ok, i), size_i = rdlane(r.size, i), any_i = rdlane(r.any, i);
+        if (mode == 0) {
+            acc.inter++;
+            acc.elems += pn + rdlane(nbs, i);
+            acc.outs += size_i;
+        }
+        acc.cnt_pending += ok_i;
+        S.nodes[L] += ok_i;
+        acc.scanned += size_i;
+        if (use_fs) {
+            uint64_t f;
+            if (mode != 2 && size_i == 0) f = P.anc[P.op_vdepth[o]];
+            else if (!any_i) f = P.anc[L];
+            else if (ok_i) f = ~0ull;
+            else f = ((uint64_t)rdlane((uint32_t)(r.conf >> 32), i) << 32) | rdlane((uint32_t)r.conf, i);
+            S.fail[d] = f;
+            if (!((f >> d) & 1ull)) {
+                S.fail[d - 1] = f;
+                S.flags[d] |= F_PRUNED;
+                acc.fsp++;
+                break;
+            }
+            S.fail[d - 1] |= f;
+        }
+    }
+    S.rem[d] = 0;
 }
 
 __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__ off,
@@ -284,11 +562,12 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
                                                     uint32_t* __restrict__ stack, uint32_t stack_words,
                                                     const uint32_t* __restrict__ host_stop, int use_fs,
                                                     const uint32_t* __restrict__ hub_slot,
-                                                    const uint32_t* __restrict__ hub_bits, uint32_t hub_words)
+                                                    const uint32_t* __restrict__ hub_bits, uint32_t hub_words,
+                                                    int serial_last)
 {
     extern __shared__ uint64_t smem_u64[];
     uint8_t* smem = reinterpret_cast<uint8_t*>(smem_u64);
-    const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
+    const uint32_t wave = uni(threadIdx.x >> 6), lane = lane_id();  // uniform: keeps per-wave values scalar
     const size_t wave_bytes = enum_wave_bytes(max_ops);
     uint8_t* wbase = smem + wave * wave_bytes;
     sgm_plan_dev& P = *reinterpret_cast<sgm_plan_dev*>(wbase);
@@ -304,7 +583,7 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
         return s == SGM_INVALID ? nullptr : hub_bits + (size_t)s * hub_words;
     };
 
-    for (uint32_t d = 0; d < SGM_MAXQ; d++) S.nodes[d] = 0;
+    clear_nodes(S, nullptr, 0);
     WaveAcc acc = {0, 0, 0, 0, 0, 0, 0};
     uint32_t cur = SGM_INVALID;  // query whose plan sits in this wave's LDS
     uint32_t n = 0, L = 0, uL = 0, root_u = 0;
@@ -454,9 +733,16 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
                         if (okb | cfb) S.flags[d] |= F_ANY;
                         if (use_fs && cfb) {
                             const uint64_t m = (f && c != SGM_INVALID) ? (P.anc[d] | P.anc[c]) : 0ull;
-                            S.fail[d - 1] |= wave_or64(m);
+                            S.fail[d - 1] |= lanes_or64(cfb, m);
                         }
                         S.rem[d] = okb;
+                        continue;
+                    }
+                    if (d == n - 2 && !serial_last) {
+                        // the last level of every remaining extension of this chunk
+                        last_chunk(P, S, d, list, off, adj, ws, op_off, op_size, op_vtx, hub_slot, hub_bits,
+                                   hub_words, vmask, uL, use_fs, acc, stop);
+                        if (acc.cnt_pending >= FLUSH_AT) flush_count(acc.cnt_pending, &Q.st->count, stop, cap);
                         continue;
                     }
                     // extend with the next candidate of this chunk
@@ -793,10 +1079,10 @@ size_t enum_smem_bytes(uint32_t max_ops) { return 4 * enum_wave_bytes(max_ops); 
 void launch_enum_cache(hipStream_t s, uint32_t blocks, const uint32_t* off, const uint32_t* adj, const uint32_t* items,
                        const sgm_equery* qs, uint32_t nq, uint32_t max_ops, uint32_t* stack, uint32_t stack_words,
                        const uint32_t* host_stop, int use_fs, const uint32_t* hub_slot, const uint32_t* hub_bits,
-                       uint32_t hub_words)
+                       uint32_t hub_words, int serial_last)
 {
     hipLaunchKernelGGL(k_enum_cache, dim3(blocks), dim3(256), 4 * enum_wave_bytes(max_ops), s, off, adj, items, qs, nq,
-                       max_ops, stack, stack_words, host_stop, use_fs, hub_slot, hub_bits, hub_words);
+                       max_ops, stack, stack_words, host_stop, use_fs, hub_slot, hub_bits, hub_words, serial_last);
 }
 
 void launch_enum_aux(hipStream_t s, uint32_t blocks, const sgm_auxplan_dev* plan, uint32_t n_items, uint32_t* stack,

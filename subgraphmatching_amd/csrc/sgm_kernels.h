@@ -51,7 +51,7 @@ void launch_enum_aux(hipStream_t s, uint32_t blocks, const sgm_auxplan_dev* plan
 void launch_enum_cache(hipStream_t s, uint32_t blocks, const uint32_t* off, const uint32_t* adj, const uint32_t* items,
                        const sgm_equery* qs, uint32_t nq, uint32_t max_ops, uint32_t* stack, uint32_t stack_words,
                        const uint32_t* host_stop, int use_fs, const uint32_t* hub_slot, const uint32_t* hub_bits,
-                       uint32_t hub_words);
+                       uint32_t hub_words, int serial_last);
 void launch_hub_bits(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* hubs, uint32_t n_hubs,
                      uint32_t* bits, uint32_t words);
 }  // namespace sgm

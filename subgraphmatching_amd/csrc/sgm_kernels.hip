@@ -193,14 +193,7 @@ __device__ void gql_mark_nbrs(const sgm_qdesc* __restrict__ d, const uint32_t* _
     const uint32_t lane = lane_id();
     for (uint32_t b = 0; b < total; b += 64) {
         const uint32_t t = b + lane;
-        // owner lane j: the number of lanes whose inclusive end is <= t (ends are sorted)
-        uint32_t j = 0;
-#pragma unroll
-        for (uint32_t s = 32; s > 0; s >>= 1) {
-            const uint32_t e = (uint32_t)__shfl((int)incl, (int)(j + s - 1));
-            if (e <= t) j += s;
-        }
-        j = j > 63u ? 63u : j;
+        const uint32_t j = wave_seg_owner(incl, t);
         const uint32_t oj = (uint32_t)__shfl((int)o, (int)j);
         const uint32_t sj = (uint32_t)__shfl((int)(incl - dv), (int)j);
         if (t >= total) continue;

@@ -42,6 +42,33 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
     return v;
 }
 
+// Segmented walks: lanes hold the inclusive prefix sums `incl` of per-lane segment lengths;
+// returns the lane owning element t of the concatenation (the number of lanes whose
+// segment ends at or before t; ends are sorted).  Every lane must be active.
+__device__ __forceinline__ uint32_t wave_seg_owner(uint32_t incl, uint32_t t)
+{
+    uint32_t j = 0;
+#pragma unroll
+    for (uint32_t s = 32; s > 0; s >>= 1) {
+        const uint32_t e = (uint32_t)__shfl((int)incl, (int)(j + s - 1));
+        if (e <= t) j += s;
+    }
+    return j;
+}
+
+// OR of x over the lanes in `lanes` (a ballot), one readlane pair per set bit: cheaper than a
+// butterfly when few lanes contribute, and it keeps no shuffle indices live.
+__device__ __forceinline__ uint64_t lanes_or64(uint64_t lanes, uint64_t x)
+{
+    uint64_t r = 0;
+    while (lanes) {
+        const uint32_t l = (uint32_t)__ffsll((long long)lanes) - 1u;
+        lanes &= lanes - 1;
+        r |= ((uint64_t)rdlane((uint32_t)(x >> 32), l) << 32) | rdlane((uint32_t)x, l);
+    }
+    return r;
+}
+
 __device__ __forceinline__ uint64_t wave_or64(uint64_t x)
 {
     uint32_t lo = wave_or32((uint32_t)x), hi = wave_or32((uint32_t)(x >> 32));

@@ -112,15 +112,22 @@ def _compare(g, q, order, cap=None, check_work=True):
     ok = sgm.Filter_GQL(g, q, can, build_tables=False)
     ok_r, c_r = O.gql(g, q)
     assert ok == ok_r
+    elems = {}
     for fs, method, flags in ((True, L.SGM_ENUM_CACHE_FILTER_FS, 0), (False, L.SGM_ENUM_CACHE_FILTER, 0),
-                              (True, L.SGM_ENUM_CACHE_FILTER_FS, L.SGM_ENUM_F_NO_BITMAPS)):
+                              (True, L.SGM_ENUM_CACHE_FILTER_FS, L.SGM_ENUM_F_NO_BITMAPS),
+                              (True, L.SGM_ENUM_CACHE_FILTER_FS, L.SGM_ENUM_F_SERIAL_LAST),
+                              (False, L.SGM_ENUM_CACHE_FILTER, L.SGM_ENUM_F_SERIAL_LAST)):
         res = sgm.enumerate_query(can, order, method=method, cap=cap, flags=flags)
         ref = O.enum_cache(g, q, c_r, order, cap=cap, fs=fs)
-        assert res.ans == ref["count"], (q.name, fs, res.ans, ref["count"])
+        assert res.ans == ref["count"], (q.name, fs, flags, res.ans, ref["count"])
         if check_work and not ref["capped"]:
-            assert res.intersections == ref["intersections"], (q.name, fs)
-            assert res.nodes == ref["nodes"], (q.name, fs, res.nodes, ref["nodes"])
-            assert res.fs_prunes == ref["fs_prunes"], (q.name, fs)
+            assert res.intersections == ref["intersections"], (q.name, fs, flags)
+            assert res.nodes == ref["nodes"], (q.name, fs, flags, res.nodes, ref["nodes"])
+            assert res.fs_prunes == ref["fs_prunes"], (q.name, fs, flags)
+            # the chunked and the one-at-a-time last level read the same elements
+            key = (fs, flags & ~L.SGM_ENUM_F_SERIAL_LAST & ~L.SGM_ENUM_F_NO_BITMAPS)
+            got = (res.elems_read, res.elems_out, res.elems_scanned)
+            assert elems.setdefault(key, got) == got, (q.name, fs, flags, elems[key], got)
     res_a = sgm.enumerate_query(can, order, method=L.SGM_ENUM_AUX_TABLES, cap=cap)
     ref_a = O.enum_all(g, q, c_r, order, cap=cap)
     assert res_a.ans == ref_a["count"], (q.name, "aux", res_a.ans, ref_a["count"])
