@@ -79,6 +79,8 @@ def parse():
     ap.add_argument("--blocks", type=int, default=0,
                     help="persistent enumeration blocks of 4 waves (0 = library default / autotuned)")
     ap.add_argument("--batch", type=int, default=64, help="rmat20: queries per graph-replayed batch")
+    ap.add_argument("--no-overlap", action="store_true",
+                    help="rmat20: one runner, each batch waited for before the next is filtered")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="wall-clock budget of the CPU baseline sample (0 disables it)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
@@ -190,24 +192,43 @@ def setup_rmat(args):
         tuned["blocks"] = args.blocks
     print(f"[bench] autotuned on the first batch: {tuned} ({tuned_ms:.2f} ms)", flush=True)
 
+    # two runners on their own streams: batch k+1 is filtered and launched while batch k
+    # enumerates, and its grid takes the CUs batch k's waves leave at the end
+    runners = [runner] + ([sgm.Batch(g, slots=B, ctx=runner.ctx)] if not args.no_overlap else [])
+
     def step():
         rec = new_rec()
         t0 = time.perf_counter()
         counts = [0] * nq
-        pos = 0
-        for part in chunks:
-            ok, _, _ = runner.filter(part)
-            f_ms, _ = runner.last_ms()
-            rec["filter_ms"] += f_ms
-            rec["gql_ms"] += f_ms
-            res = runner.enumerate(cap=args.cap, **tuned)
-            _, e_ms = runner.last_ms()
+        pending = None  # (runner, ok flags, position of the batch's first query)
+
+        def collect(p):
+            r, ok, start = p
+            res = r.enumerate_wait()
+            _, e_ms = r.last_ms()
             rec["enum_ms"] += e_ms
             rec["enum_launches"] += 1
-            for good, r in zip(ok, res):
-                add_result(rec, r if good else None)
-                counts[by_size[pos]] = rec["counts"][-1]
-                pos += 1
+            for j, (good, x) in enumerate(zip(ok, res)):
+                add_result(rec, x if good else None)
+                counts[by_size[start + j]] = rec["counts"][-1]
+
+        pos = 0
+        for k, part in enumerate(chunks):
+            r = runners[k % len(runners)]
+            if pending is not None and pending[0] is r:
+                collect(pending)
+                pending = None
+            ok, _, _ = r.filter(part)
+            f_ms, _ = r.last_ms()
+            rec["filter_ms"] += f_ms
+            rec["gql_ms"] += f_ms
+            r.enumerate_start(cap=args.cap, **tuned)
+            if pending is not None:
+                collect(pending)
+            pending = (r, ok, pos)
+            pos += len(part)
+        if pending is not None:
+            collect(pending)
         rec["counts"] = counts  # in query order
         rec["wall_s"] = time.perf_counter() - t0
         rec["filter_wall_s"] = rec["filter_ms"] / 1e3
@@ -235,7 +256,9 @@ def setup_rmat(args):
     config = {"workload": "rmat_s20_e2e7_abcd.57.19.19.05_seed3__1000x4-32v_mixed_rw_seed33__gql_ri_cache_filter_fs",
               "queries_per_step": nq, "cap_per_query": args.cap, "order": "RI", "filter": "NLF+GQL",
               "enumeration": "cache+filter+failing-sets",
-              "launch": f"hipGraph-replayed batches of {B} (filter graph + persistent enumeration graph)",
+              "launch": f"hipGraph-replayed batches of {B} (filter graph + persistent enumeration graph)"
+                        + ("" if args.no_overlap else "; two runners on two streams, batch k+1 filtered and "
+                                                      "launched while batch k enumerates"),
               "autotuned": tuned}
     return g, queries, step, config, {"single_launch": single_launch}
 
@@ -406,6 +429,11 @@ def main():
               "and nothing in this build uses a second GPU", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "rmat20" and not args.no_overlap:
+        # the two overlapped runners' streams need hardware queues of their own (HIP shares
+        # its default 4 among the context, stop, null and runner streams); set before the
+        # first HIP call of the process
+        os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
     t_setup = time.perf_counter()
     setup = {"rmat20": setup_rmat, "rmat20-learned": setup_rmat_learned,
              "rmat22-serve": setup_rmat_serve}.get(args.workload, setup_chunglu)

@@ -172,6 +172,14 @@ int sgm_batch_order_gql(const sgm_batch* b, uint32_t i, uint32_t* order);
 /* Enumerate the loaded batch (cache + filter methods) in one persistent launch; stats[i]
  * per query, stats[i].kernel_ms = the batch's enumeration time. */
 int sgm_batch_enumerate(sgm_batch* b, const uint32_t* const* orders, const sgm_enum_opts* opts, sgm_stats* stats);
+/* The same in two halves: _launch queues the enumeration on the batch's own stream and
+ * returns at once; _wait blocks until it is done and fills stats.  Between the two the
+ * host may filter and launch another batch of the same context, whose grid then runs on
+ * the CUs this batch's waves leave as they run out of work.  The batch itself accepts no
+ * other call in between (SGM_ESTATE).  A stop request (sgm_stop) ends every launch in
+ * flight. */
+int sgm_batch_enumerate_launch(sgm_batch* b, const uint32_t* const* orders, const sgm_enum_opts* opts);
+int sgm_batch_enumerate_wait(sgm_batch* b, sgm_stats* stats);
 /* Device time of the last sgm_batch_filter / sgm_batch_enumerate (HIP events). */
 int sgm_batch_last_ms(const sgm_batch* b, double* filter_ms, double* enum_ms);
 /* Autotuner: enumerates the loaded batch with each persistent grid size (2, 4, 8 blocks per

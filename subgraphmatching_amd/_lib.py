@@ -45,7 +45,7 @@ EXPORTS = [
     "sgm_cand_exist_download", "sgm_build_tables", "sgm_table_info", "sgm_table_download",
     "sgm_order_ri", "sgm_order_gql", "sgm_plan_export", "sgm_enumerate", "sgm_enumerate_batch",
     "sgm_batch_create", "sgm_batch_destroy", "sgm_batch_filter", "sgm_batch_order_gql", "sgm_batch_enumerate",
-    "sgm_batch_last_ms", "sgm_batch_autotune",
+    "sgm_batch_enumerate_launch", "sgm_batch_enumerate_wait", "sgm_batch_last_ms", "sgm_batch_autotune",
     "sgm_costmodel_create", "sgm_costmodel_destroy", "sgm_costmodel_param_count", "sgm_costmodel_params",
     "sgm_costmodel_features", "sgm_costmodel_predict", "sgm_costmodel_grad", "sgm_costmodel_train",
     "sgm_costmodel_timing", "sgm_order_sample", "sgm_batch_rank_orders", "sgm_gql_work",
@@ -124,6 +124,8 @@ def lib() -> C.CDLL:
         "sgm_batch_filter": (C.c_int, [vp, C.c_uint32, vp, vp, vp, C.c_int, vp, vp, vp]),
         "sgm_batch_order_gql": (C.c_int, [vp, C.c_uint32, vp]),
         "sgm_batch_enumerate": (C.c_int, [vp, vp, C.POINTER(EnumOpts), vp]),
+        "sgm_batch_enumerate_launch": (C.c_int, [vp, vp, C.POINTER(EnumOpts)]),
+        "sgm_batch_enumerate_wait": (C.c_int, [vp, vp]),
         "sgm_batch_last_ms": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "sgm_batch_autotune": (C.c_int, [vp, vp, C.POINTER(EnumOpts), C.POINTER(EnumOpts),
                                          C.POINTER(C.c_double)]),

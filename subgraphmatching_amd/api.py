@@ -557,6 +557,21 @@ class Batch:
             _check(code, self.ctx)
         return [_result(st, q.getVertexCnt()) for st, q in zip(stats, self.queries)]
 
+    def enumerate_start(self, orders=None, method=L.SGM_ENUM_CACHE_FILTER_FS, cap=None, blocks=0, stack_words=0,
+                        flags=0, hub_min_degree=0):
+        """Launch the loaded batch's enumeration and return at once
+        (sgm_batch_enumerate_launch); ``enumerate_wait`` returns the results.  In between,
+        another Batch of the same context can filter and launch the next batch, which then
+        fills the GPU as this one runs out of work."""
+        arr, _keep = self._orders(orders)
+        opts = _opts(method, cap, blocks, stack_words, flags, hub_min_degree)
+        _check(L.lib().sgm_batch_enumerate_launch(self.handle, arr, C.byref(opts)), self.ctx)
+
+    def enumerate_wait(self):
+        stats = (L.Stats * len(self.queries))()
+        _check(L.lib().sgm_batch_enumerate_wait(self.handle, stats), self.ctx)
+        return [_result(st, q.getVertexCnt()) for st, q in zip(stats, self.queries)]
+
     def rank_orders(self, model, n_samples=32, seed=0, margin=0.5):
         """Learned orders of the loaded batch (sgm_batch_rank_orders): returns (orders,
         predicted log2 nodes of the chosen order, of the RI order, learned-order flags)."""

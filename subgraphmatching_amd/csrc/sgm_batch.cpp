@@ -9,6 +9,11 @@
 // (+ failing-set) launch over all slots.  Unused slots have n = 0 (filter) and no items
 // (enumeration).  Orders are chosen on the host between the two graphs, from the query
 // alone (RI) or from the candidate statistics (GQL order, a learned cost model).
+//
+// Every runner has its own stream, and an enumeration can be launched without waiting
+// (sgm_batch_enumerate_launch / _wait): with two runners the host filters batch i+1 while
+// batch i enumerates, and batch i+1's persistent grid takes over the GPU as batch i's
+// waves run out of work, instead of the GPU idling through each batch's tail.
 #include "sgm_costmodel.h"
 #include "sgm_objects.h"
 
@@ -25,6 +30,7 @@ struct EnumGraph {
 struct sgm_batch {
     sgm_graph* g = nullptr;
     sgm_ctx* c = nullptr;
+    hipStream_t stream = nullptr;            // all of this runner's work (default priority)
     uint32_t slots = 0;
     uint32_t* d_vmask = nullptr;             // slots * V
     uint32_t* d_flags = nullptr;             // slots * SGM_GQL_FLAG_WORDS
@@ -58,6 +64,15 @@ struct sgm_batch {
     hipGraphExec_t gf_nlf = nullptr;
     std::vector<std::pair<uint32_t, hipGraphExec_t>> gf_gql;  // by number of GQL phases
     std::vector<EnumGraph> ge;
+    // an enumeration launched and not yet waited for
+    bool pending = false;
+    sgm_enum_opts pend_o{};
+    std::vector<sgm_equery> pend_eq;
+    uint32_t pend_bucket = 0, pend_blocks = 0;
+    uint64_t pend_words = 0;
+    int pend_use_fs = 0, pend_serial = 0;
+    const uint32_t* pend_hub = nullptr;
+    sgm_stats_dev* h_got = nullptr;          // pinned: per-slot statistics copied back
 };
 
 namespace {
@@ -96,15 +111,15 @@ int load_query(sgm_ctx* c, const uint32_t* qoff, const uint32_t* qadj, uint32_t 
     return SGM_OK;
 }
 
-// Records `body` on the context stream into an executable graph.
+// Records `body` (launches on stream s) into an executable graph.
 template <class F>
-int capture(sgm_ctx* c, F&& body, hipGraphExec_t* out)
+int capture(sgm_ctx* c, hipStream_t s, F&& body, hipGraphExec_t* out)
 {
     hipGraph_t graph = nullptr;
-    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     body();
     const hipError_t le = hipGetLastError();
-    const hipError_t ee = hipStreamEndCapture(c->stream, &graph);
+    const hipError_t ee = hipStreamEndCapture(s, &graph);
     if (le != hipSuccess || ee != hipSuccess) {
         if (graph) (void)hipGraphDestroy(graph);
         return set_err(c, SGM_EHIP, "graph capture: %s", hipGetErrorString(le != hipSuccess ? le : ee));
@@ -132,7 +147,8 @@ void sgm_batch_destroy(sgm_batch* b)
     if (b->c) {
         forget(b->c->batches, b);
         (void)hipSetDevice(b->c->device);
-        (void)hipStreamSynchronize(b->c->stream);
+        if (b->stream) (void)hipStreamSynchronize(b->stream);
+        if (b->pending) b->c->enum_inflight--;
     }
     if (b->gf_nlf) (void)hipGraphExecDestroy(b->gf_nlf);
     for (auto& x : b->gf_gql) (void)hipGraphExecDestroy(x.second);
@@ -155,6 +171,8 @@ void sgm_batch_destroy(sgm_batch* b)
     dfree(b->d_rank_out);
     dfree(b->d_rank_pred);
     dfree(b->d_rank_flag);
+    if (b->h_got) (void)hipHostFree(b->h_got);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
 
@@ -185,6 +203,8 @@ int sgm_batch_create(sgm_graph* g, uint32_t slots, sgm_batch** out)
     if (e == hipSuccess) e = dalloc(&b->d_rank_flag, slots);
     for (auto& ev : b->ev)
         if (e == hipSuccess) e = hipEventCreate(&ev);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&b->h_got), sizeof(sgm_stats_dev) * slots);
     if (e != hipSuccess) {
         sgm_batch_destroy(b);
         return set_err(c, e == hipErrorOutOfMemory ? SGM_ENOMEM : SGM_EHIP, "batch alloc: %s", hipGetErrorString(e));
@@ -219,6 +239,7 @@ int sgm_batch_filter(sgm_batch* b, uint32_t nq, const uint32_t* const* qoffs, co
     sgm_graph* g = b->g;
     if (nq == 0 || nq > b->slots) return set_err(c, SGM_EINVAL, "batch of %u queries for %u slots", nq, b->slots);
     if (kind != SGM_FILTER_NLF && kind != SGM_FILTER_GQL) return set_err(c, SGM_EINVAL, "unknown filter kind %d", kind);
+    if (b->pending) return set_err(c, SGM_ESTATE, "wait for the batch's enumeration before loading another batch");
     HIPCHK(c, hipSetDevice(c->device));
     b->nq = 0;
     b->q.assign(nq, sgm::QueryCSR());
@@ -248,35 +269,35 @@ int sgm_batch_filter(sgm_batch* b, uint32_t nq, const uint32_t* const* qoffs, co
         }
     }
     if (!*gexec) {
-        const int rc = capture(c, [&] {
-            sgm::launch_fill_u32(c->stream, b->d_flags, (size_t)slots * SGM_GQL_FLAG_WORDS, 0);
+        const int rc = capture(c, b->stream, [&] {
+            sgm::launch_fill_u32(b->stream, b->d_flags, (size_t)slots * SGM_GQL_FLAG_WORDS, 0);
             if (kind == SGM_FILTER_GQL)  // touch and dirty words of every slot
-                sgm::launch_fill_strided(c->stream, b->d_scratch + V, sgm_gql_clear_words(V),
+                sgm::launch_fill_strided(b->stream, b->d_scratch + V, sgm_gql_clear_words(V),
                                          sgm_gql_scratch_words(V), slots, 0);
-            sgm::launch_fill_u32(c->stream, b->d_cnt, (size_t)slots * SGM_MAXQ, 0);
-            sgm::launch_fill_u32(c->stream, reinterpret_cast<uint32_t*>(b->d_degsum), 2 * (size_t)slots * SGM_MAXQ, 0);
+            sgm::launch_fill_u32(b->stream, b->d_cnt, (size_t)slots * SGM_MAXQ, 0);
+            sgm::launch_fill_u32(b->stream, reinterpret_cast<uint32_t*>(b->d_degsum), 2 * (size_t)slots * SGM_MAXQ, 0);
             if (kind == SGM_FILTER_GQL && V) {
-                sgm::launch_nlf_rank(c->stream, g->d_gecnt, V, b->d_qd, slots);
-                sgm::launch_gql_prepass(c->stream, g->d_dtab, V, g->d_gecnt, b->d_qd, slots);
+                sgm::launch_nlf_rank(b->stream, g->d_gecnt, V, b->d_qd, slots);
+                sgm::launch_gql_prepass(b->stream, g->d_dtab, V, g->d_gecnt, b->d_qd, slots);
                 for (uint32_t ph = 0; ph < n_phases; ph++)
-                    sgm::launch_gql_phase(c->stream, g->d_roff, g->d_radj, V, g->d_gecnt, b->d_qd, slots, ph);
-                sgm::launch_unrank(c->stream, g->d_rank, V, b->d_qd, slots);
+                    sgm::launch_gql_phase(b->stream, g->d_roff, g->d_radj, V, g->d_gecnt, b->d_qd, slots, ph);
+                sgm::launch_unrank(b->stream, g->d_rank, V, b->d_qd, slots);
             } else {
-                sgm::launch_nlf(c->stream, g->d_off, V, b->d_qd, slots);
+                sgm::launch_nlf(b->stream, g->d_off, V, b->d_qd, slots);
             }
-            sgm::launch_cand_stats(c->stream, g->d_off, V, b->d_qd, slots, b->d_cnt, b->d_degsum);
+            sgm::launch_cand_stats(b->stream, g->d_off, V, b->d_qd, slots, b->d_cnt, b->d_degsum);
         }, gexec);
         if (rc) return rc;
     }
-    HIPCHK(c, hipMemcpyAsync(b->d_qd, b->hqd.data(), sizeof(sgm_qdesc) * slots, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipEventRecord(b->ev[0], c->stream));
-    HIPCHK(c, hipGraphLaunch(*gexec, c->stream));
-    HIPCHK(c, hipEventRecord(b->ev[1], c->stream));
+    HIPCHK(c, hipMemcpyAsync(b->d_qd, b->hqd.data(), sizeof(sgm_qdesc) * slots, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(c, hipEventRecord(b->ev[0], b->stream));
+    HIPCHK(c, hipGraphLaunch(*gexec, b->stream));
+    HIPCHK(c, hipEventRecord(b->ev[1], b->stream));
     HIPCHK(c, hipMemcpyAsync(b->counts.data(), b->d_cnt, sizeof(uint32_t) * SGM_MAXQ * nq, hipMemcpyDeviceToHost,
-                             c->stream));
+                             b->stream));
     HIPCHK(c, hipMemcpyAsync(b->degsum.data(), b->d_degsum, sizeof(unsigned long long) * SGM_MAXQ * nq,
-                             hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+                             hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(c, hipStreamSynchronize(b->stream));
     float ms = 0;
     HIPCHK(c, hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
     b->filter_ms = ms;
@@ -300,21 +321,22 @@ int sgm_batch_order_gql(const sgm_batch* b, uint32_t i, uint32_t* order)
     return SGM_OK;
 }
 
-int sgm_batch_enumerate(sgm_batch* b, const uint32_t* const* orders, const sgm_enum_opts* opts, sgm_stats* stats)
+int sgm_batch_enumerate_launch(sgm_batch* b, const uint32_t* const* orders, const sgm_enum_opts* opts)
 {
-    if (!b || !stats) return SGM_EINVAL;
+    if (!b) return SGM_EINVAL;
     sgm_ctx* c = b->c;
     sgm_graph* g = b->g;
     if (b->nq == 0) return set_err(c, SGM_ESTATE, "filter a batch first");
+    if (b->pending) return set_err(c, SGM_ESTATE, "the batch's previous enumeration was not waited for");
     const sgm_enum_opts o = normalize_opts(opts);
     if (o.method != SGM_ENUM_CACHE_FILTER_FS && o.method != SGM_ENUM_CACHE_FILTER)
         return set_err(c, SGM_EINVAL, "the batch runner supports the cache + filter methods (got %d)", o.method);
     HIPCHK(c, hipSetDevice(c->device));
     const uint32_t nq = b->nq, slots = b->slots, V = g->V;
-    std::memset(stats, 0, sizeof(sgm_stats) * nq);
     std::vector<sgm_plan_dev> plans(nq);
-    std::vector<sgm_equery> eq(slots);
-    uint32_t max_ops = 1, live = 0;
+    std::vector<sgm_equery>& eq = b->pend_eq;
+    eq.assign(slots, sgm_equery());
+    uint32_t max_ops = 1;
     std::vector<uint32_t> ord(SGM_MAXQ);
     for (uint32_t i = 0; i < slots; i++) {
         sgm_equery& e = eq[i];
@@ -336,22 +358,35 @@ int sgm_batch_enumerate(sgm_batch* b, const uint32_t* const* orders, const sgm_e
         sgm::plan_to_device(pl, plans[i]);
         max_ops = std::max<uint32_t>(max_ops, plans[i].n_ops);
         e.root_u = ord[0];
-        if (b->status[i] == SGM_OK) {  // the reference's driver reports 0 without enumerating
+        if (b->status[i] == SGM_OK)  // the reference's driver reports 0 without enumerating
             e.n_items = g->ge_cnt[std::min<uint32_t>(q.off[ord[0] + 1] - q.off[ord[0]], 32)];
-            live++;
-        }
     }
-    if (!live) return SGM_OK;
-    HIPCHK(c, hipMemcpyAsync(b->d_plans, plans.data(), sizeof(sgm_plan_dev) * nq, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(b->d_eq, eq.data(), sizeof(sgm_equery) * slots, hipMemcpyHostToDevice, c->stream));
+    uint32_t live = 0;
+    for (uint32_t i = 0; i < nq; i++) live += eq[i].n_items ? 1u : 0u;
     const uint32_t bucket = ops_bucket(max_ops);
-    const uint32_t blocks = default_blocks(c, o, live);
-    uint64_t words = o.stack_words ? o.stack_words : std::min<uint64_t>((uint64_t)8 * g->max_deg + 1024, 1u << 20);
+    const uint32_t blocks = default_blocks(c, o, std::max<uint32_t>(live, 1));
+    const uint64_t words = o.stack_words ? o.stack_words : std::min<uint64_t>((uint64_t)8 * g->max_deg + 1024, 1u << 20);
     const int use_fs = o.method == SGM_ENUM_CACHE_FILTER_FS ? 1 : 0;
     const int serial = (o.flags & SGM_ENUM_F_SERIAL_LAST) ? 1 : 0;
     const uint32_t* hub = nullptr;
     int rc = hub_slot_map(g, o, &hub);
     if (rc) return rc;
+    b->pend_o = o;
+    b->pend_bucket = bucket;
+    b->pend_blocks = blocks;
+    b->pend_words = words;
+    b->pend_use_fs = use_fs;
+    b->pend_serial = serial;
+    b->pend_hub = hub;
+    if (!live) {  // nothing to enumerate: the wait reports zero counts
+        std::memset(b->h_got, 0, sizeof(sgm_stats_dev) * slots);
+        b->enum_ms = 0;
+        b->pending = true;
+        c->enum_inflight++;
+        return SGM_OK;
+    }
+    HIPCHK(c, hipMemcpyAsync(b->d_plans, plans.data(), sizeof(sgm_plan_dev) * nq, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(c, hipMemcpyAsync(b->d_eq, eq.data(), sizeof(sgm_equery) * slots, hipMemcpyHostToDevice, b->stream));
     const size_t need = (size_t)blocks * 4 * words;
     if (b->stack_words_total < need) {
         dfree(b->d_stack);
@@ -369,40 +404,68 @@ int sgm_batch_enumerate(sgm_batch* b, const uint32_t* const* orders, const sgm_e
             exec = x.exec;
     const size_t st_words = sizeof(sgm_stats_dev) / sizeof(uint32_t) * slots;
     if (!exec) {
-        rc = capture(c, [&] {
-            sgm::launch_fill_u32(c->stream, reinterpret_cast<uint32_t*>(b->d_st), st_words, 0);
-            sgm::launch_fill_u32(c->stream, b->d_qstop, 2 * (size_t)slots, 0);
-            sgm::launch_fill_u32(c->stream, c->d_stop, 1, 0);
-            sgm::launch_enum_cache(c->stream, blocks, g->d_off, g->d_adj, g->d_bydeg, b->d_eq, slots, bucket, b->d_stack,
+        rc = capture(c, b->stream, [&] {
+            sgm::launch_fill_u32(b->stream, reinterpret_cast<uint32_t*>(b->d_st), st_words, 0);
+            sgm::launch_fill_u32(b->stream, b->d_qstop, 2 * (size_t)slots, 0);
+            sgm::launch_enum_cache(b->stream, blocks, g->d_off, g->d_adj, g->d_bydeg, b->d_eq, slots, bucket, b->d_stack,
                                    (uint32_t)words, c->d_stop, use_fs, hub, g->d_hub_bits, g->hub_words, serial);
         }, &exec);
         if (rc) return rc;
         b->ge.push_back(EnumGraph{bucket, (uint32_t)use_fs, blocks, (uint32_t)words, (uint32_t)serial, hub, exec});
     }
-    c->stop_requested = 0;
-    HIPCHK(c, hipEventRecord(b->ev[2], c->stream));
-    HIPCHK(c, hipGraphLaunch(exec, c->stream));
-    HIPCHK(c, hipEventRecord(b->ev[3], c->stream));
-    std::vector<sgm_stats_dev> got(nq);
-    HIPCHK(c, hipMemcpyAsync(got.data(), b->d_st, sizeof(sgm_stats_dev) * nq, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // the context's stop word is shared by every launch in flight: clear it only when none is
+    // (a stop request then ends all of them)
+    if (c->enum_inflight == 0) {
+        sgm::launch_fill_u32(b->stream, c->d_stop, 1, 0);
+        c->stop_requested = 0;
+    }
+    HIPCHK(c, hipEventRecord(b->ev[2], b->stream));
+    HIPCHK(c, hipGraphLaunch(exec, b->stream));
+    HIPCHK(c, hipEventRecord(b->ev[3], b->stream));
+    HIPCHK(c, hipMemcpyAsync(b->h_got, b->d_st, sizeof(sgm_stats_dev) * nq, hipMemcpyDeviceToHost, b->stream));
+    b->pending = true;
+    c->enum_inflight++;
+    b->enum_ms = -1;
+    return SGM_OK;
+}
+
+int sgm_batch_enumerate_wait(sgm_batch* b, sgm_stats* stats)
+{
+    if (!b || !stats) return SGM_EINVAL;
+    sgm_ctx* c = b->c;
+    sgm_graph* g = b->g;
+    if (!b->pending) return set_err(c, SGM_ESTATE, "no enumeration launched on this batch");
+    HIPCHK(c, hipSetDevice(c->device));
+    const hipError_t se = hipStreamSynchronize(b->stream);
+    b->pending = false;
+    c->enum_inflight--;
+    if (se != hipSuccess) return set_err(c, SGM_EHIP, "batch enumeration: %s", hipGetErrorString(se));
+    const uint32_t nq = b->nq, slots = b->slots;
+    const sgm_enum_opts o = b->pend_o;
+    const std::vector<sgm_equery>& eq = b->pend_eq;
+    std::memset(stats, 0, sizeof(sgm_stats) * nq);
     float ms = 0;
-    HIPCHK(c, hipEventElapsedTime(&ms, b->ev[2], b->ev[3]));
-    b->enum_ms = ms;
+    if (b->enum_ms < 0) {
+        HIPCHK(c, hipEventElapsedTime(&ms, b->ev[2], b->ev[3]));
+        b->enum_ms = ms;
+    }
     std::vector<uint32_t> again;
     for (uint32_t i = 0; i < nq; i++) {
         stats[i].kernel_ms = ms;
-        if (got[i].overflow) { again.push_back(i); continue; }
-        stats_out(got[i], o.cap, &stats[i]);
+        if (b->h_got[i].overflow) { again.push_back(i); continue; }
+        stats_out(b->h_got[i], o.cap, &stats[i]);
         if (c->stop_requested && !stats[i].capped && b->status[i] == SGM_OK) stats[i].stopped = 1;
     }
     // set-op stack overflow (rare): rerun those queries alone with a larger stack, outside
     // the graph (this allocates)
+    const uint32_t blocks = b->pend_blocks, bucket = b->pend_bucket;
+    uint64_t words = b->pend_words;
+    const size_t st_words = sizeof(sgm_stats_dev) / sizeof(uint32_t) * slots;
     for (uint32_t attempt = 0; !again.empty(); attempt++) {
         if (attempt >= 3) return set_err(c, SGM_EDEVICE, "set-op stack overflow after 3 retries");
         words *= 4;
         if (words > 0x7FFFFFFFull) return set_err(c, SGM_EDEVICE, "set-op stack would exceed 2^31 words");
-        rc = ensure_stack(c, (size_t)blocks * 4 * words * sizeof(uint32_t));
+        int rc = ensure_stack(c, (size_t)blocks * 4 * words * sizeof(uint32_t));
         if (rc) return rc;
         std::vector<sgm_equery> sub;
         for (uint32_t i : again) {
@@ -414,20 +477,18 @@ int sgm_batch_enumerate(sgm_batch* b, const uint32_t* const* orders, const sgm_e
             sub.push_back(e);
         }
         HIPCHK(c, hipMemcpyAsync(b->d_eq, sub.data(), sizeof(sgm_equery) * sub.size(), hipMemcpyHostToDevice,
-                                 c->stream));
-        sgm::launch_fill_u32(c->stream, reinterpret_cast<uint32_t*>(b->d_st), st_words, 0);
-        sgm::launch_fill_u32(c->stream, b->d_qstop, 2 * (size_t)slots, 0);
-        sgm::launch_fill_u32(c->stream, c->d_stop, 1, 0);
-        HIPCHK(c, hipEventRecord(b->ev[2], c->stream));
-        sgm::launch_enum_cache(c->stream, blocks, g->d_off, g->d_adj, g->d_bydeg, b->d_eq, (uint32_t)sub.size(),
-                               bucket, c->d_stack, (uint32_t)words, c->d_stop, use_fs, hub, g->d_hub_bits,
-                               g->hub_words, serial);
+                                 b->stream));
+        sgm::launch_fill_u32(b->stream, reinterpret_cast<uint32_t*>(b->d_st), st_words, 0);
+        sgm::launch_fill_u32(b->stream, b->d_qstop, 2 * (size_t)slots, 0);
+        HIPCHK(c, hipEventRecord(b->ev[2], b->stream));
+        sgm::launch_enum_cache(b->stream, blocks, g->d_off, g->d_adj, g->d_bydeg, b->d_eq, (uint32_t)sub.size(),
+                               bucket, c->d_stack, (uint32_t)words, c->d_stop, b->pend_use_fs, b->pend_hub,
+                               g->d_hub_bits, g->hub_words, b->pend_serial);
         HIPCHK(c, hipGetLastError());
-        HIPCHK(c, hipEventRecord(b->ev[3], c->stream));
-        std::vector<sgm_stats_dev> got2(nq);
-        HIPCHK(c, hipMemcpyAsync(got2.data(), b->d_st, sizeof(sgm_stats_dev) * sub.size(), hipMemcpyDeviceToHost,
-                                 c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipEventRecord(b->ev[3], b->stream));
+        HIPCHK(c, hipMemcpyAsync(b->h_got, b->d_st, sizeof(sgm_stats_dev) * sub.size(), hipMemcpyDeviceToHost,
+                                 b->stream));
+        HIPCHK(c, hipStreamSynchronize(b->stream));
         HIPCHK(c, hipEventElapsedTime(&ms, b->ev[2], b->ev[3]));
         b->enum_ms += ms;
         std::vector<uint32_t> next;
@@ -435,17 +496,26 @@ int sgm_batch_enumerate(sgm_batch* b, const uint32_t* const* orders, const sgm_e
             const uint32_t i = again[k];
             stats[i].kernel_ms += ms;
             stats[i].retries = attempt + 1;
-            if (got2[k].overflow) { next.push_back(i); continue; }
-            stats_out(got2[k], o.cap, &stats[i]);
+            if (b->h_got[k].overflow) { next.push_back(i); continue; }
+            stats_out(b->h_got[k], o.cap, &stats[i]);
         }
         again.swap(next);
         // the slot layout of d_eq was replaced: restore it for the captured graph
-        HIPCHK(c, hipMemcpyAsync(b->d_eq, eq.data(), sizeof(sgm_equery) * slots, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(b->d_eq, eq.data(), sizeof(sgm_equery) * slots, hipMemcpyHostToDevice, b->stream));
+        HIPCHK(c, hipStreamSynchronize(b->stream));
     }
     if (c->stop_requested)
         for (uint32_t i = 0; i < nq; i++)
             if (stats[i].stopped) return set_err(c, SGM_ESTOPPED, "stopped by sgm_stop");
     return SGM_OK;
+}
+
+int sgm_batch_enumerate(sgm_batch* b, const uint32_t* const* orders, const sgm_enum_opts* opts, sgm_stats* stats)
+{
+    if (!b || !stats) return SGM_EINVAL;
+    const int rc = sgm_batch_enumerate_launch(b, orders, opts);
+    if (rc) return rc;
+    return sgm_batch_enumerate_wait(b, stats);
 }
 
 int sgm_batch_rank_orders(sgm_batch* b, sgm_costmodel* m, uint32_t n_samples, uint64_t seed, float margin,
@@ -454,6 +524,7 @@ int sgm_batch_rank_orders(sgm_batch* b, sgm_costmodel* m, uint32_t n_samples, ui
     if (!b || !m || !orders) return SGM_EINVAL;
     sgm_ctx* c = b->c;
     if (b->nq == 0) return set_err(c, SGM_ESTATE, "filter a batch first");
+    if (b->pending) return set_err(c, SGM_ESTATE, "wait for the batch's enumeration first");
     HIPCHK(c, hipSetDevice(c->device));
     const uint32_t nq = b->nq, K = 2 + n_samples, S = nq * K;
     // RI and GQL orders on the host (cheap, and the GQL order needs the host's counts);
@@ -479,23 +550,23 @@ int sgm_batch_rank_orders(sgm_batch* b, sgm_costmodel* m, uint32_t n_samples, ui
     uint32_t *back, *fwd;
     int rc = costmodel_work_inputs(m, S, &F, &valid, &back, &fwd);
     if (rc) return rc;
-    HIPCHK(c, hipMemcpyAsync(b->d_rank_given, given.data(), given.size(), hipMemcpyHostToDevice, c->stream));
-    sgm::cm_rank_inputs(c->stream, b->d_qd, b->d_cnt, b->d_degsum, b->d_rank_given, nq, K, seed, b->d_rank_orders, F,
+    HIPCHK(c, hipMemcpyAsync(b->d_rank_given, given.data(), given.size(), hipMemcpyHostToDevice, b->stream));
+    sgm::cm_rank_inputs(b->stream, b->d_qd, b->d_cnt, b->d_degsum, b->d_rank_given, nq, K, seed, b->d_rank_orders, F,
                         valid, back, fwd);
     const float* y = nullptr;
     rc = costmodel_forward_work(m, S, &y);
     if (rc) return rc;
-    sgm::cm_choose(c->stream, y, b->d_rank_orders, nq, K, margin, b->d_rank_out, b->d_rank_pred,
+    sgm::cm_choose(b->stream, y, b->d_rank_orders, nq, K, margin, b->d_rank_out, b->d_rank_pred,
                    b->d_rank_pred + b->slots, b->d_rank_flag);
     HIPCHK(c, hipGetLastError());
     std::vector<uint32_t> out((size_t)nq * CM_ROWS);
     std::vector<float> pb(nq), pr(nq);
     std::vector<int32_t> fl(nq);
-    HIPCHK(c, hipMemcpyAsync(out.data(), b->d_rank_out, sizeof(uint32_t) * out.size(), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(pb.data(), b->d_rank_pred, sizeof(float) * nq, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(pr.data(), b->d_rank_pred + b->slots, sizeof(float) * nq, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(fl.data(), b->d_rank_flag, sizeof(int32_t) * nq, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpyAsync(out.data(), b->d_rank_out, sizeof(uint32_t) * out.size(), hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(c, hipMemcpyAsync(pb.data(), b->d_rank_pred, sizeof(float) * nq, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(c, hipMemcpyAsync(pr.data(), b->d_rank_pred + b->slots, sizeof(float) * nq, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(c, hipMemcpyAsync(fl.data(), b->d_rank_flag, sizeof(int32_t) * nq, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(c, hipStreamSynchronize(b->stream));
     for (uint32_t i = 0; i < nq; i++) {
         std::copy(out.begin() + (size_t)i * CM_ROWS, out.begin() + (size_t)i * CM_ROWS + SGM_MAXQ,
                   orders + (size_t)i * SGM_MAXQ);
@@ -520,6 +591,7 @@ int sgm_batch_autotune(sgm_batch* b, const uint32_t* const* orders, const sgm_en
     if (!b || !best) return SGM_EINVAL;
     sgm_ctx* c = b->c;
     if (b->nq == 0) return set_err(c, SGM_ESTATE, "filter a batch first");
+    if (b->pending) return set_err(c, SGM_ESTATE, "wait for the batch's enumeration first");
     const sgm_enum_opts o0 = normalize_opts(base);
     std::vector<sgm_stats> st(b->nq), ref;
     double best_t = 1e300;

@@ -387,10 +387,13 @@ __global__ void __launch_bounds__(256) k_nlf_rank(const uint32_t* __restrict__ g
     const sgm_qdesc* d = qs + blockIdx.y;
     const uint32_t n = d->n;
     if (n == 0) return;
+    __shared__ uint32_t s_P[SGM_MAXQ];  // thresholds, staged once per block
+    if (threadIdx.x < n) s_P[threadIdx.x] = ge_cnt[d->qdeg[threadIdx.x]];
+    __syncthreads();
     uint32_t* __restrict__ vmr = d->vmr;
     for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < V; x += gridDim.x * blockDim.x) {
         uint32_t m = 0;
-        for (uint32_t u = 0; u < n; u++) m |= (x < ge_cnt[d->qdeg[u]] ? 1u : 0u) << u;  // uniform loads
+        for (uint32_t u = 0; u < n; u++) m |= (x < s_P[u] ? 1u : 0u) << u;
         vmr[x] = m;
     }
 }
@@ -532,8 +535,9 @@ __global__ void __launch_bounds__(256) k_rank_scan(const sgm_cdesc* __restrict__
 // Candidate statistics of a batch without lists: cnt[q*32 + u] = |C(u)| and
 // degsum[q*32 + u] = sum of deg(v) over C(u) (the cost model's mean candidate degree).
 // Both start zeroed.  Each lane accumulates all 32 query vertices in registers over its
-// strided vertices (compile-time indices, no cross-lane work in the loop); the wave then
-// reduces once and lane 0 adds its totals with one atomic per query vertex.
+// strided vertices (compile-time indices, no cross-lane work in the loop); waves reduce
+// once, the block sums its waves in LDS and adds one atomic per query vertex (same-address
+// atomics from every wave of every block would serialise in L2).
 __global__ void __launch_bounds__(256) k_cand_stats(const uint32_t* __restrict__ off, uint32_t V,
                                                     const sgm_qdesc* __restrict__ qs, uint32_t* __restrict__ cnt,
                                                     unsigned long long* __restrict__ degsum)
@@ -558,7 +562,11 @@ __global__ void __launch_bounds__(256) k_cand_stats(const uint32_t* __restrict__
             sm[u] += bit ? dv : 0ull;
         }
     }
-    const size_t base = (size_t)blockIdx.y * SGM_MAXQ;
+    // wave sums, then block sums in LDS: one pair of atomics per block and query vertex
+    // (all blocks of a query add into the same 32 counters)
+    __shared__ uint32_t s_c[4][SGM_MAXQ];
+    __shared__ unsigned long long s_s[4][SGM_MAXQ];
+    const uint32_t wv = threadIdx.x >> 6;
 #pragma unroll
     for (int u = 0; u < SGM_MAXQ; u++) {
         uint32_t x = c[u];
@@ -568,9 +576,18 @@ __global__ void __launch_bounds__(256) k_cand_stats(const uint32_t* __restrict__
             x += (uint32_t)__shfl_xor((int)x, o);
             y += __shfl_xor(y, o);
         }
-        if (lane == 0 && x && (uint32_t)u < n) {
+        if (lane == 0) { s_c[wv][u] = x; s_s[wv][u] = (unsigned long long)y; }
+    }
+    __syncthreads();
+    const uint32_t u = threadIdx.x;
+    if (u < n) {
+        uint32_t x = 0;
+        unsigned long long y = 0;
+        for (uint32_t w = 0; w < (blockDim.x >> 6); w++) { x += s_c[w][u]; y += s_s[w][u]; }
+        const size_t base = (size_t)blockIdx.y * SGM_MAXQ;
+        if (x) {
             atomicAdd(&cnt[base + u], x);
-            atomicAdd(&degsum[base + u], (unsigned long long)y);
+            atomicAdd(&degsum[base + u], y);
         }
     }
 }
@@ -756,8 +773,8 @@ void launch_rank_adj(hipStream_t s, const uint32_t* off, const uint32_t* adj, co
 void launch_cand_stats(hipStream_t s, const uint32_t* off, uint32_t V, const sgm_qdesc* qs, uint32_t nq,
                        uint32_t* cnt, unsigned long long* degsum)
 {
-    const uint64_t W = ((uint64_t)V + 63) / 64;
-    dim3 grid(clampg((W + 3) / 4, 1, 256), nq);
+    // a few blocks per query, many vertices per thread: the per-block atomics stay few
+    dim3 grid(clampg(((uint64_t)V + 4095) / 4096, 1, 32), nq);
     hipLaunchKernelGGL(k_cand_stats, grid, dim3(256), 0, s, off, V, qs, cnt, degsum);
 }
 

@@ -39,6 +39,7 @@ struct sgm_ctx {
     double filter_ms[3] = {0, 0, 0};
     int num_cus = 256;
     volatile int stop_requested = 0;
+    int enum_inflight = 0;           // batch enumerations launched and not yet waited for
     // objects that hold this context (destroyed with it if the caller has not done so)
     std::vector<sgm_batch*> batches;
     std::vector<sgm_costmodel*> models;

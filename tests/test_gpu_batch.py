@@ -121,3 +121,32 @@ def test_batch_rmat_mixed_sizes(g_rmat):
     _check_filter(g_rmat, qs, ok, cnt, dsum)
     res = b.enumerate(cap=100000)
     assert _check_enum(g_rmat, qs, ok, res, None, 100000) > 0
+
+
+def test_batch_overlapped_runners(g_cl):
+    """Two runners of one context, pipelined: batch k+1 is filtered and launched while batch
+    k enumerates (sgm_batch_enumerate_launch / _wait); every batch's results equal the
+    oracle's, and a runner with a launch in flight refuses to load another batch."""
+    qs = _mix(g_cl, 40, seed=86, hi=18)
+    runners = [sgm.Batch(g_cl, slots=10), None]
+    runners[1] = sgm.Batch(g_cl, slots=10, ctx=runners[0].ctx)
+    parts = [qs[lo:lo + 10] for lo in range(0, len(qs), 10)]
+    pending = None
+    checked = 0
+    for k, part in enumerate(parts):
+        r = runners[k % 2]
+        say("overlapped batch", k)
+        ok, _, _ = r.filter(part)
+        r.enumerate_start(cap=100000)
+        if k == 0:
+            with pytest.raises(L.SgmError):
+                r.filter(part)  # launch in flight on this runner
+        if pending is not None:
+            pr, pok, ppart = pending
+            checked += _check_enum(g_cl, ppart, pok, pr.enumerate_wait(), None, 100000)
+        pending = (r, ok, part)
+    pr, pok, ppart = pending
+    checked += _check_enum(g_cl, ppart, pok, pr.enumerate_wait(), None, 100000)
+    assert checked > 0
+    for r in runners:
+        r.close()
