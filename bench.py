@@ -70,6 +70,8 @@ def parse():
                     help="rmat22-serve: arrival rate as a fraction of the measured capacity")
     ap.add_argument("--train-queries", type=int, default=5000,
                     help="rmat20-learned: training queries (10 logged orders each: RI, GQL, 8 sampled)")
+    ap.add_argument("--learn-label", choices=["busy", "nodes"], default="busy",
+                    help="cost-model target: wave time per query or internal search-tree nodes")
     ap.add_argument("--rank-samples", type=int, default=32,
                     help="rmat20-learned: sampled orders ranked per query besides RI and GQL")
     ap.add_argument("--queries", type=int, default=0, help="number of queries (0 = the workload's own)")
@@ -268,10 +270,11 @@ def setup_rmat_learned(args):
     import numpy as np
     import subgraphmatching_amd as sgm
     from subgraphmatching_amd.generators import query_set, rmat
-    from subgraphmatching_amd.learned import collect_training, train_model
+    from subgraphmatching_amd.learned import LABELS, collect_training, train_model
 
     g = rmat(20, 20_000_000, seed=3)
     nq = args.queries or 1000
+    label_of = LABELS[args.learn_label]
     mix = lambda count, seed: query_set(g, count, lambda i, r: int(r.integers(4, 33)),  # noqa: E731
                                         lambda i, r: i % 2 == 0, seed=seed)
     queries = mix(nq, 33)
@@ -283,18 +286,19 @@ def setup_rmat_learned(args):
     runner.filter(chunks[0])
     tuned, _ = runner.autotune(cap=args.cap)
     t0 = time.perf_counter()
-    items, labels, info = collect_training(runner, train_q, n_samples=8, cap=args.cap, seed=1, time_limit_s=5.0)
+    items, labels, info = collect_training(runner, train_q, n_samples=8, cap=args.cap, seed=1, time_limit_s=5.0,
+                                           label=args.learn_label)
     collect_s = time.perf_counter() - t0
     model, losses = train_model(items, labels, seed=7, epochs=30, minibatch=256, lr=2e-3)
     tm = model.timing()
     print(f"[bench] cost model: {len(items)} triples ({info['cut_off']} cut off) logged in {collect_s:.1f}s, "
           f"loss {losses[0]:.3f} -> {losses[-1]:.3f}, training {tm['train_ms']:.0f} ms", flush=True)
-    stats = {"learned_used": 0, "pred": [], "meas": [], "nodes": 0, "rank_s": 0.0}
+    stats = {"learned_used": 0, "pred": [], "meas": [], "nodes": 0, "busy_ns": 0, "rank_s": 0.0}
 
     def run(learned):
         rec = new_rec()
         t_start = time.perf_counter()
-        st = {"learned_used": 0, "pred": [], "meas": [], "nodes": 0, "rank_s": 0.0}
+        st = {"learned_used": 0, "pred": [], "meas": [], "nodes": 0, "busy_ns": 0, "rank_s": 0.0}
         for part in chunks:
             ok, _, _ = runner.filter(part)
             f_ms, _ = runner.last_ms()
@@ -313,9 +317,10 @@ def setup_rmat_learned(args):
                 add_result(rec, r if good else None)
                 if good:
                     st["nodes"] += sum(r.nodes[:-1])
+                    st["busy_ns"] += r.busy_ns
                     if learned:
                         st["pred"].append(float(pb[i]))
-                        st["meas"].append(float(np.log2(1.0 + sum(r.nodes[:-1]))))
+                        st["meas"].append(float(label_of(r)))
         rec["wall_s"] = time.perf_counter() - t_start
         rec["filter_wall_s"] = rec["filter_ms"] / 1e3
         return rec, st
@@ -335,6 +340,8 @@ def setup_rmat_learned(args):
                 "learned_vs_ri_speedup": ri_s / learned_s,
                 "counts_identical": all(r["counts"] == learned_counts for r, _ in ri),
                 "internal_nodes_per_step": {"ri": ri[0][1]["nodes"], "learned": stats["nodes"]},
+                "wave_busy_ms_per_step": {"ri": ri[0][1]["busy_ns"] / 1e6, "learned": stats["busy_ns"] / 1e6},
+                "label": args.learn_label,
                 "learned_orders_used_per_step": stats["learned_used"],
                 "ranking_host_s_per_step": stats["rank_s"],
                 "prediction_abs_error_log2": {"mean": float(err.mean()), "median": float(np.median(err))},
@@ -349,7 +356,8 @@ def setup_rmat_learned(args):
               "queries_per_step": nq, "cap_per_query": args.cap, "order": "learned (RI fallback, margin 0.5 log2)",
               "filter": "NLF+GQL", "enumeration": "cache+filter+failing-sets",
               "launch": f"hipGraph-replayed batches of {B}", "autotuned": tuned,
-              "training": f"{args.train_queries} queries (seed 44) x 10 orders, labels log2(1+internal nodes)",
+              "training": f"{args.train_queries} queries (seed 44) x 10 orders, labels "
+                          + ("log2(1 + wave microseconds)" if args.learn_label == "busy" else "log2(1 + internal nodes)"),
               "ranked_orders_per_query": f"RI + GQL + {args.rank_samples} sampled"}
     return g, queries, step, config, {"compare": compare}
 

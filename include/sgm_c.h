@@ -79,6 +79,8 @@ typedef struct {
     int32_t stopped;         /* sgm_stop() interrupted: count is partial */
     uint32_t retries;        /* re-runs with a larger set-op stack */
     uint32_t reserved;
+    uint64_t busy_ns;        /* cache methods: wave time spent on this query, summed over the
+                                waves (a work measure of the order; no reference counterpart) */
 } sgm_stats;
 
 /* ---- context ------------------------------------------------------------------------ */

@@ -386,6 +386,7 @@ class ExploreResult:
     capped: bool = False
     retries: int = 0
     stopped: bool = False  # ended early by a stop request: the count is partial
+    busy_ns: int = 0       # wave time spent on the query (cache methods), summed over waves
 
 
 def enumerate_query(can: CandidateParam, order, method=L.SGM_ENUM_CACHE_FILTER_FS, cap=None, blocks=0,
@@ -403,7 +404,8 @@ def enumerate_query(can: CandidateParam, order, method=L.SGM_ENUM_CACHE_FILTER_F
     res = ExploreResult(time=float(st.kernel_ms), ans=int(st.count), intersections=int(st.intersections),
                         nodes=[int(x) for x in st.nodes[:n]], fs_prunes=int(st.fs_prunes),
                         elems_read=int(st.elems_read), elems_out=int(st.elems_out),
-                        elems_scanned=int(st.elems_scanned), capped=bool(st.capped), retries=int(st.retries))
+                        elems_scanned=int(st.elems_scanned), capped=bool(st.capped), retries=int(st.retries),
+                        busy_ns=int(st.busy_ns))
     prof = Profiler.getInst()
     prof.total_intersection += res.intersections
     for d, x in enumerate(res.nodes):
@@ -438,7 +440,8 @@ def enumerate_batch(cans, orders, method=L.SGM_ENUM_CACHE_FILTER_FS, cap=None, b
         r = ExploreResult(time=float(st.kernel_ms), ans=int(st.count), intersections=int(st.intersections),
                           nodes=[int(x) for x in st.nodes[:n]], fs_prunes=int(st.fs_prunes),
                           elems_read=int(st.elems_read), elems_out=int(st.elems_out),
-                          elems_scanned=int(st.elems_scanned), capped=bool(st.capped), retries=int(st.retries))
+                          elems_scanned=int(st.elems_scanned), capped=bool(st.capped), retries=int(st.retries),
+                        busy_ns=int(st.busy_ns))
         prof.total_intersection += r.intersections
         out.append(r)
     return out
@@ -478,7 +481,7 @@ def _result(st, n):
                          nodes=[int(x) for x in st.nodes[:n]], fs_prunes=int(st.fs_prunes),
                          elems_read=int(st.elems_read), elems_out=int(st.elems_out),
                          elems_scanned=int(st.elems_scanned), capped=bool(st.capped), retries=int(st.retries),
-                         stopped=bool(st.stopped))
+                         stopped=bool(st.stopped), busy_ns=int(st.busy_ns))
 
 
 class Batch:

@@ -453,7 +453,7 @@ int sgm_batch_enumerate_wait(sgm_batch* b, sgm_stats* stats)
     for (uint32_t i = 0; i < nq; i++) {
         stats[i].kernel_ms = ms;
         if (b->h_got[i].overflow) { again.push_back(i); continue; }
-        stats_out(b->h_got[i], o.cap, &stats[i]);
+        stats_out(b->h_got[i], o.cap, &stats[i], c->wall_ns_per_tick);
         if (c->stop_requested && !stats[i].capped && b->status[i] == SGM_OK) stats[i].stopped = 1;
     }
     // set-op stack overflow (rare): rerun those queries alone with a larger stack, outside
@@ -497,7 +497,7 @@ int sgm_batch_enumerate_wait(sgm_batch* b, sgm_stats* stats)
             stats[i].kernel_ms += ms;
             stats[i].retries = attempt + 1;
             if (b->h_got[k].overflow) { next.push_back(i); continue; }
-            stats_out(b->h_got[k], o.cap, &stats[i]);
+            stats_out(b->h_got[k], o.cap, &stats[i], c->wall_ns_per_tick);
         }
         again.swap(next);
         // the slot layout of d_eq was replaced: restore it for the captured graph
@@ -554,7 +554,7 @@ int sgm_batch_rank_orders(sgm_batch* b, sgm_costmodel* m, uint32_t n_samples, ui
     sgm::cm_rank_inputs(b->stream, b->d_qd, b->d_cnt, b->d_degsum, b->d_rank_given, nq, K, seed, b->d_rank_orders, F,
                         valid, back, fwd);
     const float* y = nullptr;
-    rc = costmodel_forward_work(m, S, &y);
+    rc = costmodel_forward_work(m, S, &y, b->stream);  // same stream as the inputs and the choice
     if (rc) return rc;
     sgm::cm_choose(b->stream, y, b->d_rank_orders, nq, K, margin, b->d_rank_out, b->d_rank_pred,
                    b->d_rank_pred + b->slots, b->d_rank_flag);

@@ -36,6 +36,9 @@ int sgm_ctx_create(int device, sgm_ctx** out)
     if (hipSetDevice(device) != hipSuccess) { delete c; return SGM_EHIP; }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+    int wall_khz = 0;
+    if (hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && wall_khz > 0)
+        c->wall_ns_per_tick = 1e6 / wall_khz;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->ctl, hipStreamNonBlocking) != hipSuccess ||
         dalloc(&c->d_stop, 1) != hipSuccess || dalloc(&c->d_work, 1) != hipSuccess ||
@@ -702,8 +705,9 @@ uint32_t default_blocks(const sgm_ctx* c, const sgm_enum_opts& o, uint32_t nq)
     return std::max<uint32_t>(1, std::min<uint32_t>(b, 1u << 16));
 }
 
-void stats_out(const sgm_stats_dev& st, uint64_t cap, sgm_stats* out)
+void stats_out(const sgm_stats_dev& st, uint64_t cap, sgm_stats* out, double ns_per_tick)
 {
+    out->busy_ns = (uint64_t)((double)st.busy_ticks * ns_per_tick);
     out->count = st.count < cap ? st.count : cap;
     out->capped = st.count >= cap;
     out->intersections = st.intersections;
@@ -781,7 +785,7 @@ static int enumerate_aux(sgm_query* q, const uint32_t* order, const sgm_enum_opt
             if (attempt >= 3) return set_err(c, SGM_EDEVICE, "set-op stack overflow after %u retries", attempt);
             continue;
         }
-        stats_out(st, o.cap, out);
+        stats_out(st, o.cap, out, c->wall_ns_per_tick);
         if (c->stop_requested && !out->capped) {
             out->stopped = 1;
             return set_err(c, SGM_ESTOPPED, "stopped by sgm_stop");
@@ -885,7 +889,7 @@ int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uin
                 again.push_back(i);
                 continue;
             }
-            stats_out(got[k], o.cap, &out[i]);
+            stats_out(got[k], o.cap, &out[i], c->wall_ns_per_tick);
             if (c->stop_requested && !out[i].capped) out[i].stopped = 1;
         }
         live.swap(again);

@@ -174,11 +174,12 @@ void cast_weights(sgm_costmodel* m)
     for (int l = 1; l <= 3; l++) sgm::cm_cast(s, m->p + OW(l), CM_X, CM_H, m->wb[l - 1], m->wt[l - 1]);
 }
 
-// Forward over S samples in b; with `loss`, also the readout gradient into b.dH and the
-// output-layer gradients into m->g.
-void forward(sgm_costmodel* m, Bufs& b, uint32_t S, bool loss)
+// Forward over S samples in b on stream s (the context's unless a caller's inputs are on
+// another); with `loss`, also the readout gradient into b.dH and the output-layer gradients
+// into m->g.
+void forward(sgm_costmodel* m, Bufs& b, uint32_t S, bool loss, hipStream_t s = nullptr)
 {
-    hipStream_t s = m->c->stream;
+    if (!s) s = m->c->stream;
     const uint32_t R = S * CM_ROWS;
     sgm::cm_gemm(s, b.F, CM_F, m->w0b, CM_H, m->p + OB0, b.valid, 1, b.Z[0], CM_H, b.X[0], CM_X, nullptr, 0, R, CM_H, CM_F);
     for (int l = 1; l <= 3; l++) {
@@ -228,12 +229,12 @@ int costmodel_work_inputs(sgm_costmodel* m, uint32_t S, uint16_t** F, uint8_t** 
     return SGM_OK;
 }
 
-int costmodel_forward_work(sgm_costmodel* m, uint32_t S, const float** y)
+int costmodel_forward_work(sgm_costmodel* m, uint32_t S, const float** y, hipStream_t s)
 {
     sgm_ctx* c = m->c;
-    HIPCHK(c, hipEventRecord(m->ev[0], c->stream));
-    forward(m, m->work, S, false);
-    HIPCHK(c, hipEventRecord(m->ev[1], c->stream));
+    HIPCHK(c, hipEventRecord(m->ev[0], s));
+    forward(m, m->work, S, false, s);
+    HIPCHK(c, hipEventRecord(m->ev[1], s));
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventSynchronize(m->ev[1]));
     float ms = 0;

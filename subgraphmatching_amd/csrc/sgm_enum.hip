@@ -255,7 +255,7 @@ __device__ void last_isect(const uint32_t* __restrict__ a, uint32_t na, const ui
 // current query's plan in its own LDS slot and publishes counts and statistics to that
 // query's slot when it moves on.  A batch of one query is the single-query search.
 struct WaveAcc {
-    unsigned long long inter, fsp, elems, outs, scanned, items_done, cnt_pending;
+    unsigned long long inter, fsp, elems, outs, scanned, items_done, cnt_pending, busy;
 };
 
 // Per-depth node counters: lane d publishes (when out != null) and clears depth d, one LDS
@@ -282,9 +282,10 @@ __device__ void flush_stats(const sgm_equery& Q, WaveDFS& S, WaveAcc& a, uint32_
         if (a.outs) atomicAdd(&Q.st->elems_out, a.outs);
         if (a.scanned) atomicAdd(&Q.st->elems_scanned, a.scanned);
         if (a.items_done) atomicAdd(&Q.st->items_done, a.items_done);
+        if (a.busy) atomicAdd(&Q.st->busy_ticks, a.busy);
         if (overflow) { atomicAdd(&Q.st->overflow, 1ull); atomicExch(Q.stop, 1u); }
     }
-    a = WaveAcc{0, 0, 0, 0, 0, 0, 0};
+    a = WaveAcc{0, 0, 0, 0, 0, 0, 0, 0};
     clear_nodes(S, &Q.st->nodes[0], n);
 }
 
@@ -584,7 +585,7 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
     };
 
     clear_nodes(S, nullptr, 0);
-    WaveAcc acc = {0, 0, 0, 0, 0, 0, 0};
+    WaveAcc acc = {0, 0, 0, 0, 0, 0, 0, 0};
     uint32_t cur = SGM_INVALID;  // query whose plan sits in this wave's LDS
     uint32_t n = 0, L = 0, uL = 0, root_u = 0;
     const uint32_t* vmask = nullptr;
@@ -646,6 +647,7 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
         const uint32_t v0 = items[it];
         if (!((vmask[v0] >> root_u) & 1u)) continue;  // not (or no longer) a candidate of u0
         acc.items_done++;
+        const uint64_t t_item = wall_clock64();
         S.emb[0] = v0;
         S.nodes[0] += 1;
         S.stack_top[0] = 0;
@@ -843,6 +845,7 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
                 }
             }
         }
+        acc.busy += wall_clock64() - t_item;
         flush_count(acc.cnt_pending, &Q.st->count, stop, cap);
         if (overflow) {
             // this query's result is void (its stop word is raised); the host re-runs it

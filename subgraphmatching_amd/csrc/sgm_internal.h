@@ -57,6 +57,7 @@ struct sgm_stats_dev {
     unsigned long long elems_scanned;   // iteration-set elements classified (filter + injectivity)
     unsigned long long overflow;        // per-wave stack overflows (result invalid if > 0)
     unsigned long long items_done;
+    unsigned long long busy_ticks;      // wall-clock ticks its waves spent on the query's items
 };
 
 // One query of a batched cache + filter + failing-set enumeration.  Its depth-0 items are

@@ -38,6 +38,7 @@ struct sgm_ctx {
     hipEvent_t fev[4] = {nullptr, nullptr, nullptr, nullptr};
     double filter_ms[3] = {0, 0, 0};
     int num_cus = 256;
+    double wall_ns_per_tick = 10.0;  // device wall clock (s_memrealtime) period
     volatile int stop_requested = 0;
     int enum_inflight = 0;           // batch enumerations launched and not yet waited for
     // objects that hold this context (destroyed with it if the caller has not done so)
@@ -168,10 +169,11 @@ inline int grow(sgm_ctx* c, T** p, uint32_t& cap_now, uint32_t need)
 // Shared by the enumeration drivers (sgm_capi.cpp, sgm_batch.cpp).
 // Cost-model work buffers filled on the device, and a forward pass over them (ranking).
 int costmodel_work_inputs(sgm_costmodel* m, uint32_t S, uint16_t** F, uint8_t** valid, uint32_t** back, uint32_t** fwd);
-int costmodel_forward_work(sgm_costmodel* m, uint32_t S, const float** y);
+// forward pass over the work buffers on stream s, the stream their inputs were built on
+int costmodel_forward_work(sgm_costmodel* m, uint32_t S, const float** y, hipStream_t s);
 // Slot map of the hub bitmaps to use for opts (nullptr = binary search everywhere).
 int hub_slot_map(sgm_graph* g, const sgm_enum_opts& o, const uint32_t** out);
 int ensure_stack(sgm_ctx* c, size_t bytes);
 sgm_enum_opts normalize_opts(const sgm_enum_opts* opts);
 uint32_t default_blocks(const sgm_ctx* c, const sgm_enum_opts& o, uint32_t nq);
-void stats_out(const sgm_stats_dev& st, uint64_t cap, sgm_stats* out);
+void stats_out(const sgm_stats_dev& st, uint64_t cap, sgm_stats* out, double ns_per_tick);

@@ -37,6 +37,19 @@ def node_label(nodes) -> float:
     return math.log2(1.0 + float(sum(nodes[:-1])))
 
 
+def busy_label(result) -> float:
+    """Training target measuring what an order costs the enumerator: log2(1 + microseconds
+    of wave time spent on the query), summed over the waves (``busy_ns``).  Node counts
+    weigh every search-tree node alike, but a last-level step over a list cached at an
+    earlier depth is evaluated once per chunk of extensions while an intersection is
+    evaluated per extension, so two orders with the same node count can differ several-fold
+    in time."""
+    return math.log2(1.0 + result.busy_ns / 1000.0)
+
+
+LABELS = {"busy": busy_label, "nodes": lambda r: node_label(r.nodes)}
+
+
 class _Samples:
     """ctypes array of sgm_cm_sample with the numpy buffers it points into."""
 
@@ -163,12 +176,15 @@ def choose_orders(model: CostModel, queries, counts, degsums, cands, fallback, m
     return orders, best_p, fb_p
 
 
-def collect_training(runner, queries, n_samples=8, cap=10_000_000, seed=0, time_limit_s=5.0, log=None):
+def collect_training(runner, queries, n_samples=8, cap=10_000_000, seed=0, time_limit_s=5.0, log=None,
+                     label="busy"):
     """Training triples logged by the enumerator: for every query, its RI order, GQL order
     and ``n_samples`` sampled orders are enumerated through the batched runner (cache +
     filter + failing sets, the same search the orders are chosen for) and labelled with
-    log2(1 + search-tree nodes).  Launches cut off by the time limit leave their unfinished
-    queries out.  Returns (items, labels, info)."""
+    ``LABELS[label]``: log2(1 + wave microseconds) ("busy") or log2(1 + internal search-tree
+    nodes) ("nodes").  Launches cut off by the time limit leave their unfinished queries out.
+    Returns (items, labels, info)."""
+    label_of = LABELS[label]
     from .api import Order_RI
     items, labels = [], []
     dropped = enumerated = 0
@@ -194,7 +210,7 @@ def collect_training(runner, queries, n_samples=8, cap=10_000_000, seed=0, time_
                     continue
                 n = part[i].getVertexCnt()
                 items.append((part[i], orders[i], cnt[i, :n].copy(), dsum[i, :n].copy()))
-                labels.append(node_label(r.nodes))
+                labels.append(label_of(r))
         if log:
             log(f"collected {len(items)} samples from {lo + len(part)} queries ({dropped} cut off)")
     return items, np.asarray(labels, dtype=np.float32), {"enumerated": enumerated, "cut_off": dropped}

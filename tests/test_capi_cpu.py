@@ -58,3 +58,22 @@ def test_plan_rejects_disconnected_order():
     q = query_from_edges(4, [(0, 1), (1, 2), (2, 3)])
     with pytest.raises(L.SgmError):
         sgm.plan_export(q, [0, 2, 1, 3])
+
+
+def test_ctypes_structs_match_the_header(tmp_path):
+    """The ctypes mirrors of sgm_stats / sgm_enum_opts have the C header's size and field
+    offsets (a C program compiled against include/sgm_c.h prints them)."""
+    import shutil
+    import subprocess
+    if not shutil.which("gcc"):
+        pytest.skip("gcc not available")
+    src = tmp_path / "sizes.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "sgm_c.h"\n'
+                   'int main(void) { printf("%zu %zu %zu %zu %zu\\n", sizeof(sgm_stats), sizeof(sgm_enum_opts), '
+                   'offsetof(sgm_stats, kernel_ms), offsetof(sgm_stats, busy_ns), '
+                   'offsetof(sgm_enum_opts, hub_min_degree)); return 0; }\n')
+    exe = tmp_path / "sizes"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    assert got == [C.sizeof(L.Stats), C.sizeof(L.EnumOpts), L.Stats.kernel_ms.offset, L.Stats.busy_ns.offset,
+                   L.EnumOpts.hub_min_degree.offset]

@@ -186,6 +186,10 @@ def test_device_ranking_matches_host_features_and_keeps_counts():
     print("max |device - host| RI score", float(np.abs(host - pri).max()), flush=True)
     assert np.allclose(pri, host, atol=2e-2, rtol=1e-2)  # device features = host features
     assert np.all(pbest <= pri + 1e-6)
+    # the chosen order's score is the model's score of that order (inputs, scores and the
+    # choice are ordered on one stream)
+    best_items = [(q, orders[i], cnt[i], dsum[i]) for i, q in enumerate(qs)]
+    assert np.allclose(pbest, m.predict(best_items), atol=2e-2, rtol=1e-2)
     for q, o in zip(qs, orders):
         assert sorted(o) == list(range(q.getVertexCnt())) and _connected(q, o)
     ri_res = runner.enumerate(cap=50000)
