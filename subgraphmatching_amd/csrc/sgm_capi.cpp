@@ -61,6 +61,10 @@ void sgm_ctx_destroy(sgm_ctx* c)
     while (!c->models.empty()) sgm_costmodel_destroy(c->models.back());
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    dfree(c->d_fqd);
+    dfree(c->d_fcd);
+    dfree(c->d_fcnt);
+    if (c->h_fcnt) (void)hipHostFree(c->h_fcnt);
     dfree(c->d_stop);
     dfree(c->d_work);
     dfree(c->d_stats);
@@ -345,19 +349,35 @@ static int filter_chunk(sgm_ctx* c, sgm_graph* g, sgm_query* const* qs, uint32_t
         }
         max_n = std::max(max_n, qs[i]->q.n);
         free_tables(qs[i]);
-        dfree(qs[i]->d_list);
-        qs[i]->d_list = nullptr;
+        if (qs[i]->d_list) {
+            dfree(qs[i]->d_list);
+            qs[i]->d_list = nullptr;
+        }
         qs[i]->compacted = false;
-        HIPCHK(c, hipMemsetAsync(qs[i]->d_removed, 0, sizeof(uint32_t) * SGM_GQL_FLAG_WORDS, c->stream));
     }
-    if (kind == SGM_FILTER_GQL && V)  // touch and dirty words start clear; the rest is written before it is read
-        sgm::launch_fill_strided(c->stream, c->d_gqls + V, sgm_gql_clear_words(V), sgm_gql_scratch_words(V), nq, 0);
-    sgm_qdesc* d_qd = nullptr;
-    sgm_cdesc* d_cd = nullptr;
-    HIPCHK(c, dalloc(&d_qd, nq));
-    HIPCHK(c, dalloc(&d_cd, nq));
+    if (c->fcap < nq) {  // descriptor and count buffers, kept for later batches
+        dfree(c->d_fqd);
+        dfree(c->d_fcd);
+        dfree(c->d_fcnt);
+        if (c->h_fcnt) (void)hipHostFree(c->h_fcnt);
+        c->d_fqd = nullptr;
+        c->d_fcd = nullptr;
+        c->d_fcnt = nullptr;
+        c->h_fcnt = nullptr;
+        c->fcap = 0;
+        HIPCHK(c, dalloc(&c->d_fqd, nq));
+        HIPCHK(c, dalloc(&c->d_fcd, nq));
+        HIPCHK(c, dalloc(&c->d_fcnt, (size_t)nq * SGM_MAXQ));
+        HIPCHK(c, hipHostMalloc(reinterpret_cast<void**>(&c->h_fcnt), sizeof(uint32_t) * SGM_MAXQ * nq));
+        c->fcap = nq;
+    }
+    sgm_qdesc* d_qd = c->d_fqd;
+    sgm_cdesc* d_cd = c->d_fcd;
     HIPCHK(c, hipMemcpyAsync(d_qd, qd.data(), sizeof(sgm_qdesc) * nq, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(d_cd, cd.data(), sizeof(sgm_cdesc) * nq, hipMemcpyHostToDevice, c->stream));
+    sgm::launch_clear_flags(c->stream, d_qd, nq);
+    if (kind == SGM_FILTER_GQL && V)  // touch and dirty words start clear; the rest is written before it is read
+        sgm::launch_fill_strided(c->stream, c->d_gqls + V, sgm_gql_clear_words(V), sgm_gql_scratch_words(V), nq, 0);
     HIPCHK(c, hipEventRecord(c->fev[0], c->stream));
     if (kind == SGM_FILTER_GQL && V) sgm::launch_nlf_rank(c->stream, g->d_gecnt, V, d_qd, nq);
     else sgm::launch_nlf(c->stream, g->d_off, V, d_qd, nq);
@@ -372,14 +392,13 @@ static int filter_chunk(sgm_ctx* c, sgm_graph* g, sgm_query* const* qs, uint32_t
     sgm::launch_bitmaps(c->stream, d_cd, nq, V);
     sgm::launch_rank_scan(c->stream, d_cd, nq, V, max_n);
     HIPCHK(c, hipEventRecord(c->fev[3], c->stream));
+    sgm::launch_gather_counts(c->stream, d_cd, nq, c->d_fcnt);
     HIPCHK(c, hipGetLastError());
-    for (uint32_t i = 0; i < nq; i++)
-        HIPCHK(c, hipMemcpyAsync(qs[i]->counts, qs[i]->d_cnt, sizeof(uint32_t) * qs[i]->q.n, hipMemcpyDeviceToHost,
-                                 c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->h_fcnt, c->d_fcnt, sizeof(uint32_t) * SGM_MAXQ * nq, hipMemcpyDeviceToHost, c->stream));
     hipError_t se = hipStreamSynchronize(c->stream);
-    dfree(d_qd);
-    dfree(d_cd);
     if (se != hipSuccess) return set_err(c, SGM_EHIP, "filter: %s", hipGetErrorString(se));
+    for (uint32_t i = 0; i < nq; i++)
+        std::memcpy(qs[i]->counts, c->h_fcnt + (size_t)i * SGM_MAXQ, sizeof(uint32_t) * qs[i]->q.n);
     for (int k = 0; k < 3; k++) {
         float t = 0;
         HIPCHK(c, hipEventElapsedTime(&t, c->fev[k], c->fev[k + 1]));
@@ -571,7 +590,11 @@ int sgm_table_download(sgm_query* q, uint32_t a, uint32_t b, uint32_t* offsets, 
 // ------------------------------------------------------------------------------ orders
 static int csr_from(const uint32_t* qoff, const uint32_t* qadj, uint32_t n, sgm::QueryCSR& q)
 {
-    if (!qoff || n < 1 || n > SGM_MAXQ) return SGM_EINVAL;
+    if (!qoff || n < 1 || n > SGM_MAXQ || qoff[0] != 0 || (!qadj && qoff[n])) return SGM_EINVAL;
+    for (uint32_t u = 0; u < n; u++)
+        if (qoff[u + 1] < qoff[u]) return SGM_EINVAL;
+    for (uint32_t p = 0; p < qoff[n]; p++)
+        if (qadj[p] >= n) return SGM_EINVAL;
     q.n = n;
     q.off.assign(qoff, qoff + n + 1);
     q.adj.assign(qadj, qadj + qoff[n]);

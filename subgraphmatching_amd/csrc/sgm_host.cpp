@@ -1,6 +1,7 @@
 // Host-side query planning: RI and GQL matching orders, backward neighbours, DAF ancestor
 // masks and the intersection-cache plan.  These are O(n^2..n^3) over queries of at most 32
 // vertices and stay on the CPU, as in the reference.
+#include <unordered_map>
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
@@ -25,62 +26,55 @@ static uint32_t q_deg(const QueryCSR& q, uint32_t a) { return q.off[a + 1] - q.o
 // of the candidate touch no ordered vertex; final ties go to the lowest id.
 void order_ri(const QueryCSR& q, uint32_t* order)
 {
+    // queries have at most 32 vertices: neighbourhoods and the visited set are 32-bit masks
     const uint32_t n = q.n;
-    std::vector<char> vis(n, 0);
+    uint32_t nbr[32] = {0};
+    for (uint32_t u = 0; u < n; u++)
+        for (uint32_t p = q.off[u]; p < q.off[u + 1]; p++) nbr[u] |= 1u << q.adj[p];
     order[0] = 0;
     for (uint32_t i = 1; i < n; i++)
         if (q_deg(q, i) > q_deg(q, order[0])) order[0] = i;
-    vis[order[0]] = 1;
-    std::vector<uint32_t> ties, prev;
+    uint32_t vis = 1u << order[0];
+    uint32_t ties[32], prev[32];
     for (uint32_t i = 1; i < n; i++) {
-        ties.clear();
-        uint32_t best = 0;
+        // 1. most neighbours among the ordered vertices
+        uint32_t nt = 0, best = 0;
         for (uint32_t u = 0; u < n; u++) {
-            if (vis[u]) continue;
-            uint32_t bn = 0;
-            for (uint32_t j = 0; j < i; j++) bn += q_adj(q, u, order[j]);
-            if (bn > best) { best = bn; ties.assign(1, u); }
-            else if (bn == best) ties.push_back(u);
+            if ((vis >> u) & 1u) continue;
+            const uint32_t bn = (uint32_t)__builtin_popcount(nbr[u] & vis);
+            if (bn > best) { best = bn; nt = 0; }
+            if (bn == best) ties[nt++] = u;
         }
-        if (ties.size() != 1) {
-            prev.swap(ties);
-            ties.clear();
+        // 2. most ordered vertices sharing an unordered neighbour with u
+        if (nt != 1) {
+            const uint32_t np = nt;
+            std::copy(ties, ties + nt, prev);
+            nt = 0;
             best = 0;
-            for (uint32_t u : prev) {
-                std::vector<uint32_t> fn;
-                for (uint32_t p = q.off[u]; p < q.off[u + 1]; p++)
-                    if (!vis[q.adj[p]]) fn.push_back(q.adj[p]);
+            for (uint32_t k = 0; k < np; k++) {
+                const uint32_t u = prev[k], fn = nbr[u] & ~vis;
                 uint32_t cc = 0;
-                for (uint32_t j = 0; j < i; j++) {
-                    const uint32_t uu = order[j];
-                    bool share = false;
-                    for (uint32_t p = q.off[uu]; p < q.off[uu + 1] && !share; p++)
-                        share = std::binary_search(fn.begin(), fn.end(), q.adj[p]);
-                    cc += share;
-                }
-                if (cc > best) { best = cc; ties.assign(1, u); }
-                else if (cc == best) ties.push_back(u);
+                for (uint32_t m = vis; m; m &= m - 1) cc += (nbr[__builtin_ctz(m)] & fn) != 0u;
+                if (cc > best) { best = cc; nt = 0; }
+                if (cc == best) ties[nt++] = u;
             }
         }
-        if (ties.size() != 1) {
-            prev.swap(ties);
-            ties.clear();
+        // 3. most unordered neighbours not adjacent to any ordered vertex
+        if (nt != 1) {
+            const uint32_t np = nt;
+            std::copy(ties, ties + nt, prev);
+            nt = 0;
             best = 0;
-            for (uint32_t u : prev) {
+            for (uint32_t k = 0; k < np; k++) {
+                const uint32_t u = prev[k];
                 uint32_t cc = 0;
-                for (uint32_t p = q.off[u]; p < q.off[u + 1]; p++) {
-                    const uint32_t uu = q.adj[p];
-                    if (vis[uu]) continue;
-                    bool touches = false;
-                    for (uint32_t j = 0; j < i && !touches; j++) touches = q_adj(q, uu, order[j]);
-                    cc += !touches;
-                }
-                if (cc > best) { best = cc; ties.assign(1, u); }
-                else if (cc == best) ties.push_back(u);
+                for (uint32_t m = nbr[u] & ~vis; m; m &= m - 1) cc += (nbr[__builtin_ctz(m)] & vis) == 0u;
+                if (cc > best) { best = cc; nt = 0; }
+                if (cc == best) ties[nt++] = u;
             }
         }
         order[i] = ties[0];
-        vis[order[i]] = 1;
+        vis |= 1u << order[i];
     }
 }
 
@@ -145,18 +139,20 @@ struct OpIR {
     uint32_t depth = SGM_INVALID_U32;
     uint32_t label = 0;
     uint32_t vdepth = SGM_INVALID_U32;
-    std::vector<uint32_t> e;
+    uint32_t e = 0;  // the depth list as a set: depths are distinct and < 32
+    // depth, label, list length, then the ascending lists lexicographically.  For two sets of
+    // equal size the first differing position holds the smaller of the two lowest
+    // non-shared elements, i.e. the lowest bit of e ^ r.e, and the list holding it is less.
     bool operator<(const OpIR& r) const
     {
         if (this == &r) return false;
         if (depth != r.depth) return depth < r.depth;
         if (label != r.label) return label < r.label;
-        if (e.size() != r.e.size()) return e.size() < r.e.size();
-        for (size_t i = 0; i < e.size(); i++)
-            if (e[i] != r.e[i]) return e[i] < r.e[i];
-        return false;
+        const int sa = __builtin_popcount(e), sb = __builtin_popcount(r.e);
+        if (sa != sb) return sa < sb;
+        const uint32_t x = e ^ r.e;
+        return x != 0u && ((e >> __builtin_ctz(x)) & 1u) != 0u;
     }
-    bool same(const OpIR& r) const { return label == r.label && e == r.e; }
 };
 
 int build_plan(const QueryCSR& q, const uint32_t* order, Plan& pl)
@@ -164,59 +160,55 @@ int build_plan(const QueryCSR& q, const uint32_t* order, Plan& pl)
     const uint32_t n = q.n;
     if (n < 2 || n > 32) return -1;
     if (check_order(q, order)) return -2;
-    std::vector<uint32_t> dep_of(n);
+    uint32_t dep_of[32], back[32];  // back[u]: depths of u's neighbours, as a set
     for (uint32_t i = 0; i < n; i++) dep_of[order[i]] = i;
-    std::vector<OpIR> vir(n);
     for (uint32_t u = 0; u < n; u++) {
-        for (uint32_t p = q.off[u]; p < q.off[u + 1]; p++) vir[u].e.push_back(dep_of[q.adj[p]]);
-        std::sort(vir[u].e.begin(), vir[u].e.end());
+        back[u] = 0;
+        for (uint32_t p = q.off[u]; p < q.off[u + 1]; p++) back[u] |= 1u << dep_of[q.adj[p]];
     }
-    std::vector<OpIR> all, iter_tmp(n);
+    std::vector<OpIR> all;
+    all.reserve((size_t)n * (n - 1) / 2);
+    uint32_t iter_e[32] = {0};
     for (uint32_t dep = 0; dep + 1 < n; dep++) {
+        const uint32_t upto = dep == 31 ? ~0u : ((2u << dep) - 1u);  // depths <= dep
         for (uint32_t fd = dep + 1; fd < n; fd++) {
             OpIR o;
             o.depth = dep;
             o.vdepth = fd;
-            for (uint32_t x : vir[order[fd]].e)
-                if (x <= dep) o.e.push_back(x);
-            if (!o.e.empty()) all.push_back(o);
+            o.e = back[order[fd]] & upto;
+            if (o.e) all.push_back(o);
         }
-        OpIR it;
-        it.depth = dep;
-        it.vdepth = dep + 1;
-        for (uint32_t x : vir[order[dep + 1]].e)
-            if (x <= dep) it.e.push_back(x);
-        if (it.e.empty()) return -2;
-        iter_tmp[dep + 1] = it;
+        iter_e[dep + 1] = back[order[dep + 1]] & upto;
+        if (!iter_e[dep + 1]) return -2;
     }
     std::sort(all.begin(), all.end());
+    // duplicates (equal sets) keep the first in sorted order; parents and iteration sets
+    // are then one map probe each
     std::vector<OpIR> ops;
-    for (size_t i = 0; i < all.size(); i++) {
-        bool first = true;
-        for (size_t j = 0; j < i && first; j++) first = !all[i].same(all[j]);
-        if (first) ops.push_back(all[i]);
-    }
+    std::unordered_map<uint32_t, uint32_t> first_of;  // edge set -> op index
+    for (const OpIR& o : all)
+        if (first_of.emplace(o.e, (uint32_t)ops.size()).second) ops.push_back(o);
     if (ops.size() > SGM_MAXOPS) return -3;
     pl.n = n;
     pl.order.assign(order, order + n);
     pl.ops.resize(ops.size());
     for (size_t i = 0; i < ops.size(); i++) {
         PlanOp& po = pl.ops[i];
+        const uint32_t m = ops[i].e;
         po.depth = ops[i].depth;
         po.vdepth = ops[i].vdepth;
         po.parent = SGM_INVALID_U32;
-        po.edges = ops[i].e;
-        if (i > 0) {
-            OpIR pre = ops[i];
-            pre.e.pop_back();
-            for (size_t j = 0; j < i; j++)
-                if (pre.same(ops[j])) { po.parent = (uint32_t)j; break; }
-        }
+        po.edges.clear();
+        for (uint32_t x = m; x; x &= x - 1) po.edges.push_back((uint32_t)__builtin_ctz(x));
+        // the parent computes the list without its deepest edge
+        const auto it = first_of.find(m & ~(0x80000000u >> __builtin_clz(m)));
+        if (it != first_of.end() && it->second < i) po.parent = it->second;
     }
     pl.iter_set.assign(n, SGM_INVALID_U32);
-    for (uint32_t d = 1; d < n; d++)
-        for (size_t j = 0; j < ops.size(); j++)
-            if (iter_tmp[d].same(ops[j])) { pl.iter_set[d] = (uint32_t)j; break; }
+    for (uint32_t d = 1; d < n; d++) {
+        const auto it = first_of.find(iter_e[d]);
+        if (it != first_of.end()) pl.iter_set[d] = it->second;
+    }
     // DAF ancestors over depths (inc/prune.h:100-104)
     pl.anc.assign(n, 0);
     for (uint32_t i = 0; i < n; i++) {

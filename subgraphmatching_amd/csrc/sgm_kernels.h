@@ -37,6 +37,9 @@ void launch_nbr_deg(hipStream_t s, const uint32_t* roff, const uint32_t* radj, u
 // batch candidate statistics: cnt[q*32+u] = |C(u)|, degsum[q*32+u] = sum of degrees (zeroed first)
 void launch_cand_stats(hipStream_t s, const uint32_t* off, uint32_t V, const sgm_qdesc* qs, uint32_t nq,
                        uint32_t* cnt, unsigned long long* degsum);
+// one launch each: clear every query's flag words; gather the candidate counts (32 per query)
+void launch_clear_flags(hipStream_t s, const sgm_qdesc* qs, uint32_t nq);
+void launch_gather_counts(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t* out);
 void launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t val);
 void launch_bitmaps(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V);
 void launch_rank_scan(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V, uint32_t max_n);

@@ -592,6 +592,22 @@ __global__ void __launch_bounds__(256) k_cand_stats(const uint32_t* __restrict__
     }
 }
 
+// Per-query housekeeping of a filter batch, one launch each instead of one host call per
+// query: clear the flag words of every query (block = query) ...
+__global__ void __launch_bounds__(64) k_clear_flags(const sgm_qdesc* __restrict__ qs)
+{
+    uint32_t* f = qs[blockIdx.x].removed;
+    for (uint32_t i = threadIdx.x; i < SGM_GQL_FLAG_WORDS; i += blockDim.x) f[i] = 0;
+}
+
+// ... and gather the candidate counts into one array (query i at out[i * 32]) for a single
+// copy back.
+__global__ void __launch_bounds__(64) k_gather_counts(const sgm_cdesc* __restrict__ cs, uint32_t* __restrict__ out)
+{
+    const sgm_cdesc* c = cs + blockIdx.x;
+    if (threadIdx.x < SGM_MAXQ) out[(size_t)blockIdx.x * SGM_MAXQ + threadIdx.x] = threadIdx.x < c->n ? c->cnt[threadIdx.x] : 0u;
+}
+
 // Strided word fill: rep blockIdx.y clears p[rep * stride ... + len) (per-query scratch).
 __global__ void __launch_bounds__(256) k_fill_strided(uint32_t* __restrict__ p, size_t len, size_t stride,
                                                       uint32_t val)
@@ -776,6 +792,16 @@ void launch_cand_stats(hipStream_t s, const uint32_t* off, uint32_t V, const sgm
     // a few blocks per query, many vertices per thread: the per-block atomics stay few
     dim3 grid(clampg(((uint64_t)V + 4095) / 4096, 1, 32), nq);
     hipLaunchKernelGGL(k_cand_stats, grid, dim3(256), 0, s, off, V, qs, cnt, degsum);
+}
+
+void launch_clear_flags(hipStream_t s, const sgm_qdesc* qs, uint32_t nq)
+{
+    if (nq) hipLaunchKernelGGL(k_clear_flags, dim3(nq), dim3(64), 0, s, qs);
+}
+
+void launch_gather_counts(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t* out)
+{
+    if (nq) hipLaunchKernelGGL(k_gather_counts, dim3(nq), dim3(64), 0, s, cs, out);
 }
 
 void launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t val)

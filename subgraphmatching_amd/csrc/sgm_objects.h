@@ -41,6 +41,12 @@ struct sgm_ctx {
     double wall_ns_per_tick = 10.0;  // device wall clock (s_memrealtime) period
     volatile int stop_requested = 0;
     int enum_inflight = 0;           // batch enumerations launched and not yet waited for
+    // filter batches: descriptors and the gathered candidate counts (device, pinned host)
+    sgm_qdesc* d_fqd = nullptr;
+    sgm_cdesc* d_fcd = nullptr;
+    uint32_t* d_fcnt = nullptr;
+    uint32_t* h_fcnt = nullptr;
+    uint32_t fcap = 0;
     // objects that hold this context (destroyed with it if the caller has not done so)
     std::vector<sgm_batch*> batches;
     std::vector<sgm_costmodel*> models;
