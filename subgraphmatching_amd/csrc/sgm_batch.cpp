@@ -42,6 +42,7 @@ struct sgm_batch {
     sgm_equery* d_eq = nullptr;              // slots
     sgm_stats_dev* d_st = nullptr;           // slots
     uint32_t* d_qstop = nullptr;             // 2 * slots: stop words, then item counters
+    uint32_t* d_lastbits = nullptr;          // slots * ceil(V/32): bitmap of C(u_L) per slot
     uint32_t* d_stack = nullptr;
     size_t stack_words_total = 0;
     // learned-order ranking: given RI / GQL orders, candidate orders, the choice per query
@@ -165,6 +166,7 @@ void sgm_batch_destroy(sgm_batch* b)
     dfree(b->d_eq);
     dfree(b->d_st);
     dfree(b->d_qstop);
+    dfree(b->d_lastbits);
     dfree(b->d_stack);
     dfree(b->d_rank_given);
     dfree(b->d_rank_orders);
@@ -197,6 +199,7 @@ int sgm_batch_create(sgm_graph* g, uint32_t slots, sgm_batch** out)
     if (e == hipSuccess) e = dalloc(&b->d_eq, slots);
     if (e == hipSuccess) e = dalloc(&b->d_st, slots);
     if (e == hipSuccess) e = dalloc(&b->d_qstop, 2 * (size_t)slots);
+    if (e == hipSuccess) e = dalloc(&b->d_lastbits, (size_t)slots * ((V + 31) / 32));
     if (e == hipSuccess) e = dalloc(&b->d_rank_given, (size_t)slots * 2 * CM_ROWS);
     if (e == hipSuccess) e = dalloc(&b->d_rank_out, (size_t)slots * CM_ROWS);
     if (e == hipSuccess) e = dalloc(&b->d_rank_pred, 2 * (size_t)slots);
@@ -343,6 +346,7 @@ int sgm_batch_enumerate_launch(sgm_batch* b, const uint32_t* const* orders, cons
         std::memset(&e, 0, sizeof e);
         e.plan = b->d_plans + std::min(i, nq - 1);
         e.vmask = b->d_vmask + (size_t)i * V;
+        e.lastbits = b->d_lastbits + (size_t)i * ((V + 31) / 32);
         e.stop = b->d_qstop + i;
         e.next = b->d_qstop + slots + i;
         e.st = b->d_st + i;
@@ -407,6 +411,7 @@ int sgm_batch_enumerate_launch(sgm_batch* b, const uint32_t* const* orders, cons
         rc = capture(c, b->stream, [&] {
             sgm::launch_fill_u32(b->stream, reinterpret_cast<uint32_t*>(b->d_st), st_words, 0);
             sgm::launch_fill_u32(b->stream, b->d_qstop, 2 * (size_t)slots, 0);
+            sgm::launch_last_bits(b->stream, b->d_eq, slots, V);
             sgm::launch_enum_cache(b->stream, blocks, g->d_off, g->d_adj, g->d_bydeg, b->d_eq, slots, bucket, b->d_stack,
                                    (uint32_t)words, c->d_stop, use_fs, hub, g->d_hub_bits, g->hub_words, serial);
         }, &exec);

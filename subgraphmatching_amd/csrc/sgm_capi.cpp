@@ -65,6 +65,7 @@ void sgm_ctx_destroy(sgm_ctx* c)
     dfree(c->d_fcd);
     dfree(c->d_fcnt);
     if (c->h_fcnt) (void)hipHostFree(c->h_fcnt);
+    dfree(c->d_lastbits);
     dfree(c->d_stop);
     dfree(c->d_work);
     dfree(c->d_stats);
@@ -867,6 +868,14 @@ int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uin
     if ((rc = hub_slot_map(g, o, &hub_slot))) return rc;
     std::vector<sgm_stats_dev> st(nq);
     std::vector<sgm_equery> eq;
+    const size_t lb_words = ((size_t)g->V + 31) / 32;
+    if (c->lastbits_words < lb_words * live.size()) {
+        dfree(c->d_lastbits);
+        c->d_lastbits = nullptr;
+        c->lastbits_words = 0;
+        HIPCHK(c, dalloc(&c->d_lastbits, lb_words * live.size()));
+        c->lastbits_words = lb_words * live.size();
+    }
     for (uint32_t attempt = 0; !live.empty(); attempt++) {
         if (attempt > 3) return set_err(c, SGM_EDEVICE, "set-op stack overflow after %u retries", attempt - 1);
         if (words > 0x7FFFFFFFull) return set_err(c, SGM_EDEVICE, "set-op stack would exceed 2^31 words");
@@ -880,6 +889,7 @@ int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uin
             const uint32_t u0 = orders[i][0];
             eq[k].plan = c->d_plans + i;
             eq[k].vmask = q->d_vmask;
+            eq[k].lastbits = c->d_lastbits + lb_words * k;
             eq[k].stop = c->d_qstop + k;       // stop words: [0, m)
             eq[k].next = c->d_qstop + m + k;   // item counters: [m, 2m)
             eq[k].st = c->d_qstats + k;
@@ -893,6 +903,7 @@ int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uin
         HIPCHK(c, hipMemsetAsync(c->d_stop, 0, sizeof(uint32_t), c->stream));
         c->stop_requested = 0;
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+        sgm::launch_last_bits(c->stream, c->d_eq, m, g->V);
         sgm::launch_enum_cache(c->stream, blocks, g->d_off, g->d_adj, g->d_bydeg, c->d_eq, m, max_ops, c->d_stack,
                                (uint32_t)words, c->d_stop, use_fs, hub_slot, g->d_hub_bits, g->hub_words, serial);
         HIPCHK(c, hipGetLastError());

@@ -154,6 +154,12 @@ __device__ __forceinline__ void flush_count(unsigned long long& pending, unsigne
     pending = 0;
 }
 
+// bit v of a per-query bitmap (the last query vertex's candidates)
+__device__ __forceinline__ bool last_bit(const uint32_t* __restrict__ lb, uint32_t v)
+{
+    return ((lb[v >> 5] >> (v & 31u)) & 1u) != 0u;
+}
+
 struct LastRes {
     uint32_t size;      // iteration-set size
     uint32_t ok;        // extensions (embeddings)
@@ -164,14 +170,14 @@ struct LastRes {
 // Classify UNR elements per lane at the last depth L: filter bit of u_L, then injectivity
 // against the L mapped vertices.
 __device__ __forceinline__ void classify_last4(const bool (&valid)[UNR], const uint32_t (&w)[UNR],
-                                               const uint32_t* __restrict__ vmask, uint32_t uL, const WaveDFS& S,
+                                               const uint32_t* __restrict__ lbits, const WaveDFS& S,
                                                const uint64_t* anc, uint32_t L, LastRes& r)
 {
     bool f[UNR];
     uint32_t c[UNR];
 #pragma unroll
     for (int u = 0; u < UNR; u++) {
-        f[u] = valid[u] && ((vmask[w[u]] >> uL) & 1u);
+        f[u] = valid[u] && last_bit(lbits, w[u]);
         c[u] = SGM_INVALID;
     }
     for (uint32_t j = 0; j < L; j++) {
@@ -192,8 +198,8 @@ __device__ __forceinline__ void classify_last4(const bool (&valid)[UNR], const u
 }
 
 // Last level, scanning an existing list.
-__device__ void last_scan(const uint32_t* __restrict__ list, uint32_t size, const uint32_t* __restrict__ vmask,
-                          uint32_t uL, const WaveDFS& S, const uint64_t* anc, uint32_t L, LastRes& r,
+__device__ void last_scan(const uint32_t* __restrict__ list, uint32_t size, const uint32_t* __restrict__ lbits,
+                          const WaveDFS& S, const uint64_t* anc, uint32_t L, LastRes& r,
                           const uint32_t* stop)
 {
     const uint32_t lane = lane_id();
@@ -207,7 +213,7 @@ __device__ void last_scan(const uint32_t* __restrict__ list, uint32_t size, cons
             valid[u] = i < size;
             w[u] = valid[u] ? list[i] : 0u;
         }
-        classify_last4(valid, w, vmask, uL, S, anc, L, r);
+        classify_last4(valid, w, lbits, S, anc, L, r);
         if (((base >> 8) & 7u) == 7u && __atomic_load_n(stop, __ATOMIC_RELAXED)) break;
     }
 }
@@ -215,7 +221,7 @@ __device__ void last_scan(const uint32_t* __restrict__ list, uint32_t size, cons
 // Last level, intersecting (parent op) with N(v) and classifying in the same pass.
 __device__ void last_isect(const uint32_t* __restrict__ a, uint32_t na, const uint32_t* __restrict__ abits,
                            const uint32_t* __restrict__ b, uint32_t nb, const uint32_t* __restrict__ bbits,
-                           const uint32_t* __restrict__ vmask, uint32_t uL, const WaveDFS& S,
+                           const uint32_t* __restrict__ lbits, const WaveDFS& S,
                            const uint64_t* anc, uint32_t L, LastRes& r, unsigned long long& elems,
                            const uint32_t* stop)
 {
@@ -240,7 +246,7 @@ __device__ void last_isect(const uint32_t* __restrict__ a, uint32_t na, const ui
         member4(b, nb, bbits, key, valid, hit);
 #pragma unroll
         for (int u = 0; u < UNR; u++) size += (uint32_t)__popcll(__ballot(hit[u]));
-        classify_last4(hit, key, vmask, uL, S, anc, L, r);
+        classify_last4(hit, key, lbits, S, anc, L, r);
         if (((base >> 8) & 7u) == 7u && __atomic_load_n(stop, __ATOMIC_RELAXED)) break;
     }
     r.size = size;
@@ -254,6 +260,22 @@ __device__ void last_isect(const uint32_t* __restrict__ a, uint32_t na, const ui
 // work -- and waves flow to the queries that still have roots.  A wave keeps a copy of its
 // current query's plan in its own LDS slot and publishes counts and statistics to that
 // query's slot when it moves on.  A batch of one query is the single-query search.
+// The last level tests only bit u_L of the candidate words: a bitmap of C(u_L) (V/8 bytes
+// per query instead of 4·V) keeps those probes in L2.  Wave per 64 vertices, one ballot.
+__global__ void __launch_bounds__(256) k_last_bits(const sgm_equery* __restrict__ qs, uint32_t V)
+{
+    const sgm_equery& Q = qs[blockIdx.y];
+    const uint32_t uL = Q.plan->order[Q.plan->n - 1];
+    const uint32_t W = (V + 31) / 32, lane = lane_id();
+    const uint32_t waves = blockDim.x >> 6;
+    for (uint32_t g = blockIdx.x * waves + (threadIdx.x >> 6); g * 64 < V; g += gridDim.x * waves) {
+        const uint32_t v = g * 64 + lane;
+        const uint64_t b = __ballot(v < V && ((Q.vmask[v] >> uL) & 1u));
+        if (lane == 0) Q.lastbits[2 * g] = (uint32_t)b;
+        if (lane == 1 && 2 * g + 1 < W) Q.lastbits[2 * g + 1] = (uint32_t)(b >> 32);
+    }
+}
+
 struct WaveAcc {
     unsigned long long inter, fsp, elems, outs, scanned, items_done, cnt_pending, busy;
 };
@@ -393,7 +415,7 @@ __device__ void last_chunk(const sgm_plan_dev& P, WaveDFS& S, uint32_t d, const 
                            const uint32_t* __restrict__ ws, const uint32_t* op_off, const uint32_t* op_size,
                            const uint32_t* op_vtx, const uint32_t* __restrict__ hub_slot,
                            const uint32_t* __restrict__ hub_bits, uint32_t hub_words,
-                           const uint32_t* __restrict__ vmask, uint32_t uL, int use_fs, WaveAcc& acc,
+                           const uint32_t* __restrict__ lbits, int use_fs, WaveAcc& acc,
                            const uint32_t* stop)
 {
     const uint32_t lane = lane_id();
@@ -440,7 +462,7 @@ __device__ void last_chunk(const sgm_plan_dev& P, WaveDFS& S, uint32_t d, const 
                 w[u] = f[u] ? cl[i] : 0u;
             }
 #pragma unroll
-            for (int u = 0; u < UNR; u++) f[u] = f[u] && ((vmask[w[u]] >> uL) & 1u);
+            for (int u = 0; u < UNR; u++) f[u] = f[u] && last_bit(lbits, w[u]);
 #pragma unroll
             for (int u = 0; u < UNR; u++) {
                 uint32_t c = SGM_INVALID;
@@ -455,7 +477,7 @@ __device__ void last_chunk(const sgm_plan_dev& P, WaveDFS& S, uint32_t d, const 
             if (((base >> 8) & 7u) == 7u && __atomic_load_n(stop, __ATOMIC_RELAXED)) break;
         }
         // sibling v: an extension w == v of the cached list becomes a conflict at depth d
-        const bool self = mine && lower_found(cl, cn, v) && ((vmask[v] >> uL) & 1u);
+        const bool self = mine && lower_found(cl, cn, v) && last_bit(lbits, v);
         r.ok = b_ok - (self ? 1u : 0u);
         r.size = cn;
         r.any = b_any;
@@ -509,7 +531,7 @@ __device__ void last_chunk(const sgm_plan_dev& P, WaveDFS& S, uint32_t d, const 
             }
             bool filt[PU];
 #pragma unroll
-            for (int u = 0; u < PU; u++) filt[u] = hit[u] && ((vmask[key[u]] >> uL) & 1u);
+            for (int u = 0; u < PU; u++) filt[u] = hit[u] && last_bit(lbits, key[u]);
 #pragma unroll
             for (int u = 0; u < PU; u++) {
                 uint32_t c = SGM_INVALID;
@@ -587,7 +609,8 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
     clear_nodes(S, nullptr, 0);
     WaveAcc acc = {0, 0, 0, 0, 0, 0, 0, 0};
     uint32_t cur = SGM_INVALID;  // query whose plan sits in this wave's LDS
-    uint32_t n = 0, L = 0, uL = 0, root_u = 0;
+    uint32_t n = 0, L = 0, root_u = 0;
+    const uint32_t* lastbits = nullptr;  // bitmap of C(u_L) of the current query
     const uint32_t* vmask = nullptr;
     uint32_t* stop = nullptr;
     bool overflow = false;
@@ -638,9 +661,9 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
             cur = q;
             n = P.n;
             L = n - 1;
-            uL = P.order[L];
             root_u = Q.root_u;
             vmask = Q.vmask;
+            lastbits = Q.lastbits;
             stop = Q.stop;
         }
         const unsigned long long cap = Q.cap;
@@ -659,7 +682,7 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
             const uint32_t o = P.iter_set[1];
             op_off[o] = off[v0];
             op_size[o] = off[v0 + 1] - off[v0];
-            last_scan(adj + off[v0], op_size[o], vmask, uL, S, P.anc, L, r, stop);
+            last_scan(adj + off[v0], op_size[o], lastbits, S, P.anc, L, r, stop);
             acc.scanned += r.size;
             acc.cnt_pending += r.ok;
             S.nodes[L] += r.ok;
@@ -743,7 +766,7 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
                     if (d == n - 2 && !serial_last) {
                         // the last level of every remaining extension of this chunk
                         last_chunk(P, S, d, list, off, adj, ws, op_off, op_size, op_vtx, hub_slot, hub_bits,
-                                   hub_words, vmask, uL, use_fs, acc, stop);
+                                   hub_words, lastbits, use_fs, acc, stop);
                         if (acc.cnt_pending >= FLUSH_AT) flush_count(acc.cnt_pending, &Q.st->count, stop, cap);
                         continue;
                     }
@@ -764,18 +787,18 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
                             const uint32_t o = P.op_first[d];
                             const uint32_t p = P.op_parent[o];
                             if (p == SGM_NOPARENT) {
-                                last_scan(adj + nbo, nbs, vmask, uL, S, P.anc, L, r, stop);
+                                last_scan(adj + nbo, nbs, lastbits, S, P.anc, L, r, stop);
                             } else {
                                 acc.inter++;
                                 last_isect(op_list(op_off[p], adj, ws), op_size[p],
                                            op_vtx[p] != SGM_INVALID ? vbits(op_vtx[p]) : nullptr, adj + nbo, nbs,
-                                           vbits(v), vmask, uL, S, P.anc, L, r, acc.elems, stop);
+                                           vbits(v), lastbits, S, P.anc, L, r, acc.elems, stop);
                                 acc.outs += r.size;
                             }
                             if (r.size == 0) { empty = true; conf_v = P.op_vdepth[o]; }
                         } else {
                             const uint32_t o = P.iter_set[L];
-                            last_scan(op_list(op_off[o], adj, ws), op_size[o], vmask, uL, S, P.anc, L, r, stop);
+                            last_scan(op_list(op_off[o], adj, ws), op_size[o], lastbits, S, P.anc, L, r, stop);
                         }
                         acc.cnt_pending += r.ok;
                         if (acc.cnt_pending >= FLUSH_AT) flush_count(acc.cnt_pending, &Q.st->count, stop, cap);
@@ -1078,6 +1101,14 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
 namespace sgm {
 
 size_t enum_smem_bytes(uint32_t max_ops) { return 4 * enum_wave_bytes(max_ops); }
+
+void launch_last_bits(hipStream_t s, const sgm_equery* qs, uint32_t nq, uint32_t V)
+{
+    if (!nq || !V) return;
+    const uint32_t groups = (V + 63) / 64;
+    dim3 grid((groups + 3) / 4 < 1024 ? (groups + 3) / 4 : 1024, nq);
+    hipLaunchKernelGGL(k_last_bits, grid, dim3(256), 0, s, qs, V);
+}
 
 void launch_enum_cache(hipStream_t s, uint32_t blocks, const uint32_t* off, const uint32_t* adj, const uint32_t* items,
                        const sgm_equery* qs, uint32_t nq, uint32_t max_ops, uint32_t* stack, uint32_t stack_words,

@@ -65,6 +65,8 @@ struct sgm_stats_dev {
 struct sgm_equery {
     const sgm_plan_dev* plan;
     const uint32_t* vmask;
+    uint32_t* lastbits;             // ceil(V/32) words: bit v iff v in C(u_L), u_L = the plan's last
+                                    // vertex (k_last_bits, before the search)
     uint32_t* stop;                 // raised when the cap is reached (or the run overflowed)
     sgm_stats_dev* st;
     uint32_t* next;                 // next depth-0 item of this query

@@ -47,6 +47,9 @@ struct sgm_ctx {
     uint32_t* d_fcnt = nullptr;
     uint32_t* h_fcnt = nullptr;
     uint32_t fcap = 0;
+    // batched enumeration: bitmap of C(u_L) per query
+    uint32_t* d_lastbits = nullptr;
+    size_t lastbits_words = 0;
     // objects that hold this context (destroyed with it if the caller has not done so)
     std::vector<sgm_batch*> batches;
     std::vector<sgm_costmodel*> models;
