@@ -131,7 +131,7 @@ __device__ bool lsm_wave(const uint32_t* __restrict__ nb, uint32_t deg, const ui
 //     of u (dirty bit u, set by removals in (1, u') for u < u' and in (2, u') for u > u');
 //     otherwise every input is unchanged and v passes again.
 // Summary words (OR over groups of SGM_GQL_GROUP ranks) of the failures, touch and dirty
-// bits let a phase stream half a byte per rank and open only the groups that hold work.
+// bits let a phase stream a summary word per group and open only the groups that hold work.
 // Matchers: lsm_lane (one lane, <= GQL_LANE_DEG neighbours: a greedy matching that
 // saturates the left side proves true; a left vertex no right vertex covers, or fewer
 // useful right vertices than left vertices, proves false when the whole list was seen),
@@ -332,9 +332,9 @@ __global__ void __launch_bounds__(256) k_gql_phase(const uint32_t* __restrict__ 
         }
         remove_now(rm, xv);
     };
-    // 64 summary words per wave step; the groups whose word holds bit u are then opened 8 at
-    // a time, 8 lanes per group (coalesced rank segments)
-    const uint32_t sub = lane / SGM_GQL_GROUP;  // which of the 8 selected groups
+    // 64 summary words per wave step; the groups whose word holds bit u are then opened
+    // 64 / SGM_GQL_GROUP at a time, SGM_GQL_GROUP lanes per group (coalesced rank segments)
+    const uint32_t sub = lane / SGM_GQL_GROUP;  // which of the selected groups
     for (uint32_t base = w0 * 64; base < ng; base += stride * 64) {
         const uint32_t gi = base + lane;
         const uint32_t sw = gi < ng ? (sumA[gi] | (sumB ? sumB[gi] : 0u)) : 0u;

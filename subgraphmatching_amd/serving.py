@@ -29,12 +29,12 @@ def admit_slots(num_vertices: int, max_degree: int, mem_budget_bytes: int, waves
     """Largest slot count whose device state fits the budget together with the DFS stacks.
 
     Per slot: membership words by id plus the GQL scratch (rank-space membership, touch /
-    dirty marks and NLF-test failures, 4 words per data vertex, and three summary words per 8
+    dirty marks and NLF-test failures, 4 words per data vertex, and three summary words per 4
     vertices) and the last query vertex's candidate bitmap (one bit per data vertex), i.e.
-    under 22 bytes per data vertex, plus descriptors; per wave: the set-op stack
+    under 24 bytes per data vertex, plus descriptors; per wave: the set-op stack
     the enumeration reserves (min(8 * max_degree + 1024, 2^20) words).  Returns (slots,
     stack_bytes, per_slot_bytes)."""
-    per_slot = 22 * num_vertices + (1 << 14)
+    per_slot = 24 * num_vertices + (1 << 14)
     stack = waves * 4 * min(8 * max_degree + 1024, 1 << 20)
     free = mem_budget_bytes - stack
     if free < per_slot:

@@ -57,7 +57,7 @@ def test_poisson_arrivals_rate_and_seed():
 
 def test_admission_by_memory():
     slots, stack, per = admit_slots(1 << 22, 169000, mem_budget_bytes=32 << 30, waves=4096)
-    assert stack == 4096 * 4 * (1 << 20) and per == 22 * (1 << 22) + (1 << 14)
+    assert stack == 4096 * 4 * (1 << 20) and per == 24 * (1 << 22) + (1 << 14)
     assert slots == min(64, ((32 << 30) - stack) // per)
     with pytest.raises(ValueError):
         admit_slots(1 << 22, 169000, mem_budget_bytes=stack, waves=4096)
