@@ -415,7 +415,7 @@ __device__ void last_chunk(const sgm_plan_dev& P, WaveDFS& S, uint32_t d, const 
                            const uint32_t* __restrict__ ws, const uint32_t* op_off, const uint32_t* op_size,
                            const uint32_t* op_vtx, const uint32_t* __restrict__ hub_slot,
                            const uint32_t* __restrict__ hub_bits, uint32_t hub_words,
-                           const uint32_t* __restrict__ lbits, int use_fs, WaveAcc& acc,
+                           const uint32_t* __restrict__ lbits, uint32_t embv, int use_fs, WaveAcc& acc,
                            const uint32_t* stop)
 {
     const uint32_t lane = lane_id();
@@ -467,7 +467,7 @@ __device__ void last_chunk(const sgm_plan_dev& P, WaveDFS& S, uint32_t d, const 
             for (int u = 0; u < UNR; u++) {
                 uint32_t c = SGM_INVALID;
                 for (uint32_t j = 0; j < d; j++)
-                    if (S.emb[j] == w[u]) c = j;
+                    if (rdlane(embv, j) == w[u]) c = j;
                 const bool ok = f[u] && c == SGM_INVALID, cf = f[u] && c != SGM_INVALID;
                 const uint64_t okb = __ballot(ok), cfb = __ballot(cf);
                 b_ok += (uint32_t)__popcll(okb);
@@ -523,6 +523,10 @@ __device__ void last_chunk(const sgm_plan_dev& P, WaveDFS& S, uint32_t d, const 
                 tbits[u] = okv ? pbits
                                : (oslot == SGM_INVALID ? nullptr : hub_bits + (size_t)oslot * hub_words);
             }
+            // the filter words, loaded with the keys: they do not depend on the membership test
+            uint32_t lw[PU];
+#pragma unroll
+            for (int u = 0; u < PU; u++) lw[u] = valid[u] ? lbits[key[u] >> 5] : 0u;
             if (mode == 1) {
 #pragma unroll
                 for (int u = 0; u < PU; u++) hit[u] = valid[u];
@@ -531,13 +535,13 @@ __device__ void last_chunk(const sgm_plan_dev& P, WaveDFS& S, uint32_t d, const 
             }
             bool filt[PU];
 #pragma unroll
-            for (int u = 0; u < PU; u++) filt[u] = hit[u] && last_bit(lbits, key[u]);
+            for (int u = 0; u < PU; u++) filt[u] = hit[u] && ((lw[u] >> (key[u] & 31u)) & 1u) != 0u;
 #pragma unroll
             for (int u = 0; u < PU; u++) {
                 uint32_t c = SGM_INVALID;
                 if (filt[u]) {
                     for (uint32_t j = 0; j < d; j++)
-                        if (S.emb[j] == key[u]) c = j;
+                        if (rdlane(embv, j) == key[u]) c = j;
                     if (key[u] == ov[u]) c = d;
                 }
                 sib_accumulate(r, head[u], own[u], hit[u], filt[u], c, P.anc, L);
@@ -611,6 +615,7 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
     uint32_t cur = SGM_INVALID;  // query whose plan sits in this wave's LDS
     uint32_t n = 0, L = 0, root_u = 0;
     const uint32_t* lastbits = nullptr;  // bitmap of C(u_L) of the current query
+    uint32_t embv = 0;  // lane j: the vertex mapped at depth j (as S.emb[j]; readlane is cheaper than LDS)
     const uint32_t* vmask = nullptr;
     uint32_t* stop = nullptr;
     bool overflow = false;
@@ -672,6 +677,7 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
         acc.items_done++;
         const uint64_t t_item = wall_clock64();
         S.emb[0] = v0;
+        embv = lane == 0 ? v0 : embv;
         S.nodes[0] += 1;
         S.stack_top[0] = 0;
         bool aborted = false;
@@ -752,7 +758,7 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
                         uint32_t c = SGM_INVALID;
                         if (f)
                             for (uint32_t j = 0; j < d; j++)
-                                if (S.emb[j] == w) c = j;
+                                if (rdlane(embv, j) == w) c = j;
                         const uint64_t okb = __ballot(f && c == SGM_INVALID);
                         const uint64_t cfb = __ballot(f && c != SGM_INVALID);
                         if (okb | cfb) S.flags[d] |= F_ANY;
@@ -766,7 +772,7 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
                     if (d == n - 2 && !serial_last) {
                         // the last level of every remaining extension of this chunk
                         last_chunk(P, S, d, list, off, adj, ws, op_off, op_size, op_vtx, hub_slot, hub_bits,
-                                   hub_words, lastbits, use_fs, acc, stop);
+                                   hub_words, lastbits, embv, use_fs, acc, stop);
                         if (acc.cnt_pending >= FLUSH_AT) flush_count(acc.cnt_pending, &Q.st->count, stop, cap);
                         continue;
                     }
@@ -776,6 +782,7 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
                     S.rem[d] = rm & (rm - 1);
                     const uint32_t v = list[S.chunk_base[d] + j];
                     S.emb[d] = v;
+                    embv = lane == d ? v : embv;
                     S.nodes[d] += 1;
                     const uint32_t nbo = off[v], nbs = off[v + 1] - nbo;
                     if (d == n - 2) {
