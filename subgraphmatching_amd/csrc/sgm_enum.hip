@@ -549,33 +549,44 @@ __device__ void last_chunk(const sgm_plan_dev& P, WaveDFS& S, uint32_t d, const 
             if (((base >> 8) & 7u) == 7u && __atomic_load_n(stop, __ATOMIC_RELAXED)) break;
         }
     }
-    // apply the siblings' results in order (inc/explore.h:1036-1066 per extension)
-    for (uint32_t i = 0; i < m; i++) {
-        S.emb[d] = rdlane(v, i);
-        S.nodes[d] += 1;
-        const uint32_t ok_i = rdlane(r.ok, i), size_i = rdlane(r.size, i), any_i = rdlane(r.any, i);
-        if (mode == 0) {
-            acc.inter++;
-            acc.elems += pn + rdlane(nbs, i);
-            acc.outs += size_i;
-        }
-        acc.cnt_pending += ok_i;
-        S.nodes[L] += ok_i;
-        acc.scanned += size_i;
-        if (use_fs) {
-            uint64_t f;
-            if (mode != 2 && size_i == 0) f = P.anc[P.op_vdepth[o]];
-            else if (!any_i) f = P.anc[L];
-            else if (ok_i) f = ~0ull;
-            else f = ((uint64_t)rdlane((uint32_t)(r.conf >> 32), i) << 32) | rdlane((uint32_t)r.conf, i);
-            S.fail[d] = f;
-            if (!((f >> d) & 1ull)) {
-                S.fail[d - 1] = f;
-                S.flags[d] |= F_PRUNED;
-                acc.fsp++;
-                break;
-            }
-            S.fail[d - 1] |= f;
+    // apply the siblings' results in order (inc/explore.h:1036-1066 per extension), all lanes
+    // at once: sibling i's failing mask f_i; the first sibling whose mask omits depth d cuts
+    // the rest (it and those before it count); the parent's mask is the OR of the counted
+    // masks, or the cutting sibling's mask alone
+    uint64_t f = 0;
+    if (use_fs && mine) {
+        if (mode != 2 && r.size == 0) f = P.anc[P.op_vdepth[o]];
+        else if (!r.any) f = P.anc[L];
+        else if (r.ok) f = ~0ull;
+        else f = r.conf;
+    }
+    const uint64_t cut = __ballot(use_fs && mine && !((f >> d) & 1ull));
+    const uint32_t last = cut ? (uint32_t)__ffsll((long long)cut) - 1u : m - 1u;  // last counted sibling
+    const bool counted = mine && lane <= last;
+    const uint32_t ok_sum = rdlane(wave_incl_scan(counted ? r.ok : 0u), 63);
+    const uint32_t size_sum = rdlane(wave_incl_scan(counted ? r.size : 0u), 63);
+    const uint32_t n_counted = last + 1u;
+    S.emb[d] = rdlane(v, last);
+    S.nodes[d] += n_counted;
+    S.nodes[L] += ok_sum;
+    acc.cnt_pending += ok_sum;
+    acc.scanned += size_sum;
+    if (mode == 0) {
+        acc.inter += n_counted;
+        acc.elems += (unsigned long long)pn * n_counted + rdlane(wave_incl_scan(counted ? nbs : 0u), 63);
+        acc.outs += size_sum;
+    }
+    if (use_fs) {
+        const uint64_t fl = ((uint64_t)rdlane((uint32_t)(f >> 32), last) << 32) | rdlane((uint32_t)f, last);
+        S.fail[d] = fl;
+        if (cut) {
+            S.fail[d - 1] = fl;
+            S.flags[d] |= F_PRUNED;
+            acc.fsp++;
+        } else {
+            // an extension with embeddings has the full mask; otherwise OR the few others
+            const uint64_t all = __ballot(counted && f == ~0ull) ? ~0ull : lanes_or64(__ballot(counted), f);
+            S.fail[d - 1] |= all;
         }
     }
     S.rem[d] = 0;
