@@ -378,29 +378,27 @@ struct SibAcc {
     uint64_t conf;
 };
 
-// Adds one step's pair results to the owner lanes: the pairs of one owner occupy
-// consecutive lanes (owners ascend with the pair index); `head` marks the first valid lane
-// of each owner's run (lane 0, or the owner's first pair).
-__device__ __forceinline__ void sib_accumulate(SibAcc& r, bool head, uint32_t own, bool hit, bool filt,
-                                               uint32_t c, const uint64_t* anc, uint32_t L)
+// Adds one 64-pair group's results (pairs gbase .. gbase+63 of the concatenation) to the
+// owner lanes: owner lane j's pairs are [excl_j, incl_j), so its share of the group is a
+// contiguous lane range, counted from the wave-wide ballots with mask arithmetic alone.
+// Conflicts (rare) are added by their own lanes' owners.
+__device__ __forceinline__ void sib_accumulate(SibAcc& r, uint32_t excl, uint32_t incl, uint32_t gbase,
+                                               uint32_t own, bool hit, bool filt, uint32_t c,
+                                               const uint64_t* anc, uint32_t L)
 {
     const uint32_t lane = lane_id();
     const bool ok = filt && c == SGM_INVALID, cf = filt && c != SGM_INVALID;
     const uint64_t H = __ballot(hit), O = __ballot(ok), F = __ballot(filt);
     uint64_t CF = __ballot(cf);
-    uint64_t hd = __ballot(head);
     if (H) {
-        while (hd) {
-            const uint32_t h = (uint32_t)__ffsll((long long)hd) - 1u;
-            hd &= hd - 1;
-            const uint32_t e = hd ? (uint32_t)__ffsll((long long)hd) - 1u : 64u;
-            const uint64_t seg = (e == 64u ? ~0ull : ((1ull << e) - 1ull)) & ~((1ull << h) - 1ull);
-            if (!(H & seg)) continue;
-            const uint32_t j = rdlane(own, h);
-            const uint32_t cs = (uint32_t)__popcll(H & seg), co = (uint32_t)__popcll(O & seg);
-            const uint32_t ca = (F & seg) ? 1u : 0u;
-            if (lane == j) { r.size += cs; r.ok += co; r.any |= ca; }
-        }
+        const uint32_t lo = excl > gbase ? (excl - gbase < 64u ? excl - gbase : 64u) : 0u;
+        const uint32_t hi = incl > gbase ? (incl - gbase < 64u ? incl - gbase : 64u) : 0u;
+        const uint64_t below_hi = hi >= 64u ? ~0ull : ((1ull << hi) - 1ull);
+        const uint64_t below_lo = lo >= 64u ? ~0ull : ((1ull << lo) - 1ull);
+        const uint64_t seg = below_hi & ~below_lo;
+        r.size += (uint32_t)__popcll(H & seg);
+        r.ok += (uint32_t)__popcll(O & seg);
+        r.any |= (F & seg) ? 1u : 0u;
     }
     while (CF) {
         const uint32_t l = (uint32_t)__ffsll((long long)CF) - 1u;
@@ -504,14 +502,13 @@ __device__ void last_chunk(const sgm_plan_dev& P, WaveDFS& S, uint32_t d, const 
             uint32_t key[PU], own[PU], ov[PU], tn[PU];
             const uint32_t* tb[PU];
             const uint32_t* tbits[PU];
-            bool valid[PU], hit[PU], head[PU];
+            bool valid[PU], hit[PU];
 #pragma unroll
             for (int u = 0; u < PU; u++) {
                 const uint32_t t = base + 64 * u + lane;
                 own[u] = wave_seg_owner(incl, t);
                 valid[u] = t < total;
                 const uint32_t q = t - (uint32_t)__shfl((int)excl, (int)own[u]);
-                head[u] = valid[u] && (lane == 0 || q == 0);
                 const uint32_t onbo = (uint32_t)__shfl((int)nbo, (int)own[u]);
                 const uint32_t onbs = (uint32_t)__shfl((int)nbs, (int)own[u]);
                 const uint32_t okv = (uint32_t)__shfl((int)kv, (int)own[u]);
@@ -544,7 +541,7 @@ __device__ void last_chunk(const sgm_plan_dev& P, WaveDFS& S, uint32_t d, const 
                         if (rdlane(embv, j) == key[u]) c = j;
                     if (key[u] == ov[u]) c = d;
                 }
-                sib_accumulate(r, head[u], own[u], hit[u], filt[u], c, P.anc, L);
+                sib_accumulate(r, excl, incl, base + 64 * u, own[u], hit[u], filt[u], c, P.anc, L);
             }
             if (((base >> 8) & 7u) == 7u && __atomic_load_n(stop, __ATOMIC_RELAXED)) break;
         }
