@@ -154,6 +154,13 @@ __device__ __forceinline__ void flush_count(unsigned long long& pending, unsigne
     pending = 0;
 }
 
+// LDS state is per wave, so its values are wave-uniform; reading them through readfirstlane
+// keeps them in scalar registers (a plain LDS load lands in a vector register).
+__device__ __forceinline__ uint64_t uni64(uint64_t x)
+{
+    return ((uint64_t)uni((uint32_t)(x >> 32)) << 32) | uni((uint32_t)x);
+}
+
 // bit v of a per-query bitmap (the last query vertex's candidates)
 __device__ __forceinline__ bool last_bit(const uint32_t* __restrict__ lb, uint32_t v)
 {
@@ -418,7 +425,7 @@ __device__ void last_chunk(const sgm_plan_dev& P, WaveDFS& S, uint32_t d, const 
 {
     const uint32_t lane = lane_id();
     const uint32_t L = d + 1;
-    const uint64_t rm = S.rem[d];
+    const uint64_t rm = uni64(S.rem[d]);
     const uint32_t m = (uint32_t)__popcll(rm);
     const bool mine = lane < m;
     // chunk position of the lane-th set bit of rm: the largest p with at most `lane` set
@@ -431,23 +438,23 @@ __device__ void last_chunk(const sgm_plan_dev& P, WaveDFS& S, uint32_t d, const 
         if ((uint32_t)__popcll(rm & below) <= lane) pos = q;
     }
     pos = pos > 63u ? 63u : pos;
-    const uint32_t v = mine ? list[S.chunk_base[d] + pos] : 0u;
+    const uint32_t v = mine ? list[uni(S.chunk_base[d]) + pos] : 0u;
     auto bits_of = [&](uint32_t x) -> const uint32_t* {
         if (!hub_slot) return nullptr;
         const uint32_t s = hub_slot[x];
         return s == SGM_INVALID ? nullptr : hub_bits + (size_t)s * hub_words;
     };
-    const bool has_op = P.op_first[d] < P.op_first[d + 1];
-    const uint32_t o = P.op_first[d];
-    const uint32_t par = has_op ? P.op_parent[o] : SGM_NOPARENT;
+    const uint32_t o = uni(P.op_first[d]);
+    const bool has_op = o < uni(P.op_first[d + 1]);
+    const uint32_t par = has_op ? uni(P.op_parent[o]) : SGM_NOPARENT;
     const int mode = !has_op ? 2 : (par == SGM_NOPARENT ? 1 : 0);  // 0 isect, 1 N(v), 2 cached list
     SibAcc r = {0, 0, 0, 0};
     uint32_t nbs = 0, pn = 0;
     constexpr int PU = LAST_PU;  // pairs per lane per step
     if (mode == 2) {
-        const uint32_t ci = P.iter_set[L];
-        const uint32_t* __restrict__ cl = op_list(op_off[ci], adj, ws);
-        const uint32_t cn = op_size[ci];
+        const uint32_t ci = uni(P.iter_set[L]);
+        const uint32_t* __restrict__ cl = op_list(uni(op_off[ci]), adj, ws);
+        const uint32_t cn = uni(op_size[ci]);
         uint32_t b_ok = 0, b_any = 0;
         uint64_t b_conf = 0;
         for (uint32_t base = 0; base < cn; base += 64 * UNR) {
@@ -486,9 +493,10 @@ __device__ void last_chunk(const sgm_plan_dev& P, WaveDFS& S, uint32_t d, const 
         const uint32_t* __restrict__ pl = nullptr;
         const uint32_t* pbits = nullptr;
         if (mode == 0) {
-            pl = op_list(op_off[par], adj, ws);
-            pn = op_size[par];
-            pbits = op_vtx[par] != SGM_INVALID ? bits_of(op_vtx[par]) : nullptr;
+            pl = op_list(uni(op_off[par]), adj, ws);
+            pn = uni(op_size[par]);
+            const uint32_t pv = uni(op_vtx[par]);
+            pbits = pv != SGM_INVALID ? bits_of(pv) : nullptr;
         }
         // keys of sibling i: N(v) when it is the shorter list (always without a parent op),
         // else the parent op, tested against N(v) (its bitmap when v is a hub)
@@ -735,7 +743,8 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
                         aborted = true;
                         break;
                     }
-                    if (S.rem[d] == 0 && ((S.flags[d] & F_PRUNED) || S.pos[d] >= S.tot[d])) {
+                    const uint64_t rem_d = uni64(S.rem[d]);
+                    if (rem_d == 0 && ((uni(S.flags[d]) & F_PRUNED) || uni(S.pos[d]) >= uni(S.tot[d]))) {
                         // end of the loop at depth d (inc/explore.h:1068-1070)
                         if (use_fs && !(S.flags[d] & (F_PRUNED | F_ANY))) S.fail[d - 1] = P.anc[d];
                         d--;
@@ -751,16 +760,16 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
                         }
                         continue;
                     }
-                    const uint32_t* list = op_list(S.lst[d], adj, ws);
-                    if (S.rem[d] == 0) {
+                    const uint32_t* list = op_list(uni(S.lst[d]), adj, ws);
+                    if (rem_d == 0) {
                         // classify the next 64 elements of depth d's iteration set
-                        const uint32_t u = P.order[d];
-                        const uint32_t base = S.pos[d];
+                        const uint32_t u = uni(P.order[d]);
+                        const uint32_t base = uni(S.pos[d]), tot = uni(S.tot[d]);
                         S.pos[d] = base + 64;
-                        acc.scanned += (S.tot[d] - base < 64u) ? (S.tot[d] - base) : 64u;
+                        acc.scanned += (tot - base < 64u) ? (tot - base) : 64u;
                         S.chunk_base[d] = base;
                         const uint32_t i = base + lane;
-                        const bool valid = i < S.tot[d];
+                        const bool valid = i < tot;
                         const uint32_t w = valid ? list[i] : 0u;
                         const bool f = valid && ((vmask[w] >> u) & 1u);
                         uint32_t c = SGM_INVALID;
@@ -785,10 +794,10 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
                         continue;
                     }
                     // extend with the next candidate of this chunk
-                    const uint64_t rm = S.rem[d];
+                    const uint64_t rm = rem_d;
                     const uint32_t j = (uint32_t)__ffsll((long long)rm) - 1u;
                     S.rem[d] = rm & (rm - 1);
-                    const uint32_t v = list[S.chunk_base[d] + j];
+                    const uint32_t v = uni(list[uni(S.chunk_base[d]) + j]);
                     S.emb[d] = v;
                     embv = lane == d ? v : embv;
                     S.nodes[d] += 1;
