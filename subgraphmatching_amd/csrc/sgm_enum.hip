@@ -80,21 +80,26 @@ __device__ __forceinline__ void member4(const uint32_t* __restrict__ b, uint32_t
         for (int u = 0; u < UNR; u++) hit[u] = valid[u] && ((bits[key[u] >> 5] >> (key[u] & 31u)) & 1u);
         return;
     }
+    // last element <= key: every step issues its UNR loads before the first compare (into
+    // separate registers), and one final load per key decides membership
     uint32_t p[UNR];
 #pragma unroll
     for (int u = 0; u < UNR; u++) p[u] = 0;
     uint32_t n = nb;
     while (n > 1) {
         const uint32_t half = n >> 1;
+        uint32_t x[UNR];
 #pragma unroll
-        for (int u = 0; u < UNR; u++) p[u] = (b[p[u] + half] < key[u]) ? p[u] + half : p[u];
+        for (int u = 0; u < UNR; u++) x[u] = b[p[u] + half];
+#pragma unroll
+        for (int u = 0; u < UNR; u++) p[u] = x[u] <= key[u] ? p[u] + half : p[u];
         n -= half;
     }
+    uint32_t x[UNR];
 #pragma unroll
-    for (int u = 0; u < UNR; u++) {
-        const uint32_t lb = p[u] + (b[p[u]] < key[u] ? 1u : 0u);
-        hit[u] = valid[u] && lb < nb && b[lb] == key[u];
-    }
+    for (int u = 0; u < UNR; u++) x[u] = b[p[u]];
+#pragma unroll
+    for (int u = 0; u < UNR; u++) hit[u] = valid[u] && x[u] == key[u];
 }
 
 // Intersect sorted lists a and b into out (ascending); returns the size.  Lanes take
