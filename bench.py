@@ -140,8 +140,8 @@ def setup_chunglu(args):
         rec["nlf_ms"], rec["gql_ms"], rec["cand_ms"] = ctx.filter_ms()
         rec["filter_ms"] = rec["nlf_ms"] + rec["gql_ms"] + rec["cand_ms"]
         t1 = time.perf_counter()
-        orders = [sgm.Order_RI(q) for q in queries]
         if args.per_query:
+            orders = [sgm.Order_RI(q) for q in queries]
             results = []
             for can, order, good in zip(cans, orders, ok):
                 r = sgm.enumerate_query(can, order, L.SGM_ENUM_CACHE_FILTER_FS, cap=args.cap,
@@ -152,7 +152,8 @@ def setup_chunglu(args):
                 results.append(r)
         else:
             live = [i for i, good in enumerate(ok) if good]
-            batch = sgm.enumerate_batch([cans[i] for i in live], [orders[i] for i in live],
+            # RI orders computed by the library inside the call (orders=None)
+            batch = sgm.enumerate_batch([cans[i] for i in live], None,
                                         L.SGM_ENUM_CACHE_FILTER_FS, cap=args.cap, blocks=args.blocks)
             results = [None] * len(queries)
             for i, r in zip(live, batch):

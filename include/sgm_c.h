@@ -149,7 +149,8 @@ int sgm_plan_export(const uint32_t* qoff, const uint32_t* qadj, uint32_t n, cons
 int sgm_enumerate(sgm_query* q, const uint32_t* order, const sgm_enum_opts* opts, sgm_stats* out);
 /* Many queries of one data graph in one persistent launch (cache + filter methods; no
  * reference counterpart: the reference runs one query per call).  Each query gets its own
- * cap and its own sgm_stats; out[i].kernel_ms is the time of the whole batch's launch. */
+ * cap and its own sgm_stats; out[i].kernel_ms is the time of the whole batch's launch.
+ * orders == NULL or orders[i] == NULL: the RI order (Order_RI), computed in the call. */
 int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uint32_t nq, const sgm_enum_opts* opts,
                         sgm_stats* out);
 

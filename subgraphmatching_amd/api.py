@@ -414,10 +414,11 @@ def enumerate_query(can: CandidateParam, order, method=L.SGM_ENUM_CACHE_FILTER_F
     return res
 
 
-def enumerate_batch(cans, orders, method=L.SGM_ENUM_CACHE_FILTER_FS, cap=None, blocks=0, stack_words=0,
+def enumerate_batch(cans, orders=None, method=L.SGM_ENUM_CACHE_FILTER_FS, cap=None, blocks=0, stack_words=0,
                     flags=0):
     """Enumerate many filtered queries of one data graph in one persistent launch.
-    Returns one ExploreResult per query (``time`` is the whole batch's kernel time)."""
+    ``orders`` None (or a None entry): the RI order, computed by the library.  Returns one
+    ExploreResult per query (``time`` is the whole batch's kernel time)."""
     if not cans:
         return []
     ctx = cans[0].ctx
@@ -428,9 +429,9 @@ def enumerate_batch(cans, orders, method=L.SGM_ENUM_CACHE_FILTER_FS, cap=None, b
     opts.stack_words = stack_words
     opts.flags = flags
     nq = len(cans)
-    ords = [L.u32(o) for o in orders]
+    ords = [None if o is None else L.u32(o) for o in orders] if orders is not None else None
     qarr = (C.c_void_p * nq)(*[c._q.value for c in cans])
-    oarr = (C.c_void_p * nq)(*[o.ctypes.data for o in ords])
+    oarr = None if ords is None else (C.c_void_p * nq)(*[None if o is None else o.ctypes.data for o in ords])
     stats = (L.Stats * nq)()
     _check(L.lib().sgm_enumerate_batch(qarr, oarr, nq, C.byref(opts), stats), ctx)
     out = []

@@ -1,4 +1,5 @@
 // C ABI implementation: device residency, filter / table / enumeration drivers.
+#include <array>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -821,9 +822,9 @@ static int enumerate_aux(sgm_query* q, const uint32_t* order, const sgm_enum_opt
 int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uint32_t nq, const sgm_enum_opts* opts,
                         sgm_stats* out)
 {
-    if (!qs || !orders || !out || nq == 0) return SGM_EINVAL;
+    if (!qs || !out || nq == 0) return SGM_EINVAL;
     for (uint32_t i = 0; i < nq; i++)
-        if (!qs[i] || !orders[i]) return SGM_EINVAL;
+        if (!qs[i]) return SGM_EINVAL;
     sgm_graph* g = qs[0]->g;
     sgm_ctx* c = g->ctx;
     const sgm_enum_opts o = normalize_opts(opts);
@@ -833,6 +834,7 @@ int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uin
     // plans (host), validated before anything runs
     std::vector<sgm_plan_dev> plans(nq);
     std::vector<uint32_t> live;
+    std::vector<std::array<uint32_t, SGM_MAXQ>> ord(nq);  // the orders used (given or RI)
     uint32_t max_ops = 1;
     for (uint32_t i = 0; i < nq; i++) {
         sgm_query* q = qs[i];
@@ -840,7 +842,9 @@ int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uin
         if (!q->filtered) return set_err(c, SGM_ESTATE, "filter query %u first", i);
         if (q->q.n < 2) return set_err(c, SGM_EINVAL, "enumeration needs at least 2 query vertices (query %u)", i);
         sgm::Plan pl;
-        const int prc = sgm::build_plan(q->q, orders[i], pl);
+        if (orders && orders[i]) std::copy(orders[i], orders[i] + q->q.n, ord[i].begin());
+        else sgm::order_ri(q->q, ord[i].data());
+        const int prc = sgm::build_plan(q->q, ord[i].data(), pl);
         if (prc == -2) return set_err(c, SGM_EORDER, "order of query %u is not a connected permutation", i);
         if (prc) return set_err(c, SGM_EINVAL, "plan construction failed for query %u (%d)", i, prc);
         sgm::plan_to_device(pl, plans[i]);
@@ -886,7 +890,7 @@ int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uin
         for (size_t k = 0; k < live.size(); k++) {
             const uint32_t i = live[k];
             sgm_query* q = qs[i];
-            const uint32_t u0 = orders[i][0];
+            const uint32_t u0 = ord[i][0];
             eq[k].plan = c->d_plans + i;
             eq[k].vmask = q->d_vmask;
             eq[k].lastbits = c->d_lastbits + lb_words * k;

@@ -192,6 +192,9 @@ def test_enum_batch_equals_oracle(g_small):
             refs.append(ref)
         keep = [i for i in range(len(qs)) if cap or not refs[i]["capped"]]
         res = sgm.enumerate_batch([cans[i] for i in keep], [orders[i] for i in keep], cap=cap)
+        if not cap:  # orders=None: the library's RI orders, the same search
+            res_ri = sgm.enumerate_batch([cans[i] for i in keep], None, cap=cap)
+            assert [(r.ans, r.nodes, r.intersections) for r in res_ri] == [(r.ans, r.nodes, r.intersections) for r in res]
         for r, i in zip(res, keep):
             assert r.ans == refs[i]["count"], (qs[i].name, cap, r.ans, refs[i]["count"])
             if not cap:
