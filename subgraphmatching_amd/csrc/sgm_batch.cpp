@@ -218,14 +218,7 @@ int sgm_batch_create(sgm_graph* g, uint32_t slots, sgm_batch** out)
         std::memset(&d, 0, sizeof d);
         d.vmask = b->d_vmask + i * V;
         d.removed = b->d_flags + (size_t)i * SGM_GQL_FLAG_WORDS;
-        const size_t G = sgm_gql_groups(g->V);
-        d.vmr = b->d_scratch + sgm_gql_scratch_words(g->V) * i;
-        d.touch = d.vmr + V;
-        d.dirty = d.touch + V;
-        d.tsum = d.dirty + V;
-        d.dsum = d.tsum + G;
-        d.f0 = d.dsum + G;
-        d.fsum = d.f0 + V;
+        sgm_gql_layout(d, b->d_scratch + sgm_gql_scratch_words(g->V) * i, g->V);
     }
     b->counts.assign((size_t)slots * SGM_MAXQ, 0);
     b->degsum.assign((size_t)slots * SGM_MAXQ, 0);

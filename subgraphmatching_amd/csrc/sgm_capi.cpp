@@ -340,14 +340,7 @@ static int filter_chunk(sgm_ctx* c, sgm_graph* g, sgm_query* const* qs, uint32_t
         qd[i] = qs[i]->hq;
         cd[i] = qs[i]->hc;
         if (kind == SGM_FILTER_GQL) {
-            const size_t G = sgm_gql_groups(V);
-            qd[i].vmr = c->d_gqls + sgm_gql_scratch_words(V) * i;
-            qd[i].touch = qd[i].vmr + V;
-            qd[i].dirty = qd[i].touch + V;
-            qd[i].tsum = qd[i].dirty + V;
-            qd[i].dsum = qd[i].tsum + G;
-            qd[i].f0 = qd[i].dsum + G;
-            qd[i].fsum = qd[i].f0 + V;
+            sgm_gql_layout(qd[i], c->d_gqls + sgm_gql_scratch_words(V) * i, V);
         }
         max_n = std::max(max_n, qs[i]->q.n);
         free_tables(qs[i]);

@@ -85,13 +85,16 @@ struct sgm_equery {
 #define SGM_GQL_DIRTY_ANY (2 * SGM_MAXQ + 1)
 #define SGM_GQL_WORK (2 * SGM_MAXQ + 2)
 #define SGM_GQL_FLAG_WORDS (2 * SGM_MAXQ + 6)
-// GQL scratch per query, in words, by rank (G = SGM_GQL_GROUP ranks per summary word):
-//   vmr[V] | touch[V] | dirty[V] | tsum[V/G] | dsum[V/G] | f0[V] | fsum[V/G]
-// (touch .. dsum start clear; the rest is written before it is read)
+// GQL scratch per query, in words, by rank.  Summary words OR their group's words: level 1
+// over SGM_GQL_GROUP ranks (G1 = V/G words), level 2 over 64 level-1 words (G2 = G1/64):
+//   vmr[V] | touch[V] | dirty[V] | tsum[G1] | dsum[G1] | tsum2[G2] | dsum2[G2] | f0[V] | fsum[G1] | fsum2[G2]
+// (touch .. dsum2 start clear; the rest is written before it is read)
 #define SGM_GQL_GROUP 4u  // 2, 8, 16, 32 measured slower on power-law graphs
+#define SGM_GQL_GROUP2 64u
 inline size_t sgm_gql_groups(uint32_t V) { return ((size_t)V + SGM_GQL_GROUP - 1) / SGM_GQL_GROUP; }
-inline size_t sgm_gql_scratch_words(uint32_t V) { return 4 * (size_t)V + 3 * sgm_gql_groups(V); }
-inline size_t sgm_gql_clear_words(uint32_t V) { return 2 * (size_t)V + 2 * sgm_gql_groups(V); }
+inline size_t sgm_gql_groups2(uint32_t V) { return (sgm_gql_groups(V) + SGM_GQL_GROUP2 - 1) / SGM_GQL_GROUP2; }
+inline size_t sgm_gql_scratch_words(uint32_t V) { return 4 * (size_t)V + 3 * sgm_gql_groups(V) + 3 * sgm_gql_groups2(V); }
+inline size_t sgm_gql_clear_words(uint32_t V) { return 2 * (size_t)V + 2 * sgm_gql_groups(V) + 2 * sgm_gql_groups2(V); }
 // GQL runs in degree-rank space (rank r = position in the degree-sorted vertex list), where
 // the candidates of every NLF threshold are a prefix; `vmr` holds the rank-space membership
 // words and is copied back to vmask (vertex ids) after the last phase.
@@ -109,4 +112,23 @@ struct sgm_qdesc {
     uint32_t* fsum;                  // per group of SGM_GQL_GROUP ranks: OR of f0
     uint32_t* tsum;                  // per group: OR of touch
     uint32_t* dsum;                  // per group: OR of dirty
+    uint32_t* fsum2;                 // per SGM_GQL_GROUP2 groups: OR of fsum
+    uint32_t* tsum2;                 // ... of tsum
+    uint32_t* dsum2;                 // ... of dsum
 };
+
+// Points d's GQL scratch at `base` (sgm_gql_scratch_words(V) words, layout above).
+inline void sgm_gql_layout(sgm_qdesc& d, uint32_t* base, uint32_t V)
+{
+    const size_t G = sgm_gql_groups(V), G2 = sgm_gql_groups2(V);
+    d.vmr = base;
+    d.touch = d.vmr + V;
+    d.dirty = d.touch + V;
+    d.tsum = d.dirty + V;
+    d.dsum = d.tsum + G;
+    d.tsum2 = d.dsum + G;
+    d.dsum2 = d.tsum2 + G2;
+    d.f0 = d.dsum2 + G2;
+    d.fsum = d.f0 + V;
+    d.fsum2 = d.fsum + G;
+}

@@ -197,14 +197,16 @@ __device__ void gql_mark_nbrs(const sgm_qdesc* __restrict__ d, const uint32_t* _
         const uint32_t oj = (uint32_t)__shfl((int)o, (int)j);
         const uint32_t sj = (uint32_t)__shfl((int)(incl - dv), (int)j);
         if (t >= total) continue;
-        const uint32_t w = radj[oj + (t - sj)], gw = w / SGM_GQL_GROUP;
+        const uint32_t w = radj[oj + (t - sj)], gw = w / SGM_GQL_GROUP, gw2 = gw / SGM_GQL_GROUP2;
         if (tm && (d->touch[w] & tm) != tm) {
             atomicOr(&d->touch[w], tm);
             if ((d->tsum[gw] & tm) != tm) atomicOr(&d->tsum[gw], tm);
+            if ((d->tsum2[gw2] & tm) != tm) atomicOr(&d->tsum2[gw2], tm);
         }
         if (dm && (d->dirty[w] & dm) != dm) {
             atomicOr(&d->dirty[w], dm);
             if ((d->dsum[gw] & dm) != dm) atomicOr(&d->dsum[gw], dm);
+            if ((d->dsum2[gw2] & dm) != dm) atomicOr(&d->dsum2[gw2], dm);
         }
     }
 }
@@ -238,8 +240,12 @@ __global__ void __launch_bounds__(256) k_gql_prepass(const uint64_t* __restrict_
     const uint32_t np = (kmax + 7) / 8;
     const uint32_t lane = lane_id();
     uint32_t tests = 0;
-    const uint32_t span = (Pmax + 63) & ~63u;  // whole waves (and whole groups) iterate together
-    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < span; x += gridDim.x * blockDim.x) {
+    // a block covers the ranks of one level-2 summary word per iteration (block-uniform loop)
+    static_assert(SGM_GQL_GROUP * SGM_GQL_GROUP2 == 256, "one level-2 word per 256-thread block");
+    __shared__ uint32_t s_or[4];
+    const uint32_t span = (Pmax + 255) & ~255u;
+    for (uint32_t x0 = blockIdx.x * 256u; x0 < span; x0 += gridDim.x * 256u) {
+        const uint32_t x = x0 + threadIdx.x;
         const bool in = x < Pmax;
         uint64_t D[4];
 #pragma unroll
@@ -258,6 +264,11 @@ __global__ void __launch_bounds__(256) k_gql_prepass(const uint64_t* __restrict_
 #pragma unroll
         for (int o = 1; o < (int)SGM_GQL_GROUP; o <<= 1) g |= (uint32_t)__shfl_xor((int)g, o);
         if ((lane % SGM_GQL_GROUP) == 0 && in) d->fsum[x / SGM_GQL_GROUP] = g;
+        const uint32_t wv = wave_or32(f);
+        if (lane == 0) s_or[threadIdx.x >> 6] = wv;
+        __syncthreads();
+        if (threadIdx.x == 0) d->fsum2[x0 / 256u] = s_or[0] | s_or[1] | s_or[2] | s_or[3];
+        __syncthreads();
     }
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) tests += (uint32_t)__shfl_xor((int)tests, o);
@@ -335,29 +346,42 @@ __global__ void __launch_bounds__(256) k_gql_phase(const uint32_t* __restrict__ 
     // 64 summary words per wave step; the groups whose word holds bit u are then opened
     // 64 / SGM_GQL_GROUP at a time, SGM_GQL_GROUP lanes per group (coalesced rank segments)
     const uint32_t sub = lane / SGM_GQL_GROUP;  // which of the selected groups
-    for (uint32_t base = w0 * 64; base < ng; base += stride * 64) {
-        const uint32_t gi = base + lane;
-        const uint32_t sw = gi < ng ? (sumA[gi] | (sumB ? sumB[gi] : 0u)) : 0u;
-        uint64_t hm = __ballot((sw >> u) & 1u);
-        while (hm) {
-            uint32_t mine = SGM_INVALID;
-            for (uint32_t j = 0; j < 64 / SGM_GQL_GROUP && hm; j++) {
-                const uint32_t bit = (uint32_t)__ffsll((long long)hm) - 1u;
-                hm &= hm - 1;
-                if (sub == j) mine = base + bit;
+    // level-2 words first: lane k of a wave checks word w0 + k * stride (64 words per load,
+    // spread over the grid); each word with bit u set opens its 64 level-1 words
+    const uint32_t ng2 = (ng + SGM_GQL_GROUP2 - 1) / SGM_GQL_GROUP2;
+    const uint32_t* __restrict__ sum2A = r == 0 ? d->fsum2 : d->dsum2;
+    const uint32_t* __restrict__ sum2B = use_touch ? d->tsum2 : nullptr;
+    for (uint32_t o2 = 0; w0 + o2 * 64u * stride < ng2; o2++) {
+        const uint32_t i2 = w0 + (o2 * 64u + lane) * stride;
+        const uint32_t sw2 = i2 < ng2 ? (sum2A[i2] | (sum2B ? sum2B[i2] : 0u)) : 0u;
+        uint64_t h2 = __ballot((sw2 >> u) & 1u);
+        while (h2) {
+            const uint32_t k2 = (uint32_t)__ffsll((long long)h2) - 1u;
+            h2 &= h2 - 1;
+            const uint32_t base = rdlane(i2, k2) * SGM_GQL_GROUP2;
+            const uint32_t gi = base + lane;
+            const uint32_t sw = gi < ng ? (sumA[gi] | (sumB ? sumB[gi] : 0u)) : 0u;
+            uint64_t hm = __ballot((sw >> u) & 1u);
+            while (hm) {
+                uint32_t mine = SGM_INVALID;
+                for (uint32_t j = 0; j < 64 / SGM_GQL_GROUP && hm; j++) {
+                    const uint32_t bit = (uint32_t)__ffsll((long long)hm) - 1u;
+                    hm &= hm - 1;
+                    if (sub == j) mine = base + bit;
+                }
+                const uint32_t x = mine != SGM_INVALID ? mine * SGM_GQL_GROUP + lane % SGM_GQL_GROUP : SGM_INVALID;
+                const bool c = x < P;
+                const uint32_t fw = (c && r == 0) ? f0[x] : 0u;
+                const uint32_t tw = (c && (r == 1 || use_touch)) ? word[x] : 0u;
+                const uint32_t vw = (c && r == 1) ? vm[x] : 0u;
+                const bool rm = (fw >> u) & 1u;  // fails the NLF table test: fails now too
+                const bool need = !rm && ((tw >> u) & 1u) && (r == 0 || ((vw >> u) & 1u));
+                remove_now(rm, x);
+                const uint64_t bal = __ballot(need);
+                if (need) wq[(qt + mbcnt64(bal)) & (GQL_QCAP - 1u)] = x;
+                qt += (uint32_t)__popcll(bal);
+                while (qt - qh >= 64) drain(64);
             }
-            const uint32_t x = mine != SGM_INVALID ? mine * SGM_GQL_GROUP + lane % SGM_GQL_GROUP : SGM_INVALID;
-            const bool c = x < P;
-            const uint32_t fw = (c && r == 0) ? f0[x] : 0u;
-            const uint32_t tw = (c && (r == 1 || use_touch)) ? word[x] : 0u;
-            const uint32_t vw = (c && r == 1) ? vm[x] : 0u;
-            const bool rm = (fw >> u) & 1u;  // fails the NLF table test: fails now too
-            const bool need = !rm && ((tw >> u) & 1u) && (r == 0 || ((vw >> u) & 1u));
-            remove_now(rm, x);
-            const uint64_t bal = __ballot(need);
-            if (need) wq[(qt + mbcnt64(bal)) & (GQL_QCAP - 1u)] = x;
-            qt += (uint32_t)__popcll(bal);
-            while (qt - qh >= 64) drain(64);
         }
     }
     while (qt != qh) drain(qt - qh < 64 ? qt - qh : 64u);
@@ -756,8 +780,11 @@ void launch_gql_prepass(hipStream_t s, const uint64_t* dtab, uint32_t V, const u
 void launch_gql_phase(hipStream_t s, const uint32_t* roff, const uint32_t* radj, uint32_t V, const uint32_t* ge_cnt,
                       const sgm_qdesc* qs, uint32_t nq, uint32_t phase)
 {
-    // the phase visits short lists: a modest number of blocks per query
-    dim3 grid(clampg(8192 / (nq ? nq : 1), 8, 256), nq);
+    // the phase visits short lists and skips idle rank blocks by their level-2 summaries: a
+    // modest number of blocks per query (4096 in all measured best at 64 and 200 queries,
+    // 8192 for launches of several hundred)
+    const uint32_t total = nq > 256 ? 8192u : 4096u;
+    dim3 grid(clampg(total / (nq ? nq : 1), 8, 256), nq);
     hipLaunchKernelGGL(k_gql_phase, grid, dim3(256), 0, s, roff, radj, V, ge_cnt, qs, phase);
 }
 
