@@ -511,23 +511,35 @@ __device__ void last_chunk(const sgm_plan_dev& P, WaveDFS& S, uint32_t d, const 
         const uint32_t incl = wave_incl_scan(len);
         const uint32_t total = rdlane(incl, 63);
         const uint32_t excl = incl - len;
-        for (uint32_t base = 0; base < total; base += 64 * PU) {
-            uint32_t key[PU], own[PU], ov[PU], tn[PU];
-            const uint32_t* tb[PU];
-            const uint32_t* tbits[PU];
-            bool valid[PU], hit[PU];
+        // pair t -> owner lane and key; the keys of pass i + 1 are loaded while pass i's
+        // membership and filter probes are in flight (software pipelining over passes)
+        auto load_keys = [&](uint32_t b0, uint32_t (&k)[PU], uint32_t (&ow)[PU], bool (&va)[PU]) {
 #pragma unroll
             for (int u = 0; u < PU; u++) {
-                const uint32_t t = base + 64 * u + lane;
-                own[u] = wave_seg_owner(incl, t);
-                valid[u] = t < total;
-                const uint32_t q = t - (uint32_t)__shfl((int)excl, (int)own[u]);
+                const uint32_t t = b0 + 64 * u + lane;
+                ow[u] = wave_seg_owner(incl, t);
+                va[u] = t < total;
+                const uint32_t q = t - (uint32_t)__shfl((int)excl, (int)ow[u]);
+                const uint32_t onbo = (uint32_t)__shfl((int)nbo, (int)ow[u]);
+                const uint32_t okv = (uint32_t)__shfl((int)kv, (int)ow[u]);
+                k[u] = va[u] ? (okv ? adj[onbo + q] : pl[q]) : 0u;
+            }
+        };
+        uint32_t key[PU], own[PU];
+        bool valid[PU];
+        load_keys(0, key, own, valid);
+        for (uint32_t base = 0; base < total; base += 64 * PU) {
+            uint32_t ov[PU], tn[PU];
+            const uint32_t* tb[PU];
+            const uint32_t* tbits[PU];
+            bool hit[PU];
+#pragma unroll
+            for (int u = 0; u < PU; u++) {
                 const uint32_t onbo = (uint32_t)__shfl((int)nbo, (int)own[u]);
                 const uint32_t onbs = (uint32_t)__shfl((int)nbs, (int)own[u]);
                 const uint32_t okv = (uint32_t)__shfl((int)kv, (int)own[u]);
                 const uint32_t oslot = (uint32_t)__shfl((int)slot, (int)own[u]);
                 ov[u] = (uint32_t)__shfl((int)v, (int)own[u]);
-                key[u] = valid[u] ? (okv ? adj[onbo + q] : pl[q]) : 0u;
                 tb[u] = okv ? pl : adj + onbo;
                 tn[u] = okv ? pn : onbs;
                 tbits[u] = okv ? pbits
@@ -537,6 +549,14 @@ __device__ void last_chunk(const sgm_plan_dev& P, WaveDFS& S, uint32_t d, const 
             uint32_t lw[PU];
 #pragma unroll
             for (int u = 0; u < PU; u++) lw[u] = valid[u] ? lbits[key[u] >> 5] : 0u;
+            uint32_t nkey[PU], nown[PU];
+            bool nvalid[PU];
+            if (base + 64 * PU < total) {
+                load_keys(base + 64 * PU, nkey, nown, nvalid);
+            } else {
+#pragma unroll
+                for (int u = 0; u < PU; u++) { nkey[u] = 0; nown[u] = 0; nvalid[u] = false; }
+            }
             if (mode == 1) {
 #pragma unroll
                 for (int u = 0; u < PU; u++) hit[u] = valid[u];
@@ -557,6 +577,8 @@ __device__ void last_chunk(const sgm_plan_dev& P, WaveDFS& S, uint32_t d, const 
                 sib_accumulate(r, excl, incl, base + 64 * u, own[u], hit[u], filt[u], c, P.anc, L);
             }
             if (((base >> 8) & 7u) == 7u && __atomic_load_n(stop, __ATOMIC_RELAXED)) break;
+#pragma unroll
+            for (int u = 0; u < PU; u++) { key[u] = nkey[u]; own[u] = nown[u]; valid[u] = nvalid[u]; }
         }
     }
     // apply the siblings' results in order (inc/explore.h:1036-1066 per extension), all lanes
