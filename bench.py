@@ -45,6 +45,7 @@ sample of the same queries; the GPU counts of the sampled queries are compared w
 from __future__ import annotations
 
 import argparse
+import collections
 import json
 import os
 import sys
@@ -81,6 +82,8 @@ def parse():
     ap.add_argument("--blocks", type=int, default=0,
                     help="persistent enumeration blocks of 4 waves (0 = library default / autotuned)")
     ap.add_argument("--batch", type=int, default=64, help="rmat20: queries per graph-replayed batch")
+    ap.add_argument("--runners", type=int, default=2,
+                    help="rmat20: overlapped runners (batch k+1.. filtered and launched while batch k enumerates)")
     ap.add_argument("--no-overlap", action="store_true",
                     help="rmat20: one runner, each batch waited for before the next is filtered")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
@@ -195,15 +198,16 @@ def setup_rmat(args):
         tuned["blocks"] = args.blocks
     print(f"[bench] autotuned on the first batch: {tuned} ({tuned_ms:.2f} ms)", flush=True)
 
-    # two runners on their own streams: batch k+1 is filtered and launched while batch k
+    # runners on their own streams: batch k+1 is filtered and launched while batch k
     # enumerates, and its grid takes the CUs batch k's waves leave at the end
-    runners = [runner] + ([sgm.Batch(g, slots=B, ctx=runner.ctx)] if not args.no_overlap else [])
+    runners = [runner] + ([sgm.Batch(g, slots=B, ctx=runner.ctx) for _ in range(args.runners - 1)]
+                          if not args.no_overlap else [])
 
     def step():
         rec = new_rec()
         t0 = time.perf_counter()
         counts = [0] * nq
-        pending = None  # (runner, ok flags, position of the batch's first query)
+        pending = collections.deque()  # launched batches: (runner, ok flags, position of its first query)
 
         def collect(p):
             r, ok, start = p
@@ -218,20 +222,17 @@ def setup_rmat(args):
         pos = 0
         for k, part in enumerate(chunks):
             r = runners[k % len(runners)]
-            if pending is not None and pending[0] is r:
-                collect(pending)
-                pending = None
+            while any(p[0] is r for p in pending):  # the runner's previous batch, and those before it
+                collect(pending.popleft())
             ok, _, _ = r.filter(part)
             f_ms, _ = r.last_ms()
             rec["filter_ms"] += f_ms
             rec["gql_ms"] += f_ms
             r.enumerate_start(cap=args.cap, **tuned)
-            if pending is not None:
-                collect(pending)
-            pending = (r, ok, pos)
+            pending.append((r, ok, pos))
             pos += len(part)
-        if pending is not None:
-            collect(pending)
+        while pending:
+            collect(pending.popleft())
         rec["counts"] = counts  # in query order
         rec["wall_s"] = time.perf_counter() - t0
         rec["filter_wall_s"] = rec["filter_ms"] / 1e3
