@@ -94,6 +94,52 @@ public:
         count_labels();
     }
 
+    // Binary CSR cache written by subgraphmatching_amd.graph.save_csr (little-endian):
+    // "SGMCSR01", u64 V, u64 A (= 2E), u64 flags (bit 0: u32 labels[V] follow), u32 off[V+1],
+    // u32 adj[A].  No reference counterpart: a cache of the text format above.
+    void LoadCSR(const std::string& file_path)
+    {
+        std::ifstream in(file_path, std::ios::binary);
+        if (!in.is_open()) throw sgm::Error(SGM_EINVAL, "can't open " + file_path);
+        char magic[8];
+        uint64_t head[3];
+        if (!in.read(magic, 8) || std::string(magic, 8) != "SGMCSR01" ||
+            !in.read(reinterpret_cast<char*>(head), sizeof head))
+            throw sgm::Error(SGM_EINVAL, file_path + ": not an sgm binary CSR");
+        const uint64_t V = head[0], A = head[1];
+        if (V >= 0xFFFFFFFFull || A >= 0x80000000ull || (A & 1))
+            throw sgm::Error(SGM_EINVAL, file_path + ": sizes out of range");
+        std::vector<uint32_t> off(V + 1), adj(A);
+        if (!in.read(reinterpret_cast<char*>(off.data()), sizeof(uint32_t) * (V + 1)) ||
+            !in.read(reinterpret_cast<char*>(adj.data()), sizeof(uint32_t) * A))
+            throw sgm::Error(SGM_EINVAL, file_path + ": truncated");
+        std::vector<LabelID> labels(V, 0);
+        if (head[2] & 1) {
+            std::vector<uint32_t> lab(V);
+            if (!in.read(reinterpret_cast<char*>(lab.data()), sizeof(uint32_t) * V))
+                throw sgm::Error(SGM_EINVAL, file_path + ": truncated labels");
+            for (uint64_t v = 0; v < V; v++) labels[v] = (LabelID)lab[v];
+        }
+        if (off[0] != 0 || off[V] != A) throw sgm::Error(SGM_EINVAL, file_path + ": offsets are not a CSR");
+        uint32_t maxd = 0;
+        for (uint64_t v = 0; v < V; v++) {
+            if (off[v + 1] < off[v]) throw sgm::Error(SGM_EINVAL, file_path + ": offsets are not a CSR");
+            for (uint32_t p = off[v]; p < off[v + 1]; p++)
+                if (adj[p] >= V || (p > off[v] && adj[p] <= adj[p - 1]))
+                    throw sgm::Error(SGM_EINVAL, file_path + ": neighbour lists must be sorted, simple, in range");
+            maxd = std::max(maxd, off[v + 1] - off[v]);
+        }
+        release_device();
+        path = file_path;
+        _v_cnt = (uint32_t)V;
+        _offset = std::move(off);
+        _neb = std::move(adj);
+        _e_cnt = (uint32_t)(A / 2);
+        _max_degree = maxd;
+        _labels = std::move(labels);
+        count_labels();
+    }
+
     // Programmatic construction from an undirected edge list (each edge once).
     void AssignEdges(uint32_t V, const std::vector<std::pair<uint32_t, uint32_t>>& edges)
     {

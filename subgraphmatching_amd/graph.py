@@ -17,11 +17,12 @@ exact; ``load_text`` checks that instead of trusting it.
 from __future__ import annotations
 
 import hashlib
+import json
 import os
 
 import numpy as np
 
-__all__ = ["Graph", "Query", "csr_from_edges", "load_text", "save_text"]
+__all__ = ["Graph", "Query", "csr_from_edges", "load_cached", "load_csr", "load_text", "save_csr", "save_text"]
 
 
 def csr_from_edges(num_vertices: int, edges: np.ndarray):
@@ -185,3 +186,92 @@ def save_text(g: Graph, path: str):
         f.write(f"t {V} {e.shape[0]}\n")
         f.write("".join(f"v {i} 0 {int(deg[i])}\n" for i in range(V)))
         f.write("".join(f"e {int(a)} {int(b)}\n" for a, b in e))
+
+
+# ---------------------------------------------------------------------------------------
+# Binary CSR cache.  Text graphs parse slowly (the R-MAT 2^22 graph has 2^26 edge lines); the
+# binary form is the CSR itself, little-endian:
+#   "SGMCSR01" | u64 V | u64 A (adjacency length, 2E) | u64 flags (bit 0: labels follow)
+#   | u32 off[V+1] | u32 adj[A] | u32 labels[V] (if flagged)
+# with a JSON manifest beside it (<path>.json): format, V, E, the SHA-256 of the CSR
+# (Graph.digest) and whatever the writer records about the input (generator, parameters,
+# seed, source file).
+CSR_MAGIC = b"SGMCSR01"
+
+
+def save_csr(g: Graph, path: str, meta: dict | None = None) -> dict:
+    """Write g as a binary CSR plus its manifest; returns the manifest."""
+    V = g.getVertexCnt()
+    off = np.ascontiguousarray(g.off, dtype=np.uint32)
+    adj = np.ascontiguousarray(g.adj, dtype=np.uint32)
+    with open(path, "wb") as f:
+        f.write(CSR_MAGIC)
+        f.write(np.array([V, adj.size, 0], dtype="<u8").tobytes())
+        f.write(off.astype("<u4").tobytes())
+        f.write(adj.astype("<u4").tobytes())
+    manifest = {"format": "sgm-csr-1", "vertices": int(V), "edges": int(g.getEdgeCnt()), "sha256": g.digest(),
+                "name": g.name}
+    manifest.update(meta or {})
+    with open(path + ".json", "w") as f:
+        json.dump(manifest, f, indent=1, sort_keys=True)
+    return manifest
+
+
+def load_csr(path: str, cls=Graph, verify: bool = True):
+    """Read a binary CSR written by save_csr.  With ``verify``: offsets monotone from 0 to the
+    adjacency length, neighbour ids in range, every list strictly ascending, an even
+    adjacency length, and the SHA-256 against the manifest (when present)."""
+    with open(path, "rb") as f:
+        head = f.read(32)
+        if len(head) != 32 or head[:8] != CSR_MAGIC:
+            raise ValueError(f"{path}: not an sgm binary CSR")
+        V, A, flags = (int(x) for x in np.frombuffer(head[8:], dtype="<u8"))
+        off = np.fromfile(f, dtype="<u4", count=V + 1)
+        adj = np.fromfile(f, dtype="<u4", count=A)
+        if off.size != V + 1 or adj.size != A:
+            raise ValueError(f"{path}: truncated")
+        if flags & 1:
+            labels = np.fromfile(f, dtype="<u4", count=V)
+            if labels.size != V:
+                raise ValueError(f"{path}: truncated labels")
+            if np.any(labels != 0):
+                raise ValueError(f"{path}: labelled graphs are not on the accelerated path")
+    g = cls(off.astype(np.uint32), adj.astype(np.uint32), name=os.path.splitext(os.path.basename(path))[0])
+    if verify:
+        if off[0] != 0 or off[-1] != A or np.any(np.diff(off.astype(np.int64)) < 0):
+            raise ValueError(f"{path}: offsets are not a CSR")
+        if A and (int(adj.max()) >= V):
+            raise ValueError(f"{path}: neighbour id out of range")
+        if A % 2:
+            raise ValueError(f"{path}: odd adjacency length (the graph must be undirected)")
+        if A > 1:
+            # adj[p] < adj[p+1] unless p+1 starts a list
+            up = np.diff(adj.astype(np.int64)) > 0
+            starts = off[1:-1].astype(np.int64) - 1
+            starts = starts[(starts >= 0) & (starts < A - 1)]
+            up[starts] = True
+            if not up.all():
+                raise ValueError(f"{path}: neighbour lists must be strictly ascending")
+        mpath = path + ".json"
+        if os.path.exists(mpath):
+            with open(mpath) as mf:
+                manifest = json.load(mf)
+            if manifest.get("sha256") != g.digest():
+                raise ValueError(f"{path}: SHA-256 differs from {mpath}")
+    return g
+
+
+def load_cached(text_path: str, cls=Graph):
+    """The reference text format through a binary cache: <text_path>.csr is written on the
+    first load (with the source's name and size in its manifest) and read afterwards while
+    it is newer than the text file."""
+    csr = text_path + ".csr"
+    if os.path.exists(csr) and os.path.getmtime(csr) >= os.path.getmtime(text_path):
+        try:
+            return load_csr(csr, cls=cls)
+        except ValueError:
+            pass  # stale or damaged: rebuild below
+    g = load_text(text_path, cls=cls)
+    save_csr(g, csr, {"source": os.path.basename(text_path), "source_bytes": os.path.getsize(text_path)})
+    return g
+
