@@ -46,6 +46,9 @@ def lib():
         if not os.path.exists(path):
             build()
         _lib = C.CDLL(path)
+        if _lib.orc_abi_version() != 2:  # a stale build from before the label arguments
+            build(force=True)
+            raise RuntimeError("liboracle.so was stale and has been rebuilt; restart the process")
         assert _lib.orc_stats_size() == C.sizeof(Stats)
     return _lib
 
@@ -58,11 +61,18 @@ def _u32(a):
     return np.ascontiguousarray(a, dtype=np.uint32)
 
 
+def _lab(g):
+    """Vertex labels of a graph as a pointer, or None (every label 0)."""
+    lab = getattr(g, "labels", None)
+    return None if lab is None else _p(_u32(lab))
+
+
 def nlf(g, q):
     V, n = g.getVertexCnt(), q.getVertexCnt()
     cand = np.zeros((n, V), dtype=np.uint32)
     cnt = np.zeros(n, dtype=np.uint32)
-    ok = lib().orc_nlf(C.c_uint32(V), _p(g.off), C.c_uint32(n), _p(q.off), _p(cand), _p(cnt))
+    ok = lib().orc_nlf(C.c_uint32(V), _p(g.off), _p(g.adj), _lab(g), C.c_uint32(n), _p(q.off), _p(q.adj),
+                       _lab(q), _p(cand), _p(cnt))
     return bool(ok), [cand[u, :cnt[u]].copy() for u in range(n)]
 
 
@@ -71,8 +81,8 @@ def gql(g, q, return_stats=False):
     cand = np.zeros((n, V), dtype=np.uint32)
     cnt = np.zeros(n, dtype=np.uint32)
     stats = np.zeros(4, dtype=np.uint64)
-    ok = lib().orc_gql(C.c_uint32(V), _p(g.off), _p(g.adj), C.c_uint32(n), _p(q.off), _p(q.adj),
-                       _p(cand), _p(cnt), _p(stats))
+    ok = lib().orc_gql(C.c_uint32(V), _p(g.off), _p(g.adj), _lab(g), C.c_uint32(n), _p(q.off), _p(q.adj),
+                       _lab(q), _p(cand), _p(cnt), _p(stats))
     res = (bool(ok), [cand[u, :cnt[u]].copy() for u in range(n)])
     if return_stats:
         return res + ({"checks": int(stats[0]), "removed_r0": int(stats[1]),
@@ -115,13 +125,13 @@ def order_gql(g, q, counts):
 
 
 def plan(q, order):
-    """PlanIR as a dict: ops (depth, parent, vdepth, edge depth list) and iter_set."""
+    """PlanIR as a dict: ops (depth, label, parent, vdepth, edge depth list) and iter_set."""
     n = q.getVertexCnt()
     o = _u32(order)
     nops = C.c_uint32(0)
     arrs = [np.zeros(512, dtype=np.uint32) for _ in range(4)]
     it = np.zeros(n, dtype=np.uint32)
-    rc = lib().orc_plan_export(C.c_uint32(n), _p(q.off), _p(q.adj), _p(o), C.byref(nops),
+    rc = lib().orc_plan_export(C.c_uint32(n), _p(q.off), _p(q.adj), _lab(q), _p(o), C.byref(nops),
                                *[_p(a) for a in arrs], _p(it))
     if rc:
         raise ValueError("order is not connected (empty iteration set)")
@@ -171,7 +181,7 @@ def enum_cache(g, q, cands, order, cap=None, fs=True, time_limit_ms=0.0):
     o = _u32(order)
     st = Stats()
     rc = lib().orc_enum_cache(C.c_uint32(V), _p(g.off), _p(g.adj), C.c_uint32(n), _p(q.off),
-                              _p(q.adj), _p(m), _p(cnt), C.c_uint32(stride), _p(o),
+                              _p(q.adj), _lab(q), _p(m), _p(cnt), C.c_uint32(stride), _p(o),
                               C.c_uint64(cap if cap is not None else 2**64 - 1),
                               C.c_int(1 if fs else 0), C.c_double(time_limit_ms), C.byref(st))
     if rc:

@@ -53,19 +53,40 @@ static uint32_t isect(const uint32_t *a, uint32_t na, const uint32_t *b, uint32_
 }
 
 /* ------------------------------------------------------------------------------------
- * NLF (inc/filter.h:24-58 NLFByVertex, :80-93 _NLF_Filter).  All labels are 0, so the
- * neighbour-label-frequency test reduces to deg(v) >= deg_q(u) (a query vertex of degree
- * 0 has an empty NLF map and accepts every data vertex).  Candidates are written in
- * ascending vertex id into cand[u*V ...].  Returns 1, or 0 at the first empty set (the
- * reference stops there, :87-90).
+ * NLF (inc/filter.h:24-58 NLFByVertex, :80-93 _NLF_Filter).  v may take u iff
+ * label(v) = label(u) (the scan runs over getVertexByLabel, inc/graph.h:96-112),
+ * deg(v) >= deg_q(u), and for every label l among u's neighbours v has at least as many
+ * neighbours of label l as u has (the NLF maps, inc/graph.h:76-91).  dlab / qlab NULL
+ * means every label is 0; then the test reduces to deg(v) >= deg_q(u) (a query vertex of
+ * degree 0 has an empty NLF map and accepts every data vertex).  Candidates are written in
+ * ascending vertex id into cand[u*V ...] (sortCandidates, :13-22).  Returns 1, or 0 at the
+ * first empty set (the reference stops there, :87-90).
  * ---------------------------------------------------------------------------------- */
-int orc_nlf(uint32_t V, const uint32_t *off, uint32_t n, const uint32_t *qoff,
-            uint32_t *cand, uint32_t *cnt)
+static uint32_t lab_of(const uint32_t *lab, uint32_t v) { return lab ? lab[v] : 0u; }
+
+int orc_nlf(uint32_t V, const uint32_t *off, const uint32_t *adj, const uint32_t *dlab, uint32_t n,
+            const uint32_t *qoff, const uint32_t *qadj, const uint32_t *qlab, uint32_t *cand, uint32_t *cnt)
 {
     for (uint32_t u = 0; u < n; u++) {
         uint32_t qd = qoff[u + 1] - qoff[u], c = 0;
-        for (uint32_t v = 0; v < V; v++)
-            if (dg(off, v) >= qd) cand[(size_t)u * V + c++] = v;
+        /* u's NLF map as (label, count) pairs */
+        uint32_t ql[ORC_MAXQ], qc[ORC_MAXQ], np = 0;
+        for (uint32_t p = qoff[u]; p < qoff[u + 1]; p++) {
+            uint32_t l = lab_of(qlab, qadj[p]), k = 0;
+            while (k < np && ql[k] != l) k++;
+            if (k == np) { ql[np] = l; qc[np++] = 0; }
+            qc[k]++;
+        }
+        for (uint32_t v = 0; v < V; v++) {
+            if (lab_of(dlab, v) != lab_of(qlab, u) || dg(off, v) < qd) continue;
+            int ok = 1;
+            for (uint32_t k = 0; k < np && ok; k++) {
+                uint32_t have = 0;
+                for (uint32_t p = off[v]; p < off[v + 1]; p++) have += lab_of(dlab, adj[p]) == ql[k];
+                ok = have >= qc[k];
+            }
+            if (ok) cand[(size_t)u * V + c++] = v;
+        }
         cnt[u] = c;
         if (c == 0) return 0;
     }
@@ -129,12 +150,12 @@ static int lsm_check(uint32_t V, const uint32_t *off, const uint32_t *adj,
  * cleared immediately, :285-287).  Then compaction in ascending order (:206-220) and the
  * empty-set check (:222-230).  stats (optional, 4 entries): checks executed, removals
  * in round 0, removals in round 1, reserved. */
-int orc_gql(uint32_t V, const uint32_t *off, const uint32_t *adj, uint32_t n,
-            const uint32_t *qoff, const uint32_t *qadj, uint32_t *cand, uint32_t *cnt,
+int orc_gql(uint32_t V, const uint32_t *off, const uint32_t *adj, const uint32_t *dlab, uint32_t n,
+            const uint32_t *qoff, const uint32_t *qadj, const uint32_t *qlab, uint32_t *cand, uint32_t *cnt,
             uint64_t *stats)
 {
     if (stats) memset(stats, 0, 4 * sizeof(uint64_t));
-    if (!orc_nlf(V, off, n, qoff, cand, cnt)) return 0;
+    if (!orc_nlf(V, off, adj, dlab, n, qoff, qadj, qlab, cand, cnt)) return 0;
     uint8_t *valid = (uint8_t *)calloc((size_t)n * V, 1);
     uint32_t qmax = 0, dmax = 0;
     for (uint32_t u = 0; u < n; u++) if (qoff[u + 1] - qoff[u] > qmax) qmax = qoff[u + 1] - qoff[u];
@@ -314,7 +335,7 @@ void orc_order_gql(uint32_t V, uint32_t n, const uint32_t *qoff, const uint32_t 
  * ---------------------------------------------------------------------------------- */
 typedef struct {
     uint32_t depth;   /* depth at which the set is computed */
-    uint32_t label;   /* always 0 (unlabeled) */
+    uint32_t label;   /* label of the follow vertex (VertexSetIR::_v_label) */
     uint32_t vdepth;  /* depth of the follow vertex that produced it */
     uint32_t size;
     uint32_t e[ORC_MAXQ]; /* sorted depth indices of the backward neighbours */
@@ -474,8 +495,8 @@ static int op_equal(const orc_op *a, const orc_op *b)
 
 /* PlanIR(order, query) (inc/code_gen.h:699-811).  Returns 0 on success, -1 when an
  * iteration set is empty (the order is not connected; the reference exits, :743-746). */
-int orc_plan_build(uint32_t n, const uint32_t *qoff, const uint32_t *qadj, const uint32_t *order,
-                   orc_plan *pl)
+int orc_plan_build(uint32_t n, const uint32_t *qoff, const uint32_t *qadj, const uint32_t *qlab,
+                   const uint32_t *order, orc_plan *pl)
 {
     uint32_t dep_of[ORC_MAXQ];
     orc_op vir[ORC_MAXQ];
@@ -500,13 +521,13 @@ int orc_plan_build(uint32_t n, const uint32_t *qoff, const uint32_t *qadj, const
         for (uint32_t fd = dep + 1; fd < n; fd++) {
             const orc_op *src = &vir[order[fd]];
             orc_op o; memset(&o, 0, sizeof(o));
-            o.depth = dep; o.vdepth = fd; o.label = 0;
+            o.depth = dep; o.vdepth = fd; o.label = lab_of(qlab, order[fd]);
             for (uint32_t k = 0; k < src->size; k++) if (src->e[k] <= dep) o.e[o.size++] = src->e[k];
             if (o.size) all[na++] = o;
         }
         const orc_op *it = &vir[order[dep + 1]];
         orc_op o; memset(&o, 0, sizeof(o));
-        o.depth = dep; o.vdepth = dep + 1; o.label = 0;
+        o.depth = dep; o.vdepth = dep + 1; o.label = lab_of(qlab, order[dep + 1]);
         for (uint32_t k = 0; k < it->size; k++) if (it->e[k] <= dep) o.e[o.size++] = it->e[k];
         if (o.size == 0) return -1;
         iter_tmp[dep + 1] = o;
@@ -547,12 +568,12 @@ int orc_plan_build(uint32_t n, const uint32_t *qoff, const uint32_t *qadj, const
 }
 
 /* Flattened plan export for ctypes: arrays of n_ops entries plus iter_set[n]. */
-int orc_plan_export(uint32_t n, const uint32_t *qoff, const uint32_t *qadj, const uint32_t *order,
-                    uint32_t *n_ops, uint32_t *depth, uint32_t *parent, uint32_t *vdepth,
-                    uint32_t *emask, uint32_t *iter_set)
+int orc_plan_export(uint32_t n, const uint32_t *qoff, const uint32_t *qadj, const uint32_t *qlab,
+                    const uint32_t *order, uint32_t *n_ops, uint32_t *depth, uint32_t *parent,
+                    uint32_t *vdepth, uint32_t *emask, uint32_t *iter_set)
 {
     static orc_plan pl;
-    int rc = orc_plan_build(n, qoff, qadj, order, &pl);
+    int rc = orc_plan_build(n, qoff, qadj, qlab, order, &pl);
     if (rc) return rc;
     *n_ops = pl.n_ops;
     for (uint32_t i = 0; i < pl.n_ops; i++) {
@@ -587,7 +608,10 @@ static double thread_ms(void)
  * explore_Intersection_Cache_DAF_Filter, :968-1071 _explore_recur_cache_daf_filter,
  * :200-243 CacheBuffer::getNextCandidatesImpl(conf_dep), inc/prune.h:82-171
  * DAFFailingPrune).  use_fs = 0 disables the failing-set bookkeeping (cache + filter
- * only), which makes node counts schedule independent.
+ * only), which makes node counts schedule independent.  Labels enter through the plan
+ * (ops are keyed by the follow vertex's label) and the candidate sets only: this overload
+ * of getNextCandidatesImpl reads whole neighbour lists (data.getNeb, :214), not the
+ * label-partitioned ones of the overload without conf_dep (:170-171).
  * ---------------------------------------------------------------------------------- */
 typedef struct {
     uint32_t V, n;
@@ -694,14 +718,14 @@ static void daf_ancestors(uint32_t n, const uint32_t *qoff, const uint32_t *qadj
 }
 
 int orc_enum_cache(uint32_t V, const uint32_t *off, const uint32_t *adj, uint32_t n,
-                   const uint32_t *qoff, const uint32_t *qadj, const uint32_t *cand,
+                   const uint32_t *qoff, const uint32_t *qadj, const uint32_t *qlab, const uint32_t *cand,
                    const uint32_t *cnt, uint32_t cand_stride, const uint32_t *order, uint64_t cap,
                    int use_fs, double time_limit_ms, orc_stats *st)
 {
     memset(st, 0, sizeof(*st));
     if (n < 2 || n > ORC_MAXQ) return -2;
     static orc_plan pl;
-    if (orc_plan_build(n, qoff, qadj, order, &pl)) return -1;
+    if (orc_plan_build(n, qoff, qadj, qlab, order, &pl)) return -1;
     cache_ctx c;
     memset(&c, 0, sizeof(c));
     c.V = V; c.n = n; c.off = off; c.adj = adj; c.order = order; c.pl = &pl; c.cap = cap;
@@ -861,5 +885,5 @@ int orc_enum_all(uint32_t V, const uint32_t *off, const uint32_t *adj, uint32_t 
     return 0;
 }
 
-uint32_t orc_abi_version(void) { return 1u; }
+uint32_t orc_abi_version(void) { return 2u; }
 uint32_t orc_stats_size(void) { return (uint32_t)sizeof(orc_stats); }

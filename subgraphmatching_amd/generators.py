@@ -1,8 +1,8 @@
 """Seeded synthetic inputs: Chung-Lu and R-MAT data graphs, random-walk queries.
 
 Every generator takes an explicit integer seed and uses numpy's PCG64, so the same seed
-gives the same graph on every host.  All graphs are unlabeled, simple and undirected,
-returned as sorted CSR ``Graph`` objects.  The reference ships no generator; these
+gives the same graph on every host.  Graphs are simple and undirected, returned as sorted
+CSR ``Graph`` objects, unlabeled unless ``with_labels`` adds labels.  The reference ships no generator; these
 definitions are this project's own (the workload sizes follow the benchmark
 configurations: Chung-Lu 10^4 / 10^5 vertices, R-MAT scale 20 / 22).
 """
@@ -12,7 +12,7 @@ import numpy as np
 
 from .graph import Graph, Query, csr_from_edges
 
-__all__ = ["chung_lu", "rmat", "random_walk_query", "query_set"]
+__all__ = ["chung_lu", "rmat", "random_walk_query", "query_set", "with_labels"]
 
 
 def _dedup_keep_first(keys: np.ndarray) -> np.ndarray:
@@ -113,6 +113,15 @@ def rmat(scale: int, num_edges: int, seed: int, a=0.57, b=0.19, c=0.19, d=0.05,
     return _finish(V, keys, rng, permute, f"rmat_{scale}_{num_edges}_s{seed}")
 
 
+def with_labels(g: Graph, num_labels: int, seed: int, skew: float = 0.0) -> Graph:
+    """The same CSR with vertex labels 0..num_labels-1 drawn independently per vertex,
+    label l with probability proportional to (l+1)^-skew (0 = uniform)."""
+    rng = np.random.Generator(np.random.PCG64([seed, 0x1ABE1]))
+    p = np.arange(1, num_labels + 1, dtype=np.float64) ** -float(skew)
+    lab = rng.choice(num_labels, size=g.getVertexCnt(), p=p / p.sum())
+    return Graph(g.off, g.adj, name=f"{g.name}_L{num_labels}", labels=lab)
+
+
 def _is_dense(k: int, e: int) -> bool:
     """Dense = average degree > 3.  Below 5 vertices that is impossible (K4 has average
     degree 3), so there dense means at least k+1 edges (a diamond or K4 at k=4; the
@@ -172,7 +181,9 @@ def random_walk_query(g: Graph, k: int, dense: bool, rng: np.random.Generator,
             if _is_dense(k, len(es)) or 2.0 * len(es) / k > 3.0:
                 continue
         qoff, qadj = csr_from_edges(k, np.array(es, dtype=np.int64).reshape(-1, 2))
-        return Query(qoff, qadj, name=f"rw{k}{'d' if dense else 's'}")
+        # labelled graphs: query vertex i takes the label of the walk's i-th vertex
+        labels = None if g.labels is None else g.labels[np.array(order, dtype=np.int64)]
+        return Query(qoff, qadj, name=f"rw{k}{'d' if dense else 's'}", labels=labels)
     raise RuntimeError(f"no {'dense' if dense else 'sparse'} {k}-vertex query found")
 
 

@@ -2,8 +2,9 @@
 
 ``Graph`` / ``Query`` mirror the reference's ``Graph`` / ``Query`` classes
 (inc/graph.h:51-378, :380-495): an undirected simple graph held as a CSR with each
-adjacency list sorted ascending (inc/graph.h:259-260).  Only the unlabeled case is on the
-accelerated path, so every vertex label is 0.
+adjacency list sorted ascending (inc/graph.h:259-260), plus optional vertex labels
+(``labels`` is None when every label is 0).  Labels are u32 values; the reference numbers
+them 0..L-1 (inc/graph.h:93-112).
 
 Text format (README.md:64-84, parse rules inc/graph.h:190-266)::
 
@@ -57,7 +58,7 @@ def csr_from_edges(num_vertices: int, edges: np.ndarray):
 class Graph:
     """Undirected simple graph in sorted CSR form (inc/graph.h:51-378)."""
 
-    def __init__(self, off: np.ndarray, adj: np.ndarray, name: str = ""):
+    def __init__(self, off: np.ndarray, adj: np.ndarray, name: str = "", labels=None):
         self.off = np.ascontiguousarray(off, dtype=np.uint32)
         self.adj = np.ascontiguousarray(adj, dtype=np.uint32)
         self.name = name
@@ -65,11 +66,29 @@ class Graph:
             raise ValueError("offsets must be a 1-D array of V+1 entries")
         if int(self.off[-1]) != self.adj.size:
             raise ValueError("offsets do not cover the adjacency array")
+        self.labels = None
+        self.set_labels(labels)
+
+    def set_labels(self, labels):
+        """Attach vertex labels (V values; None or all zero = unlabeled)."""
+        if labels is None:
+            self.labels = None
+            return self
+        lab = np.ascontiguousarray(labels, dtype=np.int64)
+        if lab.shape != (self.getVertexCnt(),):
+            raise ValueError(f"expected {self.getVertexCnt()} labels, got shape {lab.shape}")
+        if lab.size and (lab.min() < 0 or lab.max() > 0xFFFFFFFE):
+            raise ValueError("labels must be in [0, 2^32 - 1)")
+        self.labels = lab.astype(np.uint32) if np.any(lab) else None
+        return self
+
+    def is_labelled(self) -> bool:
+        return self.labels is not None
 
     @classmethod
-    def from_edges(cls, num_vertices: int, edges, name: str = ""):
+    def from_edges(cls, num_vertices: int, edges, name: str = "", labels=None):
         off, adj = csr_from_edges(num_vertices, np.asarray(edges))
-        return cls(off, adj, name)
+        return cls(off, adj, name, labels=labels)
 
     # reference accessors -------------------------------------------------------------
     def getVertexCnt(self) -> int:  # inc/graph.h:291
@@ -94,7 +113,13 @@ class Graph:
         return i < nb.size and int(nb[i]) == v
 
     def getVertexLabel(self, v: int) -> int:  # inc/graph.h:359
-        return 0
+        return 0 if self.labels is None else int(self.labels[v])
+
+    def label_array(self) -> np.ndarray:
+        """Labels of every vertex (zeros when unlabeled)."""
+        if self.labels is None:
+            return np.zeros(self.getVertexCnt(), dtype=np.uint32)
+        return self.labels
 
     # helpers ---------------------------------------------------------------------------
     def degrees(self) -> np.ndarray:
@@ -108,15 +133,18 @@ class Graph:
         return np.stack([src[m], dst[m]], axis=1)
 
     def digest(self) -> str:
+        """SHA-256 of V, the CSR and (for labelled graphs) the labels."""
         h = hashlib.sha256()
         h.update(np.uint64(self.getVertexCnt()).tobytes())
         h.update(self.off.tobytes())
         h.update(self.adj.tobytes())
+        if self.labels is not None:
+            h.update(self.labels.tobytes())
         return h.hexdigest()
 
     def Load(self, path: str):  # inc/graph.h:190-266
         g = load_text(path)
-        self.off, self.adj, self.name = g.off, g.adj, g.name
+        self.off, self.adj, self.name, self.labels = g.off, g.adj, g.name, g.labels
         return self
 
 
@@ -125,8 +153,8 @@ class Query(Graph):
 
     MAX_VERTICES = 32
 
-    def __init__(self, off, adj, name: str = ""):
-        super().__init__(off, adj, name)
+    def __init__(self, off, adj, name: str = "", labels=None):
+        super().__init__(off, adj, name, labels=labels)
         if self.getVertexCnt() > self.MAX_VERTICES:
             raise ValueError(f"query has {self.getVertexCnt()} vertices; at most 32 are supported")
 
@@ -146,10 +174,10 @@ class Query(Graph):
 
 
 def load_text(path: str, cls=Graph):
-    """Read the reference text format (inc/graph.h:190-266)."""
-    labels_seen = set()
+    """Read the reference text format (inc/graph.h:190-266), labels included."""
     V = E = None
     decl_deg = None
+    labels = None
     edges = []
     with open(path) as f:
         for line in f:
@@ -159,18 +187,19 @@ def load_text(path: str, cls=Graph):
             if parts[0] == "t":
                 V, E = int(parts[1]), int(parts[2])
                 decl_deg = np.zeros(V, dtype=np.int64)
+                labels = np.zeros(V, dtype=np.int64)
             elif parts[0] == "v":
                 vid, lab, d = int(parts[1]), int(parts[2]), int(parts[3])
-                labels_seen.add(lab)
+                if not 0 <= vid < V:
+                    raise ValueError(f"{path}: vertex id {vid} out of range")
+                labels[vid] = lab
                 decl_deg[vid] = d
             elif parts[0] == "e":
                 edges.append((int(parts[1]), int(parts[2])))
     if V is None:
         raise ValueError(f"{path}: missing 't' header")
-    if labels_seen - {0}:
-        raise ValueError(f"{path}: labelled graphs are not on the accelerated path")
     g = cls.from_edges(V, np.array(edges, dtype=np.int64).reshape(-1, 2),
-                       name=os.path.splitext(os.path.basename(path))[0])
+                       name=os.path.splitext(os.path.basename(path))[0], labels=labels)
     if g.getEdgeCnt() != E or not np.array_equal(g.degrees(), decl_deg):
         raise ValueError(f"{path}: header/degree fields disagree with the edge list "
                          "(the reference requires exact degrees and a simple graph)")
@@ -178,13 +207,14 @@ def load_text(path: str, cls=Graph):
 
 
 def save_text(g: Graph, path: str):
-    """Write the reference text format (all labels 0)."""
+    """Write the reference text format."""
     V = g.getVertexCnt()
     deg = g.degrees()
+    lab = g.label_array()
     e = g.edges()
     with open(path, "w") as f:
         f.write(f"t {V} {e.shape[0]}\n")
-        f.write("".join(f"v {i} 0 {int(deg[i])}\n" for i in range(V)))
+        f.write("".join(f"v {i} {int(lab[i])} {int(deg[i])}\n" for i in range(V)))
         f.write("".join(f"e {int(a)} {int(b)}\n" for a, b in e))
 
 
@@ -206,9 +236,11 @@ def save_csr(g: Graph, path: str, meta: dict | None = None) -> dict:
     adj = np.ascontiguousarray(g.adj, dtype=np.uint32)
     with open(path, "wb") as f:
         f.write(CSR_MAGIC)
-        f.write(np.array([V, adj.size, 0], dtype="<u8").tobytes())
+        f.write(np.array([V, adj.size, 1 if g.labels is not None else 0], dtype="<u8").tobytes())
         f.write(off.astype("<u4").tobytes())
         f.write(adj.astype("<u4").tobytes())
+        if g.labels is not None:
+            f.write(g.labels.astype("<u4").tobytes())
     manifest = {"format": "sgm-csr-1", "vertices": int(V), "edges": int(g.getEdgeCnt()), "sha256": g.digest(),
                 "name": g.name}
     manifest.update(meta or {})
@@ -230,13 +262,15 @@ def load_csr(path: str, cls=Graph, verify: bool = True):
         adj = np.fromfile(f, dtype="<u4", count=A)
         if off.size != V + 1 or adj.size != A:
             raise ValueError(f"{path}: truncated")
+        labels = None
         if flags & 1:
             labels = np.fromfile(f, dtype="<u4", count=V)
             if labels.size != V:
                 raise ValueError(f"{path}: truncated labels")
-            if np.any(labels != 0):
-                raise ValueError(f"{path}: labelled graphs are not on the accelerated path")
-    g = cls(off.astype(np.uint32), adj.astype(np.uint32), name=os.path.splitext(os.path.basename(path))[0])
+            if labels.size and labels.max() == 0xFFFFFFFF:
+                raise ValueError(f"{path}: label out of range")
+    g = cls(off.astype(np.uint32), adj.astype(np.uint32), name=os.path.splitext(os.path.basename(path))[0],
+            labels=labels)
     if verify:
         if off[0] != 0 or off[-1] != A or np.any(np.diff(off.astype(np.int64)) < 0):
             raise ValueError(f"{path}: offsets are not a CSR")

@@ -1,12 +1,14 @@
 """Brute-force checkers used to pin the oracle (pure Python, tiny inputs only).
 
 These are independent of the oracle's algorithms: embeddings by exhaustive injective
-maps, the GQL local matching by Hall's condition over every subset of the left side,
+maps (labels must agree), NLF by its definition (label, degree and neighbour-label
+counts), the GQL local matching by Hall's condition over every subset of the left side,
 tables by an adjacency test on every pair.
 """
 from __future__ import annotations
 
 import itertools
+from collections import Counter
 
 import numpy as np
 
@@ -18,24 +20,39 @@ def adjsets(g):
 
 
 def count_embeddings(g, q):
-    """Number of injective maps q -> g preserving every query edge (non-induced)."""
-    A = adjsets(g)
-    QA = adjsets(q)
-    n, V = q.getVertexCnt(), g.getVertexCnt()
-    total = 0
-    for m in itertools.permutations(range(V), n):
-        if all(m[b] in A[m[a]] for a in range(n) for b in QA[a] if a < b):
-            total += 1
-    return total
+    """Number of injective, label-preserving maps q -> g preserving every query edge
+    (non-induced)."""
+    return sum(1 for _ in embeddings(g, q))
 
 
 def embeddings(g, q):
     A = adjsets(g)
     QA = adjsets(q)
     n, V = q.getVertexCnt(), g.getVertexCnt()
+    gl, ql = g.label_array().tolist(), q.label_array().tolist()
     for m in itertools.permutations(range(V), n):
-        if all(m[b] in A[m[a]] for a in range(n) for b in QA[a] if a < b):
+        if all(gl[m[a]] == ql[a] for a in range(n)) and \
+                all(m[b] in A[m[a]] for a in range(n) for b in QA[a] if a < b):
             yield m
+
+
+def nlf_by_definition(g, q):
+    """NLF candidate sets from the definition: same label, deg(v) >= deg_q(u), and for each
+    label the neighbours of v carry at least as often as u's do."""
+    A = adjsets(g)
+    gl, ql = g.label_array().tolist(), q.label_array().tolist()
+    out = []
+    for u in range(q.getVertexCnt()):
+        need = Counter(ql[w] for w in q.getNeb(u).tolist())
+        c = []
+        for v in range(g.getVertexCnt()):
+            if gl[v] != ql[u] or len(A[v]) < q.getVertexDegree(u):
+                continue
+            have = Counter(gl[w] for w in A[v])
+            if all(have[l] >= k for l, k in need.items()):
+                c.append(v)
+        out.append(c)
+    return out
 
 
 def hall_ok(left_nbrs):
@@ -53,9 +70,8 @@ def gql_by_hall(g, q, jacobi=False, rounds=2):
     once) with the local matching decided by Hall's condition.  ``jacobi`` and
     ``rounds`` exist only so tests can show the schedule matters."""
     A = adjsets(g)
-    n, V = q.getVertexCnt(), g.getVertexCnt()
-    deg = g.degrees()
-    cands = [[v for v in range(V) if deg[v] >= q.getVertexDegree(u)] for u in range(n)]
+    n = q.getVertexCnt()
+    cands = nlf_by_definition(g, q)
     if any(len(c) == 0 for c in cands):
         return False, cands
     valid = [set(c) for c in cands]
@@ -116,6 +132,6 @@ def random_connected_query(n, extra, rng):
     return query_from_edges(n, sorted(es))
 
 
-def query_from_edges(n, es):
+def query_from_edges(n, es, labels=None):
     g = Graph.from_edges(n, np.array(es, dtype=np.int64).reshape(-1, 2))
-    return Query(g.off, g.adj)
+    return Query(g.off, g.adj, labels=labels)

@@ -11,7 +11,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from subgraphmatching_amd.generators import chung_lu, rmat
+from subgraphmatching_amd.generators import chung_lu, rmat, with_labels
 from subgraphmatching_amd.graph import (CSR_MAGIC, Graph, Query, load_cached, load_csr, load_text, save_csr,
                                         save_text)
 
@@ -93,17 +93,29 @@ def test_damaged_files_are_rejected(tmp_path):
         load_csr(cp)
 
 
-def test_labelled_csr_is_refused(tmp_path):
-    g = Graph.from_edges(3, [(0, 1), (1, 2)])
-    cp = str(tmp_path / "l.csr")
-    with open(cp, "wb") as f:
-        f.write(CSR_MAGIC)
-        f.write(np.array([3, g.adj.size, 1], dtype="<u8").tobytes())
-        f.write(g.off.astype("<u4").tobytes())
-        f.write(g.adj.astype("<u4").tobytes())
-        f.write(np.array([0, 2, 0], dtype="<u4").tobytes())
-    with pytest.raises(ValueError, match="labelled"):
+def test_labelled_round_trip(tmp_path):
+    g = with_labels(chung_lu(800, 2.5, 6.0, seed=7), 5, seed=8, skew=1.0)
+    assert g.is_labelled()
+    tp, cp = str(tmp_path / "l.txt"), str(tmp_path / "l.csr")
+    save_text(g, tp)
+    gt = load_text(tp)
+    assert _same(g, gt) and np.array_equal(gt.labels, g.labels) and gt.digest() == g.digest()
+    m = save_csr(gt, cp)
+    raw = open(cp, "rb").read()
+    V = g.getVertexCnt()
+    assert int.from_bytes(raw[24:32], "little") == 1
+    assert np.array_equal(np.frombuffer(raw[-4 * V:], dtype="<u4"), g.labels)
+    gc = load_csr(cp)
+    assert np.array_equal(gc.labels, g.labels) and gc.digest() == m["sha256"]
+    # labels are part of the digest: a changed label is caught
+    bad = bytearray(raw)
+    bad[-4] ^= 1
+    open(cp, "wb").write(bytes(bad))
+    with pytest.raises(ValueError):
         load_csr(cp)
+    # all-zero labels are the unlabeled graph
+    z = Graph(g.off, g.adj, labels=np.zeros(V, dtype=np.uint32))
+    assert not z.is_labelled() and z.digest() == chung_lu(800, 2.5, 6.0, seed=7).digest()
 
 
 def test_load_cached_writes_then_reuses(tmp_path):
@@ -135,12 +147,17 @@ static unsigned long long fold(const Graph& g) {
     }
     return h;
 }
+static unsigned long long labs(const Graph& g) {
+    unsigned long long h = 0;
+    for (uint32_t v = 0; v < g.getVertexCnt(); v++) h = h * 31 + g.getVertexLabel(v);
+    return h;
+}
 int main(int argc, char** argv) {
     Graph a, b;
     a.Load(argv[1]);
     b.LoadCSR(argv[2]);
-    std::printf("%u %u %u %llu\n", a.getVertexCnt(), a.getEdgeCnt(), a.getMaxDgree(), fold(a));
-    std::printf("%u %u %u %llu\n", b.getVertexCnt(), b.getEdgeCnt(), b.getMaxDgree(), fold(b));
+    std::printf("%u %u %u %llu %llu\n", a.getVertexCnt(), a.getEdgeCnt(), a.getMaxDgree(), fold(a), labs(a));
+    std::printf("%u %u %u %llu %llu\n", b.getVertexCnt(), b.getEdgeCnt(), b.getMaxDgree(), fold(b), labs(b));
     try { Graph c; c.LoadCSR(argv[3]); std::printf("accepted\n"); }
     catch (const std::exception& e) { std::printf("rejected\n"); }
     return 0;
@@ -154,7 +171,7 @@ def test_cpp_loadcsr_matches_text_load(tmp_path):
     lib = os.path.join(PKG, "libsgm_hip.so")
     if not os.path.exists(lib):
         pytest.skip("libsgm_hip.so not built")
-    g = rmat(10, 8, seed=9)
+    g = with_labels(rmat(10, 8, seed=9), 4, seed=10)
     tp, cp, bad = str(tmp_path / "g.txt"), str(tmp_path / "g.csr"), str(tmp_path / "bad.csr")
     save_text(g, tp)
     save_csr(g, cp)
@@ -168,6 +185,7 @@ def test_cpp_loadcsr_matches_text_load(tmp_path):
     assert r.returncode == 0, r.stderr
     out = subprocess.run([str(exe), tp, cp, bad], check=True, capture_output=True, text=True).stdout.split("\n")
     assert out[0] == out[1]
-    V, E, D, _ = out[0].split()
+    V, E, D, _, h = out[0].split()
+    assert int(h) != 0
     assert (int(V), int(E), int(D)) == (g.getVertexCnt(), g.getEdgeCnt(), g.getMaxDgree())
     assert out[2] == "rejected"

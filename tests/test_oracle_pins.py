@@ -11,9 +11,9 @@ import numpy as np
 import pytest
 
 import oracle as O
-from brute import (count_embeddings, embeddings, gql_by_hall, prefix_nodes, query_from_edges,
+from brute import (count_embeddings, embeddings, gql_by_hall, nlf_by_definition, prefix_nodes, query_from_edges,
                    random_connected_query, random_graph, adjsets)
-from subgraphmatching_amd.generators import chung_lu, query_set
+from subgraphmatching_amd.generators import chung_lu, query_set, with_labels
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -21,12 +21,16 @@ def _build():
     O.build()
 
 
-def small_cases(seed, count, V=(7, 10), p=(0.3, 0.6), n=(3, 5)):
+def small_cases(seed, count, V=(7, 10), p=(0.3, 0.6), n=(3, 5), labels=0):
+    """Random small (graph, query) pairs; labels > 0 draws labels 0..labels-1 for both."""
     rng = np.random.default_rng(seed)
     for _ in range(count):
         g = random_graph(int(rng.integers(*V)), float(rng.uniform(*p)), rng)
         k = int(rng.integers(n[0], n[1] + 1))
         q = random_connected_query(k, int(rng.integers(0, 3)), rng)
+        if labels:
+            g.set_labels(rng.integers(0, labels, g.getVertexCnt()))
+            q.set_labels(rng.integers(0, labels, k))
         yield g, q
 
 
@@ -50,10 +54,35 @@ def test_nlf_empty_set_fails():
         assert not ok
 
 
+def test_labelled_nlf_by_definition():
+    """Labelled NLF (inc/filter.h:24-58): label, degree and neighbour-label counts, from
+    the definition; the plausible slips (no label test, degree only, counts ignored) are
+    each caught by some case."""
+    g = with_labels(chung_lu(400, 2.5, 6.0, seed=31), 3, seed=32, skew=1.0)
+    cases = [(g, q) for q in query_set(g, 8, 6, False, seed=33) + query_set(g, 8, 7, True, seed=34)]
+    cases += list(small_cases(35, 120, labels=2))
+    slips = {"no_label": 0, "no_counts": 0}
+    for gg, q in cases:
+        ok, c = O.nlf(gg, q)
+        exp = nlf_by_definition(gg, q)
+        assert ok == all(len(x) > 0 for x in exp)
+        if ok:
+            assert [x.tolist() for x in c] == exp
+        deg = gg.degrees()
+        gl, ql = gg.label_array(), q.label_array()
+        for u in range(q.getVertexCnt()):
+            only_deg = np.flatnonzero(deg >= q.getVertexDegree(u)).tolist()
+            lab_deg = np.flatnonzero((deg >= q.getVertexDegree(u)) & (gl == ql[u])).tolist()
+            slips["no_label"] += lab_deg != only_deg
+            slips["no_counts"] += lab_deg != exp[u]
+    assert slips["no_label"] > 0 and slips["no_counts"] > 0
+
+
 # ---------------------------------------------------------------- GQL: Hall's condition
-def test_gql_matches_hall_restatement():
+@pytest.mark.parametrize("labels", [0, 2])
+def test_gql_matches_hall_restatement(labels):
     diff_jacobi = diff_rounds = 0
-    for g, q in small_cases(1, 150):
+    for g, q in small_cases(1, 150, labels=labels):
         ok, c = O.gql(g, q)
         ok_b, c_b = gql_by_hall(g, q)
         assert ok == ok_b
@@ -197,6 +226,26 @@ def test_plan_known_answers():
     assert p["iter_set"] == [None, 0, 1, 2]
 
 
+def test_plan_labelled_known_answers():
+    """Ops are keyed by (label of the follow vertex, edge list) (inc/code_gen.h:640-653):
+    equal edge lists with different labels stay apart, sort by label before size, and a
+    parent must carry the same label (:777-787)."""
+    star = query_from_edges(3, [(0, 1), (0, 2)])
+    assert _ops(O.plan(star, [0, 1, 2])) == [(0, [0], None, 1)]
+    lab = query_from_edges(3, [(0, 1), (0, 2)], labels=[0, 5, 3])
+    p = O.plan(lab, [0, 1, 2])
+    assert _ops(p) == [(0, [0], None, 2), (0, [0], None, 1)]
+    assert p["iter_set"] == [None, 1, 0]
+    # triangle 0-1-2 plus pendant 3 on 0; vertex 2's label differs from 1's and 3's
+    es = [(0, 1), (0, 2), (1, 2), (0, 3)]
+    p = O.plan(query_from_edges(4, es, labels=[0, 1, 2, 1]), [0, 1, 2, 3])
+    assert _ops(p) == [(0, [0], None, 1), (0, [0], None, 2), (1, [0, 1], 1, 2)]
+    assert p["iter_set"] == [None, 0, 2, 0]
+    p = O.plan(query_from_edges(4, es), [0, 1, 2, 3])
+    assert _ops(p) == [(0, [0], None, 1), (1, [0, 1], 0, 2)]
+    assert p["iter_set"] == [None, 0, 1, 0]
+
+
 def test_plan_invariants():
     g = chung_lu(800, 2.5, 6.0, seed=14)
     for q in query_set(g, 12, lambda i, r: int(r.integers(4, 33)), lambda i, r: bool(i % 2), seed=15):
@@ -234,9 +283,10 @@ def test_op_sort_matches_libstdcxx():
 
 
 # ---------------------------------------------------------------- enumeration: counts
-def test_counts_match_brute_force():
+@pytest.mark.parametrize("labels", [0, 2])
+def test_counts_match_brute_force(labels):
     rng = np.random.default_rng(17)
-    for g, q in small_cases(4, 80):
+    for g, q in small_cases(4, 80, labels=labels):
         exp = count_embeddings(g, q)
         ok, c = O.gql(g, q)
         if not ok:
@@ -297,11 +347,13 @@ def test_aux_path_nodes_and_intersections_brute():
         assert st["intersections"] == sum(nodes[d - 1] * (nprev[d] - 1) for d in range(1, len(order)))
 
 
-def test_cache_path_nodes_and_intersections_brute():
+@pytest.mark.parametrize("labels", [0, 2])
+def test_cache_path_nodes_and_intersections_brute(labels):
     """Cache + filter with failing sets off: a depth-d node exists iff the prefix is a
     valid partial embedding and every op of every shallower depth was non-empty; each
-    visit of depth d evaluates depth d-1's ops in plan order up to the first empty one."""
-    for g, q in small_cases(6, 60):
+    visit of depth d evaluates depth d-1's ops in plan order up to the first empty one.
+    Ops intersect whole neighbour lists also for labelled queries (inc/explore.h:214)."""
+    for g, q in small_cases(6, 60, labels=labels):
         ok, c = O.gql(g, q)
         if not ok:
             continue
