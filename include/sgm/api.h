@@ -38,6 +38,7 @@ public:
         sgm_query* q = nullptr;
         sgm::check(sgm_query_create(g, query.offsets(), query.neighbours(), _qv_cnt, &q), "query create");
         _dev = q;
+        if (!query.unlabeled()) sgm::check(sgm_query_set_labels(q, query.getVertexLabels(), _qv_cnt), "query labels");
     }
     CandidateParam(const CandidateParam&) = delete;
     CandidateParam& operator=(const CandidateParam&) = delete;

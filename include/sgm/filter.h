@@ -10,8 +10,6 @@
 namespace sgm {
 inline bool run_filter(const Graph& data, const Query& query, CandidateParam& can, int kind)
 {
-    if (!data.unlabeled() || !query.unlabeled())
-        throw Error(SGM_EINVAL, "labelled graphs are not on the accelerated path (all labels must be 0)");
     if (&can.data() != &data || &can.query() != &query)
         throw Error(SGM_EINVAL, "CandidateParam was built for another data graph or query");
     std::lock_guard<std::mutex> lk(device_mutex());

@@ -2,9 +2,9 @@
 // inc/graph.h:13-49) for the drop-in C++ surface.
 //
 // A Graph keeps the reference's CSR on the host (sorted neighbour lists) and uploads itself
-// to HBM on first use; it stays resident until the Graph is destroyed.  Only unlabeled
-// graphs (every label 0) are on the accelerated path: Load accepts labels, and the filter
-// calls reject graphs that carry any other label.
+// to HBM on first use, labels included; it stays resident until the Graph is destroyed.
+// Labels reach the device filters (labelled NLF / GQL) and the plan; the data graph's
+// labels are fixed once it is on the device.
 #pragma once
 
 #include <algorithm>
@@ -198,6 +198,7 @@ public:
             sgm::check(sgm_graph_upload(sgm::context(), _offset.data(), _neb.data(), _v_cnt, _neb.size(), &g),
                        "graph upload");
             _dev = g;
+            if (!unlabeled()) sgm::check(sgm_graph_set_labels(g, _labels.data(), _v_cnt), "graph labels");
         }
         return _dev;
     }

@@ -112,6 +112,16 @@ uint32_t sgm_graph_max_degree(const sgm_graph* g);
 int sgm_query_create(sgm_graph* g, const uint32_t* qoff, const uint32_t* qadj, uint32_t n, sgm_query** out);
 void sgm_query_destroy(sgm_query* q);
 
+/* ---- vertex labels (replaces the labels of Graph::Load / Query, inc/graph.h:190-266, and
+ * the NLF maps and label index built from them, inc/graph.h:76-112) --------------------
+ * labels[num_vertices]: u32 values.  NULL or all zero makes the graph unlabeled again.  The
+ * filters (NLF, GQL) and the plan honour labels when the graph or the query has any other
+ * than 0; enumeration follows from the candidate sets, as in the reference's cache path
+ * (inc/explore.h:200-235 intersects whole neighbour lists).  Set labels before filtering:
+ * a query's labels reset its filter state.  sgm_batch_filter refuses labelled graphs. */
+int sgm_graph_set_labels(sgm_graph* g, const uint32_t* labels, uint32_t num_vertices);
+int sgm_query_set_labels(sgm_query* q, const uint32_t* labels, uint32_t n);
+
 /* Filter_NLF / Filter_GQL (inc/filter.h:1049-1063, _NLF_Filter :80-93, _GQL_Filter :233-313).
  * Returns SGM_OK, or SGM_EEMPTY when some C(u) is empty (the reference returns false). */
 int sgm_filter(sgm_query* q, int kind);
@@ -143,6 +153,11 @@ int sgm_order_gql(const sgm_query* q, uint32_t* order);
  * of backward-neighbour depths; iter_set[n].  Arrays hold up to 512 ops. */
 int sgm_plan_export(const uint32_t* qoff, const uint32_t* qadj, uint32_t n, const uint32_t* order, uint32_t* n_ops,
                     uint32_t* depth, uint32_t* parent, uint32_t* vdepth, uint32_t* emask, uint32_t* iter_set);
+/* The same for a labelled query (labels[n], NULL = all 0): ops are keyed by the follow
+ * vertex's label (inc/code_gen.h:640-654, 777-808). */
+int sgm_plan_export_labeled(const uint32_t* qoff, const uint32_t* qadj, const uint32_t* labels, uint32_t n,
+                            const uint32_t* order, uint32_t* n_ops, uint32_t* depth, uint32_t* parent,
+                            uint32_t* vdepth, uint32_t* emask, uint32_t* iter_set);
 
 /* ---- enumeration (replaces explore_Intersection_Cache_DAF_Filter inc/explore.h:1073 and
  * explore_Intersection_All<false x5> :2757) -------------------------------------------- */

@@ -49,6 +49,7 @@ EXPORTS = [
     "sgm_costmodel_create", "sgm_costmodel_destroy", "sgm_costmodel_param_count", "sgm_costmodel_params",
     "sgm_costmodel_features", "sgm_costmodel_predict", "sgm_costmodel_grad", "sgm_costmodel_train",
     "sgm_costmodel_timing", "sgm_order_sample", "sgm_batch_rank_orders", "sgm_gql_work",
+    "sgm_graph_set_labels", "sgm_query_set_labels", "sgm_plan_export_labeled",
 ]
 
 
@@ -117,6 +118,9 @@ def lib() -> C.CDLL:
         "sgm_order_ri": (C.c_int, [vp, vp, C.c_uint32, vp]),
         "sgm_order_gql": (C.c_int, [vp, vp]),
         "sgm_plan_export": (C.c_int, [vp, vp, C.c_uint32, vp, u32p, vp, vp, vp, vp, vp]),
+        "sgm_plan_export_labeled": (C.c_int, [vp, vp, vp, C.c_uint32, vp, u32p, vp, vp, vp, vp, vp]),
+        "sgm_graph_set_labels": (C.c_int, [vp, vp, C.c_uint32]),
+        "sgm_query_set_labels": (C.c_int, [vp, vp, C.c_uint32]),
         "sgm_enumerate": (C.c_int, [vp, vp, C.POINTER(EnumOpts), C.POINTER(Stats)]),
         "sgm_enumerate_batch": (C.c_int, [vp, vp, C.c_uint32, C.POINTER(EnumOpts), vp]),
         "sgm_batch_create": (C.c_int, [vp, C.c_uint32, C.POINTER(vp)]),

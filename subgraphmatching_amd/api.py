@@ -129,6 +129,9 @@ class DeviceGraph:
                                         C.byref(h)), ctx)
         self.handle = h
         self.max_degree = int(L.lib().sgm_graph_max_degree(h))
+        if g.labels is not None:
+            lab = L.u32(g.labels)
+            _check(L.lib().sgm_graph_set_labels(h, L.ptr(lab), g.getVertexCnt()), ctx)
 
     def close(self):
         if self.handle:
@@ -197,6 +200,9 @@ class CandidateParam:
         _check(L.lib().sgm_query_create(self.dg.handle, L.ptr(qoff), L.ptr(qadj), query.getVertexCnt(),
                                         C.byref(h)), self.ctx)
         self._q = h
+        if query.labels is not None:
+            qlab = L.u32(query.labels)
+            _check(L.lib().sgm_query_set_labels(h, L.ptr(qlab), query.getVertexCnt()), self.ctx)
         self._filtered = False
         self._counts = None
         self._cands = None
@@ -308,14 +314,16 @@ def Order_GQL(data: Graph, query: Query, can: CandidateParam) -> list:
 
 
 def plan_export(query: Query, order):
-    """The intersection-cache plan (inc/code_gen.h:682-847) as built by the library."""
+    """The intersection-cache plan (inc/code_gen.h:682-847) as built by the library; ops of
+    a labelled query are keyed by the follow vertex's label."""
     n = query.getVertexCnt()
     qoff, qadj, o = L.u32(query.off), L.u32(query.adj), L.u32(order)
+    qlab = None if query.labels is None else L.u32(query.labels)
     nops = C.c_uint32()
     arrs = [np.zeros(512, dtype=np.uint32) for _ in range(4)]
     it = np.zeros(n, dtype=np.uint32)
-    code = L.lib().sgm_plan_export(L.ptr(qoff), L.ptr(qadj), n, L.ptr(o), C.byref(nops), *[L.ptr(a) for a in arrs],
-                                   L.ptr(it))
+    code = L.lib().sgm_plan_export_labeled(L.ptr(qoff), L.ptr(qadj), None if qlab is None else L.ptr(qlab), n,
+                                           L.ptr(o), C.byref(nops), *[L.ptr(a) for a in arrs], L.ptr(it))
     if code != L.SGM_OK:
         raise L.SgmError(code, "plan")
     k = nops.value
@@ -514,7 +522,11 @@ class Batch:
             pass
 
     def filter(self, queries, kind=L.SGM_FILTER_GQL):
-        """Returns (ok[nq] bools, cand_counts[nq, 32], cand_degsum[nq, 32])."""
+        """Returns (ok[nq] bools, cand_counts[nq, 32], cand_degsum[nq, 32]).  Batches run the
+        unlabeled filters: labelled queries (and labelled graphs, refused by the library) go
+        through CandidateParam / filter_batch instead."""
+        if any(q.labels is not None for q in queries):
+            raise ValueError("batches take unlabeled queries; filter labelled ones with filter_batch")
         nq = len(queries)
         self._keep = [(L.u32(q.off), L.u32(q.adj)) for q in queries]
         offs = (C.c_void_p * nq)(*[a.ctypes.data for a, _ in self._keep])

@@ -236,6 +236,8 @@ int sgm_batch_filter(sgm_batch* b, uint32_t nq, const uint32_t* const* qoffs, co
     if (nq == 0 || nq > b->slots) return set_err(c, SGM_EINVAL, "batch of %u queries for %u slots", nq, b->slots);
     if (kind != SGM_FILTER_NLF && kind != SGM_FILTER_GQL) return set_err(c, SGM_EINVAL, "unknown filter kind %d", kind);
     if (b->pending) return set_err(c, SGM_ESTATE, "wait for the batch's enumeration before loading another batch");
+    if (g->labelled)  // the replayed filter graphs run the unlabeled kernels only
+        return set_err(c, SGM_EINVAL, "batches take unlabeled graphs; filter labelled ones with sgm_filter_batch");
     HIPCHK(c, hipSetDevice(c->device));
     b->nq = 0;
     b->q.assign(nq, sgm::QueryCSR());

@@ -209,9 +209,89 @@ int sgm_graph_upload(sgm_ctx* c, const uint32_t* off, const uint32_t* adj, uint3
     return SGM_OK;
 }
 
+// Label tables of the labelled NLF (sgm_graph::d_lab, d_run_*), built on the host from the
+// resident CSR: per vertex the labels of its neighbours, sorted, as (label, count) runs.
+// labels == nullptr: every label 0 (one run per non-isolated vertex).
+static int build_label_tables(sgm_ctx* c, sgm_graph* g, const uint32_t* labels)
+{
+    const uint32_t V = g->V;
+    std::vector<uint32_t> off((size_t)V + 1), adj, lab(V, 0u), roff((size_t)V + 1, 0u), rlab, rcnt;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpy(off.data(), g->d_off, sizeof(uint32_t) * ((size_t)V + 1), hipMemcpyDeviceToHost));
+    if (labels) {
+        lab.assign(labels, labels + V);
+        adj.resize(g->adj_len);
+        if (g->adj_len)
+            HIPCHK(c, hipMemcpy(adj.data(), g->d_adj, sizeof(uint32_t) * g->adj_len, hipMemcpyDeviceToHost));
+    }
+    rlab.reserve(V);
+    rcnt.reserve(V);
+    std::vector<uint32_t> t;
+    for (uint32_t v = 0; v < V; v++) {
+        const uint32_t d = off[v + 1] - off[v];
+        if (!labels) {
+            if (d) { rlab.push_back(0); rcnt.push_back(d); }
+        } else {
+            t.resize(d);
+            for (uint32_t k = 0; k < d; k++) t[k] = lab[adj[off[v] + k]];
+            std::sort(t.begin(), t.end());
+            for (uint32_t k = 0; k < d;) {
+                uint32_t e = k;
+                while (e < d && t[e] == t[k]) e++;
+                rlab.push_back(t[k]);
+                rcnt.push_back(e - k);
+                k = e;
+            }
+        }
+        roff[v + 1] = (uint32_t)rlab.size();
+    }
+    for (uint32_t** p : {&g->d_lab, &g->d_run_off, &g->d_run_lab, &g->d_run_cnt}) {
+        dfree(*p);
+        *p = nullptr;
+    }
+    g->has_lab = false;
+    const size_t R = std::max<size_t>(rlab.size(), 1);
+    HIPCHK(c, dalloc(&g->d_lab, std::max<uint32_t>(V, 1)));
+    HIPCHK(c, dalloc(&g->d_run_off, (size_t)V + 1));
+    HIPCHK(c, dalloc(&g->d_run_lab, R));
+    HIPCHK(c, dalloc(&g->d_run_cnt, R));
+    HIPCHK(c, hipMemcpy(g->d_lab, lab.data(), sizeof(uint32_t) * V, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(g->d_run_off, roff.data(), sizeof(uint32_t) * ((size_t)V + 1), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(g->d_run_lab, rlab.data(), sizeof(uint32_t) * rlab.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(g->d_run_cnt, rcnt.data(), sizeof(uint32_t) * rcnt.size(), hipMemcpyHostToDevice));
+    g->has_lab = true;
+    g->labelled = false;
+    for (uint32_t v = 0; v < V && !g->labelled; v++) g->labelled = lab[v] != 0;
+    return SGM_OK;
+}
+
+int sgm_graph_set_labels(sgm_graph* g, const uint32_t* labels, uint32_t num_vertices)
+{
+    if (!g) return SGM_EINVAL;
+    sgm_ctx* c = g->ctx;
+    if (num_vertices != g->V)
+        return set_err(c, SGM_EINVAL, "expected %u labels, got %u", g->V, num_vertices);
+    bool any = false;
+    for (uint32_t v = 0; labels && v < num_vertices && !any; v++) any = labels[v] != 0;
+    if (!any) {  // unlabeled: the tables go; a labelled query rebuilds them with every label 0
+        for (uint32_t** p : {&g->d_lab, &g->d_run_off, &g->d_run_lab, &g->d_run_cnt}) {
+            dfree(*p);
+            *p = nullptr;
+        }
+        g->has_lab = false;
+        g->labelled = false;
+        return SGM_OK;
+    }
+    return build_label_tables(c, g, labels);
+}
+
 void sgm_graph_destroy(sgm_graph* g)
 {
     if (!g) return;
+    dfree(g->d_lab);
+    dfree(g->d_run_off);
+    dfree(g->d_run_lab);
+    dfree(g->d_run_cnt);
     dfree(g->d_off);
     dfree(g->d_adj);
     dfree(g->d_bydeg);
@@ -250,6 +330,7 @@ void sgm_query_destroy(sgm_query* q)
     dfree(q->d_cnt);
     dfree(q->d_candoff);
     dfree(q->d_list);
+    dfree(q->d_lreq);
     dfree(q->d_qdesc);
     dfree(q->d_cdesc);
     delete q;
@@ -327,6 +408,49 @@ int sgm_query_create(sgm_graph* g, const uint32_t* qoff, const uint32_t* qadj, u
     return SGM_OK;
 }
 
+int sgm_query_set_labels(sgm_query* q, const uint32_t* labels, uint32_t n)
+{
+    if (!q) return SGM_EINVAL;
+    if (n != q->q.n) return set_err(q->g->ctx, SGM_EINVAL, "expected %u labels, got %u", q->q.n, n);
+    bool any = false;
+    for (uint32_t u = 0; labels && u < n && !any; u++) any = labels[u] != 0;
+    if (any) q->q.lab.assign(labels, labels + n);
+    else q->q.lab.clear();
+    q->labelled = any;
+    dfree(q->d_lreq);
+    q->d_lreq = nullptr;
+    q->filtered = 0;  // candidates and tables of the old labels are stale
+    return SGM_OK;
+}
+
+// The lreq block of q (sgm_qdesc::lreq): per u its neighbours' labels with multiplicities.
+static int ensure_lreq(sgm_ctx* c, sgm_query* q)
+{
+    if (q->d_lreq) return SGM_OK;
+    const sgm::QueryCSR& Q = q->q;
+    const uint32_t n = Q.n;
+    std::vector<uint32_t> first(n + 1, 0), plab, pcnt;
+    for (uint32_t u = 0; u < n; u++) {
+        std::vector<uint32_t> t;
+        for (uint32_t p = Q.off[u]; p < Q.off[u + 1]; p++) t.push_back(Q.label(Q.adj[p]));
+        std::sort(t.begin(), t.end());
+        for (size_t k = 0; k < t.size();) {
+            size_t e = k;
+            while (e < t.size() && t[e] == t[k]) e++;
+            plab.push_back(t[k]);
+            pcnt.push_back((uint32_t)(e - k));
+            k = e;
+        }
+        first[u + 1] = (uint32_t)plab.size();
+    }
+    std::vector<uint32_t> blk(first);
+    blk.insert(blk.end(), plab.begin(), plab.end());
+    blk.insert(blk.end(), pcnt.begin(), pcnt.end());
+    HIPCHK(c, dalloc(&q->d_lreq, blk.size()));
+    HIPCHK(c, hipMemcpy(q->d_lreq, blk.data(), sizeof(uint32_t) * blk.size(), hipMemcpyHostToDevice));
+    return SGM_OK;
+}
+
 // ------------------------------------------------------------------------------ filter
 // One chunk of a filter batch: NLF (+ GQL) for nq queries in one set of launches, then the
 // candidate bitmaps and counts.  Adds the device time of the three stages to ms[].
@@ -336,9 +460,23 @@ static int filter_chunk(sgm_ctx* c, sgm_graph* g, sgm_query* const* qs, uint32_t
     std::vector<sgm_qdesc> qd(nq);
     std::vector<sgm_cdesc> cd(nq);
     uint32_t max_n = 0;
+    bool any_lab = false;
     for (uint32_t i = 0; i < nq; i++) {
         qd[i] = qs[i]->hq;
         cd[i] = qs[i]->hc;
+        // labelled NLF / GQL when the graph or the query carries labels other than 0
+        qd[i].lab_on = (g->labelled || qs[i]->labelled) ? 1u : 0u;
+        if (qd[i].lab_on) {
+            if (!g->has_lab) {
+                const int rc = build_label_tables(c, g, nullptr);
+                if (rc) return rc;
+            }
+            const int rc = ensure_lreq(c, qs[i]);
+            if (rc) return rc;
+            for (uint32_t u = 0; u < qs[i]->q.n; u++) qd[i].qlab[u] = qs[i]->q.label(u);
+            qd[i].lreq = qs[i]->d_lreq;
+            any_lab = true;
+        }
         if (kind == SGM_FILTER_GQL) {
             sgm_gql_layout(qd[i], c->d_gqls + sgm_gql_scratch_words(V) * i, V);
         }
@@ -374,8 +512,17 @@ static int filter_chunk(sgm_ctx* c, sgm_graph* g, sgm_query* const* qs, uint32_t
     if (kind == SGM_FILTER_GQL && V)  // touch and dirty words start clear; the rest is written before it is read
         sgm::launch_fill_strided(c->stream, c->d_gqls + V, sgm_gql_clear_words(V), sgm_gql_scratch_words(V), nq, 0);
     HIPCHK(c, hipEventRecord(c->fev[0], c->stream));
-    if (kind == SGM_FILTER_GQL && V) sgm::launch_nlf_rank(c->stream, g->d_gecnt, V, d_qd, nq);
-    else sgm::launch_nlf(c->stream, g->d_off, V, d_qd, nq);
+    if (kind == SGM_FILTER_GQL && V) {
+        sgm::launch_nlf_rank(c->stream, g->d_gecnt, V, d_qd, nq);
+        if (any_lab)
+            sgm::launch_nlf_rank_lab(c->stream, g->d_bydeg, g->d_off, g->d_lab, g->d_run_off, g->d_run_lab,
+                                     g->d_run_cnt, V, d_qd, nq);
+    } else {
+        sgm::launch_nlf(c->stream, g->d_off, V, d_qd, nq);
+        if (any_lab && V)
+            sgm::launch_nlf_lab(c->stream, g->d_off, g->d_lab, g->d_run_off, g->d_run_lab, g->d_run_cnt, V, d_qd,
+                                nq);
+    }
     HIPCHK(c, hipEventRecord(c->fev[1], c->stream));
     if (kind == SGM_FILTER_GQL && V) {
         sgm::launch_gql_prepass(c->stream, g->d_dtab, V, g->d_gecnt, d_qd, nq);
@@ -650,8 +797,16 @@ int sgm_order_gql(const sgm_query* q, uint32_t* order)
 int sgm_plan_export(const uint32_t* qoff, const uint32_t* qadj, uint32_t n, const uint32_t* order, uint32_t* n_ops,
                     uint32_t* depth, uint32_t* parent, uint32_t* vdepth, uint32_t* emask, uint32_t* iter_set)
 {
+    return sgm_plan_export_labeled(qoff, qadj, nullptr, n, order, n_ops, depth, parent, vdepth, emask, iter_set);
+}
+
+int sgm_plan_export_labeled(const uint32_t* qoff, const uint32_t* qadj, const uint32_t* labels, uint32_t n,
+                            const uint32_t* order, uint32_t* n_ops, uint32_t* depth, uint32_t* parent,
+                            uint32_t* vdepth, uint32_t* emask, uint32_t* iter_set)
+{
     sgm::QueryCSR q;
     if (!order || !n_ops || csr_from(qoff, qadj, n, q)) return SGM_EINVAL;
+    if (labels) q.lab.assign(labels, labels + n);
     sgm::Plan pl;
     const int rc = sgm::build_plan(q, order, pl);
     if (rc == -2) return SGM_EORDER;

@@ -128,13 +128,15 @@ int check_order(const QueryCSR& q, const uint32_t* order)
 // ---------------------------------------------------------------------------------------
 // Intersection-cache plan (inc/code_gen.h:682-811).  For every depth `dep` and every later
 // vertex, the set of its backward neighbours' depths that are <= dep is a candidate set op
-// computed at depth `dep`.  Ops are ordered by (depth, label, size, lexicographic depths)
+// computed at depth `dep`, labelled with the follow vertex's label (VertexSetIR::_v_label,
+// :716).  Ops are ordered by (depth, label, size, lexicographic depths)
 // with std::sort -- the same comparator as the reference, so that ops that tie keep the
 // same relative order under the same standard library and the first copy of a duplicated
 // op (which decides its v_depth, the failing-set source when it turns out empty) is the
-// same one.  Duplicates (equal depth lists) are dropped, each op's parent is the op
-// equal to its depth list minus the last element, and the iteration set of depth d is the
-// op equal to all backward neighbours of order[d].
+// same one.  Ops are equal when label and depth list are (:652-654).  Duplicates are
+// dropped, each op's parent is the op with its label and its depth list minus the last
+// element (:777-787), and the iteration set of depth d is the op equal to order[d]'s label
+// with all its backward neighbours (:800-808).
 struct OpIR {
     uint32_t depth = SGM_INVALID_U32;
     uint32_t label = 0;
@@ -174,6 +176,7 @@ int build_plan(const QueryCSR& q, const uint32_t* order, Plan& pl)
         for (uint32_t fd = dep + 1; fd < n; fd++) {
             OpIR o;
             o.depth = dep;
+            o.label = q.label(order[fd]);
             o.vdepth = fd;
             o.e = back[order[fd]] & upto;
             if (o.e) all.push_back(o);
@@ -185,9 +188,10 @@ int build_plan(const QueryCSR& q, const uint32_t* order, Plan& pl)
     // duplicates (equal sets) keep the first in sorted order; parents and iteration sets
     // are then one map probe each
     std::vector<OpIR> ops;
-    std::unordered_map<uint32_t, uint32_t> first_of;  // edge set -> op index
+    std::unordered_map<uint64_t, uint32_t> first_of;  // (label, edge set) -> op index
+    auto key = [](uint32_t label, uint32_t e) { return ((uint64_t)label << 32) | e; };
     for (const OpIR& o : all)
-        if (first_of.emplace(o.e, (uint32_t)ops.size()).second) ops.push_back(o);
+        if (first_of.emplace(key(o.label, o.e), (uint32_t)ops.size()).second) ops.push_back(o);
     if (ops.size() > SGM_MAXOPS) return -3;
     pl.n = n;
     pl.order.assign(order, order + n);
@@ -201,12 +205,12 @@ int build_plan(const QueryCSR& q, const uint32_t* order, Plan& pl)
         po.edges.clear();
         for (uint32_t x = m; x; x &= x - 1) po.edges.push_back((uint32_t)__builtin_ctz(x));
         // the parent computes the list without its deepest edge
-        const auto it = first_of.find(m & ~(0x80000000u >> __builtin_clz(m)));
+        const auto it = first_of.find(key(ops[i].label, m & ~(0x80000000u >> __builtin_clz(m))));
         if (it != first_of.end() && it->second < i) po.parent = it->second;
     }
     pl.iter_set.assign(n, SGM_INVALID_U32);
     for (uint32_t d = 1; d < n; d++) {
-        const auto it = first_of.find(iter_e[d]);
+        const auto it = first_of.find(key(q.label(order[d]), iter_e[d]));
         if (it != first_of.end()) pl.iter_set[d] = it->second;
     }
     // DAF ancestors over depths (inc/prune.h:100-104)

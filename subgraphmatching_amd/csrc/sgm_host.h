@@ -13,6 +13,8 @@ namespace sgm {
 struct QueryCSR {
     uint32_t n = 0;
     std::vector<uint32_t> off, adj;
+    std::vector<uint32_t> lab;  // vertex labels; empty = every label 0
+    uint32_t label(uint32_t u) const { return lab.empty() ? 0u : lab[u]; }
 };
 
 struct PlanOp {

@@ -115,7 +115,16 @@ struct sgm_qdesc {
     uint32_t* fsum2;                 // per SGM_GQL_GROUP2 groups: OR of fsum
     uint32_t* tsum2;                 // ... of tsum
     uint32_t* dsum2;                 // ... of dsum
+    // Labelled NLF / GQL (lab_on = the data graph or the query carries a label other than 0).
+    // lreq: first[n+1] | label[P] | count[P] (P = first[n]): u's neighbour labels with their
+    // multiplicities, ascending by label, in entries [first[u], first[u+1]).
+    uint32_t lab_on;
+    uint32_t qlab[SGM_MAXQ];
+    const uint32_t* lreq;
 };
+
+// Words of a query's lreq block (see sgm_qdesc) for n vertices and adjacency length A.
+inline size_t sgm_lreq_words(uint32_t n, size_t A) { return (size_t)n + 1 + 2 * A; }
 
 // Points d's GQL scratch at `base` (sgm_gql_scratch_words(V) words, layout above).
 inline void sgm_gql_layout(sgm_qdesc& d, uint32_t* base, uint32_t V)

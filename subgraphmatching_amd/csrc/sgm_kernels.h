@@ -29,6 +29,13 @@ void launch_gql_phase(hipStream_t s, const uint32_t* roff, const uint32_t* radj,
 // p[rep * stride + i] = val for rep < reps, i < len
 void launch_fill_strided(hipStream_t s, uint32_t* p, size_t len, size_t stride, uint32_t reps, uint32_t val);
 void launch_nlf_rank(hipStream_t s, const uint32_t* ge_cnt, uint32_t V, const sgm_qdesc* qs, uint32_t nq);
+// labelled NLF (queries with lab_on; the unlabelled kernels skip those): by vertex id into
+// vmask, or by degree rank into vmr with every member touched (the start of a labelled GQL)
+void launch_nlf_lab(hipStream_t s, const uint32_t* off, const uint32_t* lab, const uint32_t* run_off,
+                    const uint32_t* run_lab, const uint32_t* run_cnt, uint32_t V, const sgm_qdesc* qs, uint32_t nq);
+void launch_nlf_rank_lab(hipStream_t s, const uint32_t* by_deg, const uint32_t* off, const uint32_t* lab,
+                         const uint32_t* run_off, const uint32_t* run_lab, const uint32_t* run_cnt, uint32_t V,
+                         const sgm_qdesc* qs, uint32_t nq);
 void launch_unrank(hipStream_t s, const uint32_t* rank, uint32_t V, const sgm_qdesc* qs, uint32_t nq);
 void launch_rank_adj(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* by_deg,
                      const uint32_t* rank, const uint32_t* roff, uint32_t V, uint32_t* radj);

@@ -20,13 +20,57 @@ __global__ void __launch_bounds__(256) k_nlf(const uint32_t* __restrict__ off, u
 {
     const sgm_qdesc* d = qs + blockIdx.y;
     const uint32_t n = d->n;
-    if (n == 0) return;  // unused batch slot
+    if (n == 0 || d->lab_on) return;  // unused batch slot / labelled query (k_nlf_lab)
     uint32_t* __restrict__ vm = d->vmask;
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
         uint32_t dv = off[v + 1] - off[v], m = 0;
         for (uint32_t u = 0; u < n; u++) m |= (dv >= d->qdeg[u] ? 1u : 0u) << u;
         vm[v] = m;
     }
+}
+
+// Labelled NLF (inc/filter.h:24-58): v may take u iff label(v) = label(u), deg(v) >=
+// deg_q(u), and for every label among u's neighbours v has at least as many neighbours of
+// that label (the NLF maps, inc/graph.h:76-91).  The data side holds each vertex's
+// neighbour-label counts as runs ascending by label (run_off[V+1], run_lab, run_cnt), the
+// query side u's (label, count) pairs ascending by label (sgm_qdesc::lreq); the count test
+// is one merge of the two lists.  Returns the membership word of v.
+__device__ __forceinline__ uint32_t nlf_lab_word(const sgm_qdesc* __restrict__ d, uint32_t n, uint32_t v,
+                                                 uint32_t dv, uint32_t lv, const uint32_t* __restrict__ run_off,
+                                                 const uint32_t* __restrict__ run_lab,
+                                                 const uint32_t* __restrict__ run_cnt)
+{
+    const uint32_t* __restrict__ first = d->lreq;
+    const uint32_t* __restrict__ plab = first + n + 1;
+    const uint32_t* __restrict__ pcnt = plab + first[n];
+    const uint32_t r0 = run_off[v], r1 = run_off[v + 1];
+    uint32_t m = 0;
+    for (uint32_t u = 0; u < n; u++) {
+        if (d->qlab[u] != lv || dv < d->qdeg[u]) continue;
+        bool ok = true;
+        uint32_t j = r0;
+        for (uint32_t p = first[u]; p < first[u + 1] && ok; p++) {
+            const uint32_t l = plab[p];
+            while (j < r1 && run_lab[j] < l) j++;
+            ok = j < r1 && run_lab[j] == l && run_cnt[j] >= pcnt[p];
+        }
+        if (ok) m |= 1u << u;
+    }
+    return m;
+}
+
+__global__ void __launch_bounds__(256) k_nlf_lab(const uint32_t* __restrict__ off, const uint32_t* __restrict__ lab,
+                                                 const uint32_t* __restrict__ run_off,
+                                                 const uint32_t* __restrict__ run_lab,
+                                                 const uint32_t* __restrict__ run_cnt, uint32_t V,
+                                                 const sgm_qdesc* __restrict__ qs)
+{
+    const sgm_qdesc* d = qs + blockIdx.y;
+    const uint32_t n = d->n;
+    if (n == 0 || !d->lab_on) return;
+    uint32_t* __restrict__ vm = d->vmask;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x)
+        vm[v] = nlf_lab_word(d, n, v, off[v + 1] - off[v], lab[v], run_off, run_lab, run_cnt);
 }
 
 // ================================================================== GQL
@@ -289,6 +333,9 @@ __global__ void __launch_bounds__(256) k_gql_phase(const uint32_t* __restrict__ 
     uint32_t* __restrict__ flags = d->removed;
     if (r == 1 && !((flags[SGM_GQL_DIRTY_ANY] >> u) & 1u)) return;  // no input of any check changed
     const bool use_touch = r == 0 && ((flags[SGM_GQL_TOUCH_ANY] >> u) & 1u);
+    // labelled: candidates below P_u need not be members, and every member is checked in
+    // round 1 (k_nlf_rank_lab set its touch bit)
+    const bool lab = d->lab_on != 0;
     const uint32_t low = L & ((1u << u) - 1u);
     const uint32_t high = u == 31u ? 0u : (L & (~0u << (u + 1u)));
     const uint32_t tm = r == 0 ? high : 0u;  // touch bits only matter to later round-1 phases
@@ -373,9 +420,10 @@ __global__ void __launch_bounds__(256) k_gql_phase(const uint32_t* __restrict__ 
                 const bool c = x < P;
                 const uint32_t fw = (c && r == 0) ? f0[x] : 0u;
                 const uint32_t tw = (c && (r == 1 || use_touch)) ? word[x] : 0u;
-                const uint32_t vw = (c && r == 1) ? vm[x] : 0u;
-                const bool rm = (fw >> u) & 1u;  // fails the NLF table test: fails now too
-                const bool need = !rm && ((tw >> u) & 1u) && (r == 0 || ((vw >> u) & 1u));
+                const uint32_t vw = (c && (r == 1 || lab)) ? vm[x] : 0u;
+                const bool mem = (r == 0 && !lab) || ((vw >> u) & 1u);  // x is in C(u) now
+                const bool rm = mem && ((fw >> u) & 1u);  // fails the NLF table test: fails now too
+                const bool need = mem && !rm && ((tw >> u) & 1u);
                 remove_now(rm, x);
                 const uint64_t bal = __ballot(need);
                 if (need) wq[(qt + mbcnt64(bal)) & (GQL_QCAP - 1u)] = x;
@@ -410,7 +458,7 @@ __global__ void __launch_bounds__(256) k_nlf_rank(const uint32_t* __restrict__ g
 {
     const sgm_qdesc* d = qs + blockIdx.y;
     const uint32_t n = d->n;
-    if (n == 0) return;
+    if (n == 0 || d->lab_on) return;
     __shared__ uint32_t s_P[SGM_MAXQ];  // thresholds, staged once per block
     if (threadIdx.x < n) s_P[threadIdx.x] = ge_cnt[d->qdeg[threadIdx.x]];
     __syncthreads();
@@ -420,6 +468,45 @@ __global__ void __launch_bounds__(256) k_nlf_rank(const uint32_t* __restrict__ g
         for (uint32_t u = 0; u < n; u++) m |= (x < s_P[u] ? 1u : 0u) << u;
         vmr[x] = m;
     }
+}
+
+// Labelled NLF in rank space, the start of a labelled GQL.  The degree-table shortcut (a
+// candidate that no removal has touched passes its round-1 check unexamined) holds only
+// for degree-nested sets, so every member gets a check: touch[x] = vmr[x], with the touch
+// summaries and the any-touch flags to match (those words start clear).
+__global__ void __launch_bounds__(256) k_nlf_rank_lab(const uint32_t* __restrict__ by_deg,
+                                                      const uint32_t* __restrict__ off,
+                                                      const uint32_t* __restrict__ lab,
+                                                      const uint32_t* __restrict__ run_off,
+                                                      const uint32_t* __restrict__ run_lab,
+                                                      const uint32_t* __restrict__ run_cnt, uint32_t V,
+                                                      const sgm_qdesc* __restrict__ qs)
+{
+    const sgm_qdesc* d = qs + blockIdx.y;
+    const uint32_t n = d->n;
+    if (n == 0 || !d->lab_on) return;
+    const uint32_t lane = lane_id();
+    uint32_t any = 0;
+    // block-uniform loop: every lane reaches the shuffles; rank blocks of 64 stay inside one
+    // level-2 group (SGM_GQL_GROUP * SGM_GQL_GROUP2 = 256 ranks)
+    for (uint32_t base = blockIdx.x * blockDim.x; base < V; base += gridDim.x * blockDim.x) {
+        const uint32_t x = base + threadIdx.x;
+        uint32_t m = 0;
+        if (x < V) {
+            const uint32_t v = by_deg[x];
+            m = nlf_lab_word(d, n, v, off[v + 1] - off[v], lab[v], run_off, run_lab, run_cnt);
+            d->vmr[x] = m;
+            d->touch[x] = m;
+        }
+        uint32_t g1 = m;
+#pragma unroll
+        for (uint32_t s = 1; s < SGM_GQL_GROUP; s <<= 1) g1 |= (uint32_t)__shfl_xor((int)g1, (int)s);
+        if (x < V && x % SGM_GQL_GROUP == 0) d->tsum[x / SGM_GQL_GROUP] = g1;
+        const uint32_t w = wave_or32(m);
+        if (lane == 0 && w) atomicOr(&d->tsum2[x / (SGM_GQL_GROUP * SGM_GQL_GROUP2)], w);
+        any |= w;
+    }
+    if (lane == 0 && any) atomicOr(&d->removed[SGM_GQL_TOUCH_ANY], any);
 }
 
 // Back to vertex ids after the last phase: vmask[v] = vmr[rank[v]].
@@ -798,6 +885,21 @@ void launch_nlf_rank(hipStream_t s, const uint32_t* ge_cnt, uint32_t V, const sg
 {
     dim3 grid(clampg((V + 255) / 256, 1, 4096), nq);
     hipLaunchKernelGGL(k_nlf_rank, grid, dim3(256), 0, s, ge_cnt, V, qs);
+}
+
+void launch_nlf_lab(hipStream_t s, const uint32_t* off, const uint32_t* lab, const uint32_t* run_off,
+                    const uint32_t* run_lab, const uint32_t* run_cnt, uint32_t V, const sgm_qdesc* qs, uint32_t nq)
+{
+    dim3 grid(clampg((V + 255) / 256, 1, 4096), nq);
+    hipLaunchKernelGGL(k_nlf_lab, grid, dim3(256), 0, s, off, lab, run_off, run_lab, run_cnt, V, qs);
+}
+
+void launch_nlf_rank_lab(hipStream_t s, const uint32_t* by_deg, const uint32_t* off, const uint32_t* lab,
+                         const uint32_t* run_off, const uint32_t* run_lab, const uint32_t* run_cnt, uint32_t V,
+                         const sgm_qdesc* qs, uint32_t nq)
+{
+    dim3 grid(clampg((V + 255) / 256, 1, 4096), nq);
+    hipLaunchKernelGGL(k_nlf_rank_lab, grid, dim3(256), 0, s, by_deg, off, lab, run_off, run_lab, run_cnt, V, qs);
 }
 
 void launch_unrank(hipStream_t s, const uint32_t* rank, uint32_t V, const sgm_qdesc* qs, uint32_t nq)

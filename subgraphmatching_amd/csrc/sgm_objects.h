@@ -84,6 +84,15 @@ struct sgm_graph {
     uint32_t* d_roff = nullptr;
     uint32_t* d_radj = nullptr;
     uint64_t* d_dtab = nullptr;
+    // vertex labels (labelled NLF / GQL): lab[V] by id, and each vertex's neighbour-label
+    // counts as runs ascending by label: run_off[V+1], run_lab[R], run_cnt[R].  Built by
+    // sgm_graph_set_labels, or with every label 0 for the first labelled query.
+    bool labelled = false;  // some label is not 0
+    bool has_lab = false;   // the label tables are on the device
+    uint32_t* d_lab = nullptr;
+    uint32_t* d_run_off = nullptr;
+    uint32_t* d_run_lab = nullptr;
+    uint32_t* d_run_cnt = nullptr;
     std::vector<uint32_t> hub_vertex;                   // bitmap slot -> vertex (degree descending)
     std::vector<uint32_t> hub_degree;
     std::vector<std::pair<uint32_t, uint32_t*>> hub_maps;  // (min degree, slot map) for higher thresholds
@@ -121,6 +130,8 @@ struct sgm_query {
     uint32_t* d_cnt = nullptr;
     uint32_t* d_candoff = nullptr;
     uint32_t* d_list = nullptr;
+    uint32_t* d_lreq = nullptr;       // labelled NLF requirements (sgm_qdesc::lreq), built on demand
+    bool labelled = false;            // some query label is not 0 (labels in q.lab)
     sgm_qdesc* d_qdesc = nullptr;
     sgm_cdesc* d_cdesc = nullptr;
     sgm_qdesc hq;                     // host copies of the descriptors

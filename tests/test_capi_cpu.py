@@ -11,7 +11,7 @@ import pytest
 import oracle as O
 import subgraphmatching_amd as sgm
 from subgraphmatching_amd import _lib as L
-from subgraphmatching_amd.generators import chung_lu, query_set
+from subgraphmatching_amd.generators import chung_lu, query_set, with_labels
 from brute import query_from_edges
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -52,6 +52,21 @@ def test_orders_and_plan_match_oracle():
         counts = np.random.default_rng(len(o)).integers(1, 50, q.getVertexCnt())
         og = O.order_gql(g, q, counts)
         assert sgm.plan_export(q, og) == O.plan(q, og)
+
+
+def test_labelled_plans_match_oracle():
+    """Ops keyed by (label, edge list): the library's plans of labelled queries equal the
+    oracle's, and labels change some plans (fewer shared ops)."""
+    g = with_labels(chung_lu(3000, 2.5, 7.4, seed=31), 3, seed=33)
+    qs = query_set(g, 40, lambda i, r: int(r.integers(3, 33)), lambda i, r: bool(i % 2), seed=34)
+    changed = 0
+    for q in qs:
+        assert q.is_labelled()
+        for o in (sgm.Order_RI(q), O.order_gql(g, q, np.random.default_rng(len(q.adj)).integers(1, 50, q.getVertexCnt()))):
+            p = sgm.plan_export(q, o)
+            assert p == O.plan(q, o)
+            changed += p != sgm.plan_export(sgm.Query(q.off, q.adj), o)
+    assert changed > 0
 
 
 def test_plan_rejects_disconnected_order():
