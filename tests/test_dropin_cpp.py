@@ -8,7 +8,7 @@ import subprocess
 import pytest
 
 import oracle as O
-from subgraphmatching_amd.generators import chung_lu, query_set
+from subgraphmatching_amd.generators import chung_lu, query_set, with_labels
 from subgraphmatching_amd.graph import save_text
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -92,9 +92,14 @@ def test_driver_reports_missing_gpu(tmp_path):
 
 
 @pytest.mark.gpu
-def test_driver_rows_match_oracle(tmp_path):
+@pytest.mark.parametrize("labels", [0, 3])
+def test_driver_rows_match_oracle(tmp_path, labels):
+    """The harness rows over reference-format text files (labels in the 'v' lines when
+    labels > 0) give the oracle's counts and intersection counts."""
     assert os.path.exists(DRIVER), "row driver not built"
     g = chung_lu(2000, 2.3, 8.0, seed=3)
+    if labels:
+        g = with_labels(g, labels, seed=72)
     qs = query_set(g, 6, lambda i, r: int(r.integers(4, 10)), lambda i, r: bool(i % 2), seed=71)
     data, paths = _write_inputs(tmp_path, g, qs)
     limit = 100000
