@@ -63,7 +63,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["chunglu100k", "rmat20", "rmat20-learned", "rmat22-serve"],
+    ap.add_argument("--workload", choices=["chunglu100k", "chunglu100k-labelled", "rmat20", "rmat20-learned",
+                                           "rmat22-serve"],
                     default="chunglu100k")
     ap.add_argument("--serve-mem-gb", type=float, default=32.0,
                     help="rmat22-serve: device memory budget for query slots and DFS stacks")
@@ -76,6 +77,8 @@ def parse():
     ap.add_argument("--rank-samples", type=int, default=32,
                     help="rmat20-learned: sampled orders ranked per query besides RI and GQL")
     ap.add_argument("--queries", type=int, default=0, help="number of queries (0 = the workload's own)")
+    ap.add_argument("--labels", type=int, default=8,
+                    help="chunglu100k-labelled: vertex labels 0..N-1, label l drawn with weight 1/(l+1)")
     ap.add_argument("--cap", type=int, default=10_000_000)
     ap.add_argument("--per-query", action="store_true",
                     help="chunglu100k: one enumeration launch per query instead of one batched launch")
@@ -129,9 +132,12 @@ def add_result(rec, r):
 def setup_chunglu(args):
     import subgraphmatching_amd as sgm
     from subgraphmatching_amd import _lib as L
-    from subgraphmatching_amd.generators import chung_lu, query_set
+    from subgraphmatching_amd.generators import chung_lu, query_set, with_labels
 
     g = chung_lu(100_000, 2.3, 16.0, seed=2)
+    labelled = args.workload == "chunglu100k-labelled"
+    if labelled:  # the same CSR with skewed labels; the queries' walks carry them
+        g = with_labels(g, args.labels, seed=23, skew=1.0)
     queries = query_set(g, args.queries or 200, 16, True, seed=22)
     ctx = sgm.default_context()
     cans = [sgm.CandidateParam(g, q, ctx) for q in queries]
@@ -170,9 +176,12 @@ def setup_chunglu(args):
         rec["filter_wall_s"] = t1 - t0
         return rec
 
-    config = {"workload": "chunglu_V100000_g2.3_d16_seed2__200x16v_dense_rw_seed22__gql_ri_cache_filter_fs",
+    name = "chunglu_V100000_g2.3_d16_seed2__200x16v_dense_rw_seed22__gql_ri_cache_filter_fs"
+    if labelled:
+        name = name.replace("seed2__", f"seed2_labels{args.labels}_skew1_seed23__")
+    config = {"workload": name,
               "queries_per_step": len(queries), "cap_per_query": args.cap, "order": "RI",
-              "filter": "NLF+GQL", "enumeration": "cache+filter+failing-sets",
+              "filter": "labelled NLF+GQL" if labelled else "NLF+GQL", "enumeration": "cache+filter+failing-sets",
               "launch": "per-query" if args.per_query else "one filter batch + one persistent enumeration launch"}
     return g, queries, step, config, {}
 

@@ -65,6 +65,12 @@ typedef struct {
 /* sgm_enum_opts.flags: evaluate the last level one extension at a time instead of a chunk of
  * extensions per pass (same results and statistics; for measurement) */
 #define SGM_ENUM_F_SERIAL_LAST 2u
+/* sgm_enum_opts.flags: by default, a query with at most 4096 candidate roots has the depth-1
+ * loops of its 64 highest-degree roots split into chunks for other waves.  NO_SPLIT searches
+ * every root in one wave; SPLIT_ALL splits every query (same results and statistics either
+ * way; for measurement) */
+#define SGM_ENUM_F_NO_SPLIT 4u
+#define SGM_ENUM_F_SPLIT_ALL 8u
 
 typedef struct {
     uint64_t count;          /* min(#embeddings, cap) */

@@ -55,6 +55,8 @@ EXPORTS = [
 
 SGM_ENUM_F_NO_BITMAPS = 1
 SGM_ENUM_F_SERIAL_LAST = 2
+SGM_ENUM_F_NO_SPLIT = 4
+SGM_ENUM_F_SPLIT_ALL = 8
 
 
 class EnumOpts(C.Structure):

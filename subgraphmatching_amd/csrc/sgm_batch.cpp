@@ -20,7 +20,7 @@
 namespace {
 
 struct EnumGraph {
-    uint32_t max_ops, use_fs, blocks, words, serial;
+    uint32_t max_ops, use_fs, blocks, words, serial, split;
     const uint32_t* hub;
     hipGraphExec_t exec;
 };
@@ -41,7 +41,9 @@ struct sgm_batch {
     sgm_plan_dev* d_plans = nullptr;         // slots
     sgm_equery* d_eq = nullptr;              // slots
     sgm_stats_dev* d_st = nullptr;           // slots
-    uint32_t* d_qstop = nullptr;             // 2 * slots: stop words, then item counters
+    uint32_t* d_qstop = nullptr;             // 3 * slots: stop words, item counters, split chunk counters
+    uint32_t* d_split = nullptr;             // slots * SGM_SPLIT_WORDS: depth-1 split blocks
+    unsigned long long* d_rec = nullptr;     // slots * SGM_SPLIT_FIELDS * SGM_SPLIT_ITEMS: chunk records
     uint32_t* d_lastbits = nullptr;          // slots * ceil(V/32): bitmap of C(u_L) per slot
     uint32_t* d_stack = nullptr;
     size_t stack_words_total = 0;
@@ -72,6 +74,7 @@ struct sgm_batch {
     uint32_t pend_bucket = 0, pend_blocks = 0;
     uint64_t pend_words = 0;
     int pend_use_fs = 0, pend_serial = 0;
+    bool pend_split = false;
     const uint32_t* pend_hub = nullptr;
     sgm_stats_dev* h_got = nullptr;          // pinned: per-slot statistics copied back
 };
@@ -166,6 +169,8 @@ void sgm_batch_destroy(sgm_batch* b)
     dfree(b->d_eq);
     dfree(b->d_st);
     dfree(b->d_qstop);
+    dfree(b->d_split);
+    dfree(b->d_rec);
     dfree(b->d_lastbits);
     dfree(b->d_stack);
     dfree(b->d_rank_given);
@@ -198,7 +203,9 @@ int sgm_batch_create(sgm_graph* g, uint32_t slots, sgm_batch** out)
     if (e == hipSuccess) e = dalloc(&b->d_plans, slots);
     if (e == hipSuccess) e = dalloc(&b->d_eq, slots);
     if (e == hipSuccess) e = dalloc(&b->d_st, slots);
-    if (e == hipSuccess) e = dalloc(&b->d_qstop, 2 * (size_t)slots);
+    if (e == hipSuccess) e = dalloc(&b->d_qstop, 3 * (size_t)slots);
+    if (e == hipSuccess) e = dalloc(&b->d_split, (size_t)SGM_SPLIT_WORDS * slots);
+    if (e == hipSuccess) e = dalloc(&b->d_rec, (size_t)SGM_SPLIT_FIELDS * SGM_SPLIT_ITEMS * slots);
     if (e == hipSuccess) e = dalloc(&b->d_lastbits, (size_t)slots * ((V + 31) / 32));
     if (e == hipSuccess) e = dalloc(&b->d_rank_given, (size_t)slots * 2 * CM_ROWS);
     if (e == hipSuccess) e = dalloc(&b->d_rank_out, (size_t)slots * CM_ROWS);
@@ -334,6 +341,7 @@ int sgm_batch_enumerate_launch(sgm_batch* b, const uint32_t* const* orders, cons
     std::vector<sgm_plan_dev> plans(nq);
     std::vector<sgm_equery>& eq = b->pend_eq;
     eq.assign(slots, sgm_equery());
+    bool split = false;  // some slot is split (the launch then runs the split variant)
     uint32_t max_ops = 1;
     std::vector<uint32_t> ord(SGM_MAXQ);
     for (uint32_t i = 0; i < slots; i++) {
@@ -344,6 +352,9 @@ int sgm_batch_enumerate_launch(sgm_batch* b, const uint32_t* const* orders, cons
         e.lastbits = b->d_lastbits + (size_t)i * ((V + 31) / 32);
         e.stop = b->d_qstop + i;
         e.next = b->d_qstop + slots + i;
+        e.snext = b->d_qstop + 2 * slots + i;
+        e.split = nullptr;
+        e.rec = nullptr;
         e.st = b->d_st + i;
         e.cap = (unsigned long long)o.cap;
         if (i >= nq) continue;
@@ -357,8 +368,15 @@ int sgm_batch_enumerate_launch(sgm_batch* b, const uint32_t* const* orders, cons
         sgm::plan_to_device(pl, plans[i]);
         max_ops = std::max<uint32_t>(max_ops, plans[i].n_ops);
         e.root_u = ord[0];
-        if (b->status[i] == SGM_OK)  // the reference's driver reports 0 without enumerating
+        if (b->status[i] == SGM_OK) {  // the reference's driver reports 0 without enumerating
             e.n_items = g->ge_cnt[std::min<uint32_t>(q.off[ord[0] + 1] - q.off[ord[0]], 32)];
+            const uint32_t roots = b->counts[(size_t)i * SGM_MAXQ + ord[0]];
+            if (!(o.flags & SGM_ENUM_F_NO_SPLIT) && ((o.flags & SGM_ENUM_F_SPLIT_ALL) || roots <= SGM_SPLIT_MAX_ROOTS)) {
+                e.split = b->d_split + (size_t)SGM_SPLIT_WORDS * i;
+                e.rec = b->d_rec + (size_t)SGM_SPLIT_FIELDS * SGM_SPLIT_ITEMS * i;
+                split = true;
+            }
+        }
     }
     uint32_t live = 0;
     for (uint32_t i = 0; i < nq; i++) live += eq[i].n_items ? 1u : 0u;
@@ -376,6 +394,7 @@ int sgm_batch_enumerate_launch(sgm_batch* b, const uint32_t* const* orders, cons
     b->pend_words = words;
     b->pend_use_fs = use_fs;
     b->pend_serial = serial;
+    b->pend_split = split;
     b->pend_hub = hub;
     if (!live) {  // nothing to enumerate: the wait reports zero counts
         std::memset(b->h_got, 0, sizeof(sgm_stats_dev) * slots);
@@ -399,19 +418,23 @@ int sgm_batch_enumerate_launch(sgm_batch* b, const uint32_t* const* orders, cons
     hipGraphExec_t exec = nullptr;
     for (auto& x : b->ge)
         if (x.max_ops == bucket && x.use_fs == (uint32_t)use_fs && x.blocks == blocks && x.words == words && x.hub == hub &&
-            x.serial == (uint32_t)serial)
+            x.serial == (uint32_t)serial && x.split == (uint32_t)split)
             exec = x.exec;
     const size_t st_words = sizeof(sgm_stats_dev) / sizeof(uint32_t) * slots;
     if (!exec) {
         rc = capture(c, b->stream, [&] {
             sgm::launch_fill_u32(b->stream, reinterpret_cast<uint32_t*>(b->d_st), st_words, 0);
-            sgm::launch_fill_u32(b->stream, b->d_qstop, 2 * (size_t)slots, 0);
+            sgm::launch_fill_u32(b->stream, b->d_qstop, 3 * (size_t)slots, 0);
             sgm::launch_last_bits(b->stream, b->d_eq, slots, V);
+            if (split) sgm::launch_split_roots(b->stream, g->d_off, g->d_bydeg, b->d_eq, slots);
             sgm::launch_enum_cache(b->stream, blocks, g->d_off, g->d_adj, g->d_bydeg, b->d_eq, slots, bucket, b->d_stack,
-                                   (uint32_t)words, c->d_stop, use_fs, hub, g->d_hub_bits, g->hub_words, serial);
+                                   (uint32_t)words, c->d_stop, use_fs, hub, g->d_hub_bits, g->hub_words, serial,
+                                   split);
+            if (split) sgm::launch_split_reduce(b->stream, b->d_eq, slots);
         }, &exec);
         if (rc) return rc;
-        b->ge.push_back(EnumGraph{bucket, (uint32_t)use_fs, blocks, (uint32_t)words, (uint32_t)serial, hub, exec});
+        b->ge.push_back(EnumGraph{bucket, (uint32_t)use_fs, blocks, (uint32_t)words, (uint32_t)serial, (uint32_t)split,
+                                  hub, exec});
     }
     // the context's stop word is shared by every launch in flight: clear it only when none is
     // (a stop request then ends all of them)
@@ -473,17 +496,21 @@ int sgm_batch_enumerate_wait(sgm_batch* b, sgm_stats* stats)
             const uint32_t k = (uint32_t)sub.size();
             e.stop = b->d_qstop + k;
             e.next = b->d_qstop + slots + k;
+            e.snext = b->d_qstop + 2 * slots + k;
             e.st = b->d_st + k;
             sub.push_back(e);
         }
         HIPCHK(c, hipMemcpyAsync(b->d_eq, sub.data(), sizeof(sgm_equery) * sub.size(), hipMemcpyHostToDevice,
                                  b->stream));
         sgm::launch_fill_u32(b->stream, reinterpret_cast<uint32_t*>(b->d_st), st_words, 0);
-        sgm::launch_fill_u32(b->stream, b->d_qstop, 2 * (size_t)slots, 0);
+        sgm::launch_fill_u32(b->stream, b->d_qstop, 3 * (size_t)slots, 0);
         HIPCHK(c, hipEventRecord(b->ev[2], b->stream));
-        sgm::launch_enum_cache(b->stream, blocks, g->d_off, g->d_adj, g->d_bydeg, b->d_eq, (uint32_t)sub.size(),
-                               bucket, c->d_stack, (uint32_t)words, c->d_stop, b->pend_use_fs, b->pend_hub,
-                               g->d_hub_bits, g->hub_words, b->pend_serial);
+        const uint32_t ns = (uint32_t)sub.size();
+        sgm::launch_split_roots(b->stream, g->d_off, g->d_bydeg, b->d_eq, ns);  // no-op without split blocks
+        sgm::launch_enum_cache(b->stream, blocks, g->d_off, g->d_adj, g->d_bydeg, b->d_eq, ns, bucket, c->d_stack,
+                               (uint32_t)words, c->d_stop, b->pend_use_fs, b->pend_hub, g->d_hub_bits, g->hub_words,
+                               b->pend_serial, b->pend_split);
+        sgm::launch_split_reduce(b->stream, b->d_eq, ns);
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipEventRecord(b->ev[3], b->stream));
         HIPCHK(c, hipMemcpyAsync(b->h_got, b->d_st, sizeof(sgm_stats_dev) * sub.size(), hipMemcpyDeviceToHost,

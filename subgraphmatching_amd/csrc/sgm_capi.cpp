@@ -67,6 +67,8 @@ void sgm_ctx_destroy(sgm_ctx* c)
     dfree(c->d_fcnt);
     if (c->h_fcnt) (void)hipHostFree(c->h_fcnt);
     dfree(c->d_lastbits);
+    dfree(c->d_split);
+    dfree(c->d_rec);
     dfree(c->d_stop);
     dfree(c->d_work);
     dfree(c->d_stats);
@@ -1007,8 +1009,8 @@ int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uin
     if ((rc = grow(c, &c->d_eq, cap2, nq))) return rc;
     uint32_t cap3 = c->batch_cap2;
     if ((rc = grow(c, &c->d_qstats, cap3, nq))) return rc;
-    uint32_t cap4 = 2 * c->batch_cap2;
-    if ((rc = grow(c, &c->d_qstop, cap4, 2 * nq))) return rc;  // stop words + item counters
+    uint32_t cap4 = 3 * c->batch_cap2;
+    if ((rc = grow(c, &c->d_qstop, cap4, 3 * nq))) return rc;  // stop words, item and chunk counters
     c->batch_cap2 = std::max(c->batch_cap2, nq);
     HIPCHK(c, hipMemcpyAsync(c->d_plans, plans.data(), sizeof(sgm_plan_dev) * nq, hipMemcpyHostToDevice, c->stream));
     const uint32_t blocks = default_blocks(c, o, (uint32_t)live.size());
@@ -1028,6 +1030,25 @@ int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uin
         HIPCHK(c, dalloc(&c->d_lastbits, lb_words * live.size()));
         c->lastbits_words = lb_words * live.size();
     }
+    // depth-1 splitting for queries with few candidate roots (or all, or none)
+    std::vector<char> want_split(nq, 0);
+    bool split = false;
+    for (uint32_t i : live) {
+        const uint32_t roots = qs[i]->counts[ord[i][0]];
+        want_split[i] = !(o.flags & SGM_ENUM_F_NO_SPLIT) &&
+                        ((o.flags & SGM_ENUM_F_SPLIT_ALL) || roots <= SGM_SPLIT_MAX_ROOTS);
+        split |= want_split[i] != 0;
+    }
+    if (split && c->split_cap < live.size()) {
+        dfree(c->d_split);
+        dfree(c->d_rec);
+        c->d_split = nullptr;
+        c->d_rec = nullptr;
+        c->split_cap = 0;
+        HIPCHK(c, dalloc(&c->d_split, (size_t)SGM_SPLIT_WORDS * live.size()));
+        HIPCHK(c, dalloc(&c->d_rec, (size_t)SGM_SPLIT_FIELDS * SGM_SPLIT_ITEMS * live.size()));
+        c->split_cap = (uint32_t)live.size();
+    }
     for (uint32_t attempt = 0; !live.empty(); attempt++) {
         if (attempt > 3) return set_err(c, SGM_EDEVICE, "set-op stack overflow after %u retries", attempt - 1);
         if (words > 0x7FFFFFFFull) return set_err(c, SGM_EDEVICE, "set-op stack would exceed 2^31 words");
@@ -1044,6 +1065,9 @@ int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uin
             eq[k].lastbits = c->d_lastbits + lb_words * k;
             eq[k].stop = c->d_qstop + k;       // stop words: [0, m)
             eq[k].next = c->d_qstop + m + k;   // item counters: [m, 2m)
+            eq[k].snext = c->d_qstop + 2 * m + k;  // split chunk counters: [2m, 3m)
+            eq[k].split = want_split[i] ? c->d_split + (size_t)SGM_SPLIT_WORDS * k : nullptr;
+            eq[k].rec = want_split[i] ? c->d_rec + (size_t)SGM_SPLIT_FIELDS * SGM_SPLIT_ITEMS * k : nullptr;
             eq[k].st = c->d_qstats + k;
             eq[k].cap = (unsigned long long)o.cap;
             eq[k].n_items = g->ge_cnt[std::min<uint32_t>(q->q.off[u0 + 1] - q->q.off[u0], 32)];
@@ -1051,13 +1075,16 @@ int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uin
         }
         HIPCHK(c, hipMemcpyAsync(c->d_eq, eq.data(), sizeof(sgm_equery) * m, hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipMemsetAsync(c->d_qstats, 0, sizeof(sgm_stats_dev) * m, c->stream));
-        HIPCHK(c, hipMemsetAsync(c->d_qstop, 0, sizeof(uint32_t) * 2 * m, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->d_qstop, 0, sizeof(uint32_t) * 3 * m, c->stream));
         HIPCHK(c, hipMemsetAsync(c->d_stop, 0, sizeof(uint32_t), c->stream));
         c->stop_requested = 0;
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
         sgm::launch_last_bits(c->stream, c->d_eq, m, g->V);
+        if (split) sgm::launch_split_roots(c->stream, g->d_off, g->d_bydeg, c->d_eq, m);
         sgm::launch_enum_cache(c->stream, blocks, g->d_off, g->d_adj, g->d_bydeg, c->d_eq, m, max_ops, c->d_stack,
-                               (uint32_t)words, c->d_stop, use_fs, hub_slot, g->d_hub_bits, g->hub_words, serial);
+                               (uint32_t)words, c->d_stop, use_fs, hub_slot, g->d_hub_bits, g->hub_words, serial,
+                               split);
+        if (split) sgm::launch_split_reduce(c->stream, c->d_eq, m);
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
         std::vector<sgm_stats_dev> got(m);

@@ -36,6 +36,11 @@ struct WaveDFS {
     uint32_t chunk_base[SGM_MAXQ];
     uint32_t flags[SGM_MAXQ];
     uint32_t stack_top[SGM_MAXQ + 1];
+    // the current item when it is a split chunk (kept here, not in registers, across the
+    // search): depth-1 start sp_lo (SGM_INVALID for a whole root), chunk index, root index
+    // and its cutmin word
+    uint32_t sp_lo, sp_it, sp_ri, sp_pad;
+    uint32_t* sp_cut;
 };
 
 #define F_ANY 1u
@@ -274,6 +279,112 @@ __device__ void last_isect(const uint32_t* __restrict__ a, uint32_t na, const ui
 // query's slot when it moves on.  A batch of one query is the single-query search.
 // The last level tests only bit u_L of the candidate words: a bitmap of C(u_L) (V/8 bytes
 // per query instead of 4·V) keeps those probes in L2.  Wave per 64 vertices, one ballot.
+// Split table of each query (layout at SGM_SPLIT_ROOTS): one wave per query walks the
+// degree-ordered items until SGM_SPLIT_ROOTS candidate roots are found (or SGM_SPLIT_SCAN
+// positions are seen), then cuts root r's depth-1 loop -- N(v_r), deg(v_r) elements -- into
+// min(SGM_SPLIT_CHUNKS, ceil(deg/64)) chunks of a whole number of 64-element batches.
+__global__ void __launch_bounds__(64) k_split_roots(const uint32_t* __restrict__ off,
+                                                    const uint32_t* __restrict__ items,
+                                                    const sgm_equery* __restrict__ qs)
+{
+    const sgm_equery& Q = qs[blockIdx.x];
+    uint32_t* __restrict__ sp = Q.split;
+    if (!sp) return;
+    const uint32_t lane = lane_id();
+    const uint32_t Rn = SGM_SPLIT_ROOTS;
+    uint32_t* __restrict__ rpos = sp + 2;
+    uint32_t* __restrict__ cutmin = rpos + Rn;
+    uint32_t* __restrict__ done = cutmin + Rn;
+    uint32_t* __restrict__ it3 = done + SGM_SPLIT_ITEMS;
+    uint32_t found = 0, root_first = 0;
+    const uint32_t lim = Q.n_items < SGM_SPLIT_SCAN ? Q.n_items : SGM_SPLIT_SCAN;
+    if (Q.plan->n > 2) {  // two-vertex queries have no depth-1 loop to split
+        for (uint32_t base = 0; base < lim && found < Rn; base += 64) {
+            const uint32_t p = base + lane;
+            const bool c = p < lim && ((Q.vmask[items[p]] >> Q.root_u) & 1u);
+            const uint64_t b = __ballot(c);
+            const uint32_t r = found + mbcnt64(b);
+            if (c && r < Rn) rpos[r] = p;
+            const uint32_t cnt = (uint32_t)__popcll(b);
+            if (found + cnt >= Rn) {
+                uint64_t bb = b;  // the (Rn - found)-th set bit is the last split root
+                for (uint32_t k = 0; k + 1 < Rn - found; k++) bb &= bb - 1;
+                root_first = base + (uint32_t)__ffsll((long long)bb);
+                found = Rn;
+            } else {
+                found += cnt;
+            }
+        }
+        if (found < Rn) root_first = lim;  // every candidate below lim is split
+    }
+    // chunks per root, then their (root, lo, hi) triples in root order
+    uint32_t deg = 0, nch = 0, W = 64;
+    if (lane < found) {
+        const uint32_t v = items[rpos[lane]];
+        deg = off[v + 1] - off[v];
+        const uint32_t b64 = (deg + 63) / 64;
+        const uint32_t ch = b64 < SGM_SPLIT_CHUNKS ? b64 : SGM_SPLIT_CHUNKS;
+        W = ch ? ((b64 + ch - 1) / ch) * 64 : 64;
+        nch = deg ? (deg + W - 1) / W : 1u;  // a root without neighbours still counts its node
+        cutmin[lane] = SGM_INVALID;
+    }
+    const uint32_t incl = wave_incl_scan(nch);
+    const uint32_t total = (uint32_t)__shfl((int)incl, 63);
+    for (uint32_t c = 0; c < nch; c++) {
+        const uint32_t s = incl - nch + c;
+        it3[3 * s] = lane;
+        it3[3 * s + 1] = c * W;
+        it3[3 * s + 2] = (c + 1) * W < deg ? (c + 1) * W : deg;
+    }
+    for (uint32_t s = lane; s < total; s += 64) done[s] = 0;
+    if (lane == 0) {
+        sp[0] = total;
+        sp[1] = root_first;
+    }
+}
+
+// Adds the split chunks' records to their queries' statistics: chunk s of root r counts iff
+// it ran to its end and starts at or before r's first cut-off (cutmin[r]).  Grid
+// (SGM_SPLIT_ITEMS / 256, queries).
+__global__ void __launch_bounds__(256) k_split_reduce(const sgm_equery* __restrict__ qs)
+{
+    const sgm_equery& Q = qs[blockIdx.y];
+    const uint32_t* __restrict__ sp = Q.split;
+    if (!sp) return;
+    const uint32_t ns = sp[0];
+    if (blockIdx.x * blockDim.x >= ns) return;
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t* __restrict__ cutmin = sp + 2 + SGM_SPLIT_ROOTS;
+    const uint32_t* __restrict__ done = cutmin + SGM_SPLIT_ROOTS;
+    const uint32_t* __restrict__ it3 = done + SGM_SPLIT_ITEMS;
+    bool inc = false;
+    if (s < ns) inc = done[s] && it3[3 * s + 1] <= cutmin[it3[3 * s]];
+    __shared__ unsigned long long red[4];
+    const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
+    for (uint32_t f = 0; f < SGM_SPLIT_FIELDS; f++) {
+        unsigned long long v = inc ? Q.rec[(size_t)f * SGM_SPLIT_ITEMS + s] : 0ull;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, o);
+            const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), o);
+            v += ((unsigned long long)hi << 32) | lo;
+        }
+        if (lane == 0) red[wave] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const unsigned long long t = red[0] + red[1] + red[2] + red[3];
+            unsigned long long* dst = f < SGM_MAXQ ? &Q.st->nodes[f]
+                                    : f == 32u ? &Q.st->intersections
+                                    : f == 33u ? &Q.st->fs_prunes
+                                    : f == 34u ? &Q.st->elems_read
+                                    : f == 35u ? &Q.st->elems_out
+                                               : &Q.st->elems_scanned;
+            if (t) atomicAdd(dst, t);
+        }
+        __syncthreads();
+    }
+}
+
 __global__ void __launch_bounds__(256) k_last_bits(const sgm_equery* __restrict__ qs, uint32_t V)
 {
     const sgm_equery& Q = qs[blockIdx.y];
@@ -624,6 +735,9 @@ __device__ void last_chunk(const sgm_plan_dev& P, WaveDFS& S, uint32_t d, const 
     S.rem[d] = 0;
 }
 
+// SPLIT: the launch holds queries with split chunks (the variant without them keeps the
+// register budget of the plain search).
+template <bool SPLIT>
 __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__ off,
                                                     const uint32_t* __restrict__ adj,
                                                     const uint32_t* __restrict__ items,
@@ -679,11 +793,24 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
         const uint32_t q = cls + ncls * k;
         const sgm_equery& Q = qs[q];
         uint32_t it = SGM_INVALID;
-        if (!__atomic_load_n(Q.stop, __ATOMIC_RELAXED) && __atomic_load_n(Q.next, __ATOMIC_RELAXED) < Q.n_items) {
-            if (lane == 0) it = atomicAdd(Q.next, 1u);
-            it = uni((uint32_t)__shfl((int)it, 0));
+        bool split_item = false;
+        if (!__atomic_load_n(Q.stop, __ATOMIC_RELAXED)) {
+            // split chunks of the heaviest roots first, then the remaining roots
+            const uint32_t ns = (SPLIT && Q.split) ? uni(Q.split[0]) : 0u;
+            const uint32_t rf = (SPLIT && Q.split) ? uni(Q.split[1]) : 0u;
+            if (ns && __atomic_load_n(Q.snext, __ATOMIC_RELAXED) < ns) {
+                if (lane == 0) it = atomicAdd(Q.snext, 1u);
+                it = uni((uint32_t)__shfl((int)it, 0));
+                split_item = it < ns;
+                if (!split_item) it = SGM_INVALID;
+            }
+            if (!split_item && __atomic_load_n(Q.next, __ATOMIC_RELAXED) < Q.n_items - rf) {
+                if (lane == 0) it = atomicAdd(Q.next, 1u);
+                it = uni((uint32_t)__shfl((int)it, 0));
+                it = it < Q.n_items - rf ? it + rf : SGM_INVALID;
+            }
         }
-        if (it >= Q.n_items) {
+        if (it == SGM_INVALID) {
             // exhausted or stopped (cap reached / overflow): try the next query of the class,
             // then the next class; a full pass over every class without work ends the wave
             const uint32_t cs = csize(cls);
@@ -715,13 +842,32 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
             stop = Q.stop;
         }
         const unsigned long long cap = Q.cap;
-        const uint32_t v0 = items[it];
+        // a split chunk: root sri, depth-1 positions [lo1, hi1)
+        uint32_t v0, lo1 = 0, hi1 = 0;
+        if (SPLIT && split_item) {
+            const uint32_t* t3 = Q.split + 2 + 2 * SGM_SPLIT_ROOTS + SGM_SPLIT_ITEMS + 3 * it;
+            const uint32_t sri = uni(t3[0]);
+            lo1 = uni(t3[1]);
+            hi1 = uni(t3[2]);
+            uint32_t* cutw = Q.split + 2 + SGM_SPLIT_ROOTS + sri;
+            v0 = items[uni(Q.split[2 + sri])];
+            // an earlier chunk of this root was cut off: the sequential search never gets here
+            if (uni(__atomic_load_n(cutw, __ATOMIC_RELAXED)) < lo1) continue;
+            flush_stats(Q, S, acc, n, false);  // the chunk's counters go to its record
+            S.sp_lo = lo1;
+            S.sp_it = it;
+            S.sp_ri = sri;
+            S.sp_cut = cutw;
+        } else {
+            v0 = items[it];
+            if (SPLIT) S.sp_lo = SGM_INVALID;
+        }
         if (!((vmask[v0] >> root_u) & 1u)) continue;  // not (or no longer) a candidate of u0
         acc.items_done++;
         const uint64_t t_item = wall_clock64();
         S.emb[0] = v0;
         embv = lane == 0 ? v0 : embv;
-        S.nodes[0] += 1;
+        if (lo1 == 0) S.nodes[0] += 1;  // a split root's node is counted by its first chunk
         S.stack_top[0] = 0;
         bool aborted = false;
 
@@ -758,17 +904,25 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
             }
             if (ok) {
                 S.stack_top[1] = top;
-                // enter depth 1
-                S.pos[1] = 0; S.rem[1] = 0; S.flags[1] = 0;
-                S.tot[1] = op_size[P.iter_set[1]];
+                // enter depth 1 (a split chunk walks only its part of the loop)
+                S.pos[1] = lo1; S.rem[1] = 0; S.flags[1] = 0;
+                S.tot[1] = (SPLIT && split_item) ? hi1 : op_size[P.iter_set[1]];
                 S.lst[1] = op_off[P.iter_set[1]];
                 S.fail[0] = 0;
                 uint32_t d = 1;
                 while (true) {
-                    if ((++poll & 3u) == 0 &&
-                        (__atomic_load_n(stop, __ATOMIC_RELAXED) | __atomic_load_n(host_stop, __ATOMIC_RELAXED))) {
-                        aborted = true;
-                        break;
+                    if ((++poll & 3u) == 0) {
+                        bool end = __atomic_load_n(stop, __ATOMIC_RELAXED) | __atomic_load_n(host_stop, __ATOMIC_RELAXED);
+                        const uint32_t slo = SPLIT ? uni(S.sp_lo) : SGM_INVALID;
+                        if (slo != SGM_INVALID) {  // an earlier chunk of this split root was cut off
+                            const uint32_t* cw = reinterpret_cast<const uint32_t*>(
+                                uni64(reinterpret_cast<uint64_t>(S.sp_cut)));
+                            end |= uni(__atomic_load_n(cw, __ATOMIC_RELAXED)) < slo;
+                        }
+                        if (end) {
+                            aborted = true;
+                            break;
+                        }
                     }
                     const uint64_t rem_d = uni64(S.rem[d]);
                     if (rem_d == 0 && ((uni(S.flags[d]) & F_PRUNED) || uni(S.pos[d]) >= uni(S.tot[d]))) {
@@ -917,6 +1071,27 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
                     S.fail[d] = 0;
                     d = nd;
                 }
+            }
+        }
+        if (SPLIT && uni(S.sp_lo) != SGM_INVALID) {
+            // a cut-off at depth 1 ends the loop with F_PRUNED (set only after the loop ran)
+            const bool cut1 = use_fs && !aborted && (uni(S.flags[1]) & F_PRUNED);
+            const uint32_t sit = uni(S.sp_it);
+            // the chunk's record: lane f < 32 its depth-f nodes, lanes 32..36 the other counters
+            unsigned long long* rec = Q.rec + sit;
+            unsigned long long x = 0;
+            if (lane < SGM_MAXQ) x = lane < n ? S.nodes[lane] : 0ull;
+            else if (lane == 32) x = acc.inter;
+            else if (lane == 33) x = acc.fsp;
+            else if (lane == 34) x = acc.elems;
+            else if (lane == 35) x = acc.outs;
+            else if (lane == 36) x = acc.scanned;
+            if (lane < SGM_SPLIT_FIELDS) rec[(size_t)lane * SGM_SPLIT_ITEMS] = x;
+            acc.inter = acc.fsp = acc.elems = acc.outs = acc.scanned = 0;
+            clear_nodes(S, nullptr, 0);
+            if (lane == 0 && !aborted) {
+                Q.split[2 + 2 * SGM_SPLIT_ROOTS + sit] = 1u;  // done
+                if (cut1) atomicMin(S.sp_cut, S.sp_lo);
             }
         }
         acc.busy += wall_clock64() - t_item;
@@ -1161,13 +1336,29 @@ void launch_last_bits(hipStream_t s, const sgm_equery* qs, uint32_t nq, uint32_t
     hipLaunchKernelGGL(k_last_bits, grid, dim3(256), 0, s, qs, V);
 }
 
+void launch_split_roots(hipStream_t s, const uint32_t* off, const uint32_t* items, const sgm_equery* qs, uint32_t nq)
+{
+    if (nq) hipLaunchKernelGGL(k_split_roots, dim3(nq), dim3(64), 0, s, off, items, qs);
+}
+
+void launch_split_reduce(hipStream_t s, const sgm_equery* qs, uint32_t nq)
+{
+    if (nq) hipLaunchKernelGGL(k_split_reduce, dim3(SGM_SPLIT_ITEMS / 256, nq), dim3(256), 0, s, qs);
+}
+
 void launch_enum_cache(hipStream_t s, uint32_t blocks, const uint32_t* off, const uint32_t* adj, const uint32_t* items,
                        const sgm_equery* qs, uint32_t nq, uint32_t max_ops, uint32_t* stack, uint32_t stack_words,
                        const uint32_t* host_stop, int use_fs, const uint32_t* hub_slot, const uint32_t* hub_bits,
-                       uint32_t hub_words, int serial_last)
+                       uint32_t hub_words, int serial_last, bool split)
 {
-    hipLaunchKernelGGL(k_enum_cache, dim3(blocks), dim3(256), 4 * enum_wave_bytes(max_ops), s, off, adj, items, qs, nq,
-                       max_ops, stack, stack_words, host_stop, use_fs, hub_slot, hub_bits, hub_words, serial_last);
+    if (split)
+        hipLaunchKernelGGL(k_enum_cache<true>, dim3(blocks), dim3(256), 4 * enum_wave_bytes(max_ops), s, off, adj, items,
+                           qs, nq, max_ops, stack, stack_words, host_stop, use_fs, hub_slot, hub_bits, hub_words,
+                           serial_last);
+    else
+        hipLaunchKernelGGL(k_enum_cache<false>, dim3(blocks), dim3(256), 4 * enum_wave_bytes(max_ops), s, off, adj,
+                           items, qs, nq, max_ops, stack, stack_words, host_stop, use_fs, hub_slot, hub_bits, hub_words,
+                           serial_last);
 }
 
 void launch_enum_aux(hipStream_t s, uint32_t blocks, const sgm_auxplan_dev* plan, uint32_t n_items, uint32_t* stack,

@@ -60,8 +60,33 @@ struct sgm_stats_dev {
     unsigned long long busy_ticks;      // wall-clock ticks its waves spent on the query's items
 };
 
+// Depth-1 splitting of the heaviest roots.  The first SGM_SPLIT_ROOTS candidate roots of a
+// query in degree order (among the first SGM_SPLIT_SCAN positions) are cut into at most
+// SGM_SPLIT_CHUNKS chunks of their depth-1 loop (whole 64-element batches), handed out
+// before the remaining roots.  Depth-1 siblings interact only through the failing-set
+// cut-off (a cut at depth 1 skips the rest of the loop; depth 0 never prunes,
+// inc/prune.h:119-137), so each chunk keeps its counters in a record and k_split_reduce
+// adds the chunks up to and including a root's first cutting chunk -- the counters of the
+// sequential search.  Per-query split block, u32 words:
+//   [0] n_split  [1] root_first (the first position not split)
+//   [2, 2+R) rpos (positions of the split roots)  [2+R, 2+2R) cutmin (smallest start of a
+//   cutting chunk, ~0 = none)  [2+2R, 2+2R+IC) done flags  then IC triples (root, lo, hi)
+// Records, u64, field-major (field f of item s at rec[f * IC + s]): nodes[32],
+// intersections, fs_prunes, elems_read, elems_out, elems_scanned.
+#define SGM_SPLIT_ROOTS 64u
+#define SGM_SPLIT_CHUNKS 16u
+#define SGM_SPLIT_ITEMS (SGM_SPLIT_ROOTS * SGM_SPLIT_CHUNKS)
+#define SGM_SPLIT_SCAN 4096u
+#define SGM_SPLIT_WORDS (2u + 2u * SGM_SPLIT_ROOTS + 4u * SGM_SPLIT_ITEMS)
+#define SGM_SPLIT_FIELDS 37u
+// queries with at most this many candidate roots are split by default (a launch's 4096-odd
+// waves would otherwise sit idle behind their heaviest roots); more roots keep every wave
+// busy without it, and the split variant of the search costs a few per cent
+#define SGM_SPLIT_MAX_ROOTS 4096u
+
 // One query of a batched cache + filter + failing-set enumeration.  Its depth-0 items are
-// the prefix of length n_items of the degree-sorted vertex list, handed out by `next`.
+// the prefix of length n_items of the degree-sorted vertex list: the split chunks (when
+// `split` is set) first, handed out by `snext`, then the roots from root_first on, by `next`.
 struct sgm_equery {
     const sgm_plan_dev* plan;
     const uint32_t* vmask;
@@ -70,6 +95,9 @@ struct sgm_equery {
     uint32_t* stop;                 // raised when the cap is reached (or the run overflowed)
     sgm_stats_dev* st;
     uint32_t* next;                 // next depth-0 item of this query
+    uint32_t* split;                // split block (SGM_SPLIT_WORDS words) or null: no splitting
+    uint32_t* snext;                // next split chunk
+    unsigned long long* rec;        // SGM_SPLIT_FIELDS * SGM_SPLIT_ITEMS chunk records
     unsigned long long cap;
     uint32_t n_items;
     uint32_t root_u;

@@ -60,10 +60,14 @@ void launch_enum_aux(hipStream_t s, uint32_t blocks, const sgm_auxplan_dev* plan
                      uint32_t stack_words, uint32_t* work, sgm_stats_dev* st, uint32_t* stop, unsigned long long cap);
 // bitmap of C(u_L) per query (Q.lastbits) from its candidate words; before launch_enum_cache
 void launch_last_bits(hipStream_t s, const sgm_equery* qs, uint32_t nq, uint32_t V);
+// depth-1 split tables (queries with Q.split set), before launch_enum_cache; and the chunk
+// records added to the query statistics, after it
+void launch_split_roots(hipStream_t s, const uint32_t* off, const uint32_t* items, const sgm_equery* qs, uint32_t nq);
+void launch_split_reduce(hipStream_t s, const sgm_equery* qs, uint32_t nq);
 void launch_enum_cache(hipStream_t s, uint32_t blocks, const uint32_t* off, const uint32_t* adj, const uint32_t* items,
                        const sgm_equery* qs, uint32_t nq, uint32_t max_ops, uint32_t* stack, uint32_t stack_words,
                        const uint32_t* host_stop, int use_fs, const uint32_t* hub_slot, const uint32_t* hub_bits,
-                       uint32_t hub_words, int serial_last);
+                       uint32_t hub_words, int serial_last, bool split);
 void launch_hub_bits(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* hubs, uint32_t n_hubs,
                      uint32_t* bits, uint32_t words);
 }  // namespace sgm

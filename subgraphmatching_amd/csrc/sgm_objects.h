@@ -47,9 +47,12 @@ struct sgm_ctx {
     uint32_t* d_fcnt = nullptr;
     uint32_t* h_fcnt = nullptr;
     uint32_t fcap = 0;
-    // batched enumeration: bitmap of C(u_L) per query
+    // batched enumeration: bitmap of C(u_L) per query; depth-1 split blocks and chunk records
     uint32_t* d_lastbits = nullptr;
     size_t lastbits_words = 0;
+    uint32_t* d_split = nullptr;
+    unsigned long long* d_rec = nullptr;
+    uint32_t split_cap = 0;  // queries the split buffers hold
     // objects that hold this context (destroyed with it if the caller has not done so)
     std::vector<sgm_batch*> batches;
     std::vector<sgm_costmodel*> models;

@@ -25,16 +25,20 @@ def poisson_arrivals(count: int, rate_per_s: float, seed: int) -> np.ndarray:
     return np.cumsum(rng.exponential(1.0 / rate_per_s, size=count))
 
 
+# per slot: split block (2 + 2*64 + 4*1024 u32) and chunk records (37 fields x 1024 u64)
+SPLIT_SLOT_BYTES = 4 * (2 + 2 * 64 + 4 * 1024) + 8 * 37 * 1024
+
+
 def admit_slots(num_vertices: int, max_degree: int, mem_budget_bytes: int, waves: int, max_slots: int = 64):
     """Largest slot count whose device state fits the budget together with the DFS stacks.
 
     Per slot: membership words by id plus the GQL scratch (rank-space membership, touch /
     dirty marks and NLF-test failures, 4 words per data vertex, and three summary words per 4
     vertices) and the last query vertex's candidate bitmap (one bit per data vertex), i.e.
-    under 24 bytes per data vertex, plus descriptors; per wave: the set-op stack
-    the enumeration reserves (min(8 * max_degree + 1024, 2^20) words).  Returns (slots,
-    stack_bytes, per_slot_bytes)."""
-    per_slot = 24 * num_vertices + (1 << 14)
+    under 24 bytes per data vertex, plus descriptors and the depth-1 split block and chunk
+    records (SPLIT_SLOT_BYTES); per wave: the set-op stack the enumeration reserves
+    (min(8 * max_degree + 1024, 2^20) words).  Returns (slots, stack_bytes, per_slot_bytes)."""
+    per_slot = 24 * num_vertices + (1 << 14) + SPLIT_SLOT_BYTES
     stack = waves * 4 * min(8 * max_degree + 1024, 1 << 20)
     free = mem_budget_bytes - stack
     if free < per_slot:

@@ -116,7 +116,10 @@ def _compare(g, q, order, cap=None, check_work=True):
     for fs, method, flags in ((True, L.SGM_ENUM_CACHE_FILTER_FS, 0), (False, L.SGM_ENUM_CACHE_FILTER, 0),
                               (True, L.SGM_ENUM_CACHE_FILTER_FS, L.SGM_ENUM_F_NO_BITMAPS),
                               (True, L.SGM_ENUM_CACHE_FILTER_FS, L.SGM_ENUM_F_SERIAL_LAST),
-                              (False, L.SGM_ENUM_CACHE_FILTER, L.SGM_ENUM_F_SERIAL_LAST)):
+                              (False, L.SGM_ENUM_CACHE_FILTER, L.SGM_ENUM_F_SERIAL_LAST),
+                              (True, L.SGM_ENUM_CACHE_FILTER_FS, L.SGM_ENUM_F_NO_SPLIT),
+                              (True, L.SGM_ENUM_CACHE_FILTER_FS, L.SGM_ENUM_F_SPLIT_ALL),
+                              (False, L.SGM_ENUM_CACHE_FILTER, L.SGM_ENUM_F_SPLIT_ALL)):
         res = sgm.enumerate_query(can, order, method=method, cap=cap, flags=flags)
         ref = O.enum_cache(g, q, c_r, order, cap=cap, fs=fs)
         assert res.ans == ref["count"], (q.name, fs, flags, res.ans, ref["count"])
@@ -125,7 +128,8 @@ def _compare(g, q, order, cap=None, check_work=True):
             assert res.nodes == ref["nodes"], (q.name, fs, flags, res.nodes, ref["nodes"])
             assert res.fs_prunes == ref["fs_prunes"], (q.name, fs, flags)
             # the chunked and the one-at-a-time last level read the same elements
-            key = (fs, flags & ~L.SGM_ENUM_F_SERIAL_LAST & ~L.SGM_ENUM_F_NO_BITMAPS)
+            key = (fs, flags & ~L.SGM_ENUM_F_SERIAL_LAST & ~L.SGM_ENUM_F_NO_BITMAPS & ~L.SGM_ENUM_F_NO_SPLIT
+                   & ~L.SGM_ENUM_F_SPLIT_ALL)
             got = (res.elems_read, res.elems_out, res.elems_scanned)
             assert elems.setdefault(key, got) == got, (q.name, fs, flags, elems[key], got)
     res_a = sgm.enumerate_query(can, order, method=L.SGM_ENUM_AUX_TABLES, cap=cap)

@@ -3,6 +3,7 @@ whose counts equal the batched runner's own counts."""
 import numpy as np
 import pytest
 
+from subgraphmatching_amd import serving
 from subgraphmatching_amd.serving import admit_slots, poisson_arrivals, serve
 
 
@@ -57,7 +58,7 @@ def test_poisson_arrivals_rate_and_seed():
 
 def test_admission_by_memory():
     slots, stack, per = admit_slots(1 << 22, 169000, mem_budget_bytes=32 << 30, waves=4096)
-    assert stack == 4096 * 4 * (1 << 20) and per == 24 * (1 << 22) + (1 << 14)
+    assert stack == 4096 * 4 * (1 << 20) and per == 24 * (1 << 22) + (1 << 14) + serving.SPLIT_SLOT_BYTES
     assert slots == min(64, ((32 << 30) - stack) // per)
     with pytest.raises(ValueError):
         admit_slots(1 << 22, 169000, mem_budget_bytes=stack, waves=4096)
