@@ -1032,22 +1032,23 @@ int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uin
     }
     // depth-1 splitting for queries with few candidate roots (or all, or none)
     std::vector<char> want_split(nq, 0);
-    bool split = false;
+    uint32_t n_split = 0;
     for (uint32_t i : live) {
         const uint32_t roots = qs[i]->counts[ord[i][0]];
         want_split[i] = !(o.flags & SGM_ENUM_F_NO_SPLIT) &&
                         ((o.flags & SGM_ENUM_F_SPLIT_ALL) || roots <= SGM_SPLIT_MAX_ROOTS);
-        split |= want_split[i] != 0;
+        n_split += want_split[i] ? 1u : 0u;
     }
-    if (split && c->split_cap < live.size()) {
+    const bool split = n_split > 0;
+    if (c->split_cap < n_split) {  // split blocks and records for the split queries only
         dfree(c->d_split);
         dfree(c->d_rec);
         c->d_split = nullptr;
         c->d_rec = nullptr;
         c->split_cap = 0;
-        HIPCHK(c, dalloc(&c->d_split, (size_t)SGM_SPLIT_WORDS * live.size()));
-        HIPCHK(c, dalloc(&c->d_rec, (size_t)SGM_SPLIT_FIELDS * SGM_SPLIT_ITEMS * live.size()));
-        c->split_cap = (uint32_t)live.size();
+        HIPCHK(c, dalloc(&c->d_split, (size_t)SGM_SPLIT_WORDS * n_split));
+        HIPCHK(c, dalloc(&c->d_rec, (size_t)SGM_SPLIT_FIELDS * SGM_SPLIT_ITEMS * n_split));
+        c->split_cap = n_split;
     }
     for (uint32_t attempt = 0; !live.empty(); attempt++) {
         if (attempt > 3) return set_err(c, SGM_EDEVICE, "set-op stack overflow after %u retries", attempt - 1);
@@ -1056,6 +1057,7 @@ int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uin
         if (rc) return rc;
         eq.assign(live.size(), sgm_equery());
         const uint32_t m = (uint32_t)live.size();
+        uint32_t js = 0;  // next split block
         for (size_t k = 0; k < live.size(); k++) {
             const uint32_t i = live[k];
             sgm_query* q = qs[i];
@@ -1066,8 +1068,11 @@ int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uin
             eq[k].stop = c->d_qstop + k;       // stop words: [0, m)
             eq[k].next = c->d_qstop + m + k;   // item counters: [m, 2m)
             eq[k].snext = c->d_qstop + 2 * m + k;  // split chunk counters: [2m, 3m)
-            eq[k].split = want_split[i] ? c->d_split + (size_t)SGM_SPLIT_WORDS * k : nullptr;
-            eq[k].rec = want_split[i] ? c->d_rec + (size_t)SGM_SPLIT_FIELDS * SGM_SPLIT_ITEMS * k : nullptr;
+            if (want_split[i]) {
+                eq[k].split = c->d_split + (size_t)SGM_SPLIT_WORDS * js;
+                eq[k].rec = c->d_rec + (size_t)SGM_SPLIT_FIELDS * SGM_SPLIT_ITEMS * js;
+                js++;
+            }
             eq[k].st = c->d_qstats + k;
             eq[k].cap = (unsigned long long)o.cap;
             eq[k].n_items = g->ge_cnt[std::min<uint32_t>(q->q.off[u0 + 1] - q->q.off[u0], 32)];

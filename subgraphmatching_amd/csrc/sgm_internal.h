@@ -74,7 +74,7 @@ struct sgm_stats_dev {
 // Records, u64, field-major (field f of item s at rec[f * IC + s]): nodes[32],
 // intersections, fs_prunes, elems_read, elems_out, elems_scanned.
 #define SGM_SPLIT_ROOTS 64u
-#define SGM_SPLIT_CHUNKS 16u
+#define SGM_SPLIT_CHUNKS 32u  // 16 / 32 / 64: labelled Chung-Lu 10^5 launch 217 / 182 / 172 ms
 #define SGM_SPLIT_ITEMS (SGM_SPLIT_ROOTS * SGM_SPLIT_CHUNKS)
 #define SGM_SPLIT_SCAN 4096u
 #define SGM_SPLIT_WORDS (2u + 2u * SGM_SPLIT_ROOTS + 4u * SGM_SPLIT_ITEMS)

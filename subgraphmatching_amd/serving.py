@@ -25,8 +25,8 @@ def poisson_arrivals(count: int, rate_per_s: float, seed: int) -> np.ndarray:
     return np.cumsum(rng.exponential(1.0 / rate_per_s, size=count))
 
 
-# per slot: split block (2 + 2*64 + 4*1024 u32) and chunk records (37 fields x 1024 u64)
-SPLIT_SLOT_BYTES = 4 * (2 + 2 * 64 + 4 * 1024) + 8 * 37 * 1024
+# per slot: split block (2 + 2*64 + 4*2048 u32) and chunk records (37 fields x 2048 u64)
+SPLIT_SLOT_BYTES = 4 * (2 + 2 * 64 + 4 * 2048) + 8 * 37 * 2048
 
 
 def admit_slots(num_vertices: int, max_degree: int, mem_budget_bytes: int, waves: int, max_slots: int = 64):
