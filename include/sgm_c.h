@@ -124,7 +124,8 @@ void sgm_query_destroy(sgm_query* q);
  * filters (NLF, GQL) and the plan honour labels when the graph or the query has any other
  * than 0; enumeration follows from the candidate sets, as in the reference's cache path
  * (inc/explore.h:200-235 intersects whole neighbour lists).  Set labels before filtering:
- * a query's labels reset its filter state.  sgm_batch_filter refuses labelled graphs. */
+ * a query's labels reset its filter state; batches take query labels through
+ * sgm_batch_filter_labeled. */
 int sgm_graph_set_labels(sgm_graph* g, const uint32_t* labels, uint32_t num_vertices);
 int sgm_query_set_labels(sgm_query* q, const uint32_t* labels, uint32_t n);
 
@@ -191,6 +192,11 @@ void sgm_batch_destroy(sgm_batch* b);
  * degrees of C(u).  Candidate lists and tables are not built on this path. */
 int sgm_batch_filter(sgm_batch* b, uint32_t nq, const uint32_t* const* qoffs, const uint32_t* const* qadjs,
                      const uint32_t* ns, int kind, int32_t* status, uint32_t* cand_counts, uint64_t* cand_degsum);
+/* The same with query vertex labels (qlabs NULL, or qlabs[i] NULL: all 0); a labelled
+ * graph (sgm_graph_set_labels) or labelled queries run the labelled NLF / GQL. */
+int sgm_batch_filter_labeled(sgm_batch* b, uint32_t nq, const uint32_t* const* qoffs, const uint32_t* const* qadjs,
+                             const uint32_t* const* qlabs, const uint32_t* ns, int kind, int32_t* status,
+                             uint32_t* cand_counts, uint64_t* cand_degsum);
 /* Order_GQL (inc/order.h:563-597) of loaded query i from the batch's candidate counts. */
 int sgm_batch_order_gql(const sgm_batch* b, uint32_t i, uint32_t* order);
 /* Enumerate the loaded batch (cache + filter methods) in one persistent launch; stats[i]

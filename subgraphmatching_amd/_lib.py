@@ -49,7 +49,7 @@ EXPORTS = [
     "sgm_costmodel_create", "sgm_costmodel_destroy", "sgm_costmodel_param_count", "sgm_costmodel_params",
     "sgm_costmodel_features", "sgm_costmodel_predict", "sgm_costmodel_grad", "sgm_costmodel_train",
     "sgm_costmodel_timing", "sgm_order_sample", "sgm_batch_rank_orders", "sgm_gql_work",
-    "sgm_graph_set_labels", "sgm_query_set_labels", "sgm_plan_export_labeled",
+    "sgm_graph_set_labels", "sgm_query_set_labels", "sgm_plan_export_labeled", "sgm_batch_filter_labeled",
 ]
 
 
@@ -128,6 +128,7 @@ def lib() -> C.CDLL:
         "sgm_batch_create": (C.c_int, [vp, C.c_uint32, C.POINTER(vp)]),
         "sgm_batch_destroy": (None, [vp]),
         "sgm_batch_filter": (C.c_int, [vp, C.c_uint32, vp, vp, vp, C.c_int, vp, vp, vp]),
+        "sgm_batch_filter_labeled": (C.c_int, [vp, C.c_uint32, vp, vp, vp, vp, C.c_int, vp, vp, vp]),
         "sgm_batch_order_gql": (C.c_int, [vp, C.c_uint32, vp]),
         "sgm_batch_enumerate": (C.c_int, [vp, vp, C.POINTER(EnumOpts), vp]),
         "sgm_batch_enumerate_launch": (C.c_int, [vp, vp, C.POINTER(EnumOpts)]),

@@ -522,11 +522,8 @@ class Batch:
             pass
 
     def filter(self, queries, kind=L.SGM_FILTER_GQL):
-        """Returns (ok[nq] bools, cand_counts[nq, 32], cand_degsum[nq, 32]).  Batches run the
-        unlabeled filters: labelled queries (and labelled graphs, refused by the library) go
-        through CandidateParam / filter_batch instead."""
-        if any(q.labels is not None for q in queries):
-            raise ValueError("batches take unlabeled queries; filter labelled ones with filter_batch")
+        """Returns (ok[nq] bools, cand_counts[nq, 32], cand_degsum[nq, 32]).  Labelled queries
+        (or a labelled data graph) run the labelled NLF / GQL."""
         nq = len(queries)
         self._keep = [(L.u32(q.off), L.u32(q.adj)) for q in queries]
         offs = (C.c_void_p * nq)(*[a.ctypes.data for a, _ in self._keep])
@@ -535,8 +532,14 @@ class Batch:
         status = np.zeros(nq, dtype=np.int32)
         cnt = np.zeros((nq, 32), dtype=np.uint32)
         dsum = np.zeros((nq, 32), dtype=np.uint64)
-        _check(L.lib().sgm_batch_filter(self.handle, nq, offs, adjs, L.ptr(ns), kind, L.ptr(status), L.ptr(cnt),
-                                        L.ptr(dsum)), self.ctx)
+        if any(q.labels is not None for q in queries):
+            self._keep_lab = [None if q.labels is None else L.u32(q.labels) for q in queries]
+            labs = (C.c_void_p * nq)(*[None if a is None else a.ctypes.data for a in self._keep_lab])
+            _check(L.lib().sgm_batch_filter_labeled(self.handle, nq, offs, adjs, labs, L.ptr(ns), kind, L.ptr(status),
+                                                    L.ptr(cnt), L.ptr(dsum)), self.ctx)
+        else:
+            _check(L.lib().sgm_batch_filter(self.handle, nq, offs, adjs, L.ptr(ns), kind, L.ptr(status), L.ptr(cnt),
+                                            L.ptr(dsum)), self.ctx)
         self.queries = list(queries)
         return [int(x) == L.SGM_OK for x in status], cnt, dsum
 

@@ -64,8 +64,10 @@ struct sgm_batch {
     std::vector<unsigned long long> degsum;
     std::vector<int32_t> status;
     double filter_ms = 0, enum_ms = 0;
-    hipGraphExec_t gf_nlf = nullptr;
-    std::vector<std::pair<uint32_t, hipGraphExec_t>> gf_gql;  // by number of GQL phases
+    hipGraphExec_t gf_nlf[2] = {nullptr, nullptr};            // unlabeled / labelled batches
+    std::vector<std::pair<uint32_t, hipGraphExec_t>> gf_gql;  // by 2 * (number of GQL phases) + labelled
+    uint32_t* d_lreq = nullptr;                               // slots * kLreqWords: labelled NLF requirements
+    const uint32_t* lab_tables = nullptr;                     // the graph's label table the labelled graphs captured
     std::vector<EnumGraph> ge;
     // an enumeration launched and not yet waited for
     bool pending = false;
@@ -80,6 +82,9 @@ struct sgm_batch {
 };
 
 namespace {
+
+// lreq words of the largest query (32 vertices, 32 * 31 adjacency entries)
+const size_t kLreqWords = sgm_lreq_words(SGM_MAXQ, (size_t)SGM_MAXQ * (SGM_MAXQ - 1));
 
 int load_query(sgm_ctx* c, const uint32_t* qoff, const uint32_t* qadj, uint32_t n, sgm::QueryCSR& q, sgm_qdesc& qd)
 {
@@ -154,7 +159,9 @@ void sgm_batch_destroy(sgm_batch* b)
         if (b->stream) (void)hipStreamSynchronize(b->stream);
         if (b->pending) b->c->enum_inflight--;
     }
-    if (b->gf_nlf) (void)hipGraphExecDestroy(b->gf_nlf);
+    for (auto x : b->gf_nlf)
+        if (x) (void)hipGraphExecDestroy(x);
+    dfree(b->d_lreq);
     for (auto& x : b->gf_gql) (void)hipGraphExecDestroy(x.second);
     for (auto& x : b->ge) (void)hipGraphExecDestroy(x.exec);
     for (auto& e : b->ev)
@@ -205,6 +212,7 @@ int sgm_batch_create(sgm_graph* g, uint32_t slots, sgm_batch** out)
     if (e == hipSuccess) e = dalloc(&b->d_st, slots);
     if (e == hipSuccess) e = dalloc(&b->d_qstop, 3 * (size_t)slots);
     if (e == hipSuccess) e = dalloc(&b->d_split, (size_t)SGM_SPLIT_WORDS * slots);
+    if (e == hipSuccess) e = dalloc(&b->d_lreq, kLreqWords * slots);
     if (e == hipSuccess) e = dalloc(&b->d_rec, (size_t)SGM_SPLIT_FIELDS * SGM_SPLIT_ITEMS * slots);
     if (e == hipSuccess) e = dalloc(&b->d_lastbits, (size_t)slots * ((V + 31) / 32));
     if (e == hipSuccess) e = dalloc(&b->d_rank_given, (size_t)slots * 2 * CM_ROWS);
@@ -237,22 +245,50 @@ int sgm_batch_create(sgm_graph* g, uint32_t slots, sgm_batch** out)
 int sgm_batch_filter(sgm_batch* b, uint32_t nq, const uint32_t* const* qoffs, const uint32_t* const* qadjs,
                      const uint32_t* ns, int kind, int32_t* status, uint32_t* cand_counts, uint64_t* cand_degsum)
 {
+    return sgm_batch_filter_labeled(b, nq, qoffs, qadjs, nullptr, ns, kind, status, cand_counts, cand_degsum);
+}
+
+int sgm_batch_filter_labeled(sgm_batch* b, uint32_t nq, const uint32_t* const* qoffs, const uint32_t* const* qadjs,
+                             const uint32_t* const* qlabs, const uint32_t* ns, int kind, int32_t* status,
+                             uint32_t* cand_counts, uint64_t* cand_degsum)
+{
     if (!b || !qoffs || !qadjs || !ns) return SGM_EINVAL;
     sgm_ctx* c = b->c;
     sgm_graph* g = b->g;
     if (nq == 0 || nq > b->slots) return set_err(c, SGM_EINVAL, "batch of %u queries for %u slots", nq, b->slots);
     if (kind != SGM_FILTER_NLF && kind != SGM_FILTER_GQL) return set_err(c, SGM_EINVAL, "unknown filter kind %d", kind);
     if (b->pending) return set_err(c, SGM_ESTATE, "wait for the batch's enumeration before loading another batch");
-    if (g->labelled)  // the replayed filter graphs run the unlabeled kernels only
-        return set_err(c, SGM_EINVAL, "batches take unlabeled graphs; filter labelled ones with sgm_filter_batch");
     HIPCHK(c, hipSetDevice(c->device));
     b->nq = 0;
     b->q.assign(nq, sgm::QueryCSR());
+    bool lab = false;  // some slot runs the labelled filter (then every slot's descriptor says which)
+    std::vector<uint32_t> lreq;
     for (uint32_t i = 0; i < b->slots; i++) {
         sgm_qdesc& d = b->hqd[i];
+        d.lab_on = 0;
+        d.lreq = nullptr;
         if (i < nq) {
             const int rc = load_query(c, qoffs[i], qadjs[i], ns[i], b->q[i], d);
             if (rc) return rc;
+            sgm::QueryCSR& q = b->q[i];
+            bool ql = false;
+            for (uint32_t u = 0; qlabs && qlabs[i] && u < q.n && !ql; u++) ql = qlabs[i][u] != 0;
+            if (ql) q.lab.assign(qlabs[i], qlabs[i] + q.n);
+            if (ql || g->labelled) {
+                if (!lab && !g->has_lab) {
+                    const int lrc = build_label_tables(c, g, nullptr);
+                    if (lrc) return lrc;
+                }
+                lab = true;
+                d.lab_on = 1;
+                for (uint32_t u = 0; u < q.n; u++) d.qlab[u] = q.label(u);
+                d.lreq = b->d_lreq + kLreqWords * i;
+                std::vector<uint32_t> blk = lreq_block(q);
+                blk.resize(kLreqWords, 0u);
+                lreq.insert(lreq.end(), blk.begin(), blk.end());
+            } else {
+                lreq.resize(lreq.size() + kLreqWords, 0u);
+            }
         } else {
             d.n = 0;
         }
@@ -263,13 +299,25 @@ int sgm_batch_filter(sgm_batch* b, uint32_t nq, const uint32_t* const* qoffs, co
     uint32_t max_n = 0;
     for (uint32_t i = 0; i < nq; i++) max_n = std::max(max_n, b->q[i].n);
     const uint32_t n_phases = kind == SGM_FILTER_GQL ? 2 * max_n : 0;
-    hipGraphExec_t* gexec = &b->gf_nlf;
+    if (lab && b->lab_tables != g->d_lab) {  // label tables rebuilt since capture: recapture
+        if (b->gf_nlf[1]) (void)hipGraphExecDestroy(b->gf_nlf[1]);
+        b->gf_nlf[1] = nullptr;
+        std::vector<std::pair<uint32_t, hipGraphExec_t>> keep;
+        for (auto& x : b->gf_gql) {
+            if (x.first & 1u) (void)hipGraphExecDestroy(x.second);
+            else keep.push_back(x);
+        }
+        b->gf_gql.swap(keep);
+        b->lab_tables = g->d_lab;
+    }
+    hipGraphExec_t* gexec = &b->gf_nlf[lab ? 1 : 0];
     if (kind == SGM_FILTER_GQL) {
+        const uint32_t key = 2 * n_phases + (lab ? 1u : 0u);
         gexec = nullptr;
         for (auto& x : b->gf_gql)
-            if (x.first == n_phases) gexec = &x.second;
+            if (x.first == key) gexec = &x.second;
         if (!gexec) {
-            b->gf_gql.emplace_back(n_phases, nullptr);
+            b->gf_gql.emplace_back(key, nullptr);
             gexec = &b->gf_gql.back().second;
         }
     }
@@ -283,17 +331,26 @@ int sgm_batch_filter(sgm_batch* b, uint32_t nq, const uint32_t* const* qoffs, co
             sgm::launch_fill_u32(b->stream, reinterpret_cast<uint32_t*>(b->d_degsum), 2 * (size_t)slots * SGM_MAXQ, 0);
             if (kind == SGM_FILTER_GQL && V) {
                 sgm::launch_nlf_rank(b->stream, g->d_gecnt, V, b->d_qd, slots);
+                if (lab)
+                    sgm::launch_nlf_rank_lab(b->stream, g->d_bydeg, g->d_off, g->d_lab, g->d_run_off, g->d_run_lab,
+                                             g->d_run_cnt, V, b->d_qd, slots);
                 sgm::launch_gql_prepass(b->stream, g->d_dtab, V, g->d_gecnt, b->d_qd, slots);
                 for (uint32_t ph = 0; ph < n_phases; ph++)
                     sgm::launch_gql_phase(b->stream, g->d_roff, g->d_radj, V, g->d_gecnt, b->d_qd, slots, ph);
                 sgm::launch_unrank(b->stream, g->d_rank, V, b->d_qd, slots);
             } else {
                 sgm::launch_nlf(b->stream, g->d_off, V, b->d_qd, slots);
+                if (lab && V)
+                    sgm::launch_nlf_lab(b->stream, g->d_off, g->d_lab, g->d_run_off, g->d_run_lab, g->d_run_cnt, V,
+                                        b->d_qd, slots);
             }
             sgm::launch_cand_stats(b->stream, g->d_off, V, b->d_qd, slots, b->d_cnt, b->d_degsum);
         }, gexec);
         if (rc) return rc;
     }
+    if (lab)
+        HIPCHK(c, hipMemcpyAsync(b->d_lreq, lreq.data(), sizeof(uint32_t) * lreq.size(), hipMemcpyHostToDevice,
+                                 b->stream));
     HIPCHK(c, hipMemcpyAsync(b->d_qd, b->hqd.data(), sizeof(sgm_qdesc) * slots, hipMemcpyHostToDevice, b->stream));
     HIPCHK(c, hipEventRecord(b->ev[0], b->stream));
     HIPCHK(c, hipGraphLaunch(*gexec, b->stream));

@@ -214,7 +214,7 @@ int sgm_graph_upload(sgm_ctx* c, const uint32_t* off, const uint32_t* adj, uint3
 // Label tables of the labelled NLF (sgm_graph::d_lab, d_run_*), built on the host from the
 // resident CSR: per vertex the labels of its neighbours, sorted, as (label, count) runs.
 // labels == nullptr: every label 0 (one run per non-isolated vertex).
-static int build_label_tables(sgm_ctx* c, sgm_graph* g, const uint32_t* labels)
+extern "C++" int build_label_tables(sgm_ctx* c, sgm_graph* g, const uint32_t* labels)
 {
     const uint32_t V = g->V;
     std::vector<uint32_t> off((size_t)V + 1), adj, lab(V, 0u), roff((size_t)V + 1, 0u), rlab, rcnt;
@@ -425,11 +425,10 @@ int sgm_query_set_labels(sgm_query* q, const uint32_t* labels, uint32_t n)
     return SGM_OK;
 }
 
-// The lreq block of q (sgm_qdesc::lreq): per u its neighbours' labels with multiplicities.
-static int ensure_lreq(sgm_ctx* c, sgm_query* q)
+// The lreq block of a query (sgm_qdesc::lreq): per u its neighbours' labels with
+// multiplicities, ascending by label.
+extern "C++" std::vector<uint32_t> lreq_block(const sgm::QueryCSR& Q)
 {
-    if (q->d_lreq) return SGM_OK;
-    const sgm::QueryCSR& Q = q->q;
     const uint32_t n = Q.n;
     std::vector<uint32_t> first(n + 1, 0), plab, pcnt;
     for (uint32_t u = 0; u < n; u++) {
@@ -448,6 +447,13 @@ static int ensure_lreq(sgm_ctx* c, sgm_query* q)
     std::vector<uint32_t> blk(first);
     blk.insert(blk.end(), plab.begin(), plab.end());
     blk.insert(blk.end(), pcnt.begin(), pcnt.end());
+    return blk;
+}
+
+static int ensure_lreq(sgm_ctx* c, sgm_query* q)
+{
+    if (q->d_lreq) return SGM_OK;
+    const std::vector<uint32_t> blk = lreq_block(q->q);
     HIPCHK(c, dalloc(&q->d_lreq, blk.size()));
     HIPCHK(c, hipMemcpy(q->d_lreq, blk.data(), sizeof(uint32_t) * blk.size(), hipMemcpyHostToDevice));
     return SGM_OK;

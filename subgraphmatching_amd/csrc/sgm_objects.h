@@ -189,6 +189,11 @@ inline int grow(sgm_ctx* c, T** p, uint32_t& cap_now, uint32_t need)
     return SGM_OK;
 }
 
+// Label tables of the labelled filters (labels == nullptr: every label 0), and the lreq block
+// (sgm_qdesc::lreq) of a query; shared by the filter drivers (sgm_capi.cpp, sgm_batch.cpp).
+int build_label_tables(sgm_ctx* c, sgm_graph* g, const uint32_t* labels);
+std::vector<uint32_t> lreq_block(const sgm::QueryCSR& q);
+
 // Shared by the enumeration drivers (sgm_capi.cpp, sgm_batch.cpp).
 // Cost-model work buffers filled on the device, and a forward pass over them (ranking).
 int costmodel_work_inputs(sgm_costmodel* m, uint32_t S, uint16_t** F, uint8_t** valid, uint32_t** back, uint32_t** fwd);

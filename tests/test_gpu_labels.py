@@ -168,11 +168,29 @@ def test_query_labels_reset_filter_state(gl_small):
         sgm.enumerate_query(can, sgm.Order_RI(q))
 
 
-def test_batch_refuses_labels(gl_small):
-    b = sgm.Batch(gl_small, slots=4)
-    qs = query_set(gl_small, 2, 4, True, seed=208)
-    with pytest.raises(L.SgmError):
-        b.filter([sgm.Query(q.off, q.adj) for q in qs])
-    b0 = sgm.Batch(chung_lu(2000, 2.3, 8.0, seed=3), slots=4)
-    with pytest.raises(ValueError):
-        b0.filter(qs)
+def test_labelled_batch_runner(gl_small):
+    """The replayed-graph runner with labels: per-slot candidate counts, counts and (uncapped)
+    nodes / intersections / prunes equal the oracle's; a batch mixing labelled and unlabeled
+    queries on an unlabeled graph, and unlabeled queries on the labelled graph, too."""
+    g0 = chung_lu(2000, 2.3, 8.0, seed=3)
+    qs = _mixed(gl_small, 12, 209, 3, 10)
+    cases = [(gl_small, qs), (gl_small, [sgm.Query(q.off, q.adj, name=q.name + "_u") for q in qs[:6]]),
+             (g0, qs[:6] + query_set(g0, 6, lambda i, r: int(r.integers(3, 8)), True, seed=210))]
+    for g, batch in cases:
+        b = sgm.Batch(g, slots=16)
+        for rep in range(2):  # the second filter replays the captured graph
+            ok, cnt, _ = b.filter(batch)
+            res = b.enumerate(cap=3_000_000)
+            say("batch lab", g.name, len(batch), rep)
+            for q, good, c, r in zip(batch, ok, cnt, res):
+                ok_r, c_r = O.gql(g, q)
+                assert bool(good) == ok_r, q.name
+                if not ok_r:
+                    assert r.ans == 0
+                    continue
+                assert [int(x) for x in c[:q.getVertexCnt()]] == [len(x) for x in c_r], q.name
+                ref = O.enum_cache(g, q, c_r, O.order_ri(q), cap=3_000_000)
+                assert r.ans == ref["count"], q.name
+                if not ref["capped"]:
+                    assert (r.nodes, r.intersections, r.fs_prunes) == \
+                        (ref["nodes"], ref["intersections"], ref["fs_prunes"]), q.name
