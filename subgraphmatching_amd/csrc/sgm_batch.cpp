@@ -400,7 +400,17 @@ int sgm_batch_enumerate_launch(sgm_batch* b, const uint32_t* const* orders, cons
     eq.assign(slots, sgm_equery());
     bool split = false;  // some slot is split (the launch then runs the split variant)
     uint32_t max_ops = 1;
-    std::vector<uint32_t> ord(SGM_MAXQ);
+    // orders and plans on host threads (errors reported in slot order below)
+    std::vector<std::array<uint32_t, SGM_MAXQ>> ords(nq);
+    std::vector<int> prcs(nq, 0);
+    sgm::parallel_for(nq, 16, [&](uint32_t i) {
+        const sgm::QueryCSR& q = b->q[i];
+        if (orders && orders[i]) std::copy(orders[i], orders[i] + q.n, ords[i].begin());
+        else sgm::order_ri(q, ords[i].data());
+        sgm::Plan pl;
+        prcs[i] = sgm::build_plan(q, ords[i].data(), pl);
+        if (!prcs[i]) sgm::plan_to_device(pl, plans[i]);
+    });
     for (uint32_t i = 0; i < slots; i++) {
         sgm_equery& e = eq[i];
         std::memset(&e, 0, sizeof e);
@@ -416,13 +426,10 @@ int sgm_batch_enumerate_launch(sgm_batch* b, const uint32_t* const* orders, cons
         e.cap = (unsigned long long)o.cap;
         if (i >= nq) continue;
         const sgm::QueryCSR& q = b->q[i];
-        if (orders && orders[i]) std::copy(orders[i], orders[i] + q.n, ord.begin());
-        else sgm::order_ri(q, ord.data());
-        sgm::Plan pl;
-        const int prc = sgm::build_plan(q, ord.data(), pl);
+        const std::array<uint32_t, SGM_MAXQ>& ord = ords[i];
+        const int prc = prcs[i];
         if (prc == -2) return set_err(c, SGM_EORDER, "order of query %u is not a connected permutation", i);
         if (prc) return set_err(c, SGM_EINVAL, "plan construction failed for query %u (%d)", i, prc);
-        sgm::plan_to_device(pl, plans[i]);
         max_ops = std::max<uint32_t>(max_ops, plans[i].n_ops);
         e.root_u = ord[0];
         if (b->status[i] == SGM_OK) {  // the reference's driver reports 0 without enumerating

@@ -824,11 +824,7 @@ int sgm_plan_export_labeled(const uint32_t* qoff, const uint32_t* qadj, const ui
         if (depth) depth[i] = pl.ops[i].depth;
         if (parent) parent[i] = pl.ops[i].parent;
         if (vdepth) vdepth[i] = pl.ops[i].vdepth;
-        if (emask) {
-            uint32_t m = 0;
-            for (uint32_t x : pl.ops[i].edges) m |= 1u << x;
-            emask[i] = m;
-        }
+        if (emask) emask[i] = pl.ops[i].emask;
     }
     if (iter_set)
         for (uint32_t d = 0; d < n; d++) iter_set[d] = pl.iter_set[d];
@@ -997,13 +993,22 @@ int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uin
         if (q->g != g) return set_err(c, SGM_EINVAL, "queries of a batch must share one graph");
         if (!q->filtered) return set_err(c, SGM_ESTATE, "filter query %u first", i);
         if (q->q.n < 2) return set_err(c, SGM_EINVAL, "enumeration needs at least 2 query vertices (query %u)", i);
+    }
+    // orders and plans of large launches on host threads; errors reported in query order
+    std::vector<int> prcs(nq, 0);
+    sgm::parallel_for(nq, 32, [&](uint32_t i) {
+        sgm_query* q = qs[i];
         sgm::Plan pl;
         if (orders && orders[i]) std::copy(orders[i], orders[i] + q->q.n, ord[i].begin());
         else sgm::order_ri(q->q, ord[i].data());
-        const int prc = sgm::build_plan(q->q, ord[i].data(), pl);
+        prcs[i] = sgm::build_plan(q->q, ord[i].data(), pl);
+        if (!prcs[i]) sgm::plan_to_device(pl, plans[i]);
+    });
+    for (uint32_t i = 0; i < nq; i++) {
+        sgm_query* q = qs[i];
+        const int prc = prcs[i];
         if (prc == -2) return set_err(c, SGM_EORDER, "order of query %u is not a connected permutation", i);
         if (prc) return set_err(c, SGM_EINVAL, "plan construction failed for query %u (%d)", i, prc);
-        sgm::plan_to_device(pl, plans[i]);
         max_ops = std::max<uint32_t>(max_ops, plans[i].n_ops);
         if (!q->empty) live.push_back(i);  // the reference's driver reports 0 without enumerating
     }

@@ -1,7 +1,6 @@
 // Host-side query planning: RI and GQL matching orders, backward neighbours, DAF ancestor
 // masks and the intersection-cache plan.  These are O(n^2..n^3) over queries of at most 32
 // vertices and stay on the CPU, as in the reference.
-#include <unordered_map>
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
@@ -168,8 +167,8 @@ int build_plan(const QueryCSR& q, const uint32_t* order, Plan& pl)
         back[u] = 0;
         for (uint32_t p = q.off[u]; p < q.off[u + 1]; p++) back[u] |= 1u << dep_of[q.adj[p]];
     }
-    std::vector<OpIR> all;
-    all.reserve((size_t)n * (n - 1) / 2);
+    OpIR all[32 * 31 / 2];  // at most one op per (depth, later vertex)
+    uint32_t na = 0;
     uint32_t iter_e[32] = {0};
     for (uint32_t dep = 0; dep + 1 < n; dep++) {
         const uint32_t upto = dep == 31 ? ~0u : ((2u << dep) - 1u);  // depths <= dep
@@ -179,40 +178,57 @@ int build_plan(const QueryCSR& q, const uint32_t* order, Plan& pl)
             o.label = q.label(order[fd]);
             o.vdepth = fd;
             o.e = back[order[fd]] & upto;
-            if (o.e) all.push_back(o);
+            if (o.e) all[na++] = o;
         }
         iter_e[dep + 1] = back[order[dep + 1]] & upto;
         if (!iter_e[dep + 1]) return -2;
     }
-    std::sort(all.begin(), all.end());
-    // duplicates (equal sets) keep the first in sorted order; parents and iteration sets
-    // are then one map probe each
-    std::vector<OpIR> ops;
-    std::unordered_map<uint64_t, uint32_t> first_of;  // (label, edge set) -> op index
+    std::sort(all, all + na);
+    // duplicates (equal label and set) keep the first in sorted order; parents and iteration
+    // sets are then one probe each of an open-addressing table keyed by (label, edge set)
+    // (edge sets are never empty, so key & 0xFFFFFFFF == 0 marks a free slot)
+    OpIR ops[32 * 31 / 2];
+    uint32_t nops = 0;
+    constexpr uint32_t kSlots = 2048, kBits = 11;
+    uint64_t tkey[kSlots];
+    uint32_t tval[kSlots];
+    std::fill(tkey, tkey + kSlots, 0ull);
     auto key = [](uint32_t label, uint32_t e) { return ((uint64_t)label << 32) | e; };
-    for (const OpIR& o : all)
-        if (first_of.emplace(key(o.label, o.e), (uint32_t)ops.size()).second) ops.push_back(o);
-    if (ops.size() > SGM_MAXOPS) return -3;
+    auto slot = [&](uint64_t k) {
+        uint32_t h = (uint32_t)((k * 0x9E3779B97F4A7C15ull) >> (64 - kBits));
+        while ((uint32_t)tkey[h] != 0u && tkey[h] != k) h = (h + 1) & (kSlots - 1);
+        return h;
+    };
+    auto find = [&](uint64_t k) -> uint32_t {
+        if ((uint32_t)k == 0u) return SGM_INVALID_U32;
+        const uint32_t h = slot(k);
+        return tkey[h] == k ? tval[h] : SGM_INVALID_U32;
+    };
+    for (uint32_t a = 0; a < na; a++) {
+        const uint64_t k = key(all[a].label, all[a].e);
+        const uint32_t h = slot(k);
+        if (tkey[h] == k) continue;
+        tkey[h] = k;
+        tval[h] = nops;
+        ops[nops++] = all[a];
+    }
+    if (nops > SGM_MAXOPS) return -3;
     pl.n = n;
     pl.order.assign(order, order + n);
-    pl.ops.resize(ops.size());
-    for (size_t i = 0; i < ops.size(); i++) {
+    pl.ops.resize(nops);
+    for (uint32_t i = 0; i < nops; i++) {
         PlanOp& po = pl.ops[i];
         const uint32_t m = ops[i].e;
         po.depth = ops[i].depth;
         po.vdepth = ops[i].vdepth;
+        po.emask = m;
         po.parent = SGM_INVALID_U32;
-        po.edges.clear();
-        for (uint32_t x = m; x; x &= x - 1) po.edges.push_back((uint32_t)__builtin_ctz(x));
         // the parent computes the list without its deepest edge
-        const auto it = first_of.find(key(ops[i].label, m & ~(0x80000000u >> __builtin_clz(m))));
-        if (it != first_of.end() && it->second < i) po.parent = it->second;
+        const uint32_t p = find(key(ops[i].label, m & ~(0x80000000u >> __builtin_clz(m))));
+        if (p != SGM_INVALID_U32 && p < i) po.parent = p;
     }
     pl.iter_set.assign(n, SGM_INVALID_U32);
-    for (uint32_t d = 1; d < n; d++) {
-        const auto it = first_of.find(key(q.label(order[d]), iter_e[d]));
-        if (it != first_of.end()) pl.iter_set[d] = it->second;
-    }
+    for (uint32_t d = 1; d < n; d++) pl.iter_set[d] = find(key(q.label(order[d]), iter_e[d]));
     // DAF ancestors over depths (inc/prune.h:100-104)
     pl.anc.assign(n, 0);
     for (uint32_t i = 0; i < n; i++) {

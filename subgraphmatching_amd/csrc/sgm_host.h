@@ -1,5 +1,7 @@
 // Host-side planning helpers (orders, intersection-cache plan).
 #pragma once
+#include <algorithm>
+#include <thread>
 #include <stdint.h>
 
 #include <vector>
@@ -19,7 +21,7 @@ struct QueryCSR {
 
 struct PlanOp {
     uint32_t depth, vdepth, parent;
-    std::vector<uint32_t> edges;
+    uint32_t emask;  // the op's backward-neighbour depths, as a set
 };
 
 struct Plan {
@@ -31,6 +33,28 @@ struct Plan {
 };
 
 void order_ri(const QueryCSR& q, uint32_t* order);
+
+// f(i) for i in [0, n) on up to 8 host threads (about `grain` items per thread; the calling
+// thread takes a share).  f must not throw.
+template <class F>
+void parallel_for(uint32_t n, uint32_t grain, F&& f)
+{
+    uint32_t T = grain ? n / grain : 1u;
+    const uint32_t hw = std::max(1u, std::thread::hardware_concurrency());
+    T = std::max(1u, std::min({T, 8u, hw}));
+    if (T == 1) {
+        for (uint32_t i = 0; i < n; i++) f(i);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve(T - 1);
+    for (uint32_t t = 1; t < T; t++)
+        th.emplace_back([&, t] {
+            for (uint32_t i = t; i < n; i += T) f(i);
+        });
+    for (uint32_t i = 0; i < n; i += T) f(i);
+    for (auto& x : th) x.join();
+}
 void order_gql(const QueryCSR& q, const uint32_t* cand_cnt, uint32_t num_data_vertices, uint32_t* order);
 int check_order(const QueryCSR& q, const uint32_t* order);
 int build_plan(const QueryCSR& q, const uint32_t* order, Plan& pl);
