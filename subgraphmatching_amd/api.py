@@ -221,11 +221,9 @@ class CandidateParam:
             pass
 
     def _after_filter(self):
+        # host copies (counts, lists, membership, tables) are fetched on first access
         self._filtered = True
-        n = self.query.getVertexCnt()
-        cnt = np.zeros(n, dtype=np.uint32)
-        _check(L.lib().sgm_cand_counts(self._q, L.ptr(cnt)), self.ctx)
-        self._counts = cnt
+        self._counts = None
         self._cands = None
         self._exist = None
         self.edge_matrix = _EdgeMatrix(self)
@@ -233,7 +231,11 @@ class CandidateParam:
     @property
     def candidates_count(self) -> np.ndarray:
         if self._counts is None:
-            raise RuntimeError("filter the candidates first")
+            if not self._filtered:
+                raise RuntimeError("filter the candidates first")
+            cnt = np.zeros(self.query.getVertexCnt(), dtype=np.uint32)
+            _check(L.lib().sgm_cand_counts(self._q, L.ptr(cnt)), self.ctx)
+            self._counts = cnt
         return self._counts
 
     @property

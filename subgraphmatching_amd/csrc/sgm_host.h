@@ -35,24 +35,28 @@ struct Plan {
 void order_ri(const QueryCSR& q, uint32_t* order);
 
 // f(i) for i in [0, n) on up to 8 host threads (about `grain` items per thread; the calling
-// thread takes a share).  f must not throw.
+// thread takes residue 0, and every residue whose thread could not be started).  f must
+// not throw.
 template <class F>
 void parallel_for(uint32_t n, uint32_t grain, F&& f)
 {
     uint32_t T = grain ? n / grain : 1u;
     const uint32_t hw = std::max(1u, std::thread::hardware_concurrency());
     T = std::max(1u, std::min({T, 8u, hw}));
-    if (T == 1) {
-        for (uint32_t i = 0; i < n; i++) f(i);
-        return;
-    }
     std::vector<std::thread> th;
-    th.reserve(T - 1);
-    for (uint32_t t = 1; t < T; t++)
-        th.emplace_back([&, t] {
+    uint32_t started = 1;
+    try {
+        th.reserve(T - 1);
+        for (uint32_t t = 1; t < T; t++, started++)
+            th.emplace_back([&f, n, T, t] {
+                for (uint32_t i = t; i < n; i += T) f(i);
+            });
+    } catch (...) {
+        // out of threads: the residues from `started` on run here
+    }
+    for (uint32_t t = 0; t < T; t++)
+        if (t == 0 || t >= started)
             for (uint32_t i = t; i < n; i += T) f(i);
-        });
-    for (uint32_t i = 0; i < n; i += T) f(i);
     for (auto& x : th) x.join();
 }
 void order_gql(const QueryCSR& q, const uint32_t* cand_cnt, uint32_t num_data_vertices, uint32_t* order);
