@@ -66,6 +66,7 @@ void sgm_ctx_destroy(sgm_ctx* c)
     dfree(c->d_fcd);
     dfree(c->d_fcnt);
     if (c->h_fcnt) (void)hipHostFree(c->h_fcnt);
+    if (c->h_stage) (void)hipHostFree(c->h_stage);
     dfree(c->d_lastbits);
     dfree(c->d_split);
     dfree(c->d_rec);
@@ -465,8 +466,21 @@ static int ensure_lreq(sgm_ctx* c, sgm_query* q)
 static int filter_chunk(sgm_ctx* c, sgm_graph* g, sgm_query* const* qs, uint32_t nq, int kind, double ms[3])
 {
     const uint32_t V = g->V;
-    std::vector<sgm_qdesc> qd(nq);
-    std::vector<sgm_cdesc> cd(nq);
+    // descriptors are built in pinned memory (a plain vector if none can be had)
+    const size_t qd_bytes = stage_align(sizeof(sgm_qdesc) * nq);
+    std::vector<sgm_qdesc> qd_v;
+    std::vector<sgm_cdesc> cd_v;
+    sgm_qdesc* qd = nullptr;
+    sgm_cdesc* cd = nullptr;
+    if (uint8_t* st = static_cast<uint8_t*>(host_stage(c, qd_bytes + sizeof(sgm_cdesc) * nq))) {
+        qd = reinterpret_cast<sgm_qdesc*>(st);
+        cd = reinterpret_cast<sgm_cdesc*>(st + qd_bytes);
+    } else {
+        qd_v.resize(nq);
+        cd_v.resize(nq);
+        qd = qd_v.data();
+        cd = cd_v.data();
+    }
     uint32_t max_n = 0;
     bool any_lab = false;
     for (uint32_t i = 0; i < nq; i++) {
@@ -514,8 +528,8 @@ static int filter_chunk(sgm_ctx* c, sgm_graph* g, sgm_query* const* qs, uint32_t
     }
     sgm_qdesc* d_qd = c->d_fqd;
     sgm_cdesc* d_cd = c->d_fcd;
-    HIPCHK(c, hipMemcpyAsync(d_qd, qd.data(), sizeof(sgm_qdesc) * nq, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(d_cd, cd.data(), sizeof(sgm_cdesc) * nq, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(d_qd, qd, sizeof(sgm_qdesc) * nq, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(d_cd, cd, sizeof(sgm_cdesc) * nq, hipMemcpyHostToDevice, c->stream));
     sgm::launch_clear_flags(c->stream, d_qd, nq);
     if (kind == SGM_FILTER_GQL && V)  // touch and dirty words start clear; the rest is written before it is read
         sgm::launch_fill_strided(c->stream, c->d_gqls + V, sgm_gql_clear_words(V), sgm_gql_scratch_words(V), nq, 0);
@@ -983,8 +997,28 @@ int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uin
     if (o.method != SGM_ENUM_CACHE_FILTER_FS && o.method != SGM_ENUM_CACHE_FILTER)
         return set_err(c, SGM_EINVAL, "batched enumeration supports the cache + filter methods (got %d)", o.method);
     std::memset(out, 0, sizeof(sgm_stats) * nq);
-    // plans (host), validated before anything runs
-    std::vector<sgm_plan_dev> plans(nq);
+    // plans (host), validated before anything runs; plans, query descriptors and statistics
+    // live in pinned staging (plain vectors if none can be had)
+    const size_t pl_bytes = stage_align(sizeof(sgm_plan_dev) * nq);
+    const size_t eq_bytes = stage_align(sizeof(sgm_equery) * nq);
+    std::vector<sgm_plan_dev> plans_v;
+    std::vector<sgm_equery> eq_v;
+    std::vector<sgm_stats_dev> got_v;
+    sgm_plan_dev* plans = nullptr;
+    sgm_equery* eqs = nullptr;
+    sgm_stats_dev* got = nullptr;
+    if (uint8_t* st = static_cast<uint8_t*>(host_stage(c, pl_bytes + eq_bytes + sizeof(sgm_stats_dev) * nq))) {
+        plans = reinterpret_cast<sgm_plan_dev*>(st);
+        eqs = reinterpret_cast<sgm_equery*>(st + pl_bytes);
+        got = reinterpret_cast<sgm_stats_dev*>(st + pl_bytes + eq_bytes);
+    } else {
+        plans_v.resize(nq);
+        eq_v.resize(nq);
+        got_v.resize(nq);
+        plans = plans_v.data();
+        eqs = eq_v.data();
+        got = got_v.data();
+    }
     std::vector<uint32_t> live;
     std::vector<std::array<uint32_t, SGM_MAXQ>> ord(nq);  // the orders used (given or RI)
     uint32_t max_ops = 1;
@@ -1023,7 +1057,7 @@ int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uin
     uint32_t cap4 = 3 * c->batch_cap2;
     if ((rc = grow(c, &c->d_qstop, cap4, 3 * nq))) return rc;  // stop words, item and chunk counters
     c->batch_cap2 = std::max(c->batch_cap2, nq);
-    HIPCHK(c, hipMemcpyAsync(c->d_plans, plans.data(), sizeof(sgm_plan_dev) * nq, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_plans, plans, sizeof(sgm_plan_dev) * nq, hipMemcpyHostToDevice, c->stream));
     const uint32_t blocks = default_blocks(c, o, (uint32_t)live.size());
     const uint32_t waves = blocks * 4;
     uint64_t words = o.stack_words ? o.stack_words : (uint64_t)8 * g->max_deg + 1024;
@@ -1032,7 +1066,6 @@ int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uin
     const uint32_t* hub_slot = nullptr;
     if ((rc = hub_slot_map(g, o, &hub_slot))) return rc;
     std::vector<sgm_stats_dev> st(nq);
-    std::vector<sgm_equery> eq;
     const size_t lb_words = ((size_t)g->V + 31) / 32;
     if (c->lastbits_words < lb_words * live.size()) {
         dfree(c->d_lastbits);
@@ -1066,7 +1099,8 @@ int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uin
         if (words > 0x7FFFFFFFull) return set_err(c, SGM_EDEVICE, "set-op stack would exceed 2^31 words");
         rc = ensure_stack(c, (size_t)waves * words * sizeof(uint32_t));
         if (rc) return rc;
-        eq.assign(live.size(), sgm_equery());
+        sgm_equery* eq = eqs;
+        std::fill(eq, eq + live.size(), sgm_equery());
         const uint32_t m = (uint32_t)live.size();
         uint32_t js = 0;  // next split block
         for (size_t k = 0; k < live.size(); k++) {
@@ -1089,7 +1123,7 @@ int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uin
             eq[k].n_items = g->ge_cnt[std::min<uint32_t>(q->q.off[u0 + 1] - q->q.off[u0], 32)];
             eq[k].root_u = u0;
         }
-        HIPCHK(c, hipMemcpyAsync(c->d_eq, eq.data(), sizeof(sgm_equery) * m, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->d_eq, eq, sizeof(sgm_equery) * m, hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipMemsetAsync(c->d_qstats, 0, sizeof(sgm_stats_dev) * m, c->stream));
         HIPCHK(c, hipMemsetAsync(c->d_qstop, 0, sizeof(uint32_t) * 3 * m, c->stream));
         HIPCHK(c, hipMemsetAsync(c->d_stop, 0, sizeof(uint32_t), c->stream));
@@ -1103,8 +1137,7 @@ int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uin
         if (split) sgm::launch_split_reduce(c->stream, c->d_eq, m);
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-        std::vector<sgm_stats_dev> got(m);
-        HIPCHK(c, hipMemcpyAsync(got.data(), c->d_qstats, sizeof(sgm_stats_dev) * m, hipMemcpyDeviceToHost,
+        HIPCHK(c, hipMemcpyAsync(got, c->d_qstats, sizeof(sgm_stats_dev) * m, hipMemcpyDeviceToHost,
                                  c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         float ms = 0;

@@ -53,6 +53,10 @@ struct sgm_ctx {
     uint32_t* d_split = nullptr;
     unsigned long long* d_rec = nullptr;
     uint32_t split_cap = 0;  // queries the split buffers hold
+    // pinned host staging of one call's descriptor uploads and statistics downloads (every
+    // call synchronises its stream before it returns, so the next call may reuse it)
+    void* h_stage = nullptr;
+    size_t h_stage_bytes = 0;
     // objects that hold this context (destroyed with it if the caller has not done so)
     std::vector<sgm_batch*> batches;
     std::vector<sgm_costmodel*> models;
@@ -175,6 +179,24 @@ inline void dfree(void* p)
 {
     if (p) (void)hipFree(p);
 }
+
+// The context's pinned staging area, grown to `bytes` (nullptr when pinned memory runs out).
+inline void* host_stage(sgm_ctx* c, size_t bytes)
+{
+    if (c->h_stage_bytes < bytes) {
+        if (c->h_stage) (void)hipHostFree(c->h_stage);
+        c->h_stage = nullptr;
+        c->h_stage_bytes = 0;
+        if (hipHostMalloc(&c->h_stage, bytes) != hipSuccess) {
+            c->h_stage = nullptr;
+            return nullptr;
+        }
+        c->h_stage_bytes = bytes;
+    }
+    return c->h_stage;
+}
+
+inline size_t stage_align(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // (Re)allocate a context buffer so that it holds at least `need` elements.
 template <class T>
