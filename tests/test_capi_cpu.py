@@ -69,6 +69,20 @@ def test_labelled_plans_match_oracle():
     assert changed > 0
 
 
+def test_plans_over_sampled_orders_match_oracle():
+    """Many connected orders per query (seeded samples), labelled and not: the library's plan
+    (op table keyed by label and edge set) equals the oracle's restatement of PlanIR."""
+    from subgraphmatching_amd.learned import sample_order
+    g = chung_lu(3000, 2.5, 7.4, seed=41)
+    gl = with_labels(g, 4, seed=42)
+    qs = query_set(g, 12, lambda i, r: int(r.integers(4, 33)), lambda i, r: bool(i % 2), seed=43)
+    qs += query_set(gl, 12, lambda i, r: int(r.integers(4, 33)), lambda i, r: bool(i % 2), seed=44)
+    for k, q in enumerate(qs):
+        for s in range(8):
+            o = sample_order(q, seed=1000 * k + s)
+            assert sgm.plan_export(q, o) == O.plan(q, o), (q.name, o)
+
+
 def test_plan_rejects_disconnected_order():
     q = query_from_edges(4, [(0, 1), (1, 2), (2, 3)])
     with pytest.raises(L.SgmError):
