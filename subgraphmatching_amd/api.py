@@ -84,6 +84,8 @@ class Context:
         if dg is None or dg.graph is not g:
             dg = DeviceGraph(self, g)
             self._graphs[key] = dg
+        elif dg.labels is not g.labels:  # labels attached or replaced since the upload
+            dg.set_labels(g.labels)
         return dg
 
     def __del__(self):
@@ -129,9 +131,16 @@ class DeviceGraph:
                                         C.byref(h)), ctx)
         self.handle = h
         self.max_degree = int(L.lib().sgm_graph_max_degree(h))
+        self.labels = None
         if g.labels is not None:
-            lab = L.u32(g.labels)
-            _check(L.lib().sgm_graph_set_labels(h, L.ptr(lab), g.getVertexCnt()), ctx)
+            self.set_labels(g.labels)
+
+    def set_labels(self, labels):
+        """Vertex labels of the resident graph (None: unlabeled); set them before filtering."""
+        lab = None if labels is None else L.u32(labels)
+        _check(L.lib().sgm_graph_set_labels(self.handle, None if lab is None else L.ptr(lab),
+                                            self.graph.getVertexCnt()), self.ctx)
+        self.labels = labels
 
     def close(self):
         if self.handle:

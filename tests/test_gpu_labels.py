@@ -194,3 +194,20 @@ def test_labelled_batch_runner(gl_small):
                 if not ref["capped"]:
                     assert (r.nodes, r.intersections, r.fs_prunes) == \
                         (ref["nodes"], ref["intersections"], ref["fs_prunes"]), q.name
+
+
+def test_labels_attached_after_upload():
+    """Labels set on a Graph after it went to the device reach the device filters."""
+    g = chung_lu(2000, 2.3, 8.0, seed=11)
+    q = query_set(g, 1, 5, True, seed=212)[0]
+    can = sgm.CandidateParam(g, q)
+    assert sgm.Filter_GQL(g, q, can, build_tables=False)
+    lab = np.random.default_rng(213).integers(0, 2, g.getVertexCnt())
+    g.set_labels(lab)
+    q.set_labels(lab[:q.getVertexCnt()])
+    can2 = sgm.CandidateParam(g, q)
+    ok = sgm.Filter_GQL(g, q, can2, build_tables=False)
+    ok_r, c_r = O.gql(g, q)
+    assert ok == ok_r
+    if ok:
+        _same_cands(can2, q, c_r)
