@@ -46,7 +46,7 @@ def lib():
         if not os.path.exists(path):
             build()
         _lib = C.CDLL(path)
-        if _lib.orc_abi_version() != 2:  # a stale build from before the label arguments
+        if _lib.orc_abi_version() != 3:  # a stale build from before the current arguments
             build(force=True)
             raise RuntimeError("liboracle.so was stale and has been rebuilt; restart the process")
         assert _lib.orc_stats_size() == C.sizeof(Stats)
@@ -189,15 +189,15 @@ def enum_cache(g, q, cands, order, cap=None, fs=True, time_limit_ms=0.0):
     return st.as_dict(n)
 
 
-def enum_all(g, q, cands, order, cap=None, time_limit_ms=0.0):
-    """explore_Intersection_All<false x5> restated; returns the stats dict."""
+def enum_all(g, q, cands, order, cap=None, time_limit_ms=0.0, fs=False):
+    """explore_Intersection_All<fs,false,false,false,false> restated; returns the stats dict."""
     V, n = g.getVertexCnt(), q.getVertexCnt()
     m, cnt, stride = _cand_matrix(cands, V)
     o = _u32(order)
     st = Stats()
     rc = lib().orc_enum_all(C.c_uint32(V), _p(g.off), _p(g.adj), C.c_uint32(n), _p(q.off),
                             _p(q.adj), _p(m), _p(cnt), C.c_uint32(stride), _p(o),
-                            C.c_uint64(cap if cap is not None else 2**64 - 1),
+                            C.c_uint64(cap if cap is not None else 2**64 - 1), C.c_int(1 if fs else 0),
                             C.c_double(time_limit_ms), C.byref(st))
     if rc:
         raise ValueError(f"orc_enum_all failed ({rc})")

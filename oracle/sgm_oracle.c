@@ -761,11 +761,18 @@ int orc_enum_cache(uint32_t V, const uint32_t *off, const uint32_t *adj, uint32_
 }
 
 /* ------------------------------------------------------------------------------------
- * Plain candidate-space enumeration over the auxiliary tables (inc/explore.h:2757-2848
- * explore_Intersection_All<false,false,false,false,false>, :2610-2747 _explore_recur_All,
+ * Candidate-space enumeration over the auxiliary tables (inc/explore.h:2757-2848
+ * explore_Intersection_All<fs,false,false,false,false>, :2610-2747 _explore_recur_All,
  * :284-354 CandidatesBuffer::getNextCandidatesImpl).  Candidates at depth d are the
  * intersection, over the backward neighbours p of u_d in order position, of the table
  * lists edge_matrix[p][u_d].getNeb(idx(p)); each extra neighbour is one intersection.
+ * use_fs: DAF failing sets through PruningOpt (inc/prune.h:2099-2260 on DAFFailingPrune,
+ * :82-171): at a child's entry an empty candidate set gives fail[d-1] = anc[d] and ends the
+ * child (NoCandidatesPruneCheck), a non-empty one clears fail[d-1]; an injectivity
+ * conflict with the vertex mapped at depth c gives fail[d] = anc[d] | anc[c], then
+ * fail[d-1] |= fail[d]; a full match gives fail[d] = all, fail[d-1] |= all; after each
+ * candidate (BacktrackingPrune) depth d > 0 cuts its remaining candidates when fail[d]
+ * lacks d (fail[d-1] = fail[d]), else fail[d-1] |= fail[d].
  * ---------------------------------------------------------------------------------- */
 typedef struct {
     uint32_t *off, *idx; /* CSR over C(a) of positions in C(b) */
@@ -779,9 +786,10 @@ typedef struct {
     orc_tab *tab; /* tab[a*n+b] */
     uint32_t *buf[ORC_MAXQ], *tmp;
     uint32_t emb[ORC_MAXQ], iemb[ORC_MAXQ];
-    uint8_t *vis;
+    uint8_t *vis;         /* 1 + the depth that maps the vertex, 0 = free */
     uint64_t cap;
-    int stop;
+    int stop, use_fs;
+    uint64_t fail[ORC_MAXQ], anc[ORC_MAXQ];
     double deadline;
     uint64_t tick;
     orc_stats *st;
@@ -807,33 +815,53 @@ static void rec_all(all_ctx *c, uint32_t d)
         uint32_t *sw = c->buf[d]; c->buf[d] = c->tmp; c->tmp = sw;
         tot = m;
     }
+    if (c->use_fs) {  /* NoCandidatesPruneCheck */
+        if (tot == 0) { c->fail[d - 1] = c->anc[d]; return; }
+        c->fail[d - 1] = 0;
+    }
     for (uint32_t pos = 0; pos < tot; pos++) {
         uint32_t vi = c->buf[d][pos];
         uint32_t v = c->cand[(size_t)u * c->stride + vi];
-        if (c->vis[v]) continue;
-        c->emb[u] = v; c->iemb[u] = vi; c->vis[v] = 1;
-        c->st->nodes[d]++;
-        if (d == c->n - 1) {
-            c->st->count++;
-            if (c->st->count >= c->cap) { c->stop = 1; c->st->capped = 1; return; }
+        if (c->vis[v]) {
+            if (!c->use_fs) continue;
+            c->fail[d] = c->anc[d] | c->anc[c->vis[v] - 1];  /* InjectiveConflict */
+            c->fail[d - 1] |= c->fail[d];
         } else {
-            rec_all(c, d + 1);
-            if (c->stop) return;
+            c->emb[u] = v; c->iemb[u] = vi; c->vis[v] = (uint8_t)(d + 1);
+            c->st->nodes[d]++;
+            if (d == c->n - 1) {
+                c->st->count++;
+                if (c->use_fs) { c->fail[d] = ~0ull; c->fail[d - 1] |= ~0ull; }  /* SuccessMatch */
+                if (c->st->count >= c->cap) { c->stop = 1; c->st->capped = 1; return; }
+            } else {
+                rec_all(c, d + 1);
+                if (c->stop) return;
+            }
+            c->vis[v] = 0;
         }
-        c->vis[v] = 0;
+        if (c->use_fs) {  /* BacktrackingPrune -> DAFFailingPrune::PruneCheck(d), d > 0 */
+            if (!((c->fail[d] >> d) & 1ull)) {
+                c->fail[d - 1] = c->fail[d];
+                c->st->fs_prunes++;
+                break;
+            }
+            c->fail[d - 1] |= c->fail[d];
+        }
     }
 }
 
 int orc_enum_all(uint32_t V, const uint32_t *off, const uint32_t *adj, uint32_t n,
                  const uint32_t *qoff, const uint32_t *qadj, const uint32_t *cand,
                  const uint32_t *cnt, uint32_t cand_stride, const uint32_t *order, uint64_t cap,
-                 double time_limit_ms, orc_stats *st)
+                 int use_fs, double time_limit_ms, orc_stats *st)
 {
     memset(st, 0, sizeof(*st));
     if (n < 2 || n > ORC_MAXQ) return -2;
     all_ctx c;
     memset(&c, 0, sizeof(c));
     c.n = n; c.cand = cand; c.order = order; c.stride = cand_stride; c.cap = cap; c.st = st;
+    c.use_fs = use_fs;
+    daf_ancestors(n, qoff, qadj, order, c.anc);
     uint32_t pos_of[ORC_MAXQ];
     for (uint32_t i = 0; i < n; i++) pos_of[order[i]] = i;
     /* GetPreviousNeb (inc/common_ops.h:259-313): backward neighbours sorted by position */
@@ -871,7 +899,7 @@ int orc_enum_all(uint32_t V, const uint32_t *off, const uint32_t *adj, uint32_t 
     uint32_t u0 = order[0];
     for (uint32_t i = 0; i < cnt[u0]; i++) {
         uint32_t v0 = cand[(size_t)u0 * cand_stride + i];
-        c.emb[u0] = v0; c.iemb[u0] = i; c.vis[v0] = 1;
+        c.emb[u0] = v0; c.iemb[u0] = i; c.vis[v0] = 1;  /* depth 0; depth 0 never prunes */
         st->nodes[0]++;
         rec_all(&c, 1);
         if (c.stop) break;
@@ -885,5 +913,5 @@ int orc_enum_all(uint32_t V, const uint32_t *off, const uint32_t *adj, uint32_t 
     return 0;
 }
 
-uint32_t orc_abi_version(void) { return 2u; }
+uint32_t orc_abi_version(void) { return 3u; }
 uint32_t orc_stats_size(void) { return (uint32_t)sizeof(orc_stats); }

@@ -296,6 +296,7 @@ def test_counts_match_brute_force(labels):
             assert O.enum_cache(g, q, c, order)["count"] == exp
             assert O.enum_cache(g, q, c, order, fs=False)["count"] == exp
             assert O.enum_all(g, q, c, order)["count"] == exp
+            assert O.enum_all(g, q, c, order, fs=True)["count"] == exp
             if exp > 1:
                 cap = int(rng.integers(1, exp))
                 assert O.enum_cache(g, q, c, order, cap=cap)["count"] == cap
@@ -390,7 +391,7 @@ def test_cache_path_nodes_and_intersections_brute(labels):
 def test_failing_sets_only_prune():
     """Failing sets never change counts and never add nodes; on some inputs they cut."""
     g = chung_lu(300, 2.3, 5.0, seed=19)
-    cut = 0
+    cut = acut = 0
     for q in query_set(g, 16, lambda i, r: int(r.integers(5, 9)), lambda i, r: bool(i % 2), seed=20):
         ok, c = O.gql(g, q)
         if not ok:
@@ -403,7 +404,34 @@ def test_failing_sets_only_prune():
         assert on["count"] == off["count"]
         assert all(a <= b for a, b in zip(on["nodes"], off["nodes"]))
         cut += on["fs_prunes"]
-    assert cut > 0
+        # the same on the aux-table search (explore_Intersection_All<true,...>)
+        aon = O.enum_all(g, q, c, order, fs=True, cap=2000000)
+        aoff = O.enum_all(g, q, c, order, cap=2000000)
+        if not (aon["capped"] or aoff["capped"]):
+            assert aon["count"] == aoff["count"] == off["count"]
+            assert all(a <= b for a, b in zip(aon["nodes"], aoff["nodes"]))
+            assert aon["intersections"] <= aoff["intersections"]
+            acut += aon["fs_prunes"]
+    assert cut > 0 and acut > 0
+
+
+def test_aux_daf_hand_worked():
+    """Failing sets on the aux-table search with hand-chosen candidate sets.  Data: star
+    0-{1,2,3} plus edge 4-5; query: star u0-{u1,u2}; order (u0, u1, u2); anc = {0}, {0,1},
+    {0,2}.  (a) C(u2) = {4}: the first u1 candidate's child has no candidates, fail[1] =
+    anc[2] = {0,2} lacks depth 1, so the other two u1 candidates are cut (nodes 1,1,0; one
+    prune; without failing sets 1,3,0).  (b) C(u2) = {1}: the child of u1 = 1 meets an
+    injectivity conflict, fail[2] = anc[2] | anc[1] holds depth 1, no cut; u1 = 2 then
+    matches (nodes 1,2,1; count 1)."""
+    g = query_from_edges(6, [(0, 1), (0, 2), (0, 3), (4, 5)])
+    q = query_from_edges(3, [(0, 1), (0, 2)])
+    arr = lambda *x: np.array(x, dtype=np.uint32)  # noqa: E731
+    st = O.enum_all(g, q, [arr(0), arr(1, 2, 3), arr(4)], [0, 1, 2], fs=True)
+    assert (st["count"], st["nodes"], st["fs_prunes"]) == (0, [1, 1, 0], 1)
+    st = O.enum_all(g, q, [arr(0), arr(1, 2, 3), arr(4)], [0, 1, 2])
+    assert (st["count"], st["nodes"], st["fs_prunes"]) == (0, [1, 3, 0], 0)
+    st = O.enum_all(g, q, [arr(0), arr(1, 2), arr(1)], [0, 1, 2], fs=True)
+    assert (st["count"], st["nodes"], st["fs_prunes"]) == (1, [1, 2, 1], 0)
 
 
 def test_daf_hand_worked():
