@@ -4,6 +4,7 @@
 //                                          failing sets (the "RI+cache+filter+fs" row)
 //   explore_Intersection_All<false x5>     plain candidate-space search over the aux-edge
 //                                          tables (the "RI" / "GQL" rows)
+//   explore_Intersection_All<true, false x4>  the same with DAF failing sets
 //
 // Semantics kept from the reference: `ans` is incremented by min(#embeddings,
 // output_limit); `time` is the search time in ms (here the enumeration kernel's, measured
@@ -11,7 +12,7 @@
 // total_intersection and total_iter_count are incremented; stop_flag is watched while the
 // search runs and a raised flag ends it early with the partial count, as the reference's
 // polling does (inc/explore.h:1057).  The other explore_Intersection_All pruning variants
-// are not on the accelerated path and fail to compile.
+// (GuP, conflict detection, VEQ) are not on the accelerated path and fail to compile.
 #pragma once
 
 #include "api.h"
@@ -72,9 +73,11 @@ inline void explore_Intersection_All(const Graph& data, const Query& query, cons
                                      double& time, size_t& ans, std::atomic<bool>& stop_flag,
                                      uint64_t output_limit = -1)
 {
-    static_assert(!useDAFFailingPrune && !useGuPFailingPrune && !useGUPConflictDetection &&
-                      !useBICEConflictDetection && !useVEQAutomorphismPrune,
-                  "only explore_Intersection_All<false,false,false,false,false> is on the accelerated path");
+    static_assert(!useGuPFailingPrune && !useGUPConflictDetection && !useBICEConflictDetection &&
+                      !useVEQAutomorphismPrune,
+                  "only explore_Intersection_All<false|true,false,false,false,false> (plain or DAF failing "
+                  "sets) is on the accelerated path");
     (void)data;
-    sgm::run_enumeration(query, order, cfg, time, ans, stop_flag, output_limit, SGM_ENUM_AUX_TABLES);
+    sgm::run_enumeration(query, order, cfg, time, ans, stop_flag, output_limit,
+                         useDAFFailingPrune ? SGM_ENUM_AUX_TABLES_FS : SGM_ENUM_AUX_TABLES);
 }

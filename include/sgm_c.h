@@ -41,7 +41,9 @@ enum { SGM_FILTER_NLF = 1, SGM_FILTER_GQL = 2 };
 enum {
     SGM_ENUM_CACHE_FILTER_FS = 0, /* explore_Intersection_Cache_DAF_Filter (inc/explore.h:1073) */
     SGM_ENUM_CACHE_FILTER = 1,    /* same search without failing-set pruning */
-    SGM_ENUM_AUX_TABLES = 2       /* explore_Intersection_All<false,false,false,false,false> (inc/explore.h:2757) */
+    SGM_ENUM_AUX_TABLES = 2,      /* explore_Intersection_All<false,false,false,false,false> (inc/explore.h:2757) */
+    SGM_ENUM_AUX_TABLES_FS = 3    /* explore_Intersection_All<true,false,false,false,false>: the same with DAF
+                                     failing sets (inc/prune.h:2099-2260, 82-171) */
 };
 
 typedef struct sgm_ctx sgm_ctx;

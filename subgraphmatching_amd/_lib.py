@@ -31,6 +31,7 @@ SGM_FILTER_GQL = 2
 SGM_ENUM_CACHE_FILTER_FS = 0
 SGM_ENUM_CACHE_FILTER = 1
 SGM_ENUM_AUX_TABLES = 2
+SGM_ENUM_AUX_TABLES_FS = 3
 
 ERR_NAMES = {0: "SGM_OK", -1: "SGM_EINVAL", -2: "SGM_EHIP", -3: "SGM_ENOMEM", -4: "SGM_EEMPTY",
              -5: "SGM_EORDER", -6: "SGM_ESTATE", -7: "SGM_ESTOPPED", -8: "SGM_EDEVICE",

@@ -911,7 +911,7 @@ void stats_out(const sgm_stats_dev& st, uint64_t cap, sgm_stats* out, double ns_
 
 extern "C" {
 
-// explore_Intersection_All<false x5> for one query over its device tables.
+// explore_Intersection_All<fs,false,false,false,false> for one query over its device tables.
 static int enumerate_aux(sgm_query* q, const uint32_t* order, const sgm_enum_opts& o, sgm_stats* out)
 {
     sgm_graph* g = q->g;
@@ -946,6 +946,14 @@ static int enumerate_aux(sgm_query* q, const uint32_t* order, const sgm_enum_opt
         }
     }
     ap.prev_first[n] = e;
+    // DAF ancestors over depths (inc/prune.h:100-104): anc[i] = {i} and the ancestors of every
+    // backward neighbour
+    for (uint32_t d = 0; d < n; d++) {
+        uint64_t a = 1ull << d;
+        for (uint32_t p : prev[order[d]]) a |= ap.anc[pos[p]];
+        ap.anc[d] = a;
+    }
+    const int use_fs = o.method == SGM_ENUM_AUX_TABLES_FS ? 1 : 0;
     HIPCHK(c, hipMemcpyAsync(c->d_auxplan, &ap, sizeof ap, hipMemcpyHostToDevice, c->stream));
     const uint32_t n_items = q->counts[order[0]];
     const uint32_t blocks = default_blocks(c, o, 1);
@@ -961,7 +969,7 @@ static int enumerate_aux(sgm_query* q, const uint32_t* order, const sgm_enum_opt
         c->stop_requested = 0;
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
         sgm::launch_enum_aux(c->stream, blocks, c->d_auxplan, n_items, c->d_stack, (uint32_t)words, c->d_work,
-                             c->d_stats, c->d_stop, (unsigned long long)o.cap);
+                             c->d_stats, c->d_stop, (unsigned long long)o.cap, use_fs);
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
         sgm_stats_dev st;
@@ -1169,7 +1177,7 @@ int sgm_enumerate(sgm_query* q, const uint32_t* order, const sgm_enum_opts* opts
     if (!q || !order || !out) return SGM_EINVAL;
     sgm_ctx* c = q->g->ctx;
     const sgm_enum_opts o = normalize_opts(opts);
-    if (o.method == SGM_ENUM_AUX_TABLES) {
+    if (o.method == SGM_ENUM_AUX_TABLES || o.method == SGM_ENUM_AUX_TABLES_FS) {
         std::memset(out, 0, sizeof *out);
         if (!q->filtered) return set_err(c, SGM_ESTATE, "filter the query first");
         if (q->q.n < 2) return set_err(c, SGM_EINVAL, "enumeration needs at least 2 query vertices");

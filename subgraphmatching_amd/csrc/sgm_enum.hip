@@ -1119,6 +1119,7 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
 // pass.  Per-wave intermediate lists live on the same HBM stack as the cache search.
 struct WaveAux {
     uint64_t rem[SGM_MAXQ];
+    uint64_t fail[SGM_MAXQ];  // failing sets (use_fs)
     unsigned long long nodes[SGM_MAXQ];
     const uint32_t* lst[SGM_MAXQ];
     uint32_t emb[SGM_MAXQ];
@@ -1172,24 +1173,29 @@ __device__ bool aux_list(const sgm_auxplan_dev& A, const WaveAux& S, uint32_t d,
     return true;
 }
 
-// count positions whose vertex is not used by depths < L
+// count positions whose vertex is not used by depths < L; conflicts add anc[L] | anc[j]
+// (j = the depth holding the vertex) to *cf
 __device__ __forceinline__ uint32_t aux_count_chunk(bool valid, uint32_t vidx, const uint32_t* __restrict__ cand,
-                                                    const WaveAux& S, uint32_t L)
+                                                    const WaveAux& S, uint32_t L, const uint64_t* __restrict__ anc,
+                                                    uint64_t* cf)
 {
     bool ok = false;
+    uint32_t c = SGM_INVALID;
     if (valid) {
         const uint32_t v = cand[vidx];
-        ok = true;
         for (uint32_t j = 0; j < L; j++)
-            if (S.emb[j] == v) ok = false;
+            if (S.emb[j] == v) c = j;
+        ok = c == SGM_INVALID;
     }
+    const uint64_t cb = __ballot(valid && !ok);
+    if (cf && cb) *cf |= lanes_or64(cb, (valid && !ok) ? (anc[L] | anc[c]) : 0ull);
     return (uint32_t)__popcll(__ballot(ok));
 }
 
 __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restrict__ gA, uint32_t n_items,
                                                   uint32_t* __restrict__ stack, uint32_t stack_words,
                                                   uint32_t* __restrict__ work, sgm_stats_dev* __restrict__ gst,
-                                                  uint32_t* __restrict__ stop, unsigned long long cap)
+                                                  uint32_t* __restrict__ stop, unsigned long long cap, int use_fs)
 {
     extern __shared__ uint64_t smem_u64[];
     uint8_t* smem = reinterpret_cast<uint8_t*>(smem_u64);
@@ -1200,9 +1206,10 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
     uint32_t* ws = stack + (size_t)(blockIdx.x * (blockDim.x >> 6) + wave) * stack_words;
     const uint32_t L = n - 1;
     for (uint32_t d = 0; d < SGM_MAXQ; d++) S.nodes[d] = 0;
-    unsigned long long inter = 0, elems = 0, outs = 0, scanned = 0, cnt_pending = 0, items_done = 0;
+    unsigned long long inter = 0, elems = 0, outs = 0, scanned = 0, cnt_pending = 0, items_done = 0, fsp = 0;
     bool overflow = false;
     uint32_t poll = 0;
+    const uint64_t* anc = A.anc;
     while (true) {
         uint32_t it = 0;
         if (lane == 0) it = atomicAdd(work, 1u);
@@ -1216,6 +1223,24 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
         S.nodes[0] += 1;
         S.stack_top[1] = 0;
         uint32_t d = 1;
+        // back from a finished child at depth d to d - 1, then PruneCheck(d - 1)
+        // (inc/prune.h:119-137): a failing set without d - 1 cuts d - 1's remaining candidates
+        auto back_up = [&]() -> bool {
+            d--;
+            if (d == 0) return false;
+            if (use_fs) {
+                const uint64_t f = S.fail[d];
+                if (!((f >> d) & 1ull)) {
+                    S.fail[d - 1] = f;
+                    S.rem[d] = 0;
+                    S.pos[d] = S.tot[d];
+                    fsp++;
+                } else {
+                    S.fail[d - 1] |= f;
+                }
+            }
+            return true;
+        };
         // enter depth d (d <= L): build the list; at d == L count directly
         bool entering = true;
         while (true) {
@@ -1235,19 +1260,22 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
                 }
                 if (is_last) {
                     const uint32_t* cand = A.cand[d];
-                    uint32_t ok = 0;
+                    uint32_t ok = 0, cands = size;
+                    uint64_t cf = 0;
+                    uint64_t* cfp = use_fs ? &cf : nullptr;
                     if (!last) {
                         scanned += size;
                         for (uint32_t base = 0; base < size; base += 64) {
                             const uint32_t i = base + lane;
                             const bool valid = i < size;
-                            ok += aux_count_chunk(valid, valid ? list[i] : 0u, cand, S, L);
+                            ok += aux_count_chunk(valid, valid ? list[i] : 0u, cand, S, L, anc, cfp);
                         }
                     } else {
                         const uint32_t* a = list; uint32_t na = size;
                         const uint32_t* b = last; uint32_t nb = nlast;
                         if (na > nb) { const uint32_t* t = a; a = b; b = t; uint32_t tn = na; na = nb; nb = tn; }
                         elems += na + nb;
+                        cands = 0;
                         for (uint32_t base = 0; base < na; base += 64) {
                             const uint32_t i = base + lane;
                             const uint32_t key = i < na ? a[i] : 0u;
@@ -1255,15 +1283,26 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
                             const uint32_t h = (uint32_t)__popcll(__ballot(hit));
                             outs += h;
                             scanned += h;
-                            ok += aux_count_chunk(hit, key, cand, S, L);
+                            cands += h;
+                            ok += aux_count_chunk(hit, key, cand, S, L, anc, cfp);
                         }
                     }
                     cnt_pending += ok;
                     if (cnt_pending >= FLUSH_AT) flush_count(cnt_pending, &gst->count, stop, cap);
                     S.nodes[L] += ok;
-                    d--;
-                    if (d == 0) break;
+                    // the last level's failing set: no candidates -> anc[L]; a match -> all;
+                    // otherwise the conflicts' anc[L] | anc[j]
+                    if (use_fs) S.fail[L - 1] = cands == 0 ? anc[L] : (ok ? ~0ull : cf);
+                    if (!back_up()) break;
                     continue;
+                }
+                if (use_fs) {  // NoCandidatesPruneCheck at the child's entry
+                    if (size == 0) {
+                        S.fail[d - 1] = anc[d];
+                        if (!back_up()) break;
+                        continue;
+                    }
+                    S.fail[d - 1] = 0;
                 }
                 S.stack_top[d + 1] = top;
                 S.lst[d] = list;
@@ -1273,8 +1312,7 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
                 continue;
             }
             if (S.rem[d] == 0 && S.pos[d] >= S.tot[d]) {
-                d--;
-                if (d == 0) break;
+                if (!back_up()) break;
                 continue;
             }
             const uint32_t* list = S.lst[d];
@@ -1286,12 +1324,17 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
                 const bool valid = i < S.tot[d];
                 scanned += (S.tot[d] - base < 64u) ? (S.tot[d] - base) : 64u;
                 bool ok = false;
+                uint32_t c = SGM_INVALID;
                 if (valid) {
                     const uint32_t v = A.cand[d][list[i]];
-                    ok = true;
                     for (uint32_t j = 0; j < d; j++)
-                        if (S.emb[j] == v) ok = false;
+                        if (S.emb[j] == v) c = j;
+                    ok = c == SGM_INVALID;
                 }
+                // injectivity conflicts (InjectiveConflict): fail[d-1] |= anc[d] | anc[c]; a
+                // later cut-off in this loop overwrites fail[d-1], so the order does not matter
+                const uint64_t cb = __ballot(valid && !ok);
+                if (use_fs && cb) S.fail[d - 1] |= lanes_or64(cb, (valid && !ok) ? (anc[d] | anc[c]) : 0ull);
                 S.rem[d] = __ballot(ok);
                 continue;
             }
@@ -1315,6 +1358,7 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
         atomicAdd(&gst->elems_out, outs);
         atomicAdd(&gst->elems_scanned, scanned);
         atomicAdd(&gst->items_done, items_done);
+        if (fsp) atomicAdd(&gst->fs_prunes, fsp);
         if (overflow) { atomicAdd(&gst->overflow, 1ull); atomicExch(stop, 1u); }
         for (uint32_t dd = 0; dd < n; dd++)
             if (S.nodes[dd]) atomicAdd(&gst->nodes[dd], S.nodes[dd]);
@@ -1362,11 +1406,12 @@ void launch_enum_cache(hipStream_t s, uint32_t blocks, const uint32_t* off, cons
 }
 
 void launch_enum_aux(hipStream_t s, uint32_t blocks, const sgm_auxplan_dev* plan, uint32_t n_items, uint32_t* stack,
-                     uint32_t stack_words, uint32_t* work, sgm_stats_dev* st, uint32_t* stop, unsigned long long cap)
+                     uint32_t stack_words, uint32_t* work, sgm_stats_dev* st, uint32_t* stop, unsigned long long cap,
+                     int use_fs)
 {
     const size_t smem = 4 * ((sizeof(WaveAux) + 15) & ~(size_t)15);
     hipLaunchKernelGGL(k_enum_aux, dim3(blocks), dim3(256), smem, s, plan, n_items, stack, stack_words, work, st, stop,
-                       cap);
+                       cap, use_fs);
 }
 
 }  // namespace sgm

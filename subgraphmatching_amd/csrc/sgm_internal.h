@@ -45,6 +45,7 @@ struct sgm_auxplan_dev {
     const uint32_t* tab_off[SGM_MAXPREV];
     const uint32_t* tab_idx[SGM_MAXPREV];
     const uint32_t* cand[SGM_MAXQ];   // candidate list of the query vertex at each depth
+    uint64_t anc[SGM_MAXQ];           // DAF ancestor masks over depths (failing sets, inc/prune.h:100-104)
 };
 
 struct sgm_stats_dev {

@@ -56,7 +56,7 @@ def test_pruning_variants_are_rejected_at_compile_time(tmp_path):
     src = tmp_path / "bad.cpp"
     src.write_text('#include "sgm/explore.h"\n'
                    "void f(const Graph& d, const Query& q, const Order& o, const Config& c, std::atomic<bool>& s) {\n"
-                   "  double t; size_t a; explore_Intersection_All<true,false,false,false,false>(d, q, o, c, t, a, s); }\n")
+                   "  double t; size_t a; explore_Intersection_All<false,true,false,false,false>(d, q, o, c, t, a, s); }\n")
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), str(src)],
                        capture_output=True, text=True)
     assert r.returncode != 0 and "accelerated path" in r.stderr

@@ -132,12 +132,14 @@ def _compare(g, q, order, cap=None, check_work=True):
                    & ~L.SGM_ENUM_F_SPLIT_ALL)
             got = (res.elems_read, res.elems_out, res.elems_scanned)
             assert elems.setdefault(key, got) == got, (q.name, fs, flags, elems[key], got)
-    res_a = sgm.enumerate_query(can, order, method=L.SGM_ENUM_AUX_TABLES, cap=cap)
-    ref_a = O.enum_all(g, q, c_r, order, cap=cap)
-    assert res_a.ans == ref_a["count"], (q.name, "aux", res_a.ans, ref_a["count"])
-    if check_work and not ref_a["capped"]:
-        assert res_a.intersections == ref_a["intersections"], (q.name, "aux")
-        assert res_a.nodes == ref_a["nodes"], (q.name, "aux", res_a.nodes, ref_a["nodes"])
+    for fs, method in ((False, L.SGM_ENUM_AUX_TABLES), (True, L.SGM_ENUM_AUX_TABLES_FS)):
+        res_a = sgm.enumerate_query(can, order, method=method, cap=cap)
+        ref_a = O.enum_all(g, q, c_r, order, cap=cap, fs=fs)
+        assert res_a.ans == ref_a["count"], (q.name, "aux", fs, res_a.ans, ref_a["count"])
+        if check_work and not ref_a["capped"]:
+            assert res_a.intersections == ref_a["intersections"], (q.name, "aux", fs)
+            assert res_a.nodes == ref_a["nodes"], (q.name, "aux", fs, res_a.nodes, ref_a["nodes"])
+            assert res_a.fs_prunes == ref_a["fs_prunes"], (q.name, "aux", fs)
     return res
 
 
