@@ -63,8 +63,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["chunglu100k", "chunglu100k-labelled", "rmat20", "rmat20-learned",
-                                           "rmat22-serve"],
+    ap.add_argument("--workload", choices=["chunglu100k", "chunglu100k-labelled", "chunglu100k-aux", "rmat20",
+                                           "rmat20-learned", "rmat22-serve"],
                     default="chunglu100k")
     ap.add_argument("--serve-mem-gb", type=float, default=32.0,
                     help="rmat22-serve: device memory budget for query slots and DFS stacks")
@@ -136,6 +136,7 @@ def setup_chunglu(args):
 
     g = chung_lu(100_000, 2.3, 16.0, seed=2)
     labelled = args.workload == "chunglu100k-labelled"
+    aux = args.workload == "chunglu100k-aux"  # tables + explore_Intersection_All<true,...>, per query
     if labelled:  # the same CSR with skewed labels; the queries' walks carry them
         g = with_labels(g, args.labels, seed=23, skew=1.0)
     queries = query_set(g, args.queries or 200, 16, True, seed=22)
@@ -149,7 +150,18 @@ def setup_chunglu(args):
         rec["nlf_ms"], rec["gql_ms"], rec["cand_ms"] = ctx.filter_ms()
         rec["filter_ms"] = rec["nlf_ms"] + rec["gql_ms"] + rec["cand_ms"]
         t1 = time.perf_counter()
-        if args.per_query:
+        if aux:
+            orders = [sgm.Order_RI(q) for q in queries]
+            results = []
+            for can, order, good in zip(cans, orders, ok):
+                r = None
+                if good:
+                    can.build_tables()  # FilterParamConstruct's tables (inc/table.h:7-137)
+                    r = sgm.enumerate_query(can, order, L.SGM_ENUM_AUX_TABLES_FS, cap=args.cap, blocks=args.blocks)
+                    rec["enum_ms"] += r.time
+                    rec["enum_launches"] += 1
+                results.append(r)
+        elif args.per_query:
             orders = [sgm.Order_RI(q) for q in queries]
             results = []
             for can, order, good in zip(cans, orders, ok):
@@ -179,9 +191,12 @@ def setup_chunglu(args):
     name = "chunglu_V100000_g2.3_d16_seed2__200x16v_dense_rw_seed22__gql_ri_cache_filter_fs"
     if labelled:
         name = name.replace("seed2__", f"seed2_labels{args.labels}_skew1_seed23__")
+    if aux:
+        name = name.replace("gql_ri_cache_filter_fs", "gql_tables_ri_aux_fs")
     config = {"workload": name,
               "queries_per_step": len(queries), "cap_per_query": args.cap, "order": "RI",
-              "filter": "labelled NLF+GQL" if labelled else "NLF+GQL", "enumeration": "cache+filter+failing-sets",
+              "filter": "labelled NLF+GQL" if labelled else "NLF+GQL",
+              "enumeration": "aux tables + failing sets (per query)" if aux else "cache+filter+failing-sets",
               "launch": "per-query" if args.per_query else "one filter batch + one persistent enumeration launch"}
     return g, queries, step, config, {}
 
@@ -485,6 +500,7 @@ def main():
     # intersection reads |A|+|B| and writes |A n B| u32; every iterated candidate costs a
     # 4-byte id plus one filter bit
     algo_bytes = sum(4 * (r["elems_read"] + r["elems_out"]) + 4.125 * r["elems_scanned"] for r in recs)
+    enum_kernel = "k_enum_aux" if args.workload == "chunglu100k-aux" else "k_enum_cache"
     achieved = algo_bytes / (enum_ms / 1e3) / 1e9 if enum_ms > 0 else 0.0
     traffic = measured = None
     if args.workload == "chunglu100k" and os.path.exists(args.traffic):
@@ -509,7 +525,8 @@ def main():
 
     cpu = None
     if args.cpu_seconds > 0:
-        cpu = cpu_baseline(g, queries, counts0, config["cap_per_query"], args.cpu_seconds)
+        cpu = cpu_baseline(g, queries, counts0, config["cap_per_query"], args.cpu_seconds,
+                           "aux_fs" if args.workload == "chunglu100k-aux" else "cache")
 
     out = {
         "metric": METRIC,
@@ -531,14 +548,14 @@ def main():
         "capped_queries_per_step": recs[0]["capped"],
         "device_ms_per_step": {"filter": filter_ms / K, "gql_filter": sum(r["gql_ms"] for r in recs) / K,
                                "enumerate": enum_ms / K},
-        "roofline": {"kernel": "k_enum_cache", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "roofline": {"kernel": enum_kernel, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "achieved_basis": "merge-model algorithmic bytes per launch / average launch time",
                      "measured_gbs": measured,
                      "measured_frac": measured / HBM_PEAK_GBS if measured is not None else None,
                      "algorithmic_bytes_per_launch": algo_bytes / max(launches, 1),
                      "avg_launch_ms": enum_ms / max(launches, 1),
-                     "dominant_kernel": "k_enum_cache" if enum_ms >= filter_ms else "k_gql_phase"},
+                     "dominant_kernel": enum_kernel if enum_ms >= filter_ms else "k_gql_phase"},
         "cpu_baseline": cpu,
     }
     out.update(extras)
@@ -546,10 +563,10 @@ def main():
 
 
 def _cpu_worker(job):
-    """One CPU worker: GQL + RI + cache/filter/fs enumeration of its queries, one after the
+    """One CPU worker: GQL + RI + cache/filter/fs (or aux-table/fs) enumeration of its queries, one after the
     other on one core, until its wall-clock budget ends (the query running then is cut off
     by the oracle's time limit and left out)."""
-    g, items, cap, budget_s = job
+    g, items, cap, budget_s, method = job
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     t0 = time.perf_counter()
@@ -562,7 +579,10 @@ def _cpu_worker(job):
         order = O.order_ri(q)
         cnt, ms = 0, 0.0
         if ok:
-            st = O.enum_cache(g, q, c, order, cap=cap, time_limit_ms=max(1.0, 1e3 * left))
+            if method == "aux_fs":
+                st = O.enum_all(g, q, c, order, cap=cap, time_limit_ms=max(1.0, 1e3 * left), fs=True)
+            else:
+                st = O.enum_cache(g, q, c, order, cap=cap, time_limit_ms=max(1.0, 1e3 * left))
             if st["timed_out"]:
                 break
             cnt, ms = st["count"], st["enum_ms"]
@@ -570,7 +590,7 @@ def _cpu_worker(job):
     return out, time.perf_counter() - t0
 
 
-def cpu_baseline(g, queries, gpu_counts, cap, budget_s):
+def cpu_baseline(g, queries, gpu_counts, cap, budget_s, method="cache"):
     """The project's C restatement of the same path (the oracle, single-threaded per query as
     the reference runs) on a bounded sample of the same queries: P worker processes, one per
     core (P = min(16, host cores)), each taking every P-th query for budget_s seconds of
@@ -580,7 +600,7 @@ def cpu_baseline(g, queries, gpu_counts, cap, budget_s):
     import oracle as O
     O.build()
     P = max(1, min(16, os.cpu_count() or 1))
-    jobs = [(g, [(i, queries[i]) for i in range(w, len(queries), P)], cap, budget_s) for w in range(P)]
+    jobs = [(g, [(i, queries[i]) for i in range(w, len(queries), P)], cap, budget_s, method) for w in range(P)]
     if P == 1:
         results = [_cpu_worker(jobs[0])]
     else:
@@ -595,7 +615,8 @@ def cpu_baseline(g, queries, gpu_counts, cap, budget_s):
     single = sum(c for _, c, _ in r0) / w0 if w0 > 0 else 0.0
     print(f"[bench] cpu baseline: {done} queries on {P} cores in {wall:.1f}s, parity={parity}", flush=True)
     return {"value": emb / wall if wall > 0 else 0.0, "unit": "embeddings/s", "cores": P, "kind": "port",
-            "sample": f"{done} of the {len(queries)} queries (GQL + RI + cache/filter/fs enumeration, cap {cap}) "
+            "sample": f"{done} of the {len(queries)} queries (GQL + RI + "
+                      f"{'aux-table/fs' if method == 'aux_fs' else 'cache/filter/fs'} enumeration, cap {cap}) "
                       f"on {P} worker processes, one query at a time each, {wall:.1f} s wall",
             "queries_per_s": done / wall if wall > 0 else 0.0,
             "single_core_value": single, "single_core_queries_per_s": len(r0) / w0 if w0 > 0 else 0.0,

@@ -229,6 +229,10 @@ class CandidateParam:
         except Exception:
             pass
 
+    def build_tables(self):
+        """FilterParamConstruct's aux-edge tables (inc/table.h:7-137) on the device."""
+        _check(L.lib().sgm_build_tables(self._q), self.ctx)
+
     def _after_filter(self):
         # host copies (counts, lists, membership, tables) are fetched on first access
         self._filtered = True

@@ -312,20 +312,32 @@ void sgm_graph_destroy(sgm_graph* g)
 uint32_t sgm_graph_max_degree(const sgm_graph* g) { return g ? g->max_deg : 0; }
 
 // ------------------------------------------------------------------------------ query
+// Tables are views into the query's arenas: dropping them frees nothing.
 static void free_tables(sgm_query* q)
 {
-    for (auto& t : q->tables) {
-        dfree(t.d_off);
-        dfree(t.d_idx);
-        dfree(t.d_v);
-        t = sgm_table();
-    }
+    for (auto& t : q->tables) t = sgm_table();
+}
+
+// Grows a u32 arena to at least `need` words (contents are not kept).
+static hipError_t grow_arena(uint32_t** p, size_t& cap, size_t need)
+{
+    if (cap >= need) return hipSuccess;
+    dfree(*p);
+    *p = nullptr;
+    cap = 0;
+    const hipError_t e = dalloc(p, std::max<size_t>(need, 1));
+    if (e == hipSuccess) cap = need;
+    return e;
 }
 
 void sgm_query_destroy(sgm_query* q)
 {
     if (!q) return;
     free_tables(q);
+    dfree(q->d_toff);
+    dfree(q->d_tidx);
+    dfree(q->d_tv);
+    dfree(q->d_tscr);
     dfree(q->d_vmask);
     dfree(q->d_removed);
     dfree(q->d_bm);
@@ -689,42 +701,72 @@ int sgm_build_tables(sgm_query* q)
     if (rc) return rc;
     const uint32_t n = q->q.n, W = (q->g->V + 63) / 64;
     free_tables(q);
-    uint32_t* d_rowcnt = nullptr;
-    uint32_t* d_max = nullptr;
-    uint32_t maxc = 1;
-    for (uint32_t u = 0; u < n; u++) maxc = std::max(maxc, q->counts[u]);
-    HIPCHK(c, dalloc(&d_rowcnt, maxc));
-    HIPCHK(c, dalloc(&d_max, 1));
-    int out_rc = SGM_OK;
-    for (uint32_t a = 0; a < n && out_rc == SGM_OK; a++) {
+    // every directed query edge (a, b): its offsets slice and its scratch maximum
+    std::vector<std::pair<uint32_t, uint32_t>> edges;
+    size_t off_words = 0;
+    for (uint32_t a = 0; a < n; a++) {
         for (uint32_t p = q->q.off[a]; p < q->q.off[a + 1]; p++) {
-            const uint32_t b = q->q.adj[p];
-            sgm_table& t = q->tables[(size_t)a * n + b];
-            const uint32_t rows = q->counts[a];
-            hipError_t e = dalloc(&t.d_off, (size_t)rows + 1);
-            if (e != hipSuccess) { out_rc = set_err(c, SGM_ENOMEM, "table offsets"); break; }
-            sgm::launch_table(c->stream, q->g->d_off, q->g->d_adj, q->d_vmask, q->d_list + q->cand_off[a], rows, b,
-                              nullptr, nullptr, d_rowcnt, t.d_off, d_max, nullptr, nullptr, 0);
-            uint32_t tot = 0, mx = 0;
-            e = hipMemcpyAsync(&tot, t.d_off + rows, sizeof tot, hipMemcpyDeviceToHost, c->stream);
-            if (e == hipSuccess) e = hipMemcpyAsync(&mx, d_max, sizeof mx, hipMemcpyDeviceToHost, c->stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-            if (e != hipSuccess) { out_rc = set_err(c, SGM_EHIP, "table count: %s", hipGetErrorString(e)); break; }
-            t.e_cnt = tot;
-            t.max_deg = mx;
-            e = dalloc(&t.d_idx, tot);
-            if (e == hipSuccess) e = dalloc(&t.d_v, tot);
-            if (e != hipSuccess) { out_rc = set_err(c, SGM_ENOMEM, "table of %llu entries", (unsigned long long)tot); break; }
-            sgm::launch_table(c->stream, q->g->d_off, q->g->d_adj, q->d_vmask, q->d_list + q->cand_off[a], rows, b,
-                              q->d_bm + (size_t)b * W, q->d_rank + (size_t)b * W, d_rowcnt, t.d_off, d_max, t.d_idx,
-                              t.d_v, 1);
-            t.built = true;
+            edges.emplace_back(a, q->q.adj[p]);
+            off_words += (size_t)q->counts[a] + 1;
         }
     }
+    const uint32_t m = (uint32_t)edges.size();
+    if (m == 0) return SGM_OK;
+    // scratch: row counts of every edge (edge k's at obase_k - k), totals[m], maxima[m],
+    // scan descriptors (obase, rows)[m]
+    const size_t Rw = off_words - m;
+    if (Rw + 4 * (size_t)m > 0xFFFFFFFFull) return set_err(c, SGM_EINVAL, "tables too large");
+    HIPCHK(c, grow_arena(&q->d_toff, q->toff_cap, off_words));
+    HIPCHK(c, grow_arena(&q->d_tscr, q->tscr_cap, Rw + 4 * (size_t)m));
+    uint32_t* hs = static_cast<uint32_t*>(host_stage(c, sizeof(uint32_t) * 4 * m));  // descriptors, results
+    std::vector<uint32_t> h_v;
+    if (!hs) {
+        h_v.resize(4 * (size_t)m);
+        hs = h_v.data();
+    }
+    uint32_t* h = hs + 2 * m;  // totals[m], maxima[m]
+    // pass 0: row counts of every table, their offsets in one launch, one synchronisation
+    size_t o = 0;
+    for (uint32_t k = 0; k < m; k++) {
+        const uint32_t a = edges[k].first, b = edges[k].second, rows = q->counts[a];
+        sgm_table& t = q->tables[(size_t)a * n + b];
+        t.d_off = q->d_toff + o;
+        hs[2 * k] = (uint32_t)o;
+        hs[2 * k + 1] = rows;
+        sgm::launch_table_count(c->stream, q->g->d_off, q->g->d_adj, q->d_vmask, q->d_list + q->cand_off[a], rows, b,
+                                q->d_tscr + (o - k));
+        o += (size_t)rows + 1;
+    }
+    uint32_t* d_desc = q->d_tscr + Rw + 2 * (size_t)m;
+    HIPCHK(c, hipMemcpyAsync(d_desc, hs, sizeof(uint32_t) * 2 * m, hipMemcpyHostToDevice, c->stream));
+    sgm::launch_scan_segments(c->stream, q->d_tscr, (uint32_t)Rw, m, d_desc, q->d_toff);
+    HIPCHK(c, hipMemcpyAsync(h, q->d_tscr + Rw, sizeof(uint32_t) * 2 * m, hipMemcpyDeviceToHost, c->stream));
     hipError_t se = hipStreamSynchronize(c->stream);
-    dfree(d_rowcnt);
-    dfree(d_max);
-    if (out_rc) { free_tables(q); return out_rc; }
+    if (se != hipSuccess) { free_tables(q); return set_err(c, SGM_EHIP, "table count: %s", hipGetErrorString(se)); }
+    // pass 1: entries into one arena
+    size_t ent = 0;
+    for (uint32_t k = 0; k < m; k++) ent += h[k];
+    hipError_t e = grow_arena(&q->d_tidx, q->tidx_cap, ent);
+    if (e == hipSuccess) e = grow_arena(&q->d_tv, q->tv_cap, ent);
+    if (e != hipSuccess) {
+        free_tables(q);
+        return set_err(c, SGM_ENOMEM, "tables of %llu entries", (unsigned long long)ent);
+    }
+    size_t base = 0;
+    for (uint32_t k = 0; k < m; k++) {
+        const uint32_t a = edges[k].first, b = edges[k].second, rows = q->counts[a];
+        sgm_table& t = q->tables[(size_t)a * n + b];
+        t.e_cnt = h[k];
+        t.max_deg = h[m + k];
+        t.d_idx = q->d_tidx + base;
+        t.d_v = q->d_tv + base;
+        base += h[k];
+        sgm::launch_table(c->stream, q->g->d_off, q->g->d_adj, q->d_vmask, q->d_list + q->cand_off[a], rows, b,
+                          q->d_bm + (size_t)b * W, q->d_rank + (size_t)b * W, nullptr, t.d_off, nullptr, t.d_idx,
+                          t.d_v, 1);
+        t.built = true;
+    }
+    se = hipStreamSynchronize(c->stream);
     if (se != hipSuccess) { free_tables(q); return set_err(c, SGM_EHIP, "tables: %s", hipGetErrorString(se)); }
     return SGM_OK;
 }

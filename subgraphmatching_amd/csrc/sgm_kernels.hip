@@ -806,8 +806,31 @@ __global__ void __launch_bounds__(256) k_table_fill(const uint32_t* __restrict__
 }
 
 // Exclusive scan of n u32 counts into n+1 u64-safe u32 offsets (single block, looped).
+// Exclusive scan of in[0, n) into out[0, n] (out[n] = total) and the maximum into *maxv, by
+// one block of 256 threads; *tot (when set) also receives the total.
+__device__ void scan_block(const uint32_t* __restrict__ in, uint32_t n, uint32_t* __restrict__ out,
+                           uint32_t* __restrict__ maxv, uint32_t* __restrict__ tot);
+
 __global__ void __launch_bounds__(256) k_scan_u32(const uint32_t* __restrict__ in, uint32_t n,
                                                   uint32_t* __restrict__ out, uint32_t* __restrict__ maxv)
+{
+    scan_block(in, n, out, maxv, nullptr);
+}
+
+// Many scans at once, one block each: segment s scans in = scr + (obase - s), n entries, into
+// out = toff + obase (desc[2s] = obase, desc[2s+1] = n); totals to scr[R + s], maxima to
+// scr[R + m + s] (the aux-table row counts of every directed query edge).
+__global__ void __launch_bounds__(256) k_scan_segments(uint32_t* __restrict__ scr, uint32_t R, uint32_t m,
+                                                       const uint32_t* __restrict__ desc, uint32_t* __restrict__ toff)
+{
+    const uint32_t s = blockIdx.x;
+    if (s >= m) return;
+    const uint32_t obase = desc[2 * s], n = desc[2 * s + 1];
+    scan_block(scr + (obase - s), n, toff + obase, scr + R + m + s, scr + R + s);
+}
+
+__device__ void scan_block(const uint32_t* __restrict__ in, uint32_t n, uint32_t* __restrict__ out,
+                           uint32_t* __restrict__ maxv, uint32_t* __restrict__ tot)
 {
     __shared__ uint32_t wsum[4], wmax[4];
     __shared__ uint32_t carry, cmax;
@@ -838,7 +861,11 @@ __global__ void __launch_bounds__(256) k_scan_u32(const uint32_t* __restrict__ i
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) { out[n] = carry; *maxv = cmax; }
+    if (threadIdx.x == 0) {
+        out[n] = carry;
+        *maxv = cmax;
+        if (tot) *tot = carry;
+    }
 }
 
 }  // namespace sgm
@@ -977,6 +1004,18 @@ void launch_table(hipStream_t s, const uint32_t* off, const uint32_t* adj, const
         hipLaunchKernelGGL(k_table_fill, grid, dim3(256), 0, s, off, adj, vmask, rows, nrows, b, bm_b, rank_b,
                            rowoff, eidx, ev);
     }
+}
+
+void launch_table_count(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* vmask,
+                        const uint32_t* rows, uint32_t nrows, uint32_t b, uint32_t* rowcnt)
+{
+    dim3 grid(clampg(((uint64_t)nrows + 3) / 4, 1, 8192));
+    if (nrows) hipLaunchKernelGGL(k_table_count, grid, dim3(256), 0, s, off, adj, vmask, rows, nrows, b, rowcnt);
+}
+
+void launch_scan_segments(hipStream_t s, uint32_t* scr, uint32_t R, uint32_t m, const uint32_t* desc, uint32_t* toff)
+{
+    if (m) hipLaunchKernelGGL(k_scan_segments, dim3(m), dim3(256), 0, s, scr, R, m, desc, toff);
 }
 
 void launch_hub_bits(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* hubs, uint32_t n_hubs,

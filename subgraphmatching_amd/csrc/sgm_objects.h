@@ -143,7 +143,15 @@ struct sgm_query {
     sgm_cdesc* d_cdesc = nullptr;
     sgm_qdesc hq;                     // host copies of the descriptors
     sgm_cdesc hc;
-    std::vector<sgm_table> tables;    // n*n, directed (a,b)
+    std::vector<sgm_table> tables;    // n*n, directed (a,b): views into the arenas below
+    // table arenas, grown as needed and kept across filters (no allocation when the query is
+    // filtered and tabled again): offsets, entries (positions and vertex ids), and scratch
+    // (row counts, then one maximum per directed edge)
+    uint32_t* d_toff = nullptr;
+    uint32_t* d_tidx = nullptr;
+    uint32_t* d_tv = nullptr;
+    uint32_t* d_tscr = nullptr;
+    size_t toff_cap = 0, tidx_cap = 0, tv_cap = 0, tscr_cap = 0;
 };
 
 inline int set_err(sgm_ctx* c, int code, const char* fmt, ...)
