@@ -338,6 +338,7 @@ void sgm_query_destroy(sgm_query* q)
     dfree(q->d_tidx);
     dfree(q->d_tv);
     dfree(q->d_tscr);
+    dfree(q->d_tdesc);
     dfree(q->d_vmask);
     dfree(q->d_removed);
     dfree(q->d_bm);
@@ -718,25 +719,47 @@ int sgm_build_tables(sgm_query* q)
     if (Rw + 4 * (size_t)m > 0xFFFFFFFFull) return set_err(c, SGM_EINVAL, "tables too large");
     HIPCHK(c, grow_arena(&q->d_toff, q->toff_cap, off_words));
     HIPCHK(c, grow_arena(&q->d_tscr, q->tscr_cap, Rw + 4 * (size_t)m));
-    uint32_t* hs = static_cast<uint32_t*>(host_stage(c, sizeof(uint32_t) * 4 * m));  // descriptors, results
+    if (q->tdesc_cap < m) {
+        dfree(q->d_tdesc);
+        q->d_tdesc = nullptr;
+        q->tdesc_cap = 0;
+        HIPCHK(c, dalloc(&q->d_tdesc, m));
+        q->tdesc_cap = m;
+    }
+    // pinned: scan descriptors and results (4m words), then the edge descriptors
+    const size_t hs_bytes = stage_align(sizeof(uint32_t) * 4 * m);
+    uint8_t* stage = static_cast<uint8_t*>(host_stage(c, hs_bytes + sizeof(sgm_tabdesc) * m));
     std::vector<uint32_t> h_v;
-    if (!hs) {
+    std::vector<sgm_tabdesc> td_v;
+    uint32_t* hs;
+    sgm_tabdesc* td;
+    if (stage) {
+        hs = reinterpret_cast<uint32_t*>(stage);
+        td = reinterpret_cast<sgm_tabdesc*>(stage + hs_bytes);
+    } else {
         h_v.resize(4 * (size_t)m);
+        td_v.resize(m);
         hs = h_v.data();
+        td = td_v.data();
     }
     uint32_t* h = hs + 2 * m;  // totals[m], maxima[m]
-    // pass 0: row counts of every table, their offsets in one launch, one synchronisation
+    // pass 0: row counts of every table (one launch), their offsets (one launch), one
+    // synchronisation
     size_t o = 0;
+    uint32_t max_rows = 0;
     for (uint32_t k = 0; k < m; k++) {
         const uint32_t a = edges[k].first, b = edges[k].second, rows = q->counts[a];
         sgm_table& t = q->tables[(size_t)a * n + b];
         t.d_off = q->d_toff + o;
         hs[2 * k] = (uint32_t)o;
         hs[2 * k + 1] = rows;
-        sgm::launch_table_count(c->stream, q->g->d_off, q->g->d_adj, q->d_vmask, q->d_list + q->cand_off[a], rows, b,
-                                q->d_tscr + (o - k));
+        td[k] = sgm_tabdesc{q->d_list + q->cand_off[a], rows, b, q->d_tscr + (o - k), q->d_bm + (size_t)b * W,
+                            q->d_rank + (size_t)b * W, t.d_off, nullptr, nullptr};
+        max_rows = std::max(max_rows, rows);
         o += (size_t)rows + 1;
     }
+    HIPCHK(c, hipMemcpyAsync(q->d_tdesc, td, sizeof(sgm_tabdesc) * m, hipMemcpyHostToDevice, c->stream));
+    sgm::launch_table_multi(c->stream, q->g->d_off, q->g->d_adj, q->d_vmask, q->d_tdesc, m, max_rows, 0);
     uint32_t* d_desc = q->d_tscr + Rw + 2 * (size_t)m;
     HIPCHK(c, hipMemcpyAsync(d_desc, hs, sizeof(uint32_t) * 2 * m, hipMemcpyHostToDevice, c->stream));
     sgm::launch_scan_segments(c->stream, q->d_tscr, (uint32_t)Rw, m, d_desc, q->d_toff);
@@ -754,18 +777,20 @@ int sgm_build_tables(sgm_query* q)
     }
     size_t base = 0;
     for (uint32_t k = 0; k < m; k++) {
-        const uint32_t a = edges[k].first, b = edges[k].second, rows = q->counts[a];
+        const uint32_t a = edges[k].first, b = edges[k].second;
         sgm_table& t = q->tables[(size_t)a * n + b];
         t.e_cnt = h[k];
         t.max_deg = h[m + k];
         t.d_idx = q->d_tidx + base;
         t.d_v = q->d_tv + base;
         base += h[k];
-        sgm::launch_table(c->stream, q->g->d_off, q->g->d_adj, q->d_vmask, q->d_list + q->cand_off[a], rows, b,
-                          q->d_bm + (size_t)b * W, q->d_rank + (size_t)b * W, nullptr, t.d_off, nullptr, t.d_idx,
-                          t.d_v, 1);
+        td[k].eidx = t.d_idx;
+        td[k].ev = t.d_v;
         t.built = true;
     }
+    // pass 1: every table's entries in one launch
+    HIPCHK(c, hipMemcpyAsync(q->d_tdesc, td, sizeof(sgm_tabdesc) * m, hipMemcpyHostToDevice, c->stream));
+    sgm::launch_table_multi(c->stream, q->g->d_off, q->g->d_adj, q->d_vmask, q->d_tdesc, m, max_rows, 1);
     se = hipStreamSynchronize(c->stream);
     if (se != hipSuccess) { free_tables(q); return set_err(c, SGM_EHIP, "tables: %s", hipGetErrorString(se)); }
     return SGM_OK;

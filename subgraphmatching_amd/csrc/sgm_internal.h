@@ -48,6 +48,19 @@ struct sgm_auxplan_dev {
     uint64_t anc[SGM_MAXQ];           // DAF ancestor masks over depths (failing sets, inc/prune.h:100-104)
 };
 
+// One directed query edge (a, b) of an aux-table build (inc/table.h:7-137): rows = C(a);
+// the count pass writes rowcnt, the fill pass writes positions in C(b) and vertex ids.
+struct sgm_tabdesc {
+    const uint32_t* rows;
+    uint32_t nrows, b;
+    uint32_t* rowcnt;
+    const uint64_t* bm_b;
+    const uint32_t* rank_b;
+    const uint32_t* rowoff;
+    uint32_t* eidx;
+    uint32_t* ev;
+};
+
 struct sgm_stats_dev {
     unsigned long long count;
     unsigned long long intersections;

@@ -55,6 +55,9 @@ void launch_compact(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V)
 void launch_table_count(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* vmask,
                         const uint32_t* rows, uint32_t nrows, uint32_t b, uint32_t* rowcnt);
 void launch_scan_segments(hipStream_t s, uint32_t* scr, uint32_t R, uint32_t m, const uint32_t* desc, uint32_t* toff);
+// every directed edge of a query in one launch: pass 0 row counts, pass 1 entries
+void launch_table_multi(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* vmask,
+                        const sgm_tabdesc* ds, uint32_t m, uint32_t max_rows, int pass);
 void launch_table(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* vmask,
                   const uint32_t* rows, uint32_t nrows, uint32_t b, const uint64_t* bm_b, const uint32_t* rank_b,
                   uint32_t* rowcnt, uint32_t* rowoff, uint32_t* maxv, uint32_t* eidx, uint32_t* ev, int pass);

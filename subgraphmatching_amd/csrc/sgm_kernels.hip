@@ -777,6 +777,53 @@ __global__ void __launch_bounds__(256) k_table_count(const uint32_t* __restrict_
     }
 }
 
+// Every directed edge of a query in one launch (blockIdx.y = edge): the same work as
+// k_table_count / k_table_fill.
+__global__ void __launch_bounds__(256) k_table_count_multi(const uint32_t* __restrict__ off,
+                                                           const uint32_t* __restrict__ adj,
+                                                           const uint32_t* __restrict__ vmask,
+                                                           const sgm_tabdesc* __restrict__ ds)
+{
+    const sgm_tabdesc& D = ds[blockIdx.y];
+    const uint32_t waves = blockDim.x >> 6, lane = lane_id();
+    const uint32_t* __restrict__ rows = D.rows;
+    const uint32_t nrows = D.nrows, b = D.b;
+    for (uint32_t i = blockIdx.x * waves + (threadIdx.x >> 6); i < nrows; i += gridDim.x * waves) {
+        const uint32_t v = rows[i], o = off[v], dv = off[v + 1] - o;
+        uint32_t c = 0;
+        for (uint32_t j = lane; j < dv; j += 64) c += (vmask[adj[o + j]] >> b) & 1u;
+        for (int s = 32; s > 0; s >>= 1) c += (uint32_t)__shfl_xor((int)c, s);
+        if (lane == 0) D.rowcnt[i] = c;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_table_fill_multi(const uint32_t* __restrict__ off,
+                                                          const uint32_t* __restrict__ adj,
+                                                          const uint32_t* __restrict__ vmask,
+                                                          const sgm_tabdesc* __restrict__ ds)
+{
+    const sgm_tabdesc& D = ds[blockIdx.y];
+    const uint32_t waves = blockDim.x >> 6, lane = lane_id();
+    const uint32_t* __restrict__ rows = D.rows;
+    const uint32_t nrows = D.nrows, b = D.b;
+    for (uint32_t i = blockIdx.x * waves + (threadIdx.x >> 6); i < nrows; i += gridDim.x * waves) {
+        const uint32_t v = rows[i], o = off[v], dv = off[v + 1] - o;
+        uint32_t outp = D.rowoff[i];
+        for (uint32_t base = 0; base < dv; base += 64) {
+            const uint32_t j = base + lane;
+            const uint32_t w = j < dv ? adj[o + j] : 0u;
+            const bool hit = j < dv && ((vmask[w] >> b) & 1u);
+            const uint64_t bal = __ballot(hit);
+            if (hit) {
+                const uint32_t p = outp + mbcnt64(bal);
+                D.eidx[p] = cand_rank(D.bm_b, D.rank_b, w);
+                D.ev[p] = w;
+            }
+            outp += (uint32_t)__popcll(bal);
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256) k_table_fill(const uint32_t* __restrict__ off,
                                                     const uint32_t* __restrict__ adj,
                                                     const uint32_t* __restrict__ vmask,
@@ -1011,6 +1058,16 @@ void launch_table_count(hipStream_t s, const uint32_t* off, const uint32_t* adj,
 {
     dim3 grid(clampg(((uint64_t)nrows + 3) / 4, 1, 8192));
     if (nrows) hipLaunchKernelGGL(k_table_count, grid, dim3(256), 0, s, off, adj, vmask, rows, nrows, b, rowcnt);
+}
+
+void launch_table_multi(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* vmask,
+                        const sgm_tabdesc* ds, uint32_t m, uint32_t max_rows, int pass)
+{
+    if (!m || !max_rows) return;
+    const uint32_t xcap = 16384 / m > 8 ? 16384 / m : 8u;
+    dim3 grid(clampg(((uint64_t)max_rows + 3) / 4, 1, xcap), m);
+    if (pass == 0) hipLaunchKernelGGL(k_table_count_multi, grid, dim3(256), 0, s, off, adj, vmask, ds);
+    else hipLaunchKernelGGL(k_table_fill_multi, grid, dim3(256), 0, s, off, adj, vmask, ds);
 }
 
 void launch_scan_segments(hipStream_t s, uint32_t* scr, uint32_t R, uint32_t m, const uint32_t* desc, uint32_t* toff)

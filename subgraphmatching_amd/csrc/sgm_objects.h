@@ -151,7 +151,8 @@ struct sgm_query {
     uint32_t* d_tidx = nullptr;
     uint32_t* d_tv = nullptr;
     uint32_t* d_tscr = nullptr;
-    size_t toff_cap = 0, tidx_cap = 0, tv_cap = 0, tscr_cap = 0;
+    sgm_tabdesc* d_tdesc = nullptr;  // per directed edge
+    size_t toff_cap = 0, tidx_cap = 0, tv_cap = 0, tscr_cap = 0, tdesc_cap = 0;
 };
 
 inline int set_err(sgm_ctx* c, int code, const char* fmt, ...)
