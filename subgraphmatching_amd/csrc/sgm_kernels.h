@@ -51,16 +51,11 @@ void launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t val);
 void launch_bitmaps(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V);
 void launch_rank_scan(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V, uint32_t max_n);
 void launch_compact(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V);
-// aux tables of many directed edges: row counts of one edge, then all offsets in one launch
-void launch_table_count(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* vmask,
-                        const uint32_t* rows, uint32_t nrows, uint32_t b, uint32_t* rowcnt);
+// aux tables: the row offsets of every directed edge in one launch
 void launch_scan_segments(hipStream_t s, uint32_t* scr, uint32_t R, uint32_t m, const uint32_t* desc, uint32_t* toff);
 // every directed edge of a query in one launch: pass 0 row counts, pass 1 entries
 void launch_table_multi(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* vmask,
                         const sgm_tabdesc* ds, uint32_t m, uint32_t max_rows, int pass);
-void launch_table(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* vmask,
-                  const uint32_t* rows, uint32_t nrows, uint32_t b, const uint64_t* bm_b, const uint32_t* rank_b,
-                  uint32_t* rowcnt, uint32_t* rowoff, uint32_t* maxv, uint32_t* eidx, uint32_t* ev, int pass);
 // sgm_enum.hip
 size_t enum_smem_bytes(uint32_t max_ops);
 void launch_enum_aux(hipStream_t s, uint32_t blocks, const sgm_auxplan_dev* plan, uint32_t n_items, uint32_t* stack,

@@ -751,8 +751,9 @@ __global__ void __launch_bounds__(256) k_compact(const sgm_cdesc* __restrict__ c
 // ================================================================== aux tables
 // Table of directed query edge (a,b) (inc/table.h:7-137): for the i-th candidate of a, the
 // ascending positions k in C(b) of its data neighbours that are candidates of b.
-// Pass 1 counts per row, pass 2 fills (positions are ranks in b's bitmap, so rows come out
-// sorted because adjacency lists are).  One wave per row, coalesced over N(v).
+// A count pass per row, an exclusive scan into row offsets, then a fill pass (positions are
+// ranks in b's bitmap, so rows come out sorted because adjacency lists are).  One wave per
+// row, coalesced over N(v); every directed edge of a query in one launch per pass.
 __device__ __forceinline__ uint32_t cand_rank(const uint64_t* __restrict__ bm, const uint32_t* __restrict__ rank,
                                               uint32_t w)
 {
@@ -761,24 +762,7 @@ __device__ __forceinline__ uint32_t cand_rank(const uint64_t* __restrict__ bm, c
     return rank[word] + (uint32_t)__popcll(low);
 }
 
-__global__ void __launch_bounds__(256) k_table_count(const uint32_t* __restrict__ off,
-                                                     const uint32_t* __restrict__ adj,
-                                                     const uint32_t* __restrict__ vmask,
-                                                     const uint32_t* __restrict__ rows, uint32_t nrows,
-                                                     uint32_t b, uint32_t* __restrict__ rowcnt)
-{
-    const uint32_t waves = blockDim.x >> 6, lane = lane_id();
-    for (uint32_t i = blockIdx.x * waves + (threadIdx.x >> 6); i < nrows; i += gridDim.x * waves) {
-        const uint32_t v = rows[i], o = off[v], dv = off[v + 1] - o;
-        uint32_t c = 0;
-        for (uint32_t j = lane; j < dv; j += 64) c += (vmask[adj[o + j]] >> b) & 1u;
-        for (int s = 32; s > 0; s >>= 1) c += (uint32_t)__shfl_xor((int)c, s);
-        if (lane == 0) rowcnt[i] = c;
-    }
-}
-
-// Every directed edge of a query in one launch (blockIdx.y = edge): the same work as
-// k_table_count / k_table_fill.
+// row counts of every directed edge (blockIdx.y = edge)
 __global__ void __launch_bounds__(256) k_table_count_multi(const uint32_t* __restrict__ off,
                                                            const uint32_t* __restrict__ adj,
                                                            const uint32_t* __restrict__ vmask,
@@ -797,6 +781,7 @@ __global__ void __launch_bounds__(256) k_table_count_multi(const uint32_t* __res
     }
 }
 
+// entries of every directed edge at the scanned row offsets
 __global__ void __launch_bounds__(256) k_table_fill_multi(const uint32_t* __restrict__ off,
                                                           const uint32_t* __restrict__ adj,
                                                           const uint32_t* __restrict__ vmask,
@@ -824,45 +809,11 @@ __global__ void __launch_bounds__(256) k_table_fill_multi(const uint32_t* __rest
     }
 }
 
-__global__ void __launch_bounds__(256) k_table_fill(const uint32_t* __restrict__ off,
-                                                    const uint32_t* __restrict__ adj,
-                                                    const uint32_t* __restrict__ vmask,
-                                                    const uint32_t* __restrict__ rows, uint32_t nrows,
-                                                    uint32_t b, const uint64_t* __restrict__ bm_b,
-                                                    const uint32_t* __restrict__ rank_b,
-                                                    const uint32_t* __restrict__ rowoff,
-                                                    uint32_t* __restrict__ eidx, uint32_t* __restrict__ ev)
-{
-    const uint32_t waves = blockDim.x >> 6, lane = lane_id();
-    for (uint32_t i = blockIdx.x * waves + (threadIdx.x >> 6); i < nrows; i += gridDim.x * waves) {
-        const uint32_t v = rows[i], o = off[v], dv = off[v + 1] - o;
-        uint32_t outp = rowoff[i];
-        for (uint32_t base = 0; base < dv; base += 64) {
-            const uint32_t j = base + lane;
-            const uint32_t w = j < dv ? adj[o + j] : 0u;
-            const bool hit = j < dv && ((vmask[w] >> b) & 1u);
-            const uint64_t bal = __ballot(hit);
-            if (hit) {
-                const uint32_t p = outp + mbcnt64(bal);
-                eidx[p] = cand_rank(bm_b, rank_b, w);
-                ev[p] = w;
-            }
-            outp += (uint32_t)__popcll(bal);
-        }
-    }
-}
-
 // Exclusive scan of n u32 counts into n+1 u64-safe u32 offsets (single block, looped).
 // Exclusive scan of in[0, n) into out[0, n] (out[n] = total) and the maximum into *maxv, by
 // one block of 256 threads; *tot (when set) also receives the total.
 __device__ void scan_block(const uint32_t* __restrict__ in, uint32_t n, uint32_t* __restrict__ out,
                            uint32_t* __restrict__ maxv, uint32_t* __restrict__ tot);
-
-__global__ void __launch_bounds__(256) k_scan_u32(const uint32_t* __restrict__ in, uint32_t n,
-                                                  uint32_t* __restrict__ out, uint32_t* __restrict__ maxv)
-{
-    scan_block(in, n, out, maxv, nullptr);
-}
 
 // Many scans at once, one block each: segment s scans in = scr + (obase - s), n entries, into
 // out = toff + obase (desc[2s] = obase, desc[2s+1] = n); totals to scr[R + s], maxima to
@@ -1037,27 +988,6 @@ void launch_compact(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V)
     const uint32_t W = (V + 63) / 64;
     dim3 grid(clampg((W + 3) / 4, 1, 4096), nq);
     hipLaunchKernelGGL(k_compact, grid, dim3(256), 0, s, cs, V);
-}
-
-void launch_table(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* vmask,
-                  const uint32_t* rows, uint32_t nrows, uint32_t b, const uint64_t* bm_b, const uint32_t* rank_b,
-                  uint32_t* rowcnt, uint32_t* rowoff, uint32_t* maxv, uint32_t* eidx, uint32_t* ev, int pass)
-{
-    dim3 grid(clampg(((uint64_t)nrows + 3) / 4, 1, 8192));
-    if (pass == 0) {
-        hipLaunchKernelGGL(k_table_count, grid, dim3(256), 0, s, off, adj, vmask, rows, nrows, b, rowcnt);
-        hipLaunchKernelGGL(k_scan_u32, dim3(1), dim3(256), 0, s, rowcnt, nrows, rowoff, maxv);
-    } else {
-        hipLaunchKernelGGL(k_table_fill, grid, dim3(256), 0, s, off, adj, vmask, rows, nrows, b, bm_b, rank_b,
-                           rowoff, eidx, ev);
-    }
-}
-
-void launch_table_count(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* vmask,
-                        const uint32_t* rows, uint32_t nrows, uint32_t b, uint32_t* rowcnt)
-{
-    dim3 grid(clampg(((uint64_t)nrows + 3) / 4, 1, 8192));
-    if (nrows) hipLaunchKernelGGL(k_table_count, grid, dim3(256), 0, s, off, adj, vmask, rows, nrows, b, rowcnt);
 }
 
 void launch_table_multi(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* vmask,
