@@ -151,16 +151,18 @@ def setup_chunglu(args):
         rec["filter_ms"] = rec["nlf_ms"] + rec["gql_ms"] + rec["cand_ms"]
         t1 = time.perf_counter()
         if aux:
-            orders = [sgm.Order_RI(q) for q in queries]
-            results = []
-            for can, order, good in zip(cans, orders, ok):
-                r = None
-                if good:
-                    can.build_tables()  # FilterParamConstruct's tables (inc/table.h:7-137)
-                    r = sgm.enumerate_query(can, order, L.SGM_ENUM_AUX_TABLES_FS, cap=args.cap, blocks=args.blocks)
-                    rec["enum_ms"] += r.time
-                    rec["enum_launches"] += 1
-                results.append(r)
+            live = [i for i, good in enumerate(ok) if good]
+            for i in live:
+                cans[i].build_tables()  # FilterParamConstruct's tables (inc/table.h:7-137)
+            # RI orders computed by the library; one launch over every query
+            batch = sgm.enumerate_batch([cans[i] for i in live], None, L.SGM_ENUM_AUX_TABLES_FS, cap=args.cap,
+                                        blocks=args.blocks)
+            results = [None] * len(queries)
+            for i, r in zip(live, batch):
+                results[i] = r
+            if batch:
+                rec["enum_ms"] += batch[0].time
+                rec["enum_launches"] += 1
         elif args.per_query:
             orders = [sgm.Order_RI(q) for q in queries]
             results = []
@@ -196,7 +198,7 @@ def setup_chunglu(args):
     config = {"workload": name,
               "queries_per_step": len(queries), "cap_per_query": args.cap, "order": "RI",
               "filter": "labelled NLF+GQL" if labelled else "NLF+GQL",
-              "enumeration": "aux tables + failing sets (per query)" if aux else "cache+filter+failing-sets",
+              "enumeration": "aux tables + failing sets" if aux else "cache+filter+failing-sets",
               "launch": "per-query" if args.per_query else "one filter batch + one persistent enumeration launch"}
     return g, queries, step, config, {}
 

@@ -74,6 +74,10 @@ void sgm_ctx_destroy(sgm_ctx* c)
     dfree(c->d_work);
     dfree(c->d_stats);
     dfree(c->d_auxplan);
+    dfree(c->d_auxplans);
+    dfree(c->d_afirst);
+    dfree(c->d_astats);
+    dfree(c->d_astop);
     dfree(c->d_work64);
     dfree(c->d_plans);
     dfree(c->d_eq);
@@ -978,86 +982,119 @@ void stats_out(const sgm_stats_dev& st, uint64_t cap, sgm_stats* out, double ns_
 
 extern "C" {
 
-// explore_Intersection_All<fs,false,false,false,false> for one query over its device tables.
-static int enumerate_aux(sgm_query* q, const uint32_t* order, const sgm_enum_opts& o, sgm_stats* out)
+// explore_Intersection_All<fs,false,false,false,false> for many queries in one launch over
+// their device tables (built here when missing); orders[i] is query i's order.
+static int enumerate_aux_batch(sgm_query* const* qs, const uint32_t* const* orders, uint32_t nq,
+                               const sgm_enum_opts& o, sgm_stats* out)
 {
-    sgm_graph* g = q->g;
+    sgm_graph* g = qs[0]->g;
     sgm_ctx* c = g->ctx;
-    const uint32_t n = q->q.n;
-    bool have = true;
-    for (uint32_t a = 0; a < n && have; a++)
-        for (uint32_t p = q->q.off[a]; p < q->q.off[a + 1]; p++) have &= q->tables[(size_t)a * n + q->q.adj[p]].built;
-    if (!have) {
-        int trc = sgm_build_tables(q);
-        if (trc) return trc;
-    }
-    std::vector<std::vector<uint32_t>> prev;
-    sgm::previous_neighbours(q->q, order, prev);
-    std::vector<uint32_t> pos(n);
-    for (uint32_t i = 0; i < n; i++) pos[order[i]] = i;
-    sgm_auxplan_dev ap;
-    std::memset(&ap, 0, sizeof ap);
-    ap.n = n;
-    uint32_t e = 0;
-    for (uint32_t d = 0; d < n; d++) {
-        const uint32_t u = order[d];
-        ap.order[d] = u;
-        ap.prev_first[d] = e;
-        ap.cand[d] = q->d_list + q->cand_off[u];
-        for (uint32_t p : prev[u]) {
-            const sgm_table& t = q->tables[(size_t)p * n + u];
-            ap.prev_depth[e] = pos[p];
-            ap.tab_off[e] = t.d_off;
-            ap.tab_idx[e] = t.d_idx;
-            e++;
+    std::vector<sgm_auxplan_dev> aps(nq);
+    for (uint32_t i = 0; i < nq; i++) {
+        sgm_query* q = qs[i];
+        const uint32_t* order = orders[i];
+        const uint32_t n = q->q.n;
+        bool have = true;
+        for (uint32_t a = 0; a < n && have; a++)
+            for (uint32_t p = q->q.off[a]; p < q->q.off[a + 1]; p++)
+                have &= q->tables[(size_t)a * n + q->q.adj[p]].built;
+        if (!have && !q->empty) {
+            const int trc = sgm_build_tables(q);
+            if (trc) return trc;
+        }
+        std::vector<std::vector<uint32_t>> prev;
+        sgm::previous_neighbours(q->q, order, prev);
+        std::vector<uint32_t> pos(n);
+        for (uint32_t d = 0; d < n; d++) pos[order[d]] = d;
+        sgm_auxplan_dev& ap = aps[i];
+        std::memset(&ap, 0, sizeof ap);
+        ap.n = n;
+        uint32_t e = 0;
+        for (uint32_t d = 0; d < n; d++) {
+            const uint32_t u = order[d];
+            ap.order[d] = u;
+            ap.prev_first[d] = e;
+            ap.cand[d] = q->d_list ? q->d_list + q->cand_off[u] : nullptr;
+            for (uint32_t p : prev[u]) {
+                const sgm_table& t = q->tables[(size_t)p * n + u];
+                ap.prev_depth[e] = pos[p];
+                ap.tab_off[e] = t.d_off;
+                ap.tab_idx[e] = t.d_idx;
+                e++;
+            }
+        }
+        ap.prev_first[n] = e;
+        // DAF ancestors over depths (inc/prune.h:100-104): anc[d] = {d} and the ancestors of
+        // every backward neighbour
+        for (uint32_t d = 0; d < n; d++) {
+            uint64_t a = 1ull << d;
+            for (uint32_t p : prev[order[d]]) a |= ap.anc[pos[p]];
+            ap.anc[d] = a;
         }
     }
-    ap.prev_first[n] = e;
-    // DAF ancestors over depths (inc/prune.h:100-104): anc[i] = {i} and the ancestors of every
-    // backward neighbour
-    for (uint32_t d = 0; d < n; d++) {
-        uint64_t a = 1ull << d;
-        for (uint32_t p : prev[order[d]]) a |= ap.anc[pos[p]];
-        ap.anc[d] = a;
-    }
     const int use_fs = o.method == SGM_ENUM_AUX_TABLES_FS ? 1 : 0;
-    HIPCHK(c, hipMemcpyAsync(c->d_auxplan, &ap, sizeof ap, hipMemcpyHostToDevice, c->stream));
-    const uint32_t n_items = q->counts[order[0]];
-    const uint32_t blocks = default_blocks(c, o, 1);
+    std::vector<uint32_t> live;
+    for (uint32_t i = 0; i < nq; i++)
+        if (!qs[i]->empty) live.push_back(i);  // the reference's driver reports 0 without enumerating
+    const uint32_t blocks = default_blocks(c, o, (uint32_t)std::max<size_t>(live.size(), 1));
     const uint32_t waves = blocks * 4;
     uint64_t words = o.stack_words ? o.stack_words : (uint64_t)8 * g->max_deg + 1024;
-    for (uint32_t attempt = 0;; attempt++) {
+    for (uint32_t attempt = 0; !live.empty(); attempt++) {
+        if (attempt > 3) return set_err(c, SGM_EDEVICE, "set-op stack overflow after %u retries", attempt - 1);
         if (words > 0x7FFFFFFFull) return set_err(c, SGM_EDEVICE, "set-op stack would exceed 2^31 words");
         int rc = ensure_stack(c, (size_t)waves * words * sizeof(uint32_t));
         if (rc) return rc;
-        HIPCHK(c, hipMemsetAsync(c->d_stats, 0, sizeof(sgm_stats_dev), c->stream));
+        const uint32_t m = (uint32_t)live.size();
+        if ((rc = grow(c, &c->d_auxplans, c->aux_cap, m))) return rc;
+        if ((rc = grow(c, &c->d_afirst, c->afirst_cap, m + 1))) return rc;
+        if ((rc = grow(c, &c->d_astats, c->astats_cap, m))) return rc;
+        if ((rc = grow(c, &c->d_astop, c->astop_cap, m))) return rc;
+        std::vector<sgm_auxplan_dev> sub(m);
+        std::vector<uint32_t> first(m + 1, 0);
+        for (uint32_t k = 0; k < m; k++) {
+            const uint32_t i = live[k];
+            sub[k] = aps[i];
+            first[k + 1] = first[k] + qs[i]->counts[orders[i][0]];
+        }
+        HIPCHK(c, hipMemcpyAsync(c->d_auxplans, sub.data(), sizeof(sgm_auxplan_dev) * m, hipMemcpyHostToDevice,
+                                 c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->d_afirst, first.data(), sizeof(uint32_t) * (m + 1), hipMemcpyHostToDevice,
+                                 c->stream));
+        HIPCHK(c, hipMemsetAsync(c->d_astats, 0, sizeof(sgm_stats_dev) * m, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->d_astop, 0, sizeof(uint32_t) * m, c->stream));
         HIPCHK(c, hipMemsetAsync(c->d_work, 0, sizeof(uint32_t), c->stream));
         HIPCHK(c, hipMemsetAsync(c->d_stop, 0, sizeof(uint32_t), c->stream));
         c->stop_requested = 0;
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-        sgm::launch_enum_aux(c->stream, blocks, c->d_auxplan, n_items, c->d_stack, (uint32_t)words, c->d_work,
-                             c->d_stats, c->d_stop, (unsigned long long)o.cap, use_fs);
+        sgm::launch_enum_aux(c->stream, blocks, c->d_auxplans, c->d_afirst, m, c->d_stack, (uint32_t)words, c->d_work,
+                             c->d_astats, c->d_astop, c->d_stop, (unsigned long long)o.cap, use_fs);
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-        sgm_stats_dev st;
-        HIPCHK(c, hipMemcpyAsync(&st, c->d_stats, sizeof st, hipMemcpyDeviceToHost, c->stream));
+        std::vector<sgm_stats_dev> got(m);
+        HIPCHK(c, hipMemcpyAsync(got.data(), c->d_astats, sizeof(sgm_stats_dev) * m, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         float ms = 0;
         HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-        out->kernel_ms += ms;
-        if (st.overflow) {
-            words *= 4;
-            out->retries++;
-            if (attempt >= 3) return set_err(c, SGM_EDEVICE, "set-op stack overflow after %u retries", attempt);
-            continue;
+        std::vector<uint32_t> again;
+        for (uint32_t k = 0; k < m; k++) {
+            const uint32_t i = live[k];
+            out[i].kernel_ms += ms;
+            if (got[k].overflow) {
+                out[i].retries++;
+                again.push_back(i);
+                continue;
+            }
+            stats_out(got[k], o.cap, &out[i], c->wall_ns_per_tick);
+            if (c->stop_requested && !out[i].capped) out[i].stopped = 1;
         }
-        stats_out(st, o.cap, out, c->wall_ns_per_tick);
-        if (c->stop_requested && !out->capped) {
-            out->stopped = 1;
-            return set_err(c, SGM_ESTOPPED, "stopped by sgm_stop");
-        }
-        return SGM_OK;
+        live.swap(again);
+        words *= 4;
     }
+    if (c->stop_requested) {
+        for (uint32_t i = 0; i < nq; i++)
+            if (out[i].stopped) return set_err(c, SGM_ESTOPPED, "stopped by sgm_stop");
+    }
+    return SGM_OK;
 }
 
 int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uint32_t nq, const sgm_enum_opts* opts,
@@ -1069,9 +1106,27 @@ int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uin
     sgm_graph* g = qs[0]->g;
     sgm_ctx* c = g->ctx;
     const sgm_enum_opts o = normalize_opts(opts);
-    if (o.method != SGM_ENUM_CACHE_FILTER_FS && o.method != SGM_ENUM_CACHE_FILTER)
-        return set_err(c, SGM_EINVAL, "batched enumeration supports the cache + filter methods (got %d)", o.method);
     std::memset(out, 0, sizeof(sgm_stats) * nq);
+    if (o.method == SGM_ENUM_AUX_TABLES || o.method == SGM_ENUM_AUX_TABLES_FS) {
+        // the aux-table search over many queries: orders (given or RI) checked like plans
+        std::vector<std::array<uint32_t, SGM_MAXQ>> ords(nq);
+        std::vector<const uint32_t*> optr(nq);
+        for (uint32_t i = 0; i < nq; i++) {
+            sgm_query* q = qs[i];
+            if (q->g != g) return set_err(c, SGM_EINVAL, "queries of a batch must share one graph");
+            if (!q->filtered) return set_err(c, SGM_ESTATE, "filter query %u first", i);
+            if (q->q.n < 2) return set_err(c, SGM_EINVAL, "enumeration needs at least 2 query vertices (query %u)", i);
+            if (orders && orders[i]) std::copy(orders[i], orders[i] + q->q.n, ords[i].begin());
+            else sgm::order_ri(q->q, ords[i].data());
+            if (sgm::check_order(q->q, ords[i].data()))
+                return set_err(c, SGM_EORDER, "order of query %u is not a connected permutation", i);
+            optr[i] = ords[i].data();
+        }
+        HIPCHK(c, hipSetDevice(c->device));
+        return enumerate_aux_batch(qs, optr.data(), nq, o, out);
+    }
+    if (o.method != SGM_ENUM_CACHE_FILTER_FS && o.method != SGM_ENUM_CACHE_FILTER)
+        return set_err(c, SGM_EINVAL, "unknown enumeration method %d", o.method);
     // plans (host), validated before anything runs; plans, query descriptors and statistics
     // live in pinned staging (plain vectors if none can be had)
     const size_t pl_bytes = stage_align(sizeof(sgm_plan_dev) * nq);
@@ -1254,7 +1309,7 @@ int sgm_enumerate(sgm_query* q, const uint32_t* order, const sgm_enum_opts* opts
         if (prc) return set_err(c, SGM_EINVAL, "plan construction failed (%d)", prc);
         if (q->empty) return SGM_OK;
         HIPCHK(c, hipSetDevice(c->device));
-        return enumerate_aux(q, order, o, out);
+        return enumerate_aux_batch(&q, &order, 1, o, out);
     }
     if (o.method != SGM_ENUM_CACHE_FILTER_FS && o.method != SGM_ENUM_CACHE_FILTER)
         return set_err(c, SGM_EINVAL, "unknown enumeration method %d", o.method);

@@ -1192,34 +1192,80 @@ __device__ __forceinline__ uint32_t aux_count_chunk(bool valid, uint32_t vidx, c
     return (uint32_t)__popcll(__ballot(ok));
 }
 
-__global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restrict__ gA, uint32_t n_items,
+// Many queries per launch: items are the depth-0 candidates of every query, query q owning
+// [item_first[q], item_first[q+1]); plans, statistics and stop words are per query.
+__global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restrict__ plans,
+                                                  const uint32_t* __restrict__ item_first, uint32_t nq,
                                                   uint32_t* __restrict__ stack, uint32_t stack_words,
-                                                  uint32_t* __restrict__ work, sgm_stats_dev* __restrict__ gst,
-                                                  uint32_t* __restrict__ stop, unsigned long long cap, int use_fs)
+                                                  uint32_t* __restrict__ work, sgm_stats_dev* __restrict__ stats,
+                                                  uint32_t* __restrict__ stops, const uint32_t* __restrict__ host_stop,
+                                                  unsigned long long cap, int use_fs)
 {
     extern __shared__ uint64_t smem_u64[];
     uint8_t* smem = reinterpret_cast<uint8_t*>(smem_u64);
-    const sgm_auxplan_dev& A = *gA;
-    const uint32_t n = A.n;
     const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
     WaveAux& S = *reinterpret_cast<WaveAux*>(smem + wave * ((sizeof(WaveAux) + 15) & ~(size_t)15));
     uint32_t* ws = stack + (size_t)(blockIdx.x * (blockDim.x >> 6) + wave) * stack_words;
-    const uint32_t L = n - 1;
     for (uint32_t d = 0; d < SGM_MAXQ; d++) S.nodes[d] = 0;
     unsigned long long inter = 0, elems = 0, outs = 0, scanned = 0, cnt_pending = 0, items_done = 0, fsp = 0;
     bool overflow = false;
     uint32_t poll = 0;
-    const uint64_t* anc = A.anc;
+    uint32_t cur = SGM_INVALID, n = 0, L = 0;
+    const sgm_auxplan_dev* Ap = plans;
+    const uint64_t* anc = nullptr;
+    sgm_stats_dev* gst = nullptr;
+    uint32_t* stop = nullptr;
+    // the current query's counters to its statistics (an overflow voids the query: the host
+    // re-runs it with a larger stack)
+    auto flush_all = [&](bool ovf) {
+        if (cur == SGM_INVALID) return;
+        flush_count(cnt_pending, &gst->count, stop, cap);
+        if (lane == 0) {
+            if (inter) atomicAdd(&gst->intersections, inter);
+            if (elems) atomicAdd(&gst->elems_read, elems);
+            if (outs) atomicAdd(&gst->elems_out, outs);
+            if (scanned) atomicAdd(&gst->elems_scanned, scanned);
+            if (items_done) atomicAdd(&gst->items_done, items_done);
+            if (fsp) atomicAdd(&gst->fs_prunes, fsp);
+            if (ovf) { atomicAdd(&gst->overflow, 1ull); atomicExch(stop, 1u); }
+            for (uint32_t dd = 0; dd < n; dd++)
+                if (S.nodes[dd]) atomicAdd(&gst->nodes[dd], S.nodes[dd]);
+        }
+        inter = elems = outs = scanned = items_done = fsp = 0;
+        for (uint32_t dd = 0; dd < SGM_MAXQ; dd++) S.nodes[dd] = 0;
+    };
+    const uint32_t total = item_first[nq];
     while (true) {
         uint32_t it = 0;
         if (lane == 0) it = atomicAdd(work, 1u);
         it = uni(__shfl((int)it, 0));
-        if (it >= n_items) break;
-        if (__atomic_load_n(stop, __ATOMIC_RELAXED)) break;
+        if (it >= total) break;
+        if (__atomic_load_n(host_stop, __ATOMIC_RELAXED)) break;
+        // the item's query: the last q with item_first[q] <= it
+        uint32_t lo = 0, hi = nq;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (item_first[mid] <= it) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t q = uni(lo);
+        if (q != cur) {
+            flush_all(false);
+            cur = q;
+            Ap = plans + q;
+            n = Ap->n;
+            L = n - 1;
+            anc = Ap->anc;
+            gst = stats + q;
+            stop = stops + q;
+        }
+        if (__atomic_load_n(stop, __ATOMIC_RELAXED)) continue;  // capped (or voided) query
+        const sgm_auxplan_dev& A = *Ap;
+        const uint32_t root = it - item_first[q];
         items_done++;
         bool aborted = false;
-        S.emb[0] = A.cand[0][it];
-        S.iemb[0] = it;
+        S.emb[0] = A.cand[0][root];
+        S.iemb[0] = root;
         S.nodes[0] += 1;
         S.stack_top[1] = 0;
         uint32_t d = 1;
@@ -1244,7 +1290,11 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
         // enter depth d (d <= L): build the list; at d == L count directly
         bool entering = true;
         while (true) {
-            if ((++poll & 63u) == 0 && __atomic_load_n(stop, __ATOMIC_RELAXED)) { aborted = true; break; }
+            if ((++poll & 63u) == 0 &&
+                (__atomic_load_n(stop, __ATOMIC_RELAXED) | __atomic_load_n(host_stop, __ATOMIC_RELAXED))) {
+                aborted = true;
+                break;
+            }
             if (entering) {
                 entering = false;
                 const uint32_t np = A.prev_first[d + 1] - A.prev_first[d];
@@ -1349,20 +1399,14 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
             entering = true;
         }
         flush_count(cnt_pending, &gst->count, stop, cap);
-        if (aborted || overflow) break;
+        if (overflow) {
+            flush_all(true);
+            overflow = false;
+            cur = SGM_INVALID;
+        }
+        (void)aborted;  // a stopped query: its remaining items are skipped above
     }
-    flush_count(cnt_pending, &gst->count, stop, cap);
-    if (lane == 0) {
-        atomicAdd(&gst->intersections, inter);
-        atomicAdd(&gst->elems_read, elems);
-        atomicAdd(&gst->elems_out, outs);
-        atomicAdd(&gst->elems_scanned, scanned);
-        atomicAdd(&gst->items_done, items_done);
-        if (fsp) atomicAdd(&gst->fs_prunes, fsp);
-        if (overflow) { atomicAdd(&gst->overflow, 1ull); atomicExch(stop, 1u); }
-        for (uint32_t dd = 0; dd < n; dd++)
-            if (S.nodes[dd]) atomicAdd(&gst->nodes[dd], S.nodes[dd]);
-    }
+    flush_all(false);
 }
 
 }  // namespace sgm
@@ -1405,13 +1449,13 @@ void launch_enum_cache(hipStream_t s, uint32_t blocks, const uint32_t* off, cons
                            serial_last);
 }
 
-void launch_enum_aux(hipStream_t s, uint32_t blocks, const sgm_auxplan_dev* plan, uint32_t n_items, uint32_t* stack,
-                     uint32_t stack_words, uint32_t* work, sgm_stats_dev* st, uint32_t* stop, unsigned long long cap,
-                     int use_fs)
+void launch_enum_aux(hipStream_t s, uint32_t blocks, const sgm_auxplan_dev* plans, const uint32_t* item_first,
+                     uint32_t nq, uint32_t* stack, uint32_t stack_words, uint32_t* work, sgm_stats_dev* st,
+                     uint32_t* stops, const uint32_t* host_stop, unsigned long long cap, int use_fs)
 {
     const size_t smem = 4 * ((sizeof(WaveAux) + 15) & ~(size_t)15);
-    hipLaunchKernelGGL(k_enum_aux, dim3(blocks), dim3(256), smem, s, plan, n_items, stack, stack_words, work, st, stop,
-                       cap, use_fs);
+    hipLaunchKernelGGL(k_enum_aux, dim3(blocks), dim3(256), smem, s, plans, item_first, nq, stack, stack_words, work,
+                       st, stops, host_stop, cap, use_fs);
 }
 
 }  // namespace sgm

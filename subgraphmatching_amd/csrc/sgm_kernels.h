@@ -58,9 +58,9 @@ void launch_table_multi(hipStream_t s, const uint32_t* off, const uint32_t* adj,
                         const sgm_tabdesc* ds, uint32_t m, uint32_t max_rows, int pass);
 // sgm_enum.hip
 size_t enum_smem_bytes(uint32_t max_ops);
-void launch_enum_aux(hipStream_t s, uint32_t blocks, const sgm_auxplan_dev* plan, uint32_t n_items, uint32_t* stack,
-                     uint32_t stack_words, uint32_t* work, sgm_stats_dev* st, uint32_t* stop, unsigned long long cap,
-                     int use_fs);
+void launch_enum_aux(hipStream_t s, uint32_t blocks, const sgm_auxplan_dev* plans, const uint32_t* item_first,
+                     uint32_t nq, uint32_t* stack, uint32_t stack_words, uint32_t* work, sgm_stats_dev* st,
+                     uint32_t* stops, const uint32_t* host_stop, unsigned long long cap, int use_fs);
 // bitmap of C(u_L) per query (Q.lastbits) from its candidate words; before launch_enum_cache
 void launch_last_bits(hipStream_t s, const sgm_equery* qs, uint32_t nq, uint32_t V);
 // depth-1 split tables (queries with Q.split set), before launch_enum_cache; and the chunk

@@ -23,7 +23,13 @@ struct sgm_ctx {
     uint32_t* d_stop = nullptr;      // stop word polled by the enumeration kernel
     uint32_t* d_work = nullptr;      // work counter
     sgm_stats_dev* d_stats = nullptr;
-    sgm_auxplan_dev* d_auxplan = nullptr;
+    sgm_auxplan_dev* d_auxplan = nullptr;   // (unused since the batched aux search; kept allocated)
+    // batched aux-table search: per-query plans, item prefix, statistics, stop words
+    sgm_auxplan_dev* d_auxplans = nullptr;
+    uint32_t* d_afirst = nullptr;
+    sgm_stats_dev* d_astats = nullptr;
+    uint32_t* d_astop = nullptr;
+    uint32_t aux_cap = 0, afirst_cap = 0, astats_cap = 0, astop_cap = 0;
     unsigned long long* d_work64 = nullptr;  // item counter of the batched search
     sgm_plan_dev* d_plans = nullptr;          // per-query plans of a batch
     sgm_equery* d_eq = nullptr;

@@ -198,6 +198,15 @@ def test_enum_batch_equals_oracle(g_small):
             refs.append(ref)
         keep = [i for i in range(len(qs)) if cap or not refs[i]["capped"]]
         res = sgm.enumerate_batch([cans[i] for i in keep], [orders[i] for i in keep], cap=cap)
+        if not cap:
+            # the aux-table search over the same queries in one launch (plain and failing sets)
+            for fs, method in ((False, L.SGM_ENUM_AUX_TABLES), (True, L.SGM_ENUM_AUX_TABLES_FS)):
+                ra = sgm.enumerate_batch([cans[i] for i in keep], [orders[i] for i in keep], method=method)
+                for r, i in zip(ra, keep):
+                    ok_r, c_r = O.gql(g_small, qs[i])
+                    ref_a = O.enum_all(g_small, qs[i], c_r, orders[i], fs=fs)
+                    assert (r.ans, r.nodes, r.intersections, r.fs_prunes) == \
+                        (ref_a["count"], ref_a["nodes"], ref_a["intersections"], ref_a["fs_prunes"]), (qs[i].name, fs)
         if not cap:  # orders=None: the library's RI orders, the same search
             res_ri = sgm.enumerate_batch([cans[i] for i in keep], None, cap=cap)
             assert [(r.ans, r.nodes, r.intersections) for r in res_ri] == [(r.ans, r.nodes, r.intersections) for r in res]
