@@ -171,9 +171,10 @@ int sgm_plan_export_labeled(const uint32_t* qoff, const uint32_t* qadj, const ui
 /* ---- enumeration (replaces explore_Intersection_Cache_DAF_Filter inc/explore.h:1073 and
  * explore_Intersection_All<false x5> :2757) -------------------------------------------- */
 int sgm_enumerate(sgm_query* q, const uint32_t* order, const sgm_enum_opts* opts, sgm_stats* out);
-/* Many queries of one data graph in one persistent launch (cache + filter methods; no
- * reference counterpart: the reference runs one query per call).  Each query gets its own
- * cap and its own sgm_stats; out[i].kernel_ms is the time of the whole batch's launch.
+/* Many queries of one data graph in one persistent launch (every method: cache + filter
+ * with or without failing sets, and the aux-table search, whose missing tables are built in
+ * the call; no reference counterpart: the reference runs one query per call).  Each query
+ * gets its own cap and its own sgm_stats; out[i].kernel_ms is the time of the whole launch.
  * orders == NULL or orders[i] == NULL: the RI order (Order_RI), computed in the call. */
 int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uint32_t nq, const sgm_enum_opts* opts,
                         sgm_stats* out);
