@@ -723,47 +723,60 @@ int sgm_build_tables(sgm_query* q)
     if (Rw + 4 * (size_t)m > 0xFFFFFFFFull) return set_err(c, SGM_EINVAL, "tables too large");
     HIPCHK(c, grow_arena(&q->d_toff, q->toff_cap, off_words));
     HIPCHK(c, grow_arena(&q->d_tscr, q->tscr_cap, Rw + 4 * (size_t)m));
-    if (q->tdesc_cap < m) {
+    if (q->tdesc_cap < n) {
         dfree(q->d_tdesc);
         q->d_tdesc = nullptr;
         q->tdesc_cap = 0;
-        HIPCHK(c, dalloc(&q->d_tdesc, m));
-        q->tdesc_cap = m;
+        HIPCHK(c, dalloc(&q->d_tdesc, n));
+        q->tdesc_cap = n;
     }
-    // pinned: scan descriptors and results (4m words), then the edge descriptors
+    // pinned: scan descriptors and results (4m words), then one descriptor per source vertex
     const size_t hs_bytes = stage_align(sizeof(uint32_t) * 4 * m);
-    uint8_t* stage = static_cast<uint8_t*>(host_stage(c, hs_bytes + sizeof(sgm_tabdesc) * m));
+    uint8_t* stage = static_cast<uint8_t*>(host_stage(c, hs_bytes + sizeof(sgm_tabsrc) * n));
     std::vector<uint32_t> h_v;
-    std::vector<sgm_tabdesc> td_v;
+    std::vector<sgm_tabsrc> td_v;
     uint32_t* hs;
-    sgm_tabdesc* td;
+    sgm_tabsrc* td;
     if (stage) {
         hs = reinterpret_cast<uint32_t*>(stage);
-        td = reinterpret_cast<sgm_tabdesc*>(stage + hs_bytes);
+        td = reinterpret_cast<sgm_tabsrc*>(stage + hs_bytes);
     } else {
         h_v.resize(4 * (size_t)m);
-        td_v.resize(m);
+        td_v.resize(n);
         hs = h_v.data();
         td = td_v.data();
     }
     uint32_t* h = hs + 2 * m;  // totals[m], maxima[m]
     // pass 0: row counts of every table (one launch), their offsets (one launch), one
     // synchronisation
+    // (edges out of one vertex are consecutive: one descriptor per run of them)
     size_t o = 0;
-    uint32_t max_rows = 0;
+    uint32_t max_rows = 0, ns = 0;
     for (uint32_t k = 0; k < m; k++) {
         const uint32_t a = edges[k].first, b = edges[k].second, rows = q->counts[a];
         sgm_table& t = q->tables[(size_t)a * n + b];
         t.d_off = q->d_toff + o;
         hs[2 * k] = (uint32_t)o;
         hs[2 * k + 1] = rows;
-        td[k] = sgm_tabdesc{q->d_list + q->cand_off[a], rows, b, q->d_tscr + (o - k), q->d_bm + (size_t)b * W,
-                            q->d_rank + (size_t)b * W, t.d_off, nullptr, nullptr};
+        if (k == 0 || edges[k - 1].first != a) {
+            sgm_tabsrc& S = td[ns];
+            S = sgm_tabsrc{};
+            S.rows = q->d_list + q->cand_off[a];
+            S.nrows = rows;
+            ns++;
+        }
+        sgm_tabsrc& S = td[ns - 1];
+        const uint32_t j = S.k++;
+        S.b[j] = b;
+        S.rowcnt[j] = q->d_tscr + (o - k);
+        S.rowoff[j] = t.d_off;
+        S.bm[j] = q->d_bm + (size_t)b * W;
+        S.rank[j] = q->d_rank + (size_t)b * W;
         max_rows = std::max(max_rows, rows);
         o += (size_t)rows + 1;
     }
-    HIPCHK(c, hipMemcpyAsync(q->d_tdesc, td, sizeof(sgm_tabdesc) * m, hipMemcpyHostToDevice, c->stream));
-    sgm::launch_table_multi(c->stream, q->g->d_off, q->g->d_adj, q->d_vmask, q->d_tdesc, m, max_rows, 0);
+    HIPCHK(c, hipMemcpyAsync(q->d_tdesc, td, sizeof(sgm_tabsrc) * ns, hipMemcpyHostToDevice, c->stream));
+    sgm::launch_table_src(c->stream, q->g->d_off, q->g->d_adj, q->d_vmask, q->d_tdesc, ns, max_rows, 0);
     uint32_t* d_desc = q->d_tscr + Rw + 2 * (size_t)m;
     HIPCHK(c, hipMemcpyAsync(d_desc, hs, sizeof(uint32_t) * 2 * m, hipMemcpyHostToDevice, c->stream));
     sgm::launch_scan_segments(c->stream, q->d_tscr, (uint32_t)Rw, m, d_desc, q->d_toff);
@@ -780,21 +793,23 @@ int sgm_build_tables(sgm_query* q)
         return set_err(c, SGM_ENOMEM, "tables of %llu entries", (unsigned long long)ent);
     }
     size_t base = 0;
-    for (uint32_t k = 0; k < m; k++) {
+    for (uint32_t k = 0, s = 0, j = 0; k < m; k++) {
         const uint32_t a = edges[k].first, b = edges[k].second;
+        if (k && edges[k - 1].first != a) s++, j = 0;
         sgm_table& t = q->tables[(size_t)a * n + b];
         t.e_cnt = h[k];
         t.max_deg = h[m + k];
         t.d_idx = q->d_tidx + base;
         t.d_v = q->d_tv + base;
         base += h[k];
-        td[k].eidx = t.d_idx;
-        td[k].ev = t.d_v;
+        td[s].eidx[j] = t.d_idx;
+        td[s].ev[j] = t.d_v;
+        j++;
         t.built = true;
     }
     // pass 1: every table's entries in one launch
-    HIPCHK(c, hipMemcpyAsync(q->d_tdesc, td, sizeof(sgm_tabdesc) * m, hipMemcpyHostToDevice, c->stream));
-    sgm::launch_table_multi(c->stream, q->g->d_off, q->g->d_adj, q->d_vmask, q->d_tdesc, m, max_rows, 1);
+    HIPCHK(c, hipMemcpyAsync(q->d_tdesc, td, sizeof(sgm_tabsrc) * ns, hipMemcpyHostToDevice, c->stream));
+    sgm::launch_table_src(c->stream, q->g->d_off, q->g->d_adj, q->d_vmask, q->d_tdesc, ns, max_rows, 1);
     se = hipStreamSynchronize(c->stream);
     if (se != hipSuccess) { free_tables(q); return set_err(c, SGM_EHIP, "tables: %s", hipGetErrorString(se)); }
     return SGM_OK;

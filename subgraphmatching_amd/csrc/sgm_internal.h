@@ -48,17 +48,19 @@ struct sgm_auxplan_dev {
     uint64_t anc[SGM_MAXQ];           // DAF ancestor masks over depths (failing sets, inc/prune.h:100-104)
 };
 
-// One directed query edge (a, b) of an aux-table build (inc/table.h:7-137): rows = C(a);
-// the count pass writes rowcnt, the fill pass writes positions in C(b) and vertex ids.
-struct sgm_tabdesc {
+// The directed query edges (a, b_j), j < k, out of one query vertex a in an aux-table build
+// (inc/table.h:7-137): rows = C(a); per b_j the count pass writes rowcnt[j], the fill pass
+// positions in C(b_j) (rank words of b_j's bitmap) and vertex ids at rowoff[j].
+struct sgm_tabsrc {
     const uint32_t* rows;
-    uint32_t nrows, b;
-    uint32_t* rowcnt;
-    const uint64_t* bm_b;
-    const uint32_t* rank_b;
-    const uint32_t* rowoff;
-    uint32_t* eidx;
-    uint32_t* ev;
+    uint32_t nrows, k;
+    uint32_t b[SGM_MAXQ];
+    uint32_t* rowcnt[SGM_MAXQ];
+    const uint32_t* rowoff[SGM_MAXQ];
+    uint32_t* eidx[SGM_MAXQ];
+    uint32_t* ev[SGM_MAXQ];
+    const uint64_t* bm[SGM_MAXQ];
+    const uint32_t* rank[SGM_MAXQ];
 };
 
 struct sgm_stats_dev {

@@ -762,54 +762,67 @@ __device__ __forceinline__ uint32_t cand_rank(const uint64_t* __restrict__ bm, c
     return rank[word] + (uint32_t)__popcll(low);
 }
 
-// row counts of every directed edge (blockIdx.y = edge)
-__global__ void __launch_bounds__(256) k_table_count_multi(const uint32_t* __restrict__ off,
-                                                           const uint32_t* __restrict__ adj,
-                                                           const uint32_t* __restrict__ vmask,
-                                                           const sgm_tabdesc* __restrict__ ds)
+// Row counts of every directed edge out of a (blockIdx.y = a's descriptor): one wave per
+// row of C(a) reads N(v) and the membership words once for all of a's neighbours b_j;
+// lane j keeps b_j's count.
+__global__ void __launch_bounds__(256) k_table_count_src(const uint32_t* __restrict__ off,
+                                                         const uint32_t* __restrict__ adj,
+                                                         const uint32_t* __restrict__ vmask,
+                                                         const sgm_tabsrc* __restrict__ ds)
 {
-    const sgm_tabdesc& D = ds[blockIdx.y];
+    const sgm_tabsrc& D = ds[blockIdx.y];
     const uint32_t waves = blockDim.x >> 6, lane = lane_id();
     const uint32_t* __restrict__ rows = D.rows;
-    const uint32_t nrows = D.nrows, b = D.b;
+    const uint32_t nrows = D.nrows, k = D.k;
     for (uint32_t i = blockIdx.x * waves + (threadIdx.x >> 6); i < nrows; i += gridDim.x * waves) {
         const uint32_t v = rows[i], o = off[v], dv = off[v + 1] - o;
-        uint32_t c = 0;
-        for (uint32_t j = lane; j < dv; j += 64) c += (vmask[adj[o + j]] >> b) & 1u;
-        for (int s = 32; s > 0; s >>= 1) c += (uint32_t)__shfl_xor((int)c, s);
-        if (lane == 0) D.rowcnt[i] = c;
+        uint32_t cnt = 0;
+        for (uint32_t base = 0; base < dv; base += 64) {
+            const uint32_t jj = base + lane;
+            const uint32_t m = jj < dv ? vmask[adj[o + jj]] : 0u;
+            for (uint32_t j = 0; j < k; j++) {
+                const uint32_t c = (uint32_t)__popcll(__ballot((m >> D.b[j]) & 1u));
+                if (lane == j) cnt += c;
+            }
+        }
+        if (lane < k) D.rowcnt[lane][i] = cnt;
     }
 }
 
-// entries of every directed edge at the scanned row offsets
-__global__ void __launch_bounds__(256) k_table_fill_multi(const uint32_t* __restrict__ off,
-                                                          const uint32_t* __restrict__ adj,
-                                                          const uint32_t* __restrict__ vmask,
-                                                          const sgm_tabdesc* __restrict__ ds)
+// Entries of every directed edge out of a at the scanned row offsets (lane j keeps b_j's
+// write position).
+__global__ void __launch_bounds__(256) k_table_fill_src(const uint32_t* __restrict__ off,
+                                                        const uint32_t* __restrict__ adj,
+                                                        const uint32_t* __restrict__ vmask,
+                                                        const sgm_tabsrc* __restrict__ ds)
 {
-    const sgm_tabdesc& D = ds[blockIdx.y];
+    const sgm_tabsrc& D = ds[blockIdx.y];
     const uint32_t waves = blockDim.x >> 6, lane = lane_id();
     const uint32_t* __restrict__ rows = D.rows;
-    const uint32_t nrows = D.nrows, b = D.b;
+    const uint32_t nrows = D.nrows, k = D.k;
     for (uint32_t i = blockIdx.x * waves + (threadIdx.x >> 6); i < nrows; i += gridDim.x * waves) {
         const uint32_t v = rows[i], o = off[v], dv = off[v + 1] - o;
-        uint32_t outp = D.rowoff[i];
+        uint32_t outp = lane < k ? D.rowoff[lane][i] : 0u;
         for (uint32_t base = 0; base < dv; base += 64) {
-            const uint32_t j = base + lane;
-            const uint32_t w = j < dv ? adj[o + j] : 0u;
-            const bool hit = j < dv && ((vmask[w] >> b) & 1u);
-            const uint64_t bal = __ballot(hit);
-            if (hit) {
-                const uint32_t p = outp + mbcnt64(bal);
-                D.eidx[p] = cand_rank(D.bm_b, D.rank_b, w);
-                D.ev[p] = w;
+            const uint32_t jj = base + lane;
+            const uint32_t w = jj < dv ? adj[o + jj] : 0u;
+            const uint32_t m = jj < dv ? vmask[w] : 0u;
+            for (uint32_t j = 0; j < k; j++) {
+                const bool hit = (m >> D.b[j]) & 1u;
+                const uint64_t bal = __ballot(hit);
+                if (!bal) continue;
+                const uint32_t start = rdlane(outp, j);
+                if (hit) {
+                    const uint32_t p = start + mbcnt64(bal);
+                    D.eidx[j][p] = cand_rank(D.bm[j], D.rank[j], w);
+                    D.ev[j][p] = w;
+                }
+                if (lane == j) outp += (uint32_t)__popcll(bal);
             }
-            outp += (uint32_t)__popcll(bal);
         }
     }
 }
 
-// Exclusive scan of n u32 counts into n+1 u64-safe u32 offsets (single block, looped).
 // Exclusive scan of in[0, n) into out[0, n] (out[n] = total) and the maximum into *maxv, by
 // one block of 256 threads; *tot (when set) also receives the total.
 __device__ void scan_block(const uint32_t* __restrict__ in, uint32_t n, uint32_t* __restrict__ out,
@@ -990,14 +1003,14 @@ void launch_compact(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V)
     hipLaunchKernelGGL(k_compact, grid, dim3(256), 0, s, cs, V);
 }
 
-void launch_table_multi(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* vmask,
-                        const sgm_tabdesc* ds, uint32_t m, uint32_t max_rows, int pass)
+void launch_table_src(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* vmask,
+                      const sgm_tabsrc* ds, uint32_t nsrc, uint32_t max_rows, int pass)
 {
-    if (!m || !max_rows) return;
-    const uint32_t xcap = 16384 / m > 8 ? 16384 / m : 8u;
-    dim3 grid(clampg(((uint64_t)max_rows + 3) / 4, 1, xcap), m);
-    if (pass == 0) hipLaunchKernelGGL(k_table_count_multi, grid, dim3(256), 0, s, off, adj, vmask, ds);
-    else hipLaunchKernelGGL(k_table_fill_multi, grid, dim3(256), 0, s, off, adj, vmask, ds);
+    if (!nsrc || !max_rows) return;
+    const uint32_t xcap = 16384 / nsrc > 8 ? 16384 / nsrc : 8u;
+    dim3 grid(clampg(((uint64_t)max_rows + 3) / 4, 1, xcap), nsrc);
+    if (pass == 0) hipLaunchKernelGGL(k_table_count_src, grid, dim3(256), 0, s, off, adj, vmask, ds);
+    else hipLaunchKernelGGL(k_table_fill_src, grid, dim3(256), 0, s, off, adj, vmask, ds);
 }
 
 void launch_scan_segments(hipStream_t s, uint32_t* scr, uint32_t R, uint32_t m, const uint32_t* desc, uint32_t* toff)

@@ -157,7 +157,7 @@ struct sgm_query {
     uint32_t* d_tidx = nullptr;
     uint32_t* d_tv = nullptr;
     uint32_t* d_tscr = nullptr;
-    sgm_tabdesc* d_tdesc = nullptr;  // per directed edge
+    sgm_tabsrc* d_tdesc = nullptr;   // per query vertex (its directed edges)
     size_t toff_cap = 0, tidx_cap = 0, tv_cap = 0, tscr_cap = 0, tdesc_cap = 0;
 };
 
