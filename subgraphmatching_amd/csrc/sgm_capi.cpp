@@ -343,6 +343,7 @@ void sgm_query_destroy(sgm_query* q)
     dfree(q->d_tv);
     dfree(q->d_tscr);
     dfree(q->d_tdesc);
+    dfree(q->d_tedesc);
     dfree(q->d_vmask);
     dfree(q->d_removed);
     dfree(q->d_bm);
@@ -730,21 +731,35 @@ int sgm_build_tables(sgm_query* q)
         HIPCHK(c, dalloc(&q->d_tdesc, n));
         q->tdesc_cap = n;
     }
-    // pinned: scan descriptors and results (4m words), then one descriptor per source vertex
+    if (q->tedesc_cap < m) {
+        dfree(q->d_tedesc);
+        q->d_tedesc = nullptr;
+        q->tedesc_cap = 0;
+        HIPCHK(c, dalloc(&q->d_tedesc, m));
+        q->tedesc_cap = m;
+    }
+    // pinned: scan descriptors and results (4m words), one count descriptor per source
+    // vertex, one fill descriptor per edge
     const size_t hs_bytes = stage_align(sizeof(uint32_t) * 4 * m);
-    uint8_t* stage = static_cast<uint8_t*>(host_stage(c, hs_bytes + sizeof(sgm_tabsrc) * n));
+    const size_t td_bytes = stage_align(sizeof(sgm_tabsrc) * n);
+    uint8_t* stage = static_cast<uint8_t*>(host_stage(c, hs_bytes + td_bytes + sizeof(sgm_tabdesc) * m));
     std::vector<uint32_t> h_v;
     std::vector<sgm_tabsrc> td_v;
+    std::vector<sgm_tabdesc> te_v;
     uint32_t* hs;
     sgm_tabsrc* td;
+    sgm_tabdesc* te;
     if (stage) {
         hs = reinterpret_cast<uint32_t*>(stage);
         td = reinterpret_cast<sgm_tabsrc*>(stage + hs_bytes);
+        te = reinterpret_cast<sgm_tabdesc*>(stage + hs_bytes + td_bytes);
     } else {
         h_v.resize(4 * (size_t)m);
         td_v.resize(n);
+        te_v.resize(m);
         hs = h_v.data();
         td = td_v.data();
+        te = te_v.data();
     }
     uint32_t* h = hs + 2 * m;  // totals[m], maxima[m]
     // pass 0: row counts of every table (one launch), their offsets (one launch), one
@@ -769,14 +784,13 @@ int sgm_build_tables(sgm_query* q)
         const uint32_t j = S.k++;
         S.b[j] = b;
         S.rowcnt[j] = q->d_tscr + (o - k);
-        S.rowoff[j] = t.d_off;
-        S.bm[j] = q->d_bm + (size_t)b * W;
-        S.rank[j] = q->d_rank + (size_t)b * W;
+        te[k] = sgm_tabdesc{S.rows, rows, b, q->d_bm + (size_t)b * W, q->d_rank + (size_t)b * W, t.d_off,
+                            nullptr, nullptr};
         max_rows = std::max(max_rows, rows);
         o += (size_t)rows + 1;
     }
     HIPCHK(c, hipMemcpyAsync(q->d_tdesc, td, sizeof(sgm_tabsrc) * ns, hipMemcpyHostToDevice, c->stream));
-    sgm::launch_table_src(c->stream, q->g->d_off, q->g->d_adj, q->d_vmask, q->d_tdesc, ns, max_rows, 0);
+    sgm::launch_table_count(c->stream, q->g->d_off, q->g->d_adj, q->d_vmask, q->d_tdesc, ns, max_rows);
     uint32_t* d_desc = q->d_tscr + Rw + 2 * (size_t)m;
     HIPCHK(c, hipMemcpyAsync(d_desc, hs, sizeof(uint32_t) * 2 * m, hipMemcpyHostToDevice, c->stream));
     sgm::launch_scan_segments(c->stream, q->d_tscr, (uint32_t)Rw, m, d_desc, q->d_toff);
@@ -793,23 +807,21 @@ int sgm_build_tables(sgm_query* q)
         return set_err(c, SGM_ENOMEM, "tables of %llu entries", (unsigned long long)ent);
     }
     size_t base = 0;
-    for (uint32_t k = 0, s = 0, j = 0; k < m; k++) {
+    for (uint32_t k = 0; k < m; k++) {
         const uint32_t a = edges[k].first, b = edges[k].second;
-        if (k && edges[k - 1].first != a) s++, j = 0;
         sgm_table& t = q->tables[(size_t)a * n + b];
         t.e_cnt = h[k];
         t.max_deg = h[m + k];
         t.d_idx = q->d_tidx + base;
         t.d_v = q->d_tv + base;
         base += h[k];
-        td[s].eidx[j] = t.d_idx;
-        td[s].ev[j] = t.d_v;
-        j++;
+        te[k].eidx = t.d_idx;
+        te[k].ev = t.d_v;
         t.built = true;
     }
     // pass 1: every table's entries in one launch
-    HIPCHK(c, hipMemcpyAsync(q->d_tdesc, td, sizeof(sgm_tabsrc) * ns, hipMemcpyHostToDevice, c->stream));
-    sgm::launch_table_src(c->stream, q->g->d_off, q->g->d_adj, q->d_vmask, q->d_tdesc, ns, max_rows, 1);
+    HIPCHK(c, hipMemcpyAsync(q->d_tedesc, te, sizeof(sgm_tabdesc) * m, hipMemcpyHostToDevice, c->stream));
+    sgm::launch_table_fill(c->stream, q->g->d_off, q->g->d_adj, q->d_vmask, q->d_tedesc, m, max_rows);
     se = hipStreamSynchronize(c->stream);
     if (se != hipSuccess) { free_tables(q); return set_err(c, SGM_EHIP, "tables: %s", hipGetErrorString(se)); }
     return SGM_OK;

@@ -48,19 +48,25 @@ struct sgm_auxplan_dev {
     uint64_t anc[SGM_MAXQ];           // DAF ancestor masks over depths (failing sets, inc/prune.h:100-104)
 };
 
-// The directed query edges (a, b_j), j < k, out of one query vertex a in an aux-table build
-// (inc/table.h:7-137): rows = C(a); per b_j the count pass writes rowcnt[j], the fill pass
-// positions in C(b_j) (rank words of b_j's bitmap) and vertex ids at rowoff[j].
+// One directed query edge (a, b) of the aux-table fill pass (inc/table.h:7-137): rows = C(a); entries go to
+// eidx / ev at the scanned row offsets rowoff, positions in C(b) from b's bitmap and ranks.
+struct sgm_tabdesc {
+    const uint32_t* rows;
+    uint32_t nrows, b;
+    const uint64_t* bm_b;
+    const uint32_t* rank_b;
+    const uint32_t* rowoff;
+    uint32_t* eidx;
+    uint32_t* ev;
+};
+
+// The directed query edges (a, b_j), j < k, out of one query vertex a in the count pass of an
+// aux-table build (inc/table.h:7-137): rows = C(a); the row counts of b_j go to rowcnt[j].
 struct sgm_tabsrc {
     const uint32_t* rows;
     uint32_t nrows, k;
     uint32_t b[SGM_MAXQ];
     uint32_t* rowcnt[SGM_MAXQ];
-    const uint32_t* rowoff[SGM_MAXQ];
-    uint32_t* eidx[SGM_MAXQ];
-    uint32_t* ev[SGM_MAXQ];
-    const uint64_t* bm[SGM_MAXQ];
-    const uint32_t* rank[SGM_MAXQ];
 };
 
 struct sgm_stats_dev {

@@ -53,10 +53,12 @@ void launch_rank_scan(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t 
 void launch_compact(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V);
 // aux tables: the row offsets of every directed edge in one launch
 void launch_scan_segments(hipStream_t s, uint32_t* scr, uint32_t R, uint32_t m, const uint32_t* desc, uint32_t* toff);
-// every directed edge of a query in one launch, grouped by source vertex: pass 0 row
-// counts, pass 1 entries
-void launch_table_src(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* vmask,
-                      const sgm_tabsrc* ds, uint32_t nsrc, uint32_t max_rows, int pass);
+// row counts of every directed edge of a query in one launch, grouped by source vertex
+void launch_table_count(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* vmask,
+                        const sgm_tabsrc* ds, uint32_t nsrc, uint32_t max_rows);
+// entries of every directed edge of a query in one launch, one edge per block row
+void launch_table_fill(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* vmask,
+                       const sgm_tabdesc* ds, uint32_t m, uint32_t max_rows);
 // sgm_enum.hip
 size_t enum_smem_bytes(uint32_t max_ops);
 void launch_enum_aux(hipStream_t s, uint32_t blocks, const sgm_auxplan_dev* plans, const uint32_t* item_first,

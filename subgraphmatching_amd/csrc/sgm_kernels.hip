@@ -789,36 +789,30 @@ __global__ void __launch_bounds__(256) k_table_count_src(const uint32_t* __restr
     }
 }
 
-// Entries of every directed edge out of a at the scanned row offsets (lane j keeps b_j's
-// write position).
-__global__ void __launch_bounds__(256) k_table_fill_src(const uint32_t* __restrict__ off,
-                                                        const uint32_t* __restrict__ adj,
-                                                        const uint32_t* __restrict__ vmask,
-                                                        const sgm_tabsrc* __restrict__ ds)
+// Entries of every directed edge (blockIdx.y = edge) at the scanned row offsets.  One
+// edge per block row keeps each wave's bitmap-rank lookups independent of the others'.
+__global__ void __launch_bounds__(256) k_table_fill_edge(const uint32_t* __restrict__ off,
+                                                         const uint32_t* __restrict__ adj,
+                                                         const uint32_t* __restrict__ vmask,
+                                                         const sgm_tabdesc* __restrict__ ds)
 {
-    const sgm_tabsrc& D = ds[blockIdx.y];
+    const sgm_tabdesc& D = ds[blockIdx.y];
     const uint32_t waves = blockDim.x >> 6, lane = lane_id();
-    const uint32_t* __restrict__ rows = D.rows;
-    const uint32_t nrows = D.nrows, k = D.k;
+    const uint32_t nrows = D.nrows, b = D.b;
     for (uint32_t i = blockIdx.x * waves + (threadIdx.x >> 6); i < nrows; i += gridDim.x * waves) {
-        const uint32_t v = rows[i], o = off[v], dv = off[v + 1] - o;
-        uint32_t outp = lane < k ? D.rowoff[lane][i] : 0u;
+        const uint32_t v = D.rows[i], o = off[v], dv = off[v + 1] - o;
+        uint32_t outp = D.rowoff[i];
         for (uint32_t base = 0; base < dv; base += 64) {
             const uint32_t jj = base + lane;
             const uint32_t w = jj < dv ? adj[o + jj] : 0u;
-            const uint32_t m = jj < dv ? vmask[w] : 0u;
-            for (uint32_t j = 0; j < k; j++) {
-                const bool hit = (m >> D.b[j]) & 1u;
-                const uint64_t bal = __ballot(hit);
-                if (!bal) continue;
-                const uint32_t start = rdlane(outp, j);
-                if (hit) {
-                    const uint32_t p = start + mbcnt64(bal);
-                    D.eidx[j][p] = cand_rank(D.bm[j], D.rank[j], w);
-                    D.ev[j][p] = w;
-                }
-                if (lane == j) outp += (uint32_t)__popcll(bal);
+            const bool hit = jj < dv && ((vmask[w] >> b) & 1u);
+            const uint64_t bal = __ballot(hit);
+            if (hit) {
+                const uint32_t p = outp + mbcnt64(bal);
+                D.eidx[p] = cand_rank(D.bm_b, D.rank_b, w);
+                D.ev[p] = w;
             }
+            outp += (uint32_t)__popcll(bal);
         }
     }
 }
@@ -1003,14 +997,22 @@ void launch_compact(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V)
     hipLaunchKernelGGL(k_compact, grid, dim3(256), 0, s, cs, V);
 }
 
-void launch_table_src(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* vmask,
-                      const sgm_tabsrc* ds, uint32_t nsrc, uint32_t max_rows, int pass)
+void launch_table_count(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* vmask,
+                        const sgm_tabsrc* ds, uint32_t nsrc, uint32_t max_rows)
 {
     if (!nsrc || !max_rows) return;
     const uint32_t xcap = 16384 / nsrc > 8 ? 16384 / nsrc : 8u;
     dim3 grid(clampg(((uint64_t)max_rows + 3) / 4, 1, xcap), nsrc);
-    if (pass == 0) hipLaunchKernelGGL(k_table_count_src, grid, dim3(256), 0, s, off, adj, vmask, ds);
-    else hipLaunchKernelGGL(k_table_fill_src, grid, dim3(256), 0, s, off, adj, vmask, ds);
+    hipLaunchKernelGGL(k_table_count_src, grid, dim3(256), 0, s, off, adj, vmask, ds);
+}
+
+void launch_table_fill(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* vmask,
+                       const sgm_tabdesc* ds, uint32_t m, uint32_t max_rows)
+{
+    if (!m || !max_rows) return;
+    const uint32_t xcap = 16384 / m > 8 ? 16384 / m : 8u;
+    dim3 grid(clampg(((uint64_t)max_rows + 3) / 4, 1, xcap), m);
+    hipLaunchKernelGGL(k_table_fill_edge, grid, dim3(256), 0, s, off, adj, vmask, ds);
 }
 
 void launch_scan_segments(hipStream_t s, uint32_t* scr, uint32_t R, uint32_t m, const uint32_t* desc, uint32_t* toff)

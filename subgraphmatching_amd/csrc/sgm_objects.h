@@ -157,8 +157,9 @@ struct sgm_query {
     uint32_t* d_tidx = nullptr;
     uint32_t* d_tv = nullptr;
     uint32_t* d_tscr = nullptr;
-    sgm_tabsrc* d_tdesc = nullptr;   // per query vertex (its directed edges)
-    size_t toff_cap = 0, tidx_cap = 0, tv_cap = 0, tscr_cap = 0, tdesc_cap = 0;
+    sgm_tabsrc* d_tdesc = nullptr;   // count pass: per query vertex (its directed edges)
+    sgm_tabdesc* d_tedesc = nullptr; // fill pass: per directed edge
+    size_t toff_cap = 0, tidx_cap = 0, tv_cap = 0, tscr_cap = 0, tdesc_cap = 0, tedesc_cap = 0;
 };
 
 inline int set_err(sgm_ctx* c, int code, const char* fmt, ...)
