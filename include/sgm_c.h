@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define SGM_ABI_VERSION 1
+#define SGM_ABI_VERSION 2
 
 enum {
     SGM_OK = 0,
@@ -58,7 +58,21 @@ typedef struct {
     uint32_t flags;        /* SGM_ENUM_F_* */
     uint32_t hub_min_degree; /* intersect against adjacency bitmaps only for lists of at least
                                 this many vertices (0 = 64, the smallest with a bitmap) */
-    uint32_t reserved[3];
+    uint32_t time_limit_ms;  /* per-query work budget (0 = none): a query's search stops once the
+                                waves working on it have spent this many milliseconds of device
+                                time on it, summed over the waves; its result is then
+                                sgm_stats.timed_out and its count is not a result (the per-query
+                                timeout of src/overall_performance.cpp:290-312, counted in device
+                                worker time as the reference counts thread CPU time,
+                                inc/explore.h:1093,1126-1127) */
+    uint32_t root_begin;     /* search only the depth-0 candidates v with root_begin <= v < root_end */
+    uint32_t root_end;       /* (vertex ids; root_end == 0: every candidate).  The depth-0 loop
+                                never prunes and each subtree's failing-set slots are reset
+                                before they are read (inc/prune.h:119-137, inc/explore.h:
+                                1030-1066), so counts and statistics over disjoint ranges add up
+                                to those of the whole search: a query splits over launches or
+                                GPUs by ranges (no reference counterpart) */
+    uint32_t reserved[2];
 } sgm_enum_opts;
 
 /* sgm_enum_opts.flags: intersect against hub adjacency lists by binary search instead of
@@ -86,7 +100,7 @@ typedef struct {
     int32_t capped;          /* the cap was reached */
     int32_t stopped;         /* sgm_stop() interrupted: count is partial */
     uint32_t retries;        /* re-runs with a larger set-op stack */
-    uint32_t reserved;
+    int32_t timed_out;       /* the query passed sgm_enum_opts.time_limit_ms: count is partial */
     uint64_t busy_ns;        /* cache methods: wave time spent on this query, summed over the
                                 waves (a work measure of the order; no reference counterpart) */
 } sgm_stats;
@@ -105,7 +119,10 @@ const char* sgm_ctx_last_error(const sgm_ctx* ctx);
  * src/optimizations_ablation.cpp:64-82). */
 int sgm_ctx_filter_ms(const sgm_ctx* ctx, double* nlf_ms, double* gql_ms, double* cand_ms);
 /* Cooperative stop of a running enumeration (replaces the std::atomic<bool>& stop_flag
- * polled by the reference, inc/explore.h:1057, src/overall_performance.cpp:302-305). */
+ * polled by the reference, inc/explore.h:1057, src/overall_performance.cpp:302-305).  The
+ * request is a plain store into a word of pinned host memory that the running kernels poll
+ * (no GPU command is queued, so it reaches a grid that occupies the whole GPU); it takes
+ * effect within a few milliseconds.  Safe to call from any host thread. */
 int sgm_stop(sgm_ctx* ctx);
 
 /* ---- data graph (replaces Graph::Load's CSR, inc/graph.h:190-266) ----------------- */

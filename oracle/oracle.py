@@ -46,7 +46,7 @@ def lib():
         if not os.path.exists(path):
             build()
         _lib = C.CDLL(path)
-        if _lib.orc_abi_version() != 3:  # a stale build from before the current arguments
+        if _lib.orc_abi_version() != 4:  # a stale build from before the current arguments
             build(force=True)
             raise RuntimeError("liboracle.so was stale and has been rebuilt; restart the process")
         assert _lib.orc_stats_size() == C.sizeof(Stats)
@@ -174,8 +174,9 @@ def _cand_matrix(cands, V):
     return m, cnt, stride
 
 
-def enum_cache(g, q, cands, order, cap=None, fs=True, time_limit_ms=0.0):
-    """explore_Intersection_Cache_DAF_Filter restated; returns the stats dict."""
+def enum_cache(g, q, cands, order, cap=None, fs=True, time_limit_ms=0.0, roots=None):
+    """explore_Intersection_Cache_DAF_Filter restated; returns the stats dict.  ``roots``
+    = (lo, hi): search only the depth-0 candidates with lo <= id < hi."""
     V, n = g.getVertexCnt(), q.getVertexCnt()
     m, cnt, stride = _cand_matrix(cands, V)
     o = _u32(order)
@@ -183,14 +184,20 @@ def enum_cache(g, q, cands, order, cap=None, fs=True, time_limit_ms=0.0):
     rc = lib().orc_enum_cache(C.c_uint32(V), _p(g.off), _p(g.adj), C.c_uint32(n), _p(q.off),
                               _p(q.adj), _lab(q), _p(m), _p(cnt), C.c_uint32(stride), _p(o),
                               C.c_uint64(cap if cap is not None else 2**64 - 1),
-                              C.c_int(1 if fs else 0), C.c_double(time_limit_ms), C.byref(st))
+                              C.c_int(1 if fs else 0), C.c_double(time_limit_ms), *_roots(roots), C.byref(st))
     if rc:
         raise ValueError(f"orc_enum_cache failed ({rc})")
     return st.as_dict(n)
 
 
-def enum_all(g, q, cands, order, cap=None, time_limit_ms=0.0, fs=False):
-    """explore_Intersection_All<fs,false,false,false,false> restated; returns the stats dict."""
+def _roots(roots):
+    lo, hi = (0, 0) if roots is None else roots
+    return C.c_uint32(int(lo)), C.c_uint32(int(hi))
+
+
+def enum_all(g, q, cands, order, cap=None, time_limit_ms=0.0, fs=False, roots=None):
+    """explore_Intersection_All<fs,false,false,false,false> restated; returns the stats dict.
+    ``roots`` as in enum_cache."""
     V, n = g.getVertexCnt(), q.getVertexCnt()
     m, cnt, stride = _cand_matrix(cands, V)
     o = _u32(order)
@@ -198,7 +205,7 @@ def enum_all(g, q, cands, order, cap=None, time_limit_ms=0.0, fs=False):
     rc = lib().orc_enum_all(C.c_uint32(V), _p(g.off), _p(g.adj), C.c_uint32(n), _p(q.off),
                             _p(q.adj), _p(m), _p(cnt), C.c_uint32(stride), _p(o),
                             C.c_uint64(cap if cap is not None else 2**64 - 1), C.c_int(1 if fs else 0),
-                            C.c_double(time_limit_ms), C.byref(st))
+                            C.c_double(time_limit_ms), *_roots(roots), C.byref(st))
     if rc:
         raise ValueError(f"orc_enum_all failed ({rc})")
     return st.as_dict(n)

@@ -717,10 +717,15 @@ static void daf_ancestors(uint32_t n, const uint32_t *qoff, const uint32_t *qadj
     }
 }
 
+/* root_lo / root_hi: only depth-0 candidates v with root_lo <= v < root_hi are searched
+ * (root_hi == 0: all).  The reference's depth-0 loop never prunes and every deeper
+ * failing-set slot is reset before it is read, so the subtree of each depth-0 candidate is
+ * searched alike whatever else runs: counts and statistics over disjoint ranges add up to
+ * those of the whole search (a restriction of the reference's loop, not a new search). */
 int orc_enum_cache(uint32_t V, const uint32_t *off, const uint32_t *adj, uint32_t n,
                    const uint32_t *qoff, const uint32_t *qadj, const uint32_t *qlab, const uint32_t *cand,
                    const uint32_t *cnt, uint32_t cand_stride, const uint32_t *order, uint64_t cap,
-                   int use_fs, double time_limit_ms, orc_stats *st)
+                   int use_fs, double time_limit_ms, uint32_t root_lo, uint32_t root_hi, orc_stats *st)
 {
     memset(st, 0, sizeof(*st));
     if (n < 2 || n > ORC_MAXQ) return -2;
@@ -747,6 +752,7 @@ int orc_enum_cache(uint32_t V, const uint32_t *off, const uint32_t *adj, uint32_
     uint32_t u0 = order[0];
     for (uint32_t i = 0; i < cnt[u0]; i++) {
         uint32_t v0 = cand[(size_t)u0 * cand_stride + i];
+        if (v0 < root_lo || (root_hi && v0 >= root_hi)) continue;
         c.emb[u0] = v0; c.vis[v0] = 1; c.revdep[v0] = 0;
         st->nodes[0]++;
         rec_cache(&c, 1);
@@ -853,7 +859,7 @@ static void rec_all(all_ctx *c, uint32_t d)
 int orc_enum_all(uint32_t V, const uint32_t *off, const uint32_t *adj, uint32_t n,
                  const uint32_t *qoff, const uint32_t *qadj, const uint32_t *cand,
                  const uint32_t *cnt, uint32_t cand_stride, const uint32_t *order, uint64_t cap,
-                 int use_fs, double time_limit_ms, orc_stats *st)
+                 int use_fs, double time_limit_ms, uint32_t root_lo, uint32_t root_hi, orc_stats *st)
 {
     memset(st, 0, sizeof(*st));
     if (n < 2 || n > ORC_MAXQ) return -2;
@@ -899,6 +905,7 @@ int orc_enum_all(uint32_t V, const uint32_t *off, const uint32_t *adj, uint32_t 
     uint32_t u0 = order[0];
     for (uint32_t i = 0; i < cnt[u0]; i++) {
         uint32_t v0 = cand[(size_t)u0 * cand_stride + i];
+        if (v0 < root_lo || (root_hi && v0 >= root_hi)) continue;  /* (see orc_enum_cache) */
         c.emb[u0] = v0; c.iemb[u0] = i; c.vis[v0] = 1;  /* depth 0; depth 0 never prunes */
         st->nodes[0]++;
         rec_all(&c, 1);
@@ -913,5 +920,5 @@ int orc_enum_all(uint32_t V, const uint32_t *off, const uint32_t *adj, uint32_t 
     return 0;
 }
 
-uint32_t orc_abi_version(void) { return 3u; }
+uint32_t orc_abi_version(void) { return 4u; }
 uint32_t orc_stats_size(void) { return (uint32_t)sizeof(orc_stats); }

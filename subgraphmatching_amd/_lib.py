@@ -63,7 +63,8 @@ SGM_ENUM_F_SPLIT_ALL = 8
 class EnumOpts(C.Structure):
     _fields_ = [("cap", C.c_uint64), ("method", C.c_int32), ("blocks", C.c_uint32),
                 ("stack_words", C.c_uint32), ("flags", C.c_uint32), ("hub_min_degree", C.c_uint32),
-                ("reserved", C.c_uint32 * 3)]
+                ("time_limit_ms", C.c_uint32), ("root_begin", C.c_uint32), ("root_end", C.c_uint32),
+                ("reserved", C.c_uint32 * 2)]
 
 
 class Stats(C.Structure):
@@ -71,7 +72,7 @@ class Stats(C.Structure):
                 ("fs_prunes", C.c_uint64), ("elems_read", C.c_uint64), ("elems_out", C.c_uint64),
                 ("elems_scanned", C.c_uint64), ("kernel_ms", C.c_double),
                 ("capped", C.c_int32), ("stopped", C.c_int32), ("retries", C.c_uint32),
-                ("reserved", C.c_uint32), ("busy_ns", C.c_uint64)]
+                ("timed_out", C.c_int32), ("busy_ns", C.c_uint64)]
 
 
 class CmSample(C.Structure):

@@ -409,26 +409,20 @@ class ExploreResult:
     capped: bool = False
     retries: int = 0
     stopped: bool = False  # ended early by a stop request: the count is partial
-    busy_ns: int = 0       # wave time spent on the query (cache methods), summed over waves
+    busy_ns: int = 0       # wave time spent on the query, summed over waves
+    timed_out: bool = False  # passed its time_limit_ms work budget: the count is partial
 
 
 def enumerate_query(can: CandidateParam, order, method=L.SGM_ENUM_CACHE_FILTER_FS, cap=None, blocks=0,
-                    stack_words=0, flags=0) -> ExploreResult:
-    opts = L.EnumOpts()
-    opts.cap = (1 << 64) - 1 if cap is None else int(cap)
-    opts.method = method
-    opts.blocks = blocks
-    opts.stack_words = stack_words
-    opts.flags = flags
+                    stack_words=0, flags=0, time_limit_ms=0, roots=None) -> ExploreResult:
+    """``time_limit_ms``: per-query work budget in device wave time (0 = none); a query that
+    passes it comes back with ``timed_out`` set and a partial count.  ``roots`` = (lo, hi):
+    search only the depth-0 candidates with lo <= id < hi (disjoint ranges add up)."""
+    opts = _opts(method, cap, blocks, stack_words, flags, time_limit_ms=time_limit_ms, roots=roots)
     st = L.Stats()
     o = L.u32(order)
     _check(L.lib().sgm_enumerate(can._q, L.ptr(o), C.byref(opts), C.byref(st)), can.ctx)
-    n = can.query.getVertexCnt()
-    res = ExploreResult(time=float(st.kernel_ms), ans=int(st.count), intersections=int(st.intersections),
-                        nodes=[int(x) for x in st.nodes[:n]], fs_prunes=int(st.fs_prunes),
-                        elems_read=int(st.elems_read), elems_out=int(st.elems_out),
-                        elems_scanned=int(st.elems_scanned), capped=bool(st.capped), retries=int(st.retries),
-                        busy_ns=int(st.busy_ns))
+    res = _result(st, can.query.getVertexCnt())
     prof = Profiler.getInst()
     prof.total_intersection += res.intersections
     for d, x in enumerate(res.nodes):
@@ -438,19 +432,16 @@ def enumerate_query(can: CandidateParam, order, method=L.SGM_ENUM_CACHE_FILTER_F
 
 
 def enumerate_batch(cans, orders=None, method=L.SGM_ENUM_CACHE_FILTER_FS, cap=None, blocks=0, stack_words=0,
-                    flags=0):
+                    flags=0, time_limit_ms=0, roots=None):
     """Enumerate many filtered queries of one data graph in one persistent launch.
     ``orders`` None (or a None entry): the RI order, computed by the library.  Returns one
-    ExploreResult per query (``time`` is the whole batch's kernel time)."""
+    ExploreResult per query (``time`` is the whole batch's kernel time).  ``time_limit_ms``:
+    per-query work budget (device wave time); queries past it come back ``timed_out``.
+    ``roots`` = (lo, hi): every query searches only its depth-0 candidates in that id range."""
     if not cans:
         return []
     ctx = cans[0].ctx
-    opts = L.EnumOpts()
-    opts.cap = (1 << 64) - 1 if cap is None else int(cap)
-    opts.method = method
-    opts.blocks = blocks
-    opts.stack_words = stack_words
-    opts.flags = flags
+    opts = _opts(method, cap, blocks, stack_words, flags, time_limit_ms=time_limit_ms, roots=roots)
     nq = len(cans)
     ords = [None if o is None else L.u32(o) for o in orders] if orders is not None else None
     qarr = (C.c_void_p * nq)(*[c._q.value for c in cans])
@@ -460,12 +451,7 @@ def enumerate_batch(cans, orders=None, method=L.SGM_ENUM_CACHE_FILTER_FS, cap=No
     out = []
     prof = Profiler.getInst()
     for can, st in zip(cans, stats):
-        n = can.query.getVertexCnt()
-        r = ExploreResult(time=float(st.kernel_ms), ans=int(st.count), intersections=int(st.intersections),
-                          nodes=[int(x) for x in st.nodes[:n]], fs_prunes=int(st.fs_prunes),
-                          elems_read=int(st.elems_read), elems_out=int(st.elems_out),
-                          elems_scanned=int(st.elems_scanned), capped=bool(st.capped), retries=int(st.retries),
-                        busy_ns=int(st.busy_ns))
+        r = _result(st, can.query.getVertexCnt())
         prof.total_intersection += r.intersections
         out.append(r)
     return out
@@ -489,7 +475,7 @@ def explore_Intersection_All(data: Graph, query: Query, order, cfg: Config,
     return enumerate_query(cfg.can, order, L.SGM_ENUM_AUX_TABLES, output_limit)
 
 
-def _opts(method, cap, blocks=0, stack_words=0, flags=0, hub_min_degree=0):
+def _opts(method, cap, blocks=0, stack_words=0, flags=0, hub_min_degree=0, time_limit_ms=0, roots=None):
     o = L.EnumOpts()
     o.cap = (1 << 64) - 1 if cap is None else int(cap)
     o.method = method
@@ -497,6 +483,12 @@ def _opts(method, cap, blocks=0, stack_words=0, flags=0, hub_min_degree=0):
     o.stack_words = stack_words
     o.flags = flags
     o.hub_min_degree = hub_min_degree
+    o.time_limit_ms = int(time_limit_ms)
+    if roots is not None:
+        lo, hi = int(roots[0]), int(roots[1])
+        if not 0 <= lo < hi <= 0xFFFFFFFF:
+            raise ValueError(f"root range must satisfy 0 <= lo < hi < 2^32 (got {roots})")
+        o.root_begin, o.root_end = lo, hi
     return o
 
 
@@ -505,7 +497,7 @@ def _result(st, n):
                          nodes=[int(x) for x in st.nodes[:n]], fs_prunes=int(st.fs_prunes),
                          elems_read=int(st.elems_read), elems_out=int(st.elems_out),
                          elems_scanned=int(st.elems_scanned), capped=bool(st.capped), retries=int(st.retries),
-                         stopped=bool(st.stopped), busy_ns=int(st.busy_ns))
+                         stopped=bool(st.stopped), busy_ns=int(st.busy_ns), timed_out=bool(st.timed_out))
 
 
 class Batch:
@@ -571,11 +563,13 @@ class Batch:
         return arr, keep
 
     def enumerate(self, orders=None, method=L.SGM_ENUM_CACHE_FILTER_FS, cap=None, blocks=0, stack_words=0,
-                  flags=0, hub_min_degree=0, time_limit_s=None):
+                  flags=0, hub_min_degree=0, time_limit_s=None, time_limit_ms=0, roots=None):
         """With ``time_limit_s``, a stop request ends the launch after that many seconds;
-        queries still running then come back with ``stopped`` set and partial counts."""
+        queries still running then come back with ``stopped`` set and partial counts.
+        ``time_limit_ms``: per-query work budget (device wave time); queries past it come back
+        ``timed_out``."""
         arr, _keep = self._orders(orders)
-        opts = _opts(method, cap, blocks, stack_words, flags, hub_min_degree)
+        opts = _opts(method, cap, blocks, stack_words, flags, hub_min_degree, time_limit_ms, roots)
         stats = (L.Stats * len(self.queries))()
         timer = None
         if time_limit_s:
@@ -592,13 +586,13 @@ class Batch:
         return [_result(st, q.getVertexCnt()) for st, q in zip(stats, self.queries)]
 
     def enumerate_start(self, orders=None, method=L.SGM_ENUM_CACHE_FILTER_FS, cap=None, blocks=0, stack_words=0,
-                        flags=0, hub_min_degree=0):
+                        flags=0, hub_min_degree=0, time_limit_ms=0, roots=None):
         """Launch the loaded batch's enumeration and return at once
         (sgm_batch_enumerate_launch); ``enumerate_wait`` returns the results.  In between,
         another Batch of the same context can filter and launch the next batch, which then
         fills the GPU as this one runs out of work."""
         arr, _keep = self._orders(orders)
-        opts = _opts(method, cap, blocks, stack_words, flags, hub_min_degree)
+        opts = _opts(method, cap, blocks, stack_words, flags, hub_min_degree, time_limit_ms, roots)
         _check(L.lib().sgm_batch_enumerate_launch(self.handle, arr, C.byref(opts)), self.ctx)
 
     def enumerate_wait(self):

@@ -67,7 +67,7 @@ struct sgm_batch {
     hipGraphExec_t gf_nlf[2] = {nullptr, nullptr};            // unlabeled / labelled batches
     std::vector<std::pair<uint32_t, hipGraphExec_t>> gf_gql;  // by 2 * (number of GQL phases) + labelled
     uint32_t* d_lreq = nullptr;                               // slots * kLreqWords: labelled NLF requirements
-    const uint32_t* lab_tables = nullptr;                     // the graph's label table the labelled graphs captured
+    uint64_t lab_gen = ~0ull;                                 // the graph's lab_gen the labelled graphs captured
     std::vector<EnumGraph> ge;
     // an enumeration launched and not yet waited for
     bool pending = false;
@@ -299,7 +299,7 @@ int sgm_batch_filter_labeled(sgm_batch* b, uint32_t nq, const uint32_t* const* q
     uint32_t max_n = 0;
     for (uint32_t i = 0; i < nq; i++) max_n = std::max(max_n, b->q[i].n);
     const uint32_t n_phases = kind == SGM_FILTER_GQL ? 2 * max_n : 0;
-    if (lab && b->lab_tables != g->d_lab) {  // label tables rebuilt since capture: recapture
+    if (lab && b->lab_gen != g->lab_gen) {  // label tables rebuilt since capture: recapture
         if (b->gf_nlf[1]) (void)hipGraphExecDestroy(b->gf_nlf[1]);
         b->gf_nlf[1] = nullptr;
         std::vector<std::pair<uint32_t, hipGraphExec_t>> keep;
@@ -308,7 +308,7 @@ int sgm_batch_filter_labeled(sgm_batch* b, uint32_t nq, const uint32_t* const* q
             else keep.push_back(x);
         }
         b->gf_gql.swap(keep);
-        b->lab_tables = g->d_lab;
+        b->lab_gen = g->lab_gen;
     }
     hipGraphExec_t* gexec = &b->gf_nlf[lab ? 1 : 0];
     if (kind == SGM_FILTER_GQL) {
@@ -424,6 +424,8 @@ int sgm_batch_enumerate_launch(sgm_batch* b, const uint32_t* const* orders, cons
         e.rec = nullptr;
         e.st = b->d_st + i;
         e.cap = (unsigned long long)o.cap;
+        e.budget = budget_ticks(c, o);
+        root_range(o, &e.root_lo, &e.root_hi);
         if (i >= nq) continue;
         const sgm::QueryCSR& q = b->q[i];
         const std::array<uint32_t, SGM_MAXQ>& ord = ords[i];
@@ -435,7 +437,8 @@ int sgm_batch_enumerate_launch(sgm_batch* b, const uint32_t* const* orders, cons
         if (b->status[i] == SGM_OK) {  // the reference's driver reports 0 without enumerating
             e.n_items = g->ge_cnt[std::min<uint32_t>(q.off[ord[0] + 1] - q.off[ord[0]], 32)];
             const uint32_t roots = b->counts[(size_t)i * SGM_MAXQ + ord[0]];
-            if (!(o.flags & SGM_ENUM_F_NO_SPLIT) && ((o.flags & SGM_ENUM_F_SPLIT_ALL) || roots <= SGM_SPLIT_MAX_ROOTS)) {
+            if (!(o.flags & SGM_ENUM_F_NO_SPLIT) &&
+                ((o.flags & SGM_ENUM_F_SPLIT_ALL) || roots <= SGM_SPLIT_MAX_ROOTS || narrow_roots(o))) {
                 e.split = b->d_split + (size_t)SGM_SPLIT_WORDS * i;
                 e.rec = b->d_rec + (size_t)SGM_SPLIT_FIELDS * SGM_SPLIT_ITEMS * i;
                 split = true;
@@ -492,8 +495,8 @@ int sgm_batch_enumerate_launch(sgm_batch* b, const uint32_t* const* orders, cons
             sgm::launch_last_bits(b->stream, b->d_eq, slots, V);
             if (split) sgm::launch_split_roots(b->stream, g->d_off, g->d_bydeg, b->d_eq, slots);
             sgm::launch_enum_cache(b->stream, blocks, g->d_off, g->d_adj, g->d_bydeg, b->d_eq, slots, bucket, b->d_stack,
-                                   (uint32_t)words, c->d_stop, use_fs, hub, g->d_hub_bits, g->hub_words, serial,
-                                   split);
+                                   (uint32_t)words, c->d_stop, c->d_hstop, use_fs, hub, g->d_hub_bits, g->hub_words,
+                                   serial, split);
             if (split) sgm::launch_split_reduce(b->stream, b->d_eq, slots);
         }, &exec);
         if (rc) return rc;
@@ -504,7 +507,7 @@ int sgm_batch_enumerate_launch(sgm_batch* b, const uint32_t* const* orders, cons
     // (a stop request then ends all of them)
     if (c->enum_inflight == 0) {
         sgm::launch_fill_u32(b->stream, c->d_stop, 1, 0);
-        c->stop_requested = 0;
+        clear_host_stop(c);
     }
     HIPCHK(c, hipEventRecord(b->ev[2], b->stream));
     HIPCHK(c, hipGraphLaunch(exec, b->stream));
@@ -572,8 +575,8 @@ int sgm_batch_enumerate_wait(sgm_batch* b, sgm_stats* stats)
         const uint32_t ns = (uint32_t)sub.size();
         sgm::launch_split_roots(b->stream, g->d_off, g->d_bydeg, b->d_eq, ns);  // no-op without split blocks
         sgm::launch_enum_cache(b->stream, blocks, g->d_off, g->d_adj, g->d_bydeg, b->d_eq, ns, bucket, c->d_stack,
-                               (uint32_t)words, c->d_stop, b->pend_use_fs, b->pend_hub, g->d_hub_bits, g->hub_words,
-                               b->pend_serial, b->pend_split);
+                               (uint32_t)words, c->d_stop, c->d_hstop, b->pend_use_fs, b->pend_hub, g->d_hub_bits,
+                               g->hub_words, b->pend_serial, b->pend_split);
         sgm::launch_split_reduce(b->stream, b->d_eq, ns);
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipEventRecord(b->ev[3], b->stream));

@@ -47,10 +47,20 @@ int sgm_ctx_create(int device, sgm_ctx** out)
         dalloc(&c->d_work64, 1) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipEventCreate(&c->fev[0]) != hipSuccess || hipEventCreate(&c->fev[1]) != hipSuccess ||
-        hipEventCreate(&c->fev[2]) != hipSuccess || hipEventCreate(&c->fev[3]) != hipSuccess) {
+        hipEventCreate(&c->fev[2]) != hipSuccess || hipEventCreate(&c->fev[3]) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&c->h_stop), 64, hipHostMallocMapped | hipHostMallocCoherent) !=
+            hipSuccess) {
+        c->h_stop = nullptr;
         sgm_ctx_destroy(c);
         return SGM_EHIP;
     }
+    void* dp = nullptr;
+    if (hipHostGetDevicePointer(&dp, c->h_stop, 0) != hipSuccess) {
+        sgm_ctx_destroy(c);
+        return SGM_EHIP;
+    }
+    c->d_hstop = static_cast<const uint32_t*>(dp);
+    __atomic_store_n(c->h_stop, 0u, __ATOMIC_SEQ_CST);
     *out = c;
     return SGM_OK;
 }
@@ -67,6 +77,7 @@ void sgm_ctx_destroy(sgm_ctx* c)
     dfree(c->d_fcnt);
     if (c->h_fcnt) (void)hipHostFree(c->h_fcnt);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
+    if (c->h_stop) (void)hipHostFree(c->h_stop);
     dfree(c->d_lastbits);
     dfree(c->d_split);
     dfree(c->d_rec);
@@ -108,10 +119,9 @@ int sgm_ctx_filter_ms(const sgm_ctx* c, double* nlf_ms, double* gql_ms, double* 
 int sgm_stop(sgm_ctx* c)
 {
     if (!c) return SGM_EINVAL;
-    static const uint32_t one = 1;
+    // a plain store: the running kernels poll the word and raise their device stop word
     c->stop_requested = 1;
-    HIPCHK(c, hipMemcpyAsync(c->d_stop, &one, sizeof one, hipMemcpyHostToDevice, c->ctl));
-    HIPCHK(c, hipStreamSynchronize(c->ctl));
+    __atomic_store_n(c->h_stop, 1u, __ATOMIC_SEQ_CST);
     return SGM_OK;
 }
 
@@ -267,6 +277,7 @@ extern "C++" int build_label_tables(sgm_ctx* c, sgm_graph* g, const uint32_t* la
     HIPCHK(c, hipMemcpy(g->d_run_lab, rlab.data(), sizeof(uint32_t) * rlab.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(g->d_run_cnt, rcnt.data(), sizeof(uint32_t) * rcnt.size(), hipMemcpyHostToDevice));
     g->has_lab = true;
+    g->lab_gen++;  // captured filter graphs that read the old tables must be recaptured
     g->labelled = false;
     for (uint32_t v = 0; v < V && !g->labelled; v++) g->labelled = lab[v] != 0;
     return SGM_OK;
@@ -280,12 +291,14 @@ int sgm_graph_set_labels(sgm_graph* g, const uint32_t* labels, uint32_t num_vert
         return set_err(c, SGM_EINVAL, "expected %u labels, got %u", g->V, num_vertices);
     bool any = false;
     for (uint32_t v = 0; labels && v < num_vertices && !any; v++) any = labels[v] != 0;
+    g->label_version++;  // queries filtered under the old labels are stale
     if (!any) {  // unlabeled: the tables go; a labelled query rebuilds them with every label 0
         for (uint32_t** p : {&g->d_lab, &g->d_run_off, &g->d_run_lab, &g->d_run_cnt}) {
             dfree(*p);
             *p = nullptr;
         }
         g->has_lab = false;
+        g->lab_gen++;
         g->labelled = false;
         return SGM_OK;
     }
@@ -621,6 +634,7 @@ int sgm_filter_batch(sgm_query* const* qs, uint32_t nq, int kind, int32_t* statu
     for (uint32_t i = 0; i < nq; i++) {
         sgm_query* q = qs[i];
         q->filtered = kind;
+        q->filter_label_version = g->label_version;
         q->empty = false;
         for (uint32_t u = 0; u < q->q.n; u++) q->empty |= q->counts[u] == 0;
         if (status) status[i] = q->empty ? SGM_EEMPTY : SGM_OK;
@@ -703,7 +717,9 @@ int sgm_build_tables(sgm_query* q)
 {
     if (!q) return SGM_EINVAL;
     sgm_ctx* c = q->g->ctx;
-    int rc = ensure_compacted(q);
+    int rc = check_filtered(q, "tables");
+    if (rc) return rc;
+    rc = ensure_compacted(q);
     if (rc) return rc;
     const uint32_t n = q->q.n, W = (q->g->V + 63) / 64;
     free_tables(q);
@@ -994,11 +1010,20 @@ uint32_t default_blocks(const sgm_ctx* c, const sgm_enum_opts& o, uint32_t nq)
     return std::max<uint32_t>(1, std::min<uint32_t>(b, 1u << 16));
 }
 
+unsigned long long budget_ticks(const sgm_ctx* c, const sgm_enum_opts& o)
+{
+    if (!o.time_limit_ms) return 0;
+    const double t = (double)o.time_limit_ms * 1e6 / c->wall_ns_per_tick;
+    return t >= 1.8e19 ? ~0ull : std::max<unsigned long long>(1, (unsigned long long)t);
+}
+
 void stats_out(const sgm_stats_dev& st, uint64_t cap, sgm_stats* out, double ns_per_tick)
 {
     out->busy_ns = (uint64_t)((double)st.busy_ticks * ns_per_tick);
     out->count = st.count < cap ? st.count : cap;
     out->capped = st.count >= cap;
+    // a query that reached its cap has its result, whatever its clock says
+    out->timed_out = (st.timed_out && !out->capped) ? 1 : 0;
     out->intersections = st.intersections;
     for (uint32_t d = 0; d < SGM_MAXQ; d++) out->nodes[d] = st.nodes[d];
     out->fs_prunes = st.fs_prunes;
@@ -1060,6 +1085,8 @@ static int enumerate_aux_batch(sgm_query* const* qs, const uint32_t* const* orde
         }
     }
     const int use_fs = o.method == SGM_ENUM_AUX_TABLES_FS ? 1 : 0;
+    uint32_t rlo, rhi;
+    root_range(o, &rlo, &rhi);
     std::vector<uint32_t> live;
     for (uint32_t i = 0; i < nq; i++)
         if (!qs[i]->empty) live.push_back(i);  // the reference's driver reports 0 without enumerating
@@ -1091,10 +1118,11 @@ static int enumerate_aux_batch(sgm_query* const* qs, const uint32_t* const* orde
         HIPCHK(c, hipMemsetAsync(c->d_astop, 0, sizeof(uint32_t) * m, c->stream));
         HIPCHK(c, hipMemsetAsync(c->d_work, 0, sizeof(uint32_t), c->stream));
         HIPCHK(c, hipMemsetAsync(c->d_stop, 0, sizeof(uint32_t), c->stream));
-        c->stop_requested = 0;
+        clear_host_stop(c);
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
         sgm::launch_enum_aux(c->stream, blocks, c->d_auxplans, c->d_afirst, m, c->d_stack, (uint32_t)words, c->d_work,
-                             c->d_astats, c->d_astop, c->d_stop, (unsigned long long)o.cap, use_fs);
+                             c->d_astats, c->d_astop, c->d_stop, c->d_hstop, (unsigned long long)o.cap,
+                             budget_ticks(c, o), rlo, rhi, use_fs);
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
         std::vector<sgm_stats_dev> got(m);
@@ -1141,7 +1169,7 @@ int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uin
         for (uint32_t i = 0; i < nq; i++) {
             sgm_query* q = qs[i];
             if (q->g != g) return set_err(c, SGM_EINVAL, "queries of a batch must share one graph");
-            if (!q->filtered) return set_err(c, SGM_ESTATE, "filter query %u first", i);
+            if (const int frc = check_filtered(q, "enumerate")) return frc;
             if (q->q.n < 2) return set_err(c, SGM_EINVAL, "enumeration needs at least 2 query vertices (query %u)", i);
             if (orders && orders[i]) std::copy(orders[i], orders[i] + q->q.n, ords[i].begin());
             else sgm::order_ri(q->q, ords[i].data());
@@ -1182,7 +1210,7 @@ int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uin
     for (uint32_t i = 0; i < nq; i++) {
         sgm_query* q = qs[i];
         if (q->g != g) return set_err(c, SGM_EINVAL, "queries of a batch must share one graph");
-        if (!q->filtered) return set_err(c, SGM_ESTATE, "filter query %u first", i);
+        if (const int frc = check_filtered(q, "enumerate")) return frc;
         if (q->q.n < 2) return set_err(c, SGM_EINVAL, "enumeration needs at least 2 query vertices (query %u)", i);
     }
     // orders and plans of large launches on host threads; errors reported in query order
@@ -1237,7 +1265,7 @@ int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uin
     for (uint32_t i : live) {
         const uint32_t roots = qs[i]->counts[ord[i][0]];
         want_split[i] = !(o.flags & SGM_ENUM_F_NO_SPLIT) &&
-                        ((o.flags & SGM_ENUM_F_SPLIT_ALL) || roots <= SGM_SPLIT_MAX_ROOTS);
+                        ((o.flags & SGM_ENUM_F_SPLIT_ALL) || roots <= SGM_SPLIT_MAX_ROOTS || narrow_roots(o));
         n_split += want_split[i] ? 1u : 0u;
     }
     const bool split = n_split > 0;
@@ -1277,6 +1305,8 @@ int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uin
             }
             eq[k].st = c->d_qstats + k;
             eq[k].cap = (unsigned long long)o.cap;
+            eq[k].budget = budget_ticks(c, o);
+            root_range(o, &eq[k].root_lo, &eq[k].root_hi);
             eq[k].n_items = g->ge_cnt[std::min<uint32_t>(q->q.off[u0 + 1] - q->q.off[u0], 32)];
             eq[k].root_u = u0;
         }
@@ -1284,13 +1314,13 @@ int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uin
         HIPCHK(c, hipMemsetAsync(c->d_qstats, 0, sizeof(sgm_stats_dev) * m, c->stream));
         HIPCHK(c, hipMemsetAsync(c->d_qstop, 0, sizeof(uint32_t) * 3 * m, c->stream));
         HIPCHK(c, hipMemsetAsync(c->d_stop, 0, sizeof(uint32_t), c->stream));
-        c->stop_requested = 0;
+        clear_host_stop(c);
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
         sgm::launch_last_bits(c->stream, c->d_eq, m, g->V);
         if (split) sgm::launch_split_roots(c->stream, g->d_off, g->d_bydeg, c->d_eq, m);
         sgm::launch_enum_cache(c->stream, blocks, g->d_off, g->d_adj, g->d_bydeg, c->d_eq, m, max_ops, c->d_stack,
-                               (uint32_t)words, c->d_stop, use_fs, hub_slot, g->d_hub_bits, g->hub_words, serial,
-                               split);
+                               (uint32_t)words, c->d_stop, c->d_hstop, use_fs, hub_slot, g->d_hub_bits, g->hub_words,
+                               serial, split);
         if (split) sgm::launch_split_reduce(c->stream, c->d_eq, m);
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
@@ -1328,7 +1358,7 @@ int sgm_enumerate(sgm_query* q, const uint32_t* order, const sgm_enum_opts* opts
     const sgm_enum_opts o = normalize_opts(opts);
     if (o.method == SGM_ENUM_AUX_TABLES || o.method == SGM_ENUM_AUX_TABLES_FS) {
         std::memset(out, 0, sizeof *out);
-        if (!q->filtered) return set_err(c, SGM_ESTATE, "filter the query first");
+        if (const int frc = check_filtered(q, "enumerate")) return frc;
         if (q->q.n < 2) return set_err(c, SGM_EINVAL, "enumeration needs at least 2 query vertices");
         sgm::Plan pl;
         const int prc = sgm::build_plan(q->q, order, pl);

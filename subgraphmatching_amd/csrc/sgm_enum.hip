@@ -164,6 +164,33 @@ __device__ __forceinline__ void flush_count(unsigned long long& pending, unsigne
     pending = 0;
 }
 
+// Copies the host's stop request (a word of pinned host memory) into the device stop word
+// that every wave polls, at most once per SGM_RELAY_TICKS of this wave's clock.
+__device__ __forceinline__ void relay_stop(uint64_t now, uint64_t& relay_at, const uint32_t* host_stop,
+                                           uint32_t* dev_stop)
+{
+    if (now < relay_at) return;
+    relay_at = now + SGM_RELAY_TICKS;
+    const uint32_t h = uni(__hip_atomic_load(host_stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
+    if (h && lane_id() == 0) atomicExch(dev_stop, 1u);
+}
+
+// Adds a wave's unpublished time on a query (busy, ticks) to the query's total; once the total
+// passes the query's budget, the query is marked timed out and its stop word raised (its
+// waves leave it at their next poll).
+__device__ __forceinline__ void charge_time(sgm_stats_dev* st, uint32_t* stop, unsigned long long budget,
+                                            unsigned long long& busy)
+{
+    unsigned long long prev = 0;
+    if (lane_id() == 0) prev = atomicAdd(&st->busy_ticks, busy);
+    prev = __shfl(prev, 0);
+    if (prev + busy > budget && lane_id() == 0) {
+        atomicExch(&st->timed_out, 1ull);
+        atomicExch(stop, 1u);
+    }
+    busy = 0;
+}
+
 // LDS state is per wave, so its values are wave-uniform; reading them through readfirstlane
 // keeps them in scalar registers (a plain LDS load lands in a vector register).
 __device__ __forceinline__ uint64_t uni64(uint64_t x)
@@ -301,7 +328,8 @@ __global__ void __launch_bounds__(64) k_split_roots(const uint32_t* __restrict__
     if (Q.plan->n > 2) {  // two-vertex queries have no depth-1 loop to split
         for (uint32_t base = 0; base < lim && found < Rn; base += 64) {
             const uint32_t p = base + lane;
-            const bool c = p < lim && ((Q.vmask[items[p]] >> Q.root_u) & 1u);
+            const uint32_t v = p < lim ? items[p] : 0u;
+            const bool c = p < lim && ((Q.vmask[v] >> Q.root_u) & 1u) && v >= Q.root_lo && v < Q.root_hi;
             const uint64_t b = __ballot(c);
             const uint32_t r = found + mbcnt64(b);
             if (c && r < Rn) rpos[r] = p;
@@ -744,6 +772,7 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
                                                     const sgm_equery* __restrict__ qs, uint32_t nq,
                                                     uint32_t max_ops,
                                                     uint32_t* __restrict__ stack, uint32_t stack_words,
+                                                    uint32_t* __restrict__ dev_stop,
                                                     const uint32_t* __restrict__ host_stop, int use_fs,
                                                     const uint32_t* __restrict__ hub_slot,
                                                     const uint32_t* __restrict__ hub_bits, uint32_t hub_words,
@@ -777,6 +806,7 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
     uint32_t* stop = nullptr;
     bool overflow = false;
     uint32_t poll = 0;
+    uint64_t relay_at = 0;
 
     // Visiting order (L2 locality): workgroups are dispatched round-robin over the 8 XCDs, so
     // blocks with the same blockIdx.x mod 8 share an L2.  Queries are split into 8 classes
@@ -789,7 +819,8 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
     uint32_t cls = blockIdx.x % ncls;
     uint32_t k = (gwave / ncls) % csize(cls), seen = 0, classes = 0;
     while (classes < ncls) {
-        if (__atomic_load_n(host_stop, __ATOMIC_RELAXED)) break;
+        if (__atomic_load_n(dev_stop, __ATOMIC_RELAXED)) break;
+        relay_stop(wall_clock64(), relay_at, host_stop, dev_stop);
         const uint32_t q = cls + ncls * k;
         const sgm_equery& Q = qs[q];
         uint32_t it = SGM_INVALID;
@@ -862,9 +893,10 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
             v0 = items[it];
             if (SPLIT) S.sp_lo = SGM_INVALID;
         }
-        if (!((vmask[v0] >> root_u) & 1u)) continue;  // not (or no longer) a candidate of u0
+        // not (or no longer) a candidate of u0, or outside the launch's root range
+        if (!((vmask[v0] >> root_u) & 1u) || v0 < Q.root_lo || v0 >= Q.root_hi) continue;
         acc.items_done++;
-        const uint64_t t_item = wall_clock64();
+        uint64_t t_mark = wall_clock64();  // the item's time up to here is in acc.busy
         S.emb[0] = v0;
         embv = lane == 0 ? v0 : embv;
         if (lo1 == 0) S.nodes[0] += 1;  // a split root's node is counted by its first chunk
@@ -912,7 +944,14 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
                 uint32_t d = 1;
                 while (true) {
                     if ((++poll & 3u) == 0) {
-                        bool end = __atomic_load_n(stop, __ATOMIC_RELAXED) | __atomic_load_n(host_stop, __ATOMIC_RELAXED);
+                        bool end = __atomic_load_n(stop, __ATOMIC_RELAXED) | __atomic_load_n(dev_stop, __ATOMIC_RELAXED);
+                        if ((poll & 255u) == 0) {  // every 256 steps: the clock (stop relay, time budget)
+                            const uint64_t now = wall_clock64();
+                            relay_stop(now, relay_at, host_stop, dev_stop);
+                            acc.busy += now - t_mark;
+                            t_mark = now;
+                            if (Q.budget && acc.busy >= SGM_CHARGE_TICKS) charge_time(Q.st, stop, Q.budget, acc.busy);
+                        }
                         const uint32_t slo = SPLIT ? uni(S.sp_lo) : SGM_INVALID;
                         if (slo != SGM_INVALID) {  // an earlier chunk of this split root was cut off
                             const uint32_t* cw = reinterpret_cast<const uint32_t*>(
@@ -1094,7 +1133,8 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
                 if (cut1) atomicMin(S.sp_cut, S.sp_lo);
             }
         }
-        acc.busy += wall_clock64() - t_item;
+        acc.busy += wall_clock64() - t_mark;
+        if (Q.budget && acc.busy >= SGM_CHARGE_TICKS) charge_time(Q.st, stop, Q.budget, acc.busy);
         flush_count(acc.cnt_pending, &Q.st->count, stop, cap);
         if (overflow) {
             // this query's result is void (its stop word is raised); the host re-runs it
@@ -1198,8 +1238,10 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
                                                   const uint32_t* __restrict__ item_first, uint32_t nq,
                                                   uint32_t* __restrict__ stack, uint32_t stack_words,
                                                   uint32_t* __restrict__ work, sgm_stats_dev* __restrict__ stats,
-                                                  uint32_t* __restrict__ stops, const uint32_t* __restrict__ host_stop,
-                                                  unsigned long long cap, int use_fs)
+                                                  uint32_t* __restrict__ stops, uint32_t* __restrict__ dev_stop,
+                                                  const uint32_t* __restrict__ host_stop, unsigned long long cap,
+                                                  unsigned long long budget, uint32_t root_lo, uint32_t root_hi,
+                                                  int use_fs)
 {
     extern __shared__ uint64_t smem_u64[];
     uint8_t* smem = reinterpret_cast<uint8_t*>(smem_u64);
@@ -1208,8 +1250,10 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
     uint32_t* ws = stack + (size_t)(blockIdx.x * (blockDim.x >> 6) + wave) * stack_words;
     for (uint32_t d = 0; d < SGM_MAXQ; d++) S.nodes[d] = 0;
     unsigned long long inter = 0, elems = 0, outs = 0, scanned = 0, cnt_pending = 0, items_done = 0, fsp = 0;
+    unsigned long long busy = 0;  // wave time on the current query not yet added to its total
     bool overflow = false;
     uint32_t poll = 0;
+    uint64_t relay_at = 0;
     uint32_t cur = SGM_INVALID, n = 0, L = 0;
     const sgm_auxplan_dev* Ap = plans;
     const uint64_t* anc = nullptr;
@@ -1227,11 +1271,12 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
             if (scanned) atomicAdd(&gst->elems_scanned, scanned);
             if (items_done) atomicAdd(&gst->items_done, items_done);
             if (fsp) atomicAdd(&gst->fs_prunes, fsp);
+            if (busy) atomicAdd(&gst->busy_ticks, busy);
             if (ovf) { atomicAdd(&gst->overflow, 1ull); atomicExch(stop, 1u); }
             for (uint32_t dd = 0; dd < n; dd++)
                 if (S.nodes[dd]) atomicAdd(&gst->nodes[dd], S.nodes[dd]);
         }
-        inter = elems = outs = scanned = items_done = fsp = 0;
+        inter = elems = outs = scanned = items_done = fsp = busy = 0;
         for (uint32_t dd = 0; dd < SGM_MAXQ; dd++) S.nodes[dd] = 0;
     };
     const uint32_t total = item_first[nq];
@@ -1240,7 +1285,8 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
         if (lane == 0) it = atomicAdd(work, 1u);
         it = uni(__shfl((int)it, 0));
         if (it >= total) break;
-        if (__atomic_load_n(host_stop, __ATOMIC_RELAXED)) break;
+        if (__atomic_load_n(dev_stop, __ATOMIC_RELAXED)) break;
+        relay_stop(wall_clock64(), relay_at, host_stop, dev_stop);
         // the item's query: the last q with item_first[q] <= it
         uint32_t lo = 0, hi = nq;
         while (hi - lo > 1) {
@@ -1262,9 +1308,12 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
         if (__atomic_load_n(stop, __ATOMIC_RELAXED)) continue;  // capped (or voided) query
         const sgm_auxplan_dev& A = *Ap;
         const uint32_t root = it - item_first[q];
+        const uint32_t v0 = A.cand[0][root];
+        if (v0 < root_lo || v0 >= root_hi) continue;  // outside the launch's root range
         items_done++;
+        uint64_t t_mark = wall_clock64();
         bool aborted = false;
-        S.emb[0] = A.cand[0][root];
+        S.emb[0] = v0;
         S.iemb[0] = root;
         S.nodes[0] += 1;
         S.stack_top[1] = 0;
@@ -1290,10 +1339,18 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
         // enter depth d (d <= L): build the list; at d == L count directly
         bool entering = true;
         while (true) {
-            if ((++poll & 63u) == 0 &&
-                (__atomic_load_n(stop, __ATOMIC_RELAXED) | __atomic_load_n(host_stop, __ATOMIC_RELAXED))) {
-                aborted = true;
-                break;
+            if ((++poll & 63u) == 0) {
+                if ((poll & 1023u) == 0) {  // every 1024 steps: the clock (stop relay, time budget)
+                    const uint64_t now = wall_clock64();
+                    relay_stop(now, relay_at, host_stop, dev_stop);
+                    busy += now - t_mark;
+                    t_mark = now;
+                    if (budget && busy >= SGM_CHARGE_TICKS) charge_time(gst, stop, budget, busy);
+                }
+                if (__atomic_load_n(stop, __ATOMIC_RELAXED) | __atomic_load_n(dev_stop, __ATOMIC_RELAXED)) {
+                    aborted = true;
+                    break;
+                }
             }
             if (entering) {
                 entering = false;
@@ -1398,6 +1455,8 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
             d++;
             entering = true;
         }
+        busy += wall_clock64() - t_mark;
+        if (budget && busy >= SGM_CHARGE_TICKS) charge_time(gst, stop, budget, busy);
         flush_count(cnt_pending, &gst->count, stop, cap);
         if (overflow) {
             flush_all(true);
@@ -1436,26 +1495,27 @@ void launch_split_reduce(hipStream_t s, const sgm_equery* qs, uint32_t nq)
 
 void launch_enum_cache(hipStream_t s, uint32_t blocks, const uint32_t* off, const uint32_t* adj, const uint32_t* items,
                        const sgm_equery* qs, uint32_t nq, uint32_t max_ops, uint32_t* stack, uint32_t stack_words,
-                       const uint32_t* host_stop, int use_fs, const uint32_t* hub_slot, const uint32_t* hub_bits,
-                       uint32_t hub_words, int serial_last, bool split)
+                       uint32_t* dev_stop, const uint32_t* host_stop, int use_fs, const uint32_t* hub_slot,
+                       const uint32_t* hub_bits, uint32_t hub_words, int serial_last, bool split)
 {
     if (split)
         hipLaunchKernelGGL(k_enum_cache<true>, dim3(blocks), dim3(256), 4 * enum_wave_bytes(max_ops), s, off, adj, items,
-                           qs, nq, max_ops, stack, stack_words, host_stop, use_fs, hub_slot, hub_bits, hub_words,
-                           serial_last);
+                           qs, nq, max_ops, stack, stack_words, dev_stop, host_stop, use_fs, hub_slot, hub_bits,
+                           hub_words, serial_last);
     else
         hipLaunchKernelGGL(k_enum_cache<false>, dim3(blocks), dim3(256), 4 * enum_wave_bytes(max_ops), s, off, adj,
-                           items, qs, nq, max_ops, stack, stack_words, host_stop, use_fs, hub_slot, hub_bits, hub_words,
-                           serial_last);
+                           items, qs, nq, max_ops, stack, stack_words, dev_stop, host_stop, use_fs, hub_slot, hub_bits,
+                           hub_words, serial_last);
 }
 
 void launch_enum_aux(hipStream_t s, uint32_t blocks, const sgm_auxplan_dev* plans, const uint32_t* item_first,
                      uint32_t nq, uint32_t* stack, uint32_t stack_words, uint32_t* work, sgm_stats_dev* st,
-                     uint32_t* stops, const uint32_t* host_stop, unsigned long long cap, int use_fs)
+                     uint32_t* stops, uint32_t* dev_stop, const uint32_t* host_stop, unsigned long long cap,
+                     unsigned long long budget, uint32_t root_lo, uint32_t root_hi, int use_fs)
 {
     const size_t smem = 4 * ((sizeof(WaveAux) + 15) & ~(size_t)15);
     hipLaunchKernelGGL(k_enum_aux, dim3(blocks), dim3(256), smem, s, plans, item_first, nq, stack, stack_words, work,
-                       st, stops, host_stop, cap, use_fs);
+                       st, stops, dev_stop, host_stop, cap, budget, root_lo, root_hi, use_fs);
 }
 
 }  // namespace sgm

@@ -80,6 +80,7 @@ struct sgm_stats_dev {
     unsigned long long overflow;        // per-wave stack overflows (result invalid if > 0)
     unsigned long long items_done;
     unsigned long long busy_ticks;      // wall-clock ticks its waves spent on the query's items
+    unsigned long long timed_out;       // the waves' time passed the query's budget (count partial)
 };
 
 // Depth-1 splitting of the heaviest roots.  The first SGM_SPLIT_ROOTS candidate roots of a
@@ -121,9 +122,21 @@ struct sgm_equery {
     uint32_t* snext;                // next split chunk
     unsigned long long* rec;        // SGM_SPLIT_FIELDS * SGM_SPLIT_ITEMS chunk records
     unsigned long long cap;
+    unsigned long long budget;      // wall-clock ticks of wave time the query may use (0 = no limit)
     uint32_t n_items;
     uint32_t root_u;
+    uint32_t root_lo, root_hi;      // only depth-0 candidates v with root_lo <= v < root_hi are searched
 };
+
+// The host's stop request lives in pinned host memory and is written by a plain CPU store
+// (a queued copy could not start while a persistent grid holds every CU).  Waves poll the
+// device stop word; each wave copies the host word into it at most once per
+// SGM_RELAY_TICKS of the device clock (100 MHz: 2.6 ms), so a full grid of 4096 waves reads
+// host memory about 1.6 million times per second.
+#define SGM_RELAY_TICKS (1ull << 18)
+// A wave adds its time on a query to the query's total (the budget check) after at most
+// SGM_CHARGE_TICKS of unpublished time (100 MHz: 82 us).
+#define SGM_CHARGE_TICKS (1ull << 13)
 
 // Per-query description used by the batched filter kernels.
 //   flags[0 .. 2*SGM_MAXQ)   removals per GQL phase (round*n + u)

@@ -61,9 +61,13 @@ void launch_table_fill(hipStream_t s, const uint32_t* off, const uint32_t* adj, 
                        const sgm_tabdesc* ds, uint32_t m, uint32_t max_rows);
 // sgm_enum.hip
 size_t enum_smem_bytes(uint32_t max_ops);
+// dev_stop: the device stop word every wave polls; host_stop: the host's request (pinned host
+// memory), copied into dev_stop by the polling waves; budget: wave-time ticks per query (0 = none);
+// only depth-0 candidates v with root_lo <= v < root_hi are searched
 void launch_enum_aux(hipStream_t s, uint32_t blocks, const sgm_auxplan_dev* plans, const uint32_t* item_first,
                      uint32_t nq, uint32_t* stack, uint32_t stack_words, uint32_t* work, sgm_stats_dev* st,
-                     uint32_t* stops, const uint32_t* host_stop, unsigned long long cap, int use_fs);
+                     uint32_t* stops, uint32_t* dev_stop, const uint32_t* host_stop, unsigned long long cap,
+                     unsigned long long budget, uint32_t root_lo, uint32_t root_hi, int use_fs);
 // bitmap of C(u_L) per query (Q.lastbits) from its candidate words; before launch_enum_cache
 void launch_last_bits(hipStream_t s, const sgm_equery* qs, uint32_t nq, uint32_t V);
 // depth-1 split tables (queries with Q.split set), before launch_enum_cache; and the chunk
@@ -72,8 +76,8 @@ void launch_split_roots(hipStream_t s, const uint32_t* off, const uint32_t* item
 void launch_split_reduce(hipStream_t s, const sgm_equery* qs, uint32_t nq);
 void launch_enum_cache(hipStream_t s, uint32_t blocks, const uint32_t* off, const uint32_t* adj, const uint32_t* items,
                        const sgm_equery* qs, uint32_t nq, uint32_t max_ops, uint32_t* stack, uint32_t stack_words,
-                       const uint32_t* host_stop, int use_fs, const uint32_t* hub_slot, const uint32_t* hub_bits,
-                       uint32_t hub_words, int serial_last, bool split);
+                       uint32_t* dev_stop, const uint32_t* host_stop, int use_fs, const uint32_t* hub_slot,
+                       const uint32_t* hub_bits, uint32_t hub_words, int serial_last, bool split);
 void launch_hub_bits(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* hubs, uint32_t n_hubs,
                      uint32_t* bits, uint32_t words);
 }  // namespace sgm

@@ -20,7 +20,11 @@ struct sgm_ctx {
     hipStream_t stream = nullptr;
     hipStream_t ctl = nullptr;       // non-blocking stream for sgm_stop
     std::string err;
-    uint32_t* d_stop = nullptr;      // stop word polled by the enumeration kernel
+    uint32_t* d_stop = nullptr;      // stop word polled by the enumeration kernels
+    // the host's stop request: one word of pinned, coherent host memory (h_stop as the host
+    // writes it, d_hstop as the kernels read it), relayed into d_stop by the polling waves
+    uint32_t* h_stop = nullptr;
+    const uint32_t* d_hstop = nullptr;
     uint32_t* d_work = nullptr;      // work counter
     sgm_stats_dev* d_stats = nullptr;
     sgm_auxplan_dev* d_auxplan = nullptr;   // (unused since the batched aux search; kept allocated)
@@ -102,6 +106,8 @@ struct sgm_graph {
     // sgm_graph_set_labels, or with every label 0 for the first labelled query.
     bool labelled = false;  // some label is not 0
     bool has_lab = false;   // the label tables are on the device
+    uint64_t lab_gen = 0;        // bumped whenever the label tables are rebuilt or dropped
+    uint64_t label_version = 0;  // bumped whenever sgm_graph_set_labels changes the labels
     uint32_t* d_lab = nullptr;
     uint32_t* d_run_off = nullptr;
     uint32_t* d_run_lab = nullptr;
@@ -132,6 +138,7 @@ struct sgm_query {
     sgm::QueryCSR q;
     uint32_t nbr[SGM_MAXQ] = {0};
     int filtered = 0;                 // SGM_FILTER_* after a successful or empty filter
+    uint64_t filter_label_version = 0; // the graph's label_version the candidates were filtered under
     bool empty = false;
     bool compacted = false;
     uint32_t counts[SGM_MAXQ] = {0};
@@ -214,6 +221,15 @@ inline void* host_stage(sgm_ctx* c, size_t bytes)
 
 inline size_t stage_align(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// SGM_ESTATE unless q's candidates are filtered under the graph's current labels.
+inline int check_filtered(sgm_query* q, const char* what)
+{
+    if (!q->filtered) return set_err(q->g->ctx, SGM_ESTATE, "%s: filter the query first", what);
+    if (q->filter_label_version != q->g->label_version)
+        return set_err(q->g->ctx, SGM_ESTATE, "%s: the graph's labels changed since the query was filtered", what);
+    return SGM_OK;
+}
+
 // (Re)allocate a context buffer so that it holds at least `need` elements.
 template <class T>
 inline int grow(sgm_ctx* c, T** p, uint32_t& cap_now, uint32_t need)
@@ -243,3 +259,24 @@ int ensure_stack(sgm_ctx* c, size_t bytes);
 sgm_enum_opts normalize_opts(const sgm_enum_opts* opts);
 uint32_t default_blocks(const sgm_ctx* c, const sgm_enum_opts& o, uint32_t nq);
 void stats_out(const sgm_stats_dev& st, uint64_t cap, sgm_stats* out, double ns_per_tick);
+// Device-clock ticks of a per-query time budget (sgm_enum_opts.time_limit_ms; 0 = none).
+unsigned long long budget_ticks(const sgm_ctx* c, const sgm_enum_opts& o);
+// The root range of opts as [lo, hi) (root_end == 0: every vertex).
+inline void root_range(const sgm_enum_opts& o, uint32_t* lo, uint32_t* hi)
+{
+    *lo = o.root_begin;
+    *hi = o.root_end ? o.root_end : 0xFFFFFFFFu;
+}
+// A root range of at most SGM_SPLIT_MAX_ROOTS ids holds at most that many roots: its queries
+// get depth-1 splitting, so a heavy root in the range is not left to a single wave.
+inline bool narrow_roots(const sgm_enum_opts& o)
+{
+    return o.root_end && o.root_end - o.root_begin <= SGM_SPLIT_MAX_ROOTS;
+}
+// Clears the host's stop request before a launch (only when no launch is in flight: a
+// request then ends every one of them).
+inline void clear_host_stop(sgm_ctx* c)
+{
+    __atomic_store_n(c->h_stop, 0u, __ATOMIC_SEQ_CST);
+    c->stop_requested = 0;
+}

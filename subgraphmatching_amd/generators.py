@@ -131,8 +131,14 @@ def _is_dense(k: int, e: int) -> bool:
     return e >= k + 1 if k == 4 else e >= k
 
 
+def _walk_arrays(g: Graph):
+    off = g.off.astype(np.int64)
+    deg = np.diff(off)
+    return off, g.adj, deg, np.flatnonzero(deg > 0)
+
+
 def random_walk_query(g: Graph, k: int, dense: bool, rng: np.random.Generator,
-                      max_attempts: int = 100000) -> Query:
+                      max_attempts: int = 100000, _arrays=None) -> Query:
     """Extract a connected k-vertex query by a random walk.
 
     Start at a random vertex of degree >= 1 and step to uniform random neighbours until k
@@ -142,10 +148,7 @@ def random_walk_query(g: Graph, k: int, dense: bool, rng: np.random.Generator,
     average degree is <= 3 and the query is not dense.  Vertices are
     relabelled 0..k-1 in visit order, so the walk itself is always one embedding.
     """
-    off = g.off.astype(np.int64)
-    adj = g.adj
-    deg = np.diff(off)
-    nz = np.flatnonzero(deg > 0)
+    off, adj, deg, nz = _arrays if _arrays is not None else _walk_arrays(g)
     if nz.size == 0:
         raise ValueError("graph has no edges")
     for _ in range(max_attempts):
@@ -191,11 +194,12 @@ def query_set(g: Graph, count: int, sizes, dense, seed: int):
     """``count`` queries; ``sizes`` and ``dense`` are either scalars or callables of the
     query index.  Each query i uses its own child seed (seed, i), so subsets reproduce."""
     out = []
+    arrays = _walk_arrays(g)  # shared by every walk (the same values each call would compute)
     for i in range(count):
         rng = np.random.Generator(np.random.PCG64([seed, i]))
         k = sizes(i, rng) if callable(sizes) else int(sizes)
         dn = dense(i, rng) if callable(dense) else bool(dense)
-        q = random_walk_query(g, k, dn, rng)
+        q = random_walk_query(g, k, dn, rng, _arrays=arrays)
         q.name = f"q{i}_{q.name}"
         out.append(q)
     return out

@@ -295,17 +295,33 @@ def load_csr(path: str, cls=Graph, verify: bool = True):
     return g
 
 
+def _file_sha256(path: str) -> str:
+    h = hashlib.sha256()
+    with open(path, "rb") as f:
+        for block in iter(lambda: f.read(1 << 22), b""):
+            h.update(block)
+    return h.hexdigest()
+
+
 def load_cached(text_path: str, cls=Graph):
     """The reference text format through a binary cache: <text_path>.csr is written on the
-    first load (with the source's name and size in its manifest) and read afterwards while
-    it is newer than the text file."""
+    first load, with the source's name, size and SHA-256 in its manifest, and read
+    afterwards only while the text file still has that size and digest (hashing the text
+    is much cheaper than parsing it; modification times are not trusted, since copies
+    can carry old ones)."""
     csr = text_path + ".csr"
-    if os.path.exists(csr) and os.path.getmtime(csr) >= os.path.getmtime(text_path):
+    src_bytes = os.path.getsize(text_path)
+    src_sha = _file_sha256(text_path)
+    mpath = csr + ".json"
+    if os.path.exists(csr) and os.path.exists(mpath):
         try:
-            return load_csr(csr, cls=cls)
+            with open(mpath) as mf:
+                manifest = json.load(mf)
+            if manifest.get("source_bytes") == src_bytes and manifest.get("source_sha256") == src_sha:
+                return load_csr(csr, cls=cls)
         except ValueError:
-            pass  # stale or damaged: rebuild below
+            pass  # damaged: rebuild below
     g = load_text(text_path, cls=cls)
-    save_csr(g, csr, {"source": os.path.basename(text_path), "source_bytes": os.path.getsize(text_path)})
+    save_csr(g, csr, {"source": os.path.basename(text_path), "source_bytes": src_bytes, "source_sha256": src_sha})
     return g
 

@@ -40,8 +40,13 @@ def _check_filter(g, qs, ok, cnt, dsum, kind=L.SGM_FILTER_GQL):
         assert [int(x) for x in dsum[i, :n]] == [int(deg[np.asarray(c, dtype=np.int64)].sum()) for c in c_r], i
 
 
+MAX_DROPPED = 0.25  # at most this fraction of a batch's live queries may pass the oracle's time limit
+
+
 def _check_enum(g, qs, ok, res, orders, cap, fs=True, limit_ms=20000.0):
-    checked = 0
+    """Compares every live query with the oracle; a query whose oracle run passes limit_ms
+    is left out, and the check fails when more than MAX_DROPPED of them are."""
+    checked = dropped = 0
     for i, q in enumerate(qs):
         if not ok[i]:
             assert res[i].ans == 0 and res[i].intersections == 0
@@ -51,12 +56,14 @@ def _check_enum(g, qs, ok, res, orders, cap, fs=True, limit_ms=20000.0):
         ref = O.enum_cache(g, q, c_r, order, cap=cap, fs=fs, time_limit_ms=limit_ms)
         if ref["timed_out"]:
             say("oracle time limit on query", i, q.getVertexCnt())
+            dropped += 1
             continue
         assert res[i].ans == ref["count"], (i, res[i].ans, ref["count"])
         if ref["count"] < cap:
             assert res[i].intersections == ref["intersections"], i
             assert res[i].nodes == ref["nodes"][:q.getVertexCnt()], i
         checked += 1
+    assert dropped <= MAX_DROPPED * (checked + dropped), (dropped, checked)
     return checked
 
 

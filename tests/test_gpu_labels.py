@@ -211,3 +211,47 @@ def test_labels_attached_after_upload():
     assert ok == ok_r
     if ok:
         _same_cands(can2, q, c_r)
+
+
+def test_batch_recaptures_after_relabel():
+    """A batch runner's captured labelled filter reads the graph's label tables; relabelling
+    the graph so that its label-run table grows must recapture it (the old graph would read
+    freed tables), and the next filter must match the oracle under the new labels."""
+    g = chung_lu(2000, 2.3, 8.0, seed=3)
+    rng = np.random.default_rng(214)
+    g.set_labels(rng.integers(0, 2, g.getVertexCnt()))
+    b = sgm.Batch(g, slots=8)
+    for labels in (2, 7):  # 7 labels: more (label, count) runs per vertex than 2
+        if labels != 2:
+            g.set_labels(rng.integers(0, labels, g.getVertexCnt()))
+            b.ctx.upload(g)  # the new labels go to the device graph
+        qs = query_set(g, 8, lambda i, r: int(r.integers(3, 7)), lambda i, r: bool(i % 2), seed=215 + labels)
+        for rep in range(2):
+            ok, cnt, _ = b.filter(qs)
+            say("relabel batch", labels, rep)
+            for q, good, c in zip(qs, ok, cnt):
+                ok_r, c_r = O.gql(g, q)
+                assert bool(good) == ok_r, q.name
+                if ok_r:
+                    assert [int(x) for x in c[:q.getVertexCnt()]] == [len(x) for x in c_r], q.name
+
+
+def test_relabel_makes_filtered_queries_stale():
+    """Candidates filtered under the graph's old labels are refused by the enumeration
+    (SGM_ESTATE) until the query is filtered again."""
+    g = chung_lu(2000, 2.3, 8.0, seed=11)
+    q = query_set(g, 1, 5, True, seed=216)[0]
+    can = sgm.CandidateParam(g, q)
+    assert sgm.Filter_GQL(g, q, can, build_tables=False)
+    order = sgm.Order_RI(q)
+    before = sgm.enumerate_query(can, order, cap=100000).ans
+    g.set_labels(np.random.default_rng(217).integers(0, 2, g.getVertexCnt()))
+    can.ctx.upload(g)
+    for method in (L.SGM_ENUM_CACHE_FILTER_FS, L.SGM_ENUM_AUX_TABLES):
+        with pytest.raises(L.SgmError) as e:
+            sgm.enumerate_query(can, order, method=method)
+        assert e.value.code == L.SGM_ESTATE
+    g.set_labels(None)
+    can.ctx.upload(g)
+    assert sgm.Filter_GQL(g, q, can, build_tables=False)  # filtered again: usable
+    assert sgm.enumerate_query(can, order, cap=100000).ans == before

@@ -83,10 +83,15 @@ def test_gql_parity_power_law_100k():
 
 
 def test_tables_parity(g_small):
-    for q in _mixed(g_small, 4, 104, 4, 9):
-        say("tables", q.name)
+    # plus a star and a fan of 20 leaves: a query vertex with 20 out-edges drives every
+    # lane of the grouped count pass (one lane per out-edge, up to 31)
+    star = query_from_edges(21, [(0, i) for i in range(1, 21)])
+    fan = query_from_edges(21, [(0, i) for i in range(1, 21)] + [(i, i + 1) for i in range(1, 20)])
+    for q in _mixed(g_small, 4, 104, 4, 9) + [star, fan]:
+        say("tables", q.name, q.getVertexCnt())
         can = sgm.CandidateParam(g_small, q)
         if not sgm.Filter_GQL(g_small, q, can):
+            assert q.getVertexCnt() < 21, "the star and the fan must pass the filter"
             continue
         c = can.candidates
         for a in range(q.getVertexCnt()):
@@ -175,6 +180,64 @@ def test_enum_parity_uncapped(g_small):
         _compare(g_small, q, order)
 
 
+def test_root_ranges_add_up_on_gpu(g_small):
+    """Disjoint root ranges (one launch each) add up to the whole search in count, nodes,
+    intersections and failing-set cut-offs, on both searches with and without failing sets,
+    and every range equals the oracle's."""
+    rng = np.random.default_rng(110)
+    V = g_small.getVertexCnt()
+    for q in _mixed(g_small, 6, 111, 4, 8):
+        can = sgm.CandidateParam(g_small, q)
+        ok_r, c_r = O.gql(g_small, q)
+        if not sgm.Filter_GQL(g_small, q, can, build_tables=False):
+            continue
+        order = O.order_ri(q)
+        cuts = sorted(int(x) for x in rng.choice(np.arange(1, V), 3, replace=False))
+        ranges = list(zip([0] + cuts, cuts + [V]))
+        for method, fs, fn in ((L.SGM_ENUM_CACHE_FILTER_FS, True, O.enum_cache),
+                               (L.SGM_ENUM_CACHE_FILTER, False, O.enum_cache),
+                               (L.SGM_ENUM_AUX_TABLES, False, O.enum_all),
+                               (L.SGM_ENUM_AUX_TABLES_FS, True, O.enum_all)):
+            whole = sgm.enumerate_query(can, order, method, cap=3_000_000)
+            if whole.capped:
+                continue
+            parts = [sgm.enumerate_query(can, order, method, roots=r) for r in ranges]
+            say("ranges", q.name, method, whole.ans, [p.ans for p in parts])
+            assert sum(p.ans for p in parts) == whole.ans
+            assert sum(p.intersections for p in parts) == whole.intersections
+            assert sum(p.fs_prunes for p in parts) == whole.fs_prunes
+            assert [sum(x) for x in zip(*[p.nodes for p in parts])] == whole.nodes
+            for r, p in zip(ranges, parts):
+                ref = fn(g_small, q, c_r, order, fs=fs, roots=r)
+                assert (p.ans, p.nodes, p.intersections, p.fs_prunes) == \
+                    (ref["count"], ref["nodes"], ref["intersections"], ref["fs_prunes"]), (q.name, method, r)
+
+
+def test_time_budget_and_host_stop(g1):
+    """A heavy uncapped query: a small work budget ends it timed out (never a result); a host
+    stop request ends it within a moment, with a larger budget as the backstop."""
+    import threading
+    import time
+    q = query_set(g1, 1, 8, False, seed=11)[0]  # more than 1e13 embeddings
+    can = sgm.CandidateParam(g1, q)
+    assert sgm.Filter_GQL(g1, q, can, build_tables=False)
+    order = sgm.Order_RI(q)
+    r = sgm.enumerate_query(can, order, time_limit_ms=500)
+    assert r.timed_out and not r.capped and r.busy_ns >= 5e8
+    # 16 blocks (64 waves) with an hour of wave time each would run about a minute
+    timer = threading.Timer(0.3, can.ctx.stop)
+    timer.start()
+    t0 = time.time()
+    with pytest.raises(L.SgmError) as e:
+        sgm.enumerate_query(can, order, blocks=16, time_limit_ms=3_600_000)
+    timer.cancel()
+    assert e.value.code == L.SGM_ESTOPPED
+    assert time.time() - t0 < 10.0
+    # the next launch is not affected by the earlier request
+    r = sgm.enumerate_query(can, order, cap=1000)
+    assert r.ans == 1000 and not r.stopped
+
+
 def test_enum_capped_config1(g1):
     for q in query_set(g1, 2, 8, False, seed=11) + query_set(g1, 2, 16, True, seed=12):
         say("capped", q.name)
@@ -231,13 +294,32 @@ def test_single_edge_and_star_queries(g_small):
 
 
 def test_empty_candidates(g_small):
-    k = g_small.getMaxDgree() + 1
-    if k > 31:
-        pytest.skip("max degree too large for a star query")
-    q = query_from_edges(k + 1, [(0, i) for i in range(1, k + 1)])
-    can = sgm.CandidateParam(g_small, q)
-    assert not sgm.Filter_GQL(g_small, q, can, build_tables=False)
-    assert sgm.enumerate_query(can, sgm.Order_RI(q)).ans == 0
+    """A query vertex whose label no data vertex carries has no candidates: the filter
+    returns false (inc/filter.h:87-90) and every enumeration entry point reports 0 without
+    searching (src/overall_performance.cpp:54-56), the batched ones next to live queries."""
+    q = query_from_edges(3, [(0, 1), (1, 2), (0, 2)], labels=[0, 0, 7])
+    live = query_set(g_small, 2, 4, True, seed=109)
+    for kind in (L.SGM_FILTER_NLF, L.SGM_FILTER_GQL):
+        can = sgm.CandidateParam(g_small, q)
+        filt = sgm.Filter_NLF if kind == L.SGM_FILTER_NLF else sgm.Filter_GQL
+        assert not filt(g_small, q, can, build_tables=False)
+        assert not (O.nlf if kind == L.SGM_FILTER_NLF else O.gql)(g_small, q)[0]
+        order = sgm.Order_RI(q)
+        for method in (L.SGM_ENUM_CACHE_FILTER_FS, L.SGM_ENUM_CACHE_FILTER, L.SGM_ENUM_AUX_TABLES,
+                       L.SGM_ENUM_AUX_TABLES_FS):
+            assert sgm.enumerate_query(can, order, method=method).ans == 0
+        cans = [sgm.CandidateParam(g_small, x) for x in live]
+        oks = sgm.filter_batch(cans + [can], kind)
+        assert oks[-1] is False and all(oks[:-1])
+        for method in (L.SGM_ENUM_CACHE_FILTER_FS, L.SGM_ENUM_AUX_TABLES, L.SGM_ENUM_AUX_TABLES_FS):
+            res = sgm.enumerate_batch(cans + [can], None, method, cap=10000)
+            assert res[-1].ans == 0
+            assert all(r.ans == 10000 for r in res[:-1])  # the live queries have many embeddings
+    b = sgm.Batch(g_small, slots=4)
+    ok, cnt, _ = b.filter(live + [q])
+    assert ok == [True, True, False]
+    res = b.enumerate(cap=10000)
+    assert res[-1].ans == 0 and all(r.ans == 10000 for r in res[:-1])
 
 
 def test_order_must_be_connected(g_small):

@@ -133,6 +133,14 @@ def test_load_cached_writes_then_reuses(tmp_path):
         f.write(b"garbage!")
     g3 = load_cached(tp)
     assert _same(g, g3) and open(cp, "rb").read(8) == CSR_MAGIC
+    # a text file replaced by another graph with an older modification time (cp -p, tar,
+    # rsync -t) is noticed by its digest, not trusted by its mtime
+    h = chung_lu(1500, 2.5, 7.4, seed=8)
+    save_text(h, tp)
+    old = os.path.getmtime(cp) - 3600
+    os.utime(tp, (old, old))
+    g4 = load_cached(tp)
+    assert _same(h, g4) and not _same(g, g4)
 
 
 CPP = r'''

@@ -303,6 +303,53 @@ def test_counts_match_brute_force(labels):
                 assert O.enum_all(g, q, c, order, cap=cap)["count"] == cap
 
 
+def test_root_range_counts_match_brute_force():
+    """A root range [lo, hi) counts exactly the embeddings whose first order vertex maps into
+    it (brute force over injective maps), on both searches, failing sets on and off."""
+    rng = np.random.default_rng(23)
+    for g, q in small_cases(24, 40):
+        ok, c = O.gql(g, q)
+        if not ok:
+            continue
+        order = O.order_ri(q)
+        V = g.getVertexCnt()
+        lo = int(rng.integers(0, V - 1))
+        hi = int(rng.integers(lo + 1, V + 1))
+        exp = sum(1 for m in embeddings(g, q) if lo <= m[order[0]] < hi)
+        for fs in (True, False):
+            assert O.enum_cache(g, q, c, order, fs=fs, roots=(lo, hi))["count"] == exp
+            assert O.enum_all(g, q, c, order, fs=fs, roots=(lo, hi))["count"] == exp
+
+
+def test_root_ranges_add_up():
+    """Disjoint root ranges add up to the whole search in every statistic (count, per-depth
+    nodes, intersections, failing-set cut-offs): the depth-0 loop never prunes, so each
+    subtree is searched alike whatever else runs."""
+    g = chung_lu(300, 2.3, 5.0, seed=19)  # inputs on which failing sets cut (see below)
+    V = g.getVertexCnt()
+    rng = np.random.default_rng(31)
+    qs = query_set(g, 16, lambda i, r: int(r.integers(5, 9)), lambda i, r: bool(i % 2), seed=20)
+    cut = 0
+    for q in qs:
+        ok, c = O.gql(g, q)
+        if not ok:
+            continue
+        order = O.order_ri(q)
+        cuts = sorted(int(x) for x in rng.choice(np.arange(1, V), 3, replace=False))
+        ranges = list(zip([0] + cuts, cuts + [V]))
+        for fn, fs in ((O.enum_cache, True), (O.enum_cache, False), (O.enum_all, True), (O.enum_all, False)):
+            whole = fn(g, q, c, order, fs=fs, cap=3_000_000)
+            if whole["capped"]:
+                continue
+            parts = [fn(g, q, c, order, fs=fs, roots=r) for r in ranges]
+            assert sum(p["count"] for p in parts) == whole["count"]
+            assert sum(p["intersections"] for p in parts) == whole["intersections"]
+            assert sum(p["fs_prunes"] for p in parts) == whole["fs_prunes"]
+            assert [sum(x) for x in zip(*[p["nodes"] for p in parts])] == whole["nodes"]
+            cut += whole["fs_prunes"] if fs else 0
+    assert cut > 0  # the failing sets did cut inside some subtree
+
+
 def test_counts_closed_forms():
     g = chung_lu(300, 2.5, 8.0, seed=18)
     deg = g.degrees().astype(np.int64)
