@@ -493,7 +493,7 @@ int sgm_batch_enumerate_launch(sgm_batch* b, const uint32_t* const* orders, cons
             sgm::launch_fill_u32(b->stream, reinterpret_cast<uint32_t*>(b->d_st), st_words, 0);
             sgm::launch_fill_u32(b->stream, b->d_qstop, 3 * (size_t)slots, 0);
             sgm::launch_last_bits(b->stream, b->d_eq, slots, V);
-            if (split) sgm::launch_split_roots(b->stream, g->d_off, g->d_bydeg, b->d_eq, slots);
+            if (split) sgm::launch_split_roots(b->stream, g->d_off, g->d_bydeg, b->d_eq, slots, V);
             sgm::launch_enum_cache(b->stream, blocks, g->d_off, g->d_adj, g->d_bydeg, b->d_eq, slots, bucket, b->d_stack,
                                    (uint32_t)words, c->d_stop, c->d_hstop, use_fs, hub, g->d_hub_bits, g->hub_words,
                                    serial, split);
@@ -573,7 +573,7 @@ int sgm_batch_enumerate_wait(sgm_batch* b, sgm_stats* stats)
         sgm::launch_fill_u32(b->stream, b->d_qstop, 3 * (size_t)slots, 0);
         HIPCHK(c, hipEventRecord(b->ev[2], b->stream));
         const uint32_t ns = (uint32_t)sub.size();
-        sgm::launch_split_roots(b->stream, g->d_off, g->d_bydeg, b->d_eq, ns);  // no-op without split blocks
+        sgm::launch_split_roots(b->stream, g->d_off, g->d_bydeg, b->d_eq, ns, g->V);  // no-op without split blocks
         sgm::launch_enum_cache(b->stream, blocks, g->d_off, g->d_adj, g->d_bydeg, b->d_eq, ns, bucket, c->d_stack,
                                (uint32_t)words, c->d_stop, c->d_hstop, b->pend_use_fs, b->pend_hub, g->d_hub_bits,
                                g->hub_words, b->pend_serial, b->pend_split);

@@ -309,30 +309,40 @@ __device__ void last_isect(const uint32_t* __restrict__ a, uint32_t na, const ui
 // Split table of each query (layout at SGM_SPLIT_ROOTS): one wave per query walks the
 // degree-ordered items until SGM_SPLIT_ROOTS candidate roots are found (or SGM_SPLIT_SCAN
 // positions are seen), then cuts root r's depth-1 loop -- N(v_r), deg(v_r) elements -- into
-// min(SGM_SPLIT_CHUNKS, ceil(deg/64)) chunks of a whole number of 64-element batches.
+// min(SGM_SPLIT_CHUNKS, ceil(deg/64)) chunks of a whole number of 64-element batches.  A
+// root range of at most SGM_SPLIT_ROOTS ids is read directly instead: all its candidates
+// are split and no root is left to the item counter (root_first = n_items).
 __global__ void __launch_bounds__(64) k_split_roots(const uint32_t* __restrict__ off,
                                                     const uint32_t* __restrict__ items,
-                                                    const sgm_equery* __restrict__ qs)
+                                                    const sgm_equery* __restrict__ qs, uint32_t V)
 {
     const sgm_equery& Q = qs[blockIdx.x];
     uint32_t* __restrict__ sp = Q.split;
     if (!sp) return;
     const uint32_t lane = lane_id();
     const uint32_t Rn = SGM_SPLIT_ROOTS;
-    uint32_t* __restrict__ rpos = sp + 2;
-    uint32_t* __restrict__ cutmin = rpos + Rn;
+    uint32_t* __restrict__ rvtx = sp + 2;
+    uint32_t* __restrict__ cutmin = rvtx + Rn;
     uint32_t* __restrict__ done = cutmin + Rn;
     uint32_t* __restrict__ it3 = done + SGM_SPLIT_ITEMS;
     uint32_t found = 0, root_first = 0;
     const uint32_t lim = Q.n_items < SGM_SPLIT_SCAN ? Q.n_items : SGM_SPLIT_SCAN;
-    if (Q.plan->n > 2) {  // two-vertex queries have no depth-1 loop to split
+    if (Q.plan->n > 2 && Q.root_hi - Q.root_lo <= Rn) {
+        // the candidates of a narrow root range, in id order (Rn <= 64: one ballot)
+        const uint32_t v = Q.root_lo + lane;
+        const bool c = lane < Q.root_hi - Q.root_lo && v < V && ((Q.vmask[v] >> Q.root_u) & 1u);
+        const uint64_t b = __ballot(c);
+        if (c) rvtx[mbcnt64(b)] = v;
+        found = (uint32_t)__popcll(b);
+        root_first = Q.n_items;
+    } else if (Q.plan->n > 2) {  // two-vertex queries have no depth-1 loop to split
         for (uint32_t base = 0; base < lim && found < Rn; base += 64) {
             const uint32_t p = base + lane;
             const uint32_t v = p < lim ? items[p] : 0u;
             const bool c = p < lim && ((Q.vmask[v] >> Q.root_u) & 1u) && v >= Q.root_lo && v < Q.root_hi;
             const uint64_t b = __ballot(c);
             const uint32_t r = found + mbcnt64(b);
-            if (c && r < Rn) rpos[r] = p;
+            if (c && r < Rn) rvtx[r] = v;
             const uint32_t cnt = (uint32_t)__popcll(b);
             if (found + cnt >= Rn) {
                 uint64_t bb = b;  // the (Rn - found)-th set bit is the last split root
@@ -348,7 +358,7 @@ __global__ void __launch_bounds__(64) k_split_roots(const uint32_t* __restrict__
     // chunks per root, then their (root, lo, hi) triples in root order
     uint32_t deg = 0, nch = 0, W = 64;
     if (lane < found) {
-        const uint32_t v = items[rpos[lane]];
+        const uint32_t v = rvtx[lane];
         deg = off[v + 1] - off[v];
         const uint32_t b64 = (deg + 63) / 64;
         const uint32_t ch = b64 < SGM_SPLIT_CHUNKS ? b64 : SGM_SPLIT_CHUNKS;
@@ -881,7 +891,7 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
             lo1 = uni(t3[1]);
             hi1 = uni(t3[2]);
             uint32_t* cutw = Q.split + 2 + SGM_SPLIT_ROOTS + sri;
-            v0 = items[uni(Q.split[2 + sri])];
+            v0 = uni(Q.split[2 + sri]);
             // an earlier chunk of this root was cut off: the sequential search never gets here
             if (uni(__atomic_load_n(cutw, __ATOMIC_RELAXED)) < lo1) continue;
             flush_stats(Q, S, acc, n, false);  // the chunk's counters go to its record
@@ -1483,9 +1493,10 @@ void launch_last_bits(hipStream_t s, const sgm_equery* qs, uint32_t nq, uint32_t
     hipLaunchKernelGGL(k_last_bits, grid, dim3(256), 0, s, qs, V);
 }
 
-void launch_split_roots(hipStream_t s, const uint32_t* off, const uint32_t* items, const sgm_equery* qs, uint32_t nq)
+void launch_split_roots(hipStream_t s, const uint32_t* off, const uint32_t* items, const sgm_equery* qs, uint32_t nq,
+                        uint32_t V)
 {
-    if (nq) hipLaunchKernelGGL(k_split_roots, dim3(nq), dim3(64), 0, s, off, items, qs);
+    if (nq) hipLaunchKernelGGL(k_split_roots, dim3(nq), dim3(64), 0, s, off, items, qs, V);
 }
 
 void launch_split_reduce(hipStream_t s, const sgm_equery* qs, uint32_t nq)

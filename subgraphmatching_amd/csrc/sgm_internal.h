@@ -92,7 +92,7 @@ struct sgm_stats_dev {
 // adds the chunks up to and including a root's first cutting chunk -- the counters of the
 // sequential search.  Per-query split block, u32 words:
 //   [0] n_split  [1] root_first (the first position not split)
-//   [2, 2+R) rpos (positions of the split roots)  [2+R, 2+2R) cutmin (smallest start of a
+//   [2, 2+R) the split roots (vertex ids)  [2+R, 2+2R) cutmin (smallest start of a
 //   cutting chunk, ~0 = none)  [2+2R, 2+2R+IC) done flags  then IC triples (root, lo, hi)
 // Records, u64, field-major (field f of item s at rec[f * IC + s]): nodes[32],
 // intersections, fs_prunes, elems_read, elems_out, elems_scanned.

@@ -267,11 +267,11 @@ inline void root_range(const sgm_enum_opts& o, uint32_t* lo, uint32_t* hi)
     *lo = o.root_begin;
     *hi = o.root_end ? o.root_end : 0xFFFFFFFFu;
 }
-// A root range of at most SGM_SPLIT_MAX_ROOTS ids holds at most that many roots: its queries
-// get depth-1 splitting, so a heavy root in the range is not left to a single wave.
+// A root range of at most SGM_SPLIT_ROOTS ids: every candidate in it gets its depth-1 loop
+// split (k_split_roots reads the range directly), so a heavy root is not left to one wave.
 inline bool narrow_roots(const sgm_enum_opts& o)
 {
-    return o.root_end && o.root_end - o.root_begin <= SGM_SPLIT_MAX_ROOTS;
+    return o.root_end && o.root_end - o.root_begin <= SGM_SPLIT_ROOTS;
 }
 // Clears the host's stop request before a launch (only when no launch is in flight: a
 // request then ends every one of them).
