@@ -188,6 +188,17 @@ int sgm_plan_export_labeled(const uint32_t* qoff, const uint32_t* qadj, const ui
 /* ---- enumeration (replaces explore_Intersection_Cache_DAF_Filter inc/explore.h:1073 and
  * explore_Intersection_All<false x5> :2757) -------------------------------------------- */
 int sgm_enumerate(sgm_query* q, const uint32_t* order, const sgm_enum_opts* opts, sgm_stats* out);
+/* sgm_enumerate without allocating: no reference counterpart.  For callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The cache search methods only (SGM_ENUM_CACHE_FILTER_FS /
+ * SGM_ENUM_CACHE_FILTER, hub_min_degree <= 64).  Every per-call device buffer (plan,
+ * descriptor, statistics, stop and item words, split blocks, the last-level bitmap and the
+ * set-op stack) comes from one static device array of 256 MiB per GPU, held until the
+ * process ends; the grid is sized so that the waves' stacks fit in it (a stack overflow
+ * re-runs with deeper stacks on fewer waves).  A graph whose fixed part (about
+ * V/8 + 1 MiB) or a stack_words request that does not fit is rejected with SGM_EINVAL.
+ * All calls on one GPU share its array: calls that overlap on one GPU race. */
+int sgm_enumerate_noalloc(sgm_query* q, const uint32_t* order, const sgm_enum_opts* opts, sgm_stats* out);
 /* Many queries of one data graph in one persistent launch (every method: cache + filter
  * with or without failing sets, and the aux-table search, whose missing tables are built in
  * the call; no reference counterpart: the reference runs one query per call).  Each query

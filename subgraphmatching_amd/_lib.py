@@ -51,6 +51,7 @@ EXPORTS = [
     "sgm_costmodel_features", "sgm_costmodel_predict", "sgm_costmodel_grad", "sgm_costmodel_train",
     "sgm_costmodel_timing", "sgm_order_sample", "sgm_batch_rank_orders", "sgm_gql_work",
     "sgm_graph_set_labels", "sgm_query_set_labels", "sgm_plan_export_labeled", "sgm_batch_filter_labeled",
+    "sgm_enumerate_noalloc",
 ]
 
 
@@ -126,6 +127,7 @@ def lib() -> C.CDLL:
         "sgm_graph_set_labels": (C.c_int, [vp, vp, C.c_uint32]),
         "sgm_query_set_labels": (C.c_int, [vp, vp, C.c_uint32]),
         "sgm_enumerate": (C.c_int, [vp, vp, C.POINTER(EnumOpts), C.POINTER(Stats)]),
+        "sgm_enumerate_noalloc": (C.c_int, [vp, vp, C.POINTER(EnumOpts), C.POINTER(Stats)]),
         "sgm_enumerate_batch": (C.c_int, [vp, vp, C.c_uint32, C.POINTER(EnumOpts), vp]),
         "sgm_batch_create": (C.c_int, [vp, C.c_uint32, C.POINTER(vp)]),
         "sgm_batch_destroy": (None, [vp]),

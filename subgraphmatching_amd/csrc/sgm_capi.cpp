@@ -61,6 +61,10 @@ int sgm_ctx_create(int device, sgm_ctx** out)
     }
     c->d_hstop = static_cast<const uint32_t*>(dp);
     __atomic_store_n(c->h_stop, 0u, __ATOMIC_SEQ_CST);
+    if (const int rc = noalloc_setup(c)) {  // the no-allocation entry point's scratch array
+        sgm_ctx_destroy(c);
+        return rc;
+    }
     *out = c;
     return SGM_OK;
 }

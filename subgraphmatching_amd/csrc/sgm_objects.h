@@ -248,6 +248,10 @@ inline int grow(sgm_ctx* c, T** p, uint32_t& cap_now, uint32_t need)
 int build_label_tables(sgm_ctx* c, sgm_graph* g, const uint32_t* labels);
 std::vector<uint32_t> lreq_block(const sgm::QueryCSR& q);
 
+// The no-allocation entry point's static scratch array (sgm_noalloc.hip): looks up its
+// address on the context's device, once per device (the device must be current).
+int noalloc_setup(sgm_ctx* c);
+
 // Shared by the enumeration drivers (sgm_capi.cpp, sgm_batch.cpp).
 // Cost-model work buffers filled on the device, and a forward pass over them (ranking).
 int costmodel_work_inputs(sgm_costmodel* m, uint32_t S, uint16_t** F, uint8_t** valid, uint32_t** back, uint32_t** fwd);
