@@ -11,6 +11,12 @@ Workloads (one step = one pass of the path over the whole query set):
     path        NLF + GQL filtering of all 200 queries in one set of launches, RI order
                 (host), cache + filter + failing-set enumeration of all 200 queries in one
                 persistent launch (per-query caps and statistics)
+  chunglu10k
+    data graph  Chung-Lu, 10,000 vertices, degree exponent 2.5, average degree 7.4, seed 1
+    query       one 8-vertex sparse random-walk query (average degree <= 3), seed 11
+    path        GQL filter, RI order, cache + filter + failing-set enumeration of the one
+                query (its embeddings pass 1.5e13, so the step is capped like the CPU side;
+                an uncapped pass under a work budget reports the count it reached)
   rmat20-learned
     the rmat20 graph and query mix with learned orders: a cost model is trained (untimed) on
     node counts the enumerator logs for 5,000 training queries (seed 44) x 10 orders, then
@@ -39,13 +45,21 @@ Workloads (one step = one pass of the path over the whole query set):
 
 Counts are exact, capped at --cap embeddings per query (default 10^7, the same cap for the
 GPU run and the CPU baseline).  The CPU baseline is the project's C restatement of the
-reference path (oracle/, the checker), run single-threaded on this host over a bounded
-sample of the same queries; the GPU counts of the sampled queries are compared with it.
+reference path (oracle/, the checker; AVX2 set intersections as in the reference's build),
+run single-threaded per query on this host over a bounded sample of the same queries; the
+GPU counts and candidate-set digests of the sampled queries are compared with it.
+
+Config 2 as written has no cap; its queries hold more than 10^9 embeddings each (no query
+finishes on either side within any budget this run can afford), so the default workload
+also runs the 200 queries uncapped once with a per-query work budget (--uncapped-budget-ms
+of wave time each) and reports how many finished, timed-out queries as timed out with the
+count they had reached (a lower bound, not a result), beside the capped headline.
 """
 from __future__ import annotations
 
 import argparse
 import collections
+import hashlib
 import json
 import os
 import sys
@@ -63,8 +77,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["chunglu100k", "chunglu100k-labelled", "chunglu100k-aux", "rmat20",
-                                           "rmat20-learned", "rmat22-serve"],
+    ap.add_argument("--workload", choices=["chunglu100k", "chunglu100k-labelled", "chunglu100k-aux", "chunglu10k",
+                                           "rmat20", "rmat20-learned", "rmat22-serve"],
                     default="chunglu100k")
     ap.add_argument("--serve-mem-gb", type=float, default=32.0,
                     help="rmat22-serve: device memory budget for query slots and DFS stacks")
@@ -80,6 +94,10 @@ def parse():
     ap.add_argument("--labels", type=int, default=8,
                     help="chunglu100k-labelled: vertex labels 0..N-1, label l drawn with weight 1/(l+1)")
     ap.add_argument("--cap", type=int, default=10_000_000)
+    ap.add_argument("--uncapped-budget-ms", type=int, default=-1,
+                    help="chunglu100k / chunglu10k: per-query wave-time budget of the uncapped pass (0 skips it; "
+                         "default 10 s per query for the 200 config-2 queries, 2000 s for the one config-1 query: "
+                         "about half a second of the whole GPU either way)")
     ap.add_argument("--per-query", action="store_true",
                     help="chunglu100k: one enumeration launch per query instead of one batched launch")
     ap.add_argument("--blocks", type=int, default=0,
@@ -96,7 +114,10 @@ def parse():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES bytes (the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings) and exit")
-    return ap.parse_args()
+    args = ap.parse_args()
+    if args.uncapped_budget_ms < 0:
+        args.uncapped_budget_ms = 2_000_000 if args.workload == "chunglu10k" else 10_000
+    return args
 
 
 def copy_calibration(nbytes):
@@ -110,7 +131,7 @@ def copy_calibration(nbytes):
 
 
 def new_rec():
-    return {"emb": 0, "inter": 0, "elems_read": 0, "elems_out": 0, "elems_scanned": 0, "enum_ms": 0.0,
+    return {"emb": 0, "inter": 0, "elems_read": 0, "elems_out": 0, "elems_scanned": 0, "probes": 0, "enum_ms": 0.0,
             "enum_launches": 0, "capped": 0, "counts": [], "gql_ms": 0.0, "nlf_ms": 0.0, "cand_ms": 0.0,
             "filter_ms": 0.0}
 
@@ -124,6 +145,7 @@ def add_result(rec, r):
     rec["elems_read"] += r.elems_read
     rec["elems_out"] += r.elems_out
     rec["elems_scanned"] += r.elems_scanned
+    rec["probes"] += r.probe_words
     rec["capped"] += int(r.capped)
     rec["counts"].append(r.ans)
 
@@ -134,12 +156,16 @@ def setup_chunglu(args):
     from subgraphmatching_amd import _lib as L
     from subgraphmatching_amd.generators import chung_lu, query_set, with_labels
 
-    g = chung_lu(100_000, 2.3, 16.0, seed=2)
+    c10k = args.workload == "chunglu10k"
+    g = chung_lu(10_000, 2.5, 7.4, seed=1) if c10k else chung_lu(100_000, 2.3, 16.0, seed=2)
     labelled = args.workload == "chunglu100k-labelled"
     aux = args.workload == "chunglu100k-aux"  # tables + explore_Intersection_All<true,...>, per query
     if labelled:  # the same CSR with skewed labels; the queries' walks carry them
         g = with_labels(g, args.labels, seed=23, skew=1.0)
-    queries = query_set(g, args.queries or 200, 16, True, seed=22)
+    if c10k:
+        queries = query_set(g, 1, 8, False, seed=11)
+    else:
+        queries = query_set(g, args.queries or 200, 16, True, seed=22)
     ctx = sgm.default_context()
     cans = [sgm.CandidateParam(g, q, ctx) for q in queries]
 
@@ -191,6 +217,8 @@ def setup_chunglu(args):
         return rec
 
     name = "chunglu_V100000_g2.3_d16_seed2__200x16v_dense_rw_seed22__gql_ri_cache_filter_fs"
+    if c10k:
+        name = "chunglu_V10000_g2.5_d7.4_seed1__1x8v_sparse_rw_seed11__gql_ri_cache_filter_fs"
     if labelled:
         name = name.replace("seed2__", f"seed2_labels{args.labels}_skew1_seed23__")
     if aux:
@@ -200,7 +228,36 @@ def setup_chunglu(args):
               "filter": "labelled NLF+GQL" if labelled else "NLF+GQL",
               "enumeration": "aux tables + failing sets" if aux else "cache+filter+failing-sets",
               "launch": "per-query" if args.per_query else "one filter batch + one persistent enumeration launch"}
-    return g, queries, step, config, {}
+
+    def uncapped(budget_ms):
+        """The same queries uncapped, one launch, each with a work budget of budget_ms of wave
+        time: finished queries are exact results; timed-out ones report the count they had
+        reached, a lower bound of their embedding count and never a result."""
+        ok = sgm.filter_batch(cans, L.SGM_FILTER_GQL)
+        live = [c for c, good in zip(cans, ok) if good]
+        t0 = time.perf_counter()
+        res = sgm.enumerate_batch(live, None, L.SGM_ENUM_CACHE_FILTER_FS, time_limit_ms=budget_ms)
+        wall = time.perf_counter() - t0
+        done = [r for r in res if not r.timed_out]
+        lb = sorted(r.ans for r in res if r.timed_out)
+        emb = sum(r.ans for r in done)
+        return {"time_limit_ms_per_query": budget_ms, "budget_kind": "wave time per query, summed over its waves",
+                "queries": len(res), "completed": len(done), "timed_out": len(lb),
+                "completed_embeddings": emb, "launch_wall_ms": 1e3 * wall,
+                "kernel_ms": res[0].time if res else 0.0,
+                "embeddings_per_s_completed": emb / wall if done and wall > 0 else None,
+                "timed_out_count_lower_bound": ({"min": lb[0], "median": lb[len(lb) // 2], "max": lb[-1]}
+                                                if lb else None)}
+
+    def gpu_digests():
+        """Candidate-set digests of every query (the oracle module's digest form)."""
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        sgm.filter_batch(cans, L.SGM_FILTER_GQL)
+        return [O.cand_digest(c.candidates) for c in cans]
+
+    extra = {} if (labelled or aux) else {"uncapped": uncapped, "digests": gpu_digests}
+    return g, queries, step, config, extra
 
 
 # ------------------------------------------------------------------------------ rmat20
@@ -498,19 +555,29 @@ def main():
     enum_ms = sum(r["enum_ms"] for r in recs)
     filter_ms = sum(r["filter_ms"] for r in recs)
     launches = sum(r["enum_launches"] for r in recs)
-    # algorithmic bytes of the enumeration kernel (merge-intersection model): every
-    # intersection reads |A|+|B| and writes |A n B| u32; every iterated candidate costs a
-    # 4-byte id plus one filter bit
-    algo_bytes = sum(4 * (r["elems_read"] + r["elems_out"]) + 4.125 * r["elems_scanned"] for r in recs)
+    # Algorithmic bytes of the enumeration kernel, per the strategy each intersection used (a
+    # lower bound of what the search must move): 4 B per key of an intersection's shorter list
+    # and per word it probes in the longer one (one bitmap word, or one word per binary-search
+    # step and the final load), 4 B per classified candidate (its filter word).
+    algo_bytes = sum(4 * (r["probes"] + r["elems_scanned"]) for r in recs)
+    # the merge model (every intersection reads |A|+|B| and writes |A n B|, every candidate
+    # 4 B + 1 bit), kept for comparison: not a bound for bitmap or binary-search intersections
+    merge_bytes = sum(4 * (r["elems_read"] + r["elems_out"]) + 4.125 * r["elems_scanned"] for r in recs)
     enum_kernel = "k_enum_aux" if args.workload == "chunglu100k-aux" else "k_enum_cache"
     achieved = algo_bytes / (enum_ms / 1e3) / 1e9 if enum_ms > 0 else 0.0
     traffic = measured = None
+    traffic_note = "no measured traffic for this workload"
     if args.workload == "chunglu100k" and os.path.exists(args.traffic):
         try:
             with open(args.traffic) as f:
-                traffic = json.load(f).get("k_enum_cache_bytes_per_launch")
-        except Exception:
-            traffic = None
+                tj = json.load(f)
+            if tj.get("source_stamp") == source_stamp():
+                traffic = tj.get("k_enum_cache_bytes_per_launch")
+                traffic_note = "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel build (profiles/traffic.json)"
+            else:
+                traffic_note = "profiles/traffic.json was measured on other kernel sources: not used"
+        except Exception as e:  # noqa: BLE001
+            traffic_note = f"profiles/traffic.json unreadable: {e}"
     if traffic and launches and not args.per_query:
         measured = traffic / (enum_ms / launches / 1e3) / 1e9
 
@@ -524,11 +591,15 @@ def main():
         sl, sl_counts = extra["single_launch"]()
         extras["single_launch"] = dict(sl, counts_match=sl_counts == counts0)
         print(f"[bench] single launch over all queries: {sl}", flush=True)
+    if "uncapped" in extra and args.uncapped_budget_ms > 0:
+        extras["uncapped"] = extra["uncapped"](args.uncapped_budget_ms)
+        print(f"[bench] uncapped pass: {extras['uncapped']}", flush=True)
 
     cpu = None
+    gpu_digests = extra["digests"]() if "digests" in extra and args.cpu_seconds > 0 else None
     if args.cpu_seconds > 0:
         cpu = cpu_baseline(g, queries, counts0, config["cap_per_query"], args.cpu_seconds,
-                           "aux_fs" if args.workload == "chunglu100k-aux" else "cache")
+                           "aux_fs" if args.workload == "chunglu100k-aux" else "cache", gpu_digests)
 
     out = {
         "metric": METRIC,
@@ -552,11 +623,15 @@ def main():
                                "enumerate": enum_ms / K},
         "roofline": {"kernel": enum_kernel, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "achieved_basis": "merge-model algorithmic bytes per launch / average launch time",
+                     "achieved_basis": "strategy bytes per launch (4 B per intersection key and per bitmap word or "
+                                       "binary-search word it probes, 4 B per classified candidate's filter "
+                                       "word; a lower bound of what the search moves) / average launch time",
+                     "traffic_basis": traffic_note,
                      "measured_gbs": measured,
                      "measured_frac": measured / HBM_PEAK_GBS if measured is not None else None,
                      "algorithmic_bytes_per_launch": algo_bytes / max(launches, 1),
                      "avg_launch_ms": enum_ms / max(launches, 1),
+                     "merge_model_gbs": merge_bytes / (enum_ms / 1e3) / 1e9 if enum_ms > 0 else 0.0,
                      "dominant_kernel": enum_kernel if enum_ms >= filter_ms else "k_gql_phase"},
         "cpu_baseline": cpu,
     }
@@ -564,10 +639,21 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def source_stamp():
+    """First 16 hex digits of the SHA-256 of the enumeration kernel's sources: a traffic
+    record applies only to the build it was measured on."""
+    h = hashlib.sha256()
+    for f in ("sgm_enum.hip", "sgm_internal.h", "sgm_wave.h"):
+        with open(os.path.join(ROOT, "subgraphmatching_amd", "csrc", f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
 def _cpu_worker(job):
     """One CPU worker: GQL + RI + cache/filter/fs (or aux-table/fs) enumeration of its queries, one after the
     other on one core, until its wall-clock budget ends (the query running then is cut off
-    by the oracle's time limit and left out)."""
+    by the oracle's time limit and left out).  Returns (query, count, enumeration thread-ms,
+    candidate digest) per finished query."""
     g, items, cap, budget_s, method = job
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -578,6 +664,7 @@ def _cpu_worker(job):
         if left <= 0:
             break
         ok, c = O.gql(g, q)
+        dig = O.cand_digest(c)
         order = O.order_ri(q)
         cnt, ms = 0, 0.0
         if ok:
@@ -588,42 +675,53 @@ def _cpu_worker(job):
             if st["timed_out"]:
                 break
             cnt, ms = st["count"], st["enum_ms"]
-        out.append((i, cnt, ms))
+        out.append((i, cnt, ms, dig))
     return out, time.perf_counter() - t0
 
 
-def cpu_baseline(g, queries, gpu_counts, cap, budget_s, method="cache"):
-    """The project's C restatement of the same path (the oracle, single-threaded per query as
-    the reference runs) on a bounded sample of the same queries: P worker processes, one per
-    core (P = min(16, host cores)), each taking every P-th query for budget_s seconds of
-    wall time.  Reports the P-core rate and worker 0's single-core rate; the GPU counts of
-    every sampled query are compared."""
+def cpu_baseline(g, queries, gpu_counts, cap, budget_s, method="cache", gpu_digests=None):
+    """The project's C restatement of the same path (the oracle: single-threaded per query as
+    the reference runs, AVX2 intersections as the reference's build) on a bounded sample of
+    the same queries: P worker processes, one per core, each taking every P-th query for
+    budget_s seconds of wall time.  P = min(16, host cores): 16 cores are one GPU's share of
+    the GPU box's host.  Reports the P-core rate and worker 0's single-core rate; the GPU
+    counts and candidate-set digests of every sampled query are compared."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     O.build()
-    P = max(1, min(16, os.cpu_count() or 1))
+    P = max(1, min(16, os.cpu_count() or 1, len(queries)))
     jobs = [(g, [(i, queries[i]) for i in range(w, len(queries), P)], cap, budget_s, method) for w in range(P)]
     if P == 1:
         results = [_cpu_worker(jobs[0])]
     else:
         import multiprocessing as mp
-        with mp.get_context("spawn").Pool(P) as pool:  # fresh interpreters: no GPU state in the workers
+        pool = mp.get_context("spawn").Pool(P)  # fresh interpreters: no GPU state in the workers
+        try:
             results = pool.map(_cpu_worker, jobs)
+        finally:
+            pool.close()  # workers exit on their own (no SIGTERM, as leaving a `with` block would send)
+            pool.join()
     done = sum(len(r) for r, _ in results)
-    emb = sum(c for r, _ in results for _, c, _ in r)
+    emb = sum(c for r, _ in results for _, c, _, _ in r)
     wall = max(w for _, w in results)
-    parity = all(c == gpu_counts[i] for r, _ in results for i, c, _ in r)
+    parity = all(c == gpu_counts[i] for r, _ in results for i, c, _, _ in r)
+    dig = None
+    if gpu_digests is not None:
+        pairs = [(d, gpu_digests[i]) for r, _ in results for i, _, _, d in r]
+        dig = {"checked": len(pairs), "match": all(a == b for a, b in pairs)}
     r0, w0 = results[0]
-    single = sum(c for _, c, _ in r0) / w0 if w0 > 0 else 0.0
-    print(f"[bench] cpu baseline: {done} queries on {P} cores in {wall:.1f}s, parity={parity}", flush=True)
+    single = sum(c for _, c, _, _ in r0) / w0 if w0 > 0 else 0.0
+    print(f"[bench] cpu baseline: {done} queries on {P} cores in {wall:.1f}s, parity={parity}, digests={dig}",
+          flush=True)
     return {"value": emb / wall if wall > 0 else 0.0, "unit": "embeddings/s", "cores": P, "kind": "port",
             "sample": f"{done} of the {len(queries)} queries (GQL + RI + "
                       f"{'aux-table/fs' if method == 'aux_fs' else 'cache/filter/fs'} enumeration, cap {cap}) "
                       f"on {P} worker processes, one query at a time each, {wall:.1f} s wall",
+            "intersection": "AVX2" if O.isect_is_avx2() else "scalar merge",
             "queries_per_s": done / wall if wall > 0 else 0.0,
             "single_core_value": single, "single_core_queries_per_s": len(r0) / w0 if w0 > 0 else 0.0,
-            "enum_thread_ms": sum(ms for r, _ in results for _, _, ms in r),
-            "counts_match_gpu": bool(parity)}
+            "enum_thread_ms": sum(ms for r, _ in results for _, _, ms, _ in r),
+            "counts_match_gpu": bool(parity), "candidate_sets_match_gpu": dig}
 
 
 if __name__ == "__main__":

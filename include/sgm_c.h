@@ -103,6 +103,9 @@ typedef struct {
     int32_t timed_out;       /* the query passed sgm_enum_opts.time_limit_ms: count is partial */
     uint64_t busy_ns;        /* cache methods: wave time spent on this query, summed over the
                                 waves (a work measure of the order; no reference counterpart) */
+    uint64_t probe_words;    /* words the intersections read under the strategy used: the keys of
+                                the shorter list plus, per key, one bitmap word or the ceil(log2 n)
+                                + 1 words of a binary search in a list of n */
 } sgm_stats;
 
 /* ---- context ------------------------------------------------------------------------ */

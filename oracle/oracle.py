@@ -190,6 +190,33 @@ def enum_cache(g, q, cands, order, cap=None, fs=True, time_limit_ms=0.0, roots=N
     return st.as_dict(n)
 
 
+def cand_digest(cands) -> str:
+    """SHA-256 over candidate sets in query-vertex order: per set its size, then its ids
+    ascending, all as little-endian u32 (the form parity checks compare)."""
+    import hashlib
+    h = hashlib.sha256()
+    for c in cands:
+        a = np.ascontiguousarray(c, dtype="<u4")
+        h.update(np.uint32(a.size).astype("<u4").tobytes())
+        h.update(a.tobytes())
+    return h.hexdigest()
+
+
+def isect_check(a, b):
+    """(same, result): the oracle's intersection of sorted u32 arrays a and b, and whether it
+    equals the scalar merge's."""
+    a, b = _u32(a), _u32(b)
+    o1 = np.zeros(max(1, min(a.size, b.size)), dtype=np.uint32)
+    o2 = np.zeros_like(o1)
+    n = C.c_uint32(0)
+    same = lib().orc_isect_test(_p(a), C.c_uint32(a.size), _p(b), C.c_uint32(b.size), _p(o1), _p(o2), C.byref(n))
+    return bool(same), o1[:n.value].copy()
+
+
+def isect_is_avx2() -> bool:
+    return bool(lib().orc_isect_avx2())
+
+
 def _roots(roots):
     lo, hi = (0, 0) if roots is None else roots
     return C.c_uint32(int(lo)), C.c_uint32(int(hi))

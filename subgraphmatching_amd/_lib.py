@@ -73,7 +73,7 @@ class Stats(C.Structure):
                 ("fs_prunes", C.c_uint64), ("elems_read", C.c_uint64), ("elems_out", C.c_uint64),
                 ("elems_scanned", C.c_uint64), ("kernel_ms", C.c_double),
                 ("capped", C.c_int32), ("stopped", C.c_int32), ("retries", C.c_uint32),
-                ("timed_out", C.c_int32), ("busy_ns", C.c_uint64)]
+                ("timed_out", C.c_int32), ("busy_ns", C.c_uint64), ("probe_words", C.c_uint64)]
 
 
 class CmSample(C.Structure):

@@ -411,6 +411,7 @@ class ExploreResult:
     stopped: bool = False  # ended early by a stop request: the count is partial
     busy_ns: int = 0       # wave time spent on the query, summed over waves
     timed_out: bool = False  # passed its time_limit_ms work budget: the count is partial
+    probe_words: int = 0     # words its intersections read: keys plus bitmap / binary-search probes
 
 
 def enumerate_query(can: CandidateParam, order, method=L.SGM_ENUM_CACHE_FILTER_FS, cap=None, blocks=0,
@@ -497,7 +498,8 @@ def _result(st, n):
                          nodes=[int(x) for x in st.nodes[:n]], fs_prunes=int(st.fs_prunes),
                          elems_read=int(st.elems_read), elems_out=int(st.elems_out),
                          elems_scanned=int(st.elems_scanned), capped=bool(st.capped), retries=int(st.retries),
-                         stopped=bool(st.stopped), busy_ns=int(st.busy_ns), timed_out=bool(st.timed_out))
+                         stopped=bool(st.stopped), busy_ns=int(st.busy_ns), timed_out=bool(st.timed_out),
+                         probe_words=int(st.probe_words))
 
 
 class Batch:

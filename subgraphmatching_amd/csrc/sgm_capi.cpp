@@ -1034,6 +1034,7 @@ void stats_out(const sgm_stats_dev& st, uint64_t cap, sgm_stats* out, double ns_
     out->elems_read = st.elems_read;
     out->elems_out = st.elems_out;
     out->elems_scanned = st.elems_scanned;
+    out->probe_words = st.probe_words;
 }
 
 extern "C" {

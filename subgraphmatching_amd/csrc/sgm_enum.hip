@@ -41,6 +41,9 @@ struct WaveDFS {
     // and its cutmin word
     uint32_t sp_lo, sp_it, sp_ri, sp_pad;
     uint32_t* sp_cut;
+    // probe_words of the wave's current query not yet published (kept in LDS: the search is
+    // at its register limit)
+    unsigned long long probes;
 };
 
 #define F_ANY 1u
@@ -107,6 +110,14 @@ __device__ __forceinline__ void member4(const uint32_t* __restrict__ b, uint32_t
     for (int u = 0; u < UNR; u++) hit[u] = valid[u] && x[u] == key[u];
 }
 
+// Words a membership test of one key in a list of n reads under the strategy used: one
+// bitmap word, or the ceil(log2 n) steps of the branch-free binary search plus its final
+// load.  An intersection's algorithmic words are its keys plus their probes (probe_words).
+__device__ __forceinline__ uint32_t probe_cost(uint32_t n, bool bitmap)
+{
+    return bitmap ? 1u : (n > 1 ? 33u - (uint32_t)__clz((int)(n - 1)) : 1u);
+}
+
 // Intersect sorted lists a and b into out (ascending); returns the size.  Lanes take
 // elements of the shorter list and test them against the longer one (bitmap or binary
 // search).  abits/bbits: bitmaps of a / b when they are hub adjacency lists, else null.
@@ -114,7 +125,7 @@ __device__ __forceinline__ void member4(const uint32_t* __restrict__ b, uint32_t
 __device__ uint32_t isect_write(const uint32_t* __restrict__ a, uint32_t na, const uint32_t* __restrict__ abits,
                                 const uint32_t* __restrict__ b, uint32_t nb, const uint32_t* __restrict__ bbits,
                                 uint32_t* __restrict__ out, unsigned long long& elems, unsigned long long& outs,
-                                const uint32_t* stop)
+                                unsigned long long* probes, const uint32_t* stop)
 {
     if (na > nb) {
         const uint32_t* t = a; a = b; b = t;
@@ -125,6 +136,7 @@ __device__ uint32_t isect_write(const uint32_t* __restrict__ a, uint32_t na, con
     uint32_t cnt = 0;
     elems += na + nb;
     if (nb == 0) return 0;
+    if (lane == 0) *probes += (unsigned long long)na * (1u + probe_cost(nb, bbits != nullptr));
     for (uint32_t base = 0; base < na; base += 64 * UNR) {
         uint32_t key[UNR];
         bool valid[UNR], hit[UNR];
@@ -267,7 +279,7 @@ __device__ void last_isect(const uint32_t* __restrict__ a, uint32_t na, const ui
                            const uint32_t* __restrict__ b, uint32_t nb, const uint32_t* __restrict__ bbits,
                            const uint32_t* __restrict__ lbits, const WaveDFS& S,
                            const uint64_t* anc, uint32_t L, LastRes& r, unsigned long long& elems,
-                           const uint32_t* stop)
+                           unsigned long long* probes, const uint32_t* stop)
 {
     if (na > nb) {
         const uint32_t* t = a; a = b; b = t;
@@ -278,6 +290,7 @@ __device__ void last_isect(const uint32_t* __restrict__ a, uint32_t na, const ui
     elems += na + nb;
     uint32_t size = 0;
     if (nb == 0) { r.size = 0; return; }
+    if (lane == 0) *probes += (unsigned long long)na * (1u + probe_cost(nb, bbits != nullptr));
     for (uint32_t base = 0; base < na; base += 64 * UNR) {
         uint32_t key[UNR];
         bool valid[UNR], hit[UNR];
@@ -416,7 +429,8 @@ __global__ void __launch_bounds__(256) k_split_reduce(const sgm_equery* __restri
                                     : f == 33u ? &Q.st->fs_prunes
                                     : f == 34u ? &Q.st->elems_read
                                     : f == 35u ? &Q.st->elems_out
-                                               : &Q.st->elems_scanned;
+                                    : f == 36u ? &Q.st->elems_scanned
+                                               : &Q.st->probe_words;
             if (t) atomicAdd(dst, t);
         }
         __syncthreads();
@@ -469,6 +483,11 @@ __device__ void flush_stats(const sgm_equery& Q, WaveDFS& S, WaveAcc& a, uint32_
         if (overflow) { atomicAdd(&Q.st->overflow, 1ull); atomicExch(Q.stop, 1u); }
     }
     a = WaveAcc{0, 0, 0, 0, 0, 0, 0, 0};
+    const unsigned long long pw = uni64(S.probes);
+    if (lane_id() == 0) {
+        if (pw) atomicAdd(&Q.st->probe_words, pw);
+        S.probes = 0;
+    }
     clear_nodes(S, &Q.st->nodes[0], n);
 }
 
@@ -604,6 +623,7 @@ __device__ void last_chunk(const sgm_plan_dev& P, WaveDFS& S, uint32_t d, const 
     const int mode = !has_op ? 2 : (par == SGM_NOPARENT ? 1 : 0);  // 0 isect, 1 N(v), 2 cached list
     SibAcc r = {0, 0, 0, 0};
     uint32_t nbs = 0, pn = 0;
+    uint32_t pcost = 0;  // this sibling's intersection: keys + probe words (probe_cost)
     constexpr int PU = LAST_PU;  // pairs per lane per step
     if (mode == 2) {
         const uint32_t ci = uni(P.iter_set[L]);
@@ -729,6 +749,9 @@ __device__ void last_chunk(const sgm_plan_dev& P, WaveDFS& S, uint32_t d, const 
 #pragma unroll
             for (int u = 0; u < PU; u++) { key[u] = nkey[u]; own[u] = nown[u]; valid[u] = nvalid[u]; }
         }
+        // (after the pair loop, so that it holds no register during it)
+        if (mode == 0 && mine && len)
+            pcost = len * (1u + probe_cost(kv ? pn : nbs, kv ? pbits != nullptr : slot != SGM_INVALID));
     }
     // apply the siblings' results in order (inc/explore.h:1036-1066 per extension), all lanes
     // at once: sibling i's failing mask f_i; the first sibling whose mask omits depth d cuts
@@ -755,6 +778,8 @@ __device__ void last_chunk(const sgm_plan_dev& P, WaveDFS& S, uint32_t d, const 
     if (mode == 0) {
         acc.inter += n_counted;
         acc.elems += (unsigned long long)pn * n_counted + rdlane(wave_incl_scan(counted ? nbs : 0u), 63);
+        const uint32_t pw = rdlane(wave_incl_scan(counted ? pcost : 0u), 63);
+        if (lane == 0) S.probes += pw;
         acc.outs += size_sum;
     }
     if (use_fs) {
@@ -808,6 +833,7 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
 
     clear_nodes(S, nullptr, 0);
     WaveAcc acc = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (lane == 0) S.probes = 0;
     uint32_t cur = SGM_INVALID;  // query whose plan sits in this wave's LDS
     uint32_t n = 0, L = 0, root_u = 0;
     const uint32_t* lastbits = nullptr;  // bitmap of C(u_L) of the current query
@@ -830,7 +856,7 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
     uint32_t k = (gwave / ncls) % csize(cls), seen = 0, classes = 0;
     while (classes < ncls) {
         if (__atomic_load_n(dev_stop, __ATOMIC_RELAXED)) break;
-        relay_stop(wall_clock64(), relay_at, host_stop, dev_stop);
+        if ((++poll & 63u) == 0) relay_stop(wall_clock64(), relay_at, host_stop, dev_stop);
         const uint32_t q = cls + ncls * k;
         const sgm_equery& Q = qs[q];
         uint32_t it = SGM_INVALID;
@@ -936,7 +962,8 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
                     acc.inter++;
                     const uint32_t sz = isect_write(op_list(op_off[p], adj, ws), op_size[p],
                                                     op_vtx[p] != SGM_INVALID ? vbits(op_vtx[p]) : nullptr,
-                                                    adj + nbo, nbs, vbits(v0), ws + top, acc.elems, acc.outs, stop);
+                                                    adj + nbo, nbs, vbits(v0), ws + top, acc.elems, acc.outs,
+                                                    &S.probes, stop);
                     op_off[o] = top | STACK_FLAG;
                     op_size[o] = sz;
                     op_vtx[o] = SGM_INVALID;
@@ -1046,7 +1073,7 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
                                 acc.inter++;
                                 last_isect(op_list(op_off[p], adj, ws), op_size[p],
                                            op_vtx[p] != SGM_INVALID ? vbits(op_vtx[p]) : nullptr, adj + nbo, nbs,
-                                           vbits(v), lastbits, S, P.anc, L, r, acc.elems, stop);
+                                           vbits(v), lastbits, S, P.anc, L, r, acc.elems, &S.probes, stop);
                                 acc.outs += r.size;
                             }
                             if (r.size == 0) { empty = true; conf_v = P.op_vdepth[o]; }
@@ -1090,7 +1117,7 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
                             const uint32_t sz = isect_write(op_list(op_off[p], adj, ws), op_size[p],
                                                             op_vtx[p] != SGM_INVALID ? vbits(op_vtx[p]) : nullptr,
                                                             adj + nbo, nbs, vbits(v), ws + top, acc.elems, acc.outs,
-                                                            stop);
+                                                            &S.probes, stop);
                             op_off[o] = top | STACK_FLAG;
                             op_size[o] = sz;
                             op_vtx[o] = SGM_INVALID;
@@ -1135,8 +1162,10 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
             else if (lane == 34) x = acc.elems;
             else if (lane == 35) x = acc.outs;
             else if (lane == 36) x = acc.scanned;
+            else if (lane == 37) x = S.probes;
             if (lane < SGM_SPLIT_FIELDS) rec[(size_t)lane * SGM_SPLIT_ITEMS] = x;
             acc.inter = acc.fsp = acc.elems = acc.outs = acc.scanned = 0;
+            if (lane == 0) S.probes = 0;
             clear_nodes(S, nullptr, 0);
             if (lane == 0 && !aborted) {
                 Q.split[2 + 2 * SGM_SPLIT_ROOTS + sit] = 1u;  // done
@@ -1168,6 +1197,7 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
 // check.  One wave per depth-0 candidate; the last level is intersected and counted in one
 // pass.  Per-wave intermediate lists live on the same HBM stack as the cache search.
 struct WaveAux {
+    unsigned long long probes;  // probe_words not yet published
     uint64_t rem[SGM_MAXQ];
     uint64_t fail[SGM_MAXQ];  // failing sets (use_fs)
     unsigned long long nodes[SGM_MAXQ];
@@ -1186,7 +1216,7 @@ struct WaveAux {
 __device__ bool aux_list(const sgm_auxplan_dev& A, const WaveAux& S, uint32_t d, uint32_t* ws, uint32_t stack_words,
                          uint32_t& top, const uint32_t*& list, uint32_t& size, bool leave_last,
                          const uint32_t*& last, uint32_t& nlast, unsigned long long& elems, unsigned long long& outs,
-                         const uint32_t* stop)
+                         unsigned long long* probes, const uint32_t* stop)
 {
     const uint32_t f = A.prev_first[d], e_end = A.prev_first[d + 1];
     const uint32_t np = e_end - f;
@@ -1204,7 +1234,8 @@ __device__ bool aux_list(const sgm_auxplan_dev& A, const WaveAux& S, uint32_t d,
         const uint32_t b = o[r], nb = o[r + 1] - b;
         const uint32_t need = size < nb ? size : nb;
         if (top + need > stack_words) return false;
-        const uint32_t sz = isect_write(list, size, nullptr, A.tab_idx[e] + b, nb, nullptr, ws + top, elems, outs, stop);
+        const uint32_t sz =
+            isect_write(list, size, nullptr, A.tab_idx[e] + b, nb, nullptr, ws + top, elems, outs, probes, stop);
         list = ws + top;
         size = sz;
         top += sz;
@@ -1259,6 +1290,7 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
     WaveAux& S = *reinterpret_cast<WaveAux*>(smem + wave * ((sizeof(WaveAux) + 15) & ~(size_t)15));
     uint32_t* ws = stack + (size_t)(blockIdx.x * (blockDim.x >> 6) + wave) * stack_words;
     for (uint32_t d = 0; d < SGM_MAXQ; d++) S.nodes[d] = 0;
+    S.probes = 0;
     unsigned long long inter = 0, elems = 0, outs = 0, scanned = 0, cnt_pending = 0, items_done = 0, fsp = 0;
     unsigned long long busy = 0;  // wave time on the current query not yet added to its total
     bool overflow = false;
@@ -1282,6 +1314,8 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
             if (items_done) atomicAdd(&gst->items_done, items_done);
             if (fsp) atomicAdd(&gst->fs_prunes, fsp);
             if (busy) atomicAdd(&gst->busy_ticks, busy);
+            if (S.probes) atomicAdd(&gst->probe_words, S.probes);
+            S.probes = 0;
             if (ovf) { atomicAdd(&gst->overflow, 1ull); atomicExch(stop, 1u); }
             for (uint32_t dd = 0; dd < n; dd++)
                 if (S.nodes[dd]) atomicAdd(&gst->nodes[dd], S.nodes[dd]);
@@ -1372,7 +1406,8 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
                 const uint32_t* last;
                 uint32_t nlast;
                 const bool is_last = d == L;
-                if (!aux_list(A, S, d, ws, stack_words, top, list, size, is_last, last, nlast, elems, outs, stop)) {
+                if (!aux_list(A, S, d, ws, stack_words, top, list, size, is_last, last, nlast, elems, outs,
+                              &S.probes, stop)) {
                     overflow = true; aborted = true; break;
                 }
                 if (is_last) {
@@ -1392,6 +1427,7 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
                         const uint32_t* b = last; uint32_t nb = nlast;
                         if (na > nb) { const uint32_t* t = a; a = b; b = t; uint32_t tn = na; na = nb; nb = tn; }
                         elems += na + nb;
+                        if (nb && lane == 0) S.probes += (unsigned long long)na * (1u + probe_cost(nb, false));
                         cands = 0;
                         for (uint32_t base = 0; base < na; base += 64) {
                             const uint32_t i = base + lane;

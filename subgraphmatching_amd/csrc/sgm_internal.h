@@ -81,6 +81,8 @@ struct sgm_stats_dev {
     unsigned long long items_done;
     unsigned long long busy_ticks;      // wall-clock ticks its waves spent on the query's items
     unsigned long long timed_out;       // the waves' time passed the query's budget (count partial)
+    unsigned long long probe_words;     // words the intersections read under the strategy used: keys
+                                        // plus one bitmap word or ceil(log2 n) + 1 search words each
 };
 
 // Depth-1 splitting of the heaviest roots.  The first SGM_SPLIT_ROOTS candidate roots of a
@@ -95,13 +97,13 @@ struct sgm_stats_dev {
 //   [2, 2+R) the split roots (vertex ids)  [2+R, 2+2R) cutmin (smallest start of a
 //   cutting chunk, ~0 = none)  [2+2R, 2+2R+IC) done flags  then IC triples (root, lo, hi)
 // Records, u64, field-major (field f of item s at rec[f * IC + s]): nodes[32],
-// intersections, fs_prunes, elems_read, elems_out, elems_scanned.
+// intersections, fs_prunes, elems_read, elems_out, elems_scanned, probe_words.
 #define SGM_SPLIT_ROOTS 64u
 #define SGM_SPLIT_CHUNKS 32u  // 16 / 32 / 64: labelled Chung-Lu 10^5 launch 217 / 182 / 172 ms
 #define SGM_SPLIT_ITEMS (SGM_SPLIT_ROOTS * SGM_SPLIT_CHUNKS)
 #define SGM_SPLIT_SCAN 4096u
 #define SGM_SPLIT_WORDS (2u + 2u * SGM_SPLIT_ROOTS + 4u * SGM_SPLIT_ITEMS)
-#define SGM_SPLIT_FIELDS 37u
+#define SGM_SPLIT_FIELDS 38u
 // queries with at most this many candidate roots are split by default (a launch's 4096-odd
 // waves would otherwise sit idle behind their heaviest roots); more roots keep every wave
 // busy without it, and the split variant of the search costs a few per cent
@@ -132,7 +134,8 @@ struct sgm_equery {
 // (a queued copy could not start while a persistent grid holds every CU).  Waves poll the
 // device stop word; each wave copies the host word into it at most once per
 // SGM_RELAY_TICKS of the device clock (100 MHz: 2.6 ms), so a full grid of 4096 waves reads
-// host memory about 1.6 million times per second.
+// host memory about 1.6 million times per second.  Between items a wave looks at the clock
+// for this only every 64 item attempts (most attempts skip a non-candidate at once).
 #define SGM_RELAY_TICKS (1ull << 18)
 // A wave adds its time on a query to the query's total (the budget check) after at most
 // SGM_CHARGE_TICKS of unpublished time (100 MHz: 82 us).

@@ -11,23 +11,10 @@ CPU oracle: they carry exact, uncapped parity to full-size inputs.
 """
 from __future__ import annotations
 
-import hashlib
-
-import numpy as np
+from oracle import cand_digest  # noqa: F401  (one definition: the oracle's)
 
 # the four enumeration variants compared on every slice: (name, cache search?, failing sets?)
 VARIANTS = [("cache_fs", True, True), ("cache", True, False), ("aux", False, False), ("aux_fs", False, True)]
-
-
-def cand_digest(cands) -> str:
-    """SHA-256 over the candidate sets in query-vertex order: per set its size, then its
-    ids ascending, all as little-endian u32."""
-    h = hashlib.sha256()
-    for c in cands:
-        a = np.ascontiguousarray(c, dtype="<u4")
-        h.update(np.uint32(a.size).astype("<u4").tobytes())
-        h.update(a.tobytes())
-    return h.hexdigest()
 
 
 def oracle_stats(O, g, q, cands, order, variant, roots, time_limit_ms=0.0):

@@ -34,6 +34,26 @@ def small_cases(seed, count, V=(7, 10), p=(0.3, 0.6), n=(3, 5), labels=0):
         yield g, q
 
 
+# ------------------------------------------------- set intersection: against set semantics
+def test_intersection_matches_sets():
+    """The oracle's intersection (AVX2 when built with it, as the reference's build is:
+    inc/common_ops.h:18-150, 248-254) equals both the scalar merge and Python's set
+    intersection on sorted lists of every size shape: empty, single, tails shorter than a
+    vector, skewed (ratio > 2: broadcast path) and balanced (the 2 x 4 path)."""
+    rng = np.random.default_rng(41)
+    assert O.isect_is_avx2()
+    sizes = [0, 1, 2, 3, 4, 5, 7, 8, 9, 15, 16, 17, 31, 64, 100, 257, 1000]
+    for na in sizes:
+        for nb in sizes:
+            for dens in (0.05, 0.5, 0.95):
+                universe = max(8, int((na + nb) / dens))
+                a = np.sort(rng.choice(universe, min(na, universe), replace=False)).astype(np.uint32)
+                b = np.sort(rng.choice(universe, min(nb, universe), replace=False)).astype(np.uint32)
+                same, got = O.isect_check(a, b)
+                assert same, (na, nb, dens)
+                assert got.tolist() == sorted(set(a.tolist()) & set(b.tolist())), (na, nb, dens)
+
+
 # ---------------------------------------------------------------- NLF: closed form
 def test_nlf_is_degree_filter():
     g = chung_lu(500, 2.5, 6.0, seed=3)
