@@ -236,3 +236,74 @@ def enum_all(g, q, cands, order, cap=None, time_limit_ms=0.0, fs=False, roots=No
     if rc:
         raise ValueError(f"orc_enum_all failed ({rc})")
     return st.as_dict(n)
+
+
+# ---- BFS-tree filters and orders (sgm_oracle_tree.c) -----------------------------------
+FILTER_KIND = {"CFL": 3, "CECI": 4, "DAF": 5}   # the C ABI's SGM_FILTER_* numbers
+ORDER_KIND = {"QSI": 1, "CFL": 2, "CECI": 3, "GuP": 4}
+
+
+def filter_tree(kind, g, q):
+    """_CFL_Filter / _CECI_Filter / _DAF_Filter + FilterParamConstruct's sort: (ok, sets).
+    ``kind`` is "CFL", "CECI" or "DAF"."""
+    V, n = g.getVertexCnt(), q.getVertexCnt()
+    cand = np.zeros((n, max(V, 1)), dtype=np.uint32)
+    cnt = np.zeros(n, dtype=np.uint32)
+    rc = lib().orc_filter_tree(C.c_int(FILTER_KIND[kind]), C.c_uint32(V), _p(g.off), _p(g.adj), _lab(g),
+                               C.c_uint32(n), _p(q.off), _p(q.adj), _lab(q), _p(cand), _p(cnt))
+    if rc < 0:
+        raise ValueError(f"orc_filter_tree({kind}) failed ({rc})")
+    return bool(rc), [cand[u, :cnt[u]].copy() for u in range(n)]
+
+
+def start_vertex(kind, g, q):
+    """CFLStartVertex / CECIStartVertex / DAFStartVertex."""
+    L = lib()
+    L.orc_start_vertex.restype = C.c_uint32
+    return int(L.orc_start_vertex(C.c_int(FILTER_KIND[kind]), C.c_uint32(g.getVertexCnt()), _p(g.off), _p(g.adj),
+                                  _lab(g), C.c_uint32(q.getVertexCnt()), _p(q.off), _p(q.adj), _lab(q)))
+
+
+def kcore(q):
+    """Query::KCore core numbers."""
+    n = q.getVertexCnt()
+    core = np.zeros(n, dtype=np.int32)
+    lib().orc_kcore(C.c_uint32(n), _p(q.off), _p(q.adj), _p(core))
+    return [int(x) for x in core]
+
+
+def bfs(q, start):
+    """BFS tree: (order, parent, level)."""
+    n = q.getVertexCnt()
+    o, par, lev = (np.zeros(n, dtype=np.uint32) for _ in range(3))
+    L = lib()
+    L.orc_bfs.restype = C.c_uint32
+    k = L.orc_bfs(C.c_uint32(n), _p(q.off), _p(q.adj), C.c_uint32(start), _p(o), _p(par), _p(lev))
+    return [int(x) for x in o[:k]], [int(x) for x in par], [int(x) for x in lev]
+
+
+def order_tree(kind, g, q, cands):
+    """Order_QSI / Order_CFL / Order_CECI / Order_GuP over the given (filtered) candidate
+    sets; None where the reference asserts (GuP on a query of maximum degree <= 2)."""
+    V, n = g.getVertexCnt(), q.getVertexCnt()
+    m, cnt, stride = _cand_matrix(cands, V)
+    o = np.zeros(n, dtype=np.uint32)
+    rc = lib().orc_order_tree(C.c_int(ORDER_KIND[kind]), C.c_uint32(V), _p(g.off), _p(g.adj), _lab(g),
+                              C.c_uint32(n), _p(q.off), _p(q.adj), _lab(q), _p(m), _p(cnt), C.c_uint32(stride),
+                              _p(o))
+    return None if rc else [int(x) for x in o]
+
+
+def heap_pop_order(keys, keep, use_std=False):
+    """Vertices in the order CFLStartVertex's max-heap visits them after trimming it to
+    ``keep`` entries: the oracle's restatement, or std::priority_queue itself."""
+    k = np.ascontiguousarray(keys, dtype=np.float64)
+    out = np.zeros(max(k.size, 1), dtype=np.uint32)
+    if use_std:
+        path = os.path.join(HERE, "libsortprobe.so")
+        if not os.path.exists(path):
+            build()
+        C.CDLL(path).probe_std_heap(C.c_uint(k.size), _p(k), C.c_uint(keep), _p(out))
+    else:
+        lib().orc_heap_test(C.c_uint32(k.size), _p(k), C.c_uint32(keep), _p(out))
+    return [int(x) for x in out[:min(keep, k.size)]]

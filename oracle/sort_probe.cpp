@@ -33,3 +33,22 @@ extern "C" void probe_std_sort(long n, const uint32_t *depth, const uint32_t *si
     std::sort(v.begin(), v.end());
     for (long i = 0; i < n; i++) perm_out[i] = v[i].payload;
 }
+
+// std::priority_queue over (vertex, key) pairs with the comparator of the reference's
+// CFLStartVertex (inc/filter.h:437-441: l.second < r.second, a max-heap on the key): push
+// (v, keys[v]) for v < count, pop while more than `keep` remain, then pop the rest into
+// out[] (the order in which CFLStartVertex visits them).
+#include <queue>
+#include <utility>
+
+extern "C" void probe_std_heap(unsigned count, const double *keys, unsigned keep, uint32_t *out) {
+    auto cmp = [](std::pair<uint32_t, double> l, std::pair<uint32_t, double> r) { return l.second < r.second; };
+    std::priority_queue<std::pair<uint32_t, double>, std::vector<std::pair<uint32_t, double>>, decltype(cmp)> q(cmp);
+    for (unsigned v = 0; v < count; v++) q.push(std::make_pair((uint32_t)v, keys[v]));
+    while (q.size() > keep) q.pop();
+    unsigned k = 0;
+    while (!q.empty()) {
+        out[k++] = q.top().first;
+        q.pop();
+    }
+}
