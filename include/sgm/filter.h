@@ -1,8 +1,9 @@
-// Candidate filters (inc/filter.h:1049-1063), drop-in over the C ABI: the NLF and GQL
-// kernels run on the device, then FilterParamConstruct's host arrays and aux-edge tables
-// are filled from the device candidate space.  Both return false when some C(u) is empty,
-// as the reference does; candidates / candidates_count then hold the filtered sets and the
-// membership arrays and tables stay unbuilt.
+// Candidate filters (inc/filter.h:1049-1087), drop-in over the C ABI: the NLF, GQL, CFL,
+// CECI and DAF kernels run on the device, then FilterParamConstruct's host arrays and
+// aux-edge tables are filled from the device candidate space.  All return false when some
+// C(u) is empty, as the reference does (Filter_CECI also where the reference ends the process
+// on an empty set, inc/filter.h:700-703); candidates / candidates_count then hold the
+// filtered sets and the membership arrays and tables stay unbuilt.
 #pragma once
 
 #include "api.h"
@@ -29,4 +30,19 @@ inline bool Filter_NLF(const Graph& data, const Query& query, CandidateParam& ca
 inline bool Filter_GQL(const Graph& data, const Query& query, CandidateParam& can)
 {
     return sgm::run_filter(data, query, can, SGM_FILTER_GQL);
+}
+
+inline bool Filter_CFL(const Graph& data, const Query& query, CandidateParam& can)
+{
+    return sgm::run_filter(data, query, can, SGM_FILTER_CFL);
+}
+
+inline bool Filter_CECI(const Graph& data, const Query& query, CandidateParam& can)
+{
+    return sgm::run_filter(data, query, can, SGM_FILTER_CECI);
+}
+
+inline bool Filter_DAF(const Graph& data, const Query& query, CandidateParam& can)
+{
+    return sgm::run_filter(data, query, can, SGM_FILTER_DAF);
 }

@@ -8,8 +8,8 @@
  * per host thread; calls on one context are not thread-safe except sgm_stop().
  *
  * Each entry point names the reference interface it replaces (paths relative to the
- * Orrinn/SubgraphMatching source tree).  Only unlabeled graphs (all labels 0) are on this
- * path, and queries have 2..32 vertices.
+ * Orrinn/SubgraphMatching source tree).  Graphs and queries may carry vertex labels (all 0
+ * when none are set); queries have 2..32 vertices.
  */
 #ifndef SGM_C_H
 #define SGM_C_H
@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define SGM_ABI_VERSION 2
+#define SGM_ABI_VERSION 3
 
 enum {
     SGM_OK = 0,
@@ -36,7 +36,14 @@ enum {
     SGM_ENODEV = -9     /* no usable GPU */
 };
 
-enum { SGM_FILTER_NLF = 1, SGM_FILTER_GQL = 2 };
+enum {
+    SGM_FILTER_NLF = 1,   /* Filter_NLF (inc/filter.h:1049) */
+    SGM_FILTER_GQL = 2,   /* Filter_GQL (inc/filter.h:1057) */
+    SGM_FILTER_CFL = 3,   /* Filter_CFL (inc/filter.h:1065, _CFL_Filter :481) */
+    SGM_FILTER_CECI = 4,  /* Filter_CECI (inc/filter.h:1073, _CECI_Filter :602); where the reference
+                             ends the process on an empty set (:700-703) this returns SGM_EEMPTY */
+    SGM_FILTER_DAF = 5    /* Filter_DAF (inc/filter.h:1081, _DAF_Filter :942) */
+};
 
 enum {
     SGM_ENUM_CACHE_FILTER_FS = 0, /* explore_Intersection_Cache_DAF_Filter (inc/explore.h:1073) */
@@ -117,7 +124,8 @@ int sgm_ctx_create(int device, sgm_ctx** out);
 void sgm_ctx_destroy(sgm_ctx* ctx);
 const char* sgm_ctx_last_error(const sgm_ctx* ctx);
 /* Device time (HIP events) of the last filter call on this context: the NLF kernel, the
- * GQL refinement phases and the candidate bitmap/count kernels (no reference
+ * refinement (the GQL phases, or the CFL / CECI / DAF steps with their NLF counts) and the
+ * candidate bitmap/count kernels (no reference
  * counterpart: the reference times filters with a host clock,
  * src/optimizations_ablation.cpp:64-82). */
 int sgm_ctx_filter_ms(const sgm_ctx* ctx, double* nlf_ms, double* gql_ms, double* cand_ms);
@@ -151,7 +159,9 @@ void sgm_query_destroy(sgm_query* q);
 int sgm_graph_set_labels(sgm_graph* g, const uint32_t* labels, uint32_t num_vertices);
 int sgm_query_set_labels(sgm_query* q, const uint32_t* labels, uint32_t n);
 
-/* Filter_NLF / Filter_GQL (inc/filter.h:1049-1063, _NLF_Filter :80-93, _GQL_Filter :233-313).
+/* Filter_NLF / Filter_GQL / Filter_CFL / Filter_CECI / Filter_DAF (inc/filter.h:1049-1087;
+ * _NLF_Filter :80-93, _GQL_Filter :233-313, _CFL_Filter :481-583, _CECI_Filter :602-920,
+ * _DAF_Filter :942-1020).
  * Returns SGM_OK, or SGM_EEMPTY when some C(u) is empty (the reference returns false). */
 int sgm_filter(sgm_query* q, int kind);
 /* Same for many queries of one graph in one set of launches; status[i] gets each result. */
@@ -178,6 +188,24 @@ int sgm_table_download(sgm_query* q, uint32_t a, uint32_t b, uint32_t* offsets, 
  * inc/code_gen.h:699) ------------------------------------------------------------------ */
 int sgm_order_ri(const uint32_t* qoff, const uint32_t* qadj, uint32_t n, uint32_t* order);
 int sgm_order_gql(const sgm_query* q, uint32_t* order);
+/* Order_QSI (inc/order.h:610-668): from the candidate space of a filtered query; builds the
+ * query's aux tables (sgm_build_tables) if they are missing. */
+int sgm_order_qsi(sgm_query* q, uint32_t* order);
+/* Order_CFL (inc/order.h:781-976): core paths, tree paths and leaves by estimated embeddings
+ * over the aux tables (built if missing; the estimates run on the device). */
+int sgm_order_cfl(sgm_query* q, uint32_t* order);
+/* Order_CECI (inc/order.h:599-607): the BFS order from CECIStartVertex, which reads the NLF
+ * counts of the query's vertices only, so q need not be filtered. */
+int sgm_order_ceci(sgm_query* q, uint32_t* order);
+/* Order_GuP (inc/order.h:537-561): 2-core by edge weights, then the tree part by path counts
+ * over the aux tables (built if missing; count_embed runs on the device).  SGM_EINVAL for a
+ * query whose vertices all have degree <= 2, where the reference asserts (:484-492). */
+int sgm_order_gup(sgm_query* q, uint32_t* order);
+/* Query::KCore core numbers (inc/graph.h:387-458) and a BFS tree (inc/common_ops.h:318-350):
+ * host helpers of the orders, exported for inspection; the BFS returns the vertex count. */
+int sgm_query_kcore(const uint32_t* qoff, const uint32_t* qadj, uint32_t n, int32_t* core);
+int sgm_query_bfs(const uint32_t* qoff, const uint32_t* qadj, uint32_t n, uint32_t start, uint32_t* order,
+                  uint32_t* parent, uint32_t* level);
 /* Plan export for inspection: per op depth, parent (UINT32_MAX = none), v_depth, bitmask
  * of backward-neighbour depths; iter_set[n].  Arrays hold up to 512 ops. */
 int sgm_plan_export(const uint32_t* qoff, const uint32_t* qadj, uint32_t n, const uint32_t* order, uint32_t* n_ops,

@@ -27,6 +27,9 @@ SGM_ENODEV = -9
 
 SGM_FILTER_NLF = 1
 SGM_FILTER_GQL = 2
+SGM_FILTER_CFL = 3
+SGM_FILTER_CECI = 4
+SGM_FILTER_DAF = 5
 
 SGM_ENUM_CACHE_FILTER_FS = 0
 SGM_ENUM_CACHE_FILTER = 1
@@ -51,7 +54,8 @@ EXPORTS = [
     "sgm_costmodel_features", "sgm_costmodel_predict", "sgm_costmodel_grad", "sgm_costmodel_train",
     "sgm_costmodel_timing", "sgm_order_sample", "sgm_batch_rank_orders", "sgm_gql_work",
     "sgm_graph_set_labels", "sgm_query_set_labels", "sgm_plan_export_labeled", "sgm_batch_filter_labeled",
-    "sgm_enumerate_noalloc",
+    "sgm_enumerate_noalloc", "sgm_order_qsi", "sgm_order_cfl", "sgm_order_ceci", "sgm_order_gup",
+    "sgm_query_kcore", "sgm_query_bfs",
 ]
 
 
@@ -154,6 +158,12 @@ def lib() -> C.CDLL:
         "sgm_order_sample": (C.c_int, [vp, vp, C.c_uint32, C.c_uint64, vp]),
         "sgm_batch_rank_orders": (C.c_int, [vp, vp, C.c_uint32, C.c_uint64, C.c_float, vp, vp, vp, vp]),
         "sgm_gql_work": (C.c_int, [vp, vp]),
+        "sgm_order_qsi": (C.c_int, [vp, vp]),
+        "sgm_order_cfl": (C.c_int, [vp, vp]),
+        "sgm_order_ceci": (C.c_int, [vp, vp]),
+        "sgm_order_gup": (C.c_int, [vp, vp]),
+        "sgm_query_kcore": (C.c_int, [vp, vp, C.c_uint32, vp]),
+        "sgm_query_bfs": (C.c_int, [vp, vp, C.c_uint32, C.c_uint32, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -29,7 +29,8 @@ from . import _lib as L
 from .graph import Graph, Query
 
 __all__ = ["Context", "default_context", "DeviceGraph", "CandidateParam", "Edges", "Filter_NLF",
-           "Filter_GQL", "Order_RI", "Order_GQL", "ReuseParam", "Config", "ExploreResult",
+           "Filter_GQL", "Filter_CFL", "Filter_CECI", "Filter_DAF", "Order_RI", "Order_GQL", "Order_QSI",
+           "Order_CFL", "Order_CECI", "Order_GuP", "query_kcore", "ReuseParam", "Config", "ExploreResult",
            "explore_Intersection_Cache_DAF_Filter", "explore_Intersection_All", "Profiler",
            "filter_batch", "enumerate_query", "enumerate_batch", "plan_export", "Batch"]
 
@@ -297,6 +298,23 @@ def Filter_GQL(data: Graph, query: Query, can: CandidateParam, build_tables: boo
     return _filter(data, query, can, L.SGM_FILTER_GQL, build_tables)
 
 
+def Filter_CFL(data: Graph, query: Query, can: CandidateParam, build_tables: bool = True) -> bool:
+    """inc/filter.h:1065-1071 (_CFL_Filter :481-583): BFS-tree generation and refinement on the
+    device, then FilterParamConstruct (tables)."""
+    return _filter(data, query, can, L.SGM_FILTER_CFL, build_tables)
+
+
+def Filter_CECI(data: Graph, query: Query, can: CandidateParam, build_tables: bool = True) -> bool:
+    """inc/filter.h:1073-1079 (_CECI_Filter :602-920).  Where the reference ends the process on
+    an empty candidate set (:700-703), this returns False."""
+    return _filter(data, query, can, L.SGM_FILTER_CECI, build_tables)
+
+
+def Filter_DAF(data: Graph, query: Query, can: CandidateParam, build_tables: bool = True) -> bool:
+    """inc/filter.h:1081-1087 (_DAF_Filter :942-1020): NLF, then three refinement passes."""
+    return _filter(data, query, can, L.SGM_FILTER_DAF, build_tables)
+
+
 def filter_batch(cans, kind=L.SGM_FILTER_GQL):
     """Filter many queries of one data graph in one set of launches; returns [bool]."""
     if not cans:
@@ -326,6 +344,54 @@ def Order_GQL(data: Graph, query: Query, can: CandidateParam) -> list:
     o = np.zeros(query.getVertexCnt(), dtype=np.uint32)
     _check(L.lib().sgm_order_gql(can._q, L.ptr(o)), can.ctx)
     return [int(x) for x in o]
+
+
+def _order_of(fn, name, query, can):
+    o = np.zeros(query.getVertexCnt(), dtype=np.uint32)
+    code = fn(can._q, L.ptr(o))
+    if code != L.SGM_OK:
+        msg = L.lib().sgm_ctx_last_error(can.ctx.handle)
+        raise L.SgmError(code, f"{name}: {msg.decode() if msg else ''}")
+    return [int(x) for x in o]
+
+
+def Order_QSI(data: Graph, query: Query, can: CandidateParam) -> list:
+    """inc/order.h:610-668 (needs filtered candidates; builds the aux tables if missing)."""
+    return _order_of(L.lib().sgm_order_qsi, "Order_QSI", query, can)
+
+
+def Order_CFL(data: Graph, query: Query, can: CandidateParam) -> list:
+    """inc/order.h:781-976 (path estimates over the aux tables run on the device)."""
+    return _order_of(L.lib().sgm_order_cfl, "Order_CFL", query, can)
+
+
+def Order_CECI(data: Graph, query: Query, can: CandidateParam | None = None) -> list:
+    """inc/order.h:599-607: the BFS order from CECIStartVertex (NLF counts only: ``can`` may be
+    unfiltered, or None for a temporary candidate space on the default context)."""
+    if can is None:
+        can = CandidateParam(data, query)
+        try:
+            return _order_of(L.lib().sgm_order_ceci, "Order_CECI", query, can)
+        finally:
+            can.close()
+    return _order_of(L.lib().sgm_order_ceci, "Order_CECI", query, can)
+
+
+def Order_GuP(data: Graph, query: Query, can: CandidateParam) -> list:
+    """inc/order.h:537-561.  Raises SgmError(SGM_EINVAL) for a query whose vertices all have
+    degree <= 2, where the reference asserts (:484-492)."""
+    return _order_of(L.lib().sgm_order_gup, "Order_GuP", query, can)
+
+
+def query_kcore(query: Query) -> list:
+    """Query::KCore core numbers (inc/graph.h:387-458)."""
+    n = query.getVertexCnt()
+    core = np.zeros(n, dtype=np.int32)
+    qoff, qadj = L.u32(query.off), L.u32(query.adj)
+    code = L.lib().sgm_query_kcore(L.ptr(qoff), L.ptr(qadj), n, L.ptr(core))
+    if code != L.SGM_OK:
+        raise L.SgmError(code, "query_kcore")
+    return [int(x) for x in core]
 
 
 def plan_export(query: Query, order):

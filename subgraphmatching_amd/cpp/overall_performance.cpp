@@ -6,7 +6,7 @@
 // is printed per row.
 //
 // usage: sgm_overall DATA.txt QUERY.txt [output_limit=100000] [timeout_s=300] [rows...]
-// rows: RI  GQL  RI+cache+filter+fs   (default: all three)
+// rows: RI  QSI  GQL  CFL  CECI  DAF  RI+cache+filter+fs   (default: all seven)
 #include <cstdio>
 #include <future>
 #include <iostream>
@@ -33,8 +33,8 @@ std::string fmt2(double x)
     return buf;
 }
 
-// One algorithm row (src/overall_performance.cpp:40-97 for "RI" and "GQL", :243-262 for
-// "RI+cache+filter+fs").  The promise is set once the search is about to start, so the
+// One algorithm row (src/overall_performance.cpp:52-127 for "RI", "QSI", "GQL", "CFL" and
+// "CECI", :158-172 for "DAF", :243-262 for "RI+cache+filter+fs").  The promise is set once the search is about to start, so the
 // timeout covers the search only, as in the reference.
 std::string run_row(const Graph& data, const Query& query, const std::string& alg, std::atomic<bool>& stop_flag,
                     std::promise<void> started, uint64_t output_limit)
@@ -68,6 +68,30 @@ std::string run_row(const Graph& data, const Query& query, const std::string& al
                 cfg.can = &can;
                 explore_Intersection_All<false, false, false, false, false>(data, query, order, cfg, time, ans,
                                                                             stop_flag, output_limit);
+            }
+        } else if (alg == "QSI" || alg == "CFL" || alg == "CECI" || alg == "DAF") {
+            // the candidate-space rows of the other filters and orders; DAF adds failing sets
+            CandidateParam can(data, query);
+            const bool ok = alg == "QSI"    ? Filter_NLF(data, query, can)
+                            : alg == "CFL"  ? Filter_CFL(data, query, can)
+                            : alg == "CECI" ? Filter_CECI(data, query, can)
+                                            : Filter_DAF(data, query, can);
+            if (ok) {
+                Order order;
+                if (alg == "QSI") Order_QSI(data, query, can, order);
+                else if (alg == "CFL") Order_CFL(data, query, can, order);
+                else if (alg == "CECI") Order_CECI(data, query, order);
+                else Order_RI(query, order);
+                start();
+                Config cfg;
+                cfg.useFilter = true;
+                cfg.can = &can;
+                if (alg == "DAF")
+                    explore_Intersection_All<true, false, false, false, false>(data, query, order, cfg, time, ans,
+                                                                               stop_flag, output_limit);
+                else
+                    explore_Intersection_All<false, false, false, false, false>(data, query, order, cfg, time, ans,
+                                                                                stop_flag, output_limit);
             }
         } else if (alg == "RI+cache+filter+fs") {
             Order order;
@@ -124,7 +148,7 @@ int main(int argc, char** argv)
     const uint32_t timeout_s = argc > 4 ? (uint32_t)std::stoul(argv[4]) : 300u;
     std::vector<std::string> rows;
     for (int i = 5; i < argc; i++) rows.push_back(argv[i]);
-    if (rows.empty()) rows = {"RI", "GQL", "RI+cache+filter+fs"};
+    if (rows.empty()) rows = {"RI", "QSI", "GQL", "CFL", "CECI", "DAF", "RI+cache+filter+fs"};
     try {
         Graph data;
         data.Load(data_path);

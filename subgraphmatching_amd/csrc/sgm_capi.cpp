@@ -80,6 +80,9 @@ void sgm_ctx_destroy(sgm_ctx* c)
     dfree(c->d_fcd);
     dfree(c->d_fcnt);
     if (c->h_fcnt) (void)hipHostFree(c->h_fcnt);
+    dfree(c->d_fdeg);
+    dfree(c->d_rprog);
+    if (c->h_rprog) (void)hipHostFree(c->h_rprog);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
     if (c->h_stop) (void)hipHostFree(c->h_stop);
     dfree(c->d_lastbits);
@@ -284,6 +287,12 @@ extern "C++" int build_label_tables(sgm_ctx* c, sgm_graph* g, const uint32_t* la
     g->lab_gen++;  // captured filter graphs that read the old tables must be recaptured
     g->labelled = false;
     for (uint32_t v = 0; v < V && !g->labelled; v++) g->labelled = lab[v] != 0;
+    g->lab_hist.clear();
+    g->max_lab_freq = 0;
+    if (g->labelled) {
+        for (uint32_t v = 0; v < V; v++) g->lab_hist[lab[v]]++;
+        for (auto& kv : g->lab_hist) g->max_lab_freq = std::max(g->max_lab_freq, kv.second);
+    }
     return SGM_OK;
 }
 
@@ -304,6 +313,8 @@ int sgm_graph_set_labels(sgm_graph* g, const uint32_t* labels, uint32_t num_vert
         g->has_lab = false;
         g->lab_gen++;
         g->labelled = false;
+        g->lab_hist.clear();
+        g->max_lab_freq = 0;
         return SGM_OK;
     }
     return build_label_tables(c, g, labels);
@@ -458,6 +469,7 @@ int sgm_query_set_labels(sgm_query* q, const uint32_t* labels, uint32_t n)
     dfree(q->d_lreq);
     q->d_lreq = nullptr;
     q->filtered = 0;  // candidates and tables of the old labels are stale
+    q->nlf_ok = false;
     return SGM_OK;
 }
 
@@ -486,7 +498,7 @@ extern "C++" std::vector<uint32_t> lreq_block(const sgm::QueryCSR& Q)
     return blk;
 }
 
-static int ensure_lreq(sgm_ctx* c, sgm_query* q)
+extern "C++" int ensure_lreq(sgm_ctx* c, sgm_query* q)
 {
     if (q->d_lreq) return SGM_OK;
     const std::vector<uint32_t> blk = lreq_block(q->q);
@@ -496,6 +508,50 @@ static int ensure_lreq(sgm_ctx* c, sgm_query* q)
 }
 
 // ------------------------------------------------------------------------------ filter
+// The refinement steps of the BFS-tree filters (CFL / CECI / DAF) for a chunk whose vmask
+// words hold the NLF sets: the NLF counts come back to the host (they pick each query's start
+// vertex), the host writes one refinement program per query, and the steps run as one
+// launch each for the whole chunk (k_refine).
+static int tree_refine(sgm_ctx* c, sgm_graph* g, sgm_query* const* qs, uint32_t nq, int kind, const sgm_qdesc* d_qd)
+{
+    const uint32_t V = g->V;
+    if (!c->d_fcnt || !c->d_fdeg || c->fcap < nq) return set_err(c, SGM_ESTATE, "filter: count buffers missing");
+    sgm::launch_fill_u32(c->stream, c->d_fcnt, (size_t)nq * SGM_MAXQ, 0u);
+    sgm::launch_fill_u32(c->stream, reinterpret_cast<uint32_t*>(c->d_fdeg), (size_t)nq * SGM_MAXQ * 2, 0u);
+    sgm::launch_cand_stats(c->stream, g->d_off, V, d_qd, nq, c->d_fcnt, c->d_fdeg);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(c->h_fcnt, c->d_fcnt, sizeof(uint32_t) * SGM_MAXQ * nq, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->rprog_cap < nq) {
+        dfree(c->d_rprog);
+        if (c->h_rprog) (void)hipHostFree(c->h_rprog);
+        c->d_rprog = nullptr;
+        c->h_rprog = nullptr;
+        c->rprog_cap = 0;
+        HIPCHK(c, dalloc(&c->d_rprog, (size_t)nq * SGM_RPROG_WORDS));
+        HIPCHK(c, hipHostMalloc(reinterpret_cast<void**>(&c->h_rprog), sizeof(uint32_t) * SGM_RPROG_WORDS * nq));
+        c->rprog_cap = nq;
+    }
+    const sgm::LabelStats ls = label_stats(g);
+    uint32_t steps = 0;
+    for (uint32_t i = 0; i < nq; i++) {
+        sgm_query* q = qs[i];
+        std::memcpy(q->nlf_cnt, c->h_fcnt + (size_t)i * SGM_MAXQ, sizeof(uint32_t) * q->q.n);
+        q->nlf_ok = true;
+        q->nlf_version = g->label_version;
+        uint32_t* prog = c->h_rprog + (size_t)i * SGM_RPROG_WORDS;
+        const int k = sgm::refine_program(kind, q->q, q->nlf_cnt, ls, prog);
+        if (k < 0) return set_err(c, SGM_EINVAL, "filter: query %u is not connected", i);
+        steps = std::max(steps, (uint32_t)k);
+    }
+    HIPCHK(c, hipMemcpyAsync(c->d_rprog, c->h_rprog, sizeof(uint32_t) * SGM_RPROG_WORDS * nq, hipMemcpyHostToDevice,
+                             c->stream));
+    if (kind == SGM_FILTER_CECI) sgm::launch_copy_vmask(c->stream, V, d_qd, nq);
+    for (uint32_t k = 0; k < steps; k++) sgm::launch_refine(c->stream, g->d_off, g->d_adj, V, d_qd, nq, c->d_rprog, k);
+    HIPCHK(c, hipGetLastError());
+    return SGM_OK;
+}
+
 // One chunk of a filter batch: NLF (+ GQL) for nq queries in one set of launches, then the
 // candidate bitmaps and counts.  Adds the device time of the three stages to ms[].
 static int filter_chunk(sgm_ctx* c, sgm_graph* g, sgm_query* const* qs, uint32_t nq, int kind, double ms[3])
@@ -534,7 +590,7 @@ static int filter_chunk(sgm_ctx* c, sgm_graph* g, sgm_query* const* qs, uint32_t
             qd[i].lreq = qs[i]->d_lreq;
             any_lab = true;
         }
-        if (kind == SGM_FILTER_GQL) {
+        if (kind != SGM_FILTER_NLF) {  // GQL scratch; the BFS-tree filters use its vmr words
             sgm_gql_layout(qd[i], c->d_gqls + sgm_gql_scratch_words(V) * i, V);
         }
         max_n = std::max(max_n, qs[i]->q.n);
@@ -549,15 +605,18 @@ static int filter_chunk(sgm_ctx* c, sgm_graph* g, sgm_query* const* qs, uint32_t
         dfree(c->d_fqd);
         dfree(c->d_fcd);
         dfree(c->d_fcnt);
+        dfree(c->d_fdeg);
         if (c->h_fcnt) (void)hipHostFree(c->h_fcnt);
         c->d_fqd = nullptr;
         c->d_fcd = nullptr;
         c->d_fcnt = nullptr;
+        c->d_fdeg = nullptr;
         c->h_fcnt = nullptr;
         c->fcap = 0;
         HIPCHK(c, dalloc(&c->d_fqd, nq));
         HIPCHK(c, dalloc(&c->d_fcd, nq));
         HIPCHK(c, dalloc(&c->d_fcnt, (size_t)nq * SGM_MAXQ));
+        HIPCHK(c, dalloc(&c->d_fdeg, (size_t)nq * SGM_MAXQ));
         HIPCHK(c, hipHostMalloc(reinterpret_cast<void**>(&c->h_fcnt), sizeof(uint32_t) * SGM_MAXQ * nq));
         c->fcap = nq;
     }
@@ -587,6 +646,10 @@ static int filter_chunk(sgm_ctx* c, sgm_graph* g, sgm_query* const* qs, uint32_t
             sgm::launch_gql_phase(c->stream, g->d_roff, g->d_radj, V, g->d_gecnt, d_qd, nq, ph);
         sgm::launch_unrank(c->stream, g->d_rank, V, d_qd, nq);
     }
+    if (kind >= SGM_FILTER_CFL && V) {
+        const int rc = tree_refine(c, g, qs, nq, kind, d_qd);
+        if (rc) return rc;
+    }
     HIPCHK(c, hipEventRecord(c->fev[2], c->stream));
     sgm::launch_bitmaps(c->stream, d_cd, nq, V);
     sgm::launch_rank_scan(c->stream, d_cd, nq, V, max_n);
@@ -611,13 +674,13 @@ int sgm_filter_batch(sgm_query* const* qs, uint32_t nq, int kind, int32_t* statu
     if (!qs || nq == 0) return SGM_EINVAL;
     sgm_graph* g = qs[0]->g;
     sgm_ctx* c = g->ctx;
-    if (kind != SGM_FILTER_NLF && kind != SGM_FILTER_GQL) return set_err(c, SGM_EINVAL, "unknown filter kind %d", kind);
+    if (kind < SGM_FILTER_NLF || kind > SGM_FILTER_DAF) return set_err(c, SGM_EINVAL, "unknown filter kind %d", kind);
     for (uint32_t i = 0; i < nq; i++)
         if (!qs[i] || qs[i]->g != g) return set_err(c, SGM_EINVAL, "queries of a batch must share one graph");
     HIPCHK(c, hipSetDevice(c->device));
     const uint64_t per_q = (uint64_t)sgm_gql_scratch_words(g->V) * sizeof(uint32_t);
     uint32_t chunk = nq;
-    if (kind == SGM_FILTER_GQL && per_q) {
+    if (kind != SGM_FILTER_NLF && per_q) {
         chunk = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(nq, kGqlScratchBytes / per_q));
         const size_t words = sgm_gql_scratch_words(g->V) * chunk;
         if (c->gqls_words < words) {

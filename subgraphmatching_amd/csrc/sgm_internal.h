@@ -141,6 +141,12 @@ struct sgm_equery {
 // SGM_CHARGE_TICKS of unpublished time (100 MHz: 82 us).
 #define SGM_CHARGE_TICKS (1ull << 13)
 
+// Refinement program of a BFS-tree filter (CFL / CECI / DAF), one per query: word 0 = step
+// count, then per step (bit | source << 8, pivot mask) -- see k_refine.  At most 3 (n - 1)
+// steps for n <= 32 query vertices.
+#define SGM_RPROG_WORDS 256u
+#define SGM_RPROG_MAXSTEPS ((SGM_RPROG_WORDS - 1u) / 2u)
+
 // Per-query description used by the batched filter kernels.
 //   flags[0 .. 2*SGM_MAXQ)   removals per GQL phase (round*n + u)
 //   flags[SGM_GQL_TOUCH_ANY] bit u: some touch[] word has bit u (round-1 checks of u need it)

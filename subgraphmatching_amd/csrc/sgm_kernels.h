@@ -59,6 +59,16 @@ void launch_table_count(hipStream_t s, const uint32_t* off, const uint32_t* adj,
 // entries of every directed edge of a query in one launch, one edge per block row
 void launch_table_fill(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* vmask,
                        const sgm_tabdesc* ds, uint32_t m, uint32_t max_rows);
+// BFS-tree filters (CFL / CECI / DAF): step `step` of every query's refinement program
+// (SGM_RPROG_WORDS words per query, see k_refine), and the copy of the NLF words into vmr
+void launch_refine(hipStream_t s, const uint32_t* off, const uint32_t* adj, uint32_t V, const sgm_qdesc* qs,
+                   uint32_t nq, const uint32_t* prog, uint32_t step);
+void launch_copy_vmask(hipStream_t s, uint32_t V, const sgm_qdesc* qs, uint32_t nq);
+// order estimates over an aux table: out[j] (= or *=) the row sum of in[idx] (in == null: row
+// lengths), *total += all row sums, all mod 2^64
+void launch_row_gather(hipStream_t s, const uint32_t* toff, const uint32_t* tidx, uint32_t rows,
+                       const unsigned long long* in, unsigned long long* out, int mul, unsigned long long* total);
+void launch_fill_u64(hipStream_t s, unsigned long long* p, size_t n, unsigned long long val);
 // sgm_enum.hip
 size_t enum_smem_bytes(uint32_t max_ops);
 // dev_stop: the device stop word every wave polls; host_stop: the host's request (pinned host

@@ -873,6 +873,107 @@ __device__ void scan_block(const uint32_t* __restrict__ in, uint32_t n, uint32_t
     }
 }
 
+// ================================================================== BFS-tree filters
+// One refinement step of the CFL, CECI and DAF filters (inc/filter.h:317-425 generate /
+// pruneCandidates, :602-788 CECI's frontier invalidation, :942-1020 DAF's passes) for every
+// query of a batch.  Query q's step s is (w, P) = prog[q][1 + 2s], prog[q][2 + 2s]: data
+// vertex v keeps bit `w & 31` of its membership word iff the OR over its neighbours of the
+// source words covers P.  Source vmask (w >> 8 == 0): "v has a neighbour in C(p) for every
+// pivot p in P" -- pruneCandidates, and generateCandidates when the bit still holds NLF(u).
+// Source vmr = the NLF words (w >> 8 == 1): "v has a neighbour in NLF(u) for every u in P"
+// -- CECI's invalidation of a frontier candidate.  A step changes only its own bit, which P
+// never holds, so the words other lanes read are unaffected by its stores.  Lanes scan short
+// lists alone; lists of more than 32 neighbours are scanned by the whole wave, 64 at a time,
+// and every scan stops as soon as P is covered.
+__global__ void __launch_bounds__(256) k_refine(const uint32_t* __restrict__ off, const uint32_t* __restrict__ adj,
+                                                uint32_t V, const sgm_qdesc* __restrict__ qs,
+                                                const uint32_t* __restrict__ prog, uint32_t step)
+{
+    const sgm_qdesc* d = qs + blockIdx.y;
+    const uint32_t* __restrict__ pr = prog + (size_t)blockIdx.y * SGM_RPROG_WORDS;
+    if (d->n == 0 || step >= pr[0]) return;
+    const uint32_t w = pr[1 + 2 * step], P = pr[2 + 2 * step];
+    const uint32_t bit = w & 31u;
+    const uint32_t* __restrict__ src = (w >> 8) ? d->vmr : d->vmask;
+    uint32_t* vm = d->vmask;
+    const uint32_t lane = lane_id();
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t base = wave * 64u; base < V; base += nwaves * 64u) {
+        const uint32_t v = base + lane;
+        const uint32_t m = v < V ? vm[v] : 0u;
+        const bool mem = (m >> bit) & 1u;
+        uint32_t b = 0, e = 0;
+        if (mem) {
+            b = off[v];
+            e = off[v + 1];
+        }
+        bool keep = true;
+        const bool alone = mem && e - b <= 32u;
+        if (alone) {
+            uint32_t acc = 0;
+            for (uint32_t p = b; p < e && (acc & P) != P; p++) acc |= src[adj[p]];
+            keep = (acc & P) == P;
+        }
+        uint64_t wide = __ballot(mem && !alone);
+        while (wide) {
+            const uint32_t l = (uint32_t)__ffsll((long long)wide) - 1u;
+            wide &= wide - 1;
+            const uint32_t lb = rdlane(b, l), le = rdlane(e, l);
+            uint32_t acc = 0;
+            for (uint32_t p = lb; p < le; p += 64u) {
+                const uint32_t x = p + lane < le ? src[adj[p + lane]] : 0u;
+                acc |= wave_or32(x);
+                if ((acc & P) == P) break;
+            }
+            if (lane == l) keep = (acc & P) == P;
+        }
+        if (mem && !keep) vm[v] = m & ~(1u << bit);
+    }
+}
+
+// vmr[v] = vmask[v] for every query of a batch (CECI keeps the NLF words beside its sets).
+__global__ void __launch_bounds__(256) k_copy_vmask(uint32_t V, const sgm_qdesc* __restrict__ qs)
+{
+    const sgm_qdesc* d = qs + blockIdx.y;
+    if (d->n == 0) return;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) d->vmr[v] = d->vmask[v];
+}
+
+// Row sums of one aux table for the CFL and GuP order estimates (inc/order.h:197-239
+// estimatePathEmbeddsingsNum, :386-410 count_embed): s_j = sum over the entries k of row j of
+// in[idx[k]] (in == null: 1 each, the row length), wrapping mod 2^64 as the reference's
+// size_t sums do; out[j] = s_j, or out[j] *= s_j when mul is set; *total += sum of s_j.
+__global__ void __launch_bounds__(256) k_row_gather(const uint32_t* __restrict__ toff, const uint32_t* __restrict__ tidx,
+                                                    uint32_t rows, const unsigned long long* __restrict__ in,
+                                                    unsigned long long* __restrict__ out, int mul,
+                                                    unsigned long long* __restrict__ total)
+{
+    unsigned long long t = 0;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < rows; j += gridDim.x * blockDim.x) {
+        const uint32_t b = toff[j], e = toff[j + 1];
+        unsigned long long s = 0;
+        if (in)
+            for (uint32_t k = b; k < e; k++) s += in[tidx[k]];
+        else
+            s = e - b;
+        out[j] = mul ? out[j] * s : s;
+        t += s;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)t, o);
+        const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(t >> 32), o);
+        t += ((unsigned long long)hi << 32) | lo;
+    }
+    if (lane_id() == 0 && t) atomicAdd(total, t);
+}
+
+__global__ void __launch_bounds__(256) k_fill_u64(unsigned long long* __restrict__ p, size_t n, unsigned long long val)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = val;
+}
+
 }  // namespace sgm
 
 // ================================================================== launch wrappers
@@ -1024,6 +1125,32 @@ void launch_hub_bits(hipStream_t s, const uint32_t* off, const uint32_t* adj, co
                      uint32_t* bits, uint32_t words)
 {
     hipLaunchKernelGGL(k_hub_bits, dim3(clampg(n_hubs, 1, 65535)), dim3(256), 0, s, off, adj, hubs, n_hubs, bits, words);
+}
+
+void launch_refine(hipStream_t s, const uint32_t* off, const uint32_t* adj, uint32_t V, const sgm_qdesc* qs,
+                   uint32_t nq, const uint32_t* prog, uint32_t step)
+{
+    dim3 grid(clampg((V + 255) / 256, 1, 2048), nq);
+    if (nq && V) hipLaunchKernelGGL(k_refine, grid, dim3(256), 0, s, off, adj, V, qs, prog, step);
+}
+
+void launch_copy_vmask(hipStream_t s, uint32_t V, const sgm_qdesc* qs, uint32_t nq)
+{
+    dim3 grid(clampg((V + 255) / 256, 1, 4096), nq);
+    if (nq && V) hipLaunchKernelGGL(k_copy_vmask, grid, dim3(256), 0, s, V, qs);
+}
+
+void launch_row_gather(hipStream_t s, const uint32_t* toff, const uint32_t* tidx, uint32_t rows,
+                       const unsigned long long* in, unsigned long long* out, int mul, unsigned long long* total)
+{
+    if (rows)
+        hipLaunchKernelGGL(k_row_gather, dim3(clampg(((uint64_t)rows + 255) / 256, 1, 1024)), dim3(256), 0, s, toff,
+                           tidx, rows, in, out, mul, total);
+}
+
+void launch_fill_u64(hipStream_t s, unsigned long long* p, size_t n, unsigned long long val)
+{
+    if (n) hipLaunchKernelGGL(k_fill_u64, dim3(clampg((n + 255) / 256, 1, 4096)), dim3(256), 0, s, p, n, val);
 }
 
 }  // namespace sgm

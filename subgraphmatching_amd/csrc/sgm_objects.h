@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/sgm_c.h"
@@ -57,6 +58,12 @@ struct sgm_ctx {
     uint32_t* d_fcnt = nullptr;
     uint32_t* h_fcnt = nullptr;
     uint32_t fcap = 0;
+    // BFS-tree filters: NLF degree sums (the statistics kernel's second output) and the
+    // per-query refinement programs (SGM_RPROG_WORDS words each)
+    unsigned long long* d_fdeg = nullptr;
+    uint32_t* d_rprog = nullptr;
+    uint32_t* h_rprog = nullptr;
+    uint32_t rprog_cap = 0;
     // batched enumeration: bitmap of C(u_L) per query; depth-1 split blocks and chunk records
     uint32_t* d_lastbits = nullptr;
     size_t lastbits_words = 0;
@@ -112,6 +119,10 @@ struct sgm_graph {
     uint32_t* d_run_off = nullptr;
     uint32_t* d_run_lab = nullptr;
     uint32_t* d_run_cnt = nullptr;
+    // label histogram of a labelled graph (Graph::getLabelFreq / getMaxLabelFreq,
+    // inc/graph.h:339-345); empty = every label 0
+    std::unordered_map<uint32_t, uint32_t> lab_hist;
+    uint32_t max_lab_freq = 0;
     std::vector<uint32_t> hub_vertex;                   // bitmap slot -> vertex (degree descending)
     std::vector<uint32_t> hub_degree;
     std::vector<std::pair<uint32_t, uint32_t*>> hub_maps;  // (min degree, slot map) for higher thresholds
@@ -152,6 +163,11 @@ struct sgm_query {
     uint32_t* d_list = nullptr;
     uint32_t* d_lreq = nullptr;       // labelled NLF requirements (sgm_qdesc::lreq), built on demand
     bool labelled = false;            // some query label is not 0 (labels in q.lab)
+    // NLF candidate counts (NLFByVertex, inc/filter.h:24-58) for the start vertices of the
+    // BFS-tree filters and orders, valid for the graph's label_version nlf_version
+    uint32_t nlf_cnt[SGM_MAXQ] = {0};
+    bool nlf_ok = false;
+    uint64_t nlf_version = 0;
     sgm_qdesc* d_qdesc = nullptr;
     sgm_cdesc* d_cdesc = nullptr;
     sgm_qdesc hq;                     // host copies of the descriptors
@@ -247,6 +263,19 @@ inline int grow(sgm_ctx* c, T** p, uint32_t& cap_now, uint32_t need)
 // (sgm_qdesc::lreq) of a query; shared by the filter drivers (sgm_capi.cpp, sgm_batch.cpp).
 int build_label_tables(sgm_ctx* c, sgm_graph* g, const uint32_t* labels);
 std::vector<uint32_t> lreq_block(const sgm::QueryCSR& q);
+int ensure_lreq(sgm_ctx* c, sgm_query* q);
+// The data graph's label statistics for the start-vertex rules.
+inline sgm::LabelStats label_stats(const sgm_graph* g)
+{
+    sgm::LabelStats ls;
+    ls.V = g->V;
+    ls.hist = g->lab_hist.empty() ? nullptr : &g->lab_hist;
+    ls.max_freq = g->lab_hist.empty() ? g->V : g->max_lab_freq;
+    return ls;
+}
+// NLF counts of every query vertex (cached in the query; computed on the device when the
+// graph or the query carries labels, from the degree histogram otherwise).
+int query_nlf_counts(sgm_query* q, uint32_t* out);
 
 // The no-allocation entry point's static scratch array (sgm_noalloc.hip): looks up its
 // address on the context's device, once per device (the device must be current).

@@ -106,3 +106,20 @@ def test_ctypes_structs_match_the_header(tmp_path):
     got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
     assert got == [C.sizeof(L.Stats), C.sizeof(L.EnumOpts), L.Stats.kernel_ms.offset, L.Stats.busy_ns.offset,
                    L.EnumOpts.hub_min_degree.offset]
+
+
+def test_kcore_and_bfs_match_oracle():
+    """The host helpers of the BFS-tree filters and orders (Query::KCore, BFS) equal the
+    oracle's on random-walk queries of every size."""
+    g = chung_lu(3000, 2.5, 7.4, seed=41)
+    qs = query_set(g, 40, lambda i, r: int(r.integers(3, 33)), lambda i, r: bool(i % 2), seed=42)
+    for q in qs:
+        n = q.getVertexCnt()
+        assert sgm.query_kcore(q) == O.kcore(q)
+        qoff, qadj = L.u32(q.off), L.u32(q.adj)
+        for s in (0, n - 1, n // 2):
+            o, par, lev = (np.zeros(n, dtype=np.uint32) for _ in range(3))
+            k = L.lib().sgm_query_bfs(L.ptr(qoff), L.ptr(qadj), n, s, L.ptr(o), L.ptr(par), L.ptr(lev))
+            ro, rp, rl = O.bfs(q, s)
+            assert k == n == len(ro)
+            assert o.tolist() == ro and par.tolist() == rp and lev.tolist() == rl

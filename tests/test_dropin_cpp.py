@@ -105,7 +105,7 @@ def test_driver_rows_match_oracle(tmp_path, labels):
     limit = 100000
     for q, qp in zip(qs, paths):
         print("query", qp, q.getVertexCnt(), flush=True)
-        r = _run(data, qp, limit, ["RI", "GQL", "RI+cache+filter+fs"])
+        r = _run(data, qp, limit, ["RI", "QSI", "GQL", "CFL", "CECI", "DAF", "RI+cache+filter+fs"])
         assert r.returncode == 0, r.stderr
         lines = r.stdout.strip().splitlines()
         assert lines[0] == "data,query_name,algorithm,time_ms,ans,output_limit,intersection_cnt"
@@ -118,6 +118,20 @@ def test_driver_rows_match_oracle(tmp_path, labels):
         ok, c = O.nlf(g, q)
         ref = O.enum_all(g, q, c, O.order_ri(q), cap=limit) if ok else {"count": 0, "intersections": 0}
         _check(got["RI"], ref, limit)
+        # QSI row: NLF + QSI order + plain search
+        if ok:
+            _check(got["QSI"], O.enum_all(g, q, c, O.order_tree("QSI", g, q, c), cap=limit), limit)
+        else:
+            assert got["QSI"][0] == 0
+        # CFL / CECI rows: their filter and order + plain search; DAF row: DAF filter + RI order
+        # + search with failing sets
+        for row in ("CFL", "CECI", "DAF"):
+            okt, ct = O.filter_tree(row, g, q)
+            if not okt:
+                assert got[row][0] == 0, row
+                continue
+            order = O.order_ri(q) if row == "DAF" else O.order_tree(row, g, q, ct)
+            _check(got[row], O.enum_all(g, q, ct, order, cap=limit, fs=row == "DAF"), limit)
         # GQL row: GQL filter + GQL order + plain search
         ok, c = O.gql(g, q)
         if ok:
