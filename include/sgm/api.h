@@ -109,22 +109,99 @@ private:
     sgm_query* _dev = nullptr;
 };
 
-// ReuseParam(query, order) (inc/api.h:170-182).  The reference builds its PlanIR here; the
-// accelerated path builds the same plan inside the enumeration call, so this keeps the
-// order and checks that it is a connected permutation (PlanIR's exit, code_gen.h:743-746).
-class ReuseParam {
+// PlanIR (inc/code_gen.h:682-847) and its ops (VertexSetIR, :573-680): the intersection-cache
+// plan exactly as the library builds and runs it (sgm_plan_export_labeled), with the
+// reference's accessors.  Op ids are positions in getTotSetOpsArr(); INVALID marks no parent
+// and no iteration set (depth 0).
+class VertexSetIR {
 public:
-    Order order;
-    ReuseParam() = default;
-    ReuseParam(const Query& query, const Order& ord) : order(ord)
+    static constexpr uint32_t INVALID = 0xFFFFFFFFu;
+    VertexSetIR() = default;
+    VertexSetIR(uint32_t vid, uint32_t parent, uint32_t depth, uint32_t v_depth, LabelID label,
+                std::vector<VertexID> edge_idx)
+        : _vid(vid), _parent_id(parent), _depth(depth), _v_depth(v_depth), _v_label(label), _edge_idx(std::move(edge_idx))
+    {
+    }
+    const std::vector<VertexID> getEdge(const Order& order) const
+    {
+        std::vector<VertexID> e(_edge_idx.size());
+        for (size_t i = 0; i < _edge_idx.size(); i++) e[i] = order[_edge_idx[i]];
+        return e;
+    }
+    const std::vector<VertexID>& getEdgeIdx() const { return _edge_idx; }
+    uint32_t getVSetID() const { return _vid; }
+    VertexID getDepth() const { return _depth; }
+    uint32_t getParentVSetID() const { return _parent_id; }
+    uint32_t getVDepth() const { return _v_depth; }
+    LabelID getLabel() const { return _v_label; }
+    bool hasEdge(VertexID v) const { return std::find(_edge_idx.begin(), _edge_idx.end(), v) != _edge_idx.end(); }
+
+private:
+    uint32_t _vid{INVALID}, _parent_id{INVALID}, _depth{INVALID}, _v_depth{INVALID};
+    LabelID _v_label{INVALID};
+    std::vector<VertexID> _edge_idx;
+};
+
+class PlanIR {
+public:
+    PlanIR() = default;
+    // Throws sgm::Error(SGM_EORDER) where the reference exits (a disconnected order,
+    // inc/code_gen.h:743-746).
+    PlanIR(const Order& order, const Query& query) : _order(order)
     {
         const uint32_t n = query.getVertexCnt();
-        if (ord.size() != n) throw sgm::Error(SGM_EORDER, "order length differs from the query size");
+        if (order.size() != n) throw sgm::Error(SGM_EORDER, "order length differs from the query size");
         std::vector<uint32_t> depth(512), parent(512), vdepth(512), emask(512), iter(32);
         uint32_t n_ops = 0;
-        sgm::check(sgm_plan_export(query.offsets(), query.neighbours(), n, ord.data(), &n_ops, depth.data(),
-                                   parent.data(), vdepth.data(), emask.data(), iter.data()),
+        sgm::check(sgm_plan_export_labeled(query.offsets(), query.neighbours(),
+                                           query.unlabeled() ? nullptr : query.getVertexLabels(), n, order.data(),
+                                           &n_ops, depth.data(), parent.data(), vdepth.data(), emask.data(),
+                                           iter.data()),
                    "plan");
+        _setOps.assign(n, {});
+        for (uint32_t i = 0; i < n_ops; i++) {
+            std::vector<VertexID> e;
+            for (uint32_t b = 0; b < 32; b++)
+                if ((emask[i] >> b) & 1u) e.push_back(b);
+            const LabelID lab = query.unlabeled() ? 0 : query.getVertexLabels()[order[vdepth[i]]];
+            _ops.emplace_back(i, parent[i], depth[i], vdepth[i], lab, std::move(e));
+            _setOps.at(depth[i]).push_back(i);
+        }
+        _iterSet.assign(iter.begin(), iter.begin() + n);
+    }
+    const std::vector<VertexID>& getOrder() const { return _order; }
+    const std::vector<std::vector<uint32_t>>& getSetOps() const { return _setOps; }
+    const std::vector<uint32_t>& getIterVSet() const { return _iterSet; }
+    uint32_t getIterVSetAt(uint32_t depth) const { return _iterSet[depth]; }
+    const std::vector<VertexSetIR>& getTotSetOpsArr() const { return _ops; }
+    uint32_t getTotOp() const { return (uint32_t)_ops.size(); }
+    const std::vector<VertexSetIR> getSetOpsArray() const { return _ops; }
+    bool isLastOp(uint32_t cur_dep) const { return cur_dep == _iterSet.size() - 2; }
+
+private:
+    std::vector<VertexID> _order;
+    std::vector<VertexSetIR> _ops;
+    std::vector<std::vector<uint32_t>> _setOps;
+    std::vector<uint32_t> _iterSet;
+};
+
+// ReuseParam (inc/api.h:167-182): owns its plan (and restrictions) as the reference does.
+// The accelerated search builds the same plan on the device side of the enumeration call;
+// this one is for callers that read reuse->plan.
+class ReuseParam {
+public:
+    Restriction* rests{nullptr};
+    PlanIR* plan{nullptr};
+    Order order;
+    ReuseParam() = default;
+    ReuseParam(Restriction* r, PlanIR* p) : rests(r), plan(p) {}
+    ReuseParam(const Query& query, const Order& ord) : order(ord) { plan = new PlanIR(ord, query); }
+    ReuseParam(const ReuseParam&) = delete;
+    ReuseParam& operator=(const ReuseParam&) = delete;
+    ~ReuseParam()
+    {
+        delete plan;
+        delete rests;
     }
 };
 

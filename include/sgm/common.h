@@ -14,6 +14,7 @@
 #include <stdexcept>
 #include <string>
 #include <thread>
+#include <utility>
 #include <vector>
 
 #include "../sgm_c.h"
@@ -21,6 +22,8 @@
 typedef uint32_t VertexID;
 typedef uint32_t LabelID;
 typedef std::vector<VertexID> Order;
+typedef std::pair<VertexID, VertexID> PartialOrder;  // inc/common_type.h:41-42
+typedef std::vector<PartialOrder> Restriction;
 
 namespace sgm {
 

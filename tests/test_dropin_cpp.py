@@ -39,7 +39,21 @@ size_t row(const Graph& data, const Query& query, std::atomic<bool>& stop_flag) 
     uint32_t cnt;
     const uint32_t* nb = can.edge_matrix[0][1]->getNeb(0, cnt);
     (void)nb;
-    return ans + Profiler::getInst().total_intersection + can.candidates_count[0] + can.candidates_exsist[0][0];
+    size_t ops = reuse.plan->getTotOp() + reuse.plan->getIterVSetAt(1) + reuse.plan->getSetOps().size();
+    // the other filters and orders of the harness rows (src/overall_performance.cpp:66-190)
+    CandidateParam c2(data, query);
+    if (Filter_CFL(data, query, c2) || Filter_CECI(data, query, c2) || Filter_DAF(data, query, c2)) {
+        Order oq, oc, og, oe;
+        Order_QSI(data, query, c2, oq);
+        Order_CFL(data, query, c2, oc);
+        Order_GuP(data, query, c2, og);
+        Order_CECI(data, query, oe);
+        Config c3;
+        c3.useFilter = true;
+        c3.can = &c2;
+        explore_Intersection_All<true, false, false, false, false>(data, query, oq, c3, time, ans, stop_flag, 100000);
+    }
+    return ans + ops + Profiler::getInst().total_intersection + can.candidates_count[0] + can.candidates_exsist[0][0];
 }
 '''
 
@@ -50,6 +64,58 @@ def test_headers_compile_with_reference_call_pattern(tmp_path):
     r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
                         str(src)], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+PLAN_PRINT = r'''
+#include <cstdio>
+#include "sgm/api.h"
+int main(int argc, char** argv) {
+    Query q;
+    q.Load(argv[1]);
+    Order order;
+    for (int i = 2; i < argc; i++) order.push_back((VertexID)std::atoi(argv[i]));
+    ReuseParam reuse(q, order);
+    const PlanIR& p = *reuse.plan;
+    for (const auto& op : p.getTotSetOpsArr()) {
+        std::printf("op %u %u %u %d", op.getVSetID(), op.getDepth(), op.getVDepth(),
+                    op.getParentVSetID() == VertexSetIR::INVALID ? -1 : (int)op.getParentVSetID());
+        for (auto e : op.getEdgeIdx()) std::printf(" %u", e);
+        std::printf("\n");
+    }
+    for (uint32_t d = 0; d < order.size(); d++)
+        std::printf("it %d\n", p.getIterVSetAt(d) == VertexSetIR::INVALID ? -1 : (int)p.getIterVSetAt(d));
+    return 0;
+}
+'''
+
+
+def test_reuse_param_plan_matches_oracle(tmp_path):
+    """ReuseParam owns a PlanIR with the reference's accessors (inc/api.h:167-182,
+    inc/code_gen.h:682-847); its ops and iteration sets equal the oracle's plan.  Host only:
+    the plan is built without a device."""
+    src = tmp_path / "plan.cpp"
+    src.write_text(PLAN_PRINT)
+    exe = tmp_path / "plan"
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                        str(exe), "-L", PKG, "-lsgm_hip", f"-Wl,-rpath,{PKG}", "-pthread"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    g = chung_lu(2000, 2.5, 7.4, seed=81)
+    for i, q in enumerate(query_set(g, 8, lambda i, r: int(r.integers(4, 17)), lambda i, r: bool(i % 2), seed=82)):
+        qp = tmp_path / f"q{i}.txt"
+        save_text(q, str(qp))
+        order = O.order_ri(q)
+        out = subprocess.run([str(exe), str(qp), *map(str, order)], capture_output=True, text=True, timeout=60)
+        assert out.returncode == 0, out.stderr
+        ref = O.plan(q, order)
+        ops, its = [], []
+        for ln in out.stdout.splitlines():
+            f = ln.split()
+            if f[0] == "op":
+                ops.append({"depth": int(f[2]), "parent": None if f[4] == "-1" else int(f[4]), "vdepth": int(f[3]),
+                            "edges": [int(x) for x in f[5:]]})
+            else:
+                its.append(None if f[1] == "-1" else int(f[1]))
+        assert ops == ref["ops"] and its == ref["iter_set"], q.name
 
 
 def test_pruning_variants_are_rejected_at_compile_time(tmp_path):
