@@ -18,11 +18,14 @@ Workloads (one step = one pass of the path over the whole query set):
                 query (its embeddings pass 1.5e13, so the step is capped like the CPU side;
                 an uncapped pass under a work budget reports the count it reached)
   rmat20-learned
-    the rmat20 graph and query mix with learned orders: a cost model is trained (untimed) on
-    node counts the enumerator logs for 5,000 training queries (seed 44) x 10 orders, then
-    each step filters, ranks RI + GQL + 32 sampled orders per query on the GPU, keeps RI
-    unless the model predicts a clear gain, and enumerates; the same steps with RI orders
-    are timed after it for the learned-vs-RI speedup
+    the rmat20 graph and query mix with learned orders.  Two cost models are trained (untimed)
+    on 5,000 training queries (seed 44) x 10 orders logged by the enumerator: the ranking model
+    on the wave time each order costs the search (--learn-label busy) and the node-count model
+    on the search-tree node counts of the plain search (failing sets off).  Each step filters,
+    ranks RI + GQL + 32 sampled orders per query on the GPU with the ranking model, keeps RI
+    unless it predicts a clear gain, and enumerates; the same pass with RI orders and one with
+    the node-count model's orders are timed after it (learned-vs-RI speedups, and predicted vs
+    measured node counts of the node-count model's orders)
   rmat22-serve
     data graph  R-MAT scale 22, average degree 32 (2^26 undirected edges), seed 5
     queries     1,200 random-walk queries, sizes uniform in 8..32, mixed sparse / dense,
@@ -48,6 +51,9 @@ GPU run and the CPU baseline).  The CPU baseline is the project's C restatement 
 reference path (oracle/, the checker; AVX2 set intersections as in the reference's build),
 run single-threaded per query on this host over a bounded sample of the same queries; the
 GPU counts and candidate-set digests of the sampled queries are compared with it.
+
+The default run also reports the rmat20-learned comparison (one timed pass each) under
+"learned_vs_ri" (--no-learned skips it).
 
 Config 2 as written has no cap; its queries hold more than 10^9 embeddings each (no query
 finishes on either side within any budget this run can afford), so the default workload
@@ -86,8 +92,13 @@ def parse():
                     help="rmat22-serve: arrival rate as a fraction of the measured capacity")
     ap.add_argument("--train-queries", type=int, default=5000,
                     help="rmat20-learned: training queries (10 logged orders each: RI, GQL, 8 sampled)")
-    ap.add_argument("--learn-label", choices=["busy", "nodes"], default="busy",
-                    help="cost-model target: wave time per query or internal search-tree nodes")
+    ap.add_argument("--learn-label", choices=["busy", "nodes", "plain-nodes"], default="busy",
+                    help="cost-model target: wave time per query, internal search-tree nodes of the cache + filter + "
+                         "failing-set search, or of the same search without failing sets (plain)")
+    ap.add_argument("--no-learned", dest="learned", action="store_false",
+                    help="chunglu100k: skip the learned-vs-RI section (R-MAT 2^20 mix, about a minute)")
+    ap.add_argument("--rank-margin", type=float, default=0.5,
+                    help="rmat20-learned: keep RI unless the model predicts a gain of more than this (log2)")
     ap.add_argument("--rank-samples", type=int, default=32,
                     help="rmat20-learned: sampled orders ranked per query besides RI and GQL")
     ap.add_argument("--queries", type=int, default=0, help="number of queries (0 = the workload's own)")
@@ -354,12 +365,12 @@ def setup_rmat(args):
 def setup_rmat_learned(args):
     import numpy as np
     import subgraphmatching_amd as sgm
+    from subgraphmatching_amd import _lib as L
     from subgraphmatching_amd.generators import query_set, rmat
     from subgraphmatching_amd.learned import LABELS, collect_training, train_model
 
     g = rmat(20, 20_000_000, seed=3)
     nq = args.queries or 1000
-    label_of = LABELS[args.learn_label]
     mix = lambda count, seed: query_set(g, count, lambda i, r: int(r.integers(4, 33)),  # noqa: E731
                                         lambda i, r: i % 2 == 0, seed=seed)
     queries = mix(nq, 33)
@@ -370,31 +381,45 @@ def setup_rmat_learned(args):
     chunks = [[queries[i] for i in by_size[lo:lo + B]] for lo in range(0, nq, B)]
     runner.filter(chunks[0])
     tuned, _ = runner.autotune(cap=args.cap)
-    t0 = time.perf_counter()
-    items, labels, info = collect_training(runner, train_q, n_samples=8, cap=args.cap, seed=1, time_limit_s=5.0,
-                                           label=args.learn_label)
-    collect_s = time.perf_counter() - t0
-    model, losses = train_model(items, labels, seed=7, epochs=30, minibatch=256, lr=2e-3)
-    tm = model.timing()
-    print(f"[bench] cost model: {len(items)} triples ({info['cut_off']} cut off) logged in {collect_s:.1f}s, "
-          f"loss {losses[0]:.3f} -> {losses[-1]:.3f}, training {tm['train_ms']:.0f} ms", flush=True)
-    stats = {"learned_used": 0, "pred": [], "meas": [], "nodes": 0, "busy_ns": 0, "rank_s": 0.0}
+    log = lambda m: print(f"[bench] {m}", flush=True)  # noqa: E731
 
-    def run(learned):
+    def fit(label):
+        t0 = time.perf_counter()
+        items, labels, info = collect_training(runner, train_q, n_samples=8, cap=args.cap, seed=1,
+                                               time_limit_s=5.0, label=label, log=log)
+        collect_s = time.perf_counter() - t0
+        model, losses = train_model(items, labels, seed=7, epochs=30, minibatch=256, lr=2e-3)
+        tm = model.timing()
+        log(f"cost model ({label}): {len(items)} triples ({info['cut_off']} cut off) logged in {collect_s:.1f}s, "
+            f"loss {losses[0]:.3f} -> {losses[-1]:.3f}, training {tm['train_ms']:.0f} ms")
+        return {"model": model, "label": label, "items": len(items), "info": info, "losses": losses, "timing": tm,
+                "collect_s": collect_s}
+
+    # the ranking model (label --learn-label), and the search-tree node-count model of the
+    # plain search (failing sets off), whose predictions are checked against measured node
+    # counts and whose ranking is timed beside it
+    rank = fit(args.learn_label)
+    nodes = fit("plain-nodes") if args.learn_label != "plain-nodes" else rank
+    stats = {}
+
+    def run(fitted):
+        """One pass over the 1,000 queries: filter, order (RI when fitted is None), enumerate."""
         rec = new_rec()
         t_start = time.perf_counter()
-        st = {"learned_used": 0, "pred": [], "meas": [], "nodes": 0, "busy_ns": 0, "rank_s": 0.0}
+        st = {"learned_used": 0, "pred": [], "meas": [], "nodes": 0, "busy_ns": 0, "rank_s": 0.0, "orders": []}
         for part in chunks:
             ok, _, _ = runner.filter(part)
             f_ms, _ = runner.last_ms()
             rec["filter_ms"] += f_ms
             rec["gql_ms"] += f_ms
             orders = None
-            if learned:
+            if fitted is not None:
                 t1 = time.perf_counter()
-                orders, pb, _, fl = runner.rank_orders(model, n_samples=args.rank_samples, seed=5, margin=0.5)
+                orders, pb, _, fl = runner.rank_orders(fitted["model"], n_samples=args.rank_samples, seed=5,
+                                                       margin=args.rank_margin)
                 st["rank_s"] += time.perf_counter() - t1
                 st["learned_used"] += int(fl.sum())
+                st["orders"].append((orders, ok, [float(x) for x in pb]))
             res = runner.enumerate(orders=orders, cap=args.cap, **tuned)
             rec["enum_ms"] += runner.last_ms()[1]
             rec["enum_launches"] += 1
@@ -403,48 +428,115 @@ def setup_rmat_learned(args):
                 if good:
                     st["nodes"] += sum(r.nodes[:-1])
                     st["busy_ns"] += r.busy_ns
-                    if learned:
+                    if fitted is not None and fitted["label"] != "plain-nodes":
                         st["pred"].append(float(pb[i]))
-                        st["meas"].append(float(label_of(r)))
+                        st["meas"].append(float(LABELS[fitted["label"]](r)))
         rec["wall_s"] = time.perf_counter() - t_start
         rec["filter_wall_s"] = rec["filter_ms"] / 1e3
         return rec, st
 
     def step():
-        rec, st = run(True)
-        stats.update(st)
+        rec, st = run(rank)
+        stats["rank"] = st
         return rec
 
+    def measured_plain(st):
+        """Predicted vs measured plain node counts of the orders a pass chose: the same batches
+        without failing sets (untimed; launches past 5 s leave their unfinished queries out)."""
+        pred, meas, cut = [], [], 0
+        for part, (orders, ok, pb) in zip(chunks, st["orders"]):
+            runner.filter(part)
+            res = runner.enumerate(orders=orders, method=L.SGM_ENUM_CACHE_FILTER, cap=args.cap, time_limit_s=5.0)
+            for i, (good, r) in enumerate(zip(ok, res)):
+                if good and not r.stopped:
+                    pred.append(pb[i])
+                    meas.append(LABELS["plain-nodes"](r))
+                cut += bool(good and r.stopped)
+        return pred, meas, cut
+
+    def pvm(pred, meas, label, cut=0):
+        pred, meas = np.array(pred, dtype=float), np.array(meas, dtype=float)
+        err = np.abs(pred - meas)
+        return {"label": label, "queries": int(len(pred)), "cut_off": cut,
+                "abs_error_log2": {"mean": float(err.mean()), "median": float(np.median(err))} if len(err) else None,
+                "within_1_log2": float((err <= 1.0).mean()) if len(err) else None,
+                "pearson_log2": float(np.corrcoef(pred, meas)[0, 1]) if len(pred) > 2 else None,
+                "mean_predicted_log2": float(pred.mean()) if len(pred) else None,
+                "mean_measured_log2": float(meas.mean()) if len(meas) else None}
+
+    def cm(f):
+        tm, ms = f["timing"], f["timing"]["train_ms"]
+        t2 = f["model"].timing()
+        return {"label": f["label"], "training_triples": f["items"], "cut_off": f["info"]["cut_off"], "epochs": 30,
+                "logged_s": f["collect_s"], "loss_first_epoch": float(f["losses"][0]),
+                "loss_last_epoch": float(f["losses"][-1]), "train_ms": ms,
+                "train_tflops": tm["train_flops"] / ms / 1e9 if ms else 0.0,
+                "train_mfma_util_vs_bf16_dense_2.5pf": tm["train_flops"] / ms / 1e9 / 2500.0 if ms else 0.0,
+                "infer_ms": t2["infer_ms"], "infer_flops": t2["infer_flops"], "layers": 3, "hidden": 64,
+                "dtype": "bf16 operands, fp32 accumulation"}
+
     def compare(K, learned_counts, learned_s):
-        ri = [run(False) for _ in range(K)]
+        ri = [run(None) for _ in range(K)]
         ri_s = sum(r["wall_s"] for r, _ in ri)
-        err = np.abs(np.array(stats["pred"]) - np.array(stats["meas"]))
-        flops, ms = tm["train_flops"], tm["train_ms"]
-        t2 = model.timing()
-        return {"ri_queries_per_s": nq * K / ri_s, "learned_queries_per_s": nq * K / learned_s,
-                "learned_vs_ri_speedup": ri_s / learned_s,
-                "counts_identical": all(r["counts"] == learned_counts for r, _ in ri),
-                "internal_nodes_per_step": {"ri": ri[0][1]["nodes"], "learned": stats["nodes"]},
-                "wave_busy_ms_per_step": {"ri": ri[0][1]["busy_ns"] / 1e6, "learned": stats["busy_ns"] / 1e6},
-                "label": args.learn_label,
-                "learned_orders_used_per_step": stats["learned_used"],
-                "ranking_host_s_per_step": stats["rank_s"],
-                "prediction_abs_error_log2": {"mean": float(err.mean()), "median": float(np.median(err))},
-                "costmodel": {"training_triples": len(items), "cut_off": info["cut_off"], "epochs": 30,
-                              "loss_first_epoch": float(losses[0]), "loss_last_epoch": float(losses[-1]),
-                              "train_ms": ms, "train_tflops": flops / ms / 1e9 if ms else 0.0,
-                              "train_mfma_util_vs_bf16_dense_2.5pf": flops / ms / 1e9 / 2500.0 if ms else 0.0,
-                              "infer_ms": t2["infer_ms"], "infer_flops": t2["infer_flops"],
-                              "layers": 3, "hidden": 64, "dtype": "bf16 operands, fp32 accumulation"}}
+        st = stats["rank"]
+        if rank["label"] == "plain-nodes":
+            p, m, cut = measured_plain(st)
+            rank_pvm = pvm(p, m, "plain-nodes", cut)
+        else:
+            rank_pvm = pvm(st["pred"], st["meas"], rank["label"])
+        out = {"ri_queries_per_s": nq * K / ri_s, "learned_queries_per_s": nq * K / learned_s,
+               "learned_vs_ri_speedup": ri_s / learned_s,
+               "counts_identical": all(r["counts"] == learned_counts for r, _ in ri),
+               "internal_nodes_per_step": {"ri": ri[0][1]["nodes"], "learned": st["nodes"]},
+               "wave_busy_ms_per_step": {"ri": ri[0][1]["busy_ns"] / 1e6, "learned": st["busy_ns"] / 1e6},
+               "label": rank["label"],
+               "learned_orders_used_per_step": st["learned_used"],
+               "ranking_host_s_per_step": st["rank_s"],
+               "predicted_vs_measured": rank_pvm,
+               "costmodel": cm(rank)}
+        if nodes is not rank:
+            # the node-count model: its predictions against the plain node counts of the
+            # orders it picks, and the same 1,000 queries timed with those orders
+            nrec, nst = run(nodes)
+            p, m, cut = measured_plain(nst)
+            out["node_count_model"] = {
+                "predicted_vs_measured_nodes": pvm(p, m, "plain-nodes", cut),
+                "queries_per_s_ranked_by_it": nq / nrec["wall_s"],
+                "speedup_vs_ri_ranked_by_it": (ri_s / K) / nrec["wall_s"],
+                "counts_identical": nrec["counts"] == learned_counts,
+                "internal_nodes_per_step": nst["nodes"], "wave_busy_ms_per_step": nst["busy_ns"] / 1e6,
+                "learned_orders_used": nst["learned_used"], "costmodel": cm(nodes)}
+        return out
 
     config = {"workload": "rmat_s20_e2e7_abcd.57.19.19.05_seed3__1000x4-32v_mixed_rw_seed33__gql_learned_order_cache_filter_fs",
-              "queries_per_step": nq, "cap_per_query": args.cap, "order": "learned (RI fallback, margin 0.5 log2)",
+              "queries_per_step": nq, "cap_per_query": args.cap, "order": f"learned (RI fallback, margin {args.rank_margin} log2)",
               "filter": "NLF+GQL", "enumeration": "cache+filter+failing-sets",
               "launch": f"hipGraph-replayed batches of {B}", "autotuned": tuned,
               "training": f"{args.train_queries} queries (seed 44) x 10 orders, labels "
-                          + ("log2(1 + wave microseconds)" if args.learn_label == "busy" else "log2(1 + internal nodes)"),
+                          + {"busy": "log2(1 + wave microseconds)",
+                             "nodes": "log2(1 + internal search-tree nodes, failing sets on)",
+                             "plain-nodes": "log2(1 + internal search-tree nodes, failing sets off)"}[args.learn_label]
+                          + ("; a second model on log2(1 + internal nodes, failing sets off)"
+                             if args.learn_label != "plain-nodes" else ""),
               "ranked_orders_per_query": f"RI + GQL + {args.rank_samples} sampled"}
     return g, queries, step, config, {"compare": compare}
+
+
+def learned_section(args):
+    """The learned-vs-RI comparison of the R-MAT 2^20 1,000-query mix (cost models trained on
+    5,000 training queries x 10 orders), run after the headline workload: one untimed and one
+    timed pass with learned orders, one with RI orders, and the node-count model's pass."""
+    import copy
+    a = copy.copy(args)
+    a.queries = 0
+    g, queries, step, config, extra = setup_rmat_learned(a)
+    step()
+    t0 = time.perf_counter()
+    rec = step()
+    out = extra["compare"](1, rec["counts"], time.perf_counter() - t0)
+    out["workload"] = config["workload"]
+    out["training"] = config["training"]
+    return out
 
 
 # ------------------------------------------------------------------------- rmat22-serve
@@ -594,6 +686,9 @@ def main():
     if "uncapped" in extra and args.uncapped_budget_ms > 0:
         extras["uncapped"] = extra["uncapped"](args.uncapped_budget_ms)
         print(f"[bench] uncapped pass: {extras['uncapped']}", flush=True)
+    if args.workload == "chunglu100k" and args.learned:
+        extras["learned_vs_ri"] = learned_section(args)
+        print(f"[bench] learned_vs_ri: {extras['learned_vs_ri']}", flush=True)
 
     cpu = None
     gpu_digests = extra["digests"]() if "digests" in extra and args.cpu_seconds > 0 else None

@@ -83,6 +83,8 @@ void sgm_ctx_destroy(sgm_ctx* c)
     dfree(c->d_fdeg);
     dfree(c->d_rprog);
     if (c->h_rprog) (void)hipHostFree(c->h_rprog);
+    dfree(c->d_tb);
+    dfree(c->d_ttot);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
     if (c->h_stop) (void)hipHostFree(c->h_stop);
     dfree(c->d_lastbits);
@@ -780,134 +782,169 @@ int sgm_cand_exist_download(sgm_query* q, uint32_t* words, uint32_t num_words)
 }
 
 // ------------------------------------------------------------------------------ tables
-int sgm_build_tables(sgm_query* q)
+// buildTables (inc/table.h:7-137) for every directed edge of many queries at once: per query
+// its candidate lists (one compaction launch each), then for the whole set one row-count
+// launch (grouped by source vertex), one scan launch (one block per directed edge), one
+// read-back of the totals, and one fill launch -- two host synchronisations in all.  Queries
+// whose candidate sets are empty are skipped (no search reads their tables).
+static int build_tables_many(sgm_query* const* qs, uint32_t nq)
 {
-    if (!q) return SGM_EINVAL;
-    sgm_ctx* c = q->g->ctx;
-    int rc = check_filtered(q, "tables");
-    if (rc) return rc;
-    rc = ensure_compacted(q);
-    if (rc) return rc;
-    const uint32_t n = q->q.n, W = (q->g->V + 63) / 64;
-    free_tables(q);
-    // every directed query edge (a, b): its offsets slice and its scratch maximum
-    std::vector<std::pair<uint32_t, uint32_t>> edges;
-    size_t off_words = 0;
-    for (uint32_t a = 0; a < n; a++) {
-        for (uint32_t p = q->q.off[a]; p < q->q.off[a + 1]; p++) {
-            edges.emplace_back(a, q->q.adj[p]);
-            off_words += (size_t)q->counts[a] + 1;
+    if (!nq) return SGM_OK;
+    sgm_graph* g = qs[0]->g;
+    sgm_ctx* c = g->ctx;
+    const uint32_t W = (g->V + 63) / 64;
+    HIPCHK(c, hipSetDevice(c->device));
+    struct Edge {
+        uint32_t q, a, b;
+        size_t o, r;  // offsets slice in the query's toff arena; row counts at tscr + r
+    };
+    std::vector<Edge> edges;
+    std::vector<uint32_t> first(nq + 1, 0);
+    uint32_t NS = 0, max_rows = 0;
+    for (uint32_t i = 0; i < nq; i++) {
+        sgm_query* q = qs[i];
+        first[i] = (uint32_t)edges.size();
+        if (q->g != g) return set_err(c, SGM_EINVAL, "tables: queries must share one graph");
+        int rc = check_filtered(q, "tables");
+        if (rc) return rc;
+        free_tables(q);
+        if (q->empty) continue;
+        if (!q->compacted) {  // candidate lists (ensure_compacted without its synchronisation)
+            const uint32_t n = q->q.n;
+            q->cand_off[0] = 0;
+            for (uint32_t u = 0; u < n; u++) q->cand_off[u + 1] = q->cand_off[u] + q->counts[u];
+            dfree(q->d_list);
+            q->d_list = nullptr;
+            HIPCHK(c, dalloc(&q->d_list, q->cand_off[n]));
+            q->hc.list = q->d_list;
+            HIPCHK(c, hipMemcpyAsync(q->d_cdesc, &q->hc, sizeof q->hc, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(c, hipMemcpyAsync(q->d_candoff, q->cand_off, sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice,
+                                     c->stream));
+            sgm::launch_compact(c->stream, q->d_cdesc, 1, g->V);
+            q->compacted = true;
         }
+        size_t o = 0, r = 0;
+        for (uint32_t a = 0; a < q->q.n; a++) {
+            if (q->q.off[a + 1] > q->q.off[a]) NS++;
+            for (uint32_t p = q->q.off[a]; p < q->q.off[a + 1]; p++) {
+                edges.push_back(Edge{i, a, q->q.adj[p], o, r});
+                o += (size_t)q->counts[a] + 1;
+                r += q->counts[a];
+                max_rows = std::max(max_rows, q->counts[a]);
+            }
+        }
+        HIPCHK(c, grow_arena(&q->d_toff, q->toff_cap, o));
+        HIPCHK(c, grow_arena(&q->d_tscr, q->tscr_cap, std::max<size_t>(r, 1)));
     }
-    const uint32_t m = (uint32_t)edges.size();
-    if (m == 0) return SGM_OK;
-    // scratch: row counts of every edge (edge k's at obase_k - k), totals[m], maxima[m],
-    // scan descriptors (obase, rows)[m]
-    const size_t Rw = off_words - m;
-    if (Rw + 4 * (size_t)m > 0xFFFFFFFFull) return set_err(c, SGM_EINVAL, "tables too large");
-    HIPCHK(c, grow_arena(&q->d_toff, q->toff_cap, off_words));
-    HIPCHK(c, grow_arena(&q->d_tscr, q->tscr_cap, Rw + 4 * (size_t)m));
-    if (q->tdesc_cap < n) {
-        dfree(q->d_tdesc);
-        q->d_tdesc = nullptr;
-        q->tdesc_cap = 0;
-        HIPCHK(c, dalloc(&q->d_tdesc, n));
-        q->tdesc_cap = n;
+    first[nq] = (uint32_t)edges.size();
+    const uint32_t M = (uint32_t)edges.size();
+    if (M == 0) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return SGM_OK;
     }
-    if (q->tedesc_cap < m) {
-        dfree(q->d_tedesc);
-        q->d_tedesc = nullptr;
-        q->tedesc_cap = 0;
-        HIPCHK(c, dalloc(&q->d_tedesc, m));
-        q->tedesc_cap = m;
+    // descriptors: [sources NS][scans M][fills M] on the device; the same plus the totals and
+    // maxima read back (2M words) in pinned staging
+    const size_t src_b = stage_align(sizeof(sgm_tabsrc) * NS), scan_b = stage_align(sizeof(sgm_scandesc) * M);
+    const size_t fill_b = stage_align(sizeof(sgm_tabdesc) * M), dev_b = src_b + scan_b + fill_b;
+    if (c->tb_bytes < dev_b) {
+        dfree(c->d_tb);
+        c->d_tb = nullptr;
+        c->tb_bytes = 0;
+        HIPCHK(c, dalloc(&c->d_tb, dev_b));
+        c->tb_bytes = dev_b;
     }
-    // pinned: scan descriptors and results (4m words), one count descriptor per source
-    // vertex, one fill descriptor per edge
-    const size_t hs_bytes = stage_align(sizeof(uint32_t) * 4 * m);
-    const size_t td_bytes = stage_align(sizeof(sgm_tabsrc) * n);
-    uint8_t* stage = static_cast<uint8_t*>(host_stage(c, hs_bytes + td_bytes + sizeof(sgm_tabdesc) * m));
-    std::vector<uint32_t> h_v;
-    std::vector<sgm_tabsrc> td_v;
-    std::vector<sgm_tabdesc> te_v;
-    uint32_t* hs;
-    sgm_tabsrc* td;
-    sgm_tabdesc* te;
-    if (stage) {
-        hs = reinterpret_cast<uint32_t*>(stage);
-        td = reinterpret_cast<sgm_tabsrc*>(stage + hs_bytes);
-        te = reinterpret_cast<sgm_tabdesc*>(stage + hs_bytes + td_bytes);
-    } else {
-        h_v.resize(4 * (size_t)m);
-        td_v.resize(n);
-        te_v.resize(m);
-        hs = h_v.data();
-        td = td_v.data();
-        te = te_v.data();
+    if (c->ttot_cap < 2 * M) {
+        dfree(c->d_ttot);
+        c->d_ttot = nullptr;
+        c->ttot_cap = 0;
+        HIPCHK(c, dalloc(&c->d_ttot, 2 * (size_t)M));
+        c->ttot_cap = 2 * M;
     }
-    uint32_t* h = hs + 2 * m;  // totals[m], maxima[m]
-    // pass 0: row counts of every table (one launch), their offsets (one launch), one
-    // synchronisation
-    // (edges out of one vertex are consecutive: one descriptor per run of them)
-    size_t o = 0;
-    uint32_t max_rows = 0, ns = 0;
-    for (uint32_t k = 0; k < m; k++) {
-        const uint32_t a = edges[k].first, b = edges[k].second, rows = q->counts[a];
-        sgm_table& t = q->tables[(size_t)a * n + b];
-        t.d_off = q->d_toff + o;
-        hs[2 * k] = (uint32_t)o;
-        hs[2 * k + 1] = rows;
-        if (k == 0 || edges[k - 1].first != a) {
-            sgm_tabsrc& S = td[ns];
+    std::vector<uint8_t> fallback;
+    uint8_t* st = static_cast<uint8_t*>(host_stage(c, dev_b + sizeof(uint32_t) * 2 * M));
+    if (!st) {
+        fallback.resize(dev_b + sizeof(uint32_t) * 2 * M);
+        st = fallback.data();
+    }
+    sgm_tabsrc* td = reinterpret_cast<sgm_tabsrc*>(st);
+    sgm_scandesc* sd = reinterpret_cast<sgm_scandesc*>(st + src_b);
+    sgm_tabdesc* te = reinterpret_cast<sgm_tabdesc*>(st + src_b + scan_b);
+    uint32_t* h = reinterpret_cast<uint32_t*>(st + dev_b);  // totals[M], maxima[M]
+    uint8_t* dev = reinterpret_cast<uint8_t*>(c->d_tb);
+    uint32_t ns = 0;
+    for (uint32_t k = 0; k < M; k++) {
+        const Edge& E = edges[k];
+        sgm_query* q = qs[E.q];
+        const uint32_t n = q->q.n, rows = q->counts[E.a];
+        sgm_table& t = q->tables[(size_t)E.a * n + E.b];
+        t.d_off = q->d_toff + E.o;
+        // edges out of one vertex are consecutive: one count descriptor per run of them
+        if (k == 0 || edges[k - 1].q != E.q || edges[k - 1].a != E.a) {
+            sgm_tabsrc& S = td[ns++];
             S = sgm_tabsrc{};
-            S.rows = q->d_list + q->cand_off[a];
+            S.rows = q->d_list + q->cand_off[E.a];
+            S.vmask = q->d_vmask;
             S.nrows = rows;
-            ns++;
         }
         sgm_tabsrc& S = td[ns - 1];
         const uint32_t j = S.k++;
-        S.b[j] = b;
-        S.rowcnt[j] = q->d_tscr + (o - k);
-        te[k] = sgm_tabdesc{S.rows, rows, b, q->d_bm + (size_t)b * W, q->d_rank + (size_t)b * W, t.d_off,
-                            nullptr, nullptr};
-        max_rows = std::max(max_rows, rows);
-        o += (size_t)rows + 1;
+        S.b[j] = E.b;
+        S.rowcnt[j] = q->d_tscr + E.r;
+        sd[k] = sgm_scandesc{q->d_tscr + E.r, t.d_off, c->d_ttot + k, c->d_ttot + M + k, rows, 0};
+        te[k] = sgm_tabdesc{S.rows, rows, E.b, q->d_bm + (size_t)E.b * W, q->d_rank + (size_t)E.b * W, t.d_off,
+                            nullptr, nullptr, q->d_vmask};
     }
-    HIPCHK(c, hipMemcpyAsync(q->d_tdesc, td, sizeof(sgm_tabsrc) * ns, hipMemcpyHostToDevice, c->stream));
-    sgm::launch_table_count(c->stream, q->g->d_off, q->g->d_adj, q->d_vmask, q->d_tdesc, ns, max_rows);
-    uint32_t* d_desc = q->d_tscr + Rw + 2 * (size_t)m;
-    HIPCHK(c, hipMemcpyAsync(d_desc, hs, sizeof(uint32_t) * 2 * m, hipMemcpyHostToDevice, c->stream));
-    sgm::launch_scan_segments(c->stream, q->d_tscr, (uint32_t)Rw, m, d_desc, q->d_toff);
-    HIPCHK(c, hipMemcpyAsync(h, q->d_tscr + Rw, sizeof(uint32_t) * 2 * m, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dev, st, src_b + scan_b, hipMemcpyHostToDevice, c->stream));
+    sgm::launch_table_count(c->stream, g->d_off, g->d_adj, reinterpret_cast<const sgm_tabsrc*>(dev), ns, max_rows);
+    sgm::launch_scan_segments(c->stream, reinterpret_cast<const sgm_scandesc*>(dev + src_b), M);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(h, c->d_ttot, sizeof(uint32_t) * 2 * M, hipMemcpyDeviceToHost, c->stream));
     hipError_t se = hipStreamSynchronize(c->stream);
-    if (se != hipSuccess) { free_tables(q); return set_err(c, SGM_EHIP, "table count: %s", hipGetErrorString(se)); }
-    // pass 1: entries into one arena
-    size_t ent = 0;
-    for (uint32_t k = 0; k < m; k++) ent += h[k];
-    hipError_t e = grow_arena(&q->d_tidx, q->tidx_cap, ent);
-    if (e == hipSuccess) e = grow_arena(&q->d_tv, q->tv_cap, ent);
-    if (e != hipSuccess) {
-        free_tables(q);
-        return set_err(c, SGM_ENOMEM, "tables of %llu entries", (unsigned long long)ent);
+    if (se != hipSuccess) {
+        for (uint32_t i = 0; i < nq; i++) free_tables(qs[i]);
+        return set_err(c, SGM_EHIP, "table count: %s", hipGetErrorString(se));
     }
-    size_t base = 0;
-    for (uint32_t k = 0; k < m; k++) {
-        const uint32_t a = edges[k].first, b = edges[k].second;
-        sgm_table& t = q->tables[(size_t)a * n + b];
-        t.e_cnt = h[k];
-        t.max_deg = h[m + k];
-        t.d_idx = q->d_tidx + base;
-        t.d_v = q->d_tv + base;
-        base += h[k];
-        te[k].eidx = t.d_idx;
-        te[k].ev = t.d_v;
-        t.built = true;
+    // entries of each query into its arena
+    for (uint32_t i = 0; i < nq; i++) {
+        sgm_query* q = qs[i];
+        size_t ent = 0;
+        for (uint32_t k = first[i]; k < first[i + 1]; k++) ent += h[k];
+        if (first[i] == first[i + 1]) continue;
+        hipError_t e = grow_arena(&q->d_tidx, q->tidx_cap, std::max<size_t>(ent, 1));
+        if (e == hipSuccess) e = grow_arena(&q->d_tv, q->tv_cap, std::max<size_t>(ent, 1));
+        if (e != hipSuccess) {
+            for (uint32_t j = 0; j < nq; j++) free_tables(qs[j]);
+            return set_err(c, SGM_ENOMEM, "tables of %llu entries", (unsigned long long)ent);
+        }
+        size_t base = 0;
+        for (uint32_t k = first[i]; k < first[i + 1]; k++) {
+            sgm_table& t = q->tables[(size_t)edges[k].a * q->q.n + edges[k].b];
+            t.e_cnt = h[k];
+            t.max_deg = h[M + k];
+            t.d_idx = q->d_tidx + base;
+            t.d_v = q->d_tv + base;
+            base += h[k];
+            te[k].eidx = t.d_idx;
+            te[k].ev = t.d_v;
+            t.built = true;
+        }
     }
-    // pass 1: every table's entries in one launch
-    HIPCHK(c, hipMemcpyAsync(q->d_tedesc, te, sizeof(sgm_tabdesc) * m, hipMemcpyHostToDevice, c->stream));
-    sgm::launch_table_fill(c->stream, q->g->d_off, q->g->d_adj, q->d_vmask, q->d_tedesc, m, max_rows);
+    HIPCHK(c, hipMemcpyAsync(dev + src_b + scan_b, te, sizeof(sgm_tabdesc) * M, hipMemcpyHostToDevice, c->stream));
+    sgm::launch_table_fill(c->stream, g->d_off, g->d_adj, reinterpret_cast<const sgm_tabdesc*>(dev + src_b + scan_b), M,
+                           max_rows);
+    HIPCHK(c, hipGetLastError());
     se = hipStreamSynchronize(c->stream);
-    if (se != hipSuccess) { free_tables(q); return set_err(c, SGM_EHIP, "tables: %s", hipGetErrorString(se)); }
+    if (se != hipSuccess) {
+        for (uint32_t i = 0; i < nq; i++) free_tables(qs[i]);
+        return set_err(c, SGM_EHIP, "tables: %s", hipGetErrorString(se));
+    }
     return SGM_OK;
+}
+
+int sgm_build_tables(sgm_query* q)
+{
+    if (!q) return SGM_EINVAL;
+    return build_tables_many(&q, 1);
 }
 
 int sgm_table_info(sgm_query* q, uint32_t a, uint32_t b, uint64_t* e_cnt, uint32_t* max_degree)
@@ -1110,18 +1147,24 @@ static int enumerate_aux_batch(sgm_query* const* qs, const uint32_t* const* orde
     sgm_graph* g = qs[0]->g;
     sgm_ctx* c = g->ctx;
     std::vector<sgm_auxplan_dev> aps(nq);
+    std::vector<sgm_query*> missing;
     for (uint32_t i = 0; i < nq; i++) {
         sgm_query* q = qs[i];
-        const uint32_t* order = orders[i];
         const uint32_t n = q->q.n;
         bool have = true;
         for (uint32_t a = 0; a < n && have; a++)
             for (uint32_t p = q->q.off[a]; p < q->q.off[a + 1]; p++)
                 have &= q->tables[(size_t)a * n + q->q.adj[p]].built;
-        if (!have && !q->empty) {
-            const int trc = sgm_build_tables(q);
-            if (trc) return trc;
-        }
+        if (!have && !q->empty) missing.push_back(q);
+    }
+    if (!missing.empty()) {  // every missing table of the launch in one build
+        const int trc = build_tables_many(missing.data(), (uint32_t)missing.size());
+        if (trc) return trc;
+    }
+    for (uint32_t i = 0; i < nq; i++) {
+        sgm_query* q = qs[i];
+        const uint32_t* order = orders[i];
+        const uint32_t n = q->q.n;
         std::vector<std::vector<uint32_t>> prev;
         sgm::previous_neighbours(q->q, order, prev);
         std::vector<uint32_t> pos(n);

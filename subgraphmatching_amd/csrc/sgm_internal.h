@@ -49,7 +49,8 @@ struct sgm_auxplan_dev {
 };
 
 // One directed query edge (a, b) of the aux-table fill pass (inc/table.h:7-137): rows = C(a); entries go to
-// eidx / ev at the scanned row offsets rowoff, positions in C(b) from b's bitmap and ranks.
+// eidx / ev at the scanned row offsets rowoff, positions in C(b) from b's bitmap and ranks; vmask = the
+// query's membership words (descriptors of many queries share one launch).
 struct sgm_tabdesc {
     const uint32_t* rows;
     uint32_t nrows, b;
@@ -58,12 +59,24 @@ struct sgm_tabdesc {
     const uint32_t* rowoff;
     uint32_t* eidx;
     uint32_t* ev;
+    const uint32_t* vmask;
+};
+
+// One exclusive scan of the aux-table build: in[0, n) into out[0, n] (out[n] = total), the
+// total to *tot and the maximum to *maxv (one block per scan, many scans per launch).
+struct sgm_scandesc {
+    const uint32_t* in;
+    uint32_t* out;
+    uint32_t* tot;
+    uint32_t* maxv;
+    uint32_t n, pad;
 };
 
 // The directed query edges (a, b_j), j < k, out of one query vertex a in the count pass of an
 // aux-table build (inc/table.h:7-137): rows = C(a); the row counts of b_j go to rowcnt[j].
 struct sgm_tabsrc {
     const uint32_t* rows;
+    const uint32_t* vmask;  // the query's membership words
     uint32_t nrows, k;
     uint32_t b[SGM_MAXQ];
     uint32_t* rowcnt[SGM_MAXQ];

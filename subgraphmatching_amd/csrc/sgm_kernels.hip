@@ -767,10 +767,10 @@ __device__ __forceinline__ uint32_t cand_rank(const uint64_t* __restrict__ bm, c
 // lane j keeps b_j's count.
 __global__ void __launch_bounds__(256) k_table_count_src(const uint32_t* __restrict__ off,
                                                          const uint32_t* __restrict__ adj,
-                                                         const uint32_t* __restrict__ vmask,
                                                          const sgm_tabsrc* __restrict__ ds)
 {
     const sgm_tabsrc& D = ds[blockIdx.y];
+    const uint32_t* __restrict__ vmask = D.vmask;
     const uint32_t waves = blockDim.x >> 6, lane = lane_id();
     const uint32_t* __restrict__ rows = D.rows;
     const uint32_t nrows = D.nrows, k = D.k;
@@ -793,10 +793,10 @@ __global__ void __launch_bounds__(256) k_table_count_src(const uint32_t* __restr
 // edge per block row keeps each wave's bitmap-rank lookups independent of the others'.
 __global__ void __launch_bounds__(256) k_table_fill_edge(const uint32_t* __restrict__ off,
                                                          const uint32_t* __restrict__ adj,
-                                                         const uint32_t* __restrict__ vmask,
                                                          const sgm_tabdesc* __restrict__ ds)
 {
     const sgm_tabdesc& D = ds[blockIdx.y];
+    const uint32_t* __restrict__ vmask = D.vmask;
     const uint32_t waves = blockDim.x >> 6, lane = lane_id();
     const uint32_t nrows = D.nrows, b = D.b;
     for (uint32_t i = blockIdx.x * waves + (threadIdx.x >> 6); i < nrows; i += gridDim.x * waves) {
@@ -822,16 +822,12 @@ __global__ void __launch_bounds__(256) k_table_fill_edge(const uint32_t* __restr
 __device__ void scan_block(const uint32_t* __restrict__ in, uint32_t n, uint32_t* __restrict__ out,
                            uint32_t* __restrict__ maxv, uint32_t* __restrict__ tot);
 
-// Many scans at once, one block each: segment s scans in = scr + (obase - s), n entries, into
-// out = toff + obase (desc[2s] = obase, desc[2s+1] = n); totals to scr[R + s], maxima to
-// scr[R + m + s] (the aux-table row counts of every directed query edge).
-__global__ void __launch_bounds__(256) k_scan_segments(uint32_t* __restrict__ scr, uint32_t R, uint32_t m,
-                                                       const uint32_t* __restrict__ desc, uint32_t* __restrict__ toff)
+// Many scans at once, one block each (the row counts of every directed query edge of many
+// queries into their row offsets, totals and maxima).
+__global__ void __launch_bounds__(256) k_scan_segments(const sgm_scandesc* __restrict__ ds)
 {
-    const uint32_t s = blockIdx.x;
-    if (s >= m) return;
-    const uint32_t obase = desc[2 * s], n = desc[2 * s + 1];
-    scan_block(scr + (obase - s), n, toff + obase, scr + R + m + s, scr + R + s);
+    const sgm_scandesc& D = ds[blockIdx.x];
+    scan_block(D.in, D.n, D.out, D.maxv, D.tot);
 }
 
 __device__ void scan_block(const uint32_t* __restrict__ in, uint32_t n, uint32_t* __restrict__ out,
@@ -1098,27 +1094,39 @@ void launch_compact(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V)
     hipLaunchKernelGGL(k_compact, grid, dim3(256), 0, s, cs, V);
 }
 
-void launch_table_count(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* vmask,
-                        const sgm_tabsrc* ds, uint32_t nsrc, uint32_t max_rows)
+// grid.y / grid.x hold at most 65535 descriptors per launch: longer lists go in pieces
+static const uint32_t kMaxGridY = 65535;
+
+void launch_table_count(hipStream_t s, const uint32_t* off, const uint32_t* adj, const sgm_tabsrc* ds, uint32_t nsrc,
+                        uint32_t max_rows)
 {
     if (!nsrc || !max_rows) return;
     const uint32_t xcap = 16384 / nsrc > 8 ? 16384 / nsrc : 8u;
-    dim3 grid(clampg(((uint64_t)max_rows + 3) / 4, 1, xcap), nsrc);
-    hipLaunchKernelGGL(k_table_count_src, grid, dim3(256), 0, s, off, adj, vmask, ds);
+    for (uint32_t s0 = 0; s0 < nsrc; s0 += kMaxGridY) {
+        const uint32_t k = nsrc - s0 < kMaxGridY ? nsrc - s0 : kMaxGridY;
+        dim3 grid(clampg(((uint64_t)max_rows + 3) / 4, 1, xcap), k);
+        hipLaunchKernelGGL(k_table_count_src, grid, dim3(256), 0, s, off, adj, ds + s0);
+    }
 }
 
-void launch_table_fill(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* vmask,
-                       const sgm_tabdesc* ds, uint32_t m, uint32_t max_rows)
+void launch_table_fill(hipStream_t s, const uint32_t* off, const uint32_t* adj, const sgm_tabdesc* ds, uint32_t m,
+                       uint32_t max_rows)
 {
     if (!m || !max_rows) return;
     const uint32_t xcap = 16384 / m > 8 ? 16384 / m : 8u;
-    dim3 grid(clampg(((uint64_t)max_rows + 3) / 4, 1, xcap), m);
-    hipLaunchKernelGGL(k_table_fill_edge, grid, dim3(256), 0, s, off, adj, vmask, ds);
+    for (uint32_t s0 = 0; s0 < m; s0 += kMaxGridY) {
+        const uint32_t k = m - s0 < kMaxGridY ? m - s0 : kMaxGridY;
+        dim3 grid(clampg(((uint64_t)max_rows + 3) / 4, 1, xcap), k);
+        hipLaunchKernelGGL(k_table_fill_edge, grid, dim3(256), 0, s, off, adj, ds + s0);
+    }
 }
 
-void launch_scan_segments(hipStream_t s, uint32_t* scr, uint32_t R, uint32_t m, const uint32_t* desc, uint32_t* toff)
+void launch_scan_segments(hipStream_t s, const sgm_scandesc* ds, uint32_t m)
 {
-    if (m) hipLaunchKernelGGL(k_scan_segments, dim3(m), dim3(256), 0, s, scr, R, m, desc, toff);
+    for (uint32_t s0 = 0; s0 < m; s0 += kMaxGridY) {
+        const uint32_t k = m - s0 < kMaxGridY ? m - s0 : kMaxGridY;
+        hipLaunchKernelGGL(k_scan_segments, dim3(k), dim3(256), 0, s, ds + s0);
+    }
 }
 
 void launch_hub_bits(hipStream_t s, const uint32_t* off, const uint32_t* adj, const uint32_t* hubs, uint32_t n_hubs,

@@ -64,6 +64,12 @@ struct sgm_ctx {
     uint32_t* d_rprog = nullptr;
     uint32_t* h_rprog = nullptr;
     uint32_t rprog_cap = 0;
+    // aux-table builds of many queries: descriptors (count / scan / fill) and the totals and
+    // maxima of every directed edge's scan
+    uint8_t* d_tb = nullptr;
+    size_t tb_bytes = 0;
+    uint32_t* d_ttot = nullptr;
+    uint32_t ttot_cap = 0;
     // batched enumeration: bitmap of C(u_L) per query; depth-1 split blocks and chunk records
     uint32_t* d_lastbits = nullptr;
     size_t lastbits_words = 0;

@@ -19,7 +19,7 @@ from .api import Context, _check, default_context
 from .graph import Query
 
 __all__ = ["CostModel", "sample_order", "candidate_orders", "choose_orders", "node_label", "collect_training",
-           "train_model"]
+           "train_model", "LABELS", "LABEL_METHOD"]
 
 
 def sample_order(q: Query, seed: int) -> list:
@@ -47,7 +47,12 @@ def busy_label(result) -> float:
     return math.log2(1.0 + result.busy_ns / 1000.0)
 
 
-LABELS = {"busy": busy_label, "nodes": lambda r: node_label(r.nodes)}
+LABELS = {"busy": busy_label, "nodes": lambda r: node_label(r.nodes), "plain-nodes": lambda r: node_label(r.nodes)}
+# the search each label's training runs are logged with: the cache + filter + failing-set
+# search the orders are chosen for, or the same search without failing sets ("plain": the
+# reference's node counts of explore_Intersection_Recursive, inc/explore.h:555-557,646-648)
+LABEL_METHOD = {"busy": L.SGM_ENUM_CACHE_FILTER_FS, "nodes": L.SGM_ENUM_CACHE_FILTER_FS,
+                "plain-nodes": L.SGM_ENUM_CACHE_FILTER}
 
 
 class _Samples:
@@ -179,12 +184,14 @@ def choose_orders(model: CostModel, queries, counts, degsums, cands, fallback, m
 def collect_training(runner, queries, n_samples=8, cap=10_000_000, seed=0, time_limit_s=5.0, log=None,
                      label="busy"):
     """Training triples logged by the enumerator: for every query, its RI order, GQL order
-    and ``n_samples`` sampled orders are enumerated through the batched runner (cache +
-    filter + failing sets, the same search the orders are chosen for) and labelled with
-    ``LABELS[label]``: log2(1 + wave microseconds) ("busy") or log2(1 + internal search-tree
-    nodes) ("nodes").  Launches cut off by the time limit leave their unfinished queries out.
-    Returns (items, labels, info)."""
+    and ``n_samples`` sampled orders are enumerated through the batched runner and labelled
+    with ``LABELS[label]``: log2(1 + internal search-tree nodes) of the cache + filter +
+    failing-set search the orders are chosen for ("nodes") or of the same search without
+    failing sets ("plain-nodes"), or log2(1 + wave microseconds) of the former ("busy").
+    Launches cut off by the time limit leave their unfinished queries out.  Returns (items,
+    labels, info)."""
     label_of = LABELS[label]
+    method = LABEL_METHOD[label]
     from .api import Order_RI
     items, labels = [], []
     dropped = enumerated = 0
@@ -200,7 +207,7 @@ def collect_training(runner, queries, n_samples=8, cap=10_000_000, seed=0, time_
             sets.append(cands)
         for k in range(2 + n_samples):
             orders = [sets[i][k] for i in range(len(part))]
-            res = runner.enumerate(orders=orders, cap=cap, time_limit_s=time_limit_s)
+            res = runner.enumerate(orders=orders, method=method, cap=cap, time_limit_s=time_limit_s)
             for i, r in enumerate(res):
                 if not ok[i]:
                     continue
