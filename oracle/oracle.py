@@ -307,3 +307,77 @@ def heap_pop_order(keys, keep, use_std=False):
     else:
         lib().orc_heap_test(C.c_uint32(k.size), _p(k), C.c_uint32(keep), _p(out))
     return [int(x) for x in out[:min(keep, k.size)]]
+
+
+# ---- dynamic-order searches (sgm_oracle_dyn.c) ------------------------------------------
+class DynStats(C.Structure):
+    _fields_ = [("count", C.c_uint64), ("intersections", C.c_uint64), ("nodes", C.c_uint64 * MAXQ),
+                ("enum_ms", C.c_double), ("timed_out", C.c_int32), ("capped", C.c_int32)]
+
+
+def _weights_ptrs(weights):
+    if weights is None:
+        return None, None
+    arrs = [np.ascontiguousarray(w, dtype=np.uint32) for w in weights]
+    ptrs = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+    return ptrs, arrs
+
+
+def dyn_bfs(g, q):
+    """The BFS order the dynamic searches start from: BFS from DAFStartVertex
+    (computeWeightArray inc/order.h:980-994; src/order_ablations.cpp:158-160)."""
+    return bfs(q, start_vertex("DAF", g, q))
+
+
+def daf_weights(g, q, cands, bfs_order=None, parent=None):
+    """computeWeightArray's per-candidate weights, one uint32 array per query vertex."""
+    if bfs_order is None:
+        bfs_order, parent, _ = dyn_bfs(g, q)
+    V, n = g.getVertexCnt(), q.getVertexCnt()
+    m, cnt, stride = _cand_matrix(cands, V)
+    out = [np.zeros(max(int(c), 1), dtype=np.uint32) for c in cnt]
+    ptrs = (C.c_void_p * n)(*[a.ctypes.data for a in out])
+    lib().orc_daf_weights(C.c_uint32(V), _p(g.off), _p(g.adj), C.c_uint32(n), _p(q.off), _p(q.adj), _p(m), _p(cnt),
+                          C.c_uint32(stride), _p(_u32(bfs_order)), _p(_u32(parent)), ptrs)
+    return [out[u][:int(cnt[u])] for u in range(n)]
+
+
+def enum_dynamic(g, q, cands, kind="DAF", cap=None, time_limit_ms=0.0, bfs_order=None, weights=None):
+    """explore_Intersection_DynamicOrder_DAF ("DAF", ranked by computeWeightArray weights) or
+    _VEQ ("VEQ", ranked by candidate counts); returns count, intersections, per-depth
+    extensions (the last not a reference counter), timed_out, capped."""
+    assert lib().orc_dyn_stats_size() == C.sizeof(DynStats)
+    if bfs_order is None:
+        bfs_order, parent, _ = dyn_bfs(g, q)
+        if kind == "DAF" and weights is None:
+            weights = daf_weights(g, q, cands, bfs_order, parent)
+    V, n = g.getVertexCnt(), q.getVertexCnt()
+    m, cnt, stride = _cand_matrix(cands, V)
+    ptrs, keep = _weights_ptrs(weights if kind == "DAF" else None)
+    st = DynStats()
+    rc = lib().orc_enum_dynamic(C.c_uint32(V), _p(g.off), _p(g.adj), C.c_uint32(n), _p(q.off), _p(q.adj), _p(m),
+                                _p(cnt), C.c_uint32(stride), _p(_u32(bfs_order)), ptrs,
+                                C.c_uint64(cap if cap is not None else 2**64 - 1), C.c_double(time_limit_ms),
+                                C.byref(st))
+    if rc:
+        raise ValueError(f"orc_enum_dynamic failed ({rc})")
+    return {"count": int(st.count), "intersections": int(st.intersections), "nodes": [int(x) for x in st.nodes[:n]],
+            "timed_out": bool(st.timed_out), "capped": bool(st.capped), "enum_ms": float(st.enum_ms)}
+
+
+def probe_dynamic(g, q, cands, bfs_order, weights, cap=None):
+    """The same search on std::priority_queue itself (libsortprobe.so): (count, intersections,
+    per-depth extensions)."""
+    path = os.path.join(HERE, "libsortprobe.so")
+    if not os.path.exists(path):
+        build()
+    P = C.CDLL(path)
+    V, n = g.getVertexCnt(), q.getVertexCnt()
+    m, cnt, stride = _cand_matrix(cands, V)
+    ptrs, keep = _weights_ptrs(weights)
+    c_, i_ = C.c_uint64(), C.c_uint64()
+    nodes = np.zeros(MAXQ, dtype=np.uint64)
+    P.probe_dyn_enum(C.c_uint32(V), _p(g.off), _p(g.adj), C.c_uint32(n), _p(q.off), _p(q.adj), _p(m), _p(cnt),
+                     C.c_uint32(stride), _p(_u32(bfs_order)), ptrs, C.c_uint64(cap if cap is not None else 2**64 - 1),
+                     C.byref(c_), C.byref(i_), _p(nodes))
+    return int(c_.value), int(i_.value), [int(x) for x in nodes[:n]]

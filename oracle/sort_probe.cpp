@@ -52,3 +52,138 @@ extern "C" void probe_std_heap(unsigned count, const double *keys, unsigned keep
         q.pop();
     }
 }
+
+// The dynamic-order search (inc/explore.h:1674-2062) restated with the reference's own
+// containers: a std::priority_queue per depth under extendable_vertex_compare, copied from the
+// depth above (vector of queues, emplace_back(back())), candidate lists from the aux tables of
+// the previous BFS neighbours.  It pins the C oracle's restated heap moves inside the whole
+// search: counts, intersections and per-depth extensions must agree.  weight: per vertex
+// pointer to cnt[u] u32 weights, or null (the VEQ variant ranks by candidate count).
+#include <functional>
+
+namespace {
+using QEnt = std::pair<std::pair<uint32_t, uint32_t>, uint32_t>;
+struct QCmp {
+    bool operator()(const QEnt& l, const QEnt& r) const
+    {
+        if (l.first.second == 1 && r.first.second != 1) return true;
+        if (l.first.second != 1 && r.first.second == 1) return false;
+        return l.second > r.second;
+    }
+};
+using DynPQ = std::priority_queue<QEnt, std::vector<QEnt>, QCmp>;
+}  // namespace
+
+extern "C" int probe_dyn_enum(uint32_t V, const uint32_t* off, const uint32_t* adj, uint32_t n, const uint32_t* qoff,
+                              const uint32_t* qadj, const uint32_t* cand, const uint32_t* cnt, uint32_t stride,
+                              const uint32_t* bfs, const uint32_t* const* weight, uint64_t cap, uint64_t* out_count,
+                              uint64_t* out_inter, uint64_t* out_nodes)
+{
+    std::vector<uint32_t> pos(n);
+    for (uint32_t i = 0; i < n; i++) pos[bfs[i]] = i;
+    std::vector<std::vector<uint32_t>> prev(n), fol(n);
+    for (uint32_t i = 0; i < n; i++) {
+        uint32_t v = bfs[i];
+        for (uint32_t j = 0; j < n; j++) {
+            uint32_t w = bfs[j];
+            bool a = false;
+            for (uint32_t p = qoff[v]; p < qoff[v + 1]; p++) a |= qadj[p] == w;
+            if (!a) continue;
+            if (j < i) prev[v].push_back(w);
+            if (j > i) fol[v].push_back(w);
+        }
+    }
+    // tables[a][b][i] = ascending positions in C(b) adjacent to C(a)[i]
+    std::vector<std::vector<std::vector<std::vector<uint32_t>>>> T(n, std::vector<std::vector<std::vector<uint32_t>>>(n));
+    std::vector<int64_t> where(V, -1);
+    for (uint32_t a = 0; a < n; a++)
+        for (uint32_t p = qoff[a]; p < qoff[a + 1]; p++) {
+            uint32_t b = qadj[p];
+            for (uint32_t k = 0; k < cnt[b]; k++) where[cand[(size_t)b * stride + k]] = k;
+            T[a][b].assign(cnt[a], {});
+            for (uint32_t i = 0; i < cnt[a]; i++) {
+                uint32_t v = cand[(size_t)a * stride + i];
+                for (uint32_t q = off[v]; q < off[v + 1]; q++)
+                    if (where[adj[q]] >= 0) T[a][b][i].push_back((uint32_t)where[adj[q]]);
+                std::sort(T[a][b][i].begin(), T[a][b][i].end());
+            }
+            for (uint32_t k = 0; k < cnt[b]; k++) where[cand[(size_t)b * stride + k]] = -1;
+        }
+    std::vector<std::vector<uint32_t>> valid(n);
+    std::vector<uint32_t> ext(n), idx(n, 0), mo(n, 0);
+    std::vector<char> vis(V, 0);
+    for (uint32_t u = 0; u < n; u++) ext[u] = (uint32_t)prev[u].size();
+    uint64_t ans = 0, inter = 0;
+    for (uint32_t d = 0; d < n; d++) out_nodes[d] = 0;
+    bool stop = false;
+    std::vector<DynPQ> vq;
+    auto cands = [&](uint32_t u) {
+        valid[u] = T[prev[u][0]][u][idx[prev[u][0]]];
+        for (size_t i = 1; i < prev[u].size(); i++) {
+            const auto& row = T[prev[u][i]][u][idx[prev[u][i]]];
+            std::vector<uint32_t> r;
+            std::set_intersection(row.begin(), row.end(), valid[u].begin(), valid[u].end(), std::back_inserter(r));
+            valid[u] = r;
+            inter++;
+        }
+    };
+    auto update = [&](uint32_t u) {
+        for (uint32_t f : fol[u])
+            if (--ext[f] == 0) {
+                cands(f);
+                uint32_t w = 0;
+                if (weight)
+                    for (uint32_t x : valid[f]) w += weight[f][x];
+                else
+                    w = (uint32_t)valid[f].size();
+                vq.back().emplace(std::make_pair(std::make_pair(f, qoff[f + 1] - qoff[f]), w));
+            }
+    };
+    auto restore = [&](uint32_t u) {
+        for (uint32_t f : fol[u]) ext[f]++;
+    };
+    std::function<void(uint32_t)> rec = [&](uint32_t d) {
+        uint32_t u = mo[d];
+        if (valid[u].empty()) return;
+        const std::vector<uint32_t> mine = valid[u];  // (valid[u] is not rewritten below this depth)
+        for (uint32_t vi : mine) {
+            uint32_t v = cand[(size_t)u * stride + vi];
+            if (vis[v]) continue;
+            out_nodes[d]++;
+            if (d == n - 1) {
+                if (++ans == cap) { stop = true; return; }
+                continue;
+            }
+            idx[u] = vi;
+            vis[v] = 1;
+            vq.emplace_back(vq.back());
+            update(u);
+            mo[d + 1] = vq.back().top().first.first;
+            vq.back().pop();
+            rec(d + 1);
+            if (stop) return;
+            vis[v] = 0;
+            vq.pop_back();
+            restore(u);
+        }
+    };
+    const uint32_t u0 = bfs[0];
+    mo[0] = u0;
+    for (uint32_t p = 0; p < cnt[u0] && !stop; p++) {
+        out_nodes[0]++;
+        idx[u0] = p;
+        vis[cand[(size_t)u0 * stride + p]] = 1;
+        vq.emplace_back(DynPQ(QCmp()));
+        update(u0);
+        mo[1] = vq.back().top().first.first;
+        vq.back().pop();
+        rec(1);
+        if (stop) break;
+        vis[cand[(size_t)u0 * stride + p]] = 0;
+        vq.pop_back();
+        restore(u0);
+    }
+    *out_count = ans;
+    *out_inter = inter;
+    return 0;
+}
