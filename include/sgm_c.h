@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define SGM_ABI_VERSION 3
+#define SGM_ABI_VERSION 4
 
 enum {
     SGM_OK = 0,
@@ -49,8 +49,14 @@ enum {
     SGM_ENUM_CACHE_FILTER_FS = 0, /* explore_Intersection_Cache_DAF_Filter (inc/explore.h:1073) */
     SGM_ENUM_CACHE_FILTER = 1,    /* same search without failing-set pruning */
     SGM_ENUM_AUX_TABLES = 2,      /* explore_Intersection_All<false,false,false,false,false> (inc/explore.h:2757) */
-    SGM_ENUM_AUX_TABLES_FS = 3    /* explore_Intersection_All<true,false,false,false,false>: the same with DAF
+    SGM_ENUM_AUX_TABLES_FS = 3,   /* explore_Intersection_All<true,false,false,false,false>: the same with DAF
                                      failing sets (inc/prune.h:2099-2260, 82-171) */
+    SGM_ENUM_DYN_DAF = 4,         /* explore_Intersection_DynamicOrder_DAF (inc/explore.h:1849-1931): the order
+                                     argument is the start (BFS) order; the next vertex is chosen per search
+                                     node by the DAF weights of sgm_daf_weights / sgm_set_daf_weights for that
+                                     order (computed in the call when none are resident for it) */
+    SGM_ENUM_DYN_VEQ = 5          /* explore_Intersection_DynamicOrder_VEQ (inc/explore.h:1993-2062): the same,
+                                     ranked by candidate counts */
 };
 
 typedef struct sgm_ctx sgm_ctx;
@@ -216,8 +222,23 @@ int sgm_plan_export_labeled(const uint32_t* qoff, const uint32_t* qadj, const ui
                             const uint32_t* order, uint32_t* n_ops, uint32_t* depth, uint32_t* parent,
                             uint32_t* vdepth, uint32_t* emask, uint32_t* iter_set);
 
-/* ---- enumeration (replaces explore_Intersection_Cache_DAF_Filter inc/explore.h:1073 and
- * explore_Intersection_All<false x5> :2757) -------------------------------------------- */
+/* ---- dynamic-order searches (explore_Intersection_DynamicOrder_DAF / _VEQ,
+ * inc/explore.h:1674-2162; driver src/order_ablations.cpp:131-172) ---------------------
+ * DAFStartVertex (inc/filter.h:922-940) of a query: the start of the BFS order both searches
+ * take (sgm_query_bfs gives the order). */
+int sgm_daf_start_vertex(sgm_query* q, uint32_t* start);
+/* computeWeightArray (inc/order.h:978-1031) on the device: bfs_order[n] receives the BFS order
+ * from DAFStartVertex; the weights stay resident with the query for SGM_ENUM_DYN_DAF over
+ * that order, and weights[u] (counts[u] words each; weights == NULL: none) receives them.
+ * q must be filtered; its aux tables are built if missing.  SGM_EEMPTY when some candidate
+ * set is empty. */
+int sgm_daf_weights(sgm_query* q, uint32_t* bfs_order, uint32_t* const* weights);
+/* The caller's weight_array for start order `order` (counts[u] words per vertex), made the
+ * resident weights of SGM_ENUM_DYN_DAF (the drop-in passes the weight_array it was given). */
+int sgm_set_daf_weights(sgm_query* q, const uint32_t* order, const uint32_t* const* weights);
+
+/* ---- enumeration (replaces explore_Intersection_Cache_DAF_Filter inc/explore.h:1073,
+ * explore_Intersection_All<false x5> :2757 and the dynamic-order searches :1849, :1993) -- */
 int sgm_enumerate(sgm_query* q, const uint32_t* order, const sgm_enum_opts* opts, sgm_stats* out);
 /* sgm_enumerate without allocating: no reference counterpart.  For callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
@@ -234,7 +255,8 @@ int sgm_enumerate_noalloc(sgm_query* q, const uint32_t* order, const sgm_enum_op
  * with or without failing sets, and the aux-table search, whose missing tables are built in
  * the call; no reference counterpart: the reference runs one query per call).  Each query
  * gets its own cap and its own sgm_stats; out[i].kernel_ms is the time of the whole launch.
- * orders == NULL or orders[i] == NULL: the RI order (Order_RI), computed in the call. */
+ * orders == NULL or orders[i] == NULL: the RI order (Order_RI), computed in the call; for
+ * the dynamic-order methods, the BFS order from DAFStartVertex. */
 int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uint32_t nq, const sgm_enum_opts* opts,
                         sgm_stats* out);
 

@@ -35,6 +35,8 @@ SGM_ENUM_CACHE_FILTER_FS = 0
 SGM_ENUM_CACHE_FILTER = 1
 SGM_ENUM_AUX_TABLES = 2
 SGM_ENUM_AUX_TABLES_FS = 3
+SGM_ENUM_DYN_DAF = 4
+SGM_ENUM_DYN_VEQ = 5
 
 ERR_NAMES = {0: "SGM_OK", -1: "SGM_EINVAL", -2: "SGM_EHIP", -3: "SGM_ENOMEM", -4: "SGM_EEMPTY",
              -5: "SGM_EORDER", -6: "SGM_ESTATE", -7: "SGM_ESTOPPED", -8: "SGM_EDEVICE",
@@ -55,7 +57,7 @@ EXPORTS = [
     "sgm_costmodel_timing", "sgm_order_sample", "sgm_batch_rank_orders", "sgm_gql_work",
     "sgm_graph_set_labels", "sgm_query_set_labels", "sgm_plan_export_labeled", "sgm_batch_filter_labeled",
     "sgm_enumerate_noalloc", "sgm_order_qsi", "sgm_order_cfl", "sgm_order_ceci", "sgm_order_gup",
-    "sgm_query_kcore", "sgm_query_bfs",
+    "sgm_query_kcore", "sgm_query_bfs", "sgm_daf_start_vertex", "sgm_daf_weights", "sgm_set_daf_weights",
 ]
 
 
@@ -164,6 +166,9 @@ def lib() -> C.CDLL:
         "sgm_order_gup": (C.c_int, [vp, vp]),
         "sgm_query_kcore": (C.c_int, [vp, vp, C.c_uint32, vp]),
         "sgm_query_bfs": (C.c_int, [vp, vp, C.c_uint32, C.c_uint32, vp, vp, vp]),
+        "sgm_daf_start_vertex": (C.c_int, [vp, vp]),
+        "sgm_daf_weights": (C.c_int, [vp, vp, vp]),
+        "sgm_set_daf_weights": (C.c_int, [vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -32,7 +32,8 @@ __all__ = ["Context", "default_context", "DeviceGraph", "CandidateParam", "Edges
            "Filter_GQL", "Filter_CFL", "Filter_CECI", "Filter_DAF", "Order_RI", "Order_GQL", "Order_QSI",
            "Order_CFL", "Order_CECI", "Order_GuP", "query_kcore", "ReuseParam", "Config", "ExploreResult",
            "explore_Intersection_Cache_DAF_Filter", "explore_Intersection_All", "Profiler",
-           "filter_batch", "enumerate_query", "enumerate_batch", "plan_export", "Batch"]
+           "filter_batch", "enumerate_query", "enumerate_batch", "plan_export", "Batch", "DAFStartVertex",
+           "computeWeightArray", "explore_Intersection_DynamicOrder_DAF", "explore_Intersection_DynamicOrder_VEQ"]
 
 
 def _check(code, ctx=None):
@@ -484,7 +485,10 @@ def enumerate_query(can: CandidateParam, order, method=L.SGM_ENUM_CACHE_FILTER_F
                     stack_words=0, flags=0, time_limit_ms=0, roots=None) -> ExploreResult:
     """``time_limit_ms``: per-query work budget in device wave time (0 = none); a query that
     passes it comes back with ``timed_out`` set and a partial count.  ``roots`` = (lo, hi):
-    search only the depth-0 candidates with lo <= id < hi (disjoint ranges add up)."""
+    search only the depth-0 candidates with lo <= id < hi (disjoint ranges add up).  For the
+    dynamic-order methods ``order`` is the start order and ``nodes`` counts the extensions
+    per depth of the search, whose vertex at a depth varies (the reference's dynamic searches
+    count no iterations, so Profiler.total_iter_count is left alone)."""
     opts = _opts(method, cap, blocks, stack_words, flags, time_limit_ms=time_limit_ms, roots=roots)
     st = L.Stats()
     o = L.u32(order)
@@ -492,8 +496,9 @@ def enumerate_query(can: CandidateParam, order, method=L.SGM_ENUM_CACHE_FILTER_F
     res = _result(st, can.query.getVertexCnt())
     prof = Profiler.getInst()
     prof.total_intersection += res.intersections
-    for d, x in enumerate(res.nodes):
-        prof.total_iter_count[d] += x
+    if method not in (L.SGM_ENUM_DYN_DAF, L.SGM_ENUM_DYN_VEQ):
+        for d, x in enumerate(res.nodes):
+            prof.total_iter_count[d] += x
     prof.daf_prune_cnt += res.fs_prunes
     return res
 
@@ -540,6 +545,71 @@ def explore_Intersection_All(data: Graph, query: Query, order, cfg: Config,
     if cfg.can is None:
         raise ValueError("the candidate-space search needs filtered candidates (cfg.can)")
     return enumerate_query(cfg.can, order, L.SGM_ENUM_AUX_TABLES, output_limit)
+
+
+def DAFStartVertex(data: Graph, query: Query, can: CandidateParam) -> int:
+    """inc/filter.h:922-940: the start vertex of the DAF BFS order (NLF counts only)."""
+    s = C.c_uint32()
+    _check(L.lib().sgm_daf_start_vertex(can._q, C.byref(s)), can.ctx)
+    return int(s.value)
+
+
+def computeWeightArray(data: Graph, query: Query, can: CandidateParam, bfs_order: list) -> list:
+    """inc/order.h:978-1031 on the device: ``bfs_order`` (a list, cleared first) receives the BFS
+    order from DAFStartVertex; returns weight_array, one uint32 array of candidates_count[u]
+    weights per query vertex (the reference fills a caller-allocated uint32_t**)."""
+    n = query.getVertexCnt()
+    cnt = can.candidates_count
+    w = [np.zeros(max(int(c), 1), dtype=np.uint32) for c in cnt]
+    o = np.zeros(n, dtype=np.uint32)
+    ptrs = (C.c_void_p * n)(*[a.ctypes.data for a in w])
+    _check(L.lib().sgm_daf_weights(can._q, L.ptr(o), ptrs), can.ctx)
+    bfs_order.clear()
+    bfs_order.extend(int(x) for x in o)
+    return [w[u][:int(cnt[u])] for u in range(n)]
+
+
+def _dyn_check(query, bfs_order, cfg):
+    cfg.Check()
+    if cfg.can is None:
+        raise ValueError("the dynamic-order search needs filtered candidates (cfg.can)")
+    if len(bfs_order) != query.getVertexCnt():
+        raise L.SgmError(L.SGM_EORDER, "bfs_order length differs from the query size")
+
+
+def _dynamic(query, bfs_order, match_order, cfg, method, output_limit):
+    # the reference writes match_order past its cleared size (inc/explore.h:1877-1879), so
+    # the caller's vector comes back empty
+    match_order.clear()
+    return enumerate_query(cfg.can, bfs_order, method, output_limit)
+
+
+def explore_Intersection_DynamicOrder_DAF(data: Graph, query: Query, bfs_order, match_order: list, cfg: Config,
+                                          weight_array, output_limit: int | None = None) -> ExploreResult:
+    """inc/explore.h:1849-1931: the next query vertex per search node by the DAF weights in
+    ``weight_array`` (from computeWeightArray over ``bfs_order``)."""
+    _dyn_check(query, bfs_order, cfg)
+    n = query.getVertexCnt()
+    if len(weight_array) != n:
+        raise ValueError("weight_array needs one array per query vertex")
+    cnt = cfg.can.candidates_count
+    ws = []
+    for u in range(n):
+        a = np.ascontiguousarray(weight_array[u], dtype=np.uint32)
+        if a.size < int(cnt[u]):
+            raise ValueError(f"weight_array[{u}] holds {a.size} weights for {int(cnt[u])} candidates")
+        ws.append(a if a.size else np.zeros(1, dtype=np.uint32))
+    o = L.u32(bfs_order)
+    ptrs = (C.c_void_p * n)(*[a.ctypes.data for a in ws])
+    _check(L.lib().sgm_set_daf_weights(cfg.can._q, L.ptr(o), ptrs), cfg.can.ctx)
+    return _dynamic(query, bfs_order, match_order, cfg, L.SGM_ENUM_DYN_DAF, output_limit)
+
+
+def explore_Intersection_DynamicOrder_VEQ(data: Graph, query: Query, bfs_order, match_order: list, cfg: Config,
+                                          output_limit: int | None = None) -> ExploreResult:
+    """inc/explore.h:1993-2062: the same search ranked by candidate counts."""
+    _dyn_check(query, bfs_order, cfg)
+    return _dynamic(query, bfs_order, match_order, cfg, L.SGM_ENUM_DYN_VEQ, output_limit)
 
 
 def _opts(method, cap, blocks=0, stack_words=0, flags=0, hub_min_degree=0, time_limit_ms=0, roots=None):

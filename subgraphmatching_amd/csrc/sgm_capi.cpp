@@ -96,6 +96,7 @@ void sgm_ctx_destroy(sgm_ctx* c)
     dfree(c->d_auxplan);
     dfree(c->d_auxplans);
     dfree(c->d_afirst);
+    dfree(c->d_dynplans);
     dfree(c->d_astats);
     dfree(c->d_astop);
     dfree(c->d_work64);
@@ -350,6 +351,7 @@ uint32_t sgm_graph_max_degree(const sgm_graph* g) { return g ? g->max_deg : 0; }
 static void free_tables(sgm_query* q)
 {
     for (auto& t : q->tables) t = sgm_table();
+    q->weights_ok = false;
 }
 
 // Grows a u32 arena to at least `need` words (contents are not kept).
@@ -374,6 +376,7 @@ void sgm_query_destroy(sgm_query* q)
     dfree(q->d_tscr);
     dfree(q->d_tdesc);
     dfree(q->d_tedesc);
+    dfree(q->d_weight);
     dfree(q->d_vmask);
     dfree(q->d_removed);
     dfree(q->d_bm);
@@ -787,7 +790,7 @@ int sgm_cand_exist_download(sgm_query* q, uint32_t* words, uint32_t num_words)
 // launch (grouped by source vertex), one scan launch (one block per directed edge), one
 // read-back of the totals, and one fill launch -- two host synchronisations in all.  Queries
 // whose candidate sets are empty are skipped (no search reads their tables).
-static int build_tables_many(sgm_query* const* qs, uint32_t nq)
+extern "C++" int build_tables_many(sgm_query* const* qs, uint32_t nq)
 {
     if (!nq) return SGM_OK;
     sgm_graph* g = qs[0]->g;
@@ -1291,6 +1294,24 @@ int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uin
         HIPCHK(c, hipSetDevice(c->device));
         return enumerate_aux_batch(qs, optr.data(), nq, o, out);
     }
+    if (o.method == SGM_ENUM_DYN_DAF || o.method == SGM_ENUM_DYN_VEQ) {
+        // start orders: given, or BFS from DAFStartVertex (the reference driver's)
+        std::vector<std::array<uint32_t, SGM_MAXQ>> ords(nq);
+        std::vector<const uint32_t*> optr(nq);
+        HIPCHK(c, hipSetDevice(c->device));
+        for (uint32_t i = 0; i < nq; i++) {
+            sgm_query* q = qs[i];
+            if (q->g != g) return set_err(c, SGM_EINVAL, "queries of a batch must share one graph");
+            if (const int frc = check_filtered(q, "enumerate")) return frc;
+            if (q->q.n < 2) return set_err(c, SGM_EINVAL, "enumeration needs at least 2 query vertices (query %u)", i);
+            if (orders && orders[i]) std::copy(orders[i], orders[i] + q->q.n, ords[i].begin());
+            else if (const int orc = dyn_default_order(q, ords[i].data())) return orc;
+            if (sgm::check_order(q->q, ords[i].data()))
+                return set_err(c, SGM_EORDER, "order of query %u is not a connected permutation", i);
+            optr[i] = ords[i].data();
+        }
+        return enumerate_dyn_batch(qs, optr.data(), nq, o, out);
+    }
     if (o.method != SGM_ENUM_CACHE_FILTER_FS && o.method != SGM_ENUM_CACHE_FILTER)
         return set_err(c, SGM_EINVAL, "unknown enumeration method %d", o.method);
     // plans (host), validated before anything runs; plans, query descriptors and statistics
@@ -1479,6 +1500,7 @@ int sgm_enumerate(sgm_query* q, const uint32_t* order, const sgm_enum_opts* opts
         HIPCHK(c, hipSetDevice(c->device));
         return enumerate_aux_batch(&q, &order, 1, o, out);
     }
+    if (o.method == SGM_ENUM_DYN_DAF || o.method == SGM_ENUM_DYN_VEQ) return sgm_enumerate_batch(&q, &order, 1, &o, out);
     if (o.method != SGM_ENUM_CACHE_FILTER_FS && o.method != SGM_ENUM_CACHE_FILTER)
         return set_err(c, SGM_EINVAL, "unknown enumeration method %d", o.method);
     return sgm_enumerate_batch(&q, &order, 1, &o, out);

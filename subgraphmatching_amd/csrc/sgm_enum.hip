@@ -1514,6 +1514,363 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
     flush_all(false);
 }
 
+// ================================================================== dynamic-order enumeration
+// explore_Intersection_DynamicOrder_DAF / _VEQ (inc/explore.h:1674-2162): the query vertex
+// matched next is chosen per search node.  A vertex becomes extendable once every previous
+// neighbour of it in the start order is matched (updataExtendableV :1726-1770); its
+// candidates are then the table row of its first previous neighbour intersected with the
+// rows of the others (getCandidatesFor :1691-1724), and it enters a priority queue ranked by
+// the sum of its candidates' DAF weights (or by their count), vertices of query degree 1
+// last.  The queue of depth d + 1 starts as a copy of depth d's (vec_rank_queue) and its top
+// is matched at d + 1.  Ties are left to the queue, so the heap moves are libstdc++'s
+// push_heap / pop_heap, replayed by every lane of the wave on its LDS copy.  One wave per
+// depth-0 candidate.  A candidate list of one previous neighbour is its table row; the
+// intersections of the others go to the wave's HBM stack, above the mark of the depth whose
+// match made the vertex extendable.  Injectivity (buffer->vis) is the only check; the last
+// level is counted in one pass.
+#define SGM_DYN_HEAP 528  // heap of depth d (1 <= d < n) holds at most n - d entries; n - d + 1 reserved
+
+struct WaveDyn {
+    unsigned long long probes;
+    unsigned long long nodes[SGM_MAXQ];
+    uint64_t rem[SGM_MAXQ];           // per depth: the untried extensions of the current chunk
+    uint64_t heap[SGM_DYN_HEAP];      // per depth: (vertex << 40) | (degree-1 flag << 32) | weight
+    const uint32_t* lst[SGM_MAXQ];    // per query vertex: its candidate list (positions in C(u))
+    uint32_t tot[SGM_MAXQ];           // per query vertex: the list's length
+    uint32_t iemb[SGM_MAXQ];          // per query vertex: the position of its match in C(u)
+    uint32_t ext[SGM_MAXQ];           // per query vertex: previous neighbours not yet matched
+    uint32_t emb[SGM_MAXQ];           // per depth: the matched data vertex
+    uint32_t mo[SGM_MAXQ];            // per depth: the matched query vertex (match_order)
+    uint32_t pos[SGM_MAXQ];
+    uint32_t chunk_base[SGM_MAXQ];
+    uint32_t stack_top[SGM_MAXQ + 1];
+    uint32_t hlen[SGM_MAXQ + 1];
+};
+
+// later reads of LDS words other lanes wrote see the writes
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ uint32_t wave_sum32(uint32_t x)
+{
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) x += (uint32_t)__shfl_xor((int)x, s);
+    return x;
+}
+
+// Rank key of a queue entry: larger = lower priority (extendable_vertex_compare,
+// inc/explore.h:1675-1686: degree 1 below the rest, then the larger weight below)
+__device__ __forceinline__ uint64_t dyn_key(uint64_t e) { return e & 0x1FFFFFFFFull; }
+
+// first word of the heap of depth d (1 <= d < n): depth k reserves n - k + 1 entries
+__device__ __forceinline__ uint32_t dyn_heap_off(uint32_t d, uint32_t n)
+{
+    return (d - 1) * (n + 1) - (d - 1) * d / 2;
+}
+
+// std::__push_heap: the value rises from `hole` past every parent that ranks below it
+__device__ void dyn_sift_up(uint64_t* f, int hole, uint64_t val)
+{
+    int parent = (hole - 1) / 2;
+    while (hole > 0 && dyn_key(f[parent]) > dyn_key(val)) {
+        f[hole] = f[parent];
+        hole = parent;
+        parent = (hole - 1) / 2;
+    }
+    f[hole] = val;
+}
+
+// std::pop_heap over f[0, len): the top's hole sinks to a leaf through the child that ranks
+// higher (the right one unless the left ranks strictly higher), the last entry takes the
+// freed slot's place by a sift-up from that leaf
+__device__ void dyn_pop(uint64_t* f, int len)
+{
+    if (len <= 1) return;
+    const int m = len - 1;
+    const uint64_t val = f[m];
+    f[m] = f[0];
+    int hole = 0, second = 0;
+    while (second < (m - 1) / 2) {
+        second = 2 * (second + 1);
+        if (dyn_key(f[second]) > dyn_key(f[second - 1])) second--;
+        f[hole] = f[second];
+        hole = second;
+    }
+    if ((m & 1) == 0 && second == (m - 2) / 2) {
+        second = 2 * (second + 1);
+        f[hole] = f[second - 1];
+        hole = second - 1;
+    }
+    dyn_sift_up(f, hole, val);
+}
+
+// Many queries per launch as in k_enum_aux: items are the depth-0 candidates of every query.
+__global__ void __launch_bounds__(256) k_enum_dyn(const sgm_dynplan_dev* __restrict__ plans,
+                                                  const uint32_t* __restrict__ item_first, uint32_t nq,
+                                                  uint32_t* __restrict__ stack, uint32_t stack_words,
+                                                  uint32_t* __restrict__ work, sgm_stats_dev* __restrict__ stats,
+                                                  uint32_t* __restrict__ stops, uint32_t* __restrict__ dev_stop,
+                                                  const uint32_t* __restrict__ host_stop, unsigned long long cap,
+                                                  unsigned long long budget, uint32_t root_lo, uint32_t root_hi)
+{
+    extern __shared__ uint64_t smem_u64[];
+    uint8_t* smem = reinterpret_cast<uint8_t*>(smem_u64);
+    const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
+    WaveDyn& S = *reinterpret_cast<WaveDyn*>(smem + wave * ((sizeof(WaveDyn) + 15) & ~(size_t)15));
+    uint32_t* ws = stack + (size_t)(blockIdx.x * (blockDim.x >> 6) + wave) * stack_words;
+    if (lane < SGM_MAXQ) S.nodes[lane] = 0;
+    S.probes = 0;
+    wave_sync();
+    unsigned long long inter = 0, elems = 0, outs = 0, scanned = 0, cnt_pending = 0, items_done = 0, busy = 0;
+    bool overflow = false;
+    uint32_t poll = 0;
+    uint64_t relay_at = 0;
+    uint32_t cur = SGM_INVALID, n = 0, L = 0;
+    const sgm_dynplan_dev* Pp = plans;
+    sgm_stats_dev* gst = nullptr;
+    uint32_t* stop = nullptr;
+    auto flush_all = [&](bool ovf) {
+        if (cur == SGM_INVALID) return;
+        flush_count(cnt_pending, &gst->count, stop, cap);
+        if (lane == 0) {
+            if (inter) atomicAdd(&gst->intersections, inter);
+            if (elems) atomicAdd(&gst->elems_read, elems);
+            if (outs) atomicAdd(&gst->elems_out, outs);
+            if (scanned) atomicAdd(&gst->elems_scanned, scanned);
+            if (items_done) atomicAdd(&gst->items_done, items_done);
+            if (busy) atomicAdd(&gst->busy_ticks, busy);
+            if (S.probes) atomicAdd(&gst->probe_words, S.probes);
+            if (ovf) { atomicAdd(&gst->overflow, 1ull); atomicExch(stop, 1u); }
+        }
+        if (lane < SGM_MAXQ) {
+            if (lane < n && S.nodes[lane]) atomicAdd(&gst->nodes[lane], S.nodes[lane]);
+            S.nodes[lane] = 0;
+        }
+        S.probes = 0;
+        wave_sync();
+        inter = elems = outs = scanned = items_done = busy = 0;
+    };
+    const uint32_t total = item_first[nq];
+    while (true) {
+        uint32_t it = 0;
+        if (lane == 0) it = atomicAdd(work, 1u);
+        it = uni(__shfl((int)it, 0));
+        if (it >= total) break;
+        if (__atomic_load_n(dev_stop, __ATOMIC_RELAXED)) break;
+        relay_stop(wall_clock64(), relay_at, host_stop, dev_stop);
+        uint32_t lo = 0, hi = nq;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (item_first[mid] <= it) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t q = uni(lo);
+        if (q != cur) {
+            flush_all(false);
+            cur = q;
+            Pp = plans + q;
+            n = Pp->n;
+            L = n - 1;
+            gst = stats + q;
+            stop = stops + q;
+        }
+        if (__atomic_load_n(stop, __ATOMIC_RELAXED)) continue;
+        const sgm_dynplan_dev& P = *Pp;
+        const uint32_t u0 = P.root;
+        const uint32_t root = it - item_first[q];
+        const uint32_t v0 = P.cand[u0][root];
+        if (v0 < root_lo || v0 >= root_hi) continue;
+        items_done++;
+        uint64_t t_mark = wall_clock64();
+        bool aborted = false;
+
+        // Match the query vertex of depth d to its chosen candidate's extensions: copy the
+        // queue of depth d into depth d + 1, make the following neighbours whose last previous
+        // neighbour this was extendable (their candidate lists, weights and queue entries, in
+        // following-neighbour order), then pop the vertex of depth d + 1.  False: the stack
+        // cannot hold an intersection.
+        auto expand = [&](uint32_t d) -> bool {
+            const uint32_t u = S.mo[d];
+            uint64_t* dst = S.heap + dyn_heap_off(d + 1, n);
+            uint32_t len = 0;
+            if (d) {
+                len = S.hlen[d];
+                if (lane < len) dst[lane] = S.heap[dyn_heap_off(d, n) + lane];
+                wave_sync();
+            }
+            uint32_t top = S.stack_top[d];
+            for (uint32_t k = P.fol_first[u]; k < P.fol_first[u + 1]; k++) {
+                const uint32_t f = P.fol[k];
+                const uint32_t left = S.ext[f] - 1u;
+                S.ext[f] = left;
+                if (left) continue;
+                const uint32_t e0 = P.prev_first[f], e1 = P.prev_first[f + 1];
+                const uint32_t* list;
+                uint32_t size;
+                {
+                    const uint32_t* o = P.tab_off[e0];
+                    const uint32_t r = S.iemb[P.prev_vtx[e0]];
+                    const uint32_t b = o[r];
+                    list = P.tab_idx[e0] + b;
+                    size = o[r + 1] - b;
+                }
+                for (uint32_t e = e0 + 1; e < e1; e++) {
+                    const uint32_t* o = P.tab_off[e];
+                    const uint32_t r = S.iemb[P.prev_vtx[e]];
+                    const uint32_t b = o[r], nb = o[r + 1] - b;
+                    const uint32_t need = size < nb ? size : nb;
+                    if (top + need > stack_words) return false;
+                    const uint32_t sz = isect_write(list, size, nullptr, P.tab_idx[e] + b, nb, nullptr, ws + top,
+                                                    elems, outs, &S.probes, stop);
+                    list = ws + top;
+                    size = sz;
+                    top += sz;
+                }
+                inter += e1 - e0 - 1;
+                S.lst[f] = list;
+                S.tot[f] = size;
+                uint32_t w = size;
+                if (const uint32_t* wf = P.weight[f]) {
+                    uint32_t s = 0;
+                    for (uint32_t i = lane; i < size; i += 64) s += wf[list[i]];
+                    w = wave_sum32(s);  // u32 sum, wrapping as the reference's
+                }
+                const uint64_t val = ((uint64_t)f << 40) | ((uint64_t)((P.deg1 >> f) & 1u) << 32) | w;
+                dst[len] = val;
+                dyn_sift_up(dst, (int)len, val);
+                len++;
+                wave_sync();
+            }
+            const uint64_t t = dst[0];
+            dyn_pop(dst, (int)len);
+            S.hlen[d + 1] = len - 1;
+            S.mo[d + 1] = (uint32_t)(t >> 40);
+            S.stack_top[d + 1] = top;
+            wave_sync();
+            return true;
+        };
+
+        if (lane < n) S.ext[lane] = P.prev_first[lane + 1] - P.prev_first[lane];
+        S.emb[0] = v0;
+        S.iemb[u0] = root;
+        S.mo[0] = u0;
+        S.nodes[0] += 1;
+        S.stack_top[0] = 0;
+        wave_sync();
+        if (!expand(0)) {
+            overflow = true;
+            aborted = true;
+        }
+        uint32_t d = 1;
+        bool entering = true;
+        // back from depth d to d - 1: the match at d - 1 is undone (restoreExtendableV)
+        auto back_up = [&]() -> bool {
+            d--;
+            if (d == 0) return false;
+            const uint32_t u = S.mo[d];
+            for (uint32_t k = P.fol_first[u]; k < P.fol_first[u + 1]; k++) S.ext[P.fol[k]] += 1u;
+            wave_sync();
+            return true;
+        };
+        while (!aborted) {
+            if ((++poll & 63u) == 0) {
+                if ((poll & 1023u) == 0) {
+                    const uint64_t now = wall_clock64();
+                    relay_stop(now, relay_at, host_stop, dev_stop);
+                    busy += now - t_mark;
+                    t_mark = now;
+                    if (budget && busy >= SGM_CHARGE_TICKS) charge_time(gst, stop, budget, busy);
+                }
+                if (__atomic_load_n(stop, __ATOMIC_RELAXED) | __atomic_load_n(dev_stop, __ATOMIC_RELAXED)) {
+                    aborted = true;
+                    break;
+                }
+            }
+            const uint32_t u = uni(S.mo[d]);
+            if (entering) {
+                entering = false;
+                const uint32_t size = uni(S.tot[u]);
+                if (d == L) {
+                    const uint32_t* list = S.lst[u];
+                    const uint32_t* cand = P.cand[u];
+                    uint32_t ok = 0;
+                    scanned += size;
+                    for (uint32_t base = 0; base < size; base += 64) {
+                        const uint32_t i = base + lane;
+                        bool good = false;
+                        if (i < size) {
+                            const uint32_t v = cand[list[i]];
+                            good = true;
+                            for (uint32_t j = 0; j < L; j++) good &= S.emb[j] != v;
+                        }
+                        ok += (uint32_t)__popcll(__ballot(good));
+                    }
+                    cnt_pending += ok;
+                    if (cnt_pending >= FLUSH_AT) flush_count(cnt_pending, &gst->count, stop, cap);
+                    S.nodes[L] += ok;
+                    if (!back_up()) break;
+                    continue;
+                }
+                if (size == 0) {
+                    if (!back_up()) break;
+                    continue;
+                }
+                S.pos[d] = 0;
+                S.rem[d] = 0;
+                continue;
+            }
+            const uint32_t* list = S.lst[u];
+            if (S.rem[d] == 0) {
+                const uint32_t tot = uni(S.tot[u]);
+                const uint32_t base = uni(S.pos[d]);
+                if (base >= tot) {
+                    if (!back_up()) break;
+                    continue;
+                }
+                S.pos[d] = base + 64;
+                S.chunk_base[d] = base;
+                const uint32_t i = base + lane;
+                scanned += (tot - base < 64u) ? (tot - base) : 64u;
+                bool good = false;
+                if (i < tot) {
+                    const uint32_t v = P.cand[u][list[i]];
+                    good = true;
+                    for (uint32_t j = 0; j < d; j++) good &= S.emb[j] != v;
+                }
+                S.rem[d] = __ballot(good);
+                continue;
+            }
+            const uint64_t rm = uni64(S.rem[d]);
+            const uint32_t j = (uint32_t)__ffsll((long long)rm) - 1u;
+            S.rem[d] = rm & (rm - 1);
+            const uint32_t vidx = list[uni(S.chunk_base[d]) + j];
+            S.emb[d] = P.cand[u][vidx];
+            S.iemb[u] = vidx;
+            S.nodes[d] += 1;
+            wave_sync();
+            if (!expand(d)) {
+                overflow = true;
+                aborted = true;
+                break;
+            }
+            d++;
+            entering = true;
+        }
+        busy += wall_clock64() - t_mark;
+        if (budget && busy >= SGM_CHARGE_TICKS) charge_time(gst, stop, budget, busy);
+        flush_count(cnt_pending, &gst->count, stop, cap);
+        if (overflow) {
+            flush_all(true);
+            overflow = false;
+            cur = SGM_INVALID;
+        }
+    }
+    flush_all(false);
+}
+
 }  // namespace sgm
 
 // ================================================================== launch wrappers
@@ -1563,6 +1920,16 @@ void launch_enum_aux(hipStream_t s, uint32_t blocks, const sgm_auxplan_dev* plan
     const size_t smem = 4 * ((sizeof(WaveAux) + 15) & ~(size_t)15);
     hipLaunchKernelGGL(k_enum_aux, dim3(blocks), dim3(256), smem, s, plans, item_first, nq, stack, stack_words, work,
                        st, stops, dev_stop, host_stop, cap, budget, root_lo, root_hi, use_fs);
+}
+
+void launch_enum_dyn(hipStream_t s, uint32_t blocks, const sgm_dynplan_dev* plans, const uint32_t* item_first,
+                     uint32_t nq, uint32_t* stack, uint32_t stack_words, uint32_t* work, sgm_stats_dev* st,
+                     uint32_t* stops, uint32_t* dev_stop, const uint32_t* host_stop, unsigned long long cap,
+                     unsigned long long budget, uint32_t root_lo, uint32_t root_hi)
+{
+    const size_t smem = 4 * ((sizeof(WaveDyn) + 15) & ~(size_t)15);
+    hipLaunchKernelGGL(k_enum_dyn, dim3(blocks), dim3(256), smem, s, plans, item_first, nq, stack, stack_words, work,
+                       st, stops, dev_stop, host_stop, cap, budget, root_lo, root_hi);
 }
 
 }  // namespace sgm

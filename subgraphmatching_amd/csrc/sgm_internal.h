@@ -48,6 +48,27 @@ struct sgm_auxplan_dev {
     uint64_t anc[SGM_MAXQ];           // DAF ancestor masks over depths (failing sets, inc/prune.h:100-104)
 };
 
+// Plan of the dynamic-order searches (explore_Intersection_DynamicOrder_DAF / _VEQ,
+// inc/explore.h:1674-2162), indexed by query vertex, not depth: the previous neighbours of u
+// in the start order (GetPreviousNeb, order-position order) are edges [prev_first[u],
+// prev_first[u+1]) with their tables (p -> u) and p itself; the following neighbours
+// (GetFollowingNeb) are fol[fol_first[u] .. fol_first[u+1]).  weight[u] holds the DAF weight
+// of each position of C(u) (computeWeightArray), or is null when the ranking is by candidate
+// count (the VEQ variant).  deg1: bit u set when u has query degree 1.
+struct sgm_dynplan_dev {
+    uint32_t n;
+    uint32_t root;                    // order[0]
+    uint32_t deg1;
+    uint32_t prev_first[SGM_MAXQ + 1];
+    uint32_t fol_first[SGM_MAXQ + 1];
+    uint8_t prev_vtx[SGM_MAXPREV];
+    uint8_t fol[SGM_MAXPREV];
+    const uint32_t* tab_off[SGM_MAXPREV];
+    const uint32_t* tab_idx[SGM_MAXPREV];
+    const uint32_t* cand[SGM_MAXQ];
+    const uint32_t* weight[SGM_MAXQ];
+};
+
 // One directed query edge (a, b) of the aux-table fill pass (inc/table.h:7-137): rows = C(a); entries go to
 // eidx / ev at the scanned row offsets rowoff, positions in C(b) from b's bitmap and ranks; vmask = the
 // query's membership words (descriptors of many queries share one launch).

@@ -68,6 +68,9 @@ void launch_copy_vmask(hipStream_t s, uint32_t V, const sgm_qdesc* qs, uint32_t 
 void launch_row_gather(hipStream_t s, const uint32_t* toff, const uint32_t* tidx, uint32_t rows,
                        const unsigned long long* in, unsigned long long* out, int mul, unsigned long long* total);
 void launch_fill_u64(hipStream_t s, unsigned long long* p, size_t n, unsigned long long val);
+// DAF weights (computeWeightArray): out[j] = min(out[j], sum of in[idx] over row j), u32 wrapping
+void launch_weight_rows(hipStream_t s, const uint32_t* toff, const uint32_t* tidx, uint32_t rows, const uint32_t* in,
+                        uint32_t* out);
 // sgm_enum.hip
 size_t enum_smem_bytes(uint32_t max_ops);
 // dev_stop: the device stop word every wave polls; host_stop: the host's request (pinned host
@@ -77,6 +80,11 @@ void launch_enum_aux(hipStream_t s, uint32_t blocks, const sgm_auxplan_dev* plan
                      uint32_t nq, uint32_t* stack, uint32_t stack_words, uint32_t* work, sgm_stats_dev* st,
                      uint32_t* stops, uint32_t* dev_stop, const uint32_t* host_stop, unsigned long long cap,
                      unsigned long long budget, uint32_t root_lo, uint32_t root_hi, int use_fs);
+// the dynamic-order searches (DAF / VEQ ranking) over sgm_dynplan_dev plans, items as above
+void launch_enum_dyn(hipStream_t s, uint32_t blocks, const sgm_dynplan_dev* plans, const uint32_t* item_first,
+                     uint32_t nq, uint32_t* stack, uint32_t stack_words, uint32_t* work, sgm_stats_dev* st,
+                     uint32_t* stops, uint32_t* dev_stop, const uint32_t* host_stop, unsigned long long cap,
+                     unsigned long long budget, uint32_t root_lo, uint32_t root_hi);
 // bitmap of C(u_L) per query (Q.lastbits) from its candidate words; before launch_enum_cache
 void launch_last_bits(hipStream_t s, const sgm_equery* qs, uint32_t nq, uint32_t V);
 // depth-1 split tables (queries with Q.split set), before launch_enum_cache; and the chunk

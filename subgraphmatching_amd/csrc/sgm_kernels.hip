@@ -965,6 +965,24 @@ __global__ void __launch_bounds__(256) k_row_gather(const uint32_t* __restrict__
     if (lane_id() == 0 && t) atomicAdd(total, t);
 }
 
+// computeWeightArray's step for one tree child (inc/order.h:1003-1022): a row's weight is the
+// u32 sum (wrapping) of the child's weights at the row's positions, kept when below the
+// current one.  Four lanes per row: rows are short and their lengths vary.
+__global__ void __launch_bounds__(256) k_weight_rows(const uint32_t* __restrict__ toff, const uint32_t* __restrict__ tidx,
+                                                     uint32_t rows, const uint32_t* __restrict__ in,
+                                                     uint32_t* __restrict__ out)
+{
+    const uint32_t sub = threadIdx.x & 3u;
+    for (uint32_t j = (blockIdx.x * blockDim.x + threadIdx.x) >> 2; j < rows; j += (gridDim.x * blockDim.x) >> 2) {
+        const uint32_t b = toff[j], e = toff[j + 1];
+        uint32_t s = 0;
+        for (uint32_t k = b + sub; k < e; k += 4) s += in[tidx[k]];
+        s += (uint32_t)__shfl_xor((int)s, 1);
+        s += (uint32_t)__shfl_xor((int)s, 2);
+        if (sub == 0 && s < out[j]) out[j] = s;
+    }
+}
+
 __global__ void __launch_bounds__(256) k_fill_u64(unsigned long long* __restrict__ p, size_t n, unsigned long long val)
 {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = val;
@@ -1154,6 +1172,14 @@ void launch_row_gather(hipStream_t s, const uint32_t* toff, const uint32_t* tidx
     if (rows)
         hipLaunchKernelGGL(k_row_gather, dim3(clampg(((uint64_t)rows + 255) / 256, 1, 1024)), dim3(256), 0, s, toff,
                            tidx, rows, in, out, mul, total);
+}
+
+void launch_weight_rows(hipStream_t s, const uint32_t* toff, const uint32_t* tidx, uint32_t rows, const uint32_t* in,
+                        uint32_t* out)
+{
+    if (rows)
+        hipLaunchKernelGGL(k_weight_rows, dim3(clampg(((uint64_t)rows * 4 + 255) / 256, 1, 2048)), dim3(256), 0, s,
+                           toff, tidx, rows, in, out);
 }
 
 void launch_fill_u64(hipStream_t s, unsigned long long* p, size_t n, unsigned long long val)

@@ -35,6 +35,8 @@ struct sgm_ctx {
     sgm_stats_dev* d_astats = nullptr;
     uint32_t* d_astop = nullptr;
     uint32_t aux_cap = 0, afirst_cap = 0, astats_cap = 0, astop_cap = 0;
+    sgm_dynplan_dev* d_dynplans = nullptr;   // dynamic-order search plans (shares the buffers above)
+    uint32_t dyn_cap = 0;
     unsigned long long* d_work64 = nullptr;  // item counter of the batched search
     sgm_plan_dev* d_plans = nullptr;          // per-query plans of a batch
     sgm_equery* d_eq = nullptr;
@@ -189,6 +191,12 @@ struct sgm_query {
     sgm_tabsrc* d_tdesc = nullptr;   // count pass: per query vertex (its directed edges)
     sgm_tabdesc* d_tedesc = nullptr; // fill pass: per directed edge
     size_t toff_cap = 0, tidx_cap = 0, tv_cap = 0, tscr_cap = 0, tdesc_cap = 0, tedesc_cap = 0;
+    // DAF weights of the dynamic-order search (computeWeightArray), per candidate position in
+    // the cand_off layout, for the start order weight_order; dropped with the tables
+    uint32_t* d_weight = nullptr;
+    size_t weight_cap = 0;
+    bool weights_ok = false;
+    uint32_t weight_order[SGM_MAXQ] = {0};
 };
 
 inline int set_err(sgm_ctx* c, int code, const char* fmt, ...)
@@ -295,6 +303,13 @@ int costmodel_forward_work(sgm_costmodel* m, uint32_t S, const float** y, hipStr
 // Slot map of the hub bitmaps to use for opts (nullptr = binary search everywhere).
 int hub_slot_map(sgm_graph* g, const sgm_enum_opts& o, const uint32_t** out);
 int ensure_stack(sgm_ctx* c, size_t bytes);
+// Aux tables (buildTables) of every directed edge of the queries, in one set of launches.
+int build_tables_many(sgm_query* const* qs, uint32_t nq);
+// The dynamic-order searches over many queries (sgm_dynamic.cpp): orders[i] = start order
+// (checked); and their default start order, BFS from DAFStartVertex.
+int enumerate_dyn_batch(sgm_query* const* qs, const uint32_t* const* orders, uint32_t nq, const sgm_enum_opts& o,
+                        sgm_stats* out);
+int dyn_default_order(sgm_query* q, uint32_t* order);
 sgm_enum_opts normalize_opts(const sgm_enum_opts* opts);
 uint32_t default_blocks(const sgm_ctx* c, const sgm_enum_opts& o, uint32_t nq);
 void stats_out(const sgm_stats_dev& st, uint64_t cap, sgm_stats* out, double ns_per_tick);
