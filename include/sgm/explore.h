@@ -1,10 +1,14 @@
-// Enumeration entry points (inc/explore.h:1073-1129 and :2757-2848), drop-in over the C ABI.
+// Enumeration entry points (inc/explore.h:1073-1129, :1849-2062 and :2757-2848), drop-in over
+// the C ABI.
 //
 //   explore_Intersection_Cache_DAF_Filter  intersection cache + candidate filter + DAF
 //                                          failing sets (the "RI+cache+filter+fs" row)
 //   explore_Intersection_All<false x5>     plain candidate-space search over the aux-edge
 //                                          tables (the "RI" / "GQL" rows)
 //   explore_Intersection_All<true, false x4>  the same with DAF failing sets
+//   explore_Intersection_DynamicOrder_DAF  the next vertex chosen per search node, by DAF
+//   explore_Intersection_DynamicOrder_VEQ  weights / by candidate counts (the "DAF" and
+//                                          "VEQ" rows of the order ablation)
 //
 // Semantics kept from the reference: `ans` is incremented by min(#embeddings,
 // output_limit); `time` is the search time in ms (here the enumeration kernel's, measured
@@ -20,7 +24,8 @@
 namespace sgm {
 
 inline void run_enumeration(const Query& query, const Order& order, const Config& cfg, double& time, size_t& ans,
-                            std::atomic<bool>& stop_flag, uint64_t output_limit, int method)
+                            std::atomic<bool>& stop_flag, uint64_t output_limit, int method,
+                            uint32_t* const* weights = nullptr)
 {
     cfg.Check();
     if (cfg.can == nullptr)
@@ -44,7 +49,8 @@ inline void run_enumeration(const Query& query, const Order& order, const Config
     int rc;
     {
         std::lock_guard<std::mutex> lk(device_mutex());
-        rc = sgm_enumerate(cfg.can->device(), order.data(), &opts, &st);
+        rc = weights ? sgm_set_daf_weights(cfg.can->device(), order.data(), weights) : SGM_OK;
+        if (rc == SGM_OK) rc = sgm_enumerate(cfg.can->device(), order.data(), &opts, &st);
     }
     done.store(true, std::memory_order_release);
     watcher.join();
@@ -53,7 +59,9 @@ inline void run_enumeration(const Query& query, const Order& order, const Config
     time = st.kernel_ms;
     Profiler& prof = Profiler::getInst();
     prof.total_intersection += (size_t)st.intersections;
-    for (uint32_t d = 0; d < query.getVertexCnt() && d < 32; d++) prof.total_iter_count[d] += (size_t)st.nodes[d];
+    // the dynamic searches count no iterations (their vertex per depth varies)
+    if (method != SGM_ENUM_DYN_DAF && method != SGM_ENUM_DYN_VEQ)
+        for (uint32_t d = 0; d < query.getVertexCnt() && d < 32; d++) prof.total_iter_count[d] += (size_t)st.nodes[d];
     prof.daf_fp.cnt += (uint32_t)st.fs_prunes;
 }
 
@@ -80,4 +88,28 @@ inline void explore_Intersection_All(const Graph& data, const Query& query, cons
     (void)data;
     sgm::run_enumeration(query, order, cfg, time, ans, stop_flag, output_limit,
                          useDAFFailingPrune ? SGM_ENUM_AUX_TABLES_FS : SGM_ENUM_AUX_TABLES);
+}
+
+// explore_Intersection_DynamicOrder_DAF (inc/explore.h:1849-1931): bfs_order is the start
+// order (computeWeightArray's), weight_array the weights ranked by (the caller's are used).
+// The reference writes match_order past its cleared size, so it comes back empty.
+inline void explore_Intersection_DynamicOrder_DAF(const Graph& data, const Query& query, const Order& bfs_order,
+                                                  Order& match_order, const Config& cfg, uint32_t** weight_array,
+                                                  double& time, size_t& ans, std::atomic<bool>& stop_flag,
+                                                  uint64_t output_limit = -1)
+{
+    (void)data;
+    if (weight_array == nullptr) throw sgm::Error(SGM_EINVAL, "weight_array is null");
+    match_order.clear();
+    sgm::run_enumeration(query, bfs_order, cfg, time, ans, stop_flag, output_limit, SGM_ENUM_DYN_DAF, weight_array);
+}
+
+// explore_Intersection_DynamicOrder_VEQ (inc/explore.h:1993-2062): ranked by candidate counts.
+inline void explore_Intersection_DynamicOrder_VEQ(const Graph& data, const Query& query, const Order& bfs_order,
+                                                  Order& match_order, const Config& cfg, double& time, size_t& ans,
+                                                  std::atomic<bool>& stop_flag, uint64_t output_limit = -1)
+{
+    (void)data;
+    match_order.clear();
+    sgm::run_enumeration(query, bfs_order, cfg, time, ans, stop_flag, output_limit, SGM_ENUM_DYN_VEQ);
 }

@@ -239,3 +239,38 @@ public:
         if (getVertexCnt() > 32) throw sgm::Error(SGM_EINVAL, file_path + ": queries have at most 32 vertices");
     }
 };
+
+// A vertex of a query's BFS tree (inc/graph.h:497-517).
+class TreeNode {
+public:
+    VertexID _id = 0;
+    uint32_t _level = 0;
+    VertexID _parent = 0;
+    std::vector<VertexID> _children;
+    size_t _estimated_embeddings_num = 0;
+};
+
+// BFS (inc/common_ops.h:318-350) of a query graph (at most 32 vertices): tree gets a new
+// TreeNode[vertex count] (the caller deletes it, as the reference's callers do); children in
+// the order they are discovered, i.e. the adjacency order of their parent.
+inline void BFS(const Graph& graph, VertexID start_vertex, TreeNode*& tree, Order& bfs_order)
+{
+    const uint32_t n = graph.getVertexCnt();
+    if (n > 32) throw sgm::Error(SGM_EINVAL, "BFS: queries have at most 32 vertices");
+    std::vector<uint32_t> order(n), parent(n), level(n);
+    const int got = sgm_query_bfs(graph.offsets(), graph.neighbours(), n, start_vertex, order.data(), parent.data(),
+                                  level.data());
+    if (got < 0) throw sgm::Error(got, "BFS");
+    tree = new TreeNode[n];
+    bfs_order.assign(order.begin(), order.begin() + got);
+    for (uint32_t i = 0; i < (uint32_t)got; i++) {
+        const VertexID u = order[i];
+        tree[u]._id = u;
+        tree[u]._level = level[u];
+        tree[u]._parent = u == start_vertex ? 0 : parent[u];
+        uint32_t cnt = 0;
+        const VertexID* nb = graph.getNeb(u, cnt);
+        for (uint32_t k = 0; k < cnt; k++)
+            if (nb[k] != start_vertex && parent[nb[k]] == u && level[nb[k]] == level[u] + 1) tree[u]._children.push_back(nb[k]);
+    }
+}

@@ -9,6 +9,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsgm_hip.so")
 DRIVER = os.path.join(HERE, "sgm_overall")
+ORDERS_DRIVER = os.path.join(HERE, "sgm_orders")
 ROOT = os.path.dirname(HERE)
 SOURCES = ["sgm_kernels.hip", "sgm_enum.hip", "sgm_costmodel.hip", "sgm_host.cpp", "sgm_capi.cpp",
            "sgm_batch.cpp", "sgm_costmodel_host.cpp", "sgm_noalloc.hip", "sgm_tree.cpp", "sgm_orders.cpp",
@@ -48,15 +49,17 @@ def build(verbose: bool = False) -> str:
 
 
 def build_driver() -> str:
-    """g++ build of the C++ row driver against the drop-in headers (no HIP headers needed)."""
-    tmp = DRIVER + ".tmp"
-    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
-           os.path.join(HERE, "cpp", "overall_performance.cpp"), "-o", tmp,
-           "-L", HERE, "-lsgm_hip", "-Wl,-rpath,$ORIGIN", "-pthread"]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"g++ failed on the row driver:\n{r.stderr}")
-    os.replace(tmp, DRIVER)
+    """g++ builds of the C++ drivers against the drop-in headers (no HIP headers needed): the
+    algorithm rows (sgm_overall) and the order comparison (sgm_orders)."""
+    for src, exe in (("overall_performance.cpp", DRIVER), ("order_ablations.cpp", ORDERS_DRIVER)):
+        tmp = exe + ".tmp"
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+               os.path.join(HERE, "cpp", src), "-o", tmp,
+               "-L", HERE, "-lsgm_hip", "-Wl,-rpath,$ORIGIN", "-pthread"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"g++ failed on {src}:\n{r.stderr}")
+        os.replace(tmp, exe)
     return DRIVER
 
 

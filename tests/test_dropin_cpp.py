@@ -14,6 +14,7 @@ from subgraphmatching_amd.graph import save_text
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "subgraphmatching_amd")
 DRIVER = os.path.join(PKG, "sgm_overall")
+ORDERS = os.path.join(PKG, "sgm_orders")
 
 # the reference harness's calling sequence (src/overall_performance.cpp:243-262)
 USAGE = r'''
@@ -53,6 +54,17 @@ size_t row(const Graph& data, const Query& query, std::atomic<bool>& stop_flag) 
         c3.can = &c2;
         explore_Intersection_All<true, false, false, false, false>(data, query, oq, c3, time, ans, stop_flag, 100000);
     }
+    // the dynamic-order rows of the order comparison (src/order_ablations.cpp:131-172)
+    uint32_t** weight_array = new uint32_t*[query.getVertexCnt()];
+    for (uint32_t i = 0; i < query.getVertexCnt(); ++i) weight_array[i] = new uint32_t[can.candidates_count[i] + 1];
+    Order bfs_order, match_order;
+    computeWeightArray(data, query, can, bfs_order, weight_array);
+    explore_Intersection_DynamicOrder_DAF(data, query, bfs_order, match_order, cfg, weight_array, time, ans, stop_flag,
+                                          100000);
+    TreeNode* tree = nullptr;
+    BFS(query, DAFStartVertex(data, query), tree, bfs_order);
+    explore_Intersection_DynamicOrder_VEQ(data, query, bfs_order, match_order, cfg, time, ans, stop_flag);
+    delete[] tree;
     return ans + ops + Profiler::getInst().total_intersection + can.candidates_count[0] + can.candidates_exsist[0][0];
 }
 '''
@@ -213,3 +225,39 @@ def _check(got, ref, limit):
     assert ans == ref["count"]
     if ref["count"] < limit:  # uncapped: the whole search ran, so the work is exact too
         assert inter == ref["intersections"]
+
+
+@pytest.mark.gpu
+def test_order_driver_rows_match_oracle(tmp_path):
+    """The order-comparison rows (GQL filter, then a static order with the plain search, or a
+    dynamic-order search) over reference-format files give the oracle's counts, and its
+    intersection counts where uncapped."""
+    assert os.path.exists(ORDERS), "order driver not built"
+    g = with_labels(chung_lu(2000, 2.4, 6.0, seed=83), 4, seed=84)
+    qs = query_set(g, 6, lambda i, r: int(r.integers(4, 11)), lambda i, r: bool(i % 2), seed=85)
+    data, paths = _write_inputs(tmp_path, g, qs)
+    limit = 100000
+    orders = ["QSI", "RI", "GQL", "CFL", "CECI", "DAF", "VEQ"]
+    for q, qp in zip(qs, paths):
+        print("query", qp, q.getVertexCnt(), flush=True)
+        r = subprocess.run([ORDERS, data, qp, str(limit), "300", *orders], capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 0, r.stderr
+        lines = r.stdout.strip().splitlines()
+        assert lines[0] == "data,query_name,explore,order,order_overhead,time_ms,ans,output_limit,intersection_cnt"
+        got = {}
+        for ln in lines[1:]:
+            f = ln.split(",")
+            assert f[0] == "data" and f[2] == "filter" and f[7] == str(limit)
+            got[f[3]] = (int(f[6]), int(f[8]))
+        ok, c = O.gql(g, q)
+        if not ok:
+            assert all(v == (0, 0) for v in got.values())
+            continue
+        static = {"RI": O.order_ri(q), "GQL": O.order_gql(g, q, [len(x) for x in c])}
+        for kind in ("QSI", "CFL", "CECI"):
+            static[kind] = O.order_tree(kind, g, q, c)
+        for kind, order in static.items():
+            _check(got[kind], O.enum_all(g, q, c, order, cap=limit), limit)
+        for kind in ("DAF", "VEQ"):
+            _check(got[kind], O.enum_dynamic(g, q, c, kind, cap=limit), limit)
