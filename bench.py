@@ -83,8 +83,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["chunglu100k", "chunglu100k-labelled", "chunglu100k-aux", "chunglu10k",
-                                           "rmat20", "rmat20-learned", "rmat22-serve"],
+    ap.add_argument("--workload", choices=["chunglu100k", "chunglu100k-labelled", "chunglu100k-aux", "chunglu100k-dyn",
+                                           "chunglu10k", "rmat20", "rmat20-learned", "rmat22-serve"],
                     default="chunglu100k")
     ap.add_argument("--serve-mem-gb", type=float, default=32.0,
                     help="rmat22-serve: device memory budget for query slots and DFS stacks")
@@ -104,7 +104,9 @@ def parse():
     ap.add_argument("--queries", type=int, default=0, help="number of queries (0 = the workload's own)")
     ap.add_argument("--labels", type=int, default=8,
                     help="chunglu100k-labelled: vertex labels 0..N-1, label l drawn with weight 1/(l+1)")
-    ap.add_argument("--cap", type=int, default=10_000_000)
+    ap.add_argument("--cap", type=int, default=None,
+                    help="embeddings per query (default 10^7; chunglu100k-dyn: 10^5, the output limit of the "
+                         "reference's order comparison)")
     ap.add_argument("--uncapped-budget-ms", type=int, default=-1,
                     help="chunglu100k / chunglu10k: per-query wave-time budget of the uncapped pass (0 skips it; "
                          "default 10 s per query for the 200 config-2 queries, 2000 s for the one config-1 query: "
@@ -126,6 +128,8 @@ def parse():
                     help="only run one device-to-device copy of BYTES bytes (the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings) and exit")
     args = ap.parse_args()
+    if args.cap is None:
+        args.cap = 100_000 if args.workload == "chunglu100k-dyn" else 10_000_000
     if args.uncapped_budget_ms < 0:
         args.uncapped_budget_ms = 2_000_000 if args.workload == "chunglu10k" else 10_000
     return args
@@ -171,6 +175,7 @@ def setup_chunglu(args):
     g = chung_lu(10_000, 2.5, 7.4, seed=1) if c10k else chung_lu(100_000, 2.3, 16.0, seed=2)
     labelled = args.workload == "chunglu100k-labelled"
     aux = args.workload == "chunglu100k-aux"  # tables + explore_Intersection_All<true,...>, per query
+    dyn = args.workload == "chunglu100k-dyn"  # computeWeightArray + explore_Intersection_DynamicOrder_DAF
     if labelled:  # the same CSR with skewed labels; the queries' walks carry them
         g = with_labels(g, args.labels, seed=23, skew=1.0)
     if c10k:
@@ -187,11 +192,22 @@ def setup_chunglu(args):
         rec["nlf_ms"], rec["gql_ms"], rec["cand_ms"] = ctx.filter_ms()
         rec["filter_ms"] = rec["nlf_ms"] + rec["gql_ms"] + rec["cand_ms"]
         t1 = time.perf_counter()
-        if aux:
+        if dyn:
             live = [i for i, good in enumerate(ok) if good]
-            for i in live:
-                cans[i].build_tables()  # FilterParamConstruct's tables (inc/table.h:7-137)
-            # RI orders computed by the library; one launch over every query
+            # start orders (BFS from DAFStartVertex), DAF weights and missing tables are made
+            # by the library inside the call; one launch over every query
+            batch = sgm.enumerate_batch([cans[i] for i in live], None, L.SGM_ENUM_DYN_DAF, cap=args.cap,
+                                        blocks=args.blocks)
+            results = [None] * len(queries)
+            for i, r in zip(live, batch):
+                results[i] = r
+            if batch:
+                rec["enum_ms"] += batch[0].time
+                rec["enum_launches"] += 1
+        elif aux:
+            live = [i for i, good in enumerate(ok) if good]
+            # FilterParamConstruct's tables (inc/table.h:7-137) of every query are built in one
+            # batch inside the call, RI orders too; then one launch over every query
             batch = sgm.enumerate_batch([cans[i] for i in live], None, L.SGM_ENUM_AUX_TABLES_FS, cap=args.cap,
                                         blocks=args.blocks)
             results = [None] * len(queries)
@@ -234,10 +250,15 @@ def setup_chunglu(args):
         name = name.replace("seed2__", f"seed2_labels{args.labels}_skew1_seed23__")
     if aux:
         name = name.replace("gql_ri_cache_filter_fs", "gql_tables_ri_aux_fs")
+    if dyn:
+        name = name.replace("gql_ri_cache_filter_fs", "gql_daf_weights_dynamic_order")
     config = {"workload": name,
-              "queries_per_step": len(queries), "cap_per_query": args.cap, "order": "RI",
+              "queries_per_step": len(queries), "cap_per_query": args.cap,
+              "order": "DAF dynamic order (BFS from DAFStartVertex, computeWeightArray weights)" if dyn else "RI",
               "filter": "labelled NLF+GQL" if labelled else "NLF+GQL",
-              "enumeration": "aux tables + failing sets" if aux else "cache+filter+failing-sets",
+              "enumeration": ("aux tables + failing sets" if aux else
+                              "dynamic-order search (explore_Intersection_DynamicOrder_DAF)" if dyn else
+                              "cache+filter+failing-sets"),
               "launch": "per-query" if args.per_query else "one filter batch + one persistent enumeration launch"}
 
     def uncapped(budget_ms):
@@ -267,7 +288,8 @@ def setup_chunglu(args):
         sgm.filter_batch(cans, L.SGM_FILTER_GQL)
         return [O.cand_digest(c.candidates) for c in cans]
 
-    extra = {} if (labelled or aux) else {"uncapped": uncapped, "digests": gpu_digests}
+    extra = ({} if (labelled or aux) else {"digests": gpu_digests} if dyn else
+             {"uncapped": uncapped, "digests": gpu_digests})
     return g, queries, step, config, extra
 
 
@@ -655,7 +677,7 @@ def main():
     # the merge model (every intersection reads |A|+|B| and writes |A n B|, every candidate
     # 4 B + 1 bit), kept for comparison: not a bound for bitmap or binary-search intersections
     merge_bytes = sum(4 * (r["elems_read"] + r["elems_out"]) + 4.125 * r["elems_scanned"] for r in recs)
-    enum_kernel = "k_enum_aux" if args.workload == "chunglu100k-aux" else "k_enum_cache"
+    enum_kernel = {"chunglu100k-aux": "k_enum_aux", "chunglu100k-dyn": "k_enum_dyn"}.get(args.workload, "k_enum_cache")
     achieved = algo_bytes / (enum_ms / 1e3) / 1e9 if enum_ms > 0 else 0.0
     traffic = measured = None
     traffic_note = "no measured traffic for this workload"
@@ -694,7 +716,8 @@ def main():
     gpu_digests = extra["digests"]() if "digests" in extra and args.cpu_seconds > 0 else None
     if args.cpu_seconds > 0:
         cpu = cpu_baseline(g, queries, counts0, config["cap_per_query"], args.cpu_seconds,
-                           "aux_fs" if args.workload == "chunglu100k-aux" else "cache", gpu_digests)
+                           {"chunglu100k-aux": "aux_fs", "chunglu100k-dyn": "dyn_daf"}.get(args.workload, "cache"),
+                           gpu_digests)
 
     out = {
         "metric": METRIC,
@@ -745,10 +768,10 @@ def source_stamp():
 
 
 def _cpu_worker(job):
-    """One CPU worker: GQL + RI + cache/filter/fs (or aux-table/fs) enumeration of its queries, one after the
-    other on one core, until its wall-clock budget ends (the query running then is cut off
-    by the oracle's time limit and left out).  Returns (query, count, enumeration thread-ms,
-    candidate digest) per finished query."""
+    """One CPU worker: GQL + RI + cache/filter/fs (or aux-table/fs, or the DAF dynamic-order
+    search) enumeration of its queries, one after the other on one core, until its wall-clock
+    budget ends (the query running then is cut off by the oracle's time limit and left out).
+    Returns (query, count, enumeration thread-ms, candidate digest) per finished query."""
     g, items, cap, budget_s, method = job
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -765,6 +788,8 @@ def _cpu_worker(job):
         if ok:
             if method == "aux_fs":
                 st = O.enum_all(g, q, c, order, cap=cap, time_limit_ms=max(1.0, 1e3 * left), fs=True)
+            elif method == "dyn_daf":  # start order and weights are computed before the clock starts
+                st = O.enum_dynamic(g, q, c, "DAF", cap=cap, time_limit_ms=max(1.0, 1e3 * left))
             else:
                 st = O.enum_cache(g, q, c, order, cap=cap, time_limit_ms=max(1.0, 1e3 * left))
             if st["timed_out"]:
@@ -809,8 +834,10 @@ def cpu_baseline(g, queries, gpu_counts, cap, budget_s, method="cache", gpu_dige
     print(f"[bench] cpu baseline: {done} queries on {P} cores in {wall:.1f}s, parity={parity}, digests={dig}",
           flush=True)
     return {"value": emb / wall if wall > 0 else 0.0, "unit": "embeddings/s", "cores": P, "kind": "port",
-            "sample": f"{done} of the {len(queries)} queries (GQL + RI + "
-                      f"{'aux-table/fs' if method == 'aux_fs' else 'cache/filter/fs'} enumeration, cap {cap}) "
+            "sample": f"{done} of the {len(queries)} queries (GQL + "
+                      + {"aux_fs": "RI + aux-table/fs", "dyn_daf": "DAF weights + dynamic-order"}.get(
+                          method, "RI + cache/filter/fs")
+                      + f" enumeration, cap {cap}) "
                       f"on {P} worker processes, one query at a time each, {wall:.1f} s wall",
             "intersection": "AVX2" if O.isect_is_avx2() else "scalar merge",
             "queries_per_s": done / wall if wall > 0 else 0.0,

@@ -128,6 +128,12 @@ int orc_isect_test(const uint32_t *a, uint32_t na, const uint32_t *b, uint32_t n
     return 1;
 }
 
+/* The same intersection for the other oracle files (the dynamic-order searches). */
+uint32_t orc_isect(const uint32_t *a, uint32_t na, const uint32_t *b, uint32_t nb, uint32_t *out)
+{
+    return isect(a, na, b, nb, out);
+}
+
 /* 1 when isect is the AVX2 intersection. */
 int orc_isect_avx2(void)
 {

@@ -25,6 +25,9 @@
 
 #define D_MAXQ 32
 
+/* the reference's Intersection as sgm_oracle.c builds it (AVX2 under -mavx2, else a merge) */
+uint32_t orc_isect(const uint32_t *a, uint32_t na, const uint32_t *b, uint32_t nb, uint32_t *out);
+
 typedef struct {
     uint64_t count;
     uint64_t intersections;
@@ -181,13 +184,9 @@ static void candidates_for(dctx *c, uint32_t u)
     for (uint32_t i = 1; i < c->np[u]; i++) {
         uint32_t p = c->prev[u][i], rr = c->emb_idx[p];
         const uint32_t *ro = c->tidx[p][u] + c->toff[p][u][rr];
-        uint32_t rk = c->toff[p][u][rr + 1] - c->toff[p][u][rr], m = 0, a = 0, b = 0;
+        uint32_t rk = c->toff[p][u][rr + 1] - c->toff[p][u][rr];
         c->st->intersections++;
-        while (a < rk && b < k) {
-            if (ro[a] < c->valid[u][b]) a++;
-            else if (ro[a] > c->valid[u][b]) b++;
-            else { c->tmp[m++] = ro[a]; a++; b++; }
-        }
+        const uint32_t m = orc_isect(ro, rk, c->valid[u], k, c->tmp);
         memcpy(c->valid[u], c->tmp, sizeof(uint32_t) * m);
         k = m;
     }

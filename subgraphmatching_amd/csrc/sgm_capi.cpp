@@ -97,6 +97,7 @@ void sgm_ctx_destroy(sgm_ctx* c)
     dfree(c->d_auxplans);
     dfree(c->d_afirst);
     dfree(c->d_dynplans);
+    dfree(c->d_wdesc);
     dfree(c->d_astats);
     dfree(c->d_astop);
     dfree(c->d_work64);
@@ -372,10 +373,7 @@ void sgm_query_destroy(sgm_query* q)
     free_tables(q);
     dfree(q->d_toff);
     dfree(q->d_tidx);
-    dfree(q->d_tv);
     dfree(q->d_tscr);
-    dfree(q->d_tdesc);
-    dfree(q->d_tedesc);
     dfree(q->d_weight);
     dfree(q->d_vmask);
     dfree(q->d_removed);
@@ -845,10 +843,10 @@ extern "C++" int build_tables_many(sgm_query* const* qs, uint32_t nq)
         HIPCHK(c, hipStreamSynchronize(c->stream));
         return SGM_OK;
     }
-    // descriptors: [sources NS][scans M][fills M] on the device; the same plus the totals and
+    // descriptors: [sources NS][scans M][fills NS] on the device; the same plus the totals and
     // maxima read back (2M words) in pinned staging
     const size_t src_b = stage_align(sizeof(sgm_tabsrc) * NS), scan_b = stage_align(sizeof(sgm_scandesc) * M);
-    const size_t fill_b = stage_align(sizeof(sgm_tabdesc) * M), dev_b = src_b + scan_b + fill_b;
+    const size_t fill_b = stage_align(sizeof(sgm_tabfill) * NS), dev_b = src_b + scan_b + fill_b;
     if (c->tb_bytes < dev_b) {
         dfree(c->d_tb);
         c->d_tb = nullptr;
@@ -871,10 +869,11 @@ extern "C++" int build_tables_many(sgm_query* const* qs, uint32_t nq)
     }
     sgm_tabsrc* td = reinterpret_cast<sgm_tabsrc*>(st);
     sgm_scandesc* sd = reinterpret_cast<sgm_scandesc*>(st + src_b);
-    sgm_tabdesc* te = reinterpret_cast<sgm_tabdesc*>(st + src_b + scan_b);
+    sgm_tabfill* tf = reinterpret_cast<sgm_tabfill*>(st + src_b + scan_b);
     uint32_t* h = reinterpret_cast<uint32_t*>(st + dev_b);  // totals[M], maxima[M]
     uint8_t* dev = reinterpret_cast<uint8_t*>(c->d_tb);
     uint32_t ns = 0;
+    std::vector<uint32_t> fill_of(M);  // each edge's fill descriptor (its source's)
     for (uint32_t k = 0; k < M; k++) {
         const Edge& E = edges[k];
         sgm_query* q = qs[E.q];
@@ -883,22 +882,35 @@ extern "C++" int build_tables_many(sgm_query* const* qs, uint32_t nq)
         t.d_off = q->d_toff + E.o;
         // edges out of one vertex are consecutive: one count descriptor per run of them
         if (k == 0 || edges[k - 1].q != E.q || edges[k - 1].a != E.a) {
-            sgm_tabsrc& S = td[ns++];
+            sgm_tabsrc& S = td[ns];
             S = sgm_tabsrc{};
             S.rows = q->d_list + q->cand_off[E.a];
             S.vmask = q->d_vmask;
             S.nrows = rows;
+            sgm_tabfill& F = tf[ns];
+            F = sgm_tabfill{};
+            F.rows = S.rows;
+            F.vmask = S.vmask;
+            F.nrows = rows;
+            ns++;
         }
         sgm_tabsrc& S = td[ns - 1];
         const uint32_t j = S.k++;
         S.b[j] = E.b;
         S.rowcnt[j] = q->d_tscr + E.r;
         sd[k] = sgm_scandesc{q->d_tscr + E.r, t.d_off, c->d_ttot + k, c->d_ttot + M + k, rows, 0};
-        te[k] = sgm_tabdesc{S.rows, rows, E.b, q->d_bm + (size_t)E.b * W, q->d_rank + (size_t)E.b * W, t.d_off,
-                            nullptr, nullptr, q->d_vmask};
+        sgm_tabfill& F = tf[ns - 1];
+        F.k = S.k;
+        F.b[j] = E.b;
+        F.rowoff[j] = t.d_off;
+        F.bm[j] = q->d_bm + (size_t)E.b * W;
+        F.rank[j] = q->d_rank + (size_t)E.b * W;
+        fill_of[k] = ns - 1;
     }
     HIPCHK(c, hipMemcpyAsync(dev, st, src_b + scan_b, hipMemcpyHostToDevice, c->stream));
-    sgm::launch_table_count(c->stream, g->d_off, g->d_adj, reinterpret_cast<const sgm_tabsrc*>(dev), ns, max_rows);
+    uint32_t kmax = 0;
+    for (uint32_t i = 0; i < ns; i++) kmax = std::max(kmax, td[i].k);
+    sgm::launch_table_count(c->stream, g->d_off, g->d_adj, reinterpret_cast<const sgm_tabsrc*>(dev), ns, max_rows, kmax);
     sgm::launch_scan_segments(c->stream, reinterpret_cast<const sgm_scandesc*>(dev + src_b), M);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(h, c->d_ttot, sizeof(uint32_t) * 2 * M, hipMemcpyDeviceToHost, c->stream));
@@ -914,7 +926,6 @@ extern "C++" int build_tables_many(sgm_query* const* qs, uint32_t nq)
         for (uint32_t k = first[i]; k < first[i + 1]; k++) ent += h[k];
         if (first[i] == first[i + 1]) continue;
         hipError_t e = grow_arena(&q->d_tidx, q->tidx_cap, std::max<size_t>(ent, 1));
-        if (e == hipSuccess) e = grow_arena(&q->d_tv, q->tv_cap, std::max<size_t>(ent, 1));
         if (e != hipSuccess) {
             for (uint32_t j = 0; j < nq; j++) free_tables(qs[j]);
             return set_err(c, SGM_ENOMEM, "tables of %llu entries", (unsigned long long)ent);
@@ -925,16 +936,18 @@ extern "C++" int build_tables_many(sgm_query* const* qs, uint32_t nq)
             t.e_cnt = h[k];
             t.max_deg = h[M + k];
             t.d_idx = q->d_tidx + base;
-            t.d_v = q->d_tv + base;
             base += h[k];
-            te[k].eidx = t.d_idx;
-            te[k].ev = t.d_v;
+            sgm_tabfill& F = tf[fill_of[k]];
+            for (uint32_t j = 0; j < F.k; j++)
+                if (F.b[j] == edges[k].b) {
+                    F.eidx[j] = t.d_idx;
+                }
             t.built = true;
         }
     }
-    HIPCHK(c, hipMemcpyAsync(dev + src_b + scan_b, te, sizeof(sgm_tabdesc) * M, hipMemcpyHostToDevice, c->stream));
-    sgm::launch_table_fill(c->stream, g->d_off, g->d_adj, reinterpret_cast<const sgm_tabdesc*>(dev + src_b + scan_b), M,
-                           max_rows);
+    HIPCHK(c, hipMemcpyAsync(dev + src_b + scan_b, tf, sizeof(sgm_tabfill) * ns, hipMemcpyHostToDevice, c->stream));
+    sgm::launch_table_fill(c->stream, g->d_off, g->d_adj, reinterpret_cast<const sgm_tabfill*>(dev + src_b + scan_b), ns,
+                           max_rows, kmax);
     HIPCHK(c, hipGetLastError());
     se = hipStreamSynchronize(c->stream);
     if (se != hipSuccess) {
@@ -968,7 +981,19 @@ int sgm_table_download(sgm_query* q, uint32_t a, uint32_t b, uint32_t* offsets, 
     if (!t.built) return set_err(c, SGM_ESTATE, "no table for (%u,%u)", a, b);
     if (offsets) HIPCHK(c, hipMemcpy(offsets, t.d_off, sizeof(uint32_t) * (q->counts[a] + 1), hipMemcpyDeviceToHost));
     if (idx && t.e_cnt) HIPCHK(c, hipMemcpy(idx, t.d_idx, sizeof(uint32_t) * t.e_cnt, hipMemcpyDeviceToHost));
-    if (vtx && t.e_cnt) HIPCHK(c, hipMemcpy(vtx, t.d_v, sizeof(uint32_t) * t.e_cnt, hipMemcpyDeviceToHost));
+    if (vtx && t.e_cnt) {
+        // _edge_v is the candidate at each position, C(b)[_edge[i]]: read from C(b) here
+        // rather than kept on the device beside the positions
+        std::vector<uint32_t> pos, cb(q->counts[b]);
+        const uint32_t* p = idx;
+        if (!p) {
+            pos.resize(t.e_cnt);
+            HIPCHK(c, hipMemcpy(pos.data(), t.d_idx, sizeof(uint32_t) * t.e_cnt, hipMemcpyDeviceToHost));
+            p = pos.data();
+        }
+        HIPCHK(c, hipMemcpy(cb.data(), q->d_list + q->cand_off[b], sizeof(uint32_t) * cb.size(), hipMemcpyDeviceToHost));
+        for (uint64_t i = 0; i < t.e_cnt; i++) vtx[i] = cb[p[i]];
+    }
     return SGM_OK;
 }
 

@@ -1,8 +1,8 @@
 // The dynamic-order searches (explore_Intersection_DynamicOrder_DAF / _VEQ,
 // inc/explore.h:1674-2162) and the DAF weight array they rank with (computeWeightArray,
-// inc/order.h:978-1031): host side.  The weights are computed on the device, one row pass
-// per tree child (k_weight_rows) from the last vertex of the start order back, and stay
-// resident with the query for the enumeration; the search itself is k_enum_dyn.
+// inc/order.h:978-1031): host side.  The weights are computed on the device (k_weight_step,
+// one launch per reverse-BFS position for a whole batch) and stay resident with the query for
+// the enumeration; the search itself is k_enum_dyn.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -68,37 +68,63 @@ bool same_order(const sgm_query* q, const uint32_t* order)
     return q->weights_ok && std::equal(order, order + q->q.n, q->weight_order);
 }
 
-// computeWeightArray (inc/order.h:997-1029) on the device for the start order `order`: from
-// its last vertex back, a vertex's weight per candidate is the minimum, over its tree
-// children whose only previous neighbour it is, of the child weights summed along the
-// candidate's table row; 1 everywhere when it has no such child; weights start at UINT32_MAX
-// (src/order_ablations.cpp:134-138).  The BFS-tree children of x with one previous neighbour
-// are exactly the following neighbours of x whose one previous neighbour is x, so the rule
-// reads the order alone.
-int compute_weights(sgm_query* q, const uint32_t* order)
+// computeWeightArray (inc/order.h:997-1029) on the device, for many queries and their start
+// orders at once: from the last vertex of each order back, a vertex's weight per candidate is
+// the minimum, over its tree children whose only previous neighbour it is, of the child
+// weights summed along the candidate's table row; 1 everywhere when it has no such child;
+// weights start at UINT32_MAX (src/order_ablations.cpp:134-138).  The BFS-tree children of x
+// with one previous neighbour are exactly the following neighbours of x whose one previous
+// neighbour is x, so the rule reads the order alone.  Launches: one fill, then one per
+// reverse position, each over every query's (vertex, child) passes at that position.  The
+// queries' tables must be built.
+int compute_weights_many(sgm_query* const* qs, const uint32_t* const* orders, uint32_t nq)
 {
-    sgm_ctx* c = q->g->ctx;
-    const uint32_t n = q->q.n;
-    int rc;
-    if ((rc = ensure_tables(q)) || (rc = ensure_weight_buffer(q))) return rc;
-    std::vector<std::vector<uint32_t>> prev, fol;
-    neighbours_in_order(q->q, order, prev, fol);
-    sgm::launch_fill_u32(c->stream, q->d_weight, q->cand_off[n], 0xFFFFFFFFu);
-    for (int i = (int)n - 1; i >= 0; i--) {
-        const uint32_t x = order[i];
-        bool one = true;
-        for (uint32_t ch : fol[x]) {
-            if (prev[ch].size() != 1) continue;
-            one = false;
-            const sgm_table& t = q->tables[(size_t)x * n + ch];
-            sgm::launch_weight_rows(c->stream, t.d_off, t.d_idx, q->counts[x], q->d_weight + q->cand_off[ch],
-                                    q->d_weight + q->cand_off[x]);
+    if (!nq) return SGM_OK;
+    sgm_ctx* c = qs[0]->g->ctx;
+    std::vector<std::vector<sgm_wdesc>> by_step(SGM_MAXQ + 1);
+    std::vector<uint32_t> max_rows(SGM_MAXQ + 1, 0);
+    auto add = [&](uint32_t k, const sgm_wdesc& d) {
+        by_step[k].push_back(d);
+        max_rows[k] = std::max(max_rows[k], d.rows);
+    };
+    for (uint32_t i = 0; i < nq; i++) {
+        sgm_query* q = qs[i];
+        const uint32_t* order = orders[i];
+        const uint32_t n = q->q.n;
+        if (const int rc = ensure_weight_buffer(q)) return rc;
+        add(0, sgm_wdesc{nullptr, nullptr, nullptr, q->d_weight, q->cand_off[n], 0xFFFFFFFFu});
+        std::vector<std::vector<uint32_t>> prev, fol;
+        neighbours_in_order(q->q, order, prev, fol);
+        for (uint32_t k = 0; k < n; k++) {
+            const uint32_t x = order[n - 1 - k];
+            uint32_t* out = q->d_weight + q->cand_off[x];
+            bool one = true;
+            for (uint32_t ch : fol[x]) {
+                if (prev[ch].size() != 1) continue;
+                one = false;
+                const sgm_table& t = q->tables[(size_t)x * n + ch];
+                add(k + 1, sgm_wdesc{t.d_off, t.d_idx, q->d_weight + q->cand_off[ch], out, q->counts[x], 0u});
+            }
+            if (one) add(k + 1, sgm_wdesc{nullptr, nullptr, nullptr, out, q->counts[x], 1u});
         }
-        if (one) sgm::launch_fill_u32(c->stream, q->d_weight + q->cand_off[x], q->counts[x], 1u);
     }
+    std::vector<sgm_wdesc> all;
+    std::vector<uint32_t> first(by_step.size() + 1, 0);
+    for (size_t k = 0; k < by_step.size(); k++) {
+        first[k] = (uint32_t)all.size();
+        all.insert(all.end(), by_step[k].begin(), by_step[k].end());
+    }
+    first[by_step.size()] = (uint32_t)all.size();
+    if (const int rc = grow(c, &c->d_wdesc, c->wdesc_cap, (uint32_t)all.size())) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->d_wdesc, all.data(), sizeof(sgm_wdesc) * all.size(), hipMemcpyHostToDevice, c->stream));
+    for (size_t k = 0; k < by_step.size(); k++)
+        sgm::launch_weight_step(c->stream, c->d_wdesc + first[k], first[k + 1] - first[k], max_rows[k]);
     HIPCHK(c, hipGetLastError());
-    std::copy(order, order + n, q->weight_order);
-    q->weights_ok = true;
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // `all` is the copy's source
+    for (uint32_t i = 0; i < nq; i++) {
+        std::copy(orders[i], orders[i] + qs[i]->q.n, qs[i]->weight_order);
+        qs[i]->weights_ok = true;
+    }
     return SGM_OK;
 }
 
@@ -140,6 +166,16 @@ extern "C++" int enumerate_dyn_batch(sgm_query* const* qs, const uint32_t* const
         const int trc = build_tables_many(missing.data(), (uint32_t)missing.size());
         if (trc) return trc;
     }
+    if (daf) {  // the weights of every query without resident ones for its order, in one pass
+        std::vector<sgm_query*> wq;
+        std::vector<const uint32_t*> wo;
+        for (uint32_t i = 0; i < nq; i++)
+            if (!qs[i]->empty && !same_order(qs[i], orders[i])) {
+                wq.push_back(qs[i]);
+                wo.push_back(orders[i]);
+            }
+        if (const int rc = compute_weights_many(wq.data(), wo.data(), (uint32_t)wq.size())) return rc;
+    }
     std::vector<sgm_dynplan_dev> plans(nq);
     uint64_t words_need = 1024;  // stack words that hold every intersection along one search path
     for (uint32_t i = 0; i < nq; i++) {
@@ -147,10 +183,6 @@ extern "C++" int enumerate_dyn_batch(sgm_query* const* qs, const uint32_t* const
         const uint32_t* order = orders[i];
         const uint32_t n = q->q.n;
         if (q->empty) continue;
-        if (daf && !same_order(q, order)) {
-            const int rc = compute_weights(q, order);
-            if (rc) return rc;
-        }
         std::vector<std::vector<uint32_t>> prev, fol;
         neighbours_in_order(q->q, order, prev, fol);
         sgm_dynplan_dev& P = plans[i];
@@ -279,7 +311,8 @@ int sgm_daf_weights(sgm_query* q, uint32_t* bfs_order, uint32_t* const* weights)
     HIPCHK(c, hipSetDevice(c->device));
     if ((rc = daf_bfs_order(q, bfs_order))) return rc;
     if (q->empty) return set_err(c, SGM_EEMPTY, "computeWeightArray: some candidate set is empty");
-    if ((rc = compute_weights(q, bfs_order))) return rc;
+    const uint32_t* bo = bfs_order;
+    if ((rc = ensure_tables(q)) || (rc = compute_weights_many(&q, &bo, 1))) return rc;
     if (weights) {
         for (uint32_t u = 0; u < q->q.n; u++)
             if (q->counts[u] && !weights[u]) return set_err(c, SGM_EINVAL, "weights[%u] is null", u);

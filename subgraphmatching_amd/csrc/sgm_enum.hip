@@ -1547,20 +1547,6 @@ struct WaveDyn {
     uint32_t hlen[SGM_MAXQ + 1];
 };
 
-// later reads of LDS words other lanes wrote see the writes
-__device__ __forceinline__ void wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
-
-__device__ __forceinline__ uint32_t wave_sum32(uint32_t x)
-{
-#pragma unroll
-    for (int s = 32; s > 0; s >>= 1) x += (uint32_t)__shfl_xor((int)x, s);
-    return x;
-}
-
 // Rank key of a queue entry: larger = lower priority (extendable_vertex_compare,
 // inc/explore.h:1675-1686: degree 1 below the rest, then the larger weight below)
 __device__ __forceinline__ uint64_t dyn_key(uint64_t e) { return e & 0x1FFFFFFFFull; }

@@ -69,18 +69,32 @@ struct sgm_dynplan_dev {
     const uint32_t* weight[SGM_MAXQ];
 };
 
-// One directed query edge (a, b) of the aux-table fill pass (inc/table.h:7-137): rows = C(a); entries go to
-// eidx / ev at the scanned row offsets rowoff, positions in C(b) from b's bitmap and ranks; vmask = the
-// query's membership words (descriptors of many queries share one launch).
-struct sgm_tabdesc {
+// The directed query edges (a, b_j), j < k, out of one query vertex a in the fill pass of an
+// aux-table build (inc/table.h:7-137): rows = C(a); edge j's entries (positions in C(b_j),
+// from b_j's bitmap and ranks) go to eidx[j] at its scanned row offsets rowoff[j]; vmask =
+// the query's membership words (descriptors of many queries share one launch).  The
+// reference's _edge_v (the candidate at each position) is not stored: C(b_j)[position].
+struct sgm_tabfill {
     const uint32_t* rows;
-    uint32_t nrows, b;
-    const uint64_t* bm_b;
-    const uint32_t* rank_b;
-    const uint32_t* rowoff;
-    uint32_t* eidx;
-    uint32_t* ev;
     const uint32_t* vmask;
+    uint32_t nrows, k;
+    uint32_t b[SGM_MAXQ];
+    const uint32_t* rowoff[SGM_MAXQ];
+    uint32_t* eidx[SGM_MAXQ];
+    const uint64_t* bm[SGM_MAXQ];
+    const uint32_t* rank[SGM_MAXQ];
+};
+
+// One row pass of computeWeightArray (inc/order.h:997-1029) in a batched launch: rows of the
+// table (x -> child) given by toff / tidx, child weights `in`, x's weights `out`; in == null
+// fills out[0, rows) with `fill` (the start value, or 1 for a vertex without a one-parent
+// child).
+struct sgm_wdesc {
+    const uint32_t* toff;
+    const uint32_t* tidx;
+    const uint32_t* in;
+    uint32_t* out;
+    uint32_t rows, fill;
 };
 
 // One exclusive scan of the aux-table build: in[0, n) into out[0, n] (out[n] = total), the

@@ -55,9 +55,9 @@ void launch_compact(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V)
 // edge), the row counts grouped by source vertex, the entries (one edge per block row)
 void launch_scan_segments(hipStream_t s, const sgm_scandesc* ds, uint32_t m);
 void launch_table_count(hipStream_t s, const uint32_t* off, const uint32_t* adj, const sgm_tabsrc* ds, uint32_t nsrc,
-                        uint32_t max_rows);
-void launch_table_fill(hipStream_t s, const uint32_t* off, const uint32_t* adj, const sgm_tabdesc* ds, uint32_t m,
-                       uint32_t max_rows);
+                        uint32_t max_rows, uint32_t kmax);
+void launch_table_fill(hipStream_t s, const uint32_t* off, const uint32_t* adj, const sgm_tabfill* ds, uint32_t nsrc,
+                       uint32_t max_rows, uint32_t kmax);
 // BFS-tree filters (CFL / CECI / DAF): step `step` of every query's refinement program
 // (SGM_RPROG_WORDS words per query, see k_refine), and the copy of the NLF words into vmr
 void launch_refine(hipStream_t s, const uint32_t* off, const uint32_t* adj, uint32_t V, const sgm_qdesc* qs,
@@ -68,9 +68,9 @@ void launch_copy_vmask(hipStream_t s, uint32_t V, const sgm_qdesc* qs, uint32_t 
 void launch_row_gather(hipStream_t s, const uint32_t* toff, const uint32_t* tidx, uint32_t rows,
                        const unsigned long long* in, unsigned long long* out, int mul, unsigned long long* total);
 void launch_fill_u64(hipStream_t s, unsigned long long* p, size_t n, unsigned long long val);
-// DAF weights (computeWeightArray): out[j] = min(out[j], sum of in[idx] over row j), u32 wrapping
-void launch_weight_rows(hipStream_t s, const uint32_t* toff, const uint32_t* tidx, uint32_t rows, const uint32_t* in,
-                        uint32_t* out);
+// DAF weights (computeWeightArray): one reverse-BFS step of many queries, m descriptors
+// (out[j] = min(out[j], u32 sum of in[idx] over row j), or a fill), max_rows = the largest
+void launch_weight_step(hipStream_t s, const sgm_wdesc* ds, uint32_t m, uint32_t max_rows);
 // sgm_enum.hip
 size_t enum_smem_bytes(uint32_t max_ops);
 // dev_stop: the device stop word every wave polls; host_stop: the host's request (pinned host

@@ -754,66 +754,135 @@ __global__ void __launch_bounds__(256) k_compact(const sgm_cdesc* __restrict__ c
 // A count pass per row, an exclusive scan into row offsets, then a fill pass (positions are
 // ranks in b's bitmap, so rows come out sorted because adjacency lists are).  One wave per
 // row, coalesced over N(v); every directed edge of a query in one launch per pass.
-__device__ __forceinline__ uint32_t cand_rank(const uint64_t* __restrict__ bm, const uint32_t* __restrict__ rank,
-                                              uint32_t w)
+__device__ __forceinline__ uint32_t cand_rank(const glb<uint64_t>* __restrict__ bm,
+                                              const glb<uint32_t>* __restrict__ rank, uint32_t w)
 {
     const uint32_t word = w >> 6, bit = w & 63u;
-    const uint64_t low = bit ? (bm[word] & ((1ull << bit) - 1ull)) : 0ull;
-    return rank[word] + (uint32_t)__popcll(low);
+    return rank[word] + (uint32_t)__popcll(bm[word] & ((1ull << bit) - 1ull));  // both loads in flight together
 }
 
-// Row counts of every directed edge out of a (blockIdx.y = a's descriptor): one wave per
-// row of C(a) reads N(v) and the membership words once for all of a's neighbours b_j;
-// lane j keeps b_j's count.
+// The aux-table passes (inc/table.h:7-137) walk the candidate rows of a query vertex a for
+// all of a's directed edges (a, b_j) at once, reading N(v) and its membership words once.
+// A wave takes 64 rows (one per lane) and walks their concatenated neighbour lists 64
+// entries at a time: rows average a few dozen entries, so one wave per row would leave most
+// lanes idle behind a chain of dependent loads.  Per (row, edge) state lives in LDS: slot
+// (lane, j) at tl[(wave * 64 + lane) * kmax + j].  In each 64-entry step the entries of one
+// row are a contiguous lane range; its last lane carries the row's hits of the step into
+// the slot.
+struct TabStep {
+    uint32_t own;    // lane whose row holds this entry (64 when none)
+    uint32_t pos;    // entry index within that row
+    bool act;        // the lane holds an entry
+    bool last;       // the last entry of its row in this step
+    uint64_t below;  // lanes of the same row before this one in this step
+};
+
+__device__ __forceinline__ TabStep tab_step(uint32_t incl, uint32_t len, uint32_t total, uint32_t b0, uint32_t lane)
+{
+    TabStep r;
+    const uint32_t t = b0 + lane;
+    r.own = wave_seg_owner(incl, t);
+    const uint32_t sj = (uint32_t)__shfl((int)(incl - len), (int)(r.own & 63u));
+    const uint32_t ej = (uint32_t)__shfl((int)incl, (int)(r.own & 63u));
+    r.act = t < total;
+    r.pos = t - sj;
+    r.last = r.act && (t + 1 == ej || lane == 63);
+    const uint32_t segstart = r.act ? (sj > b0 ? sj - b0 : 0u) : lane;
+    r.below = ((1ull << lane) - 1ull) & ~((1ull << segstart) - 1ull);
+    return r;
+}
+
+// Row counts of every directed edge out of a (blockIdx.y = a's descriptor).  Lane j holds
+// edge j's target bit and output array, read with readlane in the loops.
 __global__ void __launch_bounds__(256) k_table_count_src(const uint32_t* __restrict__ off,
                                                          const uint32_t* __restrict__ adj,
-                                                         const sgm_tabsrc* __restrict__ ds)
+                                                         const sgm_tabsrc* __restrict__ ds, uint32_t kmax)
 {
+    extern __shared__ uint32_t tl[];
     const sgm_tabsrc& D = ds[blockIdx.y];
-    const uint32_t* __restrict__ vmask = D.vmask;
-    const uint32_t waves = blockDim.x >> 6, lane = lane_id();
-    const uint32_t* __restrict__ rows = D.rows;
+    const glb<uint32_t>* __restrict__ vmask = gcast(D.vmask);
+    const glb<uint32_t>* __restrict__ rows = gcast(D.rows);
+    const uint32_t waves = blockDim.x >> 6, wave = threadIdx.x >> 6, lane = lane_id();
     const uint32_t nrows = D.nrows, k = D.k;
-    for (uint32_t i = blockIdx.x * waves + (threadIdx.x >> 6); i < nrows; i += gridDim.x * waves) {
-        const uint32_t v = rows[i], o = off[v], dv = off[v + 1] - o;
-        uint32_t cnt = 0;
-        for (uint32_t base = 0; base < dv; base += 64) {
-            const uint32_t jj = base + lane;
-            const uint32_t m = jj < dv ? vmask[adj[o + jj]] : 0u;
+    const uint32_t my_b = lane < k ? D.b[lane] : 0u;
+    const uint64_t my_out = lane < k ? (uint64_t)D.rowcnt[lane] : 0ull;
+    uint32_t* mine = tl + (wave * 64 + lane) * kmax;
+    uint32_t* slots = tl + wave * 64 * kmax;
+    for (uint32_t g = blockIdx.x * waves + wave; (uint64_t)g * 64 < nrows; g += gridDim.x * waves) {
+        const uint32_t r = g * 64 + lane;
+        const bool valid = r < nrows;
+        const uint32_t v = valid ? rows[r] : 0u;
+        const uint32_t o = valid ? off[v] : 0u, len = valid ? off[v + 1] - o : 0u;
+        for (uint32_t j = 0; j < k; j++) mine[j] = 0;
+        wave_sync();
+        const uint32_t incl = wave_incl_scan(len), total = rdlane(incl, 63);
+        for (uint32_t b0 = 0; b0 < total; b0 += 64) {
+            const TabStep st = tab_step(incl, len, total, b0, lane);
+            const uint32_t oj = (uint32_t)__shfl((int)o, (int)(st.own & 63u));
+            const uint32_t m = st.act ? vmask[adj[oj + st.pos]] : 0u;
+            const uint64_t upto = st.below | (1ull << lane);
             for (uint32_t j = 0; j < k; j++) {
-                const uint32_t c = (uint32_t)__popcll(__ballot((m >> D.b[j]) & 1u));
-                if (lane == j) cnt += c;
+                const uint64_t bal = __ballot((m >> rdlane(my_b, j)) & 1u);
+                if (bal && st.last) slots[st.own * kmax + j] += (uint32_t)__popcll(bal & upto);
             }
         }
-        if (lane < k) D.rowcnt[lane][i] = cnt;
+        wave_sync();
+        if (valid)
+            for (uint32_t j = 0; j < k; j++) gcast((uint32_t*)rdlane64(my_out, j))[r] = mine[j];
+        wave_sync();
     }
 }
 
-// Entries of every directed edge (blockIdx.y = edge) at the scanned row offsets.  One
-// edge per block row keeps each wave's bitmap-rank lookups independent of the others'.
-__global__ void __launch_bounds__(256) k_table_fill_edge(const uint32_t* __restrict__ off,
-                                                         const uint32_t* __restrict__ adj,
-                                                         const sgm_tabdesc* __restrict__ ds)
+// Entries of every directed edge out of a (blockIdx.y = a's descriptor) at the scanned row
+// offsets: a hit's position is its row's next free slot plus the row's hits before it in the
+// step, so each edge's output of a step is one nearly contiguous run.  Lane j holds edge j's
+// target bit and arrays.
+__global__ void __launch_bounds__(256) k_table_fill_src(const uint32_t* __restrict__ off,
+                                                        const uint32_t* __restrict__ adj,
+                                                        const sgm_tabfill* __restrict__ ds, uint32_t kmax)
 {
-    const sgm_tabdesc& D = ds[blockIdx.y];
-    const uint32_t* __restrict__ vmask = D.vmask;
-    const uint32_t waves = blockDim.x >> 6, lane = lane_id();
-    const uint32_t nrows = D.nrows, b = D.b;
-    for (uint32_t i = blockIdx.x * waves + (threadIdx.x >> 6); i < nrows; i += gridDim.x * waves) {
-        const uint32_t v = D.rows[i], o = off[v], dv = off[v + 1] - o;
-        uint32_t outp = D.rowoff[i];
-        for (uint32_t base = 0; base < dv; base += 64) {
-            const uint32_t jj = base + lane;
-            const uint32_t w = jj < dv ? adj[o + jj] : 0u;
-            const bool hit = jj < dv && ((vmask[w] >> b) & 1u);
-            const uint64_t bal = __ballot(hit);
-            if (hit) {
-                const uint32_t p = outp + mbcnt64(bal);
-                D.eidx[p] = cand_rank(D.bm_b, D.rank_b, w);
-                D.ev[p] = w;
+    extern __shared__ uint32_t tl[];
+    const sgm_tabfill& D = ds[blockIdx.y];
+    const glb<uint32_t>* __restrict__ vmask = gcast(D.vmask);
+    const glb<uint32_t>* __restrict__ rows = gcast(D.rows);
+    const uint32_t waves = blockDim.x >> 6, wave = threadIdx.x >> 6, lane = lane_id();
+    const uint32_t nrows = D.nrows, k = D.k;
+    const bool lj = lane < k;
+    const uint32_t my_b = lj ? D.b[lane] : 0u;
+    const uint64_t my_off = lj ? (uint64_t)D.rowoff[lane] : 0ull, my_idx = lj ? (uint64_t)D.eidx[lane] : 0ull;
+    const uint64_t my_bm = lj ? (uint64_t)D.bm[lane] : 0ull, my_rank = lj ? (uint64_t)D.rank[lane] : 0ull;
+    uint32_t* mine = tl + (wave * 64 + lane) * kmax;
+    uint32_t* slots = tl + wave * 64 * kmax;
+    for (uint32_t g = blockIdx.x * waves + wave; (uint64_t)g * 64 < nrows; g += gridDim.x * waves) {
+        const uint32_t r = g * 64 + lane;
+        const bool valid = r < nrows;
+        const uint32_t v = valid ? rows[r] : 0u;
+        const uint32_t o = valid ? off[v] : 0u, len = valid ? off[v + 1] - o : 0u;
+        for (uint32_t j = 0; j < k; j++)
+            mine[j] = valid ? gcast((const uint32_t*)rdlane64(my_off, j))[r] : 0u;
+        wave_sync();
+        const uint32_t incl = wave_incl_scan(len), total = rdlane(incl, 63);
+        for (uint32_t b0 = 0; b0 < total; b0 += 64) {
+            const TabStep st = tab_step(incl, len, total, b0, lane);
+            const uint32_t oj = (uint32_t)__shfl((int)o, (int)(st.own & 63u));
+            const uint32_t w = st.act ? adj[oj + st.pos] : 0u;
+            const uint32_t m = st.act ? vmask[w] : 0u;
+            const uint64_t upto = st.below | (1ull << lane);
+            for (uint32_t j = 0; j < k; j++) {
+                const bool hit = (m >> rdlane(my_b, j)) & 1u;
+                const uint64_t bal = __ballot(hit);
+                if (!bal) continue;
+                uint32_t* slot = slots + (st.own & 63u) * kmax + j;
+                if (hit) {
+                    const uint32_t p = *slot + (uint32_t)__popcll(bal & st.below);
+                    const uint32_t rk = cand_rank(gcast((const uint64_t*)rdlane64(my_bm, j)),
+                                                  gcast((const uint32_t*)rdlane64(my_rank, j)), w);
+                    gcast((uint32_t*)rdlane64(my_idx, j))[p] = rk;
+                }
+                if (st.last) *slot += (uint32_t)__popcll(bal & upto);
             }
-            outp += (uint32_t)__popcll(bal);
         }
+        wave_sync();
     }
 }
 
@@ -965,21 +1034,47 @@ __global__ void __launch_bounds__(256) k_row_gather(const uint32_t* __restrict__
     if (lane_id() == 0 && t) atomicAdd(total, t);
 }
 
-// computeWeightArray's step for one tree child (inc/order.h:1003-1022): a row's weight is the
-// u32 sum (wrapping) of the child's weights at the row's positions, kept when below the
-// current one.  Four lanes per row: rows are short and their lengths vary.
-__global__ void __launch_bounds__(256) k_weight_rows(const uint32_t* __restrict__ toff, const uint32_t* __restrict__ tidx,
-                                                     uint32_t rows, const uint32_t* __restrict__ in,
-                                                     uint32_t* __restrict__ out)
+// computeWeightArray's row passes (inc/order.h:1003-1022), one descriptor per (vertex, tree
+// child) of one reverse-BFS step of many queries (blockIdx.y): a row's weight is the u32 sum
+// (wrapping) of the child's weights at the row's positions, kept when below the current one
+// (atomicMin: a vertex's children share the step).  A wave takes 64 rows, one per lane;
+// rows longer than 16 entries (power-law hubs) are summed by the whole wave, one at a time.
+__global__ void __launch_bounds__(256) k_weight_step(const sgm_wdesc* __restrict__ ds)
 {
-    const uint32_t sub = threadIdx.x & 3u;
-    for (uint32_t j = (blockIdx.x * blockDim.x + threadIdx.x) >> 2; j < rows; j += (gridDim.x * blockDim.x) >> 2) {
-        const uint32_t b = toff[j], e = toff[j + 1];
+    const sgm_wdesc& D = ds[blockIdx.y];
+    const uint32_t rows = D.rows, lane = lane_id();
+    const uint32_t* __restrict__ toff = D.toff;
+    const uint32_t* __restrict__ tidx = D.tidx;
+    const uint32_t* __restrict__ in = D.in;
+    uint32_t* __restrict__ out = D.out;
+    const uint32_t waves = blockDim.x >> 6;
+    for (uint32_t g = blockIdx.x * waves + (threadIdx.x >> 6); (uint64_t)g * 64 < rows; g += gridDim.x * waves) {
+        const uint32_t j = g * 64 + lane;
+        const bool valid = j < rows;
+        if (!in) {
+            if (valid) out[j] = D.fill;
+            continue;
+        }
+        uint32_t b = 0, e = 0;
+        if (valid) {
+            b = toff[j];
+            e = toff[j + 1];
+        }
+        const bool lng = valid && e - b > 16;
         uint32_t s = 0;
-        for (uint32_t k = b + sub; k < e; k += 4) s += in[tidx[k]];
-        s += (uint32_t)__shfl_xor((int)s, 1);
-        s += (uint32_t)__shfl_xor((int)s, 2);
-        if (sub == 0 && s < out[j]) out[j] = s;
+        if (valid && !lng)
+            for (uint32_t k = b; k < e; k++) s += in[tidx[k]];
+        uint64_t lm = __ballot(lng);
+        while (lm) {
+            const uint32_t l = (uint32_t)__ffsll((long long)lm) - 1u;
+            lm &= lm - 1;
+            const uint32_t bl = rdlane(b, l), el = rdlane(e, l);
+            uint32_t ws = 0;
+            for (uint32_t k = bl + lane; k < el; k += 64) ws += in[tidx[k]];
+            ws = wave_sum32(ws);
+            if (lane == l) s = ws;
+        }
+        if (valid) atomicMin(out + j, s);
     }
 }
 
@@ -1116,26 +1211,28 @@ void launch_compact(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V)
 static const uint32_t kMaxGridY = 65535;
 
 void launch_table_count(hipStream_t s, const uint32_t* off, const uint32_t* adj, const sgm_tabsrc* ds, uint32_t nsrc,
-                        uint32_t max_rows)
+                        uint32_t max_rows, uint32_t kmax)
 {
-    if (!nsrc || !max_rows) return;
+    if (!nsrc || !max_rows || !kmax) return;
     const uint32_t xcap = 16384 / nsrc > 8 ? 16384 / nsrc : 8u;
+    const size_t smem = sizeof(uint32_t) * 4 * 64 * kmax;
     for (uint32_t s0 = 0; s0 < nsrc; s0 += kMaxGridY) {
         const uint32_t k = nsrc - s0 < kMaxGridY ? nsrc - s0 : kMaxGridY;
-        dim3 grid(clampg(((uint64_t)max_rows + 3) / 4, 1, xcap), k);
-        hipLaunchKernelGGL(k_table_count_src, grid, dim3(256), 0, s, off, adj, ds + s0);
+        dim3 grid(clampg(((uint64_t)max_rows + 255) / 256, 1, xcap), k);
+        hipLaunchKernelGGL(k_table_count_src, grid, dim3(256), smem, s, off, adj, ds + s0, kmax);
     }
 }
 
-void launch_table_fill(hipStream_t s, const uint32_t* off, const uint32_t* adj, const sgm_tabdesc* ds, uint32_t m,
-                       uint32_t max_rows)
+void launch_table_fill(hipStream_t s, const uint32_t* off, const uint32_t* adj, const sgm_tabfill* ds, uint32_t nsrc,
+                       uint32_t max_rows, uint32_t kmax)
 {
-    if (!m || !max_rows) return;
-    const uint32_t xcap = 16384 / m > 8 ? 16384 / m : 8u;
-    for (uint32_t s0 = 0; s0 < m; s0 += kMaxGridY) {
-        const uint32_t k = m - s0 < kMaxGridY ? m - s0 : kMaxGridY;
-        dim3 grid(clampg(((uint64_t)max_rows + 3) / 4, 1, xcap), k);
-        hipLaunchKernelGGL(k_table_fill_edge, grid, dim3(256), 0, s, off, adj, ds + s0);
+    if (!nsrc || !max_rows || !kmax) return;
+    const uint32_t xcap = 16384 / nsrc > 8 ? 16384 / nsrc : 8u;
+    const size_t smem = sizeof(uint32_t) * 4 * 64 * kmax;
+    for (uint32_t s0 = 0; s0 < nsrc; s0 += kMaxGridY) {
+        const uint32_t k = nsrc - s0 < kMaxGridY ? nsrc - s0 : kMaxGridY;
+        dim3 grid(clampg(((uint64_t)max_rows + 255) / 256, 1, xcap), k);
+        hipLaunchKernelGGL(k_table_fill_src, grid, dim3(256), smem, s, off, adj, ds + s0, kmax);
     }
 }
 
@@ -1174,12 +1271,14 @@ void launch_row_gather(hipStream_t s, const uint32_t* toff, const uint32_t* tidx
                            tidx, rows, in, out, mul, total);
 }
 
-void launch_weight_rows(hipStream_t s, const uint32_t* toff, const uint32_t* tidx, uint32_t rows, const uint32_t* in,
-                        uint32_t* out)
+void launch_weight_step(hipStream_t s, const sgm_wdesc* ds, uint32_t m, uint32_t max_rows)
 {
-    if (rows)
-        hipLaunchKernelGGL(k_weight_rows, dim3(clampg(((uint64_t)rows * 4 + 255) / 256, 1, 2048)), dim3(256), 0, s,
-                           toff, tidx, rows, in, out);
+    if (!m || !max_rows) return;
+    for (uint32_t s0 = 0; s0 < m; s0 += kMaxGridY) {
+        const uint32_t k = m - s0 < kMaxGridY ? m - s0 : kMaxGridY;
+        dim3 grid(clampg(((uint64_t)max_rows + 255) / 256, 1, 1024), k);
+        hipLaunchKernelGGL(k_weight_step, grid, dim3(256), 0, s, ds + s0);
+    }
 }
 
 void launch_fill_u64(hipStream_t s, unsigned long long* p, size_t n, unsigned long long val)

@@ -37,6 +37,8 @@ struct sgm_ctx {
     uint32_t aux_cap = 0, afirst_cap = 0, astats_cap = 0, astop_cap = 0;
     sgm_dynplan_dev* d_dynplans = nullptr;   // dynamic-order search plans (shares the buffers above)
     uint32_t dyn_cap = 0;
+    sgm_wdesc* d_wdesc = nullptr;            // DAF weight row passes of a batch
+    uint32_t wdesc_cap = 0;
     unsigned long long* d_work64 = nullptr;  // item counter of the batched search
     sgm_plan_dev* d_plans = nullptr;          // per-query plans of a batch
     sgm_equery* d_eq = nullptr;
@@ -149,7 +151,6 @@ struct sgm_table {
     uint32_t max_deg = 0;
     uint32_t* d_off = nullptr;  // |C(a)|+1
     uint32_t* d_idx = nullptr;
-    uint32_t* d_v = nullptr;
 };
 
 struct sgm_query {
@@ -182,15 +183,12 @@ struct sgm_query {
     sgm_cdesc hc;
     std::vector<sgm_table> tables;    // n*n, directed (a,b): views into the arenas below
     // table arenas, grown as needed and kept across filters (no allocation when the query is
-    // filtered and tabled again): offsets, entries (positions and vertex ids), and scratch
+    // filtered and tabled again): offsets, entries (positions in C(b)), and scratch
     // (row counts, then one maximum per directed edge)
     uint32_t* d_toff = nullptr;
     uint32_t* d_tidx = nullptr;
-    uint32_t* d_tv = nullptr;
     uint32_t* d_tscr = nullptr;
-    sgm_tabsrc* d_tdesc = nullptr;   // count pass: per query vertex (its directed edges)
-    sgm_tabdesc* d_tedesc = nullptr; // fill pass: per directed edge
-    size_t toff_cap = 0, tidx_cap = 0, tv_cap = 0, tscr_cap = 0, tdesc_cap = 0, tedesc_cap = 0;
+    size_t toff_cap = 0, tidx_cap = 0, tscr_cap = 0;
     // DAF weights of the dynamic-order search (computeWeightArray), per candidate position in
     // the cand_off layout, for the start order weight_order; dropped with the tables
     uint32_t* d_weight = nullptr;

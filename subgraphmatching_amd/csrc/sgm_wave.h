@@ -5,6 +5,24 @@
 
 namespace sgm {
 
+// ------------------------------------------------------------------ global pointers
+// A pointer read from device memory (a descriptor or plan field) is accessed with flat
+// instructions unless its type names the global address space; flat accesses count against
+// both the vector-memory and the LDS/scalar wait counters, so every LDS or scalar-load wait
+// then also waits for them.  gcast() types such a pointer as global.
+template <class T>
+using glb = __attribute__((address_space(1))) T;
+template <class T>
+__device__ __forceinline__ const glb<T>* gcast(const T* p)
+{
+    return (const glb<T>*)p;
+}
+template <class T>
+__device__ __forceinline__ glb<T>* gcast(T* p)
+{
+    return (glb<T>*)p;
+}
+
 // ------------------------------------------------------------------ wave primitives
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
@@ -16,6 +34,12 @@ __device__ __forceinline__ uint32_t mbcnt64(uint64_t m)
 __device__ __forceinline__ uint32_t rdlane(uint32_t x, uint32_t l)
 {
     return (uint32_t)__builtin_amdgcn_readlane((int)x, (int)l);
+}
+
+// a 64-bit value (a pointer) held by lane l, as a wave-uniform value
+__device__ __forceinline__ uint64_t rdlane64(uint64_t x, uint32_t l)
+{
+    return ((uint64_t)rdlane((uint32_t)(x >> 32), l) << 32) | rdlane((uint32_t)x, l);
 }
 
 __device__ __forceinline__ uint32_t uni(uint32_t x)
@@ -67,6 +91,20 @@ __device__ __forceinline__ uint64_t lanes_or64(uint64_t lanes, uint64_t x)
         r |= ((uint64_t)rdlane((uint32_t)(x >> 32), l) << 32) | rdlane((uint32_t)x, l);
     }
     return r;
+}
+
+// later reads of LDS words other lanes wrote see the writes
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ uint32_t wave_sum32(uint32_t x)
+{
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) x += (uint32_t)__shfl_xor((int)x, s);
+    return x;
 }
 
 __device__ __forceinline__ uint64_t wave_or64(uint64_t x)
