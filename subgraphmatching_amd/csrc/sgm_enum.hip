@@ -1208,6 +1208,18 @@ struct WaveAux {
     uint32_t tot[SGM_MAXQ];
     uint32_t chunk_base[SGM_MAXQ];
     uint32_t stack_top[SGM_MAXQ + 1];
+    // Per depth d >= 1, the rows of its backward neighbours at depths below d-1, intersected
+    // once per visit of depth d-1 (fF[d], fFn[d]; null when d's only backward neighbour is
+    // at d-1), and its edge from depth d-1 (fE[d]; SGM_INVALID when none, and then fF[d] is
+    // the whole list).  Every sibling at d-1 then needs at most one intersection.
+    const uint32_t* fF[SGM_MAXQ];
+    uint32_t fFn[SGM_MAXQ], fE[SGM_MAXQ];
+    // the last level over a chunk of depth-(L-1) siblings (queries of 3+ vertices), when L
+    // has no edge from L-1: fF[L]'s matches, size and conflict bits against depths < L-1
+    uint32_t baseA, candsA;
+    uint64_t cfA;
+    uint32_t sok[64], scnt[64];  // per sibling lane: matches and list size (segmented sums)
+    uint64_t scf[64];            // per sibling lane: conflict bits
 };
 
 // Build the candidate list of depth d into (*list, *size); all but the last intersection
@@ -1254,23 +1266,218 @@ __device__ bool aux_list(const sgm_auxplan_dev& A, const WaveAux& S, uint32_t d,
     return true;
 }
 
-// count positions whose vertex is not used by depths < L; conflicts add anc[L] | anc[j]
+// fF / fFn / fE of depth dn (1 <= dn <= L), at a time when the depths below dn-1 are
+// matched: their rows intersected on the stack from *top (a single row is aliased).
+__device__ bool aux_prefix(const sgm_auxplan_dev& A, WaveAux& S, uint32_t dn, uint32_t* ws, uint32_t stack_words,
+                           uint32_t& top, unsigned long long& elems, unsigned long long& outs,
+                           unsigned long long* probes, const uint32_t* stop)
+{
+    const uint32_t f0 = A.prev_first[dn], f1 = A.prev_first[dn + 1];
+    const uint32_t* F = nullptr;
+    uint32_t Fn = 0, E = SGM_INVALID;
+    for (uint32_t e = f0; e < f1; e++) {
+        if (A.prev_depth[e] + 1 == dn) {
+            E = e;
+            continue;
+        }
+        const uint32_t* o = A.tab_off[e];
+        const uint32_t r = S.iemb[A.prev_depth[e]];
+        const uint32_t b = o[r], nb = o[r + 1] - b;
+        if (!F) {
+            F = A.tab_idx[e] + b;
+            Fn = nb;
+            continue;
+        }
+        const uint32_t need = Fn < nb ? Fn : nb;
+        if (top + need > stack_words) return false;
+        const uint32_t sz = isect_write(F, Fn, nullptr, A.tab_idx[e] + b, nb, nullptr, ws + top, elems, outs, probes,
+                                        stop);
+        F = ws + top;
+        Fn = sz;
+        top += sz;
+    }
+    S.fF[dn] = F;
+    S.fFn[dn] = Fn;
+    S.fE[dn] = E;
+    return true;
+}
+
+// The candidate list of depth d (1 <= d < L): fF[d] intersected with the row of d's edge
+// from d-1 at that depth's match (the same set as the intersection of all its backward
+// neighbours' rows in order, getNextCandidatesImpl inc/explore.h:284-354).
+__device__ bool aux_list_at(const sgm_auxplan_dev& A, const WaveAux& S, uint32_t d, uint32_t* ws,
+                            uint32_t stack_words, uint32_t& top, const uint32_t*& list, uint32_t& size,
+                            unsigned long long& elems, unsigned long long& outs, unsigned long long* probes,
+                            const uint32_t* stop)
+{
+    const uint32_t E = uni(S.fE[d]);
+    const uint32_t* F = S.fF[d];
+    const uint32_t Fn = uni(S.fFn[d]);
+    if (E == SGM_INVALID) {
+        list = F;
+        size = Fn;
+        return true;
+    }
+    const uint32_t* o = A.tab_off[E];
+    const uint32_t r = S.iemb[d - 1];
+    const uint32_t b = o[r], nb = o[r + 1] - b;
+    if (!F) {
+        list = A.tab_idx[E] + b;
+        size = nb;
+        return true;
+    }
+    const uint32_t need = Fn < nb ? Fn : nb;
+    if (top + need > stack_words) return false;
+    size = isect_write(F, Fn, nullptr, A.tab_idx[E] + b, nb, nullptr, ws + top, elems, outs, probes, stop);
+    list = ws + top;
+    top += size;
+    return true;
+}
+
+// count positions whose vertex is not used by depths < nd; conflicts add anc[L] | anc[j]
 // (j = the depth holding the vertex) to *cf
 __device__ __forceinline__ uint32_t aux_count_chunk(bool valid, uint32_t vidx, const uint32_t* __restrict__ cand,
                                                     const WaveAux& S, uint32_t L, const uint64_t* __restrict__ anc,
-                                                    uint64_t* cf)
+                                                    uint64_t* cf, uint32_t nd)
 {
     bool ok = false;
     uint32_t c = SGM_INVALID;
     if (valid) {
         const uint32_t v = cand[vidx];
-        for (uint32_t j = 0; j < L; j++)
+        for (uint32_t j = 0; j < nd; j++)
             if (S.emb[j] == v) c = j;
         ok = c == SGM_INVALID;
     }
     const uint64_t cb = __ballot(valid && !ok);
     if (cf && cb) *cf |= lanes_or64(cb, (valid && !ok) ? (anc[L] | anc[c]) : 0ull);
     return (uint32_t)__popcll(__ballot(ok));
+}
+
+// The last level for every sibling of one chunk at depth L-1 at once (the siblings in
+// S.rem[L-1], whose list starts at chunk_base): for sibling s the candidates are lastF
+// intersected with s's row of the table from depth L-1 (lastE), or lastF alone when the
+// last vertex has no backward neighbour at L-1.  The per-sibling results are then applied in
+// sibling order exactly as one-at-a-time evaluation would (PruneCheck after each: the first
+// sibling whose failing set lacks L-1 cuts the rest of the loop), and S.rem[L-1] is
+// consumed.  Counters: nodes[L-1] and nodes[L], one intersection per extra backward
+// neighbour of the last vertex for each evaluated sibling, as the reference counts them.
+__device__ void aux_last_chunk(const sgm_auxplan_dev& A, WaveAux& S, uint32_t L, uint32_t base, int use_fs,
+                               const uint64_t* __restrict__ anc, unsigned long long& inter,
+                               unsigned long long& elems, unsigned long long& outs, unsigned long long& scanned,
+                               unsigned long long& fsp, unsigned long long& cnt_pending)
+{
+    const uint32_t lane = lane_id(), d = L - 1;
+    const uint64_t sib = uni64(S.rem[d]);
+    S.rem[d] = 0;
+    const bool me = (sib >> lane) & 1ull;
+    const uint32_t idx = me ? S.lst[d][base + lane] : 0u;
+    const uint32_t* F = S.fF[L];
+    const uint32_t Fn = uni(S.fFn[L]), E = uni(S.fE[L]);
+    const uint32_t* candL = A.cand[L];
+    const uint32_t nsib = (uint32_t)__popcll(sib);
+    uint32_t ok_s = 0, cands_s = 0;
+    uint64_t cf_s = 0;
+    if (E == SGM_INVALID) {
+        // every sibling's list is F: its own vertex is the one extra vertex to exclude
+        const uint32_t v = me ? A.cand[d][idx] : 0u;
+        uint32_t lo = 0, hi = Fn;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (candL[F[mid]] < v) lo = mid + 1;
+            else hi = mid;
+        }
+        const bool hit = me && lo < Fn && candL[F[lo]] == v;
+        cands_s = uni(S.candsA);
+        ok_s = uni(S.baseA) - (hit ? 1u : 0u);
+        cf_s = uni64(S.cfA) | (hit ? (anc[L] | anc[d]) : 0ull);
+        if (lane == 0) S.probes += (unsigned long long)nsib * (1u + probe_cost(Fn, false));
+        elems += (unsigned long long)nsib * Fn;
+    } else {
+        // sibling s's row R from depth L-1, intersected with F when the last vertex has other
+        // backward neighbours: the shorter of the two is walked and each key probed in the
+        // other; the siblings' walks are concatenated, 64 keys per step
+        const uint32_t* o = A.tab_off[E];
+        const uint32_t* tidx = A.tab_idx[E];
+        const uint32_t rb = me ? o[idx] : 0u, rlen = me ? o[idx + 1] - rb : 0u;
+        const bool walk_f = F && Fn < rlen;
+        const uint32_t len = walk_f ? Fn : rlen;
+        const uint32_t incl = wave_incl_scan(len), total = rdlane(incl, 63);
+        S.sok[lane] = 0;
+        S.scnt[lane] = 0;
+        S.scf[lane] = 0;
+        wave_sync();
+        for (uint32_t b0 = 0; b0 < total; b0 += 64) {
+            const TabStep st = tab_step(incl, len, total, b0, lane);
+            const uint32_t own = st.own & 63u;
+            const uint32_t rbj = (uint32_t)__shfl((int)rb, (int)own);
+            const uint32_t rlj = (uint32_t)__shfl((int)rlen, (int)own);
+            const bool wf = (uint32_t)__shfl((int)walk_f, (int)own) != 0;
+            uint32_t x = 0;
+            bool in = false;
+            if (st.act) {
+                if (wf) {
+                    x = F[st.pos];
+                    in = lower_found(tidx + rbj, rlj, x);
+                } else {
+                    x = tidx[rbj + st.pos];
+                    in = !F || lower_found(F, Fn, x);
+                }
+            }
+            uint32_t c = SGM_INVALID;
+            if (in) {
+                const uint32_t v = candL[x];
+                for (uint32_t j = 0; j < d; j++)
+                    if (S.emb[j] == v) c = j;
+            }
+            const bool good = in && c == SGM_INVALID;
+            const uint64_t bin = __ballot(in), bok = __ballot(good);
+            const uint64_t upto = st.below | (1ull << lane);
+            if (st.last) {
+                S.scnt[st.own] += (uint32_t)__popcll(bin & upto);
+                S.sok[st.own] += (uint32_t)__popcll(bok & upto);
+            }
+            if (use_fs && in && !good) atomicOr((unsigned long long*)&S.scf[st.own & 63u], anc[L] | anc[c]);
+        }
+        wave_sync();
+        cands_s = S.scnt[lane];
+        ok_s = S.sok[lane];
+        cf_s = S.scf[lane];
+        const uint32_t found = wave_sum32(me ? cands_s : 0u);
+        if (F) {
+            // keys of the shorter list, each with a binary search in the longer
+            const uint32_t pw = wave_sum32(me ? len * (1u + probe_cost(walk_f ? rlen : Fn, false)) : 0u);
+            if (lane == 0) S.probes += pw;
+            elems += (unsigned long long)wave_sum32(me ? rlen : 0u) + (unsigned long long)nsib * Fn;
+            outs += found;
+            scanned += found;
+        } else {
+            scanned += total;
+        }
+    }
+    // PruneCheck after each sibling, in order (inc/prune.h:119-137): fail = anc[L] for an
+    // empty list, all ones after a match, else the conflicts' bits
+    uint64_t counted = sib;
+    if (use_fs) {
+        const uint64_t fs = cands_s == 0 ? anc[L] : (ok_s ? ~0ull : cf_s);
+        const uint64_t cut = __ballot(me && !((fs >> d) & 1ull));
+        if (cut) {
+            const uint32_t c = (uint32_t)__ffsll((long long)cut) - 1u;
+            counted = sib & ((2ull << c) - 1ull);
+            S.fail[d - 1] = rdlane64(fs, c);
+            S.pos[d] = S.tot[d];
+            fsp++;
+        } else {
+            S.fail[d - 1] |= wave_or64(me ? fs : 0ull);
+        }
+    }
+    const bool cnt_me = (counted >> lane) & 1ull;
+    const uint32_t nc = (uint32_t)__popcll(counted);
+    const uint32_t okc = wave_sum32(cnt_me ? ok_s : 0u);
+    S.nodes[d] += nc;
+    S.nodes[L] += okc;
+    cnt_pending += okc;
+    inter += (unsigned long long)nc * (A.prev_first[L + 1] - A.prev_first[L] - 1u);
+    wave_sync();
 }
 
 // Many queries per launch: items are the depth-0 candidates of every query, query q owning
@@ -1360,6 +1567,10 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
         S.emb[0] = v0;
         S.iemb[0] = root;
         S.nodes[0] += 1;
+        {
+            uint32_t top0 = 0;  // depth 1's backward neighbours are all at depth 0: nothing written
+            aux_prefix(A, S, 1, ws, stack_words, top0, elems, outs, &S.probes, stop);
+        }
         S.stack_top[1] = 0;
         uint32_t d = 1;
         // back from a finished child at depth d to d - 1, then PruneCheck(d - 1)
@@ -1406,8 +1617,13 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
                 const uint32_t* last;
                 uint32_t nlast;
                 const bool is_last = d == L;
-                if (!aux_list(A, S, d, ws, stack_words, top, list, size, is_last, last, nlast, elems, outs,
-                              &S.probes, stop)) {
+                // the last level is reached one sibling at a time only for 2-vertex queries
+                // (otherwise aux_last_chunk takes it a chunk of siblings at a time)
+                const bool built = is_last ? aux_list(A, S, d, ws, stack_words, top, list, size, true, last, nlast,
+                                                      elems, outs, &S.probes, stop)
+                                           : aux_list_at(A, S, d, ws, stack_words, top, list, size, elems, outs,
+                                                         &S.probes, stop);
+                if (!built) {
                     overflow = true; aborted = true; break;
                 }
                 if (is_last) {
@@ -1420,7 +1636,7 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
                         for (uint32_t base = 0; base < size; base += 64) {
                             const uint32_t i = base + lane;
                             const bool valid = i < size;
-                            ok += aux_count_chunk(valid, valid ? list[i] : 0u, cand, S, L, anc, cfp);
+                            ok += aux_count_chunk(valid, valid ? list[i] : 0u, cand, S, L, anc, cfp, L);
                         }
                     } else {
                         const uint32_t* a = list; uint32_t na = size;
@@ -1437,7 +1653,7 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
                             outs += h;
                             scanned += h;
                             cands += h;
-                            ok += aux_count_chunk(hit, key, cand, S, L, anc, cfp);
+                            ok += aux_count_chunk(hit, key, cand, S, L, anc, cfp, L);
                         }
                     }
                     cnt_pending += ok;
@@ -1456,6 +1672,34 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
                         continue;
                     }
                     S.fail[d - 1] = 0;
+                }
+                if (size) {
+                    // depth d+1's rows from depths below d are the same for every sibling of
+                    // this visit: intersect them once
+                    if (!aux_prefix(A, S, d + 1, ws, stack_words, top, elems, outs, &S.probes, stop)) {
+                        overflow = true; aborted = true; break;
+                    }
+                }
+                if (d + 1 == L && size) {
+                    const uint32_t* F = S.fF[L];
+                    const uint32_t Fn = uni(S.fFn[L]);
+                    if (uni(S.fE[L]) == SGM_INVALID) {
+                        // no edge from L-1: F is every sibling's list; its matches against the
+                        // depths below L-1 once, the sibling's own vertex per chunk
+                        const uint32_t* cand = A.cand[L];
+                        uint32_t ok = 0;
+                        uint64_t cf = 0;
+                        for (uint32_t base = 0; base < Fn; base += 64) {
+                            const uint32_t i = base + lane;
+                            const bool valid = i < Fn;
+                            ok += aux_count_chunk(valid, valid ? F[i] : 0u, cand, S, L, anc, use_fs ? &cf : nullptr,
+                                                  L - 1);
+                        }
+                        scanned += Fn;
+                        S.baseA = ok;
+                        S.candsA = Fn;
+                        S.cfA = cf;
+                    }
                 }
                 S.stack_top[d + 1] = top;
                 S.lst[d] = list;
@@ -1489,6 +1733,18 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
                 const uint64_t cb = __ballot(valid && !ok);
                 if (use_fs && cb) S.fail[d - 1] |= lanes_or64(cb, (valid && !ok) ? (anc[d] | anc[c]) : 0ull);
                 S.rem[d] = __ballot(ok);
+                if (d + 1 == L && S.rem[d]) {
+                    aux_last_chunk(A, S, L, base, use_fs, anc, inter, elems, outs, scanned, fsp, cnt_pending);
+                    // a chunk can add many embeddings: publish them and leave at once when the
+                    // query's cap (or a stop) is reached, instead of at the next poll
+                    if (cnt_pending >= FLUSH_AT) {
+                        flush_count(cnt_pending, &gst->count, stop, cap);
+                        if (__atomic_load_n(stop, __ATOMIC_RELAXED)) {
+                            aborted = true;
+                            break;
+                        }
+                    }
+                }
                 continue;
             }
             const uint64_t rm = S.rem[d];
@@ -1795,8 +2051,14 @@ __global__ void __launch_bounds__(256) k_enum_dyn(const sgm_dynplan_dev* __restr
                         ok += (uint32_t)__popcll(__ballot(good));
                     }
                     cnt_pending += ok;
-                    if (cnt_pending >= FLUSH_AT) flush_count(cnt_pending, &gst->count, stop, cap);
                     S.nodes[L] += ok;
+                    if (cnt_pending >= FLUSH_AT) {
+                        flush_count(cnt_pending, &gst->count, stop, cap);
+                        if (__atomic_load_n(stop, __ATOMIC_RELAXED)) {
+                            aborted = true;
+                            break;
+                        }
+                    }
                     if (!back_up()) break;
                     continue;
                 }

@@ -769,29 +769,6 @@ __device__ __forceinline__ uint32_t cand_rank(const glb<uint64_t>* __restrict__ 
 // (lane, j) at tl[(wave * 64 + lane) * kmax + j].  In each 64-entry step the entries of one
 // row are a contiguous lane range; its last lane carries the row's hits of the step into
 // the slot.
-struct TabStep {
-    uint32_t own;    // lane whose row holds this entry (64 when none)
-    uint32_t pos;    // entry index within that row
-    bool act;        // the lane holds an entry
-    bool last;       // the last entry of its row in this step
-    uint64_t below;  // lanes of the same row before this one in this step
-};
-
-__device__ __forceinline__ TabStep tab_step(uint32_t incl, uint32_t len, uint32_t total, uint32_t b0, uint32_t lane)
-{
-    TabStep r;
-    const uint32_t t = b0 + lane;
-    r.own = wave_seg_owner(incl, t);
-    const uint32_t sj = (uint32_t)__shfl((int)(incl - len), (int)(r.own & 63u));
-    const uint32_t ej = (uint32_t)__shfl((int)incl, (int)(r.own & 63u));
-    r.act = t < total;
-    r.pos = t - sj;
-    r.last = r.act && (t + 1 == ej || lane == 63);
-    const uint32_t segstart = r.act ? (sj > b0 ? sj - b0 : 0u) : lane;
-    r.below = ((1ull << lane) - 1ull) & ~((1ull << segstart) - 1ull);
-    return r;
-}
-
 // Row counts of every directed edge out of a (blockIdx.y = a's descriptor).  Lane j holds
 // edge j's target bit and output array, read with readlane in the loops.
 __global__ void __launch_bounds__(256) k_table_count_src(const uint32_t* __restrict__ off,

@@ -113,6 +113,32 @@ __device__ __forceinline__ uint64_t wave_or64(uint64_t x)
     return ((uint64_t)hi << 32) | lo;
 }
 
+// One 64-entry step of a segmented walk over the concatenated lists of the wave's lanes
+// (lane l's list of len_l entries; incl = the inclusive prefix sums of the lengths, total =
+// their sum): the entries of one list are a contiguous lane range of a step.
+struct TabStep {
+    uint32_t own;    // lane whose row holds this entry (64 when none)
+    uint32_t pos;    // entry index within that row
+    bool act;        // the lane holds an entry
+    bool last;       // the last entry of its row in this step
+    uint64_t below;  // lanes of the same row before this one in this step
+};
+
+__device__ __forceinline__ TabStep tab_step(uint32_t incl, uint32_t len, uint32_t total, uint32_t b0, uint32_t lane)
+{
+    TabStep r;
+    const uint32_t t = b0 + lane;
+    r.own = wave_seg_owner(incl, t);
+    const uint32_t sj = (uint32_t)__shfl((int)(incl - len), (int)(r.own & 63u));
+    const uint32_t ej = (uint32_t)__shfl((int)incl, (int)(r.own & 63u));
+    r.act = t < total;
+    r.pos = t - sj;
+    r.last = r.act && (t + 1 == ej || lane == 63);
+    const uint32_t segstart = r.act ? (sj > b0 ? sj - b0 : 0u) : lane;
+    r.below = ((1ull << lane) - 1ull) & ~((1ull << segstart) - 1ull);
+    return r;
+}
+
 __device__ __forceinline__ bool lower_found(const uint32_t* __restrict__ a, uint32_t n, uint32_t key)
 {
     uint32_t lo = 0, hi = n;
