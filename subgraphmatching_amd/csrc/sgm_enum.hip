@@ -211,7 +211,7 @@ __device__ __forceinline__ uint64_t uni64(uint64_t x)
 }
 
 // bit v of a per-query bitmap (the last query vertex's candidates)
-__device__ __forceinline__ bool last_bit(const uint32_t* __restrict__ lb, uint32_t v)
+__device__ __forceinline__ bool last_bit(const glb<uint32_t>* __restrict__ lb, uint32_t v)
 {
     return ((lb[v >> 5] >> (v & 31u)) & 1u) != 0u;
 }
@@ -226,7 +226,7 @@ struct LastRes {
 // Classify UNR elements per lane at the last depth L: filter bit of u_L, then injectivity
 // against the L mapped vertices.
 __device__ __forceinline__ void classify_last4(const bool (&valid)[UNR], const uint32_t (&w)[UNR],
-                                               const uint32_t* __restrict__ lbits, const WaveDFS& S,
+                                               const glb<uint32_t>* __restrict__ lbits, const WaveDFS& S,
                                                const uint64_t* anc, uint32_t L, LastRes& r)
 {
     bool f[UNR];
@@ -254,7 +254,7 @@ __device__ __forceinline__ void classify_last4(const bool (&valid)[UNR], const u
 }
 
 // Last level, scanning an existing list.
-__device__ void last_scan(const uint32_t* __restrict__ list, uint32_t size, const uint32_t* __restrict__ lbits,
+__device__ void last_scan(const uint32_t* __restrict__ list, uint32_t size, const glb<uint32_t>* __restrict__ lbits,
                           const WaveDFS& S, const uint64_t* anc, uint32_t L, LastRes& r,
                           const uint32_t* stop)
 {
@@ -277,7 +277,7 @@ __device__ void last_scan(const uint32_t* __restrict__ list, uint32_t size, cons
 // Last level, intersecting (parent op) with N(v) and classifying in the same pass.
 __device__ void last_isect(const uint32_t* __restrict__ a, uint32_t na, const uint32_t* __restrict__ abits,
                            const uint32_t* __restrict__ b, uint32_t nb, const uint32_t* __restrict__ bbits,
-                           const uint32_t* __restrict__ lbits, const WaveDFS& S,
+                           const glb<uint32_t>* __restrict__ lbits, const WaveDFS& S,
                            const uint64_t* anc, uint32_t L, LastRes& r, unsigned long long& elems,
                            unsigned long long* probes, const uint32_t* stop)
 {
@@ -593,7 +593,7 @@ __device__ void last_chunk(const sgm_plan_dev& P, WaveDFS& S, uint32_t d, const 
                            const uint32_t* __restrict__ ws, const uint32_t* op_off, const uint32_t* op_size,
                            const uint32_t* op_vtx, const uint32_t* __restrict__ hub_slot,
                            const uint32_t* __restrict__ hub_bits, uint32_t hub_words,
-                           const uint32_t* __restrict__ lbits, uint32_t embv, int use_fs, WaveAcc& acc,
+                           const glb<uint32_t>* __restrict__ lbits, uint32_t embv, int use_fs, WaveAcc& acc,
                            const uint32_t* stop)
 {
     const uint32_t lane = lane_id();
@@ -836,9 +836,9 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
     if (lane == 0) S.probes = 0;
     uint32_t cur = SGM_INVALID;  // query whose plan sits in this wave's LDS
     uint32_t n = 0, L = 0, root_u = 0;
-    const uint32_t* lastbits = nullptr;  // bitmap of C(u_L) of the current query
+    const glb<uint32_t>* lastbits = nullptr;  // bitmap of C(u_L) of the current query
     uint32_t embv = 0;  // lane j: the vertex mapped at depth j (as S.emb[j]; readlane is cheaper than LDS)
-    const uint32_t* vmask = nullptr;
+    const glb<uint32_t>* vmask = nullptr;  // global-typed: loaded from the descriptor, otherwise flat
     uint32_t* stop = nullptr;
     bool overflow = false;
     uint32_t poll = 0;
@@ -904,8 +904,8 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
             n = P.n;
             L = n - 1;
             root_u = Q.root_u;
-            vmask = Q.vmask;
-            lastbits = Q.lastbits;
+            vmask = gcast(Q.vmask);
+            lastbits = gcast(Q.lastbits);
             stop = Q.stop;
         }
         const unsigned long long cap = Q.cap;
