@@ -222,20 +222,43 @@ def _roots(roots):
     return C.c_uint32(int(lo)), C.c_uint32(int(hi))
 
 
-def enum_all(g, q, cands, order, cap=None, time_limit_ms=0.0, fs=False, roots=None):
-    """explore_Intersection_All<fs,false,false,false,false> restated; returns the stats dict.
-    ``roots`` as in enum_cache."""
+def enum_all(g, q, cands, order, cap=None, time_limit_ms=0.0, fs=False, roots=None, guards=None):
+    """explore_Intersection_All<fs,false,cd,false,false> restated; returns the stats dict.
+    ``roots`` as in enum_cache.  ``guards``: (goff, guard words) from gup_guards for GuP's
+    conflict detection (cd); its prunes are counted in fs_prunes."""
     V, n = g.getVertexCnt(), q.getVertexCnt()
     m, cnt, stride = _cand_matrix(cands, V)
     o = _u32(order)
     st = Stats()
-    rc = lib().orc_enum_all(C.c_uint32(V), _p(g.off), _p(g.adj), C.c_uint32(n), _p(q.off),
-                            _p(q.adj), _p(m), _p(cnt), C.c_uint32(stride), _p(o),
-                            C.c_uint64(cap if cap is not None else 2**64 - 1), C.c_int(1 if fs else 0),
-                            C.c_double(time_limit_ms), *_roots(roots), C.byref(st))
+    goff = gw = None
+    if guards is not None:
+        goff, gw = _u32(guards[0]), _u32(guards[1])
+    rc = lib().orc_enum_all_cd(C.c_uint32(V), _p(g.off), _p(g.adj), C.c_uint32(n), _p(q.off),
+                               _p(q.adj), _p(m), _p(cnt), C.c_uint32(stride), _p(o),
+                               C.c_uint64(cap if cap is not None else 2**64 - 1), C.c_int(1 if fs else 0),
+                               C.c_double(time_limit_ms), *_roots(roots),
+                               _p(goff) if goff is not None else None, _p(gw) if gw is not None else None,
+                               C.byref(st))
     if rc:
         raise ValueError(f"orc_enum_all failed ({rc})")
     return st.as_dict(n)
+
+
+def gup_guards(g, q, cands, order, r=3):
+    """GUPConflictDetection::Init's reservation guards for ``order``: (goff[n+1], words) with 4
+    words per (depth, candidate position) in depth order: size (0 = none), then the guard's
+    data vertices ascending."""
+    V, n = g.getVertexCnt(), q.getVertexCnt()
+    m, cnt, stride = _cand_matrix(cands, V)
+    o = _u32(order)
+    goff = np.zeros(n + 1, dtype=np.uint32)
+    total = int(sum(int(cnt[u]) for u in order))
+    words = np.zeros(4 * max(total, 1), dtype=np.uint32)
+    rc = lib().orc_gup_guards(C.c_uint32(V), _p(g.off), _p(g.adj), C.c_uint32(n), _p(q.off), _p(q.adj), _p(m),
+                              _p(cnt), C.c_uint32(stride), _p(o), C.c_uint32(r), _p(goff), _p(words))
+    if rc:
+        raise ValueError(f"orc_gup_guards failed ({rc})")
+    return goff, words[:4 * total]
 
 
 # ---- BFS-tree filters and orders (sgm_oracle_tree.c) -----------------------------------
@@ -363,6 +386,22 @@ def enum_dynamic(g, q, cands, kind="DAF", cap=None, time_limit_ms=0.0, bfs_order
         raise ValueError(f"orc_enum_dynamic failed ({rc})")
     return {"count": int(st.count), "intersections": int(st.intersections), "nodes": [int(x) for x in st.nodes[:n]],
             "timed_out": bool(st.timed_out), "capped": bool(st.capped), "enum_ms": float(st.enum_ms)}
+
+
+def probe_gup_guards(g, q, cands, order, r=3):
+    """The guards built with std::partition / std::unordered_set themselves (libsortprobe.so)."""
+    path = os.path.join(HERE, "libsortprobe.so")
+    if not os.path.exists(path):
+        build()
+    P = C.CDLL(path)
+    V, n = g.getVertexCnt(), q.getVertexCnt()
+    m, cnt, stride = _cand_matrix(cands, V)
+    goff = np.zeros(n + 1, dtype=np.uint32)
+    total = int(sum(int(cnt[u]) for u in order))
+    words = np.zeros(4 * max(total, 1), dtype=np.uint32)
+    P.probe_gup_guards(C.c_uint32(V), _p(g.off), _p(g.adj), C.c_uint32(n), _p(q.off), _p(q.adj), _p(m), _p(cnt),
+                       C.c_uint32(stride), _p(_u32(order)), C.c_uint32(r), _p(goff), _p(words))
+    return goff, words[:4 * total]
 
 
 def probe_dynamic(g, q, cands, bfs_order, weights, cap=None):

@@ -891,6 +891,8 @@ typedef struct {
     double deadline;
     uint64_t tick;
     orc_stats *st;
+    /* GuP conflict detection (sgm_oracle_gup.c): 4 words per (depth, position), from goff */
+    const uint32_t *guard, *goff;
 } all_ctx;
 
 static void rec_all(all_ctx *c, uint32_t d)
@@ -920,10 +922,20 @@ static void rec_all(all_ctx *c, uint32_t d)
     for (uint32_t pos = 0; pos < tot; pos++) {
         uint32_t vi = c->buf[d][pos];
         uint32_t v = c->cand[(size_t)u * c->stride + vi];
+        const uint32_t *g = c->guard ? c->guard + 4 * (size_t)(c->goff[d] + vi) : NULL;
+        uint32_t gk = 0;
+        if (g && g[0] && !c->vis[v])
+            while (gk < g[0] && c->vis[g[1 + gk]]) gk++;
         if (c->vis[v]) {
             if (!c->use_fs) continue;
             c->fail[d] = c->anc[d] | c->anc[c->vis[v] - 1];  /* InjectiveConflict */
             c->fail[d - 1] |= c->fail[d];
+        } else if (g && g[0] && gk == g[0]) {
+            /* GUPConflictDetection::matchCheck (inc/prune.h:577-595): every vertex of v's
+             * guard is matched, so v is pruned; with failing sets ForwardPrune sets fail[d]
+             * to all (inc/prune.h:2296-2300) */
+            c->st->fs_prunes++;
+            if (c->use_fs) { c->fail[d] = ~0ull; c->fail[d - 1] |= ~0ull; }
         } else {
             c->emb[u] = v; c->iemb[u] = vi; c->vis[v] = (uint8_t)(d + 1);
             c->st->nodes[d]++;
@@ -948,10 +960,13 @@ static void rec_all(all_ctx *c, uint32_t d)
     }
 }
 
-int orc_enum_all(uint32_t V, const uint32_t *off, const uint32_t *adj, uint32_t n,
-                 const uint32_t *qoff, const uint32_t *qadj, const uint32_t *cand,
-                 const uint32_t *cnt, uint32_t cand_stride, const uint32_t *order, uint64_t cap,
-                 int use_fs, double time_limit_ms, uint32_t root_lo, uint32_t root_hi, orc_stats *st)
+/* explore_Intersection_All<fs, false, cd, false, false>: guard/goff from orc_gup_guards for
+ * GuP's conflict detection (cd), or null; its prunes are counted in fs_prunes. */
+int orc_enum_all_cd(uint32_t V, const uint32_t *off, const uint32_t *adj, uint32_t n,
+                    const uint32_t *qoff, const uint32_t *qadj, const uint32_t *cand,
+                    const uint32_t *cnt, uint32_t cand_stride, const uint32_t *order, uint64_t cap,
+                    int use_fs, double time_limit_ms, uint32_t root_lo, uint32_t root_hi, const uint32_t *goff,
+                    const uint32_t *guard, orc_stats *st)
 {
     memset(st, 0, sizeof(*st));
     if (n < 2 || n > ORC_MAXQ) return -2;
@@ -959,6 +974,8 @@ int orc_enum_all(uint32_t V, const uint32_t *off, const uint32_t *adj, uint32_t 
     memset(&c, 0, sizeof(c));
     c.n = n; c.cand = cand; c.order = order; c.stride = cand_stride; c.cap = cap; c.st = st;
     c.use_fs = use_fs;
+    c.guard = guard;
+    c.goff = goff;
     daf_ancestors(n, qoff, qadj, order, c.anc);
     uint32_t pos_of[ORC_MAXQ];
     for (uint32_t i = 0; i < n; i++) pos_of[order[i]] = i;
@@ -1010,6 +1027,15 @@ int orc_enum_all(uint32_t V, const uint32_t *off, const uint32_t *adj, uint32_t 
     for (uint32_t d = 0; d < n; d++) free(c.buf[d]);
     free(c.tmp); free(c.vis);
     return 0;
+}
+
+int orc_enum_all(uint32_t V, const uint32_t *off, const uint32_t *adj, uint32_t n,
+                 const uint32_t *qoff, const uint32_t *qadj, const uint32_t *cand,
+                 const uint32_t *cnt, uint32_t cand_stride, const uint32_t *order, uint64_t cap,
+                 int use_fs, double time_limit_ms, uint32_t root_lo, uint32_t root_hi, orc_stats *st)
+{
+    return orc_enum_all_cd(V, off, adj, n, qoff, qadj, cand, cnt, cand_stride, order, cap, use_fs, time_limit_ms,
+                           root_lo, root_hi, NULL, NULL, st);
 }
 
 uint32_t orc_abi_version(void) { return 4u; }

@@ -187,3 +187,134 @@ extern "C" int probe_dyn_enum(uint32_t V, const uint32_t* off, const uint32_t* a
     *out_inter = inter;
     return 0;
 }
+
+// ----------------------------------------------------------------------------------------
+// GuP's reservation guards written with the standard containers the reference uses
+// (std::unordered_set for the chosen vertices, std::partition over the reservation
+// candidates, std::vector<pair>::erase, std::sort over the per-neighbour guards), to check
+// the restatement of their element order in sgm_oracle_gup.c.  Tables come from the data
+// graph directly; same output layout as orc_gup_guards.
+#include <unordered_set>
+
+extern "C" int probe_gup_guards(uint32_t V, const uint32_t* off, const uint32_t* adj, uint32_t n, const uint32_t* qoff,
+                                const uint32_t* qadj, const uint32_t* cand, const uint32_t* cnt, uint32_t stride,
+                                const uint32_t* order, uint32_t r, uint32_t* goff, uint32_t* out)
+{
+    std::vector<uint32_t> pos_of(n), former(n, 0), match(V, 0);
+    for (uint32_t i = 0; i < n; i++) pos_of[order[i]] = i;
+    for (uint32_t i = 1; i < n; i++) former[i] = former[i - 1] | (1u << (i - 1));
+    std::vector<std::vector<uint32_t>> fdep(n);
+    for (uint32_t i = 0; i < n; i++) {
+        for (uint32_t p = qoff[order[i]]; p < qoff[order[i] + 1]; p++)
+            if (pos_of[qadj[p]] > i) fdep[i].push_back(pos_of[qadj[p]]);
+        std::sort(fdep[i].begin(), fdep[i].end());
+        for (uint32_t j = 0; j < cnt[order[i]]; j++) match[cand[(size_t)order[i] * stride + j]] |= 1u << i;
+    }
+    // row j of table (a, b): positions in C(b) of the neighbours of C(a)[j]
+    auto row = [&](uint32_t a, uint32_t b, uint32_t j) {
+        std::vector<uint32_t> rw;
+        const uint32_t v = cand[(size_t)a * stride + j];
+        const uint32_t* cb = cand + (size_t)b * stride;
+        for (uint32_t q = off[v]; q < off[v + 1]; q++) {
+            const uint32_t* it = std::lower_bound(cb, cb + cnt[b], adj[q]);
+            if (it != cb + cnt[b] && *it == adj[q]) rw.push_back((uint32_t)(it - cb));
+        }
+        return rw;
+    };
+    goff[0] = 0;
+    for (uint32_t i = 0; i < n; i++) goff[i + 1] = goff[i] + cnt[order[i]];
+    std::fill(out, out + 4 * (size_t)goff[n], 0u);
+    std::vector<uint32_t> freq(V, 0);
+    auto pc = [](uint32_t x) { return (uint32_t)__builtin_popcount(x); };
+    for (int i = (int)n - 1; i >= 0; i--) {
+        const uint32_t u = order[i];
+        for (uint32_t j = 0; j < cnt[u]; j++) {
+            const uint32_t v = cand[(size_t)u * stride + j];
+            std::vector<std::pair<uint32_t, std::vector<uint32_t>>> all;
+            bool failed = false;
+            for (auto it = fdep[i].rbegin(); it != fdep[i].rend() && !failed; ++it) {
+                const uint32_t ufd = *it, uf = order[ufd];
+                std::unordered_set<uint32_t> chosen;
+                std::vector<uint32_t> rev;
+                std::vector<std::pair<uint32_t, uint32_t>> gr;
+                uint32_t dom = 0;
+                const std::vector<uint32_t> rw = row(u, uf, j);
+                for (uint32_t fi : rw) {
+                    const uint32_t vf = cand[(size_t)uf * stride + fi];
+                    const bool vf_ok = (match[vf] & former[i]) != 0;
+                    const uint32_t* g = out + 4 * (size_t)(goff[ufd] + fi);
+                    std::vector<uint32_t> ws = g[0] ? std::vector<uint32_t>(g + 1, g + 1 + g[0])
+                                                    : std::vector<uint32_t>{vf};
+                    for (uint32_t w : ws) {
+                        if (w == v) continue;
+                        if (chosen.count(vf)) break;
+                        const bool w_ok = (match[w] & former[i]) != 0;
+                        if (vf_ok && (vf == w || !w_ok)) {
+                            chosen.insert(vf);
+                            dom |= match[vf];
+                        } else if (!vf_ok && w_ok) {
+                            chosen.insert(w);
+                            dom |= match[w];
+                        }
+                        if ((!vf_ok && !w_ok) || chosen.size() > r || chosen.size() > pc(former[i] & dom)) {
+                            failed = true;
+                            break;
+                        }
+                    }
+                    if (failed) break;
+                }
+                if (failed) break;
+                for (uint32_t fi : rw) {
+                    const uint32_t vf = cand[(size_t)uf * stride + fi];
+                    const uint32_t* g = out + 4 * (size_t)(goff[ufd] + fi);
+                    if (!g[0] || chosen.count(vf)) continue;
+                    for (uint32_t k = 0; k < g[0]; k++)
+                        if (g[1 + k] != v && !chosen.count(g[1 + k])) gr.emplace_back(vf, g[1 + k]);
+                }
+                for (auto& e : gr) {
+                    if (freq[e.first] == 0) rev.push_back(e.first);
+                    if (freq[e.second] == 0) rev.push_back(e.second);
+                    freq[e.first]++;
+                    freq[e.second]++;
+                }
+                while (chosen.size() < r) {
+                    auto mid = std::partition(rev.begin(), rev.end(), [&](uint32_t x) {
+                        return freq[x] > 0 && chosen.size() + 1 <= pc((dom | match[x]) & former[i]);
+                    });
+                    const size_t valid = (size_t)(mid - rev.begin());
+                    if (valid == 0) break;
+                    size_t mi = 0;
+                    uint32_t mf = 0;
+                    for (size_t k = 0; k < valid; k++)
+                        if (freq[rev[k]] >= mf) { mi = k; mf = freq[rev[k]]; }
+                    const uint32_t pick = rev[mi];
+                    chosen.insert(pick);
+                    for (auto e = gr.begin(); e != gr.end();) {
+                        if (e->first == pick || e->second == pick) {
+                            freq[e->first]--;
+                            freq[e->second]--;
+                            e = gr.erase(e);
+                        } else {
+                            ++e;
+                        }
+                    }
+                }
+                for (uint32_t x : rev) freq[x] = 0;
+                if (gr.empty() && !chosen.empty()) {
+                    std::vector<uint32_t> gv(chosen.begin(), chosen.end());
+                    std::sort(gv.begin(), gv.end());
+                    all.emplace_back(ufd, gv);
+                }
+            }
+            if (failed || all.empty()) continue;
+            std::sort(all.begin(), all.end(), [](const auto& a, const auto& b) {
+                if (a.second.size() != b.second.size()) return a.second.size() < b.second.size();
+                return a.first > b.first;
+            });
+            uint32_t* o = out + 4 * (size_t)(goff[i] + j);
+            o[0] = (uint32_t)all[0].second.size();
+            for (size_t k = 0; k < all[0].second.size(); k++) o[1 + k] = all[0].second[k];
+        }
+    }
+    return 0;
+}
