@@ -64,7 +64,8 @@ inline void check(int rc, const char* what)
 
 // Per-thread search counters (inc/common_type.h:150-226): the fields the accelerated path
 // fills are total_intersection and total_iter_count (extensions per depth); daf_fp.cnt
-// counts failing-set cut-offs.
+// counts failing-set cut-offs (and the guard prunes too when both prunings run), gup_cd.cnt
+// GuP's conflict-detection prunes.
 class Profiler {
 public:
     Profiler(const Profiler&) = delete;
@@ -84,6 +85,7 @@ public:
         void reset() { cnt = 0; mem_overhead = 0; mem_overhead_KB = 0.0; }
     };
     PruneStatistics daf_fp;
+    PruneStatistics gup_cd;
 
     size_t total_intersection{0};
     size_t total_iter_count[300];
@@ -92,6 +94,7 @@ public:
     {
         total_intersection = 0;
         daf_fp.reset();
+        gup_cd.reset();
         std::memset(total_iter_count, 0, sizeof total_iter_count);
     }
 

@@ -6,6 +6,8 @@
 //   explore_Intersection_All<false x5>     plain candidate-space search over the aux-edge
 //                                          tables (the "RI" / "GQL" rows)
 //   explore_Intersection_All<true, false x4>  the same with DAF failing sets
+//   explore_Intersection_All<f, false, true, false, false>  with GuP's conflict detection
+//                                          (reservation guards computed on the host)
 //   explore_Intersection_DynamicOrder_DAF  the next vertex chosen per search node, by DAF
 //   explore_Intersection_DynamicOrder_VEQ  weights / by candidate counts (the "DAF" and
 //                                          "VEQ" rows of the order ablation)
@@ -62,7 +64,9 @@ inline void run_enumeration(const Query& query, const Order& order, const Config
     // the dynamic searches count no iterations (their vertex per depth varies)
     if (method != SGM_ENUM_DYN_DAF && method != SGM_ENUM_DYN_VEQ)
         for (uint32_t d = 0; d < query.getVertexCnt() && d < 32; d++) prof.total_iter_count[d] += (size_t)st.nodes[d];
-    prof.daf_fp.cnt += (uint32_t)st.fs_prunes;
+    // the prunes of the method's pruning (DAF cut-offs and / or GuP guard prunes)
+    if (method == SGM_ENUM_AUX_TABLES_GUPCD) prof.gup_cd.cnt += (uint32_t)st.fs_prunes;
+    else prof.daf_fp.cnt += (uint32_t)st.fs_prunes;
 }
 
 }  // namespace sgm
@@ -81,13 +85,14 @@ inline void explore_Intersection_All(const Graph& data, const Query& query, cons
                                      double& time, size_t& ans, std::atomic<bool>& stop_flag,
                                      uint64_t output_limit = -1)
 {
-    static_assert(!useGuPFailingPrune && !useGUPConflictDetection && !useBICEConflictDetection &&
-                      !useVEQAutomorphismPrune,
-                  "only explore_Intersection_All<false|true,false,false,false,false> (plain or DAF failing "
-                  "sets) is on the accelerated path");
+    static_assert(!useGuPFailingPrune && !useBICEConflictDetection && !useVEQAutomorphismPrune,
+                  "only explore_Intersection_All<false|true,false,false|true,false,false> (plain, DAF failing "
+                  "sets, GuP conflict detection) is on the accelerated path");
     (void)data;
     sgm::run_enumeration(query, order, cfg, time, ans, stop_flag, output_limit,
-                         useDAFFailingPrune ? SGM_ENUM_AUX_TABLES_FS : SGM_ENUM_AUX_TABLES);
+                         useGUPConflictDetection
+                             ? (useDAFFailingPrune ? SGM_ENUM_AUX_TABLES_FS_GUPCD : SGM_ENUM_AUX_TABLES_GUPCD)
+                             : (useDAFFailingPrune ? SGM_ENUM_AUX_TABLES_FS : SGM_ENUM_AUX_TABLES));
 }
 
 // explore_Intersection_DynamicOrder_DAF (inc/explore.h:1849-1931): bfs_order is the start

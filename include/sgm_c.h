@@ -55,8 +55,16 @@ enum {
                                      argument is the start (BFS) order; the next vertex is chosen per search
                                      node by the DAF weights of sgm_daf_weights / sgm_set_daf_weights for that
                                      order (computed in the call when none are resident for it) */
-    SGM_ENUM_DYN_VEQ = 5          /* explore_Intersection_DynamicOrder_VEQ (inc/explore.h:1993-2062): the same,
+    SGM_ENUM_DYN_VEQ = 5,         /* explore_Intersection_DynamicOrder_VEQ (inc/explore.h:1993-2062): the same,
                                      ranked by candidate counts */
+    SGM_ENUM_AUX_TABLES_GUPCD = 6,    /* explore_Intersection_All<false,false,true,false,false>: the aux-table
+                                         search with GuP's conflict detection (inc/prune.h:488-923): a candidate
+                                         whose reservation guard is wholly matched is pruned; the guards (up to
+                                         3 vertices, gupConflictDetectionSize) are computed on the host for the
+                                         order in the call and kept; prunes are counted in sgm_stats.fs_prunes */
+    SGM_ENUM_AUX_TABLES_FS_GUPCD = 7  /* explore_Intersection_All<true,false,true,false,false>: both; a pruned
+                                         candidate gives a full failing set (inc/prune.h:2296-2300);
+                                         fs_prunes counts failing-set cut-offs and guard prunes together */
 };
 
 typedef struct sgm_ctx sgm_ctx;
@@ -236,6 +244,12 @@ int sgm_daf_weights(sgm_query* q, uint32_t* bfs_order, uint32_t* const* weights)
 /* The caller's weight_array for start order `order` (counts[u] words per vertex), made the
  * resident weights of SGM_ENUM_DYN_DAF (the drop-in passes the weight_array it was given). */
 int sgm_set_daf_weights(sgm_query* q, const uint32_t* order, const uint32_t* const* weights);
+/* GUPConflictDetection::Init (inc/prune.h:828-905) for `order` (q filtered; its aux tables are
+ * built if missing): goff[n+1] = the first record of each depth (records in depth order, one
+ * per candidate position of the vertex at that depth); words (words_cap >= 4 * goff[n]) = 4
+ * per record: the guard's size (0 = none), then its data vertices ascending.  The guards
+ * stay resident for SGM_ENUM_AUX_TABLES_GUPCD over that order; goff / words may be NULL. */
+int sgm_gup_guards(sgm_query* q, const uint32_t* order, uint32_t* goff, uint32_t* words, uint64_t words_cap);
 
 /* ---- enumeration (replaces explore_Intersection_Cache_DAF_Filter inc/explore.h:1073,
  * explore_Intersection_All<false x5> :2757 and the dynamic-order searches :1849, :1993) -- */

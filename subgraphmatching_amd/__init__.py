@@ -8,6 +8,6 @@ from .api import (Batch, CandidateParam, Config, Context, DAFStartVertex, Explor
                   Order_RI, Profiler, query_kcore, ReuseParam, default_context, enumerate_batch, enumerate_query,
                   explore_Intersection_All, explore_Intersection_Cache_DAF_Filter,
                   explore_Intersection_DynamicOrder_DAF, explore_Intersection_DynamicOrder_VEQ, computeWeightArray,
-                  filter_batch, plan_export)
+                  filter_batch, gup_guards, plan_export)
 
 __version__ = "0.1.0"

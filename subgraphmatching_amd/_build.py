@@ -13,7 +13,7 @@ ORDERS_DRIVER = os.path.join(HERE, "sgm_orders")
 ROOT = os.path.dirname(HERE)
 SOURCES = ["sgm_kernels.hip", "sgm_enum.hip", "sgm_costmodel.hip", "sgm_host.cpp", "sgm_capi.cpp",
            "sgm_batch.cpp", "sgm_costmodel_host.cpp", "sgm_noalloc.hip", "sgm_tree.cpp", "sgm_orders.cpp",
-           "sgm_dynamic.cpp"]
+           "sgm_dynamic.cpp", "sgm_gup.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SGM_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -37,6 +37,8 @@ SGM_ENUM_AUX_TABLES = 2
 SGM_ENUM_AUX_TABLES_FS = 3
 SGM_ENUM_DYN_DAF = 4
 SGM_ENUM_DYN_VEQ = 5
+SGM_ENUM_AUX_TABLES_GUPCD = 6
+SGM_ENUM_AUX_TABLES_FS_GUPCD = 7
 
 ERR_NAMES = {0: "SGM_OK", -1: "SGM_EINVAL", -2: "SGM_EHIP", -3: "SGM_ENOMEM", -4: "SGM_EEMPTY",
              -5: "SGM_EORDER", -6: "SGM_ESTATE", -7: "SGM_ESTOPPED", -8: "SGM_EDEVICE",
@@ -58,6 +60,7 @@ EXPORTS = [
     "sgm_graph_set_labels", "sgm_query_set_labels", "sgm_plan_export_labeled", "sgm_batch_filter_labeled",
     "sgm_enumerate_noalloc", "sgm_order_qsi", "sgm_order_cfl", "sgm_order_ceci", "sgm_order_gup",
     "sgm_query_kcore", "sgm_query_bfs", "sgm_daf_start_vertex", "sgm_daf_weights", "sgm_set_daf_weights",
+    "sgm_gup_guards",
 ]
 
 
@@ -169,6 +172,7 @@ def lib() -> C.CDLL:
         "sgm_daf_start_vertex": (C.c_int, [vp, vp]),
         "sgm_daf_weights": (C.c_int, [vp, vp, vp]),
         "sgm_set_daf_weights": (C.c_int, [vp, vp, vp]),
+        "sgm_gup_guards": (C.c_int, [vp, vp, vp, vp, C.c_uint64]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -33,7 +33,8 @@ __all__ = ["Context", "default_context", "DeviceGraph", "CandidateParam", "Edges
            "Order_CFL", "Order_CECI", "Order_GuP", "query_kcore", "ReuseParam", "Config", "ExploreResult",
            "explore_Intersection_Cache_DAF_Filter", "explore_Intersection_All", "Profiler",
            "filter_batch", "enumerate_query", "enumerate_batch", "plan_export", "Batch", "DAFStartVertex",
-           "computeWeightArray", "explore_Intersection_DynamicOrder_DAF", "explore_Intersection_DynamicOrder_VEQ"]
+           "computeWeightArray", "explore_Intersection_DynamicOrder_DAF", "explore_Intersection_DynamicOrder_VEQ",
+           "gup_guards"]
 
 
 def _check(code, ctx=None):
@@ -461,6 +462,7 @@ class Profiler:
         self.total_intersection = 0
         self.total_iter_count = [0] * 300
         self.daf_prune_cnt = 0
+        self.gup_cd_cnt = 0
 
 
 @dataclass
@@ -499,7 +501,10 @@ def enumerate_query(can: CandidateParam, order, method=L.SGM_ENUM_CACHE_FILTER_F
     if method not in (L.SGM_ENUM_DYN_DAF, L.SGM_ENUM_DYN_VEQ):
         for d, x in enumerate(res.nodes):
             prof.total_iter_count[d] += x
-    prof.daf_prune_cnt += res.fs_prunes
+    if method == L.SGM_ENUM_AUX_TABLES_GUPCD:
+        prof.gup_cd_cnt += res.fs_prunes
+    else:
+        prof.daf_prune_cnt += res.fs_prunes
     return res
 
 
@@ -538,13 +543,36 @@ def explore_Intersection_Cache_DAF_Filter(data: Graph, query: Query, order, cfg:
     return enumerate_query(cfg.can, order, L.SGM_ENUM_CACHE_FILTER_FS, output_limit)
 
 
-def explore_Intersection_All(data: Graph, query: Query, order, cfg: Config,
-                             output_limit: int | None = None) -> ExploreResult:
-    """inc/explore.h:2757-2848 with all pruning disabled (the "RI" / "GQL" rows)."""
+def explore_Intersection_All(data: Graph, query: Query, order, cfg: Config, output_limit: int | None = None,
+                             DAFFailingPrune: bool = False, GuPFailingPrune: bool = False,
+                             GUPConflictDetection: bool = False, BICEConflictDetection: bool = False,
+                             VEQAutomorphismPrune: bool = False) -> ExploreResult:
+    """inc/explore.h:2757-2848, the template flags as keywords: plain (the "RI" / "GQL" rows),
+    with DAF failing sets, with GuP's conflict detection, or both.  GuP failing sets, BICE
+    and VEQ pruning are not on the accelerated path (NotImplementedError)."""
     cfg.Check()
     if cfg.can is None:
         raise ValueError("the candidate-space search needs filtered candidates (cfg.can)")
-    return enumerate_query(cfg.can, order, L.SGM_ENUM_AUX_TABLES, output_limit)
+    if GuPFailingPrune or BICEConflictDetection or VEQAutomorphismPrune:
+        raise NotImplementedError("explore_Intersection_All with GuP failing sets, BICE or VEQ pruning is not on "
+                                  "the accelerated path")
+    method = {(False, False): L.SGM_ENUM_AUX_TABLES, (True, False): L.SGM_ENUM_AUX_TABLES_FS,
+              (False, True): L.SGM_ENUM_AUX_TABLES_GUPCD,
+              (True, True): L.SGM_ENUM_AUX_TABLES_FS_GUPCD}[(bool(DAFFailingPrune), bool(GUPConflictDetection))]
+    return enumerate_query(cfg.can, order, method, output_limit)
+
+
+def gup_guards(can: CandidateParam, order):
+    """GUPConflictDetection::Init's reservation guards for ``order`` (inc/prune.h:828-905),
+    computed by the library: (goff[n+1], words) with 4 words per (depth, candidate position)
+    in depth order -- the guard's size (0 = none), then its data vertices ascending."""
+    n = can.query.getVertexCnt()
+    o = L.u32(order)
+    goff = np.zeros(n + 1, dtype=np.uint32)
+    _check(L.lib().sgm_gup_guards(can._q, L.ptr(o), L.ptr(goff), None, 0), can.ctx)
+    words = np.zeros(max(4 * int(goff[n]), 1), dtype=np.uint32)
+    _check(L.lib().sgm_gup_guards(can._q, L.ptr(o), None, L.ptr(words), words.size), can.ctx)
+    return goff, words[:4 * int(goff[n])]
 
 
 def DAFStartVertex(data: Graph, query: Query, can: CandidateParam) -> int:

@@ -353,6 +353,7 @@ static void free_tables(sgm_query* q)
 {
     for (auto& t : q->tables) t = sgm_table();
     q->weights_ok = false;
+    q->guards_ok = false;
 }
 
 // Grows a u32 arena to at least `need` words (contents are not kept).
@@ -375,6 +376,7 @@ void sgm_query_destroy(sgm_query* q)
     dfree(q->d_tidx);
     dfree(q->d_tscr);
     dfree(q->d_weight);
+    dfree(q->d_guard);
     dfree(q->d_vmask);
     dfree(q->d_removed);
     dfree(q->d_bm);
@@ -1189,6 +1191,14 @@ static int enumerate_aux_batch(sgm_query* const* qs, const uint32_t* const* orde
         const int trc = build_tables_many(missing.data(), (uint32_t)missing.size());
         if (trc) return trc;
     }
+    const bool gup_cd = o.method == SGM_ENUM_AUX_TABLES_GUPCD || o.method == SGM_ENUM_AUX_TABLES_FS_GUPCD;
+    if (gup_cd)  // GuP's reservation guards for each query's order (kept while order and tables stay)
+        for (uint32_t i = 0; i < nq; i++) {
+            sgm_query* q = qs[i];
+            if (q->empty || (q->guards_ok && std::equal(orders[i], orders[i] + q->q.n, q->guard_order))) continue;
+            const int grc = gup_guards_build(q, orders[i]);
+            if (grc) return grc;
+        }
     for (uint32_t i = 0; i < nq; i++) {
         sgm_query* q = qs[i];
         const uint32_t* order = orders[i];
@@ -1206,6 +1216,7 @@ static int enumerate_aux_batch(sgm_query* const* qs, const uint32_t* const* orde
             ap.order[d] = u;
             ap.prev_first[d] = e;
             ap.cand[d] = q->d_list ? q->d_list + q->cand_off[u] : nullptr;
+            ap.guard[d] = gup_cd && q->d_guard ? q->d_guard + 4 * (size_t)q->guard_off[d] : nullptr;
             for (uint32_t p : prev[u]) {
                 const sgm_table& t = q->tables[(size_t)p * n + u];
                 ap.prev_depth[e] = pos[p];
@@ -1223,7 +1234,7 @@ static int enumerate_aux_batch(sgm_query* const* qs, const uint32_t* const* orde
             ap.anc[d] = a;
         }
     }
-    const int use_fs = o.method == SGM_ENUM_AUX_TABLES_FS ? 1 : 0;
+    const int use_fs = o.method == SGM_ENUM_AUX_TABLES_FS || o.method == SGM_ENUM_AUX_TABLES_FS_GUPCD ? 1 : 0;
     uint32_t rlo, rhi;
     root_range(o, &rlo, &rhi);
     std::vector<uint32_t> live;
@@ -1301,7 +1312,7 @@ int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uin
     sgm_ctx* c = g->ctx;
     const sgm_enum_opts o = normalize_opts(opts);
     std::memset(out, 0, sizeof(sgm_stats) * nq);
-    if (o.method == SGM_ENUM_AUX_TABLES || o.method == SGM_ENUM_AUX_TABLES_FS) {
+    if (is_aux_method(o.method)) {
         // the aux-table search over many queries: orders (given or RI) checked like plans
         std::vector<std::array<uint32_t, SGM_MAXQ>> ords(nq);
         std::vector<const uint32_t*> optr(nq);
@@ -1513,7 +1524,7 @@ int sgm_enumerate(sgm_query* q, const uint32_t* order, const sgm_enum_opts* opts
     if (!q || !order || !out) return SGM_EINVAL;
     sgm_ctx* c = q->g->ctx;
     const sgm_enum_opts o = normalize_opts(opts);
-    if (o.method == SGM_ENUM_AUX_TABLES || o.method == SGM_ENUM_AUX_TABLES_FS) {
+    if (is_aux_method(o.method)) {
         std::memset(out, 0, sizeof *out);
         if (const int frc = check_filtered(q, "enumerate")) return frc;
         if (q->q.n < 2) return set_err(c, SGM_EINVAL, "enumeration needs at least 2 query vertices");

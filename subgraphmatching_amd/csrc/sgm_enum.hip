@@ -1199,6 +1199,7 @@ __global__ void __launch_bounds__(256) k_enum_cache(const uint32_t* __restrict__
 struct WaveAux {
     unsigned long long probes;  // probe_words not yet published
     uint64_t rem[SGM_MAXQ];
+    uint64_t cdm[SGM_MAXQ];     // GuP conflict detection: pruned lanes of the chunk not yet counted
     uint64_t fail[SGM_MAXQ];  // failing sets (use_fs)
     unsigned long long nodes[SGM_MAXQ];
     const uint32_t* lst[SGM_MAXQ];
@@ -1457,12 +1458,15 @@ __device__ void aux_last_chunk(const sgm_auxplan_dev& A, WaveAux& S, uint32_t L,
     // PruneCheck after each sibling, in order (inc/prune.h:119-137): fail = anc[L] for an
     // empty list, all ones after a match, else the conflicts' bits
     uint64_t counted = sib;
+    uint64_t gprune = uni64(S.cdm[d]);  // the chunk's guard prunes, up to a cut
+    S.cdm[d] = 0;
     if (use_fs) {
         const uint64_t fs = cands_s == 0 ? anc[L] : (ok_s ? ~0ull : cf_s);
         const uint64_t cut = __ballot(me && !((fs >> d) & 1ull));
         if (cut) {
             const uint32_t c = (uint32_t)__ffsll((long long)cut) - 1u;
             counted = sib & ((2ull << c) - 1ull);
+            gprune &= (2ull << c) - 1ull;
             S.fail[d - 1] = rdlane64(fs, c);
             S.pos[d] = S.tot[d];
             fsp++;
@@ -1475,6 +1479,7 @@ __device__ void aux_last_chunk(const sgm_auxplan_dev& A, WaveAux& S, uint32_t L,
     const uint32_t okc = wave_sum32(cnt_me ? ok_s : 0u);
     S.nodes[d] += nc;
     S.nodes[L] += okc;
+    fsp += (unsigned long long)__popcll(gprune);
     cnt_pending += okc;
     inter += (unsigned long long)nc * (A.prev_first[L + 1] - A.prev_first[L] - 1u);
     wave_sync();
@@ -1583,6 +1588,7 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
                 if (!((f >> d) & 1ull)) {
                     S.fail[d - 1] = f;
                     S.rem[d] = 0;
+                    S.cdm[d] = 0;  // candidates after the cut are never tested
                     S.pos[d] = S.tot[d];
                     fsp++;
                 } else {
@@ -1706,7 +1712,12 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
                 S.tot[d] = size;
                 S.pos[d] = 0;
                 S.rem[d] = 0;
+                S.cdm[d] = 0;
                 continue;
+            }
+            if (S.rem[d] == 0 && S.cdm[d]) {  // the chunk's remaining guard prunes
+                fsp += (unsigned long long)__popcll(uni64(S.cdm[d]));
+                S.cdm[d] = 0;
             }
             if (S.rem[d] == 0 && S.pos[d] >= S.tot[d]) {
                 if (!back_up()) break;
@@ -1732,6 +1743,28 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
                 // later cut-off in this loop overwrites fail[d-1], so the order does not matter
                 const uint64_t cb = __ballot(valid && !ok);
                 if (use_fs && cb) S.fail[d - 1] |= lanes_or64(cb, (valid && !ok) ? (anc[d] | anc[c]) : 0ull);
+                if (const uint32_t* gd = A.guard[d]) {
+                    // GUPConflictDetection::matchCheck (inc/prune.h:577-595): a candidate whose
+                    // guard is wholly matched is pruned; with failing sets its fail[d] is all
+                    // ones (a later cut-off overwrites fail[d-1], so the order does not matter)
+                    bool pr = false;
+                    if (ok) {
+                        const uint32_t* g = gd + 4 * (size_t)list[i];
+                        const uint32_t gs = g[0];
+                        uint32_t k = 0;
+                        for (; k < gs; k++) {
+                            const uint32_t w = g[1 + k];
+                            bool m = false;
+                            for (uint32_t j = 0; j < d; j++) m |= S.emb[j] == w;
+                            if (!m) break;
+                        }
+                        pr = gs && k == gs;
+                    }
+                    const uint64_t pb = __ballot(pr);
+                    if (pb && use_fs) S.fail[d - 1] = ~0ull;
+                    ok = ok && !pr;
+                    S.cdm[d] = pb;  // counted as the loop reaches them
+                }
                 S.rem[d] = __ballot(ok);
                 if (d + 1 == L && S.rem[d]) {
                     aux_last_chunk(A, S, L, base, use_fs, anc, inter, elems, outs, scanned, fsp, cnt_pending);
@@ -1750,6 +1783,11 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
             const uint64_t rm = S.rem[d];
             const uint32_t j = (uint32_t)__ffsll((long long)rm) - 1u;
             S.rem[d] = rm & (rm - 1);
+            if (S.cdm[d]) {  // guard prunes of the lanes before this sibling
+                const uint64_t below = uni64(S.cdm[d]) & ((1ull << j) - 1ull);
+                fsp += (unsigned long long)__popcll(below);
+                S.cdm[d] &= ~below;
+            }
             const uint32_t vidx = list[S.chunk_base[d] + j];
             S.emb[d] = A.cand[d][vidx];
             S.iemb[d] = vidx;

@@ -46,6 +46,9 @@ struct sgm_auxplan_dev {
     const uint32_t* tab_idx[SGM_MAXPREV];
     const uint32_t* cand[SGM_MAXQ];   // candidate list of the query vertex at each depth
     uint64_t anc[SGM_MAXQ];           // DAF ancestor masks over depths (failing sets, inc/prune.h:100-104)
+    // GuP conflict detection (inc/prune.h:488-923), or null: per depth, 4 words per candidate
+    // position -- the guard's size (0 = none), then its data vertices
+    const uint32_t* guard[SGM_MAXQ];
 };
 
 // Plan of the dynamic-order searches (explore_Intersection_DynamicOrder_DAF / _VEQ,

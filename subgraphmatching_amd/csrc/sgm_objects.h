@@ -195,6 +195,13 @@ struct sgm_query {
     size_t weight_cap = 0;
     bool weights_ok = false;
     uint32_t weight_order[SGM_MAXQ] = {0};
+    // GuP conflict-detection guards (4 words per (depth, candidate position), depth d's from
+    // guard_off[d]) for the order guard_order; dropped with the tables
+    uint32_t* d_guard = nullptr;
+    size_t guard_cap = 0;
+    bool guards_ok = false;
+    uint32_t guard_order[SGM_MAXQ] = {0};
+    uint32_t guard_off[SGM_MAXQ + 1] = {0};
 };
 
 inline int set_err(sgm_ctx* c, int code, const char* fmt, ...)
@@ -307,12 +314,20 @@ int build_tables_many(sgm_query* const* qs, uint32_t nq);
 // (checked); and their default start order, BFS from DAFStartVertex.
 int enumerate_dyn_batch(sgm_query* const* qs, const uint32_t* const* orders, uint32_t nq, const sgm_enum_opts& o,
                         sgm_stats* out);
+// GuP conflict-detection guards of q for `order` (sgm_gup.cpp; tables built), to q->d_guard.
+int gup_guards_build(sgm_query* q, const uint32_t* order);
 int dyn_default_order(sgm_query* q, uint32_t* order);
 sgm_enum_opts normalize_opts(const sgm_enum_opts* opts);
 uint32_t default_blocks(const sgm_ctx* c, const sgm_enum_opts& o, uint32_t nq);
 void stats_out(const sgm_stats_dev& st, uint64_t cap, sgm_stats* out, double ns_per_tick);
 // Device-clock ticks of a per-query time budget (sgm_enum_opts.time_limit_ms; 0 = none).
 unsigned long long budget_ticks(const sgm_ctx* c, const sgm_enum_opts& o);
+// The aux-table search methods (plain, DAF failing sets, GuP conflict detection, both).
+inline bool is_aux_method(int m)
+{
+    return m == SGM_ENUM_AUX_TABLES || m == SGM_ENUM_AUX_TABLES_FS || m == SGM_ENUM_AUX_TABLES_GUPCD ||
+           m == SGM_ENUM_AUX_TABLES_FS_GUPCD;
+}
 // The root range of opts as [lo, hi) (root_end == 0: every vertex).
 inline void root_range(const sgm_enum_opts& o, uint32_t* lo, uint32_t* hi)
 {

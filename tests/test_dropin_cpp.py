@@ -53,6 +53,8 @@ size_t row(const Graph& data, const Query& query, std::atomic<bool>& stop_flag) 
         c3.useFilter = true;
         c3.can = &c2;
         explore_Intersection_All<true, false, false, false, false>(data, query, oq, c3, time, ans, stop_flag, 100000);
+        explore_Intersection_All<false, false, true, false, false>(data, query, oq, c3, time, ans, stop_flag, 100000);
+        (void)Profiler::getInst().gup_cd.cnt;
     }
     // the dynamic-order rows of the order comparison (src/order_ablations.cpp:131-172)
     uint32_t** weight_array = new uint32_t*[query.getVertexCnt()];
