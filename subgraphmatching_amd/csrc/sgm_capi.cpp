@@ -1235,6 +1235,8 @@ static int enumerate_aux_batch(sgm_query* const* qs, const uint32_t* const* orde
         }
     }
     const int use_fs = o.method == SGM_ENUM_AUX_TABLES_FS || o.method == SGM_ENUM_AUX_TABLES_FS_GUPCD ? 1 : 0;
+    const uint32_t* hub_slot = nullptr;  // adjacency bitmaps of the hubs, for the last level's probes
+    if (const int hrc = hub_slot_map(g, o, &hub_slot)) return hrc;
     uint32_t rlo, rhi;
     root_range(o, &rlo, &rhi);
     std::vector<uint32_t> live;
@@ -1272,7 +1274,7 @@ static int enumerate_aux_batch(sgm_query* const* qs, const uint32_t* const* orde
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
         sgm::launch_enum_aux(c->stream, blocks, c->d_auxplans, c->d_afirst, m, c->d_stack, (uint32_t)words, c->d_work,
                              c->d_astats, c->d_astop, c->d_stop, c->d_hstop, (unsigned long long)o.cap,
-                             budget_ticks(c, o), rlo, rhi, use_fs);
+                             budget_ticks(c, o), rlo, rhi, use_fs, hub_slot, g->d_hub_bits, g->hub_words);
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
         std::vector<sgm_stats_dev> got(m);

@@ -1365,7 +1365,9 @@ __device__ __forceinline__ uint32_t aux_count_chunk(bool valid, uint32_t vidx, c
 __device__ void aux_last_chunk(const sgm_auxplan_dev& A, WaveAux& S, uint32_t L, uint32_t base, int use_fs,
                                const uint64_t* __restrict__ anc, unsigned long long& inter,
                                unsigned long long& elems, unsigned long long& outs, unsigned long long& scanned,
-                               unsigned long long& fsp, unsigned long long& cnt_pending)
+                               unsigned long long& fsp, unsigned long long& cnt_pending,
+                               const uint32_t* __restrict__ hub_slot, const uint32_t* __restrict__ hub_bits,
+                               uint32_t hub_words)
 {
     const uint32_t lane = lane_id(), d = L - 1;
     const uint64_t sib = uni64(S.rem[d]);
@@ -1401,6 +1403,9 @@ __device__ void aux_last_chunk(const sgm_auxplan_dev& A, WaveAux& S, uint32_t L,
         const uint32_t* tidx = A.tab_idx[E];
         const uint32_t rb = me ? o[idx] : 0u, rlen = me ? o[idx + 1] - rb : 0u;
         const bool walk_f = F && Fn < rlen;
+        // F's keys probe a sibling's row; when the sibling's vertex has an adjacency bitmap,
+        // "x in the row" is "C(u_L)[x] adjacent to it": one bitmap word instead of a search
+        const uint32_t hslot = (walk_f && hub_slot) ? hub_slot[A.cand[d][idx]] : SGM_INVALID;
         const uint32_t len = walk_f ? Fn : rlen;
         const uint32_t incl = wave_incl_scan(len), total = rdlane(incl, 63);
         S.sok[lane] = 0;
@@ -1413,12 +1418,18 @@ __device__ void aux_last_chunk(const sgm_auxplan_dev& A, WaveAux& S, uint32_t L,
             const uint32_t rbj = (uint32_t)__shfl((int)rb, (int)own);
             const uint32_t rlj = (uint32_t)__shfl((int)rlen, (int)own);
             const bool wf = (uint32_t)__shfl((int)walk_f, (int)own) != 0;
+            const uint32_t hs = (uint32_t)__shfl((int)hslot, (int)own);
             uint32_t x = 0;
             bool in = false;
             if (st.act) {
                 if (wf) {
                     x = F[st.pos];
-                    in = lower_found(tidx + rbj, rlj, x);
+                    if (hs != SGM_INVALID) {
+                        const uint32_t w = candL[x];
+                        in = (hub_bits[(size_t)hs * hub_words + (w >> 5)] >> (w & 31u)) & 1u;
+                    } else {
+                        in = lower_found(tidx + rbj, rlj, x);
+                    }
                 } else {
                     x = tidx[rbj + st.pos];
                     in = !F || lower_found(F, Fn, x);
@@ -1445,8 +1456,9 @@ __device__ void aux_last_chunk(const sgm_auxplan_dev& A, WaveAux& S, uint32_t L,
         cf_s = S.scf[lane];
         const uint32_t found = wave_sum32(me ? cands_s : 0u);
         if (F) {
-            // keys of the shorter list, each with a binary search in the longer
-            const uint32_t pw = wave_sum32(me ? len * (1u + probe_cost(walk_f ? rlen : Fn, false)) : 0u);
+            // keys of the shorter list, each with a bitmap word or a binary search in the longer
+            const uint32_t pw =
+                wave_sum32(me ? len * (1u + probe_cost(walk_f ? rlen : Fn, hslot != SGM_INVALID)) : 0u);
             if (lane == 0) S.probes += pw;
             elems += (unsigned long long)wave_sum32(me ? rlen : 0u) + (unsigned long long)nsib * Fn;
             outs += found;
@@ -1494,7 +1506,8 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
                                                   uint32_t* __restrict__ stops, uint32_t* __restrict__ dev_stop,
                                                   const uint32_t* __restrict__ host_stop, unsigned long long cap,
                                                   unsigned long long budget, uint32_t root_lo, uint32_t root_hi,
-                                                  int use_fs)
+                                                  int use_fs, const uint32_t* __restrict__ hub_slot,
+                                                  const uint32_t* __restrict__ hub_bits, uint32_t hub_words)
 {
     extern __shared__ uint64_t smem_u64[];
     uint8_t* smem = reinterpret_cast<uint8_t*>(smem_u64);
@@ -1767,7 +1780,8 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
                 }
                 S.rem[d] = __ballot(ok);
                 if (d + 1 == L && S.rem[d]) {
-                    aux_last_chunk(A, S, L, base, use_fs, anc, inter, elems, outs, scanned, fsp, cnt_pending);
+                    aux_last_chunk(A, S, L, base, use_fs, anc, inter, elems, outs, scanned, fsp, cnt_pending, hub_slot,
+                                   hub_bits, hub_words);
                     // a chunk can add many embeddings: publish them and leave at once when the
                     // query's cap (or a stop) is reached, instead of at the next poll
                     if (cnt_pending >= FLUSH_AT) {
@@ -2201,11 +2215,13 @@ void launch_enum_cache(hipStream_t s, uint32_t blocks, const uint32_t* off, cons
 void launch_enum_aux(hipStream_t s, uint32_t blocks, const sgm_auxplan_dev* plans, const uint32_t* item_first,
                      uint32_t nq, uint32_t* stack, uint32_t stack_words, uint32_t* work, sgm_stats_dev* st,
                      uint32_t* stops, uint32_t* dev_stop, const uint32_t* host_stop, unsigned long long cap,
-                     unsigned long long budget, uint32_t root_lo, uint32_t root_hi, int use_fs)
+                     unsigned long long budget, uint32_t root_lo, uint32_t root_hi, int use_fs,
+                     const uint32_t* hub_slot, const uint32_t* hub_bits, uint32_t hub_words)
 {
     const size_t smem = 4 * ((sizeof(WaveAux) + 15) & ~(size_t)15);
     hipLaunchKernelGGL(k_enum_aux, dim3(blocks), dim3(256), smem, s, plans, item_first, nq, stack, stack_words, work,
-                       st, stops, dev_stop, host_stop, cap, budget, root_lo, root_hi, use_fs);
+                       st, stops, dev_stop, host_stop, cap, budget, root_lo, root_hi, use_fs, hub_slot, hub_bits,
+                       hub_words);
 }
 
 void launch_enum_dyn(hipStream_t s, uint32_t blocks, const sgm_dynplan_dev* plans, const uint32_t* item_first,

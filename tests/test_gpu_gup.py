@@ -121,3 +121,34 @@ def test_batch_and_dropin(cases):
         sgm.explore_Intersection_All(g, q, O.order_ri(q), sgm.Config(useFilter=True, can=can), GuPFailingPrune=True)
     for c in cans:
         c.close()
+
+
+@pytest.mark.parametrize("fs", [False, True])
+def test_aux_last_level_hub_bitmaps(fs):
+    """The aux-table search's last level probes a sibling's row through the sibling's
+    adjacency bitmap when it is a hub (degree >= 64): the same results as the binary
+    searches of SGM_ENUM_F_NO_BITMAPS, and the oracle's, uncapped."""
+    g = with_labels(chung_lu(4000, 2.2, 12.0, seed=95), 3, seed=96)
+    assert int(np.diff(g.off).max()) >= 64
+    method = L.SGM_ENUM_AUX_TABLES_FS if fs else L.SGM_ENUM_AUX_TABLES
+    checked = 0
+    cap = 10_000_000
+    for q in mixed(g, 14, 97, 4, 9):
+        can = sgm.CandidateParam(g, q)
+        ok = sgm.Filter_GQL(g, q, can)
+        ok_r, c_r = O.gql(g, q)
+        assert ok == ok_r
+        if ok:
+            order = O.order_ri(q)
+            ref = O.enum_all(g, q, c_r, order, cap=cap, fs=fs)
+            r = sgm.enumerate_query(can, order, method, cap=cap)
+            rb = sgm.enumerate_query(can, order, method, cap=cap, flags=L.SGM_ENUM_F_NO_BITMAPS)
+            assert r.ans == rb.ans == ref["count"], q.name
+            if not ref["capped"]:
+                for x in (r, rb):
+                    assert (x.intersections, x.nodes, x.fs_prunes) == (ref["intersections"], ref["nodes"],
+                                                                         ref["fs_prunes"]), q.name
+                checked += 1
+        can.close()
+    say("checked", checked)
+    assert checked >= 4
