@@ -100,6 +100,7 @@ void sgm_ctx_destroy(sgm_ctx* c)
     dfree(c->d_wdesc);
     dfree(c->d_astats);
     dfree(c->d_astop);
+    dfree(c->d_anext);
     dfree(c->d_work64);
     dfree(c->d_plans);
     dfree(c->d_eq);
@@ -1255,6 +1256,7 @@ static int enumerate_aux_batch(sgm_query* const* qs, const uint32_t* const* orde
         if ((rc = grow(c, &c->d_afirst, c->afirst_cap, m + 1))) return rc;
         if ((rc = grow(c, &c->d_astats, c->astats_cap, m))) return rc;
         if ((rc = grow(c, &c->d_astop, c->astop_cap, m))) return rc;
+        if ((rc = grow(c, &c->d_anext, c->anext_cap, m))) return rc;
         std::vector<sgm_auxplan_dev> sub(m);
         std::vector<uint32_t> first(m + 1, 0);
         for (uint32_t k = 0; k < m; k++) {
@@ -1268,11 +1270,11 @@ static int enumerate_aux_batch(sgm_query* const* qs, const uint32_t* const* orde
                                  c->stream));
         HIPCHK(c, hipMemsetAsync(c->d_astats, 0, sizeof(sgm_stats_dev) * m, c->stream));
         HIPCHK(c, hipMemsetAsync(c->d_astop, 0, sizeof(uint32_t) * m, c->stream));
-        HIPCHK(c, hipMemsetAsync(c->d_work, 0, sizeof(uint32_t), c->stream));
+        HIPCHK(c, hipMemsetAsync(c->d_anext, 0, sizeof(uint32_t) * m, c->stream));
         HIPCHK(c, hipMemsetAsync(c->d_stop, 0, sizeof(uint32_t), c->stream));
         clear_host_stop(c);
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-        sgm::launch_enum_aux(c->stream, blocks, c->d_auxplans, c->d_afirst, m, c->d_stack, (uint32_t)words, c->d_work,
+        sgm::launch_enum_aux(c->stream, blocks, c->d_auxplans, c->d_afirst, m, c->d_stack, (uint32_t)words, c->d_anext,
                              c->d_astats, c->d_astop, c->d_stop, c->d_hstop, (unsigned long long)o.cap,
                              budget_ticks(c, o), rlo, rhi, use_fs, hub_slot, g->d_hub_bits, g->hub_words);
         HIPCHK(c, hipGetLastError());

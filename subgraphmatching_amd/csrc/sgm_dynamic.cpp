@@ -238,6 +238,7 @@ extern "C++" int enumerate_dyn_batch(sgm_query* const* qs, const uint32_t* const
         if ((rc = grow(c, &c->d_afirst, c->afirst_cap, m + 1))) return rc;
         if ((rc = grow(c, &c->d_astats, c->astats_cap, m))) return rc;
         if ((rc = grow(c, &c->d_astop, c->astop_cap, m))) return rc;
+        if ((rc = grow(c, &c->d_anext, c->anext_cap, m))) return rc;
         std::vector<sgm_dynplan_dev> sub(m);
         std::vector<uint32_t> first(m + 1, 0);
         for (uint32_t k = 0; k < m; k++) {
@@ -251,11 +252,11 @@ extern "C++" int enumerate_dyn_batch(sgm_query* const* qs, const uint32_t* const
                                  c->stream));
         HIPCHK(c, hipMemsetAsync(c->d_astats, 0, sizeof(sgm_stats_dev) * m, c->stream));
         HIPCHK(c, hipMemsetAsync(c->d_astop, 0, sizeof(uint32_t) * m, c->stream));
-        HIPCHK(c, hipMemsetAsync(c->d_work, 0, sizeof(uint32_t), c->stream));
+        HIPCHK(c, hipMemsetAsync(c->d_anext, 0, sizeof(uint32_t) * m, c->stream));
         HIPCHK(c, hipMemsetAsync(c->d_stop, 0, sizeof(uint32_t), c->stream));
         clear_host_stop(c);
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-        sgm::launch_enum_dyn(c->stream, blocks, c->d_dynplans, c->d_afirst, m, c->d_stack, (uint32_t)words, c->d_work,
+        sgm::launch_enum_dyn(c->stream, blocks, c->d_dynplans, c->d_afirst, m, c->d_stack, (uint32_t)words, c->d_anext,
                              c->d_astats, c->d_astop, c->d_stop, c->d_hstop, (unsigned long long)o.cap,
                              budget_ticks(c, o), rlo, rhi);
         HIPCHK(c, hipGetLastError());

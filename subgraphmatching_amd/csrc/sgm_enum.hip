@@ -210,6 +210,67 @@ __device__ __forceinline__ uint64_t uni64(uint64_t x)
     return ((uint64_t)uni((uint32_t)(x >> 32)) << 32) | uni((uint32_t)x);
 }
 
+// A wave's walk over the queries of a launch whose items are the depth-0 candidates of each
+// query q, [0, item_first[q+1] - item_first[q]), handed out by the query's own counter
+// next[q].  Workgroups are dispatched round-robin over the 8 XCDs, so the queries are split
+// into 8 classes (q mod 8): a wave serves its own class first, starting at a wave-dependent
+// query, so the waves spread over the batch instead of all piling onto one query, and each
+// L2 mostly holds the rows of one eighth of it.  A stopped query (cap, budget, overflow) is
+// passed over without touching its items.  The wave is done once a pass over every class in
+// a row found no item (counters only grow and stops are never cleared).
+struct QueryWalk {
+    uint32_t nq, ncls, cls, k, seen, classes, gw;
+};
+
+__device__ __forceinline__ uint32_t qw_csize(const QueryWalk& w, uint32_t c)
+{
+    return (w.nq - c + w.ncls - 1) / w.ncls;
+}
+
+__device__ __forceinline__ void qw_init(QueryWalk& w, uint32_t nq, uint32_t gwave)
+{
+    w.nq = nq;
+    w.ncls = nq < 8u ? nq : 8u;
+    w.cls = blockIdx.x % w.ncls;
+    w.gw = gwave;
+    w.k = (gwave / w.ncls) % qw_csize(w, w.cls);
+    w.seen = 0;
+    w.classes = 0;
+}
+
+// The next (query, item), or false when the wave is done (or the launch stopped).
+__device__ bool qw_next(QueryWalk& w, const uint32_t* __restrict__ item_first, uint32_t* __restrict__ next,
+                        const uint32_t* __restrict__ stops, const uint32_t* __restrict__ dev_stop, uint32_t& q,
+                        uint32_t& it)
+{
+    while (w.classes < w.ncls) {
+        if (__atomic_load_n(dev_stop, __ATOMIC_RELAXED)) return false;
+        const uint32_t qq = uni(w.cls + w.ncls * w.k);
+        const uint32_t nit = item_first[qq + 1] - item_first[qq];
+        if (!__atomic_load_n(stops + qq, __ATOMIC_RELAXED) && __atomic_load_n(next + qq, __ATOMIC_RELAXED) < nit) {
+            uint32_t t = 0;
+            if (lane_id() == 0) t = atomicAdd(next + qq, 1u);
+            t = uni((uint32_t)__shfl((int)t, 0));
+            if (t < nit) {
+                w.seen = 0;
+                w.classes = 0;
+                q = qq;
+                it = t;
+                return true;
+            }
+        }
+        const uint32_t cs = qw_csize(w, w.cls);
+        w.k = w.k + 1 == cs ? 0 : w.k + 1;
+        if (++w.seen == cs) {
+            w.cls = w.cls + 1 == w.ncls ? 0 : w.cls + 1;
+            w.k = (w.gw / w.ncls) % qw_csize(w, w.cls);
+            w.seen = 0;
+            w.classes++;
+        }
+    }
+    return false;
+}
+
 // bit v of a per-query bitmap (the last query vertex's candidates)
 __device__ __forceinline__ bool last_bit(const glb<uint32_t>* __restrict__ lb, uint32_t v)
 {
@@ -1498,11 +1559,12 @@ __device__ void aux_last_chunk(const sgm_auxplan_dev& A, WaveAux& S, uint32_t L,
 }
 
 // Many queries per launch: items are the depth-0 candidates of every query, query q owning
-// [item_first[q], item_first[q+1]); plans, statistics and stop words are per query.
+// item_first[q+1] - item_first[q] of them, handed out by its counter next[q] (zero at
+// launch) in the QueryWalk order; plans, statistics and stop words are per query.
 __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restrict__ plans,
                                                   const uint32_t* __restrict__ item_first, uint32_t nq,
                                                   uint32_t* __restrict__ stack, uint32_t stack_words,
-                                                  uint32_t* __restrict__ work, sgm_stats_dev* __restrict__ stats,
+                                                  uint32_t* __restrict__ next, sgm_stats_dev* __restrict__ stats,
                                                   uint32_t* __restrict__ stops, uint32_t* __restrict__ dev_stop,
                                                   const uint32_t* __restrict__ host_stop, unsigned long long cap,
                                                   unsigned long long budget, uint32_t root_lo, uint32_t root_hi,
@@ -1548,22 +1610,12 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
         inter = elems = outs = scanned = items_done = fsp = busy = 0;
         for (uint32_t dd = 0; dd < SGM_MAXQ; dd++) S.nodes[dd] = 0;
     };
-    const uint32_t total = item_first[nq];
+    QueryWalk walk;
+    qw_init(walk, nq, blockIdx.x * (blockDim.x >> 6) + wave);
     while (true) {
-        uint32_t it = 0;
-        if (lane == 0) it = atomicAdd(work, 1u);
-        it = uni(__shfl((int)it, 0));
-        if (it >= total) break;
-        if (__atomic_load_n(dev_stop, __ATOMIC_RELAXED)) break;
+        uint32_t q = 0, it = 0;
+        if (!qw_next(walk, item_first, next, stops, dev_stop, q, it)) break;
         relay_stop(wall_clock64(), relay_at, host_stop, dev_stop);
-        // the item's query: the last q with item_first[q] <= it
-        uint32_t lo = 0, hi = nq;
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (item_first[mid] <= it) lo = mid;
-            else hi = mid;
-        }
-        const uint32_t q = uni(lo);
         if (q != cur) {
             flush_all(false);
             cur = q;
@@ -1576,7 +1628,7 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
         }
         if (__atomic_load_n(stop, __ATOMIC_RELAXED)) continue;  // capped (or voided) query
         const sgm_auxplan_dev& A = *Ap;
-        const uint32_t root = it - item_first[q];
+        const uint32_t root = it;
         const uint32_t v0 = A.cand[0][root];
         if (v0 < root_lo || v0 >= root_hi) continue;  // outside the launch's root range
         items_done++;
@@ -1708,13 +1760,35 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
                         const uint32_t* cand = A.cand[L];
                         uint32_t ok = 0;
                         uint64_t cf = 0;
-                        for (uint32_t base = 0; base < Fn; base += 64) {
-                            const uint32_t i = base + lane;
-                            const bool valid = i < Fn;
-                            ok += aux_count_chunk(valid, valid ? F[i] : 0u, cand, S, L, anc, use_fs ? &cf : nullptr,
-                                                  L - 1);
+                        if (Fn <= 64) {
+                            for (uint32_t base = 0; base < Fn; base += 64) {
+                                const uint32_t i = base + lane;
+                                const bool valid = i < Fn;
+                                ok += aux_count_chunk(valid, valid ? F[i] : 0u, cand, S, L, anc,
+                                                      use_fs ? &cf : nullptr, L - 1);
+                            }
+                            scanned += Fn;
+                        } else {
+                            // a long F (a hub's row): lane j looks for depth j's vertex in F
+                            // (F's positions ascend with their vertices) instead of F's every
+                            // entry being compared with the matched vertices
+                            const uint32_t nd = L - 1;
+                            bool hit = false;
+                            if (lane < nd) {
+                                const uint32_t v = S.emb[lane];
+                                uint32_t lo = 0, hi = Fn;
+                                while (lo < hi) {
+                                    const uint32_t mid = (lo + hi) >> 1;
+                                    if (cand[F[mid]] < v) lo = mid + 1;
+                                    else hi = mid;
+                                }
+                                hit = lo < Fn && cand[F[lo]] == v;
+                            }
+                            const uint64_t hb = __ballot(hit);
+                            ok = Fn - (uint32_t)__popcll(hb);
+                            if (use_fs && hb) cf = lanes_or64(hb, hit ? (anc[L] | anc[lane]) : 0ull);
+                            if (lane == 0) S.probes += (unsigned long long)nd * (1u + 2u * probe_cost(Fn, false));
                         }
-                        scanned += Fn;
                         S.baseA = ok;
                         S.candsA = Fn;
                         S.cfA = cf;
@@ -1905,7 +1979,7 @@ __device__ void dyn_pop(uint64_t* f, int len)
 __global__ void __launch_bounds__(256) k_enum_dyn(const sgm_dynplan_dev* __restrict__ plans,
                                                   const uint32_t* __restrict__ item_first, uint32_t nq,
                                                   uint32_t* __restrict__ stack, uint32_t stack_words,
-                                                  uint32_t* __restrict__ work, sgm_stats_dev* __restrict__ stats,
+                                                  uint32_t* __restrict__ next, sgm_stats_dev* __restrict__ stats,
                                                   uint32_t* __restrict__ stops, uint32_t* __restrict__ dev_stop,
                                                   const uint32_t* __restrict__ host_stop, unsigned long long cap,
                                                   unsigned long long budget, uint32_t root_lo, uint32_t root_hi)
@@ -1947,21 +2021,12 @@ __global__ void __launch_bounds__(256) k_enum_dyn(const sgm_dynplan_dev* __restr
         wave_sync();
         inter = elems = outs = scanned = items_done = busy = 0;
     };
-    const uint32_t total = item_first[nq];
+    QueryWalk walk;
+    qw_init(walk, nq, blockIdx.x * (blockDim.x >> 6) + wave);
     while (true) {
-        uint32_t it = 0;
-        if (lane == 0) it = atomicAdd(work, 1u);
-        it = uni(__shfl((int)it, 0));
-        if (it >= total) break;
-        if (__atomic_load_n(dev_stop, __ATOMIC_RELAXED)) break;
+        uint32_t q = 0, it = 0;
+        if (!qw_next(walk, item_first, next, stops, dev_stop, q, it)) break;
         relay_stop(wall_clock64(), relay_at, host_stop, dev_stop);
-        uint32_t lo = 0, hi = nq;
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (item_first[mid] <= it) lo = mid;
-            else hi = mid;
-        }
-        const uint32_t q = uni(lo);
         if (q != cur) {
             flush_all(false);
             cur = q;
@@ -1974,7 +2039,7 @@ __global__ void __launch_bounds__(256) k_enum_dyn(const sgm_dynplan_dev* __restr
         if (__atomic_load_n(stop, __ATOMIC_RELAXED)) continue;
         const sgm_dynplan_dev& P = *Pp;
         const uint32_t u0 = P.root;
-        const uint32_t root = it - item_first[q];
+        const uint32_t root = it;
         const uint32_t v0 = P.cand[u0][root];
         if (v0 < root_lo || v0 >= root_hi) continue;
         items_done++;
@@ -2213,24 +2278,24 @@ void launch_enum_cache(hipStream_t s, uint32_t blocks, const uint32_t* off, cons
 }
 
 void launch_enum_aux(hipStream_t s, uint32_t blocks, const sgm_auxplan_dev* plans, const uint32_t* item_first,
-                     uint32_t nq, uint32_t* stack, uint32_t stack_words, uint32_t* work, sgm_stats_dev* st,
+                     uint32_t nq, uint32_t* stack, uint32_t stack_words, uint32_t* next, sgm_stats_dev* st,
                      uint32_t* stops, uint32_t* dev_stop, const uint32_t* host_stop, unsigned long long cap,
                      unsigned long long budget, uint32_t root_lo, uint32_t root_hi, int use_fs,
                      const uint32_t* hub_slot, const uint32_t* hub_bits, uint32_t hub_words)
 {
     const size_t smem = 4 * ((sizeof(WaveAux) + 15) & ~(size_t)15);
-    hipLaunchKernelGGL(k_enum_aux, dim3(blocks), dim3(256), smem, s, plans, item_first, nq, stack, stack_words, work,
+    hipLaunchKernelGGL(k_enum_aux, dim3(blocks), dim3(256), smem, s, plans, item_first, nq, stack, stack_words, next,
                        st, stops, dev_stop, host_stop, cap, budget, root_lo, root_hi, use_fs, hub_slot, hub_bits,
                        hub_words);
 }
 
 void launch_enum_dyn(hipStream_t s, uint32_t blocks, const sgm_dynplan_dev* plans, const uint32_t* item_first,
-                     uint32_t nq, uint32_t* stack, uint32_t stack_words, uint32_t* work, sgm_stats_dev* st,
+                     uint32_t nq, uint32_t* stack, uint32_t stack_words, uint32_t* next, sgm_stats_dev* st,
                      uint32_t* stops, uint32_t* dev_stop, const uint32_t* host_stop, unsigned long long cap,
                      unsigned long long budget, uint32_t root_lo, uint32_t root_hi)
 {
     const size_t smem = 4 * ((sizeof(WaveDyn) + 15) & ~(size_t)15);
-    hipLaunchKernelGGL(k_enum_dyn, dim3(blocks), dim3(256), smem, s, plans, item_first, nq, stack, stack_words, work,
+    hipLaunchKernelGGL(k_enum_dyn, dim3(blocks), dim3(256), smem, s, plans, item_first, nq, stack, stack_words, next,
                        st, stops, dev_stop, host_stop, cap, budget, root_lo, root_hi);
 }
 

@@ -77,13 +77,13 @@ size_t enum_smem_bytes(uint32_t max_ops);
 // memory), copied into dev_stop by the polling waves; budget: wave-time ticks per query (0 = none);
 // only depth-0 candidates v with root_lo <= v < root_hi are searched
 void launch_enum_aux(hipStream_t s, uint32_t blocks, const sgm_auxplan_dev* plans, const uint32_t* item_first,
-                     uint32_t nq, uint32_t* stack, uint32_t stack_words, uint32_t* work, sgm_stats_dev* st,
+                     uint32_t nq, uint32_t* stack, uint32_t stack_words, uint32_t* next, sgm_stats_dev* st,
                      uint32_t* stops, uint32_t* dev_stop, const uint32_t* host_stop, unsigned long long cap,
                      unsigned long long budget, uint32_t root_lo, uint32_t root_hi, int use_fs,
                      const uint32_t* hub_slot, const uint32_t* hub_bits, uint32_t hub_words);
 // the dynamic-order searches (DAF / VEQ ranking) over sgm_dynplan_dev plans, items as above
 void launch_enum_dyn(hipStream_t s, uint32_t blocks, const sgm_dynplan_dev* plans, const uint32_t* item_first,
-                     uint32_t nq, uint32_t* stack, uint32_t stack_words, uint32_t* work, sgm_stats_dev* st,
+                     uint32_t nq, uint32_t* stack, uint32_t stack_words, uint32_t* next, sgm_stats_dev* st,
                      uint32_t* stops, uint32_t* dev_stop, const uint32_t* host_stop, unsigned long long cap,
                      unsigned long long budget, uint32_t root_lo, uint32_t root_hi);
 // bitmap of C(u_L) per query (Q.lastbits) from its candidate words; before launch_enum_cache

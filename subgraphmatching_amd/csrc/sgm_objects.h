@@ -34,7 +34,8 @@ struct sgm_ctx {
     uint32_t* d_afirst = nullptr;
     sgm_stats_dev* d_astats = nullptr;
     uint32_t* d_astop = nullptr;
-    uint32_t aux_cap = 0, afirst_cap = 0, astats_cap = 0, astop_cap = 0;
+    uint32_t* d_anext = nullptr;  // per-query item counters
+    uint32_t aux_cap = 0, afirst_cap = 0, astats_cap = 0, astop_cap = 0, anext_cap = 0;
     sgm_dynplan_dev* d_dynplans = nullptr;   // dynamic-order search plans (shares the buffers above)
     uint32_t dyn_cap = 0;
     sgm_wdesc* d_wdesc = nullptr;            // DAF weight row passes of a batch
