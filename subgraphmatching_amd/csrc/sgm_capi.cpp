@@ -85,6 +85,7 @@ void sgm_ctx_destroy(sgm_ctx* c)
     if (c->h_rprog) (void)hipHostFree(c->h_rprog);
     dfree(c->d_tb);
     dfree(c->d_ttot);
+    dfree(c->d_poff);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
     if (c->h_stop) (void)hipHostFree(c->h_stop);
     dfree(c->d_lastbits);
@@ -376,6 +377,7 @@ void sgm_query_destroy(sgm_query* q)
     dfree(q->d_toff);
     dfree(q->d_tidx);
     dfree(q->d_tscr);
+    dfree(q->d_tpool);
     dfree(q->d_weight);
     dfree(q->d_guard);
     dfree(q->d_vmask);
@@ -786,28 +788,182 @@ int sgm_cand_exist_download(sgm_query* q, uint32_t* words, uint32_t num_words)
 }
 
 // ------------------------------------------------------------------------------ tables
-// buildTables (inc/table.h:7-137) for every directed edge of many queries at once: per query
-// its candidate lists (one compaction launch each), then for the whole set one row-count
-// launch (grouped by source vertex), one scan launch (one block per directed edge), one
-// read-back of the totals, and one fill launch -- two host synchronisations in all.  Queries
+// buildTables (inc/table.h:7-137) for every directed edge of many queries at once.  The row
+// of edge (a, b) at candidate v of a is N(v) n C(b) as positions in C(b): it depends on
+// (b, v) only, so each query's rows are made once per (b, v) in a row pool (v in U_b, the
+// union of C(a) over b's query neighbours a), and edge (a, b) either aliases b's pool (when
+// every row of it belongs to C(a), the usual case for the neighbour with the largest
+// candidate set) or gets a copy of the pool rows of C(a).  Per pass of queries (their pool
+// offsets share one scratch of n (V + 1) words per query): the pool row lengths (one launch),
+// the scans (pool offsets per (query, b), row offsets per edge), one read-back of the
+// totals, then the pool rows and the copies -- two host synchronisations per pass.  Queries
 // whose candidate sets are empty are skipped (no search reads their tables).
+static int build_tables_pass(sgm_ctx* c, sgm_graph* g, sgm_query* const* qs, const uint32_t* idx, uint32_t nqp,
+                             size_t scratch_words)
+{
+    const uint32_t V = g->V, W = (V + 63) / 64;
+    const size_t per_b = (size_t)V + 1;
+    HIPCHK(c, grow_arena(&c->d_poff, c->poff_cap, scratch_words));
+    struct Edge {
+        uint32_t k, a, b;
+        size_t o, r;  // offsets slice in the query's toff arena; row counts at tscr + r
+    };
+    std::vector<Edge> edges;
+    std::vector<size_t> sbase(nqp);  // each query's block in the scratch
+    std::vector<uint32_t> pfirst(nqp + 1, 0);  // each query's first pool scan
+    uint32_t kmax = 0, max_rows = 0;
+    size_t sw = 0;
+    for (uint32_t k = 0; k < nqp; k++) {
+        sgm_query* q = qs[idx[k]];
+        sbase[k] = sw;
+        sw += (size_t)q->q.n * per_b;
+        pfirst[k + 1] = pfirst[k] + q->q.n;
+        kmax = std::max(kmax, q->q.n);
+        size_t o = 0, r = 0;
+        for (uint32_t a = 0; a < q->q.n; a++)
+            for (uint32_t p = q->q.off[a]; p < q->q.off[a + 1]; p++) {
+                edges.push_back(Edge{k, a, q->q.adj[p], o, r});
+                o += (size_t)q->counts[a] + 1;
+                r += q->counts[a];
+                max_rows = std::max(max_rows, q->counts[a]);
+            }
+    }
+    const uint32_t M = (uint32_t)edges.size(), NP = pfirst[nqp];
+    if (M == 0) return SGM_OK;
+    // device descriptors [pools nqp][edges M][scans NP + M]; pinned staging holds the same
+    // plus the totals and maxima read back (pool scans, then edge scans)
+    const size_t pool_b = stage_align(sizeof(sgm_pooldesc) * nqp), edge_b = stage_align(sizeof(sgm_edgedesc) * M);
+    const size_t scan_b = stage_align(sizeof(sgm_scandesc) * (NP + M)), dev_b = pool_b + edge_b + scan_b;
+    const uint32_t NT = NP + M;
+    if (c->tb_bytes < dev_b) {
+        dfree(c->d_tb);
+        c->d_tb = nullptr;
+        c->tb_bytes = 0;
+        HIPCHK(c, dalloc(&c->d_tb, dev_b));
+        c->tb_bytes = dev_b;
+    }
+    if (c->ttot_cap < 2 * NT) {
+        dfree(c->d_ttot);
+        c->d_ttot = nullptr;
+        c->ttot_cap = 0;
+        HIPCHK(c, dalloc(&c->d_ttot, 2 * (size_t)NT));
+        c->ttot_cap = 2 * NT;
+    }
+    std::vector<uint8_t> fallback;
+    uint8_t* st = static_cast<uint8_t*>(host_stage(c, dev_b + sizeof(uint32_t) * 2 * NT));
+    if (!st) {
+        fallback.resize(dev_b + sizeof(uint32_t) * 2 * NT);
+        st = fallback.data();
+    }
+    sgm_pooldesc* pd = reinterpret_cast<sgm_pooldesc*>(st);
+    sgm_edgedesc* ed = reinterpret_cast<sgm_edgedesc*>(st + pool_b);
+    sgm_scandesc* sd = reinterpret_cast<sgm_scandesc*>(st + pool_b + edge_b);
+    uint32_t* h = reinterpret_cast<uint32_t*>(st + dev_b);  // totals[NT], maxima[NT]
+    uint8_t* dev = reinterpret_cast<uint8_t*>(c->d_tb);
+    for (uint32_t k = 0; k < nqp; k++) {
+        sgm_query* q = qs[idx[k]];
+        const uint32_t n = q->q.n;
+        sgm_pooldesc& P = pd[k];
+        std::memset(&P, 0, sizeof P);
+        P.vmask = q->d_vmask;
+        P.poff = c->d_poff + sbase[k];
+        P.bm = q->d_bm;
+        P.rank = q->d_rank;
+        P.n = n;
+        for (uint32_t u = 0; u < n; u++)
+            for (uint32_t p = q->q.off[u]; p < q->q.off[u + 1]; p++) P.nbr[u] |= 1u << q->q.adj[p];
+        for (uint32_t b = 0; b < n; b++) {
+            const uint32_t j = pfirst[k] + b;
+            sd[j] = sgm_scandesc{P.poff + b * per_b, P.poff + b * per_b, c->d_ttot + j, c->d_ttot + NT + j, V, 0};
+        }
+    }
+    for (uint32_t e = 0; e < M; e++) {
+        const Edge& E = edges[e];
+        sgm_query* q = qs[idx[E.k]];
+        const uint32_t n = q->q.n, rows = q->counts[E.a];
+        sgm_table& t = q->tables[(size_t)E.a * n + E.b];
+        t.d_off = q->d_toff + E.o;
+        ed[e] = sgm_edgedesc{q->d_list + q->cand_off[E.a], pd[E.k].poff + E.b * per_b, q->d_tscr + E.r, t.d_off,
+                             nullptr, nullptr, rows, 0};
+        sd[NP + e] = sgm_scandesc{q->d_tscr + E.r, t.d_off, c->d_ttot + NP + e, c->d_ttot + NT + NP + e, rows, 0};
+    }
+    HIPCHK(c, hipMemcpyAsync(dev, st, dev_b, hipMemcpyHostToDevice, c->stream));
+    const sgm_pooldesc* dpd = reinterpret_cast<const sgm_pooldesc*>(dev);
+    const sgm_edgedesc* ded = reinterpret_cast<const sgm_edgedesc*>(dev + pool_b);
+    const sgm_scandesc* dsd = reinterpret_cast<const sgm_scandesc*>(dev + pool_b + edge_b);
+    sgm::launch_pool_count(c->stream, g->d_off, g->d_adj, V, dpd, nqp, kmax);
+    sgm::launch_scan_segments(c->stream, dsd, NP);
+    sgm::launch_row_counts(c->stream, ded, M, max_rows);
+    sgm::launch_scan_segments(c->stream, dsd + NP, M);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(h, c->d_ttot, sizeof(uint32_t) * 2 * NT, hipMemcpyDeviceToHost, c->stream));
+    hipError_t se = hipStreamSynchronize(c->stream);
+    if (se != hipSuccess) return set_err(c, SGM_EHIP, "table count: %s", hipGetErrorString(se));
+    // pools, and the entries of the edges that do not alias one, into each query's arenas
+    uint32_t ncopy = 0;
+    for (uint32_t k = 0; k < nqp; k++) {
+        sgm_query* q = qs[idx[k]];
+        const uint32_t n = q->q.n;
+        sgm_pooldesc& P = pd[k];
+        size_t pool = 0;
+        for (uint32_t b = 0; b < n; b++) {
+            P.pbase[b] = pool;
+            pool += h[pfirst[k] + b];
+        }
+        if (grow_arena(&q->d_tpool, q->tpool_cap, std::max<size_t>(pool, 1)) != hipSuccess)
+            return set_err(c, SGM_ENOMEM, "table row pool of %llu entries", (unsigned long long)pool);
+        P.pool = q->d_tpool;
+    }
+    std::vector<size_t> own(nqp, 0);  // entries of the copied edges per query
+    for (uint32_t e = 0; e < M; e++) {
+        const Edge& E = edges[e];
+        if (h[NP + e] != h[pfirst[E.k] + E.b]) own[E.k] += h[NP + e];
+    }
+    for (uint32_t k = 0; k < nqp; k++) {
+        sgm_query* q = qs[idx[k]];
+        if (own[k] && grow_arena(&q->d_tidx, q->tidx_cap, own[k]) != hipSuccess)
+            return set_err(c, SGM_ENOMEM, "tables of %llu entries", (unsigned long long)own[k]);
+        own[k] = 0;
+    }
+    for (uint32_t e = 0; e < M; e++) {
+        const Edge& E = edges[e];
+        sgm_query* q = qs[idx[E.k]];
+        sgm_table& t = q->tables[(size_t)E.a * q->q.n + E.b];
+        const uint32_t* pool_b = q->d_tpool + pd[E.k].pbase[E.b];
+        t.e_cnt = h[NP + e];
+        t.max_deg = h[NT + NP + e];
+        if (t.e_cnt == h[pfirst[E.k] + E.b]) {
+            // every row of b's pool is a row of C(a) (C(a) is a subset of U_b, and the rows of
+            // U_b \ C(a) are empty): the pool is the edge's table, offsets included
+            t.d_idx = const_cast<uint32_t*>(pool_b);
+        } else {
+            t.d_idx = q->d_tidx + own[E.k];
+            own[E.k] += t.e_cnt;
+            sgm_edgedesc& D = ed[ncopy++];
+            D = ed[e];
+            D.src = pool_b;
+            D.dst = t.d_idx;
+        }
+        t.built = true;
+    }
+    HIPCHK(c, hipMemcpyAsync(dev, st, pool_b + sizeof(sgm_edgedesc) * ncopy, hipMemcpyHostToDevice, c->stream));
+    sgm::launch_pool_fill(c->stream, g->d_off, g->d_adj, V, dpd, nqp, kmax);
+    sgm::launch_edge_copy(c->stream, ded, ncopy, max_rows);
+    HIPCHK(c, hipGetLastError());
+    se = hipStreamSynchronize(c->stream);
+    if (se != hipSuccess) return set_err(c, SGM_EHIP, "tables: %s", hipGetErrorString(se));
+    return SGM_OK;
+}
+
 extern "C++" int build_tables_many(sgm_query* const* qs, uint32_t nq)
 {
     if (!nq) return SGM_OK;
     sgm_graph* g = qs[0]->g;
     sgm_ctx* c = g->ctx;
-    const uint32_t W = (g->V + 63) / 64;
     HIPCHK(c, hipSetDevice(c->device));
-    struct Edge {
-        uint32_t q, a, b;
-        size_t o, r;  // offsets slice in the query's toff arena; row counts at tscr + r
-    };
-    std::vector<Edge> edges;
-    std::vector<uint32_t> first(nq + 1, 0);
-    uint32_t NS = 0, max_rows = 0;
+    std::vector<uint32_t> todo;
     for (uint32_t i = 0; i < nq; i++) {
         sgm_query* q = qs[i];
-        first[i] = (uint32_t)edges.size();
         if (q->g != g) return set_err(c, SGM_EINVAL, "tables: queries must share one graph");
         int rc = check_filtered(q, "tables");
         if (rc) return rc;
@@ -829,135 +985,32 @@ extern "C++" int build_tables_many(sgm_query* const* qs, uint32_t nq)
         }
         size_t o = 0, r = 0;
         for (uint32_t a = 0; a < q->q.n; a++) {
-            if (q->q.off[a + 1] > q->q.off[a]) NS++;
-            for (uint32_t p = q->q.off[a]; p < q->q.off[a + 1]; p++) {
-                edges.push_back(Edge{i, a, q->q.adj[p], o, r});
-                o += (size_t)q->counts[a] + 1;
-                r += q->counts[a];
-                max_rows = std::max(max_rows, q->counts[a]);
-            }
+            const uint32_t deg = q->q.off[a + 1] - q->q.off[a];
+            o += (size_t)deg * (q->counts[a] + 1);
+            r += (size_t)deg * q->counts[a];
         }
-        HIPCHK(c, grow_arena(&q->d_toff, q->toff_cap, o));
+        HIPCHK(c, grow_arena(&q->d_toff, q->toff_cap, std::max<size_t>(o, 1)));
         HIPCHK(c, grow_arena(&q->d_tscr, q->tscr_cap, std::max<size_t>(r, 1)));
+        todo.push_back(i);
     }
-    first[nq] = (uint32_t)edges.size();
-    const uint32_t M = (uint32_t)edges.size();
-    if (M == 0) {
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        return SGM_OK;
+    // passes of queries whose pool offsets fit one scratch of at most 2^29 words (a single
+    // larger query takes a pass of its own)
+    const size_t per_q_b = (size_t)g->V + 1, limit = (size_t)1 << 29;
+    int rc = SGM_OK;
+    for (size_t k0 = 0; k0 < todo.size() && rc == SGM_OK;) {
+        size_t k1 = k0, words = 0;
+        while (k1 < todo.size() && (k1 == k0 || words + qs[todo[k1]]->q.n * per_q_b <= limit))
+            words += qs[todo[k1++]]->q.n * per_q_b;
+        rc = build_tables_pass(c, g, qs, todo.data() + k0, (uint32_t)(k1 - k0), words);
+        k0 = k1;
     }
-    // descriptors: [sources NS][scans M][fills NS] on the device; the same plus the totals and
-    // maxima read back (2M words) in pinned staging
-    const size_t src_b = stage_align(sizeof(sgm_tabsrc) * NS), scan_b = stage_align(sizeof(sgm_scandesc) * M);
-    const size_t fill_b = stage_align(sizeof(sgm_tabfill) * NS), dev_b = src_b + scan_b + fill_b;
-    if (c->tb_bytes < dev_b) {
-        dfree(c->d_tb);
-        c->d_tb = nullptr;
-        c->tb_bytes = 0;
-        HIPCHK(c, dalloc(&c->d_tb, dev_b));
-        c->tb_bytes = dev_b;
+    if (rc == SGM_OK && todo.empty()) {
+        const hipError_t se = hipStreamSynchronize(c->stream);
+        if (se != hipSuccess) rc = set_err(c, SGM_EHIP, "tables: %s", hipGetErrorString(se));
     }
-    if (c->ttot_cap < 2 * M) {
-        dfree(c->d_ttot);
-        c->d_ttot = nullptr;
-        c->ttot_cap = 0;
-        HIPCHK(c, dalloc(&c->d_ttot, 2 * (size_t)M));
-        c->ttot_cap = 2 * M;
-    }
-    std::vector<uint8_t> fallback;
-    uint8_t* st = static_cast<uint8_t*>(host_stage(c, dev_b + sizeof(uint32_t) * 2 * M));
-    if (!st) {
-        fallback.resize(dev_b + sizeof(uint32_t) * 2 * M);
-        st = fallback.data();
-    }
-    sgm_tabsrc* td = reinterpret_cast<sgm_tabsrc*>(st);
-    sgm_scandesc* sd = reinterpret_cast<sgm_scandesc*>(st + src_b);
-    sgm_tabfill* tf = reinterpret_cast<sgm_tabfill*>(st + src_b + scan_b);
-    uint32_t* h = reinterpret_cast<uint32_t*>(st + dev_b);  // totals[M], maxima[M]
-    uint8_t* dev = reinterpret_cast<uint8_t*>(c->d_tb);
-    uint32_t ns = 0;
-    std::vector<uint32_t> fill_of(M);  // each edge's fill descriptor (its source's)
-    for (uint32_t k = 0; k < M; k++) {
-        const Edge& E = edges[k];
-        sgm_query* q = qs[E.q];
-        const uint32_t n = q->q.n, rows = q->counts[E.a];
-        sgm_table& t = q->tables[(size_t)E.a * n + E.b];
-        t.d_off = q->d_toff + E.o;
-        // edges out of one vertex are consecutive: one count descriptor per run of them
-        if (k == 0 || edges[k - 1].q != E.q || edges[k - 1].a != E.a) {
-            sgm_tabsrc& S = td[ns];
-            S = sgm_tabsrc{};
-            S.rows = q->d_list + q->cand_off[E.a];
-            S.vmask = q->d_vmask;
-            S.nrows = rows;
-            sgm_tabfill& F = tf[ns];
-            F = sgm_tabfill{};
-            F.rows = S.rows;
-            F.vmask = S.vmask;
-            F.nrows = rows;
-            ns++;
-        }
-        sgm_tabsrc& S = td[ns - 1];
-        const uint32_t j = S.k++;
-        S.b[j] = E.b;
-        S.rowcnt[j] = q->d_tscr + E.r;
-        sd[k] = sgm_scandesc{q->d_tscr + E.r, t.d_off, c->d_ttot + k, c->d_ttot + M + k, rows, 0};
-        sgm_tabfill& F = tf[ns - 1];
-        F.k = S.k;
-        F.b[j] = E.b;
-        F.rowoff[j] = t.d_off;
-        F.bm[j] = q->d_bm + (size_t)E.b * W;
-        F.rank[j] = q->d_rank + (size_t)E.b * W;
-        fill_of[k] = ns - 1;
-    }
-    HIPCHK(c, hipMemcpyAsync(dev, st, src_b + scan_b, hipMemcpyHostToDevice, c->stream));
-    uint32_t kmax = 0;
-    for (uint32_t i = 0; i < ns; i++) kmax = std::max(kmax, td[i].k);
-    sgm::launch_table_count(c->stream, g->d_off, g->d_adj, reinterpret_cast<const sgm_tabsrc*>(dev), ns, max_rows, kmax);
-    sgm::launch_scan_segments(c->stream, reinterpret_cast<const sgm_scandesc*>(dev + src_b), M);
-    HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipMemcpyAsync(h, c->d_ttot, sizeof(uint32_t) * 2 * M, hipMemcpyDeviceToHost, c->stream));
-    hipError_t se = hipStreamSynchronize(c->stream);
-    if (se != hipSuccess) {
+    if (rc != SGM_OK)
         for (uint32_t i = 0; i < nq; i++) free_tables(qs[i]);
-        return set_err(c, SGM_EHIP, "table count: %s", hipGetErrorString(se));
-    }
-    // entries of each query into its arena
-    for (uint32_t i = 0; i < nq; i++) {
-        sgm_query* q = qs[i];
-        size_t ent = 0;
-        for (uint32_t k = first[i]; k < first[i + 1]; k++) ent += h[k];
-        if (first[i] == first[i + 1]) continue;
-        hipError_t e = grow_arena(&q->d_tidx, q->tidx_cap, std::max<size_t>(ent, 1));
-        if (e != hipSuccess) {
-            for (uint32_t j = 0; j < nq; j++) free_tables(qs[j]);
-            return set_err(c, SGM_ENOMEM, "tables of %llu entries", (unsigned long long)ent);
-        }
-        size_t base = 0;
-        for (uint32_t k = first[i]; k < first[i + 1]; k++) {
-            sgm_table& t = q->tables[(size_t)edges[k].a * q->q.n + edges[k].b];
-            t.e_cnt = h[k];
-            t.max_deg = h[M + k];
-            t.d_idx = q->d_tidx + base;
-            base += h[k];
-            sgm_tabfill& F = tf[fill_of[k]];
-            for (uint32_t j = 0; j < F.k; j++)
-                if (F.b[j] == edges[k].b) {
-                    F.eidx[j] = t.d_idx;
-                }
-            t.built = true;
-        }
-    }
-    HIPCHK(c, hipMemcpyAsync(dev + src_b + scan_b, tf, sizeof(sgm_tabfill) * ns, hipMemcpyHostToDevice, c->stream));
-    sgm::launch_table_fill(c->stream, g->d_off, g->d_adj, reinterpret_cast<const sgm_tabfill*>(dev + src_b + scan_b), ns,
-                           max_rows, kmax);
-    HIPCHK(c, hipGetLastError());
-    se = hipStreamSynchronize(c->stream);
-    if (se != hipSuccess) {
-        for (uint32_t i = 0; i < nq; i++) free_tables(qs[i]);
-        return set_err(c, SGM_EHIP, "tables: %s", hipGetErrorString(se));
-    }
-    return SGM_OK;
+    return rc;
 }
 
 int sgm_build_tables(sgm_query* q)

@@ -77,16 +77,6 @@ struct sgm_dynplan_dev {
 // from b_j's bitmap and ranks) go to eidx[j] at its scanned row offsets rowoff[j]; vmask =
 // the query's membership words (descriptors of many queries share one launch).  The
 // reference's _edge_v (the candidate at each position) is not stored: C(b_j)[position].
-struct sgm_tabfill {
-    const uint32_t* rows;
-    const uint32_t* vmask;
-    uint32_t nrows, k;
-    uint32_t b[SGM_MAXQ];
-    const uint32_t* rowoff[SGM_MAXQ];
-    uint32_t* eidx[SGM_MAXQ];
-    const uint64_t* bm[SGM_MAXQ];
-    const uint32_t* rank[SGM_MAXQ];
-};
 
 // One row pass of computeWeightArray (inc/order.h:997-1029) in a batched launch: rows of the
 // table (x -> child) given by toff / tidx, child weights `in`, x's weights `out`; in == null
@@ -110,14 +100,32 @@ struct sgm_scandesc {
     uint32_t n, pad;
 };
 
-// The directed query edges (a, b_j), j < k, out of one query vertex a in the count pass of an
-// aux-table build (inc/table.h:7-137): rows = C(a); the row counts of b_j go to rowcnt[j].
-struct sgm_tabsrc {
+// One query of an aux-table build (inc/table.h:7-137): its row pool holds, for every query
+// vertex b and data vertex v in U_b (v a candidate of some query neighbour of b), the row
+// N(v) n C(b) as positions in C(b).  poff: n blocks of V + 1 words (row lengths, then the
+// pool offsets of b); nbr[u]: the query neighbours of u as a bit mask; pool + pbase[b]: b's
+// rows.
+struct sgm_pooldesc {
+    const uint32_t* vmask;
+    uint32_t* poff;
+    uint32_t* pool;
+    const uint64_t* bm;    // n * W candidate bitmaps
+    const uint32_t* rank;  // n * W bitmap word ranks
+    uint32_t n, pad;
+    uint32_t nbr[SGM_MAXQ];
+    unsigned long long pbase[SGM_MAXQ];
+};
+
+// One directed query edge (a, b) of an aux-table build: rows = C(a), b's pool offsets and
+// entries, the edge's row counts and scanned row offsets, and its entries when they are a copy.
+struct sgm_edgedesc {
     const uint32_t* rows;
-    const uint32_t* vmask;  // the query's membership words
-    uint32_t nrows, k;
-    uint32_t b[SGM_MAXQ];
-    uint32_t* rowcnt[SGM_MAXQ];
+    const uint32_t* poff;
+    uint32_t* rowcnt;
+    const uint32_t* rowoff;
+    const uint32_t* src;
+    uint32_t* dst;
+    uint32_t nrows, pad;
 };
 
 struct sgm_stats_dev {

@@ -51,13 +51,16 @@ void launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t val);
 void launch_bitmaps(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V);
 void launch_rank_scan(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V, uint32_t max_n);
 void launch_compact(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V);
-// aux tables of many queries at once: the row offsets of every directed edge (one scan per
-// edge), the row counts grouped by source vertex, the entries (one edge per block row)
+// aux tables of many queries at once: the row pools' lengths (one block row per query), one
+// scan per segment (pool offsets per (query, b), row offsets per directed edge), the edges'
+// row counts, the pools' entries, and the copies of the edges that do not alias a pool
 void launch_scan_segments(hipStream_t s, const sgm_scandesc* ds, uint32_t m);
-void launch_table_count(hipStream_t s, const uint32_t* off, const uint32_t* adj, const sgm_tabsrc* ds, uint32_t nsrc,
-                        uint32_t max_rows, uint32_t kmax);
-void launch_table_fill(hipStream_t s, const uint32_t* off, const uint32_t* adj, const sgm_tabfill* ds, uint32_t nsrc,
-                       uint32_t max_rows, uint32_t kmax);
+void launch_pool_count(hipStream_t s, const uint32_t* off, const uint32_t* adj, uint32_t V, const sgm_pooldesc* ds,
+                       uint32_t nq, uint32_t kmax);
+void launch_pool_fill(hipStream_t s, const uint32_t* off, const uint32_t* adj, uint32_t V, const sgm_pooldesc* ds,
+                      uint32_t nq, uint32_t kmax);
+void launch_row_counts(hipStream_t s, const sgm_edgedesc* ds, uint32_t m, uint32_t max_rows);
+void launch_edge_copy(hipStream_t s, const sgm_edgedesc* ds, uint32_t m, uint32_t max_rows);
 // BFS-tree filters (CFL / CECI / DAF): step `step` of every query's refinement program
 // (SGM_RPROG_WORDS words per query, see k_refine), and the copy of the NLF words into vmr
 void launch_refine(hipStream_t s, const uint32_t* off, const uint32_t* adj, uint32_t V, const sgm_qdesc* qs,

@@ -749,11 +749,15 @@ __global__ void __launch_bounds__(256) k_compact(const sgm_cdesc* __restrict__ c
 }
 
 // ================================================================== aux tables
-// Table of directed query edge (a,b) (inc/table.h:7-137): for the i-th candidate of a, the
-// ascending positions k in C(b) of its data neighbours that are candidates of b.
-// A count pass per row, an exclusive scan into row offsets, then a fill pass (positions are
-// ranks in b's bitmap, so rows come out sorted because adjacency lists are).  One wave per
-// row, coalesced over N(v); every directed edge of a query in one launch per pass.
+// Table of directed query edge (a,b) (inc/table.h:7-137): for the i-th candidate v of a, the
+// ascending positions k in C(b) of v's data neighbours that are candidates of b.  That row
+// depends on (b, v) only, not on a, so a build makes each query's rows once per (b, v) in a
+// row pool -- for the v in U_b, the union of C(a) over b's query neighbours a -- and gives
+// edge (a,b) either the pool of b itself (when its rows are all in C(a)) or a copy of the
+// pool rows of C(a).  Passes: row lengths per (b, v) in one walk over N(v) for all b at once,
+// a scan per b into pool offsets, the row counts and offsets of every edge, then the pool
+// rows and the copies.  Positions are ranks in b's bitmap, so rows come out sorted because
+// adjacency lists are.
 __device__ __forceinline__ uint32_t cand_rank(const glb<uint64_t>* __restrict__ bm,
                                               const glb<uint32_t>* __restrict__ rank, uint32_t w)
 {
@@ -761,105 +765,151 @@ __device__ __forceinline__ uint32_t cand_rank(const glb<uint64_t>* __restrict__ 
     return rank[word] + (uint32_t)__popcll(bm[word] & ((1ull << bit) - 1ull));  // both loads in flight together
 }
 
-// The aux-table passes (inc/table.h:7-137) walk the candidate rows of a query vertex a for
-// all of a's directed edges (a, b_j) at once, reading N(v) and its membership words once.
-// A wave takes 64 rows (one per lane) and walks their concatenated neighbour lists 64
-// entries at a time: rows average a few dozen entries, so one wave per row would leave most
-// lanes idle behind a chain of dependent loads.  Per (row, edge) state lives in LDS: slot
-// (lane, j) at tl[(wave * 64 + lane) * kmax + j].  In each 64-entry step the entries of one
-// row are a contiguous lane range; its last lane carries the row's hits of the step into
-// the slot.
-// Row counts of every directed edge out of a (blockIdx.y = a's descriptor).  Lane j holds
-// edge j's target bit and output array, read with readlane in the loops.
-__global__ void __launch_bounds__(256) k_table_count_src(const uint32_t* __restrict__ off,
-                                                         const uint32_t* __restrict__ adj,
-                                                         const sgm_tabsrc* __restrict__ ds, uint32_t kmax)
+// U_v: the query vertices b whose pool holds a row of v (v is a candidate of a neighbour of b)
+__device__ __forceinline__ uint32_t pool_umask(const sgm_pooldesc& D, uint32_t vm)
+{
+    uint32_t u = 0;
+    for (uint32_t m = vm; m; m &= m - 1) u |= D.nbr[__ffs(m) - 1];
+    return u;
+}
+
+// A wave takes 64 consecutive data vertices (one per lane) and walks their concatenated
+// neighbour lists 64 entries at a time (rows average a few dozen entries, so one wave per
+// vertex would leave most lanes idle behind a chain of dependent loads).  Per (vertex, b)
+// state lives in LDS: slot (lane, b) at tl[(wave * 64 + lane) * kmax + b].  In each step the
+// entries of one vertex are a contiguous lane range; its last lane carries the vertex's hits
+// of the step into the slot.
+// Row lengths: poff[b * (V + 1) + v] = |N(v) n C(b)| when b is in U_v, else 0 (blockIdx.y =
+// the query).
+__global__ void __launch_bounds__(256) k_pool_count(const uint32_t* __restrict__ off, const uint32_t* __restrict__ adj,
+                                                    uint32_t V, const sgm_pooldesc* __restrict__ ds, uint32_t kmax)
 {
     extern __shared__ uint32_t tl[];
-    const sgm_tabsrc& D = ds[blockIdx.y];
+    const sgm_pooldesc& D = ds[blockIdx.y];
     const glb<uint32_t>* __restrict__ vmask = gcast(D.vmask);
-    const glb<uint32_t>* __restrict__ rows = gcast(D.rows);
+    glb<uint32_t>* __restrict__ poff = gcast(D.poff);
     const uint32_t waves = blockDim.x >> 6, wave = threadIdx.x >> 6, lane = lane_id();
-    const uint32_t nrows = D.nrows, k = D.k;
-    const uint32_t my_b = lane < k ? D.b[lane] : 0u;
-    const uint64_t my_out = lane < k ? (uint64_t)D.rowcnt[lane] : 0ull;
+    const uint32_t n = D.n;
     uint32_t* mine = tl + (wave * 64 + lane) * kmax;
     uint32_t* slots = tl + wave * 64 * kmax;
-    for (uint32_t g = blockIdx.x * waves + wave; (uint64_t)g * 64 < nrows; g += gridDim.x * waves) {
-        const uint32_t r = g * 64 + lane;
-        const bool valid = r < nrows;
-        const uint32_t v = valid ? rows[r] : 0u;
-        const uint32_t o = valid ? off[v] : 0u, len = valid ? off[v + 1] - o : 0u;
-        for (uint32_t j = 0; j < k; j++) mine[j] = 0;
+    for (uint32_t g = blockIdx.x * waves + wave; (uint64_t)g * 64 < V; g += gridDim.x * waves) {
+        const uint32_t v = g * 64 + lane;
+        const bool valid = v < V;
+        const uint32_t um = valid ? pool_umask(D, vmask[v]) : 0u;
+        const uint32_t o = um ? off[v] : 0u, len = um ? off[v + 1] - o : 0u;
+        for (uint32_t b = 0; b < n; b++) mine[b] = 0;
         wave_sync();
+        const uint32_t ub = uni(wave_or32(um));
         const uint32_t incl = wave_incl_scan(len), total = rdlane(incl, 63);
         for (uint32_t b0 = 0; b0 < total; b0 += 64) {
             const TabStep st = tab_step(incl, len, total, b0, lane);
-            const uint32_t oj = (uint32_t)__shfl((int)o, (int)(st.own & 63u));
-            const uint32_t m = st.act ? vmask[adj[oj + st.pos]] : 0u;
+            const uint32_t own = st.own & 63u;
+            const uint32_t oj = (uint32_t)__shfl((int)o, (int)own);
+            const uint32_t uj = (uint32_t)__shfl((int)um, (int)own);
+            const uint32_t m = st.act ? vmask[adj[oj + st.pos]] & uj : 0u;
             const uint64_t upto = st.below | (1ull << lane);
-            for (uint32_t j = 0; j < k; j++) {
-                const uint64_t bal = __ballot((m >> rdlane(my_b, j)) & 1u);
-                if (bal && st.last) slots[st.own * kmax + j] += (uint32_t)__popcll(bal & upto);
+            for (uint32_t bs = ub; bs; bs &= bs - 1) {
+                const uint32_t b = __ffs(bs) - 1;
+                const uint64_t bal = __ballot((m >> b) & 1u);
+                if (bal && st.last) slots[own * kmax + b] += (uint32_t)__popcll(bal & upto);
             }
         }
         wave_sync();
         if (valid)
-            for (uint32_t j = 0; j < k; j++) gcast((uint32_t*)rdlane64(my_out, j))[r] = mine[j];
+            for (uint32_t b = 0; b < n; b++) poff[(size_t)b * (V + 1) + v] = mine[b];
         wave_sync();
     }
 }
 
-// Entries of every directed edge out of a (blockIdx.y = a's descriptor) at the scanned row
-// offsets: a hit's position is its row's next free slot plus the row's hits before it in the
-// step, so each edge's output of a step is one nearly contiguous run.  Lane j holds edge j's
-// target bit and arrays.
-__global__ void __launch_bounds__(256) k_table_fill_src(const uint32_t* __restrict__ off,
-                                                        const uint32_t* __restrict__ adj,
-                                                        const sgm_tabfill* __restrict__ ds, uint32_t kmax)
+// The pool rows: entries of (b, v) at pool + pbase[b] + poff[b][v] on (blockIdx.y = the query).
+__global__ void __launch_bounds__(256) k_pool_fill(const uint32_t* __restrict__ off, const uint32_t* __restrict__ adj,
+                                                   uint32_t V, const sgm_pooldesc* __restrict__ ds, uint32_t kmax)
 {
     extern __shared__ uint32_t tl[];
-    const sgm_tabfill& D = ds[blockIdx.y];
+    const sgm_pooldesc& D = ds[blockIdx.y];
     const glb<uint32_t>* __restrict__ vmask = gcast(D.vmask);
-    const glb<uint32_t>* __restrict__ rows = gcast(D.rows);
+    const glb<uint32_t>* __restrict__ poff = gcast((const uint32_t*)D.poff);
+    const glb<uint64_t>* __restrict__ bm = gcast(D.bm);
+    const glb<uint32_t>* __restrict__ rank = gcast(D.rank);
     const uint32_t waves = blockDim.x >> 6, wave = threadIdx.x >> 6, lane = lane_id();
-    const uint32_t nrows = D.nrows, k = D.k;
-    const bool lj = lane < k;
-    const uint32_t my_b = lj ? D.b[lane] : 0u;
-    const uint64_t my_off = lj ? (uint64_t)D.rowoff[lane] : 0ull, my_idx = lj ? (uint64_t)D.eidx[lane] : 0ull;
-    const uint64_t my_bm = lj ? (uint64_t)D.bm[lane] : 0ull, my_rank = lj ? (uint64_t)D.rank[lane] : 0ull;
+    const uint32_t n = D.n, W = (V + 63) / 64;
+    // lane b holds b's pool base
+    const uint64_t my_base = lane < n ? (uint64_t)(size_t)(D.pool + D.pbase[lane]) : 0ull;
     uint32_t* mine = tl + (wave * 64 + lane) * kmax;
     uint32_t* slots = tl + wave * 64 * kmax;
-    for (uint32_t g = blockIdx.x * waves + wave; (uint64_t)g * 64 < nrows; g += gridDim.x * waves) {
-        const uint32_t r = g * 64 + lane;
-        const bool valid = r < nrows;
-        const uint32_t v = valid ? rows[r] : 0u;
-        const uint32_t o = valid ? off[v] : 0u, len = valid ? off[v + 1] - o : 0u;
-        for (uint32_t j = 0; j < k; j++)
-            mine[j] = valid ? gcast((const uint32_t*)rdlane64(my_off, j))[r] : 0u;
+    for (uint32_t g = blockIdx.x * waves + wave; (uint64_t)g * 64 < V; g += gridDim.x * waves) {
+        const uint32_t v = g * 64 + lane;
+        const bool valid = v < V;
+        const uint32_t um = valid ? pool_umask(D, vmask[v]) : 0u;
+        const uint32_t o = um ? off[v] : 0u, len = um ? off[v + 1] - o : 0u;
+        for (uint32_t b = 0; b < n; b++) mine[b] = (um >> b) & 1u ? poff[(size_t)b * (V + 1) + v] : 0u;
         wave_sync();
+        const uint32_t ub = uni(wave_or32(um));
         const uint32_t incl = wave_incl_scan(len), total = rdlane(incl, 63);
         for (uint32_t b0 = 0; b0 < total; b0 += 64) {
             const TabStep st = tab_step(incl, len, total, b0, lane);
-            const uint32_t oj = (uint32_t)__shfl((int)o, (int)(st.own & 63u));
+            const uint32_t own = st.own & 63u;
+            const uint32_t oj = (uint32_t)__shfl((int)o, (int)own);
+            const uint32_t uj = (uint32_t)__shfl((int)um, (int)own);
             const uint32_t w = st.act ? adj[oj + st.pos] : 0u;
-            const uint32_t m = st.act ? vmask[w] : 0u;
+            const uint32_t m = st.act ? vmask[w] & uj : 0u;
             const uint64_t upto = st.below | (1ull << lane);
-            for (uint32_t j = 0; j < k; j++) {
-                const bool hit = (m >> rdlane(my_b, j)) & 1u;
+            for (uint32_t bs = ub; bs; bs &= bs - 1) {
+                const uint32_t b = __ffs(bs) - 1;
+                const bool hit = (m >> b) & 1u;
                 const uint64_t bal = __ballot(hit);
                 if (!bal) continue;
-                uint32_t* slot = slots + (st.own & 63u) * kmax + j;
+                uint32_t* slot = slots + own * kmax + b;
                 if (hit) {
                     const uint32_t p = *slot + (uint32_t)__popcll(bal & st.below);
-                    const uint32_t rk = cand_rank(gcast((const uint64_t*)rdlane64(my_bm, j)),
-                                                  gcast((const uint32_t*)rdlane64(my_rank, j)), w);
-                    gcast((uint32_t*)rdlane64(my_idx, j))[p] = rk;
+                    gcast((uint32_t*)rdlane64(my_base, b))[p] = cand_rank(bm + (size_t)b * W, rank + (size_t)b * W, w);
                 }
                 if (st.last) *slot += (uint32_t)__popcll(bal & upto);
             }
         }
         wave_sync();
+    }
+}
+
+// Row counts of every directed edge (a, b): out[r] = the pool row length of C(a)[r] in b's
+// offsets (blockIdx.y = the edge).
+__global__ void __launch_bounds__(256) k_row_counts(const sgm_edgedesc* __restrict__ ds)
+{
+    const sgm_edgedesc& D = ds[blockIdx.y];
+    const glb<uint32_t>* __restrict__ rows = gcast(D.rows);
+    const glb<uint32_t>* __restrict__ poff = gcast(D.poff);
+    glb<uint32_t>* __restrict__ out = gcast(D.rowcnt);
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < D.nrows; r += gridDim.x * blockDim.x) {
+        const uint32_t v = rows[r];
+        out[r] = poff[v + 1] - poff[v];
+    }
+}
+
+// An edge's table as a copy of the pool rows of C(a) (blockIdx.y = the edge): a wave takes 64
+// rows and copies their concatenation 64 entries at a time; the destination of the wave's
+// rows is one contiguous run from the first row's offset.
+__global__ void __launch_bounds__(256) k_edge_copy(const sgm_edgedesc* __restrict__ ds)
+{
+    const sgm_edgedesc& D = ds[blockIdx.y];
+    const glb<uint32_t>* __restrict__ rows = gcast(D.rows);
+    const glb<uint32_t>* __restrict__ poff = gcast(D.poff);
+    const glb<uint32_t>* __restrict__ rowoff = gcast((const uint32_t*)D.rowoff);
+    const glb<uint32_t>* __restrict__ src = gcast(D.src);
+    glb<uint32_t>* __restrict__ dst = gcast(D.dst);
+    const uint32_t waves = blockDim.x >> 6, wave = threadIdx.x >> 6, lane = lane_id();
+    const uint32_t nrows = D.nrows;
+    for (uint32_t g = blockIdx.x * waves + wave; (uint64_t)g * 64 < nrows; g += gridDim.x * waves) {
+        const uint32_t r = g * 64 + lane;
+        const bool valid = r < nrows;
+        const uint32_t d0 = rowoff[g * 64];
+        const uint32_t s = valid ? poff[rows[r]] : 0u;
+        const uint32_t len = valid ? rowoff[r + 1] - rowoff[r] : 0u;
+        const uint32_t incl = wave_incl_scan(len), total = rdlane(incl, 63);
+        for (uint32_t b0 = 0; b0 < total; b0 += 64) {
+            const TabStep st = tab_step(incl, len, total, b0, lane);
+            const uint32_t sj = (uint32_t)__shfl((int)s, (int)(st.own & 63u));
+            if (st.act) dst[(size_t)d0 + b0 + lane] = src[(size_t)sj + st.pos];
+        }
     }
 }
 
@@ -1187,29 +1237,50 @@ void launch_compact(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V)
 // grid.y / grid.x hold at most 65535 descriptors per launch: longer lists go in pieces
 static const uint32_t kMaxGridY = 65535;
 
-void launch_table_count(hipStream_t s, const uint32_t* off, const uint32_t* adj, const sgm_tabsrc* ds, uint32_t nsrc,
-                        uint32_t max_rows, uint32_t kmax)
+void launch_pool_count(hipStream_t s, const uint32_t* off, const uint32_t* adj, uint32_t V, const sgm_pooldesc* ds,
+                       uint32_t nq, uint32_t kmax)
 {
-    if (!nsrc || !max_rows || !kmax) return;
-    const uint32_t xcap = 16384 / nsrc > 8 ? 16384 / nsrc : 8u;
+    if (!nq || !V || !kmax) return;
+    const uint32_t xcap = 16384 / nq > 8 ? 16384 / nq : 8u;
     const size_t smem = sizeof(uint32_t) * 4 * 64 * kmax;
-    for (uint32_t s0 = 0; s0 < nsrc; s0 += kMaxGridY) {
-        const uint32_t k = nsrc - s0 < kMaxGridY ? nsrc - s0 : kMaxGridY;
-        dim3 grid(clampg(((uint64_t)max_rows + 255) / 256, 1, xcap), k);
-        hipLaunchKernelGGL(k_table_count_src, grid, dim3(256), smem, s, off, adj, ds + s0, kmax);
+    for (uint32_t s0 = 0; s0 < nq; s0 += kMaxGridY) {
+        const uint32_t k = nq - s0 < kMaxGridY ? nq - s0 : kMaxGridY;
+        dim3 grid(clampg(((uint64_t)V + 255) / 256, 1, xcap), k);
+        hipLaunchKernelGGL(k_pool_count, grid, dim3(256), smem, s, off, adj, V, ds + s0, kmax);
     }
 }
 
-void launch_table_fill(hipStream_t s, const uint32_t* off, const uint32_t* adj, const sgm_tabfill* ds, uint32_t nsrc,
-                       uint32_t max_rows, uint32_t kmax)
+void launch_pool_fill(hipStream_t s, const uint32_t* off, const uint32_t* adj, uint32_t V, const sgm_pooldesc* ds,
+                      uint32_t nq, uint32_t kmax)
 {
-    if (!nsrc || !max_rows || !kmax) return;
-    const uint32_t xcap = 16384 / nsrc > 8 ? 16384 / nsrc : 8u;
+    if (!nq || !V || !kmax) return;
+    const uint32_t xcap = 16384 / nq > 8 ? 16384 / nq : 8u;
     const size_t smem = sizeof(uint32_t) * 4 * 64 * kmax;
-    for (uint32_t s0 = 0; s0 < nsrc; s0 += kMaxGridY) {
-        const uint32_t k = nsrc - s0 < kMaxGridY ? nsrc - s0 : kMaxGridY;
+    for (uint32_t s0 = 0; s0 < nq; s0 += kMaxGridY) {
+        const uint32_t k = nq - s0 < kMaxGridY ? nq - s0 : kMaxGridY;
+        dim3 grid(clampg(((uint64_t)V + 255) / 256, 1, xcap), k);
+        hipLaunchKernelGGL(k_pool_fill, grid, dim3(256), smem, s, off, adj, V, ds + s0, kmax);
+    }
+}
+
+void launch_row_counts(hipStream_t s, const sgm_edgedesc* ds, uint32_t m, uint32_t max_rows)
+{
+    if (!m || !max_rows) return;
+    for (uint32_t s0 = 0; s0 < m; s0 += kMaxGridY) {
+        const uint32_t k = m - s0 < kMaxGridY ? m - s0 : kMaxGridY;
+        dim3 grid(clampg(((uint64_t)max_rows + 255) / 256, 1, 64), k);
+        hipLaunchKernelGGL(k_row_counts, grid, dim3(256), 0, s, ds + s0);
+    }
+}
+
+void launch_edge_copy(hipStream_t s, const sgm_edgedesc* ds, uint32_t m, uint32_t max_rows)
+{
+    if (!m || !max_rows) return;
+    const uint32_t xcap = 16384 / m > 8 ? 16384 / m : 8u;
+    for (uint32_t s0 = 0; s0 < m; s0 += kMaxGridY) {
+        const uint32_t k = m - s0 < kMaxGridY ? m - s0 : kMaxGridY;
         dim3 grid(clampg(((uint64_t)max_rows + 255) / 256, 1, xcap), k);
-        hipLaunchKernelGGL(k_table_fill_src, grid, dim3(256), smem, s, off, adj, ds + s0, kmax);
+        hipLaunchKernelGGL(k_edge_copy, grid, dim3(256), 0, s, ds + s0);
     }
 }
 

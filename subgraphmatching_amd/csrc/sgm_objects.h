@@ -75,6 +75,8 @@ struct sgm_ctx {
     size_t tb_bytes = 0;
     uint32_t* d_ttot = nullptr;
     uint32_t ttot_cap = 0;
+    uint32_t* d_poff = nullptr;  // row pool lengths / offsets of the queries of one build pass
+    size_t poff_cap = 0;
     // batched enumeration: bitmap of C(u_L) per query; depth-1 split blocks and chunk records
     uint32_t* d_lastbits = nullptr;
     size_t lastbits_words = 0;
@@ -189,7 +191,8 @@ struct sgm_query {
     uint32_t* d_toff = nullptr;
     uint32_t* d_tidx = nullptr;
     uint32_t* d_tscr = nullptr;
-    size_t toff_cap = 0, tidx_cap = 0, tscr_cap = 0;
+    uint32_t* d_tpool = nullptr;  // the row pool (edges whose rows are all of b's pool alias it)
+    size_t toff_cap = 0, tidx_cap = 0, tscr_cap = 0, tpool_cap = 0;
     // DAF weights of the dynamic-order search (computeWeightArray), per candidate position in
     // the cand_off layout, for the start order weight_order; dropped with the tables
     uint32_t* d_weight = nullptr;
