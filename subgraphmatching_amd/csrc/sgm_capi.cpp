@@ -86,6 +86,7 @@ void sgm_ctx_destroy(sgm_ctx* c)
     dfree(c->d_tb);
     dfree(c->d_ttot);
     dfree(c->d_poff);
+    dfree(c->d_rrec);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
     if (c->h_stop) (void)hipHostFree(c->h_stop);
     dfree(c->d_lastbits);
@@ -603,11 +604,7 @@ static int filter_chunk(sgm_ctx* c, sgm_graph* g, sgm_query* const* qs, uint32_t
         }
         max_n = std::max(max_n, qs[i]->q.n);
         free_tables(qs[i]);
-        if (qs[i]->d_list) {
-            dfree(qs[i]->d_list);
-            qs[i]->d_list = nullptr;
-        }
-        qs[i]->compacted = false;
+        qs[i]->compacted = false;  // the list arena is kept: a free would synchronise the device
     }
     if (c->fcap < nq) {  // descriptor and count buffers, kept for later batches
         dfree(c->d_fqd);
@@ -752,8 +749,7 @@ static int ensure_compacted(sgm_query* q)
     q->cand_off[0] = 0;
     for (uint32_t u = 0; u < n; u++) q->cand_off[u + 1] = q->cand_off[u] + q->counts[u];
     HIPCHK(c, hipSetDevice(c->device));
-    dfree(q->d_list);
-    HIPCHK(c, dalloc(&q->d_list, q->cand_off[n]));
+    HIPCHK(c, grow_arena(&q->d_list, q->list_cap, std::max<size_t>(q->cand_off[n], 1)));
     q->hc.list = q->d_list;
     HIPCHK(c, hipMemcpy(q->d_cdesc, &q->hc, sizeof q->hc, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(q->d_candoff, q->cand_off, sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice));
@@ -798,25 +794,34 @@ int sgm_cand_exist_download(sgm_query* q, uint32_t* words, uint32_t num_words)
 // the scans (pool offsets per (query, b), row offsets per edge), one read-back of the
 // totals, then the pool rows and the copies -- two host synchronisations per pass.  Queries
 // whose candidate sets are empty are skipped (no search reads their tables).
-static int build_tables_pass(sgm_ctx* c, sgm_graph* g, sgm_query* const* qs, const uint32_t* idx, uint32_t nqp,
-                             size_t scratch_words)
+static inline uint32_t rank_stride(uint32_t n) { return (n + 3) & ~3u; }
+
+static int build_tables_pass(sgm_ctx* c, sgm_graph* g, sgm_query* const* qs, const uint32_t* idx, uint32_t nqp)
 {
-    const uint32_t V = g->V, W = (V + 63) / 64;
+    const uint32_t V = g->V;
     const size_t per_b = (size_t)V + 1;
-    HIPCHK(c, grow_arena(&c->d_poff, c->poff_cap, scratch_words));
+    size_t pw = 0, rw = 0;
+    for (uint32_t k = 0; k < nqp; k++) {
+        pw += (size_t)qs[idx[k]]->q.n * per_b;
+        rw += (size_t)rank_stride(qs[idx[k]]->q.n) * V;
+    }
+    HIPCHK(c, grow_arena(&c->d_poff, c->poff_cap, pw));
+    HIPCHK(c, grow_arena(&c->d_rrec, c->rrec_cap, rw));
     struct Edge {
         uint32_t k, a, b;
         size_t o, r;  // offsets slice in the query's toff arena; row counts at tscr + r
     };
     std::vector<Edge> edges;
-    std::vector<size_t> sbase(nqp);  // each query's block in the scratch
+    std::vector<size_t> sbase(nqp), rbase(nqp);  // each query's blocks in the scratches
     std::vector<uint32_t> pfirst(nqp + 1, 0);  // each query's first pool scan
     uint32_t kmax = 0, max_rows = 0;
-    size_t sw = 0;
+    size_t sw = 0, rs = 0;
     for (uint32_t k = 0; k < nqp; k++) {
         sgm_query* q = qs[idx[k]];
         sbase[k] = sw;
         sw += (size_t)q->q.n * per_b;
+        rbase[k] = rs;
+        rs += (size_t)rank_stride(q->q.n) * V;
         pfirst[k + 1] = pfirst[k] + q->q.n;
         kmax = std::max(kmax, q->q.n);
         size_t o = 0, r = 0;
@@ -867,9 +872,11 @@ static int build_tables_pass(sgm_ctx* c, sgm_graph* g, sgm_query* const* qs, con
         std::memset(&P, 0, sizeof P);
         P.vmask = q->d_vmask;
         P.poff = c->d_poff + sbase[k];
+        P.rrec = c->d_rrec + rbase[k];
         P.bm = q->d_bm;
         P.rank = q->d_rank;
         P.n = n;
+        P.rstride = rank_stride(n);
         for (uint32_t u = 0; u < n; u++)
             for (uint32_t p = q->q.off[u]; p < q->q.off[u + 1]; p++) P.nbr[u] |= 1u << q->q.adj[p];
         for (uint32_t b = 0; b < n; b++) {
@@ -973,9 +980,7 @@ extern "C++" int build_tables_many(sgm_query* const* qs, uint32_t nq)
             const uint32_t n = q->q.n;
             q->cand_off[0] = 0;
             for (uint32_t u = 0; u < n; u++) q->cand_off[u + 1] = q->cand_off[u] + q->counts[u];
-            dfree(q->d_list);
-            q->d_list = nullptr;
-            HIPCHK(c, dalloc(&q->d_list, q->cand_off[n]));
+            HIPCHK(c, grow_arena(&q->d_list, q->list_cap, std::max<size_t>(q->cand_off[n], 1)));
             q->hc.list = q->d_list;
             HIPCHK(c, hipMemcpyAsync(q->d_cdesc, &q->hc, sizeof q->hc, hipMemcpyHostToDevice, c->stream));
             HIPCHK(c, hipMemcpyAsync(q->d_candoff, q->cand_off, sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice,
@@ -993,15 +998,17 @@ extern "C++" int build_tables_many(sgm_query* const* qs, uint32_t nq)
         HIPCHK(c, grow_arena(&q->d_tscr, q->tscr_cap, std::max<size_t>(r, 1)));
         todo.push_back(i);
     }
-    // passes of queries whose pool offsets fit one scratch of at most 2^29 words (a single
-    // larger query takes a pass of its own)
-    const size_t per_q_b = (size_t)g->V + 1, limit = (size_t)1 << 29;
+    // passes of queries whose pool offsets and rank records fit scratches of at most 2^30
+    // words in all (a single larger query takes a pass of its own)
+    auto words_of = [&](const sgm_query* q) {
+        return (size_t)q->q.n * ((size_t)g->V + 1) + (size_t)rank_stride(q->q.n) * g->V;
+    };
+    const size_t limit = (size_t)1 << 30;
     int rc = SGM_OK;
     for (size_t k0 = 0; k0 < todo.size() && rc == SGM_OK;) {
         size_t k1 = k0, words = 0;
-        while (k1 < todo.size() && (k1 == k0 || words + qs[todo[k1]]->q.n * per_q_b <= limit))
-            words += qs[todo[k1++]]->q.n * per_q_b;
-        rc = build_tables_pass(c, g, qs, todo.data() + k0, (uint32_t)(k1 - k0), words);
+        while (k1 < todo.size() && (k1 == k0 || words + words_of(qs[todo[k1]]) <= limit)) words += words_of(qs[todo[k1++]]);
+        rc = build_tables_pass(c, g, qs, todo.data() + k0, (uint32_t)(k1 - k0));
         k0 = k1;
     }
     if (rc == SGM_OK && todo.empty()) {

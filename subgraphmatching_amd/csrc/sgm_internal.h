@@ -103,15 +103,17 @@ struct sgm_scandesc {
 // One query of an aux-table build (inc/table.h:7-137): its row pool holds, for every query
 // vertex b and data vertex v in U_b (v a candidate of some query neighbour of b), the row
 // N(v) n C(b) as positions in C(b).  poff: n blocks of V + 1 words (row lengths, then the
-// pool offsets of b); nbr[u]: the query neighbours of u as a bit mask; pool + pbase[b]: b's
+// pool offsets of b); rrec: V records of rstride words (a multiple of 4 >= n), the position
+// of w in every C(b); nbr[u]: the query neighbours of u as a bit mask; pool + pbase[b]: b's
 // rows.
 struct sgm_pooldesc {
     const uint32_t* vmask;
     uint32_t* poff;
+    uint32_t* rrec;
     uint32_t* pool;
     const uint64_t* bm;    // n * W candidate bitmaps
     const uint32_t* rank;  // n * W bitmap word ranks
-    uint32_t n, pad;
+    uint32_t n, rstride;
     uint32_t nbr[SGM_MAXQ];
     unsigned long long pbase[SGM_MAXQ];
 };

@@ -821,7 +821,29 @@ __global__ void __launch_bounds__(256) k_pool_count(const uint32_t* __restrict__
     }
 }
 
-// The pool rows: entries of (b, v) at pool + pbase[b] + poff[b][v] on (blockIdx.y = the query).
+// Rank records of one query (blockIdx.y): rrec[w * rstride + b] = the position of w in C(b),
+// or SGM_INVALID when w is not a candidate of b (b < rstride, a multiple of 4 >= n), so the
+// pool pass reads every query vertex's rank of a neighbour in one 16-byte-aligned record.
+__global__ void __launch_bounds__(256) k_pool_ranks(uint32_t V, const sgm_pooldesc* __restrict__ ds)
+{
+    const sgm_pooldesc& D = ds[blockIdx.y];
+    const glb<uint32_t>* __restrict__ vmask = gcast(D.vmask);
+    const glb<uint64_t>* __restrict__ bm = gcast(D.bm);
+    const glb<uint32_t>* __restrict__ rank = gcast(D.rank);
+    glb<uint32_t>* __restrict__ rrec = gcast(D.rrec);
+    const uint32_t n = D.n, rs = D.rstride, W = (V + 63) / 64;
+    const uint64_t total = (uint64_t)V * rs;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t w = (uint32_t)(i / rs), b = (uint32_t)(i % rs);
+        uint32_t r = SGM_INVALID;
+        if (b < n && ((vmask[w] >> b) & 1u)) r = cand_rank(bm + (size_t)b * W, rank + (size_t)b * W, w);
+        rrec[i] = r;
+    }
+}
+
+// The pool rows: entries of (b, v) at pool + pbase[b] + poff[b][v] (blockIdx.y = the query).
+// A step loads each lane's neighbour's rank record once (rstride / 4 16-byte loads) and
+// takes every query vertex from registers.
 __global__ void __launch_bounds__(256) k_pool_fill(const uint32_t* __restrict__ off, const uint32_t* __restrict__ adj,
                                                    uint32_t V, const sgm_pooldesc* __restrict__ ds, uint32_t kmax)
 {
@@ -829,10 +851,10 @@ __global__ void __launch_bounds__(256) k_pool_fill(const uint32_t* __restrict__ 
     const sgm_pooldesc& D = ds[blockIdx.y];
     const glb<uint32_t>* __restrict__ vmask = gcast(D.vmask);
     const glb<uint32_t>* __restrict__ poff = gcast((const uint32_t*)D.poff);
-    const glb<uint64_t>* __restrict__ bm = gcast(D.bm);
-    const glb<uint32_t>* __restrict__ rank = gcast(D.rank);
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const glb<u32x4>* __restrict__ rrec = gcast((const u32x4*)D.rrec);
     const uint32_t waves = blockDim.x >> 6, wave = threadIdx.x >> 6, lane = lane_id();
-    const uint32_t n = D.n, W = (V + 63) / 64;
+    const uint32_t n = D.n, nl = D.rstride >> 2;
     // lane b holds b's pool base
     const uint64_t my_base = lane < n ? (uint64_t)(size_t)(D.pool + D.pbase[lane]) : 0ull;
     uint32_t* mine = tl + (wave * 64 + lane) * kmax;
@@ -850,19 +872,31 @@ __global__ void __launch_bounds__(256) k_pool_fill(const uint32_t* __restrict__ 
             const TabStep st = tab_step(incl, len, total, b0, lane);
             const uint32_t own = st.own & 63u;
             const uint32_t oj = (uint32_t)__shfl((int)o, (int)own);
+            // every lane takes part in the shuffle (the owner lane may hold no entry this step)
             const uint32_t uj = (uint32_t)__shfl((int)um, (int)own);
             const uint32_t w = st.act ? adj[oj + st.pos] : 0u;
-            const uint32_t m = st.act ? vmask[w] & uj : 0u;
+            uint32_t rec[SGM_MAXQ];
+#pragma unroll
+            for (uint32_t i = 0; i < SGM_MAXQ / 4; i++) {
+                u32x4 x = {SGM_INVALID, SGM_INVALID, SGM_INVALID, SGM_INVALID};
+                if (i < nl && st.act) x = rrec[(size_t)w * nl + i];
+                rec[4 * i] = x.x;
+                rec[4 * i + 1] = x.y;
+                rec[4 * i + 2] = x.z;
+                rec[4 * i + 3] = x.w;
+            }
             const uint64_t upto = st.below | (1ull << lane);
-            for (uint32_t bs = ub; bs; bs &= bs - 1) {
-                const uint32_t b = __ffs(bs) - 1;
-                const bool hit = (m >> b) & 1u;
+#pragma unroll
+            for (uint32_t b = 0; b < SGM_MAXQ; b++) {
+                if (!((ub >> b) & 1u)) continue;
+                const bool hit = ((uj >> b) & 1u) && rec[b] != SGM_INVALID;
                 const uint64_t bal = __ballot(hit);
                 if (!bal) continue;
                 uint32_t* slot = slots + own * kmax + b;
                 if (hit) {
                     const uint32_t p = *slot + (uint32_t)__popcll(bal & st.below);
-                    gcast((uint32_t*)rdlane64(my_base, b))[p] = cand_rank(bm + (size_t)b * W, rank + (size_t)b * W, w);
+                    // streaming store: the pools are written once here
+                    __builtin_nontemporal_store(rec[b], gcast((uint32_t*)rdlane64(my_base, b)) + p);
                 }
                 if (st.last) *slot += (uint32_t)__popcll(bal & upto);
             }
@@ -905,10 +939,20 @@ __global__ void __launch_bounds__(256) k_edge_copy(const sgm_edgedesc* __restric
         const uint32_t s = valid ? poff[rows[r]] : 0u;
         const uint32_t len = valid ? rowoff[r + 1] - rowoff[r] : 0u;
         const uint32_t incl = wave_incl_scan(len), total = rdlane(incl, 63);
-        for (uint32_t b0 = 0; b0 < total; b0 += 64) {
-            const TabStep st = tab_step(incl, len, total, b0, lane);
-            const uint32_t sj = (uint32_t)__shfl((int)s, (int)(st.own & 63u));
-            if (st.act) dst[(size_t)d0 + b0 + lane] = src[(size_t)sj + st.pos];
+        // four 64-entry steps at a time: their owner searches and loads overlap
+        for (uint32_t b0 = 0; b0 < total; b0 += 256) {
+            uint32_t val[4];
+            bool act[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const TabStep st = tab_step(incl, len, total, b0 + 64u * u, lane);
+                const uint32_t sj = (uint32_t)__shfl((int)s, (int)(st.own & 63u));
+                act[u] = st.act;
+                val[u] = st.act ? src[(size_t)sj + st.pos] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (act[u]) __builtin_nontemporal_store(val[u], dst + ((size_t)d0 + b0 + 64u * u + lane));
         }
     }
 }
@@ -1254,6 +1298,11 @@ void launch_pool_fill(hipStream_t s, const uint32_t* off, const uint32_t* adj, u
                       uint32_t nq, uint32_t kmax)
 {
     if (!nq || !V || !kmax) return;
+    for (uint32_t s0 = 0; s0 < nq; s0 += kMaxGridY) {
+        const uint32_t k = nq - s0 < kMaxGridY ? nq - s0 : kMaxGridY;
+        dim3 grid(clampg(((uint64_t)V * 4 + 255) / 256, 1, 16384 / k + 1), k);
+        hipLaunchKernelGGL(k_pool_ranks, grid, dim3(256), 0, s, V, ds + s0);
+    }
     const uint32_t xcap = 16384 / nq > 8 ? 16384 / nq : 8u;
     const size_t smem = sizeof(uint32_t) * 4 * 64 * kmax;
     for (uint32_t s0 = 0; s0 < nq; s0 += kMaxGridY) {

@@ -76,7 +76,8 @@ struct sgm_ctx {
     uint32_t* d_ttot = nullptr;
     uint32_t ttot_cap = 0;
     uint32_t* d_poff = nullptr;  // row pool lengths / offsets of the queries of one build pass
-    size_t poff_cap = 0;
+    uint32_t* d_rrec = nullptr;  // their rank records
+    size_t poff_cap = 0, rrec_cap = 0;
     // batched enumeration: bitmap of C(u_L) per query; depth-1 split blocks and chunk records
     uint32_t* d_lastbits = nullptr;
     size_t lastbits_words = 0;
@@ -173,6 +174,7 @@ struct sgm_query {
     uint32_t* d_cnt = nullptr;
     uint32_t* d_candoff = nullptr;
     uint32_t* d_list = nullptr;
+    size_t list_cap = 0;  // d_list's capacity (kept across filters)
     uint32_t* d_lreq = nullptr;       // labelled NLF requirements (sgm_qdesc::lreq), built on demand
     bool labelled = false;            // some query label is not 0 (labels in q.lab)
     // NLF candidate counts (NLFByVertex, inc/filter.h:24-58) for the start vertices of the
