@@ -159,6 +159,53 @@ __device__ uint32_t isect_write(const uint32_t* __restrict__ a, uint32_t na, con
     return cnt;
 }
 
+// Intersect two sorted lists of positions in C(u) (cand: C(u)'s vertices) into out.  A list
+// that is the table row of a hub data vertex (its slot, else SGM_INVALID) is probed through
+// the vertex's adjacency bitmap when the other list is walked: position x is in the row iff
+// cand[x] is adjacent to the vertex -- two loads per key instead of a binary search.
+__device__ uint32_t isect_pos(const uint32_t* __restrict__ a, uint32_t na, uint32_t aslot,
+                              const uint32_t* __restrict__ b, uint32_t nb, uint32_t bslot,
+                              const uint32_t* __restrict__ cand, const uint32_t* __restrict__ hub_bits,
+                              uint32_t hub_words, uint32_t* __restrict__ out, unsigned long long& elems,
+                              unsigned long long& outs, unsigned long long* probes, const uint32_t* stop)
+{
+    if (na > nb) {
+        const uint32_t* t = a; a = b; b = t;
+        uint32_t tn = na; na = nb; nb = tn;
+        bslot = aslot;
+    }
+    if (bslot == SGM_INVALID || !hub_bits)
+        return isect_write(a, na, nullptr, b, nb, nullptr, out, elems, outs, probes, stop);
+    const uint32_t lane = lane_id();
+    const uint32_t* __restrict__ bits = hub_bits + (size_t)bslot * hub_words;
+    uint32_t cnt = 0;
+    elems += na + nb;
+    if (nb == 0) return 0;
+    if (lane == 0) *probes += (unsigned long long)na * 3u;  // key, its vertex, the bitmap word
+    for (uint32_t base = 0; base < na; base += 64 * UNR) {
+        uint32_t key[UNR], w[UNR];
+        bool valid[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            const uint32_t i = base + 64 * u + lane;
+            valid[u] = i < na;
+            key[u] = valid[u] ? a[i] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; u++) w[u] = valid[u] ? cand[key[u]] : 0u;
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            const bool hit = valid[u] && ((bits[w[u] >> 5] >> (w[u] & 31u)) & 1u);
+            const uint64_t bal = __ballot(hit);
+            if (hit) out[cnt + mbcnt64(bal)] = key[u];
+            cnt += (uint32_t)__popcll(bal);
+        }
+        if (((base >> 8) & 7u) == 7u && __atomic_load_n(stop, __ATOMIC_RELAXED)) break;
+    }
+    outs += cnt;
+    return cnt;
+}
+
 // A wave publishes its embedding count at least every FLUSH_AT embeddings (and at the end
 // of each depth-0 item), so the cap can stop the whole grid while waves are still deep
 // inside huge subtrees; the stop word is raised once the global count reaches the cap,
@@ -1276,6 +1323,7 @@ struct WaveAux {
     // the whole list).  Every sibling at d-1 then needs at most one intersection.
     const uint32_t* fF[SGM_MAXQ];
     uint32_t fFn[SGM_MAXQ], fE[SGM_MAXQ];
+    uint32_t fFs[SGM_MAXQ];  // hub slot of fF[d] when it is one hub's row, else SGM_INVALID
     // the last level over a chunk of depth-(L-1) siblings (queries of 3+ vertices), when L
     // has no edge from L-1: fF[L]'s matches, size and conflict bits against depths < L-1
     uint32_t baseA, candsA;
@@ -1332,35 +1380,41 @@ __device__ bool aux_list(const sgm_auxplan_dev& A, const WaveAux& S, uint32_t d,
 // matched: their rows intersected on the stack from *top (a single row is aliased).
 __device__ bool aux_prefix(const sgm_auxplan_dev& A, WaveAux& S, uint32_t dn, uint32_t* ws, uint32_t stack_words,
                            uint32_t& top, unsigned long long& elems, unsigned long long& outs,
-                           unsigned long long* probes, const uint32_t* stop)
+                           unsigned long long* probes, const uint32_t* stop, const uint32_t* __restrict__ hub_slot,
+                           const uint32_t* __restrict__ hub_bits, uint32_t hub_words)
 {
     const uint32_t f0 = A.prev_first[dn], f1 = A.prev_first[dn + 1];
     const uint32_t* F = nullptr;
-    uint32_t Fn = 0, E = SGM_INVALID;
+    uint32_t Fn = 0, E = SGM_INVALID, Fs = SGM_INVALID;
     for (uint32_t e = f0; e < f1; e++) {
-        if (A.prev_depth[e] + 1 == dn) {
+        const uint32_t pd = A.prev_depth[e];
+        if (pd + 1 == dn) {
             E = e;
             continue;
         }
         const uint32_t* o = A.tab_off[e];
-        const uint32_t r = S.iemb[A.prev_depth[e]];
+        const uint32_t r = S.iemb[pd];
         const uint32_t b = o[r], nb = o[r + 1] - b;
+        const uint32_t rs = hub_slot ? uni(hub_slot[uni(S.emb[pd])]) : SGM_INVALID;
         if (!F) {
             F = A.tab_idx[e] + b;
             Fn = nb;
+            Fs = rs;
             continue;
         }
         const uint32_t need = Fn < nb ? Fn : nb;
         if (top + need > stack_words) return false;
-        const uint32_t sz = isect_write(F, Fn, nullptr, A.tab_idx[e] + b, nb, nullptr, ws + top, elems, outs, probes,
-                                        stop);
+        const uint32_t sz = isect_pos(F, Fn, Fs, A.tab_idx[e] + b, nb, rs, A.cand[dn], hub_bits, hub_words, ws + top,
+                                      elems, outs, probes, stop);
         F = ws + top;
         Fn = sz;
+        Fs = SGM_INVALID;
         top += sz;
     }
     S.fF[dn] = F;
     S.fFn[dn] = Fn;
     S.fE[dn] = E;
+    S.fFs[dn] = Fs;
     return true;
 }
 
@@ -1370,7 +1424,8 @@ __device__ bool aux_prefix(const sgm_auxplan_dev& A, WaveAux& S, uint32_t dn, ui
 __device__ bool aux_list_at(const sgm_auxplan_dev& A, const WaveAux& S, uint32_t d, uint32_t* ws,
                             uint32_t stack_words, uint32_t& top, const uint32_t*& list, uint32_t& size,
                             unsigned long long& elems, unsigned long long& outs, unsigned long long* probes,
-                            const uint32_t* stop)
+                            const uint32_t* stop, const uint32_t* __restrict__ hub_slot,
+                            const uint32_t* __restrict__ hub_bits, uint32_t hub_words)
 {
     const uint32_t E = uni(S.fE[d]);
     const uint32_t* F = S.fF[d];
@@ -1390,7 +1445,9 @@ __device__ bool aux_list_at(const sgm_auxplan_dev& A, const WaveAux& S, uint32_t
     }
     const uint32_t need = Fn < nb ? Fn : nb;
     if (top + need > stack_words) return false;
-    size = isect_write(F, Fn, nullptr, A.tab_idx[E] + b, nb, nullptr, ws + top, elems, outs, probes, stop);
+    const uint32_t rs = hub_slot ? uni(hub_slot[uni(S.emb[d - 1])]) : SGM_INVALID;
+    size = isect_pos(F, Fn, uni(S.fFs[d]), A.tab_idx[E] + b, nb, rs, A.cand[d], hub_bits, hub_words, ws + top, elems,
+                     outs, probes, stop);
     list = ws + top;
     top += size;
     return true;
@@ -1467,6 +1524,7 @@ __device__ void aux_last_chunk(const sgm_auxplan_dev& A, WaveAux& S, uint32_t L,
         // F's keys probe a sibling's row; when the sibling's vertex has an adjacency bitmap,
         // "x in the row" is "C(u_L)[x] adjacent to it": one bitmap word instead of a search
         const uint32_t hslot = (walk_f && hub_slot) ? hub_slot[A.cand[d][idx]] : SGM_INVALID;
+        const uint32_t fslot = (F && hub_slot) ? uni(S.fFs[L]) : SGM_INVALID;  // F as one hub's row
         const uint32_t len = walk_f ? Fn : rlen;
         const uint32_t incl = wave_incl_scan(len), total = rdlane(incl, 63);
         S.sok[lane] = 0;
@@ -1493,7 +1551,14 @@ __device__ void aux_last_chunk(const sgm_auxplan_dev& A, WaveAux& S, uint32_t L,
                     }
                 } else {
                     x = tidx[rbj + st.pos];
-                    in = !F || lower_found(F, Fn, x);
+                    if (!F) {
+                        in = true;
+                    } else if (fslot != SGM_INVALID) {
+                        const uint32_t w = candL[x];
+                        in = (hub_bits[(size_t)fslot * hub_words + (w >> 5)] >> (w & 31u)) & 1u;
+                    } else {
+                        in = lower_found(F, Fn, x);
+                    }
                 }
             }
             uint32_t c = SGM_INVALID;
@@ -1518,8 +1583,8 @@ __device__ void aux_last_chunk(const sgm_auxplan_dev& A, WaveAux& S, uint32_t L,
         const uint32_t found = wave_sum32(me ? cands_s : 0u);
         if (F) {
             // keys of the shorter list, each with a bitmap word or a binary search in the longer
-            const uint32_t pw =
-                wave_sum32(me ? len * (1u + probe_cost(walk_f ? rlen : Fn, hslot != SGM_INVALID)) : 0u);
+            const bool pbits = walk_f ? hslot != SGM_INVALID : fslot != SGM_INVALID;
+            const uint32_t pw = wave_sum32(me ? len * (1u + probe_cost(walk_f ? rlen : Fn, pbits)) : 0u);
             if (lane == 0) S.probes += pw;
             elems += (unsigned long long)wave_sum32(me ? rlen : 0u) + (unsigned long long)nsib * Fn;
             outs += found;
@@ -1639,7 +1704,7 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
         S.nodes[0] += 1;
         {
             uint32_t top0 = 0;  // depth 1's backward neighbours are all at depth 0: nothing written
-            aux_prefix(A, S, 1, ws, stack_words, top0, elems, outs, &S.probes, stop);
+            aux_prefix(A, S, 1, ws, stack_words, top0, elems, outs, &S.probes, stop, hub_slot, hub_bits, hub_words);
         }
         S.stack_top[1] = 0;
         uint32_t d = 1;
@@ -1693,7 +1758,7 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
                 const bool built = is_last ? aux_list(A, S, d, ws, stack_words, top, list, size, true, last, nlast,
                                                       elems, outs, &S.probes, stop)
                                            : aux_list_at(A, S, d, ws, stack_words, top, list, size, elems, outs,
-                                                         &S.probes, stop);
+                                                         &S.probes, stop, hub_slot, hub_bits, hub_words);
                 if (!built) {
                     overflow = true; aborted = true; break;
                 }
@@ -1747,7 +1812,8 @@ __global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restr
                 if (size) {
                     // depth d+1's rows from depths below d are the same for every sibling of
                     // this visit: intersect them once
-                    if (!aux_prefix(A, S, d + 1, ws, stack_words, top, elems, outs, &S.probes, stop)) {
+                    if (!aux_prefix(A, S, d + 1, ws, stack_words, top, elems, outs, &S.probes, stop, hub_slot,
+                                    hub_bits, hub_words)) {
                         overflow = true; aborted = true; break;
                     }
                 }
