@@ -93,7 +93,10 @@ typedef struct {
                                 1030-1066), so counts and statistics over disjoint ranges add up
                                 to those of the whole search: a query splits over launches or
                                 GPUs by ranges (no reference counterpart) */
-    uint32_t reserved[2];
+    uint32_t split_chunks;   /* depth-1 splitting of the heaviest roots (cache + filter methods):
+                                at most this many chunks of a root's depth-1 loop, 1..32 (0 = 32);
+                                counts and statistics do not depend on it */
+    uint32_t reserved[1];
 } sgm_enum_opts;
 
 /* sgm_enum_opts.flags: intersect against hub adjacency lists by binary search instead of
@@ -312,8 +315,10 @@ int sgm_batch_enumerate_wait(sgm_batch* b, sgm_stats* stats);
 int sgm_batch_last_ms(const sgm_batch* b, double* filter_ms, double* enum_ms);
 /* Autotuner: enumerates the loaded batch with each persistent grid size (2, 4, 8 blocks per
  * CU) and intersection strategy (adjacency bitmaps for lists of >= 64, 256, 1024 vertices,
- * or binary search only), checks that every setting gives the same counts, and returns the
- * fastest setting in *best (cap and method from *base). */
+ * or binary search only); then, from the fastest of those, with 8 and 16 work chunks per
+ * heavy root (split_chunks; the default is 32) and with half and twice the automatic
+ * per-wave stack (stack_words).  Checks that every setting gives the same counts and returns
+ * the fastest in *best (cap and method from *base). */
 int sgm_batch_autotune(sgm_batch* b, const uint32_t* const* orders, const sgm_enum_opts* base, sgm_enum_opts* best,
                        double* best_ms);
 

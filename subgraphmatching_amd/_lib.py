@@ -74,7 +74,7 @@ class EnumOpts(C.Structure):
     _fields_ = [("cap", C.c_uint64), ("method", C.c_int32), ("blocks", C.c_uint32),
                 ("stack_words", C.c_uint32), ("flags", C.c_uint32), ("hub_min_degree", C.c_uint32),
                 ("time_limit_ms", C.c_uint32), ("root_begin", C.c_uint32), ("root_end", C.c_uint32),
-                ("reserved", C.c_uint32 * 2)]
+                ("split_chunks", C.c_uint32), ("reserved", C.c_uint32 * 1)]
 
 
 class Stats(C.Structure):

@@ -484,14 +484,15 @@ class ExploreResult:
 
 
 def enumerate_query(can: CandidateParam, order, method=L.SGM_ENUM_CACHE_FILTER_FS, cap=None, blocks=0,
-                    stack_words=0, flags=0, time_limit_ms=0, roots=None) -> ExploreResult:
+                    stack_words=0, flags=0, time_limit_ms=0, roots=None, split_chunks=0) -> ExploreResult:
     """``time_limit_ms``: per-query work budget in device wave time (0 = none); a query that
     passes it comes back with ``timed_out`` set and a partial count.  ``roots`` = (lo, hi):
     search only the depth-0 candidates with lo <= id < hi (disjoint ranges add up).  For the
     dynamic-order methods ``order`` is the start order and ``nodes`` counts the extensions
     per depth of the search, whose vertex at a depth varies (the reference's dynamic searches
     count no iterations, so Profiler.total_iter_count is left alone)."""
-    opts = _opts(method, cap, blocks, stack_words, flags, time_limit_ms=time_limit_ms, roots=roots)
+    opts = _opts(method, cap, blocks, stack_words, flags, time_limit_ms=time_limit_ms, roots=roots,
+                 split_chunks=split_chunks)
     st = L.Stats()
     o = L.u32(order)
     _check(L.lib().sgm_enumerate(can._q, L.ptr(o), C.byref(opts), C.byref(st)), can.ctx)
@@ -509,7 +510,7 @@ def enumerate_query(can: CandidateParam, order, method=L.SGM_ENUM_CACHE_FILTER_F
 
 
 def enumerate_batch(cans, orders=None, method=L.SGM_ENUM_CACHE_FILTER_FS, cap=None, blocks=0, stack_words=0,
-                    flags=0, time_limit_ms=0, roots=None):
+                    flags=0, time_limit_ms=0, roots=None, split_chunks=0):
     """Enumerate many filtered queries of one data graph in one persistent launch.
     ``orders`` None (or a None entry): the RI order, computed by the library.  Returns one
     ExploreResult per query (``time`` is the whole batch's kernel time).  ``time_limit_ms``:
@@ -518,7 +519,8 @@ def enumerate_batch(cans, orders=None, method=L.SGM_ENUM_CACHE_FILTER_FS, cap=No
     if not cans:
         return []
     ctx = cans[0].ctx
-    opts = _opts(method, cap, blocks, stack_words, flags, time_limit_ms=time_limit_ms, roots=roots)
+    opts = _opts(method, cap, blocks, stack_words, flags, time_limit_ms=time_limit_ms, roots=roots,
+                 split_chunks=split_chunks)
     nq = len(cans)
     ords = [None if o is None else L.u32(o) for o in orders] if orders is not None else None
     qarr = (C.c_void_p * nq)(*[c._q.value for c in cans])
@@ -640,7 +642,8 @@ def explore_Intersection_DynamicOrder_VEQ(data: Graph, query: Query, bfs_order, 
     return _dynamic(query, bfs_order, match_order, cfg, L.SGM_ENUM_DYN_VEQ, output_limit)
 
 
-def _opts(method, cap, blocks=0, stack_words=0, flags=0, hub_min_degree=0, time_limit_ms=0, roots=None):
+def _opts(method, cap, blocks=0, stack_words=0, flags=0, hub_min_degree=0, time_limit_ms=0, roots=None,
+          split_chunks=0):
     o = L.EnumOpts()
     o.cap = (1 << 64) - 1 if cap is None else int(cap)
     o.method = method
@@ -649,6 +652,7 @@ def _opts(method, cap, blocks=0, stack_words=0, flags=0, hub_min_degree=0, time_
     o.flags = flags
     o.hub_min_degree = hub_min_degree
     o.time_limit_ms = int(time_limit_ms)
+    o.split_chunks = int(split_chunks)
     if roots is not None:
         lo, hi = int(roots[0]), int(roots[1])
         if not 0 <= lo < hi <= 0xFFFFFFFF:
@@ -729,13 +733,13 @@ class Batch:
         return arr, keep
 
     def enumerate(self, orders=None, method=L.SGM_ENUM_CACHE_FILTER_FS, cap=None, blocks=0, stack_words=0,
-                  flags=0, hub_min_degree=0, time_limit_s=None, time_limit_ms=0, roots=None):
+                  flags=0, hub_min_degree=0, time_limit_s=None, time_limit_ms=0, roots=None, split_chunks=0):
         """With ``time_limit_s``, a stop request ends the launch after that many seconds;
         queries still running then come back with ``stopped`` set and partial counts.
         ``time_limit_ms``: per-query work budget (device wave time); queries past it come back
-        ``timed_out``."""
+        ``timed_out``.  ``split_chunks``: chunks per heavy root's depth-1 loop (0 = 32)."""
         arr, _keep = self._orders(orders)
-        opts = _opts(method, cap, blocks, stack_words, flags, hub_min_degree, time_limit_ms, roots)
+        opts = _opts(method, cap, blocks, stack_words, flags, hub_min_degree, time_limit_ms, roots, split_chunks)
         stats = (L.Stats * len(self.queries))()
         timer = None
         if time_limit_s:
@@ -752,13 +756,13 @@ class Batch:
         return [_result(st, q.getVertexCnt()) for st, q in zip(stats, self.queries)]
 
     def enumerate_start(self, orders=None, method=L.SGM_ENUM_CACHE_FILTER_FS, cap=None, blocks=0, stack_words=0,
-                        flags=0, hub_min_degree=0, time_limit_ms=0, roots=None):
+                        flags=0, hub_min_degree=0, time_limit_ms=0, roots=None, split_chunks=0):
         """Launch the loaded batch's enumeration and return at once
         (sgm_batch_enumerate_launch); ``enumerate_wait`` returns the results.  In between,
         another Batch of the same context can filter and launch the next batch, which then
         fills the GPU as this one runs out of work."""
         arr, _keep = self._orders(orders)
-        opts = _opts(method, cap, blocks, stack_words, flags, hub_min_degree, time_limit_ms, roots)
+        opts = _opts(method, cap, blocks, stack_words, flags, hub_min_degree, time_limit_ms, roots, split_chunks)
         _check(L.lib().sgm_batch_enumerate_launch(self.handle, arr, C.byref(opts)), self.ctx)
 
     def enumerate_wait(self):
@@ -781,14 +785,15 @@ class Batch:
         return orders, pb, pr, fl.astype(bool)
 
     def autotune(self, orders=None, method=L.SGM_ENUM_CACHE_FILTER_FS, cap=None):
-        """Returns ({blocks, flags, hub_min_degree}, best_ms) of the fastest setting."""
+        """Returns ({blocks, flags, hub_min_degree, split_chunks, stack_words}, best_ms) of the
+        fastest setting (sgm_batch_autotune)."""
         arr, _keep = self._orders(orders)
         base = _opts(method, cap)
         best = L.EnumOpts()
         ms = C.c_double()
         _check(L.lib().sgm_batch_autotune(self.handle, arr, C.byref(base), C.byref(best), C.byref(ms)), self.ctx)
-        return {"blocks": int(best.blocks), "flags": int(best.flags),
-                "hub_min_degree": int(best.hub_min_degree)}, float(ms.value)
+        return {"blocks": int(best.blocks), "flags": int(best.flags), "hub_min_degree": int(best.hub_min_degree),
+                "split_chunks": int(best.split_chunks), "stack_words": int(best.stack_words)}, float(ms.value)
 
     def last_ms(self):
         f, e = C.c_double(), C.c_double()

@@ -20,7 +20,7 @@
 namespace {
 
 struct EnumGraph {
-    uint32_t max_ops, use_fs, blocks, words, serial, split;
+    uint32_t max_ops, use_fs, blocks, words, serial, split, chunks;
     const uint32_t* hub;
     hipGraphExec_t exec;
 };
@@ -485,7 +485,7 @@ int sgm_batch_enumerate_launch(sgm_batch* b, const uint32_t* const* orders, cons
     hipGraphExec_t exec = nullptr;
     for (auto& x : b->ge)
         if (x.max_ops == bucket && x.use_fs == (uint32_t)use_fs && x.blocks == blocks && x.words == words && x.hub == hub &&
-            x.serial == (uint32_t)serial && x.split == (uint32_t)split)
+            x.serial == (uint32_t)serial && x.split == (uint32_t)split && x.chunks == o.split_chunks)
             exec = x.exec;
     const size_t st_words = sizeof(sgm_stats_dev) / sizeof(uint32_t) * slots;
     if (!exec) {
@@ -493,7 +493,7 @@ int sgm_batch_enumerate_launch(sgm_batch* b, const uint32_t* const* orders, cons
             sgm::launch_fill_u32(b->stream, reinterpret_cast<uint32_t*>(b->d_st), st_words, 0);
             sgm::launch_fill_u32(b->stream, b->d_qstop, 3 * (size_t)slots, 0);
             sgm::launch_last_bits(b->stream, b->d_eq, slots, V);
-            if (split) sgm::launch_split_roots(b->stream, g->d_off, g->d_bydeg, b->d_eq, slots, V);
+            if (split) sgm::launch_split_roots(b->stream, g->d_off, g->d_bydeg, b->d_eq, slots, V, o.split_chunks);
             sgm::launch_enum_cache(b->stream, blocks, g->d_off, g->d_adj, g->d_bydeg, b->d_eq, slots, bucket, b->d_stack,
                                    (uint32_t)words, c->d_stop, c->d_hstop, use_fs, hub, g->d_hub_bits, g->hub_words,
                                    serial, split);
@@ -501,7 +501,7 @@ int sgm_batch_enumerate_launch(sgm_batch* b, const uint32_t* const* orders, cons
         }, &exec);
         if (rc) return rc;
         b->ge.push_back(EnumGraph{bucket, (uint32_t)use_fs, blocks, (uint32_t)words, (uint32_t)serial, (uint32_t)split,
-                                  hub, exec});
+                                  o.split_chunks, hub, exec});
     }
     // the context's stop word is shared by every launch in flight: clear it only when none is
     // (a stop request then ends all of them)
@@ -573,7 +573,7 @@ int sgm_batch_enumerate_wait(sgm_batch* b, sgm_stats* stats)
         sgm::launch_fill_u32(b->stream, b->d_qstop, 3 * (size_t)slots, 0);
         HIPCHK(c, hipEventRecord(b->ev[2], b->stream));
         const uint32_t ns = (uint32_t)sub.size();
-        sgm::launch_split_roots(b->stream, g->d_off, g->d_bydeg, b->d_eq, ns, g->V);  // no-op without split blocks
+        sgm::launch_split_roots(b->stream, g->d_off, g->d_bydeg, b->d_eq, ns, g->V, o.split_chunks);  // no-op without split blocks
         sgm::launch_enum_cache(b->stream, blocks, g->d_off, g->d_adj, g->d_bydeg, b->d_eq, ns, bucket, c->d_stack,
                                (uint32_t)words, c->d_stop, c->d_hstop, b->pend_use_fs, b->pend_hub, g->d_hub_bits,
                                g->hub_words, b->pend_serial, b->pend_split);
@@ -690,7 +690,20 @@ int sgm_batch_autotune(sgm_batch* b, const uint32_t* const* orders, const sgm_en
     std::vector<sgm_stats> st(b->nq), ref;
     double best_t = 1e300;
     sgm_enum_opts winner = o0;
-    // persistent grid (blocks of 4 waves per CU) x intersection strategy (adjacency bitmaps
+    auto trial = [&](const sgm_enum_opts& o) -> int {
+        const int rc = sgm_batch_enumerate(b, orders, &o, st.data());
+        if (rc) return rc;
+        if (ref.empty()) {
+            ref = st;
+        } else {
+            for (uint32_t i = 0; i < b->nq; i++)
+                if (st[i].count != ref[i].count)
+                    return set_err(c, SGM_EDEVICE, "autotune: counts differ between settings (query %u)", i);
+        }
+        if (b->enum_ms < best_t) { best_t = b->enum_ms; winner = o; }
+        return SGM_OK;
+    };
+    // 1. persistent grid (blocks of 4 waves per CU) x intersection strategy (adjacency bitmaps
     // for lists of at least T vertices; the rest by binary search)
     static const uint32_t per_cu[] = {2, 4, 8};
     static const uint32_t hub_min[] = {64, 256, 1024, 0};  // 0: no bitmaps
@@ -700,17 +713,31 @@ int sgm_batch_autotune(sgm_batch* b, const uint32_t* const* orders, const sgm_en
             o.blocks = (uint32_t)c->num_cus * pc;
             o.hub_min_degree = hm;
             o.flags = hm ? (o0.flags & ~SGM_ENUM_F_NO_BITMAPS) : (o0.flags | SGM_ENUM_F_NO_BITMAPS);
-            const int rc = sgm_batch_enumerate(b, orders, &o, st.data());
-            if (rc) return rc;
-            if (ref.empty()) {
-                ref = st;
-            } else {
-                for (uint32_t i = 0; i < b->nq; i++)
-                    if (st[i].count != ref[i].count)
-                        return set_err(c, SGM_EDEVICE, "autotune: counts differ between settings (query %u)", i);
-            }
-            if (b->enum_ms < best_t) { best_t = b->enum_ms; winner = o; }
+            if (const int rc = trial(o)) return rc;
         }
+    // 2. from the best of those, the work-chunk size: how many chunks the depth-1 loop of each
+    // of a query's heaviest roots is cut into (smaller chunks spread a heavy root over more
+    // waves, larger ones cost fewer records)
+    {
+        const sgm_enum_opts w = winner;
+        for (uint32_t ch : {8u, 16u}) {
+            sgm_enum_opts o = w;
+            o.split_chunks = ch;
+            if (const int rc = trial(o)) return rc;
+        }
+    }
+    // 3. the per-wave set-op stack: half and twice the automatic size (a stack that overflows
+    // re-runs its queries with a larger one, which the timing charges)
+    {
+        const sgm_enum_opts w = winner;
+        const uint32_t auto_words =
+            (uint32_t)std::min<uint64_t>((uint64_t)8 * b->g->max_deg + 1024, 1u << 20);
+        for (uint32_t words : {auto_words / 2, auto_words * 2}) {
+            sgm_enum_opts o = w;
+            o.stack_words = std::max<uint32_t>(words, 1024);
+            if (const int rc = trial(o)) return rc;
+        }
+    }
     *best = winner;
     if (best_ms) *best_ms = best_t;
     return SGM_OK;

@@ -1549,7 +1549,7 @@ int sgm_enumerate_batch(sgm_query* const* qs, const uint32_t* const* orders, uin
         clear_host_stop(c);
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
         sgm::launch_last_bits(c->stream, c->d_eq, m, g->V);
-        if (split) sgm::launch_split_roots(c->stream, g->d_off, g->d_bydeg, c->d_eq, m, g->V);
+        if (split) sgm::launch_split_roots(c->stream, g->d_off, g->d_bydeg, c->d_eq, m, g->V, o.split_chunks);
         sgm::launch_enum_cache(c->stream, blocks, g->d_off, g->d_adj, g->d_bydeg, c->d_eq, m, max_ops, c->d_stack,
                                (uint32_t)words, c->d_stop, c->d_hstop, use_fs, hub_slot, g->d_hub_bits, g->hub_words,
                                serial, split);

@@ -430,12 +430,14 @@ __device__ void last_isect(const uint32_t* __restrict__ a, uint32_t na, const ui
 // Split table of each query (layout at SGM_SPLIT_ROOTS): one wave per query walks the
 // degree-ordered items until SGM_SPLIT_ROOTS candidate roots are found (or SGM_SPLIT_SCAN
 // positions are seen), then cuts root r's depth-1 loop -- N(v_r), deg(v_r) elements -- into
-// min(SGM_SPLIT_CHUNKS, ceil(deg/64)) chunks of a whole number of 64-element batches.  A
+// min(max_chunks, ceil(deg/64)) chunks of a whole number of 64-element batches
+// (max_chunks <= SGM_SPLIT_CHUNKS).  A
 // root range of at most SGM_SPLIT_ROOTS ids is read directly instead: all its candidates
 // are split and no root is left to the item counter (root_first = n_items).
 __global__ void __launch_bounds__(64) k_split_roots(const uint32_t* __restrict__ off,
                                                     const uint32_t* __restrict__ items,
-                                                    const sgm_equery* __restrict__ qs, uint32_t V)
+                                                    const sgm_equery* __restrict__ qs, uint32_t V,
+                                                    uint32_t max_chunks)
 {
     const sgm_equery& Q = qs[blockIdx.x];
     uint32_t* __restrict__ sp = Q.split;
@@ -482,7 +484,7 @@ __global__ void __launch_bounds__(64) k_split_roots(const uint32_t* __restrict__
         const uint32_t v = rvtx[lane];
         deg = off[v + 1] - off[v];
         const uint32_t b64 = (deg + 63) / 64;
-        const uint32_t ch = b64 < SGM_SPLIT_CHUNKS ? b64 : SGM_SPLIT_CHUNKS;
+        const uint32_t ch = b64 < max_chunks ? b64 : max_chunks;
         W = ch ? ((b64 + ch - 1) / ch) * 64 : 64;
         nch = deg ? (deg + W - 1) / W : 1u;  // a root without neighbours still counts its node
         cutmin[lane] = SGM_INVALID;
@@ -2318,9 +2320,10 @@ void launch_last_bits(hipStream_t s, const sgm_equery* qs, uint32_t nq, uint32_t
 }
 
 void launch_split_roots(hipStream_t s, const uint32_t* off, const uint32_t* items, const sgm_equery* qs, uint32_t nq,
-                        uint32_t V)
+                        uint32_t V, uint32_t max_chunks)
 {
-    if (nq) hipLaunchKernelGGL(k_split_roots, dim3(nq), dim3(64), 0, s, off, items, qs, V);
+    if (max_chunks == 0 || max_chunks > SGM_SPLIT_CHUNKS) max_chunks = SGM_SPLIT_CHUNKS;
+    if (nq) hipLaunchKernelGGL(k_split_roots, dim3(nq), dim3(64), 0, s, off, items, qs, V, max_chunks);
 }
 
 void launch_split_reduce(hipStream_t s, const sgm_equery* qs, uint32_t nq)

@@ -94,7 +94,7 @@ void launch_last_bits(hipStream_t s, const sgm_equery* qs, uint32_t nq, uint32_t
 // depth-1 split tables (queries with Q.split set), before launch_enum_cache; and the chunk
 // records added to the query statistics, after it
 void launch_split_roots(hipStream_t s, const uint32_t* off, const uint32_t* items, const sgm_equery* qs, uint32_t nq,
-                        uint32_t V);
+                        uint32_t V, uint32_t max_chunks);
 void launch_split_reduce(hipStream_t s, const sgm_equery* qs, uint32_t nq);
 void launch_enum_cache(hipStream_t s, uint32_t blocks, const uint32_t* off, const uint32_t* adj, const uint32_t* items,
                        const sgm_equery* qs, uint32_t nq, uint32_t max_ops, uint32_t* stack, uint32_t stack_words,

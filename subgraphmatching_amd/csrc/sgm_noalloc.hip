@@ -132,7 +132,7 @@ extern "C" int sgm_enumerate_noalloc(sgm_query* q, const uint32_t* order, const 
         clear_host_stop(c);
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
         sgm::launch_last_bits(c->stream, d_eq, 1, V);
-        if (split) sgm::launch_split_roots(c->stream, g->d_off, g->d_bydeg, d_eq, 1, V);
+        if (split) sgm::launch_split_roots(c->stream, g->d_off, g->d_bydeg, d_eq, 1, V, o.split_chunks);
         sgm::launch_enum_cache(c->stream, blocks, g->d_off, g->d_adj, g->d_bydeg, d_eq, 1, plan.n_ops, d_stack,
                                (uint32_t)words, c->d_stop, c->d_hstop, use_fs, hub_slot, g->d_hub_bits, g->hub_words,
                                serial, split);

@@ -117,8 +117,10 @@ def test_batch_autotune_keeps_counts(g_cl):
     best, ms = b.autotune(cap=50000)
     say("autotune", best, ms)
     assert ms > 0 and best["blocks"] > 0
-    tuned = b.enumerate(cap=50000, blocks=best["blocks"], flags=best["flags"], hub_min_degree=best["hub_min_degree"])
+    assert best["split_chunks"] in (0, 8, 16) and best["stack_words"] >= 0
+    tuned = b.enumerate(cap=50000, **best)
     assert [r.ans for r in tuned] == [r.ans for r in base]
+
 
 
 def test_batch_rmat_mixed_sizes(g_rmat):

@@ -118,14 +118,17 @@ def _compare(g, q, order, cap=None, check_work=True):
     ok_r, c_r = O.gql(g, q)
     assert ok == ok_r
     elems = {}
-    for fs, method, flags in ((True, L.SGM_ENUM_CACHE_FILTER_FS, 0), (False, L.SGM_ENUM_CACHE_FILTER, 0),
-                              (True, L.SGM_ENUM_CACHE_FILTER_FS, L.SGM_ENUM_F_NO_BITMAPS),
-                              (True, L.SGM_ENUM_CACHE_FILTER_FS, L.SGM_ENUM_F_SERIAL_LAST),
-                              (False, L.SGM_ENUM_CACHE_FILTER, L.SGM_ENUM_F_SERIAL_LAST),
-                              (True, L.SGM_ENUM_CACHE_FILTER_FS, L.SGM_ENUM_F_NO_SPLIT),
-                              (True, L.SGM_ENUM_CACHE_FILTER_FS, L.SGM_ENUM_F_SPLIT_ALL),
-                              (False, L.SGM_ENUM_CACHE_FILTER, L.SGM_ENUM_F_SPLIT_ALL)):
-        res = sgm.enumerate_query(can, order, method=method, cap=cap, flags=flags)
+    # (failing sets, method, flags, work chunks per split root: 0 = 32)
+    for fs, method, flags, chunks in ((True, L.SGM_ENUM_CACHE_FILTER_FS, 0, 0), (False, L.SGM_ENUM_CACHE_FILTER, 0, 0),
+                                      (True, L.SGM_ENUM_CACHE_FILTER_FS, L.SGM_ENUM_F_NO_BITMAPS, 0),
+                                      (True, L.SGM_ENUM_CACHE_FILTER_FS, L.SGM_ENUM_F_SERIAL_LAST, 0),
+                                      (False, L.SGM_ENUM_CACHE_FILTER, L.SGM_ENUM_F_SERIAL_LAST, 0),
+                                      (True, L.SGM_ENUM_CACHE_FILTER_FS, L.SGM_ENUM_F_NO_SPLIT, 0),
+                                      (True, L.SGM_ENUM_CACHE_FILTER_FS, L.SGM_ENUM_F_SPLIT_ALL, 0),
+                                      (True, L.SGM_ENUM_CACHE_FILTER_FS, L.SGM_ENUM_F_SPLIT_ALL, 1),
+                                      (True, L.SGM_ENUM_CACHE_FILTER_FS, L.SGM_ENUM_F_SPLIT_ALL, 8),
+                                      (False, L.SGM_ENUM_CACHE_FILTER, L.SGM_ENUM_F_SPLIT_ALL, 0)):
+        res = sgm.enumerate_query(can, order, method=method, cap=cap, flags=flags, split_chunks=chunks)
         ref = O.enum_cache(g, q, c_r, order, cap=cap, fs=fs)
         assert res.ans == ref["count"], (q.name, fs, flags, res.ans, ref["count"])
         if check_work and not ref["capped"]:
