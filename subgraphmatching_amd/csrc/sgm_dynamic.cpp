@@ -216,6 +216,8 @@ extern "C++" int enumerate_dyn_batch(sgm_query* const* qs, const uint32_t* const
     }
     uint32_t rlo, rhi;
     root_range(o, &rlo, &rhi);
+    const uint32_t* hub_slot = nullptr;  // adjacency bitmaps of the hubs, for row probes
+    if (const int hrc = hub_slot_map(g, o, &hub_slot)) return hrc;
     std::vector<uint32_t> live;
     for (uint32_t i = 0; i < nq; i++)
         if (!qs[i]->empty) live.push_back(i);
@@ -258,7 +260,7 @@ extern "C++" int enumerate_dyn_batch(sgm_query* const* qs, const uint32_t* const
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
         sgm::launch_enum_dyn(c->stream, blocks, c->d_dynplans, c->d_afirst, m, c->d_stack, (uint32_t)words, c->d_anext,
                              c->d_astats, c->d_astop, c->d_stop, c->d_hstop, (unsigned long long)o.cap,
-                             budget_ticks(c, o), rlo, rhi);
+                             budget_ticks(c, o), rlo, rhi, hub_slot, g->d_hub_bits, g->hub_words);
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
         std::vector<sgm_stats_dev> got(m);

@@ -2050,7 +2050,9 @@ __global__ void __launch_bounds__(256) k_enum_dyn(const sgm_dynplan_dev* __restr
                                                   uint32_t* __restrict__ next, sgm_stats_dev* __restrict__ stats,
                                                   uint32_t* __restrict__ stops, uint32_t* __restrict__ dev_stop,
                                                   const uint32_t* __restrict__ host_stop, unsigned long long cap,
-                                                  unsigned long long budget, uint32_t root_lo, uint32_t root_hi)
+                                                  unsigned long long budget, uint32_t root_lo, uint32_t root_hi,
+                                                  const uint32_t* __restrict__ hub_slot,
+                                                  const uint32_t* __restrict__ hub_bits, uint32_t hub_words)
 {
     extern __shared__ uint64_t smem_u64[];
     uint8_t* smem = reinterpret_cast<uint8_t*>(smem_u64);
@@ -2137,23 +2139,32 @@ __global__ void __launch_bounds__(256) k_enum_dyn(const sgm_dynplan_dev* __restr
                 const uint32_t e0 = P.prev_first[f], e1 = P.prev_first[f + 1];
                 const uint32_t* list;
                 uint32_t size;
+                // a row of a hub data vertex is probed through its adjacency bitmap
+                auto row_slot = [&](uint32_t p) -> uint32_t {
+                    return hub_slot ? uni(hub_slot[uni(P.cand[p][S.iemb[p]])]) : SGM_INVALID;
+                };
+                uint32_t lslot;
                 {
+                    const uint32_t p = P.prev_vtx[e0];
                     const uint32_t* o = P.tab_off[e0];
-                    const uint32_t r = S.iemb[P.prev_vtx[e0]];
+                    const uint32_t r = S.iemb[p];
                     const uint32_t b = o[r];
                     list = P.tab_idx[e0] + b;
                     size = o[r + 1] - b;
+                    lslot = e1 - e0 > 1 ? row_slot(p) : SGM_INVALID;
                 }
                 for (uint32_t e = e0 + 1; e < e1; e++) {
+                    const uint32_t p = P.prev_vtx[e];
                     const uint32_t* o = P.tab_off[e];
-                    const uint32_t r = S.iemb[P.prev_vtx[e]];
+                    const uint32_t r = S.iemb[p];
                     const uint32_t b = o[r], nb = o[r + 1] - b;
                     const uint32_t need = size < nb ? size : nb;
                     if (top + need > stack_words) return false;
-                    const uint32_t sz = isect_write(list, size, nullptr, P.tab_idx[e] + b, nb, nullptr, ws + top,
-                                                    elems, outs, &S.probes, stop);
+                    const uint32_t sz = isect_pos(list, size, lslot, P.tab_idx[e] + b, nb, row_slot(p), P.cand[f],
+                                                  hub_bits, hub_words, ws + top, elems, outs, &S.probes, stop);
                     list = ws + top;
                     size = sz;
+                    lslot = SGM_INVALID;
                     top += sz;
                 }
                 inter += e1 - e0 - 1;
@@ -2361,11 +2372,12 @@ void launch_enum_aux(hipStream_t s, uint32_t blocks, const sgm_auxplan_dev* plan
 void launch_enum_dyn(hipStream_t s, uint32_t blocks, const sgm_dynplan_dev* plans, const uint32_t* item_first,
                      uint32_t nq, uint32_t* stack, uint32_t stack_words, uint32_t* next, sgm_stats_dev* st,
                      uint32_t* stops, uint32_t* dev_stop, const uint32_t* host_stop, unsigned long long cap,
-                     unsigned long long budget, uint32_t root_lo, uint32_t root_hi)
+                     unsigned long long budget, uint32_t root_lo, uint32_t root_hi, const uint32_t* hub_slot,
+                     const uint32_t* hub_bits, uint32_t hub_words)
 {
     const size_t smem = 4 * ((sizeof(WaveDyn) + 15) & ~(size_t)15);
     hipLaunchKernelGGL(k_enum_dyn, dim3(blocks), dim3(256), smem, s, plans, item_first, nq, stack, stack_words, next,
-                       st, stops, dev_stop, host_stop, cap, budget, root_lo, root_hi);
+                       st, stops, dev_stop, host_stop, cap, budget, root_lo, root_hi, hub_slot, hub_bits, hub_words);
 }
 
 }  // namespace sgm

@@ -88,7 +88,8 @@ void launch_enum_aux(hipStream_t s, uint32_t blocks, const sgm_auxplan_dev* plan
 void launch_enum_dyn(hipStream_t s, uint32_t blocks, const sgm_dynplan_dev* plans, const uint32_t* item_first,
                      uint32_t nq, uint32_t* stack, uint32_t stack_words, uint32_t* next, sgm_stats_dev* st,
                      uint32_t* stops, uint32_t* dev_stop, const uint32_t* host_stop, unsigned long long cap,
-                     unsigned long long budget, uint32_t root_lo, uint32_t root_hi);
+                     unsigned long long budget, uint32_t root_lo, uint32_t root_hi, const uint32_t* hub_slot,
+                     const uint32_t* hub_bits, uint32_t hub_words);
 // bitmap of C(u_L) per query (Q.lastbits) from its candidate words; before launch_enum_cache
 void launch_last_bits(hipStream_t s, const sgm_equery* qs, uint32_t nq, uint32_t V);
 // depth-1 split tables (queries with Q.split set), before launch_enum_cache; and the chunk

@@ -92,6 +92,8 @@ def test_uncapped_parity(g2k, kind):
         r = sgm.enumerate_query(can, bfs, METHOD[kind])
         _same(r, ref, (kind, q.name))
         assert not r.capped and r.retries == 0
+        # hub rows intersected by binary search instead of their adjacency bitmaps
+        _same(sgm.enumerate_query(can, bfs, METHOD[kind], flags=L.SGM_ENUM_F_NO_BITMAPS), ref, (kind, q.name, "nb"))
         checked += 1
         can.close()
     assert checked >= 6
