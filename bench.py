@@ -681,14 +681,17 @@ def main():
     achieved = algo_bytes / (enum_ms / 1e3) / 1e9 if enum_ms > 0 else 0.0
     traffic = measured = None
     traffic_note = "no measured traffic for this workload"
-    if args.workload == "chunglu100k" and os.path.exists(args.traffic):
+    # the workloads whose launches' HBM traffic profiles/traffic.json records
+    traffic_key = {"chunglu100k": "k_enum_cache_bytes_per_launch", "chunglu100k-aux": "k_enum_aux_bytes_per_launch",
+                   "chunglu100k-dyn": "k_enum_dyn_bytes_per_launch"}.get(args.workload)
+    if traffic_key and os.path.exists(args.traffic):
         try:
             with open(args.traffic) as f:
                 tj = json.load(f)
-            if tj.get("source_stamp") == source_stamp():
-                traffic = tj.get("k_enum_cache_bytes_per_launch")
+            if tj.get("source_stamp") == source_stamp() and tj.get(traffic_key):
+                traffic = tj.get(traffic_key)
                 traffic_note = "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel build (profiles/traffic.json)"
-            else:
+            elif tj.get("source_stamp") != source_stamp():
                 traffic_note = "profiles/traffic.json was measured on other kernel sources: not used"
         except Exception as e:  # noqa: BLE001
             traffic_note = f"profiles/traffic.json unreadable: {e}"
