@@ -85,7 +85,6 @@ void sgm_ctx_destroy(sgm_ctx* c)
     if (c->h_rprog) (void)hipHostFree(c->h_rprog);
     dfree(c->d_tb);
     dfree(c->d_ttot);
-    dfree(c->d_poff);
     dfree(c->d_rrec);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
     if (c->h_stop) (void)hipHostFree(c->h_stop);
@@ -375,9 +374,7 @@ void sgm_query_destroy(sgm_query* q)
 {
     if (!q) return;
     free_tables(q);
-    dfree(q->d_toff);
-    dfree(q->d_tidx);
-    dfree(q->d_tscr);
+    dfree(q->d_tvoff);
     dfree(q->d_tpool);
     dfree(q->d_weight);
     dfree(q->d_guard);
@@ -787,58 +784,48 @@ int sgm_cand_exist_download(sgm_query* q, uint32_t* words, uint32_t num_words)
 // buildTables (inc/table.h:7-137) for every directed edge of many queries at once.  The row
 // of edge (a, b) at candidate v of a is N(v) n C(b) as positions in C(b): it depends on
 // (b, v) only, so each query's rows are made once per (b, v) in a row pool (v in U_b, the
-// union of C(a) over b's query neighbours a), and edge (a, b) either aliases b's pool (when
-// every row of it belongs to C(a), the usual case for the neighbour with the largest
-// candidate set) or gets a copy of the pool rows of C(a).  Per pass of queries (their pool
-// offsets share one scratch of n (V + 1) words per query): the pool row lengths (one launch),
-// the scans (pool offsets per (query, b), row offsets per edge), one read-back of the
-// totals, then the pool rows and the copies -- two host synchronisations per pass.  Queries
-// whose candidate sets are empty are skipped (no search reads their tables).
+// union of C(a) over b's query neighbours a), with offsets by data vertex id, and every edge
+// into b reads b's pool at its own candidates' rows (rows of vertices outside U_b are empty).
+// Per pass of queries (whose rank records share one scratch of ceil(n/4)*4 V words per
+// query): the pool row lengths (one launch), a scan per (query, b), the edges' entry totals
+// and longest rows, one read-back of the totals, then the rank records and the pool rows --
+// two host synchronisations per pass.  Queries whose candidate sets are empty are skipped
+// (no search reads their tables).
 static inline uint32_t rank_stride(uint32_t n) { return (n + 3) & ~3u; }
 
 static int build_tables_pass(sgm_ctx* c, sgm_graph* g, sgm_query* const* qs, const uint32_t* idx, uint32_t nqp)
 {
     const uint32_t V = g->V;
     const size_t per_b = (size_t)V + 1;
-    size_t pw = 0, rw = 0;
+    std::vector<size_t> rbase(nqp);  // each query's block of rank records
+    size_t rw = 0;
     for (uint32_t k = 0; k < nqp; k++) {
-        pw += (size_t)qs[idx[k]]->q.n * per_b;
+        rbase[k] = rw;
         rw += (size_t)rank_stride(qs[idx[k]]->q.n) * V;
     }
-    HIPCHK(c, grow_arena(&c->d_poff, c->poff_cap, pw));
-    HIPCHK(c, grow_arena(&c->d_rrec, c->rrec_cap, rw));
+    HIPCHK(c, grow_arena(&c->d_rrec, c->rrec_cap, std::max<size_t>(rw, 1)));
     struct Edge {
         uint32_t k, a, b;
-        size_t o, r;  // offsets slice in the query's toff arena; row counts at tscr + r
     };
     std::vector<Edge> edges;
-    std::vector<size_t> sbase(nqp), rbase(nqp);  // each query's blocks in the scratches
     std::vector<uint32_t> pfirst(nqp + 1, 0);  // each query's first pool scan
     uint32_t kmax = 0, max_rows = 0;
-    size_t sw = 0, rs = 0;
     for (uint32_t k = 0; k < nqp; k++) {
         sgm_query* q = qs[idx[k]];
-        sbase[k] = sw;
-        sw += (size_t)q->q.n * per_b;
-        rbase[k] = rs;
-        rs += (size_t)rank_stride(q->q.n) * V;
         pfirst[k + 1] = pfirst[k] + q->q.n;
         kmax = std::max(kmax, q->q.n);
-        size_t o = 0, r = 0;
         for (uint32_t a = 0; a < q->q.n; a++)
             for (uint32_t p = q->q.off[a]; p < q->q.off[a + 1]; p++) {
-                edges.push_back(Edge{k, a, q->q.adj[p], o, r});
-                o += (size_t)q->counts[a] + 1;
-                r += q->counts[a];
+                edges.push_back(Edge{k, a, q->q.adj[p]});
                 max_rows = std::max(max_rows, q->counts[a]);
             }
     }
     const uint32_t M = (uint32_t)edges.size(), NP = pfirst[nqp];
     if (M == 0) return SGM_OK;
-    // device descriptors [pools nqp][edges M][scans NP + M]; pinned staging holds the same
-    // plus the totals and maxima read back (pool scans, then edge scans)
+    // device descriptors [pools nqp][edges M][pool scans NP]; pinned staging holds the same
+    // plus the totals and maxima read back (pool scans, then edges)
     const size_t pool_b = stage_align(sizeof(sgm_pooldesc) * nqp), edge_b = stage_align(sizeof(sgm_edgedesc) * M);
-    const size_t scan_b = stage_align(sizeof(sgm_scandesc) * (NP + M)), dev_b = pool_b + edge_b + scan_b;
+    const size_t scan_b = stage_align(sizeof(sgm_scandesc) * NP), dev_b = pool_b + edge_b + scan_b;
     const uint32_t NT = NP + M;
     if (c->tb_bytes < dev_b) {
         dfree(c->d_tb);
@@ -871,7 +858,7 @@ static int build_tables_pass(sgm_ctx* c, sgm_graph* g, sgm_query* const* qs, con
         sgm_pooldesc& P = pd[k];
         std::memset(&P, 0, sizeof P);
         P.vmask = q->d_vmask;
-        P.poff = c->d_poff + sbase[k];
+        P.poff = q->d_tvoff;
         P.rrec = c->d_rrec + rbase[k];
         P.bm = q->d_bm;
         P.rank = q->d_rank;
@@ -887,27 +874,21 @@ static int build_tables_pass(sgm_ctx* c, sgm_graph* g, sgm_query* const* qs, con
     for (uint32_t e = 0; e < M; e++) {
         const Edge& E = edges[e];
         sgm_query* q = qs[idx[E.k]];
-        const uint32_t n = q->q.n, rows = q->counts[E.a];
-        sgm_table& t = q->tables[(size_t)E.a * n + E.b];
-        t.d_off = q->d_toff + E.o;
-        ed[e] = sgm_edgedesc{q->d_list + q->cand_off[E.a], pd[E.k].poff + E.b * per_b, q->d_tscr + E.r, t.d_off,
-                             nullptr, nullptr, rows, 0};
-        sd[NP + e] = sgm_scandesc{q->d_tscr + E.r, t.d_off, c->d_ttot + NP + e, c->d_ttot + NT + NP + e, rows, 0};
+        ed[e] = sgm_edgedesc{q->d_list + q->cand_off[E.a], pd[E.k].poff + E.b * per_b, q->counts[E.a], NP + e};
     }
     HIPCHK(c, hipMemcpyAsync(dev, st, dev_b, hipMemcpyHostToDevice, c->stream));
     const sgm_pooldesc* dpd = reinterpret_cast<const sgm_pooldesc*>(dev);
     const sgm_edgedesc* ded = reinterpret_cast<const sgm_edgedesc*>(dev + pool_b);
     const sgm_scandesc* dsd = reinterpret_cast<const sgm_scandesc*>(dev + pool_b + edge_b);
+    sgm::launch_fill_u32(c->stream, c->d_ttot, 2 * (size_t)NT, 0);
     sgm::launch_pool_count(c->stream, g->d_off, g->d_adj, V, dpd, nqp, kmax);
     sgm::launch_scan_segments(c->stream, dsd, NP);
-    sgm::launch_row_counts(c->stream, ded, M, max_rows);
-    sgm::launch_scan_segments(c->stream, dsd + NP, M);
+    sgm::launch_edge_stats(c->stream, ded, M, max_rows, c->d_ttot, c->d_ttot + NT);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(h, c->d_ttot, sizeof(uint32_t) * 2 * NT, hipMemcpyDeviceToHost, c->stream));
     hipError_t se = hipStreamSynchronize(c->stream);
     if (se != hipSuccess) return set_err(c, SGM_EHIP, "table count: %s", hipGetErrorString(se));
-    // pools, and the entries of the edges that do not alias one, into each query's arenas
-    uint32_t ncopy = 0;
+    // the pools into each query's arena
     for (uint32_t k = 0; k < nqp; k++) {
         sgm_query* q = qs[idx[k]];
         const uint32_t n = q->q.n;
@@ -921,41 +902,18 @@ static int build_tables_pass(sgm_ctx* c, sgm_graph* g, sgm_query* const* qs, con
             return set_err(c, SGM_ENOMEM, "table row pool of %llu entries", (unsigned long long)pool);
         P.pool = q->d_tpool;
     }
-    std::vector<size_t> own(nqp, 0);  // entries of the copied edges per query
-    for (uint32_t e = 0; e < M; e++) {
-        const Edge& E = edges[e];
-        if (h[NP + e] != h[pfirst[E.k] + E.b]) own[E.k] += h[NP + e];
-    }
-    for (uint32_t k = 0; k < nqp; k++) {
-        sgm_query* q = qs[idx[k]];
-        if (own[k] && grow_arena(&q->d_tidx, q->tidx_cap, own[k]) != hipSuccess)
-            return set_err(c, SGM_ENOMEM, "tables of %llu entries", (unsigned long long)own[k]);
-        own[k] = 0;
-    }
     for (uint32_t e = 0; e < M; e++) {
         const Edge& E = edges[e];
         sgm_query* q = qs[idx[E.k]];
         sgm_table& t = q->tables[(size_t)E.a * q->q.n + E.b];
-        const uint32_t* pool_b = q->d_tpool + pd[E.k].pbase[E.b];
         t.e_cnt = h[NP + e];
         t.max_deg = h[NT + NP + e];
-        if (t.e_cnt == h[pfirst[E.k] + E.b]) {
-            // every row of b's pool is a row of C(a) (C(a) is a subset of U_b, and the rows of
-            // U_b \ C(a) are empty): the pool is the edge's table, offsets included
-            t.d_idx = const_cast<uint32_t*>(pool_b);
-        } else {
-            t.d_idx = q->d_tidx + own[E.k];
-            own[E.k] += t.e_cnt;
-            sgm_edgedesc& D = ed[ncopy++];
-            D = ed[e];
-            D.src = pool_b;
-            D.dst = t.d_idx;
-        }
+        t.d_voff = pd[E.k].poff + E.b * per_b;
+        t.d_pool = q->d_tpool + pd[E.k].pbase[E.b];
         t.built = true;
     }
-    HIPCHK(c, hipMemcpyAsync(dev, st, pool_b + sizeof(sgm_edgedesc) * ncopy, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dev, st, pool_b, hipMemcpyHostToDevice, c->stream));
     sgm::launch_pool_fill(c->stream, g->d_off, g->d_adj, V, dpd, nqp, kmax);
-    sgm::launch_edge_copy(c->stream, ded, ncopy, max_rows);
     HIPCHK(c, hipGetLastError());
     se = hipStreamSynchronize(c->stream);
     if (se != hipSuccess) return set_err(c, SGM_EHIP, "tables: %s", hipGetErrorString(se));
@@ -988,21 +946,12 @@ extern "C++" int build_tables_many(sgm_query* const* qs, uint32_t nq)
             sgm::launch_compact(c->stream, q->d_cdesc, 1, g->V);
             q->compacted = true;
         }
-        size_t o = 0, r = 0;
-        for (uint32_t a = 0; a < q->q.n; a++) {
-            const uint32_t deg = q->q.off[a + 1] - q->q.off[a];
-            o += (size_t)deg * (q->counts[a] + 1);
-            r += (size_t)deg * q->counts[a];
-        }
-        HIPCHK(c, grow_arena(&q->d_toff, q->toff_cap, std::max<size_t>(o, 1)));
-        HIPCHK(c, grow_arena(&q->d_tscr, q->tscr_cap, std::max<size_t>(r, 1)));
+        HIPCHK(c, grow_arena(&q->d_tvoff, q->tvoff_cap, (size_t)q->q.n * ((size_t)g->V + 1)));
         todo.push_back(i);
     }
-    // passes of queries whose pool offsets and rank records fit scratches of at most 2^30
-    // words in all (a single larger query takes a pass of its own)
-    auto words_of = [&](const sgm_query* q) {
-        return (size_t)q->q.n * ((size_t)g->V + 1) + (size_t)rank_stride(q->q.n) * g->V;
-    };
+    // passes of queries whose rank records fit a scratch of at most 2^30 words (a single
+    // larger query takes a pass of its own)
+    auto words_of = [&](const sgm_query* q) { return (size_t)rank_stride(q->q.n) * g->V; };
     const size_t limit = (size_t)1 << 30;
     int rc = SGM_OK;
     for (size_t k0 = 0; k0 < todo.size() && rc == SGM_OK;) {
@@ -1018,6 +967,32 @@ extern "C++" int build_tables_many(sgm_query* const* qs, uint32_t nq)
     if (rc != SGM_OK)
         for (uint32_t i = 0; i < nq; i++) free_tables(qs[i]);
     return rc;
+}
+
+// Edge (a, b)'s table in the reference's layout (offsets per candidate row of a, then the
+// positions in C(b)), gathered on the host from b's pool at a's candidates' rows.
+extern "C++" int table_fetch(sgm_query* q, uint32_t a, uint32_t b, std::vector<uint32_t>& off, std::vector<uint32_t>& idx)
+{
+    sgm_ctx* c = q->g->ctx;
+    const uint32_t V = q->g->V;
+    const sgm_table& t = q->tables[(size_t)a * q->q.n + b];
+    if (!t.built) return set_err(c, SGM_ESTATE, "no table for (%u,%u)", a, b);
+    std::vector<uint32_t> ca(q->counts[a]), voff((size_t)V + 1);
+    if (!ca.empty())
+        HIPCHK(c, hipMemcpy(ca.data(), q->d_list + q->cand_off[a], sizeof(uint32_t) * ca.size(), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(voff.data(), t.d_voff, sizeof(uint32_t) * voff.size(), hipMemcpyDeviceToHost));
+    std::vector<uint32_t> pool(voff[V]);
+    if (!pool.empty())
+        HIPCHK(c, hipMemcpy(pool.data(), t.d_pool, sizeof(uint32_t) * pool.size(), hipMemcpyDeviceToHost));
+    off.assign(ca.size() + 1, 0);
+    idx.clear();
+    idx.reserve(t.e_cnt);
+    for (size_t r = 0; r < ca.size(); r++) {
+        const uint32_t v = ca[r];
+        idx.insert(idx.end(), pool.begin() + voff[v], pool.begin() + voff[v + 1]);
+        off[r + 1] = (uint32_t)idx.size();
+    }
+    return SGM_OK;
 }
 
 int sgm_build_tables(sgm_query* q)
@@ -1040,22 +1015,16 @@ int sgm_table_download(sgm_query* q, uint32_t a, uint32_t b, uint32_t* offsets, 
 {
     if (!q || a >= q->q.n || b >= q->q.n) return SGM_EINVAL;
     sgm_ctx* c = q->g->ctx;
-    const sgm_table& t = q->tables[(size_t)a * q->q.n + b];
-    if (!t.built) return set_err(c, SGM_ESTATE, "no table for (%u,%u)", a, b);
-    if (offsets) HIPCHK(c, hipMemcpy(offsets, t.d_off, sizeof(uint32_t) * (q->counts[a] + 1), hipMemcpyDeviceToHost));
-    if (idx && t.e_cnt) HIPCHK(c, hipMemcpy(idx, t.d_idx, sizeof(uint32_t) * t.e_cnt, hipMemcpyDeviceToHost));
-    if (vtx && t.e_cnt) {
+    std::vector<uint32_t> off, pos;
+    if (const int rc = table_fetch(q, a, b, off, pos)) return rc;
+    if (offsets) std::copy(off.begin(), off.end(), offsets);
+    if (idx) std::copy(pos.begin(), pos.end(), idx);
+    if (vtx && !pos.empty()) {
         // _edge_v is the candidate at each position, C(b)[_edge[i]]: read from C(b) here
         // rather than kept on the device beside the positions
-        std::vector<uint32_t> pos, cb(q->counts[b]);
-        const uint32_t* p = idx;
-        if (!p) {
-            pos.resize(t.e_cnt);
-            HIPCHK(c, hipMemcpy(pos.data(), t.d_idx, sizeof(uint32_t) * t.e_cnt, hipMemcpyDeviceToHost));
-            p = pos.data();
-        }
+        std::vector<uint32_t> cb(q->counts[b]);
         HIPCHK(c, hipMemcpy(cb.data(), q->d_list + q->cand_off[b], sizeof(uint32_t) * cb.size(), hipMemcpyDeviceToHost));
-        for (uint64_t i = 0; i < t.e_cnt; i++) vtx[i] = cb[p[i]];
+        for (size_t i = 0; i < pos.size(); i++) vtx[i] = cb[pos[i]];
     }
     return SGM_OK;
 }
@@ -1281,8 +1250,8 @@ static int enumerate_aux_batch(sgm_query* const* qs, const uint32_t* const* orde
             for (uint32_t p : prev[u]) {
                 const sgm_table& t = q->tables[(size_t)p * n + u];
                 ap.prev_depth[e] = pos[p];
-                ap.tab_off[e] = t.d_off;
-                ap.tab_idx[e] = t.d_idx;
+                ap.tab_off[e] = t.d_voff;
+                ap.tab_idx[e] = t.d_pool;
                 e++;
             }
         }

@@ -92,7 +92,7 @@ int compute_weights_many(sgm_query* const* qs, const uint32_t* const* orders, ui
         const uint32_t* order = orders[i];
         const uint32_t n = q->q.n;
         if (const int rc = ensure_weight_buffer(q)) return rc;
-        add(0, sgm_wdesc{nullptr, nullptr, nullptr, q->d_weight, q->cand_off[n], 0xFFFFFFFFu});
+        add(0, sgm_wdesc{nullptr, nullptr, nullptr, nullptr, q->d_weight, q->cand_off[n], 0xFFFFFFFFu});
         std::vector<std::vector<uint32_t>> prev, fol;
         neighbours_in_order(q->q, order, prev, fol);
         for (uint32_t k = 0; k < n; k++) {
@@ -103,9 +103,10 @@ int compute_weights_many(sgm_query* const* qs, const uint32_t* const* orders, ui
                 if (prev[ch].size() != 1) continue;
                 one = false;
                 const sgm_table& t = q->tables[(size_t)x * n + ch];
-                add(k + 1, sgm_wdesc{t.d_off, t.d_idx, q->d_weight + q->cand_off[ch], out, q->counts[x], 0u});
+                add(k + 1, sgm_wdesc{t.d_voff, t.d_pool, q->d_list + q->cand_off[x], q->d_weight + q->cand_off[ch], out,
+                                     q->counts[x], 0u});
             }
-            if (one) add(k + 1, sgm_wdesc{nullptr, nullptr, nullptr, out, q->counts[x], 1u});
+            if (one) add(k + 1, sgm_wdesc{nullptr, nullptr, nullptr, nullptr, out, q->counts[x], 1u});
         }
     }
     std::vector<sgm_wdesc> all;
@@ -199,8 +200,8 @@ extern "C++" int enumerate_dyn_batch(sgm_query* const* qs, const uint32_t* const
             for (uint32_t p : prev[u]) {
                 const sgm_table& t = q->tables[(size_t)p * n + u];
                 P.prev_vtx[e] = (uint8_t)p;
-                P.tab_off[e] = t.d_off;
-                P.tab_idx[e] = t.d_idx;
+                P.tab_off[e] = t.d_voff;
+                P.tab_idx[e] = t.d_pool;
                 shortest = std::min(shortest, t.max_deg);
                 if (e > P.prev_first[u]) path_words += shortest;
                 e++;

@@ -1346,16 +1346,16 @@ __device__ bool aux_list(const sgm_auxplan_dev& A, const WaveAux& S, uint32_t d,
     const uint32_t np = e_end - f;
     {
         const uint32_t* o = A.tab_off[f];
-        const uint32_t r = S.iemb[A.prev_depth[f]];
-        const uint32_t b = o[r];
+        const uint32_t v = S.emb[A.prev_depth[f]];  // rows are indexed by data vertex
+        const uint32_t b = o[v];
         list = A.tab_idx[f] + b;
-        size = o[r + 1] - b;
+        size = o[v + 1] - b;
     }
     const uint32_t stop_at = leave_last ? e_end - 1 : e_end;
     for (uint32_t e = f + 1; e < stop_at; e++) {
         const uint32_t* o = A.tab_off[e];
-        const uint32_t r = S.iemb[A.prev_depth[e]];
-        const uint32_t b = o[r], nb = o[r + 1] - b;
+        const uint32_t v = S.emb[A.prev_depth[e]];
+        const uint32_t b = o[v], nb = o[v + 1] - b;
         const uint32_t need = size < nb ? size : nb;
         if (top + need > stack_words) return false;
         const uint32_t sz =
@@ -1367,10 +1367,10 @@ __device__ bool aux_list(const sgm_auxplan_dev& A, const WaveAux& S, uint32_t d,
     if (leave_last && np > 1) {
         const uint32_t e = e_end - 1;
         const uint32_t* o = A.tab_off[e];
-        const uint32_t r = S.iemb[A.prev_depth[e]];
-        const uint32_t b = o[r];
+        const uint32_t v = S.emb[A.prev_depth[e]];
+        const uint32_t b = o[v];
         last = A.tab_idx[e] + b;
-        nlast = o[r + 1] - b;
+        nlast = o[v + 1] - b;
     } else {
         last = nullptr;
         nlast = 0;
@@ -1395,8 +1395,8 @@ __device__ bool aux_prefix(const sgm_auxplan_dev& A, WaveAux& S, uint32_t dn, ui
             continue;
         }
         const uint32_t* o = A.tab_off[e];
-        const uint32_t r = S.iemb[pd];
-        const uint32_t b = o[r], nb = o[r + 1] - b;
+        const uint32_t v = S.emb[pd];
+        const uint32_t b = o[v], nb = o[v + 1] - b;
         const uint32_t rs = hub_slot ? uni(hub_slot[uni(S.emb[pd])]) : SGM_INVALID;
         if (!F) {
             F = A.tab_idx[e] + b;
@@ -1438,8 +1438,8 @@ __device__ bool aux_list_at(const sgm_auxplan_dev& A, const WaveAux& S, uint32_t
         return true;
     }
     const uint32_t* o = A.tab_off[E];
-    const uint32_t r = S.iemb[d - 1];
-    const uint32_t b = o[r], nb = o[r + 1] - b;
+    const uint32_t v = S.emb[d - 1];
+    const uint32_t b = o[v], nb = o[v + 1] - b;
     if (!F) {
         list = A.tab_idx[E] + b;
         size = nb;
@@ -1521,11 +1521,12 @@ __device__ void aux_last_chunk(const sgm_auxplan_dev& A, WaveAux& S, uint32_t L,
         // other; the siblings' walks are concatenated, 64 keys per step
         const uint32_t* o = A.tab_off[E];
         const uint32_t* tidx = A.tab_idx[E];
-        const uint32_t rb = me ? o[idx] : 0u, rlen = me ? o[idx + 1] - rb : 0u;
+        const uint32_t vs = me ? A.cand[d][idx] : 0u;  // the sibling's vertex: its row's index
+        const uint32_t rb = me ? o[vs] : 0u, rlen = me ? o[vs + 1] - rb : 0u;
         const bool walk_f = F && Fn < rlen;
         // F's keys probe a sibling's row; when the sibling's vertex has an adjacency bitmap,
         // "x in the row" is "C(u_L)[x] adjacent to it": one bitmap word instead of a search
-        const uint32_t hslot = (walk_f && hub_slot) ? hub_slot[A.cand[d][idx]] : SGM_INVALID;
+        const uint32_t hslot = (walk_f && hub_slot) ? hub_slot[vs] : SGM_INVALID;
         const uint32_t fslot = (F && hub_slot) ? uni(S.fFs[L]) : SGM_INVALID;  // F as one hub's row
         const uint32_t len = walk_f ? Fn : rlen;
         const uint32_t incl = wave_incl_scan(len), total = rdlane(incl, 63);
@@ -2140,27 +2141,25 @@ __global__ void __launch_bounds__(256) k_enum_dyn(const sgm_dynplan_dev* __restr
                 const uint32_t* list;
                 uint32_t size;
                 // a row of a hub data vertex is probed through its adjacency bitmap
-                auto row_slot = [&](uint32_t p) -> uint32_t {
-                    return hub_slot ? uni(hub_slot[uni(P.cand[p][S.iemb[p]])]) : SGM_INVALID;
-                };
+                auto row_slot = [&](uint32_t v) -> uint32_t { return hub_slot ? uni(hub_slot[v]) : SGM_INVALID; };
                 uint32_t lslot;
                 {
                     const uint32_t p = P.prev_vtx[e0];
+                    const uint32_t vp = uni(P.cand[p][S.iemb[p]]);  // p's match: its row's index
                     const uint32_t* o = P.tab_off[e0];
-                    const uint32_t r = S.iemb[p];
-                    const uint32_t b = o[r];
+                    const uint32_t b = o[vp];
                     list = P.tab_idx[e0] + b;
-                    size = o[r + 1] - b;
-                    lslot = e1 - e0 > 1 ? row_slot(p) : SGM_INVALID;
+                    size = o[vp + 1] - b;
+                    lslot = e1 - e0 > 1 ? row_slot(vp) : SGM_INVALID;
                 }
                 for (uint32_t e = e0 + 1; e < e1; e++) {
                     const uint32_t p = P.prev_vtx[e];
+                    const uint32_t vp = uni(P.cand[p][S.iemb[p]]);
                     const uint32_t* o = P.tab_off[e];
-                    const uint32_t r = S.iemb[p];
-                    const uint32_t b = o[r], nb = o[r + 1] - b;
+                    const uint32_t b = o[vp], nb = o[vp + 1] - b;
                     const uint32_t need = size < nb ? size : nb;
                     if (top + need > stack_words) return false;
-                    const uint32_t sz = isect_pos(list, size, lslot, P.tab_idx[e] + b, nb, row_slot(p), P.cand[f],
+                    const uint32_t sz = isect_pos(list, size, lslot, P.tab_idx[e] + b, nb, row_slot(vp), P.cand[f],
                                                   hub_bits, hub_words, ws + top, elems, outs, &S.probes, stop);
                     list = ws + top;
                     size = sz;

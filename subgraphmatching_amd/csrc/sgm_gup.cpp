@@ -201,10 +201,8 @@ extern "C++" int gup_guards_build(sgm_query* q, const uint32_t* order)
             if (!t.built) return set_err(c, SGM_ESTATE, "conflict detection: table (%u,%u) missing", a, b);
             auto& to = G.toff[(size_t)a * n + b];
             auto& ti = G.tidx[(size_t)a * n + b];
-            to.resize(q->counts[a] + 1);
-            ti.resize(std::max<uint64_t>(t.e_cnt, 1));
-            HIPCHK(c, hipMemcpy(to.data(), t.d_off, sizeof(uint32_t) * to.size(), hipMemcpyDeviceToHost));
-            if (t.e_cnt) HIPCHK(c, hipMemcpy(ti.data(), t.d_idx, sizeof(uint32_t) * t.e_cnt, hipMemcpyDeviceToHost));
+            if (const int rc = table_fetch(q, a, b, to, ti)) return rc;
+            if (ti.empty()) ti.resize(1);
         }
     G.goff.assign(n + 1, 0);
     for (uint32_t i = 0; i < n; i++) G.goff[i + 1] = G.goff[i] + q->counts[order[i]];

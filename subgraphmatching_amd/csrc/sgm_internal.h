@@ -42,6 +42,8 @@ struct sgm_auxplan_dev {
     uint32_t order[SGM_MAXQ];
     uint32_t prev_first[SGM_MAXQ + 1];
     uint32_t prev_depth[SGM_MAXPREV];
+    // table of backward edge e: the row of data vertex v is tab_idx[e][tab_off[e][v] ..
+    // tab_off[e][v + 1]) -- offsets by vertex id, shared by every edge into the same vertex
     const uint32_t* tab_off[SGM_MAXPREV];
     const uint32_t* tab_idx[SGM_MAXPREV];
     const uint32_t* cand[SGM_MAXQ];   // candidate list of the query vertex at each depth
@@ -66,7 +68,7 @@ struct sgm_dynplan_dev {
     uint32_t fol_first[SGM_MAXQ + 1];
     uint8_t prev_vtx[SGM_MAXPREV];
     uint8_t fol[SGM_MAXPREV];
-    const uint32_t* tab_off[SGM_MAXPREV];
+    const uint32_t* tab_off[SGM_MAXPREV];  // rows by data vertex id, as in sgm_auxplan_dev
     const uint32_t* tab_idx[SGM_MAXPREV];
     const uint32_t* cand[SGM_MAXQ];
     const uint32_t* weight[SGM_MAXQ];
@@ -83,8 +85,9 @@ struct sgm_dynplan_dev {
 // fills out[0, rows) with `fill` (the start value, or 1 for a vertex without a one-parent
 // child).
 struct sgm_wdesc {
-    const uint32_t* toff;
+    const uint32_t* toff;  // the table's row offsets by data vertex id
     const uint32_t* tidx;
+    const uint32_t* rowv;  // C(x): the data vertex of each row
     const uint32_t* in;
     uint32_t* out;
     uint32_t rows, fill;
@@ -118,16 +121,12 @@ struct sgm_pooldesc {
     unsigned long long pbase[SGM_MAXQ];
 };
 
-// One directed query edge (a, b) of an aux-table build: rows = C(a), b's pool offsets and
-// entries, the edge's row counts and scanned row offsets, and its entries when they are a copy.
+// One directed query edge (a, b) of an aux-table build: rows = C(a), b's pool offsets; its
+// entry total and longest row go to slot `slot` of the totals and maxima.
 struct sgm_edgedesc {
     const uint32_t* rows;
-    const uint32_t* poff;
-    uint32_t* rowcnt;
-    const uint32_t* rowoff;
-    const uint32_t* src;
-    uint32_t* dst;
-    uint32_t nrows, pad;
+    const uint32_t* voff;
+    uint32_t nrows, slot;
 };
 
 struct sgm_stats_dev {

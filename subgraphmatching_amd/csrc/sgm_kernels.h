@@ -52,23 +52,23 @@ void launch_bitmaps(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V)
 void launch_rank_scan(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V, uint32_t max_n);
 void launch_compact(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V);
 // aux tables of many queries at once: the row pools' lengths (one block row per query), one
-// scan per segment (pool offsets per (query, b), row offsets per directed edge), the edges'
-// row counts, the pools' entries, and the copies of the edges that do not alias a pool
+// scan per segment (pool offsets per (query, b)), the edges' entry totals and longest rows,
+// and the pools' entries
 void launch_scan_segments(hipStream_t s, const sgm_scandesc* ds, uint32_t m);
 void launch_pool_count(hipStream_t s, const uint32_t* off, const uint32_t* adj, uint32_t V, const sgm_pooldesc* ds,
                        uint32_t nq, uint32_t kmax);
 void launch_pool_fill(hipStream_t s, const uint32_t* off, const uint32_t* adj, uint32_t V, const sgm_pooldesc* ds,
                       uint32_t nq, uint32_t kmax);
-void launch_row_counts(hipStream_t s, const sgm_edgedesc* ds, uint32_t m, uint32_t max_rows);
-void launch_edge_copy(hipStream_t s, const sgm_edgedesc* ds, uint32_t m, uint32_t max_rows);
+void launch_edge_stats(hipStream_t s, const sgm_edgedesc* ds, uint32_t m, uint32_t max_rows, uint32_t* tot,
+                       uint32_t* maxv);
 // BFS-tree filters (CFL / CECI / DAF): step `step` of every query's refinement program
 // (SGM_RPROG_WORDS words per query, see k_refine), and the copy of the NLF words into vmr
 void launch_refine(hipStream_t s, const uint32_t* off, const uint32_t* adj, uint32_t V, const sgm_qdesc* qs,
                    uint32_t nq, const uint32_t* prog, uint32_t step);
 void launch_copy_vmask(hipStream_t s, uint32_t V, const sgm_qdesc* qs, uint32_t nq);
-// order estimates over an aux table: out[j] (= or *=) the row sum of in[idx] (in == null: row
-// lengths), *total += all row sums, all mod 2^64
-void launch_row_gather(hipStream_t s, const uint32_t* toff, const uint32_t* tidx, uint32_t rows,
+// order estimates over an aux table: out[j] (= or *=) the row sum of in[idx] over the row of
+// data vertex rowv[j] (in == null: row lengths), *total += all row sums, all mod 2^64
+void launch_row_gather(hipStream_t s, const uint32_t* toff, const uint32_t* tidx, const uint32_t* rowv, uint32_t rows,
                        const unsigned long long* in, unsigned long long* out, int mul, unsigned long long* total);
 void launch_fill_u64(hipStream_t s, unsigned long long* p, size_t n, unsigned long long val);
 // DAF weights (computeWeightArray): one reverse-BFS step of many queries, m descriptors

@@ -905,55 +905,30 @@ __global__ void __launch_bounds__(256) k_pool_fill(const uint32_t* __restrict__ 
     }
 }
 
-// Row counts of every directed edge (a, b): out[r] = the pool row length of C(a)[r] in b's
-// offsets (blockIdx.y = the edge).
-__global__ void __launch_bounds__(256) k_row_counts(const sgm_edgedesc* __restrict__ ds)
+// Entry total and longest row of every directed edge (a, b) over the pool rows of C(a)
+// (blockIdx.y = the edge; tot and maxv cleared before the launch).
+__global__ void __launch_bounds__(256) k_edge_stats(const sgm_edgedesc* __restrict__ ds, uint32_t* __restrict__ tot,
+                                                    uint32_t* __restrict__ maxv)
 {
     const sgm_edgedesc& D = ds[blockIdx.y];
     const glb<uint32_t>* __restrict__ rows = gcast(D.rows);
-    const glb<uint32_t>* __restrict__ poff = gcast(D.poff);
-    glb<uint32_t>* __restrict__ out = gcast(D.rowcnt);
+    const glb<uint32_t>* __restrict__ voff = gcast(D.voff);
+    uint32_t sum = 0, mx = 0;
     for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < D.nrows; r += gridDim.x * blockDim.x) {
         const uint32_t v = rows[r];
-        out[r] = poff[v + 1] - poff[v];
+        const uint32_t len = voff[v + 1] - voff[v];
+        sum += len;
+        mx = len > mx ? len : mx;
     }
-}
-
-// An edge's table as a copy of the pool rows of C(a) (blockIdx.y = the edge): a wave takes 64
-// rows and copies their concatenation 64 entries at a time; the destination of the wave's
-// rows is one contiguous run from the first row's offset.
-__global__ void __launch_bounds__(256) k_edge_copy(const sgm_edgedesc* __restrict__ ds)
-{
-    const sgm_edgedesc& D = ds[blockIdx.y];
-    const glb<uint32_t>* __restrict__ rows = gcast(D.rows);
-    const glb<uint32_t>* __restrict__ poff = gcast(D.poff);
-    const glb<uint32_t>* __restrict__ rowoff = gcast((const uint32_t*)D.rowoff);
-    const glb<uint32_t>* __restrict__ src = gcast(D.src);
-    glb<uint32_t>* __restrict__ dst = gcast(D.dst);
-    const uint32_t waves = blockDim.x >> 6, wave = threadIdx.x >> 6, lane = lane_id();
-    const uint32_t nrows = D.nrows;
-    for (uint32_t g = blockIdx.x * waves + wave; (uint64_t)g * 64 < nrows; g += gridDim.x * waves) {
-        const uint32_t r = g * 64 + lane;
-        const bool valid = r < nrows;
-        const uint32_t d0 = rowoff[g * 64];
-        const uint32_t s = valid ? poff[rows[r]] : 0u;
-        const uint32_t len = valid ? rowoff[r + 1] - rowoff[r] : 0u;
-        const uint32_t incl = wave_incl_scan(len), total = rdlane(incl, 63);
-        // four 64-entry steps at a time: their owner searches and loads overlap
-        for (uint32_t b0 = 0; b0 < total; b0 += 256) {
-            uint32_t val[4];
-            bool act[4];
+    sum = wave_sum32(sum);
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const TabStep st = tab_step(incl, len, total, b0 + 64u * u, lane);
-                const uint32_t sj = (uint32_t)__shfl((int)s, (int)(st.own & 63u));
-                act[u] = st.act;
-                val[u] = st.act ? src[(size_t)sj + st.pos] : 0u;
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++)
-                if (act[u]) __builtin_nontemporal_store(val[u], dst + ((size_t)d0 + b0 + 64u * u + lane));
-        }
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t y = (uint32_t)__shfl_xor((int)mx, o);
+        mx = y > mx ? y : mx;
+    }
+    if (lane_id() == 0) {
+        if (sum) atomicAdd(tot + D.slot, sum);
+        if (mx) atomicMax(maxv + D.slot, mx);
     }
 }
 
@@ -1081,13 +1056,15 @@ __global__ void __launch_bounds__(256) k_copy_vmask(uint32_t V, const sgm_qdesc*
 // in[idx[k]] (in == null: 1 each, the row length), wrapping mod 2^64 as the reference's
 // size_t sums do; out[j] = s_j, or out[j] *= s_j when mul is set; *total += sum of s_j.
 __global__ void __launch_bounds__(256) k_row_gather(const uint32_t* __restrict__ toff, const uint32_t* __restrict__ tidx,
-                                                    uint32_t rows, const unsigned long long* __restrict__ in,
+                                                    const uint32_t* __restrict__ rowv, uint32_t rows,
+                                                    const unsigned long long* __restrict__ in,
                                                     unsigned long long* __restrict__ out, int mul,
                                                     unsigned long long* __restrict__ total)
 {
     unsigned long long t = 0;
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < rows; j += gridDim.x * blockDim.x) {
-        const uint32_t b = toff[j], e = toff[j + 1];
+        const uint32_t v = rowv[j];
+        const uint32_t b = toff[v], e = toff[v + 1];
         unsigned long long s = 0;
         if (in)
             for (uint32_t k = b; k < e; k++) s += in[tidx[k]];
@@ -1128,8 +1105,9 @@ __global__ void __launch_bounds__(256) k_weight_step(const sgm_wdesc* __restrict
         }
         uint32_t b = 0, e = 0;
         if (valid) {
-            b = toff[j];
-            e = toff[j + 1];
+            const uint32_t v = D.rowv[j];
+            b = toff[v];
+            e = toff[v + 1];
         }
         const bool lng = valid && e - b > 16;
         uint32_t s = 0;
@@ -1312,24 +1290,14 @@ void launch_pool_fill(hipStream_t s, const uint32_t* off, const uint32_t* adj, u
     }
 }
 
-void launch_row_counts(hipStream_t s, const sgm_edgedesc* ds, uint32_t m, uint32_t max_rows)
+void launch_edge_stats(hipStream_t s, const sgm_edgedesc* ds, uint32_t m, uint32_t max_rows, uint32_t* tot,
+                       uint32_t* maxv)
 {
     if (!m || !max_rows) return;
     for (uint32_t s0 = 0; s0 < m; s0 += kMaxGridY) {
         const uint32_t k = m - s0 < kMaxGridY ? m - s0 : kMaxGridY;
         dim3 grid(clampg(((uint64_t)max_rows + 255) / 256, 1, 64), k);
-        hipLaunchKernelGGL(k_row_counts, grid, dim3(256), 0, s, ds + s0);
-    }
-}
-
-void launch_edge_copy(hipStream_t s, const sgm_edgedesc* ds, uint32_t m, uint32_t max_rows)
-{
-    if (!m || !max_rows) return;
-    const uint32_t xcap = 16384 / m > 8 ? 16384 / m : 8u;
-    for (uint32_t s0 = 0; s0 < m; s0 += kMaxGridY) {
-        const uint32_t k = m - s0 < kMaxGridY ? m - s0 : kMaxGridY;
-        dim3 grid(clampg(((uint64_t)max_rows + 255) / 256, 1, xcap), k);
-        hipLaunchKernelGGL(k_edge_copy, grid, dim3(256), 0, s, ds + s0);
+        hipLaunchKernelGGL(k_edge_stats, grid, dim3(256), 0, s, ds + s0, tot, maxv);
     }
 }
 
@@ -1360,12 +1328,12 @@ void launch_copy_vmask(hipStream_t s, uint32_t V, const sgm_qdesc* qs, uint32_t 
     if (nq && V) hipLaunchKernelGGL(k_copy_vmask, grid, dim3(256), 0, s, V, qs);
 }
 
-void launch_row_gather(hipStream_t s, const uint32_t* toff, const uint32_t* tidx, uint32_t rows,
+void launch_row_gather(hipStream_t s, const uint32_t* toff, const uint32_t* tidx, const uint32_t* rowv, uint32_t rows,
                        const unsigned long long* in, unsigned long long* out, int mul, unsigned long long* total)
 {
     if (rows)
         hipLaunchKernelGGL(k_row_gather, dim3(clampg(((uint64_t)rows + 255) / 256, 1, 1024)), dim3(256), 0, s, toff,
-                           tidx, rows, in, out, mul, total);
+                           tidx, rowv, rows, in, out, mul, total);
 }
 
 void launch_weight_step(hipStream_t s, const sgm_wdesc* ds, uint32_t m, uint32_t max_rows)

@@ -75,9 +75,8 @@ struct sgm_ctx {
     size_t tb_bytes = 0;
     uint32_t* d_ttot = nullptr;
     uint32_t ttot_cap = 0;
-    uint32_t* d_poff = nullptr;  // row pool lengths / offsets of the queries of one build pass
-    uint32_t* d_rrec = nullptr;  // their rank records
-    size_t poff_cap = 0, rrec_cap = 0;
+    uint32_t* d_rrec = nullptr;  // rank records of the queries of one table build pass
+    size_t rrec_cap = 0;
     // batched enumeration: bitmap of C(u_L) per query; depth-1 split blocks and chunk records
     uint32_t* d_lastbits = nullptr;
     size_t lastbits_words = 0;
@@ -149,12 +148,16 @@ static const uint64_t kGqlScratchBytes = 8ull << 30;
 static const uint32_t kHubMinDegree = 64;
 static const uint64_t kHubBudgetBytes = 1ull << 30;
 
+// Table of directed query edge (a, b): row r (the r-th candidate v of a) is
+// d_pool[d_voff[v] .. d_voff[v + 1]), positions in C(b).  d_voff (V + 1 offsets by data
+// vertex id) and d_pool are b's row pool, shared by every edge into b; vertices outside the
+// pool have empty rows.
 struct sgm_table {
     bool built = false;
     uint64_t e_cnt = 0;
     uint32_t max_deg = 0;
-    uint32_t* d_off = nullptr;  // |C(a)|+1
-    uint32_t* d_idx = nullptr;
+    const uint32_t* d_voff = nullptr;
+    const uint32_t* d_pool = nullptr;
 };
 
 struct sgm_query {
@@ -188,13 +191,11 @@ struct sgm_query {
     sgm_cdesc hc;
     std::vector<sgm_table> tables;    // n*n, directed (a,b): views into the arenas below
     // table arenas, grown as needed and kept across filters (no allocation when the query is
-    // filtered and tabled again): offsets, entries (positions in C(b)), and scratch
-    // (row counts, then one maximum per directed edge)
-    uint32_t* d_toff = nullptr;
-    uint32_t* d_tidx = nullptr;
-    uint32_t* d_tscr = nullptr;
-    uint32_t* d_tpool = nullptr;  // the row pool (edges whose rows are all of b's pool alias it)
-    size_t toff_cap = 0, tidx_cap = 0, tscr_cap = 0, tpool_cap = 0;
+    // filtered and tabled again): the row pools' offsets by vertex (n blocks of V + 1) and
+    // entries (positions in C(b))
+    uint32_t* d_tvoff = nullptr;
+    uint32_t* d_tpool = nullptr;
+    size_t tvoff_cap = 0, tpool_cap = 0;
     // DAF weights of the dynamic-order search (computeWeightArray), per candidate position in
     // the cand_off layout, for the start order weight_order; dropped with the tables
     uint32_t* d_weight = nullptr;
@@ -316,6 +317,8 @@ int hub_slot_map(sgm_graph* g, const sgm_enum_opts& o, const uint32_t** out);
 int ensure_stack(sgm_ctx* c, size_t bytes);
 // Aux tables (buildTables) of every directed edge of the queries, in one set of launches.
 int build_tables_many(sgm_query* const* qs, uint32_t nq);
+// edge (a, b)'s table in the reference's layout, gathered on the host from b's row pool
+int table_fetch(sgm_query* q, uint32_t a, uint32_t b, std::vector<uint32_t>& off, std::vector<uint32_t>& idx);
 // The dynamic-order searches over many queries (sgm_dynamic.cpp): orders[i] = start order
 // (checked); and their default start order, BFS from DAFStartVertex.
 int enumerate_dyn_batch(sgm_query* const* qs, const uint32_t* const* orders, uint32_t nq, const sgm_enum_opts& o,

@@ -179,12 +179,14 @@ int sgm_order_cfl(sgm_query* q, uint32_t* order)
         {
             const uint32_t s = path[L - 2], e = path[L - 1];
             const sgm_table& t = q->tables[(size_t)s * n + e];
-            sgm::launch_row_gather(c->stream, t.d_off, t.d_idx, q->counts[s], nullptr, child, 0, B.tot + (L - 2));
+            sgm::launch_row_gather(c->stream, t.d_voff, t.d_pool, q->d_list + q->cand_off[s], q->counts[s], nullptr, child,
+                                   0, B.tot + (L - 2));
         }
         for (size_t i = L - 2; i >= 1; i--) {
             const uint32_t s = path[i - 1], e = path[i];
             const sgm_table& t = q->tables[(size_t)s * n + e];
-            sgm::launch_row_gather(c->stream, t.d_off, t.d_idx, q->counts[s], child, par, 0, B.tot + (i - 1));
+            sgm::launch_row_gather(c->stream, t.d_voff, t.d_pool, q->d_list + q->cand_off[s], q->counts[s], child, par, 0,
+                                   B.tot + (i - 1));
             std::swap(child, par);
         }
         HIPCHK(c, hipGetLastError());
@@ -230,7 +232,8 @@ int sgm_order_gup(sgm_query* q, uint32_t* order)
                 rec(ch);
                 edges.emplace_back(u, ch);
                 const sgm_table& tb = q->tables[(size_t)u * n + ch];
-                sgm::launch_row_gather(c->stream, tb.d_off, tb.d_idx, q->counts[u], B.per + B.off[ch],
+                sgm::launch_row_gather(c->stream, tb.d_voff, tb.d_pool, q->d_list + q->cand_off[u], q->counts[u],
+                                       B.per + B.off[ch],
                                        B.per + B.off[u], 1, B.tot + ch);
             }
         };
