@@ -1629,7 +1629,9 @@ __device__ void aux_last_chunk(const sgm_auxplan_dev& A, WaveAux& S, uint32_t L,
 // Many queries per launch: items are the depth-0 candidates of every query, query q owning
 // item_first[q+1] - item_first[q] of them, handed out by its counter next[q] (zero at
 // launch) in the QueryWalk order; plans, statistics and stop words are per query.
-__global__ void __launch_bounds__(256) k_enum_aux(const sgm_auxplan_dev* __restrict__ plans,
+// Four waves per SIMD (at most 128 VGPRs, a few spilled): 90 ms per Chung-Lu 200-query
+// launch against 98.6 ms at the unconstrained 157 VGPRs (three waves per SIMD).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_enum_aux(const sgm_auxplan_dev* __restrict__ plans,
                                                   const uint32_t* __restrict__ item_first, uint32_t nq,
                                                   uint32_t* __restrict__ stack, uint32_t stack_words,
                                                   uint32_t* __restrict__ next, sgm_stats_dev* __restrict__ stats,
