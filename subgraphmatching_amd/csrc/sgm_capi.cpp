@@ -86,6 +86,8 @@ void sgm_ctx_destroy(sgm_ctx* c)
     dfree(c->d_tb);
     dfree(c->d_ttot);
     dfree(c->d_rrec);
+    dfree(c->d_rootord);
+    dfree(c->d_rootdesc);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
     if (c->h_stop) (void)hipHostFree(c->h_stop);
     dfree(c->d_lastbits);
@@ -969,6 +971,40 @@ extern "C++" int build_tables_many(sgm_query* const* qs, uint32_t nq)
     return rc;
 }
 
+extern "C++" int root_orders(sgm_ctx* c, sgm_query* const* qs, const uint32_t* us, uint32_t nq,
+                            std::vector<const uint32_t*>& root_order)
+{
+    root_order.assign(nq, nullptr);
+    if (!nq) return SGM_OK;
+    sgm_graph* g = qs[0]->g;
+    const uint32_t W = (g->V + 63) / 64;
+    size_t total = 0;
+    std::vector<sgm_rootdesc> ds;
+    std::vector<size_t> at(nq, 0);
+    for (uint32_t i = 0; i < nq; i++) {
+        const sgm_query* q = qs[i];
+        if (q->empty || !q->compacted) continue;
+        at[i] = total;
+        total += q->counts[us[i]];
+    }
+    HIPCHK(c, grow_arena(&c->d_rootord, c->rootord_cap, std::max<size_t>(total, 1)));
+    for (uint32_t i = 0; i < nq; i++) {
+        const sgm_query* q = qs[i];
+        if (q->empty || !q->compacted) continue;
+        root_order[i] = c->d_rootord + at[i];
+        ds.push_back(sgm_rootdesc{q->d_vmask, q->d_bm + (size_t)us[i] * W, q->d_rank + (size_t)us[i] * W,
+                                  c->d_rootord + at[i], us[i], 0});
+    }
+    if (ds.empty()) return SGM_OK;
+    if (const int rc = grow(c, &c->d_rootdesc, c->rootdesc_cap, (uint32_t)ds.size())) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->d_rootdesc, ds.data(), sizeof(sgm_rootdesc) * ds.size(), hipMemcpyHostToDevice,
+                             c->stream));
+    sgm::launch_root_order(c->stream, g->d_bydeg, g->V, c->d_rootdesc, (uint32_t)ds.size());
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // ds is read by the copy above
+    return SGM_OK;
+}
+
 // Edge (a, b)'s table in the reference's layout (offsets per candidate row of a, then the
 // positions in C(b)), gathered on the host from b's pool at a's candidates' rows.
 extern "C++" int table_fetch(sgm_query* q, uint32_t a, uint32_t b, std::vector<uint32_t>& off, std::vector<uint32_t>& idx)
@@ -1265,8 +1301,15 @@ static int enumerate_aux_batch(sgm_query* const* qs, const uint32_t* const* orde
         }
     }
     const int use_fs = o.method == SGM_ENUM_AUX_TABLES_FS || o.method == SGM_ENUM_AUX_TABLES_FS_GUPCD ? 1 : 0;
-    const uint32_t* hub_slot = nullptr;  // adjacency bitmaps of the hubs, for the last level's probes
+    const uint32_t* hub_slot = nullptr;  // adjacency bitmaps of the hubs, for the row probes
     if (const int hrc = hub_slot_map(g, o, &hub_slot)) return hrc;
+    {  // each query's roots in degree order
+        std::vector<uint32_t> us(nq);
+        for (uint32_t i = 0; i < nq; i++) us[i] = orders[i][0];
+        std::vector<const uint32_t*> ro;
+        if (const int rrc = root_orders(c, qs, us.data(), nq, ro)) return rrc;
+        for (uint32_t i = 0; i < nq; i++) aps[i].root_order = ro[i];
+    }
     uint32_t rlo, rhi;
     root_range(o, &rlo, &rhi);
     std::vector<uint32_t> live;

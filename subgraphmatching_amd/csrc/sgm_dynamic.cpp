@@ -219,6 +219,13 @@ extern "C++" int enumerate_dyn_batch(sgm_query* const* qs, const uint32_t* const
     root_range(o, &rlo, &rhi);
     const uint32_t* hub_slot = nullptr;  // adjacency bitmaps of the hubs, for row probes
     if (const int hrc = hub_slot_map(g, o, &hub_slot)) return hrc;
+    {  // each query's roots in degree order
+        std::vector<uint32_t> us(nq);
+        for (uint32_t i = 0; i < nq; i++) us[i] = plans[i].root;
+        std::vector<const uint32_t*> ro;
+        if (const int rrc = root_orders(c, qs, us.data(), nq, ro)) return rrc;
+        for (uint32_t i = 0; i < nq; i++) plans[i].root_order = ro[i];
+    }
     std::vector<uint32_t> live;
     for (uint32_t i = 0; i < nq; i++)
         if (!qs[i]->empty) live.push_back(i);

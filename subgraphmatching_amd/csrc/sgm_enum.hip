@@ -1698,7 +1698,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         }
         if (__atomic_load_n(stop, __ATOMIC_RELAXED)) continue;  // capped (or voided) query
         const sgm_auxplan_dev& A = *Ap;
-        const uint32_t root = it;
+        const uint32_t root = A.root_order ? A.root_order[it] : it;  // roots in degree order
         const uint32_t v0 = A.cand[0][root];
         if (v0 < root_lo || v0 >= root_hi) continue;  // outside the launch's root range
         items_done++;
@@ -2112,7 +2112,7 @@ __global__ void __launch_bounds__(256) k_enum_dyn(const sgm_dynplan_dev* __restr
         if (__atomic_load_n(stop, __ATOMIC_RELAXED)) continue;
         const sgm_dynplan_dev& P = *Pp;
         const uint32_t u0 = P.root;
-        const uint32_t root = it;
+        const uint32_t root = P.root_order ? P.root_order[it] : it;  // roots in degree order
         const uint32_t v0 = P.cand[u0][root];
         if (v0 < root_lo || v0 >= root_hi) continue;
         items_done++;

@@ -47,6 +47,7 @@ struct sgm_auxplan_dev {
     const uint32_t* tab_off[SGM_MAXPREV];
     const uint32_t* tab_idx[SGM_MAXPREV];
     const uint32_t* cand[SGM_MAXQ];   // candidate list of the query vertex at each depth
+    const uint32_t* root_order;       // positions in cand[0] in degree order (the items' roots)
     uint64_t anc[SGM_MAXQ];           // DAF ancestor masks over depths (failing sets, inc/prune.h:100-104)
     // GuP conflict detection (inc/prune.h:488-923), or null: per depth, 4 words per candidate
     // position -- the guard's size (0 = none), then its data vertices
@@ -72,6 +73,7 @@ struct sgm_dynplan_dev {
     const uint32_t* tab_idx[SGM_MAXPREV];
     const uint32_t* cand[SGM_MAXQ];
     const uint32_t* weight[SGM_MAXQ];
+    const uint32_t* root_order;  // positions in cand[root] in degree order (the items' roots)
 };
 
 // The directed query edges (a, b_j), j < k, out of one query vertex a in the fill pass of an
@@ -123,6 +125,16 @@ struct sgm_pooldesc {
 
 // One directed query edge (a, b) of an aux-table build: rows = C(a), b's pool offsets; its
 // entry total and longest row go to slot `slot` of the totals and maxima.
+// The depth-0 candidates of one query in degree order (descending, ties by id), as positions
+// in C(u): the root walk of the aux-table and dynamic-order searches.
+struct sgm_rootdesc {
+    const uint32_t* vmask;
+    const uint64_t* bm;    // C(u)'s bitmap words
+    const uint32_t* rank;  // their ranks
+    uint32_t* out;         // |C(u)| positions
+    uint32_t u, pad;
+};
+
 struct sgm_edgedesc {
     const uint32_t* rows;
     const uint32_t* voff;

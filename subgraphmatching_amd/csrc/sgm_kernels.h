@@ -55,6 +55,8 @@ void launch_compact(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V)
 // scan per segment (pool offsets per (query, b)), the edges' entry totals and longest rows,
 // and the pools' entries
 void launch_scan_segments(hipStream_t s, const sgm_scandesc* ds, uint32_t m);
+// each query's depth-0 candidates in degree order, as positions in C(u) (one block per query)
+void launch_root_order(hipStream_t s, const uint32_t* by_deg, uint32_t V, const sgm_rootdesc* ds, uint32_t nq);
 void launch_pool_count(hipStream_t s, const uint32_t* off, const uint32_t* adj, uint32_t V, const sgm_pooldesc* ds,
                        uint32_t nq, uint32_t kmax);
 void launch_pool_fill(hipStream_t s, const uint32_t* off, const uint32_t* adj, uint32_t V, const sgm_pooldesc* ds,

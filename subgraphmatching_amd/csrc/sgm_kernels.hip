@@ -821,6 +821,39 @@ __global__ void __launch_bounds__(256) k_pool_count(const uint32_t* __restrict__
     }
 }
 
+// The depth-0 candidates of each query (blockIdx.x) in degree order: by_deg walked in tiles
+// of 256, candidates placed at their block prefix, each as its position in C(u).  Walking the
+// roots of a capped search from the highest degree reaches the cap with fewer descents, as
+// the cache + filter search does with its degree-ordered items.
+__global__ void __launch_bounds__(256) k_root_order(const uint32_t* __restrict__ by_deg, uint32_t V,
+                                                    const sgm_rootdesc* __restrict__ ds)
+{
+    const sgm_rootdesc& D = ds[blockIdx.x];
+    const glb<uint32_t>* __restrict__ vmask = gcast(D.vmask);
+    const glb<uint64_t>* __restrict__ bm = gcast(D.bm);
+    const glb<uint32_t>* __restrict__ rank = gcast(D.rank);
+    glb<uint32_t>* __restrict__ out = gcast(D.out);
+    __shared__ uint32_t wsum[4];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
+    for (uint32_t base = 0; base < V; base += 256) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t v = i < V ? by_deg[i] : 0u;
+        const bool c = i < V && ((vmask[v] >> D.u) & 1u);
+        const uint64_t bal = __ballot(c);
+        if (lane == 0) wsum[wv] = (uint32_t)__popcll(bal);
+        __syncthreads();
+        uint32_t pre = carry;
+        for (uint32_t k = 0; k < wv; k++) pre += wsum[k];
+        if (c) out[pre + mbcnt64(bal)] = cand_rank(bm, rank, v);
+        __syncthreads();
+        if (threadIdx.x == 0) carry += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        __syncthreads();
+    }
+}
+
 // Rank records of one query (blockIdx.y): rrec[w * rstride + b] = the position of w in C(b),
 // or SGM_INVALID when w is not a candidate of b (b < rstride, a multiple of 4 >= n), so the
 // pool pass reads every query vertex's rank of a neighbour in one 16-byte-aligned record.
@@ -1258,6 +1291,11 @@ void launch_compact(hipStream_t s, const sgm_cdesc* cs, uint32_t nq, uint32_t V)
 
 // grid.y / grid.x hold at most 65535 descriptors per launch: longer lists go in pieces
 static const uint32_t kMaxGridY = 65535;
+
+void launch_root_order(hipStream_t s, const uint32_t* by_deg, uint32_t V, const sgm_rootdesc* ds, uint32_t nq)
+{
+    if (nq && V) hipLaunchKernelGGL(k_root_order, dim3(nq), dim3(256), 0, s, by_deg, V, ds);
+}
 
 void launch_pool_count(hipStream_t s, const uint32_t* off, const uint32_t* adj, uint32_t V, const sgm_pooldesc* ds,
                        uint32_t nq, uint32_t kmax)

@@ -77,6 +77,11 @@ struct sgm_ctx {
     uint32_t ttot_cap = 0;
     uint32_t* d_rrec = nullptr;  // rank records of the queries of one table build pass
     size_t rrec_cap = 0;
+    // the depth-0 candidates of an aux-table / dynamic-order launch's queries in degree order
+    uint32_t* d_rootord = nullptr;
+    size_t rootord_cap = 0;
+    sgm_rootdesc* d_rootdesc = nullptr;
+    uint32_t rootdesc_cap = 0;
     // batched enumeration: bitmap of C(u_L) per query; depth-1 split blocks and chunk records
     uint32_t* d_lastbits = nullptr;
     size_t lastbits_words = 0;
@@ -317,6 +322,10 @@ int hub_slot_map(sgm_graph* g, const sgm_enum_opts& o, const uint32_t** out);
 int ensure_stack(sgm_ctx* c, size_t bytes);
 // Aux tables (buildTables) of every directed edge of the queries, in one set of launches.
 int build_tables_many(sgm_query* const* qs, uint32_t nq);
+// root_order[i] (device, queued on the context's stream) = the candidates of query vertex
+// us[i] of qs[i] in degree order as positions in its candidate list; null for an empty query
+int root_orders(sgm_ctx* c, sgm_query* const* qs, const uint32_t* us, uint32_t nq,
+                std::vector<const uint32_t*>& root_order);
 // edge (a, b)'s table in the reference's layout, gathered on the host from b's row pool
 int table_fetch(sgm_query* q, uint32_t a, uint32_t b, std::vector<uint32_t>& off, std::vector<uint32_t>& idx);
 // The dynamic-order searches over many queries (sgm_dynamic.cpp): orders[i] = start order
