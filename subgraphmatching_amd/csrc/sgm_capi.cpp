@@ -88,6 +88,8 @@ void sgm_ctx_destroy(sgm_ctx* c)
     dfree(c->d_rrec);
     dfree(c->d_rootord);
     dfree(c->d_rootdesc);
+    dfree(c->d_cmpoff);
+    dfree(c->d_cmpdesc);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
     if (c->h_stop) (void)hipHostFree(c->h_stop);
     dfree(c->d_lastbits);
@@ -929,6 +931,11 @@ extern "C++" int build_tables_many(sgm_query* const* qs, uint32_t nq)
     sgm_ctx* c = g->ctx;
     HIPCHK(c, hipSetDevice(c->device));
     std::vector<uint32_t> todo;
+    // candidate lists of the queries not compacted yet, all in one launch (the descriptors
+    // and prefix arrays stay alive until the passes below synchronise)
+    std::vector<sgm_cdesc> cds;
+    std::vector<uint32_t> coffs;
+    std::vector<size_t> cat;
     for (uint32_t i = 0; i < nq; i++) {
         sgm_query* q = qs[i];
         if (q->g != g) return set_err(c, SGM_EINVAL, "tables: queries must share one graph");
@@ -936,20 +943,30 @@ extern "C++" int build_tables_many(sgm_query* const* qs, uint32_t nq)
         if (rc) return rc;
         free_tables(q);
         if (q->empty) continue;
-        if (!q->compacted) {  // candidate lists (ensure_compacted without its synchronisation)
+        if (!q->compacted) {
             const uint32_t n = q->q.n;
             q->cand_off[0] = 0;
             for (uint32_t u = 0; u < n; u++) q->cand_off[u + 1] = q->cand_off[u] + q->counts[u];
             HIPCHK(c, grow_arena(&q->d_list, q->list_cap, std::max<size_t>(q->cand_off[n], 1)));
             q->hc.list = q->d_list;
-            HIPCHK(c, hipMemcpyAsync(q->d_cdesc, &q->hc, sizeof q->hc, hipMemcpyHostToDevice, c->stream));
-            HIPCHK(c, hipMemcpyAsync(q->d_candoff, q->cand_off, sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice,
-                                     c->stream));
-            sgm::launch_compact(c->stream, q->d_cdesc, 1, g->V);
+            cds.push_back(q->hc);
+            cat.push_back(coffs.size());
+            coffs.insert(coffs.end(), q->cand_off, q->cand_off + n + 1);
             q->compacted = true;
         }
         HIPCHK(c, grow_arena(&q->d_tvoff, q->tvoff_cap, (size_t)q->q.n * ((size_t)g->V + 1)));
         todo.push_back(i);
+    }
+    if (!cds.empty()) {
+        HIPCHK(c, grow_arena(&c->d_cmpoff, c->cmpoff_cap, coffs.size()));
+        if (const int grc = grow(c, &c->d_cmpdesc, c->cmpdesc_cap, (uint32_t)cds.size())) return grc;
+        for (size_t k = 0; k < cds.size(); k++) cds[k].cand_off = c->d_cmpoff + cat[k];
+        HIPCHK(c, hipMemcpyAsync(c->d_cmpoff, coffs.data(), sizeof(uint32_t) * coffs.size(), hipMemcpyHostToDevice,
+                                 c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->d_cmpdesc, cds.data(), sizeof(sgm_cdesc) * cds.size(), hipMemcpyHostToDevice,
+                                 c->stream));
+        sgm::launch_compact(c->stream, c->d_cmpdesc, (uint32_t)cds.size(), g->V);
+        HIPCHK(c, hipGetLastError());
     }
     // passes of queries whose rank records fit a scratch of at most 2^30 words (a single
     // larger query takes a pass of its own)
@@ -962,7 +979,7 @@ extern "C++" int build_tables_many(sgm_query* const* qs, uint32_t nq)
         rc = build_tables_pass(c, g, qs, todo.data() + k0, (uint32_t)(k1 - k0));
         k0 = k1;
     }
-    if (rc == SGM_OK && todo.empty()) {
+    if (rc == SGM_OK) {  // (the passes end synchronised; this covers none, and the compaction)
         const hipError_t se = hipStreamSynchronize(c->stream);
         if (se != hipSuccess) rc = set_err(c, SGM_EHIP, "tables: %s", hipGetErrorString(se));
     }

@@ -82,6 +82,11 @@ struct sgm_ctx {
     size_t rootord_cap = 0;
     sgm_rootdesc* d_rootdesc = nullptr;
     uint32_t rootdesc_cap = 0;
+    // candidate-list compaction of many queries in one launch: descriptors, prefix arrays
+    sgm_cdesc* d_cmpdesc = nullptr;
+    uint32_t cmpdesc_cap = 0;
+    uint32_t* d_cmpoff = nullptr;
+    size_t cmpoff_cap = 0;
     // batched enumeration: bitmap of C(u_L) per query; depth-1 split blocks and chunk records
     uint32_t* d_lastbits = nullptr;
     size_t lastbits_words = 0;
