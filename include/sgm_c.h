@@ -198,7 +198,8 @@ int sgm_cand_exist_download(sgm_query* q, uint32_t* words, uint32_t num_words);
 int sgm_build_tables(sgm_query* q);
 /* edge_matrix[a][b] (inc/graph.h:13-49): e_cnt and _max_degree */
 int sgm_table_info(sgm_query* q, uint32_t a, uint32_t b, uint64_t* e_cnt, uint32_t* max_degree);
-/* edge_matrix[a][b]: _offset[|C(a)|+1], _edge[e_cnt], _edge_v[e_cnt] */
+/* edge_matrix[a][b]: _offset[|C(a)|+1], _edge[e_cnt], _edge_v[e_cnt] (the device keeps one row
+ * pool per query vertex b, read by every edge into b; this call gathers the edge's layout) */
 int sgm_table_download(sgm_query* q, uint32_t a, uint32_t b, uint32_t* offsets, uint32_t* idx, uint32_t* vtx);
 
 /* ---- orders and plan (host; replaces Order_RI inc/order.h:1138, Order_GQL :563, PlanIR
