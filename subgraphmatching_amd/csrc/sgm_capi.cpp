@@ -917,7 +917,7 @@ static int build_tables_pass(sgm_ctx* c, sgm_graph* g, sgm_query* const* qs, con
         t.built = true;
     }
     HIPCHK(c, hipMemcpyAsync(dev, st, pool_b, hipMemcpyHostToDevice, c->stream));
-    sgm::launch_pool_fill(c->stream, g->d_off, g->d_adj, V, dpd, nqp, kmax);
+    sgm::launch_pool_fill(c->stream, g->d_off, g->d_adj, V, dpd, nqp);
     HIPCHK(c, hipGetLastError());
     se = hipStreamSynchronize(c->stream);
     if (se != hipSuccess) return set_err(c, SGM_EHIP, "tables: %s", hipGetErrorString(se));

@@ -60,7 +60,7 @@ void launch_root_order(hipStream_t s, const uint32_t* by_deg, uint32_t V, const 
 void launch_pool_count(hipStream_t s, const uint32_t* off, const uint32_t* adj, uint32_t V, const sgm_pooldesc* ds,
                        uint32_t nq, uint32_t kmax);
 void launch_pool_fill(hipStream_t s, const uint32_t* off, const uint32_t* adj, uint32_t V, const sgm_pooldesc* ds,
-                      uint32_t nq, uint32_t kmax);
+                      uint32_t nq);
 void launch_edge_stats(hipStream_t s, const sgm_edgedesc* ds, uint32_t m, uint32_t max_rows, uint32_t* tot,
                        uint32_t* maxv);
 // BFS-tree filters (CFL / CECI / DAF): step `step` of every query's refinement program

@@ -876,11 +876,13 @@ __global__ void __launch_bounds__(256) k_pool_ranks(uint32_t V, const sgm_poolde
 
 // The pool rows: entries of (b, v) at pool + pbase[b] + poff[b][v] (blockIdx.y = the query).
 // A step loads each lane's neighbour's rank record once (rstride / 4 16-byte loads) and
-// takes every query vertex from registers.
+// takes every query vertex from registers.  For each b the rows of a wave's 64 consecutive
+// vertices are one contiguous run of b's pool, and the walk visits their entries in that
+// order, so a step's hits of b go to the run's next free positions: lane b keeps that
+// position, and a hit's place is it plus the hits of lower lanes (no per-row state).
 __global__ void __launch_bounds__(256) k_pool_fill(const uint32_t* __restrict__ off, const uint32_t* __restrict__ adj,
-                                                   uint32_t V, const sgm_pooldesc* __restrict__ ds, uint32_t kmax)
+                                                   uint32_t V, const sgm_pooldesc* __restrict__ ds)
 {
-    extern __shared__ uint32_t tl[];
     const sgm_pooldesc& D = ds[blockIdx.y];
     const glb<uint32_t>* __restrict__ vmask = gcast(D.vmask);
     const glb<uint32_t>* __restrict__ poff = gcast((const uint32_t*)D.poff);
@@ -890,15 +892,12 @@ __global__ void __launch_bounds__(256) k_pool_fill(const uint32_t* __restrict__ 
     const uint32_t n = D.n, nl = D.rstride >> 2;
     // lane b holds b's pool base
     const uint64_t my_base = lane < n ? (uint64_t)(size_t)(D.pool + D.pbase[lane]) : 0ull;
-    uint32_t* mine = tl + (wave * 64 + lane) * kmax;
-    uint32_t* slots = tl + wave * 64 * kmax;
     for (uint32_t g = blockIdx.x * waves + wave; (uint64_t)g * 64 < V; g += gridDim.x * waves) {
         const uint32_t v = g * 64 + lane;
         const bool valid = v < V;
         const uint32_t um = valid ? pool_umask(D, vmask[v]) : 0u;
         const uint32_t o = um ? off[v] : 0u, len = um ? off[v + 1] - o : 0u;
-        for (uint32_t b = 0; b < n; b++) mine[b] = (um >> b) & 1u ? poff[(size_t)b * (V + 1) + v] : 0u;
-        wave_sync();
+        uint32_t my_ptr = lane < n ? poff[(size_t)lane * (V + 1) + g * 64] : 0u;  // lane b: b's next position
         const uint32_t ub = uni(wave_or32(um));
         const uint32_t incl = wave_incl_scan(len), total = rdlane(incl, 63);
         for (uint32_t b0 = 0; b0 < total; b0 += 64) {
@@ -918,23 +917,20 @@ __global__ void __launch_bounds__(256) k_pool_fill(const uint32_t* __restrict__ 
                 rec[4 * i + 2] = x.z;
                 rec[4 * i + 3] = x.w;
             }
-            const uint64_t upto = st.below | (1ull << lane);
 #pragma unroll
             for (uint32_t b = 0; b < SGM_MAXQ; b++) {
                 if (!((ub >> b) & 1u)) continue;
                 const bool hit = ((uj >> b) & 1u) && rec[b] != SGM_INVALID;
                 const uint64_t bal = __ballot(hit);
                 if (!bal) continue;
-                uint32_t* slot = slots + own * kmax + b;
                 if (hit) {
-                    const uint32_t p = *slot + (uint32_t)__popcll(bal & st.below);
+                    const uint32_t p = rdlane(my_ptr, b) + mbcnt64(bal);
                     // streaming store: the pools are written once here
                     __builtin_nontemporal_store(rec[b], gcast((uint32_t*)rdlane64(my_base, b)) + p);
                 }
-                if (st.last) *slot += (uint32_t)__popcll(bal & upto);
+                if (lane == b) my_ptr += (uint32_t)__popcll(bal);
             }
         }
-        wave_sync();
     }
 }
 
@@ -1311,20 +1307,19 @@ void launch_pool_count(hipStream_t s, const uint32_t* off, const uint32_t* adj, 
 }
 
 void launch_pool_fill(hipStream_t s, const uint32_t* off, const uint32_t* adj, uint32_t V, const sgm_pooldesc* ds,
-                      uint32_t nq, uint32_t kmax)
+                      uint32_t nq)
 {
-    if (!nq || !V || !kmax) return;
+    if (!nq || !V) return;
     for (uint32_t s0 = 0; s0 < nq; s0 += kMaxGridY) {
         const uint32_t k = nq - s0 < kMaxGridY ? nq - s0 : kMaxGridY;
         dim3 grid(clampg(((uint64_t)V * 4 + 255) / 256, 1, 16384 / k + 1), k);
         hipLaunchKernelGGL(k_pool_ranks, grid, dim3(256), 0, s, V, ds + s0);
     }
     const uint32_t xcap = 16384 / nq > 8 ? 16384 / nq : 8u;
-    const size_t smem = sizeof(uint32_t) * 4 * 64 * kmax;
     for (uint32_t s0 = 0; s0 < nq; s0 += kMaxGridY) {
         const uint32_t k = nq - s0 < kMaxGridY ? nq - s0 : kMaxGridY;
         dim3 grid(clampg(((uint64_t)V + 255) / 256, 1, xcap), k);
-        hipLaunchKernelGGL(k_pool_fill, grid, dim3(256), smem, s, off, adj, V, ds + s0, kmax);
+        hipLaunchKernelGGL(k_pool_fill, grid, dim3(256), 0, s, off, adj, V, ds + s0);
     }
 }
 
