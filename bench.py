@@ -55,7 +55,7 @@ GPU counts and candidate-set digests of the sampled queries are compared with it
 The default run also reports the rmat20-learned comparison (one timed pass each) under
 "learned_vs_ri" (--no-learned skips it).
 
-Config 2 as written has no cap; its queries hold more than 10^9 embeddings each (no query
+The dense 16-vertex Chung-Lu 10^5 workload is specified without a cap; its queries hold more than 10^9 embeddings each (no query
 finishes on either side within any budget this run can afford), so the default workload
 also runs the 200 queries uncapped once with a per-query work budget (--uncapped-budget-ms
 of wave time each) and reports how many finished, timed-out queries as timed out with the
@@ -109,7 +109,7 @@ def parse():
                          "reference's order comparison)")
     ap.add_argument("--uncapped-budget-ms", type=int, default=-1,
                     help="chunglu100k / chunglu10k: per-query wave-time budget of the uncapped pass (0 skips it; "
-                         "default 10 s per query for the 200 config-2 queries, 2000 s for the one config-1 query: "
+                         "default 10 s per query for the 200 dense 16-vertex Chung-Lu 10^5 queries, 2000 s for the one 8-vertex Chung-Lu 10^4 query: "
                          "about half a second of the whole GPU either way)")
     ap.add_argument("--per-query", action="store_true",
                     help="chunglu100k: one enumeration launch per query instead of one batched launch")

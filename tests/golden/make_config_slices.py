@@ -1,16 +1,16 @@
 """Writes tests/golden/config_slices.json: full-size parity fixtures from the CPU oracle.
 
 Inputs are this project's seeded benchmark inputs:
-  config 1  Chung-Lu 10^4 vertices, exponent 2.5, average degree 7.4 (seed 1); the 8-vertex
+  chunglu10k   Chung-Lu 10^4 vertices, exponent 2.5, average degree 7.4 (seed 1); the 8-vertex
             sparse random-walk query of seed 11; GQL filter, RI order
-  config 2  Chung-Lu 10^5 vertices, exponent 2.3, average degree 16 (seed 2); the 200 dense
+  chunglu100k  Chung-Lu 10^5 vertices, exponent 2.3, average degree 16 (seed 2); the 200 dense
             16-vertex random-walk queries of seed 22; GQL filter, RI order
 
-Whole searches on these inputs run for hours (the config-1 query has more than 1.5e13
+Whole searches on these inputs run for hours (the 8-vertex Chung-Lu 10^4 query has more than 1.5e13
 embeddings), so the fixture holds, per query, root ranges whose searches the oracle
 finishes within a second or two (found by a seeded search over candidate roots) and the
 oracle's statistics for each range and each enumeration variant; plus the candidate-set
-digests of every config-2 query.  Usage:
+digests of every dense 16-vertex Chung-Lu 10^5 query.  Usage:
 
     python tests/golden/make_config_slices.py        (about 5 minutes on 8 cores)
 """
@@ -34,15 +34,15 @@ from slices import VARIANTS, cand_digest, oracle_stats  # noqa: E402
 from subgraphmatching_amd.generators import chung_lu, query_set  # noqa: E402
 
 OUT = os.path.join(HERE, "config_slices.json")
-C2_QUERIES = list(range(0, 200, 25))  # the config-2 queries whose slices are recorded
+C2_QUERIES = list(range(0, 200, 25))  # the Chung-Lu 10^5 queries whose slices are recorded
 
 
-def config1():
+def cl10k():
     g = chung_lu(10000, 2.5, 7.4, seed=1)
     return g, query_set(g, 1, 8, False, seed=11)
 
 
-def config2():
+def cl100k():
     g = chung_lu(100_000, 2.3, 16.0, seed=2)
     return g, query_set(g, 200, 16, True, seed=22)
 
@@ -51,7 +51,7 @@ def find_ranges(g, q, cands, order, seed, singles, windows, width, probe_ms, min
     """Seeded search for root ranges whose every variant finishes: `singles` one-root ranges
     and `windows` id windows of `width` ids.  With min_nodes == 0 a range must hold
     embeddings; otherwise its search tree must have at least min_nodes nodes (on the
-    config-2 inputs every root's subtree either dies within about a thousand nodes or holds
+    Chung-Lu 10^5 inputs every root's subtree either dies within about a thousand nodes or holds
     more than 10^8 embeddings, so exact slices there have no embeddings)."""
     rng = np.random.default_rng(seed)
     roots = np.array(cands[order[0]])
@@ -84,13 +84,13 @@ def find_ranges(g, q, cands, order, seed, singles, windows, width, probe_ms, min
 
 
 def c2_digest(i):
-    g, qs = config2()
+    g, qs = cl100k()
     ok, c = O.gql(g, qs[i])
     return i, bool(ok), cand_digest(c), [len(x) for x in c]
 
 
 def c2_slices(i):
-    g, qs = config2()
+    g, qs = cl100k()
     q = qs[i]
     ok, c = O.gql(g, q)
     order = O.order_ri(q)
@@ -100,22 +100,22 @@ def c2_slices(i):
 def main():
     out = {"about": "oracle statistics of root-range slices (see tests/slices.py); written by "
                     "tests/golden/make_config_slices.py"}
-    g1, (q1,) = config1()
+    g1, (q1,) = cl10k()
     ok, c = O.gql(g1, q1)
     order = O.order_ri(q1)
-    out["config1"] = {"graph": "chung_lu(10000, 2.5, 7.4, seed=1)", "query": "query_set(g, 1, 8, False, seed=11)[0]",
+    out["cl10k"] = {"graph": "chung_lu(10000, 2.5, 7.4, seed=1)", "query": "query_set(g, 1, 8, False, seed=11)[0]",
                       "order": order, "cand_digest": cand_digest(c),
                       "ranges": find_ranges(g1, q1, c, order, 11, 3, 0, 0, 2000.0)}
-    print("config1", len(out["config1"]["ranges"]), "ranges", flush=True)
+    print("cl10k", len(out["cl10k"]["ranges"]), "ranges", flush=True)
     pool = mp.get_context("spawn").Pool(min(8, os.cpu_count() or 1))
     dig = sorted(pool.map(c2_digest, range(200)))
-    out["config2"] = {"graph": "chung_lu(100000, 2.3, 16.0, seed=2)", "queries": "query_set(g, 200, 16, True, seed=22)",
+    out["cl100k"] = {"graph": "chung_lu(100000, 2.3, 16.0, seed=2)", "queries": "query_set(g, 200, 16, True, seed=22)",
                       "ok": [d[1] for d in dig], "cand_digest": [d[2] for d in dig], "cand_counts": [d[3] for d in dig],
                       "slices": {}}
-    print("config2 digests", flush=True)
+    print("cl100k digests", flush=True)
     for i, rec in sorted(pool.map(c2_slices, C2_QUERIES)):
-        out["config2"]["slices"][str(i)] = rec
-        print("config2 query", i, len(rec["ranges"]), "ranges", flush=True)
+        out["cl100k"]["slices"][str(i)] = rec
+        print("cl100k query", i, len(rec["ranges"]), "ranges", flush=True)
     pool.close()
     pool.join()
     with open(OUT, "w") as f:

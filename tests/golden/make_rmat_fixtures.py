@@ -1,18 +1,18 @@
 """Writes tests/golden/rmat_fixtures.json: oracle results for sampled queries of the R-MAT
 benchmark configurations, at their own sizes.
 
-  config 3  R-MAT scale 20 (a,b,c,d = .57,.19,.19,.05), 2*10^7 undirected edges, seed 3; the
+  rmat20    R-MAT scale 20 (a,b,c,d = .57,.19,.19,.05), 2*10^7 undirected edges, seed 3; the
             1,000-query mix of seed 33 (sizes uniform in 4..32, even ids dense, odd sparse);
             64 queries spread over the sizes (every 16th in size order: one batch of 64);
             GQL filter, RI order, cache + filter + failing-set search capped at 10^7
-  config 5  R-MAT scale 22, average degree 32 (2^26 undirected edges), seed 5; the serving
+  rmat22    R-MAT scale 22, average degree 32 (2^26 undirected edges), seed 5; the serving
             stream of seed 55 (1,200 queries of 8..32 vertices, even ids dense); 16 queries
             (every 75th); the same path capped at 10^6
 
 Per query: GQL success, candidate counts and the SHA-256 digest of the candidate sets, the RI
 order and the capped count (min(#embeddings, cap), order-independent, so it also checks
-learned orders).  Every sampled query reaches its cap, so for six config-3 queries and three
-config-5 queries the file also holds root-range slices (tests/slices.py): exact, uncapped
+learned orders).  Every sampled query reaches its cap, so for six R-MAT 2^20 queries and three
+R-MAT 2^22 queries the file also holds root-range slices (tests/slices.py): exact, uncapped
 sub-searches whose count, per-depth nodes, intersections and failing-set cut-offs the
 oracle computed for all four enumeration variants.  The oracle's GQL takes seconds to
 minutes per query on these graphs, so the GPU tests read these results instead of
@@ -47,14 +47,14 @@ _G = None
 _QS = None
 
 
-def config3():
+def rmat20():
     g = rmat(20, 20_000_000, seed=3)
     qs = query_set(g, 1000, lambda i, r: int(r.integers(4, 33)), lambda i, r: i % 2 == 0, seed=33)
     by_size = sorted(range(len(qs)), key=lambda i: qs[i].getVertexCnt())
     return g, qs, [by_size[k] for k in range(0, len(qs), 16)][:64]
 
 
-def config5():
+def rmat22():
     g = rmat(22, 1 << 26, seed=5)
     qs = query_set(g, 1200, lambda i, r: int(r.integers(8, 33)), lambda i, r: i % 2 == 0, seed=55)
     return g, qs, list(range(0, len(qs), 75))
@@ -101,7 +101,7 @@ def add_slices(procs):
     global _G, _QS
     with open(OUT) as f:
         out = json.load(f)
-    for name, build, count in (("config3", config3, 6), ("config5", config5, 3)):
+    for name, build, count in (("rmat20", rmat20, 6), ("rmat22", rmat22, 3)):
         _G, _QS, _ = build()
         ids = [r["id"] for r in out[name]["queries"]]
         pick = [ids[k] for k in range(0, len(ids), max(1, len(ids) // count))][:count]
@@ -124,8 +124,8 @@ def main():
         add_slices(min(8, os.cpu_count() or 1))
         return
     procs = min(8, os.cpu_count() or 1)
-    out = {"config3": run("config3", config3, 10_000_000, procs),
-           "config5": run("config5", config5, 1_000_000, procs)}
+    out = {"rmat20": run("rmat20", rmat20, 10_000_000, procs),
+           "rmat22": run("rmat22", rmat22, 1_000_000, procs)}
     with open(OUT, "w") as f:
         json.dump(out, f, indent=1)
     print("wrote", OUT)

@@ -1,14 +1,14 @@
-"""GPU parity at the benchmark configurations' own sizes (Chung-Lu 10^4 and 10^5 graphs with
+"""GPU parity at the benchmark workloads' own sizes (Chung-Lu 10^4 and 10^5 graphs with
 their random-walk queries).
 
-Whole uncapped searches on these inputs run for hours on any hardware (the config-1 query
-has more than 1.5e13 embeddings; every config-2 query more than 1e9), so enumeration parity
+Whole uncapped searches on these inputs run for hours on any hardware (the 8-vertex Chung-Lu 10^4 query
+has more than 1.5e13 embeddings; every dense 16-vertex Chung-Lu 10^5 query more than 1e9), so enumeration parity
 is shown on root-range slices (tests/slices.py): exact, uncapped sub-searches of the whole
 search whose oracle statistics are in tests/golden/config_slices.json (written by
-tests/golden/make_config_slices.py), and for one slice per config recomputed live.  Every
+tests/golden/make_config_slices.py), and for one slice per workload recomputed live.  Every
 slice compares count, per-depth nodes, intersections and failing-set cut-offs on the cache
 search and the aux-table search, with failing sets on and off.  Candidate sets are compared
-bit for bit (config 1 live, config 2 by the SHA-256 of all 200 queries' sets)."""
+bit for bit (the Chung-Lu 10^4 query live, the 200 Chung-Lu 10^5 queries by the SHA-256 of their sets)."""
 import json
 import os
 import time
@@ -41,13 +41,13 @@ def fix():
 
 
 @pytest.fixture(scope="module")
-def c1():
+def cl10k_in():
     g = chung_lu(10000, 2.5, 7.4, seed=1)
     return g, query_set(g, 1, 8, False, seed=11)[0]
 
 
 @pytest.fixture(scope="module")
-def c2():
+def cl100k_in():
     g = chung_lu(100_000, 2.3, 16.0, seed=2)
     qs = query_set(g, 200, 16, True, seed=22)
     cans = [sgm.CandidateParam(g, q) for q in qs]
@@ -70,9 +70,9 @@ def _check_ranges(can, order, ranges, tag, variants=None):
             assert gpu_stats(res) == rec[name], (tag, r, name)
 
 
-def test_config1_filter_order_and_slices(c1, fix):
-    g, q = c1
-    F = fix["config1"]
+def test_cl10k_filter_order_and_slices(cl10k_in, fix):
+    g, q = cl10k_in
+    F = fix["cl10k"]
     can = sgm.CandidateParam(g, q)
     assert sgm.Filter_GQL(g, q, can, build_tables=False)
     ok_r, c_r = O.gql(g, q)
@@ -88,7 +88,7 @@ def test_config1_filter_order_and_slices(c1, fix):
     # smallest slice runs every variant, the others one each, to keep the test near a minute
     by_size = sorted(F["ranges"], key=lambda x: x["cache_fs"]["count"])
     plan = {id(by_size[0]): VARIANTS_ALL, id(by_size[1]): ("cache_fs", "aux_fs"), id(by_size[2]): ("cache",)}
-    _check_ranges(can, order, F["ranges"], "config1", variants=lambda rec: plan[id(rec)])
+    _check_ranges(can, order, F["ranges"], "cl10k", variants=lambda rec: plan[id(rec)])
     # one slice recomputed by the oracle now (the fixture is the oracle's own output)
     r = tuple(F["ranges"][0]["roots"])
     live = oracle_stats(O, g, q, c_r, order, "cache_fs", r)
@@ -96,15 +96,15 @@ def test_config1_filter_order_and_slices(c1, fix):
     can.close()
 
 
-def test_config2_candidate_sets_all_queries(c2, fix):
+def test_cl100k_candidate_sets_all_queries(cl100k_in, fix):
     """All 200 queries' GQL candidate sets (one filter batch) against the oracle's digests."""
-    g, qs, cans, ok = c2
-    F = fix["config2"]
+    g, qs, cans, ok = cl100k_in
+    F = fix["cl100k"]
     assert ok == F["ok"]
     for i, can in enumerate(cans):
         assert [int(x) for x in can.candidates_count] == F["cand_counts"][i], i
         assert cand_digest(can.candidates) == F["cand_digest"][i], i
-    say("config2 digests", len(cans))
+    say("cl100k digests", len(cans))
     # two queries recomputed by the oracle now
     for i in (0, 199):
         ok_r, c_r = O.gql(g, qs[i])
@@ -112,15 +112,15 @@ def test_config2_candidate_sets_all_queries(c2, fix):
         assert sgm.Order_RI(qs[i]) == O.order_ri(qs[i])
 
 
-def test_config2_slices(c2, fix):
-    g, qs, cans, ok = c2
-    F = fix["config2"]["slices"]
+def test_cl100k_slices(cl100k_in, fix):
+    g, qs, cans, ok = cl100k_in
+    F = fix["cl100k"]["slices"]
     assert len(F) >= 8
     for key, rec in sorted(F.items(), key=lambda kv: int(kv[0])):
         i = int(key)
         assert rec["order"] == sgm.Order_RI(qs[i])
         assert rec["ranges"], i
-        _check_ranges(cans[i], rec["order"], rec["ranges"], f"config2 q{i}")
+        _check_ranges(cans[i], rec["order"], rec["ranges"], f"cl100k q{i}")
     # one slice recomputed by the oracle now, from the GPU's (digest-checked) candidates
     key = sorted(F, key=int)[0]
     i, rec = int(key), F[key]["ranges"][0]
@@ -128,10 +128,10 @@ def test_config2_slices(c2, fix):
     assert live == rec["cache_fs"]
 
 
-def test_config2_slices_in_one_batch(c2, fix):
+def test_cl100k_slices_in_one_batch(cl100k_in, fix):
     """The same slices from one persistent launch over the eight queries (per-query stats)."""
-    g, qs, cans, ok = c2
-    F = fix["config2"]["slices"]
+    g, qs, cans, ok = cl100k_in
+    F = fix["cl100k"]["slices"]
     keys = sorted(F, key=int)
     # one range per launch (the range applies to every query of the launch): use each
     # query's first range, one launch per distinct range, all eight queries in each; the
@@ -147,10 +147,10 @@ def test_config2_slices_in_one_batch(c2, fix):
         assert gpu_stats(mine) == F[key]["ranges"][0]["cache_fs"], key
 
 
-def test_config2_capped_counts_all_queries(c2):
-    """Every config-2 query holds more than 10^9 embeddings, so count = min(total, cap) is
+def test_cl100k_capped_counts_all_queries(cl100k_in):
+    """Every dense 16-vertex Chung-Lu 10^5 query holds more than 10^9 embeddings, so count = min(total, cap) is
     the cap itself for all 200 in one launch; the oracle agrees on two of them."""
-    g, qs, cans, ok = c2
+    g, qs, cans, ok = cl100k_in
     cap = 1_000_000
     res = sgm.enumerate_batch(cans, None, L.SGM_ENUM_CACHE_FILTER_FS, cap=cap)
     assert all(r.ans == cap and r.capped for r in res)
@@ -159,11 +159,11 @@ def test_config2_capped_counts_all_queries(c2):
         assert ref["count"] == cap
 
 
-def test_config2_uncapped_budget_times_out(c2):
-    """Config 2 as written (no cap): with a per-query budget every query times out (their
+def test_cl100k_uncapped_budget_times_out(cl100k_in):
+    """The dense 16-vertex Chung-Lu 10^5 workload as specified (no cap): with a per-query budget every query times out (their
     counts exceed what the budget can count) and is reported timed out, never as a result;
     a cap ends each query exactly at the cap instead."""
-    g, qs, cans, ok = c2
+    g, qs, cans, ok = cl100k_in
     sub = cans[:16]
     res = sgm.enumerate_batch(sub, None, L.SGM_ENUM_CACHE_FILTER_FS, time_limit_ms=2000)
     assert all(r.timed_out for r in res)

@@ -242,7 +242,7 @@ def test_time_budget_and_host_stop(g1):
     assert r.ans == 1000 and not r.stopped
 
 
-def test_enum_capped_config1(g1):
+def test_enum_capped_cl10k(g1):
     for q in query_set(g1, 2, 8, False, seed=11) + query_set(g1, 2, 16, True, seed=12):
         say("capped", q.name)
         res = _compare(g1, q, O.order_ri(q), cap=1_000_000, check_work=False)

@@ -1,12 +1,12 @@
-"""GPU parity at the R-MAT benchmark configurations' own sizes, against oracle results in
+"""GPU parity at the R-MAT benchmark workloads' own sizes, against oracle results in
 tests/golden/rmat_fixtures.json (written by tests/golden/make_rmat_fixtures.py; the oracle's
 GQL takes seconds to minutes per query on these graphs, so it is not rerun here).
 
-  config 3  R-MAT 2^20, 2*10^7 edges; one replayed batch of 64 queries of the 1,000-query mix
+  rmat20    R-MAT 2^20, 2*10^7 edges; one replayed batch of 64 queries of the 1,000-query mix
             (every 16th by size, 4..32 vertices, dense and sparse); cap 10^7
-  config 4  the same batch with learned orders (a cost model trained here on node counts the
+  rmat20 learned the same batch with learned orders (a cost model trained here on node counts the
             enumerator logs): counts equal the plain run's and the oracle's
-  config 5  R-MAT 2^22, average degree 32; 16 queries of the serving stream served on a
+  rmat22    R-MAT 2^22, average degree 32; 16 queries of the serving stream served on a
             Poisson arrival stream; cap 10^6
 
 Candidate sets are compared bit for bit through their SHA-256 digests; counts are
@@ -37,11 +37,11 @@ def fix():
 
 
 @pytest.fixture(scope="module")
-def c3(fix):
-    say("config 3: building R-MAT 2^20")
+def rmat20_in(fix):
+    say("rmat20: building R-MAT 2^20")
     g = rmat(20, 20_000_000, seed=3)
     qs = query_set(g, 1000, lambda i, r: int(r.integers(4, 33)), lambda i, r: i % 2 == 0, seed=33)
-    F = fix["config3"]
+    F = fix["rmat20"]
     sub = [qs[r["id"]] for r in F["queries"]]
     runner = sgm.Batch(g, slots=64)
     yield g, qs, sub, F, runner
@@ -71,38 +71,38 @@ def _check_counts(res, ok, recs, cap):
     return checked
 
 
-def test_config3_candidate_sets(c3):
-    g, qs, sub, F, runner = c3
-    say("config 3: candidate digests of", len(sub), "queries")
+def test_rmat20_candidate_sets(rmat20_in):
+    g, qs, sub, F, runner = rmat20_in
+    say("rmat20: candidate digests of", len(sub), "queries")
     _check_sets(g, sub, F["queries"])
 
 
-def test_config3_replayed_batch(c3):
-    g, qs, sub, F, runner = c3
+def test_rmat20_replayed_batch(rmat20_in):
+    g, qs, sub, F, runner = rmat20_in
     ok, cnt, _ = runner.filter(sub)
     for i, rec in enumerate(F["queries"]):
         assert list(cnt[i, :rec["n"]]) == rec["cand_counts"], rec["id"]
     for rep in range(2):  # the captured enumeration graph, replayed
         res = runner.enumerate(cap=F["cap"])
         checked = _check_counts(res, ok, F["queries"], F["cap"])
-        say("config 3 replay", rep, "checked", checked, "of", len(sub))
+        say("rmat20 replay", rep, "checked", checked, "of", len(sub))
         assert checked >= len(sub) - 4
 
 
-def test_config4_learned_orders_keep_counts(c3):
+def test_rmat20_learned_orders_keep_counts(rmat20_in):
     """Learned orders (cost model trained on the enumerator's node counts for a few
     training queries of the same mix) change no count."""
     from subgraphmatching_amd.learned import collect_training, train_model
-    g, qs, sub, F, runner = c3
+    g, qs, sub, F, runner = rmat20_in
     train = query_set(g, 128, lambda i, r: int(r.integers(4, 33)), lambda i, r: i % 2 == 0, seed=44)
     items, labels, info = collect_training(runner, train, n_samples=2, cap=F["cap"], seed=1, time_limit_s=5.0,
                                            label="nodes")
-    say("config 4: trained on", len(items), "triples")
+    say("rmat20 learned: trained on", len(items), "triples")
     model, losses = train_model(items, labels, seed=7, epochs=10)
     assert np.isfinite(losses).all()
     ok, _, _ = runner.filter(sub)
     orders, pred, pred_ri, used = runner.rank_orders(model, n_samples=32, seed=5, margin=0.5)
-    say("config 4: learned orders used for", int(np.sum(used)), "queries")
+    say("rmat20 learned: learned orders used for", int(np.sum(used)), "queries")
     res = runner.enumerate(orders=orders, cap=F["cap"])
     assert _check_counts(res, ok, F["queries"], F["cap"]) >= len(sub) - 4
     ri = runner.enumerate(cap=F["cap"])
@@ -110,17 +110,17 @@ def test_config4_learned_orders_keep_counts(c3):
     model.close()
 
 
-def test_config5_serving(fix):
+def test_rmat22_serving(fix):
     from subgraphmatching_amd.serving import poisson_arrivals, serve
-    F = fix["config5"]
-    say("config 5: building R-MAT 2^22")
+    F = fix["rmat22"]
+    say("rmat22: building R-MAT 2^22")
     g = rmat(22, 1 << 26, seed=5)
     qs = query_set(g, 1200, lambda i, r: int(r.integers(8, 33)), lambda i, r: i % 2 == 0, seed=55)
     sub = [qs[r["id"]] for r in F["queries"]]
     _check_sets(g, sub, F["queries"])
     runner = sgm.Batch(g, slots=8)
     out = serve(runner, sub, poisson_arrivals(len(sub), 200.0, seed=77), cap=F["cap"])
-    say("config 5: served", out.completed, "in", out.batches, "batches")
+    say("rmat22: served", out.completed, "in", out.batches, "batches")
     checked = 0
     for c, rec in zip(out.counts, F["queries"]):
         if rec["count"] is None:

@@ -82,7 +82,7 @@ def test_tree_filter_batch_equals_oracle(g3k, kind):
 
 
 def test_tree_filters_power_law_100k():
-    """The config-2 graph (Chung-Lu 10^5, exponent 2.3, average degree 16) and four of its
+    """The Chung-Lu 10^5 graph (Chung-Lu 10^5, exponent 2.3, average degree 16) and four of its
     dense 16-vertex queries."""
     g = chung_lu(100000, 2.3, 16.0, seed=2)
     qs = query_set(g, 4, 16, True, seed=22)
