@@ -424,35 +424,51 @@ def setup_rmat_learned(args):
     nodes = fit("plain-nodes") if args.learn_label != "plain-nodes" else rank
     stats = {}
 
+    # two runners on their own streams, as in the rmat20 workload: batch k+1 is filtered, ranked
+    # and launched while batch k enumerates (the same for the RI and the learned passes)
+    runners = [runner, sgm.Batch(g, slots=B, ctx=runner.ctx)]
+
     def run(fitted):
         """One pass over the 1,000 queries: filter, order (RI when fitted is None), enumerate."""
         rec = new_rec()
         t_start = time.perf_counter()
         st = {"learned_used": 0, "pred": [], "meas": [], "nodes": 0, "busy_ns": 0, "rank_s": 0.0, "orders": []}
-        for part in chunks:
-            ok, _, _ = runner.filter(part)
-            f_ms, _ = runner.last_ms()
+        pending = collections.deque()  # launched batches: (runner, ok flags, predictions)
+
+        def collect(p):
+            r, ok, pb = p
+            res = r.enumerate_wait()
+            rec["enum_ms"] += r.last_ms()[1]
+            rec["enum_launches"] += 1
+            for i, (good, x) in enumerate(zip(ok, res)):
+                add_result(rec, x if good else None)
+                if good:
+                    st["nodes"] += sum(x.nodes[:-1])
+                    st["busy_ns"] += x.busy_ns
+                    if fitted is not None and fitted["label"] != "plain-nodes":
+                        st["pred"].append(float(pb[i]))
+                        st["meas"].append(float(LABELS[fitted["label"]](x)))
+
+        for k, part in enumerate(chunks):
+            r = runners[k % len(runners)]
+            while any(p[0] is r for p in pending):  # the runner's previous batch, and those before it
+                collect(pending.popleft())
+            ok, _, _ = r.filter(part)
+            f_ms, _ = r.last_ms()
             rec["filter_ms"] += f_ms
             rec["gql_ms"] += f_ms
-            orders = None
+            orders = pb = None
             if fitted is not None:
                 t1 = time.perf_counter()
-                orders, pb, _, fl = runner.rank_orders(fitted["model"], n_samples=args.rank_samples, seed=5,
-                                                       margin=args.rank_margin)
+                orders, pb, _, fl = r.rank_orders(fitted["model"], n_samples=args.rank_samples, seed=5,
+                                                  margin=args.rank_margin)
                 st["rank_s"] += time.perf_counter() - t1
                 st["learned_used"] += int(fl.sum())
                 st["orders"].append((orders, ok, [float(x) for x in pb]))
-            res = runner.enumerate(orders=orders, cap=args.cap, **tuned)
-            rec["enum_ms"] += runner.last_ms()[1]
-            rec["enum_launches"] += 1
-            for i, (good, r) in enumerate(zip(ok, res)):
-                add_result(rec, r if good else None)
-                if good:
-                    st["nodes"] += sum(r.nodes[:-1])
-                    st["busy_ns"] += r.busy_ns
-                    if fitted is not None and fitted["label"] != "plain-nodes":
-                        st["pred"].append(float(pb[i]))
-                        st["meas"].append(float(LABELS[fitted["label"]](r)))
+            r.enumerate_start(orders=orders, cap=args.cap, **tuned)
+            pending.append((r, ok, pb))
+        while pending:
+            collect(pending.popleft())
         rec["wall_s"] = time.perf_counter() - t_start
         rec["filter_wall_s"] = rec["filter_ms"] / 1e3
         return rec, st
@@ -636,7 +652,8 @@ def main():
               "and nothing in this build uses a second GPU", file=sys.stderr)
         sys.exit(2)
 
-    if args.workload == "rmat20" and not args.no_overlap:
+    if (args.workload == "rmat20" and not args.no_overlap) or args.workload == "rmat20-learned" or \
+            (args.workload == "chunglu100k" and args.learned):
         # the two overlapped runners' streams need hardware queues of their own (HIP shares
         # its default 4 among the context, stop, null and runner streams); set before the
         # first HIP call of the process
