@@ -2176,6 +2176,7 @@ __global__ void __launch_bounds__(256) k_enum_dyn(const sgm_dynplan_dev* __restr
                     uint32_t s = 0;
                     for (uint32_t i = lane; i < size; i += 64) s += wf[list[i]];
                     w = wave_sum32(s);  // u32 sum, wrapping as the reference's
+                    if (lane == 0) S.probes += 2ull * size;  // the list's positions and their weights
                 }
                 const uint64_t val = ((uint64_t)f << 40) | ((uint64_t)((P.deg1 >> f) & 1u) << 32) | w;
                 dst[len] = val;
